@@ -1,0 +1,164 @@
+"""Pressure-solve benchmark on one MI355X.
+
+A step is one pressure solve of the 2.06M-cell jittered-tet Poisson system
+(70³ hexes × 6 Kuhn tets, ±20 % interior jitter, seed 1; fvm::laplacian with
+fixedValue 0 on xMax) by GAMG-preconditioned PCG from x = 0 to normalised residual
+1e-6, FP64, with the matrix, hierarchy, b and x resident in HBM. Every step
+re-zeroes x on the device and solves again.
+
+Reported: solves/s (value), p50/p99 solve latency, iterations, the level-0 Amul
+roofline (algorithmic bytes 24n + 16nf per launch over its HIP-event-timed average
+duration) and a CPU baseline: the oracle's restatement of the reference solver
+(GAMG-PCG with Gauss–Seidel smoothing, bit-identical to the reference) on one core.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "pressure solves/sec + Amul HBM GB/s at 1e-6 resid; p50/p99 latency, mixed vs FP64"
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(m, b, tol, budget_s=30.0):
+    """Oracle (C port of the reference solver, 1 thread) on the same system: one
+    GAMG-PCG solve (Gauss–Seidel smoothing as the reference)."""
+    from oracle import oracle as O
+    from voluma_amd import workloads as W
+
+    ni = m.n_internal
+    om = O.LduMesh(m.n_cells, m.owner[:ni], m.neighbour, m.agglomeration_weights())
+    d, up, ic, bs = O.laplacian(m, W.bc_fixed_xmax(m))
+    d, _ = O.fold(m, d, np.zeros(m.n_cells), ic, bs)
+    t0 = time.perf_counter()
+    n_solves = 0
+    iters = 0
+    while True:
+        _, p = om.solve(d, up, None, b, preconditioner="GAMG", tolerance=tol, max_iter=1000)
+        n_solves += 1
+        iters = p["iterations"]
+        if time.perf_counter() - t0 > budget_s / 3 or n_solves >= 3:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": n_solves / dt, "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"{n_solves} full GAMG(GS)-PCG solve(s) of the same system to {tol:g}, {iters} iterations, "
+                      f"{dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="tet70")
+    ap.add_argument("--tol", type=float, default=1e-6)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--pre", type=int, default=2)
+    ap.add_argument("--post", type=int, default=2)
+    ap.add_argument("--omega", type=float, default=0.8)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import voluma_amd as va
+    from voluma_amd import workloads as W
+
+    t0 = time.perf_counter()
+    m = W.make_mesh(args.workload)
+    log(f"mesh {args.workload}: {m.n_cells} cells, {m.n_internal} internal faces ({time.perf_counter() - t0:.1f} s)")
+    ctx = va.Context(0)
+    t0 = time.perf_counter()
+    lm, A, b = W.pressure_system(ctx, m, seed=1, gamg=True)
+    levels = [L["n"] for L in lm.gamg_levels()]
+    log(f"GPU setup (assembly + hierarchy) {time.perf_counter() - t0:.1f} s, levels {levels}")
+    n, nf = m.n_cells, m.n_internal
+    db = va.DeviceArray.from_host(ctx, b)
+    dx = va.DeviceArray(ctx, n)
+    zero = np.zeros(n)
+    ctl = va.SolverControls("PCG", "GAMG", args.tol, 0.0, 1000, sums=va.SUMS_FAST, gamgPreSweeps=args.pre,
+                            gamgPostSweeps=args.post, gamgOmega=args.omega)
+
+    def step():
+        dx.upload(zero)
+        return va.solve_ldu(A, db, dx, ctl)
+
+    for _ in range(args.warmup):
+        p = step()
+    ctx.synchronize()
+    ctx.timing(True)
+    ctx.read_timing(reset=True)
+    lat = []
+    iters = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        t1 = time.perf_counter()
+        p = step()
+        lat.append(time.perf_counter() - t1)
+        iters.append(p.iterations)
+        if not p.converged:
+            raise RuntimeError(f"solve {k} did not converge: {p}")
+    ctx.synchronize()
+    elapsed = time.perf_counter() - t_start
+    amul_ms, amul_cnt = ctx.read_timing(reset=True)
+    ctx.timing(False)
+    lat_ms = np.array(lat) * 1e3
+    x = dx.download()
+    # verify the last solution's true residual on the host side of the library
+    r = b - va.amul(A, x)
+    true_res = float(np.abs(r).sum() / p.normFactor)
+    amul_bytes = 24 * n + 16 * nf
+    amul_avg_s = amul_ms / max(amul_cnt, 1) / 1e3
+    achieved = amul_bytes / amul_avg_s / 1e9 if amul_cnt else None
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", f"amul_{args.workload}_traffic.json")
+    if os.path.exists(prof):
+        with open(prof) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": args.steps / elapsed,
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"{args.workload}: pressure Poisson, GAMG-PCG (damped-Jacobi V-cycle "
+                               f"{args.pre}/{args.post}, omega {args.omega}) to normalised residual {args.tol:g}, FP64",
+                   "cells": n, "internal_faces": nf, "gamg_levels": levels, "precision": "fp64"},
+        "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),
+                       "min": float(lat_ms.min())},
+        "iterations": int(np.median(iters)),
+        "final_residual": p.finalResidual,
+        "true_residual": true_res,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": None if achieved is None else achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "kernel": "k_sell<OP_AMUL> level-0 A·p", "algorithmic_bytes": amul_bytes,
+                     "avg_us": amul_avg_s * 1e6, "launches": amul_cnt},
+    }
+    if not args.no_cpu_baseline:
+        log("CPU baseline (oracle, 1 thread) ...")
+        out["cpu_baseline"] = cpu_baseline(m, b, args.tol)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

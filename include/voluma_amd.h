@@ -1,0 +1,171 @@
+/* voluma_amd.h — C ABI of the MI355X (gfx950) LDU linear-solve library.
+ *
+ * Drop-in boundary for Voluma's linear-solve path. The reference interface this
+ * replaces (paths relative to the Voluma source tree):
+ *
+ *   SolverPerformance solveLdu(const LduSystem&, std::vector<scalar>& x,
+ *                              const SolverControls&);           src/fvm/linalg/LinearSolver.h:42
+ *   void amul(const LduSystem&, std::vector<scalar>& x,
+ *             std::vector<scalar>& y);                            src/fvm/linalg/LinearSolver.h:45
+ *   struct SolverControls / SolverPerformance / LduSystem         src/fvm/linalg/LinearSolver.h:15-40
+ *   GamgHierarchy buildHierarchy(const Mesh&) (cached per mesh)   src/fvm/linalg/GAMG.h:57-58
+ *
+ * The reference's LduSystem is a non-owning view {mesh, diag (boundary-folded), upper,
+ * lower (== upper when symmetric), b}. Here the mesh addressing and the GAMG hierarchy
+ * live in a vol_mesh handle (built once per mesh, like the reference's Mesh::cached
+ * hierarchy), the coefficients in a vol_matrix handle (resident in HBM, so repeated
+ * solves with one matrix — PISO correctors — reuse them), and b/x are passed per solve.
+ *
+ * Conventions: plain pointers and sizes, int32 labels, FP64 values; every function
+ * returns VOL_OK (0) or a negative status and never throws; vol_last_error() gives
+ * the message of the calling thread's last failure. Non-convergence is not an error
+ * (perf.converged = 0), as in the reference (LinearSolver.cpp:283).
+ * Host pointers are copied; with VOL_DEVICE the pointers are device memory of the
+ * context's GPU and stay where they are.
+ */
+#ifndef VOLUMA_AMD_H
+#define VOLUMA_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (reference exception sites in parentheses) */
+#define VOL_OK 0
+#define VOL_ERR_INVALID_ARGUMENT (-1)
+#define VOL_ERR_UNKNOWN_SOLVER (-2)         /* LinearSolver.cpp:280 */
+#define VOL_ERR_UNKNOWN_PRECONDITIONER (-3) /* LinearSolver.cpp:76 */
+#define VOL_ERR_PCG_NONSYMMETRIC (-4)       /* LinearSolver.cpp:177 */
+#define VOL_ERR_SINGULAR_COARSEST (-5)      /* GAMG.cpp:336 */
+#define VOL_ERR_UNSUPPORTED (-6)            /* valid in the reference, not provided on the GPU */
+#define VOL_ERR_HIP (-7)                    /* HIP runtime failure */
+#define VOL_ERR_NO_GAMG_WEIGHTS (-8)        /* GAMG requested on a mesh created without weights */
+
+/* ---- memory location of caller buffers */
+#define VOL_HOST 0
+#define VOL_DEVICE 1
+
+/* ---- summation modes (vol_solver_controls.sums) */
+#define VOL_SUMS_CONTEXT (-1)     /* follow the context: CFD_REPRODUCIBLE or vol_set_reproducible */
+#define VOL_SUMS_FAST 0           /* deterministic tree sums (the reference's default mode analogue) */
+#define VOL_SUMS_REPRODUCIBLE 1   /* reference reproducible mode: exact sums + preconditioner swap */
+#define VOL_SUMS_EXACT 2          /* exact sums, no swap (deterministic GAMG; extension) */
+
+/* ---- arithmetic of the solve (vol_solver_controls.precision) */
+#define VOL_FP64 0
+
+typedef struct vol_context_s* vol_context;
+typedef struct vol_mesh_s* vol_mesh;
+typedef struct vol_matrix_s* vol_matrix;
+
+/* SolverControls (LinearSolver.h:15-23) plus GPU extensions. Strings as in the
+ * reference: solver PCG | PBiCGStab | GaussSeidel | symGaussSeidel | Jacobi;
+ * preconditioner diagonal | DIC | DILU | GAMG | none. Call vol_controls_default()
+ * to get the reference defaults. */
+typedef struct {
+    const char* solver;
+    const char* preconditioner;
+    double tolerance;   /* 1e-8 */
+    double relTol;      /* 0 */
+    int maxIter;        /* 1000 */
+    int minIter;        /* 0 */
+    int nSweeps;        /* 1 */
+    /* extensions */
+    int sums;           /* VOL_SUMS_*, default VOL_SUMS_CONTEXT */
+    int precision;      /* VOL_FP64 */
+    int gamgPreSweeps;  /* damped-Jacobi GAMG smoother sweeps (default 2, ≥ 1) */
+    int gamgPostSweeps; /* default 2, ≥ 1 */
+    double gamgOmega;   /* damping (default 0.8) */
+} vol_solver_controls;
+
+/* SolverPerformance (LinearSolver.h:25-31) */
+typedef struct {
+    char solver[32];
+    double initialResidual;
+    double finalResidual;
+    double residualSum;
+    double normFactor;
+    int iterations;
+    int converged;
+} vol_solver_performance;
+
+void vol_controls_default(vol_solver_controls* c);
+
+/* ---- context: one GPU, one HIP stream. Reads CFD_REPRODUCIBLE like
+ * par::Environment does (src/fvm/parallel/Comm.cpp:48). */
+int vol_context_create(int device, vol_context* out);
+int vol_context_destroy(vol_context ctx);
+int vol_set_reproducible(vol_context ctx, int on); /* par::setReproducible, Comm.h:38 */
+int vol_synchronize(vol_context ctx);
+const char* vol_last_error(void);
+
+/* Device memory of the context's GPU for VOL_DEVICE buffers. */
+int vol_device_alloc(vol_context ctx, uint64_t bytes, void** ptr);
+int vol_device_free(vol_context ctx, void* ptr);
+int vol_memcpy_to_device(vol_context ctx, void* dst, const void* src, uint64_t bytes);
+int vol_memcpy_to_host(vol_context ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ---- mesh: LDU addressing of nInternalFaces faces between nCells cells
+ * (Mesh::owner/neighbour, internal part). gamgWeights (nullable) is the per-face
+ * agglomeration weight |S_f|·nonOrthDeltaCoeff_f that buildHierarchy uses
+ * (GAMG.cpp:210-218); with it the GAMG hierarchy is built here, once.
+ * mergeLevels: pairings per level (CFD_GAMG_MERGE, GAMG.cpp:96-99; 0 → 2). */
+int vol_mesh_create(vol_context ctx, int nCells, int nInternalFaces, const int32_t* owner, const int32_t* neighbour,
+                    const double* gamgWeights, int mergeLevels, vol_mesh* out);
+int vol_mesh_destroy(vol_mesh mesh);
+/* GAMG hierarchy inspection (GamgHierarchy::levels, GAMG.h:18-31): number of levels
+ * and per-level cell counts (sizes has room for 40). */
+int vol_mesh_gamg_levels(vol_mesh mesh, int* nLevels, int32_t* sizes);
+/* level k's agg (cell → next-level cell), faceMap and faceFlip (−1 / 0 on the coarsest) */
+int vol_mesh_gamg_level(vol_mesh mesh, int level, int32_t* nFaces, int32_t* agg, int32_t* faceMap,
+                        int32_t* faceFlip);
+
+/* Boundary faces of the mesh (after the internal ones, in patch order), by owner cell:
+ * Mesh::owner()[nInternalFaces + bf]. Needed by the assembly entry points. */
+int vol_mesh_set_boundary(vol_mesh mesh, int nBoundaryFaces, const int32_t* boundaryOwner);
+
+/* ---- matrix: coefficients over a mesh. diag must already include the boundary
+ * contributions (LinearSolver.h:35, folded by FvMatrix::solve, FvMatrix.h:154-163);
+ * lower == NULL or lower == upper means symmetric (LinearSolver.h:37). */
+int vol_matrix_create(vol_mesh mesh, vol_matrix* out);
+int vol_matrix_set(vol_matrix A, const double* diag, const double* upper, const double* lower, int location);
+int vol_matrix_destroy(vol_matrix A);
+/* read back diag (folded), upper and lower (== upper when symmetric); FvMatrix::diag()/
+ * upper()/lowerPtr() (FvMatrix.h:32-44). Any output may be NULL. */
+int vol_matrix_get(vol_matrix A, double* diag, double* upper, double* lower, int location);
+
+/* y = A x (LinearSolver.cpp:121-133), x and y of length nCells. */
+int vol_amul(vol_matrix A, const double* x, double* y, int location);
+
+/* solveLdu (LinearSolver.cpp:135-285): x in/out, b of length nCells. */
+int vol_solve(vol_matrix A, const double* b, double* x, const vol_solver_controls* controls,
+              vol_solver_performance* perf, int location);
+
+/* ---- coefficient assembly on the GPU, followed by the boundary fold of FvMatrix::solve
+ * (FvMatrix.h:154-163), for a scalar field whose patches are fixedValue (bcFixed[bf] = 1,
+ * value bcValue[bf], nullable → 0) or zeroGradient (0) (BoundaryCondition.h:116-137).
+ * Face arrays (faceGamma, magSf, ndc, flux) hold nInternalFaces + nBoundaryFaces entries.
+ * A receives the folded diagonal and upper (and lower for div); bOut (nullable) gets
+ * bIn (nullable → 0) plus the folded boundary source; internalCoeffs/boundarySource
+ * (nullable, nBoundaryFaces) get the unfolded boundary terms.
+ *   fvm::laplacian(γ, ψ) without the explicit non-orthogonal correction: Fvm.h:141-154,171-180
+ *   (faceGamma nullable → uniform gamma)
+ *   fvm::div(F, ψ, upwind): Fvm.h:74-106,126-135 */
+int vol_assemble_laplacian(vol_matrix A, double gamma, const double* faceGamma, const double* magSf,
+                           const double* ndc, const int32_t* bcFixed, const double* bcValue, const double* bIn,
+                           double* bOut, double* internalCoeffs, double* boundarySource, int location);
+int vol_assemble_div_upwind(vol_matrix A, const double* flux, const int32_t* bcFixed, const double* bcValue,
+                            const double* bIn, double* bOut, double* internalCoeffs, double* boundarySource,
+                            int location);
+
+/* ---- measurement hooks (no reference counterpart): time every level-0 A·p
+ * product issued by vol_solve with HIP events on the context stream. */
+int vol_timing_enable(vol_context ctx, int on);
+int vol_timing_read(vol_context ctx, double* amulTotalMs, int64_t* amulCount, int reset);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

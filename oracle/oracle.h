@@ -1,0 +1,90 @@
+/* oracle.h — CPU restatement (plain C) of the reference's LDU linear-solve path.
+ *
+ * TEST INFRASTRUCTURE ONLY. Only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load liboracle.so, and only as the checker / CPU baseline;
+ * the product (voluma_amd) never calls it.
+ *
+ * Every function cites the reference file:line it restates (paths relative to the
+ * Voluma source tree). Pinning is done in tests/: against fixtures written by the
+ * real reference (oracle/_ref, tests/golden/), and against closed forms, dense
+ * brute force and invariants.
+ */
+#ifndef VOLUMA_ORACLE_H
+#define VOLUMA_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- exact summation (src/fvm/parallel/Comm.h:42-93, Comm.cpp:107-164) */
+double oracle_exact_sum(const double* v, long n);
+/* sequential plain sum (the reference's non-reproducible SumAcc, Comm.h:96-116) */
+double oracle_plain_sum(const double* v, long n);
+
+/* ---------------- LDU addressing ("mesh") handle
+ * n cells, nf internal faces, owner/neighbour over internal faces, optional GAMG
+ * agglomeration weights |S_f|·ndc_f (NULL: no GAMG). mergeLevels = pairings per
+ * GAMG level (reference default 2, GAMG.cpp:96-99). */
+void* oracle_mesh_new(int n, int nf, const int* own, const int* nei, const double* wgt, int mergeLevels);
+void oracle_mesh_free(void* h);
+
+/* GAMG hierarchy queries (GAMG.cpp:204-261) */
+int oracle_gamg_nlevels(void* h);
+void oracle_gamg_level_sizes(void* h, int k, int* n, int* nf);
+void oracle_gamg_level(void* h, int k, int* agg, int* faceMap, int* faceFlip, int* l, int* u, double* weight);
+
+/* y = A x (LinearSolver.cpp:121-133) */
+void oracle_amul(void* h, const double* diag, const double* upper, const double* lower, const double* x, double* y);
+
+/* Galerkin coarse coefficients of level k (GAMG.cpp:263-301); arrays sized by level */
+void oracle_gamg_coeffs(void* h, const double* diag, const double* upper, const double* lower, int k, double* cdiag,
+                        double* cupper, double* clower);
+
+/* Preconditioner application. smoother: 0 = reference Gauss–Seidel V-cycle
+ * (GAMG.cpp:363-449), 1 = the project's damped-Jacobi V-cycle (nPre/nPost sweeps,
+ * damping omega). Returns 0 or a negative error code. */
+int oracle_gamg_apply(void* h, const double* diag, const double* upper, const double* lower, const double* r, double* w,
+                      int smoother, double omega, int nPre, int nPost);
+
+/* solveLdu (LinearSolver.cpp:135-285).
+ *   mode 0: plain sequential sums, names used as given (reference default)
+ *   mode 1: reference reproducible mode: exact sums + preconditioner/smoother swap (:138-148)
+ *   mode 2: exact sums, no swap (project extension: deterministic GAMG)
+ *   gamgSmoother/omega/nPre/nPost: V-cycle variant when preconditioner == "GAMG".
+ *   hist (nullable, maxIter+1 entries): finalResidual after each iteration; hist[0] = initial.
+ *   perf[4] = initialResidual, finalResidual, residualSum, normFactor.
+ * Returns 0, or <0 on error with a message in err (reference exceptions). */
+int oracle_solve(void* h, const double* diag, const double* upper, const double* lower, int symmetric,
+                 const double* b, double* x, const char* solver, const char* precond, double tol, double relTol,
+                 int maxIter, int minIter, int nSweeps, int mode, int gamgSmoother, double omega, int nPre, int nPost,
+                 double* perf, int* iters, int* converged, double* hist, char* err, int errLen);
+
+/* ---------------- assembly (src/fvm/discretization/Fvm.h)
+ * nI internal faces, nB boundary faces; own has nI+nB entries (boundary faces after internal).
+ * Boundary condition per boundary face: bcFixed[bf] != 0 → fixedValue(bcValue[bf]), else zeroGradient
+ * (src/fvm/field/BoundaryCondition.h:116-137). */
+/* fvm::laplacian(gamma_f, ψ) coefficients (Fvm.h:141-154,171-180); gamma NULL → 1 */
+void oracle_laplacian(int n, int nI, int nB, const int* own, const int* nei, const double* magSf, const double* ndc,
+                      const double* gamma, const int* bcFixed, const double* bcValue, double* diag, double* upper,
+                      double* ic, double* bs);
+/* explicit non-orthogonal correction given a cell gradient g (n×3): Fvm.h:155-170 */
+void oracle_laplacian_nonorth(int n, int nI, const int* own, const int* nei, const double* magSf, const double* w,
+                              const double* corr, const double* gamma, const double* g, double* source, double* ffc);
+/* fvc::grad Gauss linear of a scalar with boundary face values bval (Fvc.h:215-225) */
+void oracle_grad_gauss(int n, int nI, int nB, const int* own, const int* nei, const double* w, const double* Sf,
+                       const double* V, const double* psi, const double* bval, double* g);
+/* fvm::div(F, ψ, upwind) (Fvm.h:74-106,126-135) */
+void oracle_div_upwind(int n, int nI, int nB, const int* own, const int* nei, const double* F, const int* bcFixed,
+                       const double* bcValue, double* diag, double* upper, double* lower, double* ic, double* bs);
+/* fvm::ddt Euler (Fvm.h:18-31, Schemes.h:71-86) */
+void oracle_ddt_euler(int n, const double* V, double dt, const double* psi0, double* diag, double* source);
+/* boundary fold done by FvMatrix::solve before solveLdu (FvMatrix.h:154-163) */
+void oracle_fold(int n, int nI, int nB, const int* own, const double* diag, const double* source, const double* ic,
+                 const double* bs, double* d, double* b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

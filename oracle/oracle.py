@@ -1,0 +1,223 @@
+"""ctypes binding of oracle/liboracle.so, the C restatement of the reference's
+LDU linear-solve path (oracle/oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker or the CPU baseline. The product
+package (voluma_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PATH = os.path.join(_HERE, "liboracle.so")
+_LIB = None
+
+MODE_PLAIN, MODE_REPRODUCIBLE, MODE_EXACT = 0, 1, 2
+SMOOTHER_GS, SMOOTHER_JACOBI = 0, 1
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(_PATH):
+            raise RuntimeError(f"{_PATH} not built (make -C oracle oracle)")
+        lib = ctypes.CDLL(_PATH)
+        P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        lib.oracle_exact_sum.argtypes = [P, ctypes.c_long]
+        lib.oracle_exact_sum.restype = D
+        lib.oracle_plain_sum.argtypes = [P, ctypes.c_long]
+        lib.oracle_plain_sum.restype = D
+        lib.oracle_mesh_new.argtypes = [I, I, P, P, P, I]
+        lib.oracle_mesh_new.restype = P
+        lib.oracle_mesh_free.argtypes = [P]
+        lib.oracle_gamg_nlevels.argtypes = [P]
+        lib.oracle_gamg_nlevels.restype = I
+        lib.oracle_gamg_level_sizes.argtypes = [P, I, P, P]
+        lib.oracle_gamg_level.argtypes = [P, I, P, P, P, P, P, P]
+        lib.oracle_amul.argtypes = [P] * 6
+        lib.oracle_gamg_coeffs.argtypes = [P, P, P, P, I, P, P, P]
+        lib.oracle_gamg_apply.argtypes = [P, P, P, P, P, P, I, D, I, I]
+        lib.oracle_gamg_apply.restype = I
+        lib.oracle_solve.argtypes = [P, P, P, P, I, P, P, ctypes.c_char_p, ctypes.c_char_p, D, D, I, I, I, I, I, D, I,
+                                     I, P, P, P, P, ctypes.c_char_p, I]
+        lib.oracle_solve.restype = I
+        lib.oracle_laplacian.argtypes = [I, I, I] + [P] * 10
+        lib.oracle_laplacian_nonorth.argtypes = [I, I] + [P] * 9
+        lib.oracle_grad_gauss.argtypes = [I, I, I] + [P] * 7
+        lib.oracle_div_upwind.argtypes = [I, I, I] + [P] * 10
+        lib.oracle_ddt_euler.argtypes = [I, P, D, P, P, P]
+        lib.oracle_fold.argtypes = [I, I, I] + [P] * 7
+        _LIB = lib
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i32(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def exact_sum(v) -> float:
+    v = _f64(v)
+    return float(_lib().oracle_exact_sum(_p(v), v.size))
+
+
+def plain_sum(v) -> float:
+    v = _f64(v)
+    return float(_lib().oracle_plain_sum(_p(v), v.size))
+
+
+class OracleError(RuntimeError):
+    pass
+
+
+class LduMesh:
+    """LDU addressing (+ optional GAMG weights) for the oracle's solver."""
+
+    def __init__(self, n_cells, owner, neighbour, weights=None, merge_levels=2):
+        self.n = int(n_cells)
+        self.own = _i32(owner)
+        self.nei = _i32(neighbour)
+        self.nf = int(self.own.size)
+        w = _f64(weights)
+        self._h = _lib().oracle_mesh_new(self.n, self.nf, _p(self.own), _p(self.nei), _p(w), int(merge_levels))
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib().oracle_mesh_free(self._h)
+            self._h = None
+
+    # ---------------------------------------------------------------- GAMG
+    def gamg_levels(self):
+        lib = _lib()
+        out = []
+        for k in range(lib.oracle_gamg_nlevels(self._h)):
+            n, nf = ctypes.c_int(), ctypes.c_int()
+            lib.oracle_gamg_level_sizes(self._h, k, ctypes.byref(n), ctypes.byref(nf))
+            n, nf = n.value, nf.value
+            d = {"n": n, "agg": np.zeros(n, np.int32), "faceMap": np.zeros(nf, np.int32),
+                 "faceFlip": np.zeros(nf, np.int32), "l": np.zeros(nf, np.int32), "u": np.zeros(nf, np.int32),
+                 "weight": np.zeros(nf)}
+            lib.oracle_gamg_level(self._h, k, *(_p(d[x]) for x in ("agg", "faceMap", "faceFlip", "l", "u", "weight")))
+            out.append(d)
+        return out
+
+    def gamg_coeffs(self, diag, upper, lower, k):
+        lv = self.gamg_levels()[k]
+        cd, cu, cl = np.zeros(lv["n"]), np.zeros(len(lv["l"])), np.zeros(len(lv["l"]))
+        upper = _f64(upper)
+        lower = upper if lower is None else _f64(lower)
+        _lib().oracle_gamg_coeffs(self._h, _p(_f64(diag)), _p(upper), _p(lower), k, _p(cd), _p(cu), _p(cl))
+        return cd, cu, cl
+
+    def gamg_apply(self, diag, upper, lower, r, smoother=SMOOTHER_GS, omega=1.0, n_pre=2, n_post=2):
+        upper = _f64(upper)
+        lower = upper if lower is None else _f64(lower)
+        w = np.zeros(self.n)
+        rc = _lib().oracle_gamg_apply(self._h, _p(_f64(diag)), _p(upper), _p(lower), _p(_f64(r)), _p(w),
+                                      int(smoother), float(omega), int(n_pre), int(n_post))
+        if rc:
+            raise OracleError(f"gamg_apply failed ({rc})")
+        return w
+
+    # ---------------------------------------------------------------- solve
+    def amul(self, diag, upper, lower, x):
+        upper = _f64(upper)
+        lower = upper if lower is None else _f64(lower)
+        y = np.zeros(self.n)
+        _lib().oracle_amul(self._h, _p(_f64(diag)), _p(upper), _p(lower), _p(_f64(x)), _p(y))
+        return y
+
+    def solve(self, diag, upper, lower, b, x0=None, solver="PCG", preconditioner="diagonal", tolerance=1e-8,
+              rel_tol=0.0, max_iter=1000, min_iter=0, n_sweeps=1, mode=MODE_PLAIN, smoother=SMOOTHER_GS,
+              omega=1.0, n_pre=2, n_post=2, history=False):
+        """solveLdu on (diag (boundary-folded), upper, lower or None=symmetric, b, x0).
+
+        Returns (x, perf dict[, residual history]). Raises OracleError where the
+        reference throws.
+        """
+        diag, upper, b = _f64(diag), _f64(upper), _f64(b)
+        sym = lower is None
+        lower = upper if sym else _f64(lower)
+        x = np.zeros(self.n) if x0 is None else np.array(x0, dtype=np.float64, copy=True)
+        perf = np.zeros(4)
+        it, cv = ctypes.c_int(), ctypes.c_int()
+        hist = np.full(max_iter + max(n_sweeps, 1) + 1, np.nan) if history else None
+        err = ctypes.create_string_buffer(256)
+        rc = _lib().oracle_solve(self._h, _p(diag), _p(upper), _p(lower), int(sym), _p(b), _p(x), solver.encode(),
+                                 preconditioner.encode(), float(tolerance), float(rel_tol), int(max_iter),
+                                 int(min_iter), int(n_sweeps), int(mode), int(smoother), float(omega), int(n_pre),
+                                 int(n_post), _p(perf), ctypes.byref(it), ctypes.byref(cv), _p(hist), err, 256)
+        if rc:
+            raise OracleError(err.value.decode() or f"oracle_solve failed ({rc})")
+        p = {"initialResidual": perf[0], "finalResidual": perf[1], "residualSum": perf[2], "normFactor": perf[3],
+             "iterations": it.value, "converged": bool(cv.value)}
+        if history:
+            return x, p, hist[: it.value + 1]
+        return x, p
+
+
+# -------------------------------------------------------------------- assembly
+def laplacian(mesh, bc_fixed, bc_value=None, gamma=None):
+    """fvm::laplacian coefficients: (diag, upper, internalCoeffs, boundarySource)."""
+    g = mesh.geometry()
+    n, ni, nb = mesh.n_cells, mesh.n_internal, mesh.n_boundary
+    diag, upper, ic, bs = np.zeros(n), np.zeros(ni), np.zeros(nb), np.zeros(nb)
+    bcv = np.zeros(nb) if bc_value is None else _f64(bc_value)
+    _lib().oracle_laplacian(n, ni, nb, _p(_i32(mesh.owner)), _p(_i32(mesh.neighbour)), _p(g["magSf"]), _p(g["ndc"]),
+                            _p(_f64(gamma)), _p(_i32(bc_fixed)), _p(bcv), _p(diag), _p(upper), _p(ic), _p(bs))
+    return diag, upper, ic, bs
+
+
+def laplacian_nonorth(mesh, grad, gamma=None):
+    g = mesh.geometry()
+    n, ni = mesh.n_cells, mesh.n_internal
+    source, ffc = np.zeros(n), np.zeros(ni)
+    _lib().oracle_laplacian_nonorth(n, ni, _p(_i32(mesh.owner)), _p(_i32(mesh.neighbour)), _p(g["magSf"]),
+                                    _p(g["w"]), _p(np.ascontiguousarray(g["corr"])), _p(_f64(gamma)),
+                                    _p(np.ascontiguousarray(grad, dtype=np.float64)), _p(source), _p(ffc))
+    return source, ffc
+
+
+def grad_gauss(mesh, psi, bval):
+    g = mesh.geometry()
+    n = mesh.n_cells
+    out = np.zeros((n, 3))
+    _lib().oracle_grad_gauss(n, mesh.n_internal, mesh.n_boundary, _p(_i32(mesh.owner)), _p(_i32(mesh.neighbour)),
+                             _p(g["w"]), _p(np.ascontiguousarray(g["Sf"])), _p(g["V"]), _p(_f64(psi)),
+                             _p(_f64(bval)), _p(out))
+    return out
+
+
+def div_upwind(mesh, F, bc_fixed, bc_value=None):
+    n, ni, nb = mesh.n_cells, mesh.n_internal, mesh.n_boundary
+    diag, upper, lower, ic, bs = np.zeros(n), np.zeros(ni), np.zeros(ni), np.zeros(nb), np.zeros(nb)
+    bcv = np.zeros(nb) if bc_value is None else _f64(bc_value)
+    _lib().oracle_div_upwind(n, ni, nb, _p(_i32(mesh.owner)), _p(_i32(mesh.neighbour)), _p(_f64(F)),
+                             _p(_i32(bc_fixed)), _p(bcv), _p(diag), _p(upper), _p(lower), _p(ic), _p(bs))
+    return diag, upper, lower, ic, bs
+
+
+def ddt_euler(mesh, dt, psi0):
+    n = mesh.n_cells
+    diag, source = np.zeros(n), np.zeros(n)
+    _lib().oracle_ddt_euler(n, _p(mesh.geometry()["V"]), float(dt), _p(_f64(psi0)), _p(diag), _p(source))
+    return diag, source
+
+
+def fold(mesh, diag, source, ic, bs):
+    n = mesh.n_cells
+    d, b = np.zeros(n), np.zeros(n)
+    _lib().oracle_fold(n, mesh.n_internal, mesh.n_boundary, _p(_i32(mesh.owner)), _p(_f64(diag)), _p(_f64(source)),
+                       _p(_f64(ic)), _p(_f64(bs)), _p(d), _p(b))
+    return d, b

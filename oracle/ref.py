@@ -1,0 +1,239 @@
+"""ctypes binding of oracle/_ref/libvolref.so: the reference (Voluma) compiled serial
+from its own sources by oracle/Makefile, driven through oracle/ref_driver.cpp.
+
+Test infrastructure only: used to write and re-check the golden fixtures under
+tests/golden/ and to cross-check the C oracle in this container. The GPU host never
+has the reference sources, so nothing that runs there may require this module.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_PATH = os.path.join(_HERE, "_ref", "libvolref.so")
+_LIB = None
+
+
+def available() -> bool:
+    return os.path.exists(_PATH)
+
+
+def _lib():
+    global _LIB
+    if _LIB is None:
+        if not available():
+            raise RuntimeError(f"{_PATH} not built (make -C oracle ref, needs the reference sources)")
+        lib = ctypes.CDLL(_PATH)
+        P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        lib.vr_last_error.restype = ctypes.c_char_p
+        lib.vr_set_reproducible.argtypes = [I]
+        lib.vr_box.restype = P
+        lib.vr_box.argtypes = [I, I, I]
+        lib.vr_raw.restype = P
+        lib.vr_raw.argtypes = [ctypes.c_long, P, ctypes.c_long, P, P, P, ctypes.c_long, P, I, P, P, ctypes.c_char_p]
+        lib.vr_free.argtypes = [P]
+        lib.vr_sizes.argtypes = [P, P]
+        lib.vr_patch.argtypes = [P, I, P, P]
+        lib.vr_addressing.argtypes = [P, P, P]
+        lib.vr_geometry.argtypes = [P] * 10
+        lib.vr_cellfaces.argtypes = [P, P, P]
+        lib.vr_laplacian.argtypes = [P, D, P, P, P, I, I, P, P, P, P, P, P]
+        lib.vr_laplacian.restype = I
+        lib.vr_div_upwind.argtypes = [P] * 10
+        lib.vr_div_upwind.restype = I
+        lib.vr_ddt_euler.argtypes = [P, D, P, P, P]
+        lib.vr_ddt_euler.restype = I
+        lib.vr_solve.argtypes = [P, P, P, P, I, P, P, ctypes.c_char_p, ctypes.c_char_p, D, D, I, I, I, P, P, P]
+        lib.vr_solve.restype = I
+        lib.vr_fvm_poisson_solve.argtypes = [P, D, P, P, P, P, ctypes.c_char_p, ctypes.c_char_p, D, D, I, P, P, P]
+        lib.vr_fvm_poisson_solve.restype = I
+        lib.vr_amul.argtypes = [P] * 6
+        lib.vr_amul.restype = I
+        lib.vr_gamg_hierarchy.argtypes = [P] * 10
+        lib.vr_gamg_hierarchy.restype = I
+        lib.vr_gamg_apply.argtypes = [P] * 6
+        lib.vr_gamg_apply.restype = I
+        lib.vr_exact_sum.argtypes = [P, ctypes.c_long]
+        lib.vr_exact_sum.restype = D
+        _LIB = lib
+    return _LIB
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError("reference: " + _lib().vr_last_error().decode())
+
+
+def set_reproducible(on: bool):
+    _lib().vr_set_reproducible(1 if on else 0)
+
+
+def exact_sum(v) -> float:
+    v = np.ascontiguousarray(v, dtype=np.float64)
+    return float(_lib().vr_exact_sum(_p(v), v.size))
+
+
+class RefMesh:
+    """Reference Mesh built from a voluma_amd.mesh.Mesh (raw arrays) or generateBox."""
+
+    def __init__(self, mesh=None, box=None):
+        lib = _lib()
+        if box is not None:
+            h = lib.vr_box(*box)
+        else:
+            pts = np.ascontiguousarray(mesh.points)
+            own = mesh.owner.astype(np.int64)
+            nei = mesh.neighbour.astype(np.int64)
+            names = "\n".join(("xMin", "xMax", "yMin", "yMax", "zMin", "zMax")[: len(mesh.patch_start)])
+            h = lib.vr_raw(pts.shape[0], _p(pts), mesh.n_faces, _p(mesh.face_off), _p(mesh.face_pts), _p(own),
+                           mesh.n_internal, _p(nei), len(mesh.patch_start), _p(mesh.patch_start),
+                           _p(mesh.patch_size), names.encode())
+        if not h:
+            raise RuntimeError("reference: " + lib.vr_last_error().decode())
+        self._h = h
+        s = np.zeros(4, np.int64)
+        lib.vr_sizes(h, _p(s))
+        self.n_cells, self.n_internal, self.n_faces, self.n_patches = (int(v) for v in s)
+        self.owner = np.zeros(self.n_faces, np.int32)
+        self.neighbour = np.zeros(max(self.n_internal, 1), np.int32)
+        lib.vr_addressing(h, _p(self.owner), _p(self.neighbour))
+        self.neighbour = self.neighbour[: self.n_internal]
+        self.patch_start = np.zeros(self.n_patches, np.int64)
+        self.patch_size = np.zeros(self.n_patches, np.int64)
+        for p in range(self.n_patches):
+            a, b = ctypes.c_long(), ctypes.c_long()
+            lib.vr_patch(h, p, ctypes.byref(a), ctypes.byref(b))
+            self.patch_start[p], self.patch_size[p] = a.value, b.value
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib().vr_free(self._h)
+            self._h = None
+
+    def geometry(self):
+        nc, nf = self.n_cells, self.n_faces
+        g = {"V": np.zeros(nc), "C": np.zeros((nc, 3)), "Sf": np.zeros((nf, 3)), "magSf": np.zeros(nf),
+             "Cf": np.zeros((nf, 3)), "d": np.zeros((nf, 3)), "w": np.zeros(nf), "ndc": np.zeros(nf),
+             "corr": np.zeros((nf, 3))}
+        _lib().vr_geometry(self._h, *(_p(g[k]) for k in ("V", "C", "Sf", "magSf", "Cf", "d", "w", "ndc", "corr")))
+        return g
+
+    def laplacian(self, fixed_patches, psi=None, non_orth=False, grad_scheme=0, gamma=1.0, fixed_values=None):
+        nc, ni, nf = self.n_cells, self.n_internal, self.n_faces
+        fp = np.zeros(self.n_patches, np.int32)
+        fp[list(fixed_patches)] = 1
+        fv = None if fixed_values is None else np.ascontiguousarray(fixed_values, dtype=np.float64)
+        out = {"diag": np.zeros(nc), "upper": np.zeros(ni), "source": np.zeros(nc), "ic": np.zeros(nf - ni),
+               "bs": np.zeros(nf - ni), "ffc": np.zeros(nf)}
+        psi = None if psi is None else np.ascontiguousarray(psi, dtype=np.float64)
+        _check(_lib().vr_laplacian(self._h, gamma, _p(fp), _p(fv), _p(psi), int(non_orth), int(grad_scheme),
+                                   *(_p(out[k]) for k in ("diag", "upper", "source", "ic", "bs", "ffc"))))
+        return out
+
+    def div_upwind(self, F, fixed_patches, fixed_values=None):
+        nc, ni, nf = self.n_cells, self.n_internal, self.n_faces
+        fp = np.zeros(self.n_patches, np.int32)
+        fp[list(fixed_patches)] = 1
+        fv = None if fixed_values is None else np.ascontiguousarray(fixed_values, dtype=np.float64)
+        out = {"diag": np.zeros(nc), "upper": np.zeros(ni), "lower": np.zeros(ni), "source": np.zeros(nc),
+               "ic": np.zeros(nf - ni), "bs": np.zeros(nf - ni)}
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        _check(_lib().vr_div_upwind(self._h, _p(F), _p(fp), _p(fv),
+                                    *(_p(out[k]) for k in ("diag", "upper", "lower", "source", "ic", "bs"))))
+        return out
+
+    def ddt_euler(self, dt, psi0):
+        nc = self.n_cells
+        out = {"diag": np.zeros(nc), "source": np.zeros(nc)}
+        psi0 = np.ascontiguousarray(psi0, dtype=np.float64)
+        _check(_lib().vr_ddt_euler(self._h, ctypes.c_double(dt), _p(psi0), _p(out["diag"]), _p(out["source"])))
+        return out
+
+    def solve(self, diag, upper, lower, b, x0, solver="PCG", preconditioner="diagonal", tolerance=1e-8,
+              rel_tol=0.0, max_iter=1000, min_iter=0, n_sweeps=1, reproducible=False):
+        set_reproducible(reproducible)
+        diag = np.ascontiguousarray(diag, dtype=np.float64)
+        upper = np.ascontiguousarray(upper, dtype=np.float64)
+        sym = lower is None
+        lower = upper if sym else np.ascontiguousarray(lower, dtype=np.float64)
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        x = np.array(x0, dtype=np.float64, copy=True)
+        perf = np.zeros(4)
+        it, cv = ctypes.c_int(), ctypes.c_int()
+        try:
+            _check(_lib().vr_solve(self._h, _p(diag), _p(upper), _p(lower), int(sym), _p(b), _p(x), solver.encode(),
+                                   preconditioner.encode(), tolerance, rel_tol, max_iter, min_iter, n_sweeps,
+                                   _p(perf), ctypes.byref(it), ctypes.byref(cv)))
+        finally:
+            set_reproducible(False)
+        return x, {"initialResidual": perf[0], "finalResidual": perf[1], "residualSum": perf[2],
+                   "normFactor": perf[3], "iterations": it.value, "converged": bool(cv.value)}
+
+    def fvm_poisson_solve(self, fixed_patches, su, psi0, solver="PCG", preconditioner="diagonal", tolerance=1e-8,
+                          rel_tol=0.0, max_iter=1000, gamma=1.0, reproducible=False):
+        set_reproducible(reproducible)
+        fp = np.zeros(self.n_patches, np.int32)
+        fp[list(fixed_patches)] = 1
+        su = np.ascontiguousarray(su, dtype=np.float64)
+        psi = np.array(psi0, dtype=np.float64, copy=True)
+        perf = np.zeros(4)
+        it, cv = ctypes.c_int(), ctypes.c_int()
+        try:
+            _check(_lib().vr_fvm_poisson_solve(self._h, gamma, _p(fp), None, _p(su), _p(psi), solver.encode(),
+                                               preconditioner.encode(), tolerance, rel_tol, max_iter, _p(perf),
+                                               ctypes.byref(it), ctypes.byref(cv)))
+        finally:
+            set_reproducible(False)
+        return psi, {"initialResidual": perf[0], "finalResidual": perf[1], "residualSum": perf[2],
+                     "normFactor": perf[3], "iterations": it.value, "converged": bool(cv.value)}
+
+    def amul(self, diag, upper, lower, x):
+        diag = np.ascontiguousarray(diag, dtype=np.float64)
+        upper = np.ascontiguousarray(upper, dtype=np.float64)
+        lower = upper if lower is None else np.ascontiguousarray(lower, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.zeros(self.n_cells)
+        _check(_lib().vr_amul(self._h, _p(diag), _p(upper), _p(lower), _p(x), _p(y)))
+        return y
+
+    def gamg_hierarchy(self):
+        lib = _lib()
+        nl = ctypes.c_int()
+        ln = np.zeros(40, np.int64)
+        lnf = np.zeros(40, np.int64)
+        _check(lib.vr_gamg_hierarchy(self._h, ctypes.byref(nl), _p(ln), _p(lnf), None, None, None, None, None, None))
+        L = nl.value
+        sn, snf = int(ln[:L].sum()), int(lnf[:L].sum())
+        agg = np.zeros(max(sn, 1), np.int32)
+        fmap = np.zeros(max(snf, 1), np.int32)
+        flip = np.zeros(max(snf, 1), np.int32)
+        la = np.zeros(max(snf, 1), np.int32)
+        ua = np.zeros(max(snf, 1), np.int32)
+        wt = np.zeros(max(snf, 1))
+        _check(lib.vr_gamg_hierarchy(self._h, ctypes.byref(nl), _p(ln), _p(lnf), _p(agg), _p(fmap), _p(flip), _p(la),
+                                     _p(ua), _p(wt)))
+        levels, oa, of = [], 0, 0
+        for k in range(L):
+            n, nf = int(ln[k]), int(lnf[k])
+            levels.append({"n": n, "agg": agg[oa:oa + n].copy(), "faceMap": fmap[of:of + nf].copy(),
+                           "faceFlip": flip[of:of + nf].copy(), "l": la[of:of + nf].copy(),
+                           "u": ua[of:of + nf].copy(), "weight": wt[of:of + nf].copy()})
+            oa += n
+            of += nf
+        return levels
+
+    def gamg_apply(self, diag, upper, lower, r):
+        diag = np.ascontiguousarray(diag, dtype=np.float64)
+        upper = np.ascontiguousarray(upper, dtype=np.float64)
+        lower = upper if lower is None else np.ascontiguousarray(lower, dtype=np.float64)
+        r = np.ascontiguousarray(r, dtype=np.float64)
+        w = np.zeros(self.n_cells)
+        _check(_lib().vr_gamg_apply(self._h, _p(diag), _p(upper), _p(lower), _p(r), _p(w)))
+        return w

@@ -1,0 +1,374 @@
+// Test-side driver around the reference implementation (Voluma, built serial from
+// its own sources by oracle/Makefile into oracle/_ref/libvolref.so).
+//
+// This file contains no reference code: it only calls the reference's public API
+// (generateBox, Mesh::build, fvm::laplacian/div/ddt, FvMatrix::solve, solveLdu,
+// amul, buildHierarchy, GamgPreconditioner, par::ExactSum) and copies results into
+// caller-provided flat arrays, so that tests/golden/make_golden.py can write
+// fixtures and tests can cross-check the C oracle against the real reference.
+// It is test infrastructure: nothing in the product links or loads it.
+
+#include "fvm/discretization/Fvm.h"
+#include "fvm/linalg/FvMatrix.h"
+#include "fvm/linalg/GAMG.h"
+#include "fvm/linalg/LinearSolver.h"
+#include "fvm/mesh/Mesh.h"
+#include "fvm/mesh/RawMesh.h"
+#include "fvm/parallel/Comm.h"
+
+#include <cstring>
+#include <exception>
+#include <memory>
+#include <string>
+#include <vector>
+
+using namespace cfd;
+
+namespace {
+
+struct RefMesh {
+    MeshPtr m;
+};
+
+thread_local std::string gErr;
+
+int fail(const std::exception& e) {
+    gErr = e.what();
+    return -1;
+}
+
+// Boundary conditions for a scalar field: fixedPatch[p] != 0 → fixedValue(value[p]),
+// otherwise zeroGradient.
+void setScalarBCs(ScalarField& f, const int* fixedPatch, const double* fixedValue) {
+    const Mesh& m = f.mesh();
+    for (std::size_t p = 0; p < m.patches().size(); ++p) {
+        const auto& name = m.patches()[p].name;
+        if (fixedPatch && fixedPatch[p])
+            f.fixedValue(name, fixedValue ? fixedValue[p] : 0.0);
+        else
+            f.zeroGradient(name);
+    }
+    f.correctBoundaryConditions();
+}
+
+SolverControls makeControls(const char* solver, const char* precond, double tol, double relTol, int maxIter,
+                            int minIter, int nSweeps) {
+    SolverControls c;
+    c.solver = solver;
+    c.preconditioner = precond;
+    c.tolerance = tol;
+    c.relTol = relTol;
+    c.maxIter = maxIter;
+    c.minIter = minIter;
+    c.nSweeps = nSweeps;
+    return c;
+}
+
+void writePerf(const SolverPerformance& p, double* perf, int* iters, int* conv) {
+    perf[0] = p.initialResidual;
+    perf[1] = p.finalResidual;
+    perf[2] = p.residualSum;
+    perf[3] = p.normFactor;
+    *iters = p.iterations;
+    *conv = p.converged ? 1 : 0;
+}
+
+} // namespace
+
+extern "C" {
+
+const char* vr_last_error() { return gErr.c_str(); }
+
+void vr_set_reproducible(int on) { par::setReproducible(on != 0); }
+
+void* vr_box(int nx, int ny, int nz) {
+    try {
+        BoxSpec s;
+        s.n[0] = nx;
+        s.n[1] = ny;
+        s.n[2] = nz;
+        RawMesh raw = generateBox(s);
+        auto* r = new RefMesh;
+        r->m = Mesh::build(&raw);
+        return r;
+    } catch (const std::exception& e) {
+        fail(e);
+        return nullptr;
+    }
+}
+
+// Raw polyMesh arrays → reference Mesh. Patch names are '\n'-separated.
+void* vr_raw(long nPoints, const double* pts, long nFaces, const long* faceOff, const long* facePts, const long* owner,
+             long nInternal, const long* neighbour, int nPatches, const long* pStart, const long* pSize,
+             const char* names) {
+    try {
+        RawMesh raw;
+        raw.points.resize(nPoints);
+        for (long i = 0; i < nPoints; ++i) raw.points[i] = Vec3{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+        raw.faceOffsets.assign(faceOff, faceOff + nFaces + 1);
+        raw.facePoints.assign(facePts, facePts + faceOff[nFaces]);
+        raw.owner.assign(owner, owner + nFaces);
+        raw.neighbour.assign(neighbour, neighbour + nInternal);
+        std::string all(names);
+        std::size_t pos = 0;
+        for (int p = 0; p < nPatches; ++p) {
+            RawPatch rp;
+            const std::size_t e = all.find('\n', pos);
+            rp.name = all.substr(pos, e == std::string::npos ? std::string::npos : e - pos);
+            pos = e == std::string::npos ? all.size() : e + 1;
+            rp.type = PatchType::Patch;
+            rp.start = pStart[p];
+            rp.size = pSize[p];
+            raw.patches.push_back(rp);
+        }
+        glabel nc = 0;
+        for (auto o : raw.owner) nc = std::max(nc, o + 1);
+        for (auto o : raw.neighbour) nc = std::max(nc, o + 1);
+        raw.nCells = nc;
+        auto* r = new RefMesh;
+        r->m = Mesh::build(&raw);
+        return r;
+    } catch (const std::exception& e) {
+        fail(e);
+        return nullptr;
+    }
+}
+
+void vr_free(void* h) { delete static_cast<RefMesh*>(h); }
+
+void vr_sizes(void* h, long* out) {
+    const Mesh& m = *static_cast<RefMesh*>(h)->m;
+    out[0] = m.nCells();
+    out[1] = m.nInternalFaces();
+    out[2] = m.nFaces();
+    out[3] = long(m.patches().size());
+}
+
+void vr_patch(void* h, int p, long* start, long* size) {
+    const Mesh& m = *static_cast<RefMesh*>(h)->m;
+    *start = m.patches()[p].start;
+    *size = m.patches()[p].size;
+}
+
+void vr_addressing(void* h, int* owner, int* neighbour) {
+    const Mesh& m = *static_cast<RefMesh*>(h)->m;
+    std::memcpy(owner, m.owner().data(), sizeof(int) * m.nFaces());
+    std::memcpy(neighbour, m.neighbour().data(), sizeof(int) * m.nInternalFaces());
+}
+
+static void copyVec(const std::vector<Vec3>& v, double* out) {
+    if (!out) return;
+    for (std::size_t i = 0; i < v.size(); ++i) {
+        out[3 * i] = v[i].x;
+        out[3 * i + 1] = v[i].y;
+        out[3 * i + 2] = v[i].z;
+    }
+}
+static void copySc(const std::vector<double>& v, double* out) {
+    if (out) std::memcpy(out, v.data(), sizeof(double) * v.size());
+}
+
+void vr_geometry(void* h, double* V, double* C, double* Sf, double* magSf, double* Cf, double* d, double* w,
+                 double* ndc, double* corr) {
+    const Mesh& m = *static_cast<RefMesh*>(h)->m;
+    copySc(m.V(), V);
+    copyVec(m.C(), C);
+    copyVec(m.Sf(), Sf);
+    copySc(m.magSf(), magSf);
+    copyVec(m.Cf(), Cf);
+    copyVec(m.d(), d);
+    copySc(m.w(), w);
+    copySc(m.nonOrthDeltaCoeffs(), ndc);
+    copyVec(m.corrVecs(), corr);
+}
+
+void vr_cellfaces(void* h, int* off, int* list) {
+    const Mesh& m = *static_cast<RefMesh*>(h)->m;
+    std::memcpy(off, m.cellFaceOffsets().data(), sizeof(int) * m.cellFaceOffsets().size());
+    std::memcpy(list, m.cellFaces().data(), sizeof(int) * m.cellFaces().size());
+}
+
+// fvm::laplacian(gamma (uniform), p) for a scalar p with the given field values (used
+// by the non-orthogonal correction's gradient) and fixedValue/zeroGradient patches.
+int vr_laplacian(void* h, double gamma, const int* fixedPatch, const double* fixedValue, const double* psi,
+                 int nonOrth, int gradScheme, double* diag, double* upper, double* source, double* ic, double* bs,
+                 double* ffc) {
+    try {
+        MeshPtr mp = static_cast<RefMesh*>(h)->m;
+        mp->setGradScheme(GradScheme(gradScheme));
+        ScalarField p(mp, "p", 0.0);
+        if (psi)
+            for (label c = 0; c < mp->nCells(); ++c) p[c] = psi[c];
+        setScalarBCs(p, fixedPatch, fixedValue);
+        auto M = fvm::laplacian(gamma, p, nonOrth != 0);
+        copySc(M.diag(), diag);
+        copySc(M.upper(), upper);
+        copySc(M.source(), source);
+        copySc(M.internalCoeffs(), ic);
+        copySc(M.boundarySource(), bs);
+        if (ffc) {
+            if (M.hasFaceFluxCorrection())
+                copySc(M.faceFluxCorrection(), ffc);
+            else
+                std::memset(ffc, 0, sizeof(double) * mp->nFaces());
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// fvm::div(F, psi, upwind) for a scalar psi with fixedValue/zeroGradient patches.
+int vr_div_upwind(void* h, const double* F, const int* fixedPatch, const double* fixedValue, double* diag,
+                  double* upper, double* lower, double* source, double* ic, double* bs) {
+    try {
+        MeshPtr mp = static_cast<RefMesh*>(h)->m;
+        ScalarField p(mp, "U", 0.0);
+        setScalarBCs(p, fixedPatch, fixedValue);
+        std::vector<scalar> Fv(F, F + mp->nFaces());
+        ConvectionScheme s;
+        s.type = ConvectionScheme::Type::Upwind;
+        auto M = fvm::div(Fv, p, s);
+        copySc(M.diag(), diag);
+        copySc(M.upper(), upper);
+        std::memcpy(lower, M.lowerPtr(), sizeof(double) * mp->nInternalFaces());
+        copySc(M.source(), source);
+        copySc(M.internalCoeffs(), ic);
+        copySc(M.boundarySource(), bs);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// fvm::ddt(psi) with the Euler scheme and step dt; psi0 is the old-time field.
+int vr_ddt_euler(void* h, double dt, const double* psi0, double* diag, double* source) {
+    try {
+        MeshPtr mp = static_cast<RefMesh*>(h)->m;
+        ScalarField p(mp, "U", 0.0);
+        for (label c = 0; c < mp->nCells(); ++c) p[c] = psi0[c];
+        setScalarBCs(p, nullptr, nullptr);
+        p.storeOld();
+        TimeState ts;
+        ts.dt = dt;
+        auto M = fvm::ddt(p, DdtScheme::Euler, ts);
+        copySc(M.diag(), diag);
+        copySc(M.source(), source);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// solveLdu on a raw LDU system (diag already boundary-folded). lower may equal upper.
+int vr_solve(void* h, const double* diag, const double* upper, const double* lower, int symmetric, const double* b,
+             double* x, const char* solver, const char* precond, double tol, double relTol, int maxIter, int minIter,
+             int nSweeps, double* perf, int* iters, int* conv) {
+    try {
+        const Mesh& m = *static_cast<RefMesh*>(h)->m;
+        const label n = m.nTotalCells();
+        std::vector<scalar> d(diag, diag + n), bb(b, b + n), xx(x, x + n);
+        LduSystem sys{m, d, upper, lower, bb, symmetric != 0};
+        auto p = solveLdu(sys, xx, makeControls(solver, precond, tol, relTol, maxIter, minIter, nSweeps));
+        std::memcpy(x, xx.data(), sizeof(double) * n);
+        writePerf(p, perf, iters, conv);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// FvMatrix<scalar>::solve for laplacian(gamma, p) − su (source += su·V) with the given
+// patches: exercises the boundary fold done inside FvMatrix::solve.
+int vr_fvm_poisson_solve(void* h, double gamma, const int* fixedPatch, const double* fixedValue, const double* su,
+                         double* psi, const char* solver, const char* precond, double tol, double relTol, int maxIter,
+                         double* perf, int* iters, int* conv) {
+    try {
+        MeshPtr mp = static_cast<RefMesh*>(h)->m;
+        ScalarField p(mp, "p", 0.0);
+        for (label c = 0; c < mp->nCells(); ++c) p[c] = psi[c];
+        setScalarBCs(p, fixedPatch, fixedValue);
+        auto M = fvm::laplacian(gamma, p, false);
+        std::vector<scalar> s(su, su + mp->nCells());
+        M -= s;
+        auto out = M.solve(makeControls(solver, precond, tol, relTol, maxIter, 0, 1));
+        for (label c = 0; c < mp->nCells(); ++c) psi[c] = p[c];
+        writePerf(out.at(0), perf, iters, conv);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+int vr_amul(void* h, const double* diag, const double* upper, const double* lower, const double* x, double* y) {
+    try {
+        const Mesh& m = *static_cast<RefMesh*>(h)->m;
+        const label n = m.nTotalCells();
+        std::vector<scalar> d(diag, diag + n), bb(n, 0.0), xx(x, x + n), yy(n);
+        LduSystem sys{m, d, upper, lower, bb, upper == lower};
+        amul(sys, xx, yy);
+        std::memcpy(y, yy.data(), sizeof(double) * n);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// GAMG hierarchy. Call with out arrays == nullptr first to get nLevels and per-level
+// sizes (levelN[k], levelNf[k]); then call again with arrays of total length Σ n
+// (agg) and Σ nf (faceMap, faceFlip, l, u) and Σ nf doubles (weight).
+int vr_gamg_hierarchy(void* h, int* nLevels, long* levelN, long* levelNf, int* agg, int* faceMap, int* faceFlip,
+                      int* lArr, int* uArr, double* weight) {
+    try {
+        const Mesh& m = *static_cast<RefMesh*>(h)->m;
+        const GamgHierarchy& H = gamgHierarchy(m);
+        *nLevels = int(H.levels.size());
+        long oa = 0, of = 0;
+        for (std::size_t k = 0; k < H.levels.size(); ++k) {
+            const auto& L = H.levels[k];
+            if (levelN) levelN[k] = L.n;
+            if (levelNf) levelNf[k] = long(L.l.size());
+            if (agg) {
+                for (std::size_t i = 0; i < L.agg.size(); ++i) agg[oa + i] = L.agg[i];
+                for (std::size_t i = L.agg.size(); i < std::size_t(L.n); ++i) agg[oa + i] = -1;
+            }
+            if (faceMap)
+                for (std::size_t f = 0; f < L.l.size(); ++f) {
+                    faceMap[of + f] = f < L.faceMap.size() ? L.faceMap[f] : -2;
+                    faceFlip[of + f] = f < L.faceFlip.size() ? L.faceFlip[f] : 0;
+                    lArr[of + f] = L.l[f];
+                    uArr[of + f] = L.u[f];
+                    weight[of + f] = L.weight[f];
+                }
+            oa += L.n;
+            of += long(L.l.size());
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// w = M⁻¹ r for the reference GAMG preconditioner (one V-cycle, Gauss–Seidel smoothing).
+int vr_gamg_apply(void* h, const double* diag, const double* upper, const double* lower, const double* r, double* w) {
+    try {
+        const Mesh& m = *static_cast<RefMesh*>(h)->m;
+        const label n = m.nTotalCells();
+        std::vector<scalar> d(diag, diag + n), bb(n, 0.0), rr(r, r + n), ww(n, 0.0);
+        LduSystem sys{m, d, upper, lower, bb, upper == lower};
+        GamgPreconditioner P(sys);
+        P.apply(rr, ww);
+        std::memcpy(w, ww.data(), sizeof(double) * n);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+double vr_exact_sum(const double* v, long n) {
+    par::ExactSum s;
+    for (long i = 0; i < n; ++i) s.add(v[i]);
+    return s.value();
+}
+
+} // extern "C"

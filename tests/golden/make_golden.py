@@ -1,0 +1,217 @@
+"""Write the golden fixtures under tests/golden/ from the reference build.
+
+Runs the reference (Voluma compiled serial by oracle/Makefile into
+oracle/_ref/libvolref.so, from the sources in /root/reference) on meshes made by
+voluma_amd.mesh, and stores inputs and outputs as data:
+
+* small_<mesh>.npz — full arrays for small meshes: geometry, laplacian / upwind-div
+  coefficients, GAMG hierarchy maps, solutions and SolverPerformance of every
+  solver/preconditioner pair, residual histories, one GAMG V-cycle, an Amul;
+* cfg1_box64.json, cfg2_tet70.json — the two full-size pressure systems: input and
+  hierarchy hashes, level sizes, iteration counts, residuals, solution hashes,
+  norms and samples for reproducible diagonal-PCG and default GAMG-PCG;
+* exactsum.json — ExactSum outputs for seeded random vectors.
+
+Usage (in the container that has the reference sources):
+    make -C oracle all && python tests/golden/make_golden.py [--small-only]
+"""
+from __future__ import annotations
+
+import hashlib
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+sys.path.insert(0, ROOT)
+
+from oracle import ref  # noqa: E402
+from voluma_amd import mesh as M  # noqa: E402
+from voluma_amd import workloads as W  # noqa: E402
+
+SMALL = {
+    "box8x6x5": lambda: M.box(8, 6, 5),
+    "tet5x4x3": lambda: M.tet(5, 4, 3, 0.2, 7),
+    "box12": lambda: M.box(12),
+}
+
+# (solver, preconditioner, reproducible, symmetric system?)
+CASES = [
+    ("PCG", "diagonal", False, True), ("PCG", "diagonal", True, True), ("PCG", "none", False, True),
+    ("PCG", "GAMG", False, True), ("PCG", "DIC", False, True),
+    ("PBiCGStab", "diagonal", False, False), ("PBiCGStab", "diagonal", True, False),
+    ("PBiCGStab", "none", False, False), ("PBiCGStab", "DILU", False, False), ("PBiCGStab", "GAMG", False, True),
+    ("Jacobi", "none", False, True), ("GaussSeidel", "none", False, True), ("symGaussSeidel", "none", False, True),
+    ("GaussSeidel", "none", True, True),
+]
+
+
+def sha(a) -> str:
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def mesh_hash(m) -> str:
+    h = hashlib.sha256()
+    for a in (m.points, m.face_off, m.face_pts, m.owner, m.neighbour, m.patch_start, m.patch_size):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def flux_field(m, seed):
+    """A face flux for the nonsymmetric (momentum-like) test system: seeded values
+    in [−1, 1)·|S_f| on every face."""
+    u = 2.0 * W.splitmix_uniform(m.n_faces, seed) - 1.0
+    return u * m.geometry()["magSf"]
+
+
+def momentum_like(rm, m, seed=11, nu=1e-3, dt=0.05):
+    """diag/upper/lower of a ddt + div(upwind) − ν·laplacian style LDU system, composed
+    from the reference's own fvm::div and fvm::laplacian coefficients."""
+    g = m.geometry()
+    d = rm.div_upwind(flux_field(m, seed), [W.XMAX_PATCH])
+    lap = rm.laplacian([W.XMAX_PATCH])
+    ni, own = m.n_internal, m.owner
+    diag = d["diag"] + g["V"] / dt - nu * lap["diag"]
+    np.add.at(diag, own[ni:], d["ic"] - nu * lap["ic"])  # boundary fold
+    upper = d["upper"] - nu * lap["upper"]
+    lower = d["lower"] - nu * lap["upper"]
+    return diag, upper, lower
+
+
+def small_case(name, make):
+    m = make()
+    rm = ref.RefMesh(mesh=m)
+    g = rm.geometry()
+    lap = rm.laplacian([W.XMAX_PATCH])
+    n, ni = m.n_cells, m.n_internal
+    own = m.owner
+    dsym = lap["diag"].copy()
+    np.add.at(dsym, own[ni:], lap["ic"])
+    upper = lap["upper"]
+    b = W.pressure_rhs(n, seed=3)
+    np.add.at(b, own[ni:], lap["bs"])
+    dns, uns, lns = momentum_like(rm, m)
+    bns = W.pressure_rhs(n, seed=5)
+    out = {"mesh_hash": np.frombuffer(bytes.fromhex(mesh_hash(m)), np.uint8)}
+    for k in ("V", "magSf", "ndc", "w"):
+        out["geom_" + k] = g[k]
+    out["geom_C"] = g["C"]
+    out["geom_corr"] = g["corr"]
+    for k in ("diag", "upper", "ic", "bs"):
+        out["lap_" + k] = lap[k]
+    F = flux_field(m, 11)
+    dv = rm.div_upwind(F, [W.XMAX_PATCH])
+    for k in ("diag", "upper", "lower", "ic", "bs"):
+        out["div_" + k] = dv[k]
+    out["div_flux"] = F
+    out["sym_diag"], out["sym_upper"], out["sym_b"] = dsym, upper, b
+    out["ns_diag"], out["ns_upper"], out["ns_lower"], out["ns_b"] = dns, uns, lns, bns
+    # hierarchy
+    lv = rm.gamg_hierarchy()
+    out["gamg_sizes"] = np.array([L["n"] for L in lv], np.int64)
+    out["gamg_nfaces"] = np.array([len(L["l"]) for L in lv], np.int64)
+    for key in ("agg", "faceMap", "faceFlip", "l", "u", "weight"):
+        out["gamg_" + key] = np.concatenate([L[key] for L in lv])
+    # amul and one V-cycle
+    x = 2.0 * W.splitmix_uniform(n, 21) - 1.0
+    out["amul_x"] = x
+    out["amul_y_sym"] = rm.amul(dsym, upper, None, x)
+    out["amul_y_ns"] = rm.amul(dns, uns, lns, x)
+    out["gamg_apply_r"] = x
+    out["gamg_apply_w"] = rm.gamg_apply(dsym, upper, None, x)
+    # solves
+    meta = []
+    for i, (solver, pre, repro, sym) in enumerate(CASES):
+        d, u, lo, bb = (dsym, upper, None, b) if sym else (dns, uns, lns, bns)
+        kw = dict(solver=solver, preconditioner=pre, tolerance=1e-10, rel_tol=0.0, max_iter=400, reproducible=repro)
+        xs, p = rm.solve(d, u, lo, bb, np.zeros(n), **kw)
+        out[f"solve{i}_x"] = xs
+        out[f"solve{i}_perf"] = np.array([p["initialResidual"], p["finalResidual"], p["residualSum"],
+                                          p["normFactor"], p["iterations"], p["converged"]], np.float64)
+        meta.append([solver, pre, int(repro), int(sym)])
+    out["solve_cases"] = np.array(["|".join(map(str, c)) for c in meta])
+    # residual history of two Krylov solves by re-running with maxIter = k
+    for tag, (solver, pre, repro) in {"histA": ("PCG", "diagonal", True), "histB": ("PCG", "GAMG", False)}.items():
+        _, p = rm.solve(dsym, upper, None, b, np.zeros(n), solver=solver, preconditioner=pre, tolerance=1e-10,
+                        max_iter=400, reproducible=repro)
+        hist = [p["initialResidual"]]
+        for k in range(1, min(p["iterations"], 60) + 1):
+            _, pk = rm.solve(dsym, upper, None, b, np.zeros(n), solver=solver, preconditioner=pre,
+                             tolerance=1e-10, max_iter=k, reproducible=repro)
+            hist.append(pk["finalResidual"])
+        out[tag] = np.array(hist)
+    np.savez_compressed(os.path.join(HERE, f"small_{name}.npz"), **out)
+    print(f"small_{name}: {n} cells, levels {out['gamg_sizes'].tolist()}", flush=True)
+
+
+def summary(x):
+    x = np.asarray(x)
+    idx = np.linspace(0, x.size - 1, 16).astype(np.int64)
+    return {"sha256": sha(x), "l2": float(np.sqrt(np.dot(x, x))), "sample_idx": idx.tolist(),
+            "sample": [float(v) for v in x[idx]]}
+
+
+def full_case(tag, name):
+    t0 = time.time()
+    m = W.make_mesh(name)
+    rm = ref.RefMesh(mesh=m)
+    t1 = time.time()
+    n, ni = m.n_cells, m.n_internal
+    rg = rm.geometry()
+    mg = m.geometry()
+    geom_equal = all(np.array_equal(rg[k], mg[k]) for k in ("magSf", "ndc", "w", "V", "C"))
+    lap = rm.laplacian([W.XMAX_PATCH])
+    d = lap["diag"].copy()
+    np.add.at(d, m.owner[ni:], lap["ic"])
+    b = W.pressure_rhs(n, seed=1)
+    np.add.at(b, m.owner[ni:], lap["bs"])
+    lv = rm.gamg_hierarchy()
+    res = {
+        "mesh": name, "n_cells": n, "n_internal_faces": ni, "n_faces": m.n_faces, "mesh_sha256": mesh_hash(m),
+        "geometry_bit_equal_to_meshgen": bool(geom_equal),
+        "weights_sha256": sha(m.agglomeration_weights()),
+        "diag_sha256": sha(d), "upper_sha256": sha(lap["upper"]), "b_sha256": sha(b),
+        "gamg_sizes": [int(L["n"]) for L in lv],
+        "gamg_agg_sha256": [sha(L["agg"]) for L in lv],
+        "gamg_faceMap_sha256": [sha(L["faceMap"]) for L in lv],
+        "build_s": t1 - t0,
+    }
+    for key, (pre, repro, tol) in {"repro_diag_pcg_1e-6": ("diagonal", True, 1e-6),
+                                   "gamg_pcg_1e-6": ("GAMG", False, 1e-6),
+                                   "gamg_pcg_1e-10": ("GAMG", False, 1e-10)}.items():
+        t = time.time()
+        x, p = rm.solve(d, lap["upper"], None, b, np.zeros(n), solver="PCG", preconditioner=pre, tolerance=tol,
+                        max_iter=5000, reproducible=repro)
+        res[key] = {"iterations": p["iterations"], "initialResidual": p["initialResidual"],
+                    "finalResidual": p["finalResidual"], "normFactor": p["normFactor"], "converged": p["converged"],
+                    "solve_s": time.time() - t, "x": summary(x)}
+        print(f"  {tag} {key}: {p['iterations']} iters, final {p['finalResidual']:.3e}, "
+              f"{time.time() - t:.1f} s", flush=True)
+    with open(os.path.join(HERE, f"{tag}_{name}.json"), "w") as fh:
+        json.dump(res, fh, indent=1)
+    print(f"{tag}_{name}: {n} cells, levels {res['gamg_sizes']}, geometry equal {geom_equal}", flush=True)
+
+
+def exactsum_case():
+    out = {"cases": []}
+    for seed, scale in ((1, 1.0), (2, 1e20), (3, 1e-300), (4, 1e300)):
+        v = (2.0 * W.splitmix_uniform(10007, seed) - 1.0) * scale
+        v[::7] *= 1e-12
+        out["cases"].append({"seed": seed, "scale": scale, "n": 10007, "sum": ref.exact_sum(v),
+                             "sum_reversed": ref.exact_sum(v[::-1])})
+    out["known"] = {"1e20+1-1e20": ref.exact_sum([1e20, 1.0, -1e20]), "-3.5+1.25": ref.exact_sum([-3.5, 1.25])}
+    with open(os.path.join(HERE, "exactsum.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+
+
+if __name__ == "__main__":
+    exactsum_case()
+    for nm, mk in SMALL.items():
+        small_case(nm, mk)
+    if "--small-only" not in sys.argv:
+        full_case("cfg1", "box64")
+        full_case("cfg2", "tet70")

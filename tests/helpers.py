@@ -1,0 +1,46 @@
+"""Shared test helpers: golden fixture loading and system construction."""
+import hashlib
+import json
+import os
+
+import numpy as np
+
+from voluma_amd import mesh as M
+from voluma_amd import workloads as W
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+SMALL_MESHES = {
+    "box8x6x5": lambda: M.box(8, 6, 5),
+    "tet5x4x3": lambda: M.tet(5, 4, 3, 0.2, 7),
+    "box12": lambda: M.box(12),
+}
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def mesh_hash(m):
+    h = hashlib.sha256()
+    for a in (m.points, m.face_off, m.face_pts, m.owner, m.neighbour, m.patch_start, m.patch_size):
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
+
+
+def load_small(name):
+    with np.load(os.path.join(GOLDEN, f"small_{name}.npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def load_json(name):
+    with open(os.path.join(GOLDEN, name)) as fh:
+        return json.load(fh)
+
+
+def solve_cases(g):
+    out = []
+    for i, s in enumerate(g["solve_cases"]):
+        solver, pre, repro, sym = str(s).split("|")
+        out.append((i, solver, pre, bool(int(repro)), bool(int(sym))))
+    return out

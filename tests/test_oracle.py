@@ -1,0 +1,253 @@
+"""CPU tests: the oracle (oracle/liboracle.so) pinned against the reference.
+
+Pins, per oracle function:
+* exact_sum          — reference ExactSum outputs (exactsum.json) and the known answers
+                       of the reference's own test (tests/test_mesh.cpp:13-28); permutation
+                       invariance.
+* amul               — reference amul outputs (bit-exact) and dense A·x.
+* laplacian/div      — reference fvm::laplacian / fvm::div coefficients (bit-exact) and the
+                       box closed form (upper = h, interior diag = −6h, Dirichlet −2h more).
+* gamg hierarchy     — reference buildHierarchy maps (bit-exact).
+* gamg V-cycle (GS)  — reference GamgPreconditioner::apply (bit-exact).
+* Galerkin           — dense PᵀAP.
+* solveLdu           — reference solutions and SolverPerformance for 14 solver /
+                       preconditioner / mode combinations (bit-exact), residual histories,
+                       dense numpy solve.
+* Jacobi V-cycle     — project variant: symmetry of the preconditioner and definiteness.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import SMALL_MESHES, load_json, load_small, mesh_hash, solve_cases
+from oracle import oracle as O
+from voluma_amd import workloads as W
+
+NAMES = list(SMALL_MESHES)
+
+
+# ---------------------------------------------------------------- exact sums
+def test_exact_sum_reference_known_answers():
+    # tests/test_mesh.cpp:13-28 of the reference
+    v = []
+    for i in range(10000):
+        v.append(math.ldexp(1.0 + 0.5 * ((i * 7919) % 10007) / 10007.0, (i % 133) - 66) * (1 if i % 3 else -1))
+    assert O.exact_sum(v) == O.exact_sum(v[::-1])
+    assert O.exact_sum([1e20, 1.0, -1e20]) == 1.0
+    assert O.exact_sum([-3.5, 1.25]) == -2.25
+
+
+def test_exact_sum_matches_reference_fixtures():
+    g = load_json("exactsum.json")
+    for c in g["cases"]:
+        v = (2.0 * W.splitmix_uniform(c["n"], c["seed"]) - 1.0) * c["scale"]
+        v[::7] *= 1e-12
+        assert O.exact_sum(v) == c["sum"]
+        assert O.exact_sum(v[::-1]) == c["sum_reversed"] == c["sum"]
+        rng = np.random.default_rng(c["seed"])
+        assert O.exact_sum(v[rng.permutation(v.size)]) == c["sum"]
+    assert O.exact_sum([1e20, 1.0, -1e20]) == g["known"]["1e20+1-1e20"]
+    assert O.exact_sum([-3.5, 1.25]) == g["known"]["-3.5+1.25"]
+
+
+def test_exact_sum_integers_and_nonfinite():
+    v = np.arange(-5000, 5001, dtype=np.float64) * 3.0 + 0.5
+    assert O.exact_sum(v) == float(np.sum(np.arange(-5000, 5001) * 3)) + 0.5 * v.size
+    assert math.isinf(O.exact_sum([1.0, math.inf]))
+    assert math.isnan(O.exact_sum([math.inf, -math.inf]))
+    assert O.exact_sum([]) == 0.0
+
+
+# ---------------------------------------------------------------- mesh inputs
+@pytest.mark.parametrize("name", NAMES)
+def test_mesh_generator_matches_fixture(name):
+    g = load_small(name)
+    m = SMALL_MESHES[name]()
+    assert bytes(g["mesh_hash"]).hex() == mesh_hash(m)
+    geo = m.geometry()
+    for k in ("V", "magSf", "ndc", "w"):
+        assert np.array_equal(geo[k], g["geom_" + k]), k
+    assert np.array_equal(geo["C"], g["geom_C"])
+    assert np.array_equal(geo["corr"], g["geom_corr"])
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_geometry_invariants(name):
+    m = SMALL_MESHES[name]()
+    g = m.geometry()
+    assert abs(g["V"].sum() - 1.0) < 1e-12
+    # closed cells: Σ ±S_f = 0 per cell
+    s = np.zeros((m.n_cells, 3))
+    np.add.at(s, m.owner, g["Sf"])
+    np.add.at(s, m.neighbour, -g["Sf"][: m.n_internal])
+    assert np.abs(s).max() < 1e-14
+
+
+# ---------------------------------------------------------------- assembly
+def _bc(m):
+    return W.bc_fixed_xmax(m)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_laplacian_and_div_match_reference(name):
+    g = load_small(name)
+    m = SMALL_MESHES[name]()
+    d, up, ic, bs = O.laplacian(m, _bc(m))
+    assert np.array_equal(d, g["lap_diag"]) and np.array_equal(up, g["lap_upper"])
+    assert np.array_equal(ic, g["lap_ic"]) and np.array_equal(bs, g["lap_bs"])
+    dd, uu, ll, ic2, bs2 = O.div_upwind(m, g["div_flux"], _bc(m))
+    for k, v in (("diag", dd), ("upper", uu), ("lower", ll), ("ic", ic2), ("bs", bs2)):
+        assert np.array_equal(v, g["div_" + k]), k
+    # fold: FvMatrix::solve order
+    fd, fb = O.fold(m, d, np.zeros(m.n_cells), ic, bs)
+    assert np.array_equal(fd, g["sym_diag"])
+
+
+def test_laplacian_box_closed_form():
+    m = SMALL_MESHES["box12"]()
+    h = 1.0 / 12
+    d, up, ic, bs = O.laplacian(m, _bc(m))
+    assert np.allclose(up, h, rtol=1e-13, atol=0)
+    fd, _ = O.fold(m, d, np.zeros(m.n_cells), ic, bs)
+    # diag = −Σ off-diagonals − 2h on Dirichlet (x = 1) cells
+    off = np.zeros(m.n_cells)
+    np.add.at(off, m.owner[: m.n_internal], up)
+    np.add.at(off, m.neighbour, up)
+    dirichlet = np.zeros(m.n_cells, bool)
+    dirichlet[m.owner[m.patch_faces(1)]] = True
+    assert np.allclose(fd, -off - 2 * h * dirichlet, rtol=1e-13, atol=0)
+    A1 = O.LduMesh(m.n_cells, m.owner[: m.n_internal], m.neighbour).amul(fd, up, None, np.ones(m.n_cells))
+    assert np.allclose(A1, -2 * h * dirichlet, atol=1e-14)
+
+
+def test_nonorth_correction_zero_for_uniform_gradient_on_orthogonal_box():
+    m = SMALL_MESHES["box12"]()
+    g = np.tile([1.0, 2.0, 3.0], (m.n_cells, 1))
+    src, ffc = O.laplacian_nonorth(m, g)
+    assert np.abs(src).max() < 1e-15 and np.abs(ffc).max() < 1e-15
+
+
+# ---------------------------------------------------------------- amul / GAMG
+def _dense(n, own, nei, diag, upper, lower):
+    A = np.diag(diag).astype(float)
+    for f in range(len(own)):
+        A[own[f], nei[f]] += upper[f]
+        A[nei[f], own[f]] += lower[f]
+    return A
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_amul_matches_reference_and_dense(name):
+    g = load_small(name)
+    m = SMALL_MESHES[name]()
+    om = O.LduMesh(m.n_cells, m.owner[: m.n_internal], m.neighbour)
+    x = g["amul_x"]
+    y = om.amul(g["sym_diag"], g["sym_upper"], None, x)
+    assert np.array_equal(y, g["amul_y_sym"])
+    y2 = om.amul(g["ns_diag"], g["ns_upper"], g["ns_lower"], x)
+    assert np.array_equal(y2, g["amul_y_ns"])
+    A = _dense(m.n_cells, m.owner[: m.n_internal], m.neighbour, g["ns_diag"], g["ns_upper"], g["ns_lower"])
+    assert np.allclose(A @ x, y2, rtol=1e-13, atol=1e-15)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_gamg_hierarchy_matches_reference(name):
+    g = load_small(name)
+    m = SMALL_MESHES[name]()
+    om = O.LduMesh(m.n_cells, m.owner[: m.n_internal], m.neighbour, m.agglomeration_weights())
+    lv = om.gamg_levels()
+    assert [L["n"] for L in lv] == g["gamg_sizes"].tolist()
+    for key in ("agg", "l", "u", "weight"):
+        assert np.array_equal(np.concatenate([L[key] for L in lv]), g["gamg_" + key]), key
+    nl = len(lv)
+    fm = np.concatenate([L["faceMap"] for L in lv[: nl - 1]])
+    assert np.array_equal(fm, g["gamg_faceMap"][: fm.size])
+    # the GS V-cycle of the reference
+    w = om.gamg_apply(g["sym_diag"], g["sym_upper"], None, g["gamg_apply_r"], smoother=O.SMOOTHER_GS)
+    assert np.array_equal(w, g["gamg_apply_w"])
+
+
+@pytest.mark.parametrize("name", ["box8x6x5", "tet5x4x3"])
+def test_galerkin_is_PtAP(name):
+    g = load_small(name)
+    m = SMALL_MESHES[name]()
+    om = O.LduMesh(m.n_cells, m.owner[: m.n_internal], m.neighbour, m.agglomeration_weights())
+    lv = om.gamg_levels()
+    own, nei = m.owner[: m.n_internal], m.neighbour
+    A = _dense(m.n_cells, own, nei, g["ns_diag"], g["ns_upper"], g["ns_lower"])
+    for k in range(len(lv) - 1):
+        P = np.zeros((lv[k]["n"], lv[k + 1]["n"]))
+        P[np.arange(lv[k]["n"]), lv[k]["agg"]] = 1.0
+        Ac = P.T @ A @ P
+        cd, cu, cl = om.gamg_coeffs(g["ns_diag"], g["ns_upper"], g["ns_lower"], k + 1)
+        C = _dense(lv[k + 1]["n"], lv[k + 1]["l"], lv[k + 1]["u"], cd, cu, cl)
+        assert np.allclose(C, Ac, rtol=1e-12, atol=1e-14)
+        A = Ac
+
+
+@pytest.mark.parametrize("name", ["box12", "tet5x4x3"])
+def test_jacobi_vcycle_symmetric_and_definite(name):
+    g = load_small(name)
+    m = SMALL_MESHES[name]()
+    om = O.LduMesh(m.n_cells, m.owner[: m.n_internal], m.neighbour, m.agglomeration_weights())
+    rng = np.random.default_rng(0)
+    for pre, post in ((2, 2), (1, 1)):
+        r1, r2 = rng.standard_normal(m.n_cells), rng.standard_normal(m.n_cells)
+        ap = lambda r: om.gamg_apply(g["sym_diag"], g["sym_upper"], None, r, smoother=O.SMOOTHER_JACOBI,
+                                     omega=0.8, n_pre=pre, n_post=post)
+        a, b = np.dot(ap(r1), r2), np.dot(r1, ap(r2))
+        assert abs(a - b) <= 1e-11 * (abs(a) + abs(b))
+        for _ in range(3):
+            r = rng.standard_normal(m.n_cells)
+            assert np.dot(r, ap(r)) < 0  # A is negative definite, so is the preconditioner
+
+
+# ---------------------------------------------------------------- solveLdu
+@pytest.mark.parametrize("name", NAMES)
+def test_solves_match_reference_bit_exact(name):
+    g = load_small(name)
+    m = SMALL_MESHES[name]()
+    om = O.LduMesh(m.n_cells, m.owner[: m.n_internal], m.neighbour, m.agglomeration_weights())
+    for i, solver, pre, repro, sym in solve_cases(g):
+        d, u, lo, b = ((g["sym_diag"], g["sym_upper"], None, g["sym_b"]) if sym else
+                       (g["ns_diag"], g["ns_upper"], g["ns_lower"], g["ns_b"]))
+        x, p = om.solve(d, u, lo, b, solver=solver, preconditioner=pre, tolerance=1e-10, max_iter=400,
+                        mode=O.MODE_REPRODUCIBLE if repro else O.MODE_PLAIN)
+        ref = g[f"solve{i}_perf"]
+        got = np.array([p["initialResidual"], p["finalResidual"], p["residualSum"], p["normFactor"],
+                        p["iterations"], p["converged"]])
+        assert np.array_equal(got, ref), (solver, pre, repro)
+        assert np.array_equal(x, g[f"solve{i}_x"]), (solver, pre, repro)
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_residual_histories_match_reference(name):
+    g = load_small(name)
+    m = SMALL_MESHES[name]()
+    om = O.LduMesh(m.n_cells, m.owner[: m.n_internal], m.neighbour, m.agglomeration_weights())
+    for tag, pre, mode in (("histA", "diagonal", O.MODE_REPRODUCIBLE), ("histB", "GAMG", O.MODE_PLAIN)):
+        _, _, hist = om.solve(g["sym_diag"], g["sym_upper"], None, g["sym_b"], preconditioner=pre, tolerance=1e-10,
+                              max_iter=400, mode=mode, history=True)
+        ref = g[tag]
+        assert np.array_equal(hist[: ref.size], ref)
+
+
+def test_solve_against_dense_and_error_paths():
+    g = load_small("tet5x4x3")
+    m = SMALL_MESHES["tet5x4x3"]()
+    own, nei = m.owner[: m.n_internal], m.neighbour
+    om = O.LduMesh(m.n_cells, own, nei, m.agglomeration_weights())
+    A = _dense(m.n_cells, own, nei, g["ns_diag"], g["ns_upper"], g["ns_lower"])
+    x, p = om.solve(g["ns_diag"], g["ns_upper"], g["ns_lower"], g["ns_b"], solver="PBiCGStab",
+                    preconditioner="diagonal", tolerance=1e-14, max_iter=1000)
+    assert np.allclose(x, np.linalg.solve(A, g["ns_b"]), rtol=1e-9, atol=1e-12)
+    # x0 = 0: initial residual = Σ|b| / (Σ|b| + 1e-20)
+    sb = np.abs(g["ns_b"]).sum()
+    assert p["normFactor"] == pytest.approx(sb, rel=1e-14)
+    with pytest.raises(O.OracleError, match="symmetric"):
+        om.solve(g["ns_diag"], g["ns_upper"], g["ns_lower"], g["ns_b"], solver="PCG")
+    with pytest.raises(O.OracleError, match="unknown linear solver"):
+        om.solve(g["sym_diag"], g["sym_upper"], None, g["sym_b"], solver="CG")
+    with pytest.raises(O.OracleError, match="unknown preconditioner"):
+        om.solve(g["sym_diag"], g["sym_upper"], None, g["sym_b"], preconditioner="ILU0")

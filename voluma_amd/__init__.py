@@ -1,0 +1,370 @@
+"""voluma_amd — MI355X (gfx950) LDU linear-solve path with Voluma's linalg interface.
+
+Python mirror of the reference's solver API (src/fvm/linalg/LinearSolver.h:15-45),
+bound through the C ABI of include/voluma_amd.h (``libvoluma_amd.so``):
+
+* :class:`SolverControls` / :class:`SolverPerformance` — same fields and defaults;
+* :class:`LduMesh` — owner/neighbour addressing and the cached GAMG hierarchy;
+* :class:`LduMatrix` — diag (boundary-folded) / upper / lower resident on the GPU;
+* :func:`solve_ldu` (``solveLdu``) and :func:`amul`.
+
+The native library is required: importing the solver API without it raises.
+Mesh synthesis lives in :mod:`voluma_amd.mesh`, benchmark systems in
+:mod:`voluma_amd.workloads`.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libvoluma_amd.so")
+
+VOL_OK = 0
+VOL_HOST, VOL_DEVICE = 0, 1
+SUMS_CONTEXT, SUMS_FAST, SUMS_REPRODUCIBLE, SUMS_EXACT = -1, 0, 1, 2
+FP64 = 0
+
+_ERRORS = {
+    -1: "invalid argument", -2: "unknown solver", -3: "unknown preconditioner", -4: "PCG on nonsymmetric matrix",
+    -5: "singular coarsest matrix", -6: "unsupported", -7: "HIP error", -8: "no GAMG weights",
+}
+
+
+class VolumaError(RuntimeError):
+    """A non-zero status of the C ABI (the reference would have thrown)."""
+
+    def __init__(self, code, msg):
+        super().__init__(f"[{code} {_ERRORS.get(code, '?')}] {msg}")
+        self.code = code
+
+
+class _Controls(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_char_p), ("preconditioner", ctypes.c_char_p), ("tolerance", ctypes.c_double),
+                ("relTol", ctypes.c_double), ("maxIter", ctypes.c_int), ("minIter", ctypes.c_int),
+                ("nSweeps", ctypes.c_int), ("sums", ctypes.c_int), ("precision", ctypes.c_int),
+                ("gamgPreSweeps", ctypes.c_int), ("gamgPostSweeps", ctypes.c_int), ("gamgOmega", ctypes.c_double)]
+
+
+class _Perf(ctypes.Structure):
+    _fields_ = [("solver", ctypes.c_char * 32), ("initialResidual", ctypes.c_double),
+                ("finalResidual", ctypes.c_double), ("residualSum", ctypes.c_double),
+                ("normFactor", ctypes.c_double), ("iterations", ctypes.c_int), ("converged", ctypes.c_int)]
+
+
+_LIB = None
+EXPORTED = ("vol_controls_default", "vol_last_error", "vol_context_create", "vol_context_destroy",
+            "vol_set_reproducible", "vol_synchronize", "vol_device_alloc", "vol_device_free", "vol_memcpy_to_device",
+            "vol_memcpy_to_host", "vol_mesh_create", "vol_mesh_destroy", "vol_mesh_gamg_levels",
+            "vol_mesh_gamg_level", "vol_matrix_create", "vol_matrix_set", "vol_matrix_destroy", "vol_amul",
+            "vol_solve", "vol_timing_enable", "vol_timing_read", "vol_mesh_set_boundary", "vol_assemble_laplacian",
+            "vol_assemble_div_upwind", "vol_matrix_get")
+
+
+def lib():
+    """The loaded native library (raises if it is not built)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run __graft_entry__.build() (hipcc, gfx950)")
+        L = ctypes.CDLL(LIB_PATH)
+        P, I, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.vol_last_error.restype = ctypes.c_char_p
+        L.vol_controls_default.argtypes = [ctypes.POINTER(_Controls)]
+        L.vol_context_create.argtypes = [I, ctypes.POINTER(P)]
+        L.vol_context_destroy.argtypes = [P]
+        L.vol_set_reproducible.argtypes = [P, I]
+        L.vol_synchronize.argtypes = [P]
+        L.vol_device_alloc.argtypes = [P, ctypes.c_uint64, ctypes.POINTER(P)]
+        L.vol_device_free.argtypes = [P, P]
+        L.vol_memcpy_to_device.argtypes = [P, P, P, ctypes.c_uint64]
+        L.vol_memcpy_to_host.argtypes = [P, P, P, ctypes.c_uint64]
+        L.vol_mesh_create.argtypes = [P, I, I, P, P, P, I, ctypes.POINTER(P)]
+        L.vol_mesh_destroy.argtypes = [P]
+        L.vol_mesh_gamg_levels.argtypes = [P, ctypes.POINTER(I), P]
+        L.vol_mesh_gamg_level.argtypes = [P, I, ctypes.POINTER(I), P, P, P]
+        L.vol_matrix_create.argtypes = [P, ctypes.POINTER(P)]
+        L.vol_matrix_set.argtypes = [P, P, P, P, I]
+        L.vol_matrix_destroy.argtypes = [P]
+        L.vol_amul.argtypes = [P, P, P, I]
+        L.vol_solve.argtypes = [P, P, P, ctypes.POINTER(_Controls), ctypes.POINTER(_Perf), I]
+        L.vol_timing_enable.argtypes = [P, I]
+        L.vol_timing_read.argtypes = [P, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_int64), I]
+        L.vol_mesh_set_boundary.argtypes = [P, I, P]
+        L.vol_matrix_get.argtypes = [P, P, P, P, I]
+        L.vol_assemble_laplacian.argtypes = [P, D, P, P, P, P, P, P, P, P, P, I]
+        L.vol_assemble_div_upwind.argtypes = [P, P, P, P, P, P, P, P, I]
+        for name in EXPORTED:
+            if name not in ("vol_controls_default", "vol_last_error"):
+                getattr(L, name).restype = I
+        _LIB = L
+    return _LIB
+
+
+def _check(rc):
+    if rc != VOL_OK:
+        raise VolumaError(rc, lib().vol_last_error().decode())
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+@dataclasses.dataclass
+class SolverControls:
+    """SolverControls (LinearSolver.h:15-23) + GPU extensions (sums, GAMG smoother)."""
+    solver: str = "PCG"
+    preconditioner: str = "diagonal"
+    tolerance: float = 1e-8
+    relTol: float = 0.0
+    maxIter: int = 1000
+    minIter: int = 0
+    nSweeps: int = 1
+    sums: int = SUMS_CONTEXT
+    precision: int = FP64
+    gamgPreSweeps: int = 2
+    gamgPostSweeps: int = 2
+    gamgOmega: float = 0.8
+
+    def _c(self):
+        self._keep = (self.solver.encode(), self.preconditioner.encode())
+        return _Controls(self._keep[0], self._keep[1], self.tolerance, self.relTol, self.maxIter, self.minIter,
+                         self.nSweeps, self.sums, self.precision, self.gamgPreSweeps, self.gamgPostSweeps,
+                         self.gamgOmega)
+
+
+@dataclasses.dataclass
+class SolverPerformance:
+    """SolverPerformance (LinearSolver.h:25-31)."""
+    solver: str = ""
+    field: str = ""
+    initialResidual: float = 0.0
+    finalResidual: float = 0.0
+    residualSum: float = 0.0
+    normFactor: float = 0.0
+    iterations: int = 0
+    converged: bool = False
+
+
+class Context:
+    """One GPU and one HIP stream (vol_context)."""
+
+    def __init__(self, device=0):
+        h = ctypes.c_void_p()
+        _check(lib().vol_context_create(int(device), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().vol_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_reproducible(self, on=True):
+        _check(lib().vol_set_reproducible(self._h, int(bool(on))))
+
+    def synchronize(self):
+        _check(lib().vol_synchronize(self._h))
+
+    def timing(self, on=True):
+        _check(lib().vol_timing_enable(self._h, int(bool(on))))
+
+    def read_timing(self, reset=True):
+        ms, cnt = ctypes.c_double(), ctypes.c_int64()
+        _check(lib().vol_timing_read(self._h, ctypes.byref(ms), ctypes.byref(cnt), int(bool(reset))))
+        return ms.value, cnt.value
+
+
+class DeviceArray:
+    """float64 buffer in the context's GPU memory (for VOL_DEVICE calls)."""
+
+    def __init__(self, ctx: Context, n: int, dtype=np.float64):
+        self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        p = ctypes.c_void_p()
+        _check(lib().vol_device_alloc(ctx._h, self.n * self.dtype.itemsize, ctypes.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, ctx, a, dtype=np.float64):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(ctx, a.size, dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        _check(lib().vol_memcpy_to_device(self.ctx._h, self.ptr, _p(a), a.nbytes))
+
+    def download(self):
+        out = np.empty(self.n, self.dtype)
+        _check(lib().vol_memcpy_to_host(self.ctx._h, _p(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib().vol_device_free(self.ctx._h, self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.free()
+
+
+class LduMesh:
+    """LDU addressing (internal faces) and, with weights, the GAMG hierarchy (vol_mesh)."""
+
+    def __init__(self, ctx: Context, n_cells, owner, neighbour, gamg_weights=None, merge_levels=2):
+        self.ctx = ctx
+        own = np.ascontiguousarray(owner, dtype=np.int32)
+        nei = np.ascontiguousarray(neighbour, dtype=np.int32)
+        if own.size != nei.size:
+            raise ValueError("owner/neighbour sizes differ")
+        self.n_cells, self.n_faces = int(n_cells), int(own.size)
+        w = None if gamg_weights is None else np.ascontiguousarray(gamg_weights, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _check(lib().vol_mesh_create(ctx._h, self.n_cells, self.n_faces, _p(own), _p(nei),
+                                     None if w is None else _p(w), int(merge_levels), ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().vol_mesh_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+    def set_boundary(self, boundary_owner):
+        """Owner cells of the boundary faces (Mesh::owner past nInternalFaces), for assembly."""
+        bo = np.ascontiguousarray(boundary_owner, dtype=np.int32)
+        _check(lib().vol_mesh_set_boundary(self._h, int(bo.size), _p(bo)))
+        self.n_boundary = int(bo.size)
+
+    def gamg_levels(self):
+        """[{n, agg, faceMap, faceFlip}] per level, as GamgHierarchy::levels."""
+        nl = ctypes.c_int()
+        sizes = np.zeros(40, np.int32)
+        _check(lib().vol_mesh_gamg_levels(self._h, ctypes.byref(nl), _p(sizes)))
+        out = []
+        for k in range(nl.value):
+            nf = ctypes.c_int()
+            _check(lib().vol_mesh_gamg_level(self._h, k, ctypes.byref(nf), None, None, None))
+            agg = np.zeros(int(sizes[k]), np.int32)
+            fm = np.zeros(max(nf.value, 1), np.int32)
+            ff = np.zeros(max(nf.value, 1), np.int32)
+            _check(lib().vol_mesh_gamg_level(self._h, k, ctypes.byref(nf), _p(agg), _p(fm), _p(ff)))
+            out.append({"n": int(sizes[k]), "agg": agg, "faceMap": fm[: nf.value], "faceFlip": ff[: nf.value]})
+        return out
+
+
+class LduMatrix:
+    """Coefficients over an LduMesh, resident on the GPU (vol_matrix)."""
+
+    def __init__(self, mesh: LduMesh, diag=None, upper=None, lower=None):
+        self.mesh = mesh
+        h = ctypes.c_void_p()
+        _check(lib().vol_matrix_create(mesh._h, ctypes.byref(h)))
+        self._h = h
+        if diag is not None:
+            self.set(diag, upper, lower)
+
+    def set(self, diag, upper, lower=None):
+        """diag must already include the boundary contributions; lower None → symmetric."""
+        if isinstance(diag, DeviceArray):
+            _check(lib().vol_matrix_set(self._h, diag.ptr, upper.ptr, None if lower is None else lower.ptr,
+                                        VOL_DEVICE))
+            self.symmetric = lower is None
+            return
+        d = np.ascontiguousarray(diag, dtype=np.float64)
+        u = np.ascontiguousarray(upper, dtype=np.float64)
+        lo = None if lower is None else np.ascontiguousarray(lower, dtype=np.float64)
+        if d.size != self.mesh.n_cells or u.size != self.mesh.n_faces or (lo is not None and lo.size != u.size):
+            raise ValueError("coefficient sizes do not match the mesh")
+        _check(lib().vol_matrix_set(self._h, _p(d), _p(u), None if lo is None else _p(lo), VOL_HOST))
+        self.symmetric = lo is None
+
+    def get(self):
+        """(diag, upper, lower) as held on the GPU (lower == upper when symmetric)."""
+        m = self.mesh
+        d, u, lo = np.zeros(m.n_cells), np.zeros(max(m.n_faces, 1)), np.zeros(max(m.n_faces, 1))
+        _check(lib().vol_matrix_get(self._h, _p(d), _p(u), _p(lo), VOL_HOST))
+        return d, u[: m.n_faces], lo[: m.n_faces]
+
+    def assemble_laplacian(self, magSf, ndc, bc_fixed, bc_value=None, gamma=1.0, face_gamma=None, b_in=None,
+                           return_boundary=False):
+        """fvm::laplacian(γ, ψ) + boundary fold on the GPU; returns the folded source b
+        (b_in + boundary source) and optionally (internalCoeffs, boundarySource)."""
+        m = self.mesh
+        f64 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        bcf = np.ascontiguousarray(bc_fixed, dtype=np.int32)
+        bv, fg, bi = f64(bc_value), f64(face_gamma), f64(b_in)
+        ms, nd = f64(magSf), f64(ndc)
+        b = np.zeros(m.n_cells)
+        ic = np.zeros(max(m.n_boundary, 1))
+        bs = np.zeros(max(m.n_boundary, 1))
+        q = lambda a: None if a is None else _p(a)
+        _check(lib().vol_assemble_laplacian(self._h, float(gamma), q(fg), _p(ms), _p(nd), _p(bcf), q(bv), q(bi),
+                                            _p(b), _p(ic), _p(bs), VOL_HOST))
+        self.symmetric = True
+        if return_boundary:
+            return b, ic[: m.n_boundary], bs[: m.n_boundary]
+        return b
+
+    def assemble_div_upwind(self, flux, bc_fixed, bc_value=None, b_in=None, return_boundary=False):
+        """fvm::div(F, ψ, upwind) + boundary fold on the GPU (nonsymmetric)."""
+        m = self.mesh
+        f64 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        bcf = np.ascontiguousarray(bc_fixed, dtype=np.int32)
+        F, bv, bi = f64(flux), f64(bc_value), f64(b_in)
+        b = np.zeros(m.n_cells)
+        ic = np.zeros(max(m.n_boundary, 1))
+        bs = np.zeros(max(m.n_boundary, 1))
+        q = lambda a: None if a is None else _p(a)
+        _check(lib().vol_assemble_div_upwind(self._h, _p(F), _p(bcf), q(bv), q(bi), _p(b), _p(ic), _p(bs), VOL_HOST))
+        self.symmetric = False
+        if return_boundary:
+            return b, ic[: m.n_boundary], bs[: m.n_boundary]
+        return b
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().vol_matrix_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+def amul(A: LduMatrix, x, y=None):
+    """y = A x (LinearSolver.cpp:121-133). numpy in → numpy out, DeviceArray in → in place."""
+    if isinstance(x, DeviceArray):
+        _check(lib().vol_amul(A._h, x.ptr, y.ptr, VOL_DEVICE))
+        return y
+    xx = np.ascontiguousarray(x, dtype=np.float64)
+    yy = np.empty_like(xx)
+    _check(lib().vol_amul(A._h, _p(xx), _p(yy), VOL_HOST))
+    return yy
+
+
+def solve_ldu(A: LduMatrix, b, x, controls: SolverControls = None, field="") -> SolverPerformance:
+    """solveLdu (LinearSolver.cpp:135-285): x (numpy float64 or DeviceArray) is updated in place."""
+    controls = controls or SolverControls()
+    c = controls._c()
+    pf = _Perf()
+    if isinstance(x, DeviceArray):
+        _check(lib().vol_solve(A._h, b.ptr, x.ptr, ctypes.byref(c), ctypes.byref(pf), VOL_DEVICE))
+    else:
+        if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.c_contiguous):
+            raise TypeError("x must be a contiguous float64 numpy array (updated in place)")
+        bb = np.ascontiguousarray(b, dtype=np.float64)
+        _check(lib().vol_solve(A._h, _p(bb), _p(x), ctypes.byref(c), ctypes.byref(pf), VOL_HOST))
+    return SolverPerformance(pf.solver.decode(), field, pf.initialResidual, pf.finalResidual, pf.residualSum,
+                             pf.normFactor, pf.iterations, bool(pf.converged))
+
+
+solveLdu = solve_ldu

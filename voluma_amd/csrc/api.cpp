@@ -1,0 +1,1235 @@
+// Host side (C++20) of the LDU linear-solve library: handles, setup and the Krylov
+// drivers, exported through the C ABI of include/voluma_amd.h.
+//
+// The solve loop keeps every scalar on the device (DevScalars): each reduction is a
+// kernel writing per-block partials plus a one-block finalize that performs the
+// solver's scalar step (α, β, convergence test) and raises a `done` flag; all later
+// kernels of the iteration test that flag and exit. The host therefore enqueues
+// iteration k+1 before waiting for iteration k, and never stalls the GPU.
+#include "../../include/voluma_amd.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "vol_internal.hpp"
+
+using namespace vol;
+
+namespace {
+
+thread_local std::string gLastError;
+
+struct VolError : std::runtime_error {
+    int code;
+    VolError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+void hipCheck(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw VolError(VOL_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+#define HIPCHECK(x) hipCheck((x), #x)
+
+template <class T> struct DevBuf {
+    T* p = nullptr;
+    size_t n = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    void alloc(size_t count) {
+        if (count <= n && p) return;
+        release();
+        HIPCHECK(hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1)));
+        n = count;
+    }
+    void upload(const std::vector<T>& v, hipStream_t st) {
+        alloc(v.size());
+        if (!v.empty()) HIPCHECK(hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+    }
+};
+
+struct SellDev {
+    int n = 0, nslices = 0, nslots = 0;
+    DevBuf<int> sliceOff, col, src;
+    SellView view() const {
+        SellView v;
+        v.n = n;
+        v.nslices = nslices;
+        v.nslots = nslots;
+        v.sliceOff = sliceOff.p;
+        v.col = col.p;
+        v.src = src.p;
+        return v;
+    }
+};
+
+// Row lists (row → (col, src) in the given order) → SELL-64 on the device.
+void buildSell(int n, const std::vector<int>& off, const std::vector<int>& colL, const std::vector<int>& srcL,
+               SellDev& S, hipStream_t st) {
+    S.n = n;
+    S.nslices = (n + WAVE - 1) / WAVE;
+    std::vector<int> sliceOff(S.nslices + 1, 0);
+    for (int s = 0; s < S.nslices; ++s) {
+        int w = 0;
+        for (int r = s * WAVE; r < std::min(n, (s + 1) * WAVE); ++r) w = std::max(w, off[r + 1] - off[r]);
+        sliceOff[s + 1] = sliceOff[s] + w * WAVE;
+    }
+    S.nslots = sliceOff[S.nslices];
+    std::vector<int> col(S.nslots, -1), src(S.nslots, -1);
+    for (int r = 0; r < n; ++r) {
+        const int s = r / WAVE, lane = r % WAVE;
+        for (int k = off[r]; k < off[r + 1]; ++k) {
+            const int idx = sliceOff[s] + (k - off[r]) * WAVE + lane;
+            col[idx] = colL[k];
+            src[idx] = srcL[k];
+        }
+    }
+    S.sliceOff.upload(sliceOff, st);
+    S.col.upload(col, st);
+    S.src.upload(src, st);
+}
+
+// ------------------------------------------------------------------ GAMG hierarchy (host setup)
+// Pairwise agglomeration on geometric weights, same decisions as the reference's
+// agglomerate/buildHierarchy (src/fvm/linalg/GAMG.cpp:40-94, 206-252): cells visited in
+// index order, pair with the unaggregated neighbour of strictly largest weight (first
+// in adjacency order on ties), otherwise join the strongest neighbour's aggregate;
+// coarse faces numbered by first appearance in fine-face order, weights summed in that
+// order; mergeLevels pairings per level; stop at ≤ 64 cells or when no progress.
+struct HLevel {
+    int n = 0;
+    std::vector<int> l, u;
+    std::vector<double> weight;
+    std::vector<int> off, adjCell, adjFace;
+    std::vector<char> adjIsL;
+    std::vector<int> agg, faceMap;
+    std::vector<char> faceFlip;
+
+    void adjacency() {
+        const int nf = int(l.size());
+        off.assign(n + 1, 0);
+        for (int f = 0; f < nf; ++f) {
+            ++off[l[f] + 1];
+            ++off[u[f] + 1];
+        }
+        for (int i = 0; i < n; ++i) off[i + 1] += off[i];
+        adjCell.resize(off[n]);
+        adjFace.resize(off[n]);
+        adjIsL.resize(off[n]);
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (int f = 0; f < nf; ++f) {
+            int k = fill[l[f]]++;
+            adjCell[k] = u[f];
+            adjFace[k] = f;
+            adjIsL[k] = 1;
+            k = fill[u[f]]++;
+            adjCell[k] = l[f];
+            adjFace[k] = f;
+            adjIsL[k] = 0;
+        }
+    }
+
+    HLevel pair() {
+        agg.assign(n, -1);
+        int nc = 0;
+        for (int i = 0; i < n; ++i) {
+            if (agg[i] >= 0) continue;
+            int best = -1;
+            double bw = -1;
+            for (int k = off[i]; k < off[i + 1]; ++k) {
+                const int j = adjCell[k];
+                const double wk = weight[adjFace[k]];
+                if (agg[j] < 0 && j != i && wk > bw) {
+                    bw = wk;
+                    best = j;
+                }
+            }
+            if (best >= 0) {
+                agg[i] = agg[best] = nc++;
+                continue;
+            }
+            for (int k = off[i]; k < off[i + 1]; ++k) {
+                const double wk = weight[adjFace[k]];
+                if (wk > bw) {
+                    bw = wk;
+                    best = adjCell[k];
+                }
+            }
+            agg[i] = best >= 0 ? agg[best] : nc++;
+        }
+        HLevel C;
+        C.n = nc;
+        const int nf = int(l.size());
+        faceMap.assign(nf, -1);
+        faceFlip.assign(nf, 0);
+        std::unordered_map<uint64_t, int> ids;
+        ids.reserve(size_t(nf) / 2 + 16);
+        for (int f = 0; f < nf; ++f) {
+            const int a = agg[l[f]], b = agg[u[f]];
+            if (a == b) continue;
+            const int lo = std::min(a, b), hi = std::max(a, b);
+            const uint64_t key = (uint64_t(uint32_t(lo)) << 32) | uint32_t(hi);
+            auto [it, fresh] = ids.try_emplace(key, int(C.l.size()));
+            if (fresh) {
+                C.l.push_back(lo);
+                C.u.push_back(hi);
+                C.weight.push_back(0.0);
+            }
+            faceMap[f] = it->second;
+            faceFlip[f] = char(a != lo);
+            C.weight[it->second] += weight[f];
+        }
+        return C;
+    }
+};
+
+std::vector<HLevel> buildHierarchy(int n, int nf, const int* own, const int* nei, const double* w, int mergeLevels) {
+    std::vector<HLevel> H;
+    HLevel L0;
+    L0.n = n;
+    L0.l.resize(nf);
+    L0.u.resize(nf);
+    L0.weight.assign(w, w + nf);
+    for (int f = 0; f < nf; ++f) {
+        L0.l[f] = std::min(own[f], nei[f]);
+        L0.u[f] = std::max(own[f], nei[f]);
+    }
+    L0.adjacency();
+    H.push_back(std::move(L0));
+    const int nCoarsest = 64;
+    while (H.size() < 40) {
+        HLevel& F = H.back();
+        const int nF = F.n;
+        if (nF <= nCoarsest) break;
+        HLevel C = F.pair();
+        for (int k = 1; k < mergeLevels && C.n > nCoarsest; ++k) {
+            C.adjacency();
+            HLevel C2 = C.pair();
+            for (auto& a : F.agg) a = C.agg[a];
+            for (size_t f = 0; f < F.faceMap.size(); ++f) {
+                const int cf = F.faceMap[f];
+                if (cf < 0) continue;
+                F.faceMap[f] = C.faceMap[cf];
+                F.faceFlip[f] = char(F.faceFlip[f] != C.faceFlip[cf]);
+            }
+            C = std::move(C2);
+        }
+        if (C.n >= nF) {
+            F.agg.clear();
+            break;
+        }
+        C.adjacency();
+        H.push_back(std::move(C));
+    }
+    return H;
+}
+
+struct LevelDev {
+    int n = 0, nf = 0;
+    SellDev sell;                       // coarse levels (level 0 uses the mesh SELL)
+    DevBuf<int> agg;                    // → next level
+    DevBuf<int> mOff, mList;            // next-level cell → its fine cells (ascending)
+    DevBuf<int> iOff, iList;            // next-level cell → intra-aggregate fine faces (ascending)
+    DevBuf<int> cfOff, cfList;          // next-level face → (fine face << 1 | flip), ascending
+    DevBuf<int> l, u;                   // faces (coarsest: dense assembly)
+};
+
+}  // namespace
+
+// ============================================================================ handles
+struct vol_context_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bool reproducible = false;
+    RedScratch red;
+    DevBuf<double> part, result;
+    DevBuf<long long> xpart;
+    DevBuf<int> xnf;
+    DevBuf<DevScalars> scal;
+    DevScalars* hostScal = nullptr;  // pinned, 2 slots
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> evPool;
+    size_t evUsed = 0;
+    double amulMs = 0;
+    int64_t amulCount = 0;
+
+    void ensureScratch(int blocks) {
+        const int need = std::max(blocks, RED_BLOCKS);
+        if (red.maxBlk < need) {
+            part.alloc(size_t(MAX_RED) * need);
+            red.part = part.p;
+            red.maxBlk = need;
+        }
+        if (!red.xpart) {
+            xpart.alloc(size_t(MAX_RED) * RED_BLOCKS_EXACT * NBINS);
+            xnf.alloc(size_t(MAX_RED) * RED_BLOCKS_EXACT);
+            result.alloc(MAX_RED);
+            red.xpart = xpart.p;
+            red.xnf = xnf.p;
+            red.result = result.p;
+            red.maxBlkExact = RED_BLOCKS_EXACT;
+        }
+    }
+};
+
+struct vol_mesh_s {
+    vol_context ctx = nullptr;
+    int n = 0, nf = 0;
+    std::vector<int> own, nei;
+    SellDev sell;  // level 0 / Amul
+    bool hasGamg = false;
+    std::vector<HLevel> H;  // host hierarchy (kept for inspection)
+    std::vector<std::unique_ptr<LevelDev>> lv;
+    DevBuf<char> ownerIsL;  // level-0 orientation
+    // boundary faces (for assembly): cell → boundary-face CSR
+    int nb = -1;
+    DevBuf<int> bOff, bList;
+};
+
+struct MatLevel {
+    DevBuf<double> diag, Fu, Fl, coef, bufA, bufB, r, x;
+};
+
+struct vol_matrix_s {
+    vol_mesh mesh = nullptr;
+    bool symmetric = true;
+    bool set = false;
+    DevBuf<double> diag, upper, lower, coef, rD;
+    // GAMG coefficients per level, built lazily after each vol_matrix_set
+    bool gamgReady = false;
+    std::vector<std::unique_ptr<MatLevel>> L;
+    DevBuf<double> lu;
+    int nCoarse = 0;
+    DevBuf<int> status;
+    // solve work vectors
+    DevBuf<double> vx, vb, vr, vw, vp, vq, wA, pA, r0, y, Ay, s, z, t;
+    DevBuf<double> hx, hy;  // staging for host-pointer calls
+    DevBuf<double> st0, st1, st2, st3, st4, st5;  // assembly staging
+    DevBuf<int> sti;
+};
+
+namespace {
+
+int fail(const VolError& e) {
+    gLastError = e.what();
+    return e.code;
+}
+int failStd(const std::exception& e) {
+    gLastError = e.what();
+    return VOL_ERR_INVALID_ARGUMENT;
+}
+
+#define API_BEGIN try {
+#define API_END                          \
+    }                                    \
+    catch (const VolError& e) {          \
+        return fail(e);                  \
+    }                                    \
+    catch (const std::bad_alloc&) {      \
+        gLastError = "host out of memory"; \
+        return VOL_ERR_INVALID_ARGUMENT; \
+    }                                    \
+    catch (const std::exception& e) {    \
+        return failStd(e);               \
+    }                                    \
+    return VOL_OK;
+
+void setDevice(vol_context ctx) { HIPCHECK(hipSetDevice(ctx->device)); }
+
+// ------------------------------------------------------------------ mesh setup
+void setupMesh(vol_mesh m, const double* w, int mergeLevels) {
+    hipStream_t st = m->ctx->stream;
+    const int n = m->n, nf = m->nf;
+    // level-0 rows: faces ascending; owner row → upper[f] (side 0), neighbour row → lower[f] (side 1)
+    std::vector<int> off(n + 1, 0);
+    for (int f = 0; f < nf; ++f) {
+        ++off[m->own[f] + 1];
+        ++off[m->nei[f] + 1];
+    }
+    for (int c = 0; c < n; ++c) off[c + 1] += off[c];
+    std::vector<int> col(off[n]), src(off[n]);
+    {
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (int f = 0; f < nf; ++f) {
+            int k = fill[m->own[f]]++;
+            col[k] = m->nei[f];
+            src[k] = (f << 1) | 0;
+            k = fill[m->nei[f]]++;
+            col[k] = m->own[f];
+            src[k] = (f << 1) | 1;
+        }
+    }
+    buildSell(n, off, col, src, m->sell, st);
+    if (!w) return;
+    m->hasGamg = true;
+    m->H = buildHierarchy(n, nf, m->own.data(), m->nei.data(), w, mergeLevels);
+    const int nL = int(m->H.size());
+    std::vector<char> oil(nf);
+    for (int f = 0; f < nf; ++f) oil[f] = char(m->own[f] < m->nei[f]);
+    m->ownerIsL.upload(oil, st);
+    for (int k = 0; k < nL; ++k) {
+        const HLevel& F = m->H[k];
+        auto D = std::make_unique<LevelDev>();
+        D->n = F.n;
+        D->nf = int(F.l.size());
+        D->l.upload(F.l, st);
+        D->u.upload(F.u, st);
+        if (k > 0) {  // SELL rows in adjacency (face) order: A(i, j) = upper if i is the l side
+            std::vector<int> srcL(F.adjFace.size());
+            for (size_t q = 0; q < srcL.size(); ++q) srcL[q] = (F.adjFace[q] << 1) | (F.adjIsL[q] ? 0 : 1);
+            buildSell(F.n, F.off, F.adjCell, srcL, D->sell, st);
+        }
+        if (k + 1 < nL) {
+            const HLevel& C = m->H[k + 1];
+            D->agg.upload(F.agg, st);
+            std::vector<int> mOff(C.n + 1, 0), mList(F.n);
+            for (int i = 0; i < F.n; ++i) ++mOff[F.agg[i] + 1];
+            for (int I = 0; I < C.n; ++I) mOff[I + 1] += mOff[I];
+            {
+                std::vector<int> fill(mOff.begin(), mOff.end() - 1);
+                for (int i = 0; i < F.n; ++i) mList[fill[F.agg[i]]++] = i;
+            }
+            std::vector<int> iOff(C.n + 1, 0), iList;
+            const int fnf = int(F.l.size());
+            for (int f = 0; f < fnf; ++f)
+                if (F.faceMap[f] < 0) ++iOff[F.agg[F.l[f]] + 1];
+            for (int I = 0; I < C.n; ++I) iOff[I + 1] += iOff[I];
+            iList.resize(iOff[C.n]);
+            {
+                std::vector<int> fill(iOff.begin(), iOff.end() - 1);
+                for (int f = 0; f < fnf; ++f)
+                    if (F.faceMap[f] < 0) iList[fill[F.agg[F.l[f]]]++] = f;
+            }
+            const int cnf = int(C.l.size());
+            std::vector<int> cfOff(cnf + 1, 0), cfList;
+            for (int f = 0; f < fnf; ++f)
+                if (F.faceMap[f] >= 0) ++cfOff[F.faceMap[f] + 1];
+            for (int c = 0; c < cnf; ++c) cfOff[c + 1] += cfOff[c];
+            cfList.resize(cfOff[cnf]);
+            {
+                std::vector<int> fill(cfOff.begin(), cfOff.end() - 1);
+                for (int f = 0; f < fnf; ++f)
+                    if (F.faceMap[f] >= 0) cfList[fill[F.faceMap[f]]++] = (f << 1) | (F.faceFlip[f] ? 1 : 0);
+            }
+            D->mOff.upload(mOff, st);
+            D->mList.upload(mList, st);
+            D->iOff.upload(iOff, st);
+            D->iList.upload(iList, st);
+            D->cfOff.upload(cfOff, st);
+            D->cfList.upload(cfList, st);
+        }
+        m->lv.push_back(std::move(D));
+    }
+    if (m->H.back().n > 4096)
+        throw VolError(VOL_ERR_UNSUPPORTED, "GAMG: coarsest level larger than 4096 cells (agglomeration stalled)");
+}
+
+// ------------------------------------------------------------------ matrix / GAMG coefficients
+void buildGamgCoeffs(vol_matrix A) {
+    vol_mesh m = A->mesh;
+    hipStream_t st = m->ctx->stream;
+    const int nL = int(m->lv.size());
+    if (A->L.size() != size_t(nL)) {
+        A->L.clear();
+        for (int k = 0; k < nL; ++k) {
+            auto M = std::make_unique<MatLevel>();
+            const LevelDev& D = *m->lv[k];
+            if (k > 0) {
+                M->diag.alloc(D.n);
+                M->coef.alloc(D.sell.nslots);
+                M->r.alloc(D.n);
+                M->x.alloc(D.n);
+            }
+            M->Fu.alloc(D.nf);
+            M->Fl.alloc(D.nf);
+            M->bufA.alloc(D.n);
+            M->bufB.alloc(D.n);
+            A->L.push_back(std::move(M));
+        }
+    }
+    launch_orient0(st, m->nf, m->ownerIsL.p, A->upper.p, A->symmetric ? A->upper.p : A->lower.p, A->L[0]->Fu.p,
+                   A->L[0]->Fl.p);
+    for (int k = 0; k + 1 < nL; ++k) {
+        const LevelDev& D = *m->lv[k];
+        const LevelDev& DC = *m->lv[k + 1];
+        const double* Fd = k == 0 ? A->diag.p : A->L[k]->diag.p;
+        launch_galerkin(st, DC.n, DC.nf, D.mOff.p, D.mList.p, D.iOff.p, D.iList.p, D.cfOff.p, D.cfList.p, Fd,
+                        A->L[k]->Fu.p, A->L[k]->Fl.p, A->L[k + 1]->diag.p, A->L[k + 1]->Fu.p, A->L[k + 1]->Fl.p);
+        launch_sell_coef(st, DC.sell.view(), A->L[k + 1]->Fu.p, A->L[k + 1]->Fl.p, A->L[k + 1]->coef.p, nullptr);
+    }
+    const LevelDev& DL = *m->lv[nL - 1];
+    A->nCoarse = DL.n;
+    A->lu.alloc(size_t(DL.n) * DL.n);
+    A->status.alloc(1);
+    const double* cd = nL == 1 ? A->diag.p : A->L[nL - 1]->diag.p;
+    launch_coarse_assemble(st, DL.n, DL.nf, DL.l.p, DL.u.p, cd, A->L[nL - 1]->Fu.p, A->L[nL - 1]->Fl.p, A->lu.p);
+    launch_coarse_lu(st, DL.n, A->lu.p, A->status.p);
+    int status = 0;
+    HIPCHECK(hipMemcpyAsync(&status, A->status.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (status != 0) throw VolError(VOL_ERR_SINGULAR_COARSEST, "GAMG: singular coarsest matrix");
+    A->gamgReady = true;
+}
+
+// SELL slot coefficients and 1/diag after diag/upper/lower changed.
+void matrixFinish(vol_matrix A) {
+    vol_mesh m = A->mesh;
+    hipStream_t st = m->ctx->stream;
+    A->coef.alloc(m->sell.nslots);
+    launch_sell_coef(st, m->sell.view(), A->upper.p, A->symmetric ? A->upper.p : A->lower.p, A->coef.p, nullptr);
+    A->rD.alloc(m->n);
+    VecArgs v;
+    v.n = m->n;
+    v.a = A->diag.p;
+    v.w = A->rD.p;
+    launch_vec(st, V_RECIP, v, false, m->ctx->red);
+    A->gamgReady = false;
+    A->set = true;
+}
+
+// Device copy of a caller array (host → staging, device → used in place).
+template <class T> const T* stage(DevBuf<T>& buf, const T* p, size_t n, int location, hipStream_t st) {
+    if (!p || location == VOL_DEVICE) return p;
+    buf.alloc(n);
+    if (n) HIPCHECK(hipMemcpyAsync(buf.p, p, sizeof(T) * n, hipMemcpyHostToDevice, st));
+    return buf.p;
+}
+
+// fvm::laplacian (kind ASM_LAPLACIAN) or fvm::div upwind (ASM_DIV) into A, with the fold.
+void assemble(vol_matrix A, int kind, double gammaU, const double* faceGamma, const double* magSf, const double* ndc,
+              const double* flux, const int32_t* bcFixed, const double* bcValue, const double* bIn, double* bOut,
+              double* icOut, double* bsOut, int location) {
+    vol_mesh m = A->mesh;
+    vol_context c = m->ctx;
+    if (m->nb < 0) throw VolError(VOL_ERR_INVALID_ARGUMENT, "mesh boundary not set (vol_mesh_set_boundary)");
+    if (!bcFixed && m->nb > 0) throw VolError(VOL_ERR_INVALID_ARGUMENT, "bcFixed required");
+    setDevice(c);
+    hipStream_t st = c->stream;
+    const int n = m->n, nf = m->nf, nb = m->nb;
+    const size_t nF = size_t(nf) + nb;
+    AsmArgs a;
+    a.n = n;
+    a.nf = nf;
+    a.S = m->sell.view();
+    a.bOff = m->bOff.p;
+    a.bList = m->bList.p;
+    a.gammaU = gammaU;
+    a.gamma = stage(A->st0, faceGamma, nF, location, st);
+    if (kind == ASM_LAPLACIAN) {
+        if (!magSf || !ndc) throw VolError(VOL_ERR_INVALID_ARGUMENT, "magSf and ndc required");
+        a.magSf = stage(A->st1, magSf, nF, location, st);
+        a.ndc = stage(A->st2, ndc, nF, location, st);
+    } else {
+        if (!flux) throw VolError(VOL_ERR_INVALID_ARGUMENT, "flux required");
+        a.flux = stage(A->st1, flux, nF, location, st);
+    }
+    a.bcFixed = stage(A->sti, bcFixed, size_t(nb), location, st);
+    a.bcValue = stage(A->st3, bcValue, size_t(nb), location, st);
+    a.bIn = stage(A->st4, bIn, size_t(n), location, st);
+    A->symmetric = kind == ASM_LAPLACIAN;
+    A->diag.alloc(n);
+    A->upper.alloc(nf);
+    a.diag = A->diag.p;
+    a.upper = A->upper.p;
+    if (!A->symmetric) {
+        A->lower.alloc(nf);
+        a.lower = A->lower.p;
+    }
+    const bool hostOut = location != VOL_DEVICE;
+    DevBuf<double> tb, tic, tbs;
+    if (bOut) {
+        if (hostOut) tb.alloc(n);
+        a.bOut = hostOut ? tb.p : bOut;
+    }
+    if (icOut) {
+        if (hostOut) tic.alloc(nb);
+        a.icOut = hostOut ? tic.p : icOut;
+    }
+    if (bsOut) {
+        if (hostOut) tbs.alloc(nb);
+        a.bsOut = hostOut ? tbs.p : bsOut;
+    }
+    launch_assemble(st, kind, a);
+    matrixFinish(A);
+    if (hostOut) {
+        if (bOut) HIPCHECK(hipMemcpyAsync(bOut, tb.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        if (icOut && nb) HIPCHECK(hipMemcpyAsync(icOut, tic.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+        if (bsOut && nb) HIPCHECK(hipMemcpyAsync(bsOut, tbs.p, sizeof(double) * nb, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+}
+
+// ------------------------------------------------------------------ solve
+struct Plan {
+    std::string solver, precond;
+    bool exact = false;
+    int nPre = 2, nPost = 2;
+    double omega = 0.8;
+};
+
+struct Ctx {
+    vol_context c;
+    hipStream_t st;
+    const int* done;
+    DevScalars* S;
+};
+
+void sell(const Ctx& C, int op, int xl, const SellView& V, const double* coef, const double* diag, const double* x,
+          const double* r, double* out, double omega = 1.0, const double* xc = nullptr, const int* agg = nullptr,
+          const double* dotv = nullptr, double* part = nullptr, const double* xu = nullptr) {
+    SellArgs a;
+    a.S = V;
+    a.coef = coef;
+    a.diag = diag;
+    a.x = x;
+    a.r = r;
+    a.xc = xc;
+    a.agg = agg;
+    a.omega = omega;
+    a.out = out;
+    a.dotv = dotv;
+    a.part = part;
+    a.done = C.done;
+    a.xuPtr = xu;
+    launch_sell(C.st, op, xl, a);
+}
+
+void vec(const Ctx& C, int op, int n, bool exact, const double* a, const double* b = nullptr,
+         const double* c = nullptr, const double* d = nullptr, double* w = nullptr, double* x = nullptr,
+         double* y = nullptr, const double* rD = nullptr) {
+    VecArgs v;
+    v.n = n;
+    v.a = a;
+    v.b = b;
+    v.c = c;
+    v.d = d;
+    v.w = w;
+    v.x = x;
+    v.y = y;
+    v.rD = rD;
+    v.S = C.S;
+    v.done = C.done;
+    launch_vec(C.st, op, v, exact, C.c->red);
+}
+
+void fin(const Ctx& C, int op, int nred, bool exact, int nparts) {
+    launch_finalize(C.st, op, nred, exact, nparts, C.c->red, C.S);
+}
+
+// Level-0 A·p with optional fused dot (fast mode), timed when timing is on.
+void amulTimed(const Ctx& C, vol_matrix A, const double* p, double* q, const double* dotv, bool fused) {
+    vol_context c = C.c;
+    const SellView V = A->mesh->sell.view();
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (c->timing) {
+        if (c->evUsed + 2 > c->evPool.size()) {
+            for (int k = 0; k < 64; ++k) {
+                hipEvent_t e;
+                HIPCHECK(hipEventCreate(&e));
+                c->evPool.push_back(e);
+            }
+        }
+        e0 = c->evPool[c->evUsed++];
+        e1 = c->evPool[c->evUsed++];
+        HIPCHECK(hipEventRecord(e0, C.st));
+    }
+    sell(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, p, nullptr, q, 1.0, nullptr, nullptr, fused ? dotv : nullptr,
+         fused ? c->red.part : nullptr);
+    if (c->timing) HIPCHECK(hipEventRecord(e1, C.st));
+}
+
+void collectTiming(vol_context c) {
+    if (!c->timing) return;
+    for (size_t k = 0; k + 1 < c->evUsed; k += 2) {
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, c->evPool[k], c->evPool[k + 1]));
+        c->amulMs += ms;
+        ++c->amulCount;
+    }
+    c->evUsed = 0;
+}
+
+// One V-cycle at level l: r → x (GAMG.cpp:403-443 structure, damped-Jacobi smoothing).
+void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const double* r, double* xout) {
+    vol_mesh m = A->mesh;
+    const int nL = int(m->lv.size());
+    if (l == nL - 1) {
+        launch_coarse_solve(C.st, A->nCoarse, A->lu.p, r, xout, C.done);
+        return;
+    }
+    const LevelDev& D = *m->lv[l];
+    MatLevel& M = *A->L[l];
+    const SellView V = l == 0 ? m->sell.view() : D.sell.view();
+    const double* coef = l == 0 ? A->coef.p : M.coef.p;
+    const double* diag = l == 0 ? A->diag.p : M.diag.p;
+    double* bufs[2] = {M.bufA.p, M.bufB.p};
+    int cur = 0;  // index of the buffer holding the pre-smoothed x (nPre ≥ 2)
+    for (int s = 1; s < P.nPre; ++s) {
+        const int dst = (s == 1) ? 0 : 1 - cur;
+        sell(C, OP_JACOBI, s == 1 ? XL_FROMZERO : XL_PLAIN, V, coef, diag, s == 1 ? nullptr : bufs[cur], r, bufs[dst],
+             P.omega);
+        cur = dst;
+    }
+    const int freeB = 1 - cur;
+    // residual of the pre-smoothed x, then restriction
+    sell(C, OP_RESID, P.nPre == 1 ? XL_FROMZERO : XL_PLAIN, V, coef, diag, P.nPre == 1 ? nullptr : bufs[cur], r,
+         bufs[freeB], P.omega);
+    MatLevel& MC = *A->L[l + 1];
+    launch_restrict(C.st, m->lv[l + 1]->n, D.mOff.p, D.mList.p, bufs[freeB], MC.r.p, C.done);
+    vcycle(C, A, P, l + 1, MC.r.p, MC.x.p);
+    // prolongation fused into the first post-sweep
+    int src = cur, other = freeB;
+    for (int s = 0; s < P.nPost; ++s) {
+        double* dst = (s == P.nPost - 1) ? xout : bufs[other];
+        if (s == 0)
+            sell(C, OP_JACOBI, P.nPre == 1 ? XL_ZEROPROLONG : XL_PROLONG, V, coef, diag,
+                 P.nPre == 1 ? nullptr : bufs[src], r, dst, P.omega, MC.x.p, D.agg.p);
+        else
+            sell(C, OP_JACOBI, XL_PLAIN, V, coef, diag, bufs[src], r, dst, P.omega);
+        if (s != P.nPost - 1) {
+            const int t = src;
+            src = other;
+            other = t;
+        }
+    }
+}
+
+void precondApply(const Ctx& C, vol_matrix A, const Plan& P, const double* r, double* w, int n) {
+    if (P.precond == "GAMG")
+        vcycle(C, A, P, 0, r, w);
+    else
+        vec(C, V_SCALE, n, false, r, nullptr, nullptr, nullptr, w, nullptr, nullptr,
+            P.precond == "diagonal" ? A->rD.p : nullptr);
+}
+
+void readScalars(vol_context c, DevScalars& out) {
+    HIPCHECK(hipMemcpyAsync(c->hostScal, c->scal.p, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    out = *c->hostScal;
+}
+
+bool hostConverged(const vol_solver_controls& ct, const DevScalars& S) {
+    if (S.iterations < ct.minIter) return false;
+    return S.finalResidual < ct.tolerance || (ct.relTol > 0 && S.finalResidual < ct.relTol * S.initialResidual);
+}
+
+void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_controls& ct, vol_solver_performance& perf) {
+    vol_mesh m = A->mesh;
+    vol_context c = m->ctx;
+    const int n = m->n;
+    Plan P;
+    P.solver = ct.solver ? ct.solver : "PCG";
+    P.precond = ct.preconditioner ? ct.preconditioner : "diagonal";
+    int sums = ct.sums == VOL_SUMS_CONTEXT ? (c->reproducible ? VOL_SUMS_REPRODUCIBLE : VOL_SUMS_FAST) : ct.sums;
+    if (sums < VOL_SUMS_FAST || sums > VOL_SUMS_EXACT) throw VolError(VOL_ERR_INVALID_ARGUMENT, "bad sums mode");
+    if (sums == VOL_SUMS_REPRODUCIBLE) {  // LinearSolver.cpp:138-148
+        if (P.precond != "none") P.precond = "diagonal";
+        if (P.solver == "GaussSeidel" || P.solver == "symGaussSeidel") P.solver = "Jacobi";
+    }
+    P.exact = sums != VOL_SUMS_FAST;
+    P.nPre = ct.gamgPreSweeps;
+    P.nPost = ct.gamgPostSweeps;
+    P.omega = ct.gamgOmega;
+    if (ct.precision != VOL_FP64) throw VolError(VOL_ERR_UNSUPPORTED, "only VOL_FP64 is available");
+    std::memset(&perf, 0, sizeof(perf));
+    std::strncpy(perf.solver, P.solver.c_str(), sizeof(perf.solver) - 1);
+
+    const bool isPCG = P.solver == "PCG", isBi = P.solver == "PBiCGStab", isJac = P.solver == "Jacobi";
+    if (!isPCG && !isBi && !isJac) {
+        if (P.solver == "GaussSeidel" || P.solver == "symGaussSeidel")
+            throw VolError(VOL_ERR_UNSUPPORTED,
+                           "sequential Gauss-Seidel solvers are not provided on the GPU; use Jacobi, PCG or PBiCGStab");
+        throw VolError(VOL_ERR_UNKNOWN_SOLVER, "unknown linear solver " + P.solver);
+    }
+    if (isPCG && !A->symmetric) throw VolError(VOL_ERR_PCG_NONSYMMETRIC, "PCG requires a symmetric matrix; use PBiCGStab");
+    std::string pc = P.precond;
+    if (isBi && pc == "DIC") pc = "DILU";  // LinearSolver.cpp:203
+    if (isPCG || isBi) {
+        if (pc != "none" && pc != "diagonal" && pc != "GAMG" && pc != "DIC" && pc != "DILU")
+            throw VolError(VOL_ERR_UNKNOWN_PRECONDITIONER, "unknown preconditioner " + pc);
+        if (pc == "DIC" || pc == "DILU")
+            throw VolError(VOL_ERR_UNSUPPORTED, "DIC/DILU preconditioners are not provided on the GPU; use diagonal or GAMG");
+        if (pc == "GAMG" && !m->hasGamg)
+            throw VolError(VOL_ERR_NO_GAMG_WEIGHTS, "GAMG needs a mesh created with agglomeration weights");
+        if (pc == "GAMG" && (P.nPre < 1 || P.nPost < 1 || !(P.omega > 0.0)))
+            throw VolError(VOL_ERR_INVALID_ARGUMENT, "GAMG smoother needs ≥ 1 pre/post sweep and ω > 0");
+    }
+    P.precond = pc;
+    if (ct.maxIter < 0 || ct.nSweeps < 1) throw VolError(VOL_ERR_INVALID_ARGUMENT, "bad maxIter/nSweeps");
+
+    setDevice(c);
+    hipStream_t st = c->stream;
+    c->ensureScratch(sell_grid_blocks(m->sell.nslices));
+    Ctx C{c, st, &c->scal.p->done, c->scal.p};
+    const bool ex = P.exact;
+    const bool fuse = !ex;
+    const SellView V = m->sell.view();
+    A->wA.alloc(n);
+    A->pA.alloc(n);
+    A->vr.alloc(n);
+    launch_init_scalars(st, C.S, ct.tolerance, ct.relTol, ct.maxIter, ct.minIter, ct.nSweeps, n);
+    // initial residual and normalisation (LinearSolver.cpp:150-169)
+    sell(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, x, nullptr, A->wA.p);
+    const int nparts = ex ? RED_BLOCKS_EXACT : RED_BLOCKS;
+    vec(C, V_SUM_X, n, ex, x);
+    fin(C, FIN_XREF, 1, ex, nparts);
+    sell(C, OP_AMUL, XL_UNIFORM, V, A->coef.p, A->diag.p, nullptr, nullptr, A->pA.p, 1.0, nullptr, nullptr, nullptr,
+         nullptr, &C.S->xRef);
+    vec(C, V_INIT, n, ex, A->wA.p, b, A->pA.p, nullptr, A->vr.p);
+    fin(C, FIN_INIT, 2, ex, nparts);
+    DevScalars hs;
+    readScalars(c, hs);
+    if (!hs.done) {
+        if (P.precond == "GAMG" && !A->gamgReady) buildGamgCoeffs(A);
+        const int sellParts = sell_grid_blocks(V.nslices);
+        auto iterate = [&]() {
+            if (isPCG) {  // LinearSolver.cpp:176-201
+                A->vw.alloc(n);
+                A->vp.alloc(n);
+                if (P.precond == "GAMG") {
+                    vcycle(C, A, P, 0, A->vr.p, A->vw.p);
+                    vec(C, V_DOT, n, ex, A->vw.p, A->vr.p);
+                } else {
+                    vec(C, V_PRECOND_DOT, n, ex, A->vr.p, nullptr, nullptr, nullptr, A->vw.p, nullptr, nullptr,
+                        P.precond == "diagonal" ? A->rD.p : nullptr);
+                }
+                fin(C, FIN_PCG_WRA, 1, ex, nparts);
+                vec(C, V_PCG_P, n, ex, A->vw.p, nullptr, nullptr, nullptr, nullptr, A->vp.p);
+                amulTimed(C, A, A->vp.p, A->wA.p, A->vp.p, fuse);
+                if (fuse) {
+                    fin(C, FIN_PCG_WAPA, 1, false, sellParts);
+                } else {
+                    vec(C, V_DOT, n, ex, A->wA.p, A->vp.p);
+                    fin(C, FIN_PCG_WAPA, 1, ex, nparts);
+                }
+                vec(C, V_PCG_XR, n, ex, A->vp.p, A->wA.p, nullptr, nullptr, nullptr, x, A->vr.p);
+                fin(C, FIN_PCG_NORM, 1, ex, nparts);
+            } else if (isBi) {  // LinearSolver.cpp:202-237
+                vec(C, V_DOT, n, ex, A->r0.p, A->vr.p);
+                fin(C, FIN_BI_RHO, 1, ex, nparts);
+                vec(C, V_BI_P, n, ex, A->vr.p, A->Ay.p, nullptr, nullptr, nullptr, A->vp.p);
+                precondApply(C, A, P, A->vp.p, A->y.p, n);
+                amulTimed(C, A, A->y.p, A->Ay.p, A->r0.p, fuse);
+                if (fuse) {
+                    fin(C, FIN_BI_ALPHA, 1, false, sellParts);
+                } else {
+                    vec(C, V_DOT, n, ex, A->r0.p, A->Ay.p);
+                    fin(C, FIN_BI_ALPHA, 1, ex, nparts);
+                }
+                vec(C, V_BI_S, n, ex, A->vr.p, A->Ay.p, nullptr, nullptr, A->s.p);
+                fin(C, FIN_BI_S, 1, ex, nparts);
+                vec(C, V_BI_XEXIT, n, ex, A->y.p, nullptr, nullptr, nullptr, nullptr, x);
+                launch_clear_exit(st, C.S);
+                precondApply(C, A, P, A->s.p, A->z.p, n);
+                sell(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, A->z.p, nullptr, A->t.p);
+                vec(C, V_BI_DOT2, n, ex, A->t.p, A->s.p);
+                fin(C, FIN_BI_OMEGA, 2, ex, nparts);
+                vec(C, V_BI_XR, n, ex, A->y.p, A->z.p, A->s.p, A->t.p, nullptr, x, A->vr.p);
+                fin(C, FIN_BI_NORM, 1, ex, nparts);
+            } else {  // Jacobi solver, LinearSolver.cpp:238-278
+                for (int sw = 0; sw < ct.nSweeps; ++sw) {
+                    sell(C, OP_JACREF, XL_PLAIN, V, A->coef.p, A->diag.p, x, b, A->vw.p);
+                    vec(C, V_COPY, n, ex, A->vw.p, nullptr, nullptr, nullptr, x);
+                }
+                sell(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, x, nullptr, A->wA.p);
+                vec(C, V_BMINUSAX_NORM, n, ex, A->wA.p, b, nullptr, nullptr, A->vr.p);
+                fin(C, FIN_JAC_NORM, 1, ex, nparts);
+            }
+        };
+        if (isBi) {
+            A->r0.alloc(n);
+            A->vp.alloc(n);
+            A->y.alloc(n);
+            A->Ay.alloc(n);
+            A->s.alloc(n);
+            A->z.alloc(n);
+            A->t.alloc(n);
+            HIPCHECK(hipMemcpyAsync(A->r0.p, A->vr.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+            HIPCHECK(hipMemsetAsync(A->vp.p, 0, sizeof(double) * n, st));
+            HIPCHECK(hipMemsetAsync(A->Ay.p, 0, sizeof(double) * n, st));
+        }
+        if (isPCG && P.precond != "GAMG") {
+            A->vp.alloc(n);
+            HIPCHECK(hipMemsetAsync(A->vp.p, 0, sizeof(double) * n, st));
+        }
+        if (isJac) A->vw.alloc(n);
+        // Enqueue one iteration ahead: iteration k+1 runs as no-ops if k finished.
+        hipEvent_t evs[2];
+        HIPCHECK(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+        DevScalars* hsl = c->hostScal;
+        const int maxLoops = std::max(ct.maxIter, 1) + 2;
+        int k = 0;
+        iterate();
+        HIPCHECK(hipMemcpyAsync(&hsl[0], C.S, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipEventRecord(evs[0], st));
+        for (k = 1; k <= maxLoops; ++k) {
+            iterate();
+            HIPCHECK(hipMemcpyAsync(&hsl[k & 1], C.S, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
+            HIPCHECK(hipEventRecord(evs[k & 1], st));
+            HIPCHECK(hipEventSynchronize(evs[(k - 1) & 1]));
+            if (hsl[(k - 1) & 1].done) break;
+        }
+        HIPCHECK(hipStreamSynchronize(st));
+        (void)hipEventDestroy(evs[0]);
+        (void)hipEventDestroy(evs[1]);
+        readScalars(c, hs);
+        if (!hs.done) throw VolError(VOL_ERR_HIP, "solver loop did not terminate");
+    }
+    collectTiming(c);
+    perf.initialResidual = hs.initialResidual;
+    perf.finalResidual = hs.finalResidual;
+    perf.residualSum = hs.residualSum;
+    perf.normFactor = hs.normFactor;
+    perf.iterations = hs.iterations;
+    perf.converged = hostConverged(ct, hs) ? 1 : 0;
+}
+
+}  // namespace
+
+// ============================================================================ C ABI
+extern "C" {
+
+void vol_controls_default(vol_solver_controls* c) {
+    if (!c) return;
+    c->solver = "PCG";
+    c->preconditioner = "diagonal";
+    c->tolerance = 1e-8;
+    c->relTol = 0.0;
+    c->maxIter = 1000;
+    c->minIter = 0;
+    c->nSweeps = 1;
+    c->sums = VOL_SUMS_CONTEXT;
+    c->precision = VOL_FP64;
+    c->gamgPreSweeps = 2;
+    c->gamgPostSweeps = 2;
+    c->gamgOmega = 0.8;
+}
+
+const char* vol_last_error(void) { return gLastError.c_str(); }
+
+int vol_context_create(int device, vol_context* out) {
+    API_BEGIN
+    if (!out) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null output");
+    int count = 0;
+    HIPCHECK(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count) throw VolError(VOL_ERR_INVALID_ARGUMENT, "no such GPU");
+    auto c = std::make_unique<vol_context_s>();
+    c->device = device;
+    HIPCHECK(hipSetDevice(device));
+    HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    if (const char* e = std::getenv("CFD_REPRODUCIBLE")) c->reproducible = std::string(e) != "0" && std::string(e) != "";
+    c->scal.alloc(1);
+    HIPCHECK(hipHostMalloc((void**)&c->hostScal, 2 * sizeof(DevScalars), hipHostMallocDefault));
+    c->ensureScratch(RED_BLOCKS);
+    *out = c.release();
+    API_END
+}
+
+int vol_context_destroy(vol_context ctx) {
+    API_BEGIN
+    if (!ctx) return VOL_OK;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    for (auto e : ctx->evPool) (void)hipEventDestroy(e);
+    if (ctx->hostScal) (void)hipHostFree(ctx->hostScal);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    API_END
+}
+
+int vol_set_reproducible(vol_context ctx, int on) {
+    API_BEGIN
+    if (!ctx) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null context");
+    ctx->reproducible = on != 0;
+    API_END
+}
+
+int vol_synchronize(vol_context ctx) {
+    API_BEGIN
+    if (!ctx) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null context");
+    setDevice(ctx);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    API_END
+}
+
+int vol_device_alloc(vol_context ctx, uint64_t bytes, void** ptr) {
+    API_BEGIN
+    if (!ctx || !ptr) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    setDevice(ctx);
+    HIPCHECK(hipMalloc(ptr, std::max<uint64_t>(bytes, 1)));
+    API_END
+}
+
+int vol_device_free(vol_context ctx, void* ptr) {
+    API_BEGIN
+    if (!ctx) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null context");
+    setDevice(ctx);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    if (ptr) HIPCHECK(hipFree(ptr));
+    API_END
+}
+
+int vol_memcpy_to_device(vol_context ctx, void* dst, const void* src, uint64_t bytes) {
+    API_BEGIN
+    setDevice(ctx);
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    API_END
+}
+
+int vol_memcpy_to_host(vol_context ctx, void* dst, const void* src, uint64_t bytes) {
+    API_BEGIN
+    setDevice(ctx);
+    HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    API_END
+}
+
+int vol_mesh_create(vol_context ctx, int nCells, int nInternalFaces, const int32_t* owner, const int32_t* neighbour,
+                    const double* gamgWeights, int mergeLevels, vol_mesh* out) {
+    API_BEGIN
+    if (!ctx || !out) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    if (nCells < 1 || nInternalFaces < 0 || (nInternalFaces > 0 && (!owner || !neighbour)))
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "bad mesh sizes");
+    if (int64_t(nInternalFaces) * 2 >= (int64_t(1) << 30)) throw VolError(VOL_ERR_INVALID_ARGUMENT, "mesh too large");
+    for (int f = 0; f < nInternalFaces; ++f) {
+        if (owner[f] < 0 || owner[f] >= nCells || neighbour[f] < 0 || neighbour[f] >= nCells || owner[f] == neighbour[f])
+            throw VolError(VOL_ERR_INVALID_ARGUMENT, "bad owner/neighbour at face " + std::to_string(f));
+    }
+    setDevice(ctx);
+    auto m = std::make_unique<vol_mesh_s>();
+    m->ctx = ctx;
+    m->n = nCells;
+    m->nf = nInternalFaces;
+    m->own.assign(owner, owner + nInternalFaces);
+    m->nei.assign(neighbour, neighbour + nInternalFaces);
+    setupMesh(m.get(), gamgWeights, mergeLevels > 0 ? mergeLevels : 2);
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    *out = m.release();
+    API_END
+}
+
+int vol_mesh_destroy(vol_mesh mesh) {
+    API_BEGIN
+    if (!mesh) return VOL_OK;
+    setDevice(mesh->ctx);
+    (void)hipStreamSynchronize(mesh->ctx->stream);
+    delete mesh;
+    API_END
+}
+
+int vol_mesh_gamg_levels(vol_mesh mesh, int* nLevels, int32_t* sizes) {
+    API_BEGIN
+    if (!mesh || !nLevels) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    *nLevels = int(mesh->H.size());
+    if (sizes)
+        for (size_t k = 0; k < mesh->H.size(); ++k) sizes[k] = mesh->H[k].n;
+    API_END
+}
+
+int vol_mesh_gamg_level(vol_mesh mesh, int level, int32_t* nFaces, int32_t* agg, int32_t* faceMap,
+                        int32_t* faceFlip) {
+    API_BEGIN
+    if (!mesh || level < 0 || level >= int(mesh->H.size())) throw VolError(VOL_ERR_INVALID_ARGUMENT, "bad level");
+    const HLevel& L = mesh->H[level];
+    if (nFaces) *nFaces = int(L.l.size());
+    const bool coarsest = L.agg.empty();
+    if (agg)
+        for (int i = 0; i < L.n; ++i) agg[i] = coarsest ? -1 : L.agg[i];
+    for (size_t f = 0; f < L.l.size(); ++f) {
+        if (faceMap) faceMap[f] = coarsest ? -1 : L.faceMap[f];
+        if (faceFlip) faceFlip[f] = coarsest ? 0 : L.faceFlip[f];
+    }
+    API_END
+}
+
+int vol_mesh_set_boundary(vol_mesh mesh, int nBoundaryFaces, const int32_t* boundaryOwner) {
+    API_BEGIN
+    if (!mesh || nBoundaryFaces < 0 || (nBoundaryFaces > 0 && !boundaryOwner))
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "bad boundary");
+    const int n = mesh->n;
+    std::vector<int> off(n + 1, 0), list(nBoundaryFaces);
+    for (int bf = 0; bf < nBoundaryFaces; ++bf) {
+        if (boundaryOwner[bf] < 0 || boundaryOwner[bf] >= n) throw VolError(VOL_ERR_INVALID_ARGUMENT, "bad boundary owner");
+        ++off[boundaryOwner[bf] + 1];
+    }
+    for (int c = 0; c < n; ++c) off[c + 1] += off[c];
+    {
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        for (int bf = 0; bf < nBoundaryFaces; ++bf) list[fill[boundaryOwner[bf]]++] = bf;
+    }
+    setDevice(mesh->ctx);
+    mesh->bOff.upload(off, mesh->ctx->stream);
+    mesh->bList.upload(list, mesh->ctx->stream);
+    mesh->nb = nBoundaryFaces;
+    HIPCHECK(hipStreamSynchronize(mesh->ctx->stream));
+    API_END
+}
+
+int vol_assemble_laplacian(vol_matrix A, double gamma, const double* faceGamma, const double* magSf,
+                           const double* ndc, const int32_t* bcFixed, const double* bcValue, const double* bIn,
+                           double* bOut, double* internalCoeffs, double* boundarySource, int location) {
+    API_BEGIN
+    if (!A) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null matrix");
+    assemble(A, ASM_LAPLACIAN, gamma, faceGamma, magSf, ndc, nullptr, bcFixed, bcValue, bIn, bOut, internalCoeffs,
+             boundarySource, location);
+    API_END
+}
+
+int vol_assemble_div_upwind(vol_matrix A, const double* flux, const int32_t* bcFixed, const double* bcValue,
+                            const double* bIn, double* bOut, double* internalCoeffs, double* boundarySource,
+                            int location) {
+    API_BEGIN
+    if (!A) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null matrix");
+    assemble(A, ASM_DIV, 1.0, nullptr, nullptr, nullptr, flux, bcFixed, bcValue, bIn, bOut, internalCoeffs,
+             boundarySource, location);
+    API_END
+}
+
+int vol_matrix_create(vol_mesh mesh, vol_matrix* out) {
+    API_BEGIN
+    if (!mesh || !out) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    auto A = std::make_unique<vol_matrix_s>();
+    A->mesh = mesh;
+    *out = A.release();
+    API_END
+}
+
+int vol_matrix_destroy(vol_matrix A) {
+    API_BEGIN
+    if (!A) return VOL_OK;
+    setDevice(A->mesh->ctx);
+    (void)hipStreamSynchronize(A->mesh->ctx->stream);
+    delete A;
+    API_END
+}
+
+int vol_matrix_set(vol_matrix A, const double* diag, const double* upper, const double* lower, int location) {
+    API_BEGIN
+    if (!A || !diag || (!upper && A->mesh->nf > 0)) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    vol_mesh m = A->mesh;
+    vol_context c = m->ctx;
+    setDevice(c);
+    hipStream_t st = c->stream;
+    const int n = m->n, nf = m->nf;
+    A->symmetric = (lower == nullptr || lower == upper);
+    A->diag.alloc(n);
+    A->upper.alloc(nf);
+    const hipMemcpyKind kind = location == VOL_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    HIPCHECK(hipMemcpyAsync(A->diag.p, diag, sizeof(double) * n, kind, st));
+    if (nf) HIPCHECK(hipMemcpyAsync(A->upper.p, upper, sizeof(double) * nf, kind, st));
+    if (!A->symmetric) {
+        A->lower.alloc(nf);
+        if (nf) HIPCHECK(hipMemcpyAsync(A->lower.p, lower, sizeof(double) * nf, kind, st));
+    }
+    matrixFinish(A);
+    HIPCHECK(hipStreamSynchronize(st));
+    API_END
+}
+
+int vol_matrix_get(vol_matrix A, double* diag, double* upper, double* lower, int location) {
+    API_BEGIN
+    if (!A || !A->set) throw VolError(VOL_ERR_INVALID_ARGUMENT, "matrix not set");
+    vol_mesh m = A->mesh;
+    vol_context c = m->ctx;
+    setDevice(c);
+    hipStream_t st = c->stream;
+    const hipMemcpyKind kind = location == VOL_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    if (diag) HIPCHECK(hipMemcpyAsync(diag, A->diag.p, sizeof(double) * m->n, kind, st));
+    if (upper && m->nf) HIPCHECK(hipMemcpyAsync(upper, A->upper.p, sizeof(double) * m->nf, kind, st));
+    if (lower && m->nf)
+        HIPCHECK(hipMemcpyAsync(lower, A->symmetric ? A->upper.p : A->lower.p, sizeof(double) * m->nf, kind, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    API_END
+}
+
+int vol_amul(vol_matrix A, const double* x, double* y, int location) {
+    API_BEGIN
+    if (!A || !A->set || !x || !y) throw VolError(VOL_ERR_INVALID_ARGUMENT, "matrix not set or null vector");
+    vol_mesh m = A->mesh;
+    vol_context c = m->ctx;
+    setDevice(c);
+    hipStream_t st = c->stream;
+    const int n = m->n;
+    const double* dx = x;
+    double* dy = y;
+    if (location != VOL_DEVICE) {
+        A->hx.alloc(n);
+        A->hy.alloc(n);
+        HIPCHECK(hipMemcpyAsync(A->hx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        dx = A->hx.p;
+        dy = A->hy.p;
+    }
+    Ctx C{c, st, nullptr, c->scal.p};
+    sell(C, OP_AMUL, XL_PLAIN, m->sell.view(), A->coef.p, A->diag.p, dx, nullptr, dy);
+    if (location != VOL_DEVICE) HIPCHECK(hipMemcpyAsync(y, dy, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    API_END
+}
+
+int vol_solve(vol_matrix A, const double* b, double* x, const vol_solver_controls* controls,
+              vol_solver_performance* perf, int location) {
+    API_BEGIN
+    if (!A || !A->set || !b || !x) throw VolError(VOL_ERR_INVALID_ARGUMENT, "matrix not set or null vector");
+    vol_solver_controls ct;
+    if (controls)
+        ct = *controls;
+    else
+        vol_controls_default(&ct);
+    vol_solver_performance pf;
+    vol_mesh m = A->mesh;
+    vol_context c = m->ctx;
+    setDevice(c);
+    const int n = m->n;
+    if (location == VOL_DEVICE) {
+        solveDevice(A, b, x, ct, pf);
+    } else {
+        hipStream_t st = c->stream;
+        A->vb.alloc(n);
+        A->vx.alloc(n);
+        HIPCHECK(hipMemcpyAsync(A->vb.p, b, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        HIPCHECK(hipMemcpyAsync(A->vx.p, x, sizeof(double) * n, hipMemcpyHostToDevice, st));
+        solveDevice(A, A->vb.p, A->vx.p, ct, pf);
+        HIPCHECK(hipMemcpyAsync(x, A->vx.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    if (perf) *perf = pf;
+    API_END
+}
+
+int vol_timing_enable(vol_context ctx, int on) {
+    API_BEGIN
+    if (!ctx) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null context");
+    ctx->timing = on != 0;
+    API_END
+}
+
+int vol_timing_read(vol_context ctx, double* amulTotalMs, int64_t* amulCount, int reset) {
+    API_BEGIN
+    if (!ctx) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null context");
+    if (amulTotalMs) *amulTotalMs = ctx->amulMs;
+    if (amulCount) *amulCount = ctx->amulCount;
+    if (reset) {
+        ctx->amulMs = 0;
+        ctx->amulCount = 0;
+    }
+    API_END
+}
+
+}  // extern "C"

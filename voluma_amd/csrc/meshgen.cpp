@@ -1,0 +1,478 @@
+// Mesh synthesis and finite-volume geometry for the solver's inputs (host C++20).
+//
+// The linear-solve path consumes LDU addressing (owner/neighbour) and, for GAMG,
+// the per-face agglomeration weight |S_f|·nonOrthDeltaCoeff_f. This module makes
+// those inputs for the benchmark meshes:
+//   * the box mesh with generateBox's point/face order (reference
+//     src/fvm/mesh/RawMesh.cpp:391-502): internal faces owner-major +x,+y,+z, then
+//     patches xMin,xMax,yMin,yMax,zMin,zMax;
+//   * a jittered tetrahedral mesh: an nx×ny×nz hex lattice, each hex split into the 6
+//     Kuhn tetrahedra around its main diagonal, interior vertices displaced uniformly
+//     by ±jitter·spacing per axis from a splitmix64 stream; internal faces in
+//     upper-triangular (owner, neighbour) order, boundary triangles grouped in the
+//     same six patches.
+// Geometry follows the reference's face/cell decomposition with the same operation
+// order (src/fvm/mesh/Mesh.cpp:21-48 faces, :135-154 cells, :600-628 face weights),
+// compiled without FP contraction, so weights are bit-identical to the reference's.
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace {
+
+struct V3 {
+    double x = 0, y = 0, z = 0;
+};
+inline V3 operator+(V3 a, const V3& b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline V3 operator-(V3 a, const V3& b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator*(double s, const V3& a) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 operator*(const V3& a, double s) { return {a.x * s, a.y * s, a.z * s}; }
+inline V3 operator/(const V3& a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+inline V3& operator+=(V3& a, const V3& b) { a.x += b.x; a.y += b.y; a.z += b.z; return a; }
+inline double dot(const V3& a, const V3& b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline V3 cross(const V3& a, const V3& b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline double mag(const V3& a) { return std::sqrt(dot(a, a)); }
+
+constexpr double VSMALL = 1.0e-300;
+
+struct Raw {
+    std::vector<V3> points;
+    std::vector<int64_t> faceOff{0}, facePts, owner, neighbour;
+    std::vector<int64_t> pStart, pSize;
+    int64_t nCells = 0;
+    // geometry (filled by computeGeometry)
+    bool haveGeom = false;
+    std::vector<double> V, magSf, w, ndc;
+    std::vector<V3> C, Sf, Cf, d, corr;
+    void addFace(std::initializer_list<int64_t> p) {
+        facePts.insert(facePts.end(), p.begin(), p.end());
+        faceOff.push_back(int64_t(facePts.size()));
+    }
+    int64_t nFaces() const { return int64_t(owner.size()); }
+    int64_t nInternal() const { return int64_t(neighbour.size()); }
+};
+
+Raw makeBox(int nx, int ny, int nz) {
+    if (nx < 1 || ny < 1 || nz < 1) throw std::invalid_argument("box: bad resolution");
+    Raw m;
+    auto P = [&](int i, int j, int k) -> int64_t { return i + int64_t(nx + 1) * (j + int64_t(ny + 1) * k); };
+    auto C = [&](int i, int j, int k) -> int64_t { return i + int64_t(nx) * (j + int64_t(ny) * k); };
+    // unit box: coordinate = 0 + (1 − 0)·(i/n), the reference's lo + (hi − lo)·t
+    auto coord = [](int i, int n) { return 0.0 + (1.0 - 0.0) * (double(i) / n); };
+    m.points.resize(std::size_t(nx + 1) * (ny + 1) * (nz + 1));
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 0; i <= nx; ++i) m.points[P(i, j, k)] = V3{coord(i, nx), coord(j, ny), coord(k, nz)};
+    m.nCells = int64_t(nx) * ny * nz;
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const int64_t c = C(i, j, k);
+                if (i + 1 < nx) {
+                    m.addFace({P(i + 1, j, k), P(i + 1, j + 1, k), P(i + 1, j + 1, k + 1), P(i + 1, j, k + 1)});
+                    m.owner.push_back(c);
+                    m.neighbour.push_back(C(i + 1, j, k));
+                }
+                if (j + 1 < ny) {
+                    m.addFace({P(i, j + 1, k), P(i, j + 1, k + 1), P(i + 1, j + 1, k + 1), P(i + 1, j + 1, k)});
+                    m.owner.push_back(c);
+                    m.neighbour.push_back(C(i, j + 1, k));
+                }
+                if (k + 1 < nz) {
+                    m.addFace({P(i, j, k + 1), P(i + 1, j, k + 1), P(i + 1, j + 1, k + 1), P(i, j + 1, k + 1)});
+                    m.owner.push_back(c);
+                    m.neighbour.push_back(C(i, j, k + 1));
+                }
+            }
+    auto begin = [&]() { m.pStart.push_back(m.nFaces()); };
+    auto end = [&]() { m.pSize.push_back(m.nFaces() - m.pStart.back()); };
+    for (int side = 0; side < 2; ++side) {  // xMin, xMax
+        begin();
+        const int i = side == 0 ? 0 : nx;
+        for (int k = 0; k < nz; ++k)
+            for (int j = 0; j < ny; ++j) {
+                if (side == 0)
+                    m.addFace({P(i, j, k), P(i, j, k + 1), P(i, j + 1, k + 1), P(i, j + 1, k)});
+                else
+                    m.addFace({P(i, j, k), P(i, j + 1, k), P(i, j + 1, k + 1), P(i, j, k + 1)});
+                m.owner.push_back(C(side == 0 ? 0 : nx - 1, j, k));
+            }
+        end();
+    }
+    for (int side = 0; side < 2; ++side) {  // yMin, yMax
+        begin();
+        const int j = side == 0 ? 0 : ny;
+        for (int k = 0; k < nz; ++k)
+            for (int i = 0; i < nx; ++i) {
+                if (side == 0)
+                    m.addFace({P(i, j, k), P(i + 1, j, k), P(i + 1, j, k + 1), P(i, j, k + 1)});
+                else
+                    m.addFace({P(i, j, k), P(i, j, k + 1), P(i + 1, j, k + 1), P(i + 1, j, k)});
+                m.owner.push_back(C(i, side == 0 ? 0 : ny - 1, k));
+            }
+        end();
+    }
+    for (int side = 0; side < 2; ++side) {  // zMin, zMax
+        begin();
+        const int k = side == 0 ? 0 : nz;
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                if (side == 0)
+                    m.addFace({P(i, j, k), P(i, j + 1, k), P(i + 1, j + 1, k), P(i + 1, j, k)});
+                else
+                    m.addFace({P(i, j, k), P(i + 1, j, k), P(i + 1, j + 1, k), P(i, j + 1, k)});
+                m.owner.push_back(C(i, j, side == 0 ? 0 : nz - 1));
+            }
+        end();
+    }
+    return m;
+}
+
+struct SplitMix64 {
+    uint64_t s;
+    uint64_t next() {
+        uint64_t z = (s += 0x9E3779B97F4A7C15ull);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        return z ^ (z >> 31);
+    }
+    double uniform() { return double(next() >> 11) * 0x1.0p-53; }  // [0, 1)
+};
+
+// Kuhn split of an nx×ny×nz lattice on the unit box into 6 tets per hex.
+Raw makeTet(int nx, int ny, int nz, double jitter, uint64_t seed) {
+    if (nx < 1 || ny < 1 || nz < 1) throw std::invalid_argument("tet: bad resolution");
+    if (!(jitter >= 0.0 && jitter < 0.25)) throw std::invalid_argument("tet: jitter must be in [0, 0.25)");
+    Raw m;
+    const int64_t NX = nx + 1, NY = ny + 1;
+    auto P = [&](int64_t i, int64_t j, int64_t k) -> int64_t { return i + NX * (j + NY * k); };
+    m.points.resize(std::size_t(NX) * NY * (nz + 1));
+    SplitMix64 rng{seed};
+    const double hx = 1.0 / nx, hy = 1.0 / ny, hz = 1.0 / nz;
+    for (int k = 0; k <= nz; ++k)
+        for (int j = 0; j <= ny; ++j)
+            for (int i = 0; i <= nx; ++i) {
+                V3 p{double(i) / nx, double(j) / ny, double(k) / nz};
+                if (i > 0 && i < nx && j > 0 && j < ny && k > 0 && k < nz && jitter > 0) {
+                    p.x += (2.0 * rng.uniform() - 1.0) * jitter * hx;
+                    p.y += (2.0 * rng.uniform() - 1.0) * jitter * hy;
+                    p.z += (2.0 * rng.uniform() - 1.0) * jitter * hz;
+                }
+                m.points[P(i, j, k)] = p;
+            }
+    static const int perms[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    const int64_t nHex = int64_t(nx) * ny * nz;
+    m.nCells = 6 * nHex;
+    // tet vertices as lattice indices (exact integers for orientation tests)
+    struct TetV {
+        int64_t v[4];
+        int32_t L[4][3];
+    };
+    auto tetOf = [&](int64_t cell) {
+        const int64_t hex = cell / 6, t = cell % 6;
+        const int i = int(hex % nx), j = int((hex / nx) % ny), k = int(hex / (int64_t(nx) * ny));
+        TetV T;
+        int c[3] = {i, j, k};
+        for (int s = 0; s < 4; ++s) {
+            if (s > 0) ++c[perms[t][s - 1]];
+            T.L[s][0] = c[0];
+            T.L[s][1] = c[1];
+            T.L[s][2] = c[2];
+            T.v[s] = P(c[0], c[1], c[2]);
+        }
+        return T;
+    };
+    // every tet face: key = sorted vertex triple packed in 63 bits
+    const int64_t nPts = int64_t(m.points.size());
+    int bits = 1;
+    while ((int64_t(1) << bits) < nPts) ++bits;
+    if (3 * bits > 63) throw std::invalid_argument("tet: mesh too large for face keys");
+    struct FI {
+        uint64_t key;
+        int64_t cell;
+        int32_t opp;  // local index of the vertex opposite the face
+    };
+    std::vector<FI> fi;
+    fi.reserve(std::size_t(m.nCells) * 4);
+    for (int64_t c = 0; c < m.nCells; ++c) {
+        const TetV T = tetOf(c);
+        for (int o = 0; o < 4; ++o) {
+            int64_t a[3], q = 0;
+            for (int s = 0; s < 4; ++s)
+                if (s != o) a[q++] = T.v[s];
+            std::sort(a, a + 3);
+            fi.push_back({(uint64_t(a[0]) << (2 * bits)) | (uint64_t(a[1]) << bits) | uint64_t(a[2]), c, o});
+        }
+    }
+    std::sort(fi.begin(), fi.end(), [](const FI& x, const FI& y) {
+        return x.key != y.key ? x.key < y.key : x.cell < y.cell;
+    });
+    // oriented triangle for the face of `cell` opposite local vertex `opp`: normal points away from opp
+    auto orient = [&](int64_t cell, int opp, int64_t out[3]) {
+        const TetV T = tetOf(cell);
+        int idx[3], q = 0;
+        for (int s = 0; s < 4; ++s)
+            if (s != opp) idx[q++] = s;
+        auto vec = [&](int a, int b, int64_t r[3]) {
+            for (int d = 0; d < 3; ++d) r[d] = int64_t(T.L[b][d]) - T.L[a][d];
+        };
+        int64_t e1[3], e2[3], e3[3];
+        vec(idx[0], idx[1], e1);
+        vec(idx[0], idx[2], e2);
+        vec(opp, idx[0], e3);
+        const int64_t nrm[3] = {e1[1] * e2[2] - e1[2] * e2[1], e1[2] * e2[0] - e1[0] * e2[2],
+                                e1[0] * e2[1] - e1[1] * e2[0]};
+        const int64_t s = nrm[0] * e3[0] + nrm[1] * e3[1] + nrm[2] * e3[2];
+        if (s == 0) throw std::logic_error("tet: degenerate face");
+        out[0] = T.v[idx[0]];
+        out[1] = s > 0 ? T.v[idx[1]] : T.v[idx[2]];
+        out[2] = s > 0 ? T.v[idx[2]] : T.v[idx[1]];
+    };
+    struct IF {
+        int64_t own, nei;
+        int32_t opp;
+    };
+    std::vector<IF> internal;
+    struct BF {
+        int patch;
+        int64_t own;
+        uint64_t key;
+        int32_t opp;
+    };
+    std::vector<BF> boundary;
+    internal.reserve(fi.size() / 2);
+    for (std::size_t a = 0; a < fi.size();) {
+        if (a + 1 < fi.size() && fi[a + 1].key == fi[a].key) {
+            internal.push_back({fi[a].cell, fi[a + 1].cell, fi[a].opp});  // cells ascending: own < nei
+            a += 2;
+        } else {
+            // boundary plane from lattice coordinates of its three vertices
+            const TetV T = tetOf(fi[a].cell);
+            int patch = -1;
+            const int lim[3] = {nx, ny, nz};
+            for (int d = 0; d < 3 && patch < 0; ++d)
+                for (int side = 0; side < 2 && patch < 0; ++side) {
+                    const int val = side == 0 ? 0 : lim[d];
+                    bool all = true;
+                    for (int s = 0; s < 4; ++s)
+                        if (s != fi[a].opp && T.L[s][d] != val) all = false;
+                    if (all) patch = 2 * d + side;
+                }
+            if (patch < 0) throw std::logic_error("tet: unmatched interior face");
+            boundary.push_back({patch, fi[a].cell, fi[a].key, fi[a].opp});
+            a += 1;
+        }
+    }
+    std::sort(internal.begin(), internal.end(), [](const IF& x, const IF& y) {
+        return x.own != y.own ? x.own < y.own : x.nei < y.nei;
+    });
+    std::sort(boundary.begin(), boundary.end(), [](const BF& x, const BF& y) {
+        if (x.patch != y.patch) return x.patch < y.patch;
+        if (x.own != y.own) return x.own < y.own;
+        return x.key < y.key;
+    });
+    for (const auto& f : internal) {
+        int64_t t[3];
+        orient(f.own, f.opp, t);
+        m.addFace({t[0], t[1], t[2]});
+        m.owner.push_back(f.own);
+        m.neighbour.push_back(f.nei);
+    }
+    for (int p = 0; p < 6; ++p) {
+        m.pStart.push_back(m.nFaces());
+        for (const auto& f : boundary) {
+            if (f.patch != p) continue;
+            int64_t t[3];
+            orient(f.own, f.opp, t);
+            m.addFace({t[0], t[1], t[2]});
+            m.owner.push_back(f.own);
+        }
+        m.pSize.push_back(m.nFaces() - m.pStart.back());
+    }
+    return m;
+}
+
+// Face and cell geometry in the reference's operation order.
+void computeGeometry(Raw& m) {
+    const int64_t nF = m.nFaces(), nI = m.nInternal(), nC = m.nCells;
+    m.Sf.assign(nF, V3{});
+    m.Cf.assign(nF, V3{});
+    for (int64_t f = 0; f < nF; ++f) {
+        const int64_t b = m.faceOff[f], e = m.faceOff[f + 1], n = e - b;
+        const auto& P = m.points;
+        const int64_t* fp = m.facePts.data() + b;
+        if (n == 3) {
+            m.Cf[f] = (P[fp[0]] + P[fp[1]] + P[fp[2]]) / 3.0;
+            m.Sf[f] = 0.5 * cross(P[fp[1]] - P[fp[0]], P[fp[2]] - P[fp[0]]);
+            continue;
+        }
+        V3 pAvg;
+        for (int64_t k = 0; k < n; ++k) pAvg += P[fp[k]];
+        pAvg = pAvg / double(n);
+        V3 sumN, sumAc;
+        double sumA = 0;
+        for (int64_t k = 0; k < n; ++k) {
+            const V3& p0 = P[fp[k]];
+            const V3& p1 = P[fp[(k + 1) % n]];
+            const V3 c = p0 + p1 + pAvg;
+            const V3 nk = cross(p1 - p0, pAvg - p0);
+            const double a = mag(nk);
+            sumN += nk;
+            sumA += a;
+            sumAc += a * c;
+        }
+        m.Cf[f] = sumA < VSMALL ? pAvg : sumAc / (3.0 * sumA);
+        m.Sf[f] = 0.5 * sumN;
+    }
+    // cell → face CSR over all faces in ascending face order
+    std::vector<int64_t> off(nC + 1, 0), list;
+    for (int64_t f = 0; f < nF; ++f) {
+        ++off[m.owner[f] + 1];
+        if (f < nI) ++off[m.neighbour[f] + 1];
+    }
+    for (int64_t c = 0; c < nC; ++c) off[c + 1] += off[c];
+    list.resize(off[nC]);
+    {
+        std::vector<int64_t> fill(off.begin(), off.end() - 1);
+        for (int64_t f = 0; f < nF; ++f) {
+            list[fill[m.owner[f]]++] = f;
+            if (f < nI) list[fill[m.neighbour[f]]++] = f;
+        }
+    }
+    m.C.assign(nC, V3{});
+    m.V.assign(nC, 0.0);
+    for (int64_t c = 0; c < nC; ++c) {
+        V3 cEst;
+        const int64_t nfc = off[c + 1] - off[c];
+        for (int64_t k = off[c]; k < off[c + 1]; ++k) cEst += m.Cf[list[k]];
+        cEst = cEst / double(nfc);
+        V3 cSum;
+        double vSum = 0;
+        for (int64_t k = off[c]; k < off[c + 1]; ++k) {
+            const int64_t f = list[k];
+            const V3 S = (m.owner[f] == c) ? m.Sf[f] : V3{-m.Sf[f].x, -m.Sf[f].y, -m.Sf[f].z};
+            const double pv = std::max(dot(S, m.Cf[f] - cEst), VSMALL);
+            cSum += pv * (0.75 * m.Cf[f] + 0.25 * cEst);
+            vSum += pv;
+        }
+        m.C[c] = cSum / vSum;
+        m.V[c] = vSum / 3.0;
+    }
+    m.d.assign(nF, V3{});
+    for (int64_t f = 0; f < nF; ++f)
+        m.d[f] = f < nI ? m.C[m.neighbour[f]] - m.C[m.owner[f]] : m.Cf[f] - m.C[m.owner[f]];
+    m.magSf.assign(nF, 0.0);
+    m.w.assign(nF, 0.0);
+    m.ndc.assign(nF, 0.0);
+    m.corr.assign(nF, V3{});
+    for (int64_t f = 0; f < nF; ++f) {
+        m.magSf[f] = mag(m.Sf[f]);
+        const V3 n = m.Sf[f] / m.magSf[f];
+        const V3& dd = m.d[f];
+        if (f < nI) {
+            const V3& CP = m.C[m.owner[f]];
+            const double sOwn = std::abs(dot(m.Sf[f], m.Cf[f] - CP));
+            const double sNei = std::abs(dot(m.Sf[f], CP + dd - m.Cf[f]));
+            m.w[f] = sNei / (sOwn + sNei);
+        } else {
+            m.w[f] = 1.0;
+        }
+        m.ndc[f] = 1.0 / std::max(dot(n, dd), 0.05 * mag(dd));
+        m.corr[f] = n - dd * m.ndc[f];
+    }
+    m.haveGeom = true;
+}
+
+thread_local std::string gErr;
+
+void copyV3(const std::vector<V3>& v, double* out) {
+    if (!out) return;
+    for (std::size_t i = 0; i < v.size(); ++i) {
+        out[3 * i] = v[i].x;
+        out[3 * i + 1] = v[i].y;
+        out[3 * i + 2] = v[i].z;
+    }
+}
+void copyD(const std::vector<double>& v, double* out) {
+    if (out && !v.empty()) std::memcpy(out, v.data(), sizeof(double) * v.size());
+}
+
+} // namespace
+
+extern "C" {
+
+const char* vm_last_error() { return gErr.c_str(); }
+
+void* vm_box(int nx, int ny, int nz) {
+    try {
+        return new Raw(makeBox(nx, ny, nz));
+    } catch (const std::exception& e) {
+        gErr = e.what();
+        return nullptr;
+    }
+}
+
+void* vm_tet(int nx, int ny, int nz, double jitter, uint64_t seed) {
+    try {
+        return new Raw(makeTet(nx, ny, nz, jitter, seed));
+    } catch (const std::exception& e) {
+        gErr = e.what();
+        return nullptr;
+    }
+}
+
+void vm_free(void* h) { delete static_cast<Raw*>(h); }
+
+// out: nPoints, nFaces, nInternal, nFacePoints, nCells, nPatches
+void vm_sizes(void* h, int64_t* out) {
+    const Raw& m = *static_cast<Raw*>(h);
+    out[0] = int64_t(m.points.size());
+    out[1] = m.nFaces();
+    out[2] = m.nInternal();
+    out[3] = int64_t(m.facePts.size());
+    out[4] = m.nCells;
+    out[5] = int64_t(m.pStart.size());
+}
+
+void vm_raw(void* h, double* pts, int64_t* faceOff, int64_t* facePts, int64_t* owner, int64_t* neighbour,
+            int64_t* pStart, int64_t* pSize) {
+    const Raw& m = *static_cast<Raw*>(h);
+    copyV3(m.points, pts);
+    std::memcpy(faceOff, m.faceOff.data(), sizeof(int64_t) * m.faceOff.size());
+    std::memcpy(facePts, m.facePts.data(), sizeof(int64_t) * m.facePts.size());
+    std::memcpy(owner, m.owner.data(), sizeof(int64_t) * m.owner.size());
+    if (!m.neighbour.empty()) std::memcpy(neighbour, m.neighbour.data(), sizeof(int64_t) * m.neighbour.size());
+    std::memcpy(pStart, m.pStart.data(), sizeof(int64_t) * m.pStart.size());
+    std::memcpy(pSize, m.pSize.data(), sizeof(int64_t) * m.pSize.size());
+}
+
+// Any output pointer may be null. Vectors are interleaved xyz.
+int vm_geometry(void* h, double* V, double* C, double* Sf, double* magSf, double* Cf, double* d, double* w,
+                double* ndc, double* corr) {
+    try {
+        Raw& m = *static_cast<Raw*>(h);
+        if (!m.haveGeom) computeGeometry(m);
+        copyD(m.V, V);
+        copyV3(m.C, C);
+        copyV3(m.Sf, Sf);
+        copyD(m.magSf, magSf);
+        copyV3(m.Cf, Cf);
+        copyV3(m.d, d);
+        copyD(m.w, w);
+        copyD(m.ndc, ndc);
+        copyV3(m.corr, corr);
+        return 0;
+    } catch (const std::exception& e) {
+        gErr = e.what();
+        return -1;
+    }
+}
+
+} // extern "C"

@@ -1,0 +1,596 @@
+// HIP kernels of the LDU linear-solve path for CDNA4 (gfx950, wave64).
+//
+// All kernels are memory-bound; none uses MFMA. Arithmetic is compiled with
+// -ffp-contract=off so that every a·b + c is a rounded product followed by a
+// rounded add, as in the reference (CMakeLists.txt:11).
+#include "device_common.hpp"
+#include "vol_internal.hpp"
+
+namespace vol {
+
+// ============================================================================ SELL gather
+// One wavefront per 64-row slice; lanes own rows. Block b is remapped so that each of
+// the 8 XCDs (blocks are dispatched round-robin over XCDs) walks a contiguous range of
+// slices: neighbouring rows share x gathers in that XCD's L2.
+constexpr int SELL_WAVES = 4;  // 256 threads per block
+constexpr int NXCD = 8;
+
+int sell_grid_blocks(int nslices) {
+    const int logical = (nslices + SELL_WAVES - 1) / SELL_WAVES;
+    return ((logical + NXCD - 1) / NXCD) * NXCD;
+}
+
+template <int XL>
+__device__ __forceinline__ double xload(const SellArgs& a, int j, double xu) {
+    if constexpr (XL == XL_PLAIN) return a.x[j];
+    if constexpr (XL == XL_UNIFORM) return xu;
+    if constexpr (XL == XL_FROMZERO) return a.omega * (a.r[j] / a.diag[j]);
+    if constexpr (XL == XL_PROLONG) return a.x[j] + a.xc[a.agg[j]];
+    if constexpr (XL == XL_ZEROPROLONG) return a.omega * (a.r[j] / a.diag[j]) + a.xc[a.agg[j]];
+    return 0.0;
+}
+
+template <int OP, int XL, bool DOT>
+__global__ __launch_bounds__(256) void k_sell(SellArgs a) {
+    if (a.done && *a.done) return;
+    const int nb = gridDim.x, per = nb / NXCD;
+    const int lb = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;  // XCD-contiguous logical block
+    const int slice = lb * SELL_WAVES + threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int row = slice * WAVE + lane;
+    const bool valid = slice < a.S.nslices && row < a.S.n;
+    double xu = 0.0;
+    if constexpr (XL == XL_UNIFORM) xu = *a.xuPtr;
+    double outv = 0.0;
+    if (valid) {
+        const int s0 = a.S.sliceOff[slice];
+        const int width = (a.S.sliceOff[slice + 1] - s0) / WAVE;
+        double acc;
+        if constexpr (OP == OP_AMUL) acc = a.diag[row] * xload<XL>(a, row, xu);
+        else if constexpr (OP == OP_RESID) acc = a.r[row] - a.diag[row] * xload<XL>(a, row, xu);
+        else acc = a.r[row];
+        for (int k = 0; k < width; ++k) {
+            const int idx = s0 + k * WAVE + lane;
+            const int j = a.S.col[idx];
+            if (j >= 0) {
+                const double t = a.coef[idx] * xload<XL>(a, j, xu);
+                if constexpr (OP == OP_AMUL) acc = acc + t;
+                else acc = acc - t;
+            }
+        }
+        if constexpr (OP == OP_JACOBI) {
+            const double J = acc / a.diag[row];
+            const double xo = xload<XL>(a, row, xu);
+            outv = xo + a.omega * (J - xo);
+        } else if constexpr (OP == OP_JACREF) {
+            outv = acc / a.diag[row];
+        } else {
+            outv = acc;
+        }
+        a.out[row] = outv;
+    }
+    if constexpr (DOT) {
+        __shared__ double sh[256 / WAVE];
+        const double v = block_sum(valid ? outv * a.dotv[row] : 0.0, sh);
+        if (threadIdx.x == 0) a.part[lb] = v;
+    }
+}
+
+template <int OP, int XL>
+static void sell_dispatch2(hipStream_t st, const SellArgs& a) {
+    const int nb = sell_grid_blocks(a.S.nslices);
+    if (nb == 0) return;
+    if (a.part)
+        hipLaunchKernelGGL((k_sell<OP, XL, true>), dim3(nb), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_sell<OP, XL, false>), dim3(nb), dim3(256), 0, st, a);
+}
+
+template <int OP>
+static void sell_dispatch1(hipStream_t st, int xl, const SellArgs& a) {
+    switch (xl) {
+    case XL_PLAIN: sell_dispatch2<OP, XL_PLAIN>(st, a); break;
+    case XL_UNIFORM: sell_dispatch2<OP, XL_UNIFORM>(st, a); break;
+    case XL_FROMZERO: sell_dispatch2<OP, XL_FROMZERO>(st, a); break;
+    case XL_PROLONG: sell_dispatch2<OP, XL_PROLONG>(st, a); break;
+    case XL_ZEROPROLONG: sell_dispatch2<OP, XL_ZEROPROLONG>(st, a); break;
+    }
+}
+
+void launch_sell(hipStream_t st, int op, int xl, const SellArgs& a) {
+    switch (op) {
+    case OP_AMUL: sell_dispatch1<OP_AMUL>(st, xl, a); break;
+    case OP_RESID: sell_dispatch1<OP_RESID>(st, xl, a); break;
+    case OP_JACOBI: sell_dispatch1<OP_JACOBI>(st, xl, a); break;
+    case OP_JACREF: sell_dispatch1<OP_JACREF>(st, xl, a); break;
+    }
+}
+
+// Slot coefficients from face arrays: coef[slot] = upper or lower of the slot's face.
+__global__ void k_sell_coef(SellView S, const double* upper, const double* lower, double* coef, float* coef32) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= S.nslots) return;
+    const int s = S.src[i];
+    const double v = s < 0 ? 0.0 : ((s & 1) ? lower[s >> 1] : upper[s >> 1]);
+    if (coef) coef[i] = v;
+    if (coef32) coef32[i] = float(v);
+}
+
+void launch_sell_coef(hipStream_t st, const SellView& S, const double* upper, const double* lower, double* coef,
+                      float* coef32) {
+    if (S.nslots == 0) return;
+    hipLaunchKernelGGL(k_sell_coef, dim3((S.nslots + 255) / 256), dim3(256), 0, st, S, upper, lower, coef, coef32);
+}
+
+// ============================================================================ vector kernels
+int vec_has_reduction(int op) {
+    switch (op) {
+    case V_SUM_X: case V_PRECOND_DOT: case V_DOT: case V_PCG_XR: case V_BI_S: case V_BMINUSAX_NORM: return 1;
+    case V_INIT: case V_BI_DOT2: return 2;
+    case V_BI_XR: return 1;
+    default: return 0;
+    }
+}
+
+// Element step of each vector op; returns the values to reduce in r0/r1.
+template <int OP>
+__device__ __forceinline__ void vec_elem(const VecArgs& A, int i, double& r0, double& r1) {
+    const DevScalars* S = A.S;
+    if constexpr (OP == V_SUM_X) {
+        r0 = A.a[i];
+    } else if constexpr (OP == V_INIT) {  // a = wA, b = b, c = pA; w = rA
+        const double rA = A.b[i] - A.a[i];
+        A.w[i] = rA;
+        r0 = fabs(A.a[i] - A.c[i]) + fabs(A.b[i] - A.c[i]);
+        r1 = fabs(rA);
+    } else if constexpr (OP == V_PRECOND_DOT) {  // a = r → w
+        const double w = A.rD ? A.rD[i] * A.a[i] : A.a[i];
+        A.w[i] = w;
+        r0 = w * A.a[i];
+    } else if constexpr (OP == V_DOT) {
+        r0 = A.a[i] * A.b[i];
+    } else if constexpr (OP == V_PCG_P) {  // a = w; x = p
+        A.x[i] = S->iterations == 0 ? A.a[i] : A.a[i] + S->beta * A.x[i];
+    } else if constexpr (OP == V_PCG_XR) {  // a = p, b = q; x = x, y = r
+        A.x[i] += S->alpha * A.a[i];
+        const double r = A.y[i] - S->alpha * A.b[i];
+        A.y[i] = r;
+        r0 = fabs(r);
+    } else if constexpr (OP == V_BI_P) {  // a = r, b = Ay; x = p
+        A.x[i] = S->iterations == 0 ? A.a[i] : A.a[i] + S->beta * (A.x[i] - S->omegaB * A.b[i]);
+    } else if constexpr (OP == V_SCALE) {
+        A.w[i] = A.rD ? A.rD[i] * A.a[i] : A.a[i];
+    } else if constexpr (OP == V_BI_S) {  // a = r, b = Ay; w = s
+        const double s = A.a[i] - S->alphaB * A.b[i];
+        A.w[i] = s;
+        r0 = fabs(s);
+    } else if constexpr (OP == V_BI_XEXIT) {  // a = y; x = x
+        A.x[i] += S->alphaB * A.a[i];
+    } else if constexpr (OP == V_BI_DOT2) {  // a = t, b = s
+        r0 = A.a[i] * A.a[i];
+        r1 = A.a[i] * A.b[i];
+    } else if constexpr (OP == V_BI_XR) {  // a = y, b = z, c = s, d = t; x = x, y = r
+        A.x[i] += S->alphaB * A.a[i] + S->omegaB * A.b[i];
+        const double r = A.c[i] - S->omegaB * A.d[i];
+        A.y[i] = r;
+        r0 = fabs(r);
+    } else if constexpr (OP == V_BMINUSAX_NORM) {  // a = wA, b = b; w = r
+        const double r = A.b[i] - A.a[i];
+        A.w[i] = r;
+        r0 = fabs(r);
+    } else if constexpr (OP == V_COPY) {
+        A.w[i] = A.a[i];
+    } else if constexpr (OP == V_RECIP) {
+        A.w[i] = 1.0 / A.a[i];
+    } else if constexpr (OP == V_FROMZERO) {  // a = r, b = d
+        A.w[i] = A.omega * (A.a[i] / A.b[i]);
+    }
+}
+
+template <int OP>
+__device__ __forceinline__ bool vec_should_run(const VecArgs& A) {
+    if constexpr (OP == V_BI_XEXIT) return A.S->exitPending != 0;
+    return !(A.done && *A.done);
+}
+
+template <int OP, int NRED>
+__global__ __launch_bounds__(RED_THREADS) void k_vec_fast(VecArgs A, double* part, int maxBlk) {
+    if (!vec_should_run<OP>(A)) return;
+    double s0 = 0.0, s1 = 0.0;
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < A.n; i += stride) {
+        double r0 = 0.0, r1 = 0.0;
+        vec_elem<OP>(A, i, r0, r1);
+        if constexpr (NRED > 0) s0 += r0;
+        if constexpr (NRED > 1) s1 += r1;
+    }
+    if constexpr (NRED > 0) {
+        __shared__ double sh[RED_THREADS / WAVE];
+        const double t0 = block_sum(s0, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = t0;
+        if constexpr (NRED > 1) {
+            const double t1 = block_sum(s1, sh);
+            if (threadIdx.x == 0) part[maxBlk + blockIdx.x] = t1;
+        }
+    }
+}
+
+template <int OP, int NRED>
+__global__ __launch_bounds__(RED_THREADS) void k_vec_exact(VecArgs A, long long* xpart, int* xnf, int maxBlk) {
+    if (!vec_should_run<OP>(A)) return;
+    __shared__ long long bins[NRED][NBINS];
+    __shared__ int nf[NRED];
+    for (int k = threadIdx.x; k < NRED * NBINS; k += blockDim.x) bins[k / NBINS][k % NBINS] = 0;
+    if (threadIdx.x < NRED) nf[threadIdx.x] = 0;
+    __syncthreads();
+    const int stride = gridDim.x * blockDim.x;
+    for (int base = blockIdx.x * blockDim.x; base < A.n; base += stride) {
+        const int i = base + threadIdx.x;
+        double r0 = 0.0, r1 = 0.0;
+        const bool act = i < A.n;
+        if (act) vec_elem<OP>(A, i, r0, r1);
+        exact_add_wave(bins[0], &nf[0], r0, act);
+        if constexpr (NRED > 1) exact_add_wave(bins[1], &nf[1], r1, act);
+    }
+    __syncthreads();
+    if (threadIdx.x < NRED) exact_normalize(bins[threadIdx.x]);
+    __syncthreads();
+    for (int k = threadIdx.x; k < NRED * NBINS; k += blockDim.x) {
+        const int q = k / NBINS, bn = k % NBINS;
+        xpart[(size_t(q) * maxBlk + blockIdx.x) * NBINS + bn] = bins[q][bn];
+    }
+    if (threadIdx.x < NRED) xnf[threadIdx.x * maxBlk + blockIdx.x] = nf[threadIdx.x];
+}
+
+template <int OP>
+static void vec_dispatch(hipStream_t st, const VecArgs& a, bool exact, const RedScratch& R) {
+    constexpr int NRED = (OP == V_INIT || OP == V_BI_DOT2) ? 2
+                         : (OP == V_SUM_X || OP == V_PRECOND_DOT || OP == V_DOT || OP == V_PCG_XR || OP == V_BI_S ||
+                            OP == V_BMINUSAX_NORM || OP == V_BI_XR)
+                             ? 1
+                             : 0;
+    if constexpr (NRED > 0) {
+        if (exact) {
+            hipLaunchKernelGGL((k_vec_exact<OP, NRED>), dim3(RED_BLOCKS_EXACT), dim3(RED_THREADS), 0, st, a, R.xpart,
+                               R.xnf, R.maxBlkExact);
+            return;
+        }
+    }
+    {
+        int nb = RED_BLOCKS;
+        if (NRED == 0) {
+            nb = (a.n + RED_THREADS - 1) / RED_THREADS;
+            if (nb > 4 * RED_BLOCKS) nb = 4 * RED_BLOCKS;
+            if (nb < 1) nb = 1;
+        }
+        hipLaunchKernelGGL((k_vec_fast<OP, NRED>), dim3(nb), dim3(RED_THREADS), 0, st, a, R.part, R.maxBlk);
+    }
+}
+
+void launch_vec(hipStream_t st, int op, const VecArgs& a, bool exact, const RedScratch& R) {
+    switch (op) {
+    case V_SUM_X: vec_dispatch<V_SUM_X>(st, a, exact, R); break;
+    case V_INIT: vec_dispatch<V_INIT>(st, a, exact, R); break;
+    case V_PRECOND_DOT: vec_dispatch<V_PRECOND_DOT>(st, a, exact, R); break;
+    case V_DOT: vec_dispatch<V_DOT>(st, a, exact, R); break;
+    case V_PCG_P: vec_dispatch<V_PCG_P>(st, a, exact, R); break;
+    case V_PCG_XR: vec_dispatch<V_PCG_XR>(st, a, exact, R); break;
+    case V_BI_P: vec_dispatch<V_BI_P>(st, a, exact, R); break;
+    case V_SCALE: vec_dispatch<V_SCALE>(st, a, exact, R); break;
+    case V_BI_S: vec_dispatch<V_BI_S>(st, a, exact, R); break;
+    case V_BI_XEXIT: vec_dispatch<V_BI_XEXIT>(st, a, exact, R); break;
+    case V_BI_DOT2: vec_dispatch<V_BI_DOT2>(st, a, exact, R); break;
+    case V_BI_XR: vec_dispatch<V_BI_XR>(st, a, exact, R); break;
+    case V_BMINUSAX_NORM: vec_dispatch<V_BMINUSAX_NORM>(st, a, exact, R); break;
+    case V_COPY: vec_dispatch<V_COPY>(st, a, exact, R); break;
+    case V_RECIP: vec_dispatch<V_RECIP>(st, a, exact, R); break;
+    case V_FROMZERO: vec_dispatch<V_FROMZERO>(st, a, exact, R); break;
+    }
+}
+
+// ============================================================================ finalize
+// One block: reduce the per-block partials in a fixed order (fast) or sum the exact
+// bins (exact), then run the solver's scalar step on thread 0.
+__device__ __forceinline__ bool dev_converged(const DevScalars& S) {
+    // LinearSolver.cpp:114-117
+    if (S.iterations < S.minIter) return false;
+    return S.finalResidual < S.tolerance || (S.relTol > 0 && S.finalResidual < S.relTol * S.initialResidual);
+}
+
+__global__ __launch_bounds__(256) void k_finalize(int op, int nred, int exact, int nparts, const double* part,
+                                                  int maxBlk, const long long* xpart, const int* xnf, int maxBlkExact,
+                                                  DevScalars* Sp, double* result) {
+    DevScalars& S = *Sp;
+    if (op != FIN_STORE && S.done) return;
+    __shared__ double vals[MAX_RED];
+    __shared__ double sh[256 / WAVE];
+    __shared__ long long bins[NBINS];
+    __shared__ int nfl;
+    for (int q = 0; q < nred; ++q) {
+        if (!exact) {
+            double s = 0.0;
+            for (int k = threadIdx.x; k < nparts; k += blockDim.x) s += part[size_t(q) * maxBlk + k];
+            const double t = block_sum(s, sh);
+            if (threadIdx.x == 0) vals[q] = t;
+        } else {
+            if (threadIdx.x == 0) nfl = 0;
+            __syncthreads();
+            if (threadIdx.x < NBINS) {
+                long long s = 0;
+                for (int k = 0; k < nparts; ++k) s += xpart[(size_t(q) * maxBlkExact + k) * NBINS + threadIdx.x];
+                bins[threadIdx.x] = s;
+            }
+            for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+                const int f = xnf[q * maxBlkExact + k];
+                if (f) atomicOr(&nfl, f);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) vals[q] = exact_to_double(bins, nfl);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double v0 = vals[0], v1 = nred > 1 ? vals[1] : 0.0;
+    switch (op) {
+    case FIN_XREF: S.xRef = v0 / double(S.n); break;
+    case FIN_INIT:
+        S.normFactor = v0 + 1e-20;
+        S.residualSum = v1;
+        S.initialResidual = S.residualSum / S.normFactor;
+        S.finalResidual = S.initialResidual;
+        S.iterations = 0;
+        if (S.initialResidual < S.tolerance && S.minIter <= 0) S.done = 1;
+        break;
+    case FIN_PCG_WRA:
+        S.wArAold = S.wArA;
+        S.wArA = v0;
+        if (S.iterations != 0) S.beta = S.wArA / S.wArAold;
+        break;
+    case FIN_PCG_WAPA:
+        S.wApA = v0;
+        if (fabs(S.wApA) < VSMALL) {
+            S.done = 1;
+        } else {
+            S.alpha = S.wArA / S.wApA;
+        }
+        break;
+    case FIN_PCG_NORM:
+        S.finalResidual = v0 / S.normFactor;
+        ++S.iterations;
+        if (!(S.iterations < S.maxIter && !dev_converged(S))) S.done = 1;
+        break;
+    case FIN_BI_RHO:
+        S.rhoOld = S.rho;
+        S.rho = v0;
+        if (S.iterations != 0) {
+            if (fabs(S.rhoOld) < VSMALL || fabs(S.omegaB) < VSMALL) {
+                S.done = 1;
+            } else {
+                S.beta = (S.rho / S.rhoOld) * (S.alphaB / S.omegaB);
+            }
+        }
+        break;
+    case FIN_BI_ALPHA:
+        if (fabs(v0) < VSMALL) {
+            S.done = 1;
+        } else {
+            S.alphaB = S.rho / v0;
+        }
+        break;
+    case FIN_BI_S:
+        ++S.iterations;
+        S.finalResidual = v0 / S.normFactor;
+        if (dev_converged(S)) {
+            S.exitPending = 1;
+            S.done = 1;
+        }
+        break;
+    case FIN_BI_OMEGA:
+        S.tAtA = v0;
+        S.omegaB = S.tAtA > VSMALL ? v1 / S.tAtA : 0.0;
+        break;
+    case FIN_BI_NORM:
+        S.finalResidual = v0 / S.normFactor;
+        if (!(S.iterations < S.maxIter && !dev_converged(S))) S.done = 1;
+        break;
+    case FIN_JAC_NORM:
+        S.iterations += S.nSweeps;
+        S.finalResidual = v0 / S.normFactor;
+        if (!(S.iterations < S.maxIter && !dev_converged(S))) S.done = 1;
+        break;
+    case FIN_STORE:
+        if (result) {
+            result[0] = v0;
+            if (nred > 1) result[1] = v1;
+        }
+        break;
+    }
+}
+
+void launch_finalize(hipStream_t st, int op, int nred, bool exact, int nparts, const RedScratch& R, DevScalars* S) {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, op, nred, exact ? 1 : 0, nparts, R.part, R.maxBlk,
+                       R.xpart, R.xnf, R.maxBlkExact, S, R.result);
+}
+
+__global__ void k_init_scalars(DevScalars* S, double tol, double relTol, int maxIter, int minIter, int nSweeps, int n) {
+    DevScalars z = {};
+    z.tolerance = tol;
+    z.relTol = relTol;
+    z.maxIter = maxIter;
+    z.minIter = minIter;
+    z.nSweeps = nSweeps;
+    z.n = n;
+    z.wArA = GREAT;  // LinearSolver.cpp:180
+    *S = z;
+}
+
+__global__ void k_clear_exit(DevScalars* S) { S->exitPending = 0; }
+
+void launch_clear_exit(hipStream_t st, DevScalars* S) { hipLaunchKernelGGL(k_clear_exit, dim3(1), dim3(1), 0, st, S); }
+
+void launch_init_scalars(hipStream_t st, DevScalars* S, double tol, double relTol, int maxIter, int minIter,
+                         int nSweeps, int n) {
+    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, st, S, tol, relTol, maxIter, minIter, nSweeps, n);
+}
+
+// ============================================================================ GAMG
+// Restriction: rc[I] = Σ res over the fine cells of aggregate I in ascending order,
+// which is the order of the reference's `rc[agg[i]] += res[i]` loop (GAMG.cpp:434-435).
+__global__ void k_restrict(int nc, const int* mOff, const int* mList, const double* res, double* rc, const int* done) {
+    if (done && *done) return;
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= nc) return;
+    double s = 0.0;
+    for (int k = mOff[I]; k < mOff[I + 1]; ++k) s += res[mList[k]];
+    rc[I] = s;
+}
+
+void launch_restrict(hipStream_t st, int nc, const int* mOff, const int* mList, const double* res, double* rc,
+                     const int* done) {
+    if (nc == 0) return;
+    hipLaunchKernelGGL(k_restrict, dim3((nc + 255) / 256), dim3(256), 0, st, nc, mOff, mList, res, rc, done);
+}
+
+// Galerkin coarsening (GAMG.cpp:279-301) as gathers over fine contributions listed in
+// the reference's accumulation order: per coarse cell the fine diagonals (ascending
+// cell), then its intra-aggregate faces (ascending face); per coarse face its fine faces
+// (ascending face), swapped when the fine face is flipped. cfList entries are
+// (fineFace << 1) | flip.
+__global__ void k_galerkin_cells(int nc, const int* mOff, const int* mList, const int* iOff, const int* iList,
+                                 const double* Fd, const double* Fu, const double* Fl, double* Cd) {
+    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= nc) return;
+    double s = 0.0;
+    for (int k = mOff[I]; k < mOff[I + 1]; ++k) s += Fd[mList[k]];
+    for (int k = iOff[I]; k < iOff[I + 1]; ++k) {
+        const int f = iList[k];
+        s += Fu[f] + Fl[f];
+    }
+    Cd[I] = s;
+}
+
+__global__ void k_galerkin_faces(int ncf, const int* cfOff, const int* cfList, const double* Fu, const double* Fl,
+                                 double* Cu, double* Cl) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= ncf) return;
+    double u = 0.0, l = 0.0;
+    for (int k = cfOff[c]; k < cfOff[c + 1]; ++k) {
+        const int e = cfList[k], f = e >> 1;
+        if (!(e & 1)) {
+            u += Fu[f];
+            l += Fl[f];
+        } else {
+            u += Fl[f];
+            l += Fu[f];
+        }
+    }
+    Cu[c] = u;
+    Cl[c] = l;
+}
+
+void launch_galerkin(hipStream_t st, int nc, int ncf, const int* mOff, const int* mList, const int* iOff,
+                     const int* iList, const int* cfOff, const int* cfList, const double* Fd, const double* Fu,
+                     const double* Fl, double* Cd, double* Cu, double* Cl) {
+    if (nc > 0)
+        hipLaunchKernelGGL(k_galerkin_cells, dim3((nc + 255) / 256), dim3(256), 0, st, nc, mOff, mList, iOff, iList,
+                           Fd, Fu, Fl, Cd);
+    if (ncf > 0)
+        hipLaunchKernelGGL(k_galerkin_faces, dim3((ncf + 255) / 256), dim3(256), 0, st, ncf, cfOff, cfList, Fu, Fl, Cu,
+                           Cl);
+}
+
+// Level-0 coefficients oriented to (l, u) = (min, max) cell (GAMG.cpp:266-278).
+__global__ void k_orient0(int nf, const char* ownerIsL, const double* upper, const double* lower, double* Fu,
+                          double* Fl) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= nf) return;
+    const bool o = ownerIsL[f] != 0;
+    Fu[f] = o ? upper[f] : lower[f];
+    Fl[f] = o ? lower[f] : upper[f];
+}
+
+void launch_orient0(hipStream_t st, int nf, const char* ownerIsL, const double* upper, const double* lower,
+                    double* Fu, double* Fl) {
+    if (nf == 0) return;
+    hipLaunchKernelGGL(k_orient0, dim3((nf + 255) / 256), dim3(256), 0, st, nf, ownerIsL, upper, lower, Fu, Fl);
+}
+
+// Dense coarsest matrix (GAMG.cpp:321-326): lu[i][i] = d[i]; lu[l][u] += upper; lu[u][l] += lower.
+__global__ void k_coarse_assemble(int n, int nf, const int* l, const int* u, const double* d, const double* cu,
+                                  const double* cl, double* lu) {
+    for (int k = threadIdx.x; k < n * n; k += blockDim.x) lu[k] = 0.0;
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) lu[size_t(i) * n + i] = d[i];
+    __syncthreads();
+    if (threadIdx.x == 0)  // faces in order: a pair (l,u) may repeat only across levels, keep the reference order
+        for (int f = 0; f < nf; ++f) {
+            lu[size_t(l[f]) * n + u[f]] += cu[f];
+            lu[size_t(u[f]) * n + l[f]] += cl[f];
+        }
+}
+
+void launch_coarse_assemble(hipStream_t st, int n, int nf, const int* l, const int* u, const double* d,
+                            const double* cu, const double* cl, double* lu) {
+    hipLaunchKernelGGL(k_coarse_assemble, dim3(1), dim3(256), 0, st, n, nf, l, u, d, cu, cl, lu);
+}
+
+// No-pivot Doolittle LU (GAMG.cpp:334-343). Within step k every (i, j) update is
+// independent, so the parallel form performs the same rounded operations.
+__global__ void k_coarse_lu(int n, double* lu, int* status) {
+    __shared__ int bad;
+    if (threadIdx.x == 0) bad = 0;
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        const double piv = lu[size_t(k) * n + k];
+        if (fabs(piv) < VSMALL) {
+            if (threadIdx.x == 0) bad = 1;
+            break;
+        }
+        for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) lu[size_t(i) * n + k] = lu[size_t(i) * n + k] / piv;
+        __syncthreads();
+        const int m = n - k - 1;
+        for (int t = threadIdx.x; t < m * m; t += blockDim.x) {
+            const int i = k + 1 + t / m, j = k + 1 + t % m;
+            const double fct = lu[size_t(i) * n + k];
+            if (fct == 0.0) continue;
+            lu[size_t(i) * n + j] -= fct * lu[size_t(k) * n + j];
+        }
+        __syncthreads();
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && status) status[0] = bad ? -1 : 0;
+}
+
+void launch_coarse_lu(hipStream_t st, int n, double* lu, int* status) {
+    hipLaunchKernelGGL(k_coarse_lu, dim3(1), dim3(256), 0, st, n, lu, status);
+}
+
+// Coarsest solve with the LU factors: forward substitution column by column (for every
+// row the subtractions run in ascending k, as in GAMG.cpp:413-414), then backward
+// substitution column by column from the last one (descending k per row, then the
+// division by the pivot).
+__global__ void k_coarse_solve(int n, const double* lu, const double* r, double* x, const int* done) {
+    if (done && *done) return;
+    extern __shared__ double g[];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) g[i] = r[i];
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        const double gk = g[k];
+        for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) g[i] -= lu[size_t(i) * n + k] * gk;
+        __syncthreads();
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        if (threadIdx.x == 0) g[k] /= lu[size_t(k) * n + k];
+        __syncthreads();
+        const double gk = g[k];
+        for (int i = threadIdx.x; i < k; i += blockDim.x) g[i] -= lu[size_t(i) * n + k] * gk;
+        __syncthreads();
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = g[i];
+}
+
+void launch_coarse_solve(hipStream_t st, int n, const double* lu, const double* r, double* x, const int* done) {
+    hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(256), sizeof(double) * n, st, n, lu, r, x, done);
+}
+
+}  // namespace vol

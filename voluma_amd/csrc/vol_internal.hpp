@@ -1,0 +1,189 @@
+// Internal interfaces between the host-side solver (api.cpp) and the HIP kernels
+// (solver_kernels.hip). Nothing here is part of the public C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace vol {
+
+constexpr int WAVE = 64;   // CDNA wavefront
+constexpr int NBINS = 72;  // exact-sum bins (par::ExactSum::NBINS, Comm.h:44)
+
+// ------------------------------------------------------------------ SELL-64 layout
+// Off-diagonal rows of an LDU matrix in sliced-ELLPACK form with slice height 64
+// (one wavefront): rows [64s, 64s+64) occupy slots sliceOff[s] .. sliceOff[s+1),
+// slot k of a row in slice s at sliceOff[s] + 64k + lane. Slots of a row keep
+// ascending face order, which is the per-row accumulation order of the reference's
+// face loop (src/fvm/linalg/LinearSolver.cpp:121-133). Padding slots have col = −1.
+struct SellView {
+    int n = 0;          // rows
+    int nslices = 0;
+    int nslots = 0;     // padded slot count
+    const int* sliceOff = nullptr;
+    const int* col = nullptr;  // neighbour row, −1 = padding
+    const int* src = nullptr;  // (face << 1) | side: side 0 → upper[face], 1 → lower[face]; −1 padding
+};
+
+// ------------------------------------------------------------------ device scalars
+// Krylov scalars and loop control of one solve, kept on the device so that the
+// iteration needs no host round trip. Kernels after a finalize read them.
+struct DevScalars {
+    double normFactor, initialResidual, finalResidual, residualSum, xRef;
+    double wArA, wArAold, wApA, alpha, beta;           // PCG
+    double rho, rhoOld, alphaB, omegaB, tAtA, tAsA;    // PBiCGStab
+    double tolerance, relTol;
+    int maxIter, minIter, nSweeps, n;
+    int iterations;
+    int done;         // loop finished (converged, maxIter or breakdown): later kernels skip
+    int exitPending;  // PBiCGStab exit on s: x += αy still to apply
+    int status;
+    int pad[4];
+};
+
+enum FinOp : int {
+    FIN_XREF = 0,      // xRef = Σx / n
+    FIN_INIT = 1,      // normFactor, residualSum, initial residual, early exit
+    FIN_PCG_WRA = 2,
+    FIN_PCG_WAPA = 3,
+    FIN_PCG_NORM = 4,
+    FIN_BI_RHO = 5,
+    FIN_BI_ALPHA = 6,
+    FIN_BI_S = 7,
+    FIN_BI_OMEGA = 8,
+    FIN_BI_NORM = 9,
+    FIN_JAC_NORM = 10,
+    FIN_STORE = 11,    // only store the reduced values (tests/diagnostics)
+};
+
+// Reduction scratch: fast mode uses `nblk` double partials per reduced quantity;
+// exact mode uses `nblk` × 72 int64 bins + a non-finite flag per block.
+struct RedScratch {
+    double* part = nullptr;     // [maxRed][maxBlk]
+    long long* xpart = nullptr; // [maxRed][maxBlkExact][72]
+    int* xnf = nullptr;         // [maxRed][maxBlkExact]
+    double* result = nullptr;   // [maxRed] (FIN_STORE)
+    int maxBlk = 0, maxBlkExact = 0;
+};
+
+constexpr int RED_BLOCKS = 1024;       // grid of fast reduction kernels
+constexpr int RED_BLOCKS_EXACT = 512;  // grid of exact reduction kernels
+constexpr int RED_THREADS = 256;
+constexpr int MAX_RED = 2;
+
+// ------------------------------------------------------------------ SELL gather kernels
+enum SellOp : int {
+    OP_AMUL = 0,    // out = diag∘x + Σ a·x_nb                       (LinearSolver.cpp:121-133)
+    OP_RESID = 1,   // out = r − diag∘x − Σ a·x_nb, in that order     (GAMG.cpp:387-395)
+    OP_JACOBI = 2,  // out = x + ω((r − Σ a·x_nb)/d − x)             (project smoother)
+    OP_JACREF = 3,  // out = (r − Σ a·x_nb)/d                        (LinearSolver.cpp:247-267)
+};
+enum XLoad : int {
+    XL_PLAIN = 0,        // x[j]
+    XL_UNIFORM = 1,      // a scalar
+    XL_FROMZERO = 2,     // ω·(r[j]/d[j]): the first Jacobi sweep from x = 0
+    XL_PROLONG = 3,      // x[j] + xc[agg[j]]
+    XL_ZEROPROLONG = 4,  // ω·(r[j]/d[j]) + xc[agg[j]]
+};
+
+struct SellArgs {
+    SellView S;
+    const double* coef = nullptr;
+    const double* diag = nullptr;
+    const double* x = nullptr;
+    const double* r = nullptr;
+    const double* xc = nullptr;
+    const int* agg = nullptr;
+    const double* xuPtr = nullptr;  // XL_UNIFORM value (device scalar)
+    double omega = 1.0;
+    double* out = nullptr;
+    const double* dotv = nullptr;   // fused dot: Σ out·dotv into part (fast mode only)
+    double* part = nullptr;
+    const int* done = nullptr;
+};
+
+void launch_sell(hipStream_t st, int op, int xl, const SellArgs& a);
+int sell_grid_blocks(int nslices);  // blocks used by launch_sell (== partial count when fused)
+
+void launch_sell_coef(hipStream_t st, const SellView& S, const double* upper, const double* lower, double* coef,
+                      float* coef32);
+
+// ------------------------------------------------------------------ vector kernels
+// Reductions: results reduced per block into scratch, then a 1-block finalize.
+// `exact` selects the ExactSum accumulator.
+enum VecOp : int {
+    V_SUM_X = 0,          // red0 = Σ a
+    V_INIT = 1,           // rA = b − wA; red0 = Σ(|wA − pA| + |b − pA|), red1 = Σ|rA|
+    V_PRECOND_DOT = 2,    // w = rD∘r (rD == null: w = r); red0 = Σ w·r
+    V_DOT = 3,            // red0 = Σ a·b
+    V_PCG_P = 4,          // p = (it == 0) ? w : w + β p
+    V_PCG_XR = 5,         // x += α p; r −= α q; red0 = Σ|r|
+    V_BI_P = 6,           // p = (it == 0) ? r : r + β(p − ω Ay)
+    V_SCALE = 7,          // w = rD∘r (or copy)
+    V_BI_S = 8,           // s = r − α Ay; red0 = Σ|s|
+    V_BI_XEXIT = 9,       // if exitPending: x += α y
+    V_BI_DOT2 = 10,       // red0 = Σ t·t, red1 = Σ t·s
+    V_BI_XR = 11,         // x += α y + ω z; r = s − ω t; red0 = Σ|r|
+    V_BMINUSAX_NORM = 12, // r = b − wA; red0 = Σ|r|
+    V_COPY = 13,          // a → w
+    V_RECIP = 14,         // w = 1/a
+    V_FROMZERO = 15,      // w = ω·(r/d)
+};
+
+struct VecArgs {
+    int n = 0;
+    const double *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr, *e = nullptr;
+    double *w = nullptr, *x = nullptr, *y = nullptr;
+    const double* rD = nullptr;
+    double omega = 1.0;
+    DevScalars* S = nullptr;
+    const int* done = nullptr;
+};
+
+void launch_vec(hipStream_t st, int op, const VecArgs& a, bool exact, const RedScratch& R);
+void launch_finalize(hipStream_t st, int op, int nred, bool exact, int nparts, const RedScratch& R, DevScalars* S);
+int vec_has_reduction(int op);
+
+void launch_clear_exit(hipStream_t st, DevScalars* S);
+void launch_init_scalars(hipStream_t st, DevScalars* S, double tol, double relTol, int maxIter, int minIter,
+                         int nSweeps, int n);
+
+// ------------------------------------------------------------------ GAMG kernels
+void launch_restrict(hipStream_t st, int nc, const int* mOff, const int* mList, const double* res, double* rc,
+                     const int* done);
+void launch_galerkin(hipStream_t st, int nc, int ncf, const int* mOff, const int* mList, const int* iOff,
+                     const int* iList, const int* cfOff, const int* cfList, const double* Fd, const double* Fu,
+                     const double* Fl, double* Cd, double* Cu, double* Cl);
+void launch_orient0(hipStream_t st, int nf, const char* ownerIsL, const double* upper, const double* lower,
+                    double* Fu, double* Fl);
+// Dense LU of the coarsest level, one workgroup. lu: n×n row-major (in/out). status[0] = −1 if singular.
+void launch_coarse_assemble(hipStream_t st, int n, int nf, const int* l, const int* u, const double* d,
+                            const double* cu, const double* cl, double* lu);
+void launch_coarse_lu(hipStream_t st, int n, double* lu, int* status);
+void launch_coarse_solve(hipStream_t st, int n, const double* lu, const double* r, double* x, const int* done);
+
+// ------------------------------------------------------------------ assembly
+enum AsmKind : int { ASM_LAPLACIAN = 0, ASM_DIV = 1 };
+struct AsmArgs {
+    int n = 0, nf = 0;                  // cells, internal faces
+    SellView S;                         // level-0 slots of the mesh (src = face << 1 | side)
+    const int* bOff = nullptr;          // cell → boundary faces (local index bf), ascending
+    const int* bList = nullptr;
+    const double* gamma = nullptr;      // per face (internal then boundary), or uniform gammaU
+    double gammaU = 1.0;
+    const double* magSf = nullptr;      // per face
+    const double* ndc = nullptr;        // per face
+    const double* flux = nullptr;       // per face (div)
+    const int* bcFixed = nullptr;       // per boundary face: fixedValue (1) / zeroGradient (0)
+    const double* bcValue = nullptr;    // per boundary face (nullable → 0)
+    double* diag = nullptr;             // out: folded diagonal
+    double* upper = nullptr;            // out
+    double* lower = nullptr;            // out (div)
+    double* icOut = nullptr;            // out (nullable): internalCoeffs
+    double* bsOut = nullptr;            // out (nullable): boundarySource
+    const double* bIn = nullptr;        // in (nullable): source before the fold
+    double* bOut = nullptr;             // out (nullable): folded source
+};
+void launch_assemble(hipStream_t st, int kind, const AsmArgs& a);
+
+}  // namespace vol
