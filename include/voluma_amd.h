@@ -21,7 +21,10 @@
  * the message of the calling thread's last failure. Non-convergence is not an error
  * (perf.converged = 0), as in the reference (LinearSolver.cpp:283).
  * Host pointers are copied; with VOL_DEVICE the pointers are device memory of the
- * context's GPU and stay where they are.
+ * context's GPU and stay where they are. Handles keep their parents alive (matrix →
+ * mesh → context, device allocations → context): destroy them in any order.
+ * Calls on one context are serialised on its stream; do not call into one context
+ * from two threads at once.
  */
 #ifndef VOLUMA_AMD_H
 #define VOLUMA_AMD_H

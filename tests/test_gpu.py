@@ -221,3 +221,20 @@ def test_full_size_reproducible_pcg_bit_exact(ctx, tag, name):
     idx = np.array(f["gamg_pcg_1e-10"]["x"]["sample_idx"])
     xs = np.array(f["gamg_pcg_1e-10"]["x"]["sample"])
     assert np.abs(x2[idx] - xs).max() <= 2e-3 * np.abs(xs).max()
+
+
+def test_handle_lifetimes_any_destroy_order():
+    """Handles keep their parents alive: closing the context and mesh first leaves a
+    working matrix; closing it last frees everything."""
+    c = va.Context(0)
+    g = load_small("box8x6x5")
+    m = SMALL_MESHES["box8x6x5"]()
+    lm = va.LduMesh(c, m.n_cells, m.owner[: m.n_internal], m.neighbour, m.agglomeration_weights())
+    A = va.LduMatrix(lm, g["sym_diag"], g["sym_upper"])
+    d = va.DeviceArray.from_host(c, g["amul_x"])
+    c.close()
+    lm.close()
+    assert np.array_equal(va.amul(A, g["amul_x"]), g["amul_y_sym"])
+    assert np.array_equal(d.download(), g["amul_x"])
+    A.close()
+    d.free()

@@ -185,8 +185,9 @@ class DeviceArray:
 
     def __init__(self, ctx: Context, n: int, dtype=np.float64):
         self.ctx, self.n, self.dtype = ctx, int(n), np.dtype(dtype)
+        self._ctxh = ctx._h  # the allocation holds its own reference to the context
         p = ctypes.c_void_p()
-        _check(lib().vol_device_alloc(ctx._h, self.n * self.dtype.itemsize, ctypes.byref(p)))
+        _check(lib().vol_device_alloc(self._ctxh, self.n * self.dtype.itemsize, ctypes.byref(p)))
         self.ptr = p
 
     @classmethod
@@ -199,16 +200,16 @@ class DeviceArray:
     def upload(self, a):
         a = np.ascontiguousarray(a, dtype=self.dtype)
         assert a.size == self.n
-        _check(lib().vol_memcpy_to_device(self.ctx._h, self.ptr, _p(a), a.nbytes))
+        _check(lib().vol_memcpy_to_device(self._ctxh, self.ptr, _p(a), a.nbytes))
 
     def download(self):
         out = np.empty(self.n, self.dtype)
-        _check(lib().vol_memcpy_to_host(self.ctx._h, _p(out), self.ptr, out.nbytes))
+        _check(lib().vol_memcpy_to_host(self._ctxh, _p(out), self.ptr, out.nbytes))
         return out
 
     def free(self):
         if getattr(self, "ptr", None):
-            lib().vol_device_free(self.ctx._h, self.ptr)
+            lib().vol_device_free(self._ctxh, self.ptr)
             self.ptr = None
 
     def __del__(self):

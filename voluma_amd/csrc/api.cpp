@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
@@ -252,6 +253,7 @@ struct LevelDev {
 
 // ============================================================================ handles
 struct vol_context_s {
+    std::atomic<int> refs{1};  // caller handle + meshes + device allocations
     int device = 0;
     hipStream_t stream = nullptr;
     bool reproducible = false;
@@ -288,6 +290,7 @@ struct vol_context_s {
 };
 
 struct vol_mesh_s {
+    std::atomic<int> refs{1};  // caller handle + matrices
     vol_context ctx = nullptr;
     int n = 0, nf = 0;
     std::vector<int> own, nei;
@@ -902,6 +905,27 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
 
 }  // namespace
 
+// Handles keep their parents alive (matrix → mesh → context), so callers may destroy
+// them in any order; the last release frees.
+void ctxRelease(vol_context c) {
+    if (c->refs.fetch_sub(1) != 1) return;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (auto e : c->evPool) (void)hipEventDestroy(e);
+    if (c->hostScal) (void)hipHostFree(c->hostScal);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+void meshRelease(vol_mesh m) {
+    if (m->refs.fetch_sub(1) != 1) return;
+    vol_context c = m->ctx;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    delete m;
+    ctxRelease(c);
+}
+
 // ============================================================================ C ABI
 extern "C" {
 
@@ -944,12 +968,7 @@ int vol_context_create(int device, vol_context* out) {
 int vol_context_destroy(vol_context ctx) {
     API_BEGIN
     if (!ctx) return VOL_OK;
-    (void)hipSetDevice(ctx->device);
-    (void)hipStreamSynchronize(ctx->stream);
-    for (auto e : ctx->evPool) (void)hipEventDestroy(e);
-    if (ctx->hostScal) (void)hipHostFree(ctx->hostScal);
-    (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
+    ctxRelease(ctx);
     API_END
 }
 
@@ -973,20 +992,24 @@ int vol_device_alloc(vol_context ctx, uint64_t bytes, void** ptr) {
     if (!ctx || !ptr) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
     setDevice(ctx);
     HIPCHECK(hipMalloc(ptr, std::max<uint64_t>(bytes, 1)));
+    ctx->refs.fetch_add(1);
     API_END
 }
 
 int vol_device_free(vol_context ctx, void* ptr) {
     API_BEGIN
     if (!ctx) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null context");
+    if (!ptr) return VOL_OK;
     setDevice(ctx);
     HIPCHECK(hipStreamSynchronize(ctx->stream));
-    if (ptr) HIPCHECK(hipFree(ptr));
+    HIPCHECK(hipFree(ptr));
+    ctxRelease(ctx);
     API_END
 }
 
 int vol_memcpy_to_device(vol_context ctx, void* dst, const void* src, uint64_t bytes) {
     API_BEGIN
+    if (!ctx || ((!dst || !src) && bytes)) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
     setDevice(ctx);
     HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
@@ -995,6 +1018,7 @@ int vol_memcpy_to_device(vol_context ctx, void* dst, const void* src, uint64_t b
 
 int vol_memcpy_to_host(vol_context ctx, void* dst, const void* src, uint64_t bytes) {
     API_BEGIN
+    if (!ctx || ((!dst || !src) && bytes)) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
     setDevice(ctx);
     HIPCHECK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
@@ -1021,6 +1045,7 @@ int vol_mesh_create(vol_context ctx, int nCells, int nInternalFaces, const int32
     m->nei.assign(neighbour, neighbour + nInternalFaces);
     setupMesh(m.get(), gamgWeights, mergeLevels > 0 ? mergeLevels : 2);
     HIPCHECK(hipStreamSynchronize(ctx->stream));
+    ctx->refs.fetch_add(1);
     *out = m.release();
     API_END
 }
@@ -1028,9 +1053,7 @@ int vol_mesh_create(vol_context ctx, int nCells, int nInternalFaces, const int32
 int vol_mesh_destroy(vol_mesh mesh) {
     API_BEGIN
     if (!mesh) return VOL_OK;
-    setDevice(mesh->ctx);
-    (void)hipStreamSynchronize(mesh->ctx->stream);
-    delete mesh;
+    meshRelease(mesh);
     API_END
 }
 
@@ -1107,6 +1130,7 @@ int vol_matrix_create(vol_mesh mesh, vol_matrix* out) {
     if (!mesh || !out) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
     auto A = std::make_unique<vol_matrix_s>();
     A->mesh = mesh;
+    mesh->refs.fetch_add(1);
     *out = A.release();
     API_END
 }
@@ -1114,9 +1138,11 @@ int vol_matrix_create(vol_mesh mesh, vol_matrix* out) {
 int vol_matrix_destroy(vol_matrix A) {
     API_BEGIN
     if (!A) return VOL_OK;
-    setDevice(A->mesh->ctx);
-    (void)hipStreamSynchronize(A->mesh->ctx->stream);
+    vol_mesh m = A->mesh;
+    setDevice(m->ctx);
+    (void)hipStreamSynchronize(m->ctx->stream);
     delete A;
+    meshRelease(m);
     API_END
 }
 
