@@ -635,11 +635,15 @@ static void gs_row(const Level* L, const double* Ad, const double* Au, const dou
 
 /* Project smoother: damped Jacobi x_i ← x_i + ω·(J_i − x_i), J_i = (r_i − Σ_k a_ik x_k)/d_i with the
  * off-diagonal sum in the same row order as the reference's smoother rows (GAMG.cpp:368-379).
- * fromZero: the sweep that starts from x = 0 reduces to x_i = ω·(r_i/d_i). */
+ * fromZero: the sweep that starts from x = 0 is defined as x_i = ω·(r_i·rD_i) with rD_i = 1/d_i
+ * (the reciprocal the diagonal preconditioner uses, LinearSolver.cpp:78). */
 static void jacobi_sweep(const Level* L, const double* Ad, const double* Au, const double* Al, const double* r,
                          double* x, double* xold, double omega, int fromZero) {
     if (fromZero) {
-        for (int i = 0; i < L->n; ++i) x[i] = omega * (r[i] / Ad[i]);
+        for (int i = 0; i < L->n; ++i) {
+            const double rD = 1.0 / Ad[i];
+            x[i] = omega * (r[i] * rD);
+        }
         return;
     }
     memcpy(xold, x, sizeof(double) * (size_t)L->n);

@@ -64,13 +64,14 @@ template <class T> struct DevBuf {
 };
 
 struct SellDev {
-    int n = 0, nslices = 0, nslots = 0;
+    int n = 0, nslices = 0, nslots = 0, maxWidth = 0;
     DevBuf<int> sliceOff, col, src;
     SellView view() const {
         SellView v;
         v.n = n;
         v.nslices = nslices;
         v.nslots = nslots;
+        v.maxWidth = maxWidth;
         v.sliceOff = sliceOff.p;
         v.col = col.p;
         v.src = src.p;
@@ -88,6 +89,7 @@ void buildSell(int n, const std::vector<int>& off, const std::vector<int>& colL,
         int w = 0;
         for (int r = s * WAVE; r < std::min(n, (s + 1) * WAVE); ++r) w = std::max(w, off[r + 1] - off[r]);
         sliceOff[s + 1] = sliceOff[s] + w * WAVE;
+        S.maxWidth = std::max(S.maxWidth, w);
     }
     S.nslots = sliceOff[S.nslices];
     std::vector<int> col(S.nslots, -1), src(S.nslots, -1);
@@ -257,6 +259,7 @@ struct vol_context_s {
     int device = 0;
     hipStream_t stream = nullptr;
     bool reproducible = false;
+    int tailMax = TAIL_MAX_CELLS;  // GAMG levels this small run in the one-workgroup tail kernel
     RedScratch red;
     DevBuf<double> part, result;
     DevBuf<long long> xpart;
@@ -305,7 +308,7 @@ struct vol_mesh_s {
 };
 
 struct MatLevel {
-    DevBuf<double> diag, Fu, Fl, coef, bufA, bufB, r, x;
+    DevBuf<double> diag, rD, Fu, Fl, coef, bufA, bufB, r, x;
 };
 
 struct vol_matrix_s {
@@ -316,8 +319,11 @@ struct vol_matrix_s {
     // GAMG coefficients per level, built lazily after each vol_matrix_set
     bool gamgReady = false;
     std::vector<std::unique_ptr<MatLevel>> L;
-    DevBuf<double> lu;
+    DevBuf<double> lu, luT;
     int nCoarse = 0;
+    int tailStart = -1;  // first GAMG level handled by the tail kernel (−1: none)
+    int tailLdsCoarse = 0, tailLdsBytes = 0;
+    DevBuf<TailLevel> tail;
     DevBuf<int> status;
     // solve work vectors
     DevBuf<double> vx, vb, vr, vw, vp, vq, wA, pA, r0, y, Ay, s, z, t;
@@ -454,6 +460,7 @@ void buildGamgCoeffs(vol_matrix A) {
             const LevelDev& D = *m->lv[k];
             if (k > 0) {
                 M->diag.alloc(D.n);
+                M->rD.alloc(D.n);
                 M->coef.alloc(D.sell.nslots);
                 M->r.alloc(D.n);
                 M->x.alloc(D.n);
@@ -474,14 +481,63 @@ void buildGamgCoeffs(vol_matrix A) {
         launch_galerkin(st, DC.n, DC.nf, D.mOff.p, D.mList.p, D.iOff.p, D.iList.p, D.cfOff.p, D.cfList.p, Fd,
                         A->L[k]->Fu.p, A->L[k]->Fl.p, A->L[k + 1]->diag.p, A->L[k + 1]->Fu.p, A->L[k + 1]->Fl.p);
         launch_sell_coef(st, DC.sell.view(), A->L[k + 1]->Fu.p, A->L[k + 1]->Fl.p, A->L[k + 1]->coef.p, nullptr);
+        VecArgs v;  // 1/diag of the coarse level
+        v.n = DC.n;
+        v.a = A->L[k + 1]->diag.p;
+        v.w = A->L[k + 1]->rD.p;
+        launch_vec(st, V_RECIP, v, false, m->ctx->red);
     }
     const LevelDev& DL = *m->lv[nL - 1];
     A->nCoarse = DL.n;
     A->lu.alloc(size_t(DL.n) * DL.n);
+    A->luT.alloc(size_t(DL.n) * DL.n);
     A->status.alloc(1);
     const double* cd = nL == 1 ? A->diag.p : A->L[nL - 1]->diag.p;
     launch_coarse_assemble(st, DL.n, DL.nf, DL.l.p, DL.u.p, cd, A->L[nL - 1]->Fu.p, A->L[nL - 1]->Fl.p, A->lu.p);
-    launch_coarse_lu(st, DL.n, A->lu.p, A->status.p);
+    launch_coarse_lu(st, DL.n, A->lu.p, A->luT.p, A->status.p);
+    // levels at or below the tail threshold (and their coarsest) go to one workgroup,
+    // provided their vectors fit in LDS
+    A->tailStart = -1;
+    auto ldsNeed = [&](int k0) {
+        size_t d = 0;
+        for (int k = k0; k < nL; ++k) d += 4 * size_t(m->lv[k]->n);
+        const size_t nc = size_t(DL.n);
+        d += nc <= 64 ? nc * nc : nc;
+        return d * sizeof(double);
+    };
+    for (int k = 0; k < nL; ++k)
+        if (m->lv[k]->n <= m->ctx->tailMax && ldsNeed(k) <= size_t(TAIL_LDS_BYTES)) {
+            A->tailStart = k;
+            break;
+        }
+    if (A->tailStart >= 0) {
+        std::vector<TailLevel> T;
+        int off = 0;
+        for (int k = A->tailStart; k < nL; ++k) {
+            const LevelDev& D = *m->lv[k];
+            MatLevel& M = *A->L[k];
+            TailLevel t;
+            t.S = k == 0 ? m->sell.view() : D.sell.view();
+            t.coef = k == 0 ? A->coef.p : M.coef.p;
+            t.diag = k == 0 ? A->diag.p : M.diag.p;
+            t.rD = k == 0 ? A->rD.p : M.rD.p;
+            if (k + 1 < nL) {
+                t.agg = D.agg.p;
+                t.mOff = D.mOff.p;
+                t.mList = D.mList.p;
+                t.nNext = m->lv[k + 1]->n;
+            }
+            t.ldsR = off;
+            t.ldsA = off + D.n;
+            t.ldsB = off + 2 * D.n;
+            t.ldsX = off + 3 * D.n;
+            off += 4 * D.n;
+            T.push_back(t);
+        }
+        A->tailLdsCoarse = off;
+        A->tailLdsBytes = int(ldsNeed(A->tailStart));
+        A->tail.upload(T, st);
+    }
     int status = 0;
     HIPCHECK(hipMemcpyAsync(&status, A->status.p, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
@@ -594,11 +650,13 @@ struct Ctx {
 
 void sell(const Ctx& C, int op, int xl, const SellView& V, const double* coef, const double* diag, const double* x,
           const double* r, double* out, double omega = 1.0, const double* xc = nullptr, const int* agg = nullptr,
-          const double* dotv = nullptr, double* part = nullptr, const double* xu = nullptr) {
+          const double* dotv = nullptr, double* part = nullptr, const double* xu = nullptr,
+          const double* rD = nullptr) {
     SellArgs a;
     a.S = V;
     a.coef = coef;
     a.diag = diag;
+    a.rD = rD;
     a.x = x;
     a.r = r;
     a.xc = xc;
@@ -671,8 +729,26 @@ void collectTiming(vol_context c) {
 void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const double* r, double* xout) {
     vol_mesh m = A->mesh;
     const int nL = int(m->lv.size());
+    if (l == A->tailStart) {
+        TailArgs t;
+        t.nLevels = nL - l;
+        t.L = A->tail.p;
+        t.rTop = r;
+        t.xTop = xout;
+        t.nCoarse = A->nCoarse;
+        t.lu = A->lu.p;
+        t.luT = A->luT.p;
+        t.ldsCoarse = A->tailLdsCoarse;
+        t.ldsBytes = A->tailLdsBytes;
+        t.nPre = P.nPre;
+        t.nPost = P.nPost;
+        t.omega = P.omega;
+        t.done = C.done;
+        launch_vcycle_tail(C.st, t);
+        return;
+    }
     if (l == nL - 1) {
-        launch_coarse_solve(C.st, A->nCoarse, A->lu.p, r, xout, C.done);
+        launch_coarse_solve(C.st, A->nCoarse, A->lu.p, A->luT.p, r, xout, C.done);
         return;
     }
     const LevelDev& D = *m->lv[l];
@@ -680,18 +756,19 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const double* r, d
     const SellView V = l == 0 ? m->sell.view() : D.sell.view();
     const double* coef = l == 0 ? A->coef.p : M.coef.p;
     const double* diag = l == 0 ? A->diag.p : M.diag.p;
+    const double* rD = l == 0 ? A->rD.p : M.rD.p;
     double* bufs[2] = {M.bufA.p, M.bufB.p};
     int cur = 0;  // index of the buffer holding the pre-smoothed x (nPre ≥ 2)
     for (int s = 1; s < P.nPre; ++s) {
         const int dst = (s == 1) ? 0 : 1 - cur;
         sell(C, OP_JACOBI, s == 1 ? XL_FROMZERO : XL_PLAIN, V, coef, diag, s == 1 ? nullptr : bufs[cur], r, bufs[dst],
-             P.omega);
+             P.omega, nullptr, nullptr, nullptr, nullptr, nullptr, rD);
         cur = dst;
     }
     const int freeB = 1 - cur;
     // residual of the pre-smoothed x, then restriction
     sell(C, OP_RESID, P.nPre == 1 ? XL_FROMZERO : XL_PLAIN, V, coef, diag, P.nPre == 1 ? nullptr : bufs[cur], r,
-         bufs[freeB], P.omega);
+         bufs[freeB], P.omega, nullptr, nullptr, nullptr, nullptr, nullptr, rD);
     MatLevel& MC = *A->L[l + 1];
     launch_restrict(C.st, m->lv[l + 1]->n, D.mOff.p, D.mList.p, bufs[freeB], MC.r.p, C.done);
     vcycle(C, A, P, l + 1, MC.r.p, MC.x.p);
@@ -701,7 +778,7 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const double* r, d
         double* dst = (s == P.nPost - 1) ? xout : bufs[other];
         if (s == 0)
             sell(C, OP_JACOBI, P.nPre == 1 ? XL_ZEROPROLONG : XL_PROLONG, V, coef, diag,
-                 P.nPre == 1 ? nullptr : bufs[src], r, dst, P.omega, MC.x.p, D.agg.p);
+                 P.nPre == 1 ? nullptr : bufs[src], r, dst, P.omega, MC.x.p, D.agg.p, nullptr, nullptr, nullptr, rD);
         else
             sell(C, OP_JACOBI, XL_PLAIN, V, coef, diag, bufs[src], r, dst, P.omega);
         if (s != P.nPost - 1) {
@@ -958,6 +1035,7 @@ int vol_context_create(int device, vol_context* out) {
     HIPCHECK(hipSetDevice(device));
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     if (const char* e = std::getenv("CFD_REPRODUCIBLE")) c->reproducible = std::string(e) != "0" && std::string(e) != "";
+    if (const char* e = std::getenv("VOLUMA_TAIL_MAX_CELLS")) c->tailMax = std::atoi(e);
     c->scal.alloc(1);
     HIPCHECK(hipHostMalloc((void**)&c->hostScal, 2 * sizeof(DevScalars), hipHostMallocDefault));
     c->ensureScratch(RED_BLOCKS);

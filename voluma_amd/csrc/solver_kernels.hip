@@ -24,13 +24,61 @@ template <int XL>
 __device__ __forceinline__ double xload(const SellArgs& a, int j, double xu) {
     if constexpr (XL == XL_PLAIN) return a.x[j];
     if constexpr (XL == XL_UNIFORM) return xu;
-    if constexpr (XL == XL_FROMZERO) return a.omega * (a.r[j] / a.diag[j]);
+    if constexpr (XL == XL_FROMZERO) return a.omega * (a.r[j] * a.rD[j]);
     if constexpr (XL == XL_PROLONG) return a.x[j] + a.xc[a.agg[j]];
-    if constexpr (XL == XL_ZEROPROLONG) return a.omega * (a.r[j] / a.diag[j]) + a.xc[a.agg[j]];
+    if constexpr (XL == XL_ZEROPROLONG) return a.omega * (a.r[j] * a.rD[j]) + a.xc[a.agg[j]];
     return 0.0;
 }
 
-template <int OP, int XL, bool DOT>
+// One row of a SELL gather op (the arithmetic shared by the grid kernel and the
+// one-workgroup coarse-level kernel). Slots are taken in groups of U: all column and
+// coefficient loads of a group, then all x gathers, are in flight together, and the
+// accumulation keeps the slot order.
+template <int OP, int XL, int U>
+__device__ __forceinline__ double sell_row(const SellArgs& a, int row, double xu) {
+    const int slice = row / WAVE, lane = row % WAVE;
+    const int s0 = a.S.sliceOff[slice];
+    const int width = (a.S.sliceOff[slice + 1] - s0) / WAVE;
+    double acc;
+    if constexpr (OP == OP_AMUL) acc = a.diag[row] * xload<XL>(a, row, xu);
+    else if constexpr (OP == OP_RESID) acc = a.r[row] - a.diag[row] * xload<XL>(a, row, xu);
+    else acc = a.r[row];
+    for (int k = 0; k < width; k += U) {
+        int j[U];
+        double av[U], xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int idx = s0 + (k + u) * WAVE + lane;
+            const bool in = k + u < width;
+            j[u] = in ? a.S.col[idx] : -1;
+            av[u] = in ? a.coef[idx] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = j[u] >= 0 ? xload<XL>(a, j[u], xu) : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (j[u] >= 0) {
+                const double t = av[u] * xv[u];
+                if constexpr (OP == OP_AMUL) acc = acc + t;
+                else acc = acc - t;
+            }
+        }
+    }
+    double outv;
+    if constexpr (OP == OP_JACOBI) {
+        const double J = acc / a.diag[row];
+        const double xo = xload<XL>(a, row, xu);
+        outv = xo + a.omega * (J - xo);
+    } else if constexpr (OP == OP_JACREF) {
+        outv = acc / a.diag[row];
+    } else {
+        outv = acc;
+    }
+    a.out[row] = outv;
+    return outv;
+}
+
+template <int OP, int XL, bool DOT, int U>
 __global__ __launch_bounds__(256) void k_sell(SellArgs a) {
     if (a.done && *a.done) return;
     const int nb = gridDim.x, per = nb / NXCD;
@@ -42,33 +90,7 @@ __global__ __launch_bounds__(256) void k_sell(SellArgs a) {
     double xu = 0.0;
     if constexpr (XL == XL_UNIFORM) xu = *a.xuPtr;
     double outv = 0.0;
-    if (valid) {
-        const int s0 = a.S.sliceOff[slice];
-        const int width = (a.S.sliceOff[slice + 1] - s0) / WAVE;
-        double acc;
-        if constexpr (OP == OP_AMUL) acc = a.diag[row] * xload<XL>(a, row, xu);
-        else if constexpr (OP == OP_RESID) acc = a.r[row] - a.diag[row] * xload<XL>(a, row, xu);
-        else acc = a.r[row];
-        for (int k = 0; k < width; ++k) {
-            const int idx = s0 + k * WAVE + lane;
-            const int j = a.S.col[idx];
-            if (j >= 0) {
-                const double t = a.coef[idx] * xload<XL>(a, j, xu);
-                if constexpr (OP == OP_AMUL) acc = acc + t;
-                else acc = acc - t;
-            }
-        }
-        if constexpr (OP == OP_JACOBI) {
-            const double J = acc / a.diag[row];
-            const double xo = xload<XL>(a, row, xu);
-            outv = xo + a.omega * (J - xo);
-        } else if constexpr (OP == OP_JACREF) {
-            outv = acc / a.diag[row];
-        } else {
-            outv = acc;
-        }
-        a.out[row] = outv;
-    }
+    if (valid) outv = sell_row<OP, XL, U>(a, row, xu);
     if constexpr (DOT) {
         __shared__ double sh[256 / WAVE];
         const double v = block_sum(valid ? outv * a.dotv[row] : 0.0, sh);
@@ -80,10 +102,18 @@ template <int OP, int XL>
 static void sell_dispatch2(hipStream_t st, const SellArgs& a) {
     const int nb = sell_grid_blocks(a.S.nslices);
     if (nb == 0) return;
-    if (a.part)
-        hipLaunchKernelGGL((k_sell<OP, XL, true>), dim3(nb), dim3(256), 0, st, a);
-    else
-        hipLaunchKernelGGL((k_sell<OP, XL, false>), dim3(nb), dim3(256), 0, st, a);
+    const bool wide = a.S.maxWidth > 8;
+    if (a.part) {
+        if (wide)
+            hipLaunchKernelGGL((k_sell<OP, XL, true, 16>), dim3(nb), dim3(256), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_sell<OP, XL, true, 4>), dim3(nb), dim3(256), 0, st, a);
+    } else {
+        if (wide)
+            hipLaunchKernelGGL((k_sell<OP, XL, false, 16>), dim3(nb), dim3(256), 0, st, a);
+        else
+            hipLaunchKernelGGL((k_sell<OP, XL, false, 4>), dim3(nb), dim3(256), 0, st, a);
+    }
 }
 
 template <int OP>
@@ -536,7 +566,7 @@ void launch_coarse_assemble(hipStream_t st, int n, int nf, const int* l, const i
 
 // No-pivot Doolittle LU (GAMG.cpp:334-343). Within step k every (i, j) update is
 // independent, so the parallel form performs the same rounded operations.
-__global__ void k_coarse_lu(int n, double* lu, int* status) {
+__global__ void k_coarse_lu(int n, double* lu, double* luT, int* status) {
     __shared__ int bad;
     if (threadIdx.x == 0) bad = 0;
     __syncthreads();
@@ -558,11 +588,12 @@ __global__ void k_coarse_lu(int n, double* lu, int* status) {
         __syncthreads();
     }
     __syncthreads();
+    for (int t = threadIdx.x; t < n * n; t += blockDim.x) luT[t] = lu[size_t(t % n) * n + t / n];
     if (threadIdx.x == 0 && status) status[0] = bad ? -1 : 0;
 }
 
-void launch_coarse_lu(hipStream_t st, int n, double* lu, int* status) {
-    hipLaunchKernelGGL(k_coarse_lu, dim3(1), dim3(256), 0, st, n, lu, status);
+void launch_coarse_lu(hipStream_t st, int n, double* lu, double* luT, int* status) {
+    hipLaunchKernelGGL(k_coarse_lu, dim3(1), dim3(256), 0, st, n, lu, luT, status);
 }
 
 // Coarsest solve with the LU factors: forward substitution column by column (for every
@@ -589,8 +620,185 @@ __global__ void k_coarse_solve(int n, const double* lu, const double* r, double*
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = g[i];
 }
 
-void launch_coarse_solve(hipStream_t st, int n, const double* lu, const double* r, double* x, const int* done) {
-    hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(256), sizeof(double) * n, st, n, lu, r, x, done);
+// n ≤ 64: one wavefront, lane i holds row i of the factors in registers and g_i; the
+// column steps exchange g_k by a cross-lane broadcast. Same operations and order as
+// k_coarse_solve.
+__global__ __launch_bounds__(64) void k_coarse_solve64(int n, const double* luT, const double* r, double* x,
+                                                       const int* done) {
+    if (done && *done) return;
+    const int i = threadIdx.x;
+    double row[64];
+#pragma unroll
+    for (int k = 0; k < 64; ++k) row[k] = (k < n && i < n) ? luT[size_t(k) * n + i] : 0.0;
+    double g = i < n ? r[i] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 64; ++k) {
+        if (k < n) {
+            const double gk = __shfl(g, k, WAVE);
+            if (i > k && i < n) g -= row[k] * gk;
+        }
+    }
+#pragma unroll
+    for (int k = 63; k >= 0; --k) {
+        if (k < n) {
+            if (i == k) g /= row[k];
+            const double gk = __shfl(g, k, WAVE);
+            if (i < k) g -= row[k] * gk;
+        }
+    }
+    if (i < n) x[i] = g;
+}
+
+void launch_coarse_solve(hipStream_t st, int n, const double* lu, const double* luT, const double* r, double* x,
+                         const int* done) {
+    if (n <= 64)
+        hipLaunchKernelGGL(k_coarse_solve64, dim3(1), dim3(64), 0, st, n, luT, r, x, done);
+    else
+        hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(256), sizeof(double) * n, st, n, lu, r, x, done);
+}
+
+// ============================================================================ V-cycle tail
+// The small levels of the V-cycle in one workgroup: every phase (sweep, residual,
+// restriction, coarsest solve, prolongation sweep) loops over the level's rows with
+// the block's threads and ends with a barrier. Per-row arithmetic is sell_row, so the
+// results equal the per-level kernels bit for bit.
+template <int OP, int XL>
+__device__ __forceinline__ void tail_phase(const SellArgs& a) {
+    for (int row = threadIdx.x; row < a.S.n; row += blockDim.x) sell_row<OP, XL, 16>(a, row, 0.0);
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(TailArgs t) {
+    if (t.done && *t.done) return;
+    extern __shared__ double lds[];
+    const int nl = t.nLevels;
+    const int cur = pre_smoothed_buffer(t.nPre), freeB = 1 - cur;
+    {
+        const TailLevel& L0 = t.L[0];
+        for (int i = threadIdx.x; i < L0.S.n; i += blockDim.x) lds[L0.ldsR + i] = t.rTop[i];
+        __syncthreads();
+    }
+    for (int l = 0; l + 1 < nl; ++l) {
+        const TailLevel& L = t.L[l];
+        SellArgs a;
+        a.S = L.S;
+        a.coef = L.coef;
+        a.diag = L.diag;
+        a.rD = L.rD;
+        a.r = lds + L.ldsR;
+        a.omega = t.omega;
+        double* bufs[2] = {lds + L.ldsA, lds + L.ldsB};
+        int c = 0;
+        for (int s = 1; s < t.nPre; ++s) {
+            const int dst = (s == 1) ? 0 : 1 - c;
+            a.x = bufs[c];
+            a.out = bufs[dst];
+            if (s == 1)
+                tail_phase<OP_JACOBI, XL_FROMZERO>(a);
+            else
+                tail_phase<OP_JACOBI, XL_PLAIN>(a);
+            c = dst;
+        }
+        a.x = bufs[cur];
+        a.out = bufs[freeB];
+        if (t.nPre == 1)
+            tail_phase<OP_RESID, XL_FROMZERO>(a);
+        else
+            tail_phase<OP_RESID, XL_PLAIN>(a);
+        const double* res = bufs[freeB];
+        double* rc = lds + t.L[l + 1].ldsR;
+        for (int I = threadIdx.x; I < L.nNext; I += blockDim.x) {
+            double sum = 0.0;
+            for (int k = L.mOff[I]; k < L.mOff[I + 1]; ++k) sum += res[L.mList[k]];
+            rc[I] = sum;
+        }
+        __syncthreads();
+    }
+    {  // coarsest level: LU substitution (same operations as k_coarse_solve)
+        const TailLevel& LC = t.L[nl - 1];
+        const double* r = lds + LC.ldsR;
+        double* x = (nl == 1) ? t.xTop : lds + LC.ldsX;
+        const int n = t.nCoarse;
+        double* cl = lds + t.ldsCoarse;
+        if (n <= 64) {
+            for (int q = threadIdx.x; q < n * n; q += blockDim.x) cl[q] = t.luT[q];
+            __syncthreads();
+            if (threadIdx.x < WAVE) {
+                const int i = threadIdx.x;
+                double g = i < n ? r[i] : 0.0;
+                for (int k = 0; k < n; ++k) {
+                    const double gk = __shfl(g, k, WAVE);
+                    if (i > k && i < n) g -= cl[k * n + i] * gk;
+                }
+                for (int k = n - 1; k >= 0; --k) {
+                    if (i == k) g /= cl[k * n + k];
+                    const double gk = __shfl(g, k, WAVE);
+                    if (i < k) g -= cl[k * n + i] * gk;
+                }
+                if (i < n) x[i] = g;
+            }
+        } else {
+            double* g = cl;
+            for (int i = threadIdx.x; i < n; i += blockDim.x) g[i] = r[i];
+            __syncthreads();
+            for (int k = 0; k < n; ++k) {
+                const double gk = g[k];
+                for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) g[i] -= t.lu[size_t(i) * n + k] * gk;
+                __syncthreads();
+            }
+            for (int k = n - 1; k >= 0; --k) {
+                if (threadIdx.x == 0) g[k] /= t.lu[size_t(k) * n + k];
+                __syncthreads();
+                const double gk = g[k];
+                for (int i = threadIdx.x; i < k; i += blockDim.x) g[i] -= t.lu[size_t(i) * n + k] * gk;
+                __syncthreads();
+            }
+            for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = g[i];
+        }
+        __syncthreads();
+    }
+    for (int l = nl - 2; l >= 0; --l) {
+        const TailLevel& L = t.L[l];
+        SellArgs a;
+        a.S = L.S;
+        a.coef = L.coef;
+        a.diag = L.diag;
+        a.rD = L.rD;
+        a.r = lds + L.ldsR;
+        a.omega = t.omega;
+        a.xc = lds + t.L[l + 1].ldsX;
+        a.agg = L.agg;
+        double* bufs[2] = {lds + L.ldsA, lds + L.ldsB};
+        double* xout = (l == 0) ? t.xTop : lds + L.ldsX;
+        int src = cur, other = freeB;
+        for (int s = 0; s < t.nPost; ++s) {
+            a.out = (s == t.nPost - 1) ? xout : bufs[other];
+            a.x = bufs[src];
+            if (s == 0) {
+                if (t.nPre == 1)
+                    tail_phase<OP_JACOBI, XL_ZEROPROLONG>(a);
+                else
+                    tail_phase<OP_JACOBI, XL_PROLONG>(a);
+            } else {
+                tail_phase<OP_JACOBI, XL_PLAIN>(a);
+            }
+            if (s != t.nPost - 1) {
+                const int tmp = src;
+                src = other;
+                other = tmp;
+            }
+        }
+    }
+}
+
+void launch_vcycle_tail(hipStream_t st, const TailArgs& t) {
+    static bool attr = false;
+    if (!attr) {
+        (void)hipFuncSetAttribute((const void*)k_vcycle_tail, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                  TAIL_LDS_BYTES);
+        attr = true;
+    }
+    hipLaunchKernelGGL(k_vcycle_tail, dim3(1), dim3(TAIL_THREADS), t.ldsBytes, st, t);
 }
 
 }  // namespace vol

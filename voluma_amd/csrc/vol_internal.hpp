@@ -20,6 +20,7 @@ struct SellView {
     int n = 0;          // rows
     int nslices = 0;
     int nslots = 0;     // padded slot count
+    int maxWidth = 0;   // longest row (slots)
     const int* sliceOff = nullptr;
     const int* col = nullptr;  // neighbour row, −1 = padding
     const int* src = nullptr;  // (face << 1) | side: side 0 → upper[face], 1 → lower[face]; −1 padding
@@ -81,15 +82,16 @@ enum SellOp : int {
 enum XLoad : int {
     XL_PLAIN = 0,        // x[j]
     XL_UNIFORM = 1,      // a scalar
-    XL_FROMZERO = 2,     // ω·(r[j]/d[j]): the first Jacobi sweep from x = 0
+    XL_FROMZERO = 2,     // ω·(r[j]·rD[j]), rD = 1/d: the first Jacobi sweep from x = 0
     XL_PROLONG = 3,      // x[j] + xc[agg[j]]
-    XL_ZEROPROLONG = 4,  // ω·(r[j]/d[j]) + xc[agg[j]]
+    XL_ZEROPROLONG = 4,  // ω·(r[j]·rD[j]) + xc[agg[j]]
 };
 
 struct SellArgs {
     SellView S;
     const double* coef = nullptr;
     const double* diag = nullptr;
+    const double* rD = nullptr;     // 1/diag (XL_FROMZERO / XL_ZEROPROLONG)
     const double* x = nullptr;
     const double* r = nullptr;
     const double* xc = nullptr;
@@ -159,8 +161,45 @@ void launch_orient0(hipStream_t st, int nf, const char* ownerIsL, const double* 
 // Dense LU of the coarsest level, one workgroup. lu: n×n row-major (in/out). status[0] = −1 if singular.
 void launch_coarse_assemble(hipStream_t st, int n, int nf, const int* l, const int* u, const double* d,
                             const double* cu, const double* cl, double* lu);
-void launch_coarse_lu(hipStream_t st, int n, double* lu, int* status);
-void launch_coarse_solve(hipStream_t st, int n, const double* lu, const double* r, double* x, const int* done);
+void launch_coarse_lu(hipStream_t st, int n, double* lu, double* luT, int* status);
+void launch_coarse_solve(hipStream_t st, int n, const double* lu, const double* luT, const double* r, double* x,
+                         const int* done);
+
+// Coarse levels of the V-cycle run inside one workgroup (k_vcycle_tail): the levels
+// from `first` down to the coarsest, with block barriers between the phases.
+constexpr int TAIL_MAX_CELLS = 4096;  // levels at or below this size go to the tail kernel
+constexpr int TAIL_THREADS = 1024;
+// The tail's vectors (r, two work buffers, x per level) live in LDS at the given
+// offsets (in doubles); only matrix data is read from global memory.
+constexpr int TAIL_LDS_BYTES = 160 * 1024;  // gfx950 LDS per workgroup
+struct TailLevel {
+    SellView S;
+    const double* coef = nullptr;
+    const double* diag = nullptr;
+    const double* rD = nullptr;
+    const int* agg = nullptr; // → next level
+    const int* mOff = nullptr;
+    const int* mList = nullptr;
+    int nNext = 0;
+    int ldsR = 0, ldsA = 0, ldsB = 0, ldsX = 0;
+};
+struct TailArgs {
+    int nLevels = 0;                  // levels in the tail, the last is the coarsest
+    const TailLevel* L = nullptr;     // device array
+    const double* rTop = nullptr;     // right-hand side of the first tail level (global)
+    double* xTop = nullptr;           // its V-cycle result (global)
+    int nCoarse = 0;
+    const double* lu = nullptr;
+    const double* luT = nullptr;
+    int ldsCoarse = 0;                // LU factors (n ≤ 64) or the substitution vector
+    int ldsBytes = 0;
+    int nPre = 2, nPost = 2;
+    double omega = 0.8;
+    const int* done = nullptr;
+};
+void launch_vcycle_tail(hipStream_t st, const TailArgs& t);
+// buffer (0 = bufA, 1 = bufB) holding the pre-smoothed x after nPre sweeps
+__host__ __device__ inline int pre_smoothed_buffer(int nPre) { return nPre >= 2 ? ((nPre - 2) & 1) : 0; }
 
 // ------------------------------------------------------------------ assembly
 enum AsmKind : int { ASM_LAPLACIAN = 0, ASM_DIV = 1 };
