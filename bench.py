@@ -65,9 +65,9 @@ def main():
     ap.add_argument("--workload", default="tet70")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--pre", type=int, default=2)
-    ap.add_argument("--post", type=int, default=2)
-    ap.add_argument("--omega", type=float, default=0.8)
+    ap.add_argument("--pre", type=int, default=1)
+    ap.add_argument("--post", type=int, default=1)
+    ap.add_argument("--omega", type=float, default=0.9)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
@@ -95,6 +95,16 @@ def main():
     def step():
         dx.upload(zero)
         return va.solve_ldu(A, db, dx, ctl)
+
+    # achievable HBM bandwidth on this GPU: streaming copy of 2^27 doubles (1 GiB each way)
+    ncopy = 1 << 27
+    src, dst = va.DeviceArray(ctx, ncopy), va.DeviceArray(ctx, ncopy)
+    dst.copy_from(src, 2)
+    t_copy = dst.copy_from(src, 10)
+    stream_gbs = 2 * ncopy * 8 / t_copy / 1e9
+    src.free()
+    dst.free()
+    log(f"stream copy: {stream_gbs:.0f} GB/s")
 
     for _ in range(args.warmup):
         p = step()
@@ -152,7 +162,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": None if achieved is None else achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_sell<OP_AMUL> level-0 A·p", "algorithmic_bytes": amul_bytes,
-                     "avg_us": amul_avg_s * 1e6, "launches": amul_cnt},
+                     "avg_us": amul_avg_s * 1e6, "launches": amul_cnt, "stream_copy_gbs": stream_gbs},
     }
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle, 1 thread) ...")

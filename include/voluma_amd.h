@@ -109,6 +109,10 @@ int vol_device_alloc(vol_context ctx, uint64_t bytes, void** ptr);
 int vol_device_free(vol_context ctx, void* ptr);
 int vol_memcpy_to_device(vol_context ctx, void* dst, const void* src, uint64_t bytes);
 int vol_memcpy_to_host(vol_context ctx, void* dst, const void* src, uint64_t bytes);
+/* dst[i] = src[i] for n doubles on the GPU with the library's streaming kernel,
+ * `repeats` times; *seconds = average time of one copy (HIP events). Measurement aid
+ * (achievable HBM bandwidth, counter calibration); no reference counterpart. */
+int vol_device_copy(vol_context ctx, double* dst, const double* src, int64_t n, int repeats, double* seconds);
 
 /* ---- mesh: LDU addressing of nInternalFaces faces between nCells cells
  * (Mesh::owner/neighbour, internal part). gamgWeights (nullable) is the per-face

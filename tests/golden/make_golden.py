@@ -10,7 +10,8 @@ voluma_amd.mesh, and stores inputs and outputs as data:
 * cfg1_box64.json, cfg2_tet70.json — the two full-size pressure systems: input and
   hierarchy hashes, level sizes, iteration counts, residuals, solution hashes,
   norms and samples for reproducible diagonal-PCG and default GAMG-PCG;
-* exactsum.json — ExactSum outputs for seeded random vectors.
+* exactsum.json — ExactSum outputs for seeded random vectors;
+* dropin_expected.jsonl — output of the Voluma drop-in check program with the reference solver.
 
 Usage (in the container that has the reference sources):
     make -C oracle all && python tests/golden/make_golden.py [--small-only]
@@ -208,7 +209,18 @@ def exactsum_case():
         json.dump(out, fh, indent=1)
 
 
+def dropin_case():
+    """Output of oracle/_ref/dropin_cpu (the Voluma program with the reference's own
+    solver) for the drop-in test, which runs the same program linked with the GPU shim."""
+    import subprocess
+    exe = os.path.join(ROOT, "oracle", "_ref", "dropin_cpu")
+    out = subprocess.run([exe, "24"], check=True, capture_output=True, text=True).stdout
+    with open(os.path.join(HERE, "dropin_expected.jsonl"), "w") as fh:
+        fh.write(out)
+
+
 if __name__ == "__main__":
+    dropin_case()
     exactsum_case()
     for nm, mk in SMALL.items():
         small_case(nm, mk)

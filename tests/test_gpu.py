@@ -238,3 +238,32 @@ def test_handle_lifetimes_any_destroy_order():
     assert np.array_equal(d.download(), g["amul_x"])
     A.close()
     d.free()
+
+
+def test_dropin_voluma_program_with_gpu_solver():
+    """A Voluma program (reference mesh, fvm::laplacian, FvMatrix::solve) linked with the
+    drop-in voluma_amd/integration/LinearSolver_amd.cpp instead of the reference's
+    LinearSolver.cpp/GAMG.cpp. Reproducible-mode solves match the reference's own build
+    bit for bit; default-mode solves within tolerance."""
+    import json
+    import os
+    import subprocess
+
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "oracle", "_ref", "dropin_amd")
+    if not os.path.exists(exe):
+        pytest.skip("oracle/_ref/dropin_amd not built (needs the reference sources when building)")
+    out = subprocess.run([exe, "24"], check=True, capture_output=True, text=True, timeout=120).stdout
+    got = [json.loads(l) for l in out.splitlines() if l.startswith("{")]
+    with open(os.path.join(ROOT, "tests", "golden", "dropin_expected.jsonl")) as fh:
+        ref = [json.loads(l) for l in fh if l.startswith("{")]
+    assert len(got) == len(ref) == 4
+    for g, r in zip(got, ref):
+        assert (g["solver"], g["preconditioner"], g["reproducible"]) == (r["solver"], r["preconditioner"],
+                                                                          r["reproducible"])
+        if r["reproducible"]:
+            assert g == r
+        else:
+            assert g["final"] < 1e-10 and g["l2sq"] == pytest.approx(r["l2sq"], rel=1e-8)
+            if r["preconditioner"] != "GAMG":
+                assert abs(g["iterations"] - r["iterations"]) <= 1

@@ -61,7 +61,7 @@ EXPORTED = ("vol_controls_default", "vol_last_error", "vol_context_create", "vol
             "vol_memcpy_to_host", "vol_mesh_create", "vol_mesh_destroy", "vol_mesh_gamg_levels",
             "vol_mesh_gamg_level", "vol_matrix_create", "vol_matrix_set", "vol_matrix_destroy", "vol_amul",
             "vol_solve", "vol_timing_enable", "vol_timing_read", "vol_mesh_set_boundary", "vol_assemble_laplacian",
-            "vol_assemble_div_upwind", "vol_matrix_get")
+            "vol_assemble_div_upwind", "vol_matrix_get", "vol_device_copy")
 
 
 def lib():
@@ -95,6 +95,7 @@ def lib():
         L.vol_timing_read.argtypes = [P, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_int64), I]
         L.vol_mesh_set_boundary.argtypes = [P, I, P]
         L.vol_matrix_get.argtypes = [P, P, P, P, I]
+        L.vol_device_copy.argtypes = [P, P, P, ctypes.c_int64, I, ctypes.POINTER(D)]
         L.vol_assemble_laplacian.argtypes = [P, D, P, P, P, P, P, P, P, P, P, I]
         L.vol_assemble_div_upwind.argtypes = [P, P, P, P, P, P, P, P, I]
         for name in EXPORTED:
@@ -206,6 +207,12 @@ class DeviceArray:
         out = np.empty(self.n, self.dtype)
         _check(lib().vol_memcpy_to_host(self._ctxh, _p(out), self.ptr, out.nbytes))
         return out
+
+    def copy_from(self, src: "DeviceArray", repeats=1):
+        """self ← src on the GPU (streaming kernel); returns seconds per copy."""
+        t = ctypes.c_double()
+        _check(lib().vol_device_copy(self._ctxh, self.ptr, src.ptr, min(self.n, src.n), int(repeats), ctypes.byref(t)))
+        return t.value
 
     def free(self):
         if getattr(self, "ptr", None):

@@ -64,7 +64,7 @@ template <class T> struct DevBuf {
 };
 
 struct SellDev {
-    int n = 0, nslices = 0, nslots = 0, maxWidth = 0;
+    int n = 0, nslices = 0, nslots = 0, maxWidth = 0, csrG = 0, csrIt = 0;
     DevBuf<int> sliceOff, col, src;
     SellView view() const {
         SellView v;
@@ -72,6 +72,8 @@ struct SellDev {
         v.nslices = nslices;
         v.nslots = nslots;
         v.maxWidth = maxWidth;
+        v.csrG = csrG;
+        v.csrIt = csrIt;
         v.sliceOff = sliceOff.p;
         v.col = col.p;
         v.src = src.p;
@@ -104,6 +106,28 @@ void buildSell(int n, const std::vector<int>& off, const std::vector<int>& colL,
     S.sliceOff.upload(sliceOff, st);
     S.col.upload(col, st);
     S.src.upload(src, st);
+}
+
+// Row lists → CSR with G lanes per row (coarse levels): G·IT ≥ longest row, IT ≤ 8.
+// Returns false when rows are too long for it (then SELL is used).
+bool buildCsr(int n, const std::vector<int>& off, const std::vector<int>& colL, const std::vector<int>& srcL,
+              SellDev& S, hipStream_t st) {
+    int maxW = 0;
+    for (int r = 0; r < n; ++r) maxW = std::max(maxW, off[r + 1] - off[r]);
+    int G = 4;
+    while (G < 16 && G * 4 < maxW) G *= 2;
+    const int IT = (maxW + G - 1) / G;
+    if (IT > 8) return false;
+    S.n = n;
+    S.nslices = n;
+    S.nslots = off[n];
+    S.maxWidth = maxW;
+    S.csrG = G;
+    S.csrIt = std::max(IT, 1);
+    S.sliceOff.upload(off, st);
+    S.col.upload(colL, st);
+    S.src.upload(srcL, st);
+    return true;
 }
 
 // ------------------------------------------------------------------ GAMG hierarchy (host setup)
@@ -401,7 +425,8 @@ void setupMesh(vol_mesh m, const double* w, int mergeLevels) {
         if (k > 0) {  // SELL rows in adjacency (face) order: A(i, j) = upper if i is the l side
             std::vector<int> srcL(F.adjFace.size());
             for (size_t q = 0; q < srcL.size(); ++q) srcL[q] = (F.adjFace[q] << 1) | (F.adjIsL[q] ? 0 : 1);
-            buildSell(F.n, F.off, F.adjCell, srcL, D->sell, st);
+            if (F.n > CSR_MAX_ROWS || !buildCsr(F.n, F.off, F.adjCell, srcL, D->sell, st))
+                buildSell(F.n, F.off, F.adjCell, srcL, D->sell, st);
         }
         if (k + 1 < nL) {
             const HLevel& C = m->H[k + 1];
@@ -1082,6 +1107,30 @@ int vol_device_free(vol_context ctx, void* ptr) {
     HIPCHECK(hipStreamSynchronize(ctx->stream));
     HIPCHECK(hipFree(ptr));
     ctxRelease(ctx);
+    API_END
+}
+
+int vol_device_copy(vol_context ctx, double* dst, const double* src, int64_t n, int repeats, double* seconds) {
+    API_BEGIN
+    if (!ctx || !dst || !src || n < 0 || n > (int64_t(1) << 31) - 1 || repeats < 1)
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "bad copy arguments");
+    setDevice(ctx);
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    VecArgs v;
+    v.n = int(n);
+    v.a = src;
+    v.w = dst;
+    HIPCHECK(hipEventRecord(e0, ctx->stream));
+    for (int k = 0; k < repeats; ++k) launch_vec(ctx->stream, V_COPY, v, false, ctx->red);
+    HIPCHECK(hipEventRecord(e1, ctx->stream));
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (seconds) *seconds = double(ms) * 1e-3 / repeats;
     API_END
 }
 

@@ -36,9 +36,19 @@ __device__ __forceinline__ double xload(const SellArgs& a, int j, double xu) {
 // accumulation keeps the slot order.
 template <int OP, int XL, int U>
 __device__ __forceinline__ double sell_row(const SellArgs& a, int row, double xu) {
-    const int slice = row / WAVE, lane = row % WAVE;
-    const int s0 = a.S.sliceOff[slice];
-    const int width = (a.S.sliceOff[slice + 1] - s0) / WAVE;
+    // SELL: slot k of the row at sliceOff[slice] + 64k + lane; CSR: at sliceOff[row] + k
+    int s0, width, stride;
+    if (a.S.csrG > 0) {
+        s0 = a.S.sliceOff[row];
+        width = a.S.sliceOff[row + 1] - s0;
+        stride = 1;
+    } else {
+        const int slice = row / WAVE;
+        const int b = a.S.sliceOff[slice];
+        width = (a.S.sliceOff[slice + 1] - b) / WAVE;
+        s0 = b + row % WAVE;
+        stride = WAVE;
+    }
     double acc;
     if constexpr (OP == OP_AMUL) acc = a.diag[row] * xload<XL>(a, row, xu);
     else if constexpr (OP == OP_RESID) acc = a.r[row] - a.diag[row] * xload<XL>(a, row, xu);
@@ -48,7 +58,7 @@ __device__ __forceinline__ double sell_row(const SellArgs& a, int row, double xu
         double av[U], xv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int idx = s0 + (k + u) * WAVE + lane;
+            const int idx = s0 + (k + u) * stride;
             const bool in = k + u < width;
             j[u] = in ? a.S.col[idx] : -1;
             av[u] = in ? a.coef[idx] : 0.0;
@@ -98,8 +108,89 @@ __global__ __launch_bounds__(256) void k_sell(SellArgs a) {
     }
 }
 
+// One CSR row processed by G lanes (lane g takes slots g, g+G, …): all loads of the row
+// are in flight at once, then every lane of the group accumulates the products in slot
+// order by reading them from their lanes with cross-lane shuffles, so the arithmetic is
+// identical to sell_row. Lane g == 0 writes the result. All lanes of the wave must call.
+template <int OP, int XL, int G, int IT>
+__device__ __forceinline__ double csr_row(const SellArgs& a, int row, int g, bool valid, double xu) {
+    int s0 = 0, len = 0;
+    if (valid) {
+        s0 = a.S.sliceOff[row];
+        len = a.S.sliceOff[row + 1] - s0;
+    }
+    double p[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int k = g + G * it;
+        p[it] = 0.0;
+        if (k < len) {
+            const int j = a.S.col[s0 + k];
+            p[it] = a.coef[s0 + k] * xload<XL>(a, j, xu);
+        }
+    }
+    double acc = 0.0;
+    if (valid) {
+        if constexpr (OP == OP_AMUL) acc = a.diag[row] * xload<XL>(a, row, xu);
+        else if constexpr (OP == OP_RESID) acc = a.r[row] - a.diag[row] * xload<XL>(a, row, xu);
+        else acc = a.r[row];
+    }
+    const int base = (threadIdx.x & (WAVE - 1)) - g;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const double t = __shfl(p[it], base + q, WAVE);
+            if (q + G * it < len) {
+                if constexpr (OP == OP_AMUL) acc = acc + t;
+                else acc = acc - t;
+            }
+        }
+    }
+    double outv = 0.0;
+    if (valid) {
+        if constexpr (OP == OP_JACOBI) {
+            const double J = acc / a.diag[row];
+            const double xo = xload<XL>(a, row, xu);
+            outv = xo + a.omega * (J - xo);
+        } else if constexpr (OP == OP_JACREF) {
+            outv = acc / a.diag[row];
+        } else {
+            outv = acc;
+        }
+        if (g == 0) a.out[row] = outv;
+    }
+    return outv;
+}
+
+template <int OP, int XL, int G, int IT>
+__global__ __launch_bounds__(256) void k_csrv(SellArgs a) {
+    if (a.done && *a.done) return;
+    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int row = gid / G, g = gid % G;
+    double xu = 0.0;
+    if constexpr (XL == XL_UNIFORM) xu = *a.xuPtr;
+    csr_row<OP, XL, G, IT>(a, row, g, row < a.S.n, xu);
+}
+
+template <int OP, int XL>
+static void csr_dispatch(hipStream_t st, const SellArgs& a) {
+    const long threads = long(a.S.n) * a.S.csrG;
+    const int nb = int((threads + 255) / 256);
+    if (nb == 0) return;
+    const int G = a.S.csrG, IT = a.S.csrIt;
+    if (G == 4 && IT <= 4) hipLaunchKernelGGL((k_csrv<OP, XL, 4, 4>), dim3(nb), dim3(256), 0, st, a);
+    else if (G == 8 && IT <= 4) hipLaunchKernelGGL((k_csrv<OP, XL, 8, 4>), dim3(nb), dim3(256), 0, st, a);
+    else if (G == 16 && IT <= 4) hipLaunchKernelGGL((k_csrv<OP, XL, 16, 4>), dim3(nb), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_csrv<OP, XL, 16, 8>), dim3(nb), dim3(256), 0, st, a);
+}
+
 template <int OP, int XL>
 static void sell_dispatch2(hipStream_t st, const SellArgs& a) {
+    if (a.S.csrG > 0) {
+        csr_dispatch<OP, XL>(st, a);
+        return;
+    }
     const int nb = sell_grid_blocks(a.S.nslices);
     if (nb == 0) return;
     const bool wide = a.S.maxWidth > 8;

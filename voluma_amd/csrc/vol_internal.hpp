@@ -16,11 +16,17 @@ constexpr int NBINS = 72;  // exact-sum bins (par::ExactSum::NBINS, Comm.h:44)
 // slot k of a row in slice s at sliceOff[s] + 64k + lane. Slots of a row keep
 // ascending face order, which is the per-row accumulation order of the reference's
 // face loop (src/fvm/linalg/LinearSolver.cpp:121-133). Padding slots have col = −1.
+//
+// Coarse GAMG levels, whose rows are long, use a CSR layout instead (csrG > 0): row r
+// occupies slots sliceOff[r] .. sliceOff[r+1) contiguously and is processed by csrG
+// lanes together (csrIt slots per lane).
 struct SellView {
     int n = 0;          // rows
     int nslices = 0;
     int nslots = 0;     // padded slot count
     int maxWidth = 0;   // longest row (slots)
+    int csrG = 0;       // > 0: CSR layout, lanes per row
+    int csrIt = 0;      // slots per lane (csrG·csrIt ≥ maxWidth)
     const int* sliceOff = nullptr;
     const int* col = nullptr;  // neighbour row, −1 = padding
     const int* src = nullptr;  // (face << 1) | side: side 0 → upper[face], 1 → lower[face]; −1 padding
@@ -167,7 +173,8 @@ void launch_coarse_solve(hipStream_t st, int n, const double* lu, const double* 
 
 // Coarse levels of the V-cycle run inside one workgroup (k_vcycle_tail): the levels
 // from `first` down to the coarsest, with block barriers between the phases.
-constexpr int TAIL_MAX_CELLS = 4096;  // levels at or below this size go to the tail kernel
+constexpr int TAIL_MAX_CELLS = 512;   // levels at or below this size go to the tail kernel
+constexpr int CSR_MAX_ROWS = 65536;   // coarse levels up to this size use the CSR-vector layout
 constexpr int TAIL_THREADS = 1024;
 // The tail's vectors (r, two work buffers, x per level) live in LDS at the given
 // offsets (in doubles); only matrix data is read from global memory.
