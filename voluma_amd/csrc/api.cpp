@@ -346,7 +346,7 @@ struct vol_matrix_s {
     DevBuf<double> lu, luT;
     int nCoarse = 0;
     int tailStart = -1;  // first GAMG level handled by the tail kernel (−1: none)
-    int tailLdsCoarse = 0, tailLdsBytes = 0;
+    int tailLdsBytes = 0;
     DevBuf<TailLevel> tail;
     DevBuf<int> status;
     // solve work vectors
@@ -527,8 +527,7 @@ void buildGamgCoeffs(vol_matrix A) {
         size_t d = 0;
         for (int k = k0; k < nL; ++k) d += 4 * size_t(m->lv[k]->n);
         const size_t nc = size_t(DL.n);
-        d += nc <= 64 ? nc * nc : nc;
-        return d * sizeof(double);
+        return (nc <= 64 ? nc * nc : nc) * sizeof(double) + d * sizeof(double);
     };
     for (int k = 0; k < nL; ++k)
         if (m->lv[k]->n <= m->ctx->tailMax && ldsNeed(k) <= size_t(TAIL_LDS_BYTES)) {
@@ -537,7 +536,8 @@ void buildGamgCoeffs(vol_matrix A) {
         }
     if (A->tailStart >= 0) {
         std::vector<TailLevel> T;
-        int off = 0;
+        const int coarseDoubles = DL.n <= 64 ? DL.n * DL.n : DL.n;
+        int off = coarseDoubles;  // in doubles
         for (int k = A->tailStart; k < nL; ++k) {
             const LevelDev& D = *m->lv[k];
             MatLevel& M = *A->L[k];
@@ -559,7 +559,6 @@ void buildGamgCoeffs(vol_matrix A) {
             off += 4 * D.n;
             T.push_back(t);
         }
-        A->tailLdsCoarse = off;
         A->tailLdsBytes = int(ldsNeed(A->tailStart));
         A->tail.upload(T, st);
     }
@@ -763,7 +762,6 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const double* r, d
         t.nCoarse = A->nCoarse;
         t.lu = A->lu.p;
         t.luT = A->luT.p;
-        t.ldsCoarse = A->tailLdsCoarse;
         t.ldsBytes = A->tailLdsBytes;
         t.nPre = P.nPre;
         t.nPost = P.nPost;

@@ -20,22 +20,25 @@ int sell_grid_blocks(int nslices) {
     return ((logical + NXCD - 1) / NXCD) * NXCD;
 }
 
-template <int XL>
-__device__ __forceinline__ double xload(const SellArgs& a, int j, double xu) {
+// Value of x at row j for the gather ops; T is the storage/arithmetic type of the level
+// (double, or float in the mixed-precision inner solve).
+template <int XL, class T>
+__device__ __forceinline__ T xload(const SellArgsT<T>& a, int j, T xu) {
+    const T om = T(a.omega);
     if constexpr (XL == XL_PLAIN) return a.x[j];
     if constexpr (XL == XL_UNIFORM) return xu;
-    if constexpr (XL == XL_FROMZERO) return a.omega * (a.r[j] * a.rD[j]);
+    if constexpr (XL == XL_FROMZERO) return om * (a.r[j] * a.rD[j]);
     if constexpr (XL == XL_PROLONG) return a.x[j] + a.xc[a.agg[j]];
-    if constexpr (XL == XL_ZEROPROLONG) return a.omega * (a.r[j] * a.rD[j]) + a.xc[a.agg[j]];
-    return 0.0;
+    if constexpr (XL == XL_ZEROPROLONG) return om * (a.r[j] * a.rD[j]) + a.xc[a.agg[j]];
+    return T(0);
 }
 
 // One row of a SELL gather op (the arithmetic shared by the grid kernel and the
 // one-workgroup coarse-level kernel). Slots are taken in groups of U: all column and
 // coefficient loads of a group, then all x gathers, are in flight together, and the
 // accumulation keeps the slot order.
-template <int OP, int XL, int U>
-__device__ __forceinline__ double sell_row(const SellArgs& a, int row, double xu) {
+template <int OP, int XL, int U, class T>
+__device__ __forceinline__ T sell_row(const SellArgsT<T>& a, int row, T xu) {
     // SELL: slot k of the row at sliceOff[slice] + 64k + lane; CSR: at sliceOff[row] + k
     int s0, width, stride;
     if (a.S.csrG > 0) {
@@ -49,36 +52,36 @@ __device__ __forceinline__ double sell_row(const SellArgs& a, int row, double xu
         s0 = b + row % WAVE;
         stride = WAVE;
     }
-    double acc;
+    T acc;
     if constexpr (OP == OP_AMUL) acc = a.diag[row] * xload<XL>(a, row, xu);
     else if constexpr (OP == OP_RESID) acc = a.r[row] - a.diag[row] * xload<XL>(a, row, xu);
     else acc = a.r[row];
     for (int k = 0; k < width; k += U) {
         int j[U];
-        double av[U], xv[U];
+        T av[U], xv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int idx = s0 + (k + u) * stride;
             const bool in = k + u < width;
             j[u] = in ? a.S.col[idx] : -1;
-            av[u] = in ? a.coef[idx] : 0.0;
+            av[u] = in ? a.coef[idx] : T(0);
         }
 #pragma unroll
-        for (int u = 0; u < U; ++u) xv[u] = j[u] >= 0 ? xload<XL>(a, j[u], xu) : 0.0;
+        for (int u = 0; u < U; ++u) xv[u] = j[u] >= 0 ? xload<XL>(a, j[u], xu) : T(0);
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (j[u] >= 0) {
-                const double t = av[u] * xv[u];
+                const T t = av[u] * xv[u];
                 if constexpr (OP == OP_AMUL) acc = acc + t;
                 else acc = acc - t;
             }
         }
     }
-    double outv;
+    T outv;
     if constexpr (OP == OP_JACOBI) {
-        const double J = acc / a.diag[row];
-        const double xo = xload<XL>(a, row, xu);
-        outv = xo + a.omega * (J - xo);
+        const T J = acc / a.diag[row];
+        const T xo = xload<XL>(a, row, xu);
+        outv = xo + T(a.omega) * (J - xo);
     } else if constexpr (OP == OP_JACREF) {
         outv = acc / a.diag[row];
     } else {
@@ -88,8 +91,8 @@ __device__ __forceinline__ double sell_row(const SellArgs& a, int row, double xu
     return outv;
 }
 
-template <int OP, int XL, bool DOT, int U>
-__global__ __launch_bounds__(256) void k_sell(SellArgs a) {
+template <int OP, int XL, bool DOT, int U, class T>
+__global__ __launch_bounds__(256) void k_sell(SellArgsT<T> a) {
     if (a.done && *a.done) return;
     const int nb = gridDim.x, per = nb / NXCD;
     const int lb = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;  // XCD-contiguous logical block
@@ -97,13 +100,13 @@ __global__ __launch_bounds__(256) void k_sell(SellArgs a) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int row = slice * WAVE + lane;
     const bool valid = slice < a.S.nslices && row < a.S.n;
-    double xu = 0.0;
+    T xu = T(0);
     if constexpr (XL == XL_UNIFORM) xu = *a.xuPtr;
-    double outv = 0.0;
+    T outv = T(0);
     if (valid) outv = sell_row<OP, XL, U>(a, row, xu);
-    if constexpr (DOT) {
+    if constexpr (DOT) {  // Σ out·dotv, accumulated in double
         __shared__ double sh[256 / WAVE];
-        const double v = block_sum(valid ? outv * a.dotv[row] : 0.0, sh);
+        const double v = block_sum(valid ? double(outv) * double(a.dotv[row]) : 0.0, sh);
         if (threadIdx.x == 0) a.part[lb] = v;
     }
 }
@@ -112,24 +115,24 @@ __global__ __launch_bounds__(256) void k_sell(SellArgs a) {
 // are in flight at once, then every lane of the group accumulates the products in slot
 // order by reading them from their lanes with cross-lane shuffles, so the arithmetic is
 // identical to sell_row. Lane g == 0 writes the result. All lanes of the wave must call.
-template <int OP, int XL, int G, int IT>
-__device__ __forceinline__ double csr_row(const SellArgs& a, int row, int g, bool valid, double xu) {
+template <int OP, int XL, int G, int IT, class T>
+__device__ __forceinline__ T csr_row(const SellArgsT<T>& a, int row, int g, bool valid, T xu) {
     int s0 = 0, len = 0;
     if (valid) {
         s0 = a.S.sliceOff[row];
         len = a.S.sliceOff[row + 1] - s0;
     }
-    double p[IT];
+    T p[IT];
 #pragma unroll
     for (int it = 0; it < IT; ++it) {
         const int k = g + G * it;
-        p[it] = 0.0;
+        p[it] = T(0);
         if (k < len) {
             const int j = a.S.col[s0 + k];
             p[it] = a.coef[s0 + k] * xload<XL>(a, j, xu);
         }
     }
-    double acc = 0.0;
+    T acc = T(0);
     if (valid) {
         if constexpr (OP == OP_AMUL) acc = a.diag[row] * xload<XL>(a, row, xu);
         else if constexpr (OP == OP_RESID) acc = a.r[row] - a.diag[row] * xload<XL>(a, row, xu);
@@ -140,19 +143,19 @@ __device__ __forceinline__ double csr_row(const SellArgs& a, int row, int g, boo
     for (int it = 0; it < IT; ++it) {
 #pragma unroll
         for (int q = 0; q < G; ++q) {
-            const double t = __shfl(p[it], base + q, WAVE);
+            const T t = __shfl(p[it], base + q, WAVE);
             if (q + G * it < len) {
                 if constexpr (OP == OP_AMUL) acc = acc + t;
                 else acc = acc - t;
             }
         }
     }
-    double outv = 0.0;
+    T outv = T(0);
     if (valid) {
         if constexpr (OP == OP_JACOBI) {
-            const double J = acc / a.diag[row];
-            const double xo = xload<XL>(a, row, xu);
-            outv = xo + a.omega * (J - xo);
+            const T J = acc / a.diag[row];
+            const T xo = xload<XL>(a, row, xu);
+            outv = xo + T(a.omega) * (J - xo);
         } else if constexpr (OP == OP_JACREF) {
             outv = acc / a.diag[row];
         } else {
@@ -163,30 +166,30 @@ __device__ __forceinline__ double csr_row(const SellArgs& a, int row, int g, boo
     return outv;
 }
 
-template <int OP, int XL, int G, int IT>
-__global__ __launch_bounds__(256) void k_csrv(SellArgs a) {
+template <int OP, int XL, int G, int IT, class T>
+__global__ __launch_bounds__(256) void k_csrv(SellArgsT<T> a) {
     if (a.done && *a.done) return;
     const int gid = blockIdx.x * blockDim.x + threadIdx.x;
     const int row = gid / G, g = gid % G;
-    double xu = 0.0;
+    T xu = T(0);
     if constexpr (XL == XL_UNIFORM) xu = *a.xuPtr;
     csr_row<OP, XL, G, IT>(a, row, g, row < a.S.n, xu);
 }
 
-template <int OP, int XL>
-static void csr_dispatch(hipStream_t st, const SellArgs& a) {
+template <int OP, int XL, class T>
+static void csr_dispatch(hipStream_t st, const SellArgsT<T>& a) {
     const long threads = long(a.S.n) * a.S.csrG;
     const int nb = int((threads + 255) / 256);
     if (nb == 0) return;
     const int G = a.S.csrG, IT = a.S.csrIt;
-    if (G == 4 && IT <= 4) hipLaunchKernelGGL((k_csrv<OP, XL, 4, 4>), dim3(nb), dim3(256), 0, st, a);
-    else if (G == 8 && IT <= 4) hipLaunchKernelGGL((k_csrv<OP, XL, 8, 4>), dim3(nb), dim3(256), 0, st, a);
-    else if (G == 16 && IT <= 4) hipLaunchKernelGGL((k_csrv<OP, XL, 16, 4>), dim3(nb), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_csrv<OP, XL, 16, 8>), dim3(nb), dim3(256), 0, st, a);
+    if (G == 4 && IT <= 4) hipLaunchKernelGGL((k_csrv<OP, XL, 4, 4, T>), dim3(nb), dim3(256), 0, st, a);
+    else if (G == 8 && IT <= 4) hipLaunchKernelGGL((k_csrv<OP, XL, 8, 4, T>), dim3(nb), dim3(256), 0, st, a);
+    else if (G == 16 && IT <= 4) hipLaunchKernelGGL((k_csrv<OP, XL, 16, 4, T>), dim3(nb), dim3(256), 0, st, a);
+    else hipLaunchKernelGGL((k_csrv<OP, XL, 16, 8, T>), dim3(nb), dim3(256), 0, st, a);
 }
 
-template <int OP, int XL>
-static void sell_dispatch2(hipStream_t st, const SellArgs& a) {
+template <int OP, int XL, class T>
+static void sell_dispatch2(hipStream_t st, const SellArgsT<T>& a) {
     if (a.S.csrG > 0) {
         csr_dispatch<OP, XL>(st, a);
         return;
@@ -195,37 +198,47 @@ static void sell_dispatch2(hipStream_t st, const SellArgs& a) {
     if (nb == 0) return;
     const bool wide = a.S.maxWidth > 8;
     if (a.part) {
-        if (wide)
-            hipLaunchKernelGGL((k_sell<OP, XL, true, 16>), dim3(nb), dim3(256), 0, st, a);
-        else
-            hipLaunchKernelGGL((k_sell<OP, XL, true, 4>), dim3(nb), dim3(256), 0, st, a);
+        if constexpr (OP == OP_AMUL) {
+            if (wide)
+                hipLaunchKernelGGL((k_sell<OP, XL, true, 16, T>), dim3(nb), dim3(256), 0, st, a);
+            else
+                hipLaunchKernelGGL((k_sell<OP, XL, true, 4, T>), dim3(nb), dim3(256), 0, st, a);
+        }
     } else {
         if (wide)
-            hipLaunchKernelGGL((k_sell<OP, XL, false, 16>), dim3(nb), dim3(256), 0, st, a);
+            hipLaunchKernelGGL((k_sell<OP, XL, false, 16, T>), dim3(nb), dim3(256), 0, st, a);
         else
-            hipLaunchKernelGGL((k_sell<OP, XL, false, 4>), dim3(nb), dim3(256), 0, st, a);
+            hipLaunchKernelGGL((k_sell<OP, XL, false, 4, T>), dim3(nb), dim3(256), 0, st, a);
     }
 }
 
-template <int OP>
-static void sell_dispatch1(hipStream_t st, int xl, const SellArgs& a) {
+template <int OP, class T>
+static void sell_dispatch1(hipStream_t st, int xl, const SellArgsT<T>& a) {
     switch (xl) {
     case XL_PLAIN: sell_dispatch2<OP, XL_PLAIN>(st, a); break;
-    case XL_UNIFORM: sell_dispatch2<OP, XL_UNIFORM>(st, a); break;
+    case XL_UNIFORM:
+        if constexpr (OP == OP_AMUL) sell_dispatch2<OP, XL_UNIFORM>(st, a);
+        break;
     case XL_FROMZERO: sell_dispatch2<OP, XL_FROMZERO>(st, a); break;
     case XL_PROLONG: sell_dispatch2<OP, XL_PROLONG>(st, a); break;
     case XL_ZEROPROLONG: sell_dispatch2<OP, XL_ZEROPROLONG>(st, a); break;
     }
 }
 
-void launch_sell(hipStream_t st, int op, int xl, const SellArgs& a) {
+template <class T>
+static void launch_sell_t(hipStream_t st, int op, int xl, const SellArgsT<T>& a) {
     switch (op) {
     case OP_AMUL: sell_dispatch1<OP_AMUL>(st, xl, a); break;
     case OP_RESID: sell_dispatch1<OP_RESID>(st, xl, a); break;
     case OP_JACOBI: sell_dispatch1<OP_JACOBI>(st, xl, a); break;
-    case OP_JACREF: sell_dispatch1<OP_JACREF>(st, xl, a); break;
+    case OP_JACREF:
+        if constexpr (sizeof(T) == 8) sell_dispatch1<OP_JACREF>(st, xl, a);
+        break;
     }
 }
+
+void launch_sell(hipStream_t st, int op, int xl, const SellArgs& a) { launch_sell_t(st, op, xl, a); }
+void launch_sell32(hipStream_t st, int op, int xl, const SellArgs32& a) { launch_sell_t(st, op, xl, a); }
 
 // Slot coefficients from face arrays: coef[slot] = upper or lower of the slot's face.
 __global__ void k_sell_coef(SellView S, const double* upper, const double* lower, double* coef, float* coef32) {
@@ -753,32 +766,37 @@ void launch_coarse_solve(hipStream_t st, int n, const double* lu, const double* 
 // restriction, coarsest solve, prolongation sweep) loops over the level's rows with
 // the block's threads and ends with a barrier. Per-row arithmetic is sell_row, so the
 // results equal the per-level kernels bit for bit.
-template <int OP, int XL>
-__device__ __forceinline__ void tail_phase(const SellArgs& a) {
-    for (int row = threadIdx.x; row < a.S.n; row += blockDim.x) sell_row<OP, XL, 16>(a, row, 0.0);
+template <int OP, int XL, class T>
+__device__ __forceinline__ void tail_phase(const SellArgsT<T>& a) {
+    for (int row = threadIdx.x; row < a.S.n; row += blockDim.x) sell_row<OP, XL, 16>(a, row, T(0));
     __syncthreads();
 }
 
-__global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(TailArgs t) {
+// LDS holds [coarsest LU (n ≤ 64) or substitution vector, doubles] then the tail levels'
+// vectors (r, two work buffers, x) in the level precision T. The coarsest solve runs in
+// FP64 whatever T is.
+template <class T>
+__global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(TailArgsT<T> t) {
     if (t.done && *t.done) return;
     extern __shared__ double lds[];
+    T* ldsT = reinterpret_cast<T*>(lds);
     const int nl = t.nLevels;
     const int cur = pre_smoothed_buffer(t.nPre), freeB = 1 - cur;
     {
-        const TailLevel& L0 = t.L[0];
-        for (int i = threadIdx.x; i < L0.S.n; i += blockDim.x) lds[L0.ldsR + i] = t.rTop[i];
+        const TailLevelT<T>& L0 = t.L[0];
+        for (int i = threadIdx.x; i < L0.S.n; i += blockDim.x) ldsT[L0.ldsR + i] = t.rTop[i];
         __syncthreads();
     }
     for (int l = 0; l + 1 < nl; ++l) {
-        const TailLevel& L = t.L[l];
-        SellArgs a;
+        const TailLevelT<T>& L = t.L[l];
+        SellArgsT<T> a;
         a.S = L.S;
         a.coef = L.coef;
         a.diag = L.diag;
         a.rD = L.rD;
-        a.r = lds + L.ldsR;
+        a.r = ldsT + L.ldsR;
         a.omega = t.omega;
-        double* bufs[2] = {lds + L.ldsA, lds + L.ldsB};
+        T* bufs[2] = {ldsT + L.ldsA, ldsT + L.ldsB};
         int c = 0;
         for (int s = 1; s < t.nPre; ++s) {
             const int dst = (s == 1) ? 0 : 1 - c;
@@ -796,27 +814,27 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(TailArgs t) {
             tail_phase<OP_RESID, XL_FROMZERO>(a);
         else
             tail_phase<OP_RESID, XL_PLAIN>(a);
-        const double* res = bufs[freeB];
-        double* rc = lds + t.L[l + 1].ldsR;
+        const T* res = bufs[freeB];
+        T* rc = ldsT + t.L[l + 1].ldsR;
         for (int I = threadIdx.x; I < L.nNext; I += blockDim.x) {
-            double sum = 0.0;
+            T sum = T(0);
             for (int k = L.mOff[I]; k < L.mOff[I + 1]; ++k) sum += res[L.mList[k]];
             rc[I] = sum;
         }
         __syncthreads();
     }
-    {  // coarsest level: LU substitution (same operations as k_coarse_solve)
-        const TailLevel& LC = t.L[nl - 1];
-        const double* r = lds + LC.ldsR;
-        double* x = (nl == 1) ? t.xTop : lds + LC.ldsX;
+    {  // coarsest level: LU substitution in FP64 (same operations as k_coarse_solve)
+        const TailLevelT<T>& LC = t.L[nl - 1];
+        const T* r = ldsT + LC.ldsR;
+        T* x = (nl == 1) ? t.xTop : ldsT + LC.ldsX;
         const int n = t.nCoarse;
-        double* cl = lds + t.ldsCoarse;
+        double* cl = lds;
         if (n <= 64) {
             for (int q = threadIdx.x; q < n * n; q += blockDim.x) cl[q] = t.luT[q];
             __syncthreads();
             if (threadIdx.x < WAVE) {
                 const int i = threadIdx.x;
-                double g = i < n ? r[i] : 0.0;
+                double g = i < n ? double(r[i]) : 0.0;
                 for (int k = 0; k < n; ++k) {
                     const double gk = __shfl(g, k, WAVE);
                     if (i > k && i < n) g -= cl[k * n + i] * gk;
@@ -826,11 +844,11 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(TailArgs t) {
                     const double gk = __shfl(g, k, WAVE);
                     if (i < k) g -= cl[k * n + i] * gk;
                 }
-                if (i < n) x[i] = g;
+                if (i < n) x[i] = T(g);
             }
         } else {
             double* g = cl;
-            for (int i = threadIdx.x; i < n; i += blockDim.x) g[i] = r[i];
+            for (int i = threadIdx.x; i < n; i += blockDim.x) g[i] = double(r[i]);
             __syncthreads();
             for (int k = 0; k < n; ++k) {
                 const double gk = g[k];
@@ -844,23 +862,23 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(TailArgs t) {
                 for (int i = threadIdx.x; i < k; i += blockDim.x) g[i] -= t.lu[size_t(i) * n + k] * gk;
                 __syncthreads();
             }
-            for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = g[i];
+            for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = T(g[i]);
         }
         __syncthreads();
     }
     for (int l = nl - 2; l >= 0; --l) {
-        const TailLevel& L = t.L[l];
-        SellArgs a;
+        const TailLevelT<T>& L = t.L[l];
+        SellArgsT<T> a;
         a.S = L.S;
         a.coef = L.coef;
         a.diag = L.diag;
         a.rD = L.rD;
-        a.r = lds + L.ldsR;
+        a.r = ldsT + L.ldsR;
         a.omega = t.omega;
-        a.xc = lds + t.L[l + 1].ldsX;
+        a.xc = ldsT + t.L[l + 1].ldsX;
         a.agg = L.agg;
-        double* bufs[2] = {lds + L.ldsA, lds + L.ldsB};
-        double* xout = (l == 0) ? t.xTop : lds + L.ldsX;
+        T* bufs[2] = {ldsT + L.ldsA, ldsT + L.ldsB};
+        T* xout = (l == 0) ? t.xTop : ldsT + L.ldsX;
         int src = cur, other = freeB;
         for (int s = 0; s < t.nPost; ++s) {
             a.out = (s == t.nPost - 1) ? xout : bufs[other];
@@ -882,14 +900,18 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(TailArgs t) {
     }
 }
 
-void launch_vcycle_tail(hipStream_t st, const TailArgs& t) {
+template <class T>
+static void launch_tail_t(hipStream_t st, const TailArgsT<T>& t) {
     static bool attr = false;
     if (!attr) {
-        (void)hipFuncSetAttribute((const void*)k_vcycle_tail, hipFuncAttributeMaxDynamicSharedMemorySize,
+        (void)hipFuncSetAttribute((const void*)k_vcycle_tail<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   TAIL_LDS_BYTES);
         attr = true;
     }
-    hipLaunchKernelGGL(k_vcycle_tail, dim3(1), dim3(TAIL_THREADS), t.ldsBytes, st, t);
+    hipLaunchKernelGGL(k_vcycle_tail<T>, dim3(1), dim3(TAIL_THREADS), t.ldsBytes, st, t);
 }
+
+void launch_vcycle_tail(hipStream_t st, const TailArgs& t) { launch_tail_t(st, t); }
+void launch_vcycle_tail32(hipStream_t st, const TailArgs32& t) { launch_tail_t(st, t); }
 
 }  // namespace vol

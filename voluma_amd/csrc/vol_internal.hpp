@@ -93,24 +93,27 @@ enum XLoad : int {
     XL_ZEROPROLONG = 4,  // ω·(r[j]·rD[j]) + xc[agg[j]]
 };
 
-struct SellArgs {
+template <class T> struct SellArgsT {
     SellView S;
-    const double* coef = nullptr;
-    const double* diag = nullptr;
-    const double* rD = nullptr;     // 1/diag (XL_FROMZERO / XL_ZEROPROLONG)
-    const double* x = nullptr;
-    const double* r = nullptr;
-    const double* xc = nullptr;
+    const T* coef = nullptr;
+    const T* diag = nullptr;
+    const T* rD = nullptr;          // 1/diag (XL_FROMZERO / XL_ZEROPROLONG)
+    const T* x = nullptr;
+    const T* r = nullptr;
+    const T* xc = nullptr;
     const int* agg = nullptr;
-    const double* xuPtr = nullptr;  // XL_UNIFORM value (device scalar)
+    const T* xuPtr = nullptr;       // XL_UNIFORM value (device scalar)
     double omega = 1.0;
-    double* out = nullptr;
-    const double* dotv = nullptr;   // fused dot: Σ out·dotv into part (fast mode only)
+    T* out = nullptr;
+    const T* dotv = nullptr;        // fused dot: Σ out·dotv (in double) into part (fast mode only)
     double* part = nullptr;
     const int* done = nullptr;
 };
+using SellArgs = SellArgsT<double>;
+using SellArgs32 = SellArgsT<float>;
 
 void launch_sell(hipStream_t st, int op, int xl, const SellArgs& a);
+void launch_sell32(hipStream_t st, int op, int xl, const SellArgs32& a);
 int sell_grid_blocks(int nslices);  // blocks used by launch_sell (== partial count when fused)
 
 void launch_sell_coef(hipStream_t st, const SellView& S, const double* upper, const double* lower, double* coef,
@@ -179,32 +182,36 @@ constexpr int TAIL_THREADS = 1024;
 // The tail's vectors (r, two work buffers, x per level) live in LDS at the given
 // offsets (in doubles); only matrix data is read from global memory.
 constexpr int TAIL_LDS_BYTES = 160 * 1024;  // gfx950 LDS per workgroup
-struct TailLevel {
+template <class T> struct TailLevelT {
     SellView S;
-    const double* coef = nullptr;
-    const double* diag = nullptr;
-    const double* rD = nullptr;
+    const T* coef = nullptr;
+    const T* diag = nullptr;
+    const T* rD = nullptr;
     const int* agg = nullptr; // → next level
     const int* mOff = nullptr;
     const int* mList = nullptr;
     int nNext = 0;
-    int ldsR = 0, ldsA = 0, ldsB = 0, ldsX = 0;
+    int ldsR = 0, ldsA = 0, ldsB = 0, ldsX = 0;  // offsets in elements of T
 };
-struct TailArgs {
+template <class T> struct TailArgsT {
     int nLevels = 0;                  // levels in the tail, the last is the coarsest
-    const TailLevel* L = nullptr;     // device array
-    const double* rTop = nullptr;     // right-hand side of the first tail level (global)
-    double* xTop = nullptr;           // its V-cycle result (global)
+    const TailLevelT<T>* L = nullptr; // device array
+    const T* rTop = nullptr;          // right-hand side of the first tail level (global)
+    T* xTop = nullptr;                // its V-cycle result (global)
     int nCoarse = 0;
-    const double* lu = nullptr;
+    const double* lu = nullptr;       // coarsest LU factors, always FP64
     const double* luT = nullptr;
-    int ldsCoarse = 0;                // LU factors (n ≤ 64) or the substitution vector
-    int ldsBytes = 0;
+    int ldsBytes = 0;                 // LDS: [coarsest LU / substitution vector (doubles)] [level vectors (T)]
     int nPre = 2, nPost = 2;
     double omega = 0.8;
     const int* done = nullptr;
 };
+using TailLevel = TailLevelT<double>;
+using TailArgs = TailArgsT<double>;
+using TailLevel32 = TailLevelT<float>;
+using TailArgs32 = TailArgsT<float>;
 void launch_vcycle_tail(hipStream_t st, const TailArgs& t);
+void launch_vcycle_tail32(hipStream_t st, const TailArgs32& t);
 // buffer (0 = bufA, 1 = bufB) holding the pre-smoothed x after nPre sweeps
 __host__ __device__ inline int pre_smoothed_buffer(int nPre) { return nPre >= 2 ? ((nPre - 2) & 1) : 0; }
 
