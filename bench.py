@@ -4,16 +4,21 @@ A step is one pressure solve of the 2.06M-cell jittered-tet Poisson system
 (70³ hexes × 6 Kuhn tets, ±20 % interior jitter, seed 1; fvm::laplacian with
 fixedValue 0 on xMax) by GAMG-preconditioned PCG from x = 0 to normalised residual
 1e-6, FP64, with the matrix, hierarchy, b and x resident in HBM. Every step
-re-zeroes x on the device and solves again.
+re-zeroes x on the device (device copy) and solves again.
 
 Reported: solves/s (value), p50/p99 solve latency, iterations, the level-0 Amul
 roofline (algorithmic bytes 24n + 16nf per launch over its HIP-event-timed average
 duration) and a CPU baseline: the oracle's restatement of the reference solver
 (GAMG-PCG with Gauss–Seidel smoothing, bit-identical to the reference) on one core.
+
+Beside it, "mixed": the same solves in mixed precision (FP64 defect correction around
+FP32 GAMG-PCG), timed the same way at 1e-6 and 1e-10, with the speed-up over FP64 and
+the relative error of each solution against FP64 GAMG-PCG solved to 1e-12.
 """
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -65,6 +70,7 @@ def main():
     ap.add_argument("--workload", default="tet70")
     ap.add_argument("--tol", type=float, default=1e-6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--pre", type=int, default=1)
     ap.add_argument("--post", type=int, default=1)
     ap.add_argument("--omega", type=float, default=0.9)
@@ -88,13 +94,30 @@ def main():
     n, nf = m.n_cells, m.n_internal
     db = va.DeviceArray.from_host(ctx, b)
     dx = va.DeviceArray(ctx, n)
-    zero = np.zeros(n)
+    dzero = va.DeviceArray.from_host(ctx, np.zeros(n))
     ctl = va.SolverControls("PCG", "GAMG", args.tol, 0.0, 1000, sums=va.SUMS_FAST, gamgPreSweeps=args.pre,
                             gamgPostSweeps=args.post, gamgOmega=args.omega)
 
-    def step():
-        dx.upload(zero)
-        return va.solve_ldu(A, db, dx, ctl)
+    def run(controls, steps, timing=False):
+        """steps solves from x = 0 on the device; (elapsed s, latencies s, iterations, perf, x)."""
+        if timing:
+            ctx.timing(True)
+            ctx.read_timing(reset=True)
+        lat, its = [], []
+        ctx.synchronize()
+        t_start = time.perf_counter()
+        p = None
+        for k in range(steps):
+            t1 = time.perf_counter()
+            dx.copy_from(dzero)  # x = 0 on the device
+            p = va.solve_ldu(A, db, dx, controls)
+            lat.append(time.perf_counter() - t1)
+            its.append(p.iterations)
+            if not p.converged:
+                raise RuntimeError(f"solve {k} did not converge: {p}")
+        ctx.synchronize()
+        elapsed = time.perf_counter() - t_start
+        return elapsed, np.array(lat), its, p, dx.download()
 
     # achievable HBM bandwidth on this GPU: streaming copy of 2^27 doubles (1 GiB each way)
     ncopy = 1 << 27
@@ -106,27 +129,11 @@ def main():
     dst.free()
     log(f"stream copy: {stream_gbs:.0f} GB/s")
 
-    for _ in range(args.warmup):
-        p = step()
-    ctx.synchronize()
-    ctx.timing(True)
-    ctx.read_timing(reset=True)
-    lat = []
-    iters = []
-    t_start = time.perf_counter()
-    for k in range(args.steps):
-        t1 = time.perf_counter()
-        p = step()
-        lat.append(time.perf_counter() - t1)
-        iters.append(p.iterations)
-        if not p.converged:
-            raise RuntimeError(f"solve {k} did not converge: {p}")
-    ctx.synchronize()
-    elapsed = time.perf_counter() - t_start
+    run(ctl, args.warmup)
+    elapsed, lat, iters, p, x = run(ctl, args.steps, timing=True)
     amul_ms, amul_cnt = ctx.read_timing(reset=True)
     ctx.timing(False)
-    lat_ms = np.array(lat) * 1e3
-    x = dx.download()
+    lat_ms = lat * 1e3
     # verify the last solution's true residual on the host side of the library
     r = b - va.amul(A, x)
     true_res = float(np.abs(r).sum() / p.normFactor)
@@ -138,6 +145,41 @@ def main():
     if os.path.exists(prof):
         with open(prof) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
+
+    mixed = None
+    if not args.no_mixed:
+        log("mixed precision ...")
+        mctl = dataclasses.replace(ctl, precision=va.MIXED)
+        _, _, _, pref, xref = run(dataclasses.replace(ctl, tolerance=1e-12, maxIter=3000), 1)
+        nref = np.linalg.norm(xref)
+        run(mctl, args.warmup)
+        m_el, m_lat, m_its, mp, mx = run(mctl, args.steps, timing=True)
+        a32_ms, a32_cnt = ctx.read_timing(reset=True)
+        ctx.timing(False)
+        k10 = max(2, args.steps // 4)
+        f_el10, _, f_its10, _, fx10 = run(dataclasses.replace(ctl, tolerance=1e-10, maxIter=3000), k10)
+        m_el10, _, m_its10, mp10, mx10 = run(dataclasses.replace(mctl, tolerance=1e-10, maxIter=3000), k10)
+        a32_bytes = 12 * n + 12 * nf
+        a32_s = a32_ms / max(a32_cnt, 1) / 1e3
+        mixed = {
+            "value": args.steps / m_el, "unit": "solves/s", "ms_per_step": m_el / args.steps * 1e3,
+            "speedup_vs_fp64": (args.steps / m_el) / (args.steps / elapsed),
+            "latency_ms": {"p50": float(np.percentile(m_lat * 1e3, 50)), "p99": float(np.percentile(m_lat * 1e3, 99))},
+            "iterations": int(np.median(m_its)), "outer_iterations": mp.outerIterations,
+            "final_residual": mp.finalResidual,
+            "rel_err_vs_fp64_1e-12": float(np.linalg.norm(mx - xref) / nref),
+            "fp64_rel_err_vs_fp64_1e-12": float(np.linalg.norm(x - xref) / nref),
+            "tol_1e-10": {"mixed_ms": m_el10 / k10 * 1e3, "fp64_ms": f_el10 / k10 * 1e3,
+                          "speedup_vs_fp64": f_el10 / m_el10, "mixed_iterations": int(np.median(m_its10)),
+                          "mixed_outer_iterations": mp10.outerIterations, "fp64_iterations": int(np.median(f_its10)),
+                          "rel_err_vs_fp64_1e-12": float(np.linalg.norm(mx10 - xref) / nref),
+                          "fp64_rel_err_vs_fp64_1e-12": float(np.linalg.norm(fx10 - xref) / nref)},
+            "amul32": {"achieved_gbs": a32_bytes / a32_s / 1e9 if a32_cnt else None, "avg_us": a32_s * 1e6,
+                       "algorithmic_bytes": a32_bytes, "launches": a32_cnt},
+        }
+        log(f"mixed: {mixed['value']:.2f} solves/s ({mixed['speedup_vs_fp64']:.2f}x), "
+            f"err {mixed['rel_err_vs_fp64_1e-12']:.2e}; 1e-10: {mixed['tol_1e-10']['speedup_vs_fp64']:.2f}x, "
+            f"err {mixed['tol_1e-10']['rel_err_vs_fp64_1e-12']:.2e}")
     out = {
         "metric": METRIC,
         "value": args.steps / elapsed,
@@ -156,6 +198,7 @@ def main():
                    "cells": n, "internal_faces": nf, "gamg_levels": levels, "precision": "fp64"},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),
                        "min": float(lat_ms.min())},
+        "mixed": mixed,
         "iterations": int(np.median(iters)),
         "final_residual": p.finalResidual,
         "true_residual": true_res,

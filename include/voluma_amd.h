@@ -58,6 +58,13 @@ extern "C" {
 
 /* ---- arithmetic of the solve (vol_solver_controls.precision) */
 #define VOL_FP64 0
+/* FP64 defect correction around an FP32 inner Krylov solve: each outer step forms
+ * r = b − A·x in FP64, solves A·e = r with PCG/PBiCGStab whose Amul, GAMG smoothing,
+ * restriction/prolongation and vectors are FP32 (dot products, norms and the coarsest
+ * LU solve in FP64), until the inner residual falls by innerRelTol, then x += e.
+ * Convergence is judged on the FP64 residual with the usual normalisation. Fast sums
+ * only; no counterpart in the reference. */
+#define VOL_MIXED 1
 
 typedef struct vol_context_s* vol_context;
 typedef struct vol_mesh_s* vol_mesh;
@@ -77,10 +84,11 @@ typedef struct {
     int nSweeps;        /* 1 */
     /* extensions */
     int sums;           /* VOL_SUMS_*, default VOL_SUMS_CONTEXT */
-    int precision;      /* VOL_FP64 */
+    int precision;      /* VOL_FP64 | VOL_MIXED */
     int gamgPreSweeps;  /* damped-Jacobi GAMG smoother sweeps (default 2, ≥ 1) */
     int gamgPostSweeps; /* default 2, ≥ 1 */
     double gamgOmega;   /* damping (default 0.8) */
+    double innerRelTol; /* VOL_MIXED: residual reduction per inner solve (default 1e-4, in (0, 1)) */
 } vol_solver_controls;
 
 /* SolverPerformance (LinearSolver.h:25-31) */
@@ -90,8 +98,10 @@ typedef struct {
     double finalResidual;
     double residualSum;
     double normFactor;
-    int iterations;
+    int iterations;       /* Krylov iterations (VOL_MIXED: summed over the inner solves) */
     int converged;
+    int outerIterations;  /* VOL_MIXED: defect-correction steps; 0 for VOL_FP64 */
+    int reserved;
 } vol_solver_performance;
 
 void vol_controls_default(vol_solver_controls* c);
@@ -166,6 +176,49 @@ int vol_assemble_laplacian(vol_matrix A, double gamma, const double* faceGamma, 
 int vol_assemble_div_upwind(vol_matrix A, const double* flux, const int32_t* bcFixed, const double* bcValue,
                             const double* bIn, double* bOut, double* internalCoeffs, double* boundarySource,
                             int location);
+
+/* ---- FvMatrix algebra on the GPU (FvMatrix.h:21-108,154-163,232-244; Fvm.h:18-186).
+ * A vol_matrix also holds the FvMatrix parts of one field component: the unfolded
+ * diagonal, source (nCells), internalCoeffs and boundarySource (nBoundaryFaces) and the
+ * face-flux correction (nInternalFaces + nBoundaryFaces); upper/lower are the matrix's.
+ * vol_fvm_reset zeroes them, as FvMatrix(psi) does. Each term is built alone, as the
+ * reference builds it in a fresh FvMatrix, and added with sign +1 (operator+=) or −1
+ * (operator-=); the lower part appears when a term with one (div) is added. So
+ *     UEqn = ddt(U) + div(phi, U) − laplacian(nu, U)   (IncompressibleFlow.cpp:183-185)
+ * is vol_fvm_reset, vol_fvm_ddt_euler(+1), vol_fvm_div_upwind(+1),
+ * vol_fvm_laplacian(−1), and the parts are bit-identical to the reference's.
+ * vol_fvm_finish folds internalCoeffs into the diagonal and boundarySource into
+ * b = source (FvMatrix.h:154-163) and makes A ready for vol_solve. Boundary faces are
+ * fixedValue (bcFixed 1, value bcValue, nullable → 0) or zeroGradient (0). Face arrays
+ * hold nInternalFaces + nBoundaryFaces entries (Sf, corrVecs: 3 per face). Needs
+ * vol_mesh_set_boundary. */
+typedef struct {
+    const double* psi;      /* current field (nCells): its Gauss-linear gradient (Fvc.h:208-224) */
+    const double* w;        /* face interpolation weights */
+    const double* Sf;       /* face area vectors */
+    const double* corrVecs; /* non-orthogonal correction vectors */
+    const double* V;        /* cell volumes */
+} vol_nonorth;
+int vol_fvm_reset(vol_matrix A);
+/* fvm::ddt, Euler (Fvm.h:18-31; Schemes.h:71-86): diag += V/dt, source += V·psi0/dt */
+int vol_fvm_ddt_euler(vol_matrix A, int sign, double dt, const double* V, const double* psi0, int location);
+/* fvm::div(F, ψ, upwind) (Fvm.h:74-137) */
+int vol_fvm_div_upwind(vol_matrix A, int sign, const double* flux, const int32_t* bcFixed, const double* bcValue,
+                       int location);
+/* fvm::laplacian(γ, ψ, nonOrthCorr) (Fvm.h:141-186); faceGamma nullable → uniform gamma;
+ * nonOrth nullable → no explicit correction; otherwise it is applied when some internal
+ * face has |corrVecs|² > 1e-20 (Fvm.h:157), with fc = γ|S|·(corr·(w g_P + (1−w) g_N)). */
+int vol_fvm_laplacian(vol_matrix A, int sign, double gamma, const double* faceGamma, const double* magSf,
+                      const double* ndc, const int32_t* bcFixed, const double* bcValue, const vol_nonorth* nonOrth,
+                      int location);
+/* FvMatrix −= su (FvMatrix.h:93-97): source += su·V per cell */
+int vol_fvm_source(vol_matrix A, const double* su, const double* V, int location);
+/* copy out the parts (each nullable) */
+int vol_fvm_get(vol_matrix A, double* diag, double* source, double* internalCoeffs, double* boundarySource,
+                double* faceFluxCorrection, int location);
+/* fold and finish (FvMatrix.h:154-163): A's solve diagonal = diag + Σ internalCoeffs,
+ * b (nullable, nCells) = source + Σ boundarySource per owner cell */
+int vol_fvm_finish(vol_matrix A, double* b, int location);
 
 /* ---- measurement hooks (no reference counterpart): time every level-0 A·p
  * product issued by vol_solve with HIP events on the context stream. */

@@ -221,3 +221,54 @@ def fold(mesh, diag, source, ic, bs):
     _lib().oracle_fold(n, mesh.n_internal, mesh.n_boundary, _p(_i32(mesh.owner)), _p(_f64(diag)), _p(_f64(source)),
                        _p(_f64(ic)), _p(_f64(bs)), _p(d), _p(b))
     return d, b
+
+
+def boundary_values(mesh, psi, bc_fixed, bc_value=None):
+    """Boundary face values after correctBoundaryConditions: the fixed value on
+    fixedValue faces, the owner cell's value on zeroGradient faces
+    (BoundaryCondition.h:96-137)."""
+    ni = mesh.n_internal
+    own_b = np.asarray(mesh.owner[ni:])
+    bcv = np.zeros(mesh.n_boundary) if bc_value is None else _f64(bc_value)
+    return np.where(np.asarray(bc_fixed) != 0, bcv, _f64(psi)[own_b])
+
+
+def transport(mesh, dt, nu, psi0, F, bc_fixed, bc_value=None, non_orth=True):
+    """One component of the momentum matrix as IncompressibleFlow.cpp:183-185 composes
+    it, ddt(Euler) + div(F, upwind) − laplacian(nu), restated with the FvMatrix algebra
+    of FvMatrix.h:57-80,232-239 (each term built alone, then added with s = ±1; the
+    lower part appears when the div term, which has one, is added). psi0 is both the
+    old-time and the current field. Returns the FvMatrix parts and the fold
+    (FvMatrix.h:154-163) as a dict."""
+    n, ni, nb = mesh.n_cells, mesh.n_internal, mesh.n_boundary
+    psi0 = _f64(psi0)
+    # ddt: FvMatrix(psi) zeros, then diag += c·V, source += V(c0 ψ⁰ − c00 ψ⁰⁰)
+    diag, source = ddt_euler(mesh, dt, psi0)
+    upper = np.zeros(ni)
+    ic, bs = np.zeros(nb), np.zeros(nb)
+    # += div: the div term has a lower part, so lower() is first copied from upper
+    dd, du, dl, dic, dbs = div_upwind(mesh, F, bc_fixed, bc_value)
+    diag = diag + dd
+    lower = upper.copy() + 1.0 * dl
+    upper = upper + 1.0 * du
+    source = source + np.zeros(n)
+    ic, bs = ic + dic, bs + dbs
+    # −= laplacian(nu) (symmetric: its lowerPtr is its upper)
+    gam = np.full(mesh.n_faces, float(nu))
+    ld, lu, lic, lbs = laplacian(mesh, bc_fixed, bc_value, gamma=gam)
+    lsrc = np.zeros(n)
+    lffc = None
+    if non_orth and np.any(np.einsum("ij,ij->i", mesh.geometry()["corr"][:ni], mesh.geometry()["corr"][:ni]) > 1e-20):
+        g = grad_gauss(mesh, psi0, boundary_values(mesh, psi0, bc_fixed, bc_value))
+        lsrc, lffc = laplacian_nonorth(mesh, g, gam)
+    diag = diag - ld
+    lower = lower + (-1.0) * lu
+    upper = upper + (-1.0) * lu
+    source = source - lsrc
+    ic, bs = ic - lic, bs - lbs
+    ffc = np.zeros(mesh.n_faces)
+    if lffc is not None:
+        ffc[:ni] = ffc[:ni] + (-1.0) * lffc
+    dF, bF = fold(mesh, diag, source, ic, bs)
+    return {"diag": diag, "upper": upper, "lower": lower, "source": source, "ic": ic, "bs": bs, "ffc": ffc,
+            "dF": dF, "bF": bF}

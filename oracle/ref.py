@@ -46,6 +46,8 @@ def _lib():
         lib.vr_div_upwind.restype = I
         lib.vr_ddt_euler.argtypes = [P, D, P, P, P]
         lib.vr_ddt_euler.restype = I
+        lib.vr_transport.argtypes = [P, D, D, P, P, P, P, P, I] + [P] * 9
+        lib.vr_transport.restype = I
         lib.vr_solve.argtypes = [P, P, P, P, I, P, P, ctypes.c_char_p, ctypes.c_char_p, D, D, I, I, I, P, P, P]
         lib.vr_solve.restype = I
         lib.vr_fvm_poisson_solve.argtypes = [P, D, P, P, P, P, ctypes.c_char_p, ctypes.c_char_p, D, D, I, P, P, P]
@@ -154,6 +156,25 @@ class RefMesh:
         out = {"diag": np.zeros(nc), "source": np.zeros(nc)}
         psi0 = np.ascontiguousarray(psi0, dtype=np.float64)
         _check(_lib().vr_ddt_euler(self._h, ctypes.c_double(dt), _p(psi0), _p(out["diag"]), _p(out["source"])))
+        return out
+
+    def transport(self, dt, nu, psi0, F, fixed_patches, fixed_values=None, non_orth=True, face_values=None):
+        """ddt(Euler) + div(F, upwind) − laplacian(nu) for one field component, as
+        IncompressibleFlow composes its momentum matrix; parts and the folded (d, b)."""
+        nc, ni, nf = self.n_cells, self.n_internal, self.n_faces
+        fp = np.zeros(self.n_patches, np.int32)
+        fp[list(fixed_patches)] = 1
+        fv = None if fixed_values is None else np.ascontiguousarray(fixed_values, dtype=np.float64)
+        keys = ("diag", "upper", "lower", "source", "ic", "bs", "ffc", "dF", "bF")
+        sizes = (nc, ni, ni, nc, nf - ni, nf - ni, nf, nc, nc)
+        out = {k: np.zeros(s) for k, s in zip(keys, sizes)}
+        _check(_lib().vr_transport(self._h, ctypes.c_double(dt), ctypes.c_double(nu),
+                                   _p(np.ascontiguousarray(psi0, dtype=np.float64)),
+                                   _p(np.ascontiguousarray(F, dtype=np.float64)), _p(fp), _p(fv),
+                                   None if face_values is None else _p(np.ascontiguousarray(face_values,
+                                                                                            dtype=np.float64)),
+                                   int(non_orth),
+                                   *(_p(out[k]) for k in keys)))
         return out
 
     def solve(self, diag, upper, lower, b, x0, solver="PCG", preconditioner="diagonal", tolerance=1e-8,

@@ -260,6 +260,65 @@ int vr_ddt_euler(void* h, double dt, const double* psi0, double* diag, double* s
     }
 }
 
+// One component of the momentum matrix as IncompressibleFlow::step composes it
+// (IncompressibleFlow.cpp:183-185): fvm::ddt (Euler) + fvm::div(F, upwind) −
+// fvm::laplacian(nu, nonOrth), for a scalar field with old and current value psi0.
+// The FvMatrix parts are returned as they stand, then the boundary fold that
+// FvMatrix::solve applies (FvMatrix.h:154-163) into dF/bF.
+// faceValues (nullable, nBoundaryFaces): per-face values of the fixedValue patches.
+int vr_transport(void* h, double dt, double nu, const double* psi0, const double* F, const int* fixedPatch,
+                 const double* fixedValue, const double* faceValues, int nonOrth, double* diag, double* upper,
+                 double* lower, double* source, double* ic, double* bs, double* ffc, double* dF, double* bF) {
+    try {
+        MeshPtr mp = static_cast<RefMesh*>(h)->m;
+        const Mesh& m = *mp;
+        ScalarField p(mp, "U", 0.0);
+        for (label c = 0; c < m.nCells(); ++c) p[c] = psi0[c];
+        setScalarBCs(p, fixedPatch, fixedValue);
+        if (faceValues) {
+            for (std::size_t q = 0; q < m.patches().size(); ++q) {
+                auto* fv = dynamic_cast<FixedValueBC<scalar>*>(&p.bc(label(q)));
+                if (!fv) continue;
+                const Patch& pt = m.patches()[q];
+                for (label i = 0; i < pt.size; ++i) fv->values()[i] = faceValues[pt.start + i - m.nInternalFaces()];
+            }
+            p.correctBoundaryConditions();
+        }
+        p.storeOld();
+        TimeState ts;
+        ts.dt = dt;
+        std::vector<scalar> Fv(F, F + m.nFaces());
+        ConvectionScheme cs;
+        cs.type = ConvectionScheme::Type::Upwind;
+        auto M = fvm::ddt(p, DdtScheme::Euler, ts);
+        M += fvm::div(Fv, p, cs);
+        M -= fvm::laplacian(nu, p, nonOrth != 0);
+        copySc(M.diag(), diag);
+        copySc(M.upper(), upper);
+        std::memcpy(lower, M.lowerPtr(), sizeof(double) * m.nInternalFaces());
+        copySc(M.source(), source);
+        copySc(M.internalCoeffs(), ic);
+        copySc(M.boundarySource(), bs);
+        if (M.hasFaceFluxCorrection())
+            copySc(M.faceFluxCorrection(), ffc);
+        else
+            std::memset(ffc, 0, sizeof(double) * m.nFaces());
+        for (label c = 0; c < m.nCells(); ++c) {
+            dF[c] = M.diag()[c];
+            bF[c] = M.source()[c];
+        }
+        const label nI = m.nInternalFaces();
+        for (label bf = 0; bf < m.nBoundaryFaces(); ++bf) {
+            const label c = m.owner()[nI + bf];
+            dF[c] += M.internalCoeffs()[bf];
+            bF[c] += M.boundarySource()[bf];
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
 // solveLdu on a raw LDU system (diag already boundary-folded). lower may equal upper.
 int vr_solve(void* h, const double* diag, const double* upper, const double* lower, int symmetric, const double* b,
              double* x, const char* solver, const char* precond, double tol, double relTol, int maxIter, int minIter,

@@ -62,6 +62,9 @@ def mesh_hash(m) -> str:
     return h.hexdigest()
 
 
+MOM_DT, MOM_NU, MOM_BC_VALUE = 0.05, 0.01, 0.7
+
+
 def flux_field(m, seed):
     """A face flux for the nonsymmetric (momentum-like) test system: seeded values
     in [−1, 1)·|S_f| on every face."""
@@ -109,6 +112,15 @@ def small_case(name, make):
     for k in ("diag", "upper", "lower", "ic", "bs"):
         out["div_" + k] = dv[k]
     out["div_flux"] = F
+    # momentum-matrix component: ddt(Euler) + div(F, upwind) − laplacian(nu) with the
+    # non-orthogonal correction, fixedValue 0.7 on xMax (IncompressibleFlow.cpp:183-185)
+    psi0 = 2.0 * W.splitmix_uniform(n, 31) - 1.0
+    vals = np.zeros(len(m.patch_start))
+    vals[W.XMAX_PATCH] = MOM_BC_VALUE
+    tr = rm.transport(MOM_DT, MOM_NU, psi0, F, [W.XMAX_PATCH], vals, True)
+    out["mom_psi0"] = psi0
+    for k, v in tr.items():
+        out["mom_" + k] = v
     out["sym_diag"], out["sym_upper"], out["sym_b"] = dsym, upper, b
     out["ns_diag"], out["ns_upper"], out["ns_lower"], out["ns_b"] = dns, uns, lns, bns
     # hierarchy
@@ -197,6 +209,34 @@ def full_case(tag, name):
     print(f"{tag}_{name}: {n} cells, levels {res['gamg_sizes']}, geometry equal {geom_equal}", flush=True)
 
 
+def momentum_case(tag, name):
+    """Momentum matrix of every velocity component on the full mesh (reference
+    fvm::ddt + fvm::div − fvm::laplacian with non-orthogonal correction, fixedValue U
+    on every boundary face), and the reference's reproducible PBiCGStab (DILU → diagonal
+    swap) from x = U⁰ to 1e-6."""
+    m = W.make_mesh(name)
+    rm = ref.RefMesh(mesh=m)
+    inp = W.momentum_inputs(m, 1)
+    allp = list(range(len(m.patch_start)))
+    res = {"mesh": name, "dt": inp["dt"], "nu": inp["nu"], "phi_sha256": sha(inp["phi"]),
+           "U0_sha256": sha(inp["U0"]), "Ub_sha256": sha(inp["Ub"]), "components": []}
+    for comp in range(3):
+        tr = rm.transport(inp["dt"], inp["nu"], inp["U0"][:, comp], inp["phi"], allp, None, True,
+                          face_values=inp["Ub"][:, comp])
+        x0 = np.ascontiguousarray(inp["U0"][:, comp])
+        x, p = rm.solve(tr["dF"], tr["upper"], tr["lower"], tr["bF"], x0, solver="PBiCGStab", preconditioner="DILU",
+                        tolerance=1e-6, max_iter=1000, reproducible=True)
+        res["components"].append({
+            "diag_sha256": sha(tr["dF"]), "upper_sha256": sha(tr["upper"]), "lower_sha256": sha(tr["lower"]),
+            "b_sha256": sha(tr["bF"]), "ffc_sha256": sha(tr["ffc"]),
+            "repro_pbicgstab_1e-6": {"iterations": p["iterations"], "initialResidual": p["initialResidual"],
+                                     "finalResidual": p["finalResidual"], "normFactor": p["normFactor"],
+                                     "converged": p["converged"], "x": summary(x)}})
+        print(f"  {tag} momentum component {comp}: {p['iterations']} iterations", flush=True)
+    with open(os.path.join(HERE, f"{tag}_{name}_momentum.json"), "w") as fh:
+        json.dump(res, fh, indent=1)
+
+
 def exactsum_case():
     out = {"cases": []}
     for seed, scale in ((1, 1.0), (2, 1e20), (3, 1e-300), (4, 1e300)):
@@ -227,3 +267,5 @@ if __name__ == "__main__":
     if "--small-only" not in sys.argv:
         full_case("cfg1", "box64")
         full_case("cfg2", "tet70")
+    if "--small-only" not in sys.argv or "--momentum" in sys.argv:
+        momentum_case("cfg2", "tet70")

@@ -40,7 +40,21 @@ def test_controls_defaults_match_reference():
     assert (c.gamgPreSweeps, c.gamgPostSweeps, c.gamgOmega) == (py.gamgPreSweeps, py.gamgPostSweeps, py.gamgOmega)
 
 
-def test_struct_layouts():
-    # vol_solver_controls: 2 pointers, 2 doubles, 7 ints, 1 double (with padding) on LP64
-    assert ctypes.sizeof(va._Controls) == 16 + 16 + 28 + 4 + 8
-    assert ctypes.sizeof(va._Perf) == 32 + 32 + 8
+def test_struct_layouts(tmp_path):
+    """ctypes mirrors agree with the C header: size and every field offset, from gcc."""
+    fields = {"vol_solver_controls": va._Controls, "vol_solver_performance": va._Perf}
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "voluma_amd.h"', "int main(void) {"]
+    for cname, cls in fields.items():
+        lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
+        for f, _ in cls._fields_:
+            lines.append(f'printf("{cname}.{f} %zu\\n", offsetof({cname}, {f}));')
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = dict(l.split() for l in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split("\n") if l)
+    for cname, cls in fields.items():
+        assert int(got[cname]) == ctypes.sizeof(cls), cname
+        for f, _ in cls._fields_:
+            assert int(got[f"{cname}.{f}"]) == getattr(cls, f).offset, (cname, f)

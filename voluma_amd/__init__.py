@@ -26,7 +26,7 @@ LIB_PATH = os.path.join(_HERE, "libvoluma_amd.so")
 VOL_OK = 0
 VOL_HOST, VOL_DEVICE = 0, 1
 SUMS_CONTEXT, SUMS_FAST, SUMS_REPRODUCIBLE, SUMS_EXACT = -1, 0, 1, 2
-FP64 = 0
+FP64, MIXED = 0, 1
 
 _ERRORS = {
     -1: "invalid argument", -2: "unknown solver", -3: "unknown preconditioner", -4: "PCG on nonsymmetric matrix",
@@ -46,13 +46,32 @@ class _Controls(ctypes.Structure):
     _fields_ = [("solver", ctypes.c_char_p), ("preconditioner", ctypes.c_char_p), ("tolerance", ctypes.c_double),
                 ("relTol", ctypes.c_double), ("maxIter", ctypes.c_int), ("minIter", ctypes.c_int),
                 ("nSweeps", ctypes.c_int), ("sums", ctypes.c_int), ("precision", ctypes.c_int),
-                ("gamgPreSweeps", ctypes.c_int), ("gamgPostSweeps", ctypes.c_int), ("gamgOmega", ctypes.c_double)]
+                ("gamgPreSweeps", ctypes.c_int), ("gamgPostSweeps", ctypes.c_int), ("gamgOmega", ctypes.c_double),
+                ("innerRelTol", ctypes.c_double)]
+
+
+class _NonOrth(ctypes.Structure):
+    _fields_ = [("psi", ctypes.c_void_p), ("w", ctypes.c_void_p), ("Sf", ctypes.c_void_p), ("corrVecs", ctypes.c_void_p),
+                ("V", ctypes.c_void_p)]
 
 
 class _Perf(ctypes.Structure):
     _fields_ = [("solver", ctypes.c_char * 32), ("initialResidual", ctypes.c_double),
                 ("finalResidual", ctypes.c_double), ("residualSum", ctypes.c_double),
-                ("normFactor", ctypes.c_double), ("iterations", ctypes.c_int), ("converged", ctypes.c_int)]
+                ("normFactor", ctypes.c_double), ("iterations", ctypes.c_int), ("converged", ctypes.c_int),
+                ("outerIterations", ctypes.c_int), ("reserved", ctypes.c_int)]
+
+
+def _f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _f64n(a):
+    return None if a is None else _f64(a)
+
+
+def _q(a):
+    return None if a is None else _p(a)
 
 
 _LIB = None
@@ -61,7 +80,8 @@ EXPORTED = ("vol_controls_default", "vol_last_error", "vol_context_create", "vol
             "vol_memcpy_to_host", "vol_mesh_create", "vol_mesh_destroy", "vol_mesh_gamg_levels",
             "vol_mesh_gamg_level", "vol_matrix_create", "vol_matrix_set", "vol_matrix_destroy", "vol_amul",
             "vol_solve", "vol_timing_enable", "vol_timing_read", "vol_mesh_set_boundary", "vol_assemble_laplacian",
-            "vol_assemble_div_upwind", "vol_matrix_get", "vol_device_copy")
+            "vol_assemble_div_upwind", "vol_matrix_get", "vol_device_copy", "vol_fvm_reset", "vol_fvm_ddt_euler",
+            "vol_fvm_div_upwind", "vol_fvm_laplacian", "vol_fvm_source", "vol_fvm_get", "vol_fvm_finish")
 
 
 def lib():
@@ -98,6 +118,13 @@ def lib():
         L.vol_device_copy.argtypes = [P, P, P, ctypes.c_int64, I, ctypes.POINTER(D)]
         L.vol_assemble_laplacian.argtypes = [P, D, P, P, P, P, P, P, P, P, P, I]
         L.vol_assemble_div_upwind.argtypes = [P, P, P, P, P, P, P, P, I]
+        L.vol_fvm_reset.argtypes = [P]
+        L.vol_fvm_ddt_euler.argtypes = [P, I, D, P, P, I]
+        L.vol_fvm_div_upwind.argtypes = [P, I, P, P, P, I]
+        L.vol_fvm_laplacian.argtypes = [P, I, D, P, P, P, P, P, ctypes.POINTER(_NonOrth), I]
+        L.vol_fvm_source.argtypes = [P, P, P, I]
+        L.vol_fvm_get.argtypes = [P, P, P, P, P, P, I]
+        L.vol_fvm_finish.argtypes = [P, P, I]
         for name in EXPORTED:
             if name not in ("vol_controls_default", "vol_last_error"):
                 getattr(L, name).restype = I
@@ -129,12 +156,13 @@ class SolverControls:
     gamgPreSweeps: int = 2
     gamgPostSweeps: int = 2
     gamgOmega: float = 0.8
+    innerRelTol: float = 1e-4
 
     def _c(self):
         self._keep = (self.solver.encode(), self.preconditioner.encode())
         return _Controls(self._keep[0], self._keep[1], self.tolerance, self.relTol, self.maxIter, self.minIter,
                          self.nSweeps, self.sums, self.precision, self.gamgPreSweeps, self.gamgPostSweeps,
-                         self.gamgOmega)
+                         self.gamgOmega, self.innerRelTol)
 
 
 @dataclasses.dataclass
@@ -148,6 +176,7 @@ class SolverPerformance:
     normFactor: float = 0.0
     iterations: int = 0
     converged: bool = False
+    outerIterations: int = 0
 
 
 class Context:
@@ -339,6 +368,55 @@ class LduMatrix:
             return b, ic[: m.n_boundary], bs[: m.n_boundary]
         return b
 
+    # ---- FvMatrix algebra on the GPU (host arrays)
+    def fvm_reset(self):
+        """FvMatrix(psi): zero diag/upper/source/internalCoeffs/boundarySource/ffc."""
+        _check(lib().vol_fvm_reset(self._h))
+        self._fvm_lower = False
+
+    def fvm_ddt_euler(self, sign, dt, V, psi0):
+        _check(lib().vol_fvm_ddt_euler(self._h, int(sign), ctypes.c_double(dt), _p(_f64(V)), _p(_f64(psi0)),
+                                       VOL_HOST))
+
+    def fvm_div_upwind(self, sign, flux, bc_fixed, bc_value=None):
+        bcf = np.ascontiguousarray(bc_fixed, dtype=np.int32)
+        _check(lib().vol_fvm_div_upwind(self._h, int(sign), _p(_f64(flux)), _p(bcf), _q(_f64n(bc_value)), VOL_HOST))
+        self._fvm_lower = True
+
+    def fvm_laplacian(self, sign, magSf, ndc, bc_fixed, bc_value=None, gamma=1.0, face_gamma=None, nonorth=None):
+        """nonorth: dict with psi, w, Sf (nFaces×3), corr (nFaces×3), V, or None."""
+        bcf = np.ascontiguousarray(bc_fixed, dtype=np.int32)
+        no = None
+        keep = []
+        if nonorth is not None:
+            arrs = [_f64(nonorth[k]) for k in ("psi", "w", "Sf", "corr", "V")]
+            keep = arrs
+            no = _NonOrth(*(a.ctypes.data for a in arrs))
+        ms, nd, fg, bv = _f64(magSf), _f64(ndc), _f64n(face_gamma), _f64n(bc_value)
+        _check(lib().vol_fvm_laplacian(self._h, int(sign), ctypes.c_double(gamma), _q(fg), _p(ms), _p(nd), _p(bcf),
+                                       _q(bv), None if no is None else ctypes.byref(no), VOL_HOST))
+        del keep
+
+    def fvm_source(self, su, V):
+        """FvMatrix −= su: source += su·V."""
+        _check(lib().vol_fvm_source(self._h, _p(_f64(su)), _p(_f64(V)), VOL_HOST))
+
+    def fvm_get(self):
+        m = self.mesh
+        nb = m.n_boundary
+        out = {"diag": np.zeros(m.n_cells), "source": np.zeros(m.n_cells), "ic": np.zeros(max(nb, 1)),
+               "bs": np.zeros(max(nb, 1)), "ffc": np.zeros(max(m.n_faces + nb, 1))}
+        _check(lib().vol_fvm_get(self._h, *(_p(out[k]) for k in ("diag", "source", "ic", "bs", "ffc")), VOL_HOST))
+        out["ic"], out["bs"], out["ffc"] = out["ic"][:nb], out["bs"][:nb], out["ffc"][: m.n_faces + nb]
+        return out
+
+    def fvm_finish(self):
+        """Fold and make the matrix solvable; returns b (source + folded boundary source)."""
+        b = np.zeros(self.mesh.n_cells)
+        _check(lib().vol_fvm_finish(self._h, _p(b), VOL_HOST))
+        self.symmetric = not getattr(self, "_fvm_lower", False)
+        return b
+
     def close(self):
         if getattr(self, "_h", None):
             lib().vol_matrix_destroy(self._h)
@@ -372,7 +450,7 @@ def solve_ldu(A: LduMatrix, b, x, controls: SolverControls = None, field="") -> 
         bb = np.ascontiguousarray(b, dtype=np.float64)
         _check(lib().vol_solve(A._h, _p(bb), _p(x), ctypes.byref(c), ctypes.byref(pf), VOL_HOST))
     return SolverPerformance(pf.solver.decode(), field, pf.initialResidual, pf.finalResidual, pf.residualSum,
-                             pf.normFactor, pf.iterations, bool(pf.converged))
+                             pf.normFactor, pf.iterations, bool(pf.converged), pf.outerIterations)
 
 
 solveLdu = solve_ldu

@@ -18,6 +18,7 @@
 #include <memory>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -329,10 +330,12 @@ struct vol_mesh_s {
     // boundary faces (for assembly): cell → boundary-face CSR
     int nb = -1;
     DevBuf<int> bOff, bList;
+    DevBuf<int> dOwn, dNei;  // device owner/neighbour, uploaded on first use
 };
 
 struct MatLevel {
     DevBuf<double> diag, rD, Fu, Fl, coef, bufA, bufB, r, x;
+    DevBuf<float> diag32, rD32, coef32, bufA32, bufB32, r32, x32;  // mixed precision
 };
 
 struct vol_matrix_s {
@@ -349,11 +352,22 @@ struct vol_matrix_s {
     int tailLdsBytes = 0;
     DevBuf<TailLevel> tail;
     DevBuf<int> status;
+    // FP32 copies for mixed precision, built lazily after each vol_matrix_set
+    bool mixedReady = false, mixedGamg = false;
+    DevBuf<float> coef32, diag32, rD32;
+    DevBuf<TailLevel32> tail32;
+    DevBuf<float> ir, iw, ip, iq, ie, ir0, iy, iAy, is, iz, it;  // inner-solve vectors
     // solve work vectors
     DevBuf<double> vx, vb, vr, vw, vp, vq, wA, pA, r0, y, Ay, s, z, t;
     DevBuf<double> hx, hy;  // staging for host-pointer calls
     DevBuf<double> st0, st1, st2, st3, st4, st5;  // assembly staging
     DevBuf<int> sti;
+    // FvMatrix parts of device assembly (vol_fvm_*): unfolded diag, source,
+    // internalCoeffs, boundarySource, face-flux correction; upper/lower are shared
+    bool fvmActive = false, fvmHasLower = false;
+    DevBuf<double> fdiag, fsrc, fic, fbs, fffc, fgrad, ffcT;
+    DevBuf<int> anyCorr;
+    DevBuf<double> stg[9];  // staging for host inputs
 };
 
 namespace {
@@ -567,6 +581,74 @@ void buildGamgCoeffs(vol_matrix A) {
     HIPCHECK(hipStreamSynchronize(st));
     if (status != 0) throw VolError(VOL_ERR_SINGULAR_COARSEST, "GAMG: singular coarsest matrix");
     A->gamgReady = true;
+    A->mixedReady = false;
+}
+
+void toFloat(hipStream_t st, const double* a, float* w, size_t n) {
+    launch_mix(st, MIX_TO32, int(n), a, w, nullptr, nullptr, nullptr, nullptr, nullptr);
+}
+
+// FP32 copies of the level-0 matrix and, with GAMG, of every coarse level and the tail
+// descriptors (the coarsest LU stays FP64). Values are rounded from the FP64 ones.
+void buildMixed(vol_matrix A, bool gamg) {
+    vol_mesh m = A->mesh;
+    hipStream_t st = m->ctx->stream;
+    A->coef32.alloc(m->sell.nslots);
+    A->diag32.alloc(m->n);
+    A->rD32.alloc(m->n);
+    toFloat(st, A->coef.p, A->coef32.p, m->sell.nslots);
+    toFloat(st, A->diag.p, A->diag32.p, m->n);
+    toFloat(st, A->rD.p, A->rD32.p, m->n);
+    if (gamg) {
+        const int nL = int(m->lv.size());
+        for (int k = 0; k < nL; ++k) {
+            const LevelDev& D = *m->lv[k];
+            MatLevel& M = *A->L[k];
+            if (k > 0) {
+                M.diag32.alloc(D.n);
+                M.rD32.alloc(D.n);
+                M.coef32.alloc(D.sell.nslots);
+                M.r32.alloc(D.n);
+                M.x32.alloc(D.n);
+                toFloat(st, M.diag.p, M.diag32.p, D.n);
+                toFloat(st, M.rD.p, M.rD32.p, D.n);
+                toFloat(st, M.coef.p, M.coef32.p, D.sell.nslots);
+            }
+            M.bufA32.alloc(D.n);
+            M.bufB32.alloc(D.n);
+        }
+        if (A->tailStart >= 0) {
+            const LevelDev& DL = *m->lv[nL - 1];
+            const int coarseDoubles = DL.n <= 64 ? DL.n * DL.n : DL.n;
+            int off = 2 * coarseDoubles;  // in floats, after the FP64 coarsest section
+            std::vector<TailLevel32> T;
+            for (int k = A->tailStart; k < nL; ++k) {
+                const LevelDev& D = *m->lv[k];
+                MatLevel& M = *A->L[k];
+                TailLevel32 t;
+                t.S = k == 0 ? m->sell.view() : D.sell.view();
+                t.coef = k == 0 ? A->coef32.p : M.coef32.p;
+                t.diag = k == 0 ? A->diag32.p : M.diag32.p;
+                t.rD = k == 0 ? A->rD32.p : M.rD32.p;
+                if (k + 1 < nL) {
+                    t.agg = D.agg.p;
+                    t.mOff = D.mOff.p;
+                    t.mList = D.mList.p;
+                    t.nNext = m->lv[k + 1]->n;
+                }
+                t.ldsR = off;
+                t.ldsA = off + D.n;
+                t.ldsB = off + 2 * D.n;
+                t.ldsX = off + 3 * D.n;
+                off += 4 * D.n;
+                T.push_back(t);
+            }
+            A->tail32.upload(T, st);
+        }
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    A->mixedReady = true;
+    A->mixedGamg = gamg;
 }
 
 // SELL slot coefficients and 1/diag after diag/upper/lower changed.
@@ -582,6 +664,7 @@ void matrixFinish(vol_matrix A) {
     v.w = A->rD.p;
     launch_vec(st, V_RECIP, v, false, m->ctx->red);
     A->gamgReady = false;
+    A->mixedReady = false;
     A->set = true;
 }
 
@@ -657,6 +740,85 @@ void assemble(vol_matrix A, int kind, double gammaU, const double* faceGamma, co
     HIPCHECK(hipStreamSynchronize(st));
 }
 
+// ------------------------------------------------------------------ FvMatrix assembly
+const double* stageK(vol_matrix A, size_t k, const double* p, size_t n, int location) {
+    return stage(A->stg[k], p, n, location, A->mesh->ctx->stream);
+}
+
+void fvmCheck(vol_matrix A, int sign) {
+    if (!A->fvmActive) throw VolError(VOL_ERR_INVALID_ARGUMENT, "call vol_fvm_reset first");
+    if (sign != 1 && sign != -1) throw VolError(VOL_ERR_INVALID_ARGUMENT, "sign must be +1 or -1");
+}
+
+FvmArgs fvmArgs(vol_matrix A, int sign) {
+    vol_mesh m = A->mesh;
+    FvmArgs a;
+    a.n = m->n;
+    a.nf = m->nf;
+    a.nb = m->nb;
+    a.S = m->sell.view();
+    a.bOff = m->bOff.p;
+    a.bList = m->bList.p;
+    a.sign = double(sign);
+    a.diag = A->fdiag.p;
+    a.upper = A->upper.p;
+    a.source = A->fsrc.p;
+    a.ic = A->fic.p;
+    a.bs = A->fbs.p;
+    a.ffc = A->fffc.p;
+    a.hadLower = A->fvmHasLower ? 1 : 0;
+    return a;
+}
+
+void fvmReset(vol_matrix A) {
+    vol_mesh m = A->mesh;
+    if (m->nb < 0) throw VolError(VOL_ERR_INVALID_ARGUMENT, "mesh boundary not set (vol_mesh_set_boundary)");
+    hipStream_t st = m->ctx->stream;
+    const size_t n = m->n, nf = m->nf, nb = m->nb;
+    A->fdiag.alloc(n);
+    A->upper.alloc(nf);
+    A->lower.alloc(nf);
+    A->fsrc.alloc(n);
+    A->fic.alloc(nb);
+    A->fbs.alloc(nb);
+    A->fffc.alloc(nf + nb);
+    const std::pair<DevBuf<double>*, size_t> parts[] = {{&A->fdiag, n}, {&A->upper, nf}, {&A->fsrc, n},
+                                                       {&A->fic, nb},  {&A->fbs, nb},  {&A->fffc, nf + nb}};
+    for (const auto& [b, cnt] : parts)
+        if (cnt) HIPCHECK(hipMemsetAsync(b->p, 0, sizeof(double) * cnt, st));
+    A->fvmActive = true;
+    A->fvmHasLower = false;
+    A->set = false;
+    A->gamgReady = false;
+    A->mixedReady = false;
+}
+
+void fvmTerm(vol_matrix A, int kind, FvmArgs& a, bool termLower) {
+    vol_mesh m = A->mesh;
+    hipStream_t st = m->ctx->stream;
+    const bool newLower = A->fvmHasLower || termLower;
+    a.lower = newLower ? A->lower.p : nullptr;
+    const int* own = nullptr;
+    const int* nei = nullptr;
+    if (kind == FVM_LAP && a.psi) {
+        if (!m->dOwn.p && m->nf) {
+            m->dOwn.upload(m->own, st);
+            m->dNei.upload(m->nei, st);
+        }
+        own = m->dOwn.p;
+        nei = m->dNei.p;
+        A->fgrad.alloc(3 * size_t(m->n));
+        A->ffcT.alloc(m->nf);
+        A->anyCorr.alloc(1);
+        a.grad = A->fgrad.p;
+        a.fc = A->ffcT.p;
+        a.anyCorr = A->anyCorr.p;
+    }
+    launch_fvm_term(st, kind, a, own, nei);
+    A->fvmHasLower = newLower;
+    HIPCHECK(hipStreamSynchronize(st));
+}
+
 // ------------------------------------------------------------------ solve
 struct Plan {
     std::string solver, precond;
@@ -672,11 +834,14 @@ struct Ctx {
     DevScalars* S;
 };
 
-void sell(const Ctx& C, int op, int xl, const SellView& V, const double* coef, const double* diag, const double* x,
-          const double* r, double* out, double omega = 1.0, const double* xc = nullptr, const int* agg = nullptr,
-          const double* dotv = nullptr, double* part = nullptr, const double* xu = nullptr,
-          const double* rD = nullptr) {
-    SellArgs a;
+void launchSell(hipStream_t st, int op, int xl, const SellArgs& a) { launch_sell(st, op, xl, a); }
+void launchSell(hipStream_t st, int op, int xl, const SellArgs32& a) { launch_sell32(st, op, xl, a); }
+
+template <class T>
+void sell(const Ctx& C, int op, int xl, const SellView& V, const T* coef, const T* diag, const T* x, const T* r,
+          T* out, double omega = 1.0, const T* xc = nullptr, const int* agg = nullptr, const T* dotv = nullptr,
+          double* part = nullptr, const T* xu = nullptr, const T* rD = nullptr) {
+    SellArgsT<T> a;
     a.S = V;
     a.coef = coef;
     a.diag = diag;
@@ -691,13 +856,13 @@ void sell(const Ctx& C, int op, int xl, const SellView& V, const double* coef, c
     a.part = part;
     a.done = C.done;
     a.xuPtr = xu;
-    launch_sell(C.st, op, xl, a);
+    launchSell(C.st, op, xl, a);
 }
 
-void vec(const Ctx& C, int op, int n, bool exact, const double* a, const double* b = nullptr,
-         const double* c = nullptr, const double* d = nullptr, double* w = nullptr, double* x = nullptr,
-         double* y = nullptr, const double* rD = nullptr) {
-    VecArgs v;
+template <class T>
+void vec(const Ctx& C, int op, int n, bool exact, const T* a, const T* b = nullptr, const T* c = nullptr,
+         const T* d = nullptr, T* w = nullptr, T* x = nullptr, T* y = nullptr, const T* rD = nullptr) {
+    VecArgsT<T> v;
     v.n = n;
     v.a = a;
     v.b = b;
@@ -709,17 +874,31 @@ void vec(const Ctx& C, int op, int n, bool exact, const double* a, const double*
     v.rD = rD;
     v.S = C.S;
     v.done = C.done;
-    launch_vec(C.st, op, v, exact, C.c->red);
+    if constexpr (std::is_same_v<T, double>)
+        launch_vec(C.st, op, v, exact, C.c->red);
+    else
+        launch_vec32(C.st, op, v, C.c->red);
 }
 
 void fin(const Ctx& C, int op, int nred, bool exact, int nparts) {
     launch_finalize(C.st, op, nred, exact, nparts, C.c->red, C.S);
 }
 
+// Level-0 matrix in the solve's working precision.
+template <class T> struct Level0 {
+    const T* coef;
+    const T* diag;
+    const T* rD;
+};
+Level0<double> level0(vol_matrix A, double*) { return {A->coef.p, A->diag.p, A->rD.p}; }
+Level0<float> level0(vol_matrix A, float*) { return {A->coef32.p, A->diag32.p, A->rD32.p}; }
+
 // Level-0 A·p with optional fused dot (fast mode), timed when timing is on.
-void amulTimed(const Ctx& C, vol_matrix A, const double* p, double* q, const double* dotv, bool fused) {
+template <class T>
+void amulTimed(const Ctx& C, vol_matrix A, const T* p, T* q, const T* dotv, bool fused) {
     vol_context c = C.c;
     const SellView V = A->mesh->sell.view();
+    const Level0<T> L0 = level0(A, (T*)nullptr);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (c->timing) {
         if (c->evUsed + 2 > c->evPool.size()) {
@@ -733,8 +912,8 @@ void amulTimed(const Ctx& C, vol_matrix A, const double* p, double* q, const dou
         e1 = c->evPool[c->evUsed++];
         HIPCHECK(hipEventRecord(e0, C.st));
     }
-    sell(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, p, nullptr, q, 1.0, nullptr, nullptr, fused ? dotv : nullptr,
-         fused ? c->red.part : nullptr);
+    sell<T>(C, OP_AMUL, XL_PLAIN, V, L0.coef, L0.diag, p, nullptr, q, 1.0, nullptr, nullptr, fused ? dotv : nullptr,
+            fused ? c->red.part : nullptr);
     if (c->timing) HIPCHECK(hipEventRecord(e1, C.st));
 }
 
@@ -749,61 +928,111 @@ void collectTiming(vol_context c) {
     c->evUsed = 0;
 }
 
+// Per-level data of the V-cycle in precision T.
+template <class T> struct LevelPtrs {
+    const T *coef, *diag, *rD;
+    T *bufA, *bufB, *r, *x;
+};
+LevelPtrs<double> levelPtrs(vol_matrix A, int l, double*) {
+    MatLevel& M = *A->L[l];
+    if (l == 0) return {A->coef.p, A->diag.p, A->rD.p, M.bufA.p, M.bufB.p, nullptr, nullptr};
+    return {M.coef.p, M.diag.p, M.rD.p, M.bufA.p, M.bufB.p, M.r.p, M.x.p};
+}
+LevelPtrs<float> levelPtrs(vol_matrix A, int l, float*) {
+    MatLevel& M = *A->L[l];
+    if (l == 0) return {A->coef32.p, A->diag32.p, A->rD32.p, M.bufA32.p, M.bufB32.p, nullptr, nullptr};
+    return {M.coef32.p, M.diag32.p, M.rD32.p, M.bufA32.p, M.bufB32.p, M.r32.p, M.x32.p};
+}
+void tailLaunch(const Ctx& C, vol_matrix A, const Plan& P, int l, const double* r, double* xout) {
+    TailArgs t;
+    t.nLevels = int(A->mesh->lv.size()) - l;
+    t.L = A->tail.p;
+    t.rTop = r;
+    t.xTop = xout;
+    t.nCoarse = A->nCoarse;
+    t.lu = A->lu.p;
+    t.luT = A->luT.p;
+    t.ldsBytes = A->tailLdsBytes;
+    t.nPre = P.nPre;
+    t.nPost = P.nPost;
+    t.omega = P.omega;
+    t.done = C.done;
+    launch_vcycle_tail(C.st, t);
+}
+void tailLaunch(const Ctx& C, vol_matrix A, const Plan& P, int l, const float* r, float* xout) {
+    TailArgs32 t;
+    const int nL = int(A->mesh->lv.size());
+    t.nLevels = nL - l;
+    t.L = A->tail32.p;
+    t.rTop = r;
+    t.xTop = xout;
+    t.nCoarse = A->nCoarse;
+    t.lu = A->lu.p;
+    t.luT = A->luT.p;
+    const int nc = A->nCoarse;
+    size_t bytes = size_t(nc <= 64 ? nc * nc : nc) * sizeof(double);
+    for (int k = l; k < nL; ++k) bytes += 4 * size_t(A->mesh->lv[k]->n) * sizeof(float);
+    t.ldsBytes = int(bytes);
+    t.nPre = P.nPre;
+    t.nPost = P.nPost;
+    t.omega = P.omega;
+    t.done = C.done;
+    launch_vcycle_tail32(C.st, t);
+}
+void coarseSolve(const Ctx& C, vol_matrix A, const double* r, double* x) {
+    launch_coarse_solve(C.st, A->nCoarse, A->lu.p, A->luT.p, r, x, C.done);
+}
+void coarseSolve(const Ctx& C, vol_matrix A, const float* r, float* x) {
+    launch_coarse_solve32(C.st, A->nCoarse, A->lu.p, A->luT.p, r, x, C.done);
+}
+void restrictLevel(const Ctx& C, int nc, const LevelDev& D, const double* res, double* rc) {
+    launch_restrict(C.st, nc, D.mOff.p, D.mList.p, res, rc, C.done);
+}
+void restrictLevel(const Ctx& C, int nc, const LevelDev& D, const float* res, float* rc) {
+    launch_restrict32(C.st, nc, D.mOff.p, D.mList.p, res, rc, C.done);
+}
+
 // One V-cycle at level l: r → x (GAMG.cpp:403-443 structure, damped-Jacobi smoothing).
-void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const double* r, double* xout) {
+template <class T>
+void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xout) {
     vol_mesh m = A->mesh;
     const int nL = int(m->lv.size());
     if (l == A->tailStart) {
-        TailArgs t;
-        t.nLevels = nL - l;
-        t.L = A->tail.p;
-        t.rTop = r;
-        t.xTop = xout;
-        t.nCoarse = A->nCoarse;
-        t.lu = A->lu.p;
-        t.luT = A->luT.p;
-        t.ldsBytes = A->tailLdsBytes;
-        t.nPre = P.nPre;
-        t.nPost = P.nPost;
-        t.omega = P.omega;
-        t.done = C.done;
-        launch_vcycle_tail(C.st, t);
+        tailLaunch(C, A, P, l, r, xout);
         return;
     }
     if (l == nL - 1) {
-        launch_coarse_solve(C.st, A->nCoarse, A->lu.p, A->luT.p, r, xout, C.done);
+        coarseSolve(C, A, r, xout);
         return;
     }
     const LevelDev& D = *m->lv[l];
-    MatLevel& M = *A->L[l];
     const SellView V = l == 0 ? m->sell.view() : D.sell.view();
-    const double* coef = l == 0 ? A->coef.p : M.coef.p;
-    const double* diag = l == 0 ? A->diag.p : M.diag.p;
-    const double* rD = l == 0 ? A->rD.p : M.rD.p;
-    double* bufs[2] = {M.bufA.p, M.bufB.p};
+    const LevelPtrs<T> L = levelPtrs(A, l, (T*)nullptr);
+    T* bufs[2] = {L.bufA, L.bufB};
     int cur = 0;  // index of the buffer holding the pre-smoothed x (nPre ≥ 2)
     for (int s = 1; s < P.nPre; ++s) {
         const int dst = (s == 1) ? 0 : 1 - cur;
-        sell(C, OP_JACOBI, s == 1 ? XL_FROMZERO : XL_PLAIN, V, coef, diag, s == 1 ? nullptr : bufs[cur], r, bufs[dst],
-             P.omega, nullptr, nullptr, nullptr, nullptr, nullptr, rD);
+        sell<T>(C, OP_JACOBI, s == 1 ? XL_FROMZERO : XL_PLAIN, V, L.coef, L.diag, s == 1 ? nullptr : bufs[cur], r,
+                bufs[dst], P.omega, nullptr, nullptr, nullptr, nullptr, nullptr, L.rD);
         cur = dst;
     }
     const int freeB = 1 - cur;
     // residual of the pre-smoothed x, then restriction
-    sell(C, OP_RESID, P.nPre == 1 ? XL_FROMZERO : XL_PLAIN, V, coef, diag, P.nPre == 1 ? nullptr : bufs[cur], r,
-         bufs[freeB], P.omega, nullptr, nullptr, nullptr, nullptr, nullptr, rD);
-    MatLevel& MC = *A->L[l + 1];
-    launch_restrict(C.st, m->lv[l + 1]->n, D.mOff.p, D.mList.p, bufs[freeB], MC.r.p, C.done);
-    vcycle(C, A, P, l + 1, MC.r.p, MC.x.p);
+    sell<T>(C, OP_RESID, P.nPre == 1 ? XL_FROMZERO : XL_PLAIN, V, L.coef, L.diag, P.nPre == 1 ? nullptr : bufs[cur],
+            r, bufs[freeB], P.omega, nullptr, nullptr, nullptr, nullptr, nullptr, L.rD);
+    const LevelPtrs<T> LC = levelPtrs(A, l + 1, (T*)nullptr);
+    restrictLevel(C, m->lv[l + 1]->n, D, bufs[freeB], LC.r);
+    vcycle<T>(C, A, P, l + 1, LC.r, LC.x);
     // prolongation fused into the first post-sweep
     int src = cur, other = freeB;
     for (int s = 0; s < P.nPost; ++s) {
-        double* dst = (s == P.nPost - 1) ? xout : bufs[other];
+        T* dst = (s == P.nPost - 1) ? xout : bufs[other];
         if (s == 0)
-            sell(C, OP_JACOBI, P.nPre == 1 ? XL_ZEROPROLONG : XL_PROLONG, V, coef, diag,
-                 P.nPre == 1 ? nullptr : bufs[src], r, dst, P.omega, MC.x.p, D.agg.p, nullptr, nullptr, nullptr, rD);
+            sell<T>(C, OP_JACOBI, P.nPre == 1 ? XL_ZEROPROLONG : XL_PROLONG, V, L.coef, L.diag,
+                    P.nPre == 1 ? nullptr : bufs[src], r, dst, P.omega, LC.x, D.agg.p, nullptr, nullptr, nullptr,
+                    L.rD);
         else
-            sell(C, OP_JACOBI, XL_PLAIN, V, coef, diag, bufs[src], r, dst, P.omega);
+            sell<T>(C, OP_JACOBI, XL_PLAIN, V, L.coef, L.diag, bufs[src], r, dst, P.omega);
         if (s != P.nPost - 1) {
             const int t = src;
             src = other;
@@ -812,12 +1041,13 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const double* r, d
     }
 }
 
-void precondApply(const Ctx& C, vol_matrix A, const Plan& P, const double* r, double* w, int n) {
+template <class T>
+void precondApply(const Ctx& C, vol_matrix A, const Plan& P, const T* r, T* w, int n) {
     if (P.precond == "GAMG")
-        vcycle(C, A, P, 0, r, w);
+        vcycle<T>(C, A, P, 0, r, w);
     else
-        vec(C, V_SCALE, n, false, r, nullptr, nullptr, nullptr, w, nullptr, nullptr,
-            P.precond == "diagonal" ? A->rD.p : nullptr);
+        vec<T>(C, V_SCALE, n, false, r, nullptr, nullptr, nullptr, w, nullptr, nullptr,
+               P.precond == "diagonal" ? level0(A, (T*)nullptr).rD : nullptr);
 }
 
 void readScalars(vol_context c, DevScalars& out) {
@@ -829,6 +1059,90 @@ void readScalars(vol_context c, DevScalars& out) {
 bool hostConverged(const vol_solver_controls& ct, const DevScalars& S) {
     if (S.iterations < ct.minIter) return false;
     return S.finalResidual < ct.tolerance || (ct.relTol > 0 && S.finalResidual < ct.relTol * S.initialResidual);
+}
+
+// Vectors of one Krylov solve in precision T. x is the unknown being updated (the
+// solution in FP64, the correction e in the inner solve of mixed precision); r is its
+// residual, updated recursively.
+template <class T> struct Krylov {
+    T *x, *r, *w, *p, *q;            // PCG (q = A·p)
+    T *r0, *y, *Ay, *s, *z, *t;      // PBiCGStab
+};
+
+// One PCG or PBiCGStab iteration, every kernel predicated on C.done.
+template <class T>
+void krylovIteration(const Ctx& C, vol_matrix A, const Plan& P, const Krylov<T>& k, bool isPCG, bool ex) {
+    const int n = A->mesh->n;
+    const bool fuse = !ex;
+    const int nparts = ex ? RED_BLOCKS_EXACT : RED_BLOCKS;
+    const int sellParts = sell_grid_blocks(A->mesh->sell.nslices);
+    const Level0<T> L0 = level0(A, (T*)nullptr);
+    if (isPCG) {  // LinearSolver.cpp:176-201
+        if (P.precond == "GAMG") {
+            vcycle<T>(C, A, P, 0, k.r, k.w);
+            vec<T>(C, V_DOT, n, ex, k.w, k.r);
+        } else {
+            vec<T>(C, V_PRECOND_DOT, n, ex, k.r, nullptr, nullptr, nullptr, k.w, nullptr, nullptr,
+                   P.precond == "diagonal" ? L0.rD : nullptr);
+        }
+        fin(C, FIN_PCG_WRA, 1, ex, nparts);
+        vec<T>(C, V_PCG_P, n, ex, k.w, nullptr, nullptr, nullptr, nullptr, k.p);
+        amulTimed<T>(C, A, k.p, k.q, k.p, fuse);
+        if (fuse) {
+            fin(C, FIN_PCG_WAPA, 1, false, sellParts);
+        } else {
+            vec<T>(C, V_DOT, n, ex, k.q, k.p);
+            fin(C, FIN_PCG_WAPA, 1, ex, nparts);
+        }
+        vec<T>(C, V_PCG_XR, n, ex, k.p, k.q, nullptr, nullptr, nullptr, k.x, k.r);
+        fin(C, FIN_PCG_NORM, 1, ex, nparts);
+    } else {  // PBiCGStab, LinearSolver.cpp:202-237
+        vec<T>(C, V_DOT, n, ex, k.r0, k.r);
+        fin(C, FIN_BI_RHO, 1, ex, nparts);
+        vec<T>(C, V_BI_P, n, ex, k.r, k.Ay, nullptr, nullptr, nullptr, k.p);
+        precondApply<T>(C, A, P, k.p, k.y, n);
+        amulTimed<T>(C, A, k.y, k.Ay, k.r0, fuse);
+        if (fuse) {
+            fin(C, FIN_BI_ALPHA, 1, false, sellParts);
+        } else {
+            vec<T>(C, V_DOT, n, ex, k.r0, k.Ay);
+            fin(C, FIN_BI_ALPHA, 1, ex, nparts);
+        }
+        vec<T>(C, V_BI_S, n, ex, k.r, k.Ay, nullptr, nullptr, k.s);
+        fin(C, FIN_BI_S, 1, ex, nparts);
+        vec<T>(C, V_BI_XEXIT, n, ex, k.y, nullptr, nullptr, nullptr, nullptr, k.x);
+        launch_clear_exit(C.st, C.S);
+        precondApply<T>(C, A, P, k.s, k.z, n);
+        sell<T>(C, OP_AMUL, XL_PLAIN, A->mesh->sell.view(), L0.coef, L0.diag, k.z, nullptr, k.t);
+        vec<T>(C, V_BI_DOT2, n, ex, k.t, k.s);
+        fin(C, FIN_BI_OMEGA, 2, ex, nparts);
+        vec<T>(C, V_BI_XR, n, ex, k.y, k.z, k.s, k.t, nullptr, k.x, k.r);
+        fin(C, FIN_BI_NORM, 1, ex, nparts);
+    }
+}
+
+// Runs `iterate` until C.S->done, enqueuing one iteration ahead of the host: iteration
+// k+1 is queued before the host reads the scalars after iteration k, and runs as
+// no-ops if k finished.
+template <class F> void runLoop(vol_context c, const Ctx& C, int maxLoops, F&& iterate) {
+    hipStream_t st = C.st;
+    hipEvent_t evs[2];
+    HIPCHECK(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
+    HIPCHECK(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+    DevScalars* hsl = c->hostScal;
+    iterate();
+    HIPCHECK(hipMemcpyAsync(&hsl[0], C.S, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipEventRecord(evs[0], st));
+    for (int k = 1; k <= maxLoops; ++k) {
+        iterate();
+        HIPCHECK(hipMemcpyAsync(&hsl[k & 1], C.S, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipEventRecord(evs[k & 1], st));
+        HIPCHECK(hipEventSynchronize(evs[(k - 1) & 1]));
+        if (hsl[(k - 1) & 1].done) break;
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    (void)hipEventDestroy(evs[0]);
+    (void)hipEventDestroy(evs[1]);
 }
 
 void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_controls& ct, vol_solver_performance& perf) {
@@ -848,7 +1162,9 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
     P.nPre = ct.gamgPreSweeps;
     P.nPost = ct.gamgPostSweeps;
     P.omega = ct.gamgOmega;
-    if (ct.precision != VOL_FP64) throw VolError(VOL_ERR_UNSUPPORTED, "only VOL_FP64 is available");
+    if (ct.precision != VOL_FP64 && ct.precision != VOL_MIXED)
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "precision must be VOL_FP64 or VOL_MIXED");
+    const bool mixed = ct.precision == VOL_MIXED;
     std::memset(&perf, 0, sizeof(perf));
     std::strncpy(perf.solver, P.solver.c_str(), sizeof(perf.solver) - 1);
 
@@ -874,123 +1190,96 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
     }
     P.precond = pc;
     if (ct.maxIter < 0 || ct.nSweeps < 1) throw VolError(VOL_ERR_INVALID_ARGUMENT, "bad maxIter/nSweeps");
+    if (mixed) {
+        if (P.exact) throw VolError(VOL_ERR_UNSUPPORTED, "VOL_MIXED runs with fast sums only");
+        if (isJac) throw VolError(VOL_ERR_UNSUPPORTED, "VOL_MIXED needs PCG or PBiCGStab");
+        if (!(ct.innerRelTol > 0.0 && ct.innerRelTol < 1.0))
+            throw VolError(VOL_ERR_INVALID_ARGUMENT, "innerRelTol must lie in (0, 1)");
+    }
 
     setDevice(c);
     hipStream_t st = c->stream;
     c->ensureScratch(sell_grid_blocks(m->sell.nslices));
     Ctx C{c, st, &c->scal.p->done, c->scal.p};
     const bool ex = P.exact;
-    const bool fuse = !ex;
     const SellView V = m->sell.view();
     A->wA.alloc(n);
     A->pA.alloc(n);
     A->vr.alloc(n);
-    launch_init_scalars(st, C.S, ct.tolerance, ct.relTol, ct.maxIter, ct.minIter, ct.nSweeps, n);
+    launch_init_scalars(st, C.S, ct.tolerance, ct.relTol, ct.maxIter, ct.minIter, ct.nSweeps, n,
+                        mixed ? ct.innerRelTol : 0.0);
     // initial residual and normalisation (LinearSolver.cpp:150-169)
-    sell(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, x, nullptr, A->wA.p);
+    sell<double>(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, x, nullptr, A->wA.p);
     const int nparts = ex ? RED_BLOCKS_EXACT : RED_BLOCKS;
-    vec(C, V_SUM_X, n, ex, x);
+    vec<double>(C, V_SUM_X, n, ex, x);
     fin(C, FIN_XREF, 1, ex, nparts);
-    sell(C, OP_AMUL, XL_UNIFORM, V, A->coef.p, A->diag.p, nullptr, nullptr, A->pA.p, 1.0, nullptr, nullptr, nullptr,
-         nullptr, &C.S->xRef);
-    vec(C, V_INIT, n, ex, A->wA.p, b, A->pA.p, nullptr, A->vr.p);
+    sell<double>(C, OP_AMUL, XL_UNIFORM, V, A->coef.p, A->diag.p, nullptr, nullptr, A->pA.p, 1.0, nullptr, nullptr,
+                 nullptr, nullptr, &C.S->xRef);
+    vec<double>(C, V_INIT, n, ex, A->wA.p, b, A->pA.p, nullptr, A->vr.p);
     fin(C, FIN_INIT, 2, ex, nparts);
     DevScalars hs;
     readScalars(c, hs);
     if (!hs.done) {
         if (P.precond == "GAMG" && !A->gamgReady) buildGamgCoeffs(A);
-        const int sellParts = sell_grid_blocks(V.nslices);
-        auto iterate = [&]() {
-            if (isPCG) {  // LinearSolver.cpp:176-201
-                A->vw.alloc(n);
-                A->vp.alloc(n);
-                if (P.precond == "GAMG") {
-                    vcycle(C, A, P, 0, A->vr.p, A->vw.p);
-                    vec(C, V_DOT, n, ex, A->vw.p, A->vr.p);
-                } else {
-                    vec(C, V_PRECOND_DOT, n, ex, A->vr.p, nullptr, nullptr, nullptr, A->vw.p, nullptr, nullptr,
-                        P.precond == "diagonal" ? A->rD.p : nullptr);
-                }
-                fin(C, FIN_PCG_WRA, 1, ex, nparts);
-                vec(C, V_PCG_P, n, ex, A->vw.p, nullptr, nullptr, nullptr, nullptr, A->vp.p);
-                amulTimed(C, A, A->vp.p, A->wA.p, A->vp.p, fuse);
-                if (fuse) {
-                    fin(C, FIN_PCG_WAPA, 1, false, sellParts);
-                } else {
-                    vec(C, V_DOT, n, ex, A->wA.p, A->vp.p);
-                    fin(C, FIN_PCG_WAPA, 1, ex, nparts);
-                }
-                vec(C, V_PCG_XR, n, ex, A->vp.p, A->wA.p, nullptr, nullptr, nullptr, x, A->vr.p);
-                fin(C, FIN_PCG_NORM, 1, ex, nparts);
-            } else if (isBi) {  // LinearSolver.cpp:202-237
-                vec(C, V_DOT, n, ex, A->r0.p, A->vr.p);
-                fin(C, FIN_BI_RHO, 1, ex, nparts);
-                vec(C, V_BI_P, n, ex, A->vr.p, A->Ay.p, nullptr, nullptr, nullptr, A->vp.p);
-                precondApply(C, A, P, A->vp.p, A->y.p, n);
-                amulTimed(C, A, A->y.p, A->Ay.p, A->r0.p, fuse);
-                if (fuse) {
-                    fin(C, FIN_BI_ALPHA, 1, false, sellParts);
-                } else {
-                    vec(C, V_DOT, n, ex, A->r0.p, A->Ay.p);
-                    fin(C, FIN_BI_ALPHA, 1, ex, nparts);
-                }
-                vec(C, V_BI_S, n, ex, A->vr.p, A->Ay.p, nullptr, nullptr, A->s.p);
-                fin(C, FIN_BI_S, 1, ex, nparts);
-                vec(C, V_BI_XEXIT, n, ex, A->y.p, nullptr, nullptr, nullptr, nullptr, x);
-                launch_clear_exit(st, C.S);
-                precondApply(C, A, P, A->s.p, A->z.p, n);
-                sell(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, A->z.p, nullptr, A->t.p);
-                vec(C, V_BI_DOT2, n, ex, A->t.p, A->s.p);
-                fin(C, FIN_BI_OMEGA, 2, ex, nparts);
-                vec(C, V_BI_XR, n, ex, A->y.p, A->z.p, A->s.p, A->t.p, nullptr, x, A->vr.p);
-                fin(C, FIN_BI_NORM, 1, ex, nparts);
-            } else {  // Jacobi solver, LinearSolver.cpp:238-278
-                for (int sw = 0; sw < ct.nSweeps; ++sw) {
-                    sell(C, OP_JACREF, XL_PLAIN, V, A->coef.p, A->diag.p, x, b, A->vw.p);
-                    vec(C, V_COPY, n, ex, A->vw.p, nullptr, nullptr, nullptr, x);
-                }
-                sell(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, x, nullptr, A->wA.p);
-                vec(C, V_BMINUSAX_NORM, n, ex, A->wA.p, b, nullptr, nullptr, A->vr.p);
-                fin(C, FIN_JAC_NORM, 1, ex, nparts);
-            }
-        };
-        if (isBi) {
-            A->r0.alloc(n);
-            A->vp.alloc(n);
-            A->y.alloc(n);
-            A->Ay.alloc(n);
-            A->s.alloc(n);
-            A->z.alloc(n);
-            A->t.alloc(n);
-            HIPCHECK(hipMemcpyAsync(A->r0.p, A->vr.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-            HIPCHECK(hipMemsetAsync(A->vp.p, 0, sizeof(double) * n, st));
-            HIPCHECK(hipMemsetAsync(A->Ay.p, 0, sizeof(double) * n, st));
-        }
-        if (isPCG && P.precond != "GAMG") {
-            A->vp.alloc(n);
-            HIPCHECK(hipMemsetAsync(A->vp.p, 0, sizeof(double) * n, st));
-        }
-        if (isJac) A->vw.alloc(n);
-        // Enqueue one iteration ahead: iteration k+1 runs as no-ops if k finished.
-        hipEvent_t evs[2];
-        HIPCHECK(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
-        HIPCHECK(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
-        DevScalars* hsl = c->hostScal;
         const int maxLoops = std::max(ct.maxIter, 1) + 2;
-        int k = 0;
-        iterate();
-        HIPCHECK(hipMemcpyAsync(&hsl[0], C.S, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
-        HIPCHECK(hipEventRecord(evs[0], st));
-        for (k = 1; k <= maxLoops; ++k) {
-            iterate();
-            HIPCHECK(hipMemcpyAsync(&hsl[k & 1], C.S, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
-            HIPCHECK(hipEventRecord(evs[k & 1], st));
-            HIPCHECK(hipEventSynchronize(evs[(k - 1) & 1]));
-            if (hsl[(k - 1) & 1].done) break;
+        if (mixed) {
+            if (!A->mixedReady || (P.precond == "GAMG" && !A->mixedGamg)) buildMixed(A, P.precond == "GAMG");
+            for (DevBuf<float>* v : {&A->ir, &A->iw, &A->ip, &A->iq, &A->ie}) v->alloc(n);
+            if (isBi)
+                for (DevBuf<float>* v : {&A->ir0, &A->iy, &A->iAy, &A->is, &A->iz, &A->it}) v->alloc(n);
+            Krylov<float> k{A->ie.p, A->ir.p, A->iw.p, A->ip.p, A->iq.p, A->ir0.p, A->iy.p, A->iAy.p, A->is.p, A->iz.p,
+                            A->it.p};
+            DevScalars* SI = c->scal.p + 1;
+            Ctx Ci{c, st, &SI->done, SI};
+            // inner right-hand side: the FP64 residual rounded; correction e = 0
+            fin(C, FIN_INNER_START, 0, false, 0);
+            launch_mix(st, MIX_TO32, n, A->vr.p, A->ir.p, isBi ? A->ir0.p : nullptr, A->ie.p, nullptr, nullptr,
+                       C.done);
+            for (int outer = 0; outer < maxLoops; ++outer) {
+                runLoop(c, Ci, maxLoops, [&] { krylovIteration<float>(Ci, A, P, k, isPCG, false); });
+                // x += e; r = b − A·x in FP64; convergence; next inner solve
+                launch_mix(st, MIX_ADD, n, nullptr, nullptr, nullptr, nullptr, x, A->ie.p, C.done);
+                sell<double>(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, x, nullptr, A->wA.p);
+                vec<double>(C, V_BMINUSAX_NORM, n, false, A->wA.p, b, nullptr, nullptr, A->vr.p);
+                fin(C, FIN_OUTER, 1, false, RED_BLOCKS);
+                fin(C, FIN_INNER_START, 0, false, 0);
+                launch_mix(st, MIX_TO32, n, A->vr.p, A->ir.p, isBi ? A->ir0.p : nullptr, A->ie.p, nullptr, nullptr,
+                           C.done);
+                readScalars(c, hs);
+                if (hs.done) break;
+            }
+        } else if (isPCG || isBi) {
+            A->vw.alloc(n);
+            A->vp.alloc(n);
+            Krylov<double> k{x, A->vr.p, A->vw.p, A->vp.p, A->wA.p, nullptr, nullptr, nullptr, nullptr, nullptr,
+                             nullptr};
+            if (isBi) {
+                for (DevBuf<double>* v : {&A->r0, &A->y, &A->Ay, &A->s, &A->z, &A->t}) v->alloc(n);
+                k.r0 = A->r0.p;
+                k.y = A->y.p;
+                k.Ay = A->Ay.p;
+                k.s = A->s.p;
+                k.z = A->z.p;
+                k.t = A->t.p;
+                HIPCHECK(hipMemcpyAsync(A->r0.p, A->vr.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+                HIPCHECK(hipMemsetAsync(A->vp.p, 0, sizeof(double) * n, st));
+                HIPCHECK(hipMemsetAsync(A->Ay.p, 0, sizeof(double) * n, st));
+            } else if (P.precond != "GAMG") {
+                HIPCHECK(hipMemsetAsync(A->vp.p, 0, sizeof(double) * n, st));
+            }
+            runLoop(c, C, maxLoops, [&] { krylovIteration<double>(C, A, P, k, isPCG, ex); });
+        } else {  // Jacobi solver, LinearSolver.cpp:238-278
+            A->vw.alloc(n);
+            runLoop(c, C, maxLoops, [&] {
+                for (int sw = 0; sw < ct.nSweeps; ++sw) {
+                    sell<double>(C, OP_JACREF, XL_PLAIN, V, A->coef.p, A->diag.p, x, b, A->vw.p);
+                    vec<double>(C, V_COPY, n, ex, A->vw.p, nullptr, nullptr, nullptr, x);
+                }
+                sell<double>(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, x, nullptr, A->wA.p);
+                vec<double>(C, V_BMINUSAX_NORM, n, ex, A->wA.p, b, nullptr, nullptr, A->vr.p);
+                fin(C, FIN_JAC_NORM, 1, ex, nparts);
+            });
         }
-        HIPCHECK(hipStreamSynchronize(st));
-        (void)hipEventDestroy(evs[0]);
-        (void)hipEventDestroy(evs[1]);
         readScalars(c, hs);
         if (!hs.done) throw VolError(VOL_ERR_HIP, "solver loop did not terminate");
     }
@@ -1001,6 +1290,7 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
     perf.normFactor = hs.normFactor;
     perf.iterations = hs.iterations;
     perf.converged = hostConverged(ct, hs) ? 1 : 0;
+    perf.outerIterations = hs.outer;
 }
 
 }  // namespace
@@ -1043,6 +1333,7 @@ void vol_controls_default(vol_solver_controls* c) {
     c->gamgPreSweeps = 2;
     c->gamgPostSweeps = 2;
     c->gamgOmega = 0.8;
+    c->innerRelTol = 1e-4;
 }
 
 const char* vol_last_error(void) { return gLastError.c_str(); }
@@ -1059,7 +1350,7 @@ int vol_context_create(int device, vol_context* out) {
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     if (const char* e = std::getenv("CFD_REPRODUCIBLE")) c->reproducible = std::string(e) != "0" && std::string(e) != "";
     if (const char* e = std::getenv("VOLUMA_TAIL_MAX_CELLS")) c->tailMax = std::atoi(e);
-    c->scal.alloc(1);
+    c->scal.alloc(2);
     HIPCHECK(hipHostMalloc((void**)&c->hostScal, 2 * sizeof(DevScalars), hipHostMallocDefault));
     c->ensureScratch(RED_BLOCKS);
     *out = c.release();
@@ -1250,6 +1541,135 @@ int vol_assemble_div_upwind(vol_matrix A, const double* flux, const int32_t* bcF
     API_END
 }
 
+int vol_fvm_reset(vol_matrix A) {
+    API_BEGIN
+    if (!A) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null matrix");
+    setDevice(A->mesh->ctx);
+    fvmReset(A);
+    HIPCHECK(hipStreamSynchronize(A->mesh->ctx->stream));
+    API_END
+}
+
+int vol_fvm_ddt_euler(vol_matrix A, int sign, double dt, const double* V, const double* psi0, int location) {
+    API_BEGIN
+    if (!A) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null matrix");
+    fvmCheck(A, sign);
+    if (!V || !psi0 || !(dt > 0.0)) throw VolError(VOL_ERR_INVALID_ARGUMENT, "ddt needs V, psi0 and dt > 0");
+    setDevice(A->mesh->ctx);
+    const size_t n = A->mesh->n;
+    FvmArgs a = fvmArgs(A, sign);
+    a.rDt = 1.0 / dt;  // Schemes.h:74
+    a.V = stageK(A, 0, V, n, location);
+    a.psi0 = stageK(A, 1, psi0, n, location);
+    fvmTerm(A, FVM_DDT, a, false);
+    API_END
+}
+
+int vol_fvm_div_upwind(vol_matrix A, int sign, const double* flux, const int32_t* bcFixed, const double* bcValue,
+                       int location) {
+    API_BEGIN
+    if (!A) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null matrix");
+    fvmCheck(A, sign);
+    vol_mesh m = A->mesh;
+    if (!flux || (!bcFixed && m->nb > 0)) throw VolError(VOL_ERR_INVALID_ARGUMENT, "div needs flux and bcFixed");
+    setDevice(m->ctx);
+    const size_t nF = size_t(m->nf) + m->nb;
+    FvmArgs a = fvmArgs(A, sign);
+    a.flux = stageK(A, 0, flux, nF, location);
+    a.bcFixed = stage(A->sti, bcFixed, size_t(m->nb), location, m->ctx->stream);
+    a.bcValue = stageK(A, 1, bcValue, size_t(m->nb), location);
+    fvmTerm(A, FVM_DIV, a, true);
+    API_END
+}
+
+int vol_fvm_laplacian(vol_matrix A, int sign, double gamma, const double* faceGamma, const double* magSf,
+                      const double* ndc, const int32_t* bcFixed, const double* bcValue, const vol_nonorth* nonOrth,
+                      int location) {
+    API_BEGIN
+    if (!A) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null matrix");
+    fvmCheck(A, sign);
+    vol_mesh m = A->mesh;
+    if (!magSf || !ndc || (!bcFixed && m->nb > 0))
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "laplacian needs magSf, ndc and bcFixed");
+    if (nonOrth && (!nonOrth->psi || !nonOrth->w || !nonOrth->Sf || !nonOrth->corrVecs || !nonOrth->V))
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "non-orthogonal correction needs psi, w, Sf, corrVecs and V");
+    setDevice(m->ctx);
+    const size_t n = m->n, nF = size_t(m->nf) + m->nb;
+    FvmArgs a = fvmArgs(A, sign);
+    a.gammaU = gamma;
+    a.gamma = stageK(A, 0, faceGamma, nF, location);
+    a.magSf = stageK(A, 1, magSf, nF, location);
+    a.ndc = stageK(A, 2, ndc, nF, location);
+    a.bcFixed = stage(A->sti, bcFixed, size_t(m->nb), location, m->ctx->stream);
+    a.bcValue = stageK(A, 3, bcValue, size_t(m->nb), location);
+    if (nonOrth) {
+        a.psi = stageK(A, 4, nonOrth->psi, n, location);
+        a.w = stageK(A, 5, nonOrth->w, nF, location);
+        a.Sf = stageK(A, 6, nonOrth->Sf, 3 * nF, location);
+        a.corr = stageK(A, 7, nonOrth->corrVecs, 3 * nF, location);
+        a.V = stageK(A, 8, nonOrth->V, n, location);
+    }
+    fvmTerm(A, FVM_LAP, a, false);
+    API_END
+}
+
+int vol_fvm_source(vol_matrix A, const double* su, const double* V, int location) {
+    API_BEGIN
+    if (!A || !su || !V) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    fvmCheck(A, 1);
+    vol_mesh m = A->mesh;
+    setDevice(m->ctx);
+    const size_t n = m->n;
+    launch_fvm_source(m->ctx->stream, m->n, stageK(A, 0, su, n, location), stageK(A, 1, V, n, location), A->fsrc.p);
+    HIPCHECK(hipStreamSynchronize(m->ctx->stream));
+    API_END
+}
+
+int vol_fvm_get(vol_matrix A, double* diag, double* source, double* ic, double* bs, double* ffc, int location) {
+    API_BEGIN
+    if (!A) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null matrix");
+    fvmCheck(A, 1);
+    vol_mesh m = A->mesh;
+    setDevice(m->ctx);
+    hipStream_t st = m->ctx->stream;
+    const hipMemcpyKind kind = location == VOL_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost;
+    auto cp = [&](double* dst, const DevBuf<double>& b, size_t cnt) {
+        if (dst && cnt) HIPCHECK(hipMemcpyAsync(dst, b.p, sizeof(double) * cnt, kind, st));
+    };
+    const size_t n = m->n, nf = m->nf, nb = m->nb;
+    cp(diag, A->fdiag, n);
+    cp(source, A->fsrc, n);
+    cp(ic, A->fic, nb);
+    cp(bs, A->fbs, nb);
+    cp(ffc, A->fffc, nf + nb);
+    HIPCHECK(hipStreamSynchronize(st));
+    API_END
+}
+
+int vol_fvm_finish(vol_matrix A, double* b, int location) {
+    API_BEGIN
+    if (!A) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null matrix");
+    fvmCheck(A, 1);
+    vol_mesh m = A->mesh;
+    setDevice(m->ctx);
+    hipStream_t st = m->ctx->stream;
+    const int n = m->n;
+    A->diag.alloc(n);
+    FvmArgs a = fvmArgs(A, 1);
+    DevBuf<double> tb;
+    double* bd = b;
+    if (b && location != VOL_DEVICE) {
+        tb.alloc(n);
+        bd = tb.p;
+    }
+    launch_fvm_fold(st, a, A->diag.p, bd);
+    A->symmetric = !A->fvmHasLower;
+    matrixFinish(A);
+    if (b && location != VOL_DEVICE) HIPCHECK(hipMemcpyAsync(b, tb.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    API_END
+}
+
 int vol_matrix_create(vol_mesh mesh, vol_matrix* out) {
     API_BEGIN
     if (!mesh || !out) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
@@ -1328,7 +1748,7 @@ int vol_amul(vol_matrix A, const double* x, double* y, int location) {
         dy = A->hy.p;
     }
     Ctx C{c, st, nullptr, c->scal.p};
-    sell(C, OP_AMUL, XL_PLAIN, m->sell.view(), A->coef.p, A->diag.p, dx, nullptr, dy);
+    sell<double>(C, OP_AMUL, XL_PLAIN, m->sell.view(), A->coef.p, A->diag.p, dx, nullptr, dy);
     if (location != VOL_DEVICE) HIPCHECK(hipMemcpyAsync(y, dy, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
     API_END

@@ -115,4 +115,209 @@ void launch_assemble(hipStream_t st, int kind, const AsmArgs& a) {
     }
 }
 
+// ============================================================================ FvMatrix terms
+// Laplacian coefficient of the term (γ_f |S_f| ndc_f).
+__device__ __forceinline__ double fvm_lap_coef(const FvmArgs& a, int f) {
+    const double g = a.gamma ? a.gamma[f] : a.gammaU;
+    return g * a.magSf[f] * a.ndc[f];
+}
+
+// Fvm.h:157: any internal face with |corr|² > 1e-20 switches the correction on.
+__global__ void k_any_corr(FvmArgs a) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= a.nf) return;
+    const double cx = a.corr[3 * f], cy = a.corr[3 * f + 1], cz = a.corr[3 * f + 2];
+    if (cx * cx + cy * cy + cz * cz > 1e-20) atomicOr(a.anyCorr, 1);
+}
+
+// Gauss-linear gradient (Fvc.h:216-224): per cell its internal faces in ascending
+// order (owner adds, neighbour subtracts Sf·φ_f with φ_f = w φ_P + (1−w) φ_N), then its
+// boundary faces (Sf·φ_b), then the product with 1/V.
+__global__ void k_grad_gauss(FvmArgs a) {
+    if (!*a.anyCorr) return;
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n) return;
+    const int slice = c / WAVE, lane = c % WAVE;
+    const int s0 = a.S.sliceOff[slice], width = (a.S.sliceOff[slice + 1] - s0) / WAVE;
+    double gx = 0.0, gy = 0.0, gz = 0.0;
+    for (int k = 0; k < width; ++k) {
+        const int slot = s0 + k * WAVE + lane;
+        const int e = a.S.src[slot];
+        if (e < 0) continue;
+        const int f = e >> 1, nb = a.S.col[slot];
+        const bool isOwner = (e & 1) == 0;
+        const double wf = a.w[f];
+        const double phif = isOwner ? wf * a.psi[c] + (1.0 - wf) * a.psi[nb] : wf * a.psi[nb] + (1.0 - wf) * a.psi[c];
+        const double vx = a.Sf[3 * f] * phif, vy = a.Sf[3 * f + 1] * phif, vz = a.Sf[3 * f + 2] * phif;
+        if (isOwner) {
+            gx += vx;
+            gy += vy;
+            gz += vz;
+        } else {
+            gx -= vx;
+            gy -= vy;
+            gz -= vz;
+        }
+    }
+    for (int k = a.bOff[c]; k < a.bOff[c + 1]; ++k) {
+        const int bf = a.bList[k], f = a.nf + bf;
+        const double pb = a.bcFixed[bf] ? (a.bcValue ? a.bcValue[bf] : 0.0) : a.psi[c];
+        gx += a.Sf[3 * f] * pb;
+        gy += a.Sf[3 * f + 1] * pb;
+        gz += a.Sf[3 * f + 2] * pb;
+    }
+    const double r = 1.0 / a.V[c];
+    a.grad[3 * c] = gx * r;
+    a.grad[3 * c + 1] = gy * r;
+    a.grad[3 * c + 2] = gz * r;
+}
+
+// Off-diagonal part of a term and its addition (FvMatrix.h:232-239): the lower part
+// is first copied from upper when the matrix has none yet, then lower += s·lower_term
+// (the term's upper when it has no lower part) and upper += s·upper_term. The
+// laplacian's correction flux fc (Fvm.h:162-167) is formed here and added to ffc.
+template <int KIND>
+__global__ void k_fvm_faces(FvmArgs a) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= a.nf) return;
+    double tu = 0.0, tl = 0.0;
+    constexpr bool termLower = KIND == FVM_DIV;
+    if constexpr (KIND == FVM_DIV) {
+        const double F = a.flux[f];
+        const double wUp = F >= 0 ? 1.0 : 0.0;
+        tl = -wUp * F;
+        tu = tl + F;
+    } else if constexpr (KIND == FVM_LAP) {
+        tu = fvm_lap_coef(a, f);
+    }
+    const double up = a.upper[f];
+    if (a.lower) a.lower[f] = (a.hadLower ? a.lower[f] : up) + a.sign * (termLower ? tl : tu);
+    a.upper[f] = up + a.sign * tu;
+}
+
+// Correction flux per internal face (Fvm.h:162-165): gf = w g_P + (1−w) g_N,
+// fc = (γ|S|)·(corr·gf); ffc += s·fc (FvMatrix.h:240-244).
+__global__ void k_fvm_fc(FvmArgs a, const int* own, const int* nei) {
+    if (!*a.anyCorr) return;
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= a.nf) return;
+    const int o = own[f], nb = nei[f];
+    const double wf = a.w[f], wn = 1.0 - wf;
+    const double gx = wf * a.grad[3 * o] + wn * a.grad[3 * nb];
+    const double gy = wf * a.grad[3 * o + 1] + wn * a.grad[3 * nb + 1];
+    const double gz = wf * a.grad[3 * o + 2] + wn * a.grad[3 * nb + 2];
+    const double g = a.gamma ? a.gamma[f] : a.gammaU;
+    const double fc = (g * a.magSf[f]) * (a.corr[3 * f] * gx + a.corr[3 * f + 1] * gy + a.corr[3 * f + 2] * gz);
+    a.fc[f] = fc;
+    a.ffc[f] = a.ffc[f] + a.sign * fc;
+}
+
+// Per cell: the term's diagonal and source from its faces in ascending order and its
+// boundary faces' internalCoeffs/boundarySource, then the signed addition.
+template <int KIND>
+__global__ void k_fvm_cells(FvmArgs a) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n) return;
+    double td = 0.0, ts = 0.0;
+    if constexpr (KIND == FVM_DDT) {
+        const double Vc = a.V[c];
+        td = 0.0 + a.rDt * Vc;                                     // Fvm.h:27
+        ts = 0.0 + Vc * (a.rDt * a.psi0[c] - 0.0 * a.psi0[c]);     // Fvm.h:28, c00 = 0
+    } else {
+        const int slice = c / WAVE, lane = c % WAVE;
+        const int s0 = a.S.sliceOff[slice], width = (a.S.sliceOff[slice + 1] - s0) / WAVE;
+        const bool corr = KIND == FVM_LAP && a.psi && *a.anyCorr;
+        for (int k = 0; k < width; ++k) {
+            const int e = a.S.src[s0 + k * WAVE + lane];
+            if (e < 0) continue;
+            const int f = e >> 1;
+            const bool isOwner = (e & 1) == 0;
+            if constexpr (KIND == FVM_DIV) {
+                const double F = a.flux[f];
+                const double wUp = F >= 0 ? 1.0 : 0.0;
+                const double lo = -wUp * F;
+                td -= isOwner ? lo : lo + F;                       // Fvm.h:105-106
+            } else {
+                td -= fvm_lap_coef(a, f);                          // Fvm.h:152-153
+                if (corr) {
+                    if (isOwner)
+                        ts -= a.fc[f];                             // Fvm.h:166
+                    else
+                        ts += a.fc[f];                             // Fvm.h:167
+                }
+            }
+        }
+    }
+    for (int k = a.bOff[c]; k < a.bOff[c + 1]; ++k) {
+        const int bf = a.bList[k], f = a.nf + bf;
+        double tic = 0.0, tbs = 0.0;
+        if constexpr (KIND == FVM_DIV) {                           // Fvm.h:126-135
+            const double F = a.flux[f];
+            const bool fixed = a.bcFixed[bf] != 0;
+            tic = 0.0 + F * (fixed ? 0.0 : 1.0);
+            tbs = 0.0 - F * (fixed ? (a.bcValue ? a.bcValue[bf] : 0.0) : 0.0);
+        } else if constexpr (KIND == FVM_LAP) {                    // Fvm.h:171-180
+            const double g = a.gamma ? a.gamma[f] : a.gammaU;
+            const double gs = g * a.magSf[f];
+            double gic = 0.0, gbc = 0.0;
+            if (a.bcFixed[bf]) {
+                gic = -a.ndc[f];
+                gbc = a.ndc[f] * (a.bcValue ? a.bcValue[bf] : 0.0);
+            }
+            tic = 0.0 + gs * gic;
+            tbs = 0.0 - gs * gbc;
+        }
+        a.ic[bf] = a.ic[bf] + a.sign * tic;
+        a.bs[bf] = a.bs[bf] + a.sign * tbs;
+    }
+    a.diag[c] = a.diag[c] + a.sign * td;
+    a.source[c] = a.source[c] + a.sign * ts;
+}
+
+__global__ void k_fvm_source(int n, const double* su, const double* V, double* source) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c < n) source[c] += su[c] * V[c];
+}
+
+__global__ void k_fvm_fold(FvmArgs a, double* dOut, double* bOut) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n) return;
+    double d = a.diag[c], b = a.source[c];
+    for (int k = a.bOff[c]; k < a.bOff[c + 1]; ++k) {
+        const int bf = a.bList[k];
+        d += a.ic[bf];
+        b += a.bs[bf];
+    }
+    dOut[c] = d;
+    if (bOut) bOut[c] = b;
+}
+
+void launch_fvm_term(hipStream_t st, int kind, const FvmArgs& a, const int* own, const int* nei) {
+    const dim3 fb((a.nf + 255) / 256), cb((a.n + 255) / 256), t(256);
+    if (kind == FVM_LAP && a.psi) {
+        (void)hipMemsetAsync(a.anyCorr, 0, sizeof(int), st);
+        if (a.nf > 0) hipLaunchKernelGGL(k_any_corr, fb, t, 0, st, a);
+        if (a.n > 0) hipLaunchKernelGGL(k_grad_gauss, cb, t, 0, st, a);
+        if (a.nf > 0) hipLaunchKernelGGL(k_fvm_fc, fb, t, 0, st, a, own, nei);
+    }
+    if (a.nf > 0) {
+        if (kind == FVM_DDT) hipLaunchKernelGGL(k_fvm_faces<FVM_DDT>, fb, t, 0, st, a);
+        if (kind == FVM_DIV) hipLaunchKernelGGL(k_fvm_faces<FVM_DIV>, fb, t, 0, st, a);
+        if (kind == FVM_LAP) hipLaunchKernelGGL(k_fvm_faces<FVM_LAP>, fb, t, 0, st, a);
+    }
+    if (a.n > 0) {
+        if (kind == FVM_DDT) hipLaunchKernelGGL(k_fvm_cells<FVM_DDT>, cb, t, 0, st, a);
+        if (kind == FVM_DIV) hipLaunchKernelGGL(k_fvm_cells<FVM_DIV>, cb, t, 0, st, a);
+        if (kind == FVM_LAP) hipLaunchKernelGGL(k_fvm_cells<FVM_LAP>, cb, t, 0, st, a);
+    }
+}
+
+void launch_fvm_source(hipStream_t st, int n, const double* su, const double* V, double* source) {
+    if (n > 0) hipLaunchKernelGGL(k_fvm_source, dim3((n + 255) / 256), dim3(256), 0, st, n, su, V, source);
+}
+
+void launch_fvm_fold(hipStream_t st, const FvmArgs& a, double* dOut, double* bOut) {
+    if (a.n > 0) hipLaunchKernelGGL(k_fvm_fold, dim3((a.n + 255) / 256), dim3(256), 0, st, a, dOut, bOut);
+}
+
 }  // namespace vol

@@ -3,6 +3,8 @@
 // All kernels are memory-bound; none uses MFMA. Arithmetic is compiled with
 // -ffp-contract=off so that every a·b + c is a rounded product followed by a
 // rounded add, as in the reference (CMakeLists.txt:11).
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "vol_internal.hpp"
 
@@ -266,69 +268,73 @@ int vec_has_reduction(int op) {
     }
 }
 
-// Element step of each vector op; returns the values to reduce in r0/r1.
-template <int OP>
-__device__ __forceinline__ void vec_elem(const VecArgs& A, int i, double& r0, double& r1) {
+template <class T> __device__ __forceinline__ double ld(const T* p, int i) { return double(p[i]); }
+
+// Element step of each vector op; returns the values to reduce in r0/r1. Values are
+// widened to double, combined in double and rounded once when stored; for T = double
+// this is exactly the reference's expression order.
+template <int OP, class T>
+__device__ __forceinline__ void vec_elem(const VecArgsT<T>& A, int i, double& r0, double& r1) {
     const DevScalars* S = A.S;
     if constexpr (OP == V_SUM_X) {
-        r0 = A.a[i];
+        r0 = ld(A.a, i);
     } else if constexpr (OP == V_INIT) {  // a = wA, b = b, c = pA; w = rA
-        const double rA = A.b[i] - A.a[i];
-        A.w[i] = rA;
-        r0 = fabs(A.a[i] - A.c[i]) + fabs(A.b[i] - A.c[i]);
+        const double rA = ld(A.b, i) - ld(A.a, i);
+        A.w[i] = T(rA);
+        r0 = fabs(ld(A.a, i) - ld(A.c, i)) + fabs(ld(A.b, i) - ld(A.c, i));
         r1 = fabs(rA);
     } else if constexpr (OP == V_PRECOND_DOT) {  // a = r → w
-        const double w = A.rD ? A.rD[i] * A.a[i] : A.a[i];
+        const T w = T(A.rD ? ld(A.rD, i) * ld(A.a, i) : ld(A.a, i));
         A.w[i] = w;
-        r0 = w * A.a[i];
+        r0 = double(w) * ld(A.a, i);
     } else if constexpr (OP == V_DOT) {
-        r0 = A.a[i] * A.b[i];
+        r0 = ld(A.a, i) * ld(A.b, i);
     } else if constexpr (OP == V_PCG_P) {  // a = w; x = p
-        A.x[i] = S->iterations == 0 ? A.a[i] : A.a[i] + S->beta * A.x[i];
+        A.x[i] = T(S->iterations == 0 ? ld(A.a, i) : ld(A.a, i) + S->beta * ld(A.x, i));
     } else if constexpr (OP == V_PCG_XR) {  // a = p, b = q; x = x, y = r
-        A.x[i] += S->alpha * A.a[i];
-        const double r = A.y[i] - S->alpha * A.b[i];
+        A.x[i] = T(ld(A.x, i) + S->alpha * ld(A.a, i));
+        const T r = T(ld(A.y, i) - S->alpha * ld(A.b, i));
         A.y[i] = r;
-        r0 = fabs(r);
+        r0 = fabs(double(r));
     } else if constexpr (OP == V_BI_P) {  // a = r, b = Ay; x = p
-        A.x[i] = S->iterations == 0 ? A.a[i] : A.a[i] + S->beta * (A.x[i] - S->omegaB * A.b[i]);
+        A.x[i] = T(S->iterations == 0 ? ld(A.a, i) : ld(A.a, i) + S->beta * (ld(A.x, i) - S->omegaB * ld(A.b, i)));
     } else if constexpr (OP == V_SCALE) {
-        A.w[i] = A.rD ? A.rD[i] * A.a[i] : A.a[i];
+        A.w[i] = T(A.rD ? ld(A.rD, i) * ld(A.a, i) : ld(A.a, i));
     } else if constexpr (OP == V_BI_S) {  // a = r, b = Ay; w = s
-        const double s = A.a[i] - S->alphaB * A.b[i];
+        const T s = T(ld(A.a, i) - S->alphaB * ld(A.b, i));
         A.w[i] = s;
-        r0 = fabs(s);
+        r0 = fabs(double(s));
     } else if constexpr (OP == V_BI_XEXIT) {  // a = y; x = x
-        A.x[i] += S->alphaB * A.a[i];
+        A.x[i] = T(ld(A.x, i) + S->alphaB * ld(A.a, i));
     } else if constexpr (OP == V_BI_DOT2) {  // a = t, b = s
-        r0 = A.a[i] * A.a[i];
-        r1 = A.a[i] * A.b[i];
+        r0 = ld(A.a, i) * ld(A.a, i);
+        r1 = ld(A.a, i) * ld(A.b, i);
     } else if constexpr (OP == V_BI_XR) {  // a = y, b = z, c = s, d = t; x = x, y = r
-        A.x[i] += S->alphaB * A.a[i] + S->omegaB * A.b[i];
-        const double r = A.c[i] - S->omegaB * A.d[i];
+        A.x[i] = T(ld(A.x, i) + (S->alphaB * ld(A.a, i) + S->omegaB * ld(A.b, i)));
+        const T r = T(ld(A.c, i) - S->omegaB * ld(A.d, i));
         A.y[i] = r;
-        r0 = fabs(r);
+        r0 = fabs(double(r));
     } else if constexpr (OP == V_BMINUSAX_NORM) {  // a = wA, b = b; w = r
-        const double r = A.b[i] - A.a[i];
-        A.w[i] = r;
+        const double r = ld(A.b, i) - ld(A.a, i);
+        A.w[i] = T(r);
         r0 = fabs(r);
     } else if constexpr (OP == V_COPY) {
         A.w[i] = A.a[i];
     } else if constexpr (OP == V_RECIP) {
-        A.w[i] = 1.0 / A.a[i];
+        A.w[i] = T(1.0 / ld(A.a, i));
     } else if constexpr (OP == V_FROMZERO) {  // a = r, b = d
-        A.w[i] = A.omega * (A.a[i] / A.b[i]);
+        A.w[i] = T(A.omega * (ld(A.a, i) / ld(A.b, i)));
     }
 }
 
-template <int OP>
-__device__ __forceinline__ bool vec_should_run(const VecArgs& A) {
+template <int OP, class T>
+__device__ __forceinline__ bool vec_should_run(const VecArgsT<T>& A) {
     if constexpr (OP == V_BI_XEXIT) return A.S->exitPending != 0;
     return !(A.done && *A.done);
 }
 
-template <int OP, int NRED>
-__global__ __launch_bounds__(RED_THREADS) void k_vec_fast(VecArgs A, double* part, int maxBlk) {
+template <int OP, int NRED, class T>
+__global__ __launch_bounds__(RED_THREADS) void k_vec_fast(VecArgsT<T> A, double* part, int maxBlk) {
     if (!vec_should_run<OP>(A)) return;
     double s0 = 0.0, s1 = 0.0;
     const int stride = gridDim.x * blockDim.x;
@@ -376,14 +382,14 @@ __global__ __launch_bounds__(RED_THREADS) void k_vec_exact(VecArgs A, long long*
     if (threadIdx.x < NRED) xnf[threadIdx.x * maxBlk + blockIdx.x] = nf[threadIdx.x];
 }
 
-template <int OP>
-static void vec_dispatch(hipStream_t st, const VecArgs& a, bool exact, const RedScratch& R) {
+template <int OP, class T>
+static void vec_dispatch(hipStream_t st, const VecArgsT<T>& a, bool exact, const RedScratch& R) {
     constexpr int NRED = (OP == V_INIT || OP == V_BI_DOT2) ? 2
                          : (OP == V_SUM_X || OP == V_PRECOND_DOT || OP == V_DOT || OP == V_PCG_XR || OP == V_BI_S ||
                             OP == V_BMINUSAX_NORM || OP == V_BI_XR)
                              ? 1
                              : 0;
-    if constexpr (NRED > 0) {
+    if constexpr (NRED > 0 && std::is_same_v<T, double>) {
         if (exact) {
             hipLaunchKernelGGL((k_vec_exact<OP, NRED>), dim3(RED_BLOCKS_EXACT), dim3(RED_THREADS), 0, st, a, R.xpart,
                                R.xnf, R.maxBlkExact);
@@ -397,11 +403,12 @@ static void vec_dispatch(hipStream_t st, const VecArgs& a, bool exact, const Red
             if (nb > 4 * RED_BLOCKS) nb = 4 * RED_BLOCKS;
             if (nb < 1) nb = 1;
         }
-        hipLaunchKernelGGL((k_vec_fast<OP, NRED>), dim3(nb), dim3(RED_THREADS), 0, st, a, R.part, R.maxBlk);
+        hipLaunchKernelGGL((k_vec_fast<OP, NRED, T>), dim3(nb), dim3(RED_THREADS), 0, st, a, R.part, R.maxBlk);
     }
 }
 
-void launch_vec(hipStream_t st, int op, const VecArgs& a, bool exact, const RedScratch& R) {
+template <class T>
+static void launch_vec_t(hipStream_t st, int op, const VecArgsT<T>& a, bool exact, const RedScratch& R) {
     switch (op) {
     case V_SUM_X: vec_dispatch<V_SUM_X>(st, a, exact, R); break;
     case V_INIT: vec_dispatch<V_INIT>(st, a, exact, R); break;
@@ -422,6 +429,37 @@ void launch_vec(hipStream_t st, int op, const VecArgs& a, bool exact, const RedS
     }
 }
 
+void launch_vec(hipStream_t st, int op, const VecArgs& a, bool exact, const RedScratch& R) {
+    launch_vec_t(st, op, a, exact, R);
+}
+void launch_vec32(hipStream_t st, int op, const VecArgs32& a, const RedScratch& R) {
+    launch_vec_t(st, op, a, false, R);
+}
+
+__global__ void k_mix(int op, int n, const double* a, float* w, float* w2, float* z, double* x, const float* e,
+                      const int* done) {
+    if (done && *done) return;
+    const int stride = gridDim.x * blockDim.x;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+        if (op == MIX_TO32) {
+            const float v = float(a[i]);
+            w[i] = v;
+            if (w2) w2[i] = v;
+            if (z) z[i] = 0.0f;
+        } else {
+            x[i] += double(e[i]);
+        }
+    }
+}
+
+void launch_mix(hipStream_t st, int op, int n, const double* a, float* w, float* w2, float* z, double* x,
+                const float* e, const int* done) {
+    if (n <= 0) return;
+    int nb = (n + RED_THREADS - 1) / RED_THREADS;
+    if (nb > 4 * RED_BLOCKS) nb = 4 * RED_BLOCKS;
+    hipLaunchKernelGGL(k_mix, dim3(nb), dim3(RED_THREADS), 0, st, op, n, a, w, w2, z, x, e, done);
+}
+
 // ============================================================================ finalize
 // One block: reduce the per-block partials in a fixed order (fast) or sum the exact
 // bins (exact), then run the solver's scalar step on thread 0.
@@ -435,6 +473,24 @@ __global__ __launch_bounds__(256) void k_finalize(int op, int nred, int exact, i
                                                   int maxBlk, const long long* xpart, const int* xnf, int maxBlkExact,
                                                   DevScalars* Sp, double* result) {
     DevScalars& S = *Sp;
+    if (op == FIN_INNER_START) {  // no reduction: S[1] ← fresh inner solve, or done with S[0]
+        if (threadIdx.x == 0) {
+            DevScalars& I = Sp[1];
+            const double target = fmax(S.tolerance, S.relTol > 0 ? S.relTol * S.initialResidual : 0.0);
+            DevScalars z = {};
+            z.normFactor = S.normFactor;
+            z.tolerance = fmax(S.innerRel * S.finalResidual, 0.5 * target);
+            z.relTol = 0.0;
+            z.maxIter = S.maxIter - S.iterations;
+            z.minIter = 0;
+            z.nSweeps = 1;
+            z.n = S.n;
+            z.wArA = GREAT;
+            z.done = S.done;
+            I = z;
+        }
+        return;
+    }
     if (op != FIN_STORE && S.done) return;
     __shared__ double vals[MAX_RED];
     __shared__ double sh[256 / WAVE];
@@ -532,6 +588,15 @@ __global__ __launch_bounds__(256) void k_finalize(int op, int nred, int exact, i
         S.finalResidual = v0 / S.normFactor;
         if (!(S.iterations < S.maxIter && !dev_converged(S))) S.done = 1;
         break;
+    case FIN_OUTER: {  // Sp[1] holds the inner solve that just ended
+        const DevScalars& I = Sp[1];
+        S.iterations += I.iterations;
+        ++S.outer;
+        S.finalResidual = v0 / S.normFactor;
+        // stop on convergence, the iteration budget, or an inner solve that made no progress
+        if (!(S.iterations < S.maxIter && !dev_converged(S)) || I.iterations == 0) S.done = 1;
+        break;
+    }
     case FIN_STORE:
         if (result) {
             result[0] = v0;
@@ -546,8 +611,10 @@ void launch_finalize(hipStream_t st, int op, int nred, bool exact, int nparts, c
                        R.xpart, R.xnf, R.maxBlkExact, S, R.result);
 }
 
-__global__ void k_init_scalars(DevScalars* S, double tol, double relTol, int maxIter, int minIter, int nSweeps, int n) {
+__global__ void k_init_scalars(DevScalars* S, double tol, double relTol, int maxIter, int minIter, int nSweeps, int n,
+                               double innerRel) {
     DevScalars z = {};
+    z.innerRel = innerRel;
     z.tolerance = tol;
     z.relTol = relTol;
     z.maxIter = maxIter;
@@ -563,18 +630,20 @@ __global__ void k_clear_exit(DevScalars* S) { S->exitPending = 0; }
 void launch_clear_exit(hipStream_t st, DevScalars* S) { hipLaunchKernelGGL(k_clear_exit, dim3(1), dim3(1), 0, st, S); }
 
 void launch_init_scalars(hipStream_t st, DevScalars* S, double tol, double relTol, int maxIter, int minIter,
-                         int nSweeps, int n) {
-    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, st, S, tol, relTol, maxIter, minIter, nSweeps, n);
+                         int nSweeps, int n, double innerRel) {
+    hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, st, S, tol, relTol, maxIter, minIter, nSweeps, n,
+                       innerRel);
 }
 
 // ============================================================================ GAMG
 // Restriction: rc[I] = Σ res over the fine cells of aggregate I in ascending order,
 // which is the order of the reference's `rc[agg[i]] += res[i]` loop (GAMG.cpp:434-435).
-__global__ void k_restrict(int nc, const int* mOff, const int* mList, const double* res, double* rc, const int* done) {
+template <class T>
+__global__ void k_restrict(int nc, const int* mOff, const int* mList, const T* res, T* rc, const int* done) {
     if (done && *done) return;
     const int I = blockIdx.x * blockDim.x + threadIdx.x;
     if (I >= nc) return;
-    double s = 0.0;
+    T s = T(0);
     for (int k = mOff[I]; k < mOff[I + 1]; ++k) s += res[mList[k]];
     rc[I] = s;
 }
@@ -582,7 +651,12 @@ __global__ void k_restrict(int nc, const int* mOff, const int* mList, const doub
 void launch_restrict(hipStream_t st, int nc, const int* mOff, const int* mList, const double* res, double* rc,
                      const int* done) {
     if (nc == 0) return;
-    hipLaunchKernelGGL(k_restrict, dim3((nc + 255) / 256), dim3(256), 0, st, nc, mOff, mList, res, rc, done);
+    hipLaunchKernelGGL(k_restrict<double>, dim3((nc + 255) / 256), dim3(256), 0, st, nc, mOff, mList, res, rc, done);
+}
+void launch_restrict32(hipStream_t st, int nc, const int* mOff, const int* mList, const float* res, float* rc,
+                       const int* done) {
+    if (nc == 0) return;
+    hipLaunchKernelGGL(k_restrict<float>, dim3((nc + 255) / 256), dim3(256), 0, st, nc, mOff, mList, res, rc, done);
 }
 
 // Galerkin coarsening (GAMG.cpp:279-301) as gathers over fine contributions listed in
@@ -704,10 +778,11 @@ void launch_coarse_lu(hipStream_t st, int n, double* lu, double* luT, int* statu
 // row the subtractions run in ascending k, as in GAMG.cpp:413-414), then backward
 // substitution column by column from the last one (descending k per row, then the
 // division by the pivot).
-__global__ void k_coarse_solve(int n, const double* lu, const double* r, double* x, const int* done) {
+template <class T>
+__global__ void k_coarse_solve(int n, const double* lu, const T* r, T* x, const int* done) {
     if (done && *done) return;
     extern __shared__ double g[];
-    for (int i = threadIdx.x; i < n; i += blockDim.x) g[i] = r[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) g[i] = double(r[i]);
     __syncthreads();
     for (int k = 0; k < n; ++k) {
         const double gk = g[k];
@@ -721,20 +796,20 @@ __global__ void k_coarse_solve(int n, const double* lu, const double* r, double*
         for (int i = threadIdx.x; i < k; i += blockDim.x) g[i] -= lu[size_t(i) * n + k] * gk;
         __syncthreads();
     }
-    for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = g[i];
+    for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = T(g[i]);
 }
 
 // n ≤ 64: one wavefront, lane i holds row i of the factors in registers and g_i; the
 // column steps exchange g_k by a cross-lane broadcast. Same operations and order as
 // k_coarse_solve.
-__global__ __launch_bounds__(64) void k_coarse_solve64(int n, const double* luT, const double* r, double* x,
-                                                       const int* done) {
+template <class T>
+__global__ __launch_bounds__(64) void k_coarse_solve64(int n, const double* luT, const T* r, T* x, const int* done) {
     if (done && *done) return;
     const int i = threadIdx.x;
     double row[64];
 #pragma unroll
     for (int k = 0; k < 64; ++k) row[k] = (k < n && i < n) ? luT[size_t(k) * n + i] : 0.0;
-    double g = i < n ? r[i] : 0.0;
+    double g = i < n ? double(r[i]) : 0.0;
 #pragma unroll
     for (int k = 0; k < 64; ++k) {
         if (k < n) {
@@ -750,15 +825,24 @@ __global__ __launch_bounds__(64) void k_coarse_solve64(int n, const double* luT,
             if (i < k) g -= row[k] * gk;
         }
     }
-    if (i < n) x[i] = g;
+    if (i < n) x[i] = T(g);
 }
 
+template <class T>
+static void coarse_solve_t(hipStream_t st, int n, const double* lu, const double* luT, const T* r, T* x,
+                           const int* done) {
+    if (n <= 64)
+        hipLaunchKernelGGL(k_coarse_solve64<T>, dim3(1), dim3(64), 0, st, n, luT, r, x, done);
+    else
+        hipLaunchKernelGGL(k_coarse_solve<T>, dim3(1), dim3(256), sizeof(double) * n, st, n, lu, r, x, done);
+}
 void launch_coarse_solve(hipStream_t st, int n, const double* lu, const double* luT, const double* r, double* x,
                          const int* done) {
-    if (n <= 64)
-        hipLaunchKernelGGL(k_coarse_solve64, dim3(1), dim3(64), 0, st, n, luT, r, x, done);
-    else
-        hipLaunchKernelGGL(k_coarse_solve, dim3(1), dim3(256), sizeof(double) * n, st, n, lu, r, x, done);
+    coarse_solve_t(st, n, lu, luT, r, x, done);
+}
+void launch_coarse_solve32(hipStream_t st, int n, const double* lu, const double* luT, const float* r, float* x,
+                           const int* done) {
+    coarse_solve_t(st, n, lu, luT, r, x, done);
 }
 
 // ============================================================================ V-cycle tail

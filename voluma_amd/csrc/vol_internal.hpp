@@ -42,11 +42,16 @@ struct DevScalars {
     double tolerance, relTol;
     int maxIter, minIter, nSweeps, n;
     int iterations;
+    double innerRel;  // mixed precision: residual reduction asked of each inner solve
     int done;         // loop finished (converged, maxIter or breakdown): later kernels skip
     int exitPending;  // PBiCGStab exit on s: x += αy still to apply
     int status;
-    int pad[4];
+    int outer;        // mixed precision: defect-correction steps taken
+    int pad[2];
 };
+// Mixed precision keeps two DevScalars side by side: [0] the FP64 outer loop (the
+// reported performance), [1] the current FP32 inner Krylov solve.
+
 
 enum FinOp : int {
     FIN_XREF = 0,      // xRef = Σx / n
@@ -61,6 +66,8 @@ enum FinOp : int {
     FIN_BI_NORM = 9,
     FIN_JAC_NORM = 10,
     FIN_STORE = 11,    // only store the reduced values (tests/diagnostics)
+    FIN_OUTER = 12,    // mixed: outer residual Σ|b − Ax| after x += e; convergence of the whole solve
+    FIN_INNER_START = 13,  // mixed: set up the next inner solve in S[1] from S[0]
 };
 
 // Reduction scratch: fast mode uses `nblk` double partials per reduced quantity;
@@ -141,27 +148,42 @@ enum VecOp : int {
     V_FROMZERO = 15,      // w = ω·(r/d)
 };
 
-struct VecArgs {
+// Vector element type T (double, or float for the inner solve of mixed precision);
+// element arithmetic and all reductions run in double either way.
+template <class T> struct VecArgsT {
     int n = 0;
-    const double *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr, *e = nullptr;
-    double *w = nullptr, *x = nullptr, *y = nullptr;
-    const double* rD = nullptr;
+    const T *a = nullptr, *b = nullptr, *c = nullptr, *d = nullptr, *e = nullptr;
+    T *w = nullptr, *x = nullptr, *y = nullptr;
+    const T* rD = nullptr;
     double omega = 1.0;
     DevScalars* S = nullptr;
     const int* done = nullptr;
 };
+using VecArgs = VecArgsT<double>;
+using VecArgs32 = VecArgsT<float>;
 
 void launch_vec(hipStream_t st, int op, const VecArgs& a, bool exact, const RedScratch& R);
+void launch_vec32(hipStream_t st, int op, const VecArgs32& a, const RedScratch& R);  // fast sums only
+
+// Precision changes of mixed precision (predicated on *done when done != nullptr).
+enum MixOp : int {
+    MIX_TO32 = 0,  // w = float(a); w2 = w (nullable); z = 0 (nullable)
+    MIX_ADD = 1,   // x += double(e)
+};
+void launch_mix(hipStream_t st, int op, int n, const double* a, float* w, float* w2, float* z, double* x,
+                const float* e, const int* done);
 void launch_finalize(hipStream_t st, int op, int nred, bool exact, int nparts, const RedScratch& R, DevScalars* S);
 int vec_has_reduction(int op);
 
 void launch_clear_exit(hipStream_t st, DevScalars* S);
 void launch_init_scalars(hipStream_t st, DevScalars* S, double tol, double relTol, int maxIter, int minIter,
-                         int nSweeps, int n);
+                         int nSweeps, int n, double innerRel);
 
 // ------------------------------------------------------------------ GAMG kernels
 void launch_restrict(hipStream_t st, int nc, const int* mOff, const int* mList, const double* res, double* rc,
                      const int* done);
+void launch_restrict32(hipStream_t st, int nc, const int* mOff, const int* mList, const float* res, float* rc,
+                       const int* done);
 void launch_galerkin(hipStream_t st, int nc, int ncf, const int* mOff, const int* mList, const int* iOff,
                      const int* iList, const int* cfOff, const int* cfList, const double* Fd, const double* Fu,
                      const double* Fl, double* Cd, double* Cu, double* Cl);
@@ -173,6 +195,9 @@ void launch_coarse_assemble(hipStream_t st, int n, int nf, const int* l, const i
 void launch_coarse_lu(hipStream_t st, int n, double* lu, double* luT, int* status);
 void launch_coarse_solve(hipStream_t st, int n, const double* lu, const double* luT, const double* r, double* x,
                          const int* done);
+// Same substitution in FP64 on FP32 vectors (r widened, x rounded).
+void launch_coarse_solve32(hipStream_t st, int n, const double* lu, const double* luT, const float* r, float* x,
+                           const int* done);
 
 // Coarse levels of the V-cycle run inside one workgroup (k_vcycle_tail): the levels
 // from `first` down to the coarsest, with block barriers between the phases.
@@ -238,5 +263,52 @@ struct AsmArgs {
     double* bOut = nullptr;             // out (nullable): folded source
 };
 void launch_assemble(hipStream_t st, int kind, const AsmArgs& a);
+
+// ------------------------------------------------------------------ FvMatrix algebra
+// One term (fvm::ddt Euler, fvm::div upwind, fvm::laplacian) is built per element in
+// registers exactly as the reference builds it in a fresh FvMatrix, then added to the
+// matrix parts with sign ±1 (FvMatrix::operator+= / -=, FvMatrix.h:57-80,232-239).
+enum FvmKind : int { FVM_DDT = 0, FVM_DIV = 1, FVM_LAP = 2 };
+struct FvmArgs {
+    int n = 0, nf = 0, nb = 0;          // cells, internal faces, boundary faces
+    SellView S;                         // level-0 slots (src = face << 1 | side)
+    const int* bOff = nullptr;          // cell → boundary faces, ascending
+    const int* bList = nullptr;
+    double sign = 1.0;
+    // term inputs (face arrays: internal then boundary faces)
+    double rDt = 0.0;                   // ddt: 1/dt
+    const double* V = nullptr;          // ddt, non-orthogonal gradient: cell volumes
+    const double* psi0 = nullptr;       // ddt: old-time field
+    const double* flux = nullptr;       // div
+    double gammaU = 1.0;                // laplacian: uniform γ unless gamma != nullptr
+    const double* gamma = nullptr;
+    const double* magSf = nullptr;
+    const double* ndc = nullptr;
+    const int* bcFixed = nullptr;       // per boundary face: fixedValue 1 / zeroGradient 0
+    const double* bcValue = nullptr;    // per boundary face (nullable → 0)
+    // laplacian non-orthogonal correction (psi != nullptr)
+    const double* psi = nullptr;
+    const double* w = nullptr;          // per face
+    const double* Sf = nullptr;         // 3 per face
+    const double* corr = nullptr;       // 3 per face
+    int* anyCorr = nullptr;             // device flag: some internal |corr|² > 1e-20
+    double* grad = nullptr;             // 3 per cell (scratch)
+    double* fc = nullptr;               // per internal face (scratch)
+    // FvMatrix parts (in/out)
+    double* diag = nullptr;
+    double* upper = nullptr;
+    double* lower = nullptr;            // non-null once the matrix has a lower part
+    int hadLower = 0;                   // lower held values before this term
+    double* source = nullptr;
+    double* ic = nullptr;               // internalCoeffs
+    double* bs = nullptr;               // boundarySource
+    double* ffc = nullptr;              // face-flux correction (per face; zeros = none yet)
+};
+// own/nei: device owner/neighbour of the internal faces (non-orthogonal correction)
+void launch_fvm_term(hipStream_t st, int kind, const FvmArgs& a, const int* own, const int* nei);
+// FvMatrix −= su (FvMatrix.h:93-97): source[c] += su[c]·V[c]
+void launch_fvm_source(hipStream_t st, int n, const double* su, const double* V, double* source);
+// FvMatrix::solve fold (FvMatrix.h:154-163): d = diag + Σ ic, b = source + Σ bs per cell
+void launch_fvm_fold(hipStream_t st, const FvmArgs& a, double* dOut, double* bOut);
 
 }  // namespace vol

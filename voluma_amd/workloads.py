@@ -69,3 +69,138 @@ def pressure_system(ctx, m, seed=1, gamg=True):
     g = m.geometry()
     b = A.assemble_laplacian(g["magSf"], g["ndc"], bc_fixed_xmax(m), b_in=pressure_rhs(m.n_cells, seed))
     return lm, A, b
+
+
+# ------------------------------------------------------------------ momentum system
+MOMENTUM_NU = 1e-3
+MOMENTUM_CFL = 0.5
+
+
+def _sincos(x):
+    """sin and cos by range reduction to [−π, π] and Taylor polynomials in Horner
+    form: only +, ×, floor, so the values are the same on every CPU (library sin/cos
+    may differ in the last bit between SIMD paths). Error ≲ 1e-14."""
+    two_pi = 2.0 * math.pi
+    r = x - two_pi * np.floor(x / two_pi + 0.5)
+    r2 = r * r
+    s = np.zeros_like(r)
+    c = np.zeros_like(r)
+    for k in range(14, -1, -1):  # Σ (−1)^k r^(2k+1)/(2k+1)!, Σ (−1)^k r^(2k)/(2k)!
+        s = s * r2 + ((-1.0) ** k) / math.factorial(2 * k + 1)
+        c = c * r2 + ((-1.0) ** k) / math.factorial(2 * k)
+    return s * r, c
+
+
+def _dot3(a, b):
+    return a[:, 0] * b[:, 0] + a[:, 1] * b[:, 1] + a[:, 2] * b[:, 2]
+
+
+class FourierVelocity:
+    """U = ∇×A for the seeded vector potential A(x) = Σ_k a_k sin(2π k·x + θ_k), with
+    integer wave vectors (|k_i| ≤ 2), scaled so that the RMS of |U| over 4096 seeded
+    points of the unit box is 1. The face flux is the circulation of A around each
+    face (Stokes), so the flux out of every closed cell sums to zero up to rounding.
+    Everything is evaluated with elementwise +, × and fixed-order sums, so the inputs
+    are bit-identical on every machine."""
+
+    def __init__(self, seed, n_modes=12):
+        u = splitmix_uniform(7 * n_modes, seed).reshape(n_modes, 7)
+        k = np.floor(u[:, :3] * 5.0) - 2.0
+        k[np.all(k == 0, axis=1), 0] = 1.0
+        self.k = k
+        self.a = 2.0 * u[:, 3:6] - 1.0
+        self.theta = 2.0 * math.pi * u[:, 6]
+        self.kxa = np.stack([k[:, 1] * self.a[:, 2] - k[:, 2] * self.a[:, 1],
+                             k[:, 2] * self.a[:, 0] - k[:, 0] * self.a[:, 2],
+                             k[:, 0] * self.a[:, 1] - k[:, 1] * self.a[:, 0]], axis=1)
+        q = splitmix_uniform(3 * 4096, seed + 1).reshape(4096, 3)
+        self.scale = 1.0
+        Uq = self.U(q)
+        self.scale = 1.0 / math.sqrt(math.fsum(_dot3(Uq, Uq)) / 4096.0)
+
+    def _phase(self, x):
+        x = np.asarray(x, dtype=np.float64)
+        ph = np.zeros((x.shape[0], self.k.shape[0]))
+        for j in range(self.k.shape[0]):
+            kx = x[:, 0] * self.k[j, 0] + x[:, 1] * self.k[j, 1] + x[:, 2] * self.k[j, 2]
+            ph[:, j] = 2.0 * math.pi * kx + self.theta[j]
+        return ph
+
+    def _modes(self, w, vec):
+        out = np.zeros((w.shape[0], 3))
+        for j in range(w.shape[1]):
+            for d in range(3):
+                out[:, d] = out[:, d] + w[:, j] * vec[j, d]
+        return out
+
+    def A(self, x):
+        s, _ = _sincos(self._phase(x))
+        return self.scale * self._modes(s, self.a)
+
+    def U(self, x):  # ∇×(a sin(2πk·x + θ)) = 2π cos(·) k×a
+        _, c = _sincos(self._phase(x))
+        return self.scale * (2.0 * math.pi) * self._modes(c, self.kxa)
+
+    def face_flux(self, m):
+        """Σ over the face's edges (in point order) of ½(A_i + A_j)·(x_j − x_i)."""
+        off = np.asarray(m.face_off, dtype=np.int64)
+        pts = np.asarray(m.face_pts, dtype=np.int64)
+        nf = off.size - 1
+        nslot = np.diff(off)
+        X = np.asarray(m.points, dtype=np.float64)
+        Ap = self.A(X)
+        phi = np.zeros(nf)
+        for k in range(int(nslot.max())):  # edges in point order, summed left to right
+            has = nslot > k
+            f = np.nonzero(has)[0]
+            p = pts[off[f] + k]
+            q = pts[np.where(k + 1 < nslot[f], off[f] + k + 1, off[f])]
+            phi[f] = phi[f] + _dot3(0.5 * (Ap[p] + Ap[q]), X[q] - X[p])
+        return phi
+
+
+def momentum_inputs(m, seed=1):
+    """Inputs of the momentum matrix on mesh m: face flux φ, the cell velocity U⁰ at
+    the cell centres (n×3), Dirichlet boundary values (nb×3, U at the boundary face
+    centres) and dt for Courant number 0.5 with Co = ½ Σ|φ|/V·dt
+    (IncompressibleFlow.cpp:97-107)."""
+    g = m.geometry()
+    fv = FourierVelocity(seed)
+    phi = fv.face_flux(m)
+    U0 = fv.U(g["C"])
+    Ub = fv.U(g["Cf"][m.n_internal:])
+    sum_abs = np.zeros(m.n_cells)
+    np.add.at(sum_abs, np.asarray(m.owner), np.abs(phi))
+    np.add.at(sum_abs, np.asarray(m.neighbour), np.abs(phi[: m.n_internal]))
+    co_per_dt = float(np.max(0.5 * sum_abs / g["V"]))
+    dt = MOMENTUM_CFL / co_per_dt
+    return {"phi": phi, "U0": U0, "Ub": Ub, "dt": float(dt), "nu": MOMENTUM_NU}
+
+
+def momentum_assemble(A, m, inp, comp):
+    """UEqn = ddt(U) + div(φ, U) − laplacian(ν, U) for velocity component comp, with
+    fixedValue on every boundary face (IncompressibleFlow.cpp:183-185), assembled on
+    the GPU into LduMatrix A; returns b."""
+    g = m.geometry()
+    bc = np.ones(m.n_boundary, np.int32)
+    psi = np.ascontiguousarray(inp["U0"][:, comp])
+    bv = np.ascontiguousarray(inp["Ub"][:, comp])
+    A.fvm_reset()
+    A.fvm_ddt_euler(+1, inp["dt"], g["V"], psi)
+    A.fvm_div_upwind(+1, inp["phi"], bc, bv)
+    A.fvm_laplacian(-1, g["magSf"], g["ndc"], bc, bv, gamma=inp["nu"],
+                    nonorth={"psi": psi, "w": g["w"], "Sf": g["Sf"], "corr": g["corr"], "V": g["V"]})
+    return A.fvm_finish()
+
+
+def momentum_system(ctx, m, seed=1, comp=0, gamg=False):
+    """(LduMesh, LduMatrix, b, inputs) of one momentum component on mesh m."""
+    import voluma_amd as va
+
+    ni = m.n_internal
+    lm = va.LduMesh(ctx, m.n_cells, m.owner[:ni], m.neighbour, m.agglomeration_weights() if gamg else None)
+    lm.set_boundary(m.owner[ni:])
+    A = va.LduMatrix(lm)
+    inp = momentum_inputs(m, seed)
+    b = momentum_assemble(A, m, inp, comp)
+    return lm, A, b, inp
