@@ -22,300 +22,22 @@
 #include <unordered_map>
 #include <vector>
 
-#include "vol_internal.hpp"
+#include "host_common.hpp"
 
 using namespace vol;
+using namespace vol::host;
 
 namespace {
 
 thread_local std::string gLastError;
 
-struct VolError : std::runtime_error {
-    int code;
-    VolError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
-};
-
-void hipCheck(hipError_t e, const char* what) {
-    if (e != hipSuccess) throw VolError(VOL_ERR_HIP, std::string(what) + ": " + hipGetErrorString(e));
-}
-#define HIPCHECK(x) hipCheck((x), #x)
-
-template <class T> struct DevBuf {
-    T* p = nullptr;
-    size_t n = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf&) = delete;
-    DevBuf& operator=(const DevBuf&) = delete;
-    ~DevBuf() { release(); }
-    void release() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        n = 0;
-    }
-    void alloc(size_t count) {
-        if (count <= n && p) return;
-        release();
-        HIPCHECK(hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1)));
-        n = count;
-    }
-    void upload(const std::vector<T>& v, hipStream_t st) {
-        alloc(v.size());
-        if (!v.empty()) HIPCHECK(hipMemcpyAsync(p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
-    }
-};
-
-struct SellDev {
-    int n = 0, nslices = 0, nslots = 0, maxWidth = 0, csrG = 0, csrIt = 0;
-    DevBuf<int> sliceOff, col, src;
-    SellView view() const {
-        SellView v;
-        v.n = n;
-        v.nslices = nslices;
-        v.nslots = nslots;
-        v.maxWidth = maxWidth;
-        v.csrG = csrG;
-        v.csrIt = csrIt;
-        v.sliceOff = sliceOff.p;
-        v.col = col.p;
-        v.src = src.p;
-        return v;
-    }
-};
-
-// Row lists (row → (col, src) in the given order) → SELL-64 on the device.
-void buildSell(int n, const std::vector<int>& off, const std::vector<int>& colL, const std::vector<int>& srcL,
-               SellDev& S, hipStream_t st) {
-    S.n = n;
-    S.nslices = (n + WAVE - 1) / WAVE;
-    std::vector<int> sliceOff(S.nslices + 1, 0);
-    for (int s = 0; s < S.nslices; ++s) {
-        int w = 0;
-        for (int r = s * WAVE; r < std::min(n, (s + 1) * WAVE); ++r) w = std::max(w, off[r + 1] - off[r]);
-        sliceOff[s + 1] = sliceOff[s] + w * WAVE;
-        S.maxWidth = std::max(S.maxWidth, w);
-    }
-    S.nslots = sliceOff[S.nslices];
-    std::vector<int> col(S.nslots, -1), src(S.nslots, -1);
-    for (int r = 0; r < n; ++r) {
-        const int s = r / WAVE, lane = r % WAVE;
-        for (int k = off[r]; k < off[r + 1]; ++k) {
-            const int idx = sliceOff[s] + (k - off[r]) * WAVE + lane;
-            col[idx] = colL[k];
-            src[idx] = srcL[k];
-        }
-    }
-    S.sliceOff.upload(sliceOff, st);
-    S.col.upload(col, st);
-    S.src.upload(src, st);
-}
-
-// Row lists → CSR with G lanes per row (coarse levels): G·IT ≥ longest row, IT ≤ 8.
-// Returns false when rows are too long for it (then SELL is used).
-bool buildCsr(int n, const std::vector<int>& off, const std::vector<int>& colL, const std::vector<int>& srcL,
-              SellDev& S, hipStream_t st) {
-    int maxW = 0;
-    for (int r = 0; r < n; ++r) maxW = std::max(maxW, off[r + 1] - off[r]);
-    int G = 4;
-    while (G < 16 && G * 4 < maxW) G *= 2;
-    const int IT = (maxW + G - 1) / G;
-    if (IT > 8) return false;
-    S.n = n;
-    S.nslices = n;
-    S.nslots = off[n];
-    S.maxWidth = maxW;
-    S.csrG = G;
-    S.csrIt = std::max(IT, 1);
-    S.sliceOff.upload(off, st);
-    S.col.upload(colL, st);
-    S.src.upload(srcL, st);
-    return true;
-}
-
-// ------------------------------------------------------------------ GAMG hierarchy (host setup)
-// Pairwise agglomeration on geometric weights, same decisions as the reference's
-// agglomerate/buildHierarchy (src/fvm/linalg/GAMG.cpp:40-94, 206-252): cells visited in
-// index order, pair with the unaggregated neighbour of strictly largest weight (first
-// in adjacency order on ties), otherwise join the strongest neighbour's aggregate;
-// coarse faces numbered by first appearance in fine-face order, weights summed in that
-// order; mergeLevels pairings per level; stop at ≤ 64 cells or when no progress.
-struct HLevel {
-    int n = 0;
-    std::vector<int> l, u;
-    std::vector<double> weight;
-    std::vector<int> off, adjCell, adjFace;
-    std::vector<char> adjIsL;
-    std::vector<int> agg, faceMap;
-    std::vector<char> faceFlip;
-
-    void adjacency() {
-        const int nf = int(l.size());
-        off.assign(n + 1, 0);
-        for (int f = 0; f < nf; ++f) {
-            ++off[l[f] + 1];
-            ++off[u[f] + 1];
-        }
-        for (int i = 0; i < n; ++i) off[i + 1] += off[i];
-        adjCell.resize(off[n]);
-        adjFace.resize(off[n]);
-        adjIsL.resize(off[n]);
-        std::vector<int> fill(off.begin(), off.end() - 1);
-        for (int f = 0; f < nf; ++f) {
-            int k = fill[l[f]]++;
-            adjCell[k] = u[f];
-            adjFace[k] = f;
-            adjIsL[k] = 1;
-            k = fill[u[f]]++;
-            adjCell[k] = l[f];
-            adjFace[k] = f;
-            adjIsL[k] = 0;
-        }
-    }
-
-    HLevel pair() {
-        agg.assign(n, -1);
-        int nc = 0;
-        for (int i = 0; i < n; ++i) {
-            if (agg[i] >= 0) continue;
-            int best = -1;
-            double bw = -1;
-            for (int k = off[i]; k < off[i + 1]; ++k) {
-                const int j = adjCell[k];
-                const double wk = weight[adjFace[k]];
-                if (agg[j] < 0 && j != i && wk > bw) {
-                    bw = wk;
-                    best = j;
-                }
-            }
-            if (best >= 0) {
-                agg[i] = agg[best] = nc++;
-                continue;
-            }
-            for (int k = off[i]; k < off[i + 1]; ++k) {
-                const double wk = weight[adjFace[k]];
-                if (wk > bw) {
-                    bw = wk;
-                    best = adjCell[k];
-                }
-            }
-            agg[i] = best >= 0 ? agg[best] : nc++;
-        }
-        HLevel C;
-        C.n = nc;
-        const int nf = int(l.size());
-        faceMap.assign(nf, -1);
-        faceFlip.assign(nf, 0);
-        std::unordered_map<uint64_t, int> ids;
-        ids.reserve(size_t(nf) / 2 + 16);
-        for (int f = 0; f < nf; ++f) {
-            const int a = agg[l[f]], b = agg[u[f]];
-            if (a == b) continue;
-            const int lo = std::min(a, b), hi = std::max(a, b);
-            const uint64_t key = (uint64_t(uint32_t(lo)) << 32) | uint32_t(hi);
-            auto [it, fresh] = ids.try_emplace(key, int(C.l.size()));
-            if (fresh) {
-                C.l.push_back(lo);
-                C.u.push_back(hi);
-                C.weight.push_back(0.0);
-            }
-            faceMap[f] = it->second;
-            faceFlip[f] = char(a != lo);
-            C.weight[it->second] += weight[f];
-        }
-        return C;
-    }
-};
-
-std::vector<HLevel> buildHierarchy(int n, int nf, const int* own, const int* nei, const double* w, int mergeLevels) {
-    std::vector<HLevel> H;
-    HLevel L0;
-    L0.n = n;
-    L0.l.resize(nf);
-    L0.u.resize(nf);
-    L0.weight.assign(w, w + nf);
-    for (int f = 0; f < nf; ++f) {
-        L0.l[f] = std::min(own[f], nei[f]);
-        L0.u[f] = std::max(own[f], nei[f]);
-    }
-    L0.adjacency();
-    H.push_back(std::move(L0));
-    const int nCoarsest = 64;
-    while (H.size() < 40) {
-        HLevel& F = H.back();
-        const int nF = F.n;
-        if (nF <= nCoarsest) break;
-        HLevel C = F.pair();
-        for (int k = 1; k < mergeLevels && C.n > nCoarsest; ++k) {
-            C.adjacency();
-            HLevel C2 = C.pair();
-            for (auto& a : F.agg) a = C.agg[a];
-            for (size_t f = 0; f < F.faceMap.size(); ++f) {
-                const int cf = F.faceMap[f];
-                if (cf < 0) continue;
-                F.faceMap[f] = C.faceMap[cf];
-                F.faceFlip[f] = char(F.faceFlip[f] != C.faceFlip[cf]);
-            }
-            C = std::move(C2);
-        }
-        if (C.n >= nF) {
-            F.agg.clear();
-            break;
-        }
-        C.adjacency();
-        H.push_back(std::move(C));
-    }
-    return H;
-}
-
-struct LevelDev {
-    int n = 0, nf = 0;
-    SellDev sell;                       // coarse levels (level 0 uses the mesh SELL)
-    DevBuf<int> agg;                    // → next level
-    DevBuf<int> mOff, mList;            // next-level cell → its fine cells (ascending)
-    DevBuf<int> iOff, iList;            // next-level cell → intra-aggregate fine faces (ascending)
-    DevBuf<int> cfOff, cfList;          // next-level face → (fine face << 1 | flip), ascending
-    DevBuf<int> l, u;                   // faces (coarsest: dense assembly)
-};
-
 }  // namespace
 
-// ============================================================================ handles
-struct vol_context_s {
-    std::atomic<int> refs{1};  // caller handle + meshes + device allocations
-    int device = 0;
-    hipStream_t stream = nullptr;
-    bool reproducible = false;
-    int tailMax = TAIL_MAX_CELLS;  // GAMG levels this small run in the one-workgroup tail kernel
-    RedScratch red;
-    DevBuf<double> part, result;
-    DevBuf<long long> xpart;
-    DevBuf<int> xnf;
-    DevBuf<DevScalars> scal;
-    DevScalars* hostScal = nullptr;  // pinned, 2 slots
-    // timing
-    bool timing = false;
-    std::vector<hipEvent_t> evPool;
-    size_t evUsed = 0;
-    double amulMs = 0;
-    int64_t amulCount = 0;
+namespace vol::host {
+void setLastError(const std::string& msg) { gLastError = msg; }
+}  // namespace vol::host
 
-    void ensureScratch(int blocks) {
-        const int need = std::max(blocks, RED_BLOCKS);
-        if (red.maxBlk < need) {
-            part.alloc(size_t(MAX_RED) * need);
-            red.part = part.p;
-            red.maxBlk = need;
-        }
-        if (!red.xpart) {
-            xpart.alloc(size_t(MAX_RED) * RED_BLOCKS_EXACT * NBINS);
-            xnf.alloc(size_t(MAX_RED) * RED_BLOCKS_EXACT);
-            result.alloc(MAX_RED);
-            red.xpart = xpart.p;
-            red.xnf = xnf.p;
-            red.result = result.p;
-            red.maxBlkExact = RED_BLOCKS_EXACT;
-        }
-    }
-};
+// ============================================================================ handles
 
 struct vol_mesh_s {
     std::atomic<int> refs{1};  // caller handle + matrices

@@ -469,58 +469,9 @@ __device__ __forceinline__ bool dev_converged(const DevScalars& S) {
     return S.finalResidual < S.tolerance || (S.relTol > 0 && S.finalResidual < S.relTol * S.initialResidual);
 }
 
-__global__ __launch_bounds__(256) void k_finalize(int op, int nred, int exact, int nparts, const double* part,
-                                                  int maxBlk, const long long* xpart, const int* xnf, int maxBlkExact,
-                                                  DevScalars* Sp, double* result) {
-    DevScalars& S = *Sp;
-    if (op == FIN_INNER_START) {  // no reduction: S[1] ← fresh inner solve, or done with S[0]
-        if (threadIdx.x == 0) {
-            DevScalars& I = Sp[1];
-            const double target = fmax(S.tolerance, S.relTol > 0 ? S.relTol * S.initialResidual : 0.0);
-            DevScalars z = {};
-            z.normFactor = S.normFactor;
-            z.tolerance = fmax(S.innerRel * S.finalResidual, 0.5 * target);
-            z.relTol = 0.0;
-            z.maxIter = S.maxIter - S.iterations;
-            z.minIter = 0;
-            z.nSweeps = 1;
-            z.n = S.n;
-            z.wArA = GREAT;
-            z.done = S.done;
-            I = z;
-        }
-        return;
-    }
-    if (op != FIN_STORE && S.done) return;
-    __shared__ double vals[MAX_RED];
-    __shared__ double sh[256 / WAVE];
-    __shared__ long long bins[NBINS];
-    __shared__ int nfl;
-    for (int q = 0; q < nred; ++q) {
-        if (!exact) {
-            double s = 0.0;
-            for (int k = threadIdx.x; k < nparts; k += blockDim.x) s += part[size_t(q) * maxBlk + k];
-            const double t = block_sum(s, sh);
-            if (threadIdx.x == 0) vals[q] = t;
-        } else {
-            if (threadIdx.x == 0) nfl = 0;
-            __syncthreads();
-            if (threadIdx.x < NBINS) {
-                long long s = 0;
-                for (int k = 0; k < nparts; ++k) s += xpart[(size_t(q) * maxBlkExact + k) * NBINS + threadIdx.x];
-                bins[threadIdx.x] = s;
-            }
-            for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
-                const int f = xnf[q * maxBlkExact + k];
-                if (f) atomicOr(&nfl, f);
-            }
-            __syncthreads();
-            if (threadIdx.x == 0) vals[q] = exact_to_double(bins, nfl);
-        }
-        __syncthreads();
-    }
-    if (threadIdx.x != 0) return;
-    const double v0 = vals[0], v1 = nred > 1 ? vals[1] : 0.0;
+// The solver's scalar step after a reduction (v0, v1 = reduced values). I is the inner
+// solve's scalars of mixed precision (FIN_OUTER).
+__device__ void fin_step(int op, DevScalars& S, const DevScalars& I, double v0, double v1) {
     switch (op) {
     case FIN_XREF: S.xRef = v0 / double(S.n); break;
     case FIN_INIT:
@@ -588,8 +539,7 @@ __global__ __launch_bounds__(256) void k_finalize(int op, int nred, int exact, i
         S.finalResidual = v0 / S.normFactor;
         if (!(S.iterations < S.maxIter && !dev_converged(S))) S.done = 1;
         break;
-    case FIN_OUTER: {  // Sp[1] holds the inner solve that just ended
-        const DevScalars& I = Sp[1];
+    case FIN_OUTER: {  // I holds the inner solve that just ended
         S.iterations += I.iterations;
         ++S.outer;
         S.finalResidual = v0 / S.normFactor;
@@ -597,13 +547,73 @@ __global__ __launch_bounds__(256) void k_finalize(int op, int nred, int exact, i
         if (!(S.iterations < S.maxIter && !dev_converged(S)) || I.iterations == 0) S.done = 1;
         break;
     }
-    case FIN_STORE:
+    default:
+        break;
+    }
+}
+
+// Mixed precision: I ← a fresh inner solve for the current outer residual of S, or done.
+__device__ void inner_start(const DevScalars& S, DevScalars& I) {
+    const double target = fmax(S.tolerance, S.relTol > 0 ? S.relTol * S.initialResidual : 0.0);
+    DevScalars z = {};
+    z.normFactor = S.normFactor;
+    z.tolerance = fmax(S.innerRel * S.finalResidual, 0.5 * target);
+    z.relTol = 0.0;
+    z.maxIter = S.maxIter - S.iterations;
+    z.minIter = 0;
+    z.nSweeps = 1;
+    z.n = S.n;
+    z.wArA = GREAT;
+    z.done = S.done;
+    I = z;
+}
+
+__global__ __launch_bounds__(256) void k_finalize(int op, int nred, int exact, int nparts, const double* part,
+                                                  int maxBlk, const long long* xpart, const int* xnf, int maxBlkExact,
+                                                  DevScalars* Sp, double* result) {
+    DevScalars& S = *Sp;
+    if (op == FIN_INNER_START) {  // no reduction: S[1] ← fresh inner solve, or done with S[0]
+        if (threadIdx.x == 0) inner_start(S, Sp[1]);
+        return;
+    }
+    if (op != FIN_STORE && S.done) return;
+    __shared__ double vals[MAX_RED];
+    __shared__ double sh[256 / WAVE];
+    __shared__ long long bins[NBINS];
+    __shared__ int nfl;
+    for (int q = 0; q < nred; ++q) {
+        if (!exact) {
+            double s = 0.0;
+            for (int k = threadIdx.x; k < nparts; k += blockDim.x) s += part[size_t(q) * maxBlk + k];
+            const double t = block_sum(s, sh);
+            if (threadIdx.x == 0) vals[q] = t;
+        } else {
+            if (threadIdx.x == 0) nfl = 0;
+            __syncthreads();
+            if (threadIdx.x < NBINS) {
+                long long s = 0;
+                for (int k = 0; k < nparts; ++k) s += xpart[(size_t(q) * maxBlkExact + k) * NBINS + threadIdx.x];
+                bins[threadIdx.x] = s;
+            }
+            for (int k = threadIdx.x; k < nparts; k += blockDim.x) {
+                const int f = xnf[q * maxBlkExact + k];
+                if (f) atomicOr(&nfl, f);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) vals[q] = exact_to_double(bins, nfl);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const double v0 = vals[0], v1 = nred > 1 ? vals[1] : 0.0;
+    if (op == FIN_STORE) {
         if (result) {
             result[0] = v0;
             if (nred > 1) result[1] = v1;
         }
-        break;
+        return;
     }
+    fin_step(op, S, Sp[1], v0, v1);
 }
 
 void launch_finalize(hipStream_t st, int op, int nred, bool exact, int nparts, const RedScratch& R, DevScalars* S) {
