@@ -69,6 +69,7 @@ extern "C" {
 typedef struct vol_context_s* vol_context;
 typedef struct vol_mesh_s* vol_mesh;
 typedef struct vol_matrix_s* vol_matrix;
+typedef struct vol_batch_s* vol_batch;
 
 /* SolverControls (LinearSolver.h:15-23) plus GPU extensions. Strings as in the
  * reference: solver PCG | PBiCGStab | GaussSeidel | symGaussSeidel | Jacobi;
@@ -219,6 +220,25 @@ int vol_fvm_get(vol_matrix A, double* diag, double* source, double* internalCoef
 /* fold and finish (FvMatrix.h:154-163): A's solve diagonal = diag + Σ internalCoeffs,
  * b (nullable, nCells) = source + Σ boundarySource per owner cell */
 int vol_fvm_finish(vol_matrix A, double* b, int location);
+
+/* ---- batches: many independent systems solved together (no reference counterpart).
+ * Systems are concatenated: owner/neighbour/gamgWeights hold each system's internal
+ * faces in turn with that system's own (0-based) cell numbers; diag/upper/lower, b and
+ * x are the concatenations of the systems' arrays. Each system keeps the GAMG hierarchy
+ * it has alone (built as vol_mesh_create builds it; every system needs more than 64
+ * cells), its own Krylov scalars and its own convergence test, and one kernel launch
+ * serves all systems in each phase. A system's result does not depend on the other
+ * systems of the batch; it agrees with vol_solve of that system within the solve's
+ * tolerance (the reduction trees differ). PCG with GAMG, diagonal or no
+ * preconditioner; fast sums; VOL_FP64 or VOL_MIXED. perf: one entry per system. */
+int vol_batch_create(vol_context ctx, int nSystems, const int32_t* nCells, const int32_t* nInternalFaces,
+                     const int32_t* owner, const int32_t* neighbour, const double* gamgWeights, vol_batch* out);
+int vol_batch_destroy(vol_batch batch);
+/* total cells and faces; levels[s] = GAMG levels of system s (nullable outputs) */
+int vol_batch_info(vol_batch batch, int64_t* nCellsTotal, int64_t* nFacesTotal, int32_t* levels);
+int vol_batch_set(vol_batch batch, const double* diag, const double* upper, const double* lower, int location);
+int vol_batch_solve(vol_batch batch, const double* b, double* x, const vol_solver_controls* controls,
+                    vol_solver_performance* perf, int location);
 
 /* ---- measurement hooks (no reference counterpart): time every level-0 A·p
  * product issued by vol_solve with HIP events on the context stream. */

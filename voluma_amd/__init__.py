@@ -81,7 +81,8 @@ EXPORTED = ("vol_controls_default", "vol_last_error", "vol_context_create", "vol
             "vol_mesh_gamg_level", "vol_matrix_create", "vol_matrix_set", "vol_matrix_destroy", "vol_amul",
             "vol_solve", "vol_timing_enable", "vol_timing_read", "vol_mesh_set_boundary", "vol_assemble_laplacian",
             "vol_assemble_div_upwind", "vol_matrix_get", "vol_device_copy", "vol_fvm_reset", "vol_fvm_ddt_euler",
-            "vol_fvm_div_upwind", "vol_fvm_laplacian", "vol_fvm_source", "vol_fvm_get", "vol_fvm_finish")
+            "vol_fvm_div_upwind", "vol_fvm_laplacian", "vol_fvm_source", "vol_fvm_get", "vol_fvm_finish",
+            "vol_batch_create", "vol_batch_destroy", "vol_batch_info", "vol_batch_set", "vol_batch_solve")
 
 
 def lib():
@@ -125,6 +126,11 @@ def lib():
         L.vol_fvm_source.argtypes = [P, P, P, I]
         L.vol_fvm_get.argtypes = [P, P, P, P, P, P, I]
         L.vol_fvm_finish.argtypes = [P, P, I]
+        L.vol_batch_create.argtypes = [P, I, P, P, P, P, P, ctypes.POINTER(P)]
+        L.vol_batch_destroy.argtypes = [P]
+        L.vol_batch_info.argtypes = [P, P, P, P]
+        L.vol_batch_set.argtypes = [P, P, P, P, I]
+        L.vol_batch_solve.argtypes = [P, P, P, ctypes.POINTER(_Controls), P, I]
         for name in EXPORTED:
             if name not in ("vol_controls_default", "vol_last_error"):
                 getattr(L, name).restype = I
@@ -262,6 +268,7 @@ class LduMesh:
         if own.size != nei.size:
             raise ValueError("owner/neighbour sizes differ")
         self.n_cells, self.n_faces = int(n_cells), int(own.size)
+        self.n_boundary = 0  # until set_boundary
         w = None if gamg_weights is None else np.ascontiguousarray(gamg_weights, dtype=np.float64)
         h = ctypes.c_void_p()
         _check(lib().vol_mesh_create(ctx._h, self.n_cells, self.n_faces, _p(own), _p(nei),
@@ -435,6 +442,70 @@ def amul(A: LduMatrix, x, y=None):
     yy = np.empty_like(xx)
     _check(lib().vol_amul(A._h, _p(xx), _p(yy), VOL_HOST))
     return yy
+
+
+class LduBatch:
+    """Independent systems solved together (vol_batch): each system keeps its own GAMG
+    hierarchy, Krylov scalars and convergence test; one launch per phase serves all.
+    systems: sequence of (n_cells, owner, neighbour, gamg_weights) with each system's
+    own cell numbers."""
+
+    def __init__(self, ctx: Context, systems):
+        self.ctx = ctx
+        self.n_cells = np.array([int(s[0]) for s in systems], np.int32)
+        self.n_faces = np.array([len(s[1]) for s in systems], np.int32)
+        own = np.ascontiguousarray(np.concatenate([np.asarray(s[1], np.int32) for s in systems]), dtype=np.int32)
+        nei = np.ascontiguousarray(np.concatenate([np.asarray(s[2], np.int32) for s in systems]), dtype=np.int32)
+        w = _f64(np.concatenate([np.asarray(s[3], np.float64) for s in systems]))
+        h = ctypes.c_void_p()
+        _check(lib().vol_batch_create(ctx._h, len(systems), _p(self.n_cells), _p(self.n_faces), _p(own), _p(nei),
+                                      _p(w), ctypes.byref(h)))
+        self._h = h
+        self.n_systems = len(systems)
+        self.cell_off = np.concatenate([[0], np.cumsum(self.n_cells, dtype=np.int64)])
+        self.face_off = np.concatenate([[0], np.cumsum(self.n_faces, dtype=np.int64)])
+        self.n_total = int(self.cell_off[-1])
+
+    def levels(self):
+        lv = np.zeros(self.n_systems, np.int32)
+        _check(lib().vol_batch_info(self._h, None, None, _p(lv)))
+        return lv
+
+    def set(self, diag, upper, lower=None):
+        """Concatenated diag (boundary-folded) / upper / lower of all systems."""
+        if isinstance(diag, DeviceArray):
+            _check(lib().vol_batch_set(self._h, diag.ptr, upper.ptr, None if lower is None else lower.ptr, VOL_DEVICE))
+            return
+        d, u = _f64(diag), _f64(upper)
+        lo = None if lower is None else _f64(lower)
+        if d.size != self.n_total or u.size != int(self.face_off[-1]):
+            raise ValueError("coefficient sizes do not match the batch")
+        _check(lib().vol_batch_set(self._h, _p(d), _p(u), None if lo is None else _p(lo), VOL_HOST))
+
+    def solve(self, b, x, controls: SolverControls | None = None):
+        """Solve every system; x (concatenated, in/out). Returns one SolverPerformance per system."""
+        ctl = controls or SolverControls()
+        c = ctl._c()
+        pf = (_Perf * self.n_systems)()
+        if isinstance(b, DeviceArray):
+            _check(lib().vol_batch_solve(self._h, b.ptr, x.ptr, ctypes.byref(c), ctypes.cast(pf, ctypes.c_void_p),
+                                         VOL_DEVICE))
+        else:
+            bb = _f64(b)
+            if not (isinstance(x, np.ndarray) and x.dtype == np.float64 and x.flags.c_contiguous):
+                raise TypeError("x must be a contiguous float64 array")
+            _check(lib().vol_batch_solve(self._h, _p(bb), _p(x), ctypes.byref(c), ctypes.cast(pf, ctypes.c_void_p),
+                                         VOL_HOST))
+        return [SolverPerformance(p.solver.decode(), "", p.initialResidual, p.finalResidual, p.residualSum,
+                                  p.normFactor, p.iterations, bool(p.converged), p.outerIterations) for p in pf]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().vol_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
 
 
 def solve_ldu(A: LduMatrix, b, x, controls: SolverControls = None, field="") -> SolverPerformance:

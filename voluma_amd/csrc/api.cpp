@@ -94,29 +94,7 @@ struct vol_matrix_s {
 
 namespace {
 
-int fail(const VolError& e) {
-    gLastError = e.what();
-    return e.code;
-}
-int failStd(const std::exception& e) {
-    gLastError = e.what();
-    return VOL_ERR_INVALID_ARGUMENT;
-}
 
-#define API_BEGIN try {
-#define API_END                          \
-    }                                    \
-    catch (const VolError& e) {          \
-        return fail(e);                  \
-    }                                    \
-    catch (const std::bad_alloc&) {      \
-        gLastError = "host out of memory"; \
-        return VOL_ERR_INVALID_ARGUMENT; \
-    }                                    \
-    catch (const std::exception& e) {    \
-        return failStd(e);               \
-    }                                    \
-    return VOL_OK;
 
 void setDevice(vol_context ctx) { HIPCHECK(hipSetDevice(ctx->device)); }
 
@@ -167,41 +145,8 @@ void setupMesh(vol_mesh m, const double* w, int mergeLevels) {
         if (k + 1 < nL) {
             const HLevel& C = m->H[k + 1];
             D->agg.upload(F.agg, st);
-            std::vector<int> mOff(C.n + 1, 0), mList(F.n);
-            for (int i = 0; i < F.n; ++i) ++mOff[F.agg[i] + 1];
-            for (int I = 0; I < C.n; ++I) mOff[I + 1] += mOff[I];
-            {
-                std::vector<int> fill(mOff.begin(), mOff.end() - 1);
-                for (int i = 0; i < F.n; ++i) mList[fill[F.agg[i]]++] = i;
-            }
-            std::vector<int> iOff(C.n + 1, 0), iList;
-            const int fnf = int(F.l.size());
-            for (int f = 0; f < fnf; ++f)
-                if (F.faceMap[f] < 0) ++iOff[F.agg[F.l[f]] + 1];
-            for (int I = 0; I < C.n; ++I) iOff[I + 1] += iOff[I];
-            iList.resize(iOff[C.n]);
-            {
-                std::vector<int> fill(iOff.begin(), iOff.end() - 1);
-                for (int f = 0; f < fnf; ++f)
-                    if (F.faceMap[f] < 0) iList[fill[F.agg[F.l[f]]]++] = f;
-            }
-            const int cnf = int(C.l.size());
-            std::vector<int> cfOff(cnf + 1, 0), cfList;
-            for (int f = 0; f < fnf; ++f)
-                if (F.faceMap[f] >= 0) ++cfOff[F.faceMap[f] + 1];
-            for (int c = 0; c < cnf; ++c) cfOff[c + 1] += cfOff[c];
-            cfList.resize(cfOff[cnf]);
-            {
-                std::vector<int> fill(cfOff.begin(), cfOff.end() - 1);
-                for (int f = 0; f < fnf; ++f)
-                    if (F.faceMap[f] >= 0) cfList[fill[F.faceMap[f]]++] = (f << 1) | (F.faceFlip[f] ? 1 : 0);
-            }
-            D->mOff.upload(mOff, st);
-            D->mList.upload(mList, st);
-            D->iOff.upload(iOff, st);
-            D->iList.upload(iList, st);
-            D->cfOff.upload(cfOff, st);
-            D->cfList.upload(cfList, st);
+            const GatherLists G = buildGatherLists(F.n, C.n, F.agg, F.l, F.faceMap, F.faceFlip, int(C.l.size()));
+            G.upload(*D, st);
         }
         m->lv.push_back(std::move(D));
     }

@@ -272,7 +272,86 @@ struct LevelDev {
 // Per-thread message of the last failed C ABI call (vol_last_error).
 void setLastError(const std::string& msg);
 
+// Restriction and Galerkin gather lists from fine level F to coarse level C:
+//   mOff/mList:   coarse cell → its fine cells, ascending
+//   iOff/iList:   coarse cell → fine faces inside the aggregate (faceMap < 0), ascending
+//   cfOff/cfList: coarse face → (fine face << 1 | flip), ascending
+// Fine cells with agg < 0 and faces of such cells are skipped (batches: systems whose
+// coarsest level is F).
+struct GatherLists {
+    std::vector<int> mOff, mList, iOff, iList, cfOff, cfList;
+    template <class LD> void upload(LD& D, hipStream_t st) const {
+        D.mOff.upload(mOff, st);
+        D.mList.upload(mList, st);
+        D.iOff.upload(iOff, st);
+        D.iList.upload(iList, st);
+        D.cfOff.upload(cfOff, st);
+        D.cfList.upload(cfList, st);
+    }
+};
+inline GatherLists buildGatherLists(int nFine, int nCoarse, const std::vector<int>& agg, const std::vector<int>& fl,
+                                    const std::vector<int>& faceMap, const std::vector<char>& flip, int nCoarseFaces) {
+    GatherLists G;
+    const int fnf = int(faceMap.size());
+    G.mOff.assign(nCoarse + 1, 0);
+    for (int i = 0; i < nFine; ++i)
+        if (agg[i] >= 0) ++G.mOff[agg[i] + 1];
+    for (int I = 0; I < nCoarse; ++I) G.mOff[I + 1] += G.mOff[I];
+    G.mList.resize(G.mOff[nCoarse]);
+    {
+        std::vector<int> fill(G.mOff.begin(), G.mOff.end() - 1);
+        for (int i = 0; i < nFine; ++i)
+            if (agg[i] >= 0) G.mList[fill[agg[i]]++] = i;
+    }
+    G.iOff.assign(nCoarse + 1, 0);
+    for (int f = 0; f < fnf; ++f)
+        if (faceMap[f] < 0 && agg[fl[f]] >= 0) ++G.iOff[agg[fl[f]] + 1];
+    for (int I = 0; I < nCoarse; ++I) G.iOff[I + 1] += G.iOff[I];
+    G.iList.resize(G.iOff[nCoarse]);
+    {
+        std::vector<int> fill(G.iOff.begin(), G.iOff.end() - 1);
+        for (int f = 0; f < fnf; ++f)
+            if (faceMap[f] < 0 && agg[fl[f]] >= 0) G.iList[fill[agg[fl[f]]]++] = f;
+    }
+    G.cfOff.assign(nCoarseFaces + 1, 0);
+    for (int f = 0; f < fnf; ++f)
+        if (faceMap[f] >= 0) ++G.cfOff[faceMap[f] + 1];
+    for (int c = 0; c < nCoarseFaces; ++c) G.cfOff[c + 1] += G.cfOff[c];
+    G.cfList.resize(G.cfOff[nCoarseFaces]);
+    {
+        std::vector<int> fill(G.cfOff.begin(), G.cfOff.end() - 1);
+        for (int f = 0; f < fnf; ++f)
+            if (faceMap[f] >= 0) G.cfList[fill[faceMap[f]]++] = (f << 1) | (flip[f] ? 1 : 0);
+    }
+    return G;
+}
+
+inline int fail(const VolError& e) {
+    setLastError(e.what());
+    return e.code;
+}
+inline int failStd(const std::exception& e) {
+    setLastError(e.what());
+    return VOL_ERR_INVALID_ARGUMENT;
+}
+
 }  // namespace vol::host
+
+// Body of every C ABI function: exceptions become status codes.
+#define API_BEGIN try {
+#define API_END                                           \
+    }                                                     \
+    catch (const vol::host::VolError& e) {                \
+        return vol::host::fail(e);                        \
+    }                                                     \
+    catch (const std::bad_alloc&) {                       \
+        vol::host::setLastError("host out of memory");    \
+        return VOL_ERR_INVALID_ARGUMENT;                  \
+    }                                                     \
+    catch (const std::exception& e) {                     \
+        return vol::host::failStd(e);                     \
+    }                                                     \
+    return VOL_OK;
 
 using namespace vol;  // internal header: the library's own namespaces
 using namespace vol::host;
@@ -315,3 +394,5 @@ struct vol_context_s {
     }
 };
 
+// Drops one reference of a context and frees it with the last one (api.cpp).
+void ctxRelease(vol_context c);

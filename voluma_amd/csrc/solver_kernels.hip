@@ -324,6 +324,8 @@ __device__ __forceinline__ void vec_elem(const VecArgsT<T>& A, int i, double& r0
         A.w[i] = T(1.0 / ld(A.a, i));
     } else if constexpr (OP == V_FROMZERO) {  // a = r, b = d
         A.w[i] = T(A.omega * (ld(A.a, i) / ld(A.b, i)));
+    } else if constexpr (OP == V_FILL_XREF) {
+        A.w[i] = T(S->xRef);
     }
 }
 
@@ -635,6 +637,28 @@ __global__ void k_init_scalars(DevScalars* S, double tol, double relTol, int max
     *S = z;
 }
 
+__global__ void k_binit_scalars(DevScalars* S, int nSys, const int* n, double tol, double relTol, int maxIter,
+                                int minIter, int nSweeps, double innerRel) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nSys) return;
+    DevScalars z = {};
+    z.innerRel = innerRel;
+    z.tolerance = tol;
+    z.relTol = relTol;
+    z.maxIter = maxIter;
+    z.minIter = minIter;
+    z.nSweeps = nSweeps;
+    z.n = n[s];
+    z.wArA = GREAT;  // LinearSolver.cpp:180
+    S[s] = z;
+}
+
+void launch_binit_scalars(hipStream_t st, DevScalars* S, int nSys, const int* n, double tol, double relTol,
+                          int maxIter, int minIter, int nSweeps, double innerRel) {
+    hipLaunchKernelGGL(k_binit_scalars, dim3((nSys + 63) / 64), dim3(64), 0, st, S, nSys, n, tol, relTol, maxIter,
+                       minIter, nSweeps, innerRel);
+}
+
 __global__ void k_clear_exit(DevScalars* S) { S->exitPending = 0; }
 
 void launch_clear_exit(hipStream_t st, DevScalars* S) { hipLaunchKernelGGL(k_clear_exit, dim3(1), dim3(1), 0, st, S); }
@@ -734,17 +758,25 @@ void launch_orient0(hipStream_t st, int nf, const char* ownerIsL, const double* 
 }
 
 // Dense coarsest matrix (GAMG.cpp:321-326): lu[i][i] = d[i]; lu[l][u] += upper; lu[u][l] += lower.
-__global__ void k_coarse_assemble(int n, int nf, const int* l, const int* u, const double* d, const double* cu,
-                                  const double* cl, double* lu) {
+// Dense coarsest matrix: diagonal, then the faces in order (rows/cols relative to base).
+__device__ void coarse_assemble_body(int n, int nf, const int* l, const int* u, int base, const double* d,
+                                     const double* cu, const double* cl, double* lu) {
     for (int k = threadIdx.x; k < n * n; k += blockDim.x) lu[k] = 0.0;
     __syncthreads();
     for (int i = threadIdx.x; i < n; i += blockDim.x) lu[size_t(i) * n + i] = d[i];
     __syncthreads();
     if (threadIdx.x == 0)  // faces in order: a pair (l,u) may repeat only across levels, keep the reference order
         for (int f = 0; f < nf; ++f) {
-            lu[size_t(l[f]) * n + u[f]] += cu[f];
-            lu[size_t(u[f]) * n + l[f]] += cl[f];
+            const int a = l[f] - base, b = u[f] - base;
+            lu[size_t(a) * n + b] += cu[f];
+            lu[size_t(b) * n + a] += cl[f];
         }
+    __syncthreads();
+}
+
+__global__ void k_coarse_assemble(int n, int nf, const int* l, const int* u, const double* d, const double* cu,
+                                  const double* cl, double* lu) {
+    coarse_assemble_body(n, nf, l, u, 0, d, cu, cl, lu);
 }
 
 void launch_coarse_assemble(hipStream_t st, int n, int nf, const int* l, const int* u, const double* d,
@@ -754,7 +786,7 @@ void launch_coarse_assemble(hipStream_t st, int n, int nf, const int* l, const i
 
 // No-pivot Doolittle LU (GAMG.cpp:334-343). Within step k every (i, j) update is
 // independent, so the parallel form performs the same rounded operations.
-__global__ void k_coarse_lu(int n, double* lu, double* luT, int* status) {
+__device__ void coarse_lu_body(int n, double* lu, double* luT, int* status) {
     __shared__ int bad;
     if (threadIdx.x == 0) bad = 0;
     __syncthreads();
@@ -780,6 +812,8 @@ __global__ void k_coarse_lu(int n, double* lu, double* luT, int* status) {
     if (threadIdx.x == 0 && status) status[0] = bad ? -1 : 0;
 }
 
+__global__ void k_coarse_lu(int n, double* lu, double* luT, int* status) { coarse_lu_body(n, lu, luT, status); }
+
 void launch_coarse_lu(hipStream_t st, int n, double* lu, double* luT, int* status) {
     hipLaunchKernelGGL(k_coarse_lu, dim3(1), dim3(256), 0, st, n, lu, luT, status);
 }
@@ -789,9 +823,7 @@ void launch_coarse_lu(hipStream_t st, int n, double* lu, double* luT, int* statu
 // substitution column by column from the last one (descending k per row, then the
 // division by the pivot).
 template <class T>
-__global__ void k_coarse_solve(int n, const double* lu, const T* r, T* x, const int* done) {
-    if (done && *done) return;
-    extern __shared__ double g[];
+__device__ void coarse_solve_body(int n, const double* lu, const T* r, T* x, double* g) {
     for (int i = threadIdx.x; i < n; i += blockDim.x) g[i] = double(r[i]);
     __syncthreads();
     for (int k = 0; k < n; ++k) {
@@ -809,12 +841,18 @@ __global__ void k_coarse_solve(int n, const double* lu, const T* r, T* x, const 
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = T(g[i]);
 }
 
+template <class T>
+__global__ void k_coarse_solve(int n, const double* lu, const T* r, T* x, const int* done) {
+    if (done && *done) return;
+    extern __shared__ double g[];
+    coarse_solve_body(n, lu, r, x, g);
+}
+
 // n ≤ 64: one wavefront, lane i holds row i of the factors in registers and g_i; the
 // column steps exchange g_k by a cross-lane broadcast. Same operations and order as
 // k_coarse_solve.
 template <class T>
-__global__ __launch_bounds__(64) void k_coarse_solve64(int n, const double* luT, const T* r, T* x, const int* done) {
-    if (done && *done) return;
+__device__ void coarse_solve64_body(int n, const double* luT, const T* r, T* x) {
     const int i = threadIdx.x;
     double row[64];
 #pragma unroll
@@ -839,6 +877,12 @@ __global__ __launch_bounds__(64) void k_coarse_solve64(int n, const double* luT,
 }
 
 template <class T>
+__global__ __launch_bounds__(64) void k_coarse_solve64(int n, const double* luT, const T* r, T* x, const int* done) {
+    if (done && *done) return;
+    coarse_solve64_body(n, luT, r, x);
+}
+
+template <class T>
 static void coarse_solve_t(hipStream_t st, int n, const double* lu, const double* luT, const T* r, T* x,
                            const int* done) {
     if (n <= 64)
@@ -853,6 +897,168 @@ void launch_coarse_solve(hipStream_t st, int n, const double* lu, const double* 
 void launch_coarse_solve32(hipStream_t st, int n, const double* lu, const double* luT, const float* r, float* x,
                            const int* done) {
     coarse_solve_t(st, n, lu, luT, r, x, done);
+}
+
+// ============================================================================ batches
+template <int OP, int NRED, class T>
+__global__ __launch_bounds__(RED_THREADS) void k_bvec(VecArgsT<T> A, const int4* chunks, double* part, int maxChunks) {
+    const int4 ch = chunks[blockIdx.x];
+    VecArgsT<T> a = A;
+    a.S = A.S + ch.x;
+    if constexpr (OP == V_BI_XEXIT) {
+        if (!a.S->exitPending) return;
+    } else {
+        if (a.S->done) return;
+    }
+    double s0 = 0.0, s1 = 0.0;
+    for (int i = ch.y + threadIdx.x; i < ch.z; i += blockDim.x) {
+        double r0 = 0.0, r1 = 0.0;
+        vec_elem<OP>(a, i, r0, r1);
+        if constexpr (NRED > 0) s0 += r0;
+        if constexpr (NRED > 1) s1 += r1;
+    }
+    if constexpr (NRED > 0) {
+        __shared__ double sh[RED_THREADS / WAVE];
+        const double t0 = block_sum(s0, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = t0;
+        if constexpr (NRED > 1) {
+            const double t1 = block_sum(s1, sh);
+            if (threadIdx.x == 0) part[maxChunks + blockIdx.x] = t1;
+        }
+    }
+}
+
+template <int OP, class T>
+static void bvec_dispatch(hipStream_t st, const VecArgsT<T>& a, const BatchRed& R) {
+    constexpr int NRED = (OP == V_INIT || OP == V_BI_DOT2) ? 2
+                         : (OP == V_SUM_X || OP == V_PRECOND_DOT || OP == V_DOT || OP == V_PCG_XR || OP == V_BI_S ||
+                            OP == V_BMINUSAX_NORM || OP == V_BI_XR)
+                             ? 1
+                             : 0;
+    if (R.nChunks == 0) return;
+    hipLaunchKernelGGL((k_bvec<OP, NRED, T>), dim3(R.nChunks), dim3(RED_THREADS), 0, st, a, R.chunks, R.part,
+                       R.maxChunks);
+}
+
+template <class T>
+static void launch_bvec_t(hipStream_t st, int op, const VecArgsT<T>& a, const BatchRed& R) {
+    switch (op) {
+    case V_SUM_X: bvec_dispatch<V_SUM_X>(st, a, R); break;
+    case V_INIT: bvec_dispatch<V_INIT>(st, a, R); break;
+    case V_PRECOND_DOT: bvec_dispatch<V_PRECOND_DOT>(st, a, R); break;
+    case V_DOT: bvec_dispatch<V_DOT>(st, a, R); break;
+    case V_PCG_P: bvec_dispatch<V_PCG_P>(st, a, R); break;
+    case V_PCG_XR: bvec_dispatch<V_PCG_XR>(st, a, R); break;
+    case V_BI_P: bvec_dispatch<V_BI_P>(st, a, R); break;
+    case V_SCALE: bvec_dispatch<V_SCALE>(st, a, R); break;
+    case V_BI_S: bvec_dispatch<V_BI_S>(st, a, R); break;
+    case V_BI_XEXIT: bvec_dispatch<V_BI_XEXIT>(st, a, R); break;
+    case V_BI_DOT2: bvec_dispatch<V_BI_DOT2>(st, a, R); break;
+    case V_BI_XR: bvec_dispatch<V_BI_XR>(st, a, R); break;
+    case V_BMINUSAX_NORM: bvec_dispatch<V_BMINUSAX_NORM>(st, a, R); break;
+    case V_COPY: bvec_dispatch<V_COPY>(st, a, R); break;
+    case V_FILL_XREF: bvec_dispatch<V_FILL_XREF>(st, a, R); break;
+    }
+}
+void launch_bvec(hipStream_t st, int op, const VecArgs& a, const BatchRed& R) { launch_bvec_t(st, op, a, R); }
+void launch_bvec32(hipStream_t st, int op, const VecArgs32& a, const BatchRed& R) { launch_bvec_t(st, op, a, R); }
+
+__global__ void k_bfinalize(int op, int nred, const double* part, int maxChunks, const int* sysChunkOff,
+                            DevScalars* S, DevScalars* I, int nSys) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nSys) return;
+    if (op == FIN_INNER_START) {
+        inner_start(S[s], I[s]);
+        return;
+    }
+    if (S[s].done) return;
+    double v[MAX_RED] = {0.0, 0.0};
+    for (int q = 0; q < nred; ++q)
+        for (int k = sysChunkOff[s]; k < sysChunkOff[s + 1]; ++k) v[q] += part[size_t(q) * maxChunks + k];
+    fin_step(op, S[s], I ? I[s] : S[s], v[0], v[1]);
+}
+
+void launch_bfinalize(hipStream_t st, int op, int nred, const BatchRed& R, DevScalars* S, DevScalars* I) {
+    if (R.nSys == 0) return;
+    hipLaunchKernelGGL(k_bfinalize, dim3((R.nSys + 63) / 64), dim3(64), 0, st, op, nred, R.part, R.maxChunks,
+                       R.sysChunkOff, S, I, R.nSys);
+}
+
+__global__ void k_bcount(const DevScalars* S, int nSys, int* nDone) {
+    int c = 0;
+    for (int s = threadIdx.x; s < nSys; s += blockDim.x) c += S[s].done ? 1 : 0;
+    __shared__ double sh[256 / WAVE];
+    const double t = block_sum(double(c), sh);
+    if (threadIdx.x == 0) nDone[0] = int(t);
+}
+
+void launch_bcount(hipStream_t st, const DevScalars* S, int nSys, int* nDone) {
+    hipLaunchKernelGGL(k_bcount, dim3(1), dim3(256), 0, st, S, nSys, nDone);
+}
+
+__global__ void k_bmix(int op, const int4* chunks, const DevScalars* S, const double* a, float* w, float* w2,
+                       float* z, double* x, const float* e) {
+    const int4 ch = chunks[blockIdx.x];
+    if (S[ch.x].done) return;
+    for (int i = ch.y + threadIdx.x; i < ch.z; i += blockDim.x) {
+        if (op == MIX_TO32) {
+            const float v = float(a[i]);
+            w[i] = v;
+            if (w2) w2[i] = v;
+            if (z) z[i] = 0.0f;
+        } else {
+            x[i] += double(e[i]);
+        }
+    }
+}
+
+void launch_bmix(hipStream_t st, int op, const BatchRed& R, const DevScalars* S, const double* a, float* w,
+                 float* w2, float* z, double* x, const float* e) {
+    if (R.nChunks == 0) return;
+    hipLaunchKernelGGL(k_bmix, dim3(R.nChunks), dim3(RED_THREADS), 0, st, op, R.chunks, S, a, w, w2, z, x, e);
+}
+
+__global__ __launch_bounds__(256) void k_bcoarse_factor(const CoarseDesc* D, double* lu, double* luT, int* status) {
+    const CoarseDesc d = D[blockIdx.x];
+    coarse_assemble_body(d.n, d.nf, d.l, d.u, d.rowBase, d.d + d.rowBase, d.cu, d.cl, lu + d.luOff);
+    coarse_lu_body(d.n, lu + d.luOff, luT + d.luOff, status + blockIdx.x);
+}
+
+void launch_bcoarse_factor(hipStream_t st, const CoarseDesc* D, int nSys, double* lu, double* luT, int* status) {
+    if (nSys > 0) hipLaunchKernelGGL(k_bcoarse_factor, dim3(nSys), dim3(256), 0, st, D, lu, luT, status);
+}
+
+// One workgroup per system; n ≤ 64 uses the one-wave register form, larger n the LDS
+// form (dynamic LDS of maxN doubles).
+template <class T>
+__global__ __launch_bounds__(256) void k_bcoarse_solve(const CoarseDesc* D, const double* lu, const double* luT,
+                                                       const T* const* rLev, T* const* xLev, const DevScalars* S) {
+    if (S[blockIdx.x].done) return;
+    const CoarseDesc d = D[blockIdx.x];
+    const T* r = rLev[d.level] + d.rowBase;
+    T* x = xLev[d.level] + d.rowBase;
+    if (d.n <= 64) {
+        if (threadIdx.x < WAVE) coarse_solve64_body(d.n, luT + d.luOff, r, x);
+    } else {
+        extern __shared__ double g[];
+        coarse_solve_body(d.n, lu + d.luOff, r, x, g);
+    }
+}
+
+template <class T>
+static void bcoarse_solve_t(hipStream_t st, const CoarseDesc* D, int nSys, int maxN, const double* lu,
+                            const double* luT, const T* const* rLev, T* const* xLev, const DevScalars* S) {
+    if (nSys == 0) return;
+    const size_t shm = maxN > 64 ? sizeof(double) * size_t(maxN) : 0;
+    hipLaunchKernelGGL(k_bcoarse_solve<T>, dim3(nSys), dim3(256), shm, st, D, lu, luT, rLev, xLev, S);
+}
+void launch_bcoarse_solve(hipStream_t st, const CoarseDesc* D, int nSys, int maxN, const double* lu,
+                          const double* luT, const double* const* rLev, double* const* xLev, const DevScalars* S) {
+    bcoarse_solve_t(st, D, nSys, maxN, lu, luT, rLev, xLev, S);
+}
+void launch_bcoarse_solve32(hipStream_t st, const CoarseDesc* D, int nSys, int maxN, const double* lu,
+                            const double* luT, const float* const* rLev, float* const* xLev, const DevScalars* S) {
+    bcoarse_solve_t(st, D, nSys, maxN, lu, luT, rLev, xLev, S);
 }
 
 // ============================================================================ V-cycle tail

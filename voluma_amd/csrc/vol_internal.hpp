@@ -146,6 +146,7 @@ enum VecOp : int {
     V_COPY = 13,          // a → w
     V_RECIP = 14,         // w = 1/a
     V_FROMZERO = 15,      // w = ω·(r/d)
+    V_FILL_XREF = 16,     // w = xRef of the row's system (batches)
 };
 
 // Vector element type T (double, or float for the inner solve of mixed precision);
@@ -310,5 +311,49 @@ void launch_fvm_term(hipStream_t st, int kind, const FvmArgs& a, const int* own,
 void launch_fvm_source(hipStream_t st, int n, const double* su, const double* V, double* source);
 // FvMatrix::solve fold (FvMatrix.h:154-163): d = diag + Σ ic, b = source + Σ bs per cell
 void launch_fvm_fold(hipStream_t st, const FvmArgs& a, double* dOut, double* bOut);
+
+// ------------------------------------------------------------------ batches of systems
+// Independent systems concatenated (cells, faces and every GAMG level). Vector work is
+// cut into chunks that never straddle two systems: chunk c = (system, first row, end
+// row); a segmented reduction leaves one partial per chunk and the per-system finalize
+// sums its chunks in order, so each system's arithmetic is independent of the others.
+constexpr int BATCH_CHUNK = 2048;  // rows per chunk
+struct BatchRed {
+    const int4* chunks = nullptr;  // x = system, y = first row, z = end row
+    int nChunks = 0;
+    const int* sysChunkOff = nullptr;  // system → its chunks [off[s], off[s+1])
+    int nSys = 0;
+    double* part = nullptr;            // [MAX_RED][maxChunks]
+    int maxChunks = 0;
+};
+// Coarsest level of one system: its rows and faces inside the concatenated level arrays.
+struct CoarseDesc {
+    int n = 0, nf = 0, rowBase = 0, level = 0;
+    size_t luOff = 0;
+    const int* l = nullptr;  // global rows of the level
+    const int* u = nullptr;
+    const double* d = nullptr;  // diagonal (global rows)
+    const double* cu = nullptr; // face coefficients (this system's faces)
+    const double* cl = nullptr;
+};
+// Vector op over all chunks (S = the systems' scalars; a system that is done is skipped).
+void launch_bvec(hipStream_t st, int op, const VecArgs& a, const BatchRed& R);
+void launch_bvec32(hipStream_t st, int op, const VecArgs32& a, const BatchRed& R);
+void launch_binit_scalars(hipStream_t st, DevScalars* S, int nSys, const int* n, double tol, double relTol,
+                          int maxIter, int minIter, int nSweeps, double innerRel);
+// Per-system scalar step after a segmented reduction (I: inner scalars of mixed precision).
+void launch_bfinalize(hipStream_t st, int op, int nred, const BatchRed& R, DevScalars* S, DevScalars* I);
+// nDone[0] = number of systems whose scalars say done.
+void launch_bcount(hipStream_t st, const DevScalars* S, int nSys, int* nDone);
+// Precision changes per chunk (see MixOp), skipped for systems that are done.
+void launch_bmix(hipStream_t st, int op, const BatchRed& R, const DevScalars* S, const double* a, float* w,
+                 float* w2, float* z, double* x, const float* e);
+// Dense assembly + no-pivot LU of every system's coarsest level (one workgroup each).
+void launch_bcoarse_factor(hipStream_t st, const CoarseDesc* D, int nSys, double* lu, double* luT, int* status);
+// Coarsest substitutions: r and x of system s live at rLev[D.level] + D.rowBase.
+void launch_bcoarse_solve(hipStream_t st, const CoarseDesc* D, int nSys, int maxN, const double* lu,
+                          const double* luT, const double* const* rLev, double* const* xLev, const DevScalars* S);
+void launch_bcoarse_solve32(hipStream_t st, const CoarseDesc* D, int nSys, int maxN, const double* lu,
+                            const double* luT, const float* const* rLev, float* const* xLev, const DevScalars* S);
 
 }  // namespace vol

@@ -204,3 +204,84 @@ def momentum_system(ctx, m, seed=1, comp=0, gamg=False):
     inp = momentum_inputs(m, seed)
     b = momentum_assemble(A, m, inp, comp)
     return lm, A, b, inp
+
+
+# ------------------------------------------------------------------ batches of systems
+def _exp(x):
+    """e^x by range reduction x = k·ln2 + r and a Horner polynomial (only +, ×, ldexp),
+    the same on every CPU."""
+    k = math.floor(x / math.log(2.0) + 0.5)
+    r = x - k * math.log(2.0)
+    s = 0.0
+    for j in range(24, -1, -1):
+        s = s * r + 1.0 / math.factorial(j)
+    return math.ldexp(s, int(k))
+
+
+def _cbrt(v):
+    y = max(1.0, v / 3.0)
+    for _ in range(200):  # Newton on y³ = v (division and products only)
+        y2 = (2.0 * y + v / (y * y)) / 3.0
+        if y2 == y:
+            break
+        y = y2
+    return y
+
+
+def batch_specs(n_systems=256, seed=4, n_min=5000, n_max=200000, median=30000, sigma=0.8):
+    """Seeded descriptions of the systems of a batch: lognormal cell counts (median
+    `median`, clipped to [n_min, n_max]; normal deviates from the sum of 12 uniforms),
+    box-hex or jittered-tet meshes with per-axis aspect factors in [1, 4) and one random
+    Dirichlet patch. Returns dicts (kind, nx, ny, nz, seed, patch)."""
+    u = splitmix_uniform(20 * n_systems, seed).reshape(n_systems, 20)
+    out = []
+    for s in range(n_systems):
+        z = float(sum(u[s, :12])) - 6.0
+        n = min(max(median * _exp(sigma * z), n_min), n_max)
+        kind = "tet" if u[s, 12] < 0.5 else "box"
+        fac = [1.0 + 3.0 * float(u[s, 13 + d]) for d in range(3)]
+        vol = n / (6.0 if kind == "tet" else 1.0)
+        base = _cbrt(vol / (fac[0] * fac[1] * fac[2]))
+        dims = [max(2, int(base * f + 0.5)) for f in fac]
+        cells = lambda d: d[0] * d[1] * d[2] * (6 if kind == "tet" else 1)
+        while cells(dims) < n_min:  # rounding may undershoot the floor
+            dims[dims.index(min(dims))] += 1
+        out.append({"kind": kind, "nx": dims[0], "ny": dims[1], "nz": dims[2], "seed": 1000 + s,
+                    "patch": int(u[s, 16] * 6.0)})
+    return out
+
+
+def batch_mesh(spec):
+    if spec["kind"] == "tet":
+        return _mesh.tet(spec["nx"], spec["ny"], spec["nz"], 0.2, spec["seed"])
+    return _mesh.box(spec["nx"], spec["ny"], spec["nz"])
+
+
+def batch_system(ctx, specs, rhs_seed=7):
+    """(LduBatch, b, meshes): the pressure Poisson system of every spec (fvm::laplacian(1,
+    p), fixedValue 0 on the spec's patch, zeroGradient elsewhere), assembled on the GPU
+    one system at a time and concatenated."""
+    import voluma_amd as va
+
+    meshes = [batch_mesh(sp) for sp in specs]
+    diag, upper, bs = [], [], []
+    for k, (sp, m) in enumerate(zip(specs, meshes)):
+        ni = m.n_internal
+        lm = va.LduMesh(ctx, m.n_cells, m.owner[:ni], m.neighbour, None)
+        lm.set_boundary(m.owner[ni:])
+        A = va.LduMatrix(lm)
+        g = m.geometry()
+        bc = np.zeros(m.n_boundary, np.int32)
+        s0 = int(m.patch_start[sp["patch"]]) - ni
+        bc[s0: s0 + int(m.patch_size[sp["patch"]])] = 1
+        b = A.assemble_laplacian(g["magSf"], g["ndc"], bc, b_in=pressure_rhs(m.n_cells, rhs_seed + k))
+        d, u, _ = A.get()
+        diag.append(d)
+        upper.append(u)
+        bs.append(b)
+        A.close()
+        lm.close()
+    systems = [(m.n_cells, m.owner[: m.n_internal], m.neighbour, m.agglomeration_weights()) for m in meshes]
+    B = va.LduBatch(ctx, systems)
+    B.set(np.concatenate(diag), np.concatenate(upper))
+    return B, np.concatenate(bs), meshes
