@@ -45,6 +45,7 @@ extern "C" {
 #define VOL_ERR_UNSUPPORTED (-6)            /* valid in the reference, not provided on the GPU */
 #define VOL_ERR_HIP (-7)                    /* HIP runtime failure */
 #define VOL_ERR_NO_GAMG_WEIGHTS (-8)        /* GAMG requested on a mesh created without weights */
+#define VOL_ERR_TIMEOUT (-9)                /* vol_server_wait: not finished in time (no reference counterpart) */
 
 /* ---- memory location of caller buffers */
 #define VOL_HOST 0
@@ -70,6 +71,7 @@ typedef struct vol_context_s* vol_context;
 typedef struct vol_mesh_s* vol_mesh;
 typedef struct vol_matrix_s* vol_matrix;
 typedef struct vol_batch_s* vol_batch;
+typedef struct vol_server_s* vol_server;
 
 /* SolverControls (LinearSolver.h:15-23) plus GPU extensions. Strings as in the
  * reference: solver PCG | PBiCGStab | GaussSeidel | symGaussSeidel | Jacobi;
@@ -239,6 +241,39 @@ int vol_batch_info(vol_batch batch, int64_t* nCellsTotal, int64_t* nFacesTotal, 
 int vol_batch_set(vol_batch batch, const double* diag, const double* upper, const double* lower, int location);
 int vol_batch_solve(vol_batch batch, const double* b, double* x, const vol_solver_controls* controls,
                     vol_solver_performance* perf, int location);
+
+/* ---- solve server (no reference counterpart): a request loop that keeps meshes, their
+ * GAMG hierarchies and matrices resident and runs requests concurrently. `slots`
+ * worker threads each own a context (HIP stream + scratch); a request is admitted, in
+ * arrival order, when a worker is free and its device memory (a per-worker copy of the
+ * mesh and a matrix, cached and evicted least recently used first) fits hbmBudgetBytes.
+ * A request names a mesh registered with vol_server_add_mesh and carries one matrix and
+ * nRhs right-hand sides (e.g. PISO correctors) solved one after another on that matrix
+ * with `controls`. All request buffers are host memory owned by the caller and must
+ * stay valid until vol_server_wait returns for the ticket. A request's result does not
+ * depend on what else runs. */
+typedef struct {
+    int mesh;                        /* id from vol_server_add_mesh */
+    const double* diag;              /* boundary-folded diagonal (nCells) */
+    const double* upper;             /* nInternalFaces */
+    const double* lower;             /* nullable: symmetric */
+    int nRhs;                        /* 1..8 */
+    const double* const* b;          /* nRhs right-hand sides */
+    double* const* x;                /* nRhs initial guesses, overwritten by the solutions */
+    vol_solver_controls controls;
+    vol_solver_performance* perf;    /* nullable, nRhs entries */
+} vol_request;
+int vol_server_create(int device, int slots, uint64_t hbmBudgetBytes, vol_server* out);
+/* registers a mesh (builds its GAMG hierarchy once); boundaryOwner nullable */
+int vol_server_add_mesh(vol_server s, int nCells, int nInternalFaces, const int32_t* owner, const int32_t* neighbour,
+                        const double* gamgWeights, const int32_t* boundaryOwner, int nBoundaryFaces, int* meshId);
+int vol_server_submit(vol_server s, const vol_request* request, int64_t* ticket);   /* returns at once */
+/* blocks until the request is done (VOL_ERR_TIMEOUT after timeoutSeconds); returns the
+ * request's own status, also in *requestStatus (nullable); times (nullable, 3 doubles):
+ * when it was queued, started and finished, in seconds on the server's steady clock */
+int vol_server_wait(vol_server s, int64_t ticket, double timeoutSeconds, int* requestStatus, double* times);
+int vol_server_stats(vol_server s, uint64_t* bytesInUse, int* queued, int* cachedEntries);
+int vol_server_destroy(vol_server s); /* stops the workers after their current request */
 
 /* ---- measurement hooks (no reference counterpart): time every level-0 A·p
  * product issued by vol_solve with HIP events on the context stream. */

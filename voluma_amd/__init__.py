@@ -30,7 +30,7 @@ FP64, MIXED = 0, 1
 
 _ERRORS = {
     -1: "invalid argument", -2: "unknown solver", -3: "unknown preconditioner", -4: "PCG on nonsymmetric matrix",
-    -5: "singular coarsest matrix", -6: "unsupported", -7: "HIP error", -8: "no GAMG weights",
+    -5: "singular coarsest matrix", -6: "unsupported", -7: "HIP error", -8: "no GAMG weights", -9: "timeout",
 }
 
 
@@ -62,6 +62,12 @@ class _Perf(ctypes.Structure):
                 ("outerIterations", ctypes.c_int), ("reserved", ctypes.c_int)]
 
 
+class _Request(ctypes.Structure):
+    _fields_ = [("mesh", ctypes.c_int), ("diag", ctypes.c_void_p), ("upper", ctypes.c_void_p),
+                ("lower", ctypes.c_void_p), ("nRhs", ctypes.c_int), ("b", ctypes.c_void_p), ("x", ctypes.c_void_p),
+                ("controls", _Controls), ("perf", ctypes.c_void_p)]
+
+
 def _f64(a):
     return np.ascontiguousarray(a, dtype=np.float64)
 
@@ -82,7 +88,9 @@ EXPORTED = ("vol_controls_default", "vol_last_error", "vol_context_create", "vol
             "vol_solve", "vol_timing_enable", "vol_timing_read", "vol_mesh_set_boundary", "vol_assemble_laplacian",
             "vol_assemble_div_upwind", "vol_matrix_get", "vol_device_copy", "vol_fvm_reset", "vol_fvm_ddt_euler",
             "vol_fvm_div_upwind", "vol_fvm_laplacian", "vol_fvm_source", "vol_fvm_get", "vol_fvm_finish",
-            "vol_batch_create", "vol_batch_destroy", "vol_batch_info", "vol_batch_set", "vol_batch_solve")
+            "vol_batch_create", "vol_batch_destroy", "vol_batch_info", "vol_batch_set", "vol_batch_solve",
+            "vol_server_create", "vol_server_add_mesh", "vol_server_submit", "vol_server_wait", "vol_server_stats",
+            "vol_server_destroy")
 
 
 def lib():
@@ -131,6 +139,12 @@ def lib():
         L.vol_batch_info.argtypes = [P, P, P, P]
         L.vol_batch_set.argtypes = [P, P, P, P, I]
         L.vol_batch_solve.argtypes = [P, P, P, ctypes.POINTER(_Controls), P, I]
+        L.vol_server_create.argtypes = [I, I, ctypes.c_uint64, ctypes.POINTER(P)]
+        L.vol_server_add_mesh.argtypes = [P, I, I, P, P, P, P, I, ctypes.POINTER(I)]
+        L.vol_server_submit.argtypes = [P, ctypes.POINTER(_Request), ctypes.POINTER(ctypes.c_int64)]
+        L.vol_server_wait.argtypes = [P, ctypes.c_int64, D, ctypes.POINTER(I), P]
+        L.vol_server_stats.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(I), ctypes.POINTER(I)]
+        L.vol_server_destroy.argtypes = [P]
         for name in EXPORTED:
             if name not in ("vol_controls_default", "vol_last_error"):
                 getattr(L, name).restype = I
@@ -502,6 +516,77 @@ class LduBatch:
     def close(self):
         if getattr(self, "_h", None):
             lib().vol_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()
+
+
+class Server:
+    """Solve server (vol_server): `slots` workers, each with its own HIP stream, run
+    admitted requests concurrently; meshes (and their GAMG hierarchies) are registered
+    once and kept resident, a request's matrix is reused by its right-hand sides."""
+
+    def __init__(self, device=0, slots=4, hbm_budget=200e9):
+        h = ctypes.c_void_p()
+        _check(lib().vol_server_create(int(device), int(slots), ctypes.c_uint64(int(hbm_budget)), ctypes.byref(h)))
+        self._h = h
+        self._live = {}
+
+    def add_mesh(self, n_cells, owner, neighbour, gamg_weights=None, boundary_owner=None):
+        own = np.ascontiguousarray(owner, dtype=np.int32)
+        nei = np.ascontiguousarray(neighbour, dtype=np.int32)
+        w = None if gamg_weights is None else _f64(gamg_weights)
+        bo = None if boundary_owner is None else np.ascontiguousarray(boundary_owner, dtype=np.int32)
+        mid = ctypes.c_int()
+        _check(lib().vol_server_add_mesh(self._h, int(n_cells), int(own.size), _p(own), _p(nei), _q(w), _q(bo),
+                                         0 if bo is None else int(bo.size), ctypes.byref(mid)))
+        return mid.value
+
+    def submit(self, mesh_id, diag, upper, lower, bs, x0s=None, controls: SolverControls | None = None):
+        """Queue a request; returns a ticket. bs: list of right-hand sides."""
+        d, u = _f64(diag), _f64(upper)
+        lo = None if lower is None else _f64(lower)
+        bs = [_f64(b) for b in bs]
+        xs = [np.zeros(b.size) if x0s is None else np.array(x0s[k], dtype=np.float64) for k, b in enumerate(bs)]
+        ctl = controls or SolverControls()
+        c = ctl._c()
+        bp = (ctypes.c_void_p * len(bs))(*[b.ctypes.data for b in bs])
+        xp = (ctypes.c_void_p * len(xs))(*[x.ctypes.data for x in xs])
+        pf = (_Perf * len(bs))()
+        r = _Request(int(mesh_id), d.ctypes.data, u.ctypes.data, None if lo is None else lo.ctypes.data, len(bs),
+                     ctypes.cast(bp, ctypes.c_void_p), ctypes.cast(xp, ctypes.c_void_p), c,
+                     ctypes.cast(pf, ctypes.c_void_p))
+        t = ctypes.c_int64()
+        _check(lib().vol_server_submit(self._h, ctypes.byref(r), ctypes.byref(t)))
+        self._live[t.value] = (d, u, lo, bs, xs, ctl, bp, xp, pf, r)  # alive until wait()
+        return t.value
+
+    def wait(self, ticket, timeout=600.0):
+        """(solutions, [SolverPerformance]) of a request; raises VolumaError if it failed."""
+        st = ctypes.c_int()
+        tm = np.zeros(3)
+        try:
+            _check(lib().vol_server_wait(self._h, ctypes.c_int64(ticket), ctypes.c_double(timeout), ctypes.byref(st),
+                                         _p(tm)))
+        except VolumaError as e:
+            if e.code != -9:
+                self._live.pop(ticket, None)
+            raise
+        keep = self._live.pop(ticket)
+        xs, pf = keep[4], keep[8]
+        self.last_times = {"queued": tm[0], "started": tm[1], "finished": tm[2]}
+        return xs, [SolverPerformance(p.solver.decode(), "", p.initialResidual, p.finalResidual, p.residualSum,
+                                      p.normFactor, p.iterations, bool(p.converged), p.outerIterations) for p in pf]
+
+    def stats(self):
+        b, q, c = ctypes.c_uint64(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().vol_server_stats(self._h, ctypes.byref(b), ctypes.byref(q), ctypes.byref(c)))
+        return {"bytes_in_use": b.value, "queued": q.value, "cached": c.value}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            lib().vol_server_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -99,6 +99,8 @@ namespace {
 void setDevice(vol_context ctx) { HIPCHECK(hipSetDevice(ctx->device)); }
 
 // ------------------------------------------------------------------ mesh setup
+void uploadLevels(vol_mesh m);
+
 void setupMesh(vol_mesh m, const double* w, int mergeLevels) {
     hipStream_t st = m->ctx->stream;
     const int n = m->n, nf = m->nf;
@@ -125,6 +127,13 @@ void setupMesh(vol_mesh m, const double* w, int mergeLevels) {
     if (!w) return;
     m->hasGamg = true;
     m->H = buildHierarchy(n, nf, m->own.data(), m->nei.data(), w, mergeLevels);
+    uploadLevels(m);
+}
+
+// Device structures of the GAMG levels of m->H (SELL/CSR rows, transfer lists).
+void uploadLevels(vol_mesh m) {
+    hipStream_t st = m->ctx->stream;
+    const int nf = m->nf;
     const int nL = int(m->H.size());
     std::vector<char> oil(nf);
     for (int f = 0; f < nf; ++f) oil[f] = char(m->own[f] < m->nei[f]);
@@ -964,6 +973,35 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
 
 // Handles keep their parents alive (matrix → mesh → context), so callers may destroy
 // them in any order; the last release frees.
+// Copy of mesh src for context dst: same addressing, hierarchy and boundary lists,
+// device structures built for dst (the host hierarchy is reused, not recomputed).
+vol_mesh meshCloneTo(vol_mesh src, vol_context dst) {
+    setDevice(dst);
+    auto m = std::make_unique<vol_mesh_s>();
+    m->ctx = dst;
+    m->n = src->n;
+    m->nf = src->nf;
+    m->own = src->own;
+    m->nei = src->nei;
+    setupMesh(m.get(), nullptr, 2);
+    if (src->hasGamg) {
+        m->hasGamg = true;
+        m->H = src->H;
+        uploadLevels(m.get());
+    }
+    if (src->nb >= 0) {
+        std::vector<int> off(src->n + 1), list(src->nb);
+        HIPCHECK(hipMemcpy(off.data(), src->bOff.p, sizeof(int) * off.size(), hipMemcpyDeviceToHost));
+        if (src->nb) HIPCHECK(hipMemcpy(list.data(), src->bList.p, sizeof(int) * list.size(), hipMemcpyDeviceToHost));
+        m->bOff.upload(off, dst->stream);
+        m->bList.upload(list, dst->stream);
+        m->nb = src->nb;
+    }
+    HIPCHECK(hipStreamSynchronize(dst->stream));
+    dst->refs.fetch_add(1);
+    return m.release();
+}
+
 void ctxRelease(vol_context c) {
     if (c->refs.fetch_sub(1) != 1) return;
     (void)hipSetDevice(c->device);

@@ -396,3 +396,5 @@ struct vol_context_s {
 
 // Drops one reference of a context and frees it with the last one (api.cpp).
 void ctxRelease(vol_context c);
+// Copy of a mesh (addressing, GAMG hierarchy, boundary lists) for another context (api.cpp).
+vol_mesh meshCloneTo(vol_mesh src, vol_context dst);

@@ -1202,12 +1202,12 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(TailArgsT<T> t) {
 
 template <class T>
 static void launch_tail_t(hipStream_t st, const TailArgsT<T>& t) {
-    static bool attr = false;
-    if (!attr) {
+    static const bool attr = [] {  // once per process (thread-safe initialisation)
         (void)hipFuncSetAttribute((const void*)k_vcycle_tail<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                   TAIL_LDS_BYTES);
-        attr = true;
-    }
+        return true;
+    }();
+    (void)attr;
     hipLaunchKernelGGL(k_vcycle_tail<T>, dim3(1), dim3(TAIL_THREADS), t.ldsBytes, st, t);
 }
 
