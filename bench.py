@@ -101,6 +101,38 @@ def batch_leg(va, W, ctx, args):
     return out
 
 
+def serve_leg(va, ctx, args):
+    """Solve server: 4 workers (own HIP streams), 200 GB HBM budget, a pool of 8 meshes
+    with log-uniform sizes 10k–2M cells; Poisson arrivals at --serve-rate of 60 %
+    pressure (GAMG-PCG) / 40 % momentum (PBiCGStab) requests with 2–3 right-hand sides
+    each, to 1e-6, FP64 and mixed. Also the saturated throughput (all arrivals at once)."""
+    from voluma_amd import serving as SV
+
+    t0 = time.perf_counter()
+    pool = SV.build_pool(ctx, SV.pool_specs())
+    server = va.Server(0, slots=4, hbm_budget=200e9)
+    SV.register_pool(server, pool)
+    setup_s = time.perf_counter() - t0
+    out = {"workers": 4, "hbm_budget_gb": 200, "pool_cells": [e["n"] for e in pool], "setup_s": setup_s,
+           "rate_per_s": args.serve_rate, "requests": args.serve_requests}
+    warm = SV.request_stream(pool, rate=1e6, n_requests=3 * len(pool), seed=1)
+    for prec in (va.FP64, va.MIXED):
+        SV.run_stream(server, pool, warm, prec)  # every worker/mesh pair resident, kernels warm
+    for name, prec in (("fp64", va.FP64), ("mixed", va.MIXED)):
+        stream = SV.request_stream(pool, rate=args.serve_rate, n_requests=args.serve_requests, seed=9)
+        res, st = SV.run_stream(server, pool, stream, prec)
+        if not all(p.converged for _, perf in res for p in perf):
+            raise RuntimeError(f"serve {name}: a solve did not converge")
+        sat = SV.request_stream(pool, rate=1e6, n_requests=min(120, args.serve_requests), seed=10)
+        _, st_sat = SV.run_stream(server, pool, sat, prec)
+        st["saturated_solves_per_s"] = st_sat["solves_per_s"]
+        out[name] = st
+        log(f"serve {name}: p50 {st['latency_ms']['p50']:.1f} ms, p99 {st['latency_ms']['p99']:.1f} ms at "
+            f"{args.serve_rate:g} req/s; {st['solves_per_s']:.0f} solves/s; saturated {st_sat['solves_per_s']:.0f}")
+    server.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -112,9 +144,13 @@ def main():
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-systems", type=int, default=256)
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-rate", type=float, default=40.0, help="request arrivals per second")
+    ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--pre", type=int, default=1)
     ap.add_argument("--post", type=int, default=1)
     ap.add_argument("--omega", type=float, default=0.9)
+    ap.add_argument("--inner-rel", type=float, default=1e-2, help="mixed precision: inner residual reduction")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; multi-GPU runs are not part of it",
@@ -190,7 +226,7 @@ def main():
     mixed = None
     if not args.no_mixed:
         log("mixed precision ...")
-        mctl = dataclasses.replace(ctl, precision=va.MIXED)
+        mctl = dataclasses.replace(ctl, precision=va.MIXED, innerRelTol=args.inner_rel)
         _, _, _, pref, xref = run(dataclasses.replace(ctl, tolerance=1e-12, maxIter=3000), 1)
         nref = np.linalg.norm(xref)
         run(mctl, args.warmup)
@@ -224,6 +260,9 @@ def main():
     batch = None
     if not args.no_batch:
         batch = batch_leg(va, W, ctx, args)
+    serve = None
+    if not args.no_serve:
+        serve = serve_leg(va, ctx, args)
 
     out = {
         "metric": METRIC,
@@ -245,6 +284,7 @@ def main():
                        "min": float(lat_ms.min())},
         "mixed": mixed,
         "batch": batch,
+        "serve": serve,
         "iterations": int(np.median(iters)),
         "final_residual": p.finalResidual,
         "true_residual": true_res,

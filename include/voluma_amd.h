@@ -63,8 +63,10 @@ extern "C" {
  * r = b − A·x in FP64, solves A·e = r with PCG/PBiCGStab whose Amul, GAMG smoothing,
  * restriction/prolongation and vectors are FP32 (dot products, norms and the coarsest
  * LU solve in FP64), until the inner residual falls by innerRelTol, then x += e.
- * Convergence is judged on the FP64 residual with the usual normalisation. Fast sums
- * only; no counterpart in the reference. */
+ * PCG keeps its search direction across the outer steps (a reliable update: only the
+ * recursive residual is replaced by the FP64 one); PBiCGStab restarts. Convergence is
+ * judged on the FP64 residual with the usual normalisation. Fast sums only; no
+ * counterpart in the reference. */
 #define VOL_MIXED 1
 
 typedef struct vol_context_s* vol_context;
@@ -91,7 +93,7 @@ typedef struct {
     int gamgPreSweeps;  /* damped-Jacobi GAMG smoother sweeps (default 2, ≥ 1) */
     int gamgPostSweeps; /* default 2, ≥ 1 */
     double gamgOmega;   /* damping (default 0.8) */
-    double innerRelTol; /* VOL_MIXED: residual reduction per inner solve (default 1e-4, in (0, 1)) */
+    double innerRelTol; /* VOL_MIXED: residual reduction per inner solve (default 1e-2, in (0, 1)) */
 } vol_solver_controls;
 
 /* SolverPerformance (LinearSolver.h:25-31) */

@@ -149,6 +149,35 @@ def test_gamg_exact_mode_bit_exact_vs_oracle(ctx, name, sweeps):
 
 
 @pytest.mark.parametrize("name", NAMES)
+@pytest.mark.parametrize("sweeps", [(2, 2), (1, 1)])
+def test_tail_kernel_bit_exact_vs_oracle(name, sweeps, monkeypatch):
+    """The one-workgroup kernel for the small GAMG levels (off by default, enabled by
+    VOLUMA_TAIL_MAX_CELLS) gives the same bits as the oracle, FP64 exact mode, and the
+    same mixed-precision result as the per-level kernels."""
+    monkeypatch.setenv("VOLUMA_TAIL_MAX_CELLS", "512")
+    ct = va.Context(0)
+    g, m, lm, A, b = _system(ct, name)
+    om = O.LduMesh(m.n_cells, m.owner[: m.n_internal], m.neighbour, m.agglomeration_weights())
+    pre, post = sweeps
+    xo, po = om.solve(g["sym_diag"], g["sym_upper"], None, b, preconditioner="GAMG", tolerance=1e-10, max_iter=200,
+                      mode=O.MODE_EXACT, smoother=O.SMOOTHER_JACOBI, omega=0.8, n_pre=pre, n_post=post)
+    x = np.zeros(b.size)
+    p = va.solve_ldu(A, b, x, va.SolverControls("PCG", "GAMG", 1e-10, 0.0, 200, sums=va.SUMS_EXACT,
+                                                gamgPreSweeps=pre, gamgPostSweeps=post, gamgOmega=0.8))
+    assert p.iterations == po["iterations"] and np.array_equal(x, xo)
+    ctl = va.SolverControls("PCG", "GAMG", 1e-10, 0.0, 200, gamgPreSweeps=pre, gamgPostSweeps=post,
+                            precision=va.MIXED)
+    xm = np.zeros(b.size)
+    pm = va.solve_ldu(A, b, xm, ctl)
+    monkeypatch.setenv("VOLUMA_TAIL_MAX_CELLS", "0")
+    c0 = va.Context(0)
+    _, _, _, A0, _ = _system(c0, name)
+    xm0 = np.zeros(b.size)
+    pm0 = va.solve_ldu(A0, b, xm0, ctl)
+    assert pm.iterations == pm0.iterations and np.array_equal(xm, xm0)
+
+
+@pytest.mark.parametrize("name", NAMES)
 def test_gamg_solution_vs_reference_gamg(ctx, name):
     """GPU GAMG-PCG (Jacobi smoother) vs the reference GAMG-PCG (Gauss–Seidel): both at
     normalised residual 1e-13 agree within 1e-10 relative."""

@@ -176,7 +176,7 @@ class SolverControls:
     gamgPreSweeps: int = 2
     gamgPostSweeps: int = 2
     gamgOmega: float = 0.8
-    innerRelTol: float = 1e-4
+    innerRelTol: float = 1e-2
 
     def _c(self):
         self._keep = (self.solver.encode(), self.preconditioner.encode())

@@ -71,7 +71,7 @@ struct vol_matrix_s {
     DevBuf<double> lu, luT;
     int nCoarse = 0;
     int tailStart = -1;  // first GAMG level handled by the tail kernel (−1: none)
-    int tailLdsBytes = 0;
+    int tailLdsBytes = 0, tailLdsBytes32 = 0;
     DevBuf<TailLevel> tail;
     DevBuf<int> status;
     // FP32 copies for mixed precision, built lazily after each vol_matrix_set
@@ -123,7 +123,7 @@ void setupMesh(vol_mesh m, const double* w, int mergeLevels) {
             src[k] = (f << 1) | 1;
         }
     }
-    buildSell(n, off, col, src, m->sell, st);
+    buildSell(n, off, col, src, m->sell, st, true);
     if (!w) return;
     m->hasGamg = true;
     m->H = buildHierarchy(n, nf, m->own.data(), m->nei.data(), w, mergeLevels);
@@ -164,6 +164,57 @@ void uploadLevels(vol_mesh m) {
 }
 
 // ------------------------------------------------------------------ matrix / GAMG coefficients
+// LDS layout of the tail kernel: [coarsest LU or substitution vector (doubles)]
+// [r, two work buffers, x per level (T)] [per level, when it fits: row offsets, columns,
+// coefficients, diagonal, 1/diagonal, agg, restriction lists]. Returns the bytes used.
+template <class T> int tailLayout(std::vector<TailLevelT<T>>& V, int coarseN) {
+    const size_t coarseBytes = size_t(coarseN <= 64 ? coarseN * coarseN : coarseN) * sizeof(double);
+    size_t off = coarseBytes / sizeof(T);  // in elements of T
+    for (auto& t : V) {
+        t.ldsR = int(off);
+        t.ldsA = int(off + t.S.n);
+        t.ldsB = int(off + 2 * size_t(t.S.n));
+        t.ldsX = int(off + 3 * size_t(t.S.n));
+        off += 4 * size_t(t.S.n);
+    }
+    size_t bytes = off * sizeof(T);
+    auto al = [](size_t b) { return (b + 15) & ~size_t(15); };
+    for (size_t k = 0; k + 1 < V.size(); ++k) {
+        TailLevelT<T>& t = V[k];
+        const size_t n = t.S.n, ns = t.S.nslots, nr = t.S.csrG > 0 ? n + 1 : size_t(t.S.nslices) + 1;
+        size_t o = al(bytes);
+        const size_t oRow = o;
+        o = al(o + 4 * nr);
+        const size_t oCol = o;
+        o = al(o + 4 * ns);
+        const size_t oCoef = o;
+        o = al(o + sizeof(T) * ns);
+        const size_t oDiag = o;
+        o = al(o + sizeof(T) * n);
+        const size_t oRD = o;
+        o = al(o + sizeof(T) * n);
+        const size_t oAgg = o;
+        o = al(o + (t.agg ? 4 * n : 0));
+        const size_t oMOff = o;
+        o = al(o + (t.mOff ? 4 * (size_t(t.nNext) + 1) : 0));
+        const size_t oMList = o;
+        o = al(o + (t.mList ? 4 * n : 0));
+        if (o > size_t(TAIL_LDS_BYTES)) continue;  // this level's matrix stays in global memory
+        t.ldsMat = 1;
+        t.nRowOff = int(nr);
+        t.ldsRowOff = int(oRow);
+        t.ldsCol = int(oCol);
+        t.ldsCoef = int(oCoef);
+        t.ldsDiag = int(oDiag);
+        t.ldsRD = int(oRD);
+        t.ldsAgg = int(oAgg);
+        t.ldsMOff = int(oMOff);
+        t.ldsMList = int(oMList);
+        bytes = o;
+    }
+    return int(bytes);
+}
+
 void buildGamgCoeffs(vol_matrix A) {
     vol_mesh m = A->mesh;
     hipStream_t st = m->ctx->stream;
@@ -226,8 +277,6 @@ void buildGamgCoeffs(vol_matrix A) {
         }
     if (A->tailStart >= 0) {
         std::vector<TailLevel> T;
-        const int coarseDoubles = DL.n <= 64 ? DL.n * DL.n : DL.n;
-        int off = coarseDoubles;  // in doubles
         for (int k = A->tailStart; k < nL; ++k) {
             const LevelDev& D = *m->lv[k];
             MatLevel& M = *A->L[k];
@@ -242,14 +291,9 @@ void buildGamgCoeffs(vol_matrix A) {
                 t.mList = D.mList.p;
                 t.nNext = m->lv[k + 1]->n;
             }
-            t.ldsR = off;
-            t.ldsA = off + D.n;
-            t.ldsB = off + 2 * D.n;
-            t.ldsX = off + 3 * D.n;
-            off += 4 * D.n;
             T.push_back(t);
         }
-        A->tailLdsBytes = int(ldsNeed(A->tailStart));
+        A->tailLdsBytes = tailLayout(T, DL.n);
         A->tail.upload(T, st);
     }
     int status = 0;
@@ -295,8 +339,6 @@ void buildMixed(vol_matrix A, bool gamg) {
         }
         if (A->tailStart >= 0) {
             const LevelDev& DL = *m->lv[nL - 1];
-            const int coarseDoubles = DL.n <= 64 ? DL.n * DL.n : DL.n;
-            int off = 2 * coarseDoubles;  // in floats, after the FP64 coarsest section
             std::vector<TailLevel32> T;
             for (int k = A->tailStart; k < nL; ++k) {
                 const LevelDev& D = *m->lv[k];
@@ -312,13 +354,9 @@ void buildMixed(vol_matrix A, bool gamg) {
                     t.mList = D.mList.p;
                     t.nNext = m->lv[k + 1]->n;
                 }
-                t.ldsR = off;
-                t.ldsA = off + D.n;
-                t.ldsB = off + 2 * D.n;
-                t.ldsX = off + 3 * D.n;
-                off += 4 * D.n;
                 T.push_back(t);
             }
+            A->tailLdsBytes32 = tailLayout(T, DL.n);
             A->tail32.upload(T, st);
         }
     }
@@ -645,10 +683,7 @@ void tailLaunch(const Ctx& C, vol_matrix A, const Plan& P, int l, const float* r
     t.nCoarse = A->nCoarse;
     t.lu = A->lu.p;
     t.luT = A->luT.p;
-    const int nc = A->nCoarse;
-    size_t bytes = size_t(nc <= 64 ? nc * nc : nc) * sizeof(double);
-    for (int k = l; k < nL; ++k) bytes += 4 * size_t(A->mesh->lv[k]->n) * sizeof(float);
-    t.ldsBytes = int(bytes);
+    t.ldsBytes = A->tailLdsBytes32;
     t.nPre = P.nPre;
     t.nPost = P.nPost;
     t.omega = P.omega;
@@ -669,8 +704,11 @@ void restrictLevel(const Ctx& C, int nc, const LevelDev& D, const float* res, fl
 }
 
 // One V-cycle at level l: r → x (GAMG.cpp:403-443 structure, damped-Jacobi smoothing).
+// With part != nullptr (level 0 outside the tail), the last post-sweep also leaves the
+// per-block partials of Σ x·dotv in part.
 template <class T>
-void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xout) {
+void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xout, const T* dotv = nullptr,
+            double* part = nullptr) {
     vol_mesh m = A->mesh;
     const int nL = int(m->lv.size());
     if (l == A->tailStart) {
@@ -702,13 +740,15 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
     // prolongation fused into the first post-sweep
     int src = cur, other = freeB;
     for (int s = 0; s < P.nPost; ++s) {
-        T* dst = (s == P.nPost - 1) ? xout : bufs[other];
+        const bool last = s == P.nPost - 1;
+        T* dst = last ? xout : bufs[other];
+        const T* dv = last ? dotv : nullptr;
+        double* pt = last ? part : nullptr;
         if (s == 0)
             sell<T>(C, OP_JACOBI, P.nPre == 1 ? XL_ZEROPROLONG : XL_PROLONG, V, L.coef, L.diag,
-                    P.nPre == 1 ? nullptr : bufs[src], r, dst, P.omega, LC.x, D.agg.p, nullptr, nullptr, nullptr,
-                    L.rD);
+                    P.nPre == 1 ? nullptr : bufs[src], r, dst, P.omega, LC.x, D.agg.p, dv, pt, nullptr, L.rD);
         else
-            sell<T>(C, OP_JACOBI, XL_PLAIN, V, L.coef, L.diag, bufs[src], r, dst, P.omega);
+            sell<T>(C, OP_JACOBI, XL_PLAIN, V, L.coef, L.diag, bufs[src], r, dst, P.omega, nullptr, nullptr, dv, pt);
         if (s != P.nPost - 1) {
             const int t = src;
             src = other;
@@ -754,14 +794,19 @@ void krylovIteration(const Ctx& C, vol_matrix A, const Plan& P, const Krylov<T>&
     const int sellParts = sell_grid_blocks(A->mesh->sell.nslices);
     const Level0<T> L0 = level0(A, (T*)nullptr);
     if (isPCG) {  // LinearSolver.cpp:176-201
-        if (P.precond == "GAMG") {
-            vcycle<T>(C, A, P, 0, k.r, k.w);
-            vec<T>(C, V_DOT, n, ex, k.w, k.r);
+        if (P.precond == "GAMG" && fuse && A->tailStart != 0) {  // Σ w·r fused into the last sweep
+            vcycle<T>(C, A, P, 0, k.r, k.w, k.r, C.c->red.part);
+            fin(C, FIN_PCG_WRA, 1, false, sellParts);
         } else {
-            vec<T>(C, V_PRECOND_DOT, n, ex, k.r, nullptr, nullptr, nullptr, k.w, nullptr, nullptr,
-                   P.precond == "diagonal" ? L0.rD : nullptr);
+            if (P.precond == "GAMG") {
+                vcycle<T>(C, A, P, 0, k.r, k.w);
+                vec<T>(C, V_DOT, n, ex, k.w, k.r);
+            } else {
+                vec<T>(C, V_PRECOND_DOT, n, ex, k.r, nullptr, nullptr, nullptr, k.w, nullptr, nullptr,
+                       P.precond == "diagonal" ? L0.rD : nullptr);
+            }
+            fin(C, FIN_PCG_WRA, 1, ex, nparts);
         }
-        fin(C, FIN_PCG_WRA, 1, ex, nparts);
         vec<T>(C, V_PCG_P, n, ex, k.w, nullptr, nullptr, nullptr, nullptr, k.p);
         amulTimed<T>(C, A, k.p, k.q, k.p, fuse);
         if (fuse) {
@@ -883,7 +928,7 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
     A->pA.alloc(n);
     A->vr.alloc(n);
     launch_init_scalars(st, C.S, ct.tolerance, ct.relTol, ct.maxIter, ct.minIter, ct.nSweeps, n,
-                        mixed ? ct.innerRelTol : 0.0);
+                        mixed ? ct.innerRelTol : 0.0, mixed && isPCG ? 1 : 0);
     // initial residual and normalisation (LinearSolver.cpp:150-169)
     sell<double>(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, x, nullptr, A->wA.p);
     const int nparts = ex ? RED_BLOCKS_EXACT : RED_BLOCKS;
@@ -1038,7 +1083,7 @@ void vol_controls_default(vol_solver_controls* c) {
     c->gamgPreSweeps = 2;
     c->gamgPostSweeps = 2;
     c->gamgOmega = 0.8;
-    c->innerRelTol = 1e-4;
+    c->innerRelTol = 1e-2;
 }
 
 const char* vol_last_error(void) { return gLastError.c_str(); }
@@ -1054,7 +1099,7 @@ int vol_context_create(int device, vol_context* out) {
     HIPCHECK(hipSetDevice(device));
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     if (const char* e = std::getenv("CFD_REPRODUCIBLE")) c->reproducible = std::string(e) != "0" && std::string(e) != "";
-    if (const char* e = std::getenv("VOLUMA_TAIL_MAX_CELLS")) c->tailMax = std::atoi(e);
+    if (const char* e = std::getenv("VOLUMA_TAIL_MAX_CELLS")) c->tailMax = std::min(std::atoi(e), TAIL_MAX_CELLS);
     c->scal.alloc(2);
     HIPCHECK(hipHostMalloc((void**)&c->hostScal, 2 * sizeof(DevScalars), hipHostMallocDefault));
     c->ensureScratch(RED_BLOCKS);

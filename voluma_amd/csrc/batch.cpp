@@ -151,7 +151,7 @@ void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const doubl
                     col[k] = o;
                     src[k] = (gf << 1) | 1;
                 }
-            buildSell(Lv.nSmooth, off, col, src, Lv.sell, st);
+            buildSell(Lv.nSmooth, off, col, src, Lv.sell, st, true);
             std::vector<char> oil(B->nfTotal);
             for (int s = 0; s < nS; ++s)
                 for (int f = 0; f < B->nf[s]; ++f)
@@ -199,6 +199,7 @@ void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const doubl
             G.upload(Lv, st);
             agg.resize(Lv.nSmooth);
             Lv.agg.upload(agg, st);
+
         }
         // coefficient and work storage
         if (l > 0) {
@@ -556,7 +557,7 @@ void bsolve(vol_batch B, const double* b, double* x, const vol_solver_controls& 
     if (mixed && !B->mixedReady) buildMixedB(B);
     DevScalars* S = B->S.p;
     launch_binit_scalars(st, S, B->nSys, B->sysN.p, ct.tolerance, ct.relTol, ct.maxIter, ct.minIter, ct.nSweeps,
-                         mixed ? ct.innerRelTol : 0.0);
+                         mixed ? ct.innerRelTol : 0.0, mixed ? 1 : 0);
     for (DevBuf<double>* v : {&B->vr, &B->vw, &B->vp, &B->vq, &B->wA, &B->pA}) v->alloc(n);
     const BLevel& L0 = *B->L[0];
     const SellView V = L0.sell.view();

@@ -57,8 +57,9 @@ template <class T> struct DevBuf {
 };
 
 struct SellDev {
-    int n = 0, nslices = 0, nslots = 0, maxWidth = 0, csrG = 0, csrIt = 0;
+    int n = 0, nslices = 0, nslots = 0, maxWidth = 0, csrG = 0, csrIt = 0, ellWidth = 0;
     DevBuf<int> sliceOff, col, src;
+    DevBuf<int16_t> col16;
     SellView view() const {
         SellView v;
         v.n = n;
@@ -70,13 +71,16 @@ struct SellDev {
         v.sliceOff = sliceOff.p;
         v.col = col.p;
         v.src = src.p;
+        v.col16 = col16.p;
+        v.ellWidth = ellWidth;
         return v;
     }
 };
 
-// Row lists (row → (col, src) in the given order) → SELL-64 on the device.
+// Row lists (row → (col, src) in the given order) → SELL-64 on the device. With
+// try16, int16 column offsets are added when every neighbour lies within ±32767 rows.
 inline void buildSell(int n, const std::vector<int>& off, const std::vector<int>& colL, const std::vector<int>& srcL,
-               SellDev& S, hipStream_t st) {
+               SellDev& S, hipStream_t st, bool try16 = false) {
     S.n = n;
     S.nslices = (n + WAVE - 1) / WAVE;
     std::vector<int> sliceOff(S.nslices + 1, 0);
@@ -85,6 +89,11 @@ inline void buildSell(int n, const std::vector<int>& off, const std::vector<int>
         for (int r = s * WAVE; r < std::min(n, (s + 1) * WAVE); ++r) w = std::max(w, off[r + 1] - off[r]);
         sliceOff[s + 1] = sliceOff[s] + w * WAVE;
         S.maxWidth = std::max(S.maxWidth, w);
+    }
+    // ELL (every slice at the longest row's width) when it costs at most 10 % more slots
+    if (size_t(S.nslices) * WAVE * S.maxWidth <= size_t(sliceOff[S.nslices]) * 11 / 10) {
+        S.ellWidth = S.maxWidth;
+        for (int s = 0; s <= S.nslices; ++s) sliceOff[s] = s * WAVE * S.maxWidth;
     }
     S.nslots = sliceOff[S.nslices];
     std::vector<int> col(S.nslots, -1), src(S.nslots, -1);
@@ -99,6 +108,22 @@ inline void buildSell(int n, const std::vector<int>& off, const std::vector<int>
     S.sliceOff.upload(sliceOff, st);
     S.col.upload(col, st);
     S.src.upload(src, st);
+    if (try16 && std::getenv("VOLUMA_COL16") && std::string(std::getenv("VOLUMA_COL16")) == "1") {
+        std::vector<int16_t> c16(S.nslots, int16_t(COL16_PAD));
+        bool fits = true;
+        for (int r = 0; r < n && fits; ++r) {
+            const int s = r / WAVE, lane = r % WAVE;
+            for (int k = off[r]; k < off[r + 1]; ++k) {
+                const int d = colL[k] - r;
+                if (d <= COL16_PAD || d > 32767) {
+                    fits = false;
+                    break;
+                }
+                c16[sliceOff[s] + (k - off[r]) * WAVE + lane] = int16_t(d);
+            }
+        }
+        if (fits) S.col16.upload(c16, st);
+    }
 }
 
 // Row lists → CSR with G lanes per row (coarse levels): G·IT ≥ longest row, IT ≤ 8.
@@ -361,7 +386,9 @@ struct vol_context_s {
     int device = 0;
     hipStream_t stream = nullptr;
     bool reproducible = false;
-    int tailMax = TAIL_MAX_CELLS;  // GAMG levels this small run in the one-workgroup tail kernel
+    // GAMG levels this small run in the one-workgroup tail kernel (VOLUMA_TAIL_MAX_CELLS;
+    // 0 = off, the default: on MI355X separate launches for the small levels measured faster)
+    int tailMax = 0;
     RedScratch red;
     DevBuf<double> part, result;
     DevBuf<long long> xpart;

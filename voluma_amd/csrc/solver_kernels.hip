@@ -35,6 +35,7 @@ __device__ __forceinline__ T xload(const SellArgsT<T>& a, int j, T xu) {
     return T(0);
 }
 
+
 // One row of a SELL gather op (the arithmetic shared by the grid kernel and the
 // one-workgroup coarse-level kernel). Slots are taken in groups of U: all column and
 // coefficient loads of a group, then all x gathers, are in flight together, and the
@@ -47,6 +48,10 @@ __device__ __forceinline__ T sell_row(const SellArgsT<T>& a, int row, T xu) {
         s0 = a.S.sliceOff[row];
         width = a.S.sliceOff[row + 1] - s0;
         stride = 1;
+    } else if (a.S.ellWidth > 0) {
+        width = a.S.ellWidth;
+        s0 = (row / WAVE) * WAVE * width + row % WAVE;
+        stride = WAVE;
     } else {
         const int slice = row / WAVE;
         const int b = a.S.sliceOff[slice];
@@ -65,7 +70,12 @@ __device__ __forceinline__ T sell_row(const SellArgsT<T>& a, int row, T xu) {
         for (int u = 0; u < U; ++u) {
             const int idx = s0 + (k + u) * stride;
             const bool in = k + u < width;
-            j[u] = in ? a.S.col[idx] : -1;
+            if (a.S.col16) {
+                const int d = in ? int(a.S.col16[idx]) : COL16_PAD;
+                j[u] = d == COL16_PAD ? -1 : row + d;
+            } else {
+                j[u] = in ? a.S.col[idx] : -1;
+            }
             av[u] = in ? a.coef[idx] : T(0);
         }
 #pragma unroll
@@ -199,8 +209,8 @@ static void sell_dispatch2(hipStream_t st, const SellArgsT<T>& a) {
     const int nb = sell_grid_blocks(a.S.nslices);
     if (nb == 0) return;
     const bool wide = a.S.maxWidth > 8;
-    if (a.part) {
-        if constexpr (OP == OP_AMUL) {
+    if (a.part) {  // fused Σ out·dotv: A·p, and the last sweep of the V-cycle (Σ w·r)
+        if constexpr (OP == OP_AMUL || (OP == OP_JACOBI && XL != XL_UNIFORM && XL != XL_FROMZERO)) {
             if (wide)
                 hipLaunchKernelGGL((k_sell<OP, XL, true, 16, T>), dim3(nb), dim3(256), 0, st, a);
             else
@@ -463,6 +473,7 @@ void launch_mix(hipStream_t st, int op, int n, const double* a, float* w, float*
 }
 
 // ============================================================================ finalize
+constexpr int FIN_THREADS = 1024;
 // One block: reduce the per-block partials in a fixed order (fast) or sum the exact
 // bins (exact), then run the solver's scalar step on thread 0.
 __device__ __forceinline__ bool dev_converged(const DevScalars& S) {
@@ -541,12 +552,13 @@ __device__ void fin_step(int op, DevScalars& S, const DevScalars& I, double v0, 
         S.finalResidual = v0 / S.normFactor;
         if (!(S.iterations < S.maxIter && !dev_converged(S))) S.done = 1;
         break;
-    case FIN_OUTER: {  // I holds the inner solve that just ended
-        S.iterations += I.iterations;
+    case FIN_OUTER: {  // I holds the inner solve segment that just ended
+        const int seg = I.iterations - I.itStart;
+        S.iterations += seg;
         ++S.outer;
         S.finalResidual = v0 / S.normFactor;
-        // stop on convergence, the iteration budget, or an inner solve that made no progress
-        if (!(S.iterations < S.maxIter && !dev_converged(S)) || I.iterations == 0) S.done = 1;
+        // stop on convergence, the iteration budget, or an inner segment that made no progress
+        if (!(S.iterations < S.maxIter && !dev_converged(S)) || seg == 0) S.done = 1;
         break;
     }
     default:
@@ -557,6 +569,15 @@ __device__ void fin_step(int op, DevScalars& S, const DevScalars& I, double v0, 
 // Mixed precision: I ← a fresh inner solve for the current outer residual of S, or done.
 __device__ void inner_start(const DevScalars& S, DevScalars& I) {
     const double target = fmax(S.tolerance, S.relTol > 0 ? S.relTol * S.initialResidual : 0.0);
+    if (S.keepKrylov && S.outer > 0 && !S.done && !I.exitPending) {
+        // reliable update: the recurrence goes on from the replaced (true) residual with
+        // the same search direction and ρ; only the segment's stopping point is reset
+        I.tolerance = fmax(S.innerRel * S.finalResidual, 0.5 * target);
+        I.itStart = I.iterations;
+        I.maxIter = I.iterations + (S.maxIter - S.iterations);
+        I.done = 0;
+        return;
+    }
     DevScalars z = {};
     z.normFactor = S.normFactor;
     z.tolerance = fmax(S.innerRel * S.finalResidual, 0.5 * target);
@@ -570,7 +591,7 @@ __device__ void inner_start(const DevScalars& S, DevScalars& I) {
     I = z;
 }
 
-__global__ __launch_bounds__(256) void k_finalize(int op, int nred, int exact, int nparts, const double* part,
+__global__ __launch_bounds__(FIN_THREADS) void k_finalize(int op, int nred, int exact, int nparts, const double* part,
                                                   int maxBlk, const long long* xpart, const int* xnf, int maxBlkExact,
                                                   DevScalars* Sp, double* result) {
     DevScalars& S = *Sp;
@@ -580,13 +601,24 @@ __global__ __launch_bounds__(256) void k_finalize(int op, int nred, int exact, i
     }
     if (op != FIN_STORE && S.done) return;
     __shared__ double vals[MAX_RED];
-    __shared__ double sh[256 / WAVE];
+    __shared__ double sh[FIN_THREADS / WAVE];
     __shared__ long long bins[NBINS];
     __shared__ int nfl;
     for (int q = 0; q < nred; ++q) {
         if (!exact) {
+            // eight partials in flight per thread, summed in a fixed order
             double s = 0.0;
-            for (int k = threadIdx.x; k < nparts; k += blockDim.x) s += part[size_t(q) * maxBlk + k];
+            const double* pq = part + size_t(q) * maxBlk;
+            for (int k0 = threadIdx.x; k0 < nparts; k0 += 8 * blockDim.x) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int k = k0 + u * blockDim.x;
+                    v[u] = k < nparts ? pq[k] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s += v[u];
+            }
             const double t = block_sum(s, sh);
             if (threadIdx.x == 0) vals[q] = t;
         } else {
@@ -619,14 +651,15 @@ __global__ __launch_bounds__(256) void k_finalize(int op, int nred, int exact, i
 }
 
 void launch_finalize(hipStream_t st, int op, int nred, bool exact, int nparts, const RedScratch& R, DevScalars* S) {
-    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(256), 0, st, op, nred, exact ? 1 : 0, nparts, R.part, R.maxBlk,
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(FIN_THREADS), 0, st, op, nred, exact ? 1 : 0, nparts, R.part, R.maxBlk,
                        R.xpart, R.xnf, R.maxBlkExact, S, R.result);
 }
 
 __global__ void k_init_scalars(DevScalars* S, double tol, double relTol, int maxIter, int minIter, int nSweeps, int n,
-                               double innerRel) {
+                               double innerRel, int keepKrylov) {
     DevScalars z = {};
     z.innerRel = innerRel;
+    z.keepKrylov = keepKrylov;
     z.tolerance = tol;
     z.relTol = relTol;
     z.maxIter = maxIter;
@@ -638,11 +671,12 @@ __global__ void k_init_scalars(DevScalars* S, double tol, double relTol, int max
 }
 
 __global__ void k_binit_scalars(DevScalars* S, int nSys, const int* n, double tol, double relTol, int maxIter,
-                                int minIter, int nSweeps, double innerRel) {
+                                int minIter, int nSweeps, double innerRel, int keepKrylov) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= nSys) return;
     DevScalars z = {};
     z.innerRel = innerRel;
+    z.keepKrylov = keepKrylov;
     z.tolerance = tol;
     z.relTol = relTol;
     z.maxIter = maxIter;
@@ -654,9 +688,9 @@ __global__ void k_binit_scalars(DevScalars* S, int nSys, const int* n, double to
 }
 
 void launch_binit_scalars(hipStream_t st, DevScalars* S, int nSys, const int* n, double tol, double relTol,
-                          int maxIter, int minIter, int nSweeps, double innerRel) {
+                          int maxIter, int minIter, int nSweeps, double innerRel, int keepKrylov) {
     hipLaunchKernelGGL(k_binit_scalars, dim3((nSys + 63) / 64), dim3(64), 0, st, S, nSys, n, tol, relTol, maxIter,
-                       minIter, nSweeps, innerRel);
+                       minIter, nSweeps, innerRel, keepKrylov);
 }
 
 __global__ void k_clear_exit(DevScalars* S) { S->exitPending = 0; }
@@ -664,9 +698,9 @@ __global__ void k_clear_exit(DevScalars* S) { S->exitPending = 0; }
 void launch_clear_exit(hipStream_t st, DevScalars* S) { hipLaunchKernelGGL(k_clear_exit, dim3(1), dim3(1), 0, st, S); }
 
 void launch_init_scalars(hipStream_t st, DevScalars* S, double tol, double relTol, int maxIter, int minIter,
-                         int nSweeps, int n, double innerRel) {
+                         int nSweeps, int n, double innerRel, int keepKrylov) {
     hipLaunchKernelGGL(k_init_scalars, dim3(1), dim3(1), 0, st, S, tol, relTol, maxIter, minIter, nSweeps, n,
-                       innerRel);
+                       innerRel, keepKrylov);
 }
 
 // ============================================================================ GAMG
@@ -1075,20 +1109,72 @@ __device__ __forceinline__ void tail_phase(const SellArgsT<T>& a) {
 // LDS holds [coarsest LU (n ≤ 64) or substitution vector, doubles] then the tail levels'
 // vectors (r, two work buffers, x) in the level precision T. The coarsest solve runs in
 // FP64 whatever T is.
+// A tail level with its matrix pointers redirected into LDS when it was preloaded.
+template <class T>
+__device__ __forceinline__ TailLevelT<T> tail_level(const TailLevelT<T>& G, char* base) {
+    TailLevelT<T> L = G;
+    if (L.ldsMat) {
+        L.S.sliceOff = reinterpret_cast<const int*>(base + L.ldsRowOff);
+        L.S.col = reinterpret_cast<const int*>(base + L.ldsCol);
+        L.S.col16 = nullptr;
+        L.coef = reinterpret_cast<const T*>(base + L.ldsCoef);
+        L.diag = reinterpret_cast<const T*>(base + L.ldsDiag);
+        L.rD = reinterpret_cast<const T*>(base + L.ldsRD);
+        if (L.agg) L.agg = reinterpret_cast<const int*>(base + L.ldsAgg);
+        if (L.mOff) L.mOff = reinterpret_cast<const int*>(base + L.ldsMOff);
+        if (L.mList) L.mList = reinterpret_cast<const int*>(base + L.ldsMList);
+    }
+    return L;
+}
+
+// Global → LDS with eight loads in flight per thread before the stores.
+template <class V>
+__device__ __forceinline__ void lds_copy(char* base, int off, const V* src, int n) {
+    V* dst = reinterpret_cast<V*>(base + off);
+    constexpr int UNR = 8;
+    for (int i0 = threadIdx.x; i0 < n; i0 += UNR * blockDim.x) {
+        V v[UNR];
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int i = i0 + u * blockDim.x;
+            v[u] = i < n ? src[i] : V(0);
+        }
+#pragma unroll
+        for (int u = 0; u < UNR; ++u) {
+            const int i = i0 + u * blockDim.x;
+            if (i < n) dst[i] = v[u];
+        }
+    }
+}
+
 template <class T>
 __global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(TailArgsT<T> t) {
     if (t.done && *t.done) return;
     extern __shared__ double lds[];
     T* ldsT = reinterpret_cast<T*>(lds);
+    char* base = reinterpret_cast<char*>(lds);
     const int nl = t.nLevels;
     const int cur = pre_smoothed_buffer(t.nPre), freeB = 1 - cur;
     {
         const TailLevelT<T>& L0 = t.L[0];
         for (int i = threadIdx.x; i < L0.S.n; i += blockDim.x) ldsT[L0.ldsR + i] = t.rTop[i];
+        // matrices into LDS: every later phase then reads LDS only
+        for (int l = 0; l + 1 < nl; ++l) {
+            const TailLevelT<T>& G = t.L[l];
+            if (!G.ldsMat) continue;
+            lds_copy(base, G.ldsRowOff, G.S.sliceOff, G.nRowOff);
+            lds_copy(base, G.ldsCol, G.S.col, G.S.nslots);
+            lds_copy(base, G.ldsCoef, G.coef, G.S.nslots);
+            lds_copy(base, G.ldsDiag, G.diag, G.S.n);
+            lds_copy(base, G.ldsRD, G.rD, G.S.n);
+            if (G.agg) lds_copy(base, G.ldsAgg, G.agg, G.S.n);
+            if (G.mOff) lds_copy(base, G.ldsMOff, G.mOff, G.nNext + 1);
+            if (G.mList) lds_copy(base, G.ldsMList, G.mList, G.S.n);
+        }
         __syncthreads();
     }
     for (int l = 0; l + 1 < nl; ++l) {
-        const TailLevelT<T>& L = t.L[l];
+        const TailLevelT<T> L = tail_level(t.L[l], base);
         SellArgsT<T> a;
         a.S = L.S;
         a.coef = L.coef;
@@ -1167,7 +1253,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(TailArgsT<T> t) {
         __syncthreads();
     }
     for (int l = nl - 2; l >= 0; --l) {
-        const TailLevelT<T>& L = t.L[l];
+        const TailLevelT<T> L = tail_level(t.L[l], base);
         SellArgsT<T> a;
         a.S = L.S;
         a.coef = L.coef;

@@ -30,7 +30,15 @@ struct SellView {
     const int* sliceOff = nullptr;
     const int* col = nullptr;  // neighbour row, −1 = padding
     const int* src = nullptr;  // (face << 1) | side: side 0 → upper[face], 1 → lower[face]; −1 padding
+    // SELL levels whose column offsets (neighbour − row) all fit int16 also carry them
+    // as int16 (COL16_PAD = padding); the gather kernels then read 2 bytes per slot
+    // instead of 4. Null otherwise.
+    const int16_t* col16 = nullptr;
+    // ELL form (ellWidth > 0): every slice has ellWidth slots, so slot k of row r sits at
+    // (r/64)·64·ellWidth + 64k + r%64 and no slice offset has to be loaded first.
+    int ellWidth = 0;
 };
+constexpr int COL16_PAD = -32768;
 
 // ------------------------------------------------------------------ device scalars
 // Krylov scalars and loop control of one solve, kept on the device so that the
@@ -47,7 +55,8 @@ struct DevScalars {
     int exitPending;  // PBiCGStab exit on s: x += αy still to apply
     int status;
     int outer;        // mixed precision: defect-correction steps taken
-    int pad[2];
+    int itStart;      // mixed, inner scalars: iteration count when the current segment began
+    int keepKrylov;   // mixed, outer scalars: inner PCG keeps its search direction across steps
 };
 // Mixed precision keeps two DevScalars side by side: [0] the FP64 outer loop (the
 // reported performance), [1] the current FP32 inner Krylov solve.
@@ -178,7 +187,7 @@ int vec_has_reduction(int op);
 
 void launch_clear_exit(hipStream_t st, DevScalars* S);
 void launch_init_scalars(hipStream_t st, DevScalars* S, double tol, double relTol, int maxIter, int minIter,
-                         int nSweeps, int n, double innerRel);
+                         int nSweeps, int n, double innerRel, int keepKrylov = 0);
 
 // ------------------------------------------------------------------ GAMG kernels
 void launch_restrict(hipStream_t st, int nc, const int* mOff, const int* mList, const double* res, double* rc,
@@ -218,6 +227,12 @@ template <class T> struct TailLevelT {
     const int* mList = nullptr;
     int nNext = 0;
     int ldsR = 0, ldsA = 0, ldsB = 0, ldsX = 0;  // offsets in elements of T
+    // matrix data copied into LDS at kernel start when it fits (ldsMat = 1): byte
+    // offsets of row offsets, columns, coefficients, diagonal, 1/diagonal, agg and the
+    // restriction lists
+    int ldsMat = 0;
+    int nRowOff = 0;
+    int ldsRowOff = 0, ldsCol = 0, ldsCoef = 0, ldsDiag = 0, ldsRD = 0, ldsAgg = 0, ldsMOff = 0, ldsMList = 0;
 };
 template <class T> struct TailArgsT {
     int nLevels = 0;                  // levels in the tail, the last is the coarsest
@@ -227,7 +242,7 @@ template <class T> struct TailArgsT {
     int nCoarse = 0;
     const double* lu = nullptr;       // coarsest LU factors, always FP64
     const double* luT = nullptr;
-    int ldsBytes = 0;                 // LDS: [coarsest LU / substitution vector (doubles)] [level vectors (T)]
+    int ldsBytes = 0;                 // LDS: [coarsest LU / substitution vector (doubles)] [level vectors (T)] [matrices]
     int nPre = 2, nPost = 2;
     double omega = 0.8;
     const int* done = nullptr;
@@ -340,7 +355,7 @@ struct CoarseDesc {
 void launch_bvec(hipStream_t st, int op, const VecArgs& a, const BatchRed& R);
 void launch_bvec32(hipStream_t st, int op, const VecArgs32& a, const BatchRed& R);
 void launch_binit_scalars(hipStream_t st, DevScalars* S, int nSys, const int* n, double tol, double relTol,
-                          int maxIter, int minIter, int nSweeps, double innerRel);
+                          int maxIter, int minIter, int nSweeps, double innerRel, int keepKrylov = 0);
 // Per-system scalar step after a segmented reduction (I: inner scalars of mixed precision).
 void launch_bfinalize(hipStream_t st, int op, int nred, const BatchRed& R, DevScalars* S, DevScalars* I);
 // nDone[0] = number of systems whose scalars say done.

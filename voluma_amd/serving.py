@@ -49,7 +49,7 @@ def build_pool(ctx, specs):
         g = m.geometry()
         A.assemble_laplacian(g["magSf"], g["ndc"], W.bc_fixed_xmax(m))
         pd, pu, _ = A.get()
-        inp = W.momentum_inputs(m, sp["seed"])
+        inp = W.momentum_inputs(m, sp["seed"], exact=False)
         mb = [W.momentum_assemble(A, m, inp, c) for c in range(3)]
         md, mu, ml = A.get()
         pool.append({"mesh": m, "n": m.n_cells, "p_diag": pd, "p_upper": pu, "m_diag": md, "m_upper": mu,

@@ -103,7 +103,8 @@ class FourierVelocity:
     Everything is evaluated with elementwise +, × and fixed-order sums, so the inputs
     are bit-identical on every machine."""
 
-    def __init__(self, seed, n_modes=12):
+    def __init__(self, seed, n_modes=12, exact=True):
+        self.exact = exact
         u = splitmix_uniform(7 * n_modes, seed).reshape(n_modes, 7)
         k = np.floor(u[:, :3] * 5.0) - 2.0
         k[np.all(k == 0, axis=1), 0] = 1.0
@@ -133,12 +134,16 @@ class FourierVelocity:
                 out[:, d] = out[:, d] + w[:, j] * vec[j, d]
         return out
 
+    def _sc(self, ph):
+        # exact: machine-independent bits; otherwise the (faster) library sin/cos
+        return _sincos(ph) if self.exact else (np.sin(ph), np.cos(ph))
+
     def A(self, x):
-        s, _ = _sincos(self._phase(x))
+        s, _ = self._sc(self._phase(x))
         return self.scale * self._modes(s, self.a)
 
     def U(self, x):  # ∇×(a sin(2πk·x + θ)) = 2π cos(·) k×a
-        _, c = _sincos(self._phase(x))
+        _, c = self._sc(self._phase(x))
         return self.scale * (2.0 * math.pi) * self._modes(c, self.kxa)
 
     def face_flux(self, m):
@@ -159,13 +164,14 @@ class FourierVelocity:
         return phi
 
 
-def momentum_inputs(m, seed=1):
+def momentum_inputs(m, seed=1, exact=True):
     """Inputs of the momentum matrix on mesh m: face flux φ, the cell velocity U⁰ at
     the cell centres (n×3), Dirichlet boundary values (nb×3, U at the boundary face
     centres) and dt for Courant number 0.5 with Co = ½ Σ|φ|/V·dt
-    (IncompressibleFlow.cpp:97-107)."""
+    (IncompressibleFlow.cpp:97-107). exact=False evaluates sin/cos with numpy (faster,
+    last-bit results may differ between machines)."""
     g = m.geometry()
-    fv = FourierVelocity(seed)
+    fv = FourierVelocity(seed, exact=exact)
     phi = fv.face_flux(m)
     U0 = fv.U(g["C"])
     Ub = fv.U(g["Cf"][m.n_internal:])
