@@ -148,8 +148,7 @@ void uploadLevels(vol_mesh m) {
         if (k > 0) {  // SELL rows in adjacency (face) order: A(i, j) = upper if i is the l side
             std::vector<int> srcL(F.adjFace.size());
             for (size_t q = 0; q < srcL.size(); ++q) srcL[q] = (F.adjFace[q] << 1) | (F.adjIsL[q] ? 0 : 1);
-            if (F.n > CSR_MAX_ROWS || !buildCsr(F.n, F.off, F.adjCell, srcL, D->sell, st))
-                buildSell(F.n, F.off, F.adjCell, srcL, D->sell, st);
+            buildCoarseLayout(F.n, F.off, F.adjCell, srcL, D->sell, st);
         }
         if (k + 1 < nL) {
             const HLevel& C = m->H[k + 1];
@@ -624,11 +623,19 @@ void amulTimed(const Ctx& C, vol_matrix A, const T* p, T* q, const T* dotv, bool
         }
         e0 = c->evPool[c->evUsed++];
         e1 = c->evPool[c->evUsed++];
-        HIPCHECK(hipEventRecord(e0, C.st));
     }
-    sell<T>(C, OP_AMUL, XL_PLAIN, V, L0.coef, L0.diag, p, nullptr, q, 1.0, nullptr, nullptr, fused ? dotv : nullptr,
-            fused ? c->red.part : nullptr);
-    if (c->timing) HIPCHECK(hipEventRecord(e1, C.st));
+    SellArgsT<T> a;  // the events bracket the kernel itself (recorded by the launch)
+    a.S = V;
+    a.coef = L0.coef;
+    a.diag = L0.diag;
+    a.x = p;
+    a.out = q;
+    a.dotv = fused ? dotv : nullptr;
+    a.part = fused ? c->red.part : nullptr;
+    a.done = C.done;
+    a.evStart = e0;
+    a.evStop = e1;
+    launchSell(C.st, OP_AMUL, XL_PLAIN, a);
 }
 
 void collectTiming(vol_context c) {

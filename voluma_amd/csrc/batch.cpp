@@ -18,6 +18,8 @@ using namespace vol::host;
 struct BLevel {
     int nRows = 0, nSmooth = 0, nFaces = 0, nFacesSmooth = 0;
     SellDev sell;                         // rows [0, nSmooth)
+    DevBuf<int2> sliceSys;                // SELL: first/last system of each slice
+    DevBuf<int> rowSys;                   // CSR: system of each row
     DevBuf<int> agg;                      // smoothing rows → next level (global)
     DevBuf<int> mOff, mList, iOff, iList, cfOff, cfList;
     DevBuf<int> l, u;                     // faces (global rows)
@@ -177,8 +179,25 @@ void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const doubl
                     }
                 }
             }
-            if (Lv.nSmooth > CSR_MAX_ROWS || !buildCsr(Lv.nSmooth, off, col, src, Lv.sell, st))
-                buildSell(Lv.nSmooth, off, col, src, Lv.sell, st);
+            buildCoarseLayout(Lv.nSmooth, off, col, src, Lv.sell, st);
+        }
+        // system of every smoothing row, for skipping finished systems
+        if (Lv.nSmooth > 0) {
+            std::vector<int> sysOf(Lv.nSmooth);
+            for (int s : order[l]) {
+                if (B->nLev[s] - 1 <= l) continue;
+                for (int i = 0; i < H[s][l].n; ++i) sysOf[rowOff[l][s] + i] = s;
+            }
+            if (Lv.sell.csrG > 0) {
+                Lv.rowSys.upload(sysOf, st);
+            } else {
+                std::vector<int2> ss(Lv.sell.nslices);
+                for (int q = 0; q < Lv.sell.nslices; ++q) {
+                    const int r0 = q * WAVE, r1 = std::min(Lv.nSmooth, r0 + WAVE) - 1;
+                    ss[q] = make_int2(std::min(sysOf[r0], sysOf[r1]), std::max(sysOf[r0], sysOf[r1]));
+                }
+                Lv.sliceSys.upload(ss, st);
+            }
         }
         // transfer to level l+1
         if (l + 1 < LM && Lv.nSmooth > 0) {
@@ -371,8 +390,14 @@ void bsellLaunch(hipStream_t st, int op, int xl, const SellArgs32& a) { launch_s
 
 template <class T>
 void bsell(hipStream_t st, int op, int xl, const SellView& V, const T* coef, const T* diag, const T* rD, const T* x,
-           const T* r, T* out, double omega = 1.0, const T* xc = nullptr, const int* agg = nullptr) {
+           const T* r, T* out, double omega = 1.0, const T* xc = nullptr, const int* agg = nullptr,
+           const BLevel* skip = nullptr, const DevScalars* S = nullptr) {
     SellArgsT<T> a;
+    if (skip && S) {
+        a.sliceSys = skip->sliceSys.p;
+        a.rowSys = skip->rowSys.p;
+        a.sysS = S;
+    }
     a.S = V;
     a.coef = coef;
     a.diag = diag;
@@ -419,12 +444,12 @@ void bvcycle(vol_batch B, const BPlan& P, const T* r0, T* x0, const DevScalars* 
         for (int s = 1; s < P.nPre; ++s) {
             const int dst = (s == 1) ? 0 : 1 - cur;
             bsell<T>(st, OP_JACOBI, s == 1 ? XL_FROMZERO : XL_PLAIN, V, p.coef, p.diag, p.rD,
-                     s == 1 ? nullptr : bufs[cur], r, bufs[dst], P.omega);
+                     s == 1 ? nullptr : bufs[cur], r, bufs[dst], P.omega, nullptr, nullptr, &Lv, S);
             cur = dst;
         }
         const int freeB = 1 - cur;
         bsell<T>(st, OP_RESID, P.nPre == 1 ? XL_FROMZERO : XL_PLAIN, V, p.coef, p.diag, p.rD,
-                 P.nPre == 1 ? nullptr : bufs[cur], r, bufs[freeB], P.omega);
+                 P.nPre == 1 ? nullptr : bufs[cur], r, bufs[freeB], P.omega, nullptr, nullptr, &Lv, S);
         const BLevelPtrs<T> pc = blevel(B, l + 1, (T*)nullptr);
         brestrict(st, B->L[l + 1]->nRows, Lv, bufs[freeB], pc.r);
         curL[l] = cur;
@@ -444,9 +469,10 @@ void bvcycle(vol_batch B, const BPlan& P, const T* r0, T* x0, const DevScalars* 
             T* dst = (s == P.nPost - 1) ? xout : bufs[other];
             if (s == 0)
                 bsell<T>(st, OP_JACOBI, P.nPre == 1 ? XL_ZEROPROLONG : XL_PROLONG, V, p.coef, p.diag, p.rD,
-                         P.nPre == 1 ? nullptr : bufs[src], r, dst, P.omega, pc.x, Lv.agg.p);
+                         P.nPre == 1 ? nullptr : bufs[src], r, dst, P.omega, pc.x, Lv.agg.p, &Lv, S);
             else
-                bsell<T>(st, OP_JACOBI, XL_PLAIN, V, p.coef, p.diag, p.rD, bufs[src], r, dst, P.omega);
+                bsell<T>(st, OP_JACOBI, XL_PLAIN, V, p.coef, p.diag, p.rD, bufs[src], r, dst, P.omega, nullptr,
+                         nullptr, &Lv, S);
             if (s != P.nPost - 1) std::swap(src, other);
         }
     }
@@ -489,7 +515,8 @@ void bpcgIteration(vol_batch B, const BPlan& P, const BKrylov<T>& k, DevScalars*
     }
     launch_bfinalize(st, FIN_PCG_WRA, 1, B->red, S, nullptr);
     bvec<T>(B, V_PCG_P, S, k.w, nullptr, nullptr, nullptr, k.p);
-    bsell<T>(st, OP_AMUL, XL_PLAIN, B->L[0]->sell.view(), p0.coef, p0.diag, nullptr, k.p, nullptr, k.q);
+    bsell<T>(st, OP_AMUL, XL_PLAIN, B->L[0]->sell.view(), p0.coef, p0.diag, nullptr, k.p, nullptr, k.q, 1.0, nullptr,
+             nullptr, B->L[0].get(), S);
     bvec<T>(B, V_DOT, S, k.q, k.p);
     launch_bfinalize(st, FIN_PCG_WAPA, 1, B->red, S, nullptr);
     bvec<T>(B, V_PCG_XR, S, k.p, k.q, nullptr, nullptr, k.x, k.r);
@@ -583,7 +610,8 @@ void bsolve(vol_batch B, const double* b, double* x, const vol_solver_controls& 
             for (int outer = 0; outer < maxLoops; ++outer) {
                 bloop(B, maxLoops, [&] { bpcgIteration<float>(B, P, k, SI); });
                 launch_bmix(st, MIX_ADD, B->red, S, nullptr, nullptr, nullptr, nullptr, x, B->ie.p);
-                bsell<double>(st, OP_AMUL, XL_PLAIN, V, L0.coef.p, B->diag.p, nullptr, x, nullptr, B->wA.p);
+                bsell<double>(st, OP_AMUL, XL_PLAIN, V, L0.coef.p, B->diag.p, nullptr, x, nullptr, B->wA.p, 1.0,
+                              nullptr, nullptr, &L0, S);
                 bvec<double>(B, V_BMINUSAX_NORM, S, B->wA.p, b, nullptr, B->vr.p);
                 launch_bfinalize(st, FIN_OUTER, 1, B->red, S, SI);
                 launch_bfinalize(st, FIN_INNER_START, 0, B->red, S, SI);

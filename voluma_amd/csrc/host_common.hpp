@@ -148,6 +148,75 @@ inline bool buildCsr(int n, const std::vector<int>& off, const std::vector<int>&
     return true;
 }
 
+// Time the two sweeps a 1/1 V-cycle runs on a level (residual of the first sweep from
+// zero; first post-sweep with prolongation) over layout S on stream st, with the caches
+// flushed before each launch (in a V-cycle the finer levels have evicted the level's
+// data): median of three, seconds. Values are zero; agg maps four rows to one.
+inline double timeLayout(const SellDev& S, hipStream_t st, void* flush, size_t flushBytes) {
+    DevBuf<double> coef, v;
+    DevBuf<int> agg;
+    const size_t n = S.n, nc = n / 4 + 1;
+    coef.alloc(S.nslots);
+    v.alloc(5 * n + nc);
+    HIPCHECK(hipMemsetAsync(coef.p, 0, sizeof(double) * std::max<size_t>(S.nslots, 1), st));
+    HIPCHECK(hipMemsetAsync(v.p, 0, sizeof(double) * (5 * n + nc), st));
+    std::vector<int> ag(n);
+    for (size_t i = 0; i < n; ++i) ag[i] = int(i / 4);
+    agg.upload(ag, st);
+    SellArgs a;
+    a.S = S.view();
+    a.coef = coef.p;
+    a.diag = v.p;
+    a.rD = v.p + n;
+    a.r = v.p + 2 * n;
+    a.x = v.p + 3 * n;
+    a.out = v.p + 4 * n;
+    a.xc = v.p + 5 * n;
+    a.agg = agg.p;
+    a.omega = 0.9;
+    hipEvent_t e[4];
+    for (auto& x : e) HIPCHECK(hipEventCreate(&x));
+    float t[3];
+    for (int k = 0; k < 3; ++k) {
+        float t1 = 0, t2 = 0;
+        HIPCHECK(hipMemsetAsync(flush, k + 1, flushBytes, st));
+        a.evStart = e[0];
+        a.evStop = e[1];
+        launch_sell(st, OP_RESID, XL_FROMZERO, a);
+        HIPCHECK(hipMemsetAsync(flush, k + 2, flushBytes, st));
+        a.evStart = e[2];
+        a.evStop = e[3];
+        launch_sell(st, OP_JACOBI, XL_ZEROPROLONG, a);
+        HIPCHECK(hipEventSynchronize(e[3]));
+        HIPCHECK(hipEventElapsedTime(&t1, e[0], e[1]));
+        HIPCHECK(hipEventElapsedTime(&t2, e[2], e[3]));
+        t[k] = t1 + t2;
+    }
+    for (auto& x : e) (void)hipEventDestroy(x);
+    std::sort(t, t + 3);
+    return t[1] / 1e3;
+}
+
+// Layout of a coarse GAMG level: the CSR-vector form (G lanes per row) or SELL-64,
+// whichever sweeps faster on this GPU from cold caches (measured once at mesh setup;
+// both give the same results, row by row in slot order). Levels above 4M rows use SELL.
+inline void buildCoarseLayout(int n, const std::vector<int>& off, const std::vector<int>& col,
+                              const std::vector<int>& src, SellDev& S, hipStream_t st) {
+    constexpr int TUNE_MAX_ROWS = 1 << 22;
+    bool csr = false;
+    if (n <= TUNE_MAX_ROWS) {
+        SellDev a, b;
+        if (buildCsr(n, off, col, src, a, st)) {
+            buildSell(n, off, col, src, b, st);
+            const size_t fb = size_t(512) << 20;  // larger than L2 + the 256 MB Infinity Cache
+            DevBuf<char> flush;
+            flush.alloc(fb);
+            csr = timeLayout(a, st, flush.p, fb) <= timeLayout(b, st, flush.p, fb);
+        }
+    }
+    if (!(csr && buildCsr(n, off, col, src, S, st))) buildSell(n, off, col, src, S, st);
+}
+
 // ------------------------------------------------------------------ GAMG hierarchy (host setup)
 // Pairwise agglomeration on geometric weights, same decisions as the reference's
 // agglomerate/buildHierarchy (src/fvm/linalg/GAMG.cpp:40-94, 206-252): cells visited in

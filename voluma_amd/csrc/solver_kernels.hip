@@ -3,12 +3,23 @@
 // All kernels are memory-bound; none uses MFMA. Arithmetic is compiled with
 // -ffp-contract=off so that every a·b + c is a rounded product followed by a
 // rounded add, as in the reference (CMakeLists.txt:11).
+#include <hip/hip_ext.h>
+
 #include <type_traits>
 
 #include "device_common.hpp"
 #include "vol_internal.hpp"
 
 namespace vol {
+
+// Value of v in lane `lane` (the same for the whole wavefront) as a scalar read:
+// two v_readlane, no LDS round trip as a general shuffle would take.
+__device__ __forceinline__ double lane_bcast(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = unsigned(__builtin_amdgcn_readlane(int(b & 0xffffffffll), lane));
+    const unsigned hi = unsigned(__builtin_amdgcn_readlane(int(b >> 32), lane));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 // ============================================================================ SELL gather
 // One wavefront per 64-row slice; lanes own rows. Block b is remapped so that each of
@@ -112,6 +123,14 @@ __global__ __launch_bounds__(256) void k_sell(SellArgsT<T> a) {
     const int lane = threadIdx.x & (WAVE - 1);
     const int row = slice * WAVE + lane;
     const bool valid = slice < a.S.nslices && row < a.S.n;
+    if constexpr (!DOT) {  // batches: a slice whose systems are all done is skipped (wave-uniform)
+        if (a.sliceSys && slice < a.S.nslices) {
+            const int2 r = a.sliceSys[slice];
+            bool all = true;
+            for (int q = r.x; q <= r.y; ++q) all = all && a.sysS[q].done;
+            if (all) return;
+        }
+    }
     T xu = T(0);
     if constexpr (XL == XL_UNIFORM) xu = *a.xuPtr;
     T outv = T(0);
@@ -185,7 +204,16 @@ __global__ __launch_bounds__(256) void k_csrv(SellArgsT<T> a) {
     const int row = gid / G, g = gid % G;
     T xu = T(0);
     if constexpr (XL == XL_UNIFORM) xu = *a.xuPtr;
-    csr_row<OP, XL, G, IT>(a, row, g, row < a.S.n, xu);
+    bool valid = row < a.S.n;
+    if (valid && a.rowSys) valid = !a.sysS[a.rowSys[row]].done;  // batches: skip finished systems
+    csr_row<OP, XL, G, IT>(a, row, g, valid, xu);
+}
+
+template <class T, class K> static void sell_launch(K kernel, int nb, hipStream_t st, const SellArgsT<T>& a) {
+    if (a.evStart)
+        hipExtLaunchKernelGGL(kernel, dim3(nb), dim3(256), 0, st, a.evStart, a.evStop, 0, a);
+    else
+        hipLaunchKernelGGL(kernel, dim3(nb), dim3(256), 0, st, a);
 }
 
 template <int OP, int XL, class T>
@@ -194,11 +222,12 @@ static void csr_dispatch(hipStream_t st, const SellArgsT<T>& a) {
     const int nb = int((threads + 255) / 256);
     if (nb == 0) return;
     const int G = a.S.csrG, IT = a.S.csrIt;
-    if (G == 4 && IT <= 4) hipLaunchKernelGGL((k_csrv<OP, XL, 4, 4, T>), dim3(nb), dim3(256), 0, st, a);
-    else if (G == 8 && IT <= 4) hipLaunchKernelGGL((k_csrv<OP, XL, 8, 4, T>), dim3(nb), dim3(256), 0, st, a);
-    else if (G == 16 && IT <= 4) hipLaunchKernelGGL((k_csrv<OP, XL, 16, 4, T>), dim3(nb), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL((k_csrv<OP, XL, 16, 8, T>), dim3(nb), dim3(256), 0, st, a);
+    if (G == 4 && IT <= 4) sell_launch(k_csrv<OP, XL, 4, 4, T>, nb, st, a);
+    else if (G == 8 && IT <= 4) sell_launch(k_csrv<OP, XL, 8, 4, T>, nb, st, a);
+    else if (G == 16 && IT <= 4) sell_launch(k_csrv<OP, XL, 16, 4, T>, nb, st, a);
+    else sell_launch(k_csrv<OP, XL, 16, 8, T>, nb, st, a);
 }
+
 
 template <int OP, int XL, class T>
 static void sell_dispatch2(hipStream_t st, const SellArgsT<T>& a) {
@@ -212,15 +241,15 @@ static void sell_dispatch2(hipStream_t st, const SellArgsT<T>& a) {
     if (a.part) {  // fused Σ out·dotv: A·p, and the last sweep of the V-cycle (Σ w·r)
         if constexpr (OP == OP_AMUL || (OP == OP_JACOBI && XL != XL_UNIFORM && XL != XL_FROMZERO)) {
             if (wide)
-                hipLaunchKernelGGL((k_sell<OP, XL, true, 16, T>), dim3(nb), dim3(256), 0, st, a);
+                sell_launch(k_sell<OP, XL, true, 16, T>, nb, st, a);
             else
-                hipLaunchKernelGGL((k_sell<OP, XL, true, 4, T>), dim3(nb), dim3(256), 0, st, a);
+                sell_launch(k_sell<OP, XL, true, 4, T>, nb, st, a);
         }
     } else {
         if (wide)
-            hipLaunchKernelGGL((k_sell<OP, XL, false, 16, T>), dim3(nb), dim3(256), 0, st, a);
+            sell_launch(k_sell<OP, XL, false, 16, T>, nb, st, a);
         else
-            hipLaunchKernelGGL((k_sell<OP, XL, false, 4, T>), dim3(nb), dim3(256), 0, st, a);
+            sell_launch(k_sell<OP, XL, false, 4, T>, nb, st, a);
     }
 }
 
@@ -895,7 +924,7 @@ __device__ void coarse_solve64_body(int n, const double* luT, const T* r, T* x) 
 #pragma unroll
     for (int k = 0; k < 64; ++k) {
         if (k < n) {
-            const double gk = __shfl(g, k, WAVE);
+            const double gk = lane_bcast(g, k);
             if (i > k && i < n) g -= row[k] * gk;
         }
     }
@@ -903,7 +932,7 @@ __device__ void coarse_solve64_body(int n, const double* luT, const T* r, T* x) 
     for (int k = 63; k >= 0; --k) {
         if (k < n) {
             if (i == k) g /= row[k];
-            const double gk = __shfl(g, k, WAVE);
+            const double gk = lane_bcast(g, k);
             if (i < k) g -= row[k] * gk;
         }
     }
@@ -1222,12 +1251,12 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(TailArgsT<T> t) {
                 const int i = threadIdx.x;
                 double g = i < n ? double(r[i]) : 0.0;
                 for (int k = 0; k < n; ++k) {
-                    const double gk = __shfl(g, k, WAVE);
+                    const double gk = lane_bcast(g, k);
                     if (i > k && i < n) g -= cl[k * n + i] * gk;
                 }
                 for (int k = n - 1; k >= 0; --k) {
                     if (i == k) g /= cl[k * n + k];
-                    const double gk = __shfl(g, k, WAVE);
+                    const double gk = lane_bcast(g, k);
                     if (i < k) g -= cl[k * n + i] * gk;
                 }
                 if (i < n) x[i] = T(g);

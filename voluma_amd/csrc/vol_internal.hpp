@@ -124,6 +124,14 @@ template <class T> struct SellArgsT {
     const T* dotv = nullptr;        // fused dot: Σ out·dotv (in double) into part (fast mode only)
     double* part = nullptr;
     const int* done = nullptr;
+    // batches: rows of systems whose scalars say done are skipped (SELL: per slice the
+    // first and last system of its rows; CSR: the system of each row)
+    const int2* sliceSys = nullptr;
+    const int* rowSys = nullptr;
+    const DevScalars* sysS = nullptr;
+    // host side: when set, the launch records these events at the kernel's start and end
+    // (hipExtLaunchKernelGGL), which times the kernel itself
+    hipEvent_t evStart = nullptr, evStop = nullptr;
 };
 using SellArgs = SellArgsT<double>;
 using SellArgs32 = SellArgsT<float>;
@@ -212,7 +220,6 @@ void launch_coarse_solve32(hipStream_t st, int n, const double* lu, const double
 // Coarse levels of the V-cycle run inside one workgroup (k_vcycle_tail): the levels
 // from `first` down to the coarsest, with block barriers between the phases.
 constexpr int TAIL_MAX_CELLS = 512;   // levels at or below this size go to the tail kernel
-constexpr int CSR_MAX_ROWS = 65536;   // coarse levels up to this size use the CSR-vector layout
 constexpr int TAIL_THREADS = 1024;
 // The tail's vectors (r, two work buffers, x per level) live in LDS at the given
 // offsets (in doubles); only matrix data is read from global memory.
