@@ -224,6 +224,35 @@ int vol_fvm_get(vol_matrix A, double* diag, double* source, double* internalCoef
 /* fold and finish (FvMatrix.h:154-163): A's solve diagonal = diag + Σ internalCoeffs,
  * b (nullable, nCells) = source + Σ boundarySource per owner cell */
 int vol_fvm_finish(vol_matrix A, double* b, int location);
+/* fold and finish of a copy of the matrix with −= su (FvMatrix.h:93-97), as the momentum
+ * predictor solves FvMatrix M(UEqn); M −= ngp (IncompressibleFlow.cpp:214-226): b =
+ * (source + su·V) + Σ boundarySource; the matrix's own source is left unchanged, so
+ * vol_fvm_H afterwards sees UEqn without su. */
+int vol_fvm_finish_source(vol_matrix A, const double* su, const double* V, double* b, int location);
+
+/* ---- pressure-velocity coupling on the FvMatrix parts (FvMatrix.h:111-141,179-222).
+ * Results are bit-identical to the reference's. A vector equation (UEqn) is handled one
+ * component at a time, its parts assembled per component; nComponents = 3 then selects
+ * the vector form of cmptAv, (v + v + v)/3, of the boundary coefficients
+ * (FvMatrix.h:227-228), exact for patches whose coefficient is the same in every
+ * component (fixedValue, zeroGradient). psi: the field's current cell values (nCells). */
+/* FvMatrix::relax(alpha) (FvMatrix.h:111-131): D = max(|diag + Σ ic|, Σ|offdiag|),
+ * diag += D/α − (diag + Σ ic), source += (that increment)·psi; alpha >= 1: no change.
+ * Call vol_fvm_finish (again) before solving. */
+int vol_fvm_relax(vol_matrix A, double alpha, const double* psi, int location);
+/* FvMatrix::setReference(cell, value) (FvMatrix.h:134-141): source[cell] += diag·value,
+ * diag[cell] doubled; global cell numbers are the mesh's own (single domain), and a cell
+ * outside [0, nCells) changes nothing, as on a rank that does not own it. */
+int vol_fvm_set_reference(vol_matrix A, int64_t cell, double value);
+/* FvMatrix::A() (FvMatrix.h:179-187): out[c] = (diag + Σ cmptAv(ic)) / V (nCells) */
+int vol_fvm_A(vol_matrix A, const double* V, int nComponents, double* out, int location);
+/* FvMatrix::H() (FvMatrix.h:189-208), one component: out[c] = (source − Σ offdiag·psi_nb
+ * + Σ bs − Σ (ic − cmptAv ic)·psi_c) / V (nCells) */
+int vol_fvm_H(vol_matrix A, const double* psi, const double* V, int nComponents, double* out, int location);
+/* FvMatrix::flux() (FvMatrix.h:210-222), one component: out (nInternalFaces +
+ * nBoundaryFaces) = upper·psi_N − lower·psi_P on internal faces, ic·psi_P − bs on
+ * boundary faces, + faceFluxCorrection when a laplacian term switched it on. */
+int vol_fvm_flux(vol_matrix A, const double* psi, double* out, int location);
 
 /* ---- batches: many independent systems solved together (no reference counterpart).
  * Systems are concatenated: owner/neighbour/gamgWeights hold each system's internal

@@ -1191,3 +1191,83 @@ void oracle_fold(int n, int nI, int nB, const int* own, const double* diag, cons
         b[c] += bs[bf];
     }
 }
+
+/* ---------------------------------------------------------------- pressure-velocity coupling
+ * FvMatrix parts: diag and source per cell, upper/lower per internal face (lower == upper for
+ * a matrix without a lower part), internalCoeffs/boundarySource per boundary face, and for
+ * vector equations one call per component. */
+
+/* cmptAv of the internalCoeff (FvMatrix.h:227-228): the value itself for a scalar equation,
+ * (v + v + v)/3 for a vector equation whose boundary coefficient is the same in every
+ * component (fixedValue and zeroGradient patches). */
+static double cmpt_av(double v, int nCmpt) { return nCmpt == 3 ? (v + v + v) / 3.0 : v; }
+
+/* FvMatrix::relax (FvMatrix.h:111-131): D = max(|diag + Σ_b ic|, Σ|offdiag|) per cell,
+ * diag += D/α − (diag + Σ_b ic) and source += that increment · ψ. The off-diagonal sums
+ * run in face order, owner rows over |upper|, neighbour rows over |lower|. */
+void oracle_relax(int n, int nI, int nB, const int* own, const int* nei, const double* upper, const double* lower,
+                  const double* ic, const double* psi, double alpha, double* diag, double* source) {
+    if (alpha >= 1.0) return;
+    double* sumOff = (double*)calloc((size_t)n + 1, sizeof(double));
+    double* bD = (double*)calloc((size_t)n + 1, sizeof(double));
+    for (int f = 0; f < nI; ++f) {
+        sumOff[own[f]] += fabs(upper[f]);
+        sumOff[nei[f]] += fabs(lower[f]);
+    }
+    for (int bf = 0; bf < nB; ++bf) bD[own[nI + bf]] += ic[bf];
+    for (int c = 0; c < n; ++c) {
+        const double full = diag[c] + bD[c];
+        const double mag = fabs(full);
+        const double D = mag < sumOff[c] ? sumOff[c] : mag;
+        const double inc = D / alpha - full;
+        diag[c] += inc;
+        source[c] += inc * psi[c];
+    }
+    free(sumOff);
+    free(bD);
+}
+
+/* FvMatrix::setReference (FvMatrix.h:134-141) with global cell numbers equal to the local
+ * ones: source += diag·value, diag doubled; a cell outside [0, n) changes nothing. */
+void oracle_set_reference(int n, long cell, double value, double* diag, double* source) {
+    if (cell < 0 || cell >= n) return;
+    source[cell] += diag[cell] * value;
+    diag[cell] += diag[cell];
+}
+
+/* FvMatrix::A (FvMatrix.h:179-187): (diag + Σ_b cmptAv(ic)) / V */
+void oracle_A(int n, int nI, int nB, const int* own, const double* diag, const double* ic, const double* V, int nCmpt,
+              double* a) {
+    for (int c = 0; c < n; ++c) a[c] = diag[c];
+    for (int bf = 0; bf < nB; ++bf) a[own[nI + bf]] += cmpt_av(ic[bf], nCmpt);
+    for (int c = 0; c < n; ++c) a[c] /= V[c];
+}
+
+/* FvMatrix::H (FvMatrix.h:189-208): (source − Σ offdiag·ψ_nb + Σ_b bs − Σ_b (ic − cmptAv ic)·ψ) / V,
+ * the face terms in face order, then the boundary faces in order. */
+void oracle_H(int n, int nI, int nB, const int* own, const int* nei, const double* upper, const double* lower,
+              const double* source, const double* ic, const double* bs, const double* psi, const double* V, int nCmpt,
+              double* h) {
+    for (int c = 0; c < n; ++c) h[c] = source[c];
+    for (int f = 0; f < nI; ++f) {
+        h[own[f]] -= upper[f] * psi[nei[f]];
+        h[nei[f]] -= lower[f] * psi[own[f]];
+    }
+    for (int bf = 0; bf < nB; ++bf) {
+        const int c = own[nI + bf];
+        h[c] += bs[bf];
+        h[c] -= (ic[bf] - cmpt_av(ic[bf], nCmpt)) * psi[c];
+    }
+    for (int c = 0; c < n; ++c) h[c] = h[c] / V[c];
+}
+
+/* FvMatrix::flux (FvMatrix.h:210-222): upper·ψ_N − lower·ψ_P on internal faces,
+ * ic·ψ_P − bs on boundary faces, then + ffc on every face when the matrix has one
+ * (ffc == NULL: none). */
+void oracle_flux(int nI, int nB, const int* own, const int* nei, const double* upper, const double* lower,
+                 const double* ic, const double* bs, const double* ffc, const double* psi, double* F) {
+    for (int f = 0; f < nI; ++f) F[f] = upper[f] * psi[nei[f]] - lower[f] * psi[own[f]];
+    for (int bf = 0; bf < nB; ++bf) F[nI + bf] = ic[bf] * psi[own[nI + bf]] - bs[bf];
+    if (ffc)
+        for (int f = 0; f < nI + nB; ++f) F[f] += ffc[f];
+}

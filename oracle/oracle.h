@@ -83,6 +83,21 @@ void oracle_ddt_euler(int n, const double* V, double dt, const double* psi0, dou
 /* boundary fold done by FvMatrix::solve before solveLdu (FvMatrix.h:154-163) */
 void oracle_fold(int n, int nI, int nB, const int* own, const double* diag, const double* source, const double* ic,
                  const double* bs, double* d, double* b);
+/* FvMatrix::relax (FvMatrix.h:111-131), in place on diag and source */
+void oracle_relax(int n, int nI, int nB, const int* own, const int* nei, const double* upper, const double* lower,
+                  const double* ic, const double* psi, double alpha, double* diag, double* source);
+/* FvMatrix::setReference (FvMatrix.h:134-141) */
+void oracle_set_reference(int n, long cell, double value, double* diag, double* source);
+/* FvMatrix::A (FvMatrix.h:179-187); nCmpt 3: vector equation */
+void oracle_A(int n, int nI, int nB, const int* own, const double* diag, const double* ic, const double* V, int nCmpt,
+              double* a);
+/* FvMatrix::H (FvMatrix.h:189-208), one component */
+void oracle_H(int n, int nI, int nB, const int* own, const int* nei, const double* upper, const double* lower,
+              const double* source, const double* ic, const double* bs, const double* psi, const double* V, int nCmpt,
+              double* h);
+/* FvMatrix::flux (FvMatrix.h:210-222), one component; ffc NULL when the matrix has none */
+void oracle_flux(int nI, int nB, const int* own, const int* nei, const double* upper, const double* lower,
+                 const double* ic, const double* bs, const double* ffc, const double* psi, double* F);
 
 #ifdef __cplusplus
 }

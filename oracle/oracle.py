@@ -51,6 +51,11 @@ def _lib():
         lib.oracle_div_upwind.argtypes = [I, I, I] + [P] * 10
         lib.oracle_ddt_euler.argtypes = [I, P, D, P, P, P]
         lib.oracle_fold.argtypes = [I, I, I] + [P] * 7
+        lib.oracle_relax.argtypes = [I, I, I] + [P] * 6 + [D, P, P]
+        lib.oracle_set_reference.argtypes = [I, ctypes.c_long, D, P, P]
+        lib.oracle_A.argtypes = [I, I, I, P, P, P, P, I, P]
+        lib.oracle_H.argtypes = [I, I, I] + [P] * 9 + [I, P]
+        lib.oracle_flux.argtypes = [I, I] + [P] * 9
         _LIB = lib
     return _LIB
 
@@ -271,4 +276,98 @@ def transport(mesh, dt, nu, psi0, F, bc_fixed, bc_value=None, non_orth=True):
         ffc[:ni] = ffc[:ni] + (-1.0) * lffc
     dF, bF = fold(mesh, diag, source, ic, bs)
     return {"diag": diag, "upper": upper, "lower": lower, "source": source, "ic": ic, "bs": bs, "ffc": ffc,
-            "dF": dF, "bF": bF}
+            "has_ffc": lffc is not None, "dF": dF, "bF": bF}
+
+
+def pressure_matrix(mesh, gamma, psi, bc_fixed, bc_value=None, non_orth=True):
+    """fvm::laplacian(γ, ψ) on its own (Fvm.h:142-181), as a pressure equation is
+    built: FvMatrix parts with lower = upper and the correction flux when the mesh has
+    non-orthogonal faces."""
+    n, ni = mesh.n_cells, mesh.n_internal
+    gam = np.full(mesh.n_faces, float(gamma))
+    ld, lu, lic, lbs = laplacian(mesh, bc_fixed, bc_value, gamma=gam)
+    source = np.zeros(n)
+    ffc = np.zeros(mesh.n_faces)
+    has = bool(non_orth and np.any(np.einsum("ij,ij->i", mesh.geometry()["corr"][:ni],
+                                             mesh.geometry()["corr"][:ni]) > 1e-20))
+    if has:
+        g = grad_gauss(mesh, psi, boundary_values(mesh, psi, bc_fixed, bc_value))
+        source, lffc = laplacian_nonorth(mesh, g, gam)
+        ffc[:ni] = lffc
+    return {"diag": ld, "upper": lu, "lower": lu.copy(), "source": source, "ic": lic, "bs": lbs, "ffc": ffc,
+            "has_ffc": has}
+
+
+# ---------------------------------------------------------------- pressure-velocity coupling
+def relax(mesh, M, psi, alpha):
+    """FvMatrix::relax(alpha) (FvMatrix.h:111-131): (diag, source) of the relaxed matrix."""
+    diag, source = _f64(M["diag"]).copy(), _f64(M["source"]).copy()
+    _lib().oracle_relax(mesh.n_cells, mesh.n_internal, mesh.n_boundary, _p(_i32(mesh.owner)),
+                        _p(_i32(mesh.neighbour)), _p(_f64(M["upper"])), _p(_f64(M["lower"])), _p(_f64(M["ic"])),
+                        _p(_f64(psi)), float(alpha), _p(diag), _p(source))
+    return diag, source
+
+
+def set_reference(diag, source, cell, value):
+    """FvMatrix::setReference (FvMatrix.h:134-141): (diag, source) afterwards."""
+    d, s = _f64(diag).copy(), _f64(source).copy()
+    _lib().oracle_set_reference(len(d), int(cell), float(value), _p(d), _p(s))
+    return d, s
+
+
+def A(mesh, diag, ic, n_cmpt=1):
+    """FvMatrix::A() (FvMatrix.h:179-187); n_cmpt 3 for a vector equation."""
+    out = np.zeros(mesh.n_cells)
+    _lib().oracle_A(mesh.n_cells, mesh.n_internal, mesh.n_boundary, _p(_i32(mesh.owner)), _p(_f64(diag)),
+                    _p(_f64(ic)), _p(mesh.geometry()["V"]), int(n_cmpt), _p(out))
+    return out
+
+
+def H(mesh, M, psi, n_cmpt=1):
+    """FvMatrix::H() (FvMatrix.h:189-208) of one component."""
+    out = np.zeros(mesh.n_cells)
+    _lib().oracle_H(mesh.n_cells, mesh.n_internal, mesh.n_boundary, _p(_i32(mesh.owner)), _p(_i32(mesh.neighbour)),
+                    _p(_f64(M["upper"])), _p(_f64(M["lower"])), _p(_f64(M["source"])), _p(_f64(M["ic"])),
+                    _p(_f64(M["bs"])), _p(_f64(psi)), _p(mesh.geometry()["V"]), int(n_cmpt), _p(out))
+    return out
+
+
+def flux(mesh, M, psi):
+    """FvMatrix::flux() (FvMatrix.h:210-222) of one component."""
+    out = np.zeros(mesh.n_faces)
+    ffc = _f64(M["ffc"]) if M.get("has_ffc") else None
+    _lib().oracle_flux(mesh.n_internal, mesh.n_boundary, _p(_i32(mesh.owner)), _p(_i32(mesh.neighbour)),
+                       _p(_f64(M["upper"])), _p(_f64(M["lower"])), _p(_f64(M["ic"])), _p(_f64(M["bs"])),
+                       _p(ffc) if ffc is not None else None, _p(_f64(psi)), _p(out))
+    return out
+
+
+def transport_post(mesh, dt, nu, psi0, F, bc_fixed, bc_value, alpha, ref_cell, ref_value, non_orth=True,
+                   pressure=False):
+    """The momentum matrix of :func:`transport` (or the pressure matrix of
+    :func:`pressure_matrix` when `pressure`), relax(alpha), A(), H() and flux() of the
+    relaxed matrix at ψ = psi0, then setReference. psi0 (n,) for a scalar field or (3, n)
+    for a vector field (bc_value then (3, nBoundary)); the same dict as the reference
+    driver's transport_post."""
+    psi0 = np.asarray(psi0, dtype=np.float64)
+    vec = psi0.ndim == 2
+    comps = psi0 if vec else psi0[None, :]
+    bvals = (np.asarray(bc_value, dtype=np.float64) if vec else np.asarray(bc_value, dtype=np.float64)[None, :])
+    nc = 3 if vec else 1
+    res = {k: [] for k in ("source_relaxed", "H", "flux", "source_ref")}
+    for k in range(comps.shape[0]):
+        if pressure:
+            M = pressure_matrix(mesh, nu, comps[k], bc_fixed, bvals[k], non_orth)
+        else:
+            M = transport(mesh, dt, nu, comps[k], F, bc_fixed, bvals[k], non_orth)
+        d, s = relax(mesh, M, comps[k], alpha)
+        R = dict(M, diag=d, source=s)
+        res["source_relaxed"].append(s)
+        res["H"].append(H(mesh, R, comps[k], nc))
+        res["flux"].append(flux(mesh, R, comps[k]))
+        dr, sr = set_reference(d, s, ref_cell, ref_value)
+        res["source_ref"].append(sr)
+        res["diag_relaxed"], res["A"], res["diag_ref"] = d, A(mesh, d, M["ic"], nc), dr
+    for k in ("source_relaxed", "H", "flux", "source_ref"):
+        res[k] = np.array(res[k]) if vec else res[k][0]
+    return res

@@ -48,6 +48,8 @@ def _lib():
         lib.vr_ddt_euler.restype = I
         lib.vr_transport.argtypes = [P, D, D, P, P, P, P, P, I] + [P] * 9
         lib.vr_transport.restype = I
+        lib.vr_transport_post.argtypes = [P, I, I, D, D, P, P, P, P, I, D, ctypes.c_long, D] + [P] * 7
+        lib.vr_transport_post.restype = I
         lib.vr_solve.argtypes = [P, P, P, P, I, P, P, ctypes.c_char_p, ctypes.c_char_p, D, D, I, I, I, P, P, P]
         lib.vr_solve.restype = I
         lib.vr_fvm_poisson_solve.argtypes = [P, D, P, P, P, P, ctypes.c_char_p, ctypes.c_char_p, D, D, I, P, P, P]
@@ -175,6 +177,29 @@ class RefMesh:
                                                                                             dtype=np.float64)),
                                    int(non_orth),
                                    *(_p(out[k]) for k in keys)))
+        return out
+
+    def transport_post(self, dt, nu, psi0, F, fixed_patches, face_values, alpha, ref_cell, ref_value,
+                       non_orth=True, pressure=False):
+        """Momentum matrix (ddt + upwind div − ν laplacian) of a scalar (psi0 of shape (n,))
+        or vector (shape (3, n)) field, relax(alpha), then A(), H(), flux(), then
+        setReference(ref_cell, ref_value). face_values: (nBoundary,) or (3, nBoundary).
+        pressure: the matrix is fvm::laplacian(nu) alone."""
+        psi0 = np.ascontiguousarray(psi0, dtype=np.float64)
+        nc = 1 if psi0.ndim == 1 else psi0.shape[0]
+        n, nf = self.n_cells, self.n_faces
+        fp = np.zeros(self.n_patches, np.int32)
+        fp[list(fixed_patches)] = 1
+        shp = (lambda L: (L,)) if nc == 1 else (lambda L: (nc, L))
+        out = {"diag_relaxed": np.zeros(n), "source_relaxed": np.zeros(shp(n)), "A": np.zeros(n),
+               "H": np.zeros(shp(n)), "flux": np.zeros(shp(nf)), "diag_ref": np.zeros(n),
+               "source_ref": np.zeros(shp(n))}
+        keys = ("diag_relaxed", "source_relaxed", "A", "H", "flux", "diag_ref", "source_ref")
+        _check(_lib().vr_transport_post(self._h, nc, int(pressure), ctypes.c_double(dt), ctypes.c_double(nu), _p(psi0),
+                                        _p(np.ascontiguousarray(F, dtype=np.float64)), _p(fp),
+                                        _p(np.ascontiguousarray(face_values, dtype=np.float64)), int(non_orth),
+                                        ctypes.c_double(alpha), int(ref_cell), ctypes.c_double(ref_value),
+                                        *(_p(out[k]) for k in keys)))
         return out
 
     def solve(self, diag, upper, lower, b, x0, solver="PCG", preconditioner="diagonal", tolerance=1e-8,

@@ -75,6 +75,61 @@ void writePerf(const SolverPerformance& p, double* perf, int* iters, int* conv) 
 
 } // namespace
 
+template <class T> static double& cmptRef(T& v, int k);
+template <> double& cmptRef<scalar>(scalar& v, int) { return v; }
+template <> double& cmptRef<Vec3>(Vec3& v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+template <class T>
+static void transportPost(MeshPtr mp, int nc, int pressure, double dt, double nu, const double* psi0, const double* F,
+                          const int* fixedPatch, const double* faceValues, int nonOrth, double alpha, long refCell,
+                          double refValue, double* diagRelaxed, double* sourceRelaxed, double* Aout, double* Hout,
+                          double* flux, double* diagRef, double* sourceRef) {
+    const Mesh& m = *mp;
+    const label n = m.nCells(), nI = m.nInternalFaces(), nB = m.nBoundaryFaces(), nF = m.nFaces();
+    VolField<T> p(mp, "U", Traits<T>::zero());
+    for (label c = 0; c < n; ++c)
+        for (int k = 0; k < nc; ++k) cmptRef<T>(p[c], k) = psi0[k * n + c];
+    for (std::size_t q = 0; q < m.patches().size(); ++q) {
+        const auto& name = m.patches()[q].name;
+        if (fixedPatch[q])
+            p.fixedValue(name, Traits<T>::zero());
+        else
+            p.zeroGradient(name);
+    }
+    for (std::size_t q = 0; q < m.patches().size(); ++q) {
+        auto* fv = dynamic_cast<FixedValueBC<T>*>(&p.bc(label(q)));
+        if (!fv) continue;
+        const Patch& pt = m.patches()[q];
+        for (label i = 0; i < pt.size; ++i)
+            for (int k = 0; k < nc; ++k) cmptRef<T>(fv->values()[i], k) = faceValues[k * nB + pt.start + i - nI];
+    }
+    p.correctBoundaryConditions();
+    p.storeOld();
+    TimeState ts;
+    ts.dt = dt;
+    std::vector<scalar> Fv(F, F + nF);
+    ConvectionScheme cs;
+    cs.type = ConvectionScheme::Type::Upwind;
+    FvMatrix<T> M = pressure ? fvm::laplacian(nu, p, nonOrth != 0) : fvm::ddt(p, DdtScheme::Euler, ts);
+    if (!pressure) {
+        M += fvm::div(Fv, p, cs);
+        M -= fvm::laplacian(nu, p, nonOrth != 0);
+    }
+    M.relax(alpha);
+    auto put = [&](const std::vector<T>& v, label len, double* out) {
+        for (label i = 0; i < len; ++i)
+            for (int k = 0; k < nc; ++k) out[k * len + i] = cmptRef<T>(const_cast<T&>(v[i]), k);
+    };
+    std::copy(M.diag().begin(), M.diag().begin() + n, diagRelaxed);
+    put(M.source(), n, sourceRelaxed);
+    { const auto a = M.A(); std::copy(a.begin(), a.begin() + n, Aout); }
+    put(M.H(), n, Hout);
+    put(M.flux(), nF, flux);
+    M.setReference(refCell, refValue);
+    std::copy(M.diag().begin(), M.diag().begin() + n, diagRef);
+    put(M.source(), n, sourceRef);
+}
+
 extern "C" {
 
 const char* vr_last_error() { return gErr.c_str(); }
@@ -313,6 +368,32 @@ int vr_transport(void* h, double dt, double nu, const double* psi0, const double
             dF[c] += M.internalCoeffs()[bf];
             bF[c] += M.boundarySource()[bf];
         }
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
+// The momentum matrix of vr_transport followed by FvMatrix::relax(alpha) (FvMatrix.h:111-131)
+// and the pressure-velocity coupling quantities A(), H(), flux() (FvMatrix.h:179-222) of
+// the relaxed matrix, then setReference(refCell, refValue) (FvMatrix.h:134-141) on it.
+// pressure != 0: the matrix is fvm::laplacian(nu) alone (a pressure equation, symmetric).
+// nCmpt 1: a scalar field; 3: a vector field (component arrays one after another: psi0
+// and sources n per component, faceValues nBoundary per component, flux nFaces).
+int vr_transport_post(void* h, int nCmpt, int pressure, double dt, double nu, const double* psi0, const double* F,
+                      const int* fixedPatch, const double* faceValues, int nonOrth, double alpha, long refCell,
+                      double refValue, double* diagRelaxed, double* sourceRelaxed, double* Aout, double* Hout,
+                      double* flux, double* diagRef, double* sourceRef) {
+    try {
+        MeshPtr mp = static_cast<RefMesh*>(h)->m;
+        if (nCmpt == 1)
+            transportPost<scalar>(mp, 1, pressure, dt, nu, psi0, F, fixedPatch, faceValues, nonOrth, alpha, refCell, refValue,
+                                  diagRelaxed, sourceRelaxed, Aout, Hout, flux, diagRef, sourceRef);
+        else if (nCmpt == 3)
+            transportPost<Vec3>(mp, 3, pressure, dt, nu, psi0, F, fixedPatch, faceValues, nonOrth, alpha, refCell, refValue,
+                                diagRelaxed, sourceRelaxed, Aout, Hout, flux, diagRef, sourceRef);
+        else
+            throw std::runtime_error("nCmpt must be 1 or 3");
         return 0;
     } catch (const std::exception& e) {
         return fail(e);

@@ -161,6 +161,54 @@ def small_case(name, make):
     print(f"small_{name}: {n} cells, levels {out['gamg_sizes'].tolist()}", flush=True)
 
 
+POST_ALPHA, POST_DT_VEC, POST_REF_VALUE = 0.7, 1.0, 0.25
+
+
+def bc_fixed_of(m, patches):
+    """Per boundary face: 1 on the faces of the given patches (fixedValue), 0 elsewhere."""
+    ni = m.n_internal
+    out = np.zeros(m.n_faces - ni, np.int32)
+    for p in patches:
+        out[m.patch_start[p] - ni: m.patch_start[p] - ni + m.patch_size[p]] = 1
+    return out
+
+
+def post_case(name, make):
+    """FvMatrix::relax, A(), H(), flux() and setReference (FvMatrix.h:111-141,179-222)
+    from the reference: the momentum matrix of a scalar field (fixedValue on xMax) and of
+    a vector field (fixedValue on every patch but xMax, dt = 1 so that the vector form of
+    cmptAv changes results), and a pressure matrix (laplacian alone, fixedValue on xMax)."""
+    m = make()
+    rm = ref.RefMesh(mesh=m)
+    n, ni = m.n_cells, m.n_internal
+    inp = W.momentum_inputs(m, 2)
+    out = {"mesh_hash": np.frombuffer(bytes.fromhex(mesh_hash(m)), np.uint8)}
+    xmax = [W.XMAX_PATCH]
+    other = [p for p in range(len(m.patch_start)) if p != W.XMAX_PATCH]
+    fv_x = np.where(bc_fixed_of(m, xmax) != 0, MOM_BC_VALUE, 0.0)
+    cases = {
+        "ms": dict(dt=MOM_DT, nu=MOM_NU, psi0=2.0 * W.splitmix_uniform(n, 31) - 1.0, F=flux_field(m, 11),
+                   patches=xmax, fv=fv_x, ref_cell=n // 3, pressure=False),
+        "mv": dict(dt=POST_DT_VEC, nu=inp["nu"], psi0=np.ascontiguousarray(inp["U0"].T), F=inp["phi"],
+                   patches=other, fv=np.ascontiguousarray(inp["Ub"].T), ref_cell=n // 2, pressure=False),
+        "p": dict(dt=1.0, nu=1.0, psi0=W.pressure_rhs(n, seed=41), F=np.zeros(m.n_faces), patches=xmax,
+                  fv=np.zeros(m.n_faces - ni), ref_cell=0, pressure=True),
+    }
+    for tag, c in cases.items():
+        r = rm.transport_post(c["dt"], c["nu"], c["psi0"], c["F"], c["patches"], c["fv"], POST_ALPHA, c["ref_cell"],
+                              POST_REF_VALUE, True, pressure=c["pressure"])
+        out[f"{tag}_in_psi0"] = c["psi0"]
+        out[f"{tag}_in_F"] = c["F"]
+        out[f"{tag}_in_fv"] = c["fv"]
+        out[f"{tag}_in_bcfixed"] = bc_fixed_of(m, c["patches"])
+        out[f"{tag}_in_scalars"] = np.array([c["dt"], c["nu"], POST_ALPHA, c["ref_cell"], POST_REF_VALUE,
+                                             float(c["pressure"])])
+        for k, v in r.items():
+            out[f"{tag}_{k}"] = v
+    np.savez_compressed(os.path.join(HERE, f"post_{name}.npz"), **out)
+    print(f"post_{name}: {n} cells", flush=True)
+
+
 def summary(x):
     x = np.asarray(x)
     idx = np.linspace(0, x.size - 1, 16).astype(np.int64)
@@ -260,6 +308,12 @@ def dropin_case():
 
 
 if __name__ == "__main__":
+    if "--post" in sys.argv:
+        for nm, mk in SMALL.items():
+            post_case(nm, mk)
+        sys.exit(0)
+    for nm, mk in SMALL.items():
+        post_case(nm, mk)
     dropin_case()
     exactsum_case()
     for nm, mk in SMALL.items():

@@ -44,3 +44,19 @@ def solve_cases(g):
         solver, pre, repro, sym = str(s).split("|")
         out.append((i, solver, pre, bool(int(repro)), bool(int(sym))))
     return out
+
+
+def load_post(name):
+    """Reference relax / A / H / flux / setReference fixtures of one small mesh."""
+    with np.load(os.path.join(GOLDEN, f"post_{name}.npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+POST_CASES = ("ms", "mv", "p")  # momentum scalar, momentum vector, pressure
+
+
+def post_inputs(g, tag):
+    s = g[f"{tag}_in_scalars"]
+    return {"psi0": g[f"{tag}_in_psi0"], "F": g[f"{tag}_in_F"], "fv": g[f"{tag}_in_fv"],
+            "bcfixed": g[f"{tag}_in_bcfixed"], "dt": float(s[0]), "nu": float(s[1]), "alpha": float(s[2]),
+            "ref_cell": int(s[3]), "ref_value": float(s[4]), "pressure": bool(s[5])}

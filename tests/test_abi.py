@@ -14,7 +14,7 @@ HEADER = os.path.join(ROOT, "include", "voluma_amd.h")
 
 def declared():
     txt = open(HEADER).read()
-    return sorted(set(re.findall(r"^(?:int|void|const char\*)\s+(vol_[a-z_0-9]+)\(", txt, re.M)))
+    return sorted(set(re.findall(r"^(?:int|void|const char\*)\s+(vol_[A-Za-z_0-9]+)\(", txt, re.M)))
 
 
 def test_header_declares_and_library_exports_every_symbol():
@@ -24,7 +24,7 @@ def test_header_declares_and_library_exports_every_symbol():
     for n in names:
         assert hasattr(lib, n), n
     out = subprocess.run(["nm", "-D", "--defined-only", va.LIB_PATH], capture_output=True, text=True).stdout
-    exported = sorted(set(re.findall(r" T (vol_[a-z_0-9]+)$", out, re.M)))
+    exported = sorted(set(re.findall(r" T (vol_[A-Za-z_0-9]+)$", out, re.M)))
     assert exported == names
     assert sorted(va.EXPORTED) == names
 

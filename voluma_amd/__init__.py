@@ -90,7 +90,8 @@ EXPORTED = ("vol_controls_default", "vol_last_error", "vol_context_create", "vol
             "vol_fvm_div_upwind", "vol_fvm_laplacian", "vol_fvm_source", "vol_fvm_get", "vol_fvm_finish",
             "vol_batch_create", "vol_batch_destroy", "vol_batch_info", "vol_batch_set", "vol_batch_solve",
             "vol_server_create", "vol_server_add_mesh", "vol_server_submit", "vol_server_wait", "vol_server_stats",
-            "vol_server_destroy")
+            "vol_server_destroy", "vol_fvm_finish_source", "vol_fvm_relax", "vol_fvm_set_reference", "vol_fvm_A",
+            "vol_fvm_H", "vol_fvm_flux")
 
 
 def lib():
@@ -134,6 +135,12 @@ def lib():
         L.vol_fvm_source.argtypes = [P, P, P, I]
         L.vol_fvm_get.argtypes = [P, P, P, P, P, P, I]
         L.vol_fvm_finish.argtypes = [P, P, I]
+        L.vol_fvm_finish_source.argtypes = [P, P, P, P, I]
+        L.vol_fvm_relax.argtypes = [P, D, P, I]
+        L.vol_fvm_set_reference.argtypes = [P, ctypes.c_int64, D]
+        L.vol_fvm_A.argtypes = [P, P, I, P, I]
+        L.vol_fvm_H.argtypes = [P, P, P, I, P, I]
+        L.vol_fvm_flux.argtypes = [P, P, P, I]
         L.vol_batch_create.argtypes = [P, I, P, P, P, P, P, ctypes.POINTER(P)]
         L.vol_batch_destroy.argtypes = [P]
         L.vol_batch_info.argtypes = [P, P, P, P]
@@ -437,6 +444,37 @@ class LduMatrix:
         _check(lib().vol_fvm_finish(self._h, _p(b), VOL_HOST))
         self.symmetric = not getattr(self, "_fvm_lower", False)
         return b
+
+    def fvm_finish_source(self, su, V):
+        """Fold a copy of the matrix with −= su (source + su·V); the matrix keeps its own
+        source. Returns b."""
+        b = np.zeros(self.mesh.n_cells)
+        _check(lib().vol_fvm_finish_source(self._h, _p(_f64(su)), _p(_f64(V)), _p(b), VOL_HOST))
+        self.symmetric = not getattr(self, "_fvm_lower", False)
+        return b
+
+    # ---- pressure-velocity coupling (FvMatrix::relax / setReference / A / H / flux)
+    def fvm_relax(self, alpha, psi):
+        _check(lib().vol_fvm_relax(self._h, ctypes.c_double(alpha), _p(_f64(psi)), VOL_HOST))
+
+    def fvm_set_reference(self, cell, value):
+        _check(lib().vol_fvm_set_reference(self._h, int(cell), ctypes.c_double(value)))
+
+    def fvm_A(self, V, n_components=1):
+        out = np.zeros(self.mesh.n_cells)
+        _check(lib().vol_fvm_A(self._h, _p(_f64(V)), int(n_components), _p(out), VOL_HOST))
+        return out
+
+    def fvm_H(self, psi, V, n_components=1):
+        out = np.zeros(self.mesh.n_cells)
+        _check(lib().vol_fvm_H(self._h, _p(_f64(psi)), _p(_f64(V)), int(n_components), _p(out), VOL_HOST))
+        return out
+
+    def fvm_flux(self, psi):
+        m = self.mesh
+        out = np.zeros(max(m.n_faces + m.n_boundary, 1))
+        _check(lib().vol_fvm_flux(self._h, _p(_f64(psi)), _p(out), VOL_HOST))
+        return out[: m.n_faces + m.n_boundary]
 
     def close(self):
         if getattr(self, "_h", None):

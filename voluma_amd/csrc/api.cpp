@@ -87,7 +87,9 @@ struct vol_matrix_s {
     // FvMatrix parts of device assembly (vol_fvm_*): unfolded diag, source,
     // internalCoeffs, boundarySource, face-flux correction; upper/lower are shared
     bool fvmActive = false, fvmHasLower = false;
+    bool fvmHasFfc = false;  // some laplacian term switched its non-orthogonal correction on
     DevBuf<double> fdiag, fsrc, fic, fbs, fffc, fgrad, ffcT;
+    DevBuf<double> fout;  // staging for host-pointer results
     DevBuf<int> anyCorr;
     DevBuf<double> stg[9];  // staging for host inputs
 };
@@ -501,9 +503,18 @@ void fvmReset(vol_matrix A) {
         if (cnt) HIPCHECK(hipMemsetAsync(b->p, 0, sizeof(double) * cnt, st));
     A->fvmActive = true;
     A->fvmHasLower = false;
+    A->fvmHasFfc = false;
     A->set = false;
     A->gamgReady = false;
     A->mixedReady = false;
+}
+
+// device owner/neighbour of the internal faces, uploaded on first need
+void ensureOwnNei(vol_mesh m) {
+    if (!m->dOwn.p && m->nf) {
+        m->dOwn.upload(m->own, m->ctx->stream);
+        m->dNei.upload(m->nei, m->ctx->stream);
+    }
 }
 
 void fvmTerm(vol_matrix A, int kind, FvmArgs& a, bool termLower) {
@@ -514,10 +525,7 @@ void fvmTerm(vol_matrix A, int kind, FvmArgs& a, bool termLower) {
     const int* own = nullptr;
     const int* nei = nullptr;
     if (kind == FVM_LAP && a.psi) {
-        if (!m->dOwn.p && m->nf) {
-            m->dOwn.upload(m->own, st);
-            m->dNei.upload(m->nei, st);
-        }
+        ensureOwnNei(m);
         own = m->dOwn.p;
         nei = m->dNei.p;
         A->fgrad.alloc(3 * size_t(m->n));
@@ -529,6 +537,50 @@ void fvmTerm(vol_matrix A, int kind, FvmArgs& a, bool termLower) {
     }
     launch_fvm_term(st, kind, a, own, nei);
     A->fvmHasLower = newLower;
+    if (kind == FVM_LAP && a.psi) {
+        // the matrix holds a face-flux correction from here on (FvMatrix.h:50-53,240-244)
+        int any = 0;
+        HIPCHECK(hipMemcpyAsync(&any, A->anyCorr.p, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        if (any) A->fvmHasFfc = true;
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+}
+
+// lower part of the FvMatrix parts: upper when the matrix has none (FvMatrix.h:44)
+const double* fvmLower(vol_matrix A) { return A->fvmHasLower ? A->lower.p : A->upper.p; }
+
+// Result buffer: the caller's on the device, else the staging buffer copied back by fvmOut.
+double* fvmOutBuf(vol_matrix A, double* out, size_t cnt, int location) {
+    if (location == VOL_DEVICE) return out;
+    A->fout.alloc(cnt);
+    return A->fout.p;
+}
+void fvmOut(vol_matrix A, double* out, size_t cnt, int location) {
+    hipStream_t st = A->mesh->ctx->stream;
+    if (location != VOL_DEVICE && cnt)
+        HIPCHECK(hipMemcpyAsync(out, A->fout.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+}
+
+void fvmFinish(vol_matrix A, double* b, const double* su, const double* V, int location) {
+    vol_mesh m = A->mesh;
+    setDevice(m->ctx);
+    hipStream_t st = m->ctx->stream;
+    const int n = m->n;
+    A->diag.alloc(n);
+    FvmArgs a = fvmArgs(A, 1);
+    DevBuf<double> tb;
+    double* bd = b;
+    if (b && location != VOL_DEVICE) {
+        tb.alloc(n);
+        bd = tb.p;
+    }
+    launch_fvm_fold(st, a, A->diag.p, bd, su ? stageK(A, 0, su, n, location) : nullptr,
+                    su ? stageK(A, 1, V, n, location) : nullptr);
+    A->symmetric = !A->fvmHasLower;
+    matrixFinish(A);
+    if (b && location != VOL_DEVICE) HIPCHECK(hipMemcpyAsync(b, tb.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
 }
 
@@ -1407,23 +1459,86 @@ int vol_fvm_finish(vol_matrix A, double* b, int location) {
     API_BEGIN
     if (!A) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null matrix");
     fvmCheck(A, 1);
+    fvmFinish(A, b, nullptr, nullptr, location);
+    API_END
+}
+
+int vol_fvm_finish_source(vol_matrix A, const double* su, const double* V, double* b, int location) {
+    API_BEGIN
+    if (!A || !su || !V || !b) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    fvmCheck(A, 1);
+    fvmFinish(A, b, su, V, location);
+    API_END
+}
+
+int vol_fvm_relax(vol_matrix A, double alpha, const double* psi, int location) {
+    API_BEGIN
+    if (!A || !psi) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    fvmCheck(A, 1);
+    if (alpha >= 1.0) return VOL_OK;  // FvMatrix.h:112
     vol_mesh m = A->mesh;
     setDevice(m->ctx);
-    hipStream_t st = m->ctx->stream;
-    const int n = m->n;
-    A->diag.alloc(n);
     FvmArgs a = fvmArgs(A, 1);
-    DevBuf<double> tb;
-    double* bd = b;
-    if (b && location != VOL_DEVICE) {
-        tb.alloc(n);
-        bd = tb.p;
-    }
-    launch_fvm_fold(st, a, A->diag.p, bd);
-    A->symmetric = !A->fvmHasLower;
-    matrixFinish(A);
-    if (b && location != VOL_DEVICE) HIPCHECK(hipMemcpyAsync(b, tb.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
+    launch_fvm_relax(m->ctx->stream, a, alpha, stageK(A, 0, psi, size_t(m->n), location), fvmLower(A));
+    A->set = false;  // the solver matrix comes from the next vol_fvm_finish
+    HIPCHECK(hipStreamSynchronize(m->ctx->stream));
+    API_END
+}
+
+int vol_fvm_set_reference(vol_matrix A, int64_t cell, double value) {
+    API_BEGIN
+    if (!A) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null matrix");
+    fvmCheck(A, 1);
+    vol_mesh m = A->mesh;
+    if (cell < 0 || cell >= m->n) return VOL_OK;  // no cell of this mesh carries that number
+    setDevice(m->ctx);
+    launch_fvm_set_reference(m->ctx->stream, A->fdiag.p, A->fsrc.p, int(cell), value);
+    A->set = false;
+    HIPCHECK(hipStreamSynchronize(m->ctx->stream));
+    API_END
+}
+
+int vol_fvm_A(vol_matrix A, const double* V, int nComponents, double* out, int location) {
+    API_BEGIN
+    if (!A || !V || !out) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    fvmCheck(A, 1);
+    if (nComponents != 1 && nComponents != 3) throw VolError(VOL_ERR_INVALID_ARGUMENT, "nComponents must be 1 or 3");
+    vol_mesh m = A->mesh;
+    setDevice(m->ctx);
+    const size_t n = m->n;
+    FvmArgs a = fvmArgs(A, 1);
+    launch_fvm_A(m->ctx->stream, a, stageK(A, 1, V, n, location), nComponents, fvmOutBuf(A, out, n, location));
+    fvmOut(A, out, n, location);
+    API_END
+}
+
+int vol_fvm_H(vol_matrix A, const double* psi, const double* V, int nComponents, double* out, int location) {
+    API_BEGIN
+    if (!A || !psi || !V || !out) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    fvmCheck(A, 1);
+    if (nComponents != 1 && nComponents != 3) throw VolError(VOL_ERR_INVALID_ARGUMENT, "nComponents must be 1 or 3");
+    vol_mesh m = A->mesh;
+    setDevice(m->ctx);
+    const size_t n = m->n;
+    FvmArgs a = fvmArgs(A, 1);
+    launch_fvm_H(m->ctx->stream, a, fvmLower(A), stageK(A, 0, psi, n, location), stageK(A, 1, V, n, location),
+                 nComponents, fvmOutBuf(A, out, n, location));
+    fvmOut(A, out, n, location);
+    API_END
+}
+
+int vol_fvm_flux(vol_matrix A, const double* psi, double* out, int location) {
+    API_BEGIN
+    if (!A || !psi || !out) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    fvmCheck(A, 1);
+    vol_mesh m = A->mesh;
+    setDevice(m->ctx);
+    ensureOwnNei(m);
+    const size_t nF = size_t(m->nf) + m->nb;
+    FvmArgs a = fvmArgs(A, 1);
+    launch_fvm_flux(m->ctx->stream, a, fvmLower(A), m->dOwn.p, m->dNei.p, stageK(A, 0, psi, size_t(m->n), location),
+                    A->fvmHasFfc ? 1 : 0, fvmOutBuf(A, out, nF, location));
+    fvmOut(A, out, nF, location);
     API_END
 }
 

@@ -292,6 +292,119 @@ __global__ void k_fvm_fold(FvmArgs a, double* dOut, double* bOut) {
     if (bOut) bOut[c] = b;
 }
 
+// ============================================================================ pressure-velocity coupling
+// cmptAv of a boundary coefficient (FvMatrix.h:227-228): the value for a scalar equation,
+// (v + v + v)/3 for a vector equation with the same coefficient in every component.
+__device__ __forceinline__ double cmpt_av(double v, int nCmpt) { return nCmpt == 3 ? (v + v + v) / 3.0 : v; }
+
+// FvMatrix::relax (FvMatrix.h:111-131). Per cell: Σ|offdiag| over its faces in ascending
+// order (|upper| on owner rows, |lower| on neighbour rows, as the reference's face loop
+// adds them), Σ ic over its boundary faces in order, then D = max(|diag + Σ ic|, Σ|offdiag|),
+// diag += D/α − (diag + Σ ic), source += that increment · ψ.
+__global__ void k_fvm_relax(FvmArgs a, double alpha, const double* psi, const double* lower) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n) return;
+    const int slice = c / WAVE, lane = c % WAVE;
+    const int s0 = a.S.sliceOff[slice], width = (a.S.sliceOff[slice + 1] - s0) / WAVE;
+    double sumOff = 0.0;
+    for (int k = 0; k < width; ++k) {
+        const int e = a.S.src[s0 + k * WAVE + lane];
+        if (e < 0) continue;
+        const int f = e >> 1;
+        sumOff += fabs((e & 1) ? lower[f] : a.upper[f]);
+    }
+    double bD = 0.0;
+    for (int k = a.bOff[c]; k < a.bOff[c + 1]; ++k) bD += a.ic[a.bList[k]];
+    const double d = a.diag[c];
+    const double full = d + bD;
+    const double mag = fabs(full);
+    const double D = mag < sumOff ? sumOff : mag;  // std::max(|full|, sumOff)
+    const double inc = D / alpha - full;
+    a.diag[c] = d + inc;
+    a.source[c] = a.source[c] + inc * psi[c];
+}
+
+// FvMatrix::setReference (FvMatrix.h:134-141) at one cell (checked on the host).
+__global__ void k_fvm_set_reference(double* diag, double* source, int c, double value) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const double d = diag[c];
+    source[c] = source[c] + d * value;
+    diag[c] = d + d;
+}
+
+// FvMatrix::A (FvMatrix.h:179-187): (diag + Σ_b cmptAv(ic)) / V.
+__global__ void k_fvm_A(FvmArgs a, const double* V, int nCmpt, double* out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n) return;
+    double d = a.diag[c];
+    for (int k = a.bOff[c]; k < a.bOff[c + 1]; ++k) d += cmpt_av(a.ic[a.bList[k]], nCmpt);
+    out[c] = d / V[c];
+}
+
+// FvMatrix::H (FvMatrix.h:189-208): per cell source − Σ offdiag·ψ_nb over its faces in
+// ascending order (owner rows upper·ψ_N, neighbour rows lower·ψ_P), then per boundary
+// face + bs − (ic − cmptAv ic)·ψ, then / V.
+__global__ void k_fvm_H(FvmArgs a, const double* lower, const double* psi, const double* V, int nCmpt, double* out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n) return;
+    const int slice = c / WAVE, lane = c % WAVE;
+    const int s0 = a.S.sliceOff[slice], width = (a.S.sliceOff[slice + 1] - s0) / WAVE;
+    double h = a.source[c];
+    for (int k = 0; k < width; ++k) {
+        const int slot = s0 + k * WAVE + lane;
+        const int e = a.S.src[slot];
+        if (e < 0) continue;
+        const int f = e >> 1;
+        h -= ((e & 1) ? lower[f] : a.upper[f]) * psi[a.S.col[slot]];
+    }
+    const double pc = psi[c];
+    for (int k = a.bOff[c]; k < a.bOff[c + 1]; ++k) {
+        const int bf = a.bList[k];
+        const double ic = a.ic[bf];
+        h += a.bs[bf];
+        h -= (ic - cmpt_av(ic, nCmpt)) * pc;
+    }
+    out[c] = h / V[c];
+}
+
+// FvMatrix::flux (FvMatrix.h:210-222) on the internal faces: upper·ψ_N − lower·ψ_P (+ ffc).
+__global__ void k_fvm_flux_faces(FvmArgs a, const double* lower, const int* own, const int* nei, const double* psi,
+                                 int hasFfc, double* out) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= a.nf) return;
+    double F = a.upper[f] * psi[nei[f]] - lower[f] * psi[own[f]];
+    if (hasFfc) F = F + a.ffc[f];
+    out[f] = F;
+}
+
+// ... and on the boundary faces, per owner cell: ic·ψ_P − bs (+ ffc).
+__global__ void k_fvm_flux_boundary(FvmArgs a, const double* psi, int hasFfc, double* out) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n) return;
+    const double pc = psi[c];
+    for (int k = a.bOff[c]; k < a.bOff[c + 1]; ++k) {
+        const int bf = a.bList[k];
+        double F = a.ic[bf] * pc - a.bs[bf];
+        if (hasFfc) F = F + a.ffc[a.nf + bf];
+        out[a.nf + bf] = F;
+    }
+}
+
+// fold of a copy of the matrix with an extra source term −= su (FvMatrix.h:93-97):
+// b = (source + su·V) + Σ bs per cell; the matrix's own source is left as it is.
+__global__ void k_fvm_fold_extra(FvmArgs a, const double* su, const double* V, double* dOut, double* bOut) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n) return;
+    double d = a.diag[c], b = a.source[c] + su[c] * V[c];
+    for (int k = a.bOff[c]; k < a.bOff[c + 1]; ++k) {
+        const int bf = a.bList[k];
+        d += a.ic[bf];
+        b += a.bs[bf];
+    }
+    dOut[c] = d;
+    bOut[c] = b;
+}
+
 void launch_fvm_term(hipStream_t st, int kind, const FvmArgs& a, const int* own, const int* nei) {
     const dim3 fb((a.nf + 255) / 256), cb((a.n + 255) / 256), t(256);
     if (kind == FVM_LAP && a.psi) {
@@ -316,8 +429,38 @@ void launch_fvm_source(hipStream_t st, int n, const double* su, const double* V,
     if (n > 0) hipLaunchKernelGGL(k_fvm_source, dim3((n + 255) / 256), dim3(256), 0, st, n, su, V, source);
 }
 
-void launch_fvm_fold(hipStream_t st, const FvmArgs& a, double* dOut, double* bOut) {
-    if (a.n > 0) hipLaunchKernelGGL(k_fvm_fold, dim3((a.n + 255) / 256), dim3(256), 0, st, a, dOut, bOut);
+void launch_fvm_fold(hipStream_t st, const FvmArgs& a, double* dOut, double* bOut, const double* su, const double* V) {
+    if (a.n <= 0) return;
+    if (su)
+        hipLaunchKernelGGL(k_fvm_fold_extra, dim3((a.n + 255) / 256), dim3(256), 0, st, a, su, V, dOut, bOut);
+    else
+        hipLaunchKernelGGL(k_fvm_fold, dim3((a.n + 255) / 256), dim3(256), 0, st, a, dOut, bOut);
+}
+
+void launch_fvm_relax(hipStream_t st, const FvmArgs& a, double alpha, const double* psi, const double* lower) {
+    if (a.n > 0) hipLaunchKernelGGL(k_fvm_relax, dim3((a.n + 255) / 256), dim3(256), 0, st, a, alpha, psi, lower);
+}
+
+void launch_fvm_set_reference(hipStream_t st, double* diag, double* source, int cell, double value) {
+    hipLaunchKernelGGL(k_fvm_set_reference, dim3(1), dim3(64), 0, st, diag, source, cell, value);
+}
+
+void launch_fvm_A(hipStream_t st, const FvmArgs& a, const double* V, int nCmpt, double* out) {
+    if (a.n > 0) hipLaunchKernelGGL(k_fvm_A, dim3((a.n + 255) / 256), dim3(256), 0, st, a, V, nCmpt, out);
+}
+
+void launch_fvm_H(hipStream_t st, const FvmArgs& a, const double* lower, const double* psi, const double* V, int nCmpt,
+                  double* out) {
+    if (a.n > 0) hipLaunchKernelGGL(k_fvm_H, dim3((a.n + 255) / 256), dim3(256), 0, st, a, lower, psi, V, nCmpt, out);
+}
+
+void launch_fvm_flux(hipStream_t st, const FvmArgs& a, const double* lower, const int* own, const int* nei,
+                     const double* psi, int hasFfc, double* out) {
+    if (a.nf > 0)
+        hipLaunchKernelGGL(k_fvm_flux_faces, dim3((a.nf + 255) / 256), dim3(256), 0, st, a, lower, own, nei, psi,
+                           hasFfc, out);
+    if (a.n > 0 && a.nb > 0)
+        hipLaunchKernelGGL(k_fvm_flux_boundary, dim3((a.n + 255) / 256), dim3(256), 0, st, a, psi, hasFfc, out);
 }
 
 }  // namespace vol

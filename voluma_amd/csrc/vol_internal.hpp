@@ -331,8 +331,23 @@ struct FvmArgs {
 void launch_fvm_term(hipStream_t st, int kind, const FvmArgs& a, const int* own, const int* nei);
 // FvMatrix −= su (FvMatrix.h:93-97): source[c] += su[c]·V[c]
 void launch_fvm_source(hipStream_t st, int n, const double* su, const double* V, double* source);
-// FvMatrix::solve fold (FvMatrix.h:154-163): d = diag + Σ ic, b = source + Σ bs per cell
-void launch_fvm_fold(hipStream_t st, const FvmArgs& a, double* dOut, double* bOut);
+// FvMatrix::solve fold (FvMatrix.h:154-163): d = diag + Σ ic, b = source + Σ bs per cell;
+// su != nullptr: the fold of a copy of the matrix with −= su (b = source + su·V + Σ bs)
+void launch_fvm_fold(hipStream_t st, const FvmArgs& a, double* dOut, double* bOut, const double* su = nullptr,
+                     const double* V = nullptr);
+// FvMatrix::relax (FvMatrix.h:111-131) in place on a.diag / a.source; lower = upper when
+// the matrix has no lower part
+void launch_fvm_relax(hipStream_t st, const FvmArgs& a, double alpha, const double* psi, const double* lower);
+// FvMatrix::setReference (FvMatrix.h:134-141) at cell (0 <= cell < n)
+void launch_fvm_set_reference(hipStream_t st, double* diag, double* source, int cell, double value);
+// FvMatrix::A (FvMatrix.h:179-187); nCmpt 3: vector equation
+void launch_fvm_A(hipStream_t st, const FvmArgs& a, const double* V, int nCmpt, double* out);
+// FvMatrix::H (FvMatrix.h:189-208), one component
+void launch_fvm_H(hipStream_t st, const FvmArgs& a, const double* lower, const double* psi, const double* V, int nCmpt,
+                  double* out);
+// FvMatrix::flux (FvMatrix.h:210-222): out has nf + nb entries
+void launch_fvm_flux(hipStream_t st, const FvmArgs& a, const double* lower, const int* own, const int* nei,
+                     const double* psi, int hasFfc, double* out);
 
 // ------------------------------------------------------------------ batches of systems
 // Independent systems concatenated (cells, faces and every GAMG level). Vector work is
