@@ -285,6 +285,32 @@ def momentum_case(tag, name):
         json.dump(res, fh, indent=1)
 
 
+def momentum_dilu_case(tag, name):
+    """Adds to the momentum fixture the reference's default momentum solve, PBiCGStab +
+    DILU with plain sums (IncompressibleFlow.h:30), of velocity component 0 from x = U⁰
+    to 1e-10 (1e-6 takes a single iteration on this diagonally dominant matrix)."""
+    path = os.path.join(HERE, f"{tag}_{name}_momentum.json")
+    with open(path) as fh:
+        res = json.load(fh)
+    m = W.make_mesh(name)
+    rm = ref.RefMesh(mesh=m)
+    inp = W.momentum_inputs(m, 1)
+    allp = list(range(len(m.patch_start)))
+    tr = rm.transport(inp["dt"], inp["nu"], inp["U0"][:, 0], inp["phi"], allp, None, True,
+                      face_values=inp["Ub"][:, 0])
+    assert sha(tr["dF"]) == res["components"][0]["diag_sha256"]
+    x0 = np.ascontiguousarray(inp["U0"][:, 0])
+    t = time.time()
+    x, p = rm.solve(tr["dF"], tr["upper"], tr["lower"], tr["bF"], x0, solver="PBiCGStab", preconditioner="DILU",
+                    tolerance=1e-10, max_iter=1000, reproducible=False)
+    res["components"][0]["plain_pbicgstab_dilu_1e-10"] = {
+        "iterations": p["iterations"], "initialResidual": p["initialResidual"], "finalResidual": p["finalResidual"],
+        "normFactor": p["normFactor"], "converged": p["converged"], "x": summary(x)}
+    print(f"  {tag} momentum DILU: {p['iterations']} iterations, {time.time() - t:.1f} s", flush=True)
+    with open(path, "w") as fh:
+        json.dump(res, fh, indent=1)
+
+
 def exactsum_case():
     out = {"cases": []}
     for seed, scale in ((1, 1.0), (2, 1e20), (3, 1e-300), (4, 1e300)):
@@ -308,6 +334,9 @@ def dropin_case():
 
 
 if __name__ == "__main__":
+    if "--momentum-dilu" in sys.argv:
+        momentum_dilu_case("cfg2", "tet70")
+        sys.exit(0)
     if "--post" in sys.argv:
         for nm, mk in SMALL.items():
             post_case(nm, mk)
@@ -323,3 +352,4 @@ if __name__ == "__main__":
         full_case("cfg2", "tet70")
     if "--small-only" not in sys.argv or "--momentum" in sys.argv:
         momentum_case("cfg2", "tet70")
+        momentum_dilu_case("cfg2", "tet70")

@@ -1,0 +1,100 @@
+"""GPU tests of the level-scheduled triangular sweeps: the DIC/DILU preconditioner
+(LinearSolver.cpp:56-105) and the Gauss–Seidel / symmetric Gauss–Seidel solvers
+(LinearSolver.cpp:238-278). Bar: with exact sums, every solve is bit-identical to the
+oracle's sequential restatement (which the CPU tests pin to the reference's own
+solves); at full size the DILU-preconditioned PBiCGStab of the momentum matrix, the
+reference's default momentum solver, converges like the reference's.
+"""
+import numpy as np
+import pytest
+
+import voluma_amd as va
+from helpers import SMALL_MESHES, load_json, load_small
+from oracle import oracle as O
+from voluma_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+NAMES = list(SMALL_MESHES)
+
+CASES = [("PCG", "DIC", True), ("PBiCGStab", "DILU", False), ("PBiCGStab", "DIC", False),
+         ("PBiCGStab", "DILU", True), ("GaussSeidel", "none", True), ("symGaussSeidel", "none", True),
+         ("GaussSeidel", "none", False)]
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = va.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_sweeps_exact_mode_bit_exact_vs_oracle(ctx, name):
+    g = load_small(name)
+    m = SMALL_MESHES[name]()
+    ni = m.n_internal
+    lm = va.LduMesh(ctx, m.n_cells, m.owner[:ni], m.neighbour, None)
+    om = O.LduMesh(m.n_cells, m.owner[:ni], m.neighbour, None)
+    mats = {True: (g["sym_diag"], g["sym_upper"], None, g["sym_b"]),
+            False: (g["ns_diag"], g["ns_upper"], g["ns_lower"], g["ns_b"])}
+    for solver, pre, sym in CASES:
+        d, u, lo, b = mats[sym]
+        A = va.LduMatrix(lm, d, u, lo)
+        for nsw in ((1, 2) if "Gauss" in solver else (1,)):
+            tol, mx = (1e-10, 400) if "Gauss" not in solver else (1e-6, 60)
+            xo, po = om.solve(d, u, lo, b, solver=solver, preconditioner=pre, tolerance=tol, max_iter=mx,
+                              n_sweeps=nsw, mode=O.MODE_EXACT)
+            x = np.zeros(b.size)
+            p = va.solve_ldu(A, b, x, va.SolverControls(solver, pre, tol, 0.0, mx, nSweeps=nsw, sums=va.SUMS_EXACT))
+            assert p.iterations == po["iterations"], (solver, pre, sym, nsw)
+            assert p.finalResidual == po["finalResidual"] and p.initialResidual == po["initialResidual"]
+            assert np.array_equal(x, xo), (solver, pre, sym, nsw)
+        A.close()
+    lm.close()
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_dilu_fast_mode_and_reuse(ctx, name):
+    """Fast sums: the same number of iterations as exact sums within ±1 and the same
+    solution within 1e-9; a new matrix on the same mesh refactors (no stale factor)."""
+    g = load_small(name)
+    m = SMALL_MESHES[name]()
+    ni = m.n_internal
+    lm = va.LduMesh(ctx, m.n_cells, m.owner[:ni], m.neighbour, None)
+    A = va.LduMatrix(lm, g["ns_diag"], g["ns_upper"], g["ns_lower"])
+    b = g["ns_b"]
+    xe, xf = np.zeros(b.size), np.zeros(b.size)
+    pe = va.solve_ldu(A, b, xe, va.SolverControls("PBiCGStab", "DILU", 1e-11, 0.0, 400, sums=va.SUMS_EXACT))
+    pf = va.solve_ldu(A, b, xf, va.SolverControls("PBiCGStab", "DILU", 1e-11, 0.0, 400))
+    assert abs(pe.iterations - pf.iterations) <= 1 and pf.converged
+    assert np.linalg.norm(xf - xe) <= 1e-9 * np.linalg.norm(xe)
+    # same mesh, another matrix: scaled coefficients give the scaled-back solution
+    A.set(2.0 * g["ns_diag"], 2.0 * g["ns_upper"], 2.0 * g["ns_lower"])
+    x2 = np.zeros(b.size)
+    p2 = va.solve_ldu(A, 2.0 * b, x2, va.SolverControls("PBiCGStab", "DILU", 1e-11, 0.0, 400, sums=va.SUMS_EXACT))
+    assert p2.iterations == pe.iterations and np.array_equal(x2, xe)
+    # mixed precision with the DILU preconditioner in FP32
+    xm = np.zeros(b.size)
+    pm = va.solve_ldu(A, 2.0 * b, xm, va.SolverControls("PBiCGStab", "DILU", 1e-10, 0.0, 400, precision=va.MIXED))
+    assert pm.converged and np.linalg.norm(xm - xe) <= 1e-8 * np.linalg.norm(xe)
+    A.close()
+    lm.close()
+
+
+def test_full_size_momentum_dilu_tet70(ctx):
+    """The reference's default momentum solver (PBiCGStab + DILU, plain sums) on the
+    full-size momentum matrix: the reference's iteration count (±1) and the solution
+    within 1e-9 relative of the reference's at tolerance 1e-10."""
+    f = load_json("cfg2_tet70_momentum.json")
+    m = W.make_mesh("tet70")
+    lm, A, b, inp = W.momentum_system(ctx, m, seed=1, comp=0)
+    ref = f["components"][0]["plain_pbicgstab_dilu_1e-10"]
+    x = np.ascontiguousarray(inp["U0"][:, 0]).copy()
+    p = va.solve_ldu(A, b, x, va.SolverControls("PBiCGStab", "DILU", 1e-10, 0.0, 1000))
+    assert p.converged and abs(p.iterations - ref["iterations"]) <= 1, (p.iterations, ref["iterations"])
+    assert p.initialResidual == pytest.approx(ref["initialResidual"], rel=1e-10)
+    idx = np.array(ref["x"]["sample_idx"])
+    xs = np.array(ref["x"]["sample"])
+    assert np.abs(x[idx] - xs).max() <= 1e-9 * np.abs(xs).max()
+    A.close()
+    lm.close()

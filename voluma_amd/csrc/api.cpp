@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -53,6 +54,24 @@ struct vol_mesh_s {
     int nb = -1;
     DevBuf<int> bOff, bList;
     DevBuf<int> dOwn, dNei;  // device owner/neighbour, uploaded on first use
+    // level schedules and neighbour lists of the triangular sweeps (DIC/DILU,
+    // Gauss–Seidel), built on first use
+    std::mutex sweepMu;
+    bool sweepReady = false;
+    std::vector<int> levOffF, levOffB;  // host: cells of level L are perm[levOff[L] .. levOff[L+1])
+    DevBuf<int> permF, permB, lowOff, lowCol, lowSrc, upOff, upCol, upSrc;
+    SweepView sweepView() const {
+        SweepView v;
+        v.permF = permF.p;
+        v.permB = permB.p;
+        v.lowOff = lowOff.p;
+        v.lowCol = lowCol.p;
+        v.lowSrc = lowSrc.p;
+        v.upOff = upOff.p;
+        v.upCol = upCol.p;
+        v.upSrc = upSrc.p;
+        return v;
+    }
 };
 
 struct MatLevel {
@@ -76,6 +95,11 @@ struct vol_matrix_s {
     DevBuf<int> status;
     // FP32 copies for mixed precision, built lazily after each vol_matrix_set
     bool mixedReady = false, mixedGamg = false;
+    // DIC/DILU factor (per cell) and substitution coefficients (per list entry), built
+    // lazily after each vol_matrix_set; FP32 copies for mixed precision
+    bool diluReady = false, dilu32Ready = false;
+    DevBuf<double> dluD, dluR, dluPf, dluPb;
+    DevBuf<float> dluR32, dluPf32, dluPb32;
     DevBuf<float> coef32, diag32, rD32;
     DevBuf<TailLevel32> tail32;
     DevBuf<float> ir, iw, ip, iq, ie, ir0, iy, iAy, is, iz, it;  // inner-solve vectors
@@ -311,6 +335,109 @@ void toFloat(hipStream_t st, const double* a, float* w, size_t n) {
 
 // FP32 copies of the level-0 matrix and, with GAMG, of every coarse level and the tail
 // descriptors (the coarsest LU stays FP64). Values are rounded from the FP64 ones.
+// Level schedules of the triangular sweeps (sweep_kernels.hip). Lower list of cell u:
+// its neighbours l < u in ascending (l, face) order, the order of the reference's
+// (u, l)-sorted face list (LinearSolver.cpp:29-51); upper list of cell l: its neighbours
+// u > l in descending (u, face) order, the order of the backward face loop.
+void buildSweep(vol_mesh m) {
+    std::lock_guard<std::mutex> guard(m->sweepMu);
+    if (m->sweepReady) return;
+    const int n = m->n, nf = m->nf;
+    std::vector<int> lowOff(n + 1, 0), upOff(n + 1, 0);
+    for (int f = 0; f < nf; ++f) {
+        const int o = m->own[f], e = m->nei[f];
+        ++lowOff[std::max(o, e) + 1];
+        ++upOff[std::min(o, e) + 1];
+    }
+    for (int c = 0; c < n; ++c) {
+        lowOff[c + 1] += lowOff[c];
+        upOff[c + 1] += upOff[c];
+    }
+    // entries (neighbour, face); side of the list's own row: 0 when it owns the face
+    std::vector<std::pair<int, int>> low(nf), up(nf);
+    {
+        std::vector<int> fl(lowOff.begin(), lowOff.end() - 1), fu(upOff.begin(), upOff.end() - 1);
+        for (int f = 0; f < nf; ++f) {
+            const int o = m->own[f], e = m->nei[f];
+            const int l = std::min(o, e), u = std::max(o, e);
+            low[fl[u]++] = {l, f};
+            up[fu[l]++] = {u, f};
+        }
+    }
+    std::vector<int> lowCol(nf), lowSrc(nf), upCol(nf), upSrc(nf), levF(n, 0), levB(n, 0);
+    for (int c = 0; c < n; ++c) {
+        std::sort(low.begin() + lowOff[c], low.begin() + lowOff[c + 1]);
+        std::sort(up.begin() + upOff[c], up.begin() + upOff[c + 1], std::greater<>());
+        for (int k = lowOff[c]; k < lowOff[c + 1]; ++k) {
+            const int f = low[k].second;
+            lowCol[k] = low[k].first;
+            lowSrc[k] = (f << 1) | (m->own[f] == c ? 0 : 1);
+            levF[c] = std::max(levF[c], levF[low[k].first] + 1);
+        }
+        for (int k = upOff[c]; k < upOff[c + 1]; ++k) {
+            const int f = up[k].second;
+            upCol[k] = up[k].first;
+            upSrc[k] = (f << 1) | (m->own[f] == c ? 0 : 1);
+        }
+    }
+    for (int c = n - 1; c >= 0; --c)
+        for (int k = upOff[c]; k < upOff[c + 1]; ++k) levB[c] = std::max(levB[c], levB[upCol[k]] + 1);
+    auto order = [n](const std::vector<int>& lev, std::vector<int>& off, std::vector<int>& perm) {
+        const int nl = n ? *std::max_element(lev.begin(), lev.end()) + 1 : 0;
+        off.assign(nl + 1, 0);
+        for (int c = 0; c < n; ++c) ++off[lev[c] + 1];
+        for (int L = 0; L < nl; ++L) off[L + 1] += off[L];
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        perm.assign(n, 0);
+        for (int c = 0; c < n; ++c) perm[fill[lev[c]]++] = c;
+    };
+    std::vector<int> permF, permB;
+    order(levF, m->levOffF, permF);
+    order(levB, m->levOffB, permB);
+    hipStream_t st = m->ctx->stream;
+    m->permF.upload(permF, st);
+    m->permB.upload(permB, st);
+    m->lowOff.upload(lowOff, st);
+    m->upOff.upload(upOff, st);
+    if (nf) {
+        m->lowCol.upload(lowCol, st);
+        m->lowSrc.upload(lowSrc, st);
+        m->upCol.upload(upCol, st);
+        m->upSrc.upload(upSrc, st);
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    m->sweepReady = true;
+}
+
+// DIC/DILU factor of the matrix (LinearSolver.cpp:66-78) on the forward levels.
+void buildDilu(vol_matrix A) {
+    vol_mesh m = A->mesh;
+    buildSweep(m);
+    hipStream_t st = m->ctx->stream;
+    const size_t n = m->n, nf = m->nf;
+    A->dluD.alloc(n);
+    A->dluR.alloc(n);
+    A->dluPf.alloc(std::max<size_t>(nf, 1));
+    A->dluPb.alloc(std::max<size_t>(nf, 1));
+    launch_dilu_factor(st, m->sweepView(), m->levOffF.data(), int(m->levOffF.size()) - 1, A->diag.p, A->upper.p,
+                       A->symmetric ? A->upper.p : A->lower.p, A->dluD.p, A->dluR.p, A->dluPf.p, A->dluPb.p);
+    A->diluReady = true;
+    A->dilu32Ready = false;
+}
+
+void buildDilu32(vol_matrix A) {
+    vol_mesh m = A->mesh;
+    hipStream_t st = m->ctx->stream;
+    const size_t n = m->n, nf = std::max<size_t>(m->nf, 1);
+    A->dluR32.alloc(n);
+    A->dluPf32.alloc(nf);
+    A->dluPb32.alloc(nf);
+    toFloat(st, A->dluR.p, A->dluR32.p, n);
+    toFloat(st, A->dluPf.p, A->dluPf32.p, nf);
+    toFloat(st, A->dluPb.p, A->dluPb32.p, nf);
+    A->dilu32Ready = true;
+}
+
 void buildMixed(vol_matrix A, bool gamg) {
     vol_mesh m = A->mesh;
     hipStream_t st = m->ctx->stream;
@@ -380,6 +507,8 @@ void matrixFinish(vol_matrix A) {
     launch_vec(st, V_RECIP, v, false, m->ctx->red);
     A->gamgReady = false;
     A->mixedReady = false;
+    A->diluReady = false;
+    A->dilu32Ready = false;
     A->set = true;
 }
 
@@ -818,9 +947,23 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
 
 template <class T>
 void precondApply(const Ctx& C, vol_matrix A, const Plan& P, const T* r, T* w, int n) {
-    if (P.precond == "GAMG")
+    if (P.precond == "GAMG") {
         vcycle<T>(C, A, P, 0, r, w);
-    else
+    } else if (P.precond == "DIC" || P.precond == "DILU") {
+        vol_mesh m = A->mesh;
+        const T *rD, *pf, *pb;
+        if constexpr (std::is_same_v<T, float>) {
+            rD = A->dluR32.p;
+            pf = A->dluPf32.p;
+            pb = A->dluPb32.p;
+        } else {
+            rD = A->dluR.p;
+            pf = A->dluPf.p;
+            pb = A->dluPb.p;
+        }
+        launch_dilu_apply<T>(C.st, m->sweepView(), m->levOffF.data(), int(m->levOffF.size()) - 1,
+                             m->levOffB.data(), int(m->levOffB.size()) - 1, rD, pf, pb, r, w, C.done);
+    } else
         vec<T>(C, V_SCALE, n, false, r, nullptr, nullptr, nullptr, w, nullptr, nullptr,
                P.precond == "diagonal" ? level0(A, (T*)nullptr).rD : nullptr);
 }
@@ -857,8 +1000,8 @@ void krylovIteration(const Ctx& C, vol_matrix A, const Plan& P, const Krylov<T>&
             vcycle<T>(C, A, P, 0, k.r, k.w, k.r, C.c->red.part);
             fin(C, FIN_PCG_WRA, 1, false, sellParts);
         } else {
-            if (P.precond == "GAMG") {
-                vcycle<T>(C, A, P, 0, k.r, k.w);
+            if (P.precond == "GAMG" || P.precond == "DIC" || P.precond == "DILU") {
+                precondApply<T>(C, A, P, k.r, k.w, n);
                 vec<T>(C, V_DOT, n, ex, k.w, k.r);
             } else {
                 vec<T>(C, V_PRECOND_DOT, n, ex, k.r, nullptr, nullptr, nullptr, k.w, nullptr, nullptr,
@@ -949,20 +1092,14 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
     std::strncpy(perf.solver, P.solver.c_str(), sizeof(perf.solver) - 1);
 
     const bool isPCG = P.solver == "PCG", isBi = P.solver == "PBiCGStab", isJac = P.solver == "Jacobi";
-    if (!isPCG && !isBi && !isJac) {
-        if (P.solver == "GaussSeidel" || P.solver == "symGaussSeidel")
-            throw VolError(VOL_ERR_UNSUPPORTED,
-                           "sequential Gauss-Seidel solvers are not provided on the GPU; use Jacobi, PCG or PBiCGStab");
-        throw VolError(VOL_ERR_UNKNOWN_SOLVER, "unknown linear solver " + P.solver);
-    }
+    const bool isGS = P.solver == "GaussSeidel" || P.solver == "symGaussSeidel";
+    if (!isPCG && !isBi && !isJac && !isGS) throw VolError(VOL_ERR_UNKNOWN_SOLVER, "unknown linear solver " + P.solver);
     if (isPCG && !A->symmetric) throw VolError(VOL_ERR_PCG_NONSYMMETRIC, "PCG requires a symmetric matrix; use PBiCGStab");
     std::string pc = P.precond;
     if (isBi && pc == "DIC") pc = "DILU";  // LinearSolver.cpp:203
     if (isPCG || isBi) {
         if (pc != "none" && pc != "diagonal" && pc != "GAMG" && pc != "DIC" && pc != "DILU")
             throw VolError(VOL_ERR_UNKNOWN_PRECONDITIONER, "unknown preconditioner " + pc);
-        if (pc == "DIC" || pc == "DILU")
-            throw VolError(VOL_ERR_UNSUPPORTED, "DIC/DILU preconditioners are not provided on the GPU; use diagonal or GAMG");
         if (pc == "GAMG" && !m->hasGamg)
             throw VolError(VOL_ERR_NO_GAMG_WEIGHTS, "GAMG needs a mesh created with agglomeration weights");
         if (pc == "GAMG" && (P.nPre < 1 || P.nPost < 1 || !(P.omega > 0.0)))
@@ -972,7 +1109,7 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
     if (ct.maxIter < 0 || ct.nSweeps < 1) throw VolError(VOL_ERR_INVALID_ARGUMENT, "bad maxIter/nSweeps");
     if (mixed) {
         if (P.exact) throw VolError(VOL_ERR_UNSUPPORTED, "VOL_MIXED runs with fast sums only");
-        if (isJac) throw VolError(VOL_ERR_UNSUPPORTED, "VOL_MIXED needs PCG or PBiCGStab");
+        if (isJac || isGS) throw VolError(VOL_ERR_UNSUPPORTED, "VOL_MIXED needs PCG or PBiCGStab");
         if (!(ct.innerRelTol > 0.0 && ct.innerRelTol < 1.0))
             throw VolError(VOL_ERR_INVALID_ARGUMENT, "innerRelTol must lie in (0, 1)");
     }
@@ -1001,6 +1138,10 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
     readScalars(c, hs);
     if (!hs.done) {
         if (P.precond == "GAMG" && !A->gamgReady) buildGamgCoeffs(A);
+        const bool dilu = (isPCG || isBi) && (P.precond == "DIC" || P.precond == "DILU");
+        if (dilu && !A->diluReady) buildDilu(A);
+        if (dilu && mixed && !A->dilu32Ready) buildDilu32(A);
+        if (isGS) buildSweep(m);
         const int maxLoops = std::max(ct.maxIter, 1) + 2;
         if (mixed) {
             if (!A->mixedReady || (P.precond == "GAMG" && !A->mixedGamg)) buildMixed(A, P.precond == "GAMG");
@@ -1048,10 +1189,20 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
                 HIPCHECK(hipMemsetAsync(A->vp.p, 0, sizeof(double) * n, st));
             }
             runLoop(c, C, maxLoops, [&] { krylovIteration<double>(C, A, P, k, isPCG, ex); });
-        } else {  // Jacobi solver, LinearSolver.cpp:238-278
+        } else {  // Jacobi and Gauss–Seidel solvers, LinearSolver.cpp:238-278
             A->vw.alloc(n);
+            const bool sym = P.solver == "symGaussSeidel";
+            const int nLevF = int(m->levOffF.size()) - 1, nLevB = int(m->levOffB.size()) - 1;
             runLoop(c, C, maxLoops, [&] {
                 for (int sw = 0; sw < ct.nSweeps; ++sw) {
+                    if (isGS) {
+                        launch_gs_sweep(st, V, m->permF.p, m->levOffF.data(), nLevF, A->coef.p, A->diag.p, b, x,
+                                        C.done);
+                        if (sym)
+                            launch_gs_sweep(st, V, m->permB.p, m->levOffB.data(), nLevB, A->coef.p, A->diag.p, b, x,
+                                            C.done);
+                        continue;
+                    }
                     sell<double>(C, OP_JACREF, XL_PLAIN, V, A->coef.p, A->diag.p, x, b, A->vw.p);
                     vec<double>(C, V_COPY, n, ex, A->vw.p, nullptr, nullptr, nullptr, x);
                 }

@@ -1,0 +1,144 @@
+// Level-scheduled triangular sweeps: the DIC/DILU preconditioner (factor, forward and
+// backward substitution, LinearSolver.cpp:56-105) and the Gauss–Seidel solver sweeps
+// (LinearSolver.cpp:238-264).
+//
+// The reference runs these loops one cell (or one face) at a time. A cell's update reads
+// only neighbours with a smaller index (forward) or a larger one (backward), so cells
+// are grouped into levels: the forward level of a cell is one more than the largest
+// forward level among its lower-numbered neighbours, and no two cells of one level are
+// neighbours. One launch per level updates all its cells at once, each cell with the
+// same operands, in the same order, as the sequential loop, so the results are
+// bit-identical to it. Level lists and per-cell neighbour lists belong to the mesh;
+// the factor and the coefficient products of the substitutions belong to the matrix.
+#include "device_common.hpp"
+#include "vol_internal.hpp"
+
+namespace vol {
+
+// DIC/DILU factor (LinearSolver.cpp:67-74) for the cells of one forward level:
+// d_u = diag_u − Σ_{l<u, ascending (l, face)} (A_lu·A_ul) / d_l, rD_u = 1/d_u. The same
+// thread then forms the coefficient products the substitutions use, rD_u·A_ul for its
+// lower neighbours and rD_u·A_ul' for its upper ones (row u's own coefficients).
+__global__ void k_dilu_factor(SweepView s, int lo, int hi, const double* diag, const double* upper,
+                              const double* lower, double* dF, double* rD, double* pf, double* pb) {
+    const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= hi) return;
+    const int u = s.permF[i];
+    const int k0 = s.lowOff[u], k1 = s.lowOff[u + 1];
+    double d = diag[u];
+    for (int k = k0; k < k1; ++k) {
+        const int e = s.lowSrc[k], f = e >> 1;
+        const double aul = (e & 1) ? lower[f] : upper[f];  // row u, column l
+        const double alu = (e & 1) ? upper[f] : lower[f];  // row l, column u
+        d = d - (alu * aul) / dF[s.lowCol[k]];
+    }
+    dF[u] = d;
+    const double r = 1.0 / d;
+    rD[u] = r;
+    for (int k = k0; k < k1; ++k) {
+        const int e = s.lowSrc[k], f = e >> 1;
+        pf[k] = r * ((e & 1) ? lower[f] : upper[f]);
+    }
+    for (int k = s.upOff[u]; k < s.upOff[u + 1]; ++k) {
+        const int e = s.upSrc[k], f = e >> 1;
+        pb[k] = r * ((e & 1) ? lower[f] : upper[f]);
+    }
+}
+
+// Forward substitution (LinearSolver.cpp:93-98) for one forward level:
+// w_u = rD_u·r_u − Σ_{l<u, ascending (l, face)} (rD_u·A_ul)·w_l.
+template <class T>
+__global__ void k_dilu_forward(SweepView s, int lo, int hi, const T* rD, const T* pf, const T* r, T* w,
+                               const int* done) {
+    if (done && *done) return;
+    const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= hi) return;
+    const int u = s.permF[i];
+    T acc = rD[u] * r[u];
+    for (int k = s.lowOff[u]; k < s.lowOff[u + 1]; ++k) acc = acc - pf[k] * w[s.lowCol[k]];
+    w[u] = acc;
+}
+
+// Backward substitution (LinearSolver.cpp:99-103) for one backward level:
+// w_l −= (rD_l·A_lu)·w_u over the upper neighbours u > l in descending (u, face) order.
+template <class T>
+__global__ void k_dilu_backward(SweepView s, int lo, int hi, const T* pb, T* w, const int* done) {
+    if (done && *done) return;
+    const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= hi) return;
+    const int l = s.permB[i];
+    T acc = w[l];
+    for (int k = s.upOff[l]; k < s.upOff[l + 1]; ++k) acc = acc - pb[k] * w[s.upCol[k]];
+    w[l] = acc;
+}
+
+// Gauss–Seidel update of the cells of one level (LinearSolver.cpp:247-262), in place:
+// x_c = (b_c − Σ coef·x_nb over its faces in ascending order) / diag_c. Lower-numbered
+// neighbours already hold this sweep's values (forward) or the previous pass's
+// (backward), exactly as in the sequential loop.
+__global__ void k_gs_level(SellView S, const int* perm, int lo, int hi, const double* coef, const double* diag,
+                           const double* b, double* x, const int* done) {
+    if (done && *done) return;
+    const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= hi) return;
+    const int c = perm[i];
+    const int slice = c / WAVE, lane = c % WAVE;
+    const int s0 = S.sliceOff[slice], width = (S.sliceOff[slice + 1] - s0) / WAVE;
+    double r = b[c];
+    for (int k = 0; k < width; ++k) {
+        const int slot = s0 + k * WAVE + lane;
+        if (S.src[slot] < 0) continue;
+        r = r - coef[slot] * x[S.col[slot]];
+    }
+    x[c] = r / diag[c];
+}
+
+namespace {
+constexpr int SWEEP_THREADS = 256;
+inline dim3 sweep_grid(int cnt) { return dim3((cnt + SWEEP_THREADS - 1) / SWEEP_THREADS); }
+}  // namespace
+
+void launch_dilu_factor(hipStream_t st, const SweepView& s, const int* levOffF, int nLevF, const double* diag,
+                        const double* upper, const double* lower, double* dF, double* rD, double* pf, double* pb) {
+    for (int L = 0; L < nLevF; ++L) {
+        const int lo = levOffF[L], hi = levOffF[L + 1];
+        if (hi > lo)
+            hipLaunchKernelGGL(k_dilu_factor, sweep_grid(hi - lo), dim3(SWEEP_THREADS), 0, st, s, lo, hi, diag, upper,
+                               lower, dF, rD, pf, pb);
+    }
+}
+
+template <class T>
+void launch_dilu_apply(hipStream_t st, const SweepView& s, const int* levOffF, int nLevF, const int* levOffB,
+                       int nLevB, const T* rD, const T* pf, const T* pb, const T* r, T* w, const int* done) {
+    for (int L = 0; L < nLevF; ++L) {
+        const int lo = levOffF[L], hi = levOffF[L + 1];
+        if (hi > lo)
+            hipLaunchKernelGGL(k_dilu_forward<T>, sweep_grid(hi - lo), dim3(SWEEP_THREADS), 0, st, s, lo, hi, rD, pf, r,
+                               w, done);
+    }
+    // backward level 0 holds the cells without upper neighbours: nothing to subtract
+    for (int L = 1; L < nLevB; ++L) {
+        const int lo = levOffB[L], hi = levOffB[L + 1];
+        if (hi > lo)
+            hipLaunchKernelGGL(k_dilu_backward<T>, sweep_grid(hi - lo), dim3(SWEEP_THREADS), 0, st, s, lo, hi, pb, w,
+                               done);
+    }
+}
+template void launch_dilu_apply<double>(hipStream_t, const SweepView&, const int*, int, const int*, int,
+                                        const double*, const double*, const double*, const double*, double*,
+                                        const int*);
+template void launch_dilu_apply<float>(hipStream_t, const SweepView&, const int*, int, const int*, int, const float*,
+                                       const float*, const float*, const float*, float*, const int*);
+
+void launch_gs_sweep(hipStream_t st, const SellView& S, const int* perm, const int* levOff, int nLev,
+                     const double* coef, const double* diag, const double* b, double* x, const int* done) {
+    for (int L = 0; L < nLev; ++L) {
+        const int lo = levOff[L], hi = levOff[L + 1];
+        if (hi > lo)
+            hipLaunchKernelGGL(k_gs_level, sweep_grid(hi - lo), dim3(SWEEP_THREADS), 0, st, S, perm, lo, hi, coef, diag,
+                               b, x, done);
+    }
+}
+
+}  // namespace vol

@@ -331,6 +331,33 @@ struct FvmArgs {
 void launch_fvm_term(hipStream_t st, int kind, const FvmArgs& a, const int* own, const int* nei);
 // FvMatrix −= su (FvMatrix.h:93-97): source[c] += su[c]·V[c]
 void launch_fvm_source(hipStream_t st, int n, const double* su, const double* V, double* source);
+// ------------------------------------------------------------------ level-scheduled sweeps
+// Cells in forward / backward level order and their lower / upper neighbour lists
+// (device arrays). Lower list of cell u: neighbours l < u in ascending (l, face) order;
+// upper list of cell l: neighbours u > l in descending (u, face) order; src = face << 1 |
+// side of the list's own row (0: upper[face], 1: lower[face]).
+struct SweepView {
+    const int* permF = nullptr;
+    const int* permB = nullptr;
+    const int* lowOff = nullptr;
+    const int* lowCol = nullptr;
+    const int* lowSrc = nullptr;
+    const int* upOff = nullptr;
+    const int* upCol = nullptr;
+    const int* upSrc = nullptr;
+};
+// DIC/DILU factor over the forward levels (levOffF: host array of nLevF + 1 offsets):
+// dF, rD = 1/dF per cell, pf per lower-list entry, pb per upper-list entry
+void launch_dilu_factor(hipStream_t st, const SweepView& s, const int* levOffF, int nLevF, const double* diag,
+                        const double* upper, const double* lower, double* dF, double* rD, double* pf, double* pb);
+// w = M⁻¹ r of DIC/DILU: forward then backward substitution, every launch a no-op once *done
+template <class T>
+void launch_dilu_apply(hipStream_t st, const SweepView& s, const int* levOffF, int nLevF, const int* levOffB,
+                       int nLevB, const T* rD, const T* pf, const T* pb, const T* r, T* w, const int* done);
+// one Gauss–Seidel pass over the levels of perm (forward: permF, backward: permB)
+void launch_gs_sweep(hipStream_t st, const SellView& S, const int* perm, const int* levOff, int nLev,
+                     const double* coef, const double* diag, const double* b, double* x, const int* done);
+
 // FvMatrix::solve fold (FvMatrix.h:154-163): d = diag + Σ ic, b = source + Σ bs per cell;
 // su != nullptr: the fold of a copy of the matrix with −= su (b = source + su·V + Σ bs)
 void launch_fvm_fold(hipStream_t st, const FvmArgs& a, double* dOut, double* bOut, const double* su = nullptr,
