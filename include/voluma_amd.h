@@ -90,11 +90,22 @@ typedef struct {
     /* extensions */
     int sums;           /* VOL_SUMS_*, default VOL_SUMS_CONTEXT */
     int precision;      /* VOL_FP64 | VOL_MIXED */
-    int gamgPreSweeps;  /* damped-Jacobi GAMG smoother sweeps (default 2, ≥ 1) */
-    int gamgPostSweeps; /* default 2, ≥ 1 */
-    double gamgOmega;   /* damping (default 0.8) */
+    int gamgPreSweeps;  /* GAMG smoother sweeps before the coarse correction (default 2, ≥ 1) */
+    int gamgPostSweeps; /* ... and after it (default 2, ≥ 1) */
+    double gamgOmega;   /* damped-Jacobi smoother: damping (default 0.8) */
     double innerRelTol; /* VOL_MIXED: residual reduction per inner solve (default 1e-2, in (0, 1)) */
+    int gamgSmoother;   /* VOL_GAMG_JACOBI (default) | VOL_GAMG_GAUSS_SEIDEL */
+    int reserved;
 } vol_solver_controls;
+
+/* ---- GAMG smoother (vol_solver_controls.gamgSmoother) */
+/* damped Jacobi x ← x + ω(D⁻¹(r − (A − D)x) − x), parallel on every level */
+#define VOL_GAMG_JACOBI 0
+/* the reference's smoother (GAMG.cpp:363-384,423-442): x = 0, gamgPreSweeps forward
+ * Gauss–Seidel sweeps, coarse correction, gamgPostSweeps backward sweeps, each level
+ * run as level-scheduled sweeps; with 2/2 sweeps and exact sums the V-cycle is the
+ * reference's bit for bit */
+#define VOL_GAMG_GAUSS_SEIDEL 1
 
 /* SolverPerformance (LinearSolver.h:25-31) */
 typedef struct {

@@ -98,3 +98,62 @@ def test_full_size_momentum_dilu_tet70(ctx):
     assert np.abs(x[idx] - xs).max() <= 1e-9 * np.abs(xs).max()
     A.close()
     lm.close()
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_gamg_gauss_seidel_exact_mode_bit_exact_vs_oracle(ctx, name):
+    """GAMG with the reference's smoother (Gauss–Seidel, 2 forward + 2 backward sweeps,
+    row-ordered coarsest substitution): with exact sums the PCG and PBiCGStab solves
+    are bit-identical to the oracle's restatement of the reference's GAMG."""
+    g = load_small(name)
+    m = SMALL_MESHES[name]()
+    ni = m.n_internal
+    lm = va.LduMesh(ctx, m.n_cells, m.owner[:ni], m.neighbour, m.agglomeration_weights())
+    om = O.LduMesh(m.n_cells, m.owner[:ni], m.neighbour, m.agglomeration_weights())
+    d, u, b = g["sym_diag"], g["sym_upper"], g["sym_b"]
+    A = va.LduMatrix(lm, d, u)
+    for solver in ("PCG", "PBiCGStab"):
+        xo, po = om.solve(d, u, None, b, solver=solver, preconditioner="GAMG", tolerance=1e-10, max_iter=200,
+                          mode=O.MODE_EXACT, smoother=O.SMOOTHER_GS)
+        x = np.zeros(b.size)
+        p = va.solve_ldu(A, b, x, va.SolverControls(solver, "GAMG", 1e-10, 0.0, 200, sums=va.SUMS_EXACT,
+                                                    gamgSmoother=va.GAMG_GAUSS_SEIDEL))
+        assert p.iterations == po["iterations"] and p.finalResidual == po["finalResidual"], solver
+        assert np.array_equal(x, xo), solver
+    A.close()
+    lm.close()
+
+
+@pytest.mark.parametrize("name", NAMES)
+def test_gamg_gauss_seidel_fast_mode_vs_reference(ctx, name):
+    """Fast sums with the Gauss–Seidel GAMG smoother against the reference's own GAMG
+    solves (plain sums): iterations within ±1, solution within 1e-8; mixed precision
+    with the same smoother converges to the same solution."""
+    g = load_small(name)
+    m = SMALL_MESHES[name]()
+    ni = m.n_internal
+    lm = va.LduMesh(ctx, m.n_cells, m.owner[:ni], m.neighbour, m.agglomeration_weights())
+    A = va.LduMatrix(lm, g["sym_diag"], g["sym_upper"])
+    b = g["sym_b"]
+    from helpers import solve_cases
+    seen = 0
+    for i, solver, pre, repro, sym in solve_cases(g):
+        if pre != "GAMG" or repro or not sym:
+            continue
+        x = np.zeros(b.size)
+        p = va.solve_ldu(A, b, x, va.SolverControls(solver, "GAMG", 1e-10, 0.0, 400,
+                                                    gamgSmoother=va.GAMG_GAUSS_SEIDEL))
+        ref = g[f"solve{i}_perf"]
+        assert abs(p.iterations - int(ref[4])) <= 1, (solver, p.iterations, ref[4])
+        xr = g[f"solve{i}_x"]
+        assert np.linalg.norm(x - xr) <= 1e-8 * np.linalg.norm(xr), solver
+        seen += 1
+    assert seen == 2
+    xm = np.zeros(b.size)
+    pm = va.solve_ldu(A, b, xm, va.SolverControls("PCG", "GAMG", 1e-10, 0.0, 400, precision=va.MIXED,
+                                                  gamgSmoother=va.GAMG_GAUSS_SEIDEL))
+    assert pm.converged and np.linalg.norm(xm - xr) <= 1e-8 * np.linalg.norm(xr)
+    with pytest.raises(va.VolumaError):
+        va.solve_ldu(A, b, xm, va.SolverControls("PCG", "GAMG", gamgSmoother=7))
+    A.close()
+    lm.close()

@@ -27,6 +27,7 @@ VOL_OK = 0
 VOL_HOST, VOL_DEVICE = 0, 1
 SUMS_CONTEXT, SUMS_FAST, SUMS_REPRODUCIBLE, SUMS_EXACT = -1, 0, 1, 2
 FP64, MIXED = 0, 1
+GAMG_JACOBI, GAMG_GAUSS_SEIDEL = 0, 1
 
 _ERRORS = {
     -1: "invalid argument", -2: "unknown solver", -3: "unknown preconditioner", -4: "PCG on nonsymmetric matrix",
@@ -47,7 +48,7 @@ class _Controls(ctypes.Structure):
                 ("relTol", ctypes.c_double), ("maxIter", ctypes.c_int), ("minIter", ctypes.c_int),
                 ("nSweeps", ctypes.c_int), ("sums", ctypes.c_int), ("precision", ctypes.c_int),
                 ("gamgPreSweeps", ctypes.c_int), ("gamgPostSweeps", ctypes.c_int), ("gamgOmega", ctypes.c_double),
-                ("innerRelTol", ctypes.c_double)]
+                ("innerRelTol", ctypes.c_double), ("gamgSmoother", ctypes.c_int), ("reserved", ctypes.c_int)]
 
 
 class _NonOrth(ctypes.Structure):
@@ -184,12 +185,13 @@ class SolverControls:
     gamgPostSweeps: int = 2
     gamgOmega: float = 0.8
     innerRelTol: float = 1e-2
+    gamgSmoother: int = 0  # GAMG_JACOBI | GAMG_GAUSS_SEIDEL
 
     def _c(self):
         self._keep = (self.solver.encode(), self.preconditioner.encode())
         return _Controls(self._keep[0], self._keep[1], self.tolerance, self.relTol, self.maxIter, self.minIter,
                          self.nSweeps, self.sums, self.precision, self.gamgPreSweeps, self.gamgPostSweeps,
-                         self.gamgOmega, self.innerRelTol)
+                         self.gamgOmega, self.innerRelTol, self.gamgSmoother, 0)
 
 
 @dataclasses.dataclass

@@ -60,6 +60,14 @@ struct vol_mesh_s {
     bool sweepReady = false;
     std::vector<int> levOffF, levOffB;  // host: cells of level L are perm[levOff[L] .. levOff[L+1])
     DevBuf<int> permF, permB, lowOff, lowCol, lowSrc, upOff, upCol, upSrc;
+    // Gauss–Seidel GAMG smoother: level orders of the coarse GAMG levels 1 .. nL−2
+    // (index = GAMG level; level 0 uses the fields above)
+    struct LevelOrder {
+        std::vector<int> offF, offB;
+        DevBuf<int> permF, permB;
+    };
+    bool gsLevelsReady = false;
+    std::vector<std::unique_ptr<LevelOrder>> gsLev;
     SweepView sweepView() const {
         SweepView v;
         v.permF = permF.p;
@@ -409,6 +417,65 @@ void buildSweep(vol_mesh m) {
     m->sweepReady = true;
 }
 
+// Forward / backward level order of a graph given by its edges (l[k] < u[k]): cells by
+// level, then by index; offF/offB hold the level offsets.
+void levelOrder(int n, const std::vector<int>& l, const std::vector<int>& u, std::vector<int>& offF,
+                std::vector<int>& permF, std::vector<int>& offB, std::vector<int>& permB) {
+    const int nf = int(l.size());
+    std::vector<int> lo(n + 1, 0), hi(n + 1, 0), lc(nf), hc(nf);
+    for (int k = 0; k < nf; ++k) {
+        ++lo[u[k] + 1];
+        ++hi[l[k] + 1];
+    }
+    for (int c = 0; c < n; ++c) {
+        lo[c + 1] += lo[c];
+        hi[c + 1] += hi[c];
+    }
+    {
+        std::vector<int> fl(lo.begin(), lo.end() - 1), fh(hi.begin(), hi.end() - 1);
+        for (int k = 0; k < nf; ++k) {
+            lc[fl[u[k]]++] = l[k];
+            hc[fh[l[k]]++] = u[k];
+        }
+    }
+    std::vector<int> levF(n, 0), levB(n, 0);
+    for (int c = 0; c < n; ++c)
+        for (int k = lo[c]; k < lo[c + 1]; ++k) levF[c] = std::max(levF[c], levF[lc[k]] + 1);
+    for (int c = n - 1; c >= 0; --c)
+        for (int k = hi[c]; k < hi[c + 1]; ++k) levB[c] = std::max(levB[c], levB[hc[k]] + 1);
+    auto order = [n](const std::vector<int>& lev, std::vector<int>& off, std::vector<int>& perm) {
+        const int nl = n ? *std::max_element(lev.begin(), lev.end()) + 1 : 0;
+        off.assign(nl + 1, 0);
+        for (int c = 0; c < n; ++c) ++off[lev[c] + 1];
+        for (int L = 0; L < nl; ++L) off[L + 1] += off[L];
+        std::vector<int> fill(off.begin(), off.end() - 1);
+        perm.assign(n, 0);
+        for (int c = 0; c < n; ++c) perm[fill[lev[c]]++] = c;
+    };
+    order(levF, offF, permF);
+    order(levB, offB, permB);
+}
+
+// Level orders of the coarse GAMG levels for the Gauss–Seidel smoother.
+void buildGsLevels(vol_mesh m) {
+    buildSweep(m);
+    std::lock_guard<std::mutex> guard(m->sweepMu);
+    if (m->gsLevelsReady) return;
+    const int nL = int(m->lv.size());
+    m->gsLev.clear();
+    m->gsLev.resize(std::max(nL, 1));
+    for (int k = 1; k + 1 < nL; ++k) {
+        auto o = std::make_unique<vol_mesh_s::LevelOrder>();
+        std::vector<int> pf, pb;
+        levelOrder(m->H[k].n, m->H[k].l, m->H[k].u, o->offF, pf, o->offB, pb);
+        o->permF.upload(pf, m->ctx->stream);
+        o->permB.upload(pb, m->ctx->stream);
+        m->gsLev[k] = std::move(o);
+    }
+    HIPCHECK(hipStreamSynchronize(m->ctx->stream));
+    m->gsLevelsReady = true;
+}
+
 // DIC/DILU factor of the matrix (LinearSolver.cpp:66-78) on the forward levels.
 void buildDilu(vol_matrix A) {
     vol_mesh m = A->mesh;
@@ -717,6 +784,7 @@ void fvmFinish(vol_matrix A, double* b, const double* su, const double* V, int l
 struct Plan {
     std::string solver, precond;
     bool exact = false;
+    bool gamgGS = false;  // Gauss–Seidel GAMG smoother
     int nPre = 2, nPost = 2;
     double omega = 0.8;
 };
@@ -945,9 +1013,54 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
     }
 }
 
+// One V-cycle at level l with the reference's Gauss–Seidel smoother (GAMG.cpp:403-443):
+// x = 0, nPre forward sweeps, residual r − A·x, restriction, recursion, x += xc[agg],
+// nPost backward sweeps. x is xout itself.
+template <class T> void vcycleGS(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xout) {
+    vol_mesh m = A->mesh;
+    const int nL = int(m->lv.size());
+    if (l == nL - 1) {
+        launch_coarse_solve_rows<T>(C.st, A->nCoarse, A->lu.p, r, xout, C.done);
+        return;
+    }
+    const LevelDev& D = *m->lv[l];
+    const SellView V = l == 0 ? m->sell.view() : D.sell.view();
+    const LevelPtrs<T> L = levelPtrs(A, l, (T*)nullptr);
+    const int n = l == 0 ? m->n : D.n;
+    const int *pF, *pB, *oF, *oB;
+    int nF, nB;
+    if (l == 0) {
+        pF = m->permF.p;
+        pB = m->permB.p;
+        oF = m->levOffF.data();
+        oB = m->levOffB.data();
+        nF = int(m->levOffF.size()) - 1;
+        nB = int(m->levOffB.size()) - 1;
+    } else {
+        const vol_mesh_s::LevelOrder& o = *m->gsLev[l];
+        pF = o.permF.p;
+        pB = o.permB.p;
+        oF = o.offF.data();
+        oB = o.offB.data();
+        nF = int(o.offF.size()) - 1;
+        nB = int(o.offB.size()) - 1;
+    }
+    launch_fill_zero<T>(C.st, n, xout, C.done);
+    for (int s = 0; s < P.nPre; ++s) launch_gs_sweep<T>(C.st, V, pF, oF, nF, L.coef, L.diag, r, xout, C.done);
+    sell<T>(C, OP_RESID, XL_PLAIN, V, L.coef, L.diag, xout, r, L.bufA, P.omega, nullptr, nullptr, nullptr, nullptr,
+            nullptr, L.rD);
+    const LevelPtrs<T> LC = levelPtrs(A, l + 1, (T*)nullptr);
+    restrictLevel(C, m->lv[l + 1]->n, D, L.bufA, LC.r);
+    vcycleGS<T>(C, A, P, l + 1, LC.r, LC.x);
+    launch_prolong<T>(C.st, n, D.agg.p, LC.x, xout, C.done);
+    for (int s = 0; s < P.nPost; ++s) launch_gs_sweep<T>(C.st, V, pB, oB, nB, L.coef, L.diag, r, xout, C.done);
+}
+
 template <class T>
 void precondApply(const Ctx& C, vol_matrix A, const Plan& P, const T* r, T* w, int n) {
-    if (P.precond == "GAMG") {
+    if (P.precond == "GAMG" && P.gamgGS) {
+        vcycleGS<T>(C, A, P, 0, r, w);
+    } else if (P.precond == "GAMG") {
         vcycle<T>(C, A, P, 0, r, w);
     } else if (P.precond == "DIC" || P.precond == "DILU") {
         vol_mesh m = A->mesh;
@@ -996,7 +1109,7 @@ void krylovIteration(const Ctx& C, vol_matrix A, const Plan& P, const Krylov<T>&
     const int sellParts = sell_grid_blocks(A->mesh->sell.nslices);
     const Level0<T> L0 = level0(A, (T*)nullptr);
     if (isPCG) {  // LinearSolver.cpp:176-201
-        if (P.precond == "GAMG" && fuse && A->tailStart != 0) {  // Σ w·r fused into the last sweep
+        if (P.precond == "GAMG" && !P.gamgGS && fuse && A->tailStart != 0) {  // Σ w·r fused into the last sweep
             vcycle<T>(C, A, P, 0, k.r, k.w, k.r, C.c->red.part);
             fin(C, FIN_PCG_WRA, 1, false, sellParts);
         } else {
@@ -1085,6 +1198,9 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
     P.nPre = ct.gamgPreSweeps;
     P.nPost = ct.gamgPostSweeps;
     P.omega = ct.gamgOmega;
+    if (ct.gamgSmoother != VOL_GAMG_JACOBI && ct.gamgSmoother != VOL_GAMG_GAUSS_SEIDEL)
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "gamgSmoother must be VOL_GAMG_JACOBI or VOL_GAMG_GAUSS_SEIDEL");
+    P.gamgGS = ct.gamgSmoother == VOL_GAMG_GAUSS_SEIDEL;
     if (ct.precision != VOL_FP64 && ct.precision != VOL_MIXED)
         throw VolError(VOL_ERR_INVALID_ARGUMENT, "precision must be VOL_FP64 or VOL_MIXED");
     const bool mixed = ct.precision == VOL_MIXED;
@@ -1138,6 +1254,7 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
     readScalars(c, hs);
     if (!hs.done) {
         if (P.precond == "GAMG" && !A->gamgReady) buildGamgCoeffs(A);
+        if (P.precond == "GAMG" && P.gamgGS) buildGsLevels(m);
         const bool dilu = (isPCG || isBi) && (P.precond == "DIC" || P.precond == "DILU");
         if (dilu && !A->diluReady) buildDilu(A);
         if (dilu && mixed && !A->dilu32Ready) buildDilu32(A);
@@ -1294,6 +1411,8 @@ void vol_controls_default(vol_solver_controls* c) {
     c->gamgPostSweeps = 2;
     c->gamgOmega = 0.8;
     c->innerRelTol = 1e-2;
+    c->gamgSmoother = VOL_GAMG_JACOBI;
+    c->reserved = 0;
 }
 
 const char* vol_last_error(void) { return gLastError.c_str(); }

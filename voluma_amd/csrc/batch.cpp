@@ -564,7 +564,7 @@ void bsolve(vol_batch B, const double* b, double* x, const vol_solver_controls& 
     if (!B->symmetric) throw VolError(VOL_ERR_PCG_NONSYMMETRIC, "PCG requires symmetric matrices");
     if (P.precond != "GAMG" && P.precond != "diagonal" && P.precond != "none") {
         if (P.precond == "DIC" || P.precond == "DILU")
-            throw VolError(VOL_ERR_UNSUPPORTED, "DIC/DILU preconditioners are not provided on the GPU");
+            throw VolError(VOL_ERR_UNSUPPORTED, "batched solves provide the GAMG, diagonal and none preconditioners");
         throw VolError(VOL_ERR_UNKNOWN_PRECONDITIONER, "unknown preconditioner " + P.precond);
     }
     const int sums = ct.sums == VOL_SUMS_CONTEXT ? (B->ctx->reproducible ? VOL_SUMS_REPRODUCIBLE : VOL_SUMS_FAST)
@@ -580,6 +580,8 @@ void bsolve(vol_batch B, const double* b, double* x, const vol_solver_controls& 
     P.omega = ct.gamgOmega;
     if (P.precond == "GAMG" && (P.nPre < 1 || P.nPost < 1 || !(P.omega > 0.0)))
         throw VolError(VOL_ERR_INVALID_ARGUMENT, "GAMG smoother needs ≥ 1 pre/post sweep and ω > 0");
+    if (P.precond == "GAMG" && ct.gamgSmoother != VOL_GAMG_JACOBI)
+        throw VolError(VOL_ERR_UNSUPPORTED, "batched solves smooth GAMG levels with damped Jacobi only");
     if (ct.maxIter < 0) throw VolError(VOL_ERR_INVALID_ARGUMENT, "bad maxIter");
     if (mixed && !B->mixedReady) buildMixedB(B);
     DevScalars* S = B->S.p;

@@ -72,25 +72,97 @@ __global__ void k_dilu_backward(SweepView s, int lo, int hi, const T* pb, T* w, 
     w[l] = acc;
 }
 
-// Gauss–Seidel update of the cells of one level (LinearSolver.cpp:247-262), in place:
-// x_c = (b_c − Σ coef·x_nb over its faces in ascending order) / diag_c. Lower-numbered
+// Gauss–Seidel update of the cells of one level (LinearSolver.cpp:247-262; the GAMG
+// smoother GAMG.cpp:363-384 has the same row), in place: x_c = (b_c − Σ coef·x_nb over
+// its slots, which hold its faces in ascending order) / diag_c. Lower-numbered
 // neighbours already hold this sweep's values (forward) or the previous pass's
-// (backward), exactly as in the sequential loop.
-__global__ void k_gs_level(SellView S, const int* perm, int lo, int hi, const double* coef, const double* diag,
-                           const double* b, double* x, const int* done) {
+// (backward), exactly as in the sequential loop. The slot walk covers the three
+// layouts of SellView (CSR rows, ELL, SELL slices).
+template <class T>
+__global__ void k_gs_level(SellView S, const int* perm, int lo, int hi, const T* coef, const T* diag, const T* b,
+                           T* x, const int* done) {
     if (done && *done) return;
     const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= hi) return;
     const int c = perm[i];
-    const int slice = c / WAVE, lane = c % WAVE;
-    const int s0 = S.sliceOff[slice], width = (S.sliceOff[slice + 1] - s0) / WAVE;
-    double r = b[c];
+    int s0, width, stride;
+    if (S.csrG > 0) {
+        s0 = S.sliceOff[c];
+        width = S.sliceOff[c + 1] - s0;
+        stride = 1;
+    } else if (S.ellWidth > 0) {
+        width = S.ellWidth;
+        s0 = (c / WAVE) * WAVE * width + c % WAVE;
+        stride = WAVE;
+    } else {
+        const int b0 = S.sliceOff[c / WAVE];
+        width = (S.sliceOff[c / WAVE + 1] - b0) / WAVE;
+        s0 = b0 + c % WAVE;
+        stride = WAVE;
+    }
+    T r = b[c];
     for (int k = 0; k < width; ++k) {
-        const int slot = s0 + k * WAVE + lane;
-        if (S.src[slot] < 0) continue;
-        r = r - coef[slot] * x[S.col[slot]];
+        const int slot = s0 + k * stride;
+        int j;
+        if (S.col16) {
+            const int d = int(S.col16[slot]);
+            j = d == COL16_PAD ? -1 : c + d;
+        } else {
+            j = S.col[slot];
+        }
+        if (j >= 0) r = r - coef[slot] * x[j];
     }
     x[c] = r / diag[c];
+}
+
+// x = 0 and x[i] += xc[agg[i]] (GAMG.cpp:423,438) for the Gauss–Seidel V-cycle.
+template <class T>
+__global__ void k_fill_zero(int n, T* x, const int* done) {
+    if (done && *done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = T(0);
+}
+template <class T>
+__global__ void k_prolong(int n, const int* agg, const T* xc, T* x, const int* done) {
+    if (done && *done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = x[i] + xc[agg[i]];
+}
+
+// Coarsest-level solve in the reference's order (GAMG.cpp:406-421): forward substitution
+// (rows ascending, each subtracting in ascending k: done column by column by all lanes,
+// which keeps every row's order), then backward substitution row by row from the last,
+// each row subtracting in ascending k before its division. A row's sum starts with the
+// value the previous row just produced, so the backward pass is one dependent chain,
+// run by lane 0 with the factors in LDS when they fit (n ≤ 64).
+template <class T>
+__global__ __launch_bounds__(64) void k_coarse_solve_rows(int n, const double* lu, const T* r, T* x,
+                                                          const int* done) {
+    if (done && *done) return;
+    extern __shared__ double sh[];  // g (n), then the factors (n·n) when n ≤ 64
+    const bool small = n <= 64;
+    double* gv = sh;
+    double* f = sh + n;
+    for (int i = threadIdx.x; i < n; i += blockDim.x) gv[i] = double(r[i]);
+    if (small)
+        for (int i = threadIdx.x; i < n * n; i += blockDim.x) f[i] = lu[i];
+    __syncthreads();
+    const double* a = small ? f : lu;
+    for (int k = 0; k < n; ++k) {
+        const double gk = gv[k];
+        for (int i = k + 1 + threadIdx.x; i < n; i += blockDim.x) gv[i] -= a[size_t(i) * n + k] * gk;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int i = n - 1; i >= 0; --i) {
+            double acc = gv[i];
+            const double* row = a + size_t(i) * n;
+            for (int k = i + 1; k < n; ++k) acc -= row[k] * gv[k];
+            gv[i] = acc / row[i];
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = T(gv[i]);
 }
 
 namespace {
@@ -131,14 +203,41 @@ template void launch_dilu_apply<double>(hipStream_t, const SweepView&, const int
 template void launch_dilu_apply<float>(hipStream_t, const SweepView&, const int*, int, const int*, int, const float*,
                                        const float*, const float*, const float*, float*, const int*);
 
-void launch_gs_sweep(hipStream_t st, const SellView& S, const int* perm, const int* levOff, int nLev,
-                     const double* coef, const double* diag, const double* b, double* x, const int* done) {
+template <class T>
+void launch_gs_sweep(hipStream_t st, const SellView& S, const int* perm, const int* levOff, int nLev, const T* coef,
+                     const T* diag, const T* b, T* x, const int* done) {
     for (int L = 0; L < nLev; ++L) {
         const int lo = levOff[L], hi = levOff[L + 1];
         if (hi > lo)
-            hipLaunchKernelGGL(k_gs_level, sweep_grid(hi - lo), dim3(SWEEP_THREADS), 0, st, S, perm, lo, hi, coef, diag,
-                               b, x, done);
+            hipLaunchKernelGGL(k_gs_level<T>, sweep_grid(hi - lo), dim3(SWEEP_THREADS), 0, st, S, perm, lo, hi, coef,
+                               diag, b, x, done);
     }
 }
+template void launch_gs_sweep<double>(hipStream_t, const SellView&, const int*, const int*, int, const double*,
+                                      const double*, const double*, double*, const int*);
+template void launch_gs_sweep<float>(hipStream_t, const SellView&, const int*, const int*, int, const float*,
+                                     const float*, const float*, float*, const int*);
+
+template <class T>
+void launch_coarse_solve_rows(hipStream_t st, int n, const double* lu, const T* r, T* x, const int* done) {
+    const size_t lds = sizeof(double) * (size_t(n) + (n <= 64 ? size_t(n) * n : 0));
+    hipLaunchKernelGGL(k_coarse_solve_rows<T>, dim3(1), dim3(64), lds, st, n, lu, r, x, done);
+}
+template void launch_coarse_solve_rows<double>(hipStream_t, int, const double*, const double*, double*, const int*);
+template void launch_coarse_solve_rows<float>(hipStream_t, int, const double*, const float*, float*, const int*);
+
+template <class T>
+void launch_fill_zero(hipStream_t st, int n, T* x, const int* done) {
+    if (n > 0) hipLaunchKernelGGL(k_fill_zero<T>, sweep_grid(n), dim3(SWEEP_THREADS), 0, st, n, x, done);
+}
+template void launch_fill_zero<double>(hipStream_t, int, double*, const int*);
+template void launch_fill_zero<float>(hipStream_t, int, float*, const int*);
+
+template <class T>
+void launch_prolong(hipStream_t st, int n, const int* agg, const T* xc, T* x, const int* done) {
+    if (n > 0) hipLaunchKernelGGL(k_prolong<T>, sweep_grid(n), dim3(SWEEP_THREADS), 0, st, n, agg, xc, x, done);
+}
+template void launch_prolong<double>(hipStream_t, int, const int*, const double*, double*, const int*);
+template void launch_prolong<float>(hipStream_t, int, const int*, const float*, float*, const int*);
 
 }  // namespace vol

@@ -355,8 +355,15 @@ template <class T>
 void launch_dilu_apply(hipStream_t st, const SweepView& s, const int* levOffF, int nLevF, const int* levOffB,
                        int nLevB, const T* rD, const T* pf, const T* pb, const T* r, T* w, const int* done);
 // one Gauss–Seidel pass over the levels of perm (forward: permF, backward: permB)
-void launch_gs_sweep(hipStream_t st, const SellView& S, const int* perm, const int* levOff, int nLev,
-                     const double* coef, const double* diag, const double* b, double* x, const int* done);
+template <class T>
+void launch_gs_sweep(hipStream_t st, const SellView& S, const int* perm, const int* levOff, int nLev, const T* coef,
+                     const T* diag, const T* b, T* x, const int* done);
+// coarsest solve with the LU factors in the reference's row order (Gauss–Seidel V-cycle)
+template <class T>
+void launch_coarse_solve_rows(hipStream_t st, int n, const double* lu, const T* r, T* x, const int* done);
+// x = 0; x[i] += xc[agg[i]] (Gauss–Seidel V-cycle), no-ops once *done
+template <class T> void launch_fill_zero(hipStream_t st, int n, T* x, const int* done);
+template <class T> void launch_prolong(hipStream_t st, int n, const int* agg, const T* xc, T* x, const int* done);
 
 // FvMatrix::solve fold (FvMatrix.h:154-163): d = diag + Σ ic, b = source + Σ bs per cell;
 // su != nullptr: the fold of a copy of the matrix with −= su (b = source + su·V + Σ bs)
