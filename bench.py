@@ -133,6 +133,68 @@ def serve_leg(va, ctx, args):
     return out
 
 
+def cpu_baseline_dilu(m, d, u, lo, b, x0):
+    """The same momentum solve (PBiCGStab + DILU to 1e-10, plain sums) by the oracle on
+    one host core."""
+    from oracle import oracle as O
+    om = O.LduMesh(m.n_cells, m.owner[: m.n_internal], m.neighbour, None)
+    t = time.perf_counter()
+    _, po = om.solve(d, u, lo, b, x0, solver="PBiCGStab", preconditioner="DILU", tolerance=1e-10, max_iter=1000)
+    return {"ms_per_solve": (time.perf_counter() - t) * 1e3, "iterations": po["iterations"], "cores": 1,
+            "kind": "port"}
+
+
+def reference_solvers_leg(va, W, ctx, m, A, db, dx, dzero, args, cpu):
+    """The reference's own solver choices on the GPU: GAMG-PCG with its Gauss–Seidel
+    smoother (2/2 sweeps) on the pressure system, and its default momentum solver
+    (PBiCGStab + DILU) on the momentum matrix of velocity component 0, both as
+    level-scheduled sweeps; with cpu, the momentum solve's CPU baseline too."""
+    out = {}
+    ctl = va.SolverControls("PCG", "GAMG", args.tol, 0.0, 1000, gamgSmoother=va.GAMG_GAUSS_SEIDEL)
+    for k in range(2):
+        dx.copy_from(dzero)
+        va.solve_ldu(A, db, dx, ctl)
+    ctx.synchronize()
+    steps = max(2, args.steps // 4)
+    t = time.perf_counter()
+    its = []
+    for k in range(steps):
+        dx.copy_from(dzero)
+        p = va.solve_ldu(A, db, dx, ctl)
+        its.append(p.iterations)
+    ctx.synchronize()
+    el = time.perf_counter() - t
+    out["gamg_gauss_seidel"] = {"solves_per_s": steps / el, "ms_per_solve": el / steps * 1e3,
+                                "iterations": int(np.median(its)), "converged": bool(p.converged)}
+    lm2, A2, b2, inp = W.momentum_system(ctx, m, seed=1, comp=0)
+    x0 = np.ascontiguousarray(inp["U0"][:, 0])
+    dx0, db2, dxm = va.DeviceArray.from_host(ctx, x0), va.DeviceArray.from_host(ctx, b2), va.DeviceArray(ctx, m.n_cells)
+    mctl = va.SolverControls("PBiCGStab", "DILU", 1e-10, 0.0, 1000)
+    for k in range(2):
+        dxm.copy_from(dx0)
+        va.solve_ldu(A2, db2, dxm, mctl)
+    ctx.synchronize()
+    t = time.perf_counter()
+    for k in range(steps):
+        dxm.copy_from(dx0)
+        p = va.solve_ldu(A2, db2, dxm, mctl)
+    ctx.synchronize()
+    el = time.perf_counter() - t
+    out["momentum_pbicgstab_dilu"] = {"ms_per_solve": el / steps * 1e3, "iterations": p.iterations,
+                                      "tolerance": 1e-10, "converged": bool(p.converged)}
+    if cpu:
+        d, u, lo = A2.get()
+        out["momentum_pbicgstab_dilu"]["cpu_baseline"] = cpu_baseline_dilu(m, d, u, lo, b2, x0)
+    for a in (dx0, db2, dxm):
+        a.free()
+    A2.close()
+    lm2.close()
+    log(f"reference solvers: GS-GAMG {out['gamg_gauss_seidel']['ms_per_solve']:.1f} ms "
+        f"({out['gamg_gauss_seidel']['iterations']} it), DILU momentum "
+        f"{out['momentum_pbicgstab_dilu']['ms_per_solve']:.1f} ms ({p.iterations} it)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -145,6 +207,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-ref-solvers", action="store_true", help="skip the GS-GAMG and DILU legs")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="request arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--pre", type=int, default=1)
@@ -263,6 +326,9 @@ def main():
     serve = None
     if not args.no_serve:
         serve = serve_leg(va, ctx, args)
+    ref_solvers = None
+    if not args.no_ref_solvers:
+        ref_solvers = reference_solvers_leg(va, W, ctx, m, A, db, dx, dzero, args, not args.no_cpu_baseline)
 
     out = {
         "metric": METRIC,
@@ -285,6 +351,7 @@ def main():
         "mixed": mixed,
         "batch": batch,
         "serve": serve,
+        "reference_solvers": ref_solvers,
         "iterations": int(np.median(iters)),
         "final_residual": p.finalResidual,
         "true_residual": true_res,

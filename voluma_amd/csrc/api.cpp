@@ -58,20 +58,19 @@ struct vol_mesh_s {
     // Gauss–Seidel), built on first use
     std::mutex sweepMu;
     bool sweepReady = false;
-    std::vector<int> levOffF, levOffB;  // host: cells of level L are perm[levOff[L] .. levOff[L+1])
-    DevBuf<int> permF, permB, lowOff, lowCol, lowSrc, upOff, upCol, upSrc;
-    // Gauss–Seidel GAMG smoother: level orders of the coarse GAMG levels 1 .. nL−2
-    // (index = GAMG level; level 0 uses the fields above)
+    LevelOrderDev ordF;  // forward level order of level 0 (DIC/DILU factor)
+    DevBuf<int> lowOff, lowCol, lowSrc, upOff, upCol, upSrc;
+    LevelEllDev diluF, diluB;  // substitution rows (entry source: list index)
+    LevelEllDev gsF, gsB;      // Gauss–Seidel rows of level 0 (entry source: SELL slot)
+    // Gauss–Seidel GAMG smoother: rows of the coarse GAMG levels 1 .. nL−2 (index =
+    // GAMG level; level 0 uses gsF/gsB)
     struct LevelOrder {
-        std::vector<int> offF, offB;
-        DevBuf<int> permF, permB;
+        LevelEllDev F, B;
     };
     bool gsLevelsReady = false;
     std::vector<std::unique_ptr<LevelOrder>> gsLev;
     SweepView sweepView() const {
         SweepView v;
-        v.permF = permF.p;
-        v.permB = permB.p;
         v.lowOff = lowOff.p;
         v.lowCol = lowCol.p;
         v.lowSrc = lowSrc.p;
@@ -85,6 +84,8 @@ struct vol_mesh_s {
 struct MatLevel {
     DevBuf<double> diag, rD, Fu, Fl, coef, bufA, bufB, r, x;
     DevBuf<float> diag32, rD32, coef32, bufA32, bufB32, r32, x32;  // mixed precision
+    DevBuf<double> gsCF, gsCB;  // Gauss–Seidel smoother coefficients (LevelEll layouts)
+    DevBuf<float> gsCF32, gsCB32;
 };
 
 struct vol_matrix_s {
@@ -106,8 +107,13 @@ struct vol_matrix_s {
     // DIC/DILU factor (per cell) and substitution coefficients (per list entry), built
     // lazily after each vol_matrix_set; FP32 copies for mixed precision
     bool diluReady = false, dilu32Ready = false;
-    DevBuf<double> dluD, dluR, dluPf, dluPb;
-    DevBuf<float> dluR32, dluPf32, dluPb32;
+    DevBuf<double> dluD, dluR, dluPf, dluPb, dluEf, dluEb;  // Ef/Eb: pf/pb in the LevelEll layouts
+    DevBuf<float> dluR32, dluEf32, dluEb32;
+    // Gauss–Seidel coefficients in the LevelEll layouts: level 0 (solver and smoother)
+    // and the coarse levels (smoother), FP32 copies for mixed precision
+    bool gsReady = false, gsLevReady = false, gs32Ready = false;
+    DevBuf<double> gsCF, gsCB;
+    DevBuf<float> gsCF32, gsCB32;
     DevBuf<float> coef32, diag32, rD32;
     DevBuf<TailLevel32> tail32;
     DevBuf<float> ir, iw, ip, iq, ie, ir0, iy, iAy, is, iz, it;  // inner-solve vectors
@@ -399,12 +405,21 @@ void buildSweep(vol_mesh m) {
         perm.assign(n, 0);
         for (int c = 0; c < n; ++c) perm[fill[lev[c]]++] = c;
     };
-    std::vector<int> permF, permB;
-    order(levF, m->levOffF, permF);
-    order(levB, m->levOffB, permB);
+    std::vector<int> permF, permB, offF, offB;
+    order(levF, offF, permF);
+    order(levB, offB, permB);
     hipStream_t st = m->ctx->stream;
-    m->permF.upload(permF, st);
-    m->permB.upload(permB, st);
+    m->ordF.build(offF, permF, st);
+    std::vector<int> idx(nf);
+    for (int k = 0; k < nf; ++k) idx[k] = k;
+    m->diluF.build(offF, permF, lowOff, lowCol, idx, st);
+    m->diluB.build(offB, permB, upOff, upCol, idx, st);
+    {
+        std::vector<int> ro, rc, rs;
+        sellRows(m->sell, ro, rc, rs);
+        m->gsF.build(offF, permF, ro, rc, rs, st);
+        m->gsB.build(offB, permB, ro, rc, rs, st);
+    }
     m->lowOff.upload(lowOff, st);
     m->upOff.upload(upOff, st);
     if (nf) {
@@ -466,10 +481,11 @@ void buildGsLevels(vol_mesh m) {
     m->gsLev.resize(std::max(nL, 1));
     for (int k = 1; k + 1 < nL; ++k) {
         auto o = std::make_unique<vol_mesh_s::LevelOrder>();
-        std::vector<int> pf, pb;
-        levelOrder(m->H[k].n, m->H[k].l, m->H[k].u, o->offF, pf, o->offB, pb);
-        o->permF.upload(pf, m->ctx->stream);
-        o->permB.upload(pb, m->ctx->stream);
+        std::vector<int> pf, pb, of, ob, ro, rc, rs;
+        levelOrder(m->H[k].n, m->H[k].l, m->H[k].u, of, pf, ob, pb);
+        sellRows(m->lv[k]->sell, ro, rc, rs);
+        o->F.build(of, pf, ro, rc, rs, m->ctx->stream);
+        o->B.build(ob, pb, ro, rc, rs, m->ctx->stream);
         m->gsLev[k] = std::move(o);
     }
     HIPCHECK(hipStreamSynchronize(m->ctx->stream));
@@ -486,22 +502,74 @@ void buildDilu(vol_matrix A) {
     A->dluR.alloc(n);
     A->dluPf.alloc(std::max<size_t>(nf, 1));
     A->dluPb.alloc(std::max<size_t>(nf, 1));
-    launch_dilu_factor(st, m->sweepView(), m->levOffF.data(), int(m->levOffF.size()) - 1, A->diag.p, A->upper.p,
+    launch_dilu_factor(st, m->sweepView(), m->ordF.view(), A->diag.p, A->upper.p,
                        A->symmetric ? A->upper.p : A->lower.p, A->dluD.p, A->dluR.p, A->dluPf.p, A->dluPb.p);
+    A->dluEf.alloc(std::max<size_t>(m->diluF.nEntries, 1));
+    A->dluEb.alloc(std::max<size_t>(m->diluB.nEntries, 1));
+    launch_gather_entries<double>(st, m->diluF.nEntries, m->diluF.src.p, A->dluPf.p, A->dluEf.p);
+    launch_gather_entries<double>(st, m->diluB.nEntries, m->diluB.src.p, A->dluPb.p, A->dluEb.p);
     A->diluReady = true;
     A->dilu32Ready = false;
+}
+
+// Gauss–Seidel coefficients of the matrix in the LevelEll layouts: level 0, and the
+// coarse GAMG levels when the GAMG smoother needs them.
+void buildGs(vol_matrix A, bool gamgLevels) {
+    vol_mesh m = A->mesh;
+    buildSweep(m);
+    hipStream_t st = m->ctx->stream;
+    if (!A->gsReady) {
+        A->gsCF.alloc(std::max<size_t>(m->gsF.nEntries, 1));
+        A->gsCB.alloc(std::max<size_t>(m->gsB.nEntries, 1));
+        launch_gather_entries<double>(st, m->gsF.nEntries, m->gsF.src.p, A->coef.p, A->gsCF.p);
+        launch_gather_entries<double>(st, m->gsB.nEntries, m->gsB.src.p, A->coef.p, A->gsCB.p);
+        A->gsReady = true;
+        A->gs32Ready = false;
+    }
+    if (gamgLevels && !A->gsLevReady) {
+        const int nL = int(m->lv.size());
+        for (int k = 1; k + 1 < nL; ++k) {
+            const vol_mesh_s::LevelOrder& o = *m->gsLev[k];
+            MatLevel& M = *A->L[k];
+            M.gsCF.alloc(std::max<size_t>(o.F.nEntries, 1));
+            M.gsCB.alloc(std::max<size_t>(o.B.nEntries, 1));
+            launch_gather_entries<double>(st, o.F.nEntries, o.F.src.p, M.coef.p, M.gsCF.p);
+            launch_gather_entries<double>(st, o.B.nEntries, o.B.src.p, M.coef.p, M.gsCB.p);
+        }
+        A->gsLevReady = true;
+        A->gs32Ready = false;
+    }
+}
+
+void buildGs32(vol_matrix A) {
+    vol_mesh m = A->mesh;
+    hipStream_t st = m->ctx->stream;
+    A->gsCF32.alloc(std::max<size_t>(m->gsF.nEntries, 1));
+    A->gsCB32.alloc(std::max<size_t>(m->gsB.nEntries, 1));
+    launch_gather_entries<float>(st, m->gsF.nEntries, m->gsF.src.p, A->coef32.p, A->gsCF32.p);
+    launch_gather_entries<float>(st, m->gsB.nEntries, m->gsB.src.p, A->coef32.p, A->gsCB32.p);
+    const int nL = int(m->lv.size());
+    for (int k = 1; k + 1 < nL; ++k) {
+        const vol_mesh_s::LevelOrder& o = *m->gsLev[k];
+        MatLevel& M = *A->L[k];
+        M.gsCF32.alloc(std::max<size_t>(o.F.nEntries, 1));
+        M.gsCB32.alloc(std::max<size_t>(o.B.nEntries, 1));
+        launch_gather_entries<float>(st, o.F.nEntries, o.F.src.p, M.coef32.p, M.gsCF32.p);
+        launch_gather_entries<float>(st, o.B.nEntries, o.B.src.p, M.coef32.p, M.gsCB32.p);
+    }
+    A->gs32Ready = true;
 }
 
 void buildDilu32(vol_matrix A) {
     vol_mesh m = A->mesh;
     hipStream_t st = m->ctx->stream;
-    const size_t n = m->n, nf = std::max<size_t>(m->nf, 1);
+    const size_t n = m->n, ef = std::max<size_t>(m->diluF.nEntries, 1), eb = std::max<size_t>(m->diluB.nEntries, 1);
     A->dluR32.alloc(n);
-    A->dluPf32.alloc(nf);
-    A->dluPb32.alloc(nf);
+    A->dluEf32.alloc(ef);
+    A->dluEb32.alloc(eb);
     toFloat(st, A->dluR.p, A->dluR32.p, n);
-    toFloat(st, A->dluPf.p, A->dluPf32.p, nf);
-    toFloat(st, A->dluPb.p, A->dluPb32.p, nf);
+    toFloat(st, A->dluEf.p, A->dluEf32.p, ef);
+    toFloat(st, A->dluEb.p, A->dluEb32.p, eb);
     A->dilu32Ready = true;
 }
 
@@ -576,6 +644,9 @@ void matrixFinish(vol_matrix A) {
     A->mixedReady = false;
     A->diluReady = false;
     A->dilu32Ready = false;
+    A->gsReady = false;
+    A->gsLevReady = false;
+    A->gs32Ready = false;
     A->set = true;
 }
 
@@ -1027,33 +1098,25 @@ template <class T> void vcycleGS(const Ctx& C, vol_matrix A, const Plan& P, int 
     const SellView V = l == 0 ? m->sell.view() : D.sell.view();
     const LevelPtrs<T> L = levelPtrs(A, l, (T*)nullptr);
     const int n = l == 0 ? m->n : D.n;
-    const int *pF, *pB, *oF, *oB;
-    int nF, nB;
-    if (l == 0) {
-        pF = m->permF.p;
-        pB = m->permB.p;
-        oF = m->levOffF.data();
-        oB = m->levOffB.data();
-        nF = int(m->levOffF.size()) - 1;
-        nB = int(m->levOffB.size()) - 1;
+    const LevelEll sF = l == 0 ? m->gsF.view() : m->gsLev[l]->F.view();
+    const LevelEll sB = l == 0 ? m->gsB.view() : m->gsLev[l]->B.view();
+    const T *cF, *cB;
+    if constexpr (std::is_same_v<T, float>) {
+        cF = l == 0 ? A->gsCF32.p : A->L[l]->gsCF32.p;
+        cB = l == 0 ? A->gsCB32.p : A->L[l]->gsCB32.p;
     } else {
-        const vol_mesh_s::LevelOrder& o = *m->gsLev[l];
-        pF = o.permF.p;
-        pB = o.permB.p;
-        oF = o.offF.data();
-        oB = o.offB.data();
-        nF = int(o.offF.size()) - 1;
-        nB = int(o.offB.size()) - 1;
+        cF = l == 0 ? A->gsCF.p : A->L[l]->gsCF.p;
+        cB = l == 0 ? A->gsCB.p : A->L[l]->gsCB.p;
     }
     launch_fill_zero<T>(C.st, n, xout, C.done);
-    for (int s = 0; s < P.nPre; ++s) launch_gs_sweep<T>(C.st, V, pF, oF, nF, L.coef, L.diag, r, xout, C.done);
+    for (int s = 0; s < P.nPre; ++s) launch_gs_sweep<T>(C.st, sF, cF, L.diag, r, xout, C.done);
     sell<T>(C, OP_RESID, XL_PLAIN, V, L.coef, L.diag, xout, r, L.bufA, P.omega, nullptr, nullptr, nullptr, nullptr,
             nullptr, L.rD);
     const LevelPtrs<T> LC = levelPtrs(A, l + 1, (T*)nullptr);
     restrictLevel(C, m->lv[l + 1]->n, D, L.bufA, LC.r);
     vcycleGS<T>(C, A, P, l + 1, LC.r, LC.x);
     launch_prolong<T>(C.st, n, D.agg.p, LC.x, xout, C.done);
-    for (int s = 0; s < P.nPost; ++s) launch_gs_sweep<T>(C.st, V, pB, oB, nB, L.coef, L.diag, r, xout, C.done);
+    for (int s = 0; s < P.nPost; ++s) launch_gs_sweep<T>(C.st, sB, cB, L.diag, r, xout, C.done);
 }
 
 template <class T>
@@ -1064,18 +1127,17 @@ void precondApply(const Ctx& C, vol_matrix A, const Plan& P, const T* r, T* w, i
         vcycle<T>(C, A, P, 0, r, w);
     } else if (P.precond == "DIC" || P.precond == "DILU") {
         vol_mesh m = A->mesh;
-        const T *rD, *pf, *pb;
+        const T *rD, *ef, *eb;
         if constexpr (std::is_same_v<T, float>) {
             rD = A->dluR32.p;
-            pf = A->dluPf32.p;
-            pb = A->dluPb32.p;
+            ef = A->dluEf32.p;
+            eb = A->dluEb32.p;
         } else {
             rD = A->dluR.p;
-            pf = A->dluPf.p;
-            pb = A->dluPb.p;
+            ef = A->dluEf.p;
+            eb = A->dluEb.p;
         }
-        launch_dilu_apply<T>(C.st, m->sweepView(), m->levOffF.data(), int(m->levOffF.size()) - 1,
-                             m->levOffB.data(), int(m->levOffB.size()) - 1, rD, pf, pb, r, w, C.done);
+        launch_dilu_apply<T>(C.st, m->diluF.view(), m->diluB.view(), rD, ef, eb, r, w, C.done);
     } else
         vec<T>(C, V_SCALE, n, false, r, nullptr, nullptr, nullptr, w, nullptr, nullptr,
                P.precond == "diagonal" ? level0(A, (T*)nullptr).rD : nullptr);
@@ -1255,6 +1317,7 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
     if (!hs.done) {
         if (P.precond == "GAMG" && !A->gamgReady) buildGamgCoeffs(A);
         if (P.precond == "GAMG" && P.gamgGS) buildGsLevels(m);
+        if ((P.precond == "GAMG" && P.gamgGS) || isGS) buildGs(A, P.precond == "GAMG" && P.gamgGS);
         const bool dilu = (isPCG || isBi) && (P.precond == "DIC" || P.precond == "DILU");
         if (dilu && !A->diluReady) buildDilu(A);
         if (dilu && mixed && !A->dilu32Ready) buildDilu32(A);
@@ -1262,6 +1325,7 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
         const int maxLoops = std::max(ct.maxIter, 1) + 2;
         if (mixed) {
             if (!A->mixedReady || (P.precond == "GAMG" && !A->mixedGamg)) buildMixed(A, P.precond == "GAMG");
+            if (P.precond == "GAMG" && P.gamgGS && !A->gs32Ready) buildGs32(A);
             for (DevBuf<float>* v : {&A->ir, &A->iw, &A->ip, &A->iq, &A->ie}) v->alloc(n);
             if (isBi)
                 for (DevBuf<float>* v : {&A->ir0, &A->iy, &A->iAy, &A->is, &A->iz, &A->it}) v->alloc(n);
@@ -1309,15 +1373,11 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
         } else {  // Jacobi and Gauss–Seidel solvers, LinearSolver.cpp:238-278
             A->vw.alloc(n);
             const bool sym = P.solver == "symGaussSeidel";
-            const int nLevF = int(m->levOffF.size()) - 1, nLevB = int(m->levOffB.size()) - 1;
             runLoop(c, C, maxLoops, [&] {
                 for (int sw = 0; sw < ct.nSweeps; ++sw) {
                     if (isGS) {
-                        launch_gs_sweep(st, V, m->permF.p, m->levOffF.data(), nLevF, A->coef.p, A->diag.p, b, x,
-                                        C.done);
-                        if (sym)
-                            launch_gs_sweep(st, V, m->permB.p, m->levOffB.data(), nLevB, A->coef.p, A->diag.p, b, x,
-                                            C.done);
+                        launch_gs_sweep<double>(st, m->gsF.view(), A->gsCF.p, A->diag.p, b, x, C.done);
+                        if (sym) launch_gs_sweep<double>(st, m->gsB.view(), A->gsCB.p, A->diag.p, b, x, C.done);
                         continue;
                     }
                     sell<double>(C, OP_JACREF, XL_PLAIN, V, A->coef.p, A->diag.p, x, b, A->vw.p);

@@ -56,6 +56,120 @@ template <class T> struct DevBuf {
     }
 };
 
+// One level order of the triangular sweeps in host and device form (LevelSched view):
+// offsets and permutation, and the launch groups — runs of consecutive levels of at most
+// SWEEP_RUN_MAX cells (one workgroup) and single larger levels (a grid each).
+struct LevelOrderDev {
+    std::vector<int> off, grp;
+    std::vector<char> run;
+    DevBuf<int> perm, offD;
+    void build(const std::vector<int>& offH, const std::vector<int>& permH, hipStream_t st) {
+        off = offH;
+        const int nl = int(off.size()) - 1;
+        grp.clear();
+        run.clear();
+        for (int L = 0; L < nl;) {
+            grp.push_back(L);
+            if (off[L + 1] - off[L] > SWEEP_RUN_MAX) {
+                run.push_back(0);
+                ++L;
+            } else {
+                run.push_back(1);
+                while (L < nl && off[L + 1] - off[L] <= SWEEP_RUN_MAX) ++L;
+            }
+        }
+        grp.push_back(std::max(nl, 0));
+        perm.upload(permH, st);
+        offD.upload(off, st);
+    }
+    LevelSched view() const {
+        LevelSched v;
+        v.perm = perm.p;
+        v.offD = offD.p;
+        v.offH = off.data();
+        v.nLev = int(off.size()) - 1;
+        v.grpH = grp.data();
+        v.grpRunH = run.data();
+        v.nGrp = int(grp.size()) - 1;
+        return v;
+    }
+};
+
+// Level-ordered ELL rows of a triangular sweep (LevelEll view). Rows are given as a
+// CSR by cell (rowOff, rowCol, rowSrc: neighbour and coefficient source index of each
+// entry, in the sweep's order); src keeps the source index of every entry (−1 padding)
+// for the per-matrix coefficient gather.
+struct LevelEllDev {
+    std::vector<int> off, width, base, grp;
+    std::vector<char> kind;
+    DevBuf<int> cell, col, src, offD, widthD, baseD;
+    size_t nEntries = 0;
+    void build(const std::vector<int>& levOff, const std::vector<int>& perm, const std::vector<int>& rowOff,
+               const std::vector<int>& rowCol, const std::vector<int>& rowSrc, hipStream_t st) {
+        off = levOff;
+        const int nl = int(off.size()) - 1;
+        width.assign(std::max(nl, 0), 0);
+        base.assign(std::max(nl, 0), 0);
+        size_t total = 0;
+        for (int L = 0; L < nl; ++L) {
+            int w = 0;
+            for (int i = off[L]; i < off[L + 1]; ++i) w = std::max(w, rowOff[perm[i] + 1] - rowOff[perm[i]]);
+            width[L] = w;
+            base[L] = int(total);
+            total += size_t(w) * size_t(off[L + 1] - off[L]);
+        }
+        if (total >= (size_t(1) << 31)) throw VolError(VOL_ERR_INVALID_ARGUMENT, "sweep rows exceed 2^31 entries");
+        nEntries = total;
+        std::vector<int> ec(total, -1), es(total, -1);
+        for (int L = 0; L < nl; ++L) {
+            const int size = off[L + 1] - off[L];
+            for (int t = 0; t < size; ++t) {
+                const int c = perm[off[L] + t];
+                for (int k = 0; k < rowOff[c + 1] - rowOff[c]; ++k) {
+                    const size_t e = size_t(base[L]) + size_t(k) * size + t;
+                    ec[e] = rowCol[rowOff[c] + k];
+                    es[e] = rowSrc[rowOff[c] + k];
+                }
+            }
+        }
+        grp.clear();
+        kind.clear();
+        auto kindOf = [](int size) { return size <= 1024 ? ELL_RUN1 : ELL_GRID; };
+        for (int L = 0; L < nl;) {
+            const int k = kindOf(off[L + 1] - off[L]);
+            grp.push_back(L);
+            kind.push_back(char(k));
+            ++L;
+            if (k != ELL_GRID)
+                while (L < nl && kindOf(off[L + 1] - off[L]) == k) ++L;
+        }
+        grp.push_back(std::max(nl, 0));
+        cell.upload(perm, st);
+        col.upload(ec, st);
+        src.upload(es, st);
+        offD.upload(off, st);
+        widthD.upload(width, st);
+        baseD.upload(base, st);
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+    LevelEll view() const {
+        LevelEll v;
+        v.cell = cell.p;
+        v.col = col.p;
+        v.offD = offD.p;
+        v.widthD = widthD.p;
+        v.baseD = baseD.p;
+        v.offH = off.data();
+        v.widthH = width.data();
+        v.baseH = base.data();
+        v.nLev = int(off.size()) - 1;
+        v.grpH = grp.data();
+        v.grpKindH = kind.data();
+        v.nGrp = int(grp.size()) - 1;
+        return v;
+    }
+};
+
 struct SellDev {
     int n = 0, nslices = 0, nslots = 0, maxWidth = 0, csrG = 0, csrIt = 0, ellWidth = 0;
     DevBuf<int> sliceOff, col, src;
@@ -128,6 +242,43 @@ inline void buildSell(int n, const std::vector<int>& off, const std::vector<int>
 
 // Row lists → CSR with G lanes per row (coarse levels): G·IT ≥ longest row, IT ≤ 8.
 // Returns false when rows are too long for it (then SELL is used).
+// Entries of every row of S in slot order (the order of its gathers): neighbour and slot
+// index, as a CSR by row (read back from the device).
+inline void sellRows(const SellDev& S, std::vector<int>& off, std::vector<int>& col, std::vector<int>& slot) {
+    const int n = S.n;
+    const int nOff = S.csrG > 0 ? n + 1 : S.nslices + 1;
+    std::vector<int> so(nOff), sc(S.nslots);
+    HIPCHECK(hipMemcpy(so.data(), S.sliceOff.p, sizeof(int) * so.size(), hipMemcpyDeviceToHost));
+    if (S.nslots) HIPCHECK(hipMemcpy(sc.data(), S.col.p, sizeof(int) * sc.size(), hipMemcpyDeviceToHost));
+    off.assign(n + 1, 0);
+    col.clear();
+    slot.clear();
+    for (int r = 0; r < n; ++r) {
+        int s0, width, stride;
+        if (S.csrG > 0) {
+            s0 = so[r];
+            width = so[r + 1] - s0;
+            stride = 1;
+        } else if (S.ellWidth > 0) {
+            width = S.ellWidth;
+            s0 = (r / WAVE) * WAVE * width + r % WAVE;
+            stride = WAVE;
+        } else {
+            width = (so[r / WAVE + 1] - so[r / WAVE]) / WAVE;
+            s0 = so[r / WAVE] + r % WAVE;
+            stride = WAVE;
+        }
+        for (int k = 0; k < width; ++k) {
+            const int sl = s0 + k * stride;
+            if (sc[sl] >= 0) {
+                col.push_back(sc[sl]);
+                slot.push_back(sl);
+            }
+        }
+        off[r + 1] = int(col.size());
+    }
+}
+
 inline bool buildCsr(int n, const std::vector<int>& off, const std::vector<int>& colL, const std::vector<int>& srcL,
               SellDev& S, hipStream_t st) {
     int maxW = 0;

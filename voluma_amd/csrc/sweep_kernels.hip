@@ -1,133 +1,250 @@
 // Level-scheduled triangular sweeps: the DIC/DILU preconditioner (factor, forward and
-// backward substitution, LinearSolver.cpp:56-105) and the Gauss–Seidel solver sweeps
-// (LinearSolver.cpp:238-264).
+// backward substitution, LinearSolver.cpp:56-105), the Gauss–Seidel solver sweeps
+// (LinearSolver.cpp:238-264) and the Gauss–Seidel GAMG smoother (GAMG.cpp:363-384).
 //
 // The reference runs these loops one cell (or one face) at a time. A cell's update reads
 // only neighbours with a smaller index (forward) or a larger one (backward), so cells
 // are grouped into levels: the forward level of a cell is one more than the largest
 // forward level among its lower-numbered neighbours, and no two cells of one level are
-// neighbours. One launch per level updates all its cells at once, each cell with the
-// same operands, in the same order, as the sequential loop, so the results are
-// bit-identical to it. Level lists and per-cell neighbour lists belong to the mesh;
-// the factor and the coefficient products of the substitutions belong to the matrix.
+// neighbours. All cells of a level are updated at once, each with the same operands, in
+// the same order, as in the sequential loop, so the results are bit-identical to it.
+//
+// Levels are short (10²–10⁴ cells on the meshes here) and there are hundreds of them,
+// so the cost of a level is the latency of its dependent loads, plus the launch when it
+// has one. The substitutions and Gauss–Seidel sweeps therefore read a level-ordered ELL
+// copy of their rows (LevelEll): the cells of a level are contiguous, and entry k of
+// the level's row t sits at base + k·size + t, so a row needs its cell index, its
+// entries and the gathers of its neighbours' values, and no per-row offsets. Runs of
+// consecutive levels of at most 1024 cells are swept by one workgroup of 1024 threads,
+// a row per thread, in a single launch, with a barrier between levels (global writes of
+// a workgroup are visible to it after the barrier). Larger levels get a grid launch
+// each: a level of ~10⁴ rows moves ~1 MB, more than one CU streams in the time of a
+// launch (measured: 16 rows per thread in one workgroup was 3× slower than grid
+// launches on 2M-cell meshes). The factor, run once per matrix, walks the neighbour
+// lists level by level (LevelSched).
 #include "device_common.hpp"
 #include "vol_internal.hpp"
 
 namespace vol {
 
-// DIC/DILU factor (LinearSolver.cpp:67-74) for the cells of one forward level:
-// d_u = diag_u − Σ_{l<u, ascending (l, face)} (A_lu·A_ul) / d_l, rD_u = 1/d_u. The same
-// thread then forms the coefficient products the substitutions use, rD_u·A_ul for its
-// lower neighbours and rD_u·A_ul' for its upper ones (row u's own coefficients).
-__global__ void k_dilu_factor(SweepView s, int lo, int hi, const double* diag, const double* upper,
-                              const double* lower, double* dF, double* rD, double* pf, double* pb) {
+namespace {
+constexpr int SWEEP_THREADS = 256;
+constexpr int RUN_THREADS = 1024;
+inline dim3 sweep_grid(int cnt) { return dim3((cnt + SWEEP_THREADS - 1) / SWEEP_THREADS); }
+}  // namespace
+
+// One level as a grid: thread i updates cell perm[i].
+template <class Row>
+__global__ __launch_bounds__(SWEEP_THREADS) void k_level(Row op, const int* perm, int lo, int hi, const int* done) {
+    if (done && *done) return;
     const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= hi) return;
-    const int u = s.permF[i];
-    const int k0 = s.lowOff[u], k1 = s.lowOff[u + 1];
-    double d = diag[u];
-    for (int k = k0; k < k1; ++k) {
-        const int e = s.lowSrc[k], f = e >> 1;
-        const double aul = (e & 1) ? lower[f] : upper[f];  // row u, column l
-        const double alu = (e & 1) ? upper[f] : lower[f];  // row l, column u
-        d = d - (alu * aul) / dF[s.lowCol[k]];
-    }
-    dF[u] = d;
-    const double r = 1.0 / d;
-    rD[u] = r;
-    for (int k = k0; k < k1; ++k) {
-        const int e = s.lowSrc[k], f = e >> 1;
-        pf[k] = r * ((e & 1) ? lower[f] : upper[f]);
-    }
-    for (int k = s.upOff[u]; k < s.upOff[u + 1]; ++k) {
-        const int e = s.upSrc[k], f = e >> 1;
-        pb[k] = r * ((e & 1) ? lower[f] : upper[f]);
+    if (i < hi) op(perm[i]);
+}
+
+// Levels [L0, L1) by one workgroup, a barrier after each level.
+template <class Row>
+__global__ __launch_bounds__(RUN_THREADS) void k_level_run(Row op, const int* perm, const int* off, int L0, int L1,
+                                                           const int* done) {
+    if (done && *done) return;
+    for (int L = L0; L < L1; ++L) {
+        const int lo = off[L], hi = off[L + 1];
+        for (int i = lo + int(threadIdx.x); i < hi; i += RUN_THREADS) op(perm[i]);
+        __syncthreads();
     }
 }
 
-// Forward substitution (LinearSolver.cpp:93-98) for one forward level:
-// w_u = rD_u·r_u − Σ_{l<u, ascending (l, face)} (rD_u·A_ul)·w_l.
-template <class T>
-__global__ void k_dilu_forward(SweepView s, int lo, int hi, const T* rD, const T* pf, const T* r, T* w,
-                               const int* done) {
-    if (done && *done) return;
-    const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= hi) return;
-    const int u = s.permF[i];
-    T acc = rD[u] * r[u];
-    for (int k = s.lowOff[u]; k < s.lowOff[u + 1]; ++k) acc = acc - pf[k] * w[s.lowCol[k]];
-    w[u] = acc;
-}
-
-// Backward substitution (LinearSolver.cpp:99-103) for one backward level:
-// w_l −= (rD_l·A_lu)·w_u over the upper neighbours u > l in descending (u, face) order.
-template <class T>
-__global__ void k_dilu_backward(SweepView s, int lo, int hi, const T* pb, T* w, const int* done) {
-    if (done && *done) return;
-    const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= hi) return;
-    const int l = s.permB[i];
-    T acc = w[l];
-    for (int k = s.upOff[l]; k < s.upOff[l + 1]; ++k) acc = acc - pb[k] * w[s.upCol[k]];
-    w[l] = acc;
-}
-
-// Gauss–Seidel update of the cells of one level (LinearSolver.cpp:247-262; the GAMG
-// smoother GAMG.cpp:363-384 has the same row), in place: x_c = (b_c − Σ coef·x_nb over
-// its slots, which hold its faces in ascending order) / diag_c. Lower-numbered
-// neighbours already hold this sweep's values (forward) or the previous pass's
-// (backward), exactly as in the sequential loop. The slot walk covers the three
-// layouts of SellView (CSR rows, ELL, SELL slices).
-template <class T>
-__global__ void k_gs_level(SellView S, const int* perm, int lo, int hi, const T* coef, const T* diag, const T* b,
-                           T* x, const int* done) {
-    if (done && *done) return;
-    const int i = lo + blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= hi) return;
-    const int c = perm[i];
-    int s0, width, stride;
-    if (S.csrG > 0) {
-        s0 = S.sliceOff[c];
-        width = S.sliceOff[c + 1] - s0;
-        stride = 1;
-    } else if (S.ellWidth > 0) {
-        width = S.ellWidth;
-        s0 = (c / WAVE) * WAVE * width + c % WAVE;
-        stride = WAVE;
-    } else {
-        const int b0 = S.sliceOff[c / WAVE];
-        width = (S.sliceOff[c / WAVE + 1] - b0) / WAVE;
-        s0 = b0 + c % WAVE;
-        stride = WAVE;
-    }
-    T r = b[c];
-    for (int k = 0; k < width; ++k) {
-        const int slot = s0 + k * stride;
-        int j;
-        if (S.col16) {
-            const int d = int(S.col16[slot]);
-            j = d == COL16_PAD ? -1 : c + d;
+template <class Row>
+static void launch_levels(hipStream_t st, const LevelSched& ls, int firstLevel, const Row& op, const int* done) {
+    for (int g = 0; g < ls.nGrp; ++g) {
+        const int L0 = std::max(ls.grpH[g], firstLevel), L1 = ls.grpH[g + 1];
+        if (L0 >= L1) continue;
+        if (ls.grpRunH[g]) {
+            hipLaunchKernelGGL(k_level_run<Row>, dim3(1), dim3(RUN_THREADS), 0, st, op, ls.perm, ls.offD, L0, L1,
+                               done);
         } else {
-            j = S.col[slot];
+            for (int L = L0; L < L1; ++L) {
+                const int lo = ls.offH[L], hi = ls.offH[L + 1];
+                if (hi > lo)
+                    hipLaunchKernelGGL(k_level<Row>, sweep_grid(hi - lo), dim3(SWEEP_THREADS), 0, st, op, ls.perm, lo,
+                                       hi, done);
+            }
         }
-        if (j >= 0) r = r - coef[slot] * x[j];
     }
-    x[c] = r / diag[c];
 }
 
-// x = 0 and x[i] += xc[agg[i]] (GAMG.cpp:423,438) for the Gauss–Seidel V-cycle.
-template <class T>
-__global__ void k_fill_zero(int n, T* x, const int* done) {
-    if (done && *done) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = T(0);
+// DIC/DILU factor (LinearSolver.cpp:67-74) of cell u: d_u = diag_u − Σ_{l<u, ascending
+// (l, face)} (A_lu·A_ul) / d_l, rD_u = 1/d_u. The same thread then forms the coefficient
+// products the substitutions use: rD_u·A_ul for its lower-list entries and rD_u times
+// its own row's coefficient for its upper-list entries.
+struct DiluFactorRow {
+    SweepView s;
+    const double *diag, *upper, *lower;
+    double *dF, *rD, *pf, *pb;
+    __device__ void operator()(int u) const {
+        const int k0 = s.lowOff[u], k1 = s.lowOff[u + 1];
+        double d = diag[u];
+        for (int k = k0; k < k1; ++k) {
+            const int e = s.lowSrc[k], f = e >> 1;
+            const double aul = (e & 1) ? lower[f] : upper[f];  // row u, column l
+            const double alu = (e & 1) ? upper[f] : lower[f];  // row l, column u
+            d = d - (alu * aul) / dF[s.lowCol[k]];
+        }
+        dF[u] = d;
+        const double r = 1.0 / d;
+        rD[u] = r;
+        for (int k = k0; k < k1; ++k) {
+            const int e = s.lowSrc[k], f = e >> 1;
+            pf[k] = r * ((e & 1) ? lower[f] : upper[f]);
+        }
+        for (int k = s.upOff[u]; k < s.upOff[u + 1]; ++k) {
+            const int e = s.upSrc[k], f = e >> 1;
+            pb[k] = r * ((e & 1) ? lower[f] : upper[f]);
+        }
+    }
+};
+
+void launch_dilu_factor(hipStream_t st, const SweepView& s, const LevelSched& fwd, const double* diag,
+                        const double* upper, const double* lower, double* dF, double* rD, double* pf, double* pb) {
+    launch_levels(st, fwd, 0, DiluFactorRow{s, diag, upper, lower, dF, rD, pf, pb}, nullptr);
 }
-template <class T>
-__global__ void k_prolong(int n, const int* agg, const T* xc, T* x, const int* done) {
-    if (done && *done) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] = x[i] + xc[agg[i]];
+
+// ---------------------------------------------------------------- level-ordered ELL sweeps
+// Row operations (T: double, or float in the mixed-precision inner solve):
+//   SW_GS   Gauss–Seidel (LinearSolver.cpp:247-262, GAMG.cpp:368-379):
+//           x_c = (b_c − Σ coef·x_nb) / diag_c, entries in ascending face order
+//   SW_FWD  DILU forward substitution (LinearSolver.cpp:93-98):
+//           w_u = rD_u·r_u − Σ (rD_u·A_ul)·w_l, entries in ascending (l, face) order
+//   SW_BWD  DILU backward substitution (LinearSolver.cpp:99-103):
+//           w_l = w_l − Σ (rD_l·A_lu)·w_u, entries in descending (u, face) order
+template <int OP, class T> __device__ __forceinline__ T sw_init(const SweepArgs<T>& a, int c) {
+    if constexpr (OP == SW_GS) return a.b[c];
+    else if constexpr (OP == SW_FWD) return a.rD[c] * a.r[c];
+    else return a.x[c];
 }
+template <int OP, class T> __device__ __forceinline__ T sw_fin(const SweepArgs<T>& a, int c, T acc) {
+    if constexpr (OP == SW_GS) return acc / a.diag[c];
+    else return acc;
+}
+
+// One level as a grid: thread t updates the level's row t.
+template <int OP, class T>
+__global__ __launch_bounds__(SWEEP_THREADS) void k_ell_level(LevelEll E, SweepArgs<T> a, int pos0, int size, int width,
+                                                             int base, const int* done) {
+    if (done && *done) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= size) return;
+    const int c = E.cell[pos0 + t];
+    T acc = sw_init<OP>(a, c);
+    for (int k = 0; k < width; ++k) {
+        const int e = base + k * size + t;
+        const int j = E.col[e];
+        if (j >= 0) acc = acc - a.coef[e] * a.x[j];
+    }
+    a.x[c] = sw_fin<OP>(a, c, acc);
+}
+
+// Levels [L0, L1) by one workgroup, R rows per thread, a barrier after each level.
+template <int OP, int R, class T>
+__global__ __launch_bounds__(RUN_THREADS) void k_ell_run(LevelEll E, SweepArgs<T> a, int L0, int L1,
+                                                         const int* done) {
+    if (done && *done) return;
+    for (int L = L0; L < L1; ++L) {
+        const int pos0 = E.offD[L], size = E.offD[L + 1] - pos0, width = E.widthD[L], base = E.baseD[L];
+        int cc[R];
+        T acc[R];
+#pragma unroll
+        for (int q = 0; q < R; ++q) {
+            const int t = int(threadIdx.x) + q * RUN_THREADS;
+            cc[q] = t < size ? E.cell[pos0 + t] : -1;
+            acc[q] = cc[q] >= 0 ? sw_init<OP>(a, cc[q]) : T(0);
+        }
+        for (int k = 0; k < width; ++k) {
+#pragma unroll
+            for (int q = 0; q < R; ++q) {
+                const int t = int(threadIdx.x) + q * RUN_THREADS;
+                if (cc[q] >= 0) {
+                    const int e = base + k * size + t;
+                    const int j = E.col[e];
+                    if (j >= 0) acc[q] = acc[q] - a.coef[e] * a.x[j];
+                }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < R; ++q)
+            if (cc[q] >= 0) a.x[cc[q]] = sw_fin<OP>(a, cc[q], acc[q]);
+        __syncthreads();
+    }
+}
+
+template <int OP, class T>
+static void launch_ell(hipStream_t st, const LevelEll& E, int firstLevel, const SweepArgs<T>& a, const int* done) {
+    for (int g = 0; g < E.nGrp; ++g) {
+        const int L0 = std::max(E.grpH[g], firstLevel), L1 = E.grpH[g + 1];
+        if (L0 >= L1) continue;
+        const int kind = E.grpKindH[g];
+        if (kind == ELL_RUN1) {
+            hipLaunchKernelGGL((k_ell_run<OP, 1, T>), dim3(1), dim3(RUN_THREADS), 0, st, E, a, L0, L1, done);
+        } else {
+            for (int L = L0; L < L1; ++L) {
+                const int pos0 = E.offH[L], size = E.offH[L + 1] - pos0;
+                if (size > 0)
+                    hipLaunchKernelGGL((k_ell_level<OP, T>), sweep_grid(size), dim3(SWEEP_THREADS), 0, st, E, a, pos0,
+                                       size, E.widthH[L], E.baseH[L], done);
+            }
+        }
+    }
+}
+
+template <class T>
+void launch_dilu_apply(hipStream_t st, const LevelEll& F, const LevelEll& B, const T* rD, const T* ef, const T* eb,
+                       const T* r, T* w, const int* done) {
+    SweepArgs<T> a;
+    a.rD = rD;
+    a.r = r;
+    a.x = w;
+    a.coef = ef;
+    launch_ell<SW_FWD, T>(st, F, 0, a, done);
+    a.coef = eb;
+    // backward level 0 holds the cells without upper neighbours: nothing to subtract
+    launch_ell<SW_BWD, T>(st, B, 1, a, done);
+}
+template void launch_dilu_apply<double>(hipStream_t, const LevelEll&, const LevelEll&, const double*, const double*,
+                                        const double*, const double*, double*, const int*);
+template void launch_dilu_apply<float>(hipStream_t, const LevelEll&, const LevelEll&, const float*, const float*,
+                                       const float*, const float*, float*, const int*);
+
+template <class T>
+void launch_gs_sweep(hipStream_t st, const LevelEll& E, const T* coef, const T* diag, const T* b, T* x,
+                     const int* done) {
+    SweepArgs<T> a;
+    a.coef = coef;
+    a.diag = diag;
+    a.b = b;
+    a.x = x;
+    launch_ell<SW_GS, T>(st, E, 0, a, done);
+}
+template void launch_gs_sweep<double>(hipStream_t, const LevelEll&, const double*, const double*, const double*,
+                                      double*, const int*);
+template void launch_gs_sweep<float>(hipStream_t, const LevelEll&, const float*, const float*, const float*, float*,
+                                     const int*);
+
+// Per-entry coefficients of a LevelEll from their source array: out[e] = from[src[e]]
+// (0 for padding, src < 0).
+template <class T>
+__global__ void k_gather_entries(size_t n, const int* src, const T* from, T* out) {
+    const size_t e = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (e < n) {
+        const int s = src[e];
+        out[e] = s >= 0 ? from[s] : T(0);
+    }
+}
+template <class T> void launch_gather_entries(hipStream_t st, size_t n, const int* src, const T* from, T* out) {
+    if (n > 0)
+        hipLaunchKernelGGL(k_gather_entries<T>, dim3(unsigned((n + SWEEP_THREADS - 1) / SWEEP_THREADS)),
+                           dim3(SWEEP_THREADS), 0, st, n, src, from, out);
+}
+template void launch_gather_entries<double>(hipStream_t, size_t, const int*, const double*, double*);
+template void launch_gather_entries<float>(hipStream_t, size_t, const int*, const float*, float*);
 
 // Coarsest-level solve in the reference's order (GAMG.cpp:406-421): forward substitution
 // (rows ascending, each subtracting in ascending k: done column by column by all lanes,
@@ -165,59 +282,6 @@ __global__ __launch_bounds__(64) void k_coarse_solve_rows(int n, const double* l
     for (int i = threadIdx.x; i < n; i += blockDim.x) x[i] = T(gv[i]);
 }
 
-namespace {
-constexpr int SWEEP_THREADS = 256;
-inline dim3 sweep_grid(int cnt) { return dim3((cnt + SWEEP_THREADS - 1) / SWEEP_THREADS); }
-}  // namespace
-
-void launch_dilu_factor(hipStream_t st, const SweepView& s, const int* levOffF, int nLevF, const double* diag,
-                        const double* upper, const double* lower, double* dF, double* rD, double* pf, double* pb) {
-    for (int L = 0; L < nLevF; ++L) {
-        const int lo = levOffF[L], hi = levOffF[L + 1];
-        if (hi > lo)
-            hipLaunchKernelGGL(k_dilu_factor, sweep_grid(hi - lo), dim3(SWEEP_THREADS), 0, st, s, lo, hi, diag, upper,
-                               lower, dF, rD, pf, pb);
-    }
-}
-
-template <class T>
-void launch_dilu_apply(hipStream_t st, const SweepView& s, const int* levOffF, int nLevF, const int* levOffB,
-                       int nLevB, const T* rD, const T* pf, const T* pb, const T* r, T* w, const int* done) {
-    for (int L = 0; L < nLevF; ++L) {
-        const int lo = levOffF[L], hi = levOffF[L + 1];
-        if (hi > lo)
-            hipLaunchKernelGGL(k_dilu_forward<T>, sweep_grid(hi - lo), dim3(SWEEP_THREADS), 0, st, s, lo, hi, rD, pf, r,
-                               w, done);
-    }
-    // backward level 0 holds the cells without upper neighbours: nothing to subtract
-    for (int L = 1; L < nLevB; ++L) {
-        const int lo = levOffB[L], hi = levOffB[L + 1];
-        if (hi > lo)
-            hipLaunchKernelGGL(k_dilu_backward<T>, sweep_grid(hi - lo), dim3(SWEEP_THREADS), 0, st, s, lo, hi, pb, w,
-                               done);
-    }
-}
-template void launch_dilu_apply<double>(hipStream_t, const SweepView&, const int*, int, const int*, int,
-                                        const double*, const double*, const double*, const double*, double*,
-                                        const int*);
-template void launch_dilu_apply<float>(hipStream_t, const SweepView&, const int*, int, const int*, int, const float*,
-                                       const float*, const float*, const float*, float*, const int*);
-
-template <class T>
-void launch_gs_sweep(hipStream_t st, const SellView& S, const int* perm, const int* levOff, int nLev, const T* coef,
-                     const T* diag, const T* b, T* x, const int* done) {
-    for (int L = 0; L < nLev; ++L) {
-        const int lo = levOff[L], hi = levOff[L + 1];
-        if (hi > lo)
-            hipLaunchKernelGGL(k_gs_level<T>, sweep_grid(hi - lo), dim3(SWEEP_THREADS), 0, st, S, perm, lo, hi, coef,
-                               diag, b, x, done);
-    }
-}
-template void launch_gs_sweep<double>(hipStream_t, const SellView&, const int*, const int*, int, const double*,
-                                      const double*, const double*, double*, const int*);
-template void launch_gs_sweep<float>(hipStream_t, const SellView&, const int*, const int*, int, const float*,
-                                     const float*, const float*, float*, const int*);
-
 template <class T>
 void launch_coarse_solve_rows(hipStream_t st, int n, const double* lu, const T* r, T* x, const int* done) {
     const size_t lds = sizeof(double) * (size_t(n) + (n <= 64 ? size_t(n) * n : 0));
@@ -225,6 +289,20 @@ void launch_coarse_solve_rows(hipStream_t st, int n, const double* lu, const T* 
 }
 template void launch_coarse_solve_rows<double>(hipStream_t, int, const double*, const double*, double*, const int*);
 template void launch_coarse_solve_rows<float>(hipStream_t, int, const double*, const float*, float*, const int*);
+
+// x = 0 and x[i] += xc[agg[i]] (GAMG.cpp:423,438) for the Gauss–Seidel V-cycle.
+template <class T>
+__global__ void k_fill_zero(int n, T* x, const int* done) {
+    if (done && *done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = T(0);
+}
+template <class T>
+__global__ void k_prolong(int n, const int* agg, const T* xc, T* x, const int* done) {
+    if (done && *done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = x[i] + xc[agg[i]];
+}
 
 template <class T>
 void launch_fill_zero(hipStream_t st, int n, T* x, const int* done) {

@@ -332,13 +332,10 @@ void launch_fvm_term(hipStream_t st, int kind, const FvmArgs& a, const int* own,
 // FvMatrix −= su (FvMatrix.h:93-97): source[c] += su[c]·V[c]
 void launch_fvm_source(hipStream_t st, int n, const double* su, const double* V, double* source);
 // ------------------------------------------------------------------ level-scheduled sweeps
-// Cells in forward / backward level order and their lower / upper neighbour lists
-// (device arrays). Lower list of cell u: neighbours l < u in ascending (l, face) order;
+// Lower / upper neighbour lists of the cells (device arrays). Lower list of cell u: neighbours l < u in ascending (l, face) order;
 // upper list of cell l: neighbours u > l in descending (u, face) order; src = face << 1 |
 // side of the list's own row (0: upper[face], 1: lower[face]).
 struct SweepView {
-    const int* permF = nullptr;
-    const int* permB = nullptr;
     const int* lowOff = nullptr;
     const int* lowCol = nullptr;
     const int* lowSrc = nullptr;
@@ -346,18 +343,64 @@ struct SweepView {
     const int* upCol = nullptr;
     const int* upSrc = nullptr;
 };
-// DIC/DILU factor over the forward levels (levOffF: host array of nLevF + 1 offsets):
-// dF, rD = 1/dF per cell, pf per lower-list entry, pb per upper-list entry
-void launch_dilu_factor(hipStream_t st, const SweepView& s, const int* levOffF, int nLevF, const double* diag,
+// A level order prepared for launching (DIC/DILU factor): cells of level L are
+// perm[off[L] .. off[L+1]); groups [grp[g], grp[g+1]) of consecutive levels are either
+// a run of levels of at most SWEEP_RUN_MAX cells each, swept by one workgroup in one
+// launch (grpRun[g] = 1), or a single larger level with a grid launch.
+constexpr int SWEEP_RUN_MAX = 1024;
+struct LevelSched {
+    const int* perm = nullptr;   // device
+    const int* offD = nullptr;   // device level offsets (nLev + 1)
+    const int* offH = nullptr;   // host copy of the offsets
+    int nLev = 0;
+    const int* grpH = nullptr;   // host: nGrp + 1 group boundaries (level indices)
+    const char* grpRunH = nullptr;
+    int nGrp = 0;
+};
+// DIC/DILU factor over the forward levels: dF, rD = 1/dF per cell, pf per lower-list
+// entry, pb per upper-list entry
+void launch_dilu_factor(hipStream_t st, const SweepView& s, const LevelSched& fwd, const double* diag,
                         const double* upper, const double* lower, double* dF, double* rD, double* pf, double* pb);
-// w = M⁻¹ r of DIC/DILU: forward then backward substitution, every launch a no-op once *done
+// Level-ordered ELL rows of a triangular sweep: positions in level order, the cell at
+// each position, and entry k of row t of level L at base[L] + k·size_L + t (neighbour
+// col, −1 padding; the coefficients come per matrix, in the same layout). Groups
+// [grp[g], grp[g+1]) of levels are a run of levels of ≤ 1024 cells swept by one
+// workgroup, a row per thread (ELL_RUN1), or one larger level swept by a grid (ELL_GRID).
+constexpr int ELL_GRID = 0, ELL_RUN1 = 1;
+struct LevelEll {
+    const int* cell = nullptr;    // device
+    const int* col = nullptr;     // device
+    const int* offD = nullptr;    // device: nLev + 1 level offsets
+    const int* widthD = nullptr;  // device: entries per row of each level
+    const int* baseD = nullptr;   // device: first entry of each level
+    const int* offH = nullptr;    // host copies
+    const int* widthH = nullptr;
+    const int* baseH = nullptr;
+    int nLev = 0;
+    const int* grpH = nullptr;
+    const char* grpKindH = nullptr;
+    int nGrp = 0;
+};
+enum SweepOp { SW_GS = 0, SW_FWD = 1, SW_BWD = 2 };
+template <class T> struct SweepArgs {
+    const T* coef = nullptr;  // per entry (LevelEll layout)
+    const T* b = nullptr;     // SW_GS
+    const T* diag = nullptr;  // SW_GS
+    const T* rD = nullptr;    // SW_FWD
+    const T* r = nullptr;     // SW_FWD
+    T* x = nullptr;           // updated in place (w for DILU)
+};
+// w = M⁻¹ r of DIC/DILU: forward then backward substitution (ef, eb: rD·A products per
+// entry of F and B), every launch a no-op once *done
 template <class T>
-void launch_dilu_apply(hipStream_t st, const SweepView& s, const int* levOffF, int nLevF, const int* levOffB,
-                       int nLevB, const T* rD, const T* pf, const T* pb, const T* r, T* w, const int* done);
-// one Gauss–Seidel pass over the levels of perm (forward: permF, backward: permB)
+void launch_dilu_apply(hipStream_t st, const LevelEll& F, const LevelEll& B, const T* rD, const T* ef, const T* eb,
+                       const T* r, T* w, const int* done);
+// one Gauss–Seidel pass over the levels of E (forward or backward order)
 template <class T>
-void launch_gs_sweep(hipStream_t st, const SellView& S, const int* perm, const int* levOff, int nLev, const T* coef,
-                     const T* diag, const T* b, T* x, const int* done);
+void launch_gs_sweep(hipStream_t st, const LevelEll& E, const T* coef, const T* diag, const T* b, T* x,
+                     const int* done);
+// out[e] = from[src[e]] (0 where src[e] < 0)
+template <class T> void launch_gather_entries(hipStream_t st, size_t n, const int* src, const T* from, T* out);
 // coarsest solve with the LU factors in the reference's row order (Gauss–Seidel V-cycle)
 template <class T>
 void launch_coarse_solve_rows(hipStream_t st, int n, const double* lu, const T* r, T* x, const int* done);
