@@ -265,6 +265,41 @@ int vol_fvm_H(vol_matrix A, const double* psi, const double* V, int nComponents,
  * boundary faces, + faceFluxCorrection when a laplacian term switched it on. */
 int vol_fvm_flux(vol_matrix A, const double* psi, double* out, int location);
 
+/* ---- explicit operators of the pressure corrector (Fvc.h), on the GPU.
+ * vol_mesh_set_geometry stores the mesh geometry the operators need (after
+ * vol_mesh_set_boundary): V (nCells); per face (nInternalFaces + nBoundaryFaces): Sf and
+ * d (interleaved xyz), magSf, w; skew (Mesh::skewVecs, interleaved xyz, nullable → 0)
+ * and skewed (Mesh::skewed(): whether interpolate/flux apply the skew correction; the
+ * Gauss-skew gradient always uses skew, as the reference does). Multi-component values
+ * are interleaved per cell or face, as std::vector<Vec3>/<Tensor> hold them: nComponents
+ * 1 (scalar field; its gradient a Vec3 per cell) or 3 (vector field; its gradient a
+ * row-major Tensor per cell). Results are bit-identical to the reference's. Meshes are
+ * 3-D (no empty direction). */
+int vol_mesh_set_geometry(vol_mesh mesh, const double* V, const double* Sf, const double* magSf, const double* w,
+                          const double* d, const double* skew, int skewed, int location);
+/* fvc::interpolate (Fvc.h:13-31,72-81): w c_P + (1−w) c_N on internal faces, plus
+ * gradDot(skew, w g_P + (1−w) g_N) when cellGrad is given and the mesh is skewed;
+ * boundary faces take boundaryValues (nullable → the owner cell's value). */
+int vol_fvc_interpolate(vol_mesh mesh, int nComponents, const double* cellValues, const double* cellGrad,
+                        const double* boundaryValues, double* faceValues, int location);
+/* fvc::grad (Fvc.h:185-291) with fixedValue faces (bFixesValue 1, face values bValue) and
+ * zeroGradient faces otherwise: VOL_GRAD_LEAST_SQUARES (= fvc::gradLS; nComponents 1 or
+ * 3), VOL_GRAD_GAUSS_LINEAR and VOL_GRAD_GAUSS_SKEW (nComponents 1). */
+#define VOL_GRAD_GAUSS_LINEAR 0
+#define VOL_GRAD_LEAST_SQUARES 1
+#define VOL_GRAD_GAUSS_SKEW 2
+int vol_fvc_grad(vol_mesh mesh, int scheme, int nComponents, const double* cellValues, const int32_t* bFixesValue,
+                 const double* bValue, double* cellGrad, int location);
+/* fvc::div(m, F) (Fvc.h:294-305): per cell Σ ±F / V */
+int vol_fvc_div(vol_mesh mesh, const double* faceFlux, double* out, int location);
+/* fvc::surfaceIntegrate(m, F) (Fvc.h:307-318): per cell Σ ±F · (1/V) */
+int vol_fvc_surface_integrate(vol_mesh mesh, int nComponents, const double* faceValues, double* out, int location);
+/* face flux of a cell vector field as phiHbyA is formed (IncompressibleFlow.cpp:276-291):
+ * dot(U_f, S_f) with U_f = interpolate(U, gradU) on internal faces; on boundary faces
+ * dot(Ub, S_f) where bFixesFlux, else dot(U_P, S_f). gradU nullable. */
+int vol_fvc_flux(vol_mesh mesh, const double* U, const double* gradU, const int32_t* bFixesFlux, const double* Ub,
+                 double* phi, int location);
+
 /* ---- batches: many independent systems solved together (no reference counterpart).
  * Systems are concatenated: owner/neighbour/gamgWeights hold each system's internal
  * faces in turn with that system's own (0-based) cell numbers; diag/upper/lower, b and

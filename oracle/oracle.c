@@ -1271,3 +1271,225 @@ void oracle_flux(int nI, int nB, const int* own, const int* nei, const double* u
     if (ffc)
         for (int f = 0; f < nI + nB; ++f) F[f] += ffc[f];
 }
+
+/* ---------------------------------------------------------------- explicit corrector operators
+ * Fvc.h. Multi-component values are interleaved per cell / face, as std::vector<Vec3>
+ * and std::vector<Tensor> lay them out: a Vec3 per entry for vector fields and for the
+ * gradient of a scalar, a row-major Tensor (xx xy xz yx yy yz zx zy zz) for the gradient
+ * of a vector. nc is 1 (scalar field) or 3 (vector field); face arrays have nI + nB
+ * entries. Meshes are 3-D (no empty direction). */
+
+static double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+/* gradDot(sk, w g_P + (1−w) g_N) (Types.h:156-157): the scalar sk·g for a Vec3 gradient,
+ * the Vec3 sk·T (column j: sk_x T_xj + sk_y T_yj + sk_z T_zj) for a Tensor gradient. */
+static void skew_term(int nc, const double* sk, double w, const double* gP, const double* gN, double* out) {
+    double g[9];
+    for (int k = 0; k < 3 * nc; ++k) g[k] = w * gP[k] + (1.0 - w) * gN[k];
+    if (nc == 1) {
+        out[0] = sk[0] * g[0] + sk[1] * g[1] + sk[2] * g[2];
+    } else {
+        for (int j = 0; j < 3; ++j) out[j] = sk[0] * g[j] + sk[1] * g[3 + j] + sk[2] * g[6 + j];
+    }
+}
+
+/* fvc::interpolate(m, c) (Fvc.h:24-31): w c_P + (1−w) c_N on internal faces, the owner's
+ * value on boundary faces; with grad and skew (a skewed mesh) the skew-corrected form
+ * (Fvc.h:72-81) adds gradDot(skew, w g_P + (1−w) g_N) on internal faces. */
+void oracle_fvc_interpolate(int n, int nI, int nB, const int* own, const int* nei, const double* w,
+                            const double* skew, int nc, const double* c, const double* grad, double* F) {
+    (void)n;
+    for (int i = 0; i < nI; ++i)
+        for (int k = 0; k < nc; ++k) F[nc * i + k] = w[i] * c[nc * own[i] + k] + (1.0 - w[i]) * c[nc * nei[i] + k];
+    for (int i = nI; i < nI + nB; ++i)
+        for (int k = 0; k < nc; ++k) F[nc * i + k] = c[nc * own[i] + k];
+    if (grad && skew) {
+        for (int i = 0; i < nI; ++i) {
+            double t[3];
+            skew_term(nc, skew + 3 * i, w[i], grad + 3 * nc * own[i], grad + 3 * nc * nei[i], t);
+            for (int k = 0; k < nc; ++k) F[nc * i + k] += t[k];
+        }
+    }
+}
+
+/* Tensor inverse of Fvc.h:112-126 (cofactors over the determinant, entry by entry). */
+static void inv3(const double* a, double* r) {
+    const double det = a[0] * (a[4] * a[8] - a[5] * a[7]) - a[1] * (a[3] * a[8] - a[5] * a[6]) +
+                       a[2] * (a[3] * a[7] - a[4] * a[6]);
+    r[0] = (a[4] * a[8] - a[5] * a[7]) / det;
+    r[1] = (a[2] * a[7] - a[1] * a[8]) / det;
+    r[2] = (a[1] * a[5] - a[2] * a[4]) / det;
+    r[3] = (a[5] * a[6] - a[3] * a[8]) / det;
+    r[4] = (a[0] * a[8] - a[2] * a[6]) / det;
+    r[5] = (a[2] * a[3] - a[0] * a[5]) / det;
+    r[6] = (a[3] * a[7] - a[4] * a[6]) / det;
+    r[7] = (a[1] * a[6] - a[0] * a[7]) / det;
+    r[8] = (a[0] * a[4] - a[1] * a[3]) / det;
+}
+
+/* g += lsApply(G, v, dphi) (Fvc.h:129-130): (G·v)·dphi for a scalar, outer(G·v, dphi)
+ * for a vector. */
+static void ls_add(int nc, const double* G, const double* v, const double* dphi, double* g) {
+    const double a[3] = {G[0] * v[0] + G[1] * v[1] + G[2] * v[2], G[3] * v[0] + G[4] * v[1] + G[5] * v[2],
+                         G[6] * v[0] + G[7] * v[1] + G[8] * v[2]};
+    if (nc == 1) {
+        for (int k = 0; k < 3; ++k) g[k] += a[k] * dphi[0];
+    } else {
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) g[3 * i + j] += a[i] * dphi[j];
+    }
+}
+
+/* Weighted least-squares gradient (Fvc.h:134-167) with the boundary constraints de/dphi
+ * per boundary face. */
+static void ls_grad(int n, int nI, int nB, const int* own, const int* nei, const double* d, int nc, const double* c,
+                    const double* de, const double* dphi, double* g) {
+    const int ng = 3 * nc;
+    double* dd = (double*)xcalloc((size_t)n * 9 + 1, sizeof(double));
+    double* Gi = (double*)xcalloc((size_t)n * 9 + 1, sizeof(double));
+    for (int k = 0; k < n * ng; ++k) g[k] = 0.0;
+    for (int i = 0; i < nI + nB; ++i) {
+        const double* v = i < nI ? d + 3 * i : de + 3 * (i - nI);
+        const double s = 1.0 / dot3(v, v);
+        double t[9];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) t[3 * a + b] = (v[a] * v[b]) * s;
+        for (int k = 0; k < 9; ++k) dd[9 * own[i] + k] += t[k];
+        if (i < nI)
+            for (int k = 0; k < 9; ++k) dd[9 * nei[i] + k] += t[k];
+    }
+    for (int cc = 0; cc < n; ++cc) inv3(dd + 9 * cc, Gi + 9 * cc);
+    for (int i = 0; i < nI; ++i) {
+        const double* v = d + 3 * i;
+        const double wgt = 1.0 / dot3(v, v);
+        const double vw[3] = {v[0] * wgt, v[1] * wgt, v[2] * wgt};
+        double dp[3];
+        for (int k = 0; k < nc; ++k) dp[k] = c[nc * nei[i] + k] - c[nc * own[i] + k];
+        ls_add(nc, Gi + 9 * own[i], vw, dp, g + ng * own[i]);
+        ls_add(nc, Gi + 9 * nei[i], vw, dp, g + ng * nei[i]);
+    }
+    for (int i = nI; i < nI + nB; ++i) {
+        const double* v = de + 3 * (i - nI);
+        const double s = 1.0 / dot3(v, v);
+        const double vs[3] = {v[0] * s, v[1] * s, v[2] * s};
+        ls_add(nc, Gi + 9 * own[i], vs, dphi + nc * (i - nI), g + ng * own[i]);
+    }
+    free(dd);
+    free(Gi);
+}
+
+/* Boundary constraint of a gradient face (Fvc.h:196-200, 276-281): n·max(n·d, 0.05|d|). */
+static void grad_face_de(const double* Sf, double magSf, const double* d, double* de) {
+    const double s = 1.0 / magSf;
+    const double nv[3] = {Sf[0] * s, Sf[1] * s, Sf[2] * s};
+    const double nd0 = dot3(nv, d), lim = 0.05 * sqrt(dot3(d, d));
+    const double nd = nd0 < lim ? lim : nd0;
+    for (int k = 0; k < 3; ++k) de[k] = nv[k] * nd;
+}
+
+/* fvc::gradLS(m, c, bFixesValue, bValue) (Fvc.h:185-206): value faces constrain with d and
+ * bValue − c_P, the others with the zero normal gradient. */
+void oracle_fvc_grad_ls(int n, int nI, int nB, const int* own, const int* nei, const double* d, const double* Sf,
+                        const double* magSf, int nc, const double* c, const int* bFix, const double* bVal, double* g) {
+    double* de = (double*)xcalloc((size_t)nB * 3 + 1, sizeof(double));
+    double* dphi = (double*)xcalloc((size_t)nB * 3 + 1, sizeof(double));
+    for (int i = nI; i < nI + nB; ++i) {
+        const int bf = i - nI;
+        if (bFix[bf]) {
+            for (int k = 0; k < 3; ++k) de[3 * bf + k] = d[3 * i + k];
+            for (int k = 0; k < nc; ++k) dphi[nc * bf + k] = bVal[nc * bf + k] - c[nc * own[i] + k];
+        } else {
+            grad_face_de(Sf + 3 * i, magSf[i], d + 3 * i, de + 3 * bf);
+        }
+    }
+    ls_grad(n, nI, nB, own, nei, d, nc, c, de, dphi, g);
+    free(de);
+    free(dphi);
+}
+
+/* fvc::grad(ψ, GaussSkew) of a scalar with fixedValue (bFix, face values bVal) and
+ * zeroGradient faces (Fvc.h:223-259): face values w ψ_P + (1−w) ψ_N + skew·(w g_P +
+ * (1−w) g_N) with g the least-squares gradient; on zeroGradient faces ψ_P + d_t·g_P +
+ * gb (n·d) with gb = 0·ψ_P + 0; Σ S_f ψ_f (owner +, neighbour −) · (1/V). */
+void oracle_fvc_grad_gauss_skew(int n, int nI, int nB, const int* own, const int* nei, const double* w,
+                                const double* skew, const double* d, const double* Sf, const double* magSf,
+                                const double* V, const double* psi, const int* bFix, const double* bVal, double* g) {
+    double* gls = (double*)xcalloc((size_t)n * 3 + 1, sizeof(double));
+    oracle_fvc_grad_ls(n, nI, nB, own, nei, d, Sf, magSf, 1, psi, bFix, bVal, gls);
+    for (int k = 0; k < 3 * n; ++k) g[k] = 0.0;
+    for (int i = 0; i < nI; ++i) {
+        double t;
+        skew_term(1, skew + 3 * i, w[i], gls + 3 * own[i], gls + 3 * nei[i], &t);
+        const double phif = w[i] * psi[own[i]] + (1.0 - w[i]) * psi[nei[i]] + t;
+        for (int k = 0; k < 3; ++k) {
+            const double cc = Sf[3 * i + k] * phif;
+            g[3 * own[i] + k] += cc;
+            g[3 * nei[i] + k] -= cc;
+        }
+    }
+    for (int i = nI; i < nI + nB; ++i) {
+        const int bf = i - nI, P = own[i];
+        double phif;
+        if (bFix[bf]) {
+            phif = bVal[bf];
+        } else {
+            const double* dv = d + 3 * i;
+            const double s = 1.0 / magSf[i];
+            const double nv[3] = {Sf[3 * i] * s, Sf[3 * i + 1] * s, Sf[3 * i + 2] * s};
+            const double nd = dot3(nv, dv);
+            const double dt[3] = {dv[0] - nv[0] * nd, dv[1] - nv[1] * nd, dv[2] - nv[2] * nd};
+            const double gb = 0.0 * psi[P] + 0.0;
+            phif = psi[P] + dot3(dt, gls + 3 * P) + gb * nd;
+        }
+        for (int k = 0; k < 3; ++k) g[3 * P + k] += Sf[3 * i + k] * phif;
+    }
+    for (int c = 0; c < n; ++c) {
+        const double r = 1.0 / V[c];
+        for (int k = 0; k < 3; ++k) g[3 * c + k] = g[3 * c + k] * r;
+    }
+    free(gls);
+}
+
+/* fvc::div(m, F) (Fvc.h:294-305): owner +F, neighbour −F, boundary +F, then / V. */
+void oracle_fvc_div(int n, int nI, int nB, const int* own, const int* nei, const double* V, const double* F,
+                    double* r) {
+    for (int c = 0; c < n; ++c) r[c] = 0.0;
+    for (int i = 0; i < nI; ++i) {
+        r[own[i]] += F[i];
+        r[nei[i]] -= F[i];
+    }
+    for (int i = nI; i < nI + nB; ++i) r[own[i]] += F[i];
+    for (int c = 0; c < n; ++c) r[c] /= V[c];
+}
+
+/* fvc::surfaceIntegrate(m, F) (Fvc.h:307-318), nc components per face: the same sums,
+ * then · (1/V). */
+void oracle_fvc_surface_integrate(int n, int nI, int nB, const int* own, const int* nei, const double* V, int nc,
+                                  const double* F, double* r) {
+    for (int k = 0; k < n * nc; ++k) r[k] = 0.0;
+    for (int i = 0; i < nI; ++i)
+        for (int k = 0; k < nc; ++k) {
+            r[nc * own[i] + k] += F[nc * i + k];
+            r[nc * nei[i] + k] -= F[nc * i + k];
+        }
+    for (int i = nI; i < nI + nB; ++i)
+        for (int k = 0; k < nc; ++k) r[nc * own[i] + k] += F[nc * i + k];
+    for (int c = 0; c < n; ++c) {
+        const double s = 1.0 / V[c];
+        for (int k = 0; k < nc; ++k) r[nc * c + k] = r[nc * c + k] * s;
+    }
+}
+
+/* Face flux of a cell vector field as phiHbyA is formed (IncompressibleFlow.cpp:276-291):
+ * internal faces dot(U_f, S_f) with U_f the (skew-corrected when grad) interpolate,
+ * boundary faces dot(Ub, S_f) where bFix, else dot(U_P, S_f). */
+void oracle_fvc_flux(int n, int nI, int nB, const int* own, const int* nei, const double* w, const double* skew,
+                     const double* Sf, const double* U, const double* grad, const int* bFix, const double* Ub,
+                     double* phi) {
+    double* Uf = (double*)xcalloc((size_t)(nI + nB) * 3 + 1, sizeof(double));
+    oracle_fvc_interpolate(n, nI, nB, own, nei, w, skew, 3, U, grad, Uf);
+    for (int i = 0; i < nI; ++i) phi[i] = dot3(Uf + 3 * i, Sf + 3 * i);
+    for (int i = nI; i < nI + nB; ++i)
+        phi[i] = bFix[i - nI] ? dot3(Ub + 3 * (i - nI), Sf + 3 * i) : dot3(U + 3 * own[i], Sf + 3 * i);
+    free(Uf);
+}

@@ -98,6 +98,21 @@ void oracle_H(int n, int nI, int nB, const int* own, const int* nei, const doubl
 /* FvMatrix::flux (FvMatrix.h:210-222), one component; ffc NULL when the matrix has none */
 void oracle_flux(int nI, int nB, const int* own, const int* nei, const double* upper, const double* lower,
                  const double* ic, const double* bs, const double* ffc, const double* psi, double* F);
+/* explicit corrector operators (Fvc.h); multi-component values interleaved, nc 1 or 3 */
+void oracle_fvc_interpolate(int n, int nI, int nB, const int* own, const int* nei, const double* w,
+                            const double* skew, int nc, const double* c, const double* grad, double* F);
+void oracle_fvc_grad_ls(int n, int nI, int nB, const int* own, const int* nei, const double* d, const double* Sf,
+                        const double* magSf, int nc, const double* c, const int* bFix, const double* bVal, double* g);
+void oracle_fvc_grad_gauss_skew(int n, int nI, int nB, const int* own, const int* nei, const double* w,
+                                const double* skew, const double* d, const double* Sf, const double* magSf,
+                                const double* V, const double* psi, const int* bFix, const double* bVal, double* g);
+void oracle_fvc_div(int n, int nI, int nB, const int* own, const int* nei, const double* V, const double* F,
+                    double* r);
+void oracle_fvc_surface_integrate(int n, int nI, int nB, const int* own, const int* nei, const double* V, int nc,
+                                  const double* F, double* r);
+void oracle_fvc_flux(int n, int nI, int nB, const int* own, const int* nei, const double* w, const double* skew,
+                     const double* Sf, const double* U, const double* grad, const int* bFix, const double* Ub,
+                     double* phi);
 
 #ifdef __cplusplus
 }

@@ -51,6 +51,12 @@ def _lib():
         lib.oracle_div_upwind.argtypes = [I, I, I] + [P] * 10
         lib.oracle_ddt_euler.argtypes = [I, P, D, P, P, P]
         lib.oracle_fold.argtypes = [I, I, I] + [P] * 7
+        lib.oracle_fvc_interpolate.argtypes = [I, I, I, P, P, P, P, I, P, P, P]
+        lib.oracle_fvc_grad_ls.argtypes = [I, I, I, P, P, P, P, P, I, P, P, P, P]
+        lib.oracle_fvc_grad_gauss_skew.argtypes = [I, I, I] + [P] * 12
+        lib.oracle_fvc_div.argtypes = [I, I, I, P, P, P, P, P]
+        lib.oracle_fvc_surface_integrate.argtypes = [I, I, I, P, P, P, I, P, P]
+        lib.oracle_fvc_flux.argtypes = [I, I, I] + [P] * 10
         lib.oracle_relax.argtypes = [I, I, I] + [P] * 6 + [D, P, P]
         lib.oracle_set_reference.argtypes = [I, ctypes.c_long, D, P, P]
         lib.oracle_A.argtypes = [I, I, I, P, P, P, P, I, P]
@@ -371,3 +377,86 @@ def transport_post(mesh, dt, nu, psi0, F, bc_fixed, bc_value, alpha, ref_cell, r
     for k in ("source_relaxed", "H", "flux", "source_ref"):
         res[k] = np.array(res[k]) if vec else res[k][0]
     return res
+
+
+# ---------------------------------------------------------------- explicit corrector operators
+def _geo3(mesh):
+    g = mesh.geometry()
+    return g, np.ascontiguousarray(g["skew"]) if g["skewed"] else None
+
+
+def fvc_interpolate(mesh, c, grad=None):
+    """fvc::interpolate(m, c[, grad]) (Fvc.h:24-31,72-81); c (n,) or (n, 3); grad (n, 3)
+    or (n, 9) for the skew correction (applied on skewed meshes)."""
+    g, sk = _geo3(mesh)
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    nc = 1 if c.ndim == 1 else 3
+    out = np.zeros((mesh.n_faces,) if nc == 1 else (mesh.n_faces, 3))
+    gr = None if grad is None else np.ascontiguousarray(grad, dtype=np.float64)
+    _lib().oracle_fvc_interpolate(mesh.n_cells, mesh.n_internal, mesh.n_boundary, _p(_i32(mesh.owner)),
+                                  _p(_i32(mesh.neighbour)), _p(g["w"]), None if sk is None else _p(sk), nc, _p(c),
+                                  None if gr is None else _p(gr), _p(out))
+    return out
+
+
+def fvc_grad_ls(mesh, c, b_fixed, b_value):
+    """fvc::gradLS(m, c, bFixesValue, bValue) (Fvc.h:185-206)."""
+    g = mesh.geometry()
+    c = np.ascontiguousarray(c, dtype=np.float64)
+    nc = 1 if c.ndim == 1 else 3
+    out = np.zeros((mesh.n_cells, 3 * nc))
+    _lib().oracle_fvc_grad_ls(mesh.n_cells, mesh.n_internal, mesh.n_boundary, _p(_i32(mesh.owner)),
+                              _p(_i32(mesh.neighbour)), _p(np.ascontiguousarray(g["d"])),
+                              _p(np.ascontiguousarray(g["Sf"])), _p(g["magSf"]), nc, _p(c), _p(_i32(b_fixed)),
+                              _p(np.ascontiguousarray(b_value, dtype=np.float64)), _p(out))
+    return out
+
+
+def fvc_grad_gauss(mesh, psi, b_fixed, b_value):
+    """fvc::grad(ψ, GaussLinear) with fixedValue (b_value) / zeroGradient faces."""
+    bval = np.where(np.asarray(b_fixed) != 0, _f64(b_value), _f64(psi)[np.asarray(mesh.owner[mesh.n_internal:])])
+    return grad_gauss(mesh, psi, bval)
+
+
+def fvc_grad_gauss_skew(mesh, psi, b_fixed, b_value):
+    """fvc::grad(ψ, GaussSkew) (Fvc.h:223-259); it applies the skew vectors whether or not
+    the mesh counts as skewed."""
+    g = mesh.geometry()
+    sk = np.ascontiguousarray(g["skew"])
+    out = np.zeros((mesh.n_cells, 3))
+    _lib().oracle_fvc_grad_gauss_skew(mesh.n_cells, mesh.n_internal, mesh.n_boundary, _p(_i32(mesh.owner)),
+                                      _p(_i32(mesh.neighbour)), _p(g["w"]), _p(sk), _p(np.ascontiguousarray(g["d"])),
+                                      _p(np.ascontiguousarray(g["Sf"])), _p(g["magSf"]), _p(g["V"]), _p(_f64(psi)),
+                                      _p(_i32(b_fixed)), _p(_f64(b_value)), _p(out))
+    return out
+
+
+def fvc_div(mesh, F):
+    """fvc::div(m, F) (Fvc.h:294-305)."""
+    out = np.zeros(mesh.n_cells)
+    _lib().oracle_fvc_div(mesh.n_cells, mesh.n_internal, mesh.n_boundary, _p(_i32(mesh.owner)),
+                          _p(_i32(mesh.neighbour)), _p(mesh.geometry()["V"]), _p(_f64(F)), _p(out))
+    return out
+
+
+def fvc_surface_integrate(mesh, F):
+    """fvc::surfaceIntegrate(m, F) (Fvc.h:307-318); F (nFaces,) or (nFaces, 3)."""
+    F = np.ascontiguousarray(F, dtype=np.float64)
+    nc = 1 if F.ndim == 1 else 3
+    out = np.zeros((mesh.n_cells,) if nc == 1 else (mesh.n_cells, 3))
+    _lib().oracle_fvc_surface_integrate(mesh.n_cells, mesh.n_internal, mesh.n_boundary, _p(_i32(mesh.owner)),
+                                        _p(_i32(mesh.neighbour)), _p(mesh.geometry()["V"]), nc, _p(F), _p(out))
+    return out
+
+
+def fvc_flux(mesh, U, grad, b_fixed, Ub):
+    """phiHbyA-style face flux of a cell vector field (IncompressibleFlow.cpp:276-291)."""
+    g, sk = _geo3(mesh)
+    out = np.zeros(mesh.n_faces)
+    gr = None if grad is None else np.ascontiguousarray(grad, dtype=np.float64)
+    _lib().oracle_fvc_flux(mesh.n_cells, mesh.n_internal, mesh.n_boundary, _p(_i32(mesh.owner)),
+                           _p(_i32(mesh.neighbour)), _p(g["w"]), None if sk is None else _p(sk),
+                           _p(np.ascontiguousarray(g["Sf"])), _p(np.ascontiguousarray(U, dtype=np.float64)),
+                           None if gr is None else _p(gr), _p(_i32(b_fixed)),
+                           _p(np.ascontiguousarray(Ub, dtype=np.float64)), _p(out))
+    return out

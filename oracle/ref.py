@@ -48,6 +48,9 @@ def _lib():
         lib.vr_ddt_euler.restype = I
         lib.vr_transport.argtypes = [P, D, D, P, P, P, P, P, I] + [P] * 9
         lib.vr_transport.restype = I
+        lib.vr_skew.argtypes = [P, P, P]
+        lib.vr_fvc.argtypes = [P] * 18
+        lib.vr_fvc.restype = I
         lib.vr_transport_post.argtypes = [P, I, I, D, D, P, P, P, P, I, D, ctypes.c_long, D] + [P] * 7
         lib.vr_transport_post.restype = I
         lib.vr_solve.argtypes = [P, P, P, P, I, P, P, ctypes.c_char_p, ctypes.c_char_p, D, D, I, I, I, P, P, P]
@@ -128,6 +131,27 @@ class RefMesh:
              "corr": np.zeros((nf, 3))}
         _lib().vr_geometry(self._h, *(_p(g[k]) for k in ("V", "C", "Sf", "magSf", "Cf", "d", "w", "ndc", "corr")))
         return g
+
+    def skew(self):
+        """(skewVecs (nFaces × 3), skewed flag) of the reference Mesh."""
+        sk = np.zeros((self.n_faces, 3))
+        flag = np.zeros(1, np.int32)
+        _lib().vr_skew(self._h, _p(sk), _p(flag))
+        return sk, bool(flag[0])
+
+    def fvc(self, p, U, fixed_patches, pB, UB, F):
+        """The reference's explicit corrector operators (see vr_fvc in ref_driver.cpp)."""
+        n, nf, ni = self.n_cells, self.n_faces, self.n_internal
+        fp = np.zeros(self.n_patches, np.int32)
+        fp[list(fixed_patches)] = 1
+        out = {"gLSp": np.zeros((n, 3)), "gLSU": np.zeros((n, 9)), "ip": np.zeros(nf), "ipSkew": np.zeros(nf),
+               "iU": np.zeros((nf, 3)), "gGL": np.zeros((n, 3)), "gLS": np.zeros((n, 3)), "gGS": np.zeros((n, 3)),
+               "divF": np.zeros(n), "siU": np.zeros((n, 3)), "phi": np.zeros(nf)}
+        keys = ("gLSp", "gLSU", "ip", "ipSkew", "iU", "gGL", "gLS", "gGS", "divF", "siU", "phi")
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        _check(_lib().vr_fvc(self._h, _p(c(p)), _p(c(U)), _p(fp), _p(c(pB)), _p(c(UB)), _p(c(F)),
+                             *(_p(out[k]) for k in keys)))
+        return out
 
     def laplacian(self, fixed_patches, psi=None, non_orth=False, grad_scheme=0, gamma=1.0, fixed_values=None):
         nc, ni, nf = self.n_cells, self.n_internal, self.n_faces

@@ -237,6 +237,90 @@ void vr_geometry(void* h, double* V, double* C, double* Sf, double* magSf, doubl
     copyVec(m.corrVecs(), corr);
 }
 
+void vr_skew(void* h, double* skew, int* skewed) {
+    const Mesh& m = *static_cast<RefMesh*>(h)->m;
+    copyVec(m.skewVecs(), skew);
+    *skewed = m.skewed() ? 1 : 0;
+}
+
+// The explicit operators of the pressure corrector (Fvc.h) on a scalar p and a vector U
+// (interleaved xyz) with fixedValue patches (face values pB / UB per boundary face,
+// interleaved for U) and zeroGradient elsewhere, and a face flux F:
+//   gLSp, gLSU   fvc::gradLS (value faces: fixed patches)            Fvc.h:185-206
+//   ip           fvc::interpolate(m, p)                              Fvc.h:24-31
+//   ipSkew, iU   fvc::interpolate(m, c, gradLS(c)) (skew-corrected)  Fvc.h:72-81
+//   gGL, gLS, gGS  fvc::grad(pField, GaussLinear / LeastSquares / GaussSkew)  Fvc.h:208-291
+//   divF         fvc::div(m, F)                                      Fvc.h:294-305
+//   siU          fvc::surfaceIntegrate(m, iU)                        Fvc.h:307-318
+//   phi          dot(iU_f, Sf) on internal faces; on boundary faces dot(UB, Sf) on the
+//                fixed patches and dot(U_P, Sf) elsewhere (phiHbyA, IncompressibleFlow.cpp:276-291)
+int vr_fvc(void* h, const double* p, const double* U, const int* fixedPatch, const double* pB, const double* UB,
+           const double* F, double* gLSp, double* gLSU, double* ip, double* ipSkew, double* iU, double* gGL,
+           double* gLS, double* gGS, double* divF, double* siU, double* phi) {
+    try {
+        MeshPtr mp = static_cast<RefMesh*>(h)->m;
+        const Mesh& m = *mp;
+        const label n = m.nCells(), nI = m.nInternalFaces(), nB = m.nBoundaryFaces(), nF = m.nFaces();
+        std::vector<scalar> pc(p, p + n);
+        std::vector<Vec3> Uc(n);
+        for (label c = 0; c < n; ++c) Uc[c] = Vec3{U[3 * c], U[3 * c + 1], U[3 * c + 2]};
+        std::vector<char> bFix(nB, 0);
+        std::vector<scalar> pBv(pB, pB + nB);
+        std::vector<Vec3> UBv(nB);
+        for (std::size_t q = 0; q < m.patches().size(); ++q) {
+            const Patch& pt = m.patches()[q];
+            for (label i = 0; i < pt.size; ++i) bFix[pt.start + i - nI] = fixedPatch[q] ? 1 : 0;
+        }
+        for (label b = 0; b < nB; ++b) UBv[b] = Vec3{UB[3 * b], UB[3 * b + 1], UB[3 * b + 2]};
+        const auto gp = fvc::gradLS(m, pc, bFix, pBv);
+        const auto gU = fvc::gradLS(m, Uc, bFix, UBv);
+        for (label c = 0; c < n; ++c) {
+            gLSp[3 * c] = gp[c].x;
+            gLSp[3 * c + 1] = gp[c].y;
+            gLSp[3 * c + 2] = gp[c].z;
+            for (int k = 0; k < 9; ++k) gLSU[9 * c + k] = (&gU[c].xx)[k];
+        }
+        copySc(fvc::interpolate(m, pc), ip);
+        copySc(fvc::interpolate(m, pc, gp), ipSkew);
+        const auto iUv = fvc::interpolate(m, Uc, gU);
+        copyVec(iUv, iU);
+        ScalarField pf(mp, "p", 0.0);
+        for (label c = 0; c < n; ++c) pf[c] = p[c];
+        for (std::size_t q = 0; q < m.patches().size(); ++q) {
+            const auto& name = m.patches()[q].name;
+            if (fixedPatch[q]) {
+                pf.fixedValue(name, 0.0);
+                auto* fv = dynamic_cast<FixedValueBC<scalar>*>(&pf.bc(label(q)));
+                const Patch& pt = m.patches()[q];
+                for (label i = 0; i < pt.size; ++i) fv->values()[i] = pB[pt.start + i - nI];
+            } else {
+                pf.zeroGradient(name);
+            }
+        }
+        pf.correctBoundaryConditions();
+        auto put3 = [&](const std::vector<Vec3>& g, double* out) {
+            for (label c = 0; c < n; ++c) {
+                out[3 * c] = g[c].x;
+                out[3 * c + 1] = g[c].y;
+                out[3 * c + 2] = g[c].z;
+            }
+        };
+        put3(fvc::grad(pf, GradScheme::GaussLinear), gGL);
+        put3(fvc::grad(pf, GradScheme::LeastSquares), gLS);
+        put3(fvc::grad(pf, GradScheme::GaussSkew), gGS);
+        const std::vector<scalar> Fv(F, F + nF);
+        const auto dv = fvc::div(m, Fv);
+        std::copy(dv.begin(), dv.begin() + n, divF);
+        put3(fvc::surfaceIntegrate(m, iUv), siU);
+        for (label f = 0; f < nI; ++f) phi[f] = dot(iUv[f], m.Sf()[f]);
+        for (label f = nI; f < nF; ++f)
+            phi[f] = bFix[f - nI] ? dot(UBv[f - nI], m.Sf()[f]) : dot(Uc[m.owner()[f]], m.Sf()[f]);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+
 void vr_cellfaces(void* h, int* off, int* list) {
     const Mesh& m = *static_cast<RefMesh*>(h)->m;
     std::memcpy(off, m.cellFaceOffsets().data(), sizeof(int) * m.cellFaceOffsets().size());

@@ -209,6 +209,38 @@ def post_case(name, make):
     print(f"post_{name}: {n} cells", flush=True)
 
 
+FVC_FIXED = (0,)  # fixedValue patches of the corrector-operator fixtures, with xMax
+FVC_MESHES = {"box8x6x5": SMALL["box8x6x5"], "tet5x4x3": SMALL["tet5x4x3"],
+              "tet6x5x4": lambda: M.tet(6, 5, 4, 0.24, 11)}
+
+
+def fvc_case(name, make):
+    """The reference's explicit corrector operators (Fvc.h: gradLS of a scalar and a
+    vector, interpolate plain and skew-corrected, grad Gauss linear / least squares /
+    Gauss skew, div, surfaceIntegrate, the phiHbyA face flux) on seeded fields, and the
+    mesh's skew vectors."""
+    m = make()
+    rm = ref.RefMesh(mesh=m)
+    n, nf, ni = m.n_cells, m.n_faces, m.n_internal
+    nb = nf - ni
+    fixed = sorted(set(FVC_FIXED) | {W.XMAX_PATCH})
+    inp = {"p": 2.0 * W.splitmix_uniform(n, 61) - 1.0,
+           "U": (2.0 * W.splitmix_uniform(3 * n, 62) - 1.0).reshape(n, 3),
+           "pB": 2.0 * W.splitmix_uniform(nb, 63) - 1.0,
+           "UB": (2.0 * W.splitmix_uniform(3 * nb, 64) - 1.0).reshape(nb, 3),
+           "F": (2.0 * W.splitmix_uniform(nf, 65) - 1.0) * m.geometry()["magSf"],
+           "fixed": np.array(fixed, np.int32)}
+    out = {"mesh_hash": np.frombuffer(bytes.fromhex(mesh_hash(m)), np.uint8)}
+    for k, v in inp.items():
+        out["in_" + k] = v
+    for k, v in rm.fvc(inp["p"], inp["U"], fixed, inp["pB"], inp["UB"], inp["F"]).items():
+        out[k] = v
+    sk, flag = rm.skew()
+    out["skew"], out["skewed"] = sk, np.array([int(flag)], np.int32)
+    np.savez_compressed(os.path.join(HERE, f"fvc_{name}.npz"), **out)
+    print(f"fvc_{name}: {n} cells, skewed {flag}", flush=True)
+
+
 def summary(x):
     x = np.asarray(x)
     idx = np.linspace(0, x.size - 1, 16).astype(np.int64)
@@ -341,8 +373,14 @@ if __name__ == "__main__":
         for nm, mk in SMALL.items():
             post_case(nm, mk)
         sys.exit(0)
+    if "--fvc" in sys.argv:
+        for nm, mk in FVC_MESHES.items():
+            fvc_case(nm, mk)
+        sys.exit(0)
     for nm, mk in SMALL.items():
         post_case(nm, mk)
+    for nm, mk in FVC_MESHES.items():
+        fvc_case(nm, mk)
     dropin_case()
     exactsum_case()
     for nm, mk in SMALL.items():

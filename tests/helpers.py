@@ -60,3 +60,26 @@ def post_inputs(g, tag):
     return {"psi0": g[f"{tag}_in_psi0"], "F": g[f"{tag}_in_F"], "fv": g[f"{tag}_in_fv"],
             "bcfixed": g[f"{tag}_in_bcfixed"], "dt": float(s[0]), "nu": float(s[1]), "alpha": float(s[2]),
             "ref_cell": int(s[3]), "ref_value": float(s[4]), "pressure": bool(s[5])}
+
+
+FVC_MESHES = {
+    "box8x6x5": lambda: M.box(8, 6, 5),
+    "tet5x4x3": lambda: M.tet(5, 4, 3, 0.2, 7),
+    "tet6x5x4": lambda: M.tet(6, 5, 4, 0.24, 11),
+}
+
+
+def load_fvc(name):
+    """Reference explicit corrector operators (Fvc.h) on one small mesh."""
+    with np.load(os.path.join(GOLDEN, f"fvc_{name}.npz"), allow_pickle=False) as z:
+        return {k: z[k] for k in z.files}
+
+
+def fvc_bfix(m, fixed):
+    """Per boundary face: 1 on the faces of the given patches."""
+    ni = m.n_internal
+    out = np.zeros(m.n_faces - ni, np.int32)
+    for p in fixed:
+        s = int(m.patch_start[p]) - ni
+        out[s: s + int(m.patch_size[p])] = 1
+    return out

@@ -28,6 +28,7 @@ VOL_HOST, VOL_DEVICE = 0, 1
 SUMS_CONTEXT, SUMS_FAST, SUMS_REPRODUCIBLE, SUMS_EXACT = -1, 0, 1, 2
 FP64, MIXED = 0, 1
 GAMG_JACOBI, GAMG_GAUSS_SEIDEL = 0, 1
+GRAD_GAUSS_LINEAR, GRAD_LEAST_SQUARES, GRAD_GAUSS_SKEW = 0, 1, 2
 
 _ERRORS = {
     -1: "invalid argument", -2: "unknown solver", -3: "unknown preconditioner", -4: "PCG on nonsymmetric matrix",
@@ -92,7 +93,8 @@ EXPORTED = ("vol_controls_default", "vol_last_error", "vol_context_create", "vol
             "vol_batch_create", "vol_batch_destroy", "vol_batch_info", "vol_batch_set", "vol_batch_solve",
             "vol_server_create", "vol_server_add_mesh", "vol_server_submit", "vol_server_wait", "vol_server_stats",
             "vol_server_destroy", "vol_fvm_finish_source", "vol_fvm_relax", "vol_fvm_set_reference", "vol_fvm_A",
-            "vol_fvm_H", "vol_fvm_flux")
+            "vol_fvm_H", "vol_fvm_flux", "vol_mesh_set_geometry", "vol_fvc_interpolate", "vol_fvc_grad", "vol_fvc_div",
+            "vol_fvc_surface_integrate", "vol_fvc_flux")
 
 
 def lib():
@@ -142,6 +144,12 @@ def lib():
         L.vol_fvm_A.argtypes = [P, P, I, P, I]
         L.vol_fvm_H.argtypes = [P, P, P, I, P, I]
         L.vol_fvm_flux.argtypes = [P, P, P, I]
+        L.vol_mesh_set_geometry.argtypes = [P, P, P, P, P, P, P, I, I]
+        L.vol_fvc_interpolate.argtypes = [P, I, P, P, P, P, I]
+        L.vol_fvc_grad.argtypes = [P, I, I, P, P, P, P, I]
+        L.vol_fvc_div.argtypes = [P, P, P, I]
+        L.vol_fvc_surface_integrate.argtypes = [P, I, P, P, I]
+        L.vol_fvc_flux.argtypes = [P, P, P, P, P, P, I]
         L.vol_batch_create.argtypes = [P, I, P, P, P, P, P, ctypes.POINTER(P)]
         L.vol_batch_destroy.argtypes = [P]
         L.vol_batch_info.argtypes = [P, P, P, P]
@@ -311,6 +319,55 @@ class LduMesh:
         bo = np.ascontiguousarray(boundary_owner, dtype=np.int32)
         _check(lib().vol_mesh_set_boundary(self._h, int(bo.size), _p(bo)))
         self.n_boundary = int(bo.size)
+
+    # ---- explicit corrector operators (Fvc.h) on the GPU, host arrays
+    def set_geometry(self, V, Sf, magSf, w, d, skew=None, skewed=False):
+        """Mesh geometry for the fvc operators (face arrays over internal + boundary faces)."""
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        arrs = [c(V), c(Sf), c(magSf), c(w), c(d), None if skew is None else c(skew)]
+        _check(lib().vol_mesh_set_geometry(self._h, *(_q(a) for a in arrs), int(bool(skewed)), VOL_HOST))
+
+    def _nf_total(self):
+        return self.n_faces + getattr(self, "n_boundary", 0)
+
+    def fvc_interpolate(self, c, grad=None, boundary_values=None):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        nc = 1 if c.ndim == 1 else 3
+        out = np.zeros((max(self._nf_total(), 1), nc))
+        g = None if grad is None else np.ascontiguousarray(grad, dtype=np.float64)
+        bv = None if boundary_values is None else np.ascontiguousarray(boundary_values, dtype=np.float64)
+        _check(lib().vol_fvc_interpolate(self._h, nc, _p(c), _q(g), _q(bv), _p(out), VOL_HOST))
+        out = out[: self._nf_total()]
+        return out[:, 0] if nc == 1 else out
+
+    def fvc_grad(self, scheme, c, b_fixed, b_value):
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        nc = 1 if c.ndim == 1 else 3
+        out = np.zeros((self.n_cells, 3 * nc))
+        bf = np.ascontiguousarray(b_fixed, dtype=np.int32)
+        bv = np.ascontiguousarray(b_value, dtype=np.float64)
+        _check(lib().vol_fvc_grad(self._h, int(scheme), nc, _p(c), _p(bf), _p(bv), _p(out), VOL_HOST))
+        return out
+
+    def fvc_div(self, F):
+        out = np.zeros(self.n_cells)
+        _check(lib().vol_fvc_div(self._h, _p(np.ascontiguousarray(F, dtype=np.float64)), _p(out), VOL_HOST))
+        return out
+
+    def fvc_surface_integrate(self, F):
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        nc = 1 if F.ndim == 1 else 3
+        out = np.zeros((self.n_cells, nc))
+        _check(lib().vol_fvc_surface_integrate(self._h, nc, _p(F), _p(out), VOL_HOST))
+        return out[:, 0] if nc == 1 else out
+
+    def fvc_flux(self, U, grad_U, b_fixed, Ub):
+        out = np.zeros(max(self._nf_total(), 1))
+        g = None if grad_U is None else np.ascontiguousarray(grad_U, dtype=np.float64)
+        _check(lib().vol_fvc_flux(self._h, _p(np.ascontiguousarray(U, dtype=np.float64)), _q(g),
+                                  _p(np.ascontiguousarray(b_fixed, dtype=np.int32)),
+                                  _p(np.ascontiguousarray(Ub, dtype=np.float64)), _p(out), VOL_HOST))
+        return out[: self._nf_total()]
 
     def gamg_levels(self):
         """[{n, agg, faceMap, faceFlip}] per level, as GamgHierarchy::levels."""

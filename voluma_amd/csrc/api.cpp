@@ -58,6 +58,13 @@ struct vol_mesh_s {
     // Gauss–Seidel), built on first use
     std::mutex sweepMu;
     bool sweepReady = false;
+    // geometry of the explicit corrector operators (vol_mesh_set_geometry) and their
+    // staging buffers for host pointers
+    bool geomSet = false;
+    int skewed = 0;
+    DevBuf<double> gV, gSf, gMagSf, gW, gD, gSkew;
+    DevBuf<double> fst[5], fout, fgls;
+    DevBuf<int> fsti;
     LevelOrderDev ordF;  // forward level order of level 0 (DIC/DILU factor)
     DevBuf<int> lowOff, lowCol, lowSrc, upOff, upCol, upSrc;
     LevelEllDev diluF, diluB;  // substitution rows (entry source: list index)
@@ -1454,6 +1461,63 @@ void meshRelease(vol_mesh m) {
 }
 
 // ============================================================================ C ABI
+// ------------------------------------------------------------------ explicit corrector operators
+namespace {
+template <class T> void devCopy(DevBuf<T>& dst, const T* src, size_t cnt, int location, hipStream_t st) {
+    dst.alloc(std::max<size_t>(cnt, 1));
+    if (cnt)
+        HIPCHECK(hipMemcpyAsync(dst.p, src, sizeof(T) * cnt,
+                                location == VOL_DEVICE ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, st));
+}
+
+FvcGeom fvcGeom(vol_mesh m) {
+    if (!m->geomSet) throw VolError(VOL_ERR_INVALID_ARGUMENT, "mesh geometry not set (vol_mesh_set_geometry)");
+    ensureOwnNei(m);
+    FvcGeom G;
+    G.n = m->n;
+    G.nf = m->nf;
+    G.nb = m->nb;
+    G.S = m->sell.view();
+    G.bOff = m->bOff.p;
+    G.bList = m->bList.p;
+    G.own = m->dOwn.p;
+    G.nei = m->dNei.p;
+    G.V = m->gV.p;
+    G.Sf = m->gSf.p;
+    G.magSf = m->gMagSf.p;
+    G.w = m->gW.p;
+    G.d = m->gD.p;
+    G.skew = m->gSkew.p;
+    G.skewed = m->skewed;
+    return G;
+}
+
+const double* fvcIn(vol_mesh m, int k, const double* p, size_t cnt, int location) {
+    if (!p || location == VOL_DEVICE) return p;
+    devCopy(m->fst[k], p, cnt, location, m->ctx->stream);
+    return m->fst[k].p;
+}
+const int* fvcInI(vol_mesh m, const int32_t* p, size_t cnt, int location) {
+    if (!p || location == VOL_DEVICE) return p;
+    devCopy(m->fsti, p, cnt, location, m->ctx->stream);
+    return m->fsti.p;
+}
+double* fvcOutBuf(vol_mesh m, double* out, size_t cnt, int location) {
+    if (location == VOL_DEVICE) return out;
+    m->fout.alloc(std::max<size_t>(cnt, 1));
+    return m->fout.p;
+}
+void fvcOut(vol_mesh m, double* out, size_t cnt, int location) {
+    hipStream_t st = m->ctx->stream;
+    if (location != VOL_DEVICE && cnt)
+        HIPCHECK(hipMemcpyAsync(out, m->fout.p, sizeof(double) * cnt, hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+}
+void fvcComponents(int nc) {
+    if (nc != 1 && nc != 3) throw VolError(VOL_ERR_INVALID_ARGUMENT, "nComponents must be 1 or 3");
+}
+}  // namespace
+
 extern "C" {
 
 void vol_controls_default(vol_solver_controls* c) {
@@ -1657,6 +1721,120 @@ int vol_mesh_set_boundary(vol_mesh mesh, int nBoundaryFaces, const int32_t* boun
     mesh->bList.upload(list, mesh->ctx->stream);
     mesh->nb = nBoundaryFaces;
     HIPCHECK(hipStreamSynchronize(mesh->ctx->stream));
+    API_END
+}
+
+
+int vol_mesh_set_geometry(vol_mesh mesh, const double* V, const double* Sf, const double* magSf, const double* w,
+                          const double* d, const double* skew, int skewed, int location) {
+    API_BEGIN
+    if (!mesh || !V || !Sf || !magSf || !w || !d) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    if (mesh->nb < 0) throw VolError(VOL_ERR_INVALID_ARGUMENT, "mesh boundary not set (vol_mesh_set_boundary)");
+    setDevice(mesh->ctx);
+    hipStream_t st = mesh->ctx->stream;
+    const size_t n = mesh->n, nF = size_t(mesh->nf) + mesh->nb;
+    devCopy(mesh->gV, V, n, location, st);
+    devCopy(mesh->gSf, Sf, 3 * nF, location, st);
+    devCopy(mesh->gMagSf, magSf, nF, location, st);
+    devCopy(mesh->gW, w, nF, location, st);
+    devCopy(mesh->gD, d, 3 * nF, location, st);
+    if (skew) {
+        devCopy(mesh->gSkew, skew, 3 * nF, location, st);
+    } else {
+        mesh->gSkew.alloc(std::max<size_t>(3 * nF, 1));
+        HIPCHECK(hipMemsetAsync(mesh->gSkew.p, 0, sizeof(double) * std::max<size_t>(3 * nF, 1), st));
+    }
+    mesh->skewed = skewed ? 1 : 0;
+    mesh->geomSet = true;
+    HIPCHECK(hipStreamSynchronize(st));
+    API_END
+}
+
+int vol_fvc_interpolate(vol_mesh mesh, int nComponents, const double* cellValues, const double* cellGrad,
+                        const double* boundaryValues, double* faceValues, int location) {
+    API_BEGIN
+    if (!mesh || !cellValues || !faceValues) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    fvcComponents(nComponents);
+    setDevice(mesh->ctx);
+    const FvcGeom G = fvcGeom(mesh);
+    const size_t nc = nComponents, n = G.n, nF = size_t(G.nf) + G.nb;
+    launch_fvc_interpolate(mesh->ctx->stream, G, nComponents, fvcIn(mesh, 0, cellValues, nc * n, location),
+                           fvcIn(mesh, 1, cellGrad, 3 * nc * n, location),
+                           fvcIn(mesh, 2, boundaryValues, nc * G.nb, location),
+                           fvcOutBuf(mesh, faceValues, nc * nF, location));
+    fvcOut(mesh, faceValues, nc * nF, location);
+    API_END
+}
+
+int vol_fvc_grad(vol_mesh mesh, int scheme, int nComponents, const double* cellValues, const int32_t* bFixesValue,
+                 const double* bValue, double* cellGrad, int location) {
+    API_BEGIN
+    if (!mesh || !cellValues || !cellGrad) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    fvcComponents(nComponents);
+    if (scheme != VOL_GRAD_GAUSS_LINEAR && scheme != VOL_GRAD_LEAST_SQUARES && scheme != VOL_GRAD_GAUSS_SKEW)
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "unknown gradient scheme");
+    if (scheme != VOL_GRAD_LEAST_SQUARES && nComponents != 1)
+        throw VolError(VOL_ERR_UNSUPPORTED, "Gauss gradients are provided for scalar fields");
+    setDevice(mesh->ctx);
+    const FvcGeom G = fvcGeom(mesh);
+    if (G.nb > 0 && (!bFixesValue || !bValue))
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "bFixesValue and bValue required");
+    hipStream_t st = mesh->ctx->stream;
+    const size_t nc = nComponents, n = G.n;
+    const double* c = fvcIn(mesh, 0, cellValues, nc * n, location);
+    const int* bf = fvcInI(mesh, bFixesValue, G.nb, location);
+    const double* bv = fvcIn(mesh, 1, bValue, nc * G.nb, location);
+    double* out = fvcOutBuf(mesh, cellGrad, 3 * nc * n, location);
+    if (scheme == VOL_GRAD_LEAST_SQUARES) {
+        launch_fvc_grad_ls(st, G, nComponents, c, bf, bv, out);
+    } else if (scheme == VOL_GRAD_GAUSS_LINEAR) {
+        launch_fvc_grad_gauss(st, G, false, c, nullptr, bf, bv, out);
+    } else {
+        mesh->fgls.alloc(std::max<size_t>(3 * n, 1));
+        launch_fvc_grad_ls(st, G, 1, c, bf, bv, mesh->fgls.p);
+        launch_fvc_grad_gauss(st, G, true, c, mesh->fgls.p, bf, bv, out);
+    }
+    fvcOut(mesh, cellGrad, 3 * nc * n, location);
+    API_END
+}
+
+int vol_fvc_div(vol_mesh mesh, const double* faceFlux, double* out, int location) {
+    API_BEGIN
+    if (!mesh || !faceFlux || !out) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    setDevice(mesh->ctx);
+    const FvcGeom G = fvcGeom(mesh);
+    launch_fvc_integrate(mesh->ctx->stream, G, 1, true, fvcIn(mesh, 0, faceFlux, size_t(G.nf) + G.nb, location),
+                         fvcOutBuf(mesh, out, G.n, location));
+    fvcOut(mesh, out, G.n, location);
+    API_END
+}
+
+int vol_fvc_surface_integrate(vol_mesh mesh, int nComponents, const double* faceValues, double* out, int location) {
+    API_BEGIN
+    if (!mesh || !faceValues || !out) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    fvcComponents(nComponents);
+    setDevice(mesh->ctx);
+    const FvcGeom G = fvcGeom(mesh);
+    const size_t nc = nComponents;
+    launch_fvc_integrate(mesh->ctx->stream, G, nComponents, false,
+                         fvcIn(mesh, 0, faceValues, nc * (size_t(G.nf) + G.nb), location),
+                         fvcOutBuf(mesh, out, nc * G.n, location));
+    fvcOut(mesh, out, nc * G.n, location);
+    API_END
+}
+
+int vol_fvc_flux(vol_mesh mesh, const double* U, const double* gradU, const int32_t* bFixesFlux, const double* Ub,
+                 double* phi, int location) {
+    API_BEGIN
+    if (!mesh || !U || !phi) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    setDevice(mesh->ctx);
+    const FvcGeom G = fvcGeom(mesh);
+    if (G.nb > 0 && (!bFixesFlux || !Ub)) throw VolError(VOL_ERR_INVALID_ARGUMENT, "bFixesFlux and Ub required");
+    const size_t n = G.n, nF = size_t(G.nf) + G.nb;
+    launch_fvc_flux(mesh->ctx->stream, G, fvcIn(mesh, 0, U, 3 * n, location), fvcIn(mesh, 1, gradU, 9 * n, location),
+                    fvcInI(mesh, bFixesFlux, G.nb, location), fvcIn(mesh, 2, Ub, 3 * size_t(G.nb), location),
+                    fvcOutBuf(mesh, phi, nF, location));
+    fvcOut(mesh, phi, nF, location);
     API_END
 }
 

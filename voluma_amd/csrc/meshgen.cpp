@@ -51,6 +51,8 @@ struct Raw {
     bool haveGeom = false;
     std::vector<double> V, magSf, w, ndc;
     std::vector<V3> C, Sf, Cf, d, corr;
+    std::vector<V3> skew;  // skew vectors of the internal faces (zero on boundary faces)
+    double skewMax = 0.0;
     void addFace(std::initializer_list<int64_t> p) {
         facePts.insert(facePts.end(), p.begin(), p.end());
         faceOff.push_back(int64_t(facePts.size()));
@@ -372,6 +374,8 @@ void computeGeometry(Raw& m) {
     m.w.assign(nF, 0.0);
     m.ndc.assign(nF, 0.0);
     m.corr.assign(nF, V3{});
+    m.skew.assign(nF, V3{});
+    m.skewMax = 0.0;
     for (int64_t f = 0; f < nF; ++f) {
         m.magSf[f] = mag(m.Sf[f]);
         const V3 n = m.Sf[f] / m.magSf[f];
@@ -381,6 +385,9 @@ void computeGeometry(Raw& m) {
             const double sOwn = std::abs(dot(m.Sf[f], m.Cf[f] - CP));
             const double sNei = std::abs(dot(m.Sf[f], CP + dd - m.Cf[f]));
             m.w[f] = sNei / (sOwn + sNei);
+            // face centre minus the linear-interpolation point (Mesh.cpp:618-621)
+            m.skew[f] = m.Cf[f] - (CP + dd * (1.0 - m.w[f]));
+            m.skewMax = std::max(m.skewMax, mag(m.skew[f]) / mag(dd));
         } else {
             m.w[f] = 1.0;
         }
@@ -468,6 +475,21 @@ int vm_geometry(void* h, double* V, double* C, double* Sf, double* magSf, double
         copyD(m.w, w);
         copyD(m.ndc, ndc);
         copyV3(m.corr, corr);
+        return 0;
+    } catch (const std::exception& e) {
+        gErr = e.what();
+        return -1;
+    }
+}
+
+// Skew vectors (interleaved xyz, nFaces; zero on boundary faces) and the largest
+// |skew|/|d| over internal faces (the reference calls a mesh skewed above 1e-8).
+int vm_skew(void* h, double* skew, double* skewMax) {
+    try {
+        Raw& m = *static_cast<Raw*>(h);
+        if (!m.haveGeom) computeGeometry(m);
+        copyV3(m.skew, skew);
+        if (skewMax) *skewMax = m.skewMax;
         return 0;
     } catch (const std::exception& e) {
         gErr = e.what();

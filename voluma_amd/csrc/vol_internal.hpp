@@ -331,6 +331,36 @@ struct FvmArgs {
 void launch_fvm_term(hipStream_t st, int kind, const FvmArgs& a, const int* own, const int* nei);
 // FvMatrix −= su (FvMatrix.h:93-97): source[c] += su[c]·V[c]
 void launch_fvm_source(hipStream_t st, int n, const double* su, const double* V, double* source);
+// ------------------------------------------------------------------ explicit corrector operators
+// Mesh view of the fvc kernels: level-0 slots and boundary lists for the per-cell
+// gathers, owner/neighbour for the per-face ones, and the geometry (face arrays hold
+// nf + nb entries; Sf, d, skew interleaved xyz).
+struct FvcGeom {
+    int n = 0, nf = 0, nb = 0;
+    SellView S;
+    const int* bOff = nullptr;
+    const int* bList = nullptr;
+    const int* own = nullptr;
+    const int* nei = nullptr;
+    const double *V = nullptr, *Sf = nullptr, *magSf = nullptr, *w = nullptr, *d = nullptr, *skew = nullptr;
+    int skewed = 0;
+};
+// fvc::interpolate (Fvc.h:13-31,72-81); grad nullable (skew correction when G.skewed);
+// bval nullable (boundary faces take the owner's value)
+void launch_fvc_interpolate(hipStream_t st, const FvcGeom& G, int nc, const double* c, const double* grad,
+                            const double* bval, double* out);
+// fvc::gradLS / grad(LeastSquares) (Fvc.h:134-206)
+void launch_fvc_grad_ls(hipStream_t st, const FvcGeom& G, int nc, const double* c, const int* bFix,
+                        const double* bVal, double* out);
+// fvc::grad GaussLinear / GaussSkew of a scalar (Fvc.h:213-259); gls: least-squares gradient (GaussSkew)
+void launch_fvc_grad_gauss(hipStream_t st, const FvcGeom& G, bool skew, const double* psi, const double* gls,
+                           const int* bFix, const double* bVal, double* out);
+// fvc::div (div = true, nc = 1) or fvc::surfaceIntegrate (Fvc.h:294-318)
+void launch_fvc_integrate(hipStream_t st, const FvcGeom& G, int nc, bool div, const double* F, double* out);
+// face flux of a cell vector field, phiHbyA form (IncompressibleFlow.cpp:276-291)
+void launch_fvc_flux(hipStream_t st, const FvcGeom& G, const double* U, const double* grad, const int* bFix,
+                     const double* Ub, double* phi);
+
 // ------------------------------------------------------------------ level-scheduled sweeps
 // Lower / upper neighbour lists of the cells (device arrays). Lower list of cell u: neighbours l < u in ascending (l, face) order;
 // upper list of cell l: neighbours u > l in descending (u, face) order; src = face << 1 |

@@ -42,6 +42,8 @@ def _lib():
         lib.vm_raw.argtypes = [P] * 8
         lib.vm_geometry.argtypes = [P] * 10
         lib.vm_geometry.restype = ctypes.c_int
+        lib.vm_skew.argtypes = [P, P, P]
+        lib.vm_skew.restype = ctypes.c_int
         lib.vm_last_error.restype = ctypes.c_char_p
         _LIB = lib
     return _LIB
@@ -92,7 +94,8 @@ class Mesh:
         return self.n_faces - self.n_internal
 
     def geometry(self):
-        """dict of V, C, Sf, magSf, Cf, d, w, ndc, corr (reference Mesh accessors)."""
+        """dict of V, C, Sf, magSf, Cf, d, w, ndc, corr, skew (skewVecs) and skewed
+        (reference Mesh accessors)."""
         if self._geom is None:
             nc, nf = self.n_cells, self.n_faces
             g = {"V": np.zeros(nc), "C": np.zeros((nc, 3)), "Sf": np.zeros((nf, 3)), "magSf": np.zeros(nf),
@@ -102,6 +105,11 @@ class Mesh:
                                               ("V", "C", "Sf", "magSf", "Cf", "d", "w", "ndc", "corr")))
             if rc != 0:
                 raise RuntimeError(_lib().vm_last_error().decode())
+            g["skew"] = np.zeros((nf, 3))
+            smax = np.zeros(1)
+            if _lib().vm_skew(self._h, _ptr(g["skew"]), _ptr(smax)) != 0:
+                raise RuntimeError(_lib().vm_last_error().decode())
+            g["skewed"] = bool(smax[0] > 1e-8)  # Mesh.cpp:596
             self._geom = g
         return self._geom
 
