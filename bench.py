@@ -144,6 +144,56 @@ def cpu_baseline_dilu(m, d, u, lo, b, x0):
             "kind": "port"}
 
 
+def corrector_ops_leg(va, W, ctx, m, reps=20):
+    """The explicit pressure-corrector operators (Fvc.h) on the benchmark mesh, device
+    pointers: milliseconds per call and GB/s of their algorithmic bytes (arrays each
+    must read or write once, int32 addressing included)."""
+    n, ni, nF = m.n_cells, m.n_internal, m.n_faces
+    nb = nF - ni
+    lm = va.LduMesh(ctx, n, m.owner[:ni], m.neighbour, None)
+    lm.set_boundary(m.owner[ni:])
+    g = m.geometry()
+    lm.set_geometry(g["V"], g["Sf"], g["magSf"], g["w"], g["d"], g["skew"], g["skewed"])
+    L = va.lib()
+    dev = lambda a, dt=np.float64: va.DeviceArray.from_host(ctx, np.ascontiguousarray(a, dtype=dt).ravel(), dt)
+    p = dev(2.0 * W.splitmix_uniform(n, 81) - 1.0)
+    U = dev(2.0 * W.splitmix_uniform(3 * n, 82) - 1.0)
+    bfix = dev(np.ones(nb), np.int32)
+    pB, UB = dev(np.zeros(nb)), dev(np.zeros(3 * nb))
+    gp, gU = va.DeviceArray(ctx, 3 * n), va.DeviceArray(ctx, 9 * n)
+    fS, phi, cell = va.DeviceArray(ctx, nF), va.DeviceArray(ctx, nF), va.DeviceArray(ctx, n)
+    D = va.VOL_DEVICE
+    addr = 8 * ni + 4 * nb
+    ops = {
+        "grad_ls_scalar": (lambda: L.vol_fvc_grad(lm._h, va.GRAD_LEAST_SQUARES, 1, p.ptr, bfix.ptr, pB.ptr, gp.ptr, D),
+                           32 * n + 24 * nF + 48 * nb + addr),
+        "grad_ls_vector": (lambda: L.vol_fvc_grad(lm._h, va.GRAD_LEAST_SQUARES, 3, U.ptr, bfix.ptr, UB.ptr, gU.ptr, D),
+                           96 * n + 24 * nF + 64 * nb + addr),
+        "grad_gauss_skew": (lambda: L.vol_fvc_grad(lm._h, va.GRAD_GAUSS_SKEW, 1, p.ptr, bfix.ptr, pB.ptr, gp.ptr, D),
+                            88 * n + 24 * nF + 48 * nb + 32 * ni + 24 * nF + addr),
+        "interpolate_skew": (lambda: L.vol_fvc_interpolate(lm._h, 1, p.ptr, gp.ptr, None, fS.ptr, D),
+                             32 * n + 40 * ni + 8 * nF + addr),
+        "flux_phiHbyA": (lambda: L.vol_fvc_flux(lm._h, U.ptr, gU.ptr, bfix.ptr, UB.ptr, phi.ptr, D),
+                         96 * n + 32 * ni + 32 * nF + 28 * nb + addr),
+        "div": (lambda: L.vol_fvc_div(lm._h, phi.ptr, cell.ptr, D), 16 * n + 8 * nF + addr),
+    }
+    out = {}
+    for name, (fn, nbytes) in ops.items():
+        va._check(fn())
+        ctx.synchronize()
+        t = time.perf_counter()
+        for k in range(reps):
+            va._check(fn())
+        ctx.synchronize()
+        sec = (time.perf_counter() - t) / reps
+        out[name] = {"ms": sec * 1e3, "algorithmic_bytes": nbytes, "gbs": nbytes / sec / 1e9}
+    for a in (p, U, bfix, pB, UB, gp, gU, fS, phi, cell):
+        a.free()
+    lm.close()
+    log("corrector ops: " + ", ".join(f"{k} {v['ms']:.2f} ms ({v['gbs']:.0f} GB/s)" for k, v in out.items()))
+    return out
+
+
 def reference_solvers_leg(va, W, ctx, m, A, db, dx, dzero, args, cpu):
     """The reference's own solver choices on the GPU: GAMG-PCG with its Gauss–Seidel
     smoother (2/2 sweeps) on the pressure system, and its default momentum solver
@@ -208,6 +258,7 @@ def main():
     ap.add_argument("--batch-systems", type=int, default=256)
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-ref-solvers", action="store_true", help="skip the GS-GAMG and DILU legs")
+    ap.add_argument("--no-corrector-ops", action="store_true", help="skip the fvc operator timings")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="request arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--pre", type=int, default=1)
@@ -326,6 +377,9 @@ def main():
     serve = None
     if not args.no_serve:
         serve = serve_leg(va, ctx, args)
+    corrector_ops = None
+    if not args.no_corrector_ops:
+        corrector_ops = corrector_ops_leg(va, W, ctx, m)
     ref_solvers = None
     if not args.no_ref_solvers:
         ref_solvers = reference_solvers_leg(va, W, ctx, m, A, db, dx, dzero, args, not args.no_cpu_baseline)
@@ -352,6 +406,7 @@ def main():
         "batch": batch,
         "serve": serve,
         "reference_solvers": ref_solvers,
+        "corrector_ops": corrector_ops,
         "iterations": int(np.median(iters)),
         "final_residual": p.finalResidual,
         "true_residual": true_res,
