@@ -265,6 +265,19 @@ int vol_fvm_H(vol_matrix A, const double* psi, const double* V, int nComponents,
  * boundary faces, + faceFluxCorrection when a laplacian term switched it on. */
 int vol_fvm_flux(vol_matrix A, const double* psi, double* out, int location);
 
+/* ---- mesh geometry on the GPU (Mesh.cpp:21-48 and the cell pyramid decomposition;
+ * Mesh.cpp:600-630), bit-identical to the reference's Mesh: from points (interleaved
+ * xyz) and faces (faceOffsets, nFaces + 1 entries, into facePoints; every face ≥ 3
+ * points) with owner (nFaces) and neighbour (nInternalFaces), as RawMesh holds them;
+ * outputs V, C (cells), Sf, Cf, d, magSf, w, ndc, corrVecs, skewVecs (faces; skew
+ * nullable) and *maxSkewness (host pointer, nullable; the mesh is skewed above 1e-8,
+ * Mesh.cpp:596).
+ * Labels are int32. Inputs and outputs on the host or all on the device (location). */
+int vol_geometry(vol_context ctx, int nPoints, const double* points, int nFaces, const int32_t* faceOffsets,
+                 const int32_t* facePoints, int nInternalFaces, const int32_t* owner, const int32_t* neighbour,
+                 int nCells, double* V, double* C, double* Sf, double* Cf, double* d, double* magSf, double* w,
+                 double* ndc, double* corrVecs, double* skewVecs, double* maxSkewness, int location);
+
 /* ---- explicit operators of the pressure corrector (Fvc.h), on the GPU.
  * vol_mesh_set_geometry stores the mesh geometry the operators need (after
  * vol_mesh_set_boundary): V (nCells); per face (nInternalFaces + nBoundaryFaces): Sf and

@@ -94,7 +94,7 @@ EXPORTED = ("vol_controls_default", "vol_last_error", "vol_context_create", "vol
             "vol_server_create", "vol_server_add_mesh", "vol_server_submit", "vol_server_wait", "vol_server_stats",
             "vol_server_destroy", "vol_fvm_finish_source", "vol_fvm_relax", "vol_fvm_set_reference", "vol_fvm_A",
             "vol_fvm_H", "vol_fvm_flux", "vol_mesh_set_geometry", "vol_fvc_interpolate", "vol_fvc_grad", "vol_fvc_div",
-            "vol_fvc_surface_integrate", "vol_fvc_flux")
+            "vol_fvc_surface_integrate", "vol_fvc_flux", "vol_geometry")
 
 
 def lib():
@@ -144,6 +144,7 @@ def lib():
         L.vol_fvm_A.argtypes = [P, P, I, P, I]
         L.vol_fvm_H.argtypes = [P, P, P, I, P, I]
         L.vol_fvm_flux.argtypes = [P, P, P, I]
+        L.vol_geometry.argtypes = [P, I, P, I, P, P, I, P, P, I] + [P] * 11 + [I]
         L.vol_mesh_set_geometry.argtypes = [P, P, P, P, P, P, P, I, I]
         L.vol_fvc_interpolate.argtypes = [P, I, P, P, P, P, I]
         L.vol_fvc_grad.argtypes = [P, I, I, P, P, P, P, I]
@@ -384,6 +385,25 @@ class LduMesh:
             _check(lib().vol_mesh_gamg_level(self._h, k, ctypes.byref(nf), _p(agg), _p(fm), _p(ff)))
             out.append({"n": int(sizes[k]), "agg": agg, "faceMap": fm[: nf.value], "faceFlip": ff[: nf.value]})
         return out
+
+
+def device_geometry(ctx: "Context", mesh):
+    """Mesh geometry computed on the GPU (vol_geometry) from a voluma_amd.mesh.Mesh's
+    points and faces: dict of V, C, Sf, magSf, Cf, d, w, ndc, corr, skew, skewed."""
+    nc, nf, ni = mesh.n_cells, mesh.n_faces, mesh.n_internal
+    i32 = lambda a: np.ascontiguousarray(a, dtype=np.int32)
+    pts = np.ascontiguousarray(mesh.points, dtype=np.float64)
+    g = {"V": np.zeros(nc), "C": np.zeros((nc, 3)), "Sf": np.zeros((nf, 3)), "Cf": np.zeros((nf, 3)),
+         "d": np.zeros((nf, 3)), "magSf": np.zeros(nf), "w": np.zeros(nf), "ndc": np.zeros(nf),
+         "corr": np.zeros((nf, 3)), "skew": np.zeros((nf, 3))}
+    fo, fp, own, nei = i32(mesh.face_off), i32(mesh.face_pts), i32(mesh.owner), i32(mesh.neighbour)
+    smax = ctypes.c_double()
+    _check(lib().vol_geometry(ctx._h, int(pts.shape[0]), _p(pts), nf, _p(fo), _p(fp), ni, _p(own), _p(nei), nc,
+                              *(_p(g[k]) for k in ("V", "C", "Sf", "Cf", "d", "magSf", "w", "ndc", "corr", "skew")),
+                              ctypes.byref(smax), VOL_HOST))
+    g["skewed"] = bool(smax.value > 1e-8)
+    g["max_skewness"] = smax.value
+    return g
 
 
 class LduMatrix:

@@ -1725,6 +1725,126 @@ int vol_mesh_set_boundary(vol_mesh mesh, int nBoundaryFaces, const int32_t* boun
 }
 
 
+int vol_geometry(vol_context ctx, int nPoints, const double* points, int nFaces, const int32_t* faceOffsets,
+                 const int32_t* facePoints, int nInternalFaces, const int32_t* owner, const int32_t* neighbour,
+                 int nCells, double* V, double* C, double* Sf, double* Cf, double* d, double* magSf, double* w,
+                 double* ndc, double* corrVecs, double* skewVecs, double* maxSkewness, int location) {
+    API_BEGIN
+    if (!ctx || !points || !faceOffsets || !facePoints || !owner || (nInternalFaces > 0 && !neighbour) || !V || !C ||
+        !Sf || !Cf || !d || !magSf || !w || !ndc || !corrVecs)
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    if (nPoints < 0 || nFaces < 0 || nCells <= 0 || nInternalFaces < 0 || nInternalFaces > nFaces)
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "bad mesh sizes");
+    setDevice(ctx);
+    hipStream_t st = ctx->stream;
+    const bool dev = location == VOL_DEVICE;
+    // addressing on the host: checked here (every index the kernels follow) and used for
+    // the cell → face lists
+    auto host = [&](const int32_t* p, size_t cnt) {
+        std::vector<int> v(cnt);
+        if (cnt)
+            HIPCHECK(hipMemcpy(v.data(), p, sizeof(int) * cnt, dev ? hipMemcpyDeviceToHost : hipMemcpyHostToHost));
+        return v;
+    };
+    const std::vector<int> fo = host(faceOffsets, size_t(nFaces) + 1), own = host(owner, nFaces),
+                           nei = host(neighbour, nInternalFaces);
+    if (fo[0] != 0) throw VolError(VOL_ERR_INVALID_ARGUMENT, "faceOffsets must start at 0");
+    for (int f = 0; f < nFaces; ++f)
+        if (fo[f + 1] - fo[f] < 3) throw VolError(VOL_ERR_INVALID_ARGUMENT, "face with fewer than 3 points");
+    const std::vector<int> fp = host(facePoints, size_t(fo[nFaces]));
+    for (int k : fp)
+        if (k < 0 || k >= nPoints) throw VolError(VOL_ERR_INVALID_ARGUMENT, "face point out of range");
+    std::vector<int> cfOff(size_t(nCells) + 1, 0);
+    for (int f = 0; f < nFaces; ++f) {
+        if (own[f] < 0 || own[f] >= nCells || (f < nInternalFaces && (nei[f] < 0 || nei[f] >= nCells)))
+            throw VolError(VOL_ERR_INVALID_ARGUMENT, "bad owner/neighbour at face " + std::to_string(f));
+        ++cfOff[own[f] + 1];
+        if (f < nInternalFaces) ++cfOff[nei[f] + 1];
+    }
+    for (int c = 0; c < nCells; ++c) {
+        if (cfOff[c + 1] == 0) throw VolError(VOL_ERR_INVALID_ARGUMENT, "cell without faces");
+        cfOff[c + 1] += cfOff[c];
+    }
+    std::vector<int> cfList(cfOff[nCells]);
+    {
+        std::vector<int> fill(cfOff.begin(), cfOff.end() - 1);
+        for (int f = 0; f < nFaces; ++f) {
+            cfList[fill[own[f]]++] = f;
+            if (f < nInternalFaces) cfList[fill[nei[f]]++] = f;
+        }
+    }
+    DevBuf<double> dPts, o[10];
+    DevBuf<int> dFo, dFp, dOwn, dNei, dCfo, dCfl;
+    DevBuf<unsigned long long> dMax;
+    dFo.upload(fo, st);
+    dFp.upload(fp, st);
+    dOwn.upload(own, st);
+    dNei.upload(nei, st);
+    dCfo.upload(cfOff, st);
+    dCfl.upload(cfList, st);
+    GeoArgs g;
+    g.nCells = nCells;
+    g.nFaces = nFaces;
+    g.nInternal = nInternalFaces;
+    if (dev) {
+        g.points = points;
+    } else {
+        dPts.alloc(std::max<size_t>(3 * size_t(nPoints), 1));
+        if (nPoints)
+            HIPCHECK(hipMemcpyAsync(dPts.p, points, sizeof(double) * 3 * nPoints, hipMemcpyHostToDevice, st));
+        g.points = dPts.p;
+    }
+    g.faceOff = dFo.p;
+    g.facePts = dFp.p;
+    g.owner = dOwn.p;
+    g.neighbour = dNei.p;
+    g.cfOff = dCfo.p;
+    g.cfList = dCfl.p;
+    double* outs[10] = {V, C, Sf, Cf, d, magSf, w, ndc, corrVecs, skewVecs};
+    const size_t cnt[10] = {size_t(nCells),     3 * size_t(nCells), 3 * size_t(nFaces), 3 * size_t(nFaces),
+                            3 * size_t(nFaces), size_t(nFaces),     size_t(nFaces),     size_t(nFaces),
+                            3 * size_t(nFaces), 3 * size_t(nFaces)};
+    double* dst[10];
+    for (int k = 0; k < 10; ++k) {
+        if (!outs[k]) {
+            dst[k] = nullptr;
+        } else if (dev) {
+            dst[k] = outs[k];
+        } else {
+            o[k].alloc(std::max<size_t>(cnt[k], 1));
+            dst[k] = o[k].p;
+        }
+    }
+    g.V = dst[0];
+    g.C = dst[1];
+    g.Sf = dst[2];
+    g.Cf = dst[3];
+    g.d = dst[4];
+    g.magSf = dst[5];
+    g.w = dst[6];
+    g.ndc = dst[7];
+    g.corr = dst[8];
+    g.skew = dst[9];
+    if (maxSkewness) {
+        dMax.alloc(1);
+        HIPCHECK(hipMemsetAsync(dMax.p, 0, sizeof(unsigned long long), st));
+        g.maxSkewBits = dMax.p;
+    }
+    launch_geometry(st, g);
+    if (!dev)
+        for (int k = 0; k < 10; ++k)
+            if (outs[k] && cnt[k])
+                HIPCHECK(hipMemcpyAsync(outs[k], dst[k], sizeof(double) * cnt[k], hipMemcpyDeviceToHost, st));
+    if (maxSkewness) {
+        unsigned long long bits = 0;
+        HIPCHECK(hipMemcpyAsync(&bits, dMax.p, sizeof(bits), hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipStreamSynchronize(st));
+        std::memcpy(maxSkewness, &bits, sizeof(double));
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    API_END
+}
+
 int vol_mesh_set_geometry(vol_mesh mesh, const double* V, const double* Sf, const double* magSf, const double* w,
                           const double* d, const double* skew, int skewed, int location) {
     API_BEGIN

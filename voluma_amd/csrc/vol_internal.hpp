@@ -331,6 +331,26 @@ struct FvmArgs {
 void launch_fvm_term(hipStream_t st, int kind, const FvmArgs& a, const int* own, const int* nei);
 // FvMatrix −= su (FvMatrix.h:93-97): source[c] += su[c]·V[c]
 void launch_fvm_source(hipStream_t st, int n, const double* su, const double* V, double* source);
+// ------------------------------------------------------------------ mesh geometry
+// Inputs: points (interleaved xyz), faces as offsets into facePts (int32), owner of every
+// face, neighbour of the internal faces, and the cell → face lists (faces ascending).
+// Outputs per cell: V, C; per face: Sf, Cf, d, magSf, w, ndc, corr, skew (nullable);
+// maxSkewBits (nullable): atomic max of the bits of |skew|/|d| over internal faces.
+struct GeoArgs {
+    int nCells = 0, nFaces = 0, nInternal = 0;
+    const double* points = nullptr;
+    const int* faceOff = nullptr;
+    const int* facePts = nullptr;
+    const int* owner = nullptr;
+    const int* neighbour = nullptr;
+    const int* cfOff = nullptr;
+    const int* cfList = nullptr;
+    double *V = nullptr, *C = nullptr, *Sf = nullptr, *Cf = nullptr, *d = nullptr, *magSf = nullptr, *w = nullptr,
+           *ndc = nullptr, *corr = nullptr, *skew = nullptr;
+    unsigned long long* maxSkewBits = nullptr;
+};
+void launch_geometry(hipStream_t st, const GeoArgs& g);
+
 // ------------------------------------------------------------------ explicit corrector operators
 // Mesh view of the fvc kernels: level-0 slots and boundary lists for the per-cell
 // gathers, owner/neighbour for the per-face ones, and the geometry (face arrays hold
