@@ -49,6 +49,13 @@ def _lib():
         lib.vr_transport.argtypes = [P, D, D, P, P, P, P, P, I] + [P] * 9
         lib.vr_transport.restype = I
         lib.vr_skew.argtypes = [P, P, P]
+        lib.vr_read_polymesh.argtypes = [ctypes.c_char_p]
+        lib.vr_read_polymesh.restype = P
+        lib.vr_rawmesh_sizes.argtypes = [P, P]
+        lib.vr_rawmesh_arrays.argtypes = [P] * 10
+        lib.vr_rawmesh_build.argtypes = [P]
+        lib.vr_rawmesh_build.restype = P
+        lib.vr_rawmesh_free.argtypes = [P]
         lib.vr_fvc.argtypes = [P] * 18
         lib.vr_fvc.restype = I
         lib.vr_transport_post.argtypes = [P, I, I, D, D, P, P, P, P, I, D, ctypes.c_long, D] + [P] * 7
@@ -90,9 +97,11 @@ def exact_sum(v) -> float:
 class RefMesh:
     """Reference Mesh built from a voluma_amd.mesh.Mesh (raw arrays) or generateBox."""
 
-    def __init__(self, mesh=None, box=None):
+    def __init__(self, mesh=None, box=None, handle=None):
         lib = _lib()
-        if box is not None:
+        if handle is not None:
+            h = handle
+        elif box is not None:
             h = lib.vr_box(*box)
         else:
             pts = np.ascontiguousarray(mesh.points)
@@ -307,3 +316,32 @@ class RefMesh:
         w = np.zeros(self.n_cells)
         _check(_lib().vr_gamg_apply(self._h, _p(diag), _p(upper), _p(lower), _p(r), _p(w)))
         return w
+
+
+def read_polymesh(directory):
+    """The reference's readPolyMesh of a directory: (raw arrays dict, RefMesh built from it)."""
+    lib = _lib()
+    h = lib.vr_read_polymesh(str(directory).encode())
+    if not h:
+        raise RuntimeError("reference: " + lib.vr_last_error().decode())
+    try:
+        s = np.zeros(6, np.int64)
+        lib.vr_rawmesh_sizes(h, _p(s))
+        npts, nf, ni, nfp, nc, npatch = (int(v) for v in s)
+        raw = {"points": np.zeros((npts, 3)), "face_off": np.zeros(nf + 1, np.int64),
+               "face_pts": np.zeros(nfp, np.int64), "owner": np.zeros(nf, np.int64),
+               "neighbour": np.zeros(max(ni, 1), np.int64), "patch_start": np.zeros(npatch, np.int64),
+               "patch_size": np.zeros(npatch, np.int64), "n_cells": nc}
+        names = ctypes.create_string_buffer(64 * max(npatch, 1))
+        types = ctypes.create_string_buffer(32 * max(npatch, 1))
+        lib.vr_rawmesh_arrays(h, *(_p(raw[k]) for k in ("points", "face_off", "face_pts", "owner", "neighbour",
+                                                          "patch_start", "patch_size")), names, types)
+        raw["neighbour"] = raw["neighbour"][:ni]
+        raw["patch_names"] = [names.raw[64 * p: 64 * p + 64].split(b"\0")[0].decode() for p in range(npatch)]
+        raw["patch_types"] = [types.raw[32 * p: 32 * p + 32].split(b"\0")[0].decode() for p in range(npatch)]
+        mh = lib.vr_rawmesh_build(h)
+        if not mh:
+            raise RuntimeError("reference: " + lib.vr_last_error().decode())
+        return raw, RefMesh(handle=mh)
+    finally:
+        lib.vr_rawmesh_free(h)

@@ -237,6 +237,61 @@ void vr_geometry(void* h, double* V, double* C, double* Sf, double* magSf, doubl
     copyVec(m.corrVecs(), corr);
 }
 
+// polyMesh directory read by the reference (readPolyMesh, RawMesh.cpp:223-336)
+void* vr_read_polymesh(const char* dir) {
+    try {
+        return new RawMesh(readPolyMesh(dir));
+    } catch (const std::exception& e) {
+        fail(e);
+        return nullptr;
+    }
+}
+// nPoints, nFaces, nInternalFaces, nFacePoints, nCells, nPatches
+void vr_rawmesh_sizes(void* h, long* out) {
+    const RawMesh& r = *static_cast<RawMesh*>(h);
+    out[0] = long(r.points.size());
+    out[1] = long(r.owner.size());
+    out[2] = long(r.neighbour.size());
+    out[3] = long(r.facePoints.size());
+    out[4] = long(r.nCells);
+    out[5] = long(r.patches.size());
+}
+// names / types: 64 / 32 bytes per patch
+void vr_rawmesh_arrays(void* h, double* pts, long* faceOff, long* facePts, long* owner, long* nei, long* pStart,
+                       long* pSize, char* names, char* types) {
+    const RawMesh& r = *static_cast<RawMesh*>(h);
+    for (std::size_t i = 0; i < r.points.size(); ++i) {
+        pts[3 * i] = r.points[i].x;
+        pts[3 * i + 1] = r.points[i].y;
+        pts[3 * i + 2] = r.points[i].z;
+    }
+    std::copy(r.faceOffsets.begin(), r.faceOffsets.end(), faceOff);
+    std::copy(r.facePoints.begin(), r.facePoints.end(), facePts);
+    std::copy(r.owner.begin(), r.owner.end(), owner);
+    std::copy(r.neighbour.begin(), r.neighbour.end(), nei);
+    for (std::size_t p = 0; p < r.patches.size(); ++p) {
+        pStart[p] = long(r.patches[p].start);
+        pSize[p] = long(r.patches[p].size);
+        std::strncpy(names + 64 * p, r.patches[p].name.c_str(), 63);
+        names[64 * p + 63] = 0;
+        std::strncpy(types + 32 * p, toString(r.patches[p].type).c_str(), 31);
+        types[32 * p + 31] = 0;
+    }
+}
+// the reference Mesh built from it (Mesh::build), for vr_geometry / vr_skew
+void* vr_rawmesh_build(void* h) {
+    try {
+        RawMesh raw = *static_cast<RawMesh*>(h);
+        auto* r = new RefMesh;
+        r->m = Mesh::build(&raw);
+        return r;
+    } catch (const std::exception& e) {
+        fail(e);
+        return nullptr;
+    }
+}
+void vr_rawmesh_free(void* h) { delete static_cast<RawMesh*>(h); }
+
 void vr_skew(void* h, double* skew, int* skewed) {
     const Mesh& m = *static_cast<RefMesh*>(h)->m;
     copyVec(m.skewVecs(), skew);

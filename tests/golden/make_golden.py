@@ -28,6 +28,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
+REF_ROOT = os.environ.get("VOLUMA_REF", "/root/reference")  # the reference tree (mesh data files)
 sys.path.insert(0, ROOT)
 
 from oracle import ref  # noqa: E402
@@ -241,6 +242,41 @@ def fvc_case(name, make):
     print(f"fvc_{name}: {n} cells, skewed {flag}", flush=True)
 
 
+POLYMESH = {"cavity3D_4": "meshes/cavity3D_4/polyMesh",           # 3-D tets, ASCII faceList
+            "cavity_binary": "meshes/import/cavity_binary_polyMesh"}  # 2-D hexes, binary faceCompactList
+POLYMESH_FILES = ("points", "faces", "owner", "neighbour", "boundary")
+
+
+def polymesh_case():
+    """Two of the reference's own polyMesh test meshes, copied as data (only the five
+    files the reader needs), with what the reference's readPolyMesh reads from them and,
+    for the 3-D mesh, the geometry of the Mesh it builds."""
+    import shutil
+    out = {}
+    for name, rel in POLYMESH.items():
+        src = os.path.join(REF_ROOT, rel)
+        dst = os.path.join(HERE, "polymesh", name)
+        os.makedirs(dst, exist_ok=True)
+        for f in POLYMESH_FILES:
+            shutil.copyfile(os.path.join(src, f), os.path.join(dst, f))
+        raw, rm = ref.read_polymesh(dst)
+        e = {k: sha(np.ascontiguousarray(raw[k], dtype=np.int64 if k != "points" else np.float64))
+             for k in ("points", "face_off", "face_pts", "owner", "neighbour")}
+        e.update({"n_cells": raw["n_cells"], "patch_names": raw["patch_names"], "patch_types": raw["patch_types"],
+                  "patch_start": [int(v) for v in raw["patch_start"]],
+                  "patch_size": [int(v) for v in raw["patch_size"]]})
+        if name == "cavity3D_4":
+            g = rm.geometry()
+            sk, flag = rm.skew()
+            e["geometry"] = {k: sha(v) for k, v in g.items()}
+            e["geometry"]["skew"] = sha(sk)
+            e["skewed"] = flag
+        out[name] = e
+        print(f"polymesh {name}: {raw['n_cells']} cells", flush=True)
+    with open(os.path.join(HERE, "polymesh_expected.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+
+
 def summary(x):
     x = np.asarray(x)
     idx = np.linspace(0, x.size - 1, 16).astype(np.int64)
@@ -373,6 +409,9 @@ if __name__ == "__main__":
         for nm, mk in SMALL.items():
             post_case(nm, mk)
         sys.exit(0)
+    if "--polymesh" in sys.argv:
+        polymesh_case()
+        sys.exit(0)
     if "--fvc" in sys.argv:
         for nm, mk in FVC_MESHES.items():
             fvc_case(nm, mk)
@@ -381,6 +420,7 @@ if __name__ == "__main__":
         post_case(nm, mk)
     for nm, mk in FVC_MESHES.items():
         fvc_case(nm, mk)
+    polymesh_case()
     dropin_case()
     exactsum_case()
     for nm, mk in SMALL.items():

@@ -54,3 +54,29 @@ def test_device_geometry_errors(ctx):
     b.face_pts = bad
     with pytest.raises(va.VolumaError):
         va.device_geometry(ctx, b)
+
+
+def test_read_polymesh_device_geometry_and_solve(ctx):
+    """The reference's 3-D cavity mesh read from its polyMesh files: device geometry equal
+    to the reference Mesh's, then pressure assembly and GAMG-PCG on it converge."""
+    import os
+    from helpers import GOLDEN, load_json, sha
+    from voluma_amd import mesh as M
+    e = load_json("polymesh_expected.json")["cavity3D_4"]
+    m = M.read_polymesh(os.path.join(GOLDEN, "polymesh", "cavity3D_4"))
+    g = va.device_geometry(ctx, m)
+    for k, h in e["geometry"].items():
+        assert sha(np.ascontiguousarray(g[k])) == h, k
+    ni = m.n_internal
+    lm = va.LduMesh(ctx, m.n_cells, m.owner[:ni], m.neighbour, g["magSf"][:ni] * g["ndc"][:ni])
+    lm.set_boundary(m.owner[ni:])
+    A = va.LduMatrix(lm)
+    bc = np.zeros(m.n_faces - ni, np.int32)
+    s = int(m.patch_start[1]) - ni
+    bc[s: s + int(m.patch_size[1])] = 1  # fixedValue on movingWall
+    b = A.assemble_laplacian(g["magSf"], g["ndc"], bc, b_in=W.pressure_rhs(m.n_cells, 3))
+    x = np.zeros(m.n_cells)
+    p = va.solve_ldu(A, b, x, va.SolverControls("PCG", "GAMG", 1e-8, 0.0, 500))
+    assert p.converged
+    r = b - va.amul(A, x)
+    assert np.abs(r).sum() / p.normFactor < 1e-8

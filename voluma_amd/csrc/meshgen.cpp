@@ -19,6 +19,8 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <fstream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -46,6 +48,7 @@ struct Raw {
     std::vector<V3> points;
     std::vector<int64_t> faceOff{0}, facePts, owner, neighbour;
     std::vector<int64_t> pStart, pSize;
+    std::vector<std::string> pName, pType;  // patch names / types (read meshes)
     int64_t nCells = 0;
     // geometry (filled by computeGeometry)
     bool haveGeom = false;
@@ -397,6 +400,274 @@ void computeGeometry(Raw& m) {
     m.haveGeom = true;
 }
 
+// ------------------------------------------------------------------ OpenFOAM polyMesh reader
+// One polyMesh file (RawMesh.cpp:223-336 reads the same files): the FoamFile header
+// (format ascii|binary, class, arch "LSB;label=NN;scalar=NN"), then lists "N ( ... )"
+// of labels, vectors "(x y z)", faces "k(p0 … pk-1)", or in binary form N raw little-
+// endian elements between the parentheses. C and C++ comments are skipped in ASCII.
+struct FoamFile {
+    std::string path, text, cls;
+    std::size_t pos = 0;
+    bool binary = false;
+    int labelBytes = 4, scalarBytes = 8;
+
+    explicit FoamFile(const std::string& p) : path(p) {
+        std::ifstream in(p, std::ios::binary);
+        if (!in) throw std::runtime_error(p + ": cannot open");
+        std::ostringstream ss;
+        ss << in.rdbuf();
+        text = ss.str();
+        const std::size_t h = text.find("FoamFile");
+        if (h == std::string::npos) throw std::runtime_error(p + ": no FoamFile header");
+        const std::size_t open = text.find('{', h), close = text.find('}', h);
+        if (open == std::string::npos || close == std::string::npos) throw std::runtime_error(p + ": bad header");
+        const std::string hdr = text.substr(open + 1, close - open - 1);
+        binary = value(hdr, "format") == "binary";
+        cls = value(hdr, "class");
+        const std::string arch = value(hdr, "arch");
+        if (arch.find("MSB") != std::string::npos) throw std::runtime_error(p + ": big-endian binary not supported");
+        const std::size_t l = arch.find("label="), sc = arch.find("scalar=");
+        if (l != std::string::npos) labelBytes = std::atoi(arch.c_str() + l + 6) / 8;
+        if (sc != std::string::npos) scalarBytes = std::atoi(arch.c_str() + sc + 7) / 8;
+        if ((labelBytes != 4 && labelBytes != 8) || (scalarBytes != 4 && scalarBytes != 8))
+            throw std::runtime_error(p + ": unsupported label/scalar size");
+        pos = close + 1;
+    }
+    static std::string value(const std::string& hdr, const std::string& key) {
+        std::size_t k = 0;
+        while ((k = hdr.find(key, k)) != std::string::npos) {
+            const bool startOk = k == 0 || std::isspace(static_cast<unsigned char>(hdr[k - 1]));
+            const std::size_t e = k + key.size();
+            if (startOk && e < hdr.size() && std::isspace(static_cast<unsigned char>(hdr[e]))) {
+                const std::size_t semi = hdr.find(';', e);
+                std::string v = hdr.substr(e, semi - e);
+                const auto b = v.find_first_not_of(" \t\r\n\""), en = v.find_last_not_of(" \t\r\n\"");
+                return b == std::string::npos ? std::string() : v.substr(b, en - b + 1);
+            }
+            k = e;
+        }
+        return {};
+    }
+    void skip() {
+        while (pos < text.size()) {
+            const char c = text[pos];
+            if (std::isspace(static_cast<unsigned char>(c))) {
+                ++pos;
+            } else if (text.compare(pos, 2, "//") == 0) {
+                pos = text.find('\n', pos);
+                if (pos == std::string::npos) pos = text.size();
+            } else if (text.compare(pos, 2, "/*") == 0) {
+                pos = text.find("*/", pos);
+                pos = pos == std::string::npos ? text.size() : pos + 2;
+            } else {
+                break;
+            }
+        }
+    }
+    [[noreturn]] void bad(const std::string& what) const {
+        throw std::runtime_error(path + ": " + what + " at byte " + std::to_string(pos));
+    }
+    void expect(char c) {
+        skip();
+        if (pos >= text.size() || text[pos] != c) bad(std::string("expected '") + c + "'");
+        ++pos;
+    }
+    int64_t integer() {
+        skip();
+        char* e = nullptr;
+        const long long v = std::strtoll(text.c_str() + pos, &e, 10);
+        if (e == text.c_str() + pos) bad("expected an integer");
+        pos = std::size_t(e - text.c_str());
+        return v;
+    }
+    double real() {
+        skip();
+        char* e = nullptr;
+        const double v = std::strtod(text.c_str() + pos, &e);
+        if (e == text.c_str() + pos) bad("expected a number");
+        pos = std::size_t(e - text.c_str());
+        return v;
+    }
+    // list header "N (" (binary: the raw elements follow directly)
+    int64_t listStart() {
+        const int64_t n = integer();
+        if (n < 0) bad("negative list size");
+        expect('(');
+        return n;
+    }
+    const char* rawBlock(int64_t n, int elemBytes) {
+        const std::size_t bytes = std::size_t(n) * std::size_t(elemBytes);
+        if (pos + bytes >= text.size() || text[pos + bytes] != ')') bad("binary list size mismatch");
+        const char* d = text.data() + pos;
+        pos += bytes + 1;
+        return d;
+    }
+    std::vector<int64_t> labels() {
+        const int64_t n = listStart();
+        std::vector<int64_t> v(static_cast<std::size_t>(n));
+        if (binary) {
+            const char* d = rawBlock(n, labelBytes);
+            for (int64_t i = 0; i < n; ++i) {
+                if (labelBytes == 4) {
+                    int32_t x;
+                    std::memcpy(&x, d + 4 * i, 4);
+                    v[i] = x;
+                } else {
+                    int64_t x;
+                    std::memcpy(&x, d + 8 * i, 8);
+                    v[i] = x;
+                }
+            }
+            return v;
+        }
+        for (auto& x : v) x = integer();
+        expect(')');
+        return v;
+    }
+    std::vector<V3> vectors() {
+        const int64_t n = listStart();
+        std::vector<V3> v(static_cast<std::size_t>(n));
+        if (binary) {
+            const char* d = rawBlock(n, 3 * scalarBytes);
+            for (int64_t i = 0; i < n; ++i) {
+                double c[3];
+                for (int k = 0; k < 3; ++k) {
+                    if (scalarBytes == 8) {
+                        std::memcpy(&c[k], d + 24 * i + 8 * k, 8);
+                    } else {
+                        float f;
+                        std::memcpy(&f, d + 12 * i + 4 * k, 4);
+                        c[k] = f;
+                    }
+                }
+                v[i] = V3{c[0], c[1], c[2]};
+            }
+            return v;
+        }
+        for (auto& p : v) {
+            expect('(');
+            p.x = real();
+            p.y = real();
+            p.z = real();
+            expect(')');
+        }
+        expect(')');
+        return v;
+    }
+    // next word of the boundary dictionary: a name, a value, or one of { } ;
+    std::string word() {
+        skip();
+        if (pos >= text.size()) bad("unexpected end of file");
+        const char c = text[pos];
+        if (c == '{' || c == '}' || c == ';' || c == '(' || c == ')') {
+            ++pos;
+            return std::string(1, c);
+        }
+        const std::size_t b = pos;
+        while (pos < text.size() && !std::isspace(static_cast<unsigned char>(text[pos])) &&
+               std::string("{};()").find(text[pos]) == std::string::npos)
+            ++pos;
+        std::string w = text.substr(b, pos - b);
+        if (w.size() >= 2 && w.front() == '"' && w.back() == '"') w = w.substr(1, w.size() - 2);
+        return w;
+    }
+};
+
+Raw readPolyMesh(const std::string& dir) {
+    {
+        std::ifstream probe(dir + "/points");
+        std::ifstream gz(dir + "/points.gz");
+        if (!probe && gz) throw std::runtime_error(dir + ": compressed polyMesh (*.gz) is not supported");
+    }
+    Raw m;
+    m.faceOff.clear();
+    {
+        FoamFile f(dir + "/points");
+        m.points = f.vectors();
+    }
+    {
+        FoamFile f(dir + "/faces");
+        if (f.cls == "faceCompactList") {
+            m.faceOff = f.labels();
+            m.facePts = f.labels();
+        } else {
+            if (f.binary) throw std::runtime_error(dir + "/faces: binary faces must be faceCompactList");
+            const int64_t n = f.listStart();
+            m.faceOff.reserve(std::size_t(n) + 1);
+            m.faceOff.push_back(0);
+            for (int64_t i = 0; i < n; ++i) {
+                const int64_t np = f.integer();
+                f.expect('(');
+                for (int64_t k = 0; k < np; ++k) m.facePts.push_back(f.integer());
+                f.expect(')');
+                m.faceOff.push_back(int64_t(m.facePts.size()));
+            }
+            f.expect(')');
+        }
+    }
+    {
+        FoamFile f(dir + "/owner");
+        m.owner = f.labels();
+    }
+    {
+        FoamFile f(dir + "/neighbour");
+        m.neighbour = f.labels();
+    }
+    {
+        FoamFile f(dir + "/boundary");
+        const int64_t n = f.listStart();
+        for (int64_t p = 0; p < n; ++p) {
+            std::string name = f.word(), type;
+            int64_t start = 0, size = 0;
+            if (f.word() != "{") f.bad("expected '{' after patch name");
+            int depth = 1;
+            while (depth > 0) {
+                const std::string key = f.word();
+                if (key == "}") {
+                    --depth;
+                    continue;
+                }
+                if (key == "{") {
+                    ++depth;
+                    continue;
+                }
+                if (key == ";") continue;
+                std::vector<std::string> vals;
+                for (std::string v = f.word(); v != ";"; v = f.word()) {
+                    if (v == "{") {  // a nested dictionary as the value: skipped
+                        int d = 1;
+                        while (d > 0) {
+                            const std::string w = f.word();
+                            if (w == "{") ++d;
+                            if (w == "}") --d;
+                        }
+                        break;
+                    }
+                    vals.push_back(v);
+                }
+                if (vals.empty()) continue;
+                if (key == "type") type = vals[0];
+                else if (key == "nFaces") size = std::strtoll(vals[0].c_str(), nullptr, 10);
+                else if (key == "startFace") start = std::strtoll(vals[0].c_str(), nullptr, 10);
+            }
+            m.pName.push_back(name);
+            m.pType.push_back(type);
+            m.pStart.push_back(start);
+            m.pSize.push_back(size);
+        }
+    }
+    int64_t nc = 0;
+    for (int64_t o : m.owner) nc = std::max(nc, o + 1);
+    for (int64_t o : m.neighbour) nc = std::max(nc, o + 1);
+    m.nCells = nc;
+    if (m.faceOff.size() != m.owner.size() + 1) throw std::runtime_error(dir + ": faces and owner sizes differ");
+    if (m.faceOff.empty() || m.faceOff.front() != 0 || m.faceOff.back() != int64_t(m.facePts.size()))
+        throw std::runtime_error(dir + ": inconsistent face offsets");
+    for (int64_t k : m.facePts)
+        if (k < 0 || k >= int64_t(m.points.size())) throw std::runtime_error(dir + ": face point out of range");
+    return m;
+}
+
 thread_local std::string gErr;
 
 void copyV3(const std::vector<V3>& v, double* out) {
@@ -433,6 +704,30 @@ void* vm_tet(int nx, int ny, int nz, double jitter, uint64_t seed) {
         gErr = e.what();
         return nullptr;
     }
+}
+
+// polyMesh directory (ASCII faceList / faceCompactList, or binary faceCompactList)
+void* vm_read_polymesh(const char* dir) {
+    try {
+        return new Raw(readPolyMesh(dir));
+    } catch (const std::exception& e) {
+        gErr = e.what();
+        return nullptr;
+    }
+}
+
+// Name and type of patch p (empty for generated meshes); returns -1 for a bad index.
+int vm_patch_info(void* h, int p, char* name, int nameLen, char* type, int typeLen) {
+    const Raw& m = *static_cast<Raw*>(h);
+    if (p < 0 || p >= int(m.pStart.size())) return -1;
+    auto put = [](const std::string& s, char* out, int len) {
+        if (!out || len <= 0) return;
+        std::strncpy(out, s.c_str(), std::size_t(len) - 1);
+        out[len - 1] = 0;
+    };
+    put(p < int(m.pName.size()) ? m.pName[p] : std::string(), name, nameLen);
+    put(p < int(m.pType.size()) ? m.pType[p] : std::string(), type, typeLen);
+    return 0;
 }
 
 void vm_free(void* h) { delete static_cast<Raw*>(h); }

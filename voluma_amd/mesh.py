@@ -44,6 +44,10 @@ def _lib():
         lib.vm_geometry.restype = ctypes.c_int
         lib.vm_skew.argtypes = [P, P, P]
         lib.vm_skew.restype = ctypes.c_int
+        lib.vm_read_polymesh.argtypes = [ctypes.c_char_p]
+        lib.vm_read_polymesh.restype = P
+        lib.vm_patch_info.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P, ctypes.c_int]
+        lib.vm_patch_info.restype = ctypes.c_int
         lib.vm_last_error.restype = ctypes.c_char_p
         _LIB = lib
     return _LIB
@@ -81,6 +85,12 @@ class Mesh:
                    _ptr(self.patch_start), _ptr(self.patch_size))
         self.owner = owner.astype(np.int32)
         self.neighbour = nei[:n_int].astype(np.int32)
+        self.patch_names, self.patch_types = [], []
+        for p in range(n_patch):
+            nm, tp = ctypes.create_string_buffer(256), ctypes.create_string_buffer(64)
+            lib.vm_patch_info(self._h, p, nm, 256, tp, 64)
+            self.patch_names.append(nm.value.decode())
+            self.patch_types.append(tp.value.decode())
         self._geom = None
 
     def __del__(self):
@@ -122,6 +132,13 @@ class Mesh:
     def patch_faces(self, p):
         s = int(self.patch_start[p])
         return np.arange(s, s + int(self.patch_size[p]))
+
+
+def read_polymesh(directory) -> Mesh:
+    """An OpenFOAM polyMesh directory (points, faces, owner, neighbour, boundary; ASCII,
+    or binary faceCompactList), as the reference's readPolyMesh reads it
+    (RawMesh.cpp:223-336)."""
+    return Mesh(_lib().vm_read_polymesh(str(directory).encode()))
 
 
 def box(nx, ny=None, nz=None) -> Mesh:
