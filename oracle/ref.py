@@ -51,6 +51,8 @@ def _lib():
         lib.vr_skew.argtypes = [P, P, P]
         lib.vr_read_polymesh.argtypes = [ctypes.c_char_p]
         lib.vr_read_polymesh.restype = P
+        lib.vr_read_gmsh.argtypes = [ctypes.c_char_p, D]
+        lib.vr_read_gmsh.restype = P
         lib.vr_rawmesh_sizes.argtypes = [P, P]
         lib.vr_rawmesh_arrays.argtypes = [P] * 10
         lib.vr_rawmesh_build.argtypes = [P]
@@ -320,8 +322,16 @@ class RefMesh:
 
 def read_polymesh(directory):
     """The reference's readPolyMesh of a directory: (raw arrays dict, RefMesh built from it)."""
+    return _raw_mesh(_lib().vr_read_polymesh(str(directory).encode()))
+
+
+def read_gmsh(path, scale=1.0):
+    """The reference's readGmsh of a file: (raw arrays dict, RefMesh built from it)."""
+    return _raw_mesh(_lib().vr_read_gmsh(str(path).encode(), ctypes.c_double(scale)))
+
+
+def _raw_mesh(h):
     lib = _lib()
-    h = lib.vr_read_polymesh(str(directory).encode())
     if not h:
         raise RuntimeError("reference: " + lib.vr_last_error().decode())
     try:

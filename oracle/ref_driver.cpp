@@ -13,6 +13,7 @@
 #include "fvm/linalg/GAMG.h"
 #include "fvm/linalg/LinearSolver.h"
 #include "fvm/mesh/Mesh.h"
+#include "fvm/mesh/MeshReaders.h"
 #include "fvm/mesh/RawMesh.h"
 #include "fvm/parallel/Comm.h"
 
@@ -241,6 +242,17 @@ void vr_geometry(void* h, double* V, double* C, double* Sf, double* magSf, doubl
 void* vr_read_polymesh(const char* dir) {
     try {
         return new RawMesh(readPolyMesh(dir));
+    } catch (const std::exception& e) {
+        fail(e);
+        return nullptr;
+    }
+}
+// Gmsh file read by the reference (readGmsh, MeshReaders.cpp:236-482)
+void* vr_read_gmsh(const char* file, double scale) {
+    try {
+        MeshImportOptions o;
+        o.scale = scale;
+        return new RawMesh(readGmsh(file, o));
     } catch (const std::exception& e) {
         fail(e);
         return nullptr;

@@ -277,6 +277,74 @@ def polymesh_case():
         json.dump(out, fh, indent=1)
 
 
+GMSH_FILES = {"hybrid3D_v41": "meshes/import/hybrid3D_v41.msh",  # tets, hexes, pyramids (Gmsh 4.1)
+              "tet10_v41": "meshes/import/tet10_v41.msh"}        # second-order tets (Gmsh 4.1)
+
+
+def write_gmsh22(m, path):
+    """A Gmsh 2.2 ASCII file of a tet mesh: nodes 1..n, the boundary triangles of each
+    patch (physical surface p+1) and the tets (physical volume 100), each tet's nodes
+    in first-seen order over its faces."""
+    ni = m.n_internal
+    cell_nodes = [[] for _ in range(m.n_cells)]
+    for f in range(m.n_faces):
+        pts = m.face_pts[m.face_off[f]:m.face_off[f + 1]]
+        for c in ((m.owner[f], m.neighbour[f]) if f < ni else (m.owner[f],)):
+            for q in pts:
+                if q not in cell_nodes[c]:
+                    cell_nodes[c].append(int(q))
+    lines = ["$MeshFormat", "2.2 0 8", "$EndMeshFormat", "$PhysicalNames", str(len(m.patch_start) + 1)]
+    for p in range(len(m.patch_start)):
+        lines.append(f'2 {p + 1} "side{p}{"Wall" if p % 2 else ""}"')
+    lines += ['3 100 "fluid"', "$EndPhysicalNames", "$Nodes", str(len(m.points))]
+    lines += [f"{i + 1} {x!r} {y!r} {z!r}" for i, (x, y, z) in enumerate(m.points.tolist())]
+    lines += ["$EndNodes", "$Elements"]
+    els = []
+    for p in range(len(m.patch_start)):
+        for f in range(int(m.patch_start[p]), int(m.patch_start[p] + m.patch_size[p])):
+            pts = m.face_pts[m.face_off[f]:m.face_off[f + 1]]
+            els.append(f"2 2 {p + 1} {p + 1} " + " ".join(str(int(q) + 1) for q in pts))
+    for c in range(m.n_cells):
+        els.append("4 2 100 1 " + " ".join(str(q + 1) for q in cell_nodes[c]))
+    lines.append(str(len(els)))
+    lines += [f"{k + 1} {e}" for k, e in enumerate(els)]
+    lines += ["$EndElements", ""]
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines))
+
+
+def gmsh_case():
+    """Two of the reference's own Gmsh test files (copied as data) and a Gmsh 2.2 file of
+    the jittered tet5x4x3 mesh, with what the reference's readGmsh reads from each and
+    the geometry of the Mesh it builds."""
+    import shutil
+    dst_dir = os.path.join(HERE, "gmsh")
+    os.makedirs(dst_dir, exist_ok=True)
+    files = {}
+    for name, rel in GMSH_FILES.items():
+        files[name] = os.path.join(dst_dir, name + ".msh")
+        shutil.copyfile(os.path.join(REF_ROOT, rel), files[name])
+    files["tet5x4x3_v22"] = os.path.join(dst_dir, "tet5x4x3_v22.msh")
+    write_gmsh22(SMALL["tet5x4x3"](), files["tet5x4x3_v22"])
+    out = {}
+    for name, path in files.items():
+        raw, rm = ref.read_gmsh(path)
+        e = {k: sha(np.ascontiguousarray(raw[k], dtype=np.int64 if k != "points" else np.float64))
+             for k in ("points", "face_off", "face_pts", "owner", "neighbour")}
+        e.update({"n_cells": raw["n_cells"], "patch_names": raw["patch_names"], "patch_types": raw["patch_types"],
+                  "patch_start": [int(v) for v in raw["patch_start"]],
+                  "patch_size": [int(v) for v in raw["patch_size"]]})
+        g = rm.geometry()
+        sk, flag = rm.skew()
+        e["geometry"] = {k: sha(v) for k, v in g.items()}
+        e["geometry"]["skew"] = sha(sk)
+        e["skewed"] = flag
+        out[name] = e
+        print(f"gmsh {name}: {raw['n_cells']} cells, patches {raw['patch_names']}", flush=True)
+    with open(os.path.join(HERE, "gmsh_expected.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+
+
 def summary(x):
     x = np.asarray(x)
     idx = np.linspace(0, x.size - 1, 16).astype(np.int64)
@@ -411,6 +479,7 @@ if __name__ == "__main__":
         sys.exit(0)
     if "--polymesh" in sys.argv:
         polymesh_case()
+        gmsh_case()
         sys.exit(0)
     if "--fvc" in sys.argv:
         for nm, mk in FVC_MESHES.items():
@@ -421,6 +490,7 @@ if __name__ == "__main__":
     for nm, mk in FVC_MESHES.items():
         fvc_case(nm, mk)
     polymesh_case()
+    gmsh_case()
     dropin_case()
     exactsum_case()
     for nm, mk in SMALL.items():

@@ -1,8 +1,11 @@
 """Mesh ingest: the polyMesh reader against the reference's readPolyMesh
 (RawMesh.cpp:223-336) on two of the reference's own test meshes (3-D tets in ASCII
-faceList form; 2-D hexes in binary faceCompactList form): points, faces, owner,
-neighbour, patches and cell count identical; the 3-D mesh's host geometry equal to the
-geometry of the Mesh the reference builds from it. Malformed input raises."""
+faceList form; 2-D hexes in binary faceCompactList form), and the Gmsh reader against
+the reference's readGmsh (MeshReaders.cpp:236-482) on its own hybrid (tet/hex/pyramid)
+and second-order tet test files (Gmsh 4.1) and a Gmsh 2.2 file of a jittered tet mesh:
+points, faces and their order, owner, neighbour, patches and cell count identical; the
+3-D meshes' host geometry equal to the geometry of the Mesh the reference builds from
+them. Malformed input raises."""
 import os
 
 import numpy as np
@@ -52,3 +55,36 @@ def test_polymesh_errors(tmp_path):
         (d / f).write_bytes(data)
     with pytest.raises(RuntimeError):
         M.read_polymesh(d)
+
+
+GMSH = ("hybrid3D_v41", "tet10_v41", "tet5x4x3_v22")
+
+
+@pytest.mark.parametrize("name", GMSH)
+def test_gmsh_reader_matches_reference(name):
+    e = load_json("gmsh_expected.json")[name]
+    m = M.read_mesh(os.path.join(GOLDEN, "gmsh", name + ".msh"))
+    assert sha(np.ascontiguousarray(m.points)) == e["points"]
+    for k in ("face_off", "face_pts", "owner", "neighbour"):
+        assert sha(np.ascontiguousarray(getattr(m, k), dtype=np.int64)) == e[k], k
+    assert m.n_cells == e["n_cells"]
+    assert m.patch_names == e["patch_names"] and m.patch_types == e["patch_types"]
+    assert [int(v) for v in m.patch_start] == e["patch_start"] and [int(v) for v in m.patch_size] == e["patch_size"]
+    g = m.geometry()
+    for k, h in e["geometry"].items():
+        assert sha(np.ascontiguousarray(g[k])) == h, k
+    assert g["skewed"] == e["skewed"]
+
+
+def test_gmsh_errors(tmp_path):
+    src = open(os.path.join(GOLDEN, "gmsh", "tet5x4x3_v22.msh")).read()
+    bad = tmp_path / "bad.msh"
+    bad.write_text(src.replace("$EndNodes", "$EndNodez"))  # unterminated section
+    with pytest.raises(RuntimeError):
+        M.read_gmsh(bad)
+    b2 = tmp_path / "b2.msh"
+    b2.write_text(src.replace("2.2 0 8", "2.2 1 8"))  # binary flag
+    with pytest.raises(RuntimeError):
+        M.read_gmsh(b2)
+    with pytest.raises(RuntimeError):
+        M.read_mesh(tmp_path / "missing.msh")

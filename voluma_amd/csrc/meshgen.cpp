@@ -19,8 +19,12 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <array>
+#include <cctype>
 #include <fstream>
+#include <map>
 #include <sstream>
+#include <unordered_map>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -668,6 +672,384 @@ Raw readPolyMesh(const std::string& dir) {
     return m;
 }
 
+// ------------------------------------------------------------------ Gmsh reader (3-D)
+// ASCII Gmsh 2.2 and 4.x files as the reference's readGmsh reads them
+// (MeshReaders.cpp:192-482): nodes, elements of the highest dimension become cells,
+// their faces are matched by sorted corner nodes (first element = c0), boundary faces
+// take the physical group of the 2-D element with the same nodes (else patch
+// "defaultFaces"), unused points are dropped, and the face list is assembled by
+// assembleFaces below. 2-D meshes (extruded by the reference) are not supported here.
+namespace gmsh {
+
+// Element type → (dimension, corner nodes); higher-order types keep their corners
+// (the element type numbers are Gmsh's).
+bool elementType(int t, int& dim, int& corners) {
+    static const std::map<int, std::pair<int, int>> table = {
+        {15, {0, 1}}, {1, {1, 2}},   {8, {1, 2}},   {26, {1, 2}},  {27, {1, 2}},  {28, {1, 2}},  {2, {2, 3}},
+        {9, {2, 3}},  {20, {2, 3}},  {21, {2, 3}},  {22, {2, 3}},  {23, {2, 3}},  {24, {2, 3}},  {25, {2, 3}},
+        {3, {2, 4}},  {10, {2, 4}},  {16, {2, 4}},  {36, {2, 4}},  {37, {2, 4}},  {38, {2, 4}},  {4, {3, 4}},
+        {11, {3, 4}}, {29, {3, 4}},  {30, {3, 4}},  {31, {3, 4}},  {5, {3, 8}},   {12, {3, 8}},  {17, {3, 8}},
+        {92, {3, 8}}, {93, {3, 8}},  {6, {3, 6}},   {13, {3, 6}},  {18, {3, 6}},  {90, {3, 6}},  {91, {3, 6}},
+        {7, {3, 5}},  {14, {3, 5}},  {19, {3, 5}},  {118, {3, 5}}, {119, {3, 5}}};
+    const auto it = table.find(t);
+    if (it == table.end()) return false;
+    dim = it->second.first;
+    corners = it->second.second;
+    return true;
+}
+
+// Faces of the 3-D element kinds by corner index in Gmsh's node order (their orientation
+// is corrected from the geometry afterwards).
+const std::vector<std::vector<int>>& elementFaces(int corners) {
+    static const std::vector<std::vector<int>> tet{{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {1, 2, 3}};
+    static const std::vector<std::vector<int>> pyramid{{0, 3, 2, 1}, {0, 1, 4}, {1, 2, 4}, {2, 3, 4}, {3, 0, 4}};
+    static const std::vector<std::vector<int>> prism{
+        {0, 2, 1}, {3, 4, 5}, {0, 1, 4, 3}, {1, 2, 5, 4}, {2, 0, 3, 5}};
+    static const std::vector<std::vector<int>> hex{{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                                                   {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
+    switch (corners) {
+    case 4: return tet;
+    case 5: return pyramid;
+    case 6: return prism;
+    default: return hex;
+    }
+}
+
+struct Key {
+    std::array<int64_t, 4> v{-1, -1, -1, -1};
+    bool operator==(const Key& o) const { return v == o.v; }
+};
+struct KeyHash {
+    std::size_t operator()(const Key& k) const {
+        std::size_t h = 0x9e3779b97f4a7c15ull;
+        for (int64_t x : k.v) h = (h ^ std::size_t(x + 1)) * 0x100000001b3ull;
+        return h;
+    }
+};
+Key keyOf(const std::vector<int64_t>& n) {
+    if (n.size() > 4) throw std::runtime_error("mesh import: element face with more than 4 nodes");
+    Key k;
+    std::copy(n.begin(), n.end(), k.v.begin());
+    std::sort(k.v.begin(), k.v.begin() + n.size());
+    return k;
+}
+
+struct Element {
+    int dim, phys;
+    std::vector<int64_t> nodes;
+};
+struct Face {
+    std::vector<int64_t> nodes;
+    int64_t c0, c1;
+    int patch;
+};
+struct Patch {
+    std::string name, type;
+};
+
+// Point average and half the summed fan cross products about it (the polygon's
+// centre estimate and area vector used to orient faces).
+V3 polygonNormal(const std::vector<V3>& P, const std::vector<int64_t>& f, V3& centre) {
+    V3 c;
+    for (int64_t i : f) c += P[i];
+    c = c / double(f.size());
+    V3 n;
+    for (std::size_t k = 0; k < f.size(); ++k) n += cross(P[f[k]] - c, P[f[(k + 1) % f.size()]] - c);
+    centre = c;
+    return 0.5 * n;
+}
+
+// Face list → Raw (MeshReaders.cpp:131-190): cell centres estimated as the mean of their
+// face centres; internal faces get c0 < c1 and a normal from c0 to c1, boundary faces an
+// outward normal (nodes reversed otherwise); internal faces sorted by (c0, c1), then
+// boundary faces patch by patch in input order; patches without faces are dropped.
+Raw assembleFaces(std::vector<V3> points, int64_t nCells, std::vector<Face> faces, const std::vector<Patch>& patches) {
+    std::vector<V3> cc(std::size_t(nCells), V3{});
+    std::vector<int> nf(std::size_t(nCells), 0);
+    std::vector<V3> fc(faces.size()), fn(faces.size());
+    for (std::size_t f = 0; f < faces.size(); ++f) {
+        Face& F = faces[f];
+        if (F.c0 < 0 && F.c1 >= 0) std::swap(F.c0, F.c1);
+        if (F.c0 < 0 || F.c0 >= nCells || F.c1 >= nCells) throw std::runtime_error("mesh import: face with invalid cells");
+        fn[f] = polygonNormal(points, F.nodes, fc[f]);
+        cc[F.c0] += fc[f];
+        ++nf[F.c0];
+        if (F.c1 >= 0) {
+            cc[F.c1] += fc[f];
+            ++nf[F.c1];
+        }
+    }
+    for (int64_t c = 0; c < nCells; ++c) {
+        if (nf[c] < 4) throw std::runtime_error("mesh import: cell " + std::to_string(c) + " has fewer than 4 faces");
+        cc[c] = cc[c] / double(nf[c]);
+    }
+    std::vector<std::size_t> internal;
+    std::vector<std::vector<std::size_t>> bnd(patches.size());
+    for (std::size_t f = 0; f < faces.size(); ++f) {
+        Face& F = faces[f];
+        if (F.c1 >= 0) {
+            if (F.c1 < F.c0) std::swap(F.c0, F.c1);
+            if (dot(fn[f], cc[F.c1] - cc[F.c0]) < 0) std::reverse(F.nodes.begin(), F.nodes.end());
+            internal.push_back(f);
+        } else {
+            if (F.patch < 0 || F.patch >= int(patches.size()))
+                throw std::runtime_error("mesh import: boundary face without patch");
+            if (dot(fn[f], fc[f] - cc[F.c0]) < 0) std::reverse(F.nodes.begin(), F.nodes.end());
+            bnd[F.patch].push_back(f);
+        }
+    }
+    std::sort(internal.begin(), internal.end(), [&](std::size_t a, std::size_t b) {
+        return faces[a].c0 != faces[b].c0 ? faces[a].c0 < faces[b].c0 : faces[a].c1 < faces[b].c1;
+    });
+    Raw r;
+    r.points = std::move(points);
+    r.nCells = nCells;
+    auto add = [&](const Face& F) {
+        r.facePts.insert(r.facePts.end(), F.nodes.begin(), F.nodes.end());
+        r.faceOff.push_back(int64_t(r.facePts.size()));
+        r.owner.push_back(F.c0);
+    };
+    for (std::size_t f : internal) {
+        add(faces[f]);
+        r.neighbour.push_back(faces[f].c1);
+    }
+    for (std::size_t p = 0; p < patches.size(); ++p) {
+        if (bnd[p].empty()) continue;
+        const int64_t start = r.nFaces();
+        for (std::size_t f : bnd[p]) add(faces[f]);
+        r.pName.push_back(patches[p].name);
+        r.pType.push_back(patches[p].type);
+        r.pStart.push_back(start);
+        r.pSize.push_back(r.nFaces() - start);
+    }
+    return r;
+}
+
+std::string lower(std::string s) {
+    for (char& c : s) c = char(std::tolower(static_cast<unsigned char>(c)));
+    return s;
+}
+
+Raw read(const std::string& file, double scale) {
+    std::ifstream in(file);
+    if (!in) throw std::runtime_error("cannot open " + file);
+    double version = 0;
+    std::map<std::pair<int, int>, std::string> physNames;
+    std::map<std::pair<int, int>, std::vector<int>> entityPhys;
+    std::unordered_map<int64_t, int64_t> nodeIndex;
+    std::vector<V3> points;
+    std::vector<Element> elems;
+    std::string line;
+    auto endSection = [&](const std::string& name) {
+        while (std::getline(in, line))
+            if (line.rfind("$End" + name, 0) == 0) return;
+        throw std::runtime_error(file + ": missing $End" + name);
+    };
+    while (std::getline(in, line)) {
+        if (!line.empty() && line.back() == '\r') line.pop_back();
+        if (line == "$MeshFormat") {
+            int fileType = 0, dataSize = 0;
+            in >> version >> fileType >> dataSize;
+            if (fileType != 0) throw std::runtime_error(file + ": binary Gmsh files are not supported");
+            if (version < 2 || version >= 5) throw std::runtime_error(file + ": unsupported Gmsh version");
+            endSection("MeshFormat");
+        } else if (line == "$PhysicalNames") {
+            int n;
+            in >> n;
+            for (int i = 0; i < n; ++i) {
+                int d, t;
+                in >> d >> t;
+                std::getline(in, line);
+                const auto a = line.find('"'), b = line.rfind('"');
+                physNames[{d, t}] = (a != std::string::npos && b > a) ? line.substr(a + 1, b - a - 1) : std::to_string(t);
+            }
+            endSection("PhysicalNames");
+        } else if (line == "$Entities" && version >= 4) {
+            std::size_t n[4];
+            in >> n[0] >> n[1] >> n[2] >> n[3];
+            for (int d = 0; d < 4; ++d)
+                for (std::size_t i = 0; i < n[d]; ++i) {
+                    int tag;
+                    double x;
+                    in >> tag;
+                    for (int k = 0; k < (d == 0 ? 3 : 6); ++k) in >> x;
+                    std::size_t np;
+                    in >> np;
+                    std::vector<int> ph(np);
+                    for (int& q : ph) in >> q;
+                    for (int& q : ph) q = std::abs(q);
+                    entityPhys[{d, tag}] = ph;
+                    if (d > 0) {
+                        std::size_t nb;
+                        in >> nb;
+                        for (std::size_t k = 0; k < nb; ++k) in >> tag;
+                    }
+                }
+            endSection("Entities");
+        } else if (line == "$Nodes") {
+            if (version >= 4) {
+                std::size_t nBlocks, nNodes, minTag, maxTag;
+                in >> nBlocks >> nNodes >> minTag >> maxTag;
+                points.reserve(nNodes);
+                for (std::size_t b = 0; b < nBlocks; ++b) {
+                    int d, tag, param;
+                    std::size_t n;
+                    in >> d >> tag >> param >> n;
+                    std::vector<int64_t> tags(n);
+                    for (int64_t& t : tags) in >> t;
+                    for (std::size_t i = 0; i < n; ++i) {
+                        V3 p;
+                        in >> p.x >> p.y >> p.z;
+                        double u;
+                        for (int k = 0; k < (param ? d : 0); ++k) in >> u;
+                        nodeIndex[tags[i]] = int64_t(points.size());
+                        points.push_back(p * scale);
+                    }
+                }
+            } else {
+                std::size_t n;
+                in >> n;
+                points.reserve(n);
+                for (std::size_t i = 0; i < n; ++i) {
+                    int64_t t;
+                    V3 p;
+                    in >> t >> p.x >> p.y >> p.z;
+                    nodeIndex[t] = int64_t(points.size());
+                    points.push_back(p * scale);
+                }
+            }
+            endSection("Nodes");
+        } else if (line == "$Elements") {
+            if (version >= 4) {
+                std::size_t nBlocks, nEl, minTag, maxTag;
+                in >> nBlocks >> nEl >> minTag >> maxTag;
+                elems.reserve(nEl);
+                std::getline(in, line);
+                for (std::size_t b = 0; b < nBlocks; ++b) {
+                    int d, tag, type;
+                    std::size_t n;
+                    in >> d >> tag >> type >> n;
+                    std::getline(in, line);
+                    int dim, corners;
+                    if (!elementType(type, dim, corners))
+                        throw std::runtime_error(file + ": unsupported Gmsh element type " + std::to_string(type));
+                    const auto it = entityPhys.find({d, tag});
+                    const int phys = (it != entityPhys.end() && !it->second.empty()) ? it->second[0] : -1;
+                    for (std::size_t i = 0; i < n; ++i) {
+                        std::getline(in, line);
+                        std::istringstream ls(line);
+                        int64_t id, v;
+                        ls >> id;
+                        Element e{dim, phys, {}};
+                        for (int k = 0; k < corners && ls >> v; ++k) e.nodes.push_back(v);
+                        elems.push_back(std::move(e));
+                    }
+                }
+            } else {
+                std::size_t n;
+                in >> n;
+                std::getline(in, line);
+                elems.reserve(n);
+                for (std::size_t i = 0; i < n; ++i) {
+                    std::getline(in, line);
+                    std::istringstream ls(line);
+                    int64_t id;
+                    int type, nt;
+                    ls >> id >> type >> nt;
+                    std::vector<int> tags(std::max(nt, 0));
+                    for (int& t : tags) ls >> t;
+                    int dim, corners;
+                    if (!elementType(type, dim, corners))
+                        throw std::runtime_error(file + ": unsupported Gmsh element type " + std::to_string(type));
+                    Element e{dim, nt > 0 ? tags[0] : -1, {}};
+                    if (e.phys == 0) e.phys = -1;
+                    int64_t v;
+                    for (int k = 0; k < corners && ls >> v; ++k) e.nodes.push_back(v);
+                    elems.push_back(std::move(e));
+                }
+            }
+            endSection("Elements");
+        }
+    }
+    if (points.empty() || elems.empty()) throw std::runtime_error(file + ": no nodes or elements");
+    auto node = [&](int64_t t) {
+        const auto it = nodeIndex.find(t);
+        if (it == nodeIndex.end()) throw std::runtime_error(file + ": element refers to unknown node " + std::to_string(t));
+        return it->second;
+    };
+    int meshDim = 0;
+    for (const Element& e : elems) meshDim = std::max(meshDim, e.dim);
+    if (meshDim != 3) throw std::runtime_error(file + ": only 3-D Gmsh meshes are supported");
+    std::map<int, int> physToPatch;
+    std::vector<Patch> patches;
+    for (const Element& e : elems)
+        if (e.dim == 2 && e.phys >= 0 && !physToPatch.count(e.phys)) {
+            Patch p;
+            const auto it = physNames.find({2, e.phys});
+            p.name = it != physNames.end() ? it->second : "patch" + std::to_string(e.phys);
+            const std::string l = lower(p.name);
+            p.type = l.find("symmetry") != std::string::npos ? "symmetryPlane"
+                     : l.find("wall") != std::string::npos   ? "wall"
+                                                             : "patch";
+            physToPatch[e.phys] = int(patches.size());
+            patches.push_back(p);
+        }
+    for (std::size_t a = 0; a < patches.size(); ++a)
+        for (std::size_t b = a + 1; b < patches.size(); ++b)
+            if (patches[a].name == patches[b].name)
+                throw std::runtime_error(file + ": duplicate physical name " + patches[a].name);
+    std::unordered_map<Key, int, KeyHash> bFacePatch;
+    for (const Element& e : elems)
+        if (e.dim == 2 && e.phys >= 0) {
+            std::vector<int64_t> f;
+            for (int64_t t : e.nodes) f.push_back(node(t));
+            bFacePatch[keyOf(f)] = physToPatch[e.phys];
+        }
+    const int defaultIdx = int(patches.size());
+    patches.push_back({"defaultFaces", "wall"});
+    std::vector<Face> faces;
+    std::unordered_map<Key, std::size_t, KeyHash> faceOf;
+    int64_t nCells = 0;
+    for (const Element& e : elems) {
+        if (e.dim != 3) continue;
+        const int64_t c = nCells++;
+        std::vector<int64_t> cn;
+        for (int64_t t : e.nodes) cn.push_back(node(t));
+        for (const auto& fd : elementFaces(int(cn.size()))) {
+            std::vector<int64_t> f;
+            for (int k : fd) f.push_back(cn[k]);
+            const Key k = keyOf(f);
+            const auto it = faceOf.find(k);
+            if (it == faceOf.end()) {
+                faceOf.emplace(k, faces.size());
+                faces.push_back({f, c, -1, -1});
+            } else {
+                Face& F = faces[it->second];
+                if (F.c1 >= 0) throw std::runtime_error(file + ": face shared by more than two cells");
+                F.c1 = c;
+            }
+        }
+    }
+    for (Face& F : faces)
+        if (F.c1 < 0) {
+            const auto it = bFacePatch.find(keyOf(F.nodes));
+            F.patch = it != bFacePatch.end() ? it->second : defaultIdx;
+        }
+    std::vector<int64_t> used(points.size(), -1);
+    std::vector<V3> pts;
+    for (Face& F : faces)
+        for (int64_t& v : F.nodes) {
+            if (used[v] < 0) {
+                used[v] = int64_t(pts.size());
+                pts.push_back(points[v]);
+            }
+            v = used[v];
+        }
+    return assembleFaces(std::move(pts), nCells, std::move(faces), patches);
+}
+
+}  // namespace gmsh
+
 thread_local std::string gErr;
 
 void copyV3(const std::vector<V3>& v, double* out) {
@@ -710,6 +1092,16 @@ void* vm_tet(int nx, int ny, int nz, double jitter, uint64_t seed) {
 void* vm_read_polymesh(const char* dir) {
     try {
         return new Raw(readPolyMesh(dir));
+    } catch (const std::exception& e) {
+        gErr = e.what();
+        return nullptr;
+    }
+}
+
+// ASCII Gmsh 2.2 / 4.x file of a 3-D mesh; coordinates multiplied by scale
+void* vm_read_gmsh(const char* file, double scale) {
+    try {
+        return new Raw(gmsh::read(file, scale));
     } catch (const std::exception& e) {
         gErr = e.what();
         return nullptr;

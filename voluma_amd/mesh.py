@@ -46,6 +46,8 @@ def _lib():
         lib.vm_skew.restype = ctypes.c_int
         lib.vm_read_polymesh.argtypes = [ctypes.c_char_p]
         lib.vm_read_polymesh.restype = P
+        lib.vm_read_gmsh.argtypes = [ctypes.c_char_p, ctypes.c_double]
+        lib.vm_read_gmsh.restype = P
         lib.vm_patch_info.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P, ctypes.c_int]
         lib.vm_patch_info.restype = ctypes.c_int
         lib.vm_last_error.restype = ctypes.c_char_p
@@ -139,6 +141,28 @@ def read_polymesh(directory) -> Mesh:
     or binary faceCompactList), as the reference's readPolyMesh reads it
     (RawMesh.cpp:223-336)."""
     return Mesh(_lib().vm_read_polymesh(str(directory).encode()))
+
+
+def read_gmsh(path, scale=1.0) -> Mesh:
+    """An ASCII Gmsh 2.2 / 4.x file of a 3-D mesh (tets, hexes, prisms, pyramids, also
+    higher-order), as the reference's readGmsh reads it (MeshReaders.cpp:236-482):
+    same points, faces, face order and patches."""
+    return Mesh(_lib().vm_read_gmsh(str(path).encode(), float(scale)))
+
+
+def read_mesh(path, scale=1.0) -> Mesh:
+    """A polyMesh directory or a Gmsh file, chosen as readMeshFile chooses
+    (MeshReaders.cpp:805-816); Fluent files are not read here."""
+    import os
+    if os.path.isdir(path):
+        return read_polymesh(path)
+    if not os.path.isfile(path):
+        raise RuntimeError(f"cannot open {path}")
+    with open(path, "rb") as fh:
+        first = fh.readline()
+    if first.startswith(b"$MeshFormat"):
+        return read_gmsh(path, scale)
+    raise RuntimeError(f"{path}: not a polyMesh directory or a Gmsh file")
 
 
 def box(nx, ny=None, nz=None) -> Mesh:
