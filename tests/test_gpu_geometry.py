@@ -80,3 +80,17 @@ def test_read_polymesh_device_geometry_and_solve(ctx):
     assert p.converged
     r = b - va.amul(A, x)
     assert np.abs(r).sum() / p.normFactor < 1e-8
+
+
+@pytest.mark.parametrize("name", ["hybrid3D_v41", "tet10_v41", "tet5x4x3_v22"])
+def test_read_gmsh_device_geometry(ctx, name):
+    """Gmsh files read by the library: device geometry equal to the reference Mesh's."""
+    import os
+    from helpers import GOLDEN, load_json, sha
+    from voluma_amd import mesh as M
+    e = load_json("gmsh_expected.json")[name]
+    m = M.read_gmsh(os.path.join(GOLDEN, "gmsh", name + ".msh"))
+    g = va.device_geometry(ctx, m)
+    for k, h in e["geometry"].items():
+        assert sha(np.ascontiguousarray(g[k])) == h, k
+    assert g["skewed"] == e["skewed"]
