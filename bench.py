@@ -200,12 +200,13 @@ def reference_solvers_leg(va, W, ctx, m, A, db, dx, dzero, args, cpu):
     (PBiCGStab + DILU) on the momentum matrix of velocity component 0, both as
     level-scheduled sweeps; with cpu, the momentum solve's CPU baseline too."""
     out = {}
-    ctl = va.SolverControls("PCG", "GAMG", args.tol, 0.0, 1000, gamgSmoother=va.GAMG_GAUSS_SEIDEL)
-    for k in range(2):
+    short = args.profile_legs  # a kernel-trace run: one short solve of each kind
+    ctl = va.SolverControls("PCG", "GAMG", args.tol, 0.0, 3 if short else 1000, gamgSmoother=va.GAMG_GAUSS_SEIDEL)
+    for k in range(0 if short else 2):
         dx.copy_from(dzero)
         va.solve_ldu(A, db, dx, ctl)
     ctx.synchronize()
-    steps = max(2, args.steps // 4)
+    steps = 1 if short else max(2, args.steps // 4)
     t = time.perf_counter()
     its = []
     for k in range(steps):
@@ -220,7 +221,7 @@ def reference_solvers_leg(va, W, ctx, m, A, db, dx, dzero, args, cpu):
     x0 = np.ascontiguousarray(inp["U0"][:, 0])
     dx0, db2, dxm = va.DeviceArray.from_host(ctx, x0), va.DeviceArray.from_host(ctx, b2), va.DeviceArray(ctx, m.n_cells)
     mctl = va.SolverControls("PBiCGStab", "DILU", 1e-10, 0.0, 1000)
-    for k in range(2):
+    for k in range(0 if short else 2):
         dxm.copy_from(dx0)
         va.solve_ldu(A2, db2, dxm, mctl)
     ctx.synchronize()
@@ -259,6 +260,8 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-ref-solvers", action="store_true", help="skip the GS-GAMG and DILU legs")
     ap.add_argument("--no-corrector-ops", action="store_true", help="skip the fvc operator timings")
+    ap.add_argument("--profile-legs", action="store_true",
+                    help="kernel-trace aid: one short GS-GAMG (3 iterations) and DILU solve, no warm-up")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="request arrivals per second")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--pre", type=int, default=1)
@@ -379,7 +382,7 @@ def main():
         serve = serve_leg(va, ctx, args)
     corrector_ops = None
     if not args.no_corrector_ops:
-        corrector_ops = corrector_ops_leg(va, W, ctx, m)
+        corrector_ops = corrector_ops_leg(va, W, ctx, m, 2 if args.profile_legs else 20)
     ref_solvers = None
     if not args.no_ref_solvers:
         ref_solvers = reference_solvers_leg(va, W, ctx, m, A, db, dx, dzero, args, not args.no_cpu_baseline)
