@@ -112,6 +112,7 @@ struct LevelEllDev {
         base.assign(std::max(nl, 0), 0);
         size_t total = 0;
         for (int L = 0; L < nl; ++L) {
+            if (off[L + 1] <= off[L]) throw VolError(VOL_ERR_INVALID_ARGUMENT, "empty sweep level");
             int w = 0;
             for (int i = off[L]; i < off[L + 1]; ++i) w = std::max(w, rowOff[perm[i] + 1] - rowOff[perm[i]]);
             width[L] = w;
@@ -120,7 +121,8 @@ struct LevelEllDev {
         }
         if (total >= (size_t(1) << 31)) throw VolError(VOL_ERR_INVALID_ARGUMENT, "sweep rows exceed 2^31 entries");
         nEntries = total;
-        std::vector<int> ec(total, -1), es(total, -1);
+        // at least one entry, so a sweep's stand-in reads of entry 0 stay in bounds
+        std::vector<int> ec(std::max<size_t>(total, 1), -1), es(total, -1);
         for (int L = 0; L < nl; ++L) {
             const int size = off[L + 1] - off[L];
             for (int t = 0; t < size; ++t) {
@@ -134,14 +136,47 @@ struct LevelEllDev {
         }
         grp.clear();
         kind.clear();
-        auto kindOf = [](int size) { return size <= 1024 ? ELL_RUN1 : ELL_GRID; };
+        auto kindOf = [&](int L) {
+            const int size = off[L + 1] - off[L];
+            return size <= 256 ? ELL_RUN256 : size <= 512 ? ELL_RUN512 : size <= 1024 ? ELL_RUN1024 : ELL_GRID;
+        };
         for (int L = 0; L < nl;) {
-            const int k = kindOf(off[L + 1] - off[L]);
-            grp.push_back(L);
-            kind.push_back(char(k));
-            ++L;
-            if (k != ELL_GRID)
-                while (L < nl && kindOf(off[L + 1] - off[L]) == k) ++L;
+            if (kindOf(L) == ELL_GRID) {
+                grp.push_back(L);
+                kind.push_back(char(ELL_GRID));
+                ++L;
+                continue;
+            }
+            // a stretch of run levels [L, R): segments of one class, a segment of fewer
+            // than 16 levels merged into a neighbour (the merged run takes the larger
+            // class), each segment cut into runs of at most ELL_RUN_LEVELS levels
+            int R = L;
+            while (R < nl && kindOf(R) != ELL_GRID) ++R;
+            std::vector<int> segS, segK;
+            for (int i = L; i < R; ++i)
+                if (segK.empty() || segK.back() != kindOf(i)) {
+                    segS.push_back(i);
+                    segK.push_back(kindOf(i));
+                }
+            segS.push_back(R);
+            for (bool merged = true; merged && segK.size() > 1;) {
+                merged = false;
+                for (size_t g = 0; g < segK.size(); ++g) {
+                    if (segS[g + 1] - segS[g] >= 16) continue;
+                    const size_t a = g > 0 ? g - 1 : 0;  // merge segments a and a + 1
+                    segK[a] = std::max(segK[a], segK[a + 1]);
+                    segK.erase(segK.begin() + long(a) + 1);
+                    segS.erase(segS.begin() + long(a) + 1);
+                    merged = true;
+                    break;
+                }
+            }
+            for (size_t g = 0; g < segK.size(); ++g)
+                for (int s0 = segS[g]; s0 < segS[g + 1]; s0 += ELL_RUN_LEVELS) {
+                    grp.push_back(s0);
+                    kind.push_back(char(segK[g]));
+                }
+            L = R;
         }
         grp.push_back(std::max(nl, 0));
         cell.upload(perm, st);

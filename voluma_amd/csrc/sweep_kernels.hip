@@ -17,7 +17,8 @@
 // entries and the gathers of its neighbours' values, and no per-row offsets. Runs of
 // consecutive levels of at most 1024 cells are swept by one workgroup of 1024 threads,
 // a row per thread, in a single launch, with a barrier between levels (global writes of
-// a workgroup are visible to it after the barrier). Larger levels get a grid launch
+// a workgroup are visible to it after the barrier) and the next level's operands loaded
+// while the current level's gathers are in flight. Larger levels get a grid launch
 // each: a level of ~10⁴ rows moves ~1 MB, more than one CU streams in the time of a
 // launch (measured: 16 rows per thread in one workgroup was 3× slower than grid
 // launches on 2M-cell meshes). The factor, run once per matrix, walks the neighbour
@@ -116,63 +117,150 @@ void launch_dilu_factor(hipStream_t st, const SweepView& s, const LevelSched& fw
 //           w_u = rD_u·r_u − Σ (rD_u·A_ul)·w_l, entries in ascending (l, face) order
 //   SW_BWD  DILU backward substitution (LinearSolver.cpp:99-103):
 //           w_l = w_l − Σ (rD_l·A_lu)·w_u, entries in descending (u, face) order
-template <int OP, class T> __device__ __forceinline__ T sw_init(const SweepArgs<T>& a, int c) {
-    if constexpr (OP == SW_GS) return a.b[c];
-    else if constexpr (OP == SW_FWD) return a.rD[c] * a.r[c];
-    else return a.x[c];
+// What a row needs that the sweep does not change: its cell, its first PF entries
+// (column and coefficient) and the operands of its start value (GS: b and diag; forward:
+// rD·r; backward: its own w, which only its own level writes).
+//
+// Every load is issued whatever the row's width: a thread without a row, or an entry
+// past its row, reads a valid stand-in (the level's first cell, entry 0, x[0]) whose
+// value is then discarded. With no branches around them the loads of a phase go out
+// together and the compiler waits for exactly those a result needs (vector loads return
+// in order), not for everything in flight.
+template <int PF, class T> struct RowPre {
+    int c;     // the row's cell (a stand-in when !has)
+    bool has;  // the thread has a row in this level
+    T i0, i1;
+    int col[PF];  // −1: no entry
+    T cf[PF];
+};
+
+__device__ __forceinline__ int row_cell(const LevelEll& E, int pos0, int size, int t) {
+    return E.cell[pos0 + (t < size ? t : 0)];
 }
-template <int OP, class T> __device__ __forceinline__ T sw_fin(const SweepArgs<T>& a, int c, T acc) {
-    if constexpr (OP == SW_GS) return acc / a.diag[c];
-    else return acc;
+template <int PF, class T>
+__device__ __forceinline__ void row_load_entries(const LevelEll& E, const SweepArgs<T>& a, int size, int width, int base,
+                                                 int t, RowPre<PF, T>& p) {
+    p.has = t < size;
+#pragma unroll
+    for (int k = 0; k < PF; ++k) {
+        const bool in = p.has && k < width;
+        const int e = in ? base + k * size + t : 0;
+        const int j = E.col[e];
+        const T v = a.coef[e];
+        p.col[k] = in ? j : -1;
+        p.cf[k] = v;
+    }
+}
+template <int OP, int PF, class T> __device__ __forceinline__ void row_load_start(const SweepArgs<T>& a, RowPre<PF, T>& p) {
+    if constexpr (OP == SW_GS) {
+        p.i0 = a.b[p.c];
+        p.i1 = a.diag[p.c];
+    } else if constexpr (OP == SW_FWD) {
+        p.i0 = a.rD[p.c] * a.r[p.c];
+        p.i1 = T(1);
+    } else {
+        p.i0 = a.x[p.c];
+        p.i1 = T(1);
+    }
+}
+template <int PF, class T> __device__ __forceinline__ void row_gather(const SweepArgs<T>& a, const RowPre<PF, T>& p, T* xv) {
+#pragma unroll
+    for (int k = 0; k < PF; ++k) xv[k] = a.x[p.col[k] >= 0 ? p.col[k] : 0];
+}
+// acc = start − Σ coef·x over the row's entries in order (padding entries, −1, only
+// trail a row); entries k ≥ PF in steps of SWEEP_TAIL with their loads issued together.
+// Writes the row's new value.
+constexpr int SWEEP_TAIL = 4;
+template <int OP, int PF, class T>
+__device__ __forceinline__ void row_finish(const LevelEll& E, const SweepArgs<T>& a, const RowPre<PF, T>& p, const T* xv,
+                                           int size, int width, int base, int t) {
+    T acc = p.i0;
+#pragma unroll
+    for (int k = 0; k < PF; ++k)
+        if (p.col[k] >= 0) acc = acc - p.cf[k] * xv[k];
+    for (int k0 = PF; k0 < width; k0 += SWEEP_TAIL) {
+        int j[SWEEP_TAIL];
+        T cf[SWEEP_TAIL], xt[SWEEP_TAIL];
+#pragma unroll
+        for (int q = 0; q < SWEEP_TAIL; ++q) {
+            const bool in = k0 + q < width;
+            const int e = in ? base + (k0 + q) * size + t : 0;
+            const int jj = E.col[e];
+            cf[q] = a.coef[e];
+            j[q] = in ? jj : -1;
+        }
+#pragma unroll
+        for (int q = 0; q < SWEEP_TAIL; ++q) xt[q] = a.x[j[q] >= 0 ? j[q] : 0];
+#pragma unroll
+        for (int q = 0; q < SWEEP_TAIL; ++q)
+            if (j[q] >= 0) acc = acc - cf[q] * xt[q];
+    }
+    if constexpr (OP == SW_GS) a.x[p.c] = acc / p.i1;
+    else a.x[p.c] = acc;
 }
 
 // One level as a grid: thread t updates the level's row t.
+constexpr int GRID_PF = 8;
 template <int OP, class T>
 __global__ __launch_bounds__(SWEEP_THREADS) void k_ell_level(LevelEll E, SweepArgs<T> a, int pos0, int size, int width,
                                                              int base, const int* done) {
     if (done && *done) return;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= size) return;
-    const int c = E.cell[pos0 + t];
-    T acc = sw_init<OP>(a, c);
-    for (int k = 0; k < width; ++k) {
-        const int e = base + k * size + t;
-        const int j = E.col[e];
-        if (j >= 0) acc = acc - a.coef[e] * a.x[j];
-    }
-    a.x[c] = sw_fin<OP>(a, c, acc);
+    RowPre<GRID_PF, T> p;
+    p.c = row_cell(E, pos0, size, t);
+    row_load_entries(E, a, size, width, base, t, p);
+    row_load_start<OP>(a, p);
+    T xv[GRID_PF];
+    row_gather(a, p, xv);
+    row_finish<OP>(E, a, p, xv, size, width, base, t);
 }
 
-// Levels [L0, L1) by one workgroup, R rows per thread, a barrier after each level.
-template <int OP, int R, class T>
-__global__ __launch_bounds__(RUN_THREADS) void k_ell_run(LevelEll E, SweepArgs<T> a, int L0, int L1,
-                                                         const int* done) {
+// Levels [L0, L1) by one workgroup of THREADS threads (every level has at most THREADS
+// cells), a row per thread, a barrier after each level (global writes of a workgroup are
+// visible to it after the barrier). The levels' offsets, widths and bases are staged in
+// LDS first. In level i a thread first sends out its row's neighbour gathers, then, while
+// they are in flight, loads level i+1's entries and start operands (from the cell it
+// loaded in level i−1) and level i+2's cell: none of them is written before its own level.
+// A level thus costs about one gather round trip, the store and the barrier when its rows
+// have at most PF entries. Sums keep the entry order: the result is the plain loop's.
+template <int OP, int THREADS, int PF, class T>
+__global__ __launch_bounds__(THREADS) void k_ell_run(LevelEll E, SweepArgs<T> a, int L0, int L1, const int* done) {
     if (done && *done) return;
-    for (int L = L0; L < L1; ++L) {
-        const int pos0 = E.offD[L], size = E.offD[L + 1] - pos0, width = E.widthD[L], base = E.baseD[L];
-        int cc[R];
-        T acc[R];
-#pragma unroll
-        for (int q = 0; q < R; ++q) {
-            const int t = int(threadIdx.x) + q * RUN_THREADS;
-            cc[q] = t < size ? E.cell[pos0 + t] : -1;
-            acc[q] = cc[q] >= 0 ? sw_init<OP>(a, cc[q]) : T(0);
+    __shared__ int sOff[ELL_RUN_LEVELS + 1], sWidth[ELL_RUN_LEVELS], sBase[ELL_RUN_LEVELS];
+    const int nl = L1 - L0, t = int(threadIdx.x);
+    for (int i = t; i <= nl; i += THREADS) {
+        sOff[i] = E.offD[L0 + i];
+        if (i < nl) {
+            sWidth[i] = E.widthD[L0 + i];
+            sBase[i] = E.baseD[L0 + i];
         }
-        for (int k = 0; k < width; ++k) {
-#pragma unroll
-            for (int q = 0; q < R; ++q) {
-                const int t = int(threadIdx.x) + q * RUN_THREADS;
-                if (cc[q] >= 0) {
-                    const int e = base + k * size + t;
-                    const int j = E.col[e];
-                    if (j >= 0) acc[q] = acc[q] - a.coef[e] * a.x[j];
-                }
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < R; ++q)
-            if (cc[q] >= 0) a.x[cc[q]] = sw_fin<OP>(a, cc[q], acc[q]);
+    }
+    __syncthreads();
+    // past the last level, a thread loads the last level's data again (unused)
+    auto lev = [nl](int i) { return i < nl ? i : nl - 1; };
+    RowPre<PF, T> cur, nxt;
+    int size = sOff[1] - sOff[0], width = sWidth[0], base = sBase[0];
+    cur.c = row_cell(E, sOff[0], size, t);
+    row_load_entries(E, a, size, width, base, t, cur);
+    row_load_start<OP>(a, cur);
+    int cNext = row_cell(E, sOff[lev(1)], sOff[lev(1) + 1] - sOff[lev(1)], t);
+    for (int i = 0; i < nl; ++i) {
+        T xv[PF];
+        row_gather(a, cur, xv);
+        asm volatile("" ::: "memory");  // the gathers go out before the loads below
+        const int n1 = lev(i + 1), n2 = lev(i + 2);
+        const int nSize = sOff[n1 + 1] - sOff[n1], nWidth = sWidth[n1], nBase = sBase[n1];
+        nxt.c = cNext;
+        row_load_start<OP>(a, nxt);
+        row_load_entries(E, a, nSize, nWidth, nBase, t, nxt);
+        cNext = row_cell(E, sOff[n2], sOff[n2 + 1] - sOff[n2], t);
+        if (cur.has) row_finish<OP>(E, a, cur, xv, size, width, base, t);
         __syncthreads();
+        cur = nxt;
+        size = nSize;
+        width = nWidth;
+        base = nBase;
     }
 }
 
@@ -181,10 +269,17 @@ static void launch_ell(hipStream_t st, const LevelEll& E, int firstLevel, const 
     for (int g = 0; g < E.nGrp; ++g) {
         const int L0 = std::max(E.grpH[g], firstLevel), L1 = E.grpH[g + 1];
         if (L0 >= L1) continue;
-        const int kind = E.grpKindH[g];
-        if (kind == ELL_RUN1) {
-            hipLaunchKernelGGL((k_ell_run<OP, 1, T>), dim3(1), dim3(RUN_THREADS), 0, st, E, a, L0, L1, done);
-        } else {
+        switch (E.grpKindH[g]) {
+        case ELL_RUN256:
+            hipLaunchKernelGGL((k_ell_run<OP, 256, 24, T>), dim3(1), dim3(256), 0, st, E, a, L0, L1, done);
+            break;
+        case ELL_RUN512:
+            hipLaunchKernelGGL((k_ell_run<OP, 512, 16, T>), dim3(1), dim3(512), 0, st, E, a, L0, L1, done);
+            break;
+        case ELL_RUN1024:
+            hipLaunchKernelGGL((k_ell_run<OP, 1024, 12, T>), dim3(1), dim3(1024), 0, st, E, a, L0, L1, done);
+            break;
+        default:
             for (int L = L0; L < L1; ++L) {
                 const int pos0 = E.offH[L], size = E.offH[L + 1] - pos0;
                 if (size > 0)

@@ -414,9 +414,12 @@ void launch_dilu_factor(hipStream_t st, const SweepView& s, const LevelSched& fw
 // Level-ordered ELL rows of a triangular sweep: positions in level order, the cell at
 // each position, and entry k of row t of level L at base[L] + k·size_L + t (neighbour
 // col, −1 padding; the coefficients come per matrix, in the same layout). Groups
-// [grp[g], grp[g+1]) of levels are a run of levels of ≤ 1024 cells swept by one
-// workgroup, a row per thread (ELL_RUN1), or one larger level swept by a grid (ELL_GRID).
-constexpr int ELL_GRID = 0, ELL_RUN1 = 1;
+// [grp[g], grp[g+1]) of levels are one level of more than 1024 cells swept by a grid
+// (ELL_GRID), or a run of at most ELL_RUN_LEVELS consecutive levels swept by one
+// workgroup, a row per thread, whose levels hold at most 256, 512 or 1024 cells
+// (ELL_RUN256 < ELL_RUN512 < ELL_RUN1024: the larger class covers the smaller).
+constexpr int ELL_GRID = 0, ELL_RUN256 = 1, ELL_RUN512 = 2, ELL_RUN1024 = 3;
+constexpr int ELL_RUN_LEVELS = 2048;
 struct LevelEll {
     const int* cell = nullptr;    // device
     const int* col = nullptr;     // device
