@@ -37,6 +37,8 @@ def _lib():
         lib.vr_free.argtypes = [P]
         lib.vr_sizes.argtypes = [P, P]
         lib.vr_patch.argtypes = [P, I, P, P]
+        lib.vr_layout.argtypes = [P, P]
+        lib.vr_patch_name.argtypes = [P, I, P, I]
         lib.vr_addressing.argtypes = [P, P, P]
         lib.vr_geometry.argtypes = [P] * 10
         lib.vr_cellfaces.argtypes = [P, P, P]
@@ -51,8 +53,9 @@ def _lib():
         lib.vr_skew.argtypes = [P, P, P]
         lib.vr_read_polymesh.argtypes = [ctypes.c_char_p]
         lib.vr_read_polymesh.restype = P
-        lib.vr_read_gmsh.argtypes = [ctypes.c_char_p, D]
-        lib.vr_read_gmsh.restype = P
+        for f in (lib.vr_read_gmsh, lib.vr_read_fluent, lib.vr_read_mesh_file):
+            f.argtypes = [ctypes.c_char_p, D, D, ctypes.c_char_p]
+            f.restype = P
         lib.vr_rawmesh_sizes.argtypes = [P, P]
         lib.vr_rawmesh_arrays.argtypes = [P] * 10
         lib.vr_rawmesh_build.argtypes = [P]
@@ -129,6 +132,18 @@ class RefMesh:
             a, b = ctypes.c_long(), ctypes.c_long()
             lib.vr_patch(h, p, ctypes.byref(a), ctypes.byref(b))
             self.patch_start[p], self.patch_size[p] = a.value, b.value
+
+    def layout(self):
+        """(twoD, emptyDir, solvedComponents, patch names) of the reference Mesh."""
+        lib = _lib()
+        out = np.full(6, -1, np.int64)
+        lib.vr_layout(self._h, _p(out))
+        names = []
+        for p in range(self.n_patches):
+            buf = ctypes.create_string_buffer(128)
+            lib.vr_patch_name(self._h, p, buf, 128)
+            names.append(buf.value.decode())
+        return bool(out[0]), int(out[1]), [int(v) for v in out[3:3 + int(out[2])]], names
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -325,9 +340,26 @@ def read_polymesh(directory):
     return _raw_mesh(_lib().vr_read_polymesh(str(directory).encode()))
 
 
-def read_gmsh(path, scale=1.0):
+def _overrides(patch_types):
+    return "".join(f"{k}={v}\n" for k, v in (patch_types or {}).items()).encode()
+
+
+def read_gmsh(path, scale=1.0, depth2d=0.1, patch_types=None):
     """The reference's readGmsh of a file: (raw arrays dict, RefMesh built from it)."""
-    return _raw_mesh(_lib().vr_read_gmsh(str(path).encode(), ctypes.c_double(scale)))
+    return _raw_mesh(_lib().vr_read_gmsh(str(path).encode(), ctypes.c_double(scale), ctypes.c_double(depth2d),
+                                         _overrides(patch_types)))
+
+
+def read_fluent(path, scale=1.0, depth2d=0.1, patch_types=None):
+    """The reference's readFluent of a file: (raw arrays dict, RefMesh built from it)."""
+    return _raw_mesh(_lib().vr_read_fluent(str(path).encode(), ctypes.c_double(scale), ctypes.c_double(depth2d),
+                                           _overrides(patch_types)))
+
+
+def read_mesh_file(path, scale=1.0, depth2d=0.1, patch_types=None):
+    """The reference's readMeshFile of a path: (raw arrays dict, RefMesh built from it)."""
+    return _raw_mesh(_lib().vr_read_mesh_file(str(path).encode(), ctypes.c_double(scale), ctypes.c_double(depth2d),
+                                              _overrides(patch_types)))
 
 
 def _raw_mesh(h):

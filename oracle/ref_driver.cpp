@@ -20,6 +20,7 @@
 #include <cstring>
 #include <exception>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -131,6 +132,27 @@ static void transportPost(MeshPtr mp, int nc, int pressure, double dt, double nu
     put(M.source(), n, sourceRef);
 }
 
+// A RawMesh from read(options), or null with the message kept (templates cannot sit in
+// the extern "C" block).
+template <class F> void* readWith(F read, double scale, double depth2D, const char* patchTypes) {
+    try {
+        MeshImportOptions o;
+        o.scale = scale;
+        o.depth2D = depth2D;
+        std::istringstream ss(patchTypes ? patchTypes : "");
+        std::string line;
+        while (std::getline(ss, line)) {
+            const auto eq = line.rfind('=');
+            if (line.empty() || eq == std::string::npos) continue;
+            o.patchTypes[line.substr(0, eq)] = patchTypeFromString(line.substr(eq + 1));
+        }
+        return new RawMesh(read(o));
+    } catch (const std::exception& e) {
+        fail(e);
+        return nullptr;
+    }
+}
+
 extern "C" {
 
 const char* vr_last_error() { return gErr.c_str(); }
@@ -206,6 +228,20 @@ void vr_patch(void* h, int p, long* start, long* size) {
     *size = m.patches()[p].size;
 }
 
+// twoD, emptyDir, number of solved components, the solved components (Mesh.h:102-105)
+void vr_layout(void* h, long* out) {
+    const Mesh& m = *static_cast<RefMesh*>(h)->m;
+    out[0] = m.twoD() ? 1 : 0;
+    out[1] = m.emptyDir();
+    out[2] = long(m.solvedComponents().size());
+    for (std::size_t k = 0; k < m.solvedComponents().size() && k < 3; ++k) out[3 + k] = m.solvedComponents()[k];
+}
+void vr_patch_name(void* h, int p, char* out, int len) {
+    const Mesh& m = *static_cast<RefMesh*>(h)->m;
+    std::strncpy(out, m.patches()[p].name.c_str(), std::size_t(len) - 1);
+    out[len - 1] = 0;
+}
+
 void vr_addressing(void* h, int* owner, int* neighbour) {
     const Mesh& m = *static_cast<RefMesh*>(h)->m;
     std::memcpy(owner, m.owner().data(), sizeof(int) * m.nFaces());
@@ -247,16 +283,17 @@ void* vr_read_polymesh(const char* dir) {
         return nullptr;
     }
 }
-// Gmsh file read by the reference (readGmsh, MeshReaders.cpp:236-482)
-void* vr_read_gmsh(const char* file, double scale) {
-    try {
-        MeshImportOptions o;
-        o.scale = scale;
-        return new RawMesh(readGmsh(file, o));
-    } catch (const std::exception& e) {
-        fail(e);
-        return nullptr;
-    }
+// Mesh files read by the reference's importers (MeshReaders.cpp): readGmsh
+// (:236-482), readFluent (:751-803) and readMeshFile (:805-816), with MeshImportOptions
+// {scale, depth2D, patchTypes from "name=type" lines (patchTypeFromString)}.
+void* vr_read_gmsh(const char* file, double scale, double depth2D, const char* patchTypes) {
+    return readWith([&](const MeshImportOptions& o) { return readGmsh(file, o); }, scale, depth2D, patchTypes);
+}
+void* vr_read_fluent(const char* file, double scale, double depth2D, const char* patchTypes) {
+    return readWith([&](const MeshImportOptions& o) { return readFluent(file, o); }, scale, depth2D, patchTypes);
+}
+void* vr_read_mesh_file(const char* path, double scale, double depth2D, const char* patchTypes) {
+    return readWith([&](const MeshImportOptions& o) { return readMeshFile(path, o); }, scale, depth2D, patchTypes);
 }
 // nPoints, nFaces, nInternalFaces, nFacePoints, nCells, nPatches
 void vr_rawmesh_sizes(void* h, long* out) {

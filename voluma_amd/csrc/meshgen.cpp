@@ -20,6 +20,7 @@
 #include <cstdint>
 #include <cstring>
 #include <array>
+#include <filesystem>
 #include <cctype>
 #include <fstream>
 #include <map>
@@ -679,7 +680,10 @@ Raw readPolyMesh(const std::string& dir) {
 // take the physical group of the 2-D element with the same nodes (else patch
 // "defaultFaces"), unused points are dropped, and the face list is assembled by
 // assembleFaces below. 2-D meshes (extruded by the reference) are not supported here.
-namespace gmsh {
+// ---------------------------------------------------------------- mesh import
+// Shared by the Gmsh and Fluent readers (MeshReaders.cpp): face keys, face-list assembly,
+// 2-D meshes extruded to one layer, import options.
+namespace meshio {
 
 // Element type → (dimension, corner nodes); higher-order types keep their corners
 // (the element type numbers are Gmsh's).
@@ -764,7 +768,7 @@ V3 polygonNormal(const std::vector<V3>& P, const std::vector<int64_t>& f, V3& ce
 // outward normal (nodes reversed otherwise); internal faces sorted by (c0, c1), then
 // boundary faces patch by patch in input order; patches without faces are dropped.
 Raw assembleFaces(std::vector<V3> points, int64_t nCells, std::vector<Face> faces, const std::vector<Patch>& patches) {
-    std::vector<V3> cc(std::size_t(nCells), V3{});
+    std::vector<V3> cc(static_cast<std::size_t>(nCells), V3{});
     std::vector<int> nf(std::size_t(nCells), 0);
     std::vector<V3> fc(faces.size()), fn(faces.size());
     for (std::size_t f = 0; f < faces.size(); ++f) {
@@ -830,7 +834,173 @@ std::string lower(std::string s) {
     return s;
 }
 
-Raw read(const std::string& file, double scale) {
+// MeshImportOptions (MeshReaders.h:20-24): coordinate scale, 2-D extrusion depth and
+// patch types by name that replace the reader's choice.
+struct Options {
+    double scale = 1.0;
+    double depth2D = 0.1;
+    std::map<std::string, std::string> patchTypes;
+};
+// A patch type string in the reference's spelling (patchTypeFromString then toString,
+// RawMesh.cpp:14-30): wall, symmetryPlane (also from "symmetry"), empty, cyclic, patch.
+std::string typeName(const std::string& s) {
+    if (s == "wall") return "wall";
+    if (s == "symmetry" || s == "symmetryPlane") return "symmetryPlane";
+    if (s == "empty") return "empty";
+    if (s == "cyclic") return "cyclic";
+    return "patch";
+}
+// The type a patch gets: the override for its name if there is one (MeshReaders.cpp:57-61).
+std::string patchType(const std::string& name, const std::string& t, const Options& o) {
+    const auto it = o.patchTypes.find(name);
+    return it != o.patchTypes.end() ? it->second : t;
+}
+
+// A planar polygon mesh: points (x, y used) and each cell's points in order.
+struct Mesh2D {
+    std::vector<V3> points;
+    std::vector<std::vector<int64_t>> cells;
+};
+// Twice the signed area of a cell, summed as the reference sums it (x_k·y_k+1 − x_k+1·y_k).
+double twiceArea(const Mesh2D& m, const std::vector<int64_t>& c) {
+    double a = 0;
+    for (std::size_t k = 0; k < c.size(); ++k) {
+        const V3& p = m.points[c[k]];
+        const V3& q = m.points[c[(k + 1) % c.size()]];
+        a += p.x * q.y - q.x * p.y;
+    }
+    return a;
+}
+
+// One layer of cells from a counter-clockwise 2-D mesh (Extrude2D.cpp:59-166): points
+// z = 0 then z = depth; internal faces from the edges shared by two cells (owner the
+// lower cell, its edge direction), sorted by (owner, neighbour); then each patch's
+// boundary edges sorted by cell; then "frontAndBack" (empty): per cell its reversed
+// polygon at z = 0 and its polygon at z = depth. classify(a, b) gives the patch of a
+// boundary edge. The edges are visited in (min, max) point order and sorted with
+// std::sort, as the reference does, so equal keys keep the reference's order.
+template <class Classify>
+Raw extrude2D(const Mesh2D& m, const std::vector<Patch>& patches, Classify classify, double depth) {
+    const int64_t nP = int64_t(m.points.size()), nC = int64_t(m.cells.size());
+    for (int64_t c = 0; c < nC; ++c) {
+        if (m.cells[c].size() < 3) throw std::runtime_error("extrude2D: degenerate cell");
+        if (twiceArea(m, m.cells[c]) <= 0)
+            throw std::runtime_error("extrude2D: cell " + std::to_string(c) + " is not counter-clockwise");
+    }
+    struct Use {
+        int64_t cell, a, b;
+    };
+    struct Internal {
+        int64_t own, nei, a, b;
+    };
+    std::map<std::pair<int64_t, int64_t>, std::vector<Use>> edges;
+    for (int64_t c = 0; c < nC; ++c) {
+        const auto& pl = m.cells[c];
+        for (std::size_t k = 0; k < pl.size(); ++k) {
+            const int64_t a = pl[k], b = pl[(k + 1) % pl.size()];
+            edges[{std::min(a, b), std::max(a, b)}].push_back({c, a, b});
+        }
+    }
+    std::vector<Internal> internal;
+    std::vector<std::vector<Use>> bnd(patches.size());
+    for (const auto& kv : edges) {
+        const auto& u = kv.second;
+        if (u.size() == 2) {
+            const Use& u0 = u[0].cell < u[1].cell ? u[0] : u[1];
+            const Use& u1 = u[0].cell < u[1].cell ? u[1] : u[0];
+            internal.push_back({u0.cell, u1.cell, u0.a, u0.b});
+        } else if (u.size() == 1) {
+            const int p = classify(u[0].a, u[0].b);
+            if (p < 0 || p >= int(patches.size())) throw std::runtime_error("extrude2D: unclassified boundary edge");
+            bnd[p].push_back(u[0]);
+        } else {
+            throw std::runtime_error("extrude2D: non-manifold edge");
+        }
+    }
+    std::sort(internal.begin(), internal.end(), [](const Internal& x, const Internal& y) {
+        return x.own != y.own ? x.own < y.own : x.nei < y.nei;
+    });
+    for (auto& list : bnd) std::sort(list.begin(), list.end(), [](const Use& x, const Use& y) { return x.cell < y.cell; });
+    Raw r;
+    r.points.resize(std::size_t(2 * nP));
+    for (int64_t i = 0; i < nP; ++i) {
+        r.points[i] = {m.points[i].x, m.points[i].y, 0.0};
+        r.points[nP + i] = {m.points[i].x, m.points[i].y, depth};
+    }
+    r.nCells = nC;
+    for (const Internal& f : internal) {
+        r.addFace({f.a, f.b, nP + f.b, nP + f.a});
+        r.owner.push_back(f.own);
+        r.neighbour.push_back(f.nei);
+    }
+    auto patch = [&](const std::string& name, const std::string& type) {
+        r.pName.push_back(name);
+        r.pType.push_back(type);
+        r.pStart.push_back(r.nFaces());
+    };
+    for (std::size_t p = 0; p < patches.size(); ++p) {
+        patch(patches[p].name, patches[p].type);
+        for (const Use& e : bnd[p]) {
+            r.addFace({e.a, e.b, nP + e.b, nP + e.a});
+            r.owner.push_back(e.cell);
+        }
+        r.pSize.push_back(r.nFaces() - r.pStart.back());
+    }
+    patch("frontAndBack", "empty");
+    for (int64_t c = 0; c < nC; ++c) {
+        const auto& pl = m.cells[c];
+        for (auto it = pl.rbegin(); it != pl.rend(); ++it) r.facePts.push_back(*it);
+        r.faceOff.push_back(int64_t(r.facePts.size()));
+        r.owner.push_back(c);
+        for (int64_t q : pl) r.facePts.push_back(nP + q);
+        r.faceOff.push_back(int64_t(r.facePts.size()));
+        r.owner.push_back(c);
+    }
+    r.pSize.push_back(r.nFaces() - r.pStart.back());
+    return r;
+}
+
+// An imported 2-D mesh (MeshReaders.cpp:64-96): clockwise cells reversed, the patches
+// plus "defaultFaces" (wall) for boundary edges in no group, extruded by depth2D;
+// patches without faces dropped.
+Raw extrudeImported(Mesh2D m, const std::vector<Patch>& patches,
+                    const std::map<std::pair<int64_t, int64_t>, int>& edgePatch, const Options& o) {
+    for (auto& c : m.cells)
+        if (twiceArea(m, c) < 0) std::reverse(c.begin(), c.end());
+    std::vector<Patch> specs;
+    for (const Patch& p : patches) specs.push_back({p.name, patchType(p.name, p.type, o)});
+    const int defaultIdx = int(specs.size());
+    specs.push_back({"defaultFaces", patchType("defaultFaces", "wall", o)});
+    Raw r = extrude2D(
+        m, specs,
+        [&](int64_t a, int64_t b) {
+            const auto it = edgePatch.find({std::min(a, b), std::max(a, b)});
+            return it != edgePatch.end() ? it->second : defaultIdx;
+        },
+        o.depth2D);
+    Raw k = r;
+    k.pName.clear();
+    k.pType.clear();
+    k.pStart.clear();
+    k.pSize.clear();
+    for (std::size_t p = 0; p < r.pSize.size(); ++p)
+        if (r.pSize[p] > 0) {
+            k.pName.push_back(r.pName[p]);
+            k.pType.push_back(r.pType[p]);
+            k.pStart.push_back(r.pStart[p]);
+            k.pSize.push_back(r.pSize[p]);
+        }
+    return k;
+}
+
+}  // namespace meshio
+
+// ---------------------------------------------------------------- Gmsh
+namespace gmsh {
+using namespace meshio;
+
+Raw read(const std::string& file, const Options& opt) {
+    const double scale = opt.scale;
     std::ifstream in(file);
     if (!in) throw std::runtime_error("cannot open " + file);
     double version = 0;
@@ -979,13 +1149,14 @@ Raw read(const std::string& file, double scale) {
     };
     int meshDim = 0;
     for (const Element& e : elems) meshDim = std::max(meshDim, e.dim);
-    if (meshDim != 3) throw std::runtime_error(file + ": only 3-D Gmsh meshes are supported");
+    if (meshDim < 2) throw std::runtime_error(file + ": no 2D or 3D elements");
+    // patches: the physical groups of dimension meshDim − 1
     std::map<int, int> physToPatch;
     std::vector<Patch> patches;
     for (const Element& e : elems)
-        if (e.dim == 2 && e.phys >= 0 && !physToPatch.count(e.phys)) {
+        if (e.dim == meshDim - 1 && e.phys >= 0 && !physToPatch.count(e.phys)) {
             Patch p;
-            const auto it = physNames.find({2, e.phys});
+            const auto it = physNames.find({meshDim - 1, e.phys});
             p.name = it != physNames.end() ? it->second : "patch" + std::to_string(e.phys);
             const std::string l = lower(p.name);
             p.type = l.find("symmetry") != std::string::npos ? "symmetryPlane"
@@ -998,6 +1169,27 @@ Raw read(const std::string& file, double scale) {
         for (std::size_t b = a + 1; b < patches.size(); ++b)
             if (patches[a].name == patches[b].name)
                 throw std::runtime_error(file + ": duplicate physical name " + patches[a].name);
+    if (meshDim == 2) {  // MeshReaders.cpp:406-426: every node kept, cells from 2-D elements
+        Mesh2D m2;
+        m2.points = points;
+        const double z0 = points[0].z;
+        for (const V3& p : points)
+            if (std::abs(p.z - z0) > 1e-9 * (1 + std::abs(z0)))
+                throw std::runtime_error(file + ": 2D Gmsh mesh must lie in a plane z = const");
+        std::map<std::pair<int64_t, int64_t>, int> edgePatch;
+        for (const Element& e : elems) {
+            if (e.dim == 2) {
+                std::vector<int64_t> c;
+                for (int64_t t : e.nodes) c.push_back(node(t));
+                m2.cells.push_back(c);
+            } else if (e.dim == 1 && e.phys >= 0) {
+                const int64_t a = node(e.nodes[0]), b = node(e.nodes[1]);
+                edgePatch[{std::min(a, b), std::max(a, b)}] = physToPatch[e.phys];
+            }
+        }
+        for (Patch& p : patches) p.type = patchType(p.name, p.type, opt);
+        return extrudeImported(std::move(m2), patches, edgePatch, opt);
+    }
     std::unordered_map<Key, int, KeyHash> bFacePatch;
     for (const Element& e : elems)
         if (e.dim == 2 && e.phys >= 0) {
@@ -1007,6 +1199,7 @@ Raw read(const std::string& file, double scale) {
         }
     const int defaultIdx = int(patches.size());
     patches.push_back({"defaultFaces", "wall"});
+    for (Patch& p : patches) p.type = patchType(p.name, p.type, opt);
     std::vector<Face> faces;
     std::unordered_map<Key, std::size_t, KeyHash> faceOf;
     int64_t nCells = 0;
@@ -1050,6 +1243,366 @@ Raw read(const std::string& file, double scale) {
 
 }  // namespace gmsh
 
+// ---------------------------------------------------------------- Fluent
+// Fluent mesh files (MeshReaders.cpp:484-803): parenthesised sections "(index ...)" with
+// hexadecimal headers; index 10 nodes, 12 cells, 13 faces, 39/45 zone names and types,
+// 2 dimension; index + 2000 (single precision, 32-bit integers) or + 3000 (double,
+// 64-bit) marks a binary body ending in "End of Binary Section". Other sections are
+// skipped; periodic pairs (18) import as ordinary patches; non-conformal sections are
+// rejected.
+namespace fluent {
+using namespace meshio;
+
+struct Zone {
+    int id, bc;
+    std::vector<Face> faces;  // nodes and cells 0-based, −1 for no cell
+};
+
+class Reader {
+public:
+    explicit Reader(const std::string& file) : file_(file) {
+        std::ifstream in(file, std::ios::binary);
+        if (!in) throw std::runtime_error("cannot open " + file);
+        std::ostringstream ss;
+        ss << in.rdbuf();
+        s_ = ss.str();
+    }
+    int dim = 3;
+    std::vector<V3> points;
+    int64_t nCells = 0;
+    std::vector<Zone> zones;
+    std::map<int, std::pair<std::string, std::string>> zoneInfo;  // id → (type, name)
+
+    void parse() {
+        for (;;) {
+            ws();
+            if (p_ >= s_.size()) return;
+            if (s_[p_++] != '(') continue;
+            section(int(integer(10)));
+        }
+    }
+
+private:
+    std::string file_, s_;
+    std::size_t p_ = 0;
+
+    [[noreturn]] void fail(const std::string& m) const {
+        throw std::runtime_error(file_ + ": " + m + " (at byte " + std::to_string(p_) + ")");
+    }
+    void ws() {
+        while (p_ < s_.size() && std::isspace(static_cast<unsigned char>(s_[p_]))) ++p_;
+    }
+    bool at(char c) {
+        ws();
+        return p_ < s_.size() && s_[p_] == c;
+    }
+    void expect(char c) {
+        if (!at(c)) fail(std::string("expected '") + c + "'");
+        ++p_;
+    }
+    long long integer(int base) {
+        ws();
+        const char* b = s_.c_str() + p_;
+        char* e = nullptr;
+        const long long v = std::strtoll(b, &e, base);
+        if (e == b) fail("expected integer");
+        p_ += std::size_t(e - b);
+        return v;
+    }
+    double real() {
+        ws();
+        const char* b = s_.c_str() + p_;
+        char* e = nullptr;
+        const double v = std::strtod(b, &e);
+        if (e == b) fail("expected number");
+        p_ += std::size_t(e - b);
+        return v;
+    }
+    std::string word() {
+        ws();
+        const std::size_t b = p_;
+        if (p_ < s_.size() && s_[p_] == '"') {
+            const std::size_t e = s_.find('"', p_ + 1);
+            p_ = e + 1;
+            return s_.substr(b + 1, e - b - 1);
+        }
+        while (p_ < s_.size() && !std::isspace(static_cast<unsigned char>(s_[p_])) && s_[p_] != '(' && s_[p_] != ')')
+            ++p_;
+        return s_.substr(b, p_ - b);
+    }
+    // past the ')' that closes the current level (quoted strings skipped whole)
+    void closeLevel() {
+        for (int depth = 1; p_ < s_.size() && depth > 0;) {
+            const char c = s_[p_++];
+            if (c == '"') {
+                const std::size_t e = s_.find('"', p_);
+                p_ = e == std::string::npos ? s_.size() : e + 1;
+            } else if (c == '(') {
+                ++depth;
+            } else if (c == ')') {
+                --depth;
+            }
+        }
+    }
+    void closeBinary() {
+        const std::size_t e = s_.find("End of Binary Section", p_);
+        if (e == std::string::npos) fail("missing 'End of Binary Section'");
+        p_ = s_.find(')', e);
+        if (p_ == std::string::npos) fail("unterminated binary section");
+        ++p_;
+    }
+    template <class T> T raw() {
+        if (p_ + sizeof(T) > s_.size()) fail("unexpected end of binary data");
+        T v;
+        std::memcpy(&v, s_.data() + p_, sizeof(T));
+        p_ += sizeof(T);
+        return v;
+    }
+    std::vector<long long> header() {
+        expect('(');
+        std::vector<long long> h;
+        while (!at(')')) h.push_back(integer(16));
+        ++p_;
+        return h;
+    }
+    void closeBody(int bin) {
+        if (bin) {
+            closeBinary();
+        } else {
+            expect(')');
+            expect(')');
+        }
+    }
+
+    void section(int idx) {
+        const int kind = idx % 1000, bin = idx / 1000;
+        switch (kind) {
+        case 0:
+        case 1: closeLevel(); return;
+        case 2:
+            dim = int(integer(10));
+            expect(')');
+            return;
+        case 10: nodes(bin); return;
+        case 12: cells(bin); return;
+        case 13: faces(bin); return;
+        case 39:
+        case 45: zone(); return;
+        case 58:
+        case 59:
+        case 61:
+        case 62:
+        case 63:
+            fail("non-conformal / hanging-node Fluent meshes (section " + std::to_string(idx) + ") are not supported");
+        default: break;  // 18 (periodic pairs) and anything else: skipped
+        }
+        if (bin) closeBinary();
+        else closeLevel();
+    }
+    void nodes(int bin) {
+        const auto h = header();
+        if (h.size() < 4) fail("bad node header");
+        if (h[0] == 0) {
+            points.resize(std::size_t(h[2]));
+            expect(')');
+            return;
+        }
+        const int64_t first = h[1] - 1, last = h[2] - 1;
+        const int nd = h.size() >= 5 ? int(h[4]) : dim;
+        if (int64_t(points.size()) <= last) points.resize(std::size_t(last) + 1);
+        expect('(');
+        for (int64_t i = first; i <= last; ++i) {
+            double x[3] = {0, 0, 0};
+            for (int k = 0; k < nd; ++k) x[k] = bin == 0 ? real() : bin == 2 ? double(raw<float>()) : raw<double>();
+            points[i] = {x[0], x[1], x[2]};
+        }
+        closeBody(bin);
+    }
+    void cells(int bin) {
+        const auto h = header();
+        if (h.size() < 3) fail("bad cell header");
+        nCells = std::max<int64_t>(nCells, h[2]);
+        if (h[0] != 0 && at('(')) {  // per-cell element types: not needed
+            ++p_;
+            if (bin) {
+                closeBinary();
+                return;
+            }
+            closeLevel();
+        }
+        expect(')');
+    }
+    void faces(int bin) {
+        const auto h = header();
+        if (h.size() < 5) {
+            if (h.size() >= 3 && h[0] == 0) {
+                expect(')');
+                return;
+            }
+            fail("bad face header");
+        }
+        if (h[0] == 0) {
+            expect(')');
+            return;
+        }
+        Zone z{int(h[0]), int(h[3]), {}};
+        const long long n = h[2] - h[1] + 1;
+        const int faceType = int(h[4]);
+        z.faces.reserve(std::size_t(std::max(n, 0LL)));
+        expect('(');
+        auto next = [&]() -> long long {
+            if (bin == 0) return integer(16);
+            return bin == 3 ? raw<int64_t>() : raw<int32_t>();
+        };
+        for (long long i = 0; i < n; ++i) {
+            long long nn = faceType;
+            if (faceType == 0 || faceType == 5) {  // mixed: a type per face; polygon: a count
+                nn = next();
+                if (faceType == 0 && nn == 5) nn = next();
+            }
+            Face F{{}, -1, -1, -1};
+            for (long long k = 0; k < nn; ++k) F.nodes.push_back(next() - 1);
+            F.c0 = next() - 1;
+            F.c1 = next() - 1;
+            z.faces.push_back(std::move(F));
+        }
+        closeBody(bin);
+        zones.push_back(std::move(z));
+    }
+    void zone() {
+        expect('(');
+        const int id = int(integer(10));
+        const std::string type = word();
+        const std::string name = word();
+        closeLevel();
+        zoneInfo[id] = {type, name};
+        if (at('(')) {
+            ++p_;
+            closeLevel();
+        }
+        expect(')');
+    }
+};
+
+// Patch type of a boundary zone from its type word or, without one, its bc number.
+std::string zoneType(const std::string& type, int bc) {
+    const std::string t = lower(type);
+    if (t == "wall" || (t.empty() && bc == 3)) return "wall";
+    if (t == "symmetry" || t == "axis" || (t.empty() && (bc == 7 || bc == 37))) return "symmetryPlane";
+    return "patch";
+}
+
+// 2-D cell polygons from the edges of each cell, chained from its first edge in file
+// order (MeshReaders.cpp:98-126).
+std::vector<std::vector<int64_t>> polygons(int64_t nCells, const std::vector<std::array<int64_t, 3>>& cellEdges) {
+    std::vector<std::vector<std::pair<int64_t, int64_t>>> e(static_cast<std::size_t>(nCells));
+    for (const auto& ce : cellEdges) e[ce[0]].push_back({ce[1], ce[2]});
+    std::vector<std::vector<int64_t>> cells(static_cast<std::size_t>(nCells));
+    for (int64_t c = 0; c < nCells; ++c) {
+        const auto& E = e[c];
+        const std::string cs = "mesh import: 2D cell " + std::to_string(c);
+        if (E.size() < 3) throw std::runtime_error(cs + " has < 3 edges");
+        std::vector<char> used(E.size(), 0);
+        std::vector<int64_t> poly{E[0].first, E[0].second};
+        used[0] = 1;
+        for (std::size_t k = 1; k < E.size(); ++k) {
+            const int64_t last = poly.back();
+            std::size_t j = 0;
+            for (; j < E.size(); ++j)
+                if (!used[j] && (E[j].first == last || E[j].second == last)) break;
+            if (j == E.size()) throw std::runtime_error(cs + " edges do not form a loop");
+            used[j] = 1;
+            poly.push_back(E[j].first == last ? E[j].second : E[j].first);
+        }
+        if (poly.back() != poly.front()) throw std::runtime_error(cs + " is not closed");
+        poly.pop_back();
+        cells[c] = std::move(poly);
+    }
+    return cells;
+}
+
+Raw read(const std::string& file, const Options& opt) {
+    Reader fr(file);
+    fr.parse();
+    if (fr.points.empty() || fr.zones.empty()) throw std::runtime_error(file + ": no nodes or faces found");
+    for (V3& p : fr.points) p = p * opt.scale;
+    int64_t nc = fr.nCells;
+    for (const Zone& z : fr.zones)
+        for (const Face& F : z.faces) nc = std::max({nc, F.c0 + 1, F.c1 + 1});
+    // every zone with a one-sided face becomes a patch, in zone order
+    std::vector<Patch> patches;
+    std::map<int, int> zoneToPatch;
+    for (const Zone& z : fr.zones) {
+        bool boundary = false;
+        for (const Face& F : z.faces) boundary = boundary || F.c0 < 0 || F.c1 < 0;
+        if (!boundary) continue;
+        const auto it = fr.zoneInfo.find(z.id);
+        Patch p;
+        p.name = it != fr.zoneInfo.end() ? it->second.second : "zone" + std::to_string(z.id);
+        p.type = patchType(p.name, zoneType(it != fr.zoneInfo.end() ? it->second.first : "", z.bc), opt);
+        zoneToPatch[z.id] = int(patches.size());
+        patches.push_back(p);
+    }
+    if (fr.dim == 2) {
+        Mesh2D m2;
+        m2.points = fr.points;
+        std::vector<std::array<int64_t, 3>> cellEdges;
+        std::map<std::pair<int64_t, int64_t>, int> edgePatch;
+        for (const Zone& z : fr.zones)
+            for (const Face& F : z.faces) {
+                if (F.nodes.size() != 2)
+                    throw std::runtime_error(file + ": 2D face with " + std::to_string(F.nodes.size()) + " nodes");
+                const int64_t a = F.nodes[0], b = F.nodes[1];
+                if (F.c0 >= 0) cellEdges.push_back({F.c0, a, b});
+                if (F.c1 >= 0) cellEdges.push_back({F.c1, a, b});
+                if (F.c0 < 0 || F.c1 < 0) edgePatch[{std::min(a, b), std::max(a, b)}] = zoneToPatch.at(z.id);
+            }
+        m2.cells = polygons(nc, cellEdges);
+        return extrudeImported(std::move(m2), patches, edgePatch, opt);
+    }
+    std::vector<Face> faces;
+    for (Zone& z : fr.zones)
+        for (Face& F : z.faces) {
+            if (F.c0 < 0 || F.c1 < 0) F.patch = zoneToPatch.at(z.id);
+            faces.push_back(std::move(F));
+        }
+    return assembleFaces(std::move(fr.points), nc, std::move(faces), patches);
+}
+
+}  // namespace fluent
+
+// A mesh path as readMeshFile chooses (MeshReaders.cpp:805-816): a directory is a
+// polyMesh, a file starting "$MeshFormat" Gmsh, a .msh / .cas file or one whose first
+// line holds '(' Fluent.
+Raw readMeshFile(const std::string& path, const meshio::Options& opt) {
+    namespace fs = std::filesystem;
+    if (fs::is_directory(path)) return readPolyMesh(path);
+    std::ifstream in(path);
+    if (!in) throw std::runtime_error("cannot open " + path);
+    std::string first;
+    std::getline(in, first);
+    if (first.rfind("$MeshFormat", 0) == 0) return gmsh::read(path, opt);
+    const std::string ext = meshio::lower(fs::path(path).extension().string());
+    if (ext == ".msh" || ext == ".cas" || first.find('(') != std::string::npos) return fluent::read(path, opt);
+    throw std::runtime_error(path + ": unknown mesh format");
+}
+
+// Import options from the C arguments; patchTypes holds "name=type" lines.
+meshio::Options importOptions(double scale, double depth2D, const char* patchTypes) {
+    meshio::Options o;
+    o.scale = scale;
+    o.depth2D = depth2D;
+    std::istringstream ss(patchTypes ? patchTypes : "");
+    std::string line;
+    while (std::getline(ss, line)) {
+        const auto eq = line.rfind('=');
+        if (line.empty()) continue;
+        if (eq == std::string::npos) throw std::runtime_error("patch type override '" + line + "' is not name=type");
+        o.patchTypes[line.substr(0, eq)] = meshio::typeName(line.substr(eq + 1));
+    }
+    return o;
+}
+
 thread_local std::string gErr;
 
 void copyV3(const std::vector<V3>& v, double* out) {
@@ -1065,6 +1618,16 @@ void copyD(const std::vector<double>& v, double* out) {
 }
 
 } // namespace
+
+// A new Raw from f(), or null with the message kept for vm_last_error.
+template <class F> void* guarded(F f) {
+    try {
+        return new Raw(f());
+    } catch (const std::exception& e) {
+        gErr = e.what();
+        return nullptr;
+    }
+}
 
 extern "C" {
 
@@ -1098,14 +1661,19 @@ void* vm_read_polymesh(const char* dir) {
     }
 }
 
-// ASCII Gmsh 2.2 / 4.x file of a 3-D mesh; coordinates multiplied by scale
-void* vm_read_gmsh(const char* file, double scale) {
-    try {
-        return new Raw(gmsh::read(file, scale));
-    } catch (const std::exception& e) {
-        gErr = e.what();
-        return nullptr;
-    }
+// Mesh files as the reference's importers read them (MeshImportOptions: coordinates
+// multiplied by scale, 2-D meshes extruded by depth2D, patch types overridden by the
+// "name=type" lines of patchTypes, which may be null): an ASCII Gmsh 2.2 / 4.x file,
+// a Fluent file (ASCII or binary, 2-D or 3-D), or any of these or a polyMesh directory
+// chosen by readMeshFile's rules.
+void* vm_read_gmsh(const char* file, double scale, double depth2D, const char* patchTypes) {
+    return guarded([&] { return gmsh::read(file, importOptions(scale, depth2D, patchTypes)); });
+}
+void* vm_read_fluent(const char* file, double scale, double depth2D, const char* patchTypes) {
+    return guarded([&] { return fluent::read(file, importOptions(scale, depth2D, patchTypes)); });
+}
+void* vm_read_mesh_file(const char* path, double scale, double depth2D, const char* patchTypes) {
+    return guarded([&] { return readMeshFile(path, importOptions(scale, depth2D, patchTypes)); });
 }
 
 // Name and type of patch p (empty for generated meshes); returns -1 for a bad index.

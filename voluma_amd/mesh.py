@@ -46,8 +46,9 @@ def _lib():
         lib.vm_skew.restype = ctypes.c_int
         lib.vm_read_polymesh.argtypes = [ctypes.c_char_p]
         lib.vm_read_polymesh.restype = P
-        lib.vm_read_gmsh.argtypes = [ctypes.c_char_p, ctypes.c_double]
-        lib.vm_read_gmsh.restype = P
+        for f in (lib.vm_read_gmsh, lib.vm_read_fluent, lib.vm_read_mesh_file):
+            f.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_char_p]
+            f.restype = P
         lib.vm_patch_info.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P, ctypes.c_int]
         lib.vm_patch_info.restype = ctypes.c_int
         lib.vm_last_error.restype = ctypes.c_char_p
@@ -125,6 +126,47 @@ class Mesh:
             self._geom = g
         return self._geom
 
+    def computational(self):
+        """The faces the reference's Mesh computes on (Mesh.cpp:236-330, 641-645):
+        internal faces, then the faces of each patch that is not ``empty``, patches in
+        order. A mesh with an empty patch is 2-D: its empty direction is the axis with
+        the largest Σ|S_f| component over the empty faces (the first on a tie) and it
+        solves the other two vector components. Cyclic patches are not supported.
+
+        Returns a dict: ``faces`` (raw face index of each computational face),
+        ``patch_names``/``patch_types``/``patch_start``/``patch_size`` (computational
+        numbering), ``two_d``, ``empty_dir`` (−1 in 3-D), ``solved_components``."""
+        if "cyclic" in self.patch_types:
+            raise RuntimeError("cyclic patches are not supported")
+        faces = [np.arange(self.n_internal)]
+        out = {"patch_names": [], "patch_types": [], "patch_start": [], "patch_size": []}
+        n = self.n_internal
+        empty_n = np.zeros(3)
+        sf = None
+        for p, kind in enumerate(self.patch_types):
+            f = self.patch_faces(p)
+            if kind == "empty":
+                sf = self.geometry()["Sf"] if sf is None else sf
+                for v in np.abs(sf[f]):  # in face order, as the reference sums
+                    empty_n += v
+                continue
+            faces.append(f)
+            out["patch_names"].append(self.patch_names[p])
+            out["patch_types"].append(kind)
+            out["patch_start"].append(n)
+            out["patch_size"].append(len(f))
+            n += len(f)
+        two_d = "empty" in self.patch_types
+        d = -1
+        if two_d:
+            d = 0
+            for a in (1, 2):
+                if empty_n[a] > empty_n[d]:
+                    d = a
+        out.update(faces=np.concatenate(faces), two_d=two_d, empty_dir=d,
+                   solved_components=[c for c in range(3) if not two_d or c != d])
+        return out
+
     def agglomeration_weights(self):
         """|S_f|·nonOrthDeltaCoeff_f over internal faces (reference GAMG.cpp:215)."""
         g = self.geometry()
@@ -143,26 +185,31 @@ def read_polymesh(directory) -> Mesh:
     return Mesh(_lib().vm_read_polymesh(str(directory).encode()))
 
 
-def read_gmsh(path, scale=1.0) -> Mesh:
-    """An ASCII Gmsh 2.2 / 4.x file of a 3-D mesh (tets, hexes, prisms, pyramids, also
-    higher-order), as the reference's readGmsh reads it (MeshReaders.cpp:236-482):
-    same points, faces, face order and patches."""
-    return Mesh(_lib().vm_read_gmsh(str(path).encode(), float(scale)))
+def _import_args(path, scale, depth2d, patch_types):
+    over = "".join(f"{name}={kind}\n" for name, kind in (patch_types or {}).items())
+    return str(path).encode(), float(scale), float(depth2d), over.encode()
 
 
-def read_mesh(path, scale=1.0) -> Mesh:
-    """A polyMesh directory or a Gmsh file, chosen as readMeshFile chooses
-    (MeshReaders.cpp:805-816); Fluent files are not read here."""
-    import os
-    if os.path.isdir(path):
-        return read_polymesh(path)
-    if not os.path.isfile(path):
-        raise RuntimeError(f"cannot open {path}")
-    with open(path, "rb") as fh:
-        first = fh.readline()
-    if first.startswith(b"$MeshFormat"):
-        return read_gmsh(path, scale)
-    raise RuntimeError(f"{path}: not a polyMesh directory or a Gmsh file")
+def read_gmsh(path, scale=1.0, depth2d=0.1, patch_types=None) -> Mesh:
+    """An ASCII Gmsh 2.2 / 4.x file, as the reference's readGmsh reads it
+    (MeshReaders.cpp:236-482): same points, faces, face order and patches. A 3-D mesh
+    (tets, hexes, prisms, pyramids, also higher-order) is assembled from its cells'
+    faces; a 2-D mesh (z = const) is extruded one layer of ``depth2d``. Options follow
+    MeshImportOptions: ``scale`` multiplies coordinates, ``patch_types`` maps patch
+    names to the type they get (wall, symmetry, empty, cyclic, patch)."""
+    return Mesh(_lib().vm_read_gmsh(*_import_args(path, scale, depth2d, patch_types)))
+
+
+def read_fluent(path, scale=1.0, depth2d=0.1, patch_types=None) -> Mesh:
+    """A Fluent mesh file (ASCII or binary sections, 2-D or 3-D), as the reference's
+    readFluent reads it (MeshReaders.cpp:484-803); options as for :func:`read_gmsh`."""
+    return Mesh(_lib().vm_read_fluent(*_import_args(path, scale, depth2d, patch_types)))
+
+
+def read_mesh(path, scale=1.0, depth2d=0.1, patch_types=None) -> Mesh:
+    """A polyMesh directory, a Gmsh file or a Fluent file, chosen as readMeshFile
+    chooses (MeshReaders.cpp:805-816)."""
+    return Mesh(_lib().vm_read_mesh_file(*_import_args(path, scale, depth2d, patch_types)))
 
 
 def box(nx, ny=None, nz=None) -> Mesh:
