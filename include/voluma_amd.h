@@ -313,6 +313,41 @@ int vol_fvc_surface_integrate(vol_mesh mesh, int nComponents, const double* face
 int vol_fvc_flux(vol_mesh mesh, const double* U, const double* gradU, const int32_t* bFixesFlux, const double* Ub,
                  double* phi, int location);
 
+/* ---- pressure-corrector steps (IncompressibleFlow.cpp:242-362), on a mesh with geometry.
+ * Cell vectors are interleaved Vec3 (nCells × 3), face arrays cover nInternalFaces +
+ * nBoundaryFaces, boundary arrays nBoundaryFaces; all bit-identical to the reference. */
+/* rAU = 1/A, HbyA = rAU·H (IncompressibleFlow.cpp:245-249) */
+int vol_piso_rau_hbya(vol_mesh mesh, const double* A, const double* H, double* rAU, double* HbyA, int location);
+/* boundary values of HbyA for its least-squares gradient (IncompressibleFlow.cpp:261-270):
+ * U_b + rAU_P ∇p_P on fixed-flux faces (bFixesFlux), zero elsewhere */
+int vol_piso_hbya_boundary(vol_mesh mesh, const int32_t* bFixesFlux, const double* Ub, const double* rAU,
+                           const double* gradp, double* bValue, int location);
+/* face-wise combinations over all faces: out = a + b·c (phiHbyA += rAUf·φ_F,
+ * IncompressibleFlow.cpp:294-295), a − b (φ_F − ∇p_f·S_f, :219-220; φ = phiHbyA − F,
+ * :338-339), a − b/c (φ_F − pF/rAUf, :351-352). c unused for VOL_FACE_A_MINUS_B. */
+#define VOL_FACE_A_PLUS_B_TIMES_C 0
+#define VOL_FACE_A_MINUS_B 1
+#define VOL_FACE_A_MINUS_B_OVER_C 2
+int vol_piso_face_op(vol_mesh mesh, int op, const double* a, const double* b, const double* c, double* out,
+                     int location);
+/* ddtPhiCorr (IncompressibleFlow.cpp:296-316), internal faces, phiHbyA updated in place:
+ * ddtPhiCoeff < 0 selects the automatic coupling; c0, c00 are ddtCoeffs (Schemes.h:71-86);
+ * U00 and phiOldOld are read only when c00 != 0. */
+int vol_piso_ddt_phi_corr(vol_mesh mesh, double ddtPhiCoeff, double c0, double c00, const double* U0,
+                          const double* U00, const double* phiOld, const double* phiOldOld, const double* rAUf,
+                          double* phiHbyA, int location);
+/* constrainPressure (IncompressibleFlow.cpp:17-29): the fixedFluxPressure gradient of the
+ * boundary faces [patchStart, patchStart + patchSize) (boundary numbering):
+ * gradient[i] = (phiHbyA_f − U_b·S_f) / (|S_f| rAUf_f) */
+int vol_piso_constrain_pressure(vol_mesh mesh, int patchStart, int patchSize, const double* phiHbyA,
+                                const double* Ub, const double* rAUf, double* gradient, int location);
+/* fvc::reconstruct of a face flux (Fvc.h:33-67); emptyDir: the 2-D empty direction or −1 */
+int vol_fvc_reconstruct(vol_mesh mesh, const double* faceFlux, int emptyDir, double* out, int location);
+/* velocity correction (IncompressibleFlow.cpp:347-360): U = HbyA − rAU·v (sign < 0, v = ∇p)
+ * or HbyA + rAU·v (sign > 0, v = reconstruct(...)); component emptyDir (2-D) set to 0 */
+int vol_piso_correct_U(vol_mesh mesh, const double* HbyA, const double* rAU, const double* v, int sign, int emptyDir,
+                       double* U, int location);
+
 /* ---- batches: many independent systems solved together (no reference counterpart).
  * Systems are concatenated: owner/neighbour/gamgWeights hold each system's internal
  * faces in turn with that system's own (0-based) cell numbers; diag/upper/lower, b and

@@ -1493,3 +1493,107 @@ void oracle_fvc_flux(int n, int nI, int nB, const int* own, const int* nei, cons
         phi[i] = bFix[i - nI] ? dot3(Ub + 3 * (i - nI), Sf + 3 * i) : dot3(U + 3 * own[i], Sf + 3 * i);
     free(Uf);
 }
+
+/* ---------------------------------------------------------------- pressure-corrector steps */
+
+/* rAU = 1.0 / A; HbyA = rAU * H (IncompressibleFlow.cpp:245-249) */
+void oracle_piso_rau_hbya(int n, const double* A, const double* H, double* rAU, double* HbyA) {
+    for (int c = 0; c < n; ++c) {
+        rAU[c] = 1.0 / A[c];
+        for (int k = 0; k < 3; ++k) HbyA[3 * c + k] = rAU[c] * H[3 * c + k];
+    }
+}
+
+/* bVal = U_b + rAU_o * gradp_o on fixed-flux faces, else 0 (IncompressibleFlow.cpp:261-270) */
+void oracle_piso_hbya_boundary(int nI, int nB, const int* own, const int* bFix, const double* Ub, const double* rAU,
+                               const double* gp, double* bVal) {
+    for (int i = 0; i < nB; ++i) {
+        const int o = own[nI + i];
+        for (int k = 0; k < 3; ++k) bVal[3 * i + k] = bFix[i] ? Ub[3 * i + k] + rAU[o] * gp[3 * o + k] : 0.0;
+    }
+}
+
+/* op 0: a + b*c (:294-295); 1: a - b (:219-220, :338-339); 2: a - b/c (:351-352) */
+void oracle_piso_face_op(int nF, int op, const double* a, const double* b, const double* c, double* out) {
+    for (int f = 0; f < nF; ++f) out[f] = op == 0 ? a[f] + b[f] * c[f] : op == 1 ? a[f] - b[f] : a[f] - b[f] / c[f];
+}
+
+/* ddtPhiCorr (IncompressibleFlow.cpp:296-316) */
+void oracle_piso_ddt_phi_corr(int nI, const int* own, const int* nei, const double* w, const double* Sf, double coeff,
+                              double c0, double c00, const double* U0, const double* U00, const double* phiOld,
+                              const double* phiOldOld, const double* rAUf, double* phiHbyA) {
+    for (int f = 0; f < nI; ++f) {
+        const double wf = w[f];
+        const int o = own[f], n = nei[f];
+        double v[3];
+        for (int k = 0; k < 3; ++k) v[k] = wf * U0[3 * o + k] + (1.0 - wf) * U0[3 * n + k];
+        const double Uf0 = dot3(v, Sf + 3 * f);
+        const double corr0 = phiOld[f] - Uf0;
+        double coupling = coeff;
+        if (coeff < 0) {
+            const double q = fabs(corr0) / (fabs(phiOld[f]) + 1.0e-15);
+            coupling = 1.0 - (1.0 < q ? 1.0 : q);
+        }
+        double dc = c0 * corr0;
+        if (c00 != 0.0) {
+            for (int k = 0; k < 3; ++k) v[k] = wf * U00[3 * o + k] + (1.0 - wf) * U00[3 * n + k];
+            dc -= c00 * (phiOldOld[f] - dot3(v, Sf + 3 * f));
+        }
+        phiHbyA[f] += rAUf[f] * coupling * dc;
+    }
+}
+
+/* constrainPressure (IncompressibleFlow.cpp:17-29) on boundary faces [b0, b0 + cnt) */
+void oracle_piso_constrain_pressure(int nI, int b0, int cnt, const double* Sf, const double* magSf,
+                                    const double* phiHbyA, const double* Ub, const double* rAUf, double* g) {
+    for (int i = 0; i < cnt; ++i) {
+        const int f = nI + b0 + i;
+        g[i] = (phiHbyA[f] - dot3(Ub + 3 * (b0 + i), Sf + 3 * f)) / (magSf[f] * rAUf[f]);
+    }
+}
+
+/* fvc::reconstruct (Fvc.h:33-67): face loop scattering S S^T/|S| and (phi/|S|) S into
+ * owner and neighbour, 2-D diagonal fix, adjugate inverse */
+static void rec_acc(double* A, double* b, const double* S, double ms, double phi) {
+    const double r = 1.0 / ms; /* Tensor / scalar multiplies by the reciprocal (Types.h:86) */
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) A[3 * i + j] += S[i] * S[j] * r;
+    const double s = phi / ms;
+    for (int i = 0; i < 3; ++i) b[i] += s * S[i];
+}
+void oracle_fvc_reconstruct(int n, int nI, int nB, const int* own, const int* nei, const double* Sf,
+                            const double* magSf, const double* ssf, int emptyDir, double* v) {
+    double* A = (double*)xcalloc((size_t)n * 9 + 1, sizeof(double));
+    double* b = (double*)xcalloc((size_t)n * 3 + 1, sizeof(double));
+    for (int f = 0; f < nI; ++f) {
+        rec_acc(A + 9 * own[f], b + 3 * own[f], Sf + 3 * f, magSf[f], ssf[f]);
+        rec_acc(A + 9 * nei[f], b + 3 * nei[f], Sf + 3 * f, magSf[f], ssf[f]);
+    }
+    for (int f = nI; f < nI + nB; ++f) rec_acc(A + 9 * own[f], b + 3 * own[f], Sf + 3 * f, magSf[f], ssf[f]);
+    for (int c = 0; c < n; ++c) {
+        double* t = A + 9 * c;
+        const double* r = b + 3 * c;
+        v[3 * c] = v[3 * c + 1] = v[3 * c + 2] = 0.0;
+        if (emptyDir >= 0) t[4 * emptyDir] += 1.0;
+        const double c00 = t[4] * t[8] - t[5] * t[7], c01 = t[5] * t[6] - t[3] * t[8], c02 = t[3] * t[7] - t[4] * t[6];
+        const double det = t[0] * c00 + t[1] * c01 + t[2] * c02;
+        if (fabs(det) < 1e-300) continue;
+        const double c10 = t[2] * t[7] - t[1] * t[8], c11 = t[0] * t[8] - t[2] * t[6], c12 = t[1] * t[6] - t[0] * t[7];
+        const double c20 = t[1] * t[5] - t[2] * t[4], c21 = t[2] * t[3] - t[0] * t[5], c22 = t[0] * t[4] - t[1] * t[3];
+        v[3 * c] = (c00 * r[0] + c10 * r[1] + c20 * r[2]) / det;
+        v[3 * c + 1] = (c01 * r[0] + c11 * r[1] + c21 * r[2]) / det;
+        v[3 * c + 2] = (c02 * r[0] + c12 * r[1] + c22 * r[2]) / det;
+    }
+    free(A);
+    free(b);
+}
+
+/* U = HbyA - rAU*v (sign < 0) or HbyA + rAU*v; the empty direction zeroed (:347-360) */
+void oracle_piso_correct_U(int n, const double* HbyA, const double* rAU, const double* v, int sign, int emptyDir,
+                           double* U) {
+    for (int c = 0; c < n; ++c)
+        for (int k = 0; k < 3; ++k) {
+            const double t = rAU[c] * v[3 * c + k];
+            U[3 * c + k] = k == emptyDir ? 0.0 : sign < 0 ? HbyA[3 * c + k] - t : HbyA[3 * c + k] + t;
+        }
+}

@@ -114,6 +114,23 @@ void oracle_fvc_flux(int n, int nI, int nB, const int* own, const int* nei, cons
                      const double* Sf, const double* U, const double* grad, const int* bFix, const double* Ub,
                      double* phi);
 
+/* Pressure-corrector steps (IncompressibleFlow.cpp:17-29,242-362) and fvc::reconstruct
+ * (Fvc.h:33-67). Face arrays cover nI + nB faces, boundary arrays nB; own covers all
+ * faces. */
+void oracle_piso_rau_hbya(int n, const double* A, const double* H, double* rAU, double* HbyA);
+void oracle_piso_hbya_boundary(int nI, int nB, const int* own, const int* bFix, const double* Ub, const double* rAU,
+                               const double* gp, double* bVal);
+void oracle_piso_face_op(int nF, int op, const double* a, const double* b, const double* c, double* out);
+void oracle_piso_ddt_phi_corr(int nI, const int* own, const int* nei, const double* w, const double* Sf, double coeff,
+                              double c0, double c00, const double* U0, const double* U00, const double* phiOld,
+                              const double* phiOldOld, const double* rAUf, double* phiHbyA);
+void oracle_piso_constrain_pressure(int nI, int b0, int cnt, const double* Sf, const double* magSf,
+                                    const double* phiHbyA, const double* Ub, const double* rAUf, double* g);
+void oracle_fvc_reconstruct(int n, int nI, int nB, const int* own, const int* nei, const double* Sf,
+                            const double* magSf, const double* ssf, int emptyDir, double* v);
+void oracle_piso_correct_U(int n, const double* HbyA, const double* rAU, const double* v, int sign, int emptyDir,
+                           double* U);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,13 @@ def _lib():
         lib.oracle_fvc_div.argtypes = [I, I, I, P, P, P, P, P]
         lib.oracle_fvc_surface_integrate.argtypes = [I, I, I, P, P, P, I, P, P]
         lib.oracle_fvc_flux.argtypes = [I, I, I] + [P] * 10
+        lib.oracle_piso_rau_hbya.argtypes = [I, P, P, P, P]
+        lib.oracle_piso_hbya_boundary.argtypes = [I, I, P, P, P, P, P, P]
+        lib.oracle_piso_face_op.argtypes = [I, I, P, P, P, P]
+        lib.oracle_piso_ddt_phi_corr.argtypes = [I, P, P, P, P, D, D, D, P, P, P, P, P, P]
+        lib.oracle_piso_constrain_pressure.argtypes = [I, I, I, P, P, P, P, P, P]
+        lib.oracle_fvc_reconstruct.argtypes = [I, I, I, P, P, P, P, P, I, P]
+        lib.oracle_piso_correct_U.argtypes = [I, P, P, P, I, I, P]
         lib.oracle_relax.argtypes = [I, I, I] + [P] * 6 + [D, P, P]
         lib.oracle_set_reference.argtypes = [I, ctypes.c_long, D, P, P]
         lib.oracle_A.argtypes = [I, I, I, P, P, P, P, I, P]
@@ -460,3 +467,74 @@ def fvc_flux(mesh, U, grad, b_fixed, Ub):
                            None if gr is None else _p(gr), _p(_i32(b_fixed)),
                            _p(np.ascontiguousarray(Ub, dtype=np.float64)), _p(out))
     return out
+
+
+# ---------------------------------------------------------------- pressure-corrector steps
+# mesh: n_cells, n_internal, n_boundary, owner (all faces), neighbour and geometry()
+# (Sf, magSf, w over the faces the solve uses).
+def _f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def piso_rau_hbya(A, H):
+    """rAU = 1/A, HbyA = rAU·H (IncompressibleFlow.cpp:245-249)."""
+    A, H = _f64(A), _f64(H)
+    rAU, HbyA = np.zeros(len(A)), np.zeros((len(A), 3))
+    _lib().oracle_piso_rau_hbya(len(A), _p(A), _p(H), _p(rAU), _p(HbyA))
+    return rAU, HbyA
+
+
+def piso_hbya_boundary(mesh, b_fixed, Ub, rAU, gradp):
+    """U_b + rAU_P ∇p_P on fixed-flux boundary faces, 0 elsewhere (IncompressibleFlow.cpp:261-270)."""
+    out = np.zeros((mesh.n_boundary, 3))
+    _lib().oracle_piso_hbya_boundary(mesh.n_internal, mesh.n_boundary, _p(_i32(mesh.owner)), _p(_i32(b_fixed)),
+                                     _p(_f64(Ub)), _p(_f64(rAU)), _p(_f64(gradp)), _p(out))
+    return out
+
+
+def piso_face_op(op, a, b, c=None):
+    """op 0: a + b·c, 1: a − b, 2: a − b/c (IncompressibleFlow.cpp:219-220,294-295,338-339,351-352)."""
+    a, b, c = _f64(a), _f64(b), _f64(c)
+    out = np.zeros(len(a))
+    _lib().oracle_piso_face_op(len(a), int(op), _p(a), _p(b), None if c is None else _p(c), _p(out))
+    return out
+
+
+def piso_ddt_phi_corr(mesh, coeff, c0, c00, U0, U00, phi_old, phi_old_old, rAUf, phiHbyA):
+    """phiHbyA after the ddtPhiCorr coupling (IncompressibleFlow.cpp:296-316)."""
+    g = mesh.geometry()
+    out = _f64(phiHbyA).copy()
+    _lib().oracle_piso_ddt_phi_corr(mesh.n_internal, _p(_i32(mesh.owner)), _p(_i32(mesh.neighbour)), _p(_f64(g["w"])),
+                                    _p(_f64(g["Sf"])), float(coeff), float(c0), float(c00), _p(_f64(U0)),
+                                    None if U00 is None else _p(_f64(U00)), _p(_f64(phi_old)),
+                                    None if phi_old_old is None else _p(_f64(phi_old_old)), _p(_f64(rAUf)), _p(out))
+    return out
+
+
+def piso_constrain_pressure(mesh, start, size, phiHbyA, Ub, rAUf):
+    """fixedFluxPressure gradient of boundary faces [start, start + size) (IncompressibleFlow.cpp:17-29)."""
+    g = mesh.geometry()
+    out = np.zeros(max(size, 1))
+    _lib().oracle_piso_constrain_pressure(mesh.n_internal, int(start), int(size), _p(_f64(g["Sf"])),
+                                          _p(_f64(g["magSf"])), _p(_f64(phiHbyA)), _p(_f64(Ub)), _p(_f64(rAUf)),
+                                          _p(out))
+    return out[:size]
+
+
+def fvc_reconstruct(mesh, ssf, empty_dir=-1):
+    """fvc::reconstruct of a face flux (Fvc.h:33-67): nCells × 3."""
+    g = mesh.geometry()
+    out = np.zeros((mesh.n_cells, 3))
+    _lib().oracle_fvc_reconstruct(mesh.n_cells, mesh.n_internal, mesh.n_boundary, _p(_i32(mesh.owner)),
+                                  _p(_i32(mesh.neighbour)), _p(_f64(g["Sf"])), _p(_f64(g["magSf"])), _p(_f64(ssf)),
+                                  int(empty_dir), _p(out))
+    return out
+
+
+def piso_correct_U(HbyA, rAU, v, sign, empty_dir=-1):
+    """U = HbyA − rAU·v (sign < 0) or HbyA + rAU·v, empty direction zeroed (IncompressibleFlow.cpp:347-360)."""
+    HbyA = _f64(HbyA)
+    out = np.zeros_like(HbyA)
+    _lib().oracle_piso_correct_U(len(HbyA), _p(HbyA), _p(_f64(rAU)), _p(_f64(v)), int(sign), int(empty_dir), _p(out))
+    return out
+

@@ -38,6 +38,9 @@ def _lib():
         lib.vr_sizes.argtypes = [P, P]
         lib.vr_patch.argtypes = [P, I, P, P]
         lib.vr_layout.argtypes = [P, P]
+        lib.vr_reconstruct.argtypes = [P, P, P]
+        lib.vr_reconstruct.restype = I
+        lib.vr_ddt_coeffs.argtypes = [I, D, D, I, P]
         lib.vr_patch_name.argtypes = [P, I, P, I]
         lib.vr_addressing.argtypes = [P, P, P]
         lib.vr_geometry.argtypes = [P] * 10
@@ -132,6 +135,14 @@ class RefMesh:
             a, b = ctypes.c_long(), ctypes.c_long()
             lib.vr_patch(h, p, ctypes.byref(a), ctypes.byref(b))
             self.patch_start[p], self.patch_size[p] = a.value, b.value
+
+    def reconstruct(self, ssf):
+        """fvc::reconstruct of a face flux (nFaces) on the reference Mesh: nCells × 3."""
+        out = np.zeros((self.n_cells, 3))
+        ssf = np.ascontiguousarray(ssf, dtype=np.float64)
+        if _lib().vr_reconstruct(self._h, _p(ssf), _p(out)) != 0:
+            raise RuntimeError("reference: " + _lib().vr_last_error().decode())
+        return out
 
     def layout(self):
         """(twoD, emptyDir, solvedComponents, patch names) of the reference Mesh."""
@@ -387,3 +398,12 @@ def _raw_mesh(h):
         return raw, RefMesh(handle=mh)
     finally:
         lib.vr_rawmesh_free(h)
+
+
+def ddt_coeffs(scheme, dt, dt0=-1.0, have_old_old=False):
+    """The reference's ddtCoeffs (Schemes.h:71-86): scheme "steadyState", "Euler" or
+    "backward" → (c, c0, c00)."""
+    out = np.zeros(3)
+    code = {"steadyState": 0, "Euler": 1, "backward": 2}[scheme]
+    _lib().vr_ddt_coeffs(code, ctypes.c_double(dt), ctypes.c_double(dt0), int(have_old_old), _p(out))
+    return tuple(float(v) for v in out)

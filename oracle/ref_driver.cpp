@@ -228,6 +228,34 @@ void vr_patch(void* h, int p, long* start, long* size) {
     *size = m.patches()[p].size;
 }
 
+// fvc::reconstruct (Fvc.h:33-67) of a face flux on the reference Mesh: out nCells × 3
+int vr_reconstruct(void* h, const double* ssf, double* out) {
+    try {
+        const Mesh& m = *static_cast<RefMesh*>(h)->m;
+        const std::vector<scalar> f(ssf, ssf + m.nFaces());
+        const auto v = fvc::reconstruct(m, f);
+        for (label c = 0; c < m.nCells(); ++c) {
+            out[3 * c] = v[c].x;
+            out[3 * c + 1] = v[c].y;
+            out[3 * c + 2] = v[c].z;
+        }
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
+}
+// ddtCoeffs (Schemes.h:71-86): scheme 0 steady, 1 Euler, 2 backward; out c, c0, c00
+void vr_ddt_coeffs(int scheme, double dt, double dt0, int haveOldOld, double* out) {
+    TimeState ts;
+    ts.dt = dt;
+    ts.dt0 = dt0;
+    const DdtScheme s = scheme == 0 ? DdtScheme::Steady : scheme == 1 ? DdtScheme::Euler : DdtScheme::Backward;
+    const DdtCoeffs k = ddtCoeffs(s, ts, haveOldOld != 0);
+    out[0] = k.c;
+    out[1] = k.c0;
+    out[2] = k.c00;
+}
+
 // twoD, emptyDir, number of solved components, the solved components (Mesh.h:102-105)
 void vr_layout(void* h, long* out) {
     const Mesh& m = *static_cast<RefMesh*>(h)->m;

@@ -83,3 +83,25 @@ def fvc_bfix(m, fixed):
         s = int(m.patch_start[p]) - ni
         out[s: s + int(m.patch_size[p])] = 1
     return out
+
+
+class CompView:
+    """The faces the reference's Mesh keeps of a read mesh (empty patches dropped): the
+    attributes the oracle's corrector functions read, with geometry restricted to those
+    faces, and the 2-D empty direction."""
+
+    def __init__(self, m):
+        c = m.computational()
+        f = c["faces"]
+        self.n_cells, self.n_internal = m.n_cells, m.n_internal
+        self.n_faces = len(f)
+        self.n_boundary = self.n_faces - self.n_internal
+        self.owner = np.ascontiguousarray(m.owner[f])
+        self.neighbour = m.neighbour
+        g = m.geometry()
+        self._g = {k: (g[k] if k in ("V", "C", "skewed") else np.ascontiguousarray(g[k][f])) for k in g}
+        self.empty_dir = c["empty_dir"]
+        self.comp = c
+
+    def geometry(self):
+        return self._g

@@ -29,6 +29,7 @@ SUMS_CONTEXT, SUMS_FAST, SUMS_REPRODUCIBLE, SUMS_EXACT = -1, 0, 1, 2
 FP64, MIXED = 0, 1
 GAMG_JACOBI, GAMG_GAUSS_SEIDEL = 0, 1
 GRAD_GAUSS_LINEAR, GRAD_LEAST_SQUARES, GRAD_GAUSS_SKEW = 0, 1, 2
+FACE_A_PLUS_B_TIMES_C, FACE_A_MINUS_B, FACE_A_MINUS_B_OVER_C = 0, 1, 2
 
 _ERRORS = {
     -1: "invalid argument", -2: "unknown solver", -3: "unknown preconditioner", -4: "PCG on nonsymmetric matrix",
@@ -94,7 +95,9 @@ EXPORTED = ("vol_controls_default", "vol_last_error", "vol_context_create", "vol
             "vol_server_create", "vol_server_add_mesh", "vol_server_submit", "vol_server_wait", "vol_server_stats",
             "vol_server_destroy", "vol_fvm_finish_source", "vol_fvm_relax", "vol_fvm_set_reference", "vol_fvm_A",
             "vol_fvm_H", "vol_fvm_flux", "vol_mesh_set_geometry", "vol_fvc_interpolate", "vol_fvc_grad", "vol_fvc_div",
-            "vol_fvc_surface_integrate", "vol_fvc_flux", "vol_geometry")
+            "vol_fvc_surface_integrate", "vol_fvc_flux", "vol_geometry", "vol_piso_rau_hbya",
+            "vol_piso_hbya_boundary", "vol_piso_face_op", "vol_piso_ddt_phi_corr", "vol_piso_constrain_pressure",
+            "vol_fvc_reconstruct", "vol_piso_correct_U")
 
 
 def lib():
@@ -151,6 +154,13 @@ def lib():
         L.vol_fvc_div.argtypes = [P, P, P, I]
         L.vol_fvc_surface_integrate.argtypes = [P, I, P, P, I]
         L.vol_fvc_flux.argtypes = [P, P, P, P, P, P, I]
+        L.vol_piso_rau_hbya.argtypes = [P, P, P, P, P, I]
+        L.vol_piso_hbya_boundary.argtypes = [P, P, P, P, P, P, I]
+        L.vol_piso_face_op.argtypes = [P, I, P, P, P, P, I]
+        L.vol_piso_ddt_phi_corr.argtypes = [P, D, D, D, P, P, P, P, P, P, I]
+        L.vol_piso_constrain_pressure.argtypes = [P, I, I, P, P, P, P, I]
+        L.vol_fvc_reconstruct.argtypes = [P, P, I, P, I]
+        L.vol_piso_correct_U.argtypes = [P, P, P, P, I, I, P, I]
         L.vol_batch_create.argtypes = [P, I, P, P, P, P, P, ctypes.POINTER(P)]
         L.vol_batch_destroy.argtypes = [P]
         L.vol_batch_info.argtypes = [P, P, P, P]
@@ -370,6 +380,62 @@ class LduMesh:
                                   _p(np.ascontiguousarray(Ub, dtype=np.float64)), _p(out), VOL_HOST))
         return out[: self._nf_total()]
 
+    # ---- pressure-corrector steps (IncompressibleFlow.cpp:17-29,242-362)
+    def piso_rau_hbya(self, A, H):
+        """(rAU, HbyA) = (1/A, rAU·H)."""
+        A = np.ascontiguousarray(A, dtype=np.float64)
+        rau, hbya = np.zeros(self.n_cells), np.zeros((self.n_cells, 3))
+        _check(lib().vol_piso_rau_hbya(self._h, _p(A), _p(np.ascontiguousarray(H, dtype=np.float64)), _p(rau),
+                                       _p(hbya), VOL_HOST))
+        return rau, hbya
+
+    def piso_hbya_boundary(self, b_fixed, Ub, rAU, gradp):
+        """Boundary values of HbyA for its least-squares gradient: U_b + rAU_P ∇p_P on
+        fixed-flux faces, 0 elsewhere."""
+        out = np.zeros((max(self.n_boundary, 1), 3))
+        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        _check(lib().vol_piso_hbya_boundary(self._h, _p(np.ascontiguousarray(b_fixed, dtype=np.int32)), _p(f(Ub)),
+                                            _p(f(rAU)), _p(f(gradp)), _p(out), VOL_HOST))
+        return out[: self.n_boundary]
+
+    def piso_face_op(self, op, a, b, c=None):
+        """Face-wise a + b·c, a − b or a − b/c (FACE_* constants)."""
+        f = lambda x: None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(max(self._nf_total(), 1))
+        _check(lib().vol_piso_face_op(self._h, int(op), _p(f(a)), _p(f(b)), _q(f(c)), _p(out), VOL_HOST))
+        return out[: self._nf_total()]
+
+    def piso_ddt_phi_corr(self, coeff, c0, c00, U0, U00, phi_old, phi_old_old, rAUf, phiHbyA):
+        """phiHbyA with the ddtPhiCorr coupling added (coeff < 0: automatic coupling)."""
+        f = lambda x: None if x is None else np.ascontiguousarray(x, dtype=np.float64)
+        out = f(phiHbyA).copy()
+        _check(lib().vol_piso_ddt_phi_corr(self._h, float(coeff), float(c0), float(c00), _p(f(U0)), _q(f(U00)),
+                                           _p(f(phi_old)), _q(f(phi_old_old)), _p(f(rAUf)), _p(out), VOL_HOST))
+        return out
+
+    def piso_constrain_pressure(self, start, size, phiHbyA, Ub, rAUf):
+        """fixedFluxPressure gradient of boundary faces [start, start + size)."""
+        f = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros(max(size, 1))
+        _check(lib().vol_piso_constrain_pressure(self._h, int(start), int(size), _p(f(phiHbyA)), _p(f(Ub)),
+                                                 _p(f(rAUf)), _p(out), VOL_HOST))
+        return out[:size]
+
+    def fvc_reconstruct(self, flux, empty_dir=-1):
+        """fvc::reconstruct of a face flux: nCells × 3."""
+        out = np.zeros((self.n_cells, 3))
+        _check(lib().vol_fvc_reconstruct(self._h, _p(np.ascontiguousarray(flux, dtype=np.float64)), int(empty_dir),
+                                         _p(out), VOL_HOST))
+        return out
+
+    def piso_correct_U(self, HbyA, rAU, v, sign, empty_dir=-1):
+        """U = HbyA − rAU·v (sign < 0) or HbyA + rAU·v, component empty_dir zeroed."""
+        f = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+        out = np.zeros((self.n_cells, 3))
+        _check(lib().vol_piso_correct_U(self._h, _p(f(HbyA)), _p(f(rAU)), _p(f(v)), int(sign), int(empty_dir),
+                                        _p(out), VOL_HOST))
+        return out
+
     def gamg_levels(self):
         """[{n, agg, faceMap, faceFlip}] per level, as GamgHierarchy::levels."""
         nl = ctypes.c_int()
@@ -385,6 +451,21 @@ class LduMesh:
             _check(lib().vol_mesh_gamg_level(self._h, k, ctypes.byref(nf), _p(agg), _p(fm), _p(ff)))
             out.append({"n": int(sizes[k]), "agg": agg, "faceMap": fm[: nf.value], "faceFlip": ff[: nf.value]})
         return out
+
+
+def ddt_coeffs(scheme, dt, dt0=-1.0, have_old_old=False):
+    """ddtCoeffs (Schemes.h:71-86): (c, c0, c00) of "steadyState", "Euler" or "backward"
+    (variable-step BDF2 when the old-old level exists and dt0 > 0, Euler otherwise)."""
+    if scheme == "steadyState":
+        return 0.0, 0.0, 0.0
+    if scheme not in ("Euler", "backward"):
+        raise ValueError(f"unknown ddt scheme: {scheme}")
+    rdt = 1.0 / dt
+    if scheme == "backward" and have_old_old and dt0 > 0:
+        ct = 1.0 + dt / (dt + dt0)
+        ct00 = dt * dt / (dt0 * (dt + dt0))
+        return ct * rdt, (ct + ct00) * rdt, ct00 * rdt
+    return rdt, rdt, 0.0
 
 
 def device_geometry(ctx: "Context", mesh):

@@ -63,7 +63,7 @@ struct vol_mesh_s {
     bool geomSet = false;
     int skewed = 0;
     DevBuf<double> gV, gSf, gMagSf, gW, gD, gSkew;
-    DevBuf<double> fst[5], fout, fgls;
+    DevBuf<double> fst[8], fout, fgls;
     DevBuf<int> fsti;
     LevelOrderDev ordF;  // forward level order of level 0 (DIC/DILU factor)
     DevBuf<int> lowOff, lowCol, lowSrc, upOff, upCol, upSrc;
@@ -1955,6 +1955,129 @@ int vol_fvc_flux(vol_mesh mesh, const double* U, const double* gradU, const int3
                     fvcInI(mesh, bFixesFlux, G.nb, location), fvcIn(mesh, 2, Ub, 3 * size_t(G.nb), location),
                     fvcOutBuf(mesh, phi, nF, location));
     fvcOut(mesh, phi, nF, location);
+    API_END
+}
+
+int vol_piso_rau_hbya(vol_mesh mesh, const double* A, const double* H, double* rAU, double* HbyA, int location) {
+    API_BEGIN
+    if (!mesh || !A || !H || !rAU || !HbyA) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    setDevice(mesh->ctx);
+    hipStream_t st = mesh->ctx->stream;
+    const size_t n = mesh->n;
+    const double* a = fvcIn(mesh, 0, A, n, location);
+    const double* h = fvcIn(mesh, 1, H, 3 * n, location);
+    double* r = rAU;
+    double* hb = HbyA;
+    if (location != VOL_DEVICE) {
+        mesh->fst[2].alloc(std::max<size_t>(n, 1));
+        mesh->fst[3].alloc(std::max<size_t>(3 * n, 1));
+        r = mesh->fst[2].p;
+        hb = mesh->fst[3].p;
+    }
+    launch_piso_rau_hbya(st, int(n), a, h, r, hb);
+    if (location != VOL_DEVICE && n) {
+        HIPCHECK(hipMemcpyAsync(rAU, r, sizeof(double) * n, hipMemcpyDeviceToHost, st));
+        HIPCHECK(hipMemcpyAsync(HbyA, hb, sizeof(double) * 3 * n, hipMemcpyDeviceToHost, st));
+    }
+    HIPCHECK(hipStreamSynchronize(st));
+    API_END
+}
+
+int vol_piso_hbya_boundary(vol_mesh mesh, const int32_t* bFixesFlux, const double* Ub, const double* rAU,
+                           const double* gradp, double* bValue, int location) {
+    API_BEGIN
+    if (!mesh || !rAU || !gradp || !bValue) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    setDevice(mesh->ctx);
+    const FvcGeom G = fvcGeom(mesh);
+    if (G.nb > 0 && (!bFixesFlux || !Ub)) throw VolError(VOL_ERR_INVALID_ARGUMENT, "bFixesFlux and Ub required");
+    const size_t n = G.n, nb = G.nb;
+    launch_piso_hbya_boundary(mesh->ctx->stream, G, fvcInI(mesh, bFixesFlux, nb, location),
+                              fvcIn(mesh, 0, Ub, 3 * nb, location), fvcIn(mesh, 1, rAU, n, location),
+                              fvcIn(mesh, 2, gradp, 3 * n, location), fvcOutBuf(mesh, bValue, 3 * nb, location));
+    fvcOut(mesh, bValue, 3 * nb, location);
+    API_END
+}
+
+int vol_piso_face_op(vol_mesh mesh, int op, const double* a, const double* b, const double* c, double* out,
+                     int location) {
+    API_BEGIN
+    if (!mesh || !a || !b || !out) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    if (op != VOL_FACE_A_PLUS_B_TIMES_C && op != VOL_FACE_A_MINUS_B && op != VOL_FACE_A_MINUS_B_OVER_C)
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "unknown face operation");
+    if (op != VOL_FACE_A_MINUS_B && !c) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    if (mesh->nb < 0) throw VolError(VOL_ERR_INVALID_ARGUMENT, "mesh boundary not set (vol_mesh_set_boundary)");
+    setDevice(mesh->ctx);
+    const size_t nF = size_t(mesh->nf) + mesh->nb;
+    launch_piso_face_op(mesh->ctx->stream, op, int(nF), fvcIn(mesh, 0, a, nF, location), fvcIn(mesh, 1, b, nF, location),
+                        op == VOL_FACE_A_MINUS_B ? nullptr : fvcIn(mesh, 2, c, nF, location),
+                        fvcOutBuf(mesh, out, nF, location));
+    fvcOut(mesh, out, nF, location);
+    API_END
+}
+
+int vol_piso_ddt_phi_corr(vol_mesh mesh, double ddtPhiCoeff, double c0, double c00, const double* U0,
+                          const double* U00, const double* phiOld, const double* phiOldOld, const double* rAUf,
+                          double* phiHbyA, int location) {
+    API_BEGIN
+    if (!mesh || !U0 || !phiOld || !rAUf || !phiHbyA) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    if (c00 != 0.0 && (!U00 || !phiOldOld)) throw VolError(VOL_ERR_INVALID_ARGUMENT, "U00 and phiOldOld required");
+    setDevice(mesh->ctx);
+    const FvcGeom G = fvcGeom(mesh);
+    hipStream_t st = mesh->ctx->stream;
+    const size_t n = G.n, nF = size_t(G.nf) + G.nb;
+    double* io = fvcOutBuf(mesh, phiHbyA, nF, location);
+    if (location != VOL_DEVICE && nF)
+        HIPCHECK(hipMemcpyAsync(io, phiHbyA, sizeof(double) * nF, hipMemcpyHostToDevice, st));
+    const bool two = c00 != 0.0;
+    launch_piso_ddt_phi_corr(st, G, ddtPhiCoeff, c0, c00, fvcIn(mesh, 0, U0, 3 * n, location),
+                             two ? fvcIn(mesh, 1, U00, 3 * n, location) : nullptr, fvcIn(mesh, 2, phiOld, nF, location),
+                             two ? fvcIn(mesh, 3, phiOldOld, nF, location) : nullptr,
+                             fvcIn(mesh, 4, rAUf, nF, location), io);
+    fvcOut(mesh, phiHbyA, nF, location);
+    API_END
+}
+
+int vol_piso_constrain_pressure(vol_mesh mesh, int patchStart, int patchSize, const double* phiHbyA,
+                                const double* Ub, const double* rAUf, double* gradient, int location) {
+    API_BEGIN
+    if (!mesh || !phiHbyA || !Ub || !rAUf || (patchSize > 0 && !gradient))
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    setDevice(mesh->ctx);
+    const FvcGeom G = fvcGeom(mesh);
+    if (patchStart < 0 || patchSize < 0 || int64_t(patchStart) + patchSize > G.nb)
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "patch range outside the boundary faces");
+    const size_t nF = size_t(G.nf) + G.nb;
+    launch_piso_constrain_pressure(mesh->ctx->stream, G, patchStart, patchSize, fvcIn(mesh, 0, phiHbyA, nF, location),
+                                   fvcIn(mesh, 1, Ub, 3 * size_t(G.nb), location), fvcIn(mesh, 2, rAUf, nF, location),
+                                   fvcOutBuf(mesh, gradient, size_t(patchSize), location));
+    fvcOut(mesh, gradient, size_t(patchSize), location);
+    API_END
+}
+
+int vol_fvc_reconstruct(vol_mesh mesh, const double* faceFlux, int emptyDir, double* out, int location) {
+    API_BEGIN
+    if (!mesh || !faceFlux || !out) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    if (emptyDir < -1 || emptyDir > 2) throw VolError(VOL_ERR_INVALID_ARGUMENT, "emptyDir must be -1, 0, 1 or 2");
+    setDevice(mesh->ctx);
+    const FvcGeom G = fvcGeom(mesh);
+    const size_t n = G.n, nF = size_t(G.nf) + G.nb;
+    launch_fvc_reconstruct(mesh->ctx->stream, G, fvcIn(mesh, 0, faceFlux, nF, location), emptyDir,
+                           fvcOutBuf(mesh, out, 3 * n, location));
+    fvcOut(mesh, out, 3 * n, location);
+    API_END
+}
+
+int vol_piso_correct_U(vol_mesh mesh, const double* HbyA, const double* rAU, const double* v, int sign, int emptyDir,
+                       double* U, int location) {
+    API_BEGIN
+    if (!mesh || !HbyA || !rAU || !v || !U) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    if (emptyDir < -1 || emptyDir > 2) throw VolError(VOL_ERR_INVALID_ARGUMENT, "emptyDir must be -1, 0, 1 or 2");
+    setDevice(mesh->ctx);
+    const size_t n = mesh->n;
+    launch_piso_correct_U(mesh->ctx->stream, int(n), fvcIn(mesh, 0, HbyA, 3 * n, location),
+                          fvcIn(mesh, 1, rAU, n, location), fvcIn(mesh, 2, v, 3 * n, location), sign, emptyDir,
+                          fvcOutBuf(mesh, U, 3 * n, location));
+    fvcOut(mesh, U, 3 * n, location);
     API_END
 }
 

@@ -380,6 +380,20 @@ void launch_fvc_integrate(hipStream_t st, const FvcGeom& G, int nc, bool div, co
 // face flux of a cell vector field, phiHbyA form (IncompressibleFlow.cpp:276-291)
 void launch_fvc_flux(hipStream_t st, const FvcGeom& G, const double* U, const double* grad, const int* bFix,
                      const double* Ub, double* phi);
+// pressure-corrector steps (piso_kernels.hip)
+void launch_piso_rau_hbya(hipStream_t st, int n, const double* A, const double* H, double* rAU, double* HbyA);
+void launch_piso_hbya_boundary(hipStream_t st, const FvcGeom& G, const int* bFix, const double* Ub, const double* rAU,
+                               const double* gp, double* bVal);
+void launch_piso_face_op(hipStream_t st, int op, int nF, const double* a, const double* b, const double* c,
+                         double* out);
+void launch_piso_ddt_phi_corr(hipStream_t st, const FvcGeom& G, double coeff, double c0, double c00, const double* U0,
+                              const double* U00, const double* phiOld, const double* phiOldOld, const double* rAUf,
+                              double* phiHbyA);
+void launch_piso_constrain_pressure(hipStream_t st, const FvcGeom& G, int b0, int cnt, const double* phiHbyA,
+                                    const double* Ub, const double* rAUf, double* g);
+void launch_fvc_reconstruct(hipStream_t st, const FvcGeom& G, const double* ssf, int emptyDir, double* out);
+void launch_piso_correct_U(hipStream_t st, int n, const double* HbyA, const double* rAU, const double* v, int sign,
+                           int emptyDir, double* U);
 
 // ------------------------------------------------------------------ level-scheduled sweeps
 // Lower / upper neighbour lists of the cells (device arrays). Lower list of cell u: neighbours l < u in ascending (l, face) order;
