@@ -218,6 +218,11 @@ typedef struct {
 int vol_fvm_reset(vol_matrix A);
 /* fvm::ddt, Euler (Fvm.h:18-31; Schemes.h:71-86): diag += V/dt, source += V·psi0/dt */
 int vol_fvm_ddt_euler(vol_matrix A, int sign, double dt, const double* V, const double* psi0, int location);
+/* fvm::ddt with ddtCoeffs (Fvm.h:18-31): diag += c·V, source += V·(c0·psi0 − c00·psi00);
+ * Euler: c = c0 = 1/dt, c00 = 0; backward (variable-step BDF2, Schemes.h:75-81) with the
+ * old-old field psi00 (nullable → psi0, as VolField::oldOld() is without one) */
+int vol_fvm_ddt(vol_matrix A, int sign, double c, double c0, double c00, const double* V, const double* psi0,
+                const double* psi00, int location);
 /* fvm::div(F, ψ, upwind) (Fvm.h:74-137) */
 int vol_fvm_div_upwind(vol_matrix A, int sign, const double* flux, const int32_t* bcFixed, const double* bcValue,
                        int location);
@@ -286,10 +291,16 @@ int vol_geometry(vol_context ctx, int nPoints, const double* points, int nFaces,
  * Gauss-skew gradient always uses skew, as the reference does). Multi-component values
  * are interleaved per cell or face, as std::vector<Vec3>/<Tensor> hold them: nComponents
  * 1 (scalar field; its gradient a Vec3 per cell) or 3 (vector field; its gradient a
- * row-major Tensor per cell). Results are bit-identical to the reference's. Meshes are
- * 3-D (no empty direction). */
+ * row-major Tensor per cell). Results are bit-identical to the reference's. A 2-D mesh
+ * passes the faces the reference's Mesh keeps (empty patches dropped) and its empty
+ * direction (vol_mesh_set_empty_dir). */
 int vol_mesh_set_geometry(vol_mesh mesh, const double* V, const double* Sf, const double* magSf, const double* w,
                           const double* d, const double* skew, int skewed, int location);
+/* A 2-D mesh (Mesh::twoD, emptyDir; Mesh.cpp:300-330): the empty direction (0, 1, 2; −1
+ * for 3-D, the default). Every gradient then adds 1 to the empty diagonal of its
+ * least-squares matrix and zeroes the empty component (Fvc.h:152,169-181): vol_fvc_grad
+ * and the non-orthogonal gradient of vol_fvm_laplacian. */
+int vol_mesh_set_empty_dir(vol_mesh mesh, int emptyDir);
 /* fvc::interpolate (Fvc.h:13-31,72-81): w c_P + (1−w) c_N on internal faces, plus
  * gradDot(skew, w g_P + (1−w) g_N) when cellGrad is given and the mesh is skewed;
  * boundary faces take boundaryValues (nullable → the owner cell's value). */

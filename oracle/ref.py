@@ -38,6 +38,8 @@ def _lib():
         lib.vr_sizes.argtypes = [P, P]
         lib.vr_patch.argtypes = [P, I, P, P]
         lib.vr_layout.argtypes = [P, P]
+        lib.vr_piso_run.argtypes = [P, I, D, D] + [P] * 12
+        lib.vr_piso_run.restype = I
         lib.vr_reconstruct.argtypes = [P, P, P]
         lib.vr_reconstruct.restype = I
         lib.vr_ddt_coeffs.argtypes = [I, D, D, I, P]
@@ -135,6 +137,32 @@ class RefMesh:
             a, b = ctypes.c_long(), ctypes.c_long()
             lib.vr_patch(h, p, ctypes.byref(a), ctypes.byref(b))
             self.patch_start[p], self.patch_size[p] = a.value, b.value
+
+    def piso_run(self, n_steps, nu, dt, U0, p0, phi0, u_bc, p_bc, n_corr=2, n_non_orth=0, momentum_predictor=True,
+                 ddt="Euler", grad_p=0, nonorth_corr=True, p_ref_cell=0, p_ref_value=0.0, ddt_phi_coeff=0.0,
+                 u_tol=1e-8, p_tol=1e-8, p_rel=0.01, p_final_tol=1e-8):
+        """n_steps of the reference's IncompressibleFlow::step (upwind convection) from
+        U0, p0 and phi0. u_bc: per patch None (zeroGradient) or a value (3); p_bc: per
+        patch None or a value. Returns (U, p, phi)."""
+        npatch = self.n_patches
+        ut = np.array([0 if v is None else 1 for v in u_bc], np.int32)
+        uv = np.array([[0.0, 0.0, 0.0] if v is None else list(v) for v in u_bc], np.float64)
+        pt = np.array([0 if v is None else 1 for v in p_bc], np.int32)
+        pv = np.array([0.0 if v is None else float(v) for v in p_bc], np.float64)
+        assert len(ut) == npatch and len(pt) == npatch
+        iopt = np.array([n_corr, n_non_orth, int(momentum_predictor), 2 if ddt == "backward" else 1, grad_p,
+                         int(nonorth_corr), p_ref_cell], np.int32)
+        dopt = np.array([ddt_phi_coeff, p_ref_value, u_tol, p_tol, p_rel, p_final_tol], np.float64)
+        U = np.zeros((self.n_cells, 3))
+        p = np.zeros(self.n_cells)
+        phi = np.zeros(self.n_faces)
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        U0, p0, phi0 = c(U0), c(p0), c(phi0)
+        rc = _lib().vr_piso_run(self._h, int(n_steps), ctypes.c_double(nu), ctypes.c_double(dt), _p(iopt), _p(dopt),
+                                _p(ut), _p(uv), _p(pt), _p(pv), _p(U0), _p(p0), _p(phi0), _p(U), _p(p), _p(phi))
+        if rc != 0:
+            raise RuntimeError("reference: " + _lib().vr_last_error().decode())
+        return U, p, phi
 
     def reconstruct(self, ssf):
         """fvc::reconstruct of a face flux (nFaces) on the reference Mesh: nCells × 3."""

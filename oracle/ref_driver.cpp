@@ -16,6 +16,7 @@
 #include "fvm/mesh/MeshReaders.h"
 #include "fvm/mesh/RawMesh.h"
 #include "fvm/parallel/Comm.h"
+#include "fvm/solvers/IncompressibleFlow.h"
 
 #include <cstring>
 #include <exception>
@@ -254,6 +255,62 @@ void vr_ddt_coeffs(int scheme, double dt, double dt0, int haveOldOld, double* ou
     out[0] = k.c;
     out[1] = k.c0;
     out[2] = k.c00;
+}
+
+// nSteps time steps of the reference's IncompressibleFlow (PISO, IncompressibleFlow.cpp:
+// 144-370) from U0, p0 and the face flux phi0 (kept as a restart keeps it), upwind
+// convection. Per patch: U type 1 fixedValue (uValue, 3 per patch) or 0 zeroGradient; p
+// type 1 fixedValue (pValue) or 0 zeroGradient. iopt: nCorr, nNonOrthCorr,
+// momentumPredictor, ddt scheme (1 Euler, 2 backward), gradPScheme (0 Gauss linear,
+// 1 least squares, 2 Gauss skew), laplacianNonOrthCorr, pRefCell. dopt: ddtPhiCoeff,
+// pRefValue, U tolerance, p tolerance, p relTol, pFinal tolerance. Out: U, p, phi.
+int vr_piso_run(void* h, int nSteps, double nu, double dt, const int* iopt, const double* dopt, const int* uType,
+                const double* uValue, const int* pType, const double* pValue, const double* U0, const double* p0,
+                const double* phi0, double* U, double* p, double* phi) {
+    try {
+        MeshPtr mesh = static_cast<RefMesh*>(h)->m;
+        const Mesh& m = *mesh;
+        IncompressibleFlow flow(mesh, nu);
+        for (label q = 0; q < label(m.patches().size()); ++q) {
+            const std::string& name = m.patches()[q].name;
+            if (uType[q]) flow.U().fixedValue(name, Vec3{uValue[3 * q], uValue[3 * q + 1], uValue[3 * q + 2]});
+            else flow.U().zeroGradient(name);
+            if (pType[q]) flow.p().fixedValue(name, pValue[q]);
+            else flow.p().zeroGradient(name);
+        }
+        for (label c = 0; c < m.nCells(); ++c) {
+            flow.U()[c] = Vec3{U0[3 * c], U0[3 * c + 1], U0[3 * c + 2]};
+            flow.p()[c] = p0[c];
+        }
+        flow.phi().assign(phi0, phi0 + m.nFaces());
+        PimpleControls& ct = flow.controls;
+        ct.nCorr = iopt[0];
+        ct.nNonOrthCorr = iopt[1];
+        ct.momentumPredictor = iopt[2] != 0;
+        flow.ddtScheme = iopt[3] == 2 ? DdtScheme::Backward : DdtScheme::Euler;
+        flow.gradPScheme = iopt[4] == 1 ? GradScheme::LeastSquares : iopt[4] == 2 ? GradScheme::GaussSkew
+                                                                                   : GradScheme::GaussLinear;
+        flow.laplacianNonOrthCorr = iopt[5] != 0;
+        ct.pRefCell = iopt[6];
+        ct.ddtPhiCoeff = dopt[0];
+        ct.pRefValue = dopt[1];
+        ct.UControls = SolverControls{"PBiCGStab", "DILU", dopt[2], 0.0, 1000, 0, 1};
+        ct.pControls = SolverControls{"PCG", "GAMG", dopt[3], dopt[4], 2000, 0, 1};
+        ct.pFinalControls = SolverControls{"PCG", "GAMG", dopt[5], 0.0, 2000, 0, 1};
+        flow.divScheme = ConvectionScheme::parse("upwind");
+        flow.markRestarted();
+        for (int k = 0; k < nSteps; ++k) flow.step(dt);
+        for (label c = 0; c < m.nCells(); ++c) {
+            U[3 * c] = flow.U()[c].x;
+            U[3 * c + 1] = flow.U()[c].y;
+            U[3 * c + 2] = flow.U()[c].z;
+            p[c] = flow.p()[c];
+        }
+        std::copy(flow.phi().begin(), flow.phi().end(), phi);
+        return 0;
+    } catch (const std::exception& e) {
+        return fail(e);
+    }
 }
 
 // twoD, emptyDir, number of solved components, the solved components (Mesh.h:102-105)

@@ -30,6 +30,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.abspath(os.path.join(HERE, "..", ".."))
 REF_ROOT = os.environ.get("VOLUMA_REF", "/root/reference")  # the reference tree (mesh data files)
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
 
 from oracle import ref  # noqa: E402
 from voluma_amd import mesh as M  # noqa: E402
@@ -392,6 +393,30 @@ def import_case():
         json.dump(out, fh, indent=1)
 
 
+def piso_case(name):
+    """Time steps of the reference's IncompressibleFlow (vr_piso_run) for one PISO case
+    of helpers.PISO_CASES: U, p and φ after the steps."""
+    from helpers import PISO_CASES, piso_inputs
+    mk, opt, steps, dt, repro = PISO_CASES[name]
+    m = mk()
+    if name.startswith("annulus"):
+        _, rm = ref.read_mesh_file(os.path.join(HERE, "import", "annulus_v41.msh"))
+    else:
+        rm = ref.RefMesh(m)
+    U0, p0, phi0 = piso_inputs(m)
+    kw = {k: v for k, v in opt.items() if k not in ("nu", "u_bc", "p_bc")}
+    if "ddt_phi_coeff" in kw:
+        kw["ddt_phi_coeff"] = kw["ddt_phi_coeff"]
+    ref.set_reproducible(repro)
+    try:
+        U, p, phi = rm.piso_run(steps, opt["nu"], dt, U0, p0, phi0, opt["u_bc"], opt["p_bc"], **kw)
+    finally:
+        ref.set_reproducible(False)
+    np.savez(os.path.join(HERE, f"piso_{name}.npz"), U=U, p=p, phi=phi)
+    print(f"piso {name}: |U| {np.abs(U).max():.4g} |p| {np.abs(p).max():.4g} |phi| {np.abs(phi).max():.4g}",
+          flush=True)
+
+
 def summary(x):
     x = np.asarray(x)
     idx = np.linspace(0, x.size - 1, 16).astype(np.int64)
@@ -532,6 +557,11 @@ if __name__ == "__main__":
     if "--import" in sys.argv:
         import_case()
         sys.exit(0)
+    if "--piso" in sys.argv:
+        from helpers import PISO_CASES
+        for nm in PISO_CASES:
+            piso_case(nm)
+        sys.exit(0)
     if "--fvc" in sys.argv:
         for nm, mk in FVC_MESHES.items():
             fvc_case(nm, mk)
@@ -543,6 +573,9 @@ if __name__ == "__main__":
     polymesh_case()
     gmsh_case()
     import_case()
+    from helpers import PISO_CASES
+    for nm in PISO_CASES:
+        piso_case(nm)
     dropin_case()
     exactsum_case()
     for nm, mk in SMALL.items():

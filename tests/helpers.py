@@ -105,3 +105,38 @@ class CompView:
 
     def geometry(self):
         return self._g
+
+
+# ---------------------------------------------------------------- PISO step cases
+# name → (mesh maker, options, number of steps, dt, reproducible sums). u_bc / p_bc are
+# per kept patch; the initial fields come from piso_inputs.
+PISO_CASES = {
+    "box8_lid_repro": (lambda: M.box(8, 8, 8),
+                       {"nu": 0.01, "u_bc": [(0, 0, 0)] * 5 + [(1, 0, 0)], "p_bc": [None] * 6}, 2, 0.005, True),
+    "tet5x4x3_channel_repro": (lambda: M.tet(5, 4, 3, 0.2, 7),
+                               {"nu": 0.02, "u_bc": [(1, 0, 0), None, (0, 0, 0), (0, 0, 0), (0, 0, 0), (0, 0, 0)],
+                                "p_bc": [None, 0.0, None, None, None, None], "ddt": "backward",
+                                "ddt_phi_coeff": -1.0, "grad_p": 2, "n_non_orth": 1}, 3, 0.01, True),
+    "annulus_2d_repro": (lambda: M.read_mesh(os.path.join(GOLDEN, "import", "annulus_v41.msh")),
+                         {"nu": 0.05, "u_bc": [(0, 0, 0), (0, 0, 0)], "p_bc": [None, None], "ddt_phi_coeff": 0.5},
+                         2, 0.02, True),
+    "box8_lid_default": (lambda: M.box(8, 8, 8),
+                         {"nu": 0.01, "u_bc": [(0, 0, 0)] * 5 + [(1, 0, 0)], "p_bc": [None] * 6}, 2, 0.005, False),
+}
+
+
+def piso_inputs(m):
+    """Initial U (a smooth swirl about the cell centres, zero normal component in 2-D),
+    p (a linear ramp) and the face flux φ⁰ = U_f·S_f of that U (the oracle's phiHbyA
+    flux with zero-gradient boundaries), on the faces the reference's Mesh keeps."""
+    from oracle import oracle as O
+    C = m.geometry()["C"]
+    x, y, z = C[:, 0], C[:, 1], C[:, 2]
+    c = m.computational()
+    U0 = np.stack([np.sin(np.pi * y) * 0.5, -np.sin(np.pi * x) * 0.5, 0.0 * z if c["two_d"] else 0.2 * np.cos(x)],
+                  axis=1)
+    p0 = 0.1 * x - 0.05 * y
+    cv = CompView(m)
+    nb = cv.n_boundary
+    phi0 = O.fvc_flux(cv, U0, None, np.zeros(nb, np.int32), np.zeros((nb, 3)))
+    return np.ascontiguousarray(U0), np.ascontiguousarray(p0), phi0

@@ -97,7 +97,8 @@ EXPORTED = ("vol_controls_default", "vol_last_error", "vol_context_create", "vol
             "vol_fvm_H", "vol_fvm_flux", "vol_mesh_set_geometry", "vol_fvc_interpolate", "vol_fvc_grad", "vol_fvc_div",
             "vol_fvc_surface_integrate", "vol_fvc_flux", "vol_geometry", "vol_piso_rau_hbya",
             "vol_piso_hbya_boundary", "vol_piso_face_op", "vol_piso_ddt_phi_corr", "vol_piso_constrain_pressure",
-            "vol_fvc_reconstruct", "vol_piso_correct_U")
+            "vol_fvc_reconstruct", "vol_piso_correct_U", "vol_fvm_ddt",
+            "vol_mesh_set_empty_dir")
 
 
 def lib():
@@ -136,6 +137,8 @@ def lib():
         L.vol_assemble_div_upwind.argtypes = [P, P, P, P, P, P, P, P, I]
         L.vol_fvm_reset.argtypes = [P]
         L.vol_fvm_ddt_euler.argtypes = [P, I, D, P, P, I]
+        L.vol_fvm_ddt.argtypes = [P, I, D, D, D, P, P, P, I]
+        L.vol_mesh_set_empty_dir.argtypes = [P, I]
         L.vol_fvm_div_upwind.argtypes = [P, I, P, P, P, I]
         L.vol_fvm_laplacian.argtypes = [P, I, D, P, P, P, P, P, ctypes.POINTER(_NonOrth), I]
         L.vol_fvm_source.argtypes = [P, P, P, I]
@@ -337,6 +340,10 @@ class LduMesh:
         c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
         arrs = [c(V), c(Sf), c(magSf), c(w), c(d), None if skew is None else c(skew)]
         _check(lib().vol_mesh_set_geometry(self._h, *(_q(a) for a in arrs), int(bool(skewed)), VOL_HOST))
+
+    def set_empty_dir(self, empty_dir):
+        """2-D mesh: its empty direction (−1 for 3-D) for the gradients."""
+        _check(lib().vol_mesh_set_empty_dir(self._h, int(empty_dir)))
 
     def _nf_total(self):
         return self.n_faces + getattr(self, "n_boundary", 0)
@@ -565,6 +572,12 @@ class LduMatrix:
     def fvm_ddt_euler(self, sign, dt, V, psi0):
         _check(lib().vol_fvm_ddt_euler(self._h, int(sign), ctypes.c_double(dt), _p(_f64(V)), _p(_f64(psi0)),
                                        VOL_HOST))
+
+    def fvm_ddt(self, sign, coeffs, V, psi0, psi00=None):
+        """fvm::ddt with ddtCoeffs (c, c0, c00): diag += c·V, source += V·(c0·ψ⁰ − c00·ψ⁰⁰)."""
+        c, c0, c00 = (float(v) for v in coeffs)
+        _check(lib().vol_fvm_ddt(self._h, int(sign), c, c0, c00, _p(_f64(V)), _p(_f64(psi0)), _q(_f64n(psi00)),
+                                 VOL_HOST))
 
     def fvm_div_upwind(self, sign, flux, bc_fixed, bc_value=None):
         bcf = np.ascontiguousarray(bc_fixed, dtype=np.int32)

@@ -62,6 +62,7 @@ struct vol_mesh_s {
     // staging buffers for host pointers
     bool geomSet = false;
     int skewed = 0;
+    int emptyDir = -1;  // 2-D meshes (vol_mesh_set_empty_dir)
     DevBuf<double> gV, gSf, gMagSf, gW, gD, gSkew;
     DevBuf<double> fst[8], fout, fgls;
     DevBuf<int> fsti;
@@ -749,6 +750,7 @@ FvmArgs fvmArgs(vol_matrix A, int sign) {
     a.bOff = m->bOff.p;
     a.bList = m->bList.p;
     a.sign = double(sign);
+    a.emptyDir = m->emptyDir;
     a.diag = A->fdiag.p;
     a.upper = A->upper.p;
     a.source = A->fsrc.p;
@@ -1489,6 +1491,7 @@ FvcGeom fvcGeom(vol_mesh m) {
     G.d = m->gD.p;
     G.skew = m->gSkew.p;
     G.skewed = m->skewed;
+    G.emptyDir = m->emptyDir;
     return G;
 }
 
@@ -1870,6 +1873,14 @@ int vol_mesh_set_geometry(vol_mesh mesh, const double* V, const double* Sf, cons
     API_END
 }
 
+int vol_mesh_set_empty_dir(vol_mesh mesh, int emptyDir) {
+    API_BEGIN
+    if (!mesh) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null mesh");
+    if (emptyDir < -1 || emptyDir > 2) throw VolError(VOL_ERR_INVALID_ARGUMENT, "emptyDir must be -1, 0, 1 or 2");
+    mesh->emptyDir = emptyDir;
+    API_END
+}
+
 int vol_fvc_interpolate(vol_mesh mesh, int nComponents, const double* cellValues, const double* cellGrad,
                         const double* boundaryValues, double* faceValues, int location) {
     API_BEGIN
@@ -2118,9 +2129,29 @@ int vol_fvm_ddt_euler(vol_matrix A, int sign, double dt, const double* V, const 
     setDevice(A->mesh->ctx);
     const size_t n = A->mesh->n;
     FvmArgs a = fvmArgs(A, sign);
-    a.rDt = 1.0 / dt;  // Schemes.h:74
+    a.ddtC = a.ddtC0 = 1.0 / dt;  // Schemes.h:74,83
+    a.ddtC00 = 0.0;
+    a.V = stageK(A, 0, V, n, location);
+    a.psi0 = a.psi00 = stageK(A, 1, psi0, n, location);
+    fvmTerm(A, FVM_DDT, a, false);
+    API_END
+}
+
+int vol_fvm_ddt(vol_matrix A, int sign, double c, double c0, double c00, const double* V, const double* psi0,
+                const double* psi00, int location) {
+    API_BEGIN
+    if (!A) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null matrix");
+    fvmCheck(A, sign);
+    if (!V || !psi0) throw VolError(VOL_ERR_INVALID_ARGUMENT, "ddt needs V and psi0");
+    setDevice(A->mesh->ctx);
+    const size_t n = A->mesh->n;
+    FvmArgs a = fvmArgs(A, sign);
+    a.ddtC = c;
+    a.ddtC0 = c0;
+    a.ddtC00 = c00;
     a.V = stageK(A, 0, V, n, location);
     a.psi0 = stageK(A, 1, psi0, n, location);
+    a.psi00 = psi00 ? stageK(A, 2, psi00, n, location) : a.psi0;
     fvmTerm(A, FVM_DDT, a, false);
     API_END
 }

@@ -167,9 +167,9 @@ __global__ void k_grad_gauss(FvmArgs a) {
         gz += a.Sf[3 * f + 2] * pb;
     }
     const double r = 1.0 / a.V[c];
-    a.grad[3 * c] = gx * r;
-    a.grad[3 * c + 1] = gy * r;
-    a.grad[3 * c + 2] = gz * r;
+    a.grad[3 * c] = a.emptyDir == 0 ? 0.0 : gx * r;  // 2-D: empty component zeroed (Fvc.h:169-181)
+    a.grad[3 * c + 1] = a.emptyDir == 1 ? 0.0 : gy * r;
+    a.grad[3 * c + 2] = a.emptyDir == 2 ? 0.0 : gz * r;
 }
 
 // Off-diagonal part of a term and its addition (FvMatrix.h:232-239): the lower part
@@ -221,8 +221,8 @@ __global__ void k_fvm_cells(FvmArgs a) {
     double td = 0.0, ts = 0.0;
     if constexpr (KIND == FVM_DDT) {
         const double Vc = a.V[c];
-        td = 0.0 + a.rDt * Vc;                                     // Fvm.h:27
-        ts = 0.0 + Vc * (a.rDt * a.psi0[c] - 0.0 * a.psi0[c]);     // Fvm.h:28, c00 = 0
+        td = 0.0 + a.ddtC * Vc;                                              // Fvm.h:27
+        ts = 0.0 + Vc * (a.ddtC0 * a.psi0[c] - a.ddtC00 * a.psi00[c]);       // Fvm.h:28
     } else {
         const int slice = c / WAVE, lane = c % WAVE;
         const int s0 = a.S.sliceOff[slice], width = (a.S.sliceOff[slice + 1] - s0) / WAVE;

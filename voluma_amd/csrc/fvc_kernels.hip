@@ -168,6 +168,7 @@ __global__ void k_fvc_grad_ls(FvcGeom G, const double* c, const int* bFix, const
 #pragma unroll
             for (int b = 0; b < 3; ++b) dd[3 * a + b] += (de[a] * de[b]) * s;
     }
+    if (G.emptyDir >= 0) dd[4 * G.emptyDir] += 1.0;  // 2-D (Fvc.h:152)
     double Gi[9];
     inv3(dd, Gi);
     double g[3 * NC];
@@ -199,6 +200,9 @@ __global__ void k_fvc_grad_ls(FvcGeom G, const double* c, const int* bFix, const
         for (int q = 0; q < NC; ++q) dphi[q] = fixes ? bVal[NC * bf + q] - c[NC * cell + q] : 0.0;
         ls_add<NC>(Gi, vs, dphi, g);
     }
+    if (G.emptyDir >= 0)  // 2-D: derivatives along the empty direction zeroed (Fvc.h:169-181)
+#pragma unroll
+        for (int q = 0; q < NC; ++q) g[NC * G.emptyDir + q] = 0.0;
 #pragma unroll
     for (int k = 0; k < 3 * NC; ++k) out[3 * NC * cell + k] = g[k];
 }
@@ -258,7 +262,7 @@ __global__ void k_fvc_grad_gauss(FvcGeom G, const double* psi, const double* gls
     }
     const double r = 1.0 / G.V[cell];
 #pragma unroll
-    for (int q = 0; q < 3; ++q) out[3 * cell + q] = g[q] * r;
+    for (int q = 0; q < 3; ++q) out[3 * cell + q] = q == G.emptyDir ? 0.0 : g[q] * r;
 }
 
 // ---------------------------------------------------------------- div, surfaceIntegrate

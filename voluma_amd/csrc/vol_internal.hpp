@@ -299,9 +299,10 @@ struct FvmArgs {
     const int* bList = nullptr;
     double sign = 1.0;
     // term inputs (face arrays: internal then boundary faces)
-    double rDt = 0.0;                   // ddt: 1/dt
+    double ddtC = 0.0, ddtC0 = 0.0, ddtC00 = 0.0;  // ddt: ddtCoeffs c, c0, c00 (Schemes.h:68-86)
     const double* V = nullptr;          // ddt, non-orthogonal gradient: cell volumes
     const double* psi0 = nullptr;       // ddt: old-time field
+    const double* psi00 = nullptr;      // ddt: old-old-time field (psi0 when there is none)
     const double* flux = nullptr;       // div
     double gammaU = 1.0;                // laplacian: uniform γ unless gamma != nullptr
     const double* gamma = nullptr;
@@ -316,6 +317,7 @@ struct FvmArgs {
     const double* corr = nullptr;       // 3 per face
     int* anyCorr = nullptr;             // device flag: some internal |corr|² > 1e-20
     double* grad = nullptr;             // 3 per cell (scratch)
+    int emptyDir = -1;                  // 2-D: the gradient's empty component is zeroed (Fvc.h:169-181)
     double* fc = nullptr;               // per internal face (scratch)
     // FvMatrix parts (in/out)
     double* diag = nullptr;
@@ -364,6 +366,7 @@ struct FvcGeom {
     const int* nei = nullptr;
     const double *V = nullptr, *Sf = nullptr, *magSf = nullptr, *w = nullptr, *d = nullptr, *skew = nullptr;
     int skewed = 0;
+    int emptyDir = -1;  // 2-D meshes: the empty direction (gradients: 1 on its LS diagonal, component zeroed)
 };
 // fvc::interpolate (Fvc.h:13-31,72-81); grad nullable (skew correction when G.skewed);
 // bval nullable (boundary faces take the owner's value)
