@@ -194,6 +194,74 @@ def corrector_ops_leg(va, W, ctx, m, reps=20):
     return out
 
 
+def piso_step_leg(va, W, ctx, m, args, cpu):
+    """Whole incompressible time steps resident on the GPU (vol_flow: the reference's
+    IncompressibleFlow::step, upwind convection, Euler, 2 pressure correctors) on the
+    benchmark mesh as a lid-driven cavity (zMax moves at (1, 0, 0), p zero-gradient with a
+    reference cell), from the synthetic divergence-free velocity of the momentum inputs:
+    ms per step with the reference's solvers (momentum PBiCGStab + DILU, pressure PCG +
+    GAMG smoothed by Gauss–Seidel) and with the pressure's GAMG smoothed by damped Jacobi.
+    With cpu: the reference's own step on the CPU on tet35 beside the GPU's there."""
+    from voluma_amd.piso import PisoFlow
+
+    def flow_for(mesh, inp, p_controls=None, p_final=None):
+        f = PisoFlow(ctx, mesh, inp["nu"], [(0, 0, 0)] * 5 + [(1, 0, 0)], [None] * 6, ddt="Euler",
+                     p_controls=p_controls, p_final_controls=p_final)
+        f.U, f.p, f.phi = inp["U0"], np.zeros(mesh.n_cells), inp["phi"]
+        return f
+
+    def timed(f, dt, steps):
+        f.step(dt)  # warm-up step: builds the solvers' level orders
+        ctx.synchronize()
+        t = time.perf_counter()
+        its = []
+        for _ in range(steps):
+            f.step(dt)
+            its.append([pf.iterations for pf in f.perf])
+        ctx.synchronize()
+        return (time.perf_counter() - t) / steps * 1e3, its[-1]
+
+    steps = 1 if args.profile_legs else 3
+    inp = W.momentum_inputs(m)
+    out = {"mesh": args.workload, "cells": m.n_cells, "dt": inp["dt"], "steps_timed": steps}
+    f = flow_for(m, inp)
+    out["reference_solvers_ms"], out["reference_solvers_iterations"] = timed(f, inp["dt"], steps)
+    f.close()
+    jac = va.SolverControls("PCG", "GAMG", 1e-8, 0.01, 2000, gamgPreSweeps=args.pre, gamgPostSweeps=args.post,
+                            gamgOmega=args.omega)
+    jac_final = va.SolverControls("PCG", "GAMG", 1e-8, 0.0, 2000, gamgPreSweeps=args.pre,
+                                  gamgPostSweeps=args.post, gamgOmega=args.omega)
+    f = flow_for(m, inp, jac, jac_final)
+    out["jacobi_gamg_ms"], out["jacobi_gamg_iterations"] = timed(f, inp["dt"], steps)
+    f.close()
+    log(f"PISO step ({args.workload}): {out['reference_solvers_ms']:.1f} ms with the reference's solvers, "
+        f"{out['jacobi_gamg_ms']:.1f} ms with Jacobi-smoothed GAMG")
+    if cpu and not args.profile_legs:
+        from oracle import ref
+        if ref.available():
+            ms = W.make_mesh("tet35")
+            inp35 = W.momentum_inputs(ms)
+            f = flow_for(ms, inp35)
+            gpu_ms, _ = timed(f, inp35["dt"], 2)
+            f.close()
+            f = flow_for(ms, inp35, jac, jac_final)
+            gpu_jac_ms, _ = timed(f, inp35["dt"], 2)
+            f.close()
+            rm = ref.RefMesh(ms)
+            t = time.perf_counter()
+            rm.piso_run(1, inp35["nu"], inp35["dt"], inp35["U0"], np.zeros(ms.n_cells), inp35["phi"],
+                        [(0, 0, 0)] * 5 + [(1, 0, 0)], [None] * 6)
+            cpu_ms = (time.perf_counter() - t) * 1e3
+            out["cpu_reference"] = {"mesh": "tet35", "cells": ms.n_cells, "cpu_ms_per_step": cpu_ms,
+                                    "gpu_ms_per_step": gpu_ms, "gpu_jacobi_gamg_ms_per_step": gpu_jac_ms,
+                                    "cores": 1, "kind": "reference",
+                                    "note": "the reference's IncompressibleFlow::step (serial build, first step "
+                                            "from the same fields) on the GPU host's CPU"}
+            log(f"PISO step tet35: reference CPU {cpu_ms:.0f} ms, GPU {gpu_ms:.1f} ms (reference solvers), "
+                f"{gpu_jac_ms:.1f} ms (Jacobi-smoothed GAMG)")
+    return out
+
+
 def reference_solvers_leg(va, W, ctx, m, A, db, dx, dzero, args, cpu):
     """The reference's own solver choices on the GPU: GAMG-PCG with its Gauss–Seidel
     smoother (2/2 sweeps) on the pressure system, and its default momentum solver
@@ -260,6 +328,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-ref-solvers", action="store_true", help="skip the GS-GAMG and DILU legs")
     ap.add_argument("--no-corrector-ops", action="store_true", help="skip the fvc operator timings")
+    ap.add_argument("--no-piso", action="store_true", help="skip the whole PISO time-step leg")
     ap.add_argument("--profile-legs", action="store_true",
                     help="kernel-trace aid: one short GS-GAMG (3 iterations) and DILU solve, no warm-up")
     ap.add_argument("--serve-rate", type=float, default=40.0, help="request arrivals per second")
@@ -383,6 +452,9 @@ def main():
     corrector_ops = None
     if not args.no_corrector_ops:
         corrector_ops = corrector_ops_leg(va, W, ctx, m, 2 if args.profile_legs else 20)
+    piso = None
+    if not args.no_piso:
+        piso = piso_step_leg(va, W, ctx, m, args, not args.no_cpu_baseline)
     ref_solvers = None
     if not args.no_ref_solvers:
         ref_solvers = reference_solvers_leg(va, W, ctx, m, A, db, dx, dzero, args, not args.no_cpu_baseline)
@@ -410,6 +482,7 @@ def main():
         "serve": serve,
         "reference_solvers": ref_solvers,
         "corrector_ops": corrector_ops,
+        "piso_step": piso,
         "iterations": int(np.median(iters)),
         "final_residual": p.finalResidual,
         "true_residual": true_res,

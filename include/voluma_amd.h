@@ -359,6 +359,61 @@ int vol_fvc_reconstruct(vol_mesh mesh, const double* faceFlux, int emptyDir, dou
 int vol_piso_correct_U(vol_mesh mesh, const double* HbyA, const double* rAU, const double* v, int sign, int emptyDir,
                        double* U, int location);
 
+/* boundary values of a field with fixedValue (bFixed 1: bValue) or zeroGradient (the
+ * owner cell's value) faces, as VolField::correctBoundaryConditions evaluates them
+ * (VolField.h:89-96, BoundaryCondition.h:98-138); nComponents 1 or 3 (interleaved) */
+int vol_boundary_values(vol_mesh mesh, int nComponents, const int32_t* bFixed, const double* bValue,
+                        const double* cellValues, double* out, int location);
+/* device-side strided copy dst[i·dstStride] = scale·src[i·srcStride] (scale ±1 copies or
+ * negates exactly), e.g. a component of an interleaved vector field (device pointers;
+ * no reference counterpart) */
+int vol_copy_strided(vol_context ctx, int64_t n, const double* src, int srcStride, double* dst, int dstStride,
+                     double scale);
+
+/* ---- incompressible flow time steps resident on the GPU (IncompressibleFlow.{h,cpp}).
+ * vol_flow holds U (interleaved Vec3), p and the face flux φ of a mesh on the device with
+ * their old time levels, the momentum and pressure matrices and scratch, and advances
+ * them by IncompressibleFlow::step (IncompressibleFlow.cpp:144-370) for laminar flow with
+ * upwind convection, fixedValue / zeroGradient boundaries and no body force: momentum
+ * assembly and predictor, nCorr pressure correctors (A, H, rAU/HbyA, rAUf, phiHbyA,
+ * ddtPhiCorr, pressure equation with nNonOrthCorr correctors and a reference cell when no
+ * patch fixes p, φ update, velocity correction). Nothing leaves the device during a step
+ * but the solvers' scalars. The mesh needs vol_mesh_set_boundary, vol_mesh_set_geometry
+ * (and vol_mesh_set_empty_dir in 2-D), GAMG weights when a GAMG solve is asked for; the
+ * boundary faces are the kept faces of the reference's Mesh (empty patches dropped). */
+typedef struct vol_flow_s* vol_flow;
+#define VOL_DDT_EULER 1
+#define VOL_DDT_BACKWARD 2
+typedef struct {
+    double nu;                  /* kinematic viscosity (uniform ν_eff) */
+    int nCorr;                  /* PimpleControls::nCorr (2) */
+    int nNonOrthCorr;           /* (0) */
+    int momentumPredictor;      /* (1) */
+    int ddtScheme;              /* VOL_DDT_EULER / VOL_DDT_BACKWARD (IncompressibleFlow.h:75) */
+    int gradPScheme;            /* VOL_GRAD_* for ∇p (Gauss linear) */
+    int laplacianNonOrthCorr;   /* (1) */
+    int64_t pRefCell;           /* (0) */
+    double pRefValue;           /* (0) */
+    double ddtPhiCoeff;         /* (0: off; < 0 automatic) */
+    vol_solver_controls UControls, pControls, pFinalControls;
+} vol_flow_options;
+/* PimpleControls' defaults (IncompressibleFlow.h:18-36) with the GPU's controls: U
+ * PBiCGStab + DILU 1e-8; p PCG + GAMG 1e-8 relTol 0.01 and final 1e-8, GAMG smoothed by
+ * Gauss-Seidel (the reference's smoother) */
+void vol_flow_options_default(vol_flow_options* o);
+/* faces: nInternalFaces + nBoundaryFaces of ndc and corrVecs (3 per face, interleaved);
+ * per boundary face: uFixed / uValue (3 per face) and pFixed / pValue */
+int vol_flow_create(vol_mesh mesh, const vol_flow_options* options, const double* ndc, const double* corrVecs,
+                    const int32_t* uFixed, const double* uValue, const int32_t* pFixed, const double* pValue,
+                    int location, vol_flow* out);
+/* U (nCells × 3), p (nCells), phi (faces); each nullable (left as it is) */
+int vol_flow_set_fields(vol_flow flow, const double* U, const double* p, const double* phi, int location);
+int vol_flow_get_fields(vol_flow flow, double* U, double* p, double* phi, int location);
+/* one time step; perf (nullable) receives up to maxPerf solver performances in solve
+ * order, *nPerf (nullable) their number */
+int vol_flow_step(vol_flow flow, double dt, vol_solver_performance* perf, int maxPerf, int* nPerf);
+int vol_flow_destroy(vol_flow flow);
+
 /* ---- batches: many independent systems solved together (no reference counterpart).
  * Systems are concatenated: owner/neighbour/gamgWeights hold each system's internal
  * faces in turn with that system's own (0-based) cell numbers; diag/upper/lower, b and

@@ -58,3 +58,31 @@ def test_piso_steps_default_solvers_match_reference(ctx):
         scale = np.abs(ref[k]).max()
         assert np.abs(got - ref[k]).max() <= 1e-6 * scale, k
     assert all(pf.converged for pf in flow.perf)
+
+
+def test_boundary_values_and_flow_errors(ctx):
+    """vol_boundary_values: fixed faces take their value, the others their owner cell's;
+    vol_flow rejects bad options and steps."""
+    import ctypes
+    from helpers import FVC_MESHES
+    m = FVC_MESHES["tet5x4x3"]()
+    ni, nb = m.n_internal, m.n_boundary
+    lm = va.LduMesh(ctx, m.n_cells, m.owner[:ni], m.neighbour, None)
+    lm.set_boundary(m.owner[ni:])
+    rng = np.random.default_rng(2)
+    fix = (np.arange(nb) % 2).astype(np.int32)
+    for nc in (1, 3):
+        c = rng.uniform(-1, 1, (m.n_cells, nc)).squeeze()
+        bv = rng.uniform(-1, 1, (nb, nc)).squeeze()
+        got = lm.boundary_values(fix, bv, c)
+        exp = np.where((fix == 1) if nc == 1 else (fix == 1)[:, None], bv, c[m.owner[ni:]])
+        assert got.tobytes() == np.ascontiguousarray(exp).tobytes()
+    from voluma_amd.piso import PisoFlow
+    with pytest.raises(ValueError):
+        PisoFlow(ctx, m, 0.01, [None] * 5, [None] * 6)  # one entry per patch
+    with pytest.raises(va.VolumaError):
+        PisoFlow(ctx, m, 0.01, [None] * 6, [None] * 6, n_corr=0)
+    f = PisoFlow(ctx, m, 0.01, [(0, 0, 0)] * 6, [None] * 6)
+    with pytest.raises(va.VolumaError):
+        f.step(0.0)
+    f.close()

@@ -30,6 +30,7 @@ FP64, MIXED = 0, 1
 GAMG_JACOBI, GAMG_GAUSS_SEIDEL = 0, 1
 GRAD_GAUSS_LINEAR, GRAD_LEAST_SQUARES, GRAD_GAUSS_SKEW = 0, 1, 2
 FACE_A_PLUS_B_TIMES_C, FACE_A_MINUS_B, FACE_A_MINUS_B_OVER_C = 0, 1, 2
+DDT_EULER, DDT_BACKWARD = 1, 2
 
 _ERRORS = {
     -1: "invalid argument", -2: "unknown solver", -3: "unknown preconditioner", -4: "PCG on nonsymmetric matrix",
@@ -56,6 +57,14 @@ class _Controls(ctypes.Structure):
 class _NonOrth(ctypes.Structure):
     _fields_ = [("psi", ctypes.c_void_p), ("w", ctypes.c_void_p), ("Sf", ctypes.c_void_p), ("corrVecs", ctypes.c_void_p),
                 ("V", ctypes.c_void_p)]
+
+
+class _FlowOptions(ctypes.Structure):
+    _fields_ = [("nu", ctypes.c_double), ("nCorr", ctypes.c_int), ("nNonOrthCorr", ctypes.c_int),
+                ("momentumPredictor", ctypes.c_int), ("ddtScheme", ctypes.c_int), ("gradPScheme", ctypes.c_int),
+                ("laplacianNonOrthCorr", ctypes.c_int), ("pRefCell", ctypes.c_int64), ("pRefValue", ctypes.c_double),
+                ("ddtPhiCoeff", ctypes.c_double), ("UControls", _Controls), ("pControls", _Controls),
+                ("pFinalControls", _Controls)]
 
 
 class _Perf(ctypes.Structure):
@@ -98,7 +107,8 @@ EXPORTED = ("vol_controls_default", "vol_last_error", "vol_context_create", "vol
             "vol_fvc_surface_integrate", "vol_fvc_flux", "vol_geometry", "vol_piso_rau_hbya",
             "vol_piso_hbya_boundary", "vol_piso_face_op", "vol_piso_ddt_phi_corr", "vol_piso_constrain_pressure",
             "vol_fvc_reconstruct", "vol_piso_correct_U", "vol_fvm_ddt",
-            "vol_mesh_set_empty_dir")
+            "vol_mesh_set_empty_dir", "vol_boundary_values", "vol_copy_strided", "vol_flow_options_default",
+            "vol_flow_create", "vol_flow_set_fields", "vol_flow_get_fields", "vol_flow_step", "vol_flow_destroy")
 
 
 def lib():
@@ -139,6 +149,15 @@ def lib():
         L.vol_fvm_ddt_euler.argtypes = [P, I, D, P, P, I]
         L.vol_fvm_ddt.argtypes = [P, I, D, D, D, P, P, P, I]
         L.vol_mesh_set_empty_dir.argtypes = [P, I]
+        L.vol_boundary_values.argtypes = [P, I, P, P, P, P, I]
+        L.vol_copy_strided.argtypes = [P, ctypes.c_int64, P, I, P, I, D]
+        L.vol_flow_options_default.argtypes = [ctypes.POINTER(_FlowOptions)]
+        L.vol_flow_options_default.restype = None
+        L.vol_flow_create.argtypes = [P, ctypes.POINTER(_FlowOptions), P, P, P, P, P, P, I, ctypes.POINTER(P)]
+        L.vol_flow_set_fields.argtypes = [P, P, P, P, I]
+        L.vol_flow_get_fields.argtypes = [P, P, P, P, I]
+        L.vol_flow_step.argtypes = [P, D, P, I, ctypes.POINTER(I)]
+        L.vol_flow_destroy.argtypes = [P]
         L.vol_fvm_div_upwind.argtypes = [P, I, P, P, P, I]
         L.vol_fvm_laplacian.argtypes = [P, I, D, P, P, P, P, P, ctypes.POINTER(_NonOrth), I]
         L.vol_fvm_source.argtypes = [P, P, P, I]
@@ -344,6 +363,17 @@ class LduMesh:
     def set_empty_dir(self, empty_dir):
         """2-D mesh: its empty direction (−1 for 3-D) for the gradients."""
         _check(lib().vol_mesh_set_empty_dir(self._h, int(empty_dir)))
+
+    def boundary_values(self, b_fixed, b_value, c):
+        """fixedValue (b_fixed 1: b_value) / zeroGradient (the owner's value) boundary values."""
+        c = np.ascontiguousarray(c, dtype=np.float64)
+        nc = 1 if c.ndim == 1 else 3
+        out = np.zeros((max(self.n_boundary, 1), nc))
+        _check(lib().vol_boundary_values(self._h, nc, _p(np.ascontiguousarray(b_fixed, dtype=np.int32)),
+                                         _p(np.ascontiguousarray(b_value, dtype=np.float64)), _p(c), _p(out),
+                                         VOL_HOST))
+        out = out[: self.n_boundary]
+        return out[:, 0] if nc == 1 else out
 
     def _nf_total(self):
         return self.n_faces + getattr(self, "n_boundary", 0)

@@ -1521,6 +1521,26 @@ void fvcComponents(int nc) {
 }
 }  // namespace
 
+namespace vol::host {
+vol_context meshContext(vol_mesh m) { return m->ctx; }
+void meshSizes(vol_mesh m, int& n, int& nInternal, int& nBoundary) {
+    n = m->n;
+    nInternal = m->nf;
+    nBoundary = m->nb;
+}
+bool meshHasGamg(vol_mesh m) { return m->hasGamg; }
+void meshGeometry(vol_mesh m, const double*& V, const double*& Sf, const double*& magSf, const double*& w,
+                  int& skewed, int& emptyDir) {
+    if (!m->geomSet) throw VolError(VOL_ERR_INVALID_ARGUMENT, "mesh geometry not set (vol_mesh_set_geometry)");
+    V = m->gV.p;
+    Sf = m->gSf.p;
+    magSf = m->gMagSf.p;
+    w = m->gW.p;
+    skewed = m->skewed;
+    emptyDir = m->emptyDir;
+}
+}  // namespace vol::host
+
 extern "C" {
 
 void vol_controls_default(vol_solver_controls* c) {
@@ -1870,6 +1890,39 @@ int vol_mesh_set_geometry(vol_mesh mesh, const double* V, const double* Sf, cons
     mesh->skewed = skewed ? 1 : 0;
     mesh->geomSet = true;
     HIPCHECK(hipStreamSynchronize(st));
+    API_END
+}
+
+int vol_boundary_values(vol_mesh mesh, int nComponents, const int32_t* bFixed, const double* bValue,
+                        const double* cellValues, double* out, int location) {
+    API_BEGIN
+    if (!mesh || !cellValues || !out) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    fvcComponents(nComponents);
+    if (mesh->nb < 0) throw VolError(VOL_ERR_INVALID_ARGUMENT, "mesh boundary not set (vol_mesh_set_boundary)");
+    if (mesh->nb > 0 && (!bFixed || !bValue)) throw VolError(VOL_ERR_INVALID_ARGUMENT, "bFixed and bValue required");
+    setDevice(mesh->ctx);
+    FvcGeom G;
+    G.n = mesh->n;
+    G.nf = mesh->nf;
+    G.nb = mesh->nb;
+    G.bOff = mesh->bOff.p;
+    G.bList = mesh->bList.p;
+    const size_t nc = nComponents, nb = mesh->nb;
+    launch_boundary_values(mesh->ctx->stream, G, nComponents, fvcInI(mesh, bFixed, nb, location),
+                           fvcIn(mesh, 0, bValue, nc * nb, location), fvcIn(mesh, 1, cellValues, nc * G.n, location),
+                           fvcOutBuf(mesh, out, nc * nb, location));
+    fvcOut(mesh, out, nc * nb, location);
+    API_END
+}
+
+int vol_copy_strided(vol_context ctx, int64_t n, const double* src, int srcStride, double* dst, int dstStride,
+                     double scale) {
+    API_BEGIN
+    if (!ctx || (n > 0 && (!src || !dst))) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null argument");
+    if (n < 0 || srcStride < 1 || dstStride < 1) throw VolError(VOL_ERR_INVALID_ARGUMENT, "bad count or stride");
+    setDevice(ctx);
+    launch_copy_strided(ctx->stream, n, src, srcStride, dst, dstStride, scale);
+    HIPCHECK(hipGetLastError());
     API_END
 }
 

@@ -606,6 +606,14 @@ inline GatherLists buildGatherLists(int nFine, int nCoarse, const std::vector<in
     return G;
 }
 
+// mesh accessors for translation units that do not see vol_mesh_s (api.cpp)
+vol_context meshContext(vol_mesh m);
+void meshSizes(vol_mesh m, int& n, int& nInternal, int& nBoundary);
+bool meshHasGamg(vol_mesh m);
+// the geometry vol_mesh_set_geometry stored (device pointers owned by the mesh)
+void meshGeometry(vol_mesh m, const double*& V, const double*& Sf, const double*& magSf, const double*& w,
+                  int& skewed, int& emptyDir);
+
 inline int fail(const VolError& e) {
     setLastError(e.what());
     return e.code;

@@ -168,7 +168,37 @@ __global__ void k_piso_correct_U(int n, const double* HbyA, const double* rAU, c
     }
 }
 
+// fixedValue / zeroGradient boundary values (correctBoundaryConditions): per cell over its
+// boundary faces, out = bValue on fixed faces, the cell's value elsewhere
+template <int NC>
+__global__ void k_boundary_values(FvcGeom G, const int* bFix, const double* bValue, const double* c, double* out) {
+    const int cell = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cell >= G.n) return;
+    for (int k = G.bOff[cell]; k < G.bOff[cell + 1]; ++k) {
+        const int bf = G.bList[k];
+#pragma unroll
+        for (int q = 0; q < NC; ++q) out[NC * bf + q] = bFix[bf] ? bValue[NC * bf + q] : c[NC * cell + q];
+    }
+}
+
+// dst[i·ds] = scale·src[i·ss] (scale ±1: a copy or an exact negation)
+__global__ void k_copy_strided(long long n, const double* src, int ss, double* dst, int ds, double scale) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i * ds] = scale * src[i * ss];
+}
+
 // ---------------------------------------------------------------- launchers
+void launch_boundary_values(hipStream_t st, const FvcGeom& G, int nc, const int* bFix, const double* bValue,
+                            const double* c, double* out) {
+    if (G.n <= 0 || G.nb <= 0) return;
+    if (nc == 1)
+        hipLaunchKernelGGL(k_boundary_values<1>, piso_grid(G.n), dim3(PISO_THREADS), 0, st, G, bFix, bValue, c, out);
+    else
+        hipLaunchKernelGGL(k_boundary_values<3>, piso_grid(G.n), dim3(PISO_THREADS), 0, st, G, bFix, bValue, c, out);
+}
+void launch_copy_strided(hipStream_t st, long long n, const double* src, int ss, double* dst, int ds, double scale) {
+    if (n > 0) hipLaunchKernelGGL(k_copy_strided, piso_grid(n), dim3(PISO_THREADS), 0, st, n, src, ss, dst, ds, scale);
+}
 void launch_piso_rau_hbya(hipStream_t st, int n, const double* A, const double* H, double* rAU, double* HbyA) {
     if (n > 0) hipLaunchKernelGGL(k_piso_rau_hbya, piso_grid(n), dim3(PISO_THREADS), 0, st, n, A, H, rAU, HbyA);
 }

@@ -397,6 +397,9 @@ void launch_piso_constrain_pressure(hipStream_t st, const FvcGeom& G, int b0, in
 void launch_fvc_reconstruct(hipStream_t st, const FvcGeom& G, const double* ssf, int emptyDir, double* out);
 void launch_piso_correct_U(hipStream_t st, int n, const double* HbyA, const double* rAU, const double* v, int sign,
                            int emptyDir, double* U);
+void launch_boundary_values(hipStream_t st, const FvcGeom& G, int nc, const int* bFix, const double* bValue,
+                            const double* c, double* out);
+void launch_copy_strided(hipStream_t st, long long n, const double* src, int ss, double* dst, int ds, double scale);
 
 // ------------------------------------------------------------------ level-scheduled sweeps
 // Lower / upper neighbour lists of the cells (device arrays). Lower list of cell u: neighbours l < u in ascending (l, face) order;
