@@ -16,7 +16,9 @@
 #include <cstdlib>
 #include <cstring>
 #include <memory>
+#include <map>
 #include <mutex>
+#include <tuple>
 #include <stdexcept>
 #include <string>
 #include <type_traits>
@@ -77,6 +79,9 @@ struct vol_mesh_s {
     };
     bool gsLevelsReady = false;
     std::vector<std::unique_ptr<LevelOrder>> gsLev;
+    // k consecutive Gauss–Seidel sweeps of one direction as one level schedule, per
+    // (GAMG level, backward, k); built on first use
+    std::map<std::tuple<int, int, int>, std::unique_ptr<LevelEllDev>> gsMulti;
     SweepView sweepView() const {
         SweepView v;
         v.lowOff = lowOff.p;
@@ -121,6 +126,7 @@ struct vol_matrix_s {
     // and the coarse levels (smoother), FP32 copies for mixed precision
     bool gsReady = false, gsLevReady = false, gs32Ready = false;
     DevBuf<double> gsCF, gsCB;
+    std::map<std::tuple<int, int, int>, DevBuf<double>> gsMultiCoef;  // per gsMulti schedule
     DevBuf<float> gsCF32, gsCB32;
     DevBuf<float> coef32, diag32, rD32;
     DevBuf<TailLevel32> tail32;
@@ -549,6 +555,110 @@ void buildGs(vol_matrix A, bool gamgLevels) {
     }
 }
 
+// k consecutive Gauss–Seidel sweeps over a graph (edges l[e] < u[e]) as one level
+// schedule. A forward sweep updates cells in ascending order, each reading its lower
+// neighbours' values of this sweep and its upper neighbours' of the previous one; so
+// (c, s) follows (j, s) for lower neighbours j, (j, s − 1) for upper ones and (c, s − 1):
+//   L(c, s) = 1 + max(L(j, s) over lower j, L(j, s − 1) over upper j, L(c, s − 1)).
+// Every read then sees the value the sequential sweeps would see and every write comes
+// after the reads of the value it replaces (a lower neighbour's (j, s) precedes (c, s),
+// an upper neighbour's (j, s − 1) precedes (c, s), and (c, s) precedes the upper
+// neighbours' (j, s)), so the sweeps give the sequential loop's bits with far fewer
+// levels than k separate sweeps (the second sweep adds only a few levels to the first).
+// Backward sweeps mirror it (descending order, upper neighbours updated first). Levels
+// list their cells (one entry per (cell, sweep)) in ascending cell order.
+void multiSweepOrder(int n, const std::vector<int>& l, const std::vector<int>& u, bool backward, int k,
+                     std::vector<int>& off, std::vector<int>& perm) {
+    const int ne = int(l.size());
+    std::vector<int> bo(n + 1, 0), ao(n + 1, 0), bc(ne), ac(ne);  // before / after neighbours
+    auto before = [&](int e, int& c, int& j) {
+        c = backward ? l[e] : u[e];
+        j = backward ? u[e] : l[e];
+    };
+    for (int e = 0; e < ne; ++e) {
+        int c, j;
+        before(e, c, j);
+        ++bo[c + 1];
+        ++ao[j + 1];
+    }
+    for (int c = 0; c < n; ++c) {
+        bo[c + 1] += bo[c];
+        ao[c + 1] += ao[c];
+    }
+    {
+        std::vector<int> fb(bo.begin(), bo.end() - 1), fa(ao.begin(), ao.end() - 1);
+        for (int e = 0; e < ne; ++e) {
+            int c, j;
+            before(e, c, j);
+            bc[fb[c]++] = j;
+            ac[fa[j]++] = c;
+        }
+    }
+    std::vector<int> prev(n, -1), cur(n, 0), lev;  // lev[s·n + c]
+    lev.resize(size_t(k) * n);
+    for (int s = 0; s < k; ++s) {
+        for (int t = 0; t < n; ++t) {
+            const int c = backward ? n - 1 - t : t;
+            int L = prev[c] + 1;
+            for (int q = bo[c]; q < bo[c + 1]; ++q) L = std::max(L, cur[bc[q]] + 1);
+            for (int q = ao[c]; q < ao[c + 1]; ++q) L = std::max(L, prev[ac[q]] + 1);
+            cur[c] = L;
+        }
+        std::copy(cur.begin(), cur.end(), lev.begin() + size_t(s) * n);
+        prev = cur;
+    }
+    const int nl = n ? *std::max_element(lev.begin(), lev.end()) + 1 : 0;
+    off.assign(nl + 1, 0);
+    for (int v : lev) ++off[v + 1];
+    for (int L = 0; L < nl; ++L) off[L + 1] += off[L];
+    std::vector<int> fill(off.begin(), off.end() - 1);
+    perm.assign(lev.size(), 0);
+    for (int c = 0; c < n; ++c)  // a cell's sweeps fall in distinct levels
+        for (int s = 0; s < k; ++s) perm[fill[lev[size_t(s) * n + c]]++] = c;
+}
+
+// The k-sweep schedule of GAMG level lvl (0: the mesh) and direction, built on first use.
+const LevelEllDev& gsMultiEll(vol_mesh m, int lvl, bool backward, int k) {
+    std::lock_guard<std::mutex> guard(m->sweepMu);
+    auto& slot = m->gsMulti[{lvl, backward ? 1 : 0, k}];
+    if (slot) return *slot;
+    std::vector<int> l, u;
+    int n;
+    if (lvl == 0) {
+        n = m->n;
+        l.resize(m->nf);
+        u.resize(m->nf);
+        for (int f = 0; f < m->nf; ++f) {
+            l[f] = std::min(m->own[f], m->nei[f]);
+            u[f] = std::max(m->own[f], m->nei[f]);
+        }
+    } else {
+        n = m->H[lvl].n;
+        l = m->H[lvl].l;
+        u = m->H[lvl].u;
+    }
+    std::vector<int> off, perm, ro, rc, rs;
+    multiSweepOrder(n, l, u, backward, k, off, perm);
+    sellRows(lvl == 0 ? m->sell : m->lv[lvl]->sell, ro, rc, rs);
+    auto e = std::make_unique<LevelEllDev>();
+    e->build(off, perm, ro, rc, rs, m->ctx->stream);
+    slot = std::move(e);
+    return *slot;
+}
+
+// The matrix's coefficients in that schedule's layout (level coefficients gathered by the
+// schedule's entry sources), built after each change of the matrix.
+const double* gsMultiCoef(vol_matrix A, int lvl, bool backward, int k) {
+    const LevelEllDev& E = gsMultiEll(A->mesh, lvl, backward, k);
+    auto& buf = A->gsMultiCoef[{lvl, backward ? 1 : 0, k}];
+    if (!buf.p) {
+        buf.alloc(std::max<size_t>(E.nEntries, 1));
+        launch_gather_entries<double>(A->mesh->ctx->stream, E.nEntries, E.src.p,
+                                      lvl == 0 ? A->coef.p : A->L[lvl]->coef.p, buf.p);
+    }
+    return buf.p;
+}
+
 void buildGs32(vol_matrix A) {
     vol_mesh m = A->mesh;
     hipStream_t st = m->ctx->stream;
@@ -655,6 +765,7 @@ void matrixFinish(vol_matrix A) {
     A->gsReady = false;
     A->gsLevReady = false;
     A->gs32Ready = false;
+    A->gsMultiCoef.clear();
     A->set = true;
 }
 
@@ -1118,14 +1229,32 @@ template <class T> void vcycleGS(const Ctx& C, vol_matrix A, const Plan& P, int 
         cB = l == 0 ? A->gsCB.p : A->L[l]->gsCB.p;
     }
     launch_fill_zero<T>(C.st, n, xout, C.done);
-    for (int s = 0; s < P.nPre; ++s) launch_gs_sweep<T>(C.st, sF, cF, L.diag, r, xout, C.done);
+    if constexpr (std::is_same_v<T, double>) {  // the sweeps as one pipelined schedule
+        if (P.nPre > 1) {
+            launch_gs_sweep<T>(C.st, gsMultiEll(m, l, false, P.nPre).view(), gsMultiCoef(A, l, false, P.nPre),
+                               L.diag, r, xout, C.done);
+        } else {
+            for (int s = 0; s < P.nPre; ++s) launch_gs_sweep<T>(C.st, sF, cF, L.diag, r, xout, C.done);
+        }
+    } else {
+        for (int s = 0; s < P.nPre; ++s) launch_gs_sweep<T>(C.st, sF, cF, L.diag, r, xout, C.done);
+    }
     sell<T>(C, OP_RESID, XL_PLAIN, V, L.coef, L.diag, xout, r, L.bufA, P.omega, nullptr, nullptr, nullptr, nullptr,
             nullptr, L.rD);
     const LevelPtrs<T> LC = levelPtrs(A, l + 1, (T*)nullptr);
     restrictLevel(C, m->lv[l + 1]->n, D, L.bufA, LC.r);
     vcycleGS<T>(C, A, P, l + 1, LC.r, LC.x);
     launch_prolong<T>(C.st, n, D.agg.p, LC.x, xout, C.done);
-    for (int s = 0; s < P.nPost; ++s) launch_gs_sweep<T>(C.st, sB, cB, L.diag, r, xout, C.done);
+    if constexpr (std::is_same_v<T, double>) {
+        if (P.nPost > 1) {
+            launch_gs_sweep<T>(C.st, gsMultiEll(m, l, true, P.nPost).view(), gsMultiCoef(A, l, true, P.nPost),
+                               L.diag, r, xout, C.done);
+        } else {
+            for (int s = 0; s < P.nPost; ++s) launch_gs_sweep<T>(C.st, sB, cB, L.diag, r, xout, C.done);
+        }
+    } else {
+        for (int s = 0; s < P.nPost; ++s) launch_gs_sweep<T>(C.st, sB, cB, L.diag, r, xout, C.done);
+    }
 }
 
 template <class T>
