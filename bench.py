@@ -38,15 +38,30 @@ def log(msg):
 
 
 def cpu_baseline(m, b, tol, budget_s=30.0):
-    """Oracle (C port of the reference solver, 1 thread) on the same system: one
-    GAMG-PCG solve (Gauss–Seidel smoothing as the reference)."""
+    """The same system's GAMG-PCG solve (Gauss–Seidel smoothing) on one host core: the
+    reference's own solveLdu (its serial build, oracle/_ref) when present, otherwise the
+    oracle's C port of it."""
     from oracle import oracle as O
     from voluma_amd import workloads as W
 
     ni = m.n_internal
-    om = O.LduMesh(m.n_cells, m.owner[:ni], m.neighbour, m.agglomeration_weights())
     d, up, ic, bs = O.laplacian(m, W.bc_fixed_xmax(m))
     d, _ = O.fold(m, d, np.zeros(m.n_cells), ic, bs)
+    try:
+        from oracle import ref
+        have_ref = ref.available()
+    except Exception:  # the reference build is optional
+        have_ref = False
+    if have_ref:
+        rm = ref.RefMesh(m)
+        t0 = time.perf_counter()
+        _, p = rm.solve(d, up, None, b, np.zeros(m.n_cells), solver="PCG", preconditioner="GAMG", tolerance=tol,
+                        max_iter=1000)
+        dt = time.perf_counter() - t0
+        return {"value": 1.0 / dt, "unit": "solves/s", "cores": 1, "kind": "reference",
+                "sample": f"1 full GAMG(GS)-PCG solve of the same system to {tol:g} by the reference's own "
+                          f"solveLdu (serial build), {p['iterations']} iterations, {dt:.1f} s"}
+    om = O.LduMesh(m.n_cells, m.owner[:ni], m.neighbour, m.agglomeration_weights())
     t0 = time.perf_counter()
     n_solves = 0
     iters = 0
