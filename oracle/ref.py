@@ -38,6 +38,8 @@ def _lib():
         lib.vr_sizes.argtypes = [P, P]
         lib.vr_patch.argtypes = [P, I, P, P]
         lib.vr_layout.argtypes = [P, P]
+        lib.vr_generate_box.argtypes = [I, I, I, I, I]
+        lib.vr_generate_box.restype = P
         lib.vr_piso_run.argtypes = [P, I, D, D] + [P] * 12
         lib.vr_piso_run.restype = I
         lib.vr_reconstruct.argtypes = [P, P, P]
@@ -62,6 +64,7 @@ def _lib():
             f.argtypes = [ctypes.c_char_p, D, D, ctypes.c_char_p]
             f.restype = P
         lib.vr_rawmesh_sizes.argtypes = [P, P]
+        lib.vr_rawmesh_set_points.argtypes = [P, P, ctypes.c_long]
         lib.vr_rawmesh_arrays.argtypes = [P] * 10
         lib.vr_rawmesh_build.argtypes = [P]
         lib.vr_rawmesh_build.restype = P
@@ -401,11 +404,22 @@ def read_mesh_file(path, scale=1.0, depth2d=0.1, patch_types=None):
                                               _overrides(patch_types)))
 
 
-def _raw_mesh(h):
+def generate_box(nx, ny, nz, periodic=(False, False, False), two_d=False, points=None):
+    """The reference's generateBox on the unit cube: (raw arrays dict, RefMesh built from
+    it). ``points`` (same count) replaces the box's point coordinates before the build."""
+    bits = sum(1 << d for d in range(3) if periodic[d])
+    return _raw_mesh(_lib().vr_generate_box(int(nx), int(ny), int(nz), bits, int(bool(two_d))), points)
+
+
+def _raw_mesh(h, points=None):
     lib = _lib()
     if not h:
         raise RuntimeError("reference: " + lib.vr_last_error().decode())
     try:
+        if points is not None:
+            pts = np.ascontiguousarray(points, dtype=np.float64)
+            if lib.vr_rawmesh_set_points(h, _p(pts), pts.shape[0]) != 0:
+                raise ValueError("point count differs from the mesh's")
         s = np.zeros(6, np.int64)
         lib.vr_rawmesh_sizes(h, _p(s))
         npts, nf, ni, nfp, nc, npatch = (int(v) for v in s)

@@ -380,6 +380,28 @@ void* vr_read_fluent(const char* file, double scale, double depth2D, const char*
 void* vr_read_mesh_file(const char* path, double scale, double depth2D, const char* patchTypes) {
     return readWith([&](const MeshImportOptions& o) { return readMeshFile(path, o); }, scale, depth2D, patchTypes);
 }
+// generateBox (RawMesh.cpp:391-502) on the unit cube: periodic bits 1 x, 2 y, 4 z; twoD
+void* vr_generate_box(int nx, int ny, int nz, int periodic, int twoD) {
+    try {
+        BoxSpec b;
+        b.n[0] = nx;
+        b.n[1] = ny;
+        b.n[2] = nz;
+        for (int d = 0; d < 3; ++d) b.periodic[d] = (periodic >> d & 1) != 0;
+        b.twoD = twoD != 0;
+        return new RawMesh(generateBox(b));
+    } catch (const std::exception& e) {
+        fail(e);
+        return nullptr;
+    }
+}
+// replace the point coordinates of a RawMesh (same count) before it is built
+int vr_rawmesh_set_points(void* h, const double* pts, long nPoints) {
+    RawMesh& r = *static_cast<RawMesh*>(h);
+    if (nPoints != long(r.points.size())) return -1;
+    for (long i = 0; i < nPoints; ++i) r.points[i] = Vec3{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+    return 0;
+}
 // nPoints, nFaces, nInternalFaces, nFacePoints, nCells, nPatches
 void vr_rawmesh_sizes(void* h, long* out) {
     const RawMesh& r = *static_cast<RawMesh*>(h);

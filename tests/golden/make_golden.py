@@ -393,6 +393,47 @@ def import_case():
         json.dump(out, fh, indent=1)
 
 
+# generateBox cases with cyclic pairs: (nx, ny, nz), periodic directions, 2-D, jittered
+PERIODIC_CASES = {
+    "box6x5x4_px": ((6, 5, 4), (True, False, False), False, False),
+    "box6x5x4_pxyz": ((6, 5, 4), (True, True, True), False, False),
+    "box6x5x1_py_2d": ((6, 5, 1), (False, True, False), True, False),
+    "channel_periodic": ((8, 5, 4), (True, False, False), False, True),
+}
+
+
+def periodic_case():
+    """The reference's generateBox with cyclic patch pairs (and one box with its interior
+    points moved, helpers.periodic_channel): the raw mesh, then the Mesh it builds (cyclic
+    pairs merged into internal faces, Mesh.cpp:246-300): owner, neighbour and geometry of
+    its faces, skew vectors, layout, patches, and a reconstruct of a seeded face flux."""
+    from helpers import periodic_channel
+    out = {}
+    for case, (n, per, two_d, jitter) in PERIODIC_CASES.items():
+        pts = periodic_channel().points if jitter else None
+        raw, rm = ref.generate_box(*n, per, two_d, points=pts)
+        e = {k: sha(np.ascontiguousarray(raw[k], dtype=np.int64 if k != "points" else np.float64))
+             for k in ("points", "face_off", "face_pts", "owner", "neighbour")}
+        e.update({"n_cells": raw["n_cells"], "patch_names": raw["patch_names"], "patch_types": raw["patch_types"]})
+        e["owner_c"], e["neighbour_c"] = sha(rm.owner), sha(rm.neighbour)
+        e["n_internal_c"] = rm.n_internal
+        g = rm.geometry()
+        sk, flag = rm.skew()
+        e["geometry"] = {k: sha(v) for k, v in g.items()}
+        e["geometry"]["skew"] = sha(sk)
+        e["skewed"] = flag
+        two, empty_dir, solved, names = rm.layout()
+        e["mesh"] = {"two_d": two, "empty_dir": empty_dir, "solved_components": solved, "patch_names": names,
+                     "patch_start": [int(v) for v in rm.patch_start], "patch_size": [int(v) for v in rm.patch_size]}
+        ssf = np.random.default_rng(11).uniform(-1, 1, rm.n_faces)
+        e["reconstruct"] = sha(rm.reconstruct(ssf))
+        out[case] = e
+        print(f"periodic {case}: {raw['n_cells']} cells, {rm.n_internal} internal faces, patches {names}",
+              flush=True)
+    with open(os.path.join(HERE, "periodic_expected.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+
+
 def piso_case(name):
     """Time steps of the reference's IncompressibleFlow (vr_piso_run) for one PISO case
     of helpers.PISO_CASES: U, p and φ after the steps."""
@@ -401,6 +442,8 @@ def piso_case(name):
     m = mk()
     if name.startswith("annulus"):
         _, rm = ref.read_mesh_file(os.path.join(HERE, "import", "annulus_v41.msh"))
+    elif name.startswith("channel_periodic"):
+        _, rm = ref.generate_box(8, 5, 4, (True, False, False), points=m.points)
     else:
         rm = ref.RefMesh(m)
     U0, p0, phi0 = piso_inputs(m)
@@ -556,6 +599,9 @@ if __name__ == "__main__":
         sys.exit(0)
     if "--import" in sys.argv:
         import_case()
+        sys.exit(0)
+    if "--periodic" in sys.argv:
+        periodic_case()
         sys.exit(0)
     if "--piso" in sys.argv:
         from helpers import PISO_CASES

@@ -86,25 +86,36 @@ def fvc_bfix(m, fixed):
 
 
 class CompView:
-    """The faces the reference's Mesh keeps of a read mesh (empty patches dropped): the
-    attributes the oracle's corrector functions read, with geometry restricted to those
-    faces, and the 2-D empty direction."""
+    """The faces the reference's Mesh keeps of a read mesh (cyclic pairs merged into
+    internal faces, empty patches dropped): the attributes the oracle's corrector
+    functions read, with the geometry of those faces, and the 2-D empty direction."""
 
     def __init__(self, m):
         c = m.computational()
-        f = c["faces"]
-        self.n_cells, self.n_internal = m.n_cells, m.n_internal
-        self.n_faces = len(f)
+        self.n_cells, self.n_internal = m.n_cells, c["n_internal"]
+        self.n_faces = len(c["faces"])
         self.n_boundary = self.n_faces - self.n_internal
-        self.owner = np.ascontiguousarray(m.owner[f])
-        self.neighbour = m.neighbour
-        g = m.geometry()
-        self._g = {k: (g[k] if k in ("V", "C", "skewed") else np.ascontiguousarray(g[k][f])) for k in g}
+        self.owner = c["owner"]
+        self.neighbour = c["neighbour"]
+        self._g = c["geometry"]
         self.empty_dir = c["empty_dir"]
         self.comp = c
 
     def geometry(self):
         return self._g
+
+
+def periodic_channel():
+    """An 8×5×4 unit box periodic in x (xMin/xMax a cyclic pair) with its interior points
+    moved by up to ±0.025 (a fifth of a cell in x, seeded): skewed internal and cyclic
+    faces; the boundary points stay, so the periodicity is a pure translation."""
+    m = M.box(8, 5, 4, periodic=(True, False, False))
+    pts = m.points.copy()
+    inner = np.all((pts > 1e-12) & (pts < 1 - 1e-12), axis=1)
+    rng = np.random.default_rng(3)
+    pts[inner] += rng.uniform(-0.025, 0.025, (int(inner.sum()), 3))
+    m.set_points(pts)
+    return m
 
 
 # ---------------------------------------------------------------- PISO step cases
@@ -120,6 +131,9 @@ PISO_CASES = {
     "annulus_2d_repro": (lambda: M.read_mesh(os.path.join(GOLDEN, "import", "annulus_v41.msh")),
                          {"nu": 0.05, "u_bc": [(0, 0, 0), (0, 0, 0)], "p_bc": [None, None], "ddt_phi_coeff": 0.5},
                          2, 0.02, True),
+    "channel_periodic_repro": (periodic_channel,
+                               {"nu": 0.02, "u_bc": [(0, 0, 0)] * 4, "p_bc": [None] * 4, "ddt": "backward",
+                                "grad_p": 1, "n_non_orth": 1}, 2, 0.01, True),
     "box8_lid_default": (lambda: M.box(8, 8, 8),
                          {"nu": 0.01, "u_bc": [(0, 0, 0)] * 5 + [(1, 0, 0)], "p_bc": [None] * 6}, 2, 0.005, False),
 }

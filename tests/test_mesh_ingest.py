@@ -11,7 +11,9 @@ importers extrude them, against its readMeshFile with scale / depth2D / patch ty
 options: raw mesh identical, and the faces its Mesh keeps (empty patches dropped) with
 their geometry, skew vectors, 2-D direction and solved components identical. The
 reference's own importer test checks (Fluent == polyMesh cavity; closed cells, volume
-= depth, patch areas) hold. Malformed input raises."""
+= depth, patch areas) hold. Boxes with cyclic patch pairs (one with moved interior
+points) against the reference's generateBox and Mesh: raw mesh, merged cyclic faces,
+their geometry and a reconstruct on them identical. Malformed input raises."""
 import os
 
 import numpy as np
@@ -195,3 +197,96 @@ def test_fluent_errors(tmp_path):
     tilted.write_text(g2.replace("\n2 0.5 0 0\n", "\n2 0.5 0 0.25\n"))  # a 2-D mesh off its plane
     with pytest.raises(RuntimeError):
         M.read_gmsh(tilted)
+
+
+# ---------------------------------------------------------------- cyclic patch pairs
+def _periodic(case):
+    from helpers import periodic_channel
+    e = load_json("periodic_expected.json")[case]
+    if case == "channel_periodic":
+        return e, periodic_channel()
+    n = {"box6x5x4_px": ((6, 5, 4), (True, False, False), False),
+         "box6x5x4_pxyz": ((6, 5, 4), (True, True, True), False),
+         "box6x5x1_py_2d": ((6, 5, 1), (False, True, False), True)}[case]
+    return e, M.box(*n[0], periodic=n[1], two_d=n[2])
+
+
+PERIODIC = ("box6x5x4_px", "box6x5x4_pxyz", "box6x5x1_py_2d", "channel_periodic")
+
+
+@pytest.mark.parametrize("case", PERIODIC)
+def test_periodic_mesh_matches_reference(case):
+    """generateBox with cyclic pairs (and a box whose interior points are moved, so that
+    internal and cyclic faces are skewed): raw mesh identical to the reference's; the
+    faces its Mesh computes on (cyclic pairs merged into internal faces after the
+    internal ones, Mesh.cpp:246-300) with owner, neighbour, geometry, skew vectors,
+    layout and patches identical; reconstruct of a face flux on them identical."""
+    from helpers import CompView
+    from oracle import oracle as O
+    e, m = _periodic(case)
+    assert sha(np.ascontiguousarray(m.points)) == e["points"]
+    for k in ("face_off", "face_pts", "owner", "neighbour"):
+        assert sha(np.ascontiguousarray(getattr(m, k), dtype=np.int64)) == e[k], k
+    assert m.n_cells == e["n_cells"]
+    assert m.patch_names == e["patch_names"] and m.patch_types == e["patch_types"]
+    c = m.computational()
+    assert c["n_internal"] == e["n_internal_c"]
+    assert sha(c["owner"]) == e["owner_c"] and sha(c["neighbour"]) == e["neighbour_c"]
+    for k, h in e["geometry"].items():
+        assert sha(np.ascontiguousarray(c["geometry"][k])) == h, k
+    assert c["geometry"]["skewed"] == e["skewed"]
+    lay = e["mesh"]
+    assert (c["two_d"], c["empty_dir"], c["solved_components"]) == (lay["two_d"], lay["empty_dir"],
+                                                                    lay["solved_components"])
+    assert c["patch_names"] == lay["patch_names"]
+    assert c["patch_start"] == lay["patch_start"] and c["patch_size"] == lay["patch_size"]
+    cv = CompView(m)
+    ssf = np.random.default_rng(11).uniform(-1, 1, cv.n_faces)
+    assert sha(O.fvc_reconstruct(cv, ssf, cv.empty_dir)) == e["reconstruct"]
+
+
+def test_periodic_faces_are_translations():
+    """Every merged cyclic face (an xMin face, owned by the first cell layer): d = C_N +
+    (C_fa − C_fb) − C_P is the vector to the partner cell's image one period (1 in x)
+    away, and w, ndc, corr follow from d as on any internal face (orthogonal box: w = 1/2,
+    ndc = 1/|d|, corr = 0)."""
+    m = M.box(6, 5, 4, periodic=(True, False, False))
+    c = m.computational()
+    g = c["geometry"]
+    ni_raw, ni = m.n_internal, c["n_internal"]
+    f = c["faces"][ni_raw:ni]
+    assert len(f) == 5 * 4 and all(m.patch_names[0] == "xMin" for _ in f)
+    o, n = c["owner"][ni_raw:ni], c["neighbour"][ni_raw:ni]
+    C = g["C"]
+    assert np.allclose((C[n] - np.array([1.0, 0, 0])) - C[o], g["d"][ni_raw:ni], atol=1e-15)
+    assert np.allclose(g["d"][ni_raw:ni], [-1.0 / 6, 0, 0], atol=1e-15)  # xMin's S_f points to −x
+    assert np.allclose(g["w"][ni_raw:ni], 0.5) and np.allclose(g["ndc"][ni_raw:ni], 6.0)
+    assert np.abs(g["corr"][ni_raw:ni]).max() < 1e-12
+
+
+def _foam(path, cls, body):
+    path.write_text("FoamFile\n{\n    version 2.0;\n    format ascii;\n    class " + cls + ";\n}\n" + body)
+
+
+def test_periodic_errors(tmp_path):
+    """A periodic direction of one cell is refused by the box generator (as by the
+    reference's generateBox, RawMesh.cpp:393), and a read mesh whose cyclic pair couples
+    a cell to itself, which the LDU matrices here do not hold, by computational()."""
+    with pytest.raises(RuntimeError, match="at least 2 cells"):
+        M.box(1, 3, 3, periodic=(True, False, False))
+    d = tmp_path / "polyMesh"
+    d.mkdir()
+    pts = ["(0 0 0)", "(1 0 0)", "(1 1 0)", "(0 1 0)", "(0 0 1)", "(1 0 1)", "(1 1 1)", "(0 1 1)"]
+    faces = ["4(0 4 7 3)", "4(1 2 6 5)", "4(0 1 5 4)", "4(3 7 6 2)", "4(0 3 2 1)", "4(4 5 6 7)"]
+    _foam(d / "points", "vectorField", "8\n(\n" + "\n".join(pts) + "\n)\n")
+    _foam(d / "faces", "faceList", "6\n(\n" + "\n".join(faces) + "\n)\n")
+    _foam(d / "owner", "labelList", "6\n(\n0\n0\n0\n0\n0\n0\n)\n")
+    _foam(d / "neighbour", "labelList", "0\n(\n)\n")
+    _foam(d / "boundary", "polyBoundaryMesh",
+          "3\n(\n xMin\n {\n  type cyclic;\n  neighbourPatch xMax;\n  nFaces 1;\n  startFace 0;\n }\n"
+          " xMax\n {\n  type cyclic;\n  neighbourPatch xMin;\n  nFaces 1;\n  startFace 1;\n }\n"
+          " walls\n {\n  type wall;\n  nFaces 4;\n  startFace 2;\n }\n)\n")
+    m = M.read_polymesh(str(d))
+    assert m.patch_types == ["cyclic", "cyclic", "wall"] and m.patch_neighbours[:2] == ["xMax", "xMin"]
+    with pytest.raises(RuntimeError, match="itself"):
+        m.computational()

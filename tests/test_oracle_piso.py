@@ -164,3 +164,22 @@ def test_corrector_closed_forms():
     assert np.array_equal(O.piso_face_op(0, a, b, c), a + b * c)
     assert np.array_equal(O.piso_face_op(1, a, b), a - b)
     assert np.array_equal(O.piso_face_op(2, a, b, c), a - b / c)
+
+
+def test_periodic_gamg_hierarchy_matches_reference():
+    """The oracle's GAMG agglomeration on the faces of a periodic channel (cyclic pairs
+    merged into internal faces, weights |S_f|·ndc_f of those faces) equals the hierarchy
+    the reference builds on its own Mesh of the same points."""
+    if not ref.available():
+        pytest.skip("reference build absent")
+    from helpers import periodic_channel
+    m = periodic_channel()
+    c = m.computational()
+    g, ni = c["geometry"], c["n_internal"]
+    om = O.LduMesh(m.n_cells, c["owner"][:ni], c["neighbour"], g["magSf"][:ni] * g["ndc"][:ni])
+    _, rm = ref.generate_box(8, 5, 4, (True, False, False), points=m.points)
+    mine, theirs = om.gamg_levels(), rm.gamg_hierarchy()
+    assert [L["n"] for L in mine] == [L["n"] for L in theirs]
+    for a, b in zip(mine, theirs):
+        for k in ("agg", "l", "u", "weight"):
+            assert np.array_equal(a[k], b[k]), k

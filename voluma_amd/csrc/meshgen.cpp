@@ -53,7 +53,8 @@ struct Raw {
     std::vector<V3> points;
     std::vector<int64_t> faceOff{0}, facePts, owner, neighbour;
     std::vector<int64_t> pStart, pSize;
-    std::vector<std::string> pName, pType;  // patch names / types (read meshes)
+    std::vector<std::string> pName, pType;  // patch names / types
+    std::vector<std::string> pNeighbour;     // cyclic patches: the partner patch's name
     int64_t nCells = 0;
     // geometry (filled by computeGeometry)
     bool haveGeom = false;
@@ -69,8 +70,17 @@ struct Raw {
     int64_t nInternal() const { return int64_t(neighbour.size()); }
 };
 
-Raw makeBox(int nx, int ny, int nz) {
+// generateBox (RawMesh.cpp:391-502) on the unit cube: periodic bit d (1 x, 2 y, 4 z)
+// makes the two patches of that direction a cyclic pair; twoD (nz = 1) makes zMin and
+// zMax one empty patch "frontAndBack".
+Raw makeBox(int nx, int ny, int nz, int periodic = 0, bool twoD = false) {
+    for (int d = 0; d < 3; ++d) {
+        const int nd = d == 0 ? nx : d == 1 ? ny : nz;
+        if ((periodic >> d & 1) && nd < 2 && !(d == 2 && twoD))
+            throw std::runtime_error("box: a periodic direction needs at least 2 cells");
+    }
     if (nx < 1 || ny < 1 || nz < 1) throw std::invalid_argument("box: bad resolution");
+    if (twoD && nz != 1) throw std::invalid_argument("box: twoD requires nz == 1");
     Raw m;
     auto P = [&](int i, int j, int k) -> int64_t { return i + int64_t(nx + 1) * (j + int64_t(ny + 1) * k); };
     auto C = [&](int i, int j, int k) -> int64_t { return i + int64_t(nx) * (j + int64_t(ny) * k); };
@@ -101,8 +111,23 @@ Raw makeBox(int nx, int ny, int nz) {
                     m.neighbour.push_back(C(i, j, k + 1));
                 }
             }
-    auto begin = [&]() { m.pStart.push_back(m.nFaces()); };
-    auto end = [&]() { m.pSize.push_back(m.nFaces() - m.pStart.back()); };
+    static const char* names[6] = {"xMin", "xMax", "yMin", "yMax", "zMin", "zMax"};
+    auto beginP = [&](const std::string& name, const std::string& type, const std::string& nb) {
+        m.pName.push_back(name);
+        m.pType.push_back(type);
+        m.pNeighbour.push_back(nb);
+        m.pStart.push_back(m.nFaces());
+    };
+    auto patchOf = [&](int q, int d) {  // patch q of direction d: cyclic with its partner, or a plain patch
+        const bool cyc = periodic >> d & 1;
+        beginP(names[q], cyc ? "cyclic" : "patch", cyc ? names[q ^ 1] : "");
+    };
+    int q0 = 0;
+    auto begin = [&]() { patchOf(q0, q0 / 2); };
+    auto end = [&]() {
+        m.pSize.push_back(m.nFaces() - m.pStart.back());
+        ++q0;
+    };
     for (int side = 0; side < 2; ++side) {  // xMin, xMax
         begin();
         const int i = side == 0 ? 0 : nx;
@@ -129,8 +154,9 @@ Raw makeBox(int nx, int ny, int nz) {
             }
         end();
     }
-    for (int side = 0; side < 2; ++side) {  // zMin, zMax
-        begin();
+    if (twoD) beginP("frontAndBack", "empty", "");
+    for (int side = 0; side < 2; ++side) {  // zMin, zMax (one empty patch in 2-D)
+        if (!twoD) begin();
         const int k = side == 0 ? 0 : nz;
         for (int j = 0; j < ny; ++j)
             for (int i = 0; i < nx; ++i) {
@@ -140,8 +166,9 @@ Raw makeBox(int nx, int ny, int nz) {
                     m.addFace({P(i, j, k), P(i + 1, j, k), P(i + 1, j + 1, k), P(i, j + 1, k)});
                 m.owner.push_back(C(i, j, side == 0 ? 0 : nz - 1));
             }
-        end();
+        if (!twoD) end();
     }
+    if (twoD) m.pSize.push_back(m.nFaces() - m.pStart.back());
     return m;
 }
 
@@ -622,7 +649,7 @@ Raw readPolyMesh(const std::string& dir) {
         FoamFile f(dir + "/boundary");
         const int64_t n = f.listStart();
         for (int64_t p = 0; p < n; ++p) {
-            std::string name = f.word(), type;
+            std::string name = f.word(), type, nbPatch;
             int64_t start = 0, size = 0;
             if (f.word() != "{") f.bad("expected '{' after patch name");
             int depth = 1;
@@ -654,9 +681,17 @@ Raw readPolyMesh(const std::string& dir) {
                 if (key == "type") type = vals[0];
                 else if (key == "nFaces") size = std::strtoll(vals[0].c_str(), nullptr, 10);
                 else if (key == "startFace") start = std::strtoll(vals[0].c_str(), nullptr, 10);
+                else if (key == "neighbourPatch") nbPatch = vals[0];
             }
+            // the type in the reference's spelling (patchTypeFromString, then toString)
+            type = type == "wall" ? "wall"
+                   : (type == "symmetry" || type == "symmetryPlane") ? "symmetryPlane"
+                   : type == "empty" ? "empty"
+                   : type == "cyclic" ? "cyclic"
+                                      : "patch";
             m.pName.push_back(name);
             m.pType.push_back(type);
+            m.pNeighbour.push_back(nbPatch);
             m.pStart.push_back(start);
             m.pSize.push_back(size);
         }
@@ -1640,6 +1675,34 @@ void* vm_box(int nx, int ny, int nz) {
         gErr = e.what();
         return nullptr;
     }
+}
+
+// generateBox with periodic directions (bit 1 x, 2 y, 4 z: cyclic patch pairs) and 2-D
+// (nz = 1, frontAndBack empty)
+void* vm_box_spec(int nx, int ny, int nz, int periodic, int twoD) {
+    return guarded([&] { return makeBox(nx, ny, nz, periodic, twoD != 0); });
+}
+
+// the partner patch of a cyclic patch p (empty otherwise); -1 for a bad index
+int vm_patch_neighbour(void* h, int p, char* out, int len) {
+    const Raw& m = *static_cast<Raw*>(h);
+    if (p < 0 || p >= int(m.pStart.size()) || !out || len <= 0) return -1;
+    const std::string s = p < int(m.pNeighbour.size()) ? m.pNeighbour[p] : std::string();
+    std::strncpy(out, s.c_str(), std::size_t(len) - 1);
+    out[len - 1] = 0;
+    return 0;
+}
+
+// replace the point coordinates of a mesh (same count); its geometry is recomputed
+int vm_set_points(void* h, const double* pts, int64_t nPoints) {
+    Raw& m = *static_cast<Raw*>(h);
+    if (!pts || nPoints != int64_t(m.points.size())) {
+        gErr = "vm_set_points: point count differs from the mesh's";
+        return -1;
+    }
+    for (int64_t i = 0; i < nPoints; ++i) m.points[i] = V3{pts[3 * i], pts[3 * i + 1], pts[3 * i + 2]};
+    m.haveGeom = false;
+    return 0;
 }
 
 void* vm_tet(int nx, int ny, int nz, double jitter, uint64_t seed) {

@@ -15,6 +15,7 @@ Patches are xMin, xMax, yMin, yMax, zMin, zMax in that order.
 from __future__ import annotations
 
 import ctypes
+import math
 import os
 
 import numpy as np
@@ -51,6 +52,12 @@ def _lib():
             f.restype = P
         lib.vm_patch_info.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P, ctypes.c_int]
         lib.vm_patch_info.restype = ctypes.c_int
+        lib.vm_box_spec.argtypes = [ctypes.c_int] * 5
+        lib.vm_box_spec.restype = P
+        lib.vm_patch_neighbour.argtypes = [P, ctypes.c_int, P, ctypes.c_int]
+        lib.vm_patch_neighbour.restype = ctypes.c_int
+        lib.vm_set_points.argtypes = [P, P, ctypes.c_int64]
+        lib.vm_set_points.restype = ctypes.c_int
         lib.vm_last_error.restype = ctypes.c_char_p
         _LIB = lib
     return _LIB
@@ -58,6 +65,45 @@ def _lib():
 
 def _ptr(a):
     return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _mag(v):
+    return math.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+
+
+def _max_skew(g, ni):
+    """Largest |skew|/|d| over the raw internal faces, summed as the reference does."""
+    sk, d = g["skew"][:ni], g["d"][:ni]
+    if ni == 0:
+        return 0.0
+    ms = np.sqrt(sk[:, 0] * sk[:, 0] + sk[:, 1] * sk[:, 1] + sk[:, 2] * sk[:, 2])
+    md = np.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2])
+    return float(np.max(ms / md))
+
+
+def _cyclic_geometry(geo, C, Cf, owner, neighbour, ni, n_int, fa, fb):
+    """Face quantities of merged cyclic faces (Mesh.cpp:295-300, 598-628): d = C_N +
+    (Cf_a − Cf_b) − C_P, then w, skew, ndc and corr from it, in the reference's order."""
+    for k in range(n_int - ni):
+        f = ni + k
+        o, n = owner[f], neighbour[f]
+        dd = (C[n] + (Cf[fa[k]] - Cf[fb[k]])) - C[o]
+        S, c, ms = geo["Sf"][f], geo["Cf"][f], geo["magSf"][f]
+        nv = S / ms
+        CP = C[o]
+        t = c - CP
+        s_own = abs(S[0] * t[0] + S[1] * t[1] + S[2] * t[2])
+        t = (CP + dd) - c
+        s_nei = abs(S[0] * t[0] + S[1] * t[1] + S[2] * t[2])
+        w = s_nei / (s_own + s_nei)
+        geo["d"][f] = dd
+        geo["w"][f] = w
+        geo["skew"][f] = c - (CP + dd * (1.0 - w))
+        md = _mag(dd)
+        nd = nv[0] * dd[0] + nv[1] * dd[1] + nv[2] * dd[2]
+        ndc = 1.0 / max(nd, 0.05 * md)
+        geo["ndc"][f] = ndc
+        geo["corr"][f] = nv - dd * ndc
 
 
 class Mesh:
@@ -94,6 +140,11 @@ class Mesh:
             lib.vm_patch_info(self._h, p, nm, 256, tp, 64)
             self.patch_names.append(nm.value.decode())
             self.patch_types.append(tp.value.decode())
+        self.patch_neighbours = []
+        for p in range(n_patch):
+            nb = ctypes.create_string_buffer(256)
+            lib.vm_patch_neighbour(self._h, p, nb, 256)
+            self.patch_neighbours.append(nb.value.decode())
         self._geom = None
 
     def __del__(self):
@@ -101,6 +152,15 @@ class Mesh:
         if h:
             _lib().vm_free(h)
             self._h = None
+
+    def set_points(self, points):
+        """Replace the point coordinates (same count, same topology); the geometry is
+        recomputed on next use."""
+        pts = np.ascontiguousarray(points, dtype=np.float64)
+        if pts.shape != self.points.shape or _lib().vm_set_points(self._h, _ptr(pts), pts.shape[0]) != 0:
+            raise ValueError("points must have the mesh's point count, shape (n, 3)")
+        self.points = pts.copy()
+        self._geom = None
 
     @property
     def n_boundary(self):
@@ -127,31 +187,94 @@ class Mesh:
         return self._geom
 
     def computational(self):
-        """The faces the reference's Mesh computes on (Mesh.cpp:236-330, 641-645):
-        internal faces, then the faces of each patch that is not ``empty``, patches in
-        order. A mesh with an empty patch is 2-D: its empty direction is the axis with
-        the largest Σ|S_f| component over the empty faces (the first on a tie) and it
-        solves the other two vector components. Cyclic patches are not supported.
+        """The faces the reference's Mesh computes on (Mesh.cpp:236-330, 598-645): the
+        internal faces, then each cyclic patch pair merged into internal faces, then the
+        faces of every patch that is neither empty nor cyclic, patches in order. A mesh
+        with an empty patch is 2-D: its empty direction is the axis with the largest
+        Σ|S_f| component over the empty faces (the first on a tie) and it solves the other
+        two vector components.
 
-        Returns a dict: ``faces`` (raw face index of each computational face),
+        Returns a dict: ``faces`` (the raw face of each computational face; a merged
+        cyclic face is its first patch's face), ``owner``/``neighbour`` (int32; a merged
+        face's neighbour is the partner face's cell), ``n_internal``,
         ``patch_names``/``patch_types``/``patch_start``/``patch_size`` (computational
-        numbering), ``two_d``, ``empty_dir`` (−1 in 3-D), ``solved_components``."""
-        if "cyclic" in self.patch_types:
-            raise RuntimeError("cyclic patches are not supported")
-        faces = [np.arange(self.n_internal)]
+        numbering), ``two_d``, ``empty_dir`` (−1 in 3-D), ``solved_components`` and
+        ``geometry`` (V, C and per computational face Sf, Cf, magSf, d, w, ndc, corr,
+        skew, with skewed)."""
+        g = self.geometry()
+        ni = self.n_internal
+        Cf, Sf = g["Cf"], g["Sf"]
+        faces = [np.arange(ni)]
+        own = [np.asarray(self.owner[:ni])]
+        nei = [np.asarray(self.neighbour)]
+        cyc_face, cyc_partner = [], []
+        done = [False] * len(self.patch_types)
+        for a, kind in enumerate(self.patch_types):  # cyclic pairs (Mesh.cpp:246-300)
+            if kind != "cyclic" or done[a]:
+                continue
+            b = len(self.patch_types)
+            for k, nm in enumerate(self.patch_names):
+                if nm == self.patch_neighbours[a]:
+                    b = k
+            if b == len(self.patch_types):
+                raise RuntimeError(f"cyclic patch {self.patch_names[a]}: neighbourPatch not found")
+            sa, na = int(self.patch_start[a]), int(self.patch_size[a])
+            sb, nbf = int(self.patch_start[b]), int(self.patch_size[b])
+            if na != nbf:
+                raise RuntimeError(f"cyclic patches {self.patch_names[a]}/{self.patch_names[b]} differ in size")
+            done[a] = done[b] = True
+            ca, cb = np.zeros(3), np.zeros(3)
+            min_a = 1e15  # GREAT
+            for k in range(na):
+                ca = ca + Cf[sa + k]
+                cb = cb + Cf[sb + k]
+                min_a = min(min_a, _mag(Sf[sa + k]), _mag(Sf[sb + k]))
+            sep = (cb - ca) / float(na)
+            h = 0.25 * math.sqrt(min_a)
+            key = lambda q: (math.floor(q[0] / h), math.floor(q[1] / h), math.floor(q[2] / h))
+            grid = {}
+            for k in range(nbf):
+                grid.setdefault(key(Cf[sb + k]), []).append(sb + k)
+            for k in range(na):
+                fa = sa + k
+                target = Cf[fa] + sep
+                kk = key(target)
+                best, best_d = -1, 1e15
+                for dx in (-1, 0, 1):
+                    for dy in (-1, 0, 1):
+                        for dz in (-1, 0, 1):
+                            for fb in grid.get((kk[0] + dx, kk[1] + dy, kk[2] + dz), ()):
+                                dd = _mag(Cf[fb] - target)
+                                if dd < best_d:
+                                    best_d, best = dd, fb
+                if best < 0 or best_d > 1e-3 * math.sqrt(_mag(Sf[fa])):
+                    raise RuntimeError(f"cyclic patch {self.patch_names[a]}: no matching face (non-translational?)")
+                if self.owner[fa] == self.owner[best]:
+                    raise RuntimeError(f"cyclic patch {self.patch_names[a]}: a face couples cell "
+                                       f"{int(self.owner[fa])} to itself (one cell across the periodic "
+                                       "direction), which the LDU matrices here do not hold")
+                cyc_face.append(fa)
+                cyc_partner.append(best)
+        if cyc_face:
+            cf_, cp_ = np.asarray(cyc_face), np.asarray(cyc_partner)
+            faces.append(cf_)
+            own.append(np.asarray(self.owner[cf_]))
+            nei.append(np.asarray(self.owner[cp_]))
+        n_int = ni + len(cyc_face)
         out = {"patch_names": [], "patch_types": [], "patch_start": [], "patch_size": []}
-        n = self.n_internal
+        n = n_int
         empty_n = np.zeros(3)
-        sf = None
-        for p, kind in enumerate(self.patch_types):
-            f = self.patch_faces(p)
+        for q, kind in enumerate(self.patch_types):
+            f = self.patch_faces(q)
             if kind == "empty":
-                sf = self.geometry()["Sf"] if sf is None else sf
-                for v in np.abs(sf[f]):  # in face order, as the reference sums
+                for v in np.abs(Sf[f]):  # in face order, as the reference sums
                     empty_n += v
                 continue
+            if kind == "cyclic":
+                continue
             faces.append(f)
-            out["patch_names"].append(self.patch_names[p])
+            own.append(np.asarray(self.owner[f]))
+            out["patch_names"].append(self.patch_names[q])
             out["patch_types"].append(kind)
             out["patch_start"].append(n)
             out["patch_size"].append(len(f))
@@ -163,8 +286,21 @@ class Mesh:
             for a in (1, 2):
                 if empty_n[a] > empty_n[d]:
                     d = a
-        out.update(faces=np.concatenate(faces), two_d=two_d, empty_dir=d,
-                   solved_components=[c for c in range(3) if not two_d or c != d])
+        fl = np.concatenate(faces)
+        owner = np.concatenate(own).astype(np.int32)
+        neighbour = np.concatenate(nei).astype(np.int32)
+        geo = {"V": g["V"], "C": g["C"]}
+        for k in ("Sf", "Cf", "magSf", "d", "w", "ndc", "corr", "skew"):
+            geo[k] = np.ascontiguousarray(g[k][fl])
+        skew_max = _max_skew(g, ni)
+        if cyc_face:
+            _cyclic_geometry(geo, g["C"], Cf, owner, neighbour, ni, n_int, np.asarray(cyc_face),
+                             np.asarray(cyc_partner))
+            for f in range(ni, n_int):
+                skew_max = max(skew_max, _mag(geo["skew"][f]) / _mag(geo["d"][f]))
+        geo["skewed"] = bool(skew_max > 1e-8)  # Mesh.cpp:596
+        out.update(faces=fl, owner=owner, neighbour=neighbour, n_internal=n_int, two_d=two_d, empty_dir=d,
+                   solved_components=[c for c in range(3) if not two_d or c != d], geometry=geo)
         return out
 
     def agglomeration_weights(self):
@@ -212,10 +348,15 @@ def read_mesh(path, scale=1.0, depth2d=0.1, patch_types=None) -> Mesh:
     return Mesh(_lib().vm_read_mesh_file(*_import_args(path, scale, depth2d, patch_types)))
 
 
-def box(nx, ny=None, nz=None) -> Mesh:
+def box(nx, ny=None, nz=None, periodic=(False, False, False), two_d=False) -> Mesh:
+    """The unit-cube hex mesh in generateBox's order; periodic directions make their two
+    patches a cyclic pair, two_d (nz = 1) one empty frontAndBack patch."""
     ny = nx if ny is None else ny
     nz = nx if nz is None else nz
-    return Mesh(_lib().vm_box(nx, ny, nz))
+    if not any(periodic) and not two_d:
+        return Mesh(_lib().vm_box(nx, ny, nz))
+    bits = sum(1 << d for d in range(3) if periodic[d])
+    return Mesh(_lib().vm_box_spec(nx, ny, nz, bits, int(bool(two_d))))
 
 
 def tet(nx, ny=None, nz=None, jitter=0.2, seed=1) -> Mesh:

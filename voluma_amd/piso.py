@@ -12,8 +12,9 @@ reference cell, the face flux update and the velocity correction. U, p, φ, thei
 levels and the matrices stay on the GPU between and during steps; ``U``, ``p`` and
 ``phi`` copy the fields in or out.
 
-Meshes come from ``voluma_amd.mesh``; a 2-D mesh (an empty patch) solves the two
-in-plane velocity components, the reference's ``solvedComponents``. With reproducible
+Meshes come from ``voluma_amd.mesh``; cyclic patch pairs become internal faces
+(``Mesh.computational``) and a 2-D mesh (an empty patch) solves the two in-plane
+velocity components, the reference's ``solvedComponents``. With reproducible
 sums (``Context.set_reproducible``) a step uses the reference's partition-independent
 solvers and reproduces its fields bit for bit.
 """
@@ -38,17 +39,15 @@ class PisoFlow:
         if ddt not in ("Euler", "backward"):
             raise ValueError("ddt must be 'Euler' or 'backward'")
         comp = mesh.computational()
-        f = comp["faces"]
-        g = mesh.geometry()
+        g = comp["geometry"]
         self.ctx = ctx
-        self.n, self.ni, self.nF = mesh.n_cells, mesh.n_internal, len(f)
+        self.n, self.ni, self.nF = mesh.n_cells, comp["n_internal"], len(comp["faces"])
         nb = self.nF - self.ni
-        own = np.ascontiguousarray(mesh.owner[f]).astype(np.int32)
-        pick = lambda k: np.ascontiguousarray(g[k][f])
-        magSf, ndc = pick("magSf"), pick("ndc")
-        self.mesh = va.LduMesh(ctx, self.n, own[: self.ni], mesh.neighbour, magSf[: self.ni] * ndc[: self.ni])
+        own = comp["owner"]
+        magSf, ndc = g["magSf"], g["ndc"]
+        self.mesh = va.LduMesh(ctx, self.n, own[: self.ni], comp["neighbour"], magSf[: self.ni] * ndc[: self.ni])
         self.mesh.set_boundary(own[self.ni:])
-        self.mesh.set_geometry(g["V"], pick("Sf"), magSf, pick("w"), pick("d"), pick("skew"), g["skewed"])
+        self.mesh.set_geometry(g["V"], g["Sf"], magSf, g["w"], g["d"], g["skew"], g["skewed"])
         self.mesh.set_empty_dir(comp["empty_dir"])
         nbp = len(comp["patch_names"])
         if len(u_bc) != nbp or len(p_bc) != nbp:
@@ -77,7 +76,7 @@ class PisoFlow:
         h = ctypes.c_void_p()
         u_fix_c = np.zeros(max(nb, 1), np.int32)
         u_fix_c[:nb] = u_fix
-        _check(lib().vol_flow_create(self.mesh._h, ctypes.byref(o), _p(ndc), _p(pick("corr")), _p(u_fix_c),
+        _check(lib().vol_flow_create(self.mesh._h, ctypes.byref(o), _p(ndc), _p(g["corr"]), _p(u_fix_c),
                                      _p(np.ascontiguousarray(u_val)), _p(p_fix), _p(p_val), va.VOL_HOST,
                                      ctypes.byref(h)))
         self._h = h
