@@ -100,9 +100,9 @@ struct LevelOrderDev {
 // entry, in the sweep's order); src keeps the source index of every entry (−1 padding)
 // for the per-matrix coefficient gather.
 struct LevelEllDev {
-    std::vector<int> off, width, base, grp;
+    std::vector<int> off, width, base, grp, runOff, runCnt;
     std::vector<char> kind;
-    DevBuf<int> cell, col, src, offD, widthD, baseD;
+    DevBuf<int> cell, slot, runCells, col, src, offD, widthD, baseD;
     size_t nEntries = 0;
     void build(const std::vector<int>& levOff, const std::vector<int>& perm, const std::vector<int>& rowOff,
                const std::vector<int>& rowCol, const std::vector<int>& rowSrc, hipStream_t st) {
@@ -179,7 +179,79 @@ struct LevelEllDev {
             L = R;
         }
         grp.push_back(std::max(nl, 0));
+        // LDS-resident runs: a run takes consecutive levels while the cells its rows
+        // write or read fit RUN_LDS_CELLS slots (a level that alone does not fit is swept
+        // as a grid level); slots in first-appearance order (a row's cell, then its
+        // neighbours), run rows' entries and own cells re-coded as slots
+        std::vector<int> slotv(std::max<size_t>(perm.size(), 1), 0), cells;
+        {
+            std::vector<int> g2, ro, rn;
+            std::vector<char> k2;
+            std::unordered_map<int, int> map, seen;
+            std::vector<int> fresh;
+            auto close = [&](int L, char k, int start, int cnt) {
+                g2.push_back(L);
+                k2.push_back(k);
+                ro.push_back(start);
+                rn.push_back(cnt);
+            };
+            for (size_t g = 0; g + 1 < grp.size(); ++g) {
+                const int A = grp[g], B = grp[g + 1];
+                if (kind[g] == char(ELL_GRID)) {
+                    close(A, kind[g], int(cells.size()), 0);
+                    continue;
+                }
+                for (int L = A; L < B;) {
+                    map.clear();
+                    const int start = int(cells.size());
+                    int R = L;
+                    for (; R < B; ++R) {
+                        fresh.clear();
+                        seen.clear();
+                        for (int i = off[R]; i < off[R + 1]; ++i) {
+                            const int c = perm[i];
+                            auto take = [&](int q) {
+                                if (!map.count(q) && seen.emplace(q, 0).second) fresh.push_back(q);
+                            };
+                            take(c);
+                            for (int k = rowOff[c]; k < rowOff[c + 1]; ++k) take(rowCol[k]);
+                        }
+                        if (map.size() + fresh.size() > size_t(RUN_LDS_CELLS)) break;
+                        for (int q : fresh) {
+                            map.emplace(q, int(cells.size()) - start);
+                            cells.push_back(q);
+                        }
+                    }
+                    if (R == L) {  // one level alone exceeds the slots
+                        close(L, char(ELL_GRID), start, 0);
+                        ++L;
+                        continue;
+                    }
+                    close(L, kind[g], start, int(cells.size()) - start);
+                    for (int Lq = L; Lq < R; ++Lq) {
+                        const int size = off[Lq + 1] - off[Lq];
+                        for (int t = 0; t < size; ++t) {
+                            const int c = perm[off[Lq] + t];
+                            slotv[off[Lq] + t] = map.at(c);
+                            for (int k = 0; k < rowOff[c + 1] - rowOff[c]; ++k) {
+                                const size_t e = size_t(base[Lq]) + size_t(k) * size + t;
+                                ec[e] = map.at(ec[e]);
+                            }
+                        }
+                    }
+                    L = R;
+                }
+            }
+            g2.push_back(std::max(nl, 0));
+            grp = std::move(g2);
+            kind = std::move(k2);
+            runOff = std::move(ro);
+            runCnt = std::move(rn);
+        }
         cell.upload(perm, st);
+        slot.upload(slotv, st);
+        if (cells.empty()) cells.push_back(0);
+        runCells.upload(cells, st);
         col.upload(ec, st);
         src.upload(es, st);
         offD.upload(off, st);
@@ -190,6 +262,10 @@ struct LevelEllDev {
     LevelEll view() const {
         LevelEll v;
         v.cell = cell.p;
+        v.slot = slot.p;
+        v.runCells = runCells.p;
+        v.runOffH = runOff.data();
+        v.runCntH = runCnt.data();
         v.col = col.p;
         v.offD = offD.p;
         v.widthD = widthD.p;

@@ -16,9 +16,11 @@
 // the level's row t sits at base + k·size + t, so a row needs its cell index, its
 // entries and the gathers of its neighbours' values, and no per-row offsets. Runs of
 // consecutive levels of at most 1024 cells are swept by one workgroup of 1024 threads,
-// a row per thread, in a single launch, with a barrier between levels (global writes of
-// a workgroup are visible to it after the barrier) and the next level's operands loaded
-// while the current level's gathers are in flight. Larger levels get a grid launch
+// a row per thread, in a single launch, with the run's part of x in LDS (every cell its
+// rows write or read: loaded when the run starts, updated in place, stored to x as it is
+// computed), a barrier on the LDS writes only between levels (a full barrier would also
+// wait for the next levels' operand loads in flight), and the next level's operands
+// loaded while the current level's LDS gathers are read. Larger levels get a grid launch
 // each: a level of ~10⁴ rows moves ~1 MB, more than one CU streams in the time of a
 // launch (measured: 16 rows per thread in one workgroup was 3× slower than grid
 // launches on 2M-cell meshes). The factor, run once per matrix, walks the neighbour
@@ -128,6 +130,7 @@ void launch_dilu_factor(hipStream_t st, const SweepView& s, const LevelSched& fw
 // in order), not for everything in flight.
 template <int PF, class T> struct RowPre {
     int c;     // the row's cell (a stand-in when !has)
+    int s;     // runs: the LDS slot of the row's cell
     bool has;  // the thread has a row in this level
     T i0, i1;
     int col[PF];  // −1: no entry
@@ -199,6 +202,48 @@ __device__ __forceinline__ void row_finish(const LevelEll& E, const SweepArgs<T>
     else a.x[p.c] = acc;
 }
 
+// Runs: entries hold LDS slots of x (xs), so the gathers and the tail's reads come from
+// LDS; the row writes its value to its slot and to x (read by later launches only).
+template <int PF, class T> __device__ __forceinline__ void row_gather_lds(const T* xs, const RowPre<PF, T>& p, T* xv) {
+#pragma unroll
+    for (int k = 0; k < PF; ++k) xv[k] = xs[p.col[k] >= 0 ? p.col[k] : 0];
+}
+template <int OP, int PF, class T>
+__device__ __forceinline__ void row_finish_lds(const LevelEll& E, const SweepArgs<T>& a, T* xs, const RowPre<PF, T>& p,
+                                               const T* xv, int size, int width, int base, int t) {
+    T acc = p.i0;
+#pragma unroll
+    for (int k = 0; k < PF; ++k)
+        if (p.col[k] >= 0) acc = acc - p.cf[k] * xv[k];
+    for (int k0 = PF; k0 < width; k0 += SWEEP_TAIL) {
+        int j[SWEEP_TAIL];
+        T cf[SWEEP_TAIL], xt[SWEEP_TAIL];
+#pragma unroll
+        for (int q = 0; q < SWEEP_TAIL; ++q) {
+            const bool in = k0 + q < width;
+            const int e = in ? base + (k0 + q) * size + t : 0;
+            const int jj = E.col[e];
+            cf[q] = a.coef[e];
+            j[q] = in ? jj : -1;
+        }
+#pragma unroll
+        for (int q = 0; q < SWEEP_TAIL; ++q) xt[q] = xs[j[q] >= 0 ? j[q] : 0];
+#pragma unroll
+        for (int q = 0; q < SWEEP_TAIL; ++q)
+            if (j[q] >= 0) acc = acc - cf[q] * xt[q];
+    }
+    const T v = OP == SW_GS ? acc / p.i1 : acc;
+    xs[p.s] = v;
+    a.x[p.c] = v;
+}
+// Barrier for values exchanged through LDS only: waits for the LDS writes, not for the
+// global loads in flight (the next levels' operands) or the stores to x.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // One level as a grid: thread t updates the level's row t.
 constexpr int GRID_PF = 8;
 template <int OP, class T>
@@ -217,17 +262,18 @@ __global__ __launch_bounds__(SWEEP_THREADS) void k_ell_level(LevelEll E, SweepAr
 }
 
 // Levels [L0, L1) by one workgroup of THREADS threads (every level has at most THREADS
-// cells), a row per thread, a barrier after each level (global writes of a workgroup are
-// visible to it after the barrier). The levels' offsets, widths and bases are staged in
-// LDS first. In level i a thread first sends out its row's neighbour gathers, then, while
-// they are in flight, loads level i+1's entries and start operands (from the cell it
-// loaded in level i−1) and level i+2's cell: none of them is written before its own level.
-// A level thus costs about one gather round trip, the store and the barrier when its rows
-// have at most PF entries. Sums keep the entry order: the result is the plain loop's.
+// cells), a row per thread, x of the run's cells (cells[0 .. nCells), entries holding
+// their slots) in LDS, a barrier on the LDS writes after each level. The levels' offsets,
+// widths and bases are staged in LDS first. In level i a thread first reads its row's
+// neighbour values from LDS, then loads level i+1's entries and start operands (from the
+// cell it loaded in level i−1) and level i+2's cell and slot: none of them is written
+// before its own level, and they stay in flight across the barrier. Sums keep the entry order: the result is the plain loop's.
 template <int OP, int THREADS, int PF, class T>
-__global__ __launch_bounds__(THREADS) void k_ell_run(LevelEll E, SweepArgs<T> a, int L0, int L1, const int* done) {
+__global__ __launch_bounds__(THREADS) void k_ell_run(LevelEll E, SweepArgs<T> a, int L0, int L1, const int* cells,
+                                                     int nCells, const int* done) {
     if (done && *done) return;
     __shared__ int sOff[ELL_RUN_LEVELS + 1], sWidth[ELL_RUN_LEVELS], sBase[ELL_RUN_LEVELS];
+    __shared__ T xs[RUN_LDS_CELLS];
     const int nl = L1 - L0, t = int(threadIdx.x);
     for (int i = t; i <= nl; i += THREADS) {
         sOff[i] = E.offD[L0 + i];
@@ -236,27 +282,33 @@ __global__ __launch_bounds__(THREADS) void k_ell_run(LevelEll E, SweepArgs<T> a,
             sBase[i] = E.baseD[L0 + i];
         }
     }
+    for (int i = t; i < nCells; i += THREADS) xs[i] = a.x[cells[i]];
     __syncthreads();
     // past the last level, a thread loads the last level's data again (unused)
     auto lev = [nl](int i) { return i < nl ? i : nl - 1; };
     RowPre<PF, T> cur, nxt;
     int size = sOff[1] - sOff[0], width = sWidth[0], base = sBase[0];
     cur.c = row_cell(E, sOff[0], size, t);
+    cur.s = E.slot[sOff[0] + (t < size ? t : 0)];
     row_load_entries(E, a, size, width, base, t, cur);
     row_load_start<OP>(a, cur);
-    int cNext = row_cell(E, sOff[lev(1)], sOff[lev(1) + 1] - sOff[lev(1)], t);
+    const int p1 = sOff[lev(1)], z1 = sOff[lev(1) + 1] - p1;
+    int cNext = row_cell(E, p1, z1, t), sNext = E.slot[p1 + (t < z1 ? t : 0)];
     for (int i = 0; i < nl; ++i) {
         T xv[PF];
-        row_gather(a, cur, xv);
+        row_gather_lds(xs, cur, xv);
         asm volatile("" ::: "memory");  // the gathers go out before the loads below
         const int n1 = lev(i + 1), n2 = lev(i + 2);
         const int nSize = sOff[n1 + 1] - sOff[n1], nWidth = sWidth[n1], nBase = sBase[n1];
         nxt.c = cNext;
+        nxt.s = sNext;
         row_load_start<OP>(a, nxt);
         row_load_entries(E, a, nSize, nWidth, nBase, t, nxt);
-        cNext = row_cell(E, sOff[n2], sOff[n2 + 1] - sOff[n2], t);
-        if (cur.has) row_finish<OP>(E, a, cur, xv, size, width, base, t);
-        __syncthreads();
+        const int p2 = sOff[n2], z2 = sOff[n2 + 1] - p2;
+        cNext = row_cell(E, p2, z2, t);
+        sNext = E.slot[p2 + (t < z2 ? t : 0)];
+        if (cur.has) row_finish_lds<OP>(E, a, xs, cur, xv, size, width, base, t);
+        lds_barrier();
         cur = nxt;
         size = nSize;
         width = nWidth;
@@ -271,13 +323,16 @@ static void launch_ell(hipStream_t st, const LevelEll& E, int firstLevel, const 
         if (L0 >= L1) continue;
         switch (E.grpKindH[g]) {
         case ELL_RUN256:
-            hipLaunchKernelGGL((k_ell_run<OP, 256, 24, T>), dim3(1), dim3(256), 0, st, E, a, L0, L1, done);
+            hipLaunchKernelGGL((k_ell_run<OP, 256, 24, T>), dim3(1), dim3(256), 0, st, E, a, L0, L1,
+                               E.runCells + E.runOffH[g], E.runCntH[g], done);
             break;
         case ELL_RUN512:
-            hipLaunchKernelGGL((k_ell_run<OP, 512, 16, T>), dim3(1), dim3(512), 0, st, E, a, L0, L1, done);
+            hipLaunchKernelGGL((k_ell_run<OP, 512, 16, T>), dim3(1), dim3(512), 0, st, E, a, L0, L1,
+                               E.runCells + E.runOffH[g], E.runCntH[g], done);
             break;
         case ELL_RUN1024:
-            hipLaunchKernelGGL((k_ell_run<OP, 1024, 12, T>), dim3(1), dim3(1024), 0, st, E, a, L0, L1, done);
+            hipLaunchKernelGGL((k_ell_run<OP, 1024, 12, T>), dim3(1), dim3(1024), 0, st, E, a, L0, L1,
+                               E.runCells + E.runOffH[g], E.runCntH[g], done);
             break;
         default:
             for (int L = L0; L < L1; ++L) {

@@ -438,10 +438,19 @@ void launch_dilu_factor(hipStream_t st, const SweepView& s, const LevelSched& fw
 // (ELL_GRID), or a run of at most ELL_RUN_LEVELS consecutive levels swept by one
 // workgroup, a row per thread, whose levels hold at most 256, 512 or 1024 cells
 // (ELL_RUN256 < ELL_RUN512 < ELL_RUN1024: the larger class covers the smaller).
+// A run keeps x in LDS: every cell its rows write or read has a slot there (at most
+// RUN_LDS_CELLS, runCells[runOff[g] ..] in slot order), filled from x when the run
+// starts and updated in place; entries of run rows hold the neighbour's slot instead of
+// its cell, and slot[pos] is the slot of the row's own cell.
 constexpr int ELL_GRID = 0, ELL_RUN256 = 1, ELL_RUN512 = 2, ELL_RUN1024 = 3;
 constexpr int ELL_RUN_LEVELS = 2048;
+constexpr int RUN_LDS_CELLS = 8192;
 struct LevelEll {
     const int* cell = nullptr;    // device
+    const int* slot = nullptr;    // device: per position, the LDS slot of its cell (runs)
+    const int* runCells = nullptr;  // device: cells of each run's LDS slots
+    const int* runOffH = nullptr;   // host: per group, first entry in runCells
+    const int* runCntH = nullptr;   // host: per group, number of slots (0: grid)
     const int* col = nullptr;     // device
     const int* offD = nullptr;    // device: nLev + 1 level offsets
     const int* widthD = nullptr;  // device: entries per row of each level
