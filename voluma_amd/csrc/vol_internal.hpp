@@ -444,7 +444,7 @@ void launch_dilu_factor(hipStream_t st, const SweepView& s, const LevelSched& fw
 // its cell, and slot[pos] is the slot of the row's own cell.
 constexpr int ELL_GRID = 0, ELL_RUN256 = 1, ELL_RUN512 = 2, ELL_RUN1024 = 3;
 constexpr int ELL_RUN_LEVELS = 2048;
-constexpr int RUN_LDS_CELLS = 8192;
+constexpr int RUN_LDS_CELLS = 16384;
 struct LevelEll {
     const int* cell = nullptr;    // device
     const int* slot = nullptr;    // device: per position, the LDS slot of its cell (runs)
