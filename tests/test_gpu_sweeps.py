@@ -157,3 +157,39 @@ def test_gamg_gauss_seidel_fast_mode_vs_reference(ctx, name):
         va.solve_ldu(A, b, xm, va.SolverControls("PCG", "GAMG", gamgSmoother=7))
     A.close()
     lm.close()
+
+
+def test_sweeps_with_split_runs_bit_exact_vs_oracle(ctx):
+    """A 48k-cell tet mesh (20³ hexes × 6): its forward levels all hold ≤ 1024 cells, so
+    the one-workgroup runs together cover far more cells than one run's 16384 LDS slots
+    and are cut at level boundaries. Gauss–Seidel (2 sweeps), DILU-PBiCGStab and
+    PCG + GAMG smoothed by Gauss–Seidel, with exact sums: bit-identical to the oracle."""
+    from voluma_amd import mesh as M
+    m = M.tet(20, 20, 20, 0.2, 1)
+    ni, n = m.n_internal, m.n_cells
+    w = m.agglomeration_weights()
+    rng = np.random.default_rng(4)
+    u = -w
+    d = np.zeros(n)
+    np.add.at(d, m.owner[:ni], w)
+    np.add.at(d, m.neighbour, w)
+    d = d + 0.05 * w.mean()
+    lo = u * rng.uniform(0.8, 1.2, ni)
+    b = rng.uniform(-1, 1, n)
+    lm = va.LduMesh(ctx, n, m.owner[:ni], m.neighbour, w)
+    om = O.LduMesh(n, m.owner[:ni], m.neighbour, w)
+    cases = [("GaussSeidel", "none", False, dict(n_sweeps=2), dict(nSweeps=2)),
+             ("PBiCGStab", "DILU", False, {}, {}),
+             ("PCG", "GAMG", True, dict(smoother=O.SMOOTHER_GS), dict(gamgSmoother=va.GAMG_GAUSS_SEIDEL))]
+    for solver, pre, sym, okw, vkw in cases:
+        lower = None if sym else lo
+        tol, mx = (1e-6, 40) if solver == "GaussSeidel" else (1e-10, 300)
+        xo, po = om.solve(d, u, lower, b, solver=solver, preconditioner=pre, tolerance=tol, max_iter=mx,
+                          mode=O.MODE_EXACT, **okw)
+        A = va.LduMatrix(lm, d, u, lower)
+        x = np.zeros(n)
+        p = va.solve_ldu(A, b, x, va.SolverControls(solver, pre, tol, 0.0, mx, sums=va.SUMS_EXACT, **vkw))
+        assert p.iterations == po["iterations"] and p.finalResidual == po["finalResidual"], solver
+        assert np.array_equal(x, xo), solver
+        A.close()
+    lm.close()
