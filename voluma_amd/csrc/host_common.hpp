@@ -466,6 +466,13 @@ inline void buildCoarseLayout(int n, const std::vector<int>& off, const std::vec
                               const std::vector<int>& src, SellDev& S, hipStream_t st) {
     constexpr int TUNE_MAX_ROWS = 1 << 22;
     bool csr = false;
+    // VOLUMA_COARSE_LAYOUT=csr / sell fixes the choice (measurement override)
+    const char* force = std::getenv("VOLUMA_COARSE_LAYOUT");
+    const std::string f = force ? force : "";
+    if (f == "csr" || f == "sell") {
+        if (!(f == "csr" && buildCsr(n, off, col, src, S, st))) buildSell(n, off, col, src, S, st);
+        return;
+    }
     if (n <= TUNE_MAX_ROWS) {
         SellDev a, b;
         if (buildCsr(n, off, col, src, a, st)) {
