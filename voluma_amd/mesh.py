@@ -263,12 +263,11 @@ class Mesh:
         n_int = ni + len(cyc_face)
         out = {"patch_names": [], "patch_types": [], "patch_start": [], "patch_size": []}
         n = n_int
-        empty_n = np.zeros(3)
+        empty_faces = []
         for q, kind in enumerate(self.patch_types):
             f = self.patch_faces(q)
             if kind == "empty":
-                for v in np.abs(Sf[f]):  # in face order, as the reference sums
-                    empty_n += v
+                empty_faces.append(f)
                 continue
             if kind == "cyclic":
                 continue
@@ -282,6 +281,9 @@ class Mesh:
         two_d = "empty" in self.patch_types
         d = -1
         if two_d:
+            # Σ|S_f| over the empty faces, one running sum in patch then face order, as
+            # the reference sums (numpy's cumsum adds sequentially)
+            empty_n = np.cumsum(np.abs(Sf[np.concatenate(empty_faces)]), axis=0)[-1]
             d = 0
             for a in (1, 2):
                 if empty_n[a] > empty_n[d]:
