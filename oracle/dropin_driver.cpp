@@ -14,6 +14,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <string>
 #include <vector>
 
@@ -29,7 +30,18 @@ static uint64_t fnv1a(const std::vector<double>& v, label n) {
     return h;
 }
 
+static int run(int argc, char** argv);
+
 int main(int argc, char** argv) {
+    try {
+        return run(argc, argv);
+    } catch (const std::exception& e) {  // a failed solve is reported, not an abort
+        std::fprintf(stderr, "dropin: %s\n", e.what());
+        return 1;
+    }
+}
+
+static int run(int argc, char** argv) {
     par::Environment env(argc, argv);
     const int nx = argc > 1 ? std::atoi(argv[1]) : 24;
     BoxSpec s;

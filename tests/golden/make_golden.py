@@ -10,6 +10,9 @@ voluma_amd.mesh, and stores inputs and outputs as data:
 * cfg1_box64.json, cfg2_tet70.json — the two full-size pressure systems: input and
   hierarchy hashes, level sizes, iteration counts, residuals, solution hashes,
   norms and samples for reproducible diagonal-PCG and default GAMG-PCG;
+* cfg1_box64_tight.json, cfg2_tet70_tight.json (--tight) — the benchmark's pressure
+  systems (non-orthogonal correction included): the reference's GAMG-PCG to 1e-13
+  (hash, norm, 4096 samples) and the oracle's exact-sum Jacobi-GAMG-PCG hash;
 * exactsum.json — ExactSum outputs for seeded random vectors;
 * dropin_expected.jsonl — output of the Voluma drop-in check program with the reference solver.
 
@@ -508,6 +511,69 @@ def full_case(tag, name):
     print(f"{tag}_{name}: {n} cells, levels {res['gamg_sizes']}, geometry equal {geom_equal}", flush=True)
 
 
+def dense_summary(x, k=4096):
+    """summary() plus k evenly spaced entries, for relative-error checks at full size."""
+    x = np.asarray(x)
+    out = summary(x)
+    idx = np.linspace(0, x.size - 1, k).astype(np.int64)
+    out["dense_idx_step"] = [int(idx[0]), int(idx[-1]), k]
+    out["dense"] = [float(v) for v in x[idx]]
+    return out
+
+
+# Jacobi V-cycle of the benchmark's pressure solve: 1 pre- and 1 post-sweep, ω = 0.9
+BENCH_VCYCLE = {"n_pre": 1, "n_post": 1, "omega": 0.9}
+
+
+def tight_case(tag, name):
+    """Full-size pressure system as the benchmark builds it (W.pressure_system with
+    nonorth=True: fvm::laplacian(1, p) with its non-orthogonal correction on p⁰ =
+    W.pressure_psi0, Gauss-linear gradient, fixedValue 0 on xMax), then
+    * the reference's GAMG-PCG (Gauss–Seidel V-cycle, plain sums) to normalised residual
+      1e-13 — the tight solution the GPU's solvers are compared with;
+    * the oracle's GAMG-PCG at the same tolerance (checked bit-equal to the reference);
+    * the oracle's exact-sum GAMG-PCG with the benchmark's Jacobi V-cycle to 1e-6 (its
+      bits are what the GPU's exact-sum mode must reproduce)."""
+    from oracle import oracle as O
+    m = W.make_mesh(name)
+    rm = ref.RefMesh(mesh=m)
+    n, ni = m.n_cells, m.n_internal
+    g = m.geometry()
+    psi = W.pressure_psi0(g["C"])
+    lap = rm.laplacian([W.XMAX_PATCH], psi=psi, non_orth=True, grad_scheme=0)
+    d = lap["diag"].copy()
+    np.add.at(d, m.owner[ni:], lap["ic"])
+    bf = lap["source"].copy()
+    np.add.at(bf, m.owner[ni:], lap["bs"])
+    b = W.pressure_rhs(n, seed=1) + bf
+    res = {"mesh": name, "n_cells": n, "n_internal_faces": ni, "mesh_sha256": mesh_hash(m),
+           "psi0_sha256": sha(psi), "diag_sha256": sha(d), "upper_sha256": sha(lap["upper"]), "b_sha256": sha(b),
+           "ffc_sha256": sha(lap["ffc"]), "bench_vcycle": BENCH_VCYCLE}
+    t = time.time()
+    x, p = rm.solve(d, lap["upper"], None, b, np.zeros(n), solver="PCG", preconditioner="GAMG", tolerance=1e-13,
+                    max_iter=5000)
+    res["ref_gamg_pcg_1e-13"] = {"iterations": p["iterations"], "initialResidual": p["initialResidual"],
+                                 "finalResidual": p["finalResidual"], "normFactor": p["normFactor"],
+                                 "converged": p["converged"], "solve_s": time.time() - t, "x": dense_summary(x)}
+    print(f"  {tag} ref GS-GAMG-PCG 1e-13: {p['iterations']} iters, {time.time() - t:.1f} s", flush=True)
+    om = O.LduMesh(n, m.owner[:ni], m.neighbour, m.agglomeration_weights())
+    t = time.time()
+    xo, po = om.solve(d, lap["upper"], None, b, preconditioner="GAMG", tolerance=1e-13, max_iter=5000)
+    res["oracle_gamg_pcg_1e-13_equal_to_reference"] = bool(np.array_equal(xo, x) and
+                                                           po["iterations"] == p["iterations"])
+    print(f"  {tag} oracle GS-GAMG-PCG 1e-13: equal {res['oracle_gamg_pcg_1e-13_equal_to_reference']}, "
+          f"{time.time() - t:.1f} s", flush=True)
+    t = time.time()
+    xe, pe = om.solve(d, lap["upper"], None, b, preconditioner="GAMG", tolerance=1e-6, max_iter=5000,
+                      mode=O.MODE_EXACT, smoother=O.SMOOTHER_JACOBI, **BENCH_VCYCLE)
+    res["oracle_exact_jacobi_gamg_pcg_1e-6"] = {"iterations": pe["iterations"], "finalResidual": pe["finalResidual"],
+                                                "normFactor": pe["normFactor"], "x": summary(xe)}
+    print(f"  {tag} oracle exact Jacobi-GAMG-PCG 1e-6: {pe['iterations']} iters, {time.time() - t:.1f} s",
+          flush=True)
+    with open(os.path.join(HERE, f"{tag}_{name}_tight.json"), "w") as fh:
+        json.dump(res, fh, indent=1)
+
+
 def momentum_case(tag, name):
     """Momentum matrix of every velocity component on the full mesh (reference
     fvm::ddt + fvm::div − fvm::laplacian with non-orthogonal correction, fixedValue U
@@ -585,6 +651,10 @@ def dropin_case():
 
 
 if __name__ == "__main__":
+    if "--tight" in sys.argv:
+        tight_case("cfg1", "box64")
+        tight_case("cfg2", "tet70")
+        sys.exit(0)
     if "--momentum-dilu" in sys.argv:
         momentum_dilu_case("cfg2", "tet70")
         sys.exit(0)

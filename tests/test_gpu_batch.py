@@ -27,8 +27,8 @@ def small_batch(ctx):
     return specs, B, b, meshes
 
 
-def _ctl(pre="GAMG", tol=1e-10, **kw):
-    return va.SolverControls("PCG", pre, tol, 0.0, 1000, **kw)
+def _ctl(pre="GAMG", tol=1e-10, maxIter=1000, **kw):
+    return va.SolverControls("PCG", pre, tol, 0.0, maxIter, **kw)
 
 
 def _slice(B, v, s):
@@ -101,3 +101,76 @@ def test_batch_errors(ctx, small_batch):
     with pytest.raises(va.VolumaError) as e:
         va.LduBatch(ctx, [(m.n_cells, m.owner[: m.n_internal], m.neighbour, m.agglomeration_weights())])
     assert e.value.code == -1
+
+
+# ---------------------------------------------------------------- the benchmark's batch
+BENCH_VCYCLE = dict(gamgPreSweeps=1, gamgPostSweeps=1, gamgOmega=0.9)
+
+
+@pytest.fixture(scope="module")
+def bench_batch(ctx):
+    """The benchmark's batch: 256 systems, lognormal 5k–200k cells (median 30k)."""
+    specs = W.batch_specs(n_systems=256)
+    print(f"[batch256] building {len(specs)} systems", flush=True)
+    B, b, meshes = W.batch_system(ctx, specs)
+    print(f"[batch256] {B.n_total} cells", flush=True)
+    yield specs, B, b, meshes
+    B.close()
+
+
+def _true_residuals(B, b, x, meshes, perf):
+    """Normalised true residual Σ|b − A·x| / normFactor of every system, A·x by the
+    oracle's amul (bit-identical to the GPU's)."""
+    from oracle import oracle as O
+    out = []
+    for s, m in enumerate(meshes):
+        c0, c1 = B.cell_off[s], B.cell_off[s + 1]
+        f0, f1 = B.face_off[s], B.face_off[s + 1]
+        om = O.LduMesh(m.n_cells, m.owner[: m.n_internal], m.neighbour, None)
+        r = b[c0:c1] - om.amul(B.diag_h[c0:c1], B.upper_h[f0:f1], None, x[c0:c1])
+        out.append(float(np.abs(r).sum() / perf[s].normFactor))
+    return np.array(out)
+
+
+def test_bench_batch_fp64_true_residual_and_batch_mates(ctx, bench_batch):
+    """Every system of the 256 converges and its true residual (recomputed with the
+    bit-exact A·x) is at the 1e-6 target; the smallest, median and largest systems
+    solved alone (a batch of one) are bit-identical to their solution in the batch."""
+    specs, B, b, meshes = bench_batch
+    assert B.n_systems == 256 and 9_000_000 < B.n_total < 12_000_000
+    x = np.zeros(B.n_total)
+    perf = B.solve(b, x, _ctl(tol=1e-6, **BENCH_VCYCLE))
+    assert all(p.converged and p.finalResidual < 1e-6 for p in perf)
+    tr = _true_residuals(B, b, x, meshes, perf)
+    assert np.all(tr <= 1e-6 * (1 + 1e-6)), (int(np.argmax(tr)), float(tr.max()))
+    order = np.argsort(B.n_cells, kind="stable")
+    for k in (int(order[0]), int(order[len(order) // 2]), int(order[-1])):
+        print(f"[batch256] system {k} ({B.n_cells[k]} cells) alone", flush=True)
+        B1, b1, _ = W.batch_system(ctx, [specs[k]], rhs_seed=7 + k)
+        assert np.array_equal(b1, _slice(B, b, k))
+        x1 = np.zeros(B1.n_total)
+        p1 = B1.solve(b1, x1, _ctl(tol=1e-6, **BENCH_VCYCLE))
+        assert p1[0].iterations == perf[k].iterations and p1[0].finalResidual == perf[k].finalResidual
+        assert np.array_equal(x1, _slice(B, x, k))
+        B1.close()
+
+
+def test_bench_batch_mixed_precision(ctx, bench_batch):
+    """Mixed precision on the 256 systems: every solution at residual 1e-10 within 1e-8
+    (relative L2) of the FP64 solution at 1e-12; at 1e-6 every system converges."""
+    specs, B, b, meshes = bench_batch
+    xr = np.zeros(B.n_total)
+    pr = B.solve(b, xr, _ctl(tol=1e-12, maxIter=3000, **BENCH_VCYCLE))
+    assert all(p.converged for p in pr)
+    x = np.zeros(B.n_total)
+    perf = B.solve(b, x, _ctl(tol=1e-10, maxIter=3000, precision=va.MIXED, **BENCH_VCYCLE))
+    errs = []
+    for s in range(B.n_systems):
+        assert perf[s].converged and perf[s].outerIterations >= 1
+        xs, xrs = _slice(B, x, s), _slice(B, xr, s)
+        errs.append(np.linalg.norm(xs - xrs) / np.linalg.norm(xrs))
+    assert max(errs) <= 1e-8, max(errs)
+    x6 = np.zeros(B.n_total)
+    p6 = B.solve(b, x6, _ctl(tol=1e-6, precision=va.MIXED, **BENCH_VCYCLE))
+    assert all(p.converged for p in p6)
+    print(f"[batch256] mixed 1e-10: max rel. error {max(errs):.2e}", flush=True)

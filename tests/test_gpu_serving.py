@@ -98,3 +98,42 @@ def test_hbm_budget_admission(pool):
         tiny.submit(**SV.request_args(pool, rq, va.FP64))
     assert e.value.code == -1
     tiny.close()
+
+
+# ---------------------------------------------------------------- the benchmark's pool
+@pytest.fixture(scope="module")
+def bench_pool(ctx):
+    """The benchmark's pool: 8 meshes, log-uniform 10k–2M cells (30k … 1.86M)."""
+    print("[serve] building the benchmark pool", flush=True)
+    pool = SV.build_pool(ctx, SV.pool_specs())
+    print("[serve] pool cells " + ", ".join(str(e["n"]) for e in pool), flush=True)
+    return pool
+
+
+@pytest.mark.parametrize("precision", [va.FP64, va.MIXED])
+def test_bench_pool_same_alone_and_in_the_mix(bench_pool, precision):
+    """Requests of the benchmark's mix (60 % pressure GAMG-PCG / 40 % momentum
+    PBiCGStab, 2–3 right-hand sides each, 200 GB budget, 4 workers) arriving faster than
+    they are served: a sample that includes pressure and momentum requests on the
+    1.86M-cell mesh gives the same bits alone as in the mix."""
+    pool = bench_pool
+    big = max(range(len(pool)), key=lambda i: pool[i]["n"])
+    assert pool[big]["n"] > 1_800_000
+    stream = SV.request_stream(pool, rate=200.0, n_requests=24, seed=9)
+    t_end = stream[-1]["t"]
+    stream += [{"t": t_end + 1e-3, "kind": "pressure", "pool": big, "nrhs": 3, "seed": 900},
+               {"t": t_end + 2e-3, "kind": "momentum", "pool": big, "nrhs": 3, "seed": 901}]
+    server = va.Server(0, slots=4, hbm_budget=200e9)
+    SV.register_pool(server, pool)
+    mixed, stats = SV.run_stream(server, pool, stream, precision, timeout=120.0)
+    assert stats["requests"] == len(stream)
+    sample = sorted(set([k for k, rq in enumerate(stream) if rq["pool"] == big] + [0, 5, 11, 17]))
+    print(f"[serve] {len(sample)} requests again alone", flush=True)
+    alone = _alone(server, pool, [stream[k] for k in sample], precision)
+    for k, (xa, pa) in zip(sample, alone):
+        xm, pm = mixed[k]
+        assert all(p.converged and p.finalResidual < 1e-6 for p in pm), k
+        assert [p.iterations for p in pa] == [p.iterations for p in pm], k
+        for a, b in zip(xa, xm):
+            assert np.array_equal(a, b), k
+    server.close()

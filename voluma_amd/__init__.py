@@ -92,6 +92,17 @@ def _q(a):
     return None if a is None else _p(a)
 
 
+def _sized(a, count, name, dtype=np.float64):
+    """Contiguous copy of a with `count` elements of dtype, else ValueError: the C ABI
+    copies exactly that many elements and cannot check a caller's array."""
+    if a is None:
+        raise ValueError(f"{name} is required")
+    arr = np.ascontiguousarray(a, dtype=dtype)
+    if arr.size != count:
+        raise ValueError(f"{name}: expected {count} elements, got {arr.size} (shape {arr.shape})")
+    return arr
+
+
 _LIB = None
 EXPORTED = ("vol_controls_default", "vol_last_error", "vol_context_create", "vol_context_destroy",
             "vol_set_reproducible", "vol_synchronize", "vol_device_alloc", "vol_device_free", "vol_memcpy_to_device",
@@ -364,113 +375,144 @@ class LduMesh:
         """2-D mesh: its empty direction (−1 for 3-D) for the gradients."""
         _check(lib().vol_mesh_set_empty_dir(self._h, int(empty_dir)))
 
-    def boundary_values(self, b_fixed, b_value, c):
-        """fixedValue (b_fixed 1: b_value) / zeroGradient (the owner's value) boundary values."""
-        c = np.ascontiguousarray(c, dtype=np.float64)
-        nc = 1 if c.ndim == 1 else 3
-        out = np.zeros((max(self.n_boundary, 1), nc))
-        _check(lib().vol_boundary_values(self._h, nc, _p(np.ascontiguousarray(b_fixed, dtype=np.int32)),
-                                         _p(np.ascontiguousarray(b_value, dtype=np.float64)), _p(c), _p(out),
-                                         VOL_HOST))
-        out = out[: self.n_boundary]
-        return out[:, 0] if nc == 1 else out
-
     def _nf_total(self):
         return self.n_faces + getattr(self, "n_boundary", 0)
 
+    def _ncomp(self, c, name):
+        """1 for a scalar cell field (n), 3 for a vector one (n × 3 or 3n flat with ndim 2)."""
+        c = np.asarray(c)
+        if c.ndim == 1:
+            return 1
+        if c.ndim == 2 and c.shape[1] == 3:
+            return 3
+        raise ValueError(f"{name}: expected shape ({self.n_cells},) or ({self.n_cells}, 3), got {c.shape}")
+
+    def boundary_values(self, b_fixed, b_value, c):
+        """fixedValue (b_fixed 1: b_value) / zeroGradient (the owner's value) boundary values."""
+        nc = self._ncomp(c, "c")
+        nb = self.n_boundary
+        c = _sized(c, nc * self.n_cells, "c")
+        bf = _sized(b_fixed, nb, "b_fixed", np.int32)
+        bv = _sized(b_value, nc * nb, "b_value")
+        out = np.zeros((max(nb, 1), nc))
+        _check(lib().vol_boundary_values(self._h, nc, _p(bf), _p(bv), _p(c), _p(out), VOL_HOST))
+        out = out[:nb]
+        return out[:, 0] if nc == 1 else out
+
     def fvc_interpolate(self, c, grad=None, boundary_values=None):
-        c = np.ascontiguousarray(c, dtype=np.float64)
-        nc = 1 if c.ndim == 1 else 3
+        nc = self._ncomp(c, "c")
+        n, nb = self.n_cells, self.n_boundary
+        c = _sized(c, nc * n, "c")
+        g = None if grad is None else _sized(grad, 3 * nc * n, "grad")
+        bv = None if boundary_values is None else _sized(boundary_values, nc * nb, "boundary_values")
         out = np.zeros((max(self._nf_total(), 1), nc))
-        g = None if grad is None else np.ascontiguousarray(grad, dtype=np.float64)
-        bv = None if boundary_values is None else np.ascontiguousarray(boundary_values, dtype=np.float64)
         _check(lib().vol_fvc_interpolate(self._h, nc, _p(c), _q(g), _q(bv), _p(out), VOL_HOST))
         out = out[: self._nf_total()]
         return out[:, 0] if nc == 1 else out
 
     def fvc_grad(self, scheme, c, b_fixed, b_value):
-        c = np.ascontiguousarray(c, dtype=np.float64)
-        nc = 1 if c.ndim == 1 else 3
+        nc = self._ncomp(c, "c")
+        nb = self.n_boundary
+        c = _sized(c, nc * self.n_cells, "c")
+        bf = _sized(b_fixed, nb, "b_fixed", np.int32)
+        bv = _sized(b_value, nc * nb, "b_value")
         out = np.zeros((self.n_cells, 3 * nc))
-        bf = np.ascontiguousarray(b_fixed, dtype=np.int32)
-        bv = np.ascontiguousarray(b_value, dtype=np.float64)
         _check(lib().vol_fvc_grad(self._h, int(scheme), nc, _p(c), _p(bf), _p(bv), _p(out), VOL_HOST))
         return out
 
     def fvc_div(self, F):
+        F = _sized(F, self._nf_total(), "F")
         out = np.zeros(self.n_cells)
-        _check(lib().vol_fvc_div(self._h, _p(np.ascontiguousarray(F, dtype=np.float64)), _p(out), VOL_HOST))
+        _check(lib().vol_fvc_div(self._h, _p(F), _p(out), VOL_HOST))
         return out
 
     def fvc_surface_integrate(self, F):
-        F = np.ascontiguousarray(F, dtype=np.float64)
+        F = np.asarray(F)
+        if F.ndim not in (1, 2) or (F.ndim == 2 and F.shape[1] != 3):
+            raise ValueError(f"F: expected shape ({self._nf_total()},) or ({self._nf_total()}, 3), got {F.shape}")
         nc = 1 if F.ndim == 1 else 3
+        F = _sized(F, nc * self._nf_total(), "F")
         out = np.zeros((self.n_cells, nc))
         _check(lib().vol_fvc_surface_integrate(self._h, nc, _p(F), _p(out), VOL_HOST))
         return out[:, 0] if nc == 1 else out
 
     def fvc_flux(self, U, grad_U, b_fixed, Ub):
+        n, nb = self.n_cells, self.n_boundary
+        U = _sized(U, 3 * n, "U")
+        g = None if grad_U is None else _sized(grad_U, 9 * n, "grad_U")
+        bf = _sized(b_fixed, nb, "b_fixed", np.int32)
+        ub = _sized(Ub, 3 * nb, "Ub")
         out = np.zeros(max(self._nf_total(), 1))
-        g = None if grad_U is None else np.ascontiguousarray(grad_U, dtype=np.float64)
-        _check(lib().vol_fvc_flux(self._h, _p(np.ascontiguousarray(U, dtype=np.float64)), _q(g),
-                                  _p(np.ascontiguousarray(b_fixed, dtype=np.int32)),
-                                  _p(np.ascontiguousarray(Ub, dtype=np.float64)), _p(out), VOL_HOST))
+        _check(lib().vol_fvc_flux(self._h, _p(U), _q(g), _p(bf), _p(ub), _p(out), VOL_HOST))
         return out[: self._nf_total()]
 
     # ---- pressure-corrector steps (IncompressibleFlow.cpp:17-29,242-362)
     def piso_rau_hbya(self, A, H):
         """(rAU, HbyA) = (1/A, rAU·H)."""
-        A = np.ascontiguousarray(A, dtype=np.float64)
-        rau, hbya = np.zeros(self.n_cells), np.zeros((self.n_cells, 3))
-        _check(lib().vol_piso_rau_hbya(self._h, _p(A), _p(np.ascontiguousarray(H, dtype=np.float64)), _p(rau),
-                                       _p(hbya), VOL_HOST))
+        n = self.n_cells
+        A, H = _sized(A, n, "A"), _sized(H, 3 * n, "H")
+        rau, hbya = np.zeros(n), np.zeros((n, 3))
+        _check(lib().vol_piso_rau_hbya(self._h, _p(A), _p(H), _p(rau), _p(hbya), VOL_HOST))
         return rau, hbya
 
     def piso_hbya_boundary(self, b_fixed, Ub, rAU, gradp):
         """Boundary values of HbyA for its least-squares gradient: U_b + rAU_P ∇p_P on
         fixed-flux faces, 0 elsewhere."""
-        out = np.zeros((max(self.n_boundary, 1), 3))
-        f = lambda a: np.ascontiguousarray(a, dtype=np.float64)
-        _check(lib().vol_piso_hbya_boundary(self._h, _p(np.ascontiguousarray(b_fixed, dtype=np.int32)), _p(f(Ub)),
-                                            _p(f(rAU)), _p(f(gradp)), _p(out), VOL_HOST))
-        return out[: self.n_boundary]
+        n, nb = self.n_cells, self.n_boundary
+        bf = _sized(b_fixed, nb, "b_fixed", np.int32)
+        ub, ra, gp = _sized(Ub, 3 * nb, "Ub"), _sized(rAU, n, "rAU"), _sized(gradp, 3 * n, "gradp")
+        out = np.zeros((max(nb, 1), 3))
+        _check(lib().vol_piso_hbya_boundary(self._h, _p(bf), _p(ub), _p(ra), _p(gp), _p(out), VOL_HOST))
+        return out[:nb]
 
     def piso_face_op(self, op, a, b, c=None):
         """Face-wise a + b·c, a − b or a − b/c (FACE_* constants)."""
-        f = lambda x: None if x is None else np.ascontiguousarray(x, dtype=np.float64)
-        out = np.zeros(max(self._nf_total(), 1))
-        _check(lib().vol_piso_face_op(self._h, int(op), _p(f(a)), _p(f(b)), _q(f(c)), _p(out), VOL_HOST))
-        return out[: self._nf_total()]
+        nF = self._nf_total()
+        a, b = _sized(a, nF, "a"), _sized(b, nF, "b")
+        if c is None and op != FACE_A_MINUS_B:
+            raise ValueError("c is required for this face operation")
+        cc = None if c is None else _sized(c, nF, "c")
+        out = np.zeros(max(nF, 1))
+        _check(lib().vol_piso_face_op(self._h, int(op), _p(a), _p(b), _q(cc), _p(out), VOL_HOST))
+        return out[:nF]
 
     def piso_ddt_phi_corr(self, coeff, c0, c00, U0, U00, phi_old, phi_old_old, rAUf, phiHbyA):
         """phiHbyA with the ddtPhiCorr coupling added (coeff < 0: automatic coupling)."""
-        f = lambda x: None if x is None else np.ascontiguousarray(x, dtype=np.float64)
-        out = f(phiHbyA).copy()
-        _check(lib().vol_piso_ddt_phi_corr(self._h, float(coeff), float(c0), float(c00), _p(f(U0)), _q(f(U00)),
-                                           _p(f(phi_old)), _q(f(phi_old_old)), _p(f(rAUf)), _p(out), VOL_HOST))
+        n, nF = self.n_cells, self._nf_total()
+        if phiHbyA is None:
+            raise ValueError("phiHbyA is required")
+        u0 = _sized(U0, 3 * n, "U0")
+        u00 = None if U00 is None else _sized(U00, 3 * n, "U00")
+        po = _sized(phi_old, nF, "phi_old")
+        poo = None if phi_old_old is None else _sized(phi_old_old, nF, "phi_old_old")
+        ra = _sized(rAUf, nF, "rAUf")
+        out = _sized(phiHbyA, nF, "phiHbyA").copy()
+        _check(lib().vol_piso_ddt_phi_corr(self._h, float(coeff), float(c0), float(c00), _p(u0), _q(u00), _p(po),
+                                           _q(poo), _p(ra), _p(out), VOL_HOST))
         return out
 
     def piso_constrain_pressure(self, start, size, phiHbyA, Ub, rAUf):
         """fixedFluxPressure gradient of boundary faces [start, start + size)."""
-        f = lambda x: np.ascontiguousarray(x, dtype=np.float64)
+        nF, nb = self._nf_total(), self.n_boundary
+        ph, ub, ra = _sized(phiHbyA, nF, "phiHbyA"), _sized(Ub, 3 * nb, "Ub"), _sized(rAUf, nF, "rAUf")
         out = np.zeros(max(size, 1))
-        _check(lib().vol_piso_constrain_pressure(self._h, int(start), int(size), _p(f(phiHbyA)), _p(f(Ub)),
-                                                 _p(f(rAUf)), _p(out), VOL_HOST))
+        _check(lib().vol_piso_constrain_pressure(self._h, int(start), int(size), _p(ph), _p(ub), _p(ra), _p(out),
+                                                 VOL_HOST))
         return out[:size]
 
     def fvc_reconstruct(self, flux, empty_dir=-1):
         """fvc::reconstruct of a face flux: nCells × 3."""
+        fl = _sized(flux, self._nf_total(), "flux")
         out = np.zeros((self.n_cells, 3))
-        _check(lib().vol_fvc_reconstruct(self._h, _p(np.ascontiguousarray(flux, dtype=np.float64)), int(empty_dir),
-                                         _p(out), VOL_HOST))
+        _check(lib().vol_fvc_reconstruct(self._h, _p(fl), int(empty_dir), _p(out), VOL_HOST))
         return out
 
     def piso_correct_U(self, HbyA, rAU, v, sign, empty_dir=-1):
         """U = HbyA − rAU·v (sign < 0) or HbyA + rAU·v, component empty_dir zeroed."""
-        f = lambda x: np.ascontiguousarray(x, dtype=np.float64)
-        out = np.zeros((self.n_cells, 3))
-        _check(lib().vol_piso_correct_U(self._h, _p(f(HbyA)), _p(f(rAU)), _p(f(v)), int(sign), int(empty_dir),
-                                        _p(out), VOL_HOST))
+        n = self.n_cells
+        h, ra, vv = _sized(HbyA, 3 * n, "HbyA"), _sized(rAU, n, "rAU"), _sized(v, 3 * n, "v")
+        out = np.zeros((n, 3))
+        _check(lib().vol_piso_correct_U(self._h, _p(h), _p(ra), _p(vv), int(sign), int(empty_dir), _p(out), VOL_HOST))
         return out
 
     def gamg_levels(self):
@@ -562,10 +604,11 @@ class LduMatrix:
         """fvm::laplacian(γ, ψ) + boundary fold on the GPU; returns the folded source b
         (b_in + boundary source) and optionally (internalCoeffs, boundarySource)."""
         m = self.mesh
-        f64 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
-        bcf = np.ascontiguousarray(bc_fixed, dtype=np.int32)
-        bv, fg, bi = f64(bc_value), f64(face_gamma), f64(b_in)
-        ms, nd = f64(magSf), f64(ndc)
+        n, nb, nF = m.n_cells, m.n_boundary, m.n_faces + m.n_boundary
+        opt = lambda a, cnt, name: None if a is None else _sized(a, cnt, name)
+        bcf = _sized(bc_fixed, nb, "bc_fixed", np.int32)
+        bv, fg, bi = opt(bc_value, nb, "bc_value"), opt(face_gamma, nF, "face_gamma"), opt(b_in, n, "b_in")
+        ms, nd = _sized(magSf, nF, "magSf"), _sized(ndc, nF, "ndc")
         b = np.zeros(m.n_cells)
         ic = np.zeros(max(m.n_boundary, 1))
         bs = np.zeros(max(m.n_boundary, 1))
@@ -580,9 +623,10 @@ class LduMatrix:
     def assemble_div_upwind(self, flux, bc_fixed, bc_value=None, b_in=None, return_boundary=False):
         """fvm::div(F, ψ, upwind) + boundary fold on the GPU (nonsymmetric)."""
         m = self.mesh
-        f64 = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
-        bcf = np.ascontiguousarray(bc_fixed, dtype=np.int32)
-        F, bv, bi = f64(flux), f64(bc_value), f64(b_in)
+        n, nb, nF = m.n_cells, m.n_boundary, m.n_faces + m.n_boundary
+        opt = lambda a, cnt, name: None if a is None else _sized(a, cnt, name)
+        bcf = _sized(bc_fixed, nb, "bc_fixed", np.int32)
+        F, bv, bi = _sized(flux, nF, "flux"), opt(bc_value, nb, "bc_value"), opt(b_in, n, "b_in")
         b = np.zeros(m.n_cells)
         ic = np.zeros(max(m.n_boundary, 1))
         bs = np.zeros(max(m.n_boundary, 1))
@@ -600,37 +644,50 @@ class LduMatrix:
         self._fvm_lower = False
 
     def fvm_ddt_euler(self, sign, dt, V, psi0):
-        _check(lib().vol_fvm_ddt_euler(self._h, int(sign), ctypes.c_double(dt), _p(_f64(V)), _p(_f64(psi0)),
-                                       VOL_HOST))
+        n = self.mesh.n_cells
+        _check(lib().vol_fvm_ddt_euler(self._h, int(sign), ctypes.c_double(dt), _p(_sized(V, n, "V")),
+                                       _p(_sized(psi0, n, "psi0")), VOL_HOST))
 
     def fvm_ddt(self, sign, coeffs, V, psi0, psi00=None):
         """fvm::ddt with ddtCoeffs (c, c0, c00): diag += c·V, source += V·(c0·ψ⁰ − c00·ψ⁰⁰)."""
         c, c0, c00 = (float(v) for v in coeffs)
-        _check(lib().vol_fvm_ddt(self._h, int(sign), c, c0, c00, _p(_f64(V)), _p(_f64(psi0)), _q(_f64n(psi00)),
-                                 VOL_HOST))
+        n = self.mesh.n_cells
+        p00 = None if psi00 is None else _sized(psi00, n, "psi00")
+        _check(lib().vol_fvm_ddt(self._h, int(sign), c, c0, c00, _p(_sized(V, n, "V")), _p(_sized(psi0, n, "psi0")),
+                                 _q(p00), VOL_HOST))
 
     def fvm_div_upwind(self, sign, flux, bc_fixed, bc_value=None):
-        bcf = np.ascontiguousarray(bc_fixed, dtype=np.int32)
-        _check(lib().vol_fvm_div_upwind(self._h, int(sign), _p(_f64(flux)), _p(bcf), _q(_f64n(bc_value)), VOL_HOST))
+        m = self.mesh
+        nb = m.n_boundary
+        bcf = _sized(bc_fixed, nb, "bc_fixed", np.int32)
+        bv = None if bc_value is None else _sized(bc_value, nb, "bc_value")
+        _check(lib().vol_fvm_div_upwind(self._h, int(sign), _p(_sized(flux, m.n_faces + nb, "flux")), _p(bcf), _q(bv),
+                                        VOL_HOST))
         self._fvm_lower = True
 
     def fvm_laplacian(self, sign, magSf, ndc, bc_fixed, bc_value=None, gamma=1.0, face_gamma=None, nonorth=None):
         """nonorth: dict with psi, w, Sf (nFaces×3), corr (nFaces×3), V, or None."""
-        bcf = np.ascontiguousarray(bc_fixed, dtype=np.int32)
+        m = self.mesh
+        n, nb, nF = m.n_cells, m.n_boundary, m.n_faces + m.n_boundary
+        bcf = _sized(bc_fixed, nb, "bc_fixed", np.int32)
         no = None
         keep = []
         if nonorth is not None:
-            arrs = [_f64(nonorth[k]) for k in ("psi", "w", "Sf", "corr", "V")]
+            sizes = {"psi": n, "w": nF, "Sf": 3 * nF, "corr": 3 * nF, "V": n}
+            arrs = [_sized(nonorth[k], sizes[k], "nonorth[%s]" % k) for k in ("psi", "w", "Sf", "corr", "V")]
             keep = arrs
             no = _NonOrth(*(a.ctypes.data for a in arrs))
-        ms, nd, fg, bv = _f64(magSf), _f64(ndc), _f64n(face_gamma), _f64n(bc_value)
+        ms, nd = _sized(magSf, nF, "magSf"), _sized(ndc, nF, "ndc")
+        fg = None if face_gamma is None else _sized(face_gamma, nF, "face_gamma")
+        bv = None if bc_value is None else _sized(bc_value, nb, "bc_value")
         _check(lib().vol_fvm_laplacian(self._h, int(sign), ctypes.c_double(gamma), _q(fg), _p(ms), _p(nd), _p(bcf),
                                        _q(bv), None if no is None else ctypes.byref(no), VOL_HOST))
         del keep
 
     def fvm_source(self, su, V):
         """FvMatrix −= su: source += su·V."""
-        _check(lib().vol_fvm_source(self._h, _p(_f64(su)), _p(_f64(V)), VOL_HOST))
+        n = self.mesh.n_cells
+        _check(lib().vol_fvm_source(self._h, _p(_sized(su, n, "su")), _p(_sized(V, n, "V")), VOL_HOST))
 
     def fvm_get(self):
         m = self.mesh
@@ -652,31 +709,35 @@ class LduMatrix:
         """Fold a copy of the matrix with −= su (source + su·V); the matrix keeps its own
         source. Returns b."""
         b = np.zeros(self.mesh.n_cells)
-        _check(lib().vol_fvm_finish_source(self._h, _p(_f64(su)), _p(_f64(V)), _p(b), VOL_HOST))
+        n = self.mesh.n_cells
+        _check(lib().vol_fvm_finish_source(self._h, _p(_sized(su, n, "su")), _p(_sized(V, n, "V")), _p(b), VOL_HOST))
         self.symmetric = not getattr(self, "_fvm_lower", False)
         return b
 
     # ---- pressure-velocity coupling (FvMatrix::relax / setReference / A / H / flux)
     def fvm_relax(self, alpha, psi):
-        _check(lib().vol_fvm_relax(self._h, ctypes.c_double(alpha), _p(_f64(psi)), VOL_HOST))
+        _check(lib().vol_fvm_relax(self._h, ctypes.c_double(alpha), _p(_sized(psi, self.mesh.n_cells, "psi")),
+                                   VOL_HOST))
 
     def fvm_set_reference(self, cell, value):
         _check(lib().vol_fvm_set_reference(self._h, int(cell), ctypes.c_double(value)))
 
     def fvm_A(self, V, n_components=1):
         out = np.zeros(self.mesh.n_cells)
-        _check(lib().vol_fvm_A(self._h, _p(_f64(V)), int(n_components), _p(out), VOL_HOST))
+        _check(lib().vol_fvm_A(self._h, _p(_sized(V, self.mesh.n_cells, "V")), int(n_components), _p(out), VOL_HOST))
         return out
 
     def fvm_H(self, psi, V, n_components=1):
         out = np.zeros(self.mesh.n_cells)
-        _check(lib().vol_fvm_H(self._h, _p(_f64(psi)), _p(_f64(V)), int(n_components), _p(out), VOL_HOST))
+        n = self.mesh.n_cells
+        _check(lib().vol_fvm_H(self._h, _p(_sized(psi, n, "psi")), _p(_sized(V, n, "V")), int(n_components), _p(out),
+                               VOL_HOST))
         return out
 
     def fvm_flux(self, psi):
         m = self.mesh
         out = np.zeros(max(m.n_faces + m.n_boundary, 1))
-        _check(lib().vol_fvm_flux(self._h, _p(_f64(psi)), _p(out), VOL_HOST))
+        _check(lib().vol_fvm_flux(self._h, _p(_sized(psi, m.n_cells, "psi")), _p(out), VOL_HOST))
         return out[: m.n_faces + m.n_boundary]
 
     def close(self):
@@ -693,7 +754,7 @@ def amul(A: LduMatrix, x, y=None):
     if isinstance(x, DeviceArray):
         _check(lib().vol_amul(A._h, x.ptr, y.ptr, VOL_DEVICE))
         return y
-    xx = np.ascontiguousarray(x, dtype=np.float64)
+    xx = _sized(x, A.mesh.n_cells, "x")
     yy = np.empty_like(xx)
     _check(lib().vol_amul(A._h, _p(xx), _p(yy), VOL_HOST))
     return yy

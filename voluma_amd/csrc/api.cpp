@@ -985,7 +985,11 @@ struct Ctx {
     hipStream_t st;
     const int* done;
     DevScalars* S;
+    const RedScratch* red = nullptr;  // reduction scratch: the context's unless set
+    bool noTiming = false;            // no event recording (no event objects created)
 };
+
+const RedScratch& redOf(const Ctx& C) { return C.red ? *C.red : C.c->red; }
 
 void launchSell(hipStream_t st, int op, int xl, const SellArgs& a) { launch_sell(st, op, xl, a); }
 void launchSell(hipStream_t st, int op, int xl, const SellArgs32& a) { launch_sell32(st, op, xl, a); }
@@ -1028,13 +1032,13 @@ void vec(const Ctx& C, int op, int n, bool exact, const T* a, const T* b = nullp
     v.S = C.S;
     v.done = C.done;
     if constexpr (std::is_same_v<T, double>)
-        launch_vec(C.st, op, v, exact, C.c->red);
+        launch_vec(C.st, op, v, exact, redOf(C));
     else
-        launch_vec32(C.st, op, v, C.c->red);
+        launch_vec32(C.st, op, v, redOf(C));
 }
 
 void fin(const Ctx& C, int op, int nred, bool exact, int nparts) {
-    launch_finalize(C.st, op, nred, exact, nparts, C.c->red, C.S);
+    launch_finalize(C.st, op, nred, exact, nparts, redOf(C), C.S);
 }
 
 // Level-0 matrix in the solve's working precision.
@@ -1053,7 +1057,7 @@ void amulTimed(const Ctx& C, vol_matrix A, const T* p, T* q, const T* dotv, bool
     const SellView V = A->mesh->sell.view();
     const Level0<T> L0 = level0(A, (T*)nullptr);
     hipEvent_t e0 = nullptr, e1 = nullptr;
-    if (c->timing) {
+    if (c->timing && !C.noTiming) {
         if (c->evUsed + 2 > c->evPool.size()) {
             for (int k = 0; k < 64; ++k) {
                 hipEvent_t e;
@@ -1071,7 +1075,7 @@ void amulTimed(const Ctx& C, vol_matrix A, const T* p, T* q, const T* dotv, bool
     a.x = p;
     a.out = q;
     a.dotv = fused ? dotv : nullptr;
-    a.part = fused ? c->red.part : nullptr;
+    a.part = fused ? redOf(C).part : nullptr;
     a.done = C.done;
     a.evStart = e0;
     a.evStop = e1;
@@ -1281,8 +1285,8 @@ void precondApply(const Ctx& C, vol_matrix A, const Plan& P, const T* r, T* w, i
                P.precond == "diagonal" ? level0(A, (T*)nullptr).rD : nullptr);
 }
 
-void readScalars(vol_context c, DevScalars& out) {
-    HIPCHECK(hipMemcpyAsync(c->hostScal, c->scal.p, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
+void readScalars(vol_context c, const DevScalars* S, DevScalars& out) {
+    HIPCHECK(hipMemcpyAsync(c->hostScal, S, sizeof(DevScalars), hipMemcpyDeviceToHost, c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
     out = *c->hostScal;
 }
@@ -1310,7 +1314,7 @@ void krylovIteration(const Ctx& C, vol_matrix A, const Plan& P, const Krylov<T>&
     const Level0<T> L0 = level0(A, (T*)nullptr);
     if (isPCG) {  // LinearSolver.cpp:176-201
         if (P.precond == "GAMG" && !P.gamgGS && fuse && A->tailStart != 0) {  // Σ w·r fused into the last sweep
-            vcycle<T>(C, A, P, 0, k.r, k.w, k.r, C.c->red.part);
+            vcycle<T>(C, A, P, 0, k.r, k.w, k.r, redOf(C).part);
             fin(C, FIN_PCG_WRA, 1, false, sellParts);
         } else {
             if (P.precond == "GAMG" || P.precond == "DIC" || P.precond == "DILU") {
@@ -1362,9 +1366,7 @@ void krylovIteration(const Ctx& C, vol_matrix A, const Plan& P, const Krylov<T>&
 // no-ops if k finished.
 template <class F> void runLoop(vol_context c, const Ctx& C, int maxLoops, F&& iterate) {
     hipStream_t st = C.st;
-    hipEvent_t evs[2];
-    HIPCHECK(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
-    HIPCHECK(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
+    hipEvent_t* evs = c->loopEv;
     DevScalars* hsl = c->hostScal;
     iterate();
     HIPCHECK(hipMemcpyAsync(&hsl[0], C.S, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
@@ -1377,11 +1379,20 @@ template <class F> void runLoop(vol_context c, const Ctx& C, int maxLoops, F&& i
         if (hsl[(k - 1) & 1].done) break;
     }
     HIPCHECK(hipStreamSynchronize(st));
-    (void)hipEventDestroy(evs[0]);
-    (void)hipEventDestroy(evs[1]);
 }
 
-void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_controls& ct, vol_solver_performance& perf) {
+// Work buffers of one FP64 solve given by the caller instead of the matrix's and the
+// context's own (the no-allocation entry point): vectors of n doubles, reduction scratch
+// and the solve's two DevScalars. Solvers PCG/PBiCGStab with preconditioner none or
+// diagonal only (the others keep per-matrix state).
+struct Workspace {
+    double *wA, *pA, *vr, *vw, *vp, *r0, *y, *Ay, *s, *z, *t;
+    RedScratch red;
+    DevScalars* scal;
+};
+
+void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_controls& ct, vol_solver_performance& perf,
+                 const Workspace* ws = nullptr) {
     vol_mesh m = A->mesh;
     vol_context c = m->ctx;
     const int n = m->n;
@@ -1430,28 +1441,43 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
             throw VolError(VOL_ERR_INVALID_ARGUMENT, "innerRelTol must lie in (0, 1)");
     }
 
+    if (ws) {
+        const bool simple = P.precond == "none" || P.precond == "diagonal";
+        if (mixed || !(isPCG || isBi) || !simple)
+            throw VolError(VOL_ERR_UNSUPPORTED, "the no-allocation solve runs PCG or PBiCGStab with preconditioner "
+                                                "none or diagonal in FP64");
+    }
     setDevice(c);
     hipStream_t st = c->stream;
-    c->ensureScratch(sell_grid_blocks(m->sell.nslices));
-    Ctx C{c, st, &c->scal.p->done, c->scal.p};
+    DevScalars* scal = ws ? ws->scal : c->scal.p;
+    Ctx C{c, st, &scal->done, scal};
+    if (ws) {
+        C.red = &ws->red;
+        C.noTiming = true;
+    } else {
+        c->ensureScratch(sell_grid_blocks(m->sell.nslices));
+        A->wA.alloc(n);
+        A->pA.alloc(n);
+        A->vr.alloc(n);
+    }
+    double* wA = ws ? ws->wA : A->wA.p;
+    double* pA = ws ? ws->pA : A->pA.p;
+    double* vr = ws ? ws->vr : A->vr.p;
     const bool ex = P.exact;
     const SellView V = m->sell.view();
-    A->wA.alloc(n);
-    A->pA.alloc(n);
-    A->vr.alloc(n);
     launch_init_scalars(st, C.S, ct.tolerance, ct.relTol, ct.maxIter, ct.minIter, ct.nSweeps, n,
                         mixed ? ct.innerRelTol : 0.0, mixed && isPCG ? 1 : 0);
     // initial residual and normalisation (LinearSolver.cpp:150-169)
-    sell<double>(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, x, nullptr, A->wA.p);
+    sell<double>(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, x, nullptr, wA);
     const int nparts = ex ? RED_BLOCKS_EXACT : RED_BLOCKS;
     vec<double>(C, V_SUM_X, n, ex, x);
     fin(C, FIN_XREF, 1, ex, nparts);
-    sell<double>(C, OP_AMUL, XL_UNIFORM, V, A->coef.p, A->diag.p, nullptr, nullptr, A->pA.p, 1.0, nullptr, nullptr,
+    sell<double>(C, OP_AMUL, XL_UNIFORM, V, A->coef.p, A->diag.p, nullptr, nullptr, pA, 1.0, nullptr, nullptr,
                  nullptr, nullptr, &C.S->xRef);
-    vec<double>(C, V_INIT, n, ex, A->wA.p, b, A->pA.p, nullptr, A->vr.p);
+    vec<double>(C, V_INIT, n, ex, wA, b, pA, nullptr, vr);
     fin(C, FIN_INIT, 2, ex, nparts);
     DevScalars hs;
-    readScalars(c, hs);
+    readScalars(c, C.S, hs);
     if (!hs.done) {
         if (P.precond == "GAMG" && !A->gamgReady) buildGamgCoeffs(A);
         if (P.precond == "GAMG" && P.gamgGS) buildGsLevels(m);
@@ -1485,27 +1511,39 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
                 fin(C, FIN_INNER_START, 0, false, 0);
                 launch_mix(st, MIX_TO32, n, A->vr.p, A->ir.p, isBi ? A->ir0.p : nullptr, A->ie.p, nullptr, nullptr,
                            C.done);
-                readScalars(c, hs);
+                readScalars(c, C.S, hs);
                 if (hs.done) break;
             }
         } else if (isPCG || isBi) {
-            A->vw.alloc(n);
-            A->vp.alloc(n);
-            Krylov<double> k{x, A->vr.p, A->vw.p, A->vp.p, A->wA.p, nullptr, nullptr, nullptr, nullptr, nullptr,
-                             nullptr};
+            if (!ws) {
+                A->vw.alloc(n);
+                A->vp.alloc(n);
+            }
+            Krylov<double> k{x, vr, ws ? ws->vw : A->vw.p, ws ? ws->vp : A->vp.p, wA, nullptr, nullptr, nullptr,
+                             nullptr, nullptr, nullptr};
             if (isBi) {
-                for (DevBuf<double>* v : {&A->r0, &A->y, &A->Ay, &A->s, &A->z, &A->t}) v->alloc(n);
-                k.r0 = A->r0.p;
-                k.y = A->y.p;
-                k.Ay = A->Ay.p;
-                k.s = A->s.p;
-                k.z = A->z.p;
-                k.t = A->t.p;
-                HIPCHECK(hipMemcpyAsync(A->r0.p, A->vr.p, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-                HIPCHECK(hipMemsetAsync(A->vp.p, 0, sizeof(double) * n, st));
-                HIPCHECK(hipMemsetAsync(A->Ay.p, 0, sizeof(double) * n, st));
+                if (ws) {
+                    k.r0 = ws->r0;
+                    k.y = ws->y;
+                    k.Ay = ws->Ay;
+                    k.s = ws->s;
+                    k.z = ws->z;
+                    k.t = ws->t;
+                } else {
+                    for (DevBuf<double>* v : {&A->r0, &A->y, &A->Ay, &A->s, &A->z, &A->t}) v->alloc(n);
+                    k.r0 = A->r0.p;
+                    k.y = A->y.p;
+                    k.Ay = A->Ay.p;
+                    k.s = A->s.p;
+                    k.z = A->z.p;
+                    k.t = A->t.p;
+                }
+                // r0 = r, p = 0, Ay = 0 by kernels (no memset/copy nodes)
+                vec<double>(C, V_COPY, n, false, vr, nullptr, nullptr, nullptr, k.r0);
+                launch_fill_zero<double>(st, n, k.p, nullptr);
+                launch_fill_zero<double>(st, n, k.Ay, nullptr);
             } else if (P.precond != "GAMG") {
-                HIPCHECK(hipMemsetAsync(A->vp.p, 0, sizeof(double) * n, st));
+                launch_fill_zero<double>(st, n, k.p, nullptr);
             }
             runLoop(c, C, maxLoops, [&] { krylovIteration<double>(C, A, P, k, isPCG, ex); });
         } else {  // Jacobi and Gauss–Seidel solvers, LinearSolver.cpp:238-278
@@ -1526,7 +1564,7 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
                 fin(C, FIN_JAC_NORM, 1, ex, nparts);
             });
         }
-        readScalars(c, hs);
+        readScalars(c, C.S, hs);
         if (!hs.done) throw VolError(VOL_ERR_HIP, "solver loop did not terminate");
     }
     collectTiming(c);
@@ -1577,6 +1615,8 @@ void ctxRelease(vol_context c) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     for (auto e : c->evPool) (void)hipEventDestroy(e);
+    for (auto e : c->loopEv)
+        if (e) (void)hipEventDestroy(e);
     if (c->hostScal) (void)hipHostFree(c->hostScal);
     (void)hipStreamDestroy(c->stream);
     delete c;
@@ -1708,6 +1748,7 @@ int vol_context_create(int device, vol_context* out) {
     c->scal.alloc(2);
     HIPCHECK(hipHostMalloc((void**)&c->hostScal, 2 * sizeof(DevScalars), hipHostMallocDefault));
     c->ensureScratch(RED_BLOCKS);
+    for (hipEvent_t& e : c->loopEv) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
     *out = c.release();
     API_END
 }

@@ -741,6 +741,7 @@ struct vol_context_s {
     DevBuf<int> xnf;
     DevBuf<DevScalars> scal;
     DevScalars* hostScal = nullptr;  // pinned, 2 slots
+    hipEvent_t loopEv[2] = {nullptr, nullptr};  // the solve loop's look-ahead events
     // timing
     bool timing = false;
     std::vector<hipEvent_t> evPool;

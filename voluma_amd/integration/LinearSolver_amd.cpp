@@ -11,13 +11,20 @@
 //    b/x; reproducible mode (par::reproducible()) selects VOL_SUMS_REPRODUCIBLE, which
 //    applies the reference's swap rules (LinearSolver.cpp:138-148) and exact sums;
 //  * failures become std::runtime_error with the library's message, the reference's
-//    error convention (LinearSolver.cpp:76,177,280; GAMG.cpp:336).
+//    error convention (LinearSolver.cpp:76,177,280; GAMG.cpp:336);
+//  * GAMG smooths as the reference does (Gauss–Seidel, 2 forward + 2 backward sweeps,
+//    GAMG.cpp:423-442) and merges CFD_GAMG_MERGE pairings per level (default 2,
+//    GAMG.cpp:96-99). VOLUMA_GAMG_SMOOTHER=jacobi opts in to the library's damped-Jacobi
+//    V-cycle (2/2 sweeps, ω 0.8), which converges in a different number of iterations.
 // Serial meshes only (nGhost == 0): the multi-GPU halo path is not part of this build.
 
 #include "fvm/linalg/LinearSolver.h"
 
 #include "voluma_amd.h"
 
+#include <algorithm>
+#include <cstdlib>
+#include <cstring>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -30,6 +37,20 @@ namespace {
 
 void check(int rc) {
     if (rc != VOL_OK) throw std::runtime_error(std::string("voluma_amd: ") + vol_last_error());
+}
+
+// Pairing passes merged per GAMG level, as the reference reads it (GAMG.cpp:96-99).
+int mergeLevels() {
+    const char* e = std::getenv("CFD_GAMG_MERGE");
+    return e ? std::max(1, std::atoi(e)) : 2;
+}
+
+// GAMG smoother: the reference's Gauss–Seidel unless the caller opts in to Jacobi.
+int gamgSmoother() {
+    const char* e = std::getenv("VOLUMA_GAMG_SMOOTHER");
+    if (!e || !*e || std::strcmp(e, "gaussseidel") == 0) return VOL_GAMG_GAUSS_SEIDEL;
+    if (std::strcmp(e, "jacobi") == 0) return VOL_GAMG_JACOBI;
+    throw std::runtime_error(std::string("voluma_amd: VOLUMA_GAMG_SMOOTHER must be gaussseidel or jacobi, not ") + e);
 }
 
 vol_context context() {
@@ -58,7 +79,8 @@ AmdHandles buildHandles(const Mesh& m) {
     std::vector<double> w(nI);
     for (label f = 0; f < nI; ++f) w[f] = m.magSf()[f] * m.nonOrthDeltaCoeffs()[f];
     AmdHandles h;
-    check(vol_mesh_create(context(), m.nCells(), nI, m.owner().data(), m.neighbour().data(), w.data(), 0, &h.mesh));
+    check(vol_mesh_create(context(), m.nCells(), nI, m.owner().data(), m.neighbour().data(), w.data(), mergeLevels(),
+                          &h.mesh));
     check(vol_matrix_create(h.mesh, &h.A));
     return h;
 }
@@ -93,6 +115,7 @@ SolverPerformance solveLdu(const LduSystem& s, std::vector<scalar>& x, const Sol
     c.minIter = ctrl.minIter;
     c.nSweeps = ctrl.nSweeps;
     c.sums = par::reproducible() ? VOL_SUMS_REPRODUCIBLE : VOL_SUMS_FAST;
+    c.gamgSmoother = gamgSmoother();
     vol_solver_performance p;
     check(vol_solve(h.A, s.b.data(), x.data(), &c, &p, VOL_HOST));
     SolverPerformance perf;

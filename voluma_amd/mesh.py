@@ -217,11 +217,13 @@ class Mesh:
                 if nm == self.patch_neighbours[a]:
                     b = k
             if b == len(self.patch_types):
-                raise RuntimeError(f"cyclic patch {self.patch_names[a]}: neighbourPatch not found")
+                raise RuntimeError(f"periodic patch {self.patch_names[a]!r} names a partner "
+                                   f"{self.patch_neighbours[a]!r} that the mesh does not have")
             sa, na = int(self.patch_start[a]), int(self.patch_size[a])
             sb, nbf = int(self.patch_start[b]), int(self.patch_size[b])
             if na != nbf:
-                raise RuntimeError(f"cyclic patches {self.patch_names[a]}/{self.patch_names[b]} differ in size")
+                raise RuntimeError(f"periodic pair {self.patch_names[a]!r}/{self.patch_names[b]!r}: {na} faces "
+                                   f"against {nbf}")
             done[a] = done[b] = True
             ca, cb = np.zeros(3), np.zeros(3)
             min_a = 1e15  # GREAT
@@ -248,7 +250,8 @@ class Mesh:
                                 if dd < best_d:
                                     best_d, best = dd, fb
                 if best < 0 or best_d > 1e-3 * math.sqrt(_mag(Sf[fa])):
-                    raise RuntimeError(f"cyclic patch {self.patch_names[a]}: no matching face (non-translational?)")
+                    raise RuntimeError(f"periodic patch {self.patch_names[a]!r}: face {fa} has no partner face at its "
+                                       "translated centre (only translational periodicity is merged)")
                 if self.owner[fa] == self.owner[best]:
                     raise RuntimeError(f"cyclic patch {self.patch_names[a]}: a face couples cell "
                                        f"{int(self.owner[fa])} to itself (one cell across the periodic "
@@ -283,7 +286,8 @@ class Mesh:
         if two_d:
             # Σ|S_f| over the empty faces, one running sum in patch then face order, as
             # the reference sums (numpy's cumsum adds sequentially)
-            empty_n = np.cumsum(np.abs(Sf[np.concatenate(empty_faces)]), axis=0)[-1]
+            ef = np.concatenate(empty_faces)
+            empty_n = np.cumsum(np.abs(Sf[ef]), axis=0)[-1] if ef.size else np.zeros(3)
             d = 0
             for a in (1, 2):
                 if empty_n[a] > empty_n[d]:

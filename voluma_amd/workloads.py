@@ -58,8 +58,22 @@ def bc_fixed_xmax(m):
     return bc
 
 
-def pressure_system(ctx, m, seed=1, gamg=True):
-    """(LduMesh, LduMatrix, b) of the pressure Poisson system on mesh m, assembled on the GPU."""
+def pressure_psi0(C):
+    """Previous pressure field of the non-orthogonal correction: p⁰ = cos(2πx)·sin(2πy) + z
+    at the cell centres C (n × 3), with the machine-independent sin/cos of _sincos."""
+    sx, cx = _sincos(2.0 * math.pi * C[:, 0])
+    sy, cy = _sincos(2.0 * math.pi * C[:, 1])
+    return cx * sy + C[:, 2]
+
+
+def pressure_system(ctx, m, seed=1, gamg=True, nonorth=False):
+    """(LduMesh, LduMatrix, b) of the pressure Poisson system on mesh m, assembled on the GPU.
+
+    nonorth=False: b = rhs + folded boundary source (the correction's explicit part is
+    left out, which is what a solve from p⁰ = 0 sees). nonorth=True: fvm::laplacian(1, p)
+    with its non-orthogonal correction evaluated on p⁰ = pressure_psi0 (Fvm.h:155-170:
+    Gauss-linear gradient, source[o] −= fc, source[n] += fc) and b = rhs + that folded
+    source. The matrix is the same either way."""
     import voluma_amd as va
 
     ni = m.n_internal
@@ -67,7 +81,15 @@ def pressure_system(ctx, m, seed=1, gamg=True):
     lm.set_boundary(m.owner[ni:])
     A = va.LduMatrix(lm)
     g = m.geometry()
-    b = A.assemble_laplacian(g["magSf"], g["ndc"], bc_fixed_xmax(m), b_in=pressure_rhs(m.n_cells, seed))
+    rhs = pressure_rhs(m.n_cells, seed)
+    if not nonorth:
+        b = A.assemble_laplacian(g["magSf"], g["ndc"], bc_fixed_xmax(m), b_in=rhs)
+        return lm, A, b
+    psi = pressure_psi0(g["C"])
+    A.fvm_reset()
+    A.fvm_laplacian(+1, g["magSf"], g["ndc"], bc_fixed_xmax(m), None,
+                    nonorth={"psi": psi, "w": g["w"], "Sf": g["Sf"], "corr": g["corr"], "V": g["V"]})
+    b = rhs + A.fvm_finish()
     return lm, A, b
 
 
@@ -289,5 +311,6 @@ def batch_system(ctx, specs, rhs_seed=7):
         lm.close()
     systems = [(m.n_cells, m.owner[: m.n_internal], m.neighbour, m.agglomeration_weights()) for m in meshes]
     B = va.LduBatch(ctx, systems)
-    B.set(np.concatenate(diag), np.concatenate(upper))
+    B.diag_h, B.upper_h = np.concatenate(diag), np.concatenate(upper)  # host copies, for checks
+    B.set(B.diag_h, B.upper_h)
     return B, np.concatenate(bs), meshes
