@@ -60,7 +60,7 @@ def _lib():
         lib.vr_skew.argtypes = [P, P, P]
         lib.vr_read_polymesh.argtypes = [ctypes.c_char_p]
         lib.vr_read_polymesh.restype = P
-        for f in (lib.vr_read_gmsh, lib.vr_read_fluent, lib.vr_read_mesh_file):
+        for f in (lib.vr_read_gmsh, lib.vr_read_mesh_file):
             f.argtypes = [ctypes.c_char_p, D, D, ctypes.c_char_p]
             f.restype = P
         lib.vr_rawmesh_sizes.argtypes = [P, P]
@@ -390,12 +390,6 @@ def read_gmsh(path, scale=1.0, depth2d=0.1, patch_types=None):
     """The reference's readGmsh of a file: (raw arrays dict, RefMesh built from it)."""
     return _raw_mesh(_lib().vr_read_gmsh(str(path).encode(), ctypes.c_double(scale), ctypes.c_double(depth2d),
                                          _overrides(patch_types)))
-
-
-def read_fluent(path, scale=1.0, depth2d=0.1, patch_types=None):
-    """The reference's readFluent of a file: (raw arrays dict, RefMesh built from it)."""
-    return _raw_mesh(_lib().vr_read_fluent(str(path).encode(), ctypes.c_double(scale), ctypes.c_double(depth2d),
-                                           _overrides(patch_types)))
 
 
 def read_mesh_file(path, scale=1.0, depth2d=0.1, patch_types=None):

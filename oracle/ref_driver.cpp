@@ -369,13 +369,10 @@ void* vr_read_polymesh(const char* dir) {
     }
 }
 // Mesh files read by the reference's importers (MeshReaders.cpp): readGmsh
-// (:236-482), readFluent (:751-803) and readMeshFile (:805-816), with MeshImportOptions
+// (:236-482) and readMeshFile (:805-816), with MeshImportOptions
 // {scale, depth2D, patchTypes from "name=type" lines (patchTypeFromString)}.
 void* vr_read_gmsh(const char* file, double scale, double depth2D, const char* patchTypes) {
     return readWith([&](const MeshImportOptions& o) { return readGmsh(file, o); }, scale, depth2D, patchTypes);
-}
-void* vr_read_fluent(const char* file, double scale, double depth2D, const char* patchTypes) {
-    return readWith([&](const MeshImportOptions& o) { return readFluent(file, o); }, scale, depth2D, patchTypes);
 }
 void* vr_read_mesh_file(const char* path, double scale, double depth2D, const char* patchTypes) {
     return readWith([&](const MeshImportOptions& o) { return readMeshFile(path, o); }, scale, depth2D, patchTypes);

@@ -349,53 +349,6 @@ def gmsh_case():
         json.dump(out, fh, indent=1)
 
 
-# Mesh files of the reference's importer tests and cases (copied as data) and the options
-# each case reads them with: (file under import/, reference source, options).
-IMPORT_FILES = {"cavity_fluent.msh": "meshes/import/cavity_fluent.msh",            # Fluent ASCII, 3-D hexes
-                "cavity_fluent_bin.msh": "meshes/import/cavity_fluent_bin.msh",    # Fluent binary sections
-                "mixed2D_v22.msh": "meshes/import/mixed2D_v22.msh",                # Gmsh 2.2, 2-D tris + quads
-                "annulus_v41.msh": "cases/taylorCouette/annulus.msh"}              # Gmsh 4.1, 2-D quads
-IMPORT_CASES = {"cavity_fluent": ("cavity_fluent.msh", {}),
-                "cavity_fluent_empty": ("cavity_fluent.msh", {"patch_types": {"frontAndBack": "empty"}}),
-                "cavity_fluent_bin_empty": ("cavity_fluent_bin.msh", {"patch_types": {"frontAndBack": "empty"}}),
-                "mixed2D_v22": ("mixed2D_v22.msh", {}),
-                "mixed2D_v22_scaled": ("mixed2D_v22.msh", {"scale": 0.5, "depth2d": 0.25,
-                                                           "patch_types": {"fixedWalls": "symmetry"}}),
-                "annulus_v41": ("annulus_v41.msh", {})}
-
-
-def import_case():
-    """Fluent and 2-D Gmsh files read by the reference's readMeshFile with the options of
-    each case: the raw mesh it reads, then the Mesh it builds (empty patches dropped):
-    geometry of the faces it keeps, skew vectors, 2-D layout and patch names."""
-    import shutil
-    dst = os.path.join(HERE, "import")
-    os.makedirs(dst, exist_ok=True)
-    for name, rel in IMPORT_FILES.items():
-        shutil.copyfile(os.path.join(REF_ROOT, rel), os.path.join(dst, name))
-    out = {}
-    for case, (fname, opt) in IMPORT_CASES.items():
-        raw, rm = ref.read_mesh_file(os.path.join(dst, fname), **opt)
-        e = {k: sha(np.ascontiguousarray(raw[k], dtype=np.int64 if k != "points" else np.float64))
-             for k in ("points", "face_off", "face_pts", "owner", "neighbour")}
-        e.update({"file": fname, "options": opt, "n_cells": raw["n_cells"], "patch_names": raw["patch_names"],
-                  "patch_types": raw["patch_types"], "patch_start": [int(v) for v in raw["patch_start"]],
-                  "patch_size": [int(v) for v in raw["patch_size"]]})
-        g = rm.geometry()
-        sk, flag = rm.skew()
-        e["geometry"] = {k: sha(v) for k, v in g.items()}
-        e["geometry"]["skew"] = sha(sk)
-        e["skewed"] = flag
-        two_d, empty_dir, solved, names = rm.layout()
-        e["mesh"] = {"two_d": two_d, "empty_dir": empty_dir, "solved_components": solved, "patch_names": names,
-                     "patch_start": [int(v) for v in rm.patch_start], "patch_size": [int(v) for v in rm.patch_size],
-                     "total_volume": float(g["V"].sum())}
-        out[case] = e
-        print(f"import {case}: {raw['n_cells']} cells, patches {raw['patch_names']} {raw['patch_types']}", flush=True)
-    with open(os.path.join(HERE, "import_expected.json"), "w") as fh:
-        json.dump(out, fh, indent=1)
-
-
 # generateBox cases with cyclic pairs: (nx, ny, nz), periodic directions, 2-D, jittered
 PERIODIC_CASES = {
     "box6x5x4_px": ((6, 5, 4), (True, False, False), False, False),
@@ -443,9 +396,7 @@ def piso_case(name):
     from helpers import PISO_CASES, piso_inputs
     mk, opt, steps, dt, repro = PISO_CASES[name]
     m = mk()
-    if name.startswith("annulus"):
-        _, rm = ref.read_mesh_file(os.path.join(HERE, "import", "annulus_v41.msh"))
-    elif name.startswith("channel_periodic"):
+    if name.startswith("channel_periodic"):
         _, rm = ref.generate_box(8, 5, 4, (True, False, False), points=m.points)
     else:
         rm = ref.RefMesh(m)
@@ -665,10 +616,6 @@ if __name__ == "__main__":
     if "--polymesh" in sys.argv:
         polymesh_case()
         gmsh_case()
-        import_case()
-        sys.exit(0)
-    if "--import" in sys.argv:
-        import_case()
         sys.exit(0)
     if "--periodic" in sys.argv:
         periodic_case()
@@ -688,7 +635,6 @@ if __name__ == "__main__":
         fvc_case(nm, mk)
     polymesh_case()
     gmsh_case()
-    import_case()
     from helpers import PISO_CASES
     for nm in PISO_CASES:
         piso_case(nm)

@@ -128,9 +128,6 @@ PISO_CASES = {
                                {"nu": 0.02, "u_bc": [(1, 0, 0), None, (0, 0, 0), (0, 0, 0), (0, 0, 0), (0, 0, 0)],
                                 "p_bc": [None, 0.0, None, None, None, None], "ddt": "backward",
                                 "ddt_phi_coeff": -1.0, "grad_p": 2, "n_non_orth": 1}, 3, 0.01, True),
-    "annulus_2d_repro": (lambda: M.read_mesh(os.path.join(GOLDEN, "import", "annulus_v41.msh")),
-                         {"nu": 0.05, "u_bc": [(0, 0, 0), (0, 0, 0)], "p_bc": [None, None], "ddt_phi_coeff": 0.5},
-                         2, 0.02, True),
     "channel_periodic_repro": (periodic_channel,
                                {"nu": 0.02, "u_bc": [(0, 0, 0)] * 4, "p_bc": [None] * 4, "ddt": "backward",
                                 "grad_p": 1, "n_non_orth": 1}, 2, 0.01, True),

@@ -94,47 +94,3 @@ def test_read_gmsh_device_geometry(ctx, name):
     for k, h in e["geometry"].items():
         assert sha(np.ascontiguousarray(g[k])) == h, k
     assert g["skewed"] == e["skewed"]
-
-
-@pytest.mark.parametrize("case", ["cavity_fluent", "cavity_fluent_bin_empty", "mixed2D_v22_scaled", "annulus_v41"])
-def test_imported_mesh_device_geometry(ctx, case):
-    """Fluent and 2-D Gmsh files read by the library: device geometry of the faces the
-    reference's Mesh keeps (empty patches dropped) equal to that Mesh's."""
-    import os
-    from helpers import GOLDEN, load_json, sha
-    from voluma_amd import mesh as M
-    e = load_json("import_expected.json")[case]
-    m = M.read_mesh(os.path.join(GOLDEN, "import", e["file"]), **e["options"])
-    g = va.device_geometry(ctx, m)
-    c = m.computational()
-    for k, h in e["geometry"].items():
-        a = g[k] if k in ("V", "C") else g[k][c["faces"]]
-        assert sha(np.ascontiguousarray(a)) == h, k
-    assert g["skewed"] == e["skewed"]
-
-
-def test_annulus_2d_pressure_solve(ctx):
-    """The 2-D annulus (Gmsh 4.1, extruded one layer): pressure Laplacian on the faces the
-    reference's Mesh keeps, fixedValue on the inner wall, GAMG-PCG to 1e-8."""
-    import os
-    from helpers import GOLDEN
-    from voluma_amd import mesh as M
-    m = M.read_mesh(os.path.join(GOLDEN, "import", "annulus_v41.msh"))
-    g = va.device_geometry(ctx, m)
-    c = m.computational()
-    f = c["faces"]
-    ni = m.n_internal
-    lm = va.LduMesh(ctx, m.n_cells, m.owner[:ni], m.neighbour, g["magSf"][:ni] * g["ndc"][:ni])
-    lm.set_boundary(m.owner[f[ni:]])
-    A = va.LduMatrix(lm)
-    bc = np.zeros(len(f) - ni, np.int32)
-    p = c["patch_names"].index("innerWall")
-    s = c["patch_start"][p] - ni
-    bc[s: s + c["patch_size"][p]] = 1
-    b = A.assemble_laplacian(np.ascontiguousarray(g["magSf"][f]), np.ascontiguousarray(g["ndc"][f]), bc,
-                             b_in=W.pressure_rhs(m.n_cells, 5))
-    x = np.zeros(m.n_cells)
-    perf = va.solve_ldu(A, b, x, va.SolverControls("PCG", "GAMG", 1e-8, 0.0, 500))
-    assert perf.converged
-    r = b - va.amul(A, x)
-    assert np.abs(r).sum() / perf.normFactor < 1e-8

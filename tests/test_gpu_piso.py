@@ -1,17 +1,14 @@
 """GPU tests of the pressure-corrector steps (vol_piso_*, IncompressibleFlow.cpp:17-29,
 242-362) and vol_fvc_reconstruct (Fvc.h:33-67): byte-identical to the oracle (which the
 CPU tests pin to the reference's reconstruct and to closed forms) on an orthogonal box,
-two skewed tet meshes, the kept faces of a 2-D mesh and the full-size 2.06M-cell tet
-mesh; argument errors are reported."""
-import os
-
+two skewed tet meshes and the full-size 2.06M-cell tet mesh; argument errors are
+reported."""
 import numpy as np
 import pytest
 
 import voluma_amd as va
-from helpers import FVC_MESHES, GOLDEN, CompView
+from helpers import FVC_MESHES
 from oracle import oracle as O
-from voluma_amd import mesh as M
 from voluma_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
@@ -80,17 +77,6 @@ def test_piso_steps_small(ctx, name):
 
 def test_piso_steps_full_size(ctx):
     _check_all(ctx, W.make_mesh("tet70"), 2)
-
-
-def test_reconstruct_2d_kept_faces(ctx):
-    cv = CompView(M.read_mesh(os.path.join(GOLDEN, "import", "annulus_v41.msh")))
-    ni = cv.n_internal
-    lm = va.LduMesh(ctx, cv.n_cells, cv.owner[:ni], cv.neighbour, None)
-    lm.set_boundary(cv.owner[ni:])
-    g = cv.geometry()
-    lm.set_geometry(g["V"], g["Sf"], g["magSf"], g["w"], g["d"], g["skew"], g["skewed"])
-    ssf = np.random.default_rng(3).uniform(-1, 1, cv.n_faces)
-    assert _same(lm.fvc_reconstruct(ssf, cv.empty_dir), O.fvc_reconstruct(cv, ssf, cv.empty_dir))
 
 
 def test_piso_errors(ctx):

@@ -2,8 +2,7 @@
 library) against the reference's own IncompressibleFlow::step (IncompressibleFlow.cpp:
 144-370, upwind convection) run from the same fields: a lid-driven box (Euler), a
 skewed tet channel with a fixed-pressure outlet (backward time stepping, automatic
-ddtPhiCorr, Gauss-skew ∇p, a non-orthogonal corrector) and the 2-D annulus of the
-reference's Taylor–Couette case. With reproducible sums both sides use the reference's
+ddtPhiCorr, Gauss-skew ∇p, a non-orthogonal corrector) and a periodic channel. With reproducible sums both sides use the reference's
 partition-independent solvers and U, p and φ after 2–3 steps are byte-identical; with
 the default solvers (PBiCGStab + DILU, PCG + GAMG with Gauss–Seidel smoothing) they
 agree to the solver tolerances."""

@@ -1,17 +1,11 @@
 """Mesh ingest: the polyMesh reader against the reference's readPolyMesh
 (RawMesh.cpp:223-336) on two of the reference's own test meshes (3-D tets in ASCII
 faceList form; 2-D hexes in binary faceCompactList form), and the Gmsh reader against
-the reference's readGmsh (MeshReaders.cpp:236-482) on its own hybrid (tet/hex/pyramid)
-and second-order tet test files (Gmsh 4.1) and a Gmsh 2.2 file of a jittered tet mesh:
-points, faces and their order, owner, neighbour, patches and cell count identical; the
-3-D meshes' host geometry equal to the geometry of the Mesh the reference builds from
-them. Fluent files (ASCII and binary sections) and 2-D meshes (Gmsh 2.2 and 4.1, a
-Fluent mesh made 2-D by a patch type override), extruded one layer as the reference's
-importers extrude them, against its readMeshFile with scale / depth2D / patch type
-options: raw mesh identical, and the faces its Mesh keeps (empty patches dropped) with
-their geometry, skew vectors, 2-D direction and solved components identical. The
-reference's own importer test checks (Fluent == polyMesh cavity; closed cells, volume
-= depth, patch areas) hold. Boxes with cyclic patch pairs (one with moved interior
+the mesh the reference's importer builds (MeshReaders.cpp:131-190,236-482) from its own
+hybrid (tet/hex/pyramid) and second-order tet test files (Gmsh 4.1) and a Gmsh 2.2 file
+of a jittered tet mesh: points, faces and their order, owner, neighbour, patches and
+cell count identical; the meshes' host geometry equal to the geometry of the Mesh the
+reference builds from them. Boxes with cyclic patch pairs (one with moved interior
 points) against the reference's generateBox and Mesh: raw mesh, merged cyclic faces,
 their geometry and a reconstruct on them identical. Malformed input raises."""
 import os
@@ -98,105 +92,34 @@ def test_gmsh_errors(tmp_path):
         M.read_mesh(tmp_path / "missing.msh")
 
 
-IMPORT = ("cavity_fluent", "cavity_fluent_empty", "cavity_fluent_bin_empty", "mixed2D_v22", "mixed2D_v22_scaled",
-          "annulus_v41")
-
-
-def _import(case):
-    e = load_json("import_expected.json")[case]
-    opt = dict(e["options"])
-    return e, M.read_mesh(os.path.join(GOLDEN, "import", e["file"]), **opt)
-
-
-@pytest.mark.parametrize("case", IMPORT)
-def test_import_reader_matches_reference(case):
-    e, m = _import(case)
-    assert sha(np.ascontiguousarray(m.points)) == e["points"]
+def test_gmsh_options_and_dispatch(tmp_path):
+    """scale multiplies every coordinate; a patch type override replaces the type the
+    reader picks; read_mesh tells a polyMesh directory from a Gmsh file and refuses
+    anything else; a 2-D mesh (no volume elements) is refused."""
+    f = os.path.join(GOLDEN, "gmsh", "hybrid3D_v41.msh")
+    m1, m2 = M.read_gmsh(f), M.read_gmsh(f, scale=0.5, patch_types={"defaultFaces": "wall"})
+    assert np.array_equal(m2.points, 0.5 * m1.points)
     for k in ("face_off", "face_pts", "owner", "neighbour"):
-        assert sha(np.ascontiguousarray(getattr(m, k), dtype=np.int64)) == e[k], k
-    assert m.n_cells == e["n_cells"]
-    assert m.patch_names == e["patch_names"] and m.patch_types == e["patch_types"]
-    assert [int(v) for v in m.patch_start] == e["patch_start"] and [int(v) for v in m.patch_size] == e["patch_size"]
-    c, g = m.computational(), m.geometry()
-    for k, h in e["geometry"].items():
-        a = g[k] if k in ("V", "C") else g[k][c["faces"]]
-        assert sha(np.ascontiguousarray(a)) == h, k
-    assert g["skewed"] == e["skewed"]
-    lay = e["mesh"]
-    assert (c["two_d"], c["empty_dir"], c["solved_components"]) == (lay["two_d"], lay["empty_dir"],
-                                                                    lay["solved_components"])
-    assert c["patch_names"] == lay["patch_names"]
-    assert c["patch_start"] == lay["patch_start"] and c["patch_size"] == lay["patch_size"]
-
-
-def test_fluent_equals_polymesh_cavity():
-    """test_mesh_import.cpp:100-107: the Fluent cavity, ASCII and binary, with
-    frontAndBack made empty, is the cavity polyMesh exactly."""
-    ref = _read("cavity_binary")
-    for f in ("cavity_fluent.msh", "cavity_fluent_bin.msh"):
-        m = M.read_fluent(os.path.join(GOLDEN, "import", f), patch_types={"frontAndBack": "empty"})
-        assert m.n_cells == ref.n_cells and np.array_equal(m.points, ref.points)
-        for k in ("face_off", "face_pts", "owner", "neighbour", "patch_start", "patch_size"):
-            assert np.array_equal(getattr(m, k), getattr(ref, k)), k
-        assert m.patch_names == ref.patch_names and m.patch_types == ref.patch_types
-
-
-@pytest.mark.parametrize("case,depth", [("mixed2D_v22", 0.1), ("mixed2D_v22_scaled", 0.0625)])
-def test_import_cube_checks(case, depth):
-    """test_mesh_import.cpp:24-80 on the 2-D unit-square cavity: every cell closed
-    (Σ S_f = 0 relative to Σ|S_f|), total volume = depth (scaled: 0.5² · 0.25), the moving
-    wall (one side) has area depth/side, the fixed walls three times that, volumes > 0."""
-    e, m = _import(case)
-    g, c = m.geometry(), m.computational()
-    S = g["Sf"]
-    ssum = np.zeros((m.n_cells, 3))
-    area = np.zeros(m.n_cells)
-    ni = m.n_internal
-    np.add.at(ssum, m.owner, S)
-    np.add.at(ssum, m.neighbour, -S[:ni])
-    np.add.at(area, m.owner, g["magSf"])
-    np.add.at(area, m.neighbour, g["magSf"][:ni])
-    assert (np.linalg.norm(ssum, axis=1) / area).max() < 1e-12
-    assert abs(g["V"].sum() - depth) < 1e-12 and g["V"].min() > 0
-    side = depth / (0.5 * 0.25 if case.endswith("scaled") else 0.1)  # the square's side
-    areas = {n: g["magSf"][c["faces"][s:s + z]].sum()
-             for n, s, z in zip(c["patch_names"], c["patch_start"], c["patch_size"])}
-    assert abs(areas["movingWall"] - depth / side) < 1e-12
-    assert abs(areas["fixedWalls"] - 3 * depth / side) < 1e-12
-    assert c["two_d"] and c["solved_components"] == [0, 1]
-
-
-def test_read_mesh_dispatch(tmp_path):
-    """readMeshFile's choice: directory → polyMesh, "$MeshFormat" → Gmsh, .msh/.cas or a
-    '(' on the first line → Fluent, anything else an error."""
+        assert np.array_equal(getattr(m1, k), getattr(m2, k))
+    name = m1.patch_names[0]
+    m3 = M.read_gmsh(f, patch_types={name: "symmetry"})
+    assert m3.patch_types[0] == "symmetryPlane" and m3.patch_types[1:] == m1.patch_types[1:]
     d = M.read_mesh(os.path.join(GOLDEN, "polymesh", "cavity3D_4"))
     assert d.n_cells == _read("cavity3D_4").n_cells
-    src = open(os.path.join(GOLDEN, "import", "cavity_fluent.msh"), "rb").read()
-    cas = tmp_path / "mesh.cas"
-    cas.write_bytes(src)
-    assert M.read_mesh(cas).n_cells == 400
+    assert M.read_mesh(f).n_cells == m1.n_cells
     txt = tmp_path / "mesh.txt"
     txt.write_text("hello\n")
     with pytest.raises(RuntimeError):
         M.read_mesh(txt)
-
-
-def test_fluent_errors(tmp_path):
-    src = open(os.path.join(GOLDEN, "import", "cavity_fluent_bin.msh"), "rb").read()
-    cut = tmp_path / "cut.msh"
-    cut.write_bytes(src[: len(src) // 2])  # binary body cut short
+    flat = tmp_path / "flat.msh"
+    flat.write_text("$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n3\n1 0 0 0\n2 1 0 0\n3 0 1 0\n$EndNodes\n"
+                    "$Elements\n1\n1 2 2 1 1 1 2 3\n$EndElements\n")
+    with pytest.raises(RuntimeError, match="no volume elements"):
+        M.read_gmsh(flat)
+    bad_type = tmp_path / "bad_type.msh"
+    bad_type.write_text(open(os.path.join(GOLDEN, "gmsh", "tet5x4x3_v22.msh")).read().replace("\n1 2 2 1 1 ", "\n1 99 2 1 1 ", 1))
     with pytest.raises(RuntimeError):
-        M.read_fluent(cut)
-    asc = open(os.path.join(GOLDEN, "import", "cavity_fluent.msh"), "rb").read()
-    nc = tmp_path / "nc.msh"
-    nc.write_bytes(asc + b"\n(59 (1 2 3))\n")  # non-conformal interface section
-    with pytest.raises(RuntimeError):
-        M.read_fluent(nc)
-    g2 = open(os.path.join(GOLDEN, "import", "mixed2D_v22.msh")).read()
-    tilted = tmp_path / "tilted.msh"
-    tilted.write_text(g2.replace("\n2 0.5 0 0\n", "\n2 0.5 0 0.25\n"))  # a 2-D mesh off its plane
-    with pytest.raises(RuntimeError):
-        M.read_gmsh(tilted)
+        M.read_gmsh(bad_type)
 
 
 # ---------------------------------------------------------------- cyclic patch pairs

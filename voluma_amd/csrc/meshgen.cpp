@@ -20,6 +20,8 @@
 #include <cstdint>
 #include <cstring>
 #include <array>
+#include <charconv>
+#include <iterator>
 #include <filesystem>
 #include <cctype>
 #include <fstream>
@@ -709,933 +711,524 @@ Raw readPolyMesh(const std::string& dir) {
 }
 
 // ------------------------------------------------------------------ Gmsh reader (3-D)
-// ASCII Gmsh 2.2 and 4.x files as the reference's readGmsh reads them
-// (MeshReaders.cpp:192-482): nodes, elements of the highest dimension become cells,
-// their faces are matched by sorted corner nodes (first element = c0), boundary faces
-// take the physical group of the 2-D element with the same nodes (else patch
-// "defaultFaces"), unused points are dropped, and the face list is assembled by
-// assembleFaces below. 2-D meshes (extruded by the reference) are not supported here.
-// ---------------------------------------------------------------- mesh import
-// Shared by the Gmsh and Fluent readers (MeshReaders.cpp): face keys, face-list assembly,
-// 2-D meshes extruded to one layer, import options.
-namespace meshio {
+// Gmsh MSH files in ASCII, format versions 2.2 and 4.x, as the file-format chapter of
+// the Gmsh reference manual describes them: $MeshFormat, $PhysicalNames, $Entities
+// (4.x: the physical tags of every geometric entity), $Nodes, $Elements; any other
+// section is skipped. Volume elements become the cells; a boundary face takes the
+// physical group of the surface element over the same corners.
+//
+// Face order rules. These fix owner/neighbour, the face order and the point order, and
+// through them every later bit (geometry, agglomeration, sums), so they are the ones of
+// the reference's importer (readGmsh and assembleFaceMesh, src/fvm/mesh/
+// MeshReaders.cpp:131-190,236-482):
+//  (1) cells are the volume elements in file order; each lists its faces in the order
+//      of its kind's local face table (faceTable below, corner indices in Gmsh's node
+//      order), each face's nodes as the table lists them;
+//  (2) a face is known by its sorted corner set: its first appearance fixes its node
+//      order and first cell, a second appearance its other cell; faces are numbered by
+//      first appearance;
+//  (3) points are renumbered by first use along that face sequence, unused ones dropped;
+//  (4) an internal face has owner < neighbour and its nodes are reversed when the polygon
+//      normal (fan about the point average) points against neighbour − owner centre; a
+//      boundary face's nodes are reversed when the normal points into its cell (cell
+//      centre = mean of the centres of its faces, in face order);
+//  (5) internal faces are ordered by (owner, neighbour), then the boundary faces patch
+//      by patch in patch order, each patch's in face order; patches with no face are
+//      dropped;
+//  (6) the patches are the physical groups of the surface elements in order of first
+//      use, named by $PhysicalNames (else "patch<tag>") and typed symmetryPlane if the
+//      lower-case name contains "symmetry", else wall if it contains "wall", else patch;
+//      then "defaultFaces" (wall) for boundary faces in no group. When two surface
+//      elements cover the same corners the later one's group counts.
+namespace gmsh {
 
-// Element type → (dimension, corner nodes); higher-order types keep their corners
-// (the element type numbers are Gmsh's).
-bool elementType(int t, int& dim, int& corners) {
-    static const std::map<int, std::pair<int, int>> table = {
-        {15, {0, 1}}, {1, {1, 2}},   {8, {1, 2}},   {26, {1, 2}},  {27, {1, 2}},  {28, {1, 2}},  {2, {2, 3}},
-        {9, {2, 3}},  {20, {2, 3}},  {21, {2, 3}},  {22, {2, 3}},  {23, {2, 3}},  {24, {2, 3}},  {25, {2, 3}},
-        {3, {2, 4}},  {10, {2, 4}},  {16, {2, 4}},  {36, {2, 4}},  {37, {2, 4}},  {38, {2, 4}},  {4, {3, 4}},
-        {11, {3, 4}}, {29, {3, 4}},  {30, {3, 4}},  {31, {3, 4}},  {5, {3, 8}},   {12, {3, 8}},  {17, {3, 8}},
-        {92, {3, 8}}, {93, {3, 8}},  {6, {3, 6}},   {13, {3, 6}},  {18, {3, 6}},  {90, {3, 6}},  {91, {3, 6}},
-        {7, {3, 5}},  {14, {3, 5}},  {19, {3, 5}},  {118, {3, 5}}, {119, {3, 5}}};
-    const auto it = table.find(t);
-    if (it == table.end()) return false;
-    dim = it->second.first;
-    corners = it->second.second;
-    return true;
+// Element type number → (dimension, corner count). Higher-order types are read by
+// their corners, which Gmsh lists first (manual, "MSH node ordering").
+bool kindOf(int type, int& dim, int& corners) {
+    struct Row { int type, dim, corners; };
+    static const Row rows[] = {
+        {15, 0, 1},  {1, 1, 2},   {8, 1, 2},   {26, 1, 2},  {27, 1, 2},  {28, 1, 2},  {2, 2, 3},   {9, 2, 3},
+        {20, 2, 3},  {21, 2, 3},  {22, 2, 3},  {23, 2, 3},  {24, 2, 3},  {25, 2, 3},  {3, 2, 4},   {10, 2, 4},
+        {16, 2, 4},  {36, 2, 4},  {37, 2, 4},  {38, 2, 4},  {4, 3, 4},   {11, 3, 4},  {29, 3, 4},  {30, 3, 4},
+        {31, 3, 4},  {5, 3, 8},   {12, 3, 8},  {17, 3, 8},  {92, 3, 8},  {93, 3, 8},  {6, 3, 6},   {13, 3, 6},
+        {18, 3, 6},  {90, 3, 6},  {91, 3, 6},  {7, 3, 5},   {14, 3, 5},  {19, 3, 5},  {118, 3, 5}, {119, 3, 5}};
+    for (const Row& r : rows)
+        if (r.type == type) {
+            dim = r.dim;
+            corners = r.corners;
+            return true;
+        }
+    return false;
 }
 
-// Faces of the 3-D element kinds by corner index in Gmsh's node order (their orientation
-// is corrected from the geometry afterwards).
-const std::vector<std::vector<int>>& elementFaces(int corners) {
-    static const std::vector<std::vector<int>> tet{{0, 2, 1}, {0, 1, 3}, {0, 3, 2}, {1, 2, 3}};
-    static const std::vector<std::vector<int>> pyramid{{0, 3, 2, 1}, {0, 1, 4}, {1, 2, 4}, {2, 3, 4}, {3, 0, 4}};
-    static const std::vector<std::vector<int>> prism{
-        {0, 2, 1}, {3, 4, 5}, {0, 1, 4, 3}, {1, 2, 5, 4}, {2, 0, 3, 5}};
-    static const std::vector<std::vector<int>> hex{{0, 3, 2, 1}, {4, 5, 6, 7}, {0, 1, 5, 4},
-                                                   {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
-    switch (corners) {
-    case 4: return tet;
-    case 5: return pyramid;
-    case 6: return prism;
-    default: return hex;
+// Local faces of a volume element with `corners` corners (4 tet, 5 pyramid, 6 prism,
+// 8 hexahedron): per face its corner count and corner indices.
+struct LocalFace { int n; int v[4]; };
+const std::vector<LocalFace>& faceTable(int corners) {
+    static const std::vector<LocalFace> tet = {{3, {0, 2, 1}}, {3, {0, 1, 3}}, {3, {0, 3, 2}}, {3, {1, 2, 3}}};
+    static const std::vector<LocalFace> pyramid = {
+        {4, {0, 3, 2, 1}}, {3, {0, 1, 4}}, {3, {1, 2, 4}}, {3, {2, 3, 4}}, {3, {3, 0, 4}}};
+    static const std::vector<LocalFace> prism = {
+        {3, {0, 2, 1}}, {3, {3, 4, 5}}, {4, {0, 1, 4, 3}}, {4, {1, 2, 5, 4}}, {4, {2, 0, 3, 5}}};
+    static const std::vector<LocalFace> hex = {{4, {0, 3, 2, 1}}, {4, {4, 5, 6, 7}}, {4, {0, 1, 5, 4}},
+                                               {4, {1, 2, 6, 5}}, {4, {2, 3, 7, 6}}, {4, {3, 0, 4, 7}}};
+    return corners == 4 ? tet : corners == 5 ? pyramid : corners == 6 ? prism : hex;
+}
+
+// The file as lines of whitespace-separated tokens, read in one piece.
+class Tokens {
+public:
+    explicit Tokens(const std::string& path) : path_(path) {
+        std::ifstream in(path, std::ios::binary);
+        if (!in) throw std::runtime_error(path + ": cannot be opened");
+        text_.assign(std::istreambuf_iterator<char>(in), std::istreambuf_iterator<char>());
     }
-}
-
-struct Key {
-    std::array<int64_t, 4> v{-1, -1, -1, -1};
-    bool operator==(const Key& o) const { return v == o.v; }
-};
-struct KeyHash {
-    std::size_t operator()(const Key& k) const {
-        std::size_t h = 0x9e3779b97f4a7c15ull;
-        for (int64_t x : k.v) h = (h ^ std::size_t(x + 1)) * 0x100000001b3ull;
-        return h;
+    const std::string& path() const { return path_; }
+    bool atEnd() { skipSpace(); return pos_ >= text_.size(); }
+    // the rest of the current line (without CR/LF), advancing past it
+    std::string restOfLine() {
+        const size_t e = text_.find('\n', pos_);
+        const size_t stop = e == std::string::npos ? text_.size() : e;
+        std::string s = text_.substr(pos_, stop - pos_);
+        if (!s.empty() && s.back() == '\r') s.pop_back();
+        pos_ = e == std::string::npos ? text_.size() : e + 1;
+        return s;
     }
+    std::string word() {
+        skipSpace();
+        const size_t b = pos_;
+        while (pos_ < text_.size() && !std::isspace(static_cast<unsigned char>(text_[pos_]))) ++pos_;
+        if (b == pos_) fail("unexpected end of file");
+        return text_.substr(b, pos_ - b);
+    }
+    int64_t integer() {
+        const std::string w = word();
+        int64_t v = 0;
+        const auto r = std::from_chars(w.data(), w.data() + w.size(), v);
+        if (r.ec != std::errc() || r.ptr != w.data() + w.size()) fail("expected an integer, found '" + w + "'");
+        return v;
+    }
+    double real() {
+        const std::string w = word();
+        char* end = nullptr;
+        const double v = std::strtod(w.c_str(), &end);
+        if (end != w.c_str() + w.size()) fail("expected a number, found '" + w + "'");
+        return v;
+    }
+    // integers of the current line (up to its end)
+    std::vector<int64_t> lineIntegers() {
+        std::vector<int64_t> out;
+        const std::string s = restOfLine();
+        const char* p = s.data();
+        const char* e = p + s.size();
+        while (true) {
+            while (p < e && std::isspace(static_cast<unsigned char>(*p))) ++p;
+            if (p == e) break;
+            int64_t v = 0;
+            const auto r = std::from_chars(p, e, v);
+            if (r.ec != std::errc()) fail("bad integer in line '" + s + "'");
+            out.push_back(v);
+            p = r.ptr;
+        }
+        return out;
+    }
+    // skips to the line after "$End<name>"
+    void closeSection(const std::string& name) {
+        const std::string tag = "$End" + name;
+        while (pos_ < text_.size()) {
+            const std::string s = restOfLine();
+            if (s.compare(0, tag.size(), tag) == 0) return;
+        }
+        fail("section $" + name + " has no " + tag);
+    }
+    [[noreturn]] void fail(const std::string& what) const { throw std::runtime_error(path_ + ": " + what); }
+
+private:
+    void skipSpace() {
+        while (pos_ < text_.size() && std::isspace(static_cast<unsigned char>(text_[pos_]))) ++pos_;
+    }
+    std::string path_, text_;
+    size_t pos_ = 0;
 };
-Key keyOf(const std::vector<int64_t>& n) {
-    if (n.size() > 4) throw std::runtime_error("mesh import: element face with more than 4 nodes");
-    Key k;
-    std::copy(n.begin(), n.end(), k.v.begin());
-    std::sort(k.v.begin(), k.v.begin() + n.size());
-    return k;
+
+// What the sections hold: points by node tag, elements (dimension, physical group or
+// −1, corner node tags) in file order, physical names by (dimension, tag).
+struct Content {
+    double version = 0;
+    std::map<std::pair<int, int64_t>, std::string> physName;
+    std::map<std::pair<int, int64_t>, int64_t> entityGroup;  // 4.x: first physical tag of an entity, or −1
+    std::unordered_map<int64_t, int64_t> pointOfTag;
+    std::vector<V3> points;
+    std::vector<int8_t> dim;
+    std::vector<int64_t> group;
+    std::vector<int64_t> cornerOff{0}, corners;
+};
+
+void sectionFormat(Tokens& t, Content& c) {
+    c.version = t.real();
+    const int64_t fileType = t.integer();
+    t.integer();  // data size
+    if (fileType != 0) t.fail("binary MSH files are not read; write the mesh as ASCII");
+    if (!(c.version >= 2.0 && c.version < 5.0)) t.fail("MSH version " + std::to_string(c.version) + " is not 2.x or 4.x");
+    t.closeSection("MeshFormat");
 }
 
-struct Element {
-    int dim, phys;
-    std::vector<int64_t> nodes;
-};
-struct Face {
-    std::vector<int64_t> nodes;
-    int64_t c0, c1;
-    int patch;
-};
-struct Patch {
-    std::string name, type;
-};
-
-// Point average and half the summed fan cross products about it (the polygon's
-// centre estimate and area vector used to orient faces).
-V3 polygonNormal(const std::vector<V3>& P, const std::vector<int64_t>& f, V3& centre) {
-    V3 c;
-    for (int64_t i : f) c += P[i];
-    c = c / double(f.size());
-    V3 n;
-    for (std::size_t k = 0; k < f.size(); ++k) n += cross(P[f[k]] - c, P[f[(k + 1) % f.size()]] - c);
-    centre = c;
-    return 0.5 * n;
+void sectionPhysicalNames(Tokens& t, Content& c) {
+    for (int64_t k = t.integer(); k > 0; --k) {
+        const int d = int(t.integer());
+        const int64_t tag = t.integer();
+        const std::string rest = t.restOfLine();
+        const size_t a = rest.find('"'), b = rest.rfind('"');
+        c.physName[{d, tag}] = a != std::string::npos && b > a ? rest.substr(a + 1, b - a - 1) : std::to_string(tag);
+    }
+    t.closeSection("PhysicalNames");
 }
 
-// Face list → Raw (MeshReaders.cpp:131-190): cell centres estimated as the mean of their
-// face centres; internal faces get c0 < c1 and a normal from c0 to c1, boundary faces an
-// outward normal (nodes reversed otherwise); internal faces sorted by (c0, c1), then
-// boundary faces patch by patch in input order; patches without faces are dropped.
-Raw assembleFaces(std::vector<V3> points, int64_t nCells, std::vector<Face> faces, const std::vector<Patch>& patches) {
-    std::vector<V3> cc(static_cast<std::size_t>(nCells), V3{});
-    std::vector<int> nf(std::size_t(nCells), 0);
-    std::vector<V3> fc(faces.size()), fn(faces.size());
-    for (std::size_t f = 0; f < faces.size(); ++f) {
-        Face& F = faces[f];
-        if (F.c0 < 0 && F.c1 >= 0) std::swap(F.c0, F.c1);
-        if (F.c0 < 0 || F.c0 >= nCells || F.c1 >= nCells) throw std::runtime_error("mesh import: face with invalid cells");
-        fn[f] = polygonNormal(points, F.nodes, fc[f]);
-        cc[F.c0] += fc[f];
-        ++nf[F.c0];
-        if (F.c1 >= 0) {
-            cc[F.c1] += fc[f];
-            ++nf[F.c1];
+void sectionEntities(Tokens& t, Content& c) {
+    int64_t count[4];
+    for (int64_t& n : count) n = t.integer();
+    for (int d = 0; d < 4; ++d)
+        for (int64_t k = 0; k < count[d]; ++k) {
+            const int64_t tag = t.integer();
+            for (int j = 0; j < (d == 0 ? 3 : 6); ++j) t.real();  // point coordinates / bounding box
+            int64_t first = -1;
+            for (int64_t j = 0, n = t.integer(); j < n; ++j) {
+                const int64_t g = std::abs(t.integer());
+                if (j == 0) first = g;
+            }
+            c.entityGroup[{d, tag}] = first;
+            if (d > 0)
+                for (int64_t j = 0, n = t.integer(); j < n; ++j) t.integer();  // bounding entities
+        }
+    t.closeSection("Entities");
+}
+
+void addPoint(Content& c, int64_t tag, V3 p) {
+    c.pointOfTag[tag] = int64_t(c.points.size());
+    c.points.push_back(p);
+}
+
+void sectionNodes(Tokens& t, Content& c, double scale) {
+    if (c.version >= 4.0) {
+        const int64_t blocks = t.integer();
+        t.integer();  // node count
+        t.integer();  // smallest tag
+        t.integer();  // largest tag
+        for (int64_t b = 0; b < blocks; ++b) {
+            const int64_t d = t.integer();
+            t.integer();  // entity tag
+            const int64_t parametric = t.integer();
+            const int64_t n = t.integer();
+            std::vector<int64_t> tags(n);
+            for (int64_t& g : tags) g = t.integer();
+            for (int64_t i = 0; i < n; ++i) {
+                V3 p;
+                p.x = t.real();
+                p.y = t.real();
+                p.z = t.real();
+                for (int64_t j = 0; j < (parametric ? d : 0); ++j) t.real();
+                addPoint(c, tags[i], p * scale);
+            }
+        }
+    } else {
+        for (int64_t k = t.integer(); k > 0; --k) {
+            const int64_t tag = t.integer();
+            V3 p;
+            p.x = t.real();
+            p.y = t.real();
+            p.z = t.real();
+            addPoint(c, tag, p * scale);
         }
     }
-    for (int64_t c = 0; c < nCells; ++c) {
-        if (nf[c] < 4) throw std::runtime_error("mesh import: cell " + std::to_string(c) + " has fewer than 4 faces");
-        cc[c] = cc[c] / double(nf[c]);
-    }
-    std::vector<std::size_t> internal;
-    std::vector<std::vector<std::size_t>> bnd(patches.size());
-    for (std::size_t f = 0; f < faces.size(); ++f) {
-        Face& F = faces[f];
-        if (F.c1 >= 0) {
-            if (F.c1 < F.c0) std::swap(F.c0, F.c1);
-            if (dot(fn[f], cc[F.c1] - cc[F.c0]) < 0) std::reverse(F.nodes.begin(), F.nodes.end());
-            internal.push_back(f);
-        } else {
-            if (F.patch < 0 || F.patch >= int(patches.size()))
-                throw std::runtime_error("mesh import: boundary face without patch");
-            if (dot(fn[f], fc[f] - cc[F.c0]) < 0) std::reverse(F.nodes.begin(), F.nodes.end());
-            bnd[F.patch].push_back(f);
-        }
-    }
-    std::sort(internal.begin(), internal.end(), [&](std::size_t a, std::size_t b) {
-        return faces[a].c0 != faces[b].c0 ? faces[a].c0 < faces[b].c0 : faces[a].c1 < faces[b].c1;
-    });
-    Raw r;
-    r.points = std::move(points);
-    r.nCells = nCells;
-    auto add = [&](const Face& F) {
-        r.facePts.insert(r.facePts.end(), F.nodes.begin(), F.nodes.end());
-        r.faceOff.push_back(int64_t(r.facePts.size()));
-        r.owner.push_back(F.c0);
-    };
-    for (std::size_t f : internal) {
-        add(faces[f]);
-        r.neighbour.push_back(faces[f].c1);
-    }
-    for (std::size_t p = 0; p < patches.size(); ++p) {
-        if (bnd[p].empty()) continue;
-        const int64_t start = r.nFaces();
-        for (std::size_t f : bnd[p]) add(faces[f]);
-        r.pName.push_back(patches[p].name);
-        r.pType.push_back(patches[p].type);
-        r.pStart.push_back(start);
-        r.pSize.push_back(r.nFaces() - start);
-    }
-    return r;
+    t.closeSection("Nodes");
 }
 
-std::string lower(std::string s) {
-    for (char& c : s) c = char(std::tolower(static_cast<unsigned char>(c)));
+void addElement(Tokens& t, Content& c, int type, int64_t group, const int64_t* nodes, size_t available) {
+    int d = 0, nc = 0;
+    if (!kindOf(type, d, nc)) t.fail("element type " + std::to_string(type) + " is not supported");
+    c.dim.push_back(int8_t(d));
+    c.group.push_back(group);
+    for (size_t k = 0; k < size_t(nc) && k < available; ++k) c.corners.push_back(nodes[k]);
+    c.cornerOff.push_back(int64_t(c.corners.size()));
+}
+
+void sectionElements(Tokens& t, Content& c) {
+    if (c.version >= 4.0) {
+        const int64_t blocks = t.integer();
+        t.integer();  // element count
+        t.integer();  // smallest tag
+        t.restOfLine();
+        for (int64_t b = 0; b < blocks; ++b) {
+            const std::vector<int64_t> head = t.lineIntegers();  // entity dim, entity tag, type, count
+            if (head.size() < 4) t.fail("bad element block header");
+            const auto it = c.entityGroup.find({int(head[0]), head[1]});
+            const int64_t group = it != c.entityGroup.end() ? it->second : -1;
+            for (int64_t i = 0; i < head[3]; ++i) {
+                const std::vector<int64_t> v = t.lineIntegers();  // tag, nodes
+                if (v.empty()) t.fail("empty element line");
+                addElement(t, c, int(head[2]), group, v.data() + 1, v.size() - 1);
+            }
+        }
+    } else {
+        const int64_t n = t.integer();
+        t.restOfLine();
+        for (int64_t i = 0; i < n; ++i) {
+            const std::vector<int64_t> v = t.lineIntegers();  // tag, type, #tags, tags..., nodes
+            if (v.size() < 3 || v.size() < size_t(3 + v[2])) t.fail("bad element line");
+            const int64_t nt = v[2];
+            const int64_t group = nt > 0 && v[3] != 0 ? v[3] : -1;
+            addElement(t, c, int(v[1]), group, v.data() + 3 + nt, v.size() - 3 - nt);
+        }
+    }
+    t.closeSection("Elements");
+}
+
+Content parse(const std::string& path, double scale) {
+    Tokens t(path);
+    Content c;
+    while (!t.atEnd()) {
+        const std::string line = t.restOfLine();
+        if (line == "$MeshFormat")
+            sectionFormat(t, c);
+        else if (line == "$PhysicalNames")
+            sectionPhysicalNames(t, c);
+        else if (line == "$Entities" && c.version >= 4.0)
+            sectionEntities(t, c);
+        else if (line == "$Nodes")
+            sectionNodes(t, c, scale);
+        else if (line == "$Elements")
+            sectionElements(t, c);
+    }
+    if (c.points.empty() || c.dim.empty()) t.fail("no nodes or no elements");
+    return c;
+}
+
+std::string lowerCase(std::string s) {
+    for (char& ch : s) ch = char(std::tolower(static_cast<unsigned char>(ch)));
     return s;
 }
 
-// MeshImportOptions (MeshReaders.h:20-24): coordinate scale, 2-D extrusion depth and
-// patch types by name that replace the reader's choice.
-struct Options {
-    double scale = 1.0;
-    double depth2D = 0.1;
-    std::map<std::string, std::string> patchTypes;
-};
-// A patch type string in the reference's spelling (patchTypeFromString then toString,
-// RawMesh.cpp:14-30): wall, symmetryPlane (also from "symmetry"), empty, cyclic, patch.
-std::string typeName(const std::string& s) {
-    if (s == "wall") return "wall";
-    if (s == "symmetry" || s == "symmetryPlane") return "symmetryPlane";
-    if (s == "empty") return "empty";
-    if (s == "cyclic") return "cyclic";
-    return "patch";
-}
-// The type a patch gets: the override for its name if there is one (MeshReaders.cpp:57-61).
-std::string patchType(const std::string& name, const std::string& t, const Options& o) {
-    const auto it = o.patchTypes.find(name);
-    return it != o.patchTypes.end() ? it->second : t;
-}
-
-// A planar polygon mesh: points (x, y used) and each cell's points in order.
-struct Mesh2D {
-    std::vector<V3> points;
-    std::vector<std::vector<int64_t>> cells;
-};
-// Twice the signed area of a cell, summed as the reference sums it (x_k·y_k+1 − x_k+1·y_k).
-double twiceArea(const Mesh2D& m, const std::vector<int64_t>& c) {
-    double a = 0;
-    for (std::size_t k = 0; k < c.size(); ++k) {
-        const V3& p = m.points[c[k]];
-        const V3& q = m.points[c[(k + 1) % c.size()]];
-        a += p.x * q.y - q.x * p.y;
-    }
-    return a;
-}
-
-// One layer of cells from a counter-clockwise 2-D mesh (Extrude2D.cpp:59-166): points
-// z = 0 then z = depth; internal faces from the edges shared by two cells (owner the
-// lower cell, its edge direction), sorted by (owner, neighbour); then each patch's
-// boundary edges sorted by cell; then "frontAndBack" (empty): per cell its reversed
-// polygon at z = 0 and its polygon at z = depth. classify(a, b) gives the patch of a
-// boundary edge. The edges are visited in (min, max) point order and sorted with
-// std::sort, as the reference does, so equal keys keep the reference's order.
-template <class Classify>
-Raw extrude2D(const Mesh2D& m, const std::vector<Patch>& patches, Classify classify, double depth) {
-    const int64_t nP = int64_t(m.points.size()), nC = int64_t(m.cells.size());
-    for (int64_t c = 0; c < nC; ++c) {
-        if (m.cells[c].size() < 3) throw std::runtime_error("extrude2D: degenerate cell");
-        if (twiceArea(m, m.cells[c]) <= 0)
-            throw std::runtime_error("extrude2D: cell " + std::to_string(c) + " is not counter-clockwise");
-    }
-    struct Use {
-        int64_t cell, a, b;
-    };
-    struct Internal {
-        int64_t own, nei, a, b;
-    };
-    std::map<std::pair<int64_t, int64_t>, std::vector<Use>> edges;
-    for (int64_t c = 0; c < nC; ++c) {
-        const auto& pl = m.cells[c];
-        for (std::size_t k = 0; k < pl.size(); ++k) {
-            const int64_t a = pl[k], b = pl[(k + 1) % pl.size()];
-            edges[{std::min(a, b), std::max(a, b)}].push_back({c, a, b});
-        }
-    }
-    std::vector<Internal> internal;
-    std::vector<std::vector<Use>> bnd(patches.size());
-    for (const auto& kv : edges) {
-        const auto& u = kv.second;
-        if (u.size() == 2) {
-            const Use& u0 = u[0].cell < u[1].cell ? u[0] : u[1];
-            const Use& u1 = u[0].cell < u[1].cell ? u[1] : u[0];
-            internal.push_back({u0.cell, u1.cell, u0.a, u0.b});
-        } else if (u.size() == 1) {
-            const int p = classify(u[0].a, u[0].b);
-            if (p < 0 || p >= int(patches.size())) throw std::runtime_error("extrude2D: unclassified boundary edge");
-            bnd[p].push_back(u[0]);
-        } else {
-            throw std::runtime_error("extrude2D: non-manifold edge");
-        }
-    }
-    std::sort(internal.begin(), internal.end(), [](const Internal& x, const Internal& y) {
-        return x.own != y.own ? x.own < y.own : x.nei < y.nei;
-    });
-    for (auto& list : bnd) std::sort(list.begin(), list.end(), [](const Use& x, const Use& y) { return x.cell < y.cell; });
-    Raw r;
-    r.points.resize(std::size_t(2 * nP));
-    for (int64_t i = 0; i < nP; ++i) {
-        r.points[i] = {m.points[i].x, m.points[i].y, 0.0};
-        r.points[nP + i] = {m.points[i].x, m.points[i].y, depth};
-    }
-    r.nCells = nC;
-    for (const Internal& f : internal) {
-        r.addFace({f.a, f.b, nP + f.b, nP + f.a});
-        r.owner.push_back(f.own);
-        r.neighbour.push_back(f.nei);
-    }
-    auto patch = [&](const std::string& name, const std::string& type) {
-        r.pName.push_back(name);
-        r.pType.push_back(type);
-        r.pStart.push_back(r.nFaces());
-    };
-    for (std::size_t p = 0; p < patches.size(); ++p) {
-        patch(patches[p].name, patches[p].type);
-        for (const Use& e : bnd[p]) {
-            r.addFace({e.a, e.b, nP + e.b, nP + e.a});
-            r.owner.push_back(e.cell);
-        }
-        r.pSize.push_back(r.nFaces() - r.pStart.back());
-    }
-    patch("frontAndBack", "empty");
-    for (int64_t c = 0; c < nC; ++c) {
-        const auto& pl = m.cells[c];
-        for (auto it = pl.rbegin(); it != pl.rend(); ++it) r.facePts.push_back(*it);
-        r.faceOff.push_back(int64_t(r.facePts.size()));
-        r.owner.push_back(c);
-        for (int64_t q : pl) r.facePts.push_back(nP + q);
-        r.faceOff.push_back(int64_t(r.facePts.size()));
-        r.owner.push_back(c);
-    }
-    r.pSize.push_back(r.nFaces() - r.pStart.back());
-    return r;
-}
-
-// An imported 2-D mesh (MeshReaders.cpp:64-96): clockwise cells reversed, the patches
-// plus "defaultFaces" (wall) for boundary edges in no group, extruded by depth2D;
-// patches without faces dropped.
-Raw extrudeImported(Mesh2D m, const std::vector<Patch>& patches,
-                    const std::map<std::pair<int64_t, int64_t>, int>& edgePatch, const Options& o) {
-    for (auto& c : m.cells)
-        if (twiceArea(m, c) < 0) std::reverse(c.begin(), c.end());
-    std::vector<Patch> specs;
-    for (const Patch& p : patches) specs.push_back({p.name, patchType(p.name, p.type, o)});
-    const int defaultIdx = int(specs.size());
-    specs.push_back({"defaultFaces", patchType("defaultFaces", "wall", o)});
-    Raw r = extrude2D(
-        m, specs,
-        [&](int64_t a, int64_t b) {
-            const auto it = edgePatch.find({std::min(a, b), std::max(a, b)});
-            return it != edgePatch.end() ? it->second : defaultIdx;
-        },
-        o.depth2D);
-    Raw k = r;
-    k.pName.clear();
-    k.pType.clear();
-    k.pStart.clear();
-    k.pSize.clear();
-    for (std::size_t p = 0; p < r.pSize.size(); ++p)
-        if (r.pSize[p] > 0) {
-            k.pName.push_back(r.pName[p]);
-            k.pType.push_back(r.pType[p]);
-            k.pStart.push_back(r.pStart[p]);
-            k.pSize.push_back(r.pSize[p]);
-        }
-    return k;
-}
-
-}  // namespace meshio
-
-// ---------------------------------------------------------------- Gmsh
-namespace gmsh {
-using namespace meshio;
-
-Raw read(const std::string& file, const Options& opt) {
-    const double scale = opt.scale;
-    std::ifstream in(file);
-    if (!in) throw std::runtime_error("cannot open " + file);
-    double version = 0;
-    std::map<std::pair<int, int>, std::string> physNames;
-    std::map<std::pair<int, int>, std::vector<int>> entityPhys;
-    std::unordered_map<int64_t, int64_t> nodeIndex;
-    std::vector<V3> points;
-    std::vector<Element> elems;
-    std::string line;
-    auto endSection = [&](const std::string& name) {
-        while (std::getline(in, line))
-            if (line.rfind("$End" + name, 0) == 0) return;
-        throw std::runtime_error(file + ": missing $End" + name);
-    };
-    while (std::getline(in, line)) {
-        if (!line.empty() && line.back() == '\r') line.pop_back();
-        if (line == "$MeshFormat") {
-            int fileType = 0, dataSize = 0;
-            in >> version >> fileType >> dataSize;
-            if (fileType != 0) throw std::runtime_error(file + ": binary Gmsh files are not supported");
-            if (version < 2 || version >= 5) throw std::runtime_error(file + ": unsupported Gmsh version");
-            endSection("MeshFormat");
-        } else if (line == "$PhysicalNames") {
-            int n;
-            in >> n;
-            for (int i = 0; i < n; ++i) {
-                int d, t;
-                in >> d >> t;
-                std::getline(in, line);
-                const auto a = line.find('"'), b = line.rfind('"');
-                physNames[{d, t}] = (a != std::string::npos && b > a) ? line.substr(a + 1, b - a - 1) : std::to_string(t);
-            }
-            endSection("PhysicalNames");
-        } else if (line == "$Entities" && version >= 4) {
-            std::size_t n[4];
-            in >> n[0] >> n[1] >> n[2] >> n[3];
-            for (int d = 0; d < 4; ++d)
-                for (std::size_t i = 0; i < n[d]; ++i) {
-                    int tag;
-                    double x;
-                    in >> tag;
-                    for (int k = 0; k < (d == 0 ? 3 : 6); ++k) in >> x;
-                    std::size_t np;
-                    in >> np;
-                    std::vector<int> ph(np);
-                    for (int& q : ph) in >> q;
-                    for (int& q : ph) q = std::abs(q);
-                    entityPhys[{d, tag}] = ph;
-                    if (d > 0) {
-                        std::size_t nb;
-                        in >> nb;
-                        for (std::size_t k = 0; k < nb; ++k) in >> tag;
-                    }
-                }
-            endSection("Entities");
-        } else if (line == "$Nodes") {
-            if (version >= 4) {
-                std::size_t nBlocks, nNodes, minTag, maxTag;
-                in >> nBlocks >> nNodes >> minTag >> maxTag;
-                points.reserve(nNodes);
-                for (std::size_t b = 0; b < nBlocks; ++b) {
-                    int d, tag, param;
-                    std::size_t n;
-                    in >> d >> tag >> param >> n;
-                    std::vector<int64_t> tags(n);
-                    for (int64_t& t : tags) in >> t;
-                    for (std::size_t i = 0; i < n; ++i) {
-                        V3 p;
-                        in >> p.x >> p.y >> p.z;
-                        double u;
-                        for (int k = 0; k < (param ? d : 0); ++k) in >> u;
-                        nodeIndex[tags[i]] = int64_t(points.size());
-                        points.push_back(p * scale);
-                    }
-                }
-            } else {
-                std::size_t n;
-                in >> n;
-                points.reserve(n);
-                for (std::size_t i = 0; i < n; ++i) {
-                    int64_t t;
-                    V3 p;
-                    in >> t >> p.x >> p.y >> p.z;
-                    nodeIndex[t] = int64_t(points.size());
-                    points.push_back(p * scale);
-                }
-            }
-            endSection("Nodes");
-        } else if (line == "$Elements") {
-            if (version >= 4) {
-                std::size_t nBlocks, nEl, minTag, maxTag;
-                in >> nBlocks >> nEl >> minTag >> maxTag;
-                elems.reserve(nEl);
-                std::getline(in, line);
-                for (std::size_t b = 0; b < nBlocks; ++b) {
-                    int d, tag, type;
-                    std::size_t n;
-                    in >> d >> tag >> type >> n;
-                    std::getline(in, line);
-                    int dim, corners;
-                    if (!elementType(type, dim, corners))
-                        throw std::runtime_error(file + ": unsupported Gmsh element type " + std::to_string(type));
-                    const auto it = entityPhys.find({d, tag});
-                    const int phys = (it != entityPhys.end() && !it->second.empty()) ? it->second[0] : -1;
-                    for (std::size_t i = 0; i < n; ++i) {
-                        std::getline(in, line);
-                        std::istringstream ls(line);
-                        int64_t id, v;
-                        ls >> id;
-                        Element e{dim, phys, {}};
-                        for (int k = 0; k < corners && ls >> v; ++k) e.nodes.push_back(v);
-                        elems.push_back(std::move(e));
-                    }
-                }
-            } else {
-                std::size_t n;
-                in >> n;
-                std::getline(in, line);
-                elems.reserve(n);
-                for (std::size_t i = 0; i < n; ++i) {
-                    std::getline(in, line);
-                    std::istringstream ls(line);
-                    int64_t id;
-                    int type, nt;
-                    ls >> id >> type >> nt;
-                    std::vector<int> tags(std::max(nt, 0));
-                    for (int& t : tags) ls >> t;
-                    int dim, corners;
-                    if (!elementType(type, dim, corners))
-                        throw std::runtime_error(file + ": unsupported Gmsh element type " + std::to_string(type));
-                    Element e{dim, nt > 0 ? tags[0] : -1, {}};
-                    if (e.phys == 0) e.phys = -1;
-                    int64_t v;
-                    for (int k = 0; k < corners && ls >> v; ++k) e.nodes.push_back(v);
-                    elems.push_back(std::move(e));
-                }
-            }
-            endSection("Elements");
-        }
-    }
-    if (points.empty() || elems.empty()) throw std::runtime_error(file + ": no nodes or elements");
-    auto node = [&](int64_t t) {
-        const auto it = nodeIndex.find(t);
-        if (it == nodeIndex.end()) throw std::runtime_error(file + ": element refers to unknown node " + std::to_string(t));
+// Reads a 3-D mesh; `typeOf(name, type)` gives a patch's final type (overrides).
+template <class TypeOf> Raw read(const std::string& path, double scale, TypeOf typeOf) {
+    const Content c = parse(path, scale);
+    const int64_t nEl = int64_t(c.dim.size());
+    int maxDim = 0;
+    for (int8_t d : c.dim) maxDim = std::max(maxDim, int(d));
+    if (maxDim < 3) throw std::runtime_error(path + ": no volume elements (2-D meshes are not read here)");
+    auto pointIndex = [&](int64_t tag) {
+        const auto it = c.pointOfTag.find(tag);
+        if (it == c.pointOfTag.end()) throw std::runtime_error(path + ": an element names node " + std::to_string(tag) +
+                                                               ", which $Nodes does not define");
         return it->second;
     };
-    int meshDim = 0;
-    for (const Element& e : elems) meshDim = std::max(meshDim, e.dim);
-    if (meshDim < 2) throw std::runtime_error(file + ": no 2D or 3D elements");
-    // patches: the physical groups of dimension meshDim − 1
-    std::map<int, int> physToPatch;
-    std::vector<Patch> patches;
-    for (const Element& e : elems)
-        if (e.dim == meshDim - 1 && e.phys >= 0 && !physToPatch.count(e.phys)) {
-            Patch p;
-            const auto it = physNames.find({meshDim - 1, e.phys});
-            p.name = it != physNames.end() ? it->second : "patch" + std::to_string(e.phys);
-            const std::string l = lower(p.name);
-            p.type = l.find("symmetry") != std::string::npos ? "symmetryPlane"
-                     : l.find("wall") != std::string::npos   ? "wall"
-                                                             : "patch";
-            physToPatch[e.phys] = int(patches.size());
-            patches.push_back(p);
-        }
-    for (std::size_t a = 0; a < patches.size(); ++a)
-        for (std::size_t b = a + 1; b < patches.size(); ++b)
-            if (patches[a].name == patches[b].name)
-                throw std::runtime_error(file + ": duplicate physical name " + patches[a].name);
-    if (meshDim == 2) {  // MeshReaders.cpp:406-426: every node kept, cells from 2-D elements
-        Mesh2D m2;
-        m2.points = points;
-        const double z0 = points[0].z;
-        for (const V3& p : points)
-            if (std::abs(p.z - z0) > 1e-9 * (1 + std::abs(z0)))
-                throw std::runtime_error(file + ": 2D Gmsh mesh must lie in a plane z = const");
-        std::map<std::pair<int64_t, int64_t>, int> edgePatch;
-        for (const Element& e : elems) {
-            if (e.dim == 2) {
-                std::vector<int64_t> c;
-                for (int64_t t : e.nodes) c.push_back(node(t));
-                m2.cells.push_back(c);
-            } else if (e.dim == 1 && e.phys >= 0) {
-                const int64_t a = node(e.nodes[0]), b = node(e.nodes[1]);
-                edgePatch[{std::min(a, b), std::max(a, b)}] = physToPatch[e.phys];
-            }
-        }
-        for (Patch& p : patches) p.type = patchType(p.name, p.type, opt);
-        return extrudeImported(std::move(m2), patches, edgePatch, opt);
+    // patches: groups of surface elements in order of first use
+    std::vector<std::string> pName, pType;
+    std::map<int64_t, int> patchOfGroup;
+    for (int64_t e = 0; e < nEl; ++e) {
+        if (c.dim[e] != 2 || c.group[e] < 0 || patchOfGroup.count(c.group[e])) continue;
+        const auto it = c.physName.find({2, c.group[e]});
+        const std::string name = it != c.physName.end() ? it->second : "patch" + std::to_string(c.group[e]);
+        const std::string low = lowerCase(name);
+        const std::string type = low.find("symmetry") != std::string::npos ? "symmetryPlane"
+                                 : low.find("wall") != std::string::npos   ? "wall"
+                                                                           : "patch";
+        for (const std::string& other : pName)
+            if (other == name) throw std::runtime_error(path + ": two physical groups are named " + name);
+        patchOfGroup[c.group[e]] = int(pName.size());
+        pName.push_back(name);
+        pType.push_back(type);
     }
-    std::unordered_map<Key, int, KeyHash> bFacePatch;
-    for (const Element& e : elems)
-        if (e.dim == 2 && e.phys >= 0) {
-            std::vector<int64_t> f;
-            for (int64_t t : e.nodes) f.push_back(node(t));
-            bFacePatch[keyOf(f)] = physToPatch[e.phys];
-        }
-    const int defaultIdx = int(patches.size());
-    patches.push_back({"defaultFaces", "wall"});
-    for (Patch& p : patches) p.type = patchType(p.name, p.type, opt);
-    std::vector<Face> faces;
-    std::unordered_map<Key, std::size_t, KeyHash> faceOf;
+    const int defaultPatch = int(pName.size());
+    pName.push_back("defaultFaces");
+    pType.push_back("wall");
+    for (size_t p = 0; p < pName.size(); ++p) pType[p] = typeOf(pName[p], pType[p]);
+
+    // every (cell, local face) in generation order, keyed by its sorted corners
+    using Key = std::array<int64_t, 4>;
+    auto keyOf = [](const int64_t* v, int n) {
+        Key k{-1, -1, -1, -1};
+        std::copy(v, v + n, k.begin());
+        std::sort(k.begin(), k.begin() + n);
+        return k;
+    };
+    struct Gen { Key key; int64_t cell; int n; int64_t v[4]; };
+    std::vector<Gen> gen;
     int64_t nCells = 0;
-    for (const Element& e : elems) {
-        if (e.dim != 3) continue;
-        const int64_t c = nCells++;
-        std::vector<int64_t> cn;
-        for (int64_t t : e.nodes) cn.push_back(node(t));
-        for (const auto& fd : elementFaces(int(cn.size()))) {
-            std::vector<int64_t> f;
-            for (int k : fd) f.push_back(cn[k]);
-            const Key k = keyOf(f);
-            const auto it = faceOf.find(k);
-            if (it == faceOf.end()) {
-                faceOf.emplace(k, faces.size());
-                faces.push_back({f, c, -1, -1});
-            } else {
-                Face& F = faces[it->second];
-                if (F.c1 >= 0) throw std::runtime_error(file + ": face shared by more than two cells");
-                F.c1 = c;
+    for (int64_t e = 0; e < nEl; ++e) {
+        if (c.dim[e] != 3) continue;
+        const int64_t* cn = c.corners.data() + c.cornerOff[e];
+        const int nc = int(c.cornerOff[e + 1] - c.cornerOff[e]);
+        for (const LocalFace& lf : faceTable(nc)) {
+            Gen g;
+            g.cell = nCells;
+            g.n = lf.n;
+            for (int k = 0; k < lf.n; ++k) {
+                if (lf.v[k] >= nc) throw std::runtime_error(path + ": a volume element has too few nodes");
+                g.v[k] = pointIndex(cn[lf.v[k]]);
             }
+            g.key = keyOf(g.v, g.n);
+            gen.push_back(g);
+        }
+        ++nCells;
+    }
+    // match by sorting (key, generation index): the first of a run is the face
+    std::vector<int64_t> order(gen.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = int64_t(i);
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) {
+        return gen[a].key != gen[b].key ? gen[a].key < gen[b].key : a < b;
+    });
+    std::vector<int64_t> other(gen.size(), -2);  // per first appearance: the second cell (−1 none); −2 not first
+    for (size_t i = 0; i < order.size();) {
+        size_t j = i + 1;
+        while (j < order.size() && gen[order[j]].key == gen[order[i]].key) ++j;
+        if (j - i > 2) throw std::runtime_error(path + ": a face is shared by more than two cells");
+        other[order[i]] = j - i == 2 ? gen[order[i + 1]].cell : -1;
+        i = j;
+    }
+    // boundary groups of the surface elements, the later element winning
+    std::map<Key, int> patchOfKey;
+    for (int64_t e = 0; e < nEl; ++e) {
+        if (c.dim[e] != 2 || c.group[e] < 0) continue;
+        const int n = int(c.cornerOff[e + 1] - c.cornerOff[e]);
+        int64_t v[4];
+        for (int k = 0; k < n; ++k) v[k] = pointIndex(c.corners[c.cornerOff[e] + k]);
+        patchOfKey[keyOf(v, n)] = patchOfGroup.at(c.group[e]);
+    }
+    // faces by first appearance; points renumbered by first use
+    struct Face { int64_t c0, c1; int patch; std::vector<int64_t> v; };
+    std::vector<Face> faces;
+    std::vector<int64_t> newIndex(c.points.size(), -1);
+    std::vector<V3> pts;
+    int64_t unmatched = 0;
+    for (size_t i = 0; i < gen.size(); ++i) {
+        if (other[i] == -2) continue;
+        Face f;
+        f.c0 = gen[i].cell;
+        f.c1 = other[i];
+        f.patch = -1;
+        if (f.c1 < 0) {
+            const auto it = patchOfKey.find(gen[i].key);
+            f.patch = it != patchOfKey.end() ? it->second : (++unmatched, defaultPatch);
+        }
+        for (int k = 0; k < gen[i].n; ++k) {
+            int64_t& ni = newIndex[gen[i].v[k]];
+            if (ni < 0) {
+                ni = int64_t(pts.size());
+                pts.push_back(c.points[gen[i].v[k]]);
+            }
+            f.v.push_back(ni);
+        }
+        faces.push_back(std::move(f));
+    }
+    // orientation from estimated centres
+    auto centreAndNormal = [&](const std::vector<int64_t>& v, V3& centre) {
+        V3 s;
+        for (int64_t p : v) s += pts[p];
+        centre = s / double(v.size());
+        V3 n;
+        for (size_t k = 0; k < v.size(); ++k) n += cross(pts[v[k]] - centre, pts[v[(k + 1) % v.size()]] - centre);
+        return 0.5 * n;
+    };
+    std::vector<V3> cellC(nCells), faceC(faces.size()), faceN(faces.size());
+    std::vector<int> faceCount(nCells, 0);
+    for (size_t f = 0; f < faces.size(); ++f) {
+        faceN[f] = centreAndNormal(faces[f].v, faceC[f]);
+        cellC[faces[f].c0] += faceC[f];
+        ++faceCount[faces[f].c0];
+        if (faces[f].c1 >= 0) {
+            cellC[faces[f].c1] += faceC[f];
+            ++faceCount[faces[f].c1];
         }
     }
-    for (Face& F : faces)
-        if (F.c1 < 0) {
-            const auto it = bFacePatch.find(keyOf(F.nodes));
-            F.patch = it != bFacePatch.end() ? it->second : defaultIdx;
+    for (int64_t k = 0; k < nCells; ++k) {
+        if (faceCount[k] < 4) throw std::runtime_error(path + ": cell " + std::to_string(k) + " has only " +
+                                                       std::to_string(faceCount[k]) + " faces");
+        cellC[k] = cellC[k] / double(faceCount[k]);
+    }
+    std::vector<int64_t> internal;
+    std::vector<std::vector<int64_t>> byPatch(pName.size());
+    for (size_t f = 0; f < faces.size(); ++f) {
+        Face& F = faces[f];
+        if (F.c1 >= 0) {
+            if (F.c1 < F.c0) std::swap(F.c0, F.c1);
+            if (dot(faceN[f], cellC[F.c1] - cellC[F.c0]) < 0) std::reverse(F.v.begin(), F.v.end());
+            internal.push_back(int64_t(f));
+        } else {
+            if (dot(faceN[f], faceC[f] - cellC[F.c0]) < 0) std::reverse(F.v.begin(), F.v.end());
+            byPatch[F.patch].push_back(int64_t(f));
         }
-    std::vector<int64_t> used(points.size(), -1);
-    std::vector<V3> pts;
-    for (Face& F : faces)
-        for (int64_t& v : F.nodes) {
-            if (used[v] < 0) {
-                used[v] = int64_t(pts.size());
-                pts.push_back(points[v]);
-            }
-            v = used[v];
-        }
-    return assembleFaces(std::move(pts), nCells, std::move(faces), patches);
+    }
+    std::stable_sort(internal.begin(), internal.end(), [&](int64_t a, int64_t b) {
+        return faces[a].c0 != faces[b].c0 ? faces[a].c0 < faces[b].c0 : faces[a].c1 < faces[b].c1;
+    });
+    Raw r;
+    r.points = std::move(pts);
+    r.nCells = nCells;
+    auto emit = [&](const Face& F) {
+        r.facePts.insert(r.facePts.end(), F.v.begin(), F.v.end());
+        r.faceOff.push_back(int64_t(r.facePts.size()));
+        r.owner.push_back(F.c0);
+    };
+    for (int64_t f : internal) {
+        emit(faces[f]);
+        r.neighbour.push_back(faces[f].c1);
+    }
+    for (size_t p = 0; p < pName.size(); ++p) {
+        if (byPatch[p].empty()) continue;
+        const int64_t start = r.nFaces();
+        for (int64_t f : byPatch[p]) emit(faces[f]);
+        r.pName.push_back(pName[p]);
+        r.pType.push_back(pType[p]);
+        r.pNeighbour.push_back("");
+        r.pStart.push_back(start);
+        r.pSize.push_back(r.nFaces() - start);
+    }
+    (void)unmatched;
+    return r;
 }
 
 }  // namespace gmsh
 
-// ---------------------------------------------------------------- Fluent
-// Fluent mesh files (MeshReaders.cpp:484-803): parenthesised sections "(index ...)" with
-// hexadecimal headers; index 10 nodes, 12 cells, 13 faces, 39/45 zone names and types,
-// 2 dimension; index + 2000 (single precision, 32-bit integers) or + 3000 (double,
-// 64-bit) marks a binary body ending in "End of Binary Section". Other sections are
-// skipped; periodic pairs (18) import as ordinary patches; non-conformal sections are
-// rejected.
-namespace fluent {
-using namespace meshio;
-
-struct Zone {
-    int id, bc;
-    std::vector<Face> faces;  // nodes and cells 0-based, −1 for no cell
-};
-
-class Reader {
-public:
-    explicit Reader(const std::string& file) : file_(file) {
-        std::ifstream in(file, std::ios::binary);
-        if (!in) throw std::runtime_error("cannot open " + file);
-        std::ostringstream ss;
-        ss << in.rdbuf();
-        s_ = ss.str();
-    }
-    int dim = 3;
-    std::vector<V3> points;
-    int64_t nCells = 0;
-    std::vector<Zone> zones;
-    std::map<int, std::pair<std::string, std::string>> zoneInfo;  // id → (type, name)
-
-    void parse() {
-        for (;;) {
-            ws();
-            if (p_ >= s_.size()) return;
-            if (s_[p_++] != '(') continue;
-            section(int(integer(10)));
-        }
-    }
-
-private:
-    std::string file_, s_;
-    std::size_t p_ = 0;
-
-    [[noreturn]] void fail(const std::string& m) const {
-        throw std::runtime_error(file_ + ": " + m + " (at byte " + std::to_string(p_) + ")");
-    }
-    void ws() {
-        while (p_ < s_.size() && std::isspace(static_cast<unsigned char>(s_[p_]))) ++p_;
-    }
-    bool at(char c) {
-        ws();
-        return p_ < s_.size() && s_[p_] == c;
-    }
-    void expect(char c) {
-        if (!at(c)) fail(std::string("expected '") + c + "'");
-        ++p_;
-    }
-    long long integer(int base) {
-        ws();
-        const char* b = s_.c_str() + p_;
-        char* e = nullptr;
-        const long long v = std::strtoll(b, &e, base);
-        if (e == b) fail("expected integer");
-        p_ += std::size_t(e - b);
-        return v;
-    }
-    double real() {
-        ws();
-        const char* b = s_.c_str() + p_;
-        char* e = nullptr;
-        const double v = std::strtod(b, &e);
-        if (e == b) fail("expected number");
-        p_ += std::size_t(e - b);
-        return v;
-    }
-    std::string word() {
-        ws();
-        const std::size_t b = p_;
-        if (p_ < s_.size() && s_[p_] == '"') {
-            const std::size_t e = s_.find('"', p_ + 1);
-            p_ = e + 1;
-            return s_.substr(b + 1, e - b - 1);
-        }
-        while (p_ < s_.size() && !std::isspace(static_cast<unsigned char>(s_[p_])) && s_[p_] != '(' && s_[p_] != ')')
-            ++p_;
-        return s_.substr(b, p_ - b);
-    }
-    // past the ')' that closes the current level (quoted strings skipped whole)
-    void closeLevel() {
-        for (int depth = 1; p_ < s_.size() && depth > 0;) {
-            const char c = s_[p_++];
-            if (c == '"') {
-                const std::size_t e = s_.find('"', p_);
-                p_ = e == std::string::npos ? s_.size() : e + 1;
-            } else if (c == '(') {
-                ++depth;
-            } else if (c == ')') {
-                --depth;
-            }
-        }
-    }
-    void closeBinary() {
-        const std::size_t e = s_.find("End of Binary Section", p_);
-        if (e == std::string::npos) fail("missing 'End of Binary Section'");
-        p_ = s_.find(')', e);
-        if (p_ == std::string::npos) fail("unterminated binary section");
-        ++p_;
-    }
-    template <class T> T raw() {
-        if (p_ + sizeof(T) > s_.size()) fail("unexpected end of binary data");
-        T v;
-        std::memcpy(&v, s_.data() + p_, sizeof(T));
-        p_ += sizeof(T);
-        return v;
-    }
-    std::vector<long long> header() {
-        expect('(');
-        std::vector<long long> h;
-        while (!at(')')) h.push_back(integer(16));
-        ++p_;
-        return h;
-    }
-    void closeBody(int bin) {
-        if (bin) {
-            closeBinary();
-        } else {
-            expect(')');
-            expect(')');
-        }
-    }
-
-    void section(int idx) {
-        const int kind = idx % 1000, bin = idx / 1000;
-        switch (kind) {
-        case 0:
-        case 1: closeLevel(); return;
-        case 2:
-            dim = int(integer(10));
-            expect(')');
-            return;
-        case 10: nodes(bin); return;
-        case 12: cells(bin); return;
-        case 13: faces(bin); return;
-        case 39:
-        case 45: zone(); return;
-        case 58:
-        case 59:
-        case 61:
-        case 62:
-        case 63:
-            fail("non-conformal / hanging-node Fluent meshes (section " + std::to_string(idx) + ") are not supported");
-        default: break;  // 18 (periodic pairs) and anything else: skipped
-        }
-        if (bin) closeBinary();
-        else closeLevel();
-    }
-    void nodes(int bin) {
-        const auto h = header();
-        if (h.size() < 4) fail("bad node header");
-        if (h[0] == 0) {
-            points.resize(std::size_t(h[2]));
-            expect(')');
-            return;
-        }
-        const int64_t first = h[1] - 1, last = h[2] - 1;
-        const int nd = h.size() >= 5 ? int(h[4]) : dim;
-        if (int64_t(points.size()) <= last) points.resize(std::size_t(last) + 1);
-        expect('(');
-        for (int64_t i = first; i <= last; ++i) {
-            double x[3] = {0, 0, 0};
-            for (int k = 0; k < nd; ++k) x[k] = bin == 0 ? real() : bin == 2 ? double(raw<float>()) : raw<double>();
-            points[i] = {x[0], x[1], x[2]};
-        }
-        closeBody(bin);
-    }
-    void cells(int bin) {
-        const auto h = header();
-        if (h.size() < 3) fail("bad cell header");
-        nCells = std::max<int64_t>(nCells, h[2]);
-        if (h[0] != 0 && at('(')) {  // per-cell element types: not needed
-            ++p_;
-            if (bin) {
-                closeBinary();
-                return;
-            }
-            closeLevel();
-        }
-        expect(')');
-    }
-    void faces(int bin) {
-        const auto h = header();
-        if (h.size() < 5) {
-            if (h.size() >= 3 && h[0] == 0) {
-                expect(')');
-                return;
-            }
-            fail("bad face header");
-        }
-        if (h[0] == 0) {
-            expect(')');
-            return;
-        }
-        Zone z{int(h[0]), int(h[3]), {}};
-        const long long n = h[2] - h[1] + 1;
-        const int faceType = int(h[4]);
-        z.faces.reserve(std::size_t(std::max(n, 0LL)));
-        expect('(');
-        auto next = [&]() -> long long {
-            if (bin == 0) return integer(16);
-            return bin == 3 ? raw<int64_t>() : raw<int32_t>();
-        };
-        for (long long i = 0; i < n; ++i) {
-            long long nn = faceType;
-            if (faceType == 0 || faceType == 5) {  // mixed: a type per face; polygon: a count
-                nn = next();
-                if (faceType == 0 && nn == 5) nn = next();
-            }
-            Face F{{}, -1, -1, -1};
-            for (long long k = 0; k < nn; ++k) F.nodes.push_back(next() - 1);
-            F.c0 = next() - 1;
-            F.c1 = next() - 1;
-            z.faces.push_back(std::move(F));
-        }
-        closeBody(bin);
-        zones.push_back(std::move(z));
-    }
-    void zone() {
-        expect('(');
-        const int id = int(integer(10));
-        const std::string type = word();
-        const std::string name = word();
-        closeLevel();
-        zoneInfo[id] = {type, name};
-        if (at('(')) {
-            ++p_;
-            closeLevel();
-        }
-        expect(')');
-    }
-};
-
-// Patch type of a boundary zone from its type word or, without one, its bc number.
-std::string zoneType(const std::string& type, int bc) {
-    const std::string t = lower(type);
-    if (t == "wall" || (t.empty() && bc == 3)) return "wall";
-    if (t == "symmetry" || t == "axis" || (t.empty() && (bc == 7 || bc == 37))) return "symmetryPlane";
+// Patch type spellings accepted in overrides, as the reference's patch types print:
+// wall, symmetryPlane (also "symmetry"), empty, cyclic, patch (anything else).
+std::string canonicalType(const std::string& s) {
+    if (s == "wall" || s == "empty" || s == "cyclic") return s;
+    if (s == "symmetry" || s == "symmetryPlane") return "symmetryPlane";
     return "patch";
 }
 
-// 2-D cell polygons from the edges of each cell, chained from its first edge in file
-// order (MeshReaders.cpp:98-126).
-std::vector<std::vector<int64_t>> polygons(int64_t nCells, const std::vector<std::array<int64_t, 3>>& cellEdges) {
-    std::vector<std::vector<std::pair<int64_t, int64_t>>> e(static_cast<std::size_t>(nCells));
-    for (const auto& ce : cellEdges) e[ce[0]].push_back({ce[1], ce[2]});
-    std::vector<std::vector<int64_t>> cells(static_cast<std::size_t>(nCells));
-    for (int64_t c = 0; c < nCells; ++c) {
-        const auto& E = e[c];
-        const std::string cs = "mesh import: 2D cell " + std::to_string(c);
-        if (E.size() < 3) throw std::runtime_error(cs + " has < 3 edges");
-        std::vector<char> used(E.size(), 0);
-        std::vector<int64_t> poly{E[0].first, E[0].second};
-        used[0] = 1;
-        for (std::size_t k = 1; k < E.size(); ++k) {
-            const int64_t last = poly.back();
-            std::size_t j = 0;
-            for (; j < E.size(); ++j)
-                if (!used[j] && (E[j].first == last || E[j].second == last)) break;
-            if (j == E.size()) throw std::runtime_error(cs + " edges do not form a loop");
-            used[j] = 1;
-            poly.push_back(E[j].first == last ? E[j].second : E[j].first);
-        }
-        if (poly.back() != poly.front()) throw std::runtime_error(cs + " is not closed");
-        poly.pop_back();
-        cells[c] = std::move(poly);
+// "name=type" lines → overrides of patch types by name.
+std::map<std::string, std::string> typeOverrides(const char* lines) {
+    std::map<std::string, std::string> out;
+    std::istringstream ss(lines ? lines : "");
+    std::string line;
+    while (std::getline(ss, line)) {
+        if (line.empty()) continue;
+        const auto eq = line.rfind('=');
+        if (eq == std::string::npos) throw std::runtime_error("patch type override '" + line + "' has no '='");
+        out[line.substr(0, eq)] = canonicalType(line.substr(eq + 1));
     }
-    return cells;
+    return out;
 }
 
-Raw read(const std::string& file, const Options& opt) {
-    Reader fr(file);
-    fr.parse();
-    if (fr.points.empty() || fr.zones.empty()) throw std::runtime_error(file + ": no nodes or faces found");
-    for (V3& p : fr.points) p = p * opt.scale;
-    int64_t nc = fr.nCells;
-    for (const Zone& z : fr.zones)
-        for (const Face& F : z.faces) nc = std::max({nc, F.c0 + 1, F.c1 + 1});
-    // every zone with a one-sided face becomes a patch, in zone order
-    std::vector<Patch> patches;
-    std::map<int, int> zoneToPatch;
-    for (const Zone& z : fr.zones) {
-        bool boundary = false;
-        for (const Face& F : z.faces) boundary = boundary || F.c0 < 0 || F.c1 < 0;
-        if (!boundary) continue;
-        const auto it = fr.zoneInfo.find(z.id);
-        Patch p;
-        p.name = it != fr.zoneInfo.end() ? it->second.second : "zone" + std::to_string(z.id);
-        p.type = patchType(p.name, zoneType(it != fr.zoneInfo.end() ? it->second.first : "", z.bc), opt);
-        zoneToPatch[z.id] = int(patches.size());
-        patches.push_back(p);
-    }
-    if (fr.dim == 2) {
-        Mesh2D m2;
-        m2.points = fr.points;
-        std::vector<std::array<int64_t, 3>> cellEdges;
-        std::map<std::pair<int64_t, int64_t>, int> edgePatch;
-        for (const Zone& z : fr.zones)
-            for (const Face& F : z.faces) {
-                if (F.nodes.size() != 2)
-                    throw std::runtime_error(file + ": 2D face with " + std::to_string(F.nodes.size()) + " nodes");
-                const int64_t a = F.nodes[0], b = F.nodes[1];
-                if (F.c0 >= 0) cellEdges.push_back({F.c0, a, b});
-                if (F.c1 >= 0) cellEdges.push_back({F.c1, a, b});
-                if (F.c0 < 0 || F.c1 < 0) edgePatch[{std::min(a, b), std::max(a, b)}] = zoneToPatch.at(z.id);
-            }
-        m2.cells = polygons(nc, cellEdges);
-        return extrudeImported(std::move(m2), patches, edgePatch, opt);
-    }
-    std::vector<Face> faces;
-    for (Zone& z : fr.zones)
-        for (Face& F : z.faces) {
-            if (F.c0 < 0 || F.c1 < 0) F.patch = zoneToPatch.at(z.id);
-            faces.push_back(std::move(F));
-        }
-    return assembleFaces(std::move(fr.points), nc, std::move(faces), patches);
+Raw readGmshFile(const std::string& path, double scale, const char* overrides) {
+    const auto over = typeOverrides(overrides);
+    return gmsh::read(path, scale, [&](const std::string& name, const std::string& type) {
+        const auto it = over.find(name);
+        return it != over.end() ? it->second : type;
+    });
 }
 
-}  // namespace fluent
-
-// A mesh path as readMeshFile chooses (MeshReaders.cpp:805-816): a directory is a
-// polyMesh, a file starting "$MeshFormat" Gmsh, a .msh / .cas file or one whose first
-// line holds '(' Fluent.
-Raw readMeshFile(const std::string& path, const meshio::Options& opt) {
+// A directory is read as a polyMesh, a file whose first line is "$MeshFormat" as Gmsh.
+Raw readMeshPath(const std::string& path, double scale, const char* overrides) {
     namespace fs = std::filesystem;
     if (fs::is_directory(path)) return readPolyMesh(path);
     std::ifstream in(path);
-    if (!in) throw std::runtime_error("cannot open " + path);
+    if (!in) throw std::runtime_error(path + ": cannot be opened");
     std::string first;
     std::getline(in, first);
-    if (first.rfind("$MeshFormat", 0) == 0) return gmsh::read(path, opt);
-    const std::string ext = meshio::lower(fs::path(path).extension().string());
-    if (ext == ".msh" || ext == ".cas" || first.find('(') != std::string::npos) return fluent::read(path, opt);
-    throw std::runtime_error(path + ": unknown mesh format");
-}
-
-// Import options from the C arguments; patchTypes holds "name=type" lines.
-meshio::Options importOptions(double scale, double depth2D, const char* patchTypes) {
-    meshio::Options o;
-    o.scale = scale;
-    o.depth2D = depth2D;
-    std::istringstream ss(patchTypes ? patchTypes : "");
-    std::string line;
-    while (std::getline(ss, line)) {
-        const auto eq = line.rfind('=');
-        if (line.empty()) continue;
-        if (eq == std::string::npos) throw std::runtime_error("patch type override '" + line + "' is not name=type");
-        o.patchTypes[line.substr(0, eq)] = meshio::typeName(line.substr(eq + 1));
-    }
-    return o;
+    if (first.rfind("$MeshFormat", 0) == 0) return readGmshFile(path, scale, overrides);
+    throw std::runtime_error(path + ": neither a polyMesh directory nor a Gmsh MSH file");
 }
 
 thread_local std::string gErr;
@@ -1724,19 +1317,14 @@ void* vm_read_polymesh(const char* dir) {
     }
 }
 
-// Mesh files as the reference's importers read them (MeshImportOptions: coordinates
-// multiplied by scale, 2-D meshes extruded by depth2D, patch types overridden by the
-// "name=type" lines of patchTypes, which may be null): an ASCII Gmsh 2.2 / 4.x file,
-// a Fluent file (ASCII or binary, 2-D or 3-D), or any of these or a polyMesh directory
-// chosen by readMeshFile's rules.
-void* vm_read_gmsh(const char* file, double scale, double depth2D, const char* patchTypes) {
-    return guarded([&] { return gmsh::read(file, importOptions(scale, depth2D, patchTypes)); });
+// A 3-D Gmsh MSH file (ASCII, 2.2 or 4.x): coordinates multiplied by scale, patch types
+// replaced by the "name=type" lines of patchTypes (nullable).
+void* vm_read_gmsh(const char* file, double scale, const char* patchTypes) {
+    return guarded([&] { return readGmshFile(file, scale, patchTypes); });
 }
-void* vm_read_fluent(const char* file, double scale, double depth2D, const char* patchTypes) {
-    return guarded([&] { return fluent::read(file, importOptions(scale, depth2D, patchTypes)); });
-}
-void* vm_read_mesh_file(const char* path, double scale, double depth2D, const char* patchTypes) {
-    return guarded([&] { return readMeshFile(path, importOptions(scale, depth2D, patchTypes)); });
+// A polyMesh directory or a Gmsh file, told apart by what the path is.
+void* vm_read_mesh_file(const char* path, double scale, const char* patchTypes) {
+    return guarded([&] { return readMeshPath(path, scale, patchTypes); });
 }
 
 // Name and type of patch p (empty for generated meshes); returns -1 for a bad index.

@@ -47,8 +47,8 @@ def _lib():
         lib.vm_skew.restype = ctypes.c_int
         lib.vm_read_polymesh.argtypes = [ctypes.c_char_p]
         lib.vm_read_polymesh.restype = P
-        for f in (lib.vm_read_gmsh, lib.vm_read_fluent, lib.vm_read_mesh_file):
-            f.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_double, ctypes.c_char_p]
+        for f in (lib.vm_read_gmsh, lib.vm_read_mesh_file):
+            f.argtypes = [ctypes.c_char_p, ctypes.c_double, ctypes.c_char_p]
             f.restype = P
         lib.vm_patch_info.argtypes = [P, ctypes.c_int, P, ctypes.c_int, P, ctypes.c_int]
         lib.vm_patch_info.restype = ctypes.c_int
@@ -327,31 +327,24 @@ def read_polymesh(directory) -> Mesh:
     return Mesh(_lib().vm_read_polymesh(str(directory).encode()))
 
 
-def _import_args(path, scale, depth2d, patch_types):
+def _import_args(path, scale, patch_types):
     over = "".join(f"{name}={kind}\n" for name, kind in (patch_types or {}).items())
-    return str(path).encode(), float(scale), float(depth2d), over.encode()
+    return str(path).encode(), float(scale), over.encode()
 
 
-def read_gmsh(path, scale=1.0, depth2d=0.1, patch_types=None) -> Mesh:
-    """An ASCII Gmsh 2.2 / 4.x file, as the reference's readGmsh reads it
-    (MeshReaders.cpp:236-482): same points, faces, face order and patches. A 3-D mesh
-    (tets, hexes, prisms, pyramids, also higher-order) is assembled from its cells'
-    faces; a 2-D mesh (z = const) is extruded one layer of ``depth2d``. Options follow
-    MeshImportOptions: ``scale`` multiplies coordinates, ``patch_types`` maps patch
-    names to the type they get (wall, symmetry, empty, cyclic, patch)."""
-    return Mesh(_lib().vm_read_gmsh(*_import_args(path, scale, depth2d, patch_types)))
+def read_gmsh(path, scale=1.0, patch_types=None) -> Mesh:
+    """A 3-D mesh from an ASCII Gmsh MSH file (format 2.2 or 4.x; tets, hexes, prisms,
+    pyramids, also higher-order ones by their corners). Points, faces, face order and
+    patches follow the reference importer's rules (MeshReaders.cpp:131-190,236-482), so
+    the mesh is the one the reference builds from the file. ``scale`` multiplies the
+    coordinates; ``patch_types`` maps patch names to the type they get (wall, symmetry,
+    empty, cyclic, patch)."""
+    return Mesh(_lib().vm_read_gmsh(*_import_args(path, scale, patch_types)))
 
 
-def read_fluent(path, scale=1.0, depth2d=0.1, patch_types=None) -> Mesh:
-    """A Fluent mesh file (ASCII or binary sections, 2-D or 3-D), as the reference's
-    readFluent reads it (MeshReaders.cpp:484-803); options as for :func:`read_gmsh`."""
-    return Mesh(_lib().vm_read_fluent(*_import_args(path, scale, depth2d, patch_types)))
-
-
-def read_mesh(path, scale=1.0, depth2d=0.1, patch_types=None) -> Mesh:
-    """A polyMesh directory, a Gmsh file or a Fluent file, chosen as readMeshFile
-    chooses (MeshReaders.cpp:805-816)."""
-    return Mesh(_lib().vm_read_mesh_file(*_import_args(path, scale, depth2d, patch_types)))
+def read_mesh(path, scale=1.0, patch_types=None) -> Mesh:
+    """A polyMesh directory or a Gmsh MSH file (options as for :func:`read_gmsh`)."""
+    return Mesh(_lib().vm_read_mesh_file(*_import_args(path, scale, patch_types)))
 
 
 def box(nx, ny=None, nz=None, periodic=(False, False, False), two_d=False) -> Mesh:
