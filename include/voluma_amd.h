@@ -176,6 +176,23 @@ int vol_amul(vol_matrix A, const double* x, double* y, int location);
 int vol_solve(vol_matrix A, const double* b, double* x, const vol_solver_controls* controls,
               vol_solver_performance* perf, int location);
 
+/* No counterpart in the reference. For callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): the
+ * same solve as vol_solve with b and x in device memory (VOL_DEVICE), whose work
+ * vectors, reduction partials and loop scalars live in a fixed array of
+ * VOL_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends) instead of buffers the
+ * library allocates. It calls no allocation routine. Limits: at most
+ * VOL_NOALLOC_MAX_CELLS cells (VOL_ERR_INVALID_ARGUMENT beyond); solver PCG or
+ * PBiCGStab, preconditioner none or diagonal, precision VOL_FP64, any sums mode
+ * (VOL_ERR_UNSUPPORTED otherwise; reproducible mode swaps every preconditioner to
+ * diagonal as vol_solve does). Results are bit-identical to vol_solve's. Every call on
+ * one GPU uses that GPU's one array: calls that overlap on a GPU (two threads or two
+ * contexts) race. */
+#define VOL_NOALLOC_MAX_CELLS 262144
+#define VOL_NOALLOC_SCRATCH_BYTES (24u * 1024u * 1024u)
+int vol_solve_noalloc(vol_matrix A, const double* b, double* x, const vol_solver_controls* controls,
+                      vol_solver_performance* perf);
+
 /* ---- coefficient assembly on the GPU, followed by the boundary fold of FvMatrix::solve
  * (FvMatrix.h:154-163), for a scalar field whose patches are fixedValue (bcFixed[bf] = 1,
  * value bcValue[bf], nullable → 0) or zeroGradient (0) (BoundaryCondition.h:116-137).

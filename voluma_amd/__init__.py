@@ -119,7 +119,8 @@ EXPORTED = ("vol_controls_default", "vol_last_error", "vol_context_create", "vol
             "vol_piso_hbya_boundary", "vol_piso_face_op", "vol_piso_ddt_phi_corr", "vol_piso_constrain_pressure",
             "vol_fvc_reconstruct", "vol_piso_correct_U", "vol_fvm_ddt",
             "vol_mesh_set_empty_dir", "vol_boundary_values", "vol_copy_strided", "vol_flow_options_default",
-            "vol_flow_create", "vol_flow_set_fields", "vol_flow_get_fields", "vol_flow_step", "vol_flow_destroy")
+            "vol_flow_create", "vol_flow_set_fields", "vol_flow_get_fields", "vol_flow_step", "vol_flow_destroy",
+            "vol_solve_noalloc")
 
 
 def lib():
@@ -149,6 +150,7 @@ def lib():
         L.vol_matrix_destroy.argtypes = [P]
         L.vol_amul.argtypes = [P, P, P, I]
         L.vol_solve.argtypes = [P, P, P, ctypes.POINTER(_Controls), ctypes.POINTER(_Perf), I]
+        L.vol_solve_noalloc.argtypes = [P, P, P, ctypes.POINTER(_Controls), ctypes.POINTER(_Perf)]
         L.vol_timing_enable.argtypes = [P, I]
         L.vol_timing_read.argtypes = [P, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_int64), I]
         L.vol_mesh_set_boundary.argtypes = [P, I, P]

@@ -34,6 +34,53 @@ namespace {
 
 thread_local std::string gLastError;
 
+// ---------------------------------------------------------------- no-allocation scratch
+// Work memory of vol_solve_noalloc: one fixed array per GPU, held until the process
+// ends. Layout (256-byte aligned pieces): 11 vectors of VOL_NOALLOC_MAX_CELLS doubles,
+// the fast-sum partials (MAX_RED × NOALLOC_PART_BLOCKS doubles), the exact-sum bins
+// (MAX_RED × RED_BLOCKS_EXACT × 72 int64) and non-finite flags, the stored results
+// and the solve's two DevScalars. Every piece is written by the solve before it is read
+// (the partials and bins by each reducing block, DevScalars by launch_init_scalars), so
+// nothing has to start at zero; calls that overlap on one GPU share it and race.
+constexpr int NOALLOC_VECTORS = 11;
+constexpr int NOALLOC_PART_BLOCKS = 1024;  // ≥ max(RED_BLOCKS, SELL grid blocks at the cell limit)
+constexpr size_t noallocRound(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t NOALLOC_VEC_BYTES = noallocRound(sizeof(double) * size_t(VOL_NOALLOC_MAX_CELLS));
+constexpr size_t NOALLOC_PART_OFF = NOALLOC_VECTORS * NOALLOC_VEC_BYTES;
+constexpr size_t NOALLOC_XPART_OFF = NOALLOC_PART_OFF + noallocRound(sizeof(double) * MAX_RED * NOALLOC_PART_BLOCKS);
+constexpr size_t NOALLOC_XNF_OFF =
+    NOALLOC_XPART_OFF + noallocRound(sizeof(long long) * MAX_RED * RED_BLOCKS_EXACT * NBINS);
+constexpr size_t NOALLOC_RESULT_OFF = NOALLOC_XNF_OFF + noallocRound(sizeof(int) * MAX_RED * RED_BLOCKS_EXACT);
+constexpr size_t NOALLOC_SCAL_OFF = NOALLOC_RESULT_OFF + noallocRound(sizeof(double) * MAX_RED);
+constexpr size_t NOALLOC_BYTES = NOALLOC_SCAL_OFF + noallocRound(2 * sizeof(DevScalars));
+static_assert(NOALLOC_BYTES <= VOL_NOALLOC_SCRATCH_BYTES, "scratch layout exceeds the published size");
+
+static __device__ __align__(256) unsigned char scratch[VOL_NOALLOC_SCRATCH_BYTES];
+
+// Device address of `scratch` on each GPU (each holds its own copy), looked up once
+// per device with that device current. The host-side name `scratch` is not a device
+// address and is never passed to a kernel or a copy.
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex gScratchMu;
+unsigned char* gScratchPtr[NOALLOC_MAX_DEVICES] = {};
+
+// Called by vol_context_create with `device` current.
+void registerScratch(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return;  // such devices get no no-allocation solve
+    std::lock_guard<std::mutex> lk(gScratchMu);
+    if (gScratchPtr[device]) return;
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+        throw std::runtime_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    gScratchPtr[device] = static_cast<unsigned char*>(p);
+}
+
+unsigned char* scratchOf(int device) {
+    std::lock_guard<std::mutex> lk(gScratchMu);
+    return device >= 0 && device < NOALLOC_MAX_DEVICES ? gScratchPtr[device] : nullptr;
+}
+
 }  // namespace
 
 namespace vol::host {
@@ -1749,6 +1796,7 @@ int vol_context_create(int device, vol_context* out) {
     HIPCHECK(hipHostMalloc((void**)&c->hostScal, 2 * sizeof(DevScalars), hipHostMallocDefault));
     c->ensureScratch(RED_BLOCKS);
     for (hipEvent_t& e : c->loopEv) HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    registerScratch(device);
     *out = c.release();
     API_END
 }
@@ -2657,6 +2705,53 @@ int vol_solve(vol_matrix A, const double* b, double* x, const vol_solver_control
         HIPCHECK(hipMemcpyAsync(x, A->vx.p, sizeof(double) * n, hipMemcpyDeviceToHost, st));
         HIPCHECK(hipStreamSynchronize(st));
     }
+    if (perf) *perf = pf;
+    API_END
+}
+
+int vol_solve_noalloc(vol_matrix A, const double* b, double* x, const vol_solver_controls* controls,
+                      vol_solver_performance* perf) {
+    API_BEGIN
+    if (!A || !A->set || !b || !x) throw VolError(VOL_ERR_INVALID_ARGUMENT, "matrix not set or null vector");
+    vol_mesh m = A->mesh;
+    vol_context c = m->ctx;
+    const int n = m->n;
+    if (n > VOL_NOALLOC_MAX_CELLS)
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "vol_solve_noalloc: " + std::to_string(n) +
+                                                     " cells exceed its scratch limit of " +
+                                                     std::to_string(VOL_NOALLOC_MAX_CELLS));
+    if (sell_grid_blocks(m->sell.nslices) > NOALLOC_PART_BLOCKS)
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "vol_solve_noalloc: matrix layout exceeds the scratch partials");
+    unsigned char* base = scratchOf(c->device);
+    if (!base) throw VolError(VOL_ERR_UNSUPPORTED, "vol_solve_noalloc: no scratch on this device");
+    vol_solver_controls ct;
+    if (controls)
+        ct = *controls;
+    else
+        vol_controls_default(&ct);
+    Workspace ws;
+    double* v[NOALLOC_VECTORS];
+    for (int k = 0; k < NOALLOC_VECTORS; ++k) v[k] = reinterpret_cast<double*>(base + k * NOALLOC_VEC_BYTES);
+    ws.wA = v[0];
+    ws.pA = v[1];
+    ws.vr = v[2];
+    ws.vw = v[3];
+    ws.vp = v[4];
+    ws.r0 = v[5];
+    ws.y = v[6];
+    ws.Ay = v[7];
+    ws.s = v[8];
+    ws.z = v[9];
+    ws.t = v[10];
+    ws.red.part = reinterpret_cast<double*>(base + NOALLOC_PART_OFF);
+    ws.red.maxBlk = NOALLOC_PART_BLOCKS;
+    ws.red.xpart = reinterpret_cast<long long*>(base + NOALLOC_XPART_OFF);
+    ws.red.xnf = reinterpret_cast<int*>(base + NOALLOC_XNF_OFF);
+    ws.red.result = reinterpret_cast<double*>(base + NOALLOC_RESULT_OFF);
+    ws.red.maxBlkExact = RED_BLOCKS_EXACT;
+    ws.scal = reinterpret_cast<DevScalars*>(base + NOALLOC_SCAL_OFF);
+    vol_solver_performance pf;
+    solveDevice(A, b, x, ct, pf, &ws);
     if (perf) *perf = pf;
     API_END
 }
