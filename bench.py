@@ -148,11 +148,17 @@ def serve_leg(va, ctx, args):
     server = va.Server(0, slots=4, hbm_budget=200e9)
     SV.register_pool(server, pool)
     setup_s = time.perf_counter() - t0
+    t1 = time.perf_counter()
+    for e in pool:  # every worker's copy, matrix storage and batch of copies, before the stream
+        server.prefetch(e["id"])
+    prefetch_s = time.perf_counter() - t1
     out = {"workers": 4, "hbm_budget_gb": 200, "pool_cells": [e["n"] for e in pool], "setup_s": setup_s,
+           "prefetch_s": prefetch_s,
            "rate_per_s": args.serve_rate, "requests": args.serve_requests}
     warm = SV.request_stream(pool, rate=1e6, n_requests=3 * len(pool), seed=1)
     for prec in (va.FP64, va.MIXED):
-        SV.run_stream(server, pool, warm, prec)  # every worker/mesh pair resident, kernels warm
+        SV.run_stream(server, pool, warm, prec)  # kernels and captured iterations warm
+    out["bytes_in_use_gb"] = server.stats()["bytes_in_use"] / 1e9
     for name, prec in (("fp64", va.FP64), ("mixed", va.MIXED)):
         stream = SV.request_stream(pool, rate=args.serve_rate, n_requests=args.serve_requests, seed=9)
         res, st = SV.run_stream(server, pool, stream, prec)
@@ -389,6 +395,17 @@ def main():
     lm, A, b = W.pressure_system(ctx, m, seed=1, gamg=True, nonorth=True)
     levels = [L["n"] for L in lm.gamg_levels()]
     log(f"GPU setup (assembly + hierarchy) {time.perf_counter() - t0:.1f} s, levels {levels}")
+    agg_s, agg_passes, agg_rounds = lm.gamg_setup_info()
+    os.environ["VOLUMA_AGGLOMERATION"] = "host"  # the same hierarchy by the host loop, for comparison
+    lm_host = va.LduMesh(ctx, m.n_cells, m.owner[: m.n_internal], m.neighbour, m.agglomeration_weights())
+    del os.environ["VOLUMA_AGGLOMERATION"]
+    agglomeration = {"device_s": agg_s, "passes": agg_passes, "rounds": agg_rounds,
+                     "host_s": lm_host.gamg_setup_info()[0],
+                     "same_maps": all(np.array_equal(a["agg"], b["agg"]) and np.array_equal(a["faceMap"], b["faceMap"])
+                                      for a, b in zip(lm.gamg_levels(), lm_host.gamg_levels()))}
+    lm_host.close()
+    log(f"agglomeration: device {agg_s * 1e3:.1f} ms ({agg_passes} passes, {agg_rounds} rounds), host "
+        f"{agglomeration['host_s'] * 1e3:.1f} ms, same maps {agglomeration['same_maps']}")
     n, nf = m.n_cells, m.n_internal
     db = va.DeviceArray.from_host(ctx, b)
     dx = va.DeviceArray(ctx, n)
@@ -543,6 +560,7 @@ def main():
         "final_residual": p.finalResidual,
         "true_residual": true_res,
         "setup_ms": setup_ms,
+        "agglomeration": agglomeration,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": None if achieved is None else achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_sell<OP_AMUL> level-0 A·p", "algorithmic_bytes": amul_bytes,

@@ -154,6 +154,10 @@ int vol_mesh_gamg_levels(vol_mesh mesh, int* nLevels, int32_t* sizes);
 /* level k's agg (cell → next-level cell), faceMap and faceFlip (−1 / 0 on the coarsest) */
 int vol_mesh_gamg_level(vol_mesh mesh, int level, int32_t* nFaces, int32_t* agg, int32_t* faceMap,
                         int32_t* faceFlip);
+/* How the hierarchy was built (no reference counterpart; measurement aid): seconds taken,
+ * pairing passes and decision rounds run on the GPU (0 / 0 when built on the host:
+ * meshes below 32768 cells, or VOLUMA_AGGLOMERATION=host). Any output may be NULL. */
+int vol_mesh_gamg_setup_info(vol_mesh mesh, double* seconds, int* passes, int* rounds);
 
 /* Boundary faces of the mesh (after the internal ones, in patch order), by owner cell:
  * Mesh::owner()[nInternalFaces + bf]. Needed by the assembly entry points. */
@@ -446,6 +450,11 @@ int vol_batch_create(vol_context ctx, int nSystems, const int32_t* nCells, const
 int vol_batch_destroy(vol_batch batch);
 /* total cells and faces; levels[s] = GAMG levels of system s (nullable outputs) */
 int vol_batch_info(vol_batch batch, int64_t* nCellsTotal, int64_t* nFacesTotal, int32_t* levels);
+/* Batched iterations after the first replay one PCG iteration captured as a HIP graph
+ * (VOLUMA_BATCH_GRAPH=0 launches every kernel instead; the results are the same bits).
+ * Counts since the last reset: graph replays and eagerly launched iterations. No
+ * reference counterpart. */
+int vol_batch_graph_stats(vol_batch batch, int* graphLaunches, int* eagerIterations, int reset);
 int vol_batch_set(vol_batch batch, const double* diag, const double* upper, const double* lower, int location);
 int vol_batch_solve(vol_batch batch, const double* b, double* x, const vol_solver_controls* controls,
                     vol_solver_performance* perf, int location);
@@ -454,12 +463,17 @@ int vol_batch_solve(vol_batch batch, const double* b, double* x, const vol_solve
  * GAMG hierarchies and matrices resident and runs requests concurrently. `slots`
  * worker threads each own a context (HIP stream + scratch); a request is admitted, in
  * arrival order, when a worker is free and its device memory (a per-worker copy of the
- * mesh and a matrix, cached and evicted least recently used first) fits hbmBudgetBytes.
+ * mesh, a matrix and, for a mesh with a hierarchy, a batch of 4 copies; cached, and
+ * evicted least recently used first) fits hbmBudgetBytes. An entry is counted at an
+ * estimate until its first request ends, then at what its allocations measured.
  * A request names a mesh registered with vol_server_add_mesh and carries one matrix and
- * nRhs right-hand sides (e.g. PISO correctors) solved one after another on that matrix
- * with `controls`. All request buffers are host memory owned by the caller and must
- * stay valid until vol_server_wait returns for the ticket. A request's result does not
- * depend on what else runs. */
+ * nRhs right-hand sides (e.g. PISO correctors) on that matrix, solved with `controls`.
+ * Pressure requests (PCG + GAMG, damped-Jacobi V-cycle, fast sums) run as systems of the
+ * worker's batch of copies, together with the requests queued behind them for the same
+ * mesh and controls (at most 4 right-hand sides); other requests solve their
+ * right-hand sides one after another. All request buffers are host memory owned by the
+ * caller and must stay valid until vol_server_wait returns for the ticket. A request's
+ * result does not depend on what else runs. */
 typedef struct {
     int mesh;                        /* id from vol_server_add_mesh */
     const double* diag;              /* boundary-folded diagonal (nCells) */
@@ -476,6 +490,11 @@ int vol_server_create(int device, int slots, uint64_t hbmBudgetBytes, vol_server
 int vol_server_add_mesh(vol_server s, int nCells, int nInternalFaces, const int32_t* owner, const int32_t* neighbour,
                         const double* gamgWeights, const int32_t* boundaryOwner, int nBoundaryFaces, int* meshId);
 int vol_server_submit(vol_server s, const vol_request* request, int64_t* ticket);   /* returns at once */
+/* makes a registered mesh resident on every worker (copy, matrix storage, batch of
+ * copies), queued behind the requests submitted before it; blocks until done
+ * (VOL_ERR_TIMEOUT after timeoutSeconds). Lets a caller pay the first-use set-up
+ * before its latency-sensitive stream. */
+int vol_server_prefetch(vol_server s, int meshId, double timeoutSeconds);
 /* blocks until the request is done (VOL_ERR_TIMEOUT after timeoutSeconds); returns the
  * request's own status, also in *requestStatus (nullable); times (nullable, 3 doubles):
  * when it was queued, started and finished, in seconds on the server's steady clock */

@@ -174,3 +174,39 @@ def test_bench_batch_mixed_precision(ctx, bench_batch):
     p6 = B.solve(b, x6, _ctl(tol=1e-6, precision=va.MIXED, **BENCH_VCYCLE))
     assert all(p.converged for p in p6)
     print(f"[batch256] mixed 1e-10: max rel. error {max(errs):.2e}", flush=True)
+
+
+@pytest.mark.parametrize("prec", [va.FP64, va.MIXED])
+def test_batch_graph_replay_matches_eager(ctx, small_batch, prec):
+    """Batched iterations replayed from a captured HIP graph give the bits of eagerly
+    launched ones, over several solves in a row with eager work between the replays."""
+    import os
+    specs, B, b, meshes = small_batch
+    ctl = _ctl(tol=1e-9, precision=prec, **BENCH_VCYCLE)
+    old = os.environ.get("VOLUMA_BATCH_GRAPH")
+    try:
+        os.environ["VOLUMA_BATCH_GRAPH"] = "0"
+        B.graph_stats(reset=True)
+        xe = np.zeros(B.n_total)
+        pe = B.solve(b, xe, ctl)
+        g, e = B.graph_stats(reset=True)
+        assert g == 0 and e > 0
+        os.environ["VOLUMA_BATCH_GRAPH"] = "1"
+        for rep in range(3):
+            xg = np.zeros(B.n_total)
+            pg = B.solve(b, xg, ctl)
+            g, e = B.graph_stats(reset=True)
+            assert g > 0, "no graph replays"
+            assert np.array_equal(xg, xe), rep
+            assert [p.iterations for p in pg] == [p.iterations for p in pe]
+            assert [p.finalResidual for p in pg] == [p.finalResidual for p in pe]
+            # eager work between the replays
+            os.environ["VOLUMA_BATCH_GRAPH"] = "0"
+            x2 = np.zeros(B.n_total)
+            B.solve(b, x2, _ctl(tol=1e-6, precision=prec, **BENCH_VCYCLE))
+            os.environ["VOLUMA_BATCH_GRAPH"] = "1"
+    finally:
+        if old is None:
+            os.environ.pop("VOLUMA_BATCH_GRAPH", None)
+        else:
+            os.environ["VOLUMA_BATCH_GRAPH"] = old

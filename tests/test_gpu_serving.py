@@ -1,8 +1,9 @@
 """GPU tests of the solve server (vol_server_*): continuous admission of requests on
-concurrent streams, meshes and hierarchies resident, admission by an HBM budget. No
-reference counterpart; the bar is that each request's result is the same alone and in
-the mix (bit for bit: every request runs on its own worker buffers), and equal to a
-plain vol_solve of the same system.
+concurrent streams, queued pressure requests batched into shared launches, meshes and
+hierarchies resident, admission by an HBM budget. No reference counterpart; the bar is
+that each request's result is the same alone and in the mix (bit for bit: a system's
+arithmetic in a batch does not depend on its batch mates), and equal to a direct solve
+of the same system (a batch of one for pressure, vol_solve for momentum).
 """
 import numpy as np
 import pytest
@@ -25,9 +26,9 @@ def pool(ctx):
     return SV.build_pool(ctx, SV.pool_specs(n_meshes=3, seed=5, n_min=2000, n_max=30000))
 
 
-def _est(e):  # the server's admission estimate per resident mesh copy + matrix
+def _est(e):  # the server's admission estimate per resident mesh copy + matrix + batch of 4 copies
     m = e["mesh"]
-    return m.n_cells * 400 + m.n_internal * 96 + (1 << 20)
+    return (m.n_cells * 400 + m.n_internal * 96 + (1 << 20)) * 5
 
 
 def _alone(server, pool, stream, precision):
@@ -54,20 +55,89 @@ def test_results_same_alone_and_in_the_mix(pool, precision):
     server.close()
 
 
-def test_server_result_equals_plain_solve(ctx, pool):
+def test_server_results_equal_direct_solves(ctx, pool):
+    """Pressure requests run as systems of a batch of copies of their mesh: the same
+    bits as a batch of one system (vol_batch) of that matrix and right-hand side.
+    Momentum requests run with vol_solve: the same bits as a plain vol_solve."""
     server = va.Server(0, slots=2)
     SV.register_pool(server, pool)
+    e = pool[1]
+    m = e["mesh"]
     rq = {"t": 0.0, "kind": "pressure", "pool": 1, "nrhs": 2, "seed": 42}
     args = SV.request_args(pool, rq, va.FP64)
     xs, perf = server.wait(server.submit(**args))
-    e = pool[1]
-    m = e["mesh"]
-    lm = va.LduMesh(ctx, m.n_cells, m.owner[: m.n_internal], m.neighbour, m.agglomeration_weights())
-    A = va.LduMatrix(lm, e["p_diag"], e["p_upper"])
+    B1 = va.LduBatch(ctx, [(m.n_cells, m.owner[: m.n_internal], m.neighbour, m.agglomeration_weights())])
+    B1.set(e["p_diag"], e["p_upper"])
     for k, b in enumerate(args["bs"]):
         x = np.zeros(m.n_cells)
+        p = B1.solve(b, x, args["controls"])[0]
+        assert np.array_equal(x, xs[k]) and p.iterations == perf[k].iterations
+    B1.close()
+    rq = {"t": 0.0, "kind": "momentum", "pool": 1, "nrhs": 2, "seed": 43}
+    args = SV.request_args(pool, rq, va.FP64)
+    xs, perf = server.wait(server.submit(**args))
+    lm = va.LduMesh(ctx, m.n_cells, m.owner[: m.n_internal], m.neighbour, None)
+    A = va.LduMatrix(lm, e["m_diag"], e["m_upper"], e["m_lower"])
+    for k, b in enumerate(args["bs"]):
+        x = np.array(args["x0s"][k], dtype=np.float64)
         p = va.solve_ldu(A, b, x, args["controls"])
         assert np.array_equal(x, xs[k]) and p.iterations == perf[k].iterations
+    server.close()
+
+
+@pytest.mark.parametrize("precision", [va.FP64, va.MIXED])
+def test_queued_pressure_requests_share_a_batch(pool, precision):
+    """With one worker busy, pressure requests that queue behind each other on one mesh
+    are solved together (the same start and end), and each result is the bits it has
+    alone."""
+    server = va.Server(0, slots=1)
+    SV.register_pool(server, pool)
+    busy = {"t": 0.0, "kind": "momentum", "pool": 2, "nrhs": 3, "seed": 50}
+    reqs = [{"t": 0.0, "kind": "pressure", "pool": 0, "nrhs": 2, "seed": 60 + k} for k in range(2)]
+    alone = _alone(server, pool, reqs, precision)
+    t_busy = server.submit(**SV.request_args(pool, busy, precision))
+    tickets = [server.submit(**SV.request_args(pool, rq, precision)) for rq in reqs]
+    server.wait(t_busy)
+    got, times = [], []
+    for t in tickets:
+        got.append(server.wait(t))
+        times.append(dict(server.last_times))
+    assert abs(times[0]["started"] - times[1]["started"]) < 1e-3
+    assert abs(times[0]["finished"] - times[1]["finished"]) < 1e-3
+    for (xa, pa), (xg, pg) in zip(alone, got):
+        assert [p.iterations for p in pa] == [p.iterations for p in pg]
+        for a, b in zip(xa, xg):
+            assert np.array_equal(a, b)
+    server.close()
+
+
+def test_prefetch_makes_every_worker_resident(pool):
+    """prefetch builds each worker's copy, matrix storage and batch of copies of a mesh;
+    requests after it give the same bits as on a server that built them on first use,
+    and the entries are then counted at their measured device bytes, not the estimate."""
+    stream = SV.request_stream(pool, rate=1e6, n_requests=10, seed=6)
+    ref = va.Server(0, slots=4)
+    SV.register_pool(ref, pool)
+    expect = _alone(ref, pool, stream, va.FP64)
+    ref.close()
+    server = va.Server(0, slots=4, hbm_budget=200e9)
+    SV.register_pool(server, pool)
+    reg = server.stats()["bytes_in_use"]
+    for e in pool:
+        server.prefetch(e["id"])
+    st = server.stats()
+    assert st["cached"] == 4 * len(pool) and st["queued"] == 0
+    resident = st["bytes_in_use"] - reg
+    est = 4 * sum(_est(e) for e in pool)
+    print(f"[serve] resident {resident / 1e6:.1f} MB measured, {est / 1e6:.1f} MB estimated", flush=True)
+    assert 0 < resident <= est
+    got, _ = SV.run_stream(server, pool, stream, va.FP64, timeout=120.0)
+    for (xa, pa), (xg, pg) in zip(expect, got):
+        assert [p.iterations for p in pa] == [p.iterations for p in pg]
+        for a, b in zip(xa, xg):
+            assert np.array_equal(a, b)
+    with pytest.raises(va.VolumaError):
+        server.prefetch(len(pool))
     server.close()
 
 

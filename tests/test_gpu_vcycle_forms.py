@@ -1,0 +1,88 @@
+"""GPU tests of the damped-Jacobi V-cycle's equivalent launch forms. Each form is a
+context switch read from the environment when the context is created:
+
+* coarse levels take z = ω·(r·rD) from the restriction (VOLUMA_COARSE_Z);
+* FP64 PCG keeps the level-0 z with its residual (VOLUMA_LEVEL0_Z);
+* the level-0 post-sweep reads z + xc[agg] from a streaming prolongation pass
+  (VOLUMA_PROLONG_SPLIT=1);
+* the small levels run in one persistent grid launch with grid barriers
+  (VOLUMA_GRID_MAX_CELLS, VOLUMA_GRID_BLOCKS).
+
+Every form computes the same per-row arithmetic in the same order, so solutions,
+iteration counts and residuals must equal the plain per-level launches bit for bit, in
+fast and exact sums, FP64 and mixed precision, with 1/1 and 2/2 sweeps."""
+import os
+
+import numpy as np
+import pytest
+
+import voluma_amd as va
+from voluma_amd import mesh as M
+from voluma_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+PLAIN = {"VOLUMA_COARSE_Z": "0", "VOLUMA_LEVEL0_Z": "0", "VOLUMA_PROLONG_SPLIT": "0", "VOLUMA_GRID_MAX_CELLS": "0"}
+FORMS = {
+    "default": {},
+    "prolong_split": {"VOLUMA_PROLONG_SPLIT": "1"},
+    "grid": {"VOLUMA_GRID_MAX_CELLS": "40000"},
+    "grid_few_blocks": {"VOLUMA_GRID_MAX_CELLS": "40000", "VOLUMA_GRID_BLOCKS": "7"},
+    "grid_no_coarse_z": {"VOLUMA_GRID_MAX_CELLS": "40000", "VOLUMA_COARSE_Z": "0"},
+    "grid_from_level1": {"VOLUMA_GRID_MAX_CELLS": "100000000"},
+}
+KEYS = sorted(set(PLAIN) | {"VOLUMA_GRID_BLOCKS"})
+
+
+def _context(env):
+    old = {k: os.environ.get(k) for k in KEYS}
+    try:
+        for k in KEYS:
+            os.environ.pop(k, None)
+        os.environ.update(env)
+        return va.Context(0)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+def _controls(pre, prec, sums, tol):
+    jac = dict(gamgPreSweeps=pre, gamgPostSweeps=pre, gamgOmega=0.9 if pre == 1 else 0.8)
+    return va.SolverControls("PCG", "GAMG", tol, 0.0, 2000, sums=sums, precision=prec, **jac)
+
+
+CASES = [(1, va.FP64, va.SUMS_FAST, 1e-8), (2, va.FP64, va.SUMS_FAST, 1e-8), (1, va.FP64, va.SUMS_EXACT, 1e-7),
+         (1, va.MIXED, va.SUMS_FAST, 1e-8)]
+
+
+def _solves(env, m):
+    ctx = _context(env)
+    lm, A, b = W.pressure_system(ctx, m, seed=1, gamg=True, nonorth=False)
+    out = []
+    for pre, prec, sums, tol in CASES:
+        x = np.zeros(m.n_cells)
+        p = va.solve_ldu(A, b, x, _controls(pre, prec, sums, tol))
+        out.append((x, p))
+    A.close()
+    lm.close()
+    ctx.close()
+    return out
+
+
+@pytest.mark.parametrize("mesh", ["box64", "tet"])
+def test_vcycle_forms_bit_identical(mesh):
+    m = W.make_mesh("box64") if mesh == "box64" else M.tet(34, 30, 28, 0.2, 3)
+    print(f"[forms] {mesh}: {m.n_cells} cells, plain per-level launches", flush=True)
+    ref = _solves(PLAIN, m)
+    for (pre, prec, sums, tol), (x, p) in zip(CASES, ref):
+        assert p.converged and p.finalResidual < tol, (pre, prec, sums)
+    for name, env in FORMS.items():
+        print(f"[forms] {mesh}: {name}", flush=True)
+        got = _solves(env, m)
+        for case, (xr, pr), (xg, pg) in zip(CASES, ref, got):
+            assert pg.iterations == pr.iterations, (name, case)
+            assert pg.finalResidual == pr.finalResidual and pg.initialResidual == pr.initialResidual, (name, case)
+            assert np.array_equal(xg, xr), (name, case)

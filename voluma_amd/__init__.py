@@ -120,7 +120,7 @@ EXPORTED = ("vol_controls_default", "vol_last_error", "vol_context_create", "vol
             "vol_fvc_reconstruct", "vol_piso_correct_U", "vol_fvm_ddt",
             "vol_mesh_set_empty_dir", "vol_boundary_values", "vol_copy_strided", "vol_flow_options_default",
             "vol_flow_create", "vol_flow_set_fields", "vol_flow_get_fields", "vol_flow_step", "vol_flow_destroy",
-            "vol_solve_noalloc")
+            "vol_solve_noalloc", "vol_mesh_gamg_setup_info", "vol_batch_graph_stats", "vol_server_prefetch")
 
 
 def lib():
@@ -151,6 +151,8 @@ def lib():
         L.vol_amul.argtypes = [P, P, P, I]
         L.vol_solve.argtypes = [P, P, P, ctypes.POINTER(_Controls), ctypes.POINTER(_Perf), I]
         L.vol_solve_noalloc.argtypes = [P, P, P, ctypes.POINTER(_Controls), ctypes.POINTER(_Perf)]
+        L.vol_mesh_gamg_setup_info.argtypes = [P, ctypes.POINTER(D), ctypes.POINTER(I), ctypes.POINTER(I)]
+        L.vol_batch_graph_stats.argtypes = [P, ctypes.POINTER(I), ctypes.POINTER(I), I]
         L.vol_timing_enable.argtypes = [P, I]
         L.vol_timing_read.argtypes = [P, ctypes.POINTER(D), ctypes.POINTER(ctypes.c_int64), I]
         L.vol_mesh_set_boundary.argtypes = [P, I, P]
@@ -205,6 +207,7 @@ def lib():
         L.vol_server_add_mesh.argtypes = [P, I, I, P, P, P, P, I, ctypes.POINTER(I)]
         L.vol_server_submit.argtypes = [P, ctypes.POINTER(_Request), ctypes.POINTER(ctypes.c_int64)]
         L.vol_server_wait.argtypes = [P, ctypes.c_int64, D, ctypes.POINTER(I), P]
+        L.vol_server_prefetch.argtypes = [P, I, D]
         L.vol_server_stats.argtypes = [P, ctypes.POINTER(ctypes.c_uint64), ctypes.POINTER(I), ctypes.POINTER(I)]
         L.vol_server_destroy.argtypes = [P]
         for name in EXPORTED:
@@ -517,6 +520,13 @@ class LduMesh:
         _check(lib().vol_piso_correct_U(self._h, _p(h), _p(ra), _p(vv), int(sign), int(empty_dir), _p(out), VOL_HOST))
         return out
 
+    def gamg_setup_info(self):
+        """(seconds, pairing passes, decision rounds) of the hierarchy build; passes and
+        rounds are 0 when the hierarchy was built on the host."""
+        s, p, r = ctypes.c_double(), ctypes.c_int(), ctypes.c_int()
+        _check(lib().vol_mesh_gamg_setup_info(self._h, ctypes.byref(s), ctypes.byref(p), ctypes.byref(r)))
+        return s.value, p.value, r.value
+
     def gamg_levels(self):
         """[{n, agg, faceMap, faceFlip}] per level, as GamgHierarchy::levels."""
         nl = ctypes.c_int()
@@ -784,6 +794,12 @@ class LduBatch:
         self.face_off = np.concatenate([[0], np.cumsum(self.n_faces, dtype=np.int64)])
         self.n_total = int(self.cell_off[-1])
 
+    def graph_stats(self, reset=True):
+        """(graph replays, eager iterations) of the batched iterations since the last reset."""
+        g, e = ctypes.c_int(), ctypes.c_int()
+        _check(lib().vol_batch_graph_stats(self._h, ctypes.byref(g), ctypes.byref(e), int(bool(reset))))
+        return g.value, e.value
+
     def levels(self):
         lv = np.zeros(self.n_systems, np.int32)
         _check(lib().vol_batch_info(self._h, None, None, _p(lv)))
@@ -865,6 +881,11 @@ class Server:
         _check(lib().vol_server_submit(self._h, ctypes.byref(r), ctypes.byref(t)))
         self._live[t.value] = (d, u, lo, bs, xs, ctl, bp, xp, pf, r)  # alive until wait()
         return t.value
+
+    def prefetch(self, mesh_id, timeout=600.0):
+        """Make a registered mesh resident on every worker (copy, matrix storage, batch of
+        copies) so that its first requests pay no set-up; blocks until done."""
+        _check(lib().vol_server_prefetch(self._h, int(mesh_id), ctypes.c_double(timeout)))
 
     def wait(self, ticket, timeout=600.0):
         """(solutions, [SolverPerformance]) of a request; raises VolumaError if it failed."""

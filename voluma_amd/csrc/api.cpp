@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <atomic>
 #include <cstdint>
 #include <cstdlib>
@@ -97,6 +98,8 @@ struct vol_mesh_s {
     SellDev sell;  // level 0 / Amul
     bool hasGamg = false;
     std::vector<HLevel> H;  // host hierarchy (kept for inspection)
+    AgglomerationStats aggStats;  // pairing passes / decision rounds of the device build
+    double aggSeconds = 0.0;      // time to build H
     std::vector<std::unique_ptr<LevelDev>> lv;
     DevBuf<char> ownerIsL;  // level-0 orientation
     // boundary faces (for assembly): cell → boundary-face CSR
@@ -143,7 +146,8 @@ struct vol_mesh_s {
 
 struct MatLevel {
     DevBuf<double> diag, rD, Fu, Fl, coef, bufA, bufB, r, x;
-    DevBuf<float> diag32, rD32, coef32, bufA32, bufB32, r32, x32;  // mixed precision
+    DevBuf<double> z;  // ω·(r·rD) of the restricted residual: the first smoothing from zero
+    DevBuf<float> diag32, rD32, coef32, bufA32, bufB32, r32, x32, z32;  // mixed precision
     DevBuf<double> gsCF, gsCB;  // Gauss–Seidel smoother coefficients (LevelEll layouts)
     DevBuf<float> gsCF32, gsCB32;
 };
@@ -153,6 +157,7 @@ struct vol_matrix_s {
     bool symmetric = true;
     bool set = false;
     DevBuf<double> diag, upper, lower, coef, rD;
+    DevBuf<double> z0;  // level-0 first Jacobi sweep from zero, ω·(r·rD), written with r
     // GAMG coefficients per level, built lazily after each vol_matrix_set
     bool gamgReady = false;
     std::vector<std::unique_ptr<MatLevel>> L;
@@ -161,6 +166,10 @@ struct vol_matrix_s {
     int tailStart = -1;  // first GAMG level handled by the tail kernel (−1: none)
     int tailLdsBytes = 0, tailLdsBytes32 = 0;
     DevBuf<TailLevel> tail;
+    int gridStart = -1;  // first GAMG level of the persistent grid launch (−1: none)
+    DevBuf<GridLevel> grid;
+    DevBuf<GridLevel32> grid32;
+    DevBuf<GridBarrier> gridBar;
     DevBuf<int> status;
     // FP32 copies for mixed precision, built lazily after each vol_matrix_set
     bool mixedReady = false, mixedGamg = false;
@@ -200,7 +209,7 @@ namespace {
 void setDevice(vol_context ctx) { HIPCHECK(hipSetDevice(ctx->device)); }
 
 // ------------------------------------------------------------------ mesh setup
-void uploadLevels(vol_mesh m);
+void uploadLevels(vol_mesh m, const std::vector<int>* layout = nullptr);
 
 void setupMesh(vol_mesh m, const double* w, int mergeLevels) {
     hipStream_t st = m->ctx->stream;
@@ -227,12 +236,27 @@ void setupMesh(vol_mesh m, const double* w, int mergeLevels) {
     buildSell(n, off, col, src, m->sell, st, true);
     if (!w) return;
     m->hasGamg = true;
-    m->H = buildHierarchy(n, nf, m->own.data(), m->nei.data(), w, mergeLevels);
+    // agglomeration on the GPU for meshes of ≥ 32768 cells (below, the launches of the
+    // decision rounds cost more than the host loop); VOLUMA_AGGLOMERATION=host / device
+    // fixes the choice. Both give the same hierarchy.
+    const char* where = std::getenv("VOLUMA_AGGLOMERATION");
+    const std::string wh = where ? where : "";
+    const bool onHost = wh == "host" || (wh != "device" && n < 32768);
+    const auto t0 = std::chrono::steady_clock::now();
+    if (onHost) {
+        m->H = buildHierarchy(n, nf, m->own.data(), m->nei.data(), w, mergeLevels);
+        m->aggStats = AgglomerationStats{};
+    } else {
+        m->H = buildHierarchyDevice(st, n, nf, m->own.data(), m->nei.data(), w, mergeLevels, &m->aggStats);
+    }
+    m->aggSeconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     uploadLevels(m);
 }
 
 // Device structures of the GAMG levels of m->H (SELL/CSR rows, transfer lists).
-void uploadLevels(vol_mesh m) {
+// layout (nullable): the coarse layout of every level, as chosen for the mesh this one
+// copies (otherwise measured level by level).
+void uploadLevels(vol_mesh m, const std::vector<int>* layout) {
     hipStream_t st = m->ctx->stream;
     const int nf = m->nf;
     const int nL = int(m->H.size());
@@ -249,7 +273,7 @@ void uploadLevels(vol_mesh m) {
         if (k > 0) {  // SELL rows in adjacency (face) order: A(i, j) = upper if i is the l side
             std::vector<int> srcL(F.adjFace.size());
             for (size_t q = 0; q < srcL.size(); ++q) srcL[q] = (F.adjFace[q] << 1) | (F.adjIsL[q] ? 0 : 1);
-            buildCoarseLayout(F.n, F.off, F.adjCell, srcL, D->sell, st);
+            D->layout = buildCoarseLayout(F.n, F.off, F.adjCell, srcL, D->sell, st, layout ? (*layout)[k] : -1);
         }
         if (k + 1 < nL) {
             const HLevel& C = m->H[k + 1];
@@ -315,6 +339,46 @@ template <class T> int tailLayout(std::vector<TailLevelT<T>>& V, int coarseN) {
     return int(bytes);
 }
 
+// Descriptors of the grid launch's levels (gridStart … coarsest) in precision T.
+template <class T> std::vector<GridLevelT<T>> gridLevels(vol_matrix A) {
+    vol_mesh m = A->mesh;
+    const int nL = int(m->lv.size());
+    std::vector<GridLevelT<T>> G;
+    for (int k = A->gridStart; k < nL; ++k) {
+        const LevelDev& D = *m->lv[k];
+        MatLevel& M = *A->L[k];
+        GridLevelT<T> g;
+        g.S = D.sell.view();
+        if constexpr (std::is_same_v<T, double>) {
+            g.coef = M.coef.p;
+            g.diag = M.diag.p;
+            g.rD = M.rD.p;
+            g.r = M.r.p;
+            g.bufA = M.bufA.p;
+            g.bufB = M.bufB.p;
+            g.x = M.x.p;
+            g.z = M.z.p;
+        } else {
+            g.coef = M.coef32.p;
+            g.diag = M.diag32.p;
+            g.rD = M.rD32.p;
+            g.r = M.r32.p;
+            g.bufA = M.bufA32.p;
+            g.bufB = M.bufB32.p;
+            g.x = M.x32.p;
+            g.z = M.z32.p;
+        }
+        if (k + 1 < nL) {
+            g.agg = D.agg.p;
+            g.mOff = D.mOff.p;
+            g.mList = D.mList.p;
+            g.nNext = m->lv[k + 1]->n;
+        }
+        G.push_back(g);
+    }
+    return G;
+}
+
 void buildGamgCoeffs(vol_matrix A) {
     vol_mesh m = A->mesh;
     hipStream_t st = m->ctx->stream;
@@ -330,6 +394,7 @@ void buildGamgCoeffs(vol_matrix A) {
                 M->coef.alloc(D.sell.nslots);
                 M->r.alloc(D.n);
                 M->x.alloc(D.n);
+                M->z.alloc(D.n);
             }
             M->Fu.alloc(D.nf);
             M->Fl.alloc(D.nf);
@@ -395,6 +460,22 @@ void buildGamgCoeffs(vol_matrix A) {
         }
         A->tailLdsBytes = tailLayout(T, DL.n);
         A->tail.upload(T, st);
+    }
+    // without the tail: the levels from the first of ≤ gridMax cells (below level 0, above
+    // the coarsest) down to the coarsest in one persistent grid launch
+    A->gridStart = -1;
+    if (A->tailStart < 0 && m->ctx->gridMax > 0)
+        for (int k = 1; k + 1 < nL; ++k)
+            if (m->lv[k]->n <= m->ctx->gridMax) {
+                A->gridStart = k;
+                break;
+            }
+    if (A->gridStart >= 0) {
+        A->grid.upload(gridLevels<double>(A), st);
+        if (!A->gridBar.p) {
+            A->gridBar.alloc(1);
+            HIPCHECK(hipMemsetAsync(A->gridBar.p, 0, sizeof(GridBarrier), st));
+        }
     }
     int status = 0;
     HIPCHECK(hipMemcpyAsync(&status, A->status.p, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -758,6 +839,7 @@ void buildMixed(vol_matrix A, bool gamg) {
                 M.coef32.alloc(D.sell.nslots);
                 M.r32.alloc(D.n);
                 M.x32.alloc(D.n);
+                M.z32.alloc(D.n);
                 toFloat(st, M.diag.p, M.diag32.p, D.n);
                 toFloat(st, M.rD.p, M.rD32.p, D.n);
                 toFloat(st, M.coef.p, M.coef32.p, D.sell.nslots);
@@ -787,6 +869,7 @@ void buildMixed(vol_matrix A, bool gamg) {
             A->tailLdsBytes32 = tailLayout(T, DL.n);
             A->tail32.upload(T, st);
         }
+        if (A->gridStart >= 0) A->grid32.upload(gridLevels<float>(A), st);
     }
     HIPCHECK(hipStreamSynchronize(st));
     A->mixedReady = true;
@@ -1034,6 +1117,9 @@ struct Ctx {
     DevScalars* S;
     const RedScratch* red = nullptr;  // reduction scratch: the context's unless set
     bool noTiming = false;            // no event recording (no event objects created)
+    // set by runLoop: the host-visible slot the iteration's last finalize copies the
+    // scalars into (so that no separate copy is needed to watch the loop)
+    DevScalars* hostOut = nullptr;
 };
 
 const RedScratch& redOf(const Ctx& C) { return C.red ? *C.red : C.c->red; }
@@ -1065,8 +1151,10 @@ void sell(const Ctx& C, int op, int xl, const SellView& V, const T* coef, const 
 
 template <class T>
 void vec(const Ctx& C, int op, int n, bool exact, const T* a, const T* b = nullptr, const T* c = nullptr,
-         const T* d = nullptr, T* w = nullptr, T* x = nullptr, T* y = nullptr, const T* rD = nullptr) {
+         const T* d = nullptr, T* w = nullptr, T* x = nullptr, T* y = nullptr, const T* rD = nullptr,
+         double omega = 1.0) {
     VecArgsT<T> v;
+    v.omega = omega;
     v.n = n;
     v.a = a;
     v.b = b;
@@ -1084,8 +1172,8 @@ void vec(const Ctx& C, int op, int n, bool exact, const T* a, const T* b = nullp
         launch_vec32(C.st, op, v, redOf(C));
 }
 
-void fin(const Ctx& C, int op, int nred, bool exact, int nparts) {
-    launch_finalize(C.st, op, nred, exact, nparts, redOf(C), C.S);
+void fin(const Ctx& C, int op, int nred, bool exact, int nparts, DevScalars* hostOut = nullptr) {
+    launch_finalize(C.st, op, nred, exact, nparts, redOf(C), C.S, hostOut);
 }
 
 // Level-0 matrix in the solve's working precision.
@@ -1143,17 +1231,17 @@ void collectTiming(vol_context c) {
 // Per-level data of the V-cycle in precision T.
 template <class T> struct LevelPtrs {
     const T *coef, *diag, *rD;
-    T *bufA, *bufB, *r, *x;
+    T *bufA, *bufB, *r, *x, *z;
 };
 LevelPtrs<double> levelPtrs(vol_matrix A, int l, double*) {
     MatLevel& M = *A->L[l];
-    if (l == 0) return {A->coef.p, A->diag.p, A->rD.p, M.bufA.p, M.bufB.p, nullptr, nullptr};
-    return {M.coef.p, M.diag.p, M.rD.p, M.bufA.p, M.bufB.p, M.r.p, M.x.p};
+    if (l == 0) return {A->coef.p, A->diag.p, A->rD.p, M.bufA.p, M.bufB.p, nullptr, nullptr, nullptr};
+    return {M.coef.p, M.diag.p, M.rD.p, M.bufA.p, M.bufB.p, M.r.p, M.x.p, M.z.p};
 }
 LevelPtrs<float> levelPtrs(vol_matrix A, int l, float*) {
     MatLevel& M = *A->L[l];
-    if (l == 0) return {A->coef32.p, A->diag32.p, A->rD32.p, M.bufA32.p, M.bufB32.p, nullptr, nullptr};
-    return {M.coef32.p, M.diag32.p, M.rD32.p, M.bufA32.p, M.bufB32.p, M.r32.p, M.x32.p};
+    if (l == 0) return {A->coef32.p, A->diag32.p, A->rD32.p, M.bufA32.p, M.bufB32.p, nullptr, nullptr, nullptr};
+    return {M.coef32.p, M.diag32.p, M.rD32.p, M.bufA32.p, M.bufB32.p, M.r32.p, M.x32.p, M.z32.p};
 }
 void tailLaunch(const Ctx& C, vol_matrix A, const Plan& P, int l, const double* r, double* xout) {
     TailArgs t;
@@ -1188,17 +1276,64 @@ void tailLaunch(const Ctx& C, vol_matrix A, const Plan& P, int l, const float* r
     t.done = C.done;
     launch_vcycle_tail32(C.st, t);
 }
+// After a solve (stream synchronised): a grid launch whose barrier wait gave up has set
+// the fault flag; its results are void. The barrier is reset for the next launch.
+void checkGridBarrier(vol_matrix A, hipStream_t st) {
+    if (!A->gridBar.p) return;
+    GridBarrier b{};
+    HIPCHECK(hipMemcpyAsync(&b, A->gridBar.p, sizeof(b), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (b.fault) {
+        HIPCHECK(hipMemsetAsync(A->gridBar.p, 0, sizeof(GridBarrier), st));
+        HIPCHECK(hipStreamSynchronize(st));
+        throw VolError(VOL_ERR_HIP, "GAMG grid launch: a barrier wait timed out (workgroups not co-resident)");
+    }
+}
+
+template <class T> void gridLaunch(const Ctx& C, vol_matrix A, const Plan& P, int l, T*) {
+    GridArgsT<T> g;
+    g.nLevels = int(A->mesh->lv.size()) - l;
+    if constexpr (std::is_same_v<T, double>)
+        g.L = A->grid.p;
+    else
+        g.L = A->grid32.p;
+    g.nCoarse = A->nCoarse;
+    g.lu = A->lu.p;
+    g.luT = A->luT.p;
+    g.nPre = P.nPre;
+    g.nPost = P.nPost;
+    g.omega = P.omega;
+    g.useZ = P.nPre == 1 && A->mesh->ctx->coarseZ ? 1 : 0;
+    g.done = C.done;
+    g.bar = A->gridBar.p;
+    if constexpr (std::is_same_v<T, double>)
+        launch_vcycle_grid(C.st, A->mesh->ctx->gridBlocks, g);
+    else
+        launch_vcycle_grid32(C.st, A->mesh->ctx->gridBlocks, g);
+}
+
 void coarseSolve(const Ctx& C, vol_matrix A, const double* r, double* x) {
     launch_coarse_solve(C.st, A->nCoarse, A->lu.p, A->luT.p, r, x, C.done);
 }
 void coarseSolve(const Ctx& C, vol_matrix A, const float* r, float* x) {
     launch_coarse_solve32(C.st, A->nCoarse, A->lu.p, A->luT.p, r, x, C.done);
 }
-void restrictLevel(const Ctx& C, int nc, const LevelDev& D, const double* res, double* rc) {
-    launch_restrict(C.st, nc, D.mOff.p, D.mList.p, res, rc, C.done);
+// rc = restriction of res; with zc: also zc = ω·(rc·rDc), the first Jacobi sweep from
+// zero of the coarse level, so that its sweeps gather one vector instead of two
+void restrictLevel(const Ctx& C, int nc, const LevelDev& D, const double* res, double* rc, double* zc = nullptr,
+                   const double* rDc = nullptr, double omega = 1.0) {
+    launch_restrict(C.st, nc, D.mOff.p, D.mList.p, res, rc, C.done, zc, rDc, omega);
 }
-void restrictLevel(const Ctx& C, int nc, const LevelDev& D, const float* res, float* rc) {
-    launch_restrict32(C.st, nc, D.mOff.p, D.mList.p, res, rc, C.done);
+void restrictLevel(const Ctx& C, int nc, const LevelDev& D, const float* res, float* rc, float* zc = nullptr,
+                   const float* rDc = nullptr, double omega = 1.0) {
+    launch_restrict32(C.st, nc, D.mOff.p, D.mList.p, res, rc, C.done, zc, rDc, omega);
+}
+
+// Does level l of the Jacobi V-cycle get its z = ω·(r·rD) from the restriction? (a
+// level smoothed by grid launches, below level 0, with one pre-sweep)
+bool levelUsesZ(vol_matrix A, const Plan& P, int l) {
+    const int nL = int(A->mesh->lv.size());
+    return P.nPre == 1 && l >= 1 && l < nL - 1 && l != A->tailStart && A->mesh->ctx->coarseZ;
 }
 
 // One V-cycle at level l: r → x (GAMG.cpp:403-443 structure, damped-Jacobi smoothing).
@@ -1206,11 +1341,15 @@ void restrictLevel(const Ctx& C, int nc, const LevelDev& D, const float* res, fl
 // per-block partials of Σ x·dotv in part.
 template <class T>
 void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xout, const T* dotv = nullptr,
-            double* part = nullptr) {
+            double* part = nullptr, T* z0 = nullptr) {
     vol_mesh m = A->mesh;
     const int nL = int(m->lv.size());
     if (l == A->tailStart) {
         tailLaunch(C, A, P, l, r, xout);
+        return;
+    }
+    if (l == A->gridStart) {  // r and xout are the level's own r and x buffers
+        gridLaunch(C, A, P, l, (T*)nullptr);
         return;
     }
     if (l == nL - 1) {
@@ -1220,6 +1359,8 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
     const LevelDev& D = *m->lv[l];
     const SellView V = l == 0 ? m->sell.view() : D.sell.view();
     const LevelPtrs<T> L = levelPtrs(A, l, (T*)nullptr);
+    const bool useZ = levelUsesZ(A, P, l);  // then L.z holds the first sweep from zero
+    T* const z = useZ ? L.z : (l == 0 && P.nPre == 1 ? z0 : nullptr);  // given: ω·(r·rD)
     T* bufs[2] = {L.bufA, L.bufB};
     int cur = 0;  // index of the buffer holding the pre-smoothed x (nPre ≥ 2)
     for (int s = 1; s < P.nPre; ++s) {
@@ -1230,10 +1371,17 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
     }
     const int freeB = 1 - cur;
     // residual of the pre-smoothed x, then restriction
-    sell<T>(C, OP_RESID, P.nPre == 1 ? XL_FROMZERO : XL_PLAIN, V, L.coef, L.diag, P.nPre == 1 ? nullptr : bufs[cur],
-            r, bufs[freeB], P.omega, nullptr, nullptr, nullptr, nullptr, nullptr, L.rD);
+    if (z)
+        sell<T>(C, OP_RESID, XL_PLAIN, V, L.coef, L.diag, z, r, bufs[freeB], P.omega);
+    else
+        sell<T>(C, OP_RESID, P.nPre == 1 ? XL_FROMZERO : XL_PLAIN, V, L.coef, L.diag,
+                P.nPre == 1 ? nullptr : bufs[cur], r, bufs[freeB], P.omega, nullptr, nullptr, nullptr, nullptr, nullptr,
+                L.rD);
     const LevelPtrs<T> LC = levelPtrs(A, l + 1, (T*)nullptr);
-    restrictLevel(C, m->lv[l + 1]->n, D, bufs[freeB], LC.r);
+    if (levelUsesZ(A, P, l + 1))
+        restrictLevel(C, m->lv[l + 1]->n, D, bufs[freeB], LC.r, LC.z, LC.rD, P.omega);
+    else
+        restrictLevel(C, m->lv[l + 1]->n, D, bufs[freeB], LC.r);
     vcycle<T>(C, A, P, l + 1, LC.r, LC.x);
     // prolongation fused into the first post-sweep
     int src = cur, other = freeB;
@@ -1242,7 +1390,13 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
         T* dst = last ? xout : bufs[other];
         const T* dv = last ? dotv : nullptr;
         double* pt = last ? part : nullptr;
-        if (s == 0)
+        if (s == 0 && z && l == 0 && m->ctx->prolongSplit) {
+            // z += xc[agg] streamed first, so that the sweep gathers one vector
+            launch_prolong<T>(C.st, m->n, D.agg.p, LC.x, z, C.done);
+            sell<T>(C, OP_JACOBI, XL_PLAIN, V, L.coef, L.diag, z, r, dst, P.omega, nullptr, nullptr, dv, pt);
+        } else if (s == 0 && z)
+            sell<T>(C, OP_JACOBI, XL_PROLONG, V, L.coef, L.diag, z, r, dst, P.omega, LC.x, D.agg.p, dv, pt);
+        else if (s == 0)
             sell<T>(C, OP_JACOBI, P.nPre == 1 ? XL_ZEROPROLONG : XL_PROLONG, V, L.coef, L.diag,
                     P.nPre == 1 ? nullptr : bufs[src], r, dst, P.omega, LC.x, D.agg.p, dv, pt, nullptr, L.rD);
         else
@@ -1349,6 +1503,7 @@ bool hostConverged(const vol_solver_controls& ct, const DevScalars& S) {
 template <class T> struct Krylov {
     T *x, *r, *w, *p, *q;            // PCG (q = A·p)
     T *r0, *y, *Ay, *s, *z, *t;      // PBiCGStab
+    T* z0 = nullptr;                 // ω·(r·rD) kept with r (PCG + Jacobi GAMG, one pre-sweep)
 };
 
 // One PCG or PBiCGStab iteration, every kernel predicated on C.done.
@@ -1361,7 +1516,7 @@ void krylovIteration(const Ctx& C, vol_matrix A, const Plan& P, const Krylov<T>&
     const Level0<T> L0 = level0(A, (T*)nullptr);
     if (isPCG) {  // LinearSolver.cpp:176-201
         if (P.precond == "GAMG" && !P.gamgGS && fuse && A->tailStart != 0) {  // Σ w·r fused into the last sweep
-            vcycle<T>(C, A, P, 0, k.r, k.w, k.r, redOf(C).part);
+            vcycle<T>(C, A, P, 0, k.r, k.w, k.r, redOf(C).part, k.z0);
             fin(C, FIN_PCG_WRA, 1, false, sellParts);
         } else {
             if (P.precond == "GAMG" || P.precond == "DIC" || P.precond == "DILU") {
@@ -1381,8 +1536,8 @@ void krylovIteration(const Ctx& C, vol_matrix A, const Plan& P, const Krylov<T>&
             vec<T>(C, V_DOT, n, ex, k.q, k.p);
             fin(C, FIN_PCG_WAPA, 1, ex, nparts);
         }
-        vec<T>(C, V_PCG_XR, n, ex, k.p, k.q, nullptr, nullptr, nullptr, k.x, k.r);
-        fin(C, FIN_PCG_NORM, 1, ex, nparts);
+        vec<T>(C, V_PCG_XR, n, ex, k.p, k.q, nullptr, nullptr, k.z0, k.x, k.r, k.z0 ? L0.rD : nullptr, P.omega);
+        fin(C, FIN_PCG_NORM, 1, ex, nparts, C.hostOut);
     } else {  // PBiCGStab, LinearSolver.cpp:202-237
         vec<T>(C, V_DOT, n, ex, k.r0, k.r);
         fin(C, FIN_BI_RHO, 1, ex, nparts);
@@ -1404,28 +1559,31 @@ void krylovIteration(const Ctx& C, vol_matrix A, const Plan& P, const Krylov<T>&
         vec<T>(C, V_BI_DOT2, n, ex, k.t, k.s);
         fin(C, FIN_BI_OMEGA, 2, ex, nparts);
         vec<T>(C, V_BI_XR, n, ex, k.y, k.z, k.s, k.t, nullptr, k.x, k.r);
-        fin(C, FIN_BI_NORM, 1, ex, nparts);
+        fin(C, FIN_BI_NORM, 1, ex, nparts, C.hostOut);
     }
 }
 
 // Runs `iterate` until C.S->done, enqueuing one iteration ahead of the host: iteration
 // k+1 is queued before the host reads the scalars after iteration k, and runs as
-// no-ops if k finished.
-template <class F> void runLoop(vol_context c, const Ctx& C, int maxLoops, F&& iterate) {
+// no-ops if k finished. The iteration's last finalize writes the scalars into the
+// pinned slot C.hostOut (two slots, alternating), so watching the loop costs an event
+// per iteration and no copy.
+template <class F> void runLoop(vol_context c, Ctx& C, int maxLoops, F&& iterate) {
     hipStream_t st = C.st;
     hipEvent_t* evs = c->loopEv;
     DevScalars* hsl = c->hostScal;
+    C.hostOut = &hsl[0];
     iterate();
-    HIPCHECK(hipMemcpyAsync(&hsl[0], C.S, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipEventRecord(evs[0], st));
     for (int k = 1; k <= maxLoops; ++k) {
+        C.hostOut = &hsl[k & 1];
         iterate();
-        HIPCHECK(hipMemcpyAsync(&hsl[k & 1], C.S, sizeof(DevScalars), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipEventRecord(evs[k & 1], st));
         HIPCHECK(hipEventSynchronize(evs[(k - 1) & 1]));
         if (hsl[(k - 1) & 1].done) break;
     }
     HIPCHECK(hipStreamSynchronize(st));
+    C.hostOut = nullptr;
 }
 
 // Work buffers of one FP64 solve given by the caller instead of the matrix's and the
@@ -1521,7 +1679,12 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
     fin(C, FIN_XREF, 1, ex, nparts);
     sell<double>(C, OP_AMUL, XL_UNIFORM, V, A->coef.p, A->diag.p, nullptr, nullptr, pA, 1.0, nullptr, nullptr,
                  nullptr, nullptr, &C.S->xRef);
-    vec<double>(C, V_INIT, n, ex, wA, b, pA, nullptr, vr);
+    // FP64 PCG with the fused Jacobi V-cycle keeps z0 = ω·(r·rD) with r (prologue and
+    // every x/r update), the level-0 first sweep from zero
+    const bool useZ0 = isPCG && !mixed && P.precond == "GAMG" && !P.gamgGS && !ex && P.nPre == 1 && c->level0Z && !ws;
+    if (useZ0) A->z0.alloc(n);
+    vec<double>(C, V_INIT, n, ex, wA, b, pA, nullptr, vr, nullptr, useZ0 ? A->z0.p : nullptr,
+                useZ0 ? A->rD.p : nullptr, P.omega);
     fin(C, FIN_INIT, 2, ex, nparts);
     DevScalars hs;
     readScalars(c, C.S, hs);
@@ -1568,6 +1731,7 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
             }
             Krylov<double> k{x, vr, ws ? ws->vw : A->vw.p, ws ? ws->vp : A->vp.p, wA, nullptr, nullptr, nullptr,
                              nullptr, nullptr, nullptr};
+            if (useZ0 && isPCG) k.z0 = A->z0.p;
             if (isBi) {
                 if (ws) {
                     k.r0 = ws->r0;
@@ -1608,12 +1772,13 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
                 }
                 sell<double>(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, x, nullptr, A->wA.p);
                 vec<double>(C, V_BMINUSAX_NORM, n, ex, A->wA.p, b, nullptr, nullptr, A->vr.p);
-                fin(C, FIN_JAC_NORM, 1, ex, nparts);
+                fin(C, FIN_JAC_NORM, 1, ex, nparts, C.hostOut);
             });
         }
         readScalars(c, C.S, hs);
         if (!hs.done) throw VolError(VOL_ERR_HIP, "solver loop did not terminate");
     }
+    checkGridBarrier(A, C.st);
     collectTiming(c);
     perf.initialResidual = hs.initialResidual;
     perf.finalResidual = hs.finalResidual;
@@ -1642,7 +1807,8 @@ vol_mesh meshCloneTo(vol_mesh src, vol_context dst) {
     if (src->hasGamg) {
         m->hasGamg = true;
         m->H = src->H;
-        uploadLevels(m.get());
+        const std::vector<int> layout = meshView(src).layout;
+        uploadLevels(m.get(), &layout);
     }
     if (src->nb >= 0) {
         std::vector<int> off(src->n + 1), list(src->nb);
@@ -1667,6 +1833,17 @@ void ctxRelease(vol_context c) {
     if (c->hostScal) (void)hipHostFree(c->hostScal);
     (void)hipStreamDestroy(c->stream);
     delete c;
+}
+
+MeshView meshView(vol_mesh m) {
+    MeshView v;
+    v.n = m->n;
+    v.nf = m->nf;
+    v.own = m->own.data();
+    v.nei = m->nei.data();
+    v.H = m->hasGamg ? &m->H : nullptr;
+    for (const auto& D : m->lv) v.layout.push_back(D->layout);
+    return v;
 }
 
 void meshRelease(vol_mesh m) {
@@ -1792,6 +1969,25 @@ int vol_context_create(int device, vol_context* out) {
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     if (const char* e = std::getenv("CFD_REPRODUCIBLE")) c->reproducible = std::string(e) != "0" && std::string(e) != "";
     if (const char* e = std::getenv("VOLUMA_TAIL_MAX_CELLS")) c->tailMax = std::min(std::atoi(e), TAIL_MAX_CELLS);
+    if (const char* e = std::getenv("VOLUMA_GRID_MAX_CELLS")) c->gridMax = std::max(0, std::atoi(e));
+    auto envOff = [](const char* name) {
+        const char* e = std::getenv(name);
+        return e && std::string(e) == "0";
+    };
+    c->coarseZ = !envOff("VOLUMA_COARSE_Z");
+    c->level0Z = !envOff("VOLUMA_LEVEL0_Z");
+    if (const char* e = std::getenv("VOLUMA_PROLONG_SPLIT")) c->prolongSplit = std::string(e) == "1";
+    {
+        // all workgroups of a grid launch must be resident together, with room for the
+        // grid launches of other contexts on the same GPU (at most a quarter each)
+        int cus = 0;
+        HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+        const int per = std::min(vcycle_grid_blocks_per_cu(false), vcycle_grid_blocks_per_cu(true));
+        c->gridBlocks = std::min(128, std::max(1, cus * per / 4));
+        if (const char* e = std::getenv("VOLUMA_GRID_BLOCKS"))
+            c->gridBlocks = std::min(c->gridBlocks, std::max(1, std::atoi(e)));
+        if (per <= 0) c->gridMax = 0;
+    }
     c->scal.alloc(2);
     HIPCHECK(hipHostMalloc((void**)&c->hostScal, 2 * sizeof(DevScalars), hipHostMallocDefault));
     c->ensureScratch(RED_BLOCKS);
@@ -1923,6 +2119,15 @@ int vol_mesh_gamg_levels(vol_mesh mesh, int* nLevels, int32_t* sizes) {
     *nLevels = int(mesh->H.size());
     if (sizes)
         for (size_t k = 0; k < mesh->H.size(); ++k) sizes[k] = mesh->H[k].n;
+    API_END
+}
+
+int vol_mesh_gamg_setup_info(vol_mesh mesh, double* seconds, int* passes, int* rounds) {
+    API_BEGIN
+    if (!mesh) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null mesh");
+    if (seconds) *seconds = mesh->aggSeconds;
+    if (passes) *passes = mesh->aggStats.passes;
+    if (rounds) *rounds = mesh->aggStats.rounds;
     API_END
 }
 

@@ -59,15 +59,30 @@ struct vol_batch_s {
     DevBuf<double> diag, upper, lower;
     DevBuf<double> vr, vw, vp, vq, wA, pA, hb, hx;
     DevBuf<float> ir, iw, ip, iq, ie;
+    std::vector<int> layoutHint;  // coarse layout per level (0 CSR, 1 SELL) when known
+    // one PCG iteration captured as a HIP graph per key (plan, precision, buffers),
+    // replayed every iteration; at most MAX_GRAPHS kept, the oldest dropped first
+    static constexpr size_t MAX_GRAPHS = 4;
+    std::vector<std::pair<std::string, hipGraphExec_t>> graphs;
+    hipGraphExec_t graph = nullptr;  // the one of the running solve
+    int graphLaunches = 0, eagerIterations = 0;  // statistics of the last solve
+    ~vol_batch_s() {
+        for (auto& g : graphs) (void)hipGraphExecDestroy(g.second);
+    }
 };
 
 namespace {
 
 // ------------------------------------------------------------------ setup
-void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const double* w) {
+// pre (nullable): per system a hierarchy built already (the same as buildHierarchy of
+// its addressing and weights), used instead of building one
+void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const double* w,
+                const std::vector<const std::vector<HLevel>*>* pre = nullptr) {
     hipStream_t st = B->ctx->stream;
     const int nS = B->nSys;
-    std::vector<std::vector<HLevel>> H(nS);
+    std::vector<std::vector<HLevel>> built(pre ? 0 : nS);
+    std::vector<const std::vector<HLevel>*> Hp(nS);
+    auto H = [&](int s) -> const std::vector<HLevel>& { return *Hp[s]; };
     B->cellOff.assign(nS + 1, 0);
     B->faceOff.assign(nS + 1, 0);
     for (int s = 0; s < nS; ++s) {
@@ -83,12 +98,17 @@ void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const doubl
             if (o < 0 || o >= B->n[s] || q < 0 || q >= B->n[s] || o == q)
                 throw VolError(VOL_ERR_INVALID_ARGUMENT, "system " + std::to_string(s) + ": bad owner/neighbour");
         }
-        H[s] = buildHierarchy(B->n[s], B->nf[s], own + fo, nei + fo, w + fo, 2);
-        B->nLev[s] = int(H[s].size());
+        if (pre) {
+            Hp[s] = (*pre)[s];
+        } else {
+            built[s] = buildHierarchy(B->n[s], B->nf[s], own + fo, nei + fo, w + fo, 2);
+            Hp[s] = &built[s];
+        }
+        B->nLev[s] = int(H(s).size());
         if (B->nLev[s] < 2)
             throw VolError(VOL_ERR_INVALID_ARGUMENT,
                            "system " + std::to_string(s) + ": a batched system needs more than 64 cells");
-        if (H[s].back().n > 4096)
+        if (H(s).back().n > 4096)
             throw VolError(VOL_ERR_UNSUPPORTED, "system " + std::to_string(s) + ": coarsest level above 4096 cells");
         B->nLevMax = std::max(B->nLevMax, B->nLev[s]);
     }
@@ -106,8 +126,8 @@ void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const doubl
                 order[l].push_back(s);
                 rowOff[l][s] = Lv->nRows;
                 faceOffL[l][s] = Lv->nFaces;
-                Lv->nRows += H[s][l].n;
-                Lv->nFaces += int(H[s][l].l.size());
+                Lv->nRows += H(s)[l].n;
+                Lv->nFaces += int(H(s)[l].l.size());
                 if (smooth) {
                     Lv->nSmooth = Lv->nRows;
                     Lv->nFacesSmooth = Lv->nFaces;
@@ -122,7 +142,7 @@ void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const doubl
         BLevel& Lv = *B->L[l];
         std::vector<int> gl(Lv.nFaces), gu(Lv.nFaces);
         for (int s : order[l]) {
-            const HLevel& F = H[s][l];
+            const HLevel& F = H(s)[l];
             for (size_t f = 0; f < F.l.size(); ++f) {
                 gl[faceOffL[l][s] + f] = F.l[f] + rowOff[l][s];
                 gu[faceOffL[l][s] + f] = F.u[f] + rowOff[l][s];
@@ -162,7 +182,7 @@ void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const doubl
         } else if (Lv.nSmooth > 0) {
             for (int s : order[l]) {
                 if (B->nLev[s] - 1 <= l) continue;
-                const HLevel& F = H[s][l];
+                const HLevel& F = H(s)[l];
                 for (int i = 0; i < F.n; ++i) off[rowOff[l][s] + i + 1] = F.off[i + 1] - F.off[i];
             }
             for (int c = 0; c < Lv.nSmooth; ++c) off[c + 1] += off[c];
@@ -170,7 +190,7 @@ void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const doubl
             src.resize(off[Lv.nSmooth]);
             for (int s : order[l]) {
                 if (B->nLev[s] - 1 <= l) continue;
-                const HLevel& F = H[s][l];
+                const HLevel& F = H(s)[l];
                 for (int i = 0; i < F.n; ++i) {
                     int k = off[rowOff[l][s] + i];
                     for (int q = F.off[i]; q < F.off[i + 1]; ++q, ++k) {
@@ -179,14 +199,15 @@ void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const doubl
                     }
                 }
             }
-            buildCoarseLayout(Lv.nSmooth, off, col, src, Lv.sell, st);
+            buildCoarseLayout(Lv.nSmooth, off, col, src, Lv.sell, st,
+                              l < int(B->layoutHint.size()) ? B->layoutHint[l] : -1);
         }
         // system of every smoothing row, for skipping finished systems
         if (Lv.nSmooth > 0) {
             std::vector<int> sysOf(Lv.nSmooth);
             for (int s : order[l]) {
                 if (B->nLev[s] - 1 <= l) continue;
-                for (int i = 0; i < H[s][l].n; ++i) sysOf[rowOff[l][s] + i] = s;
+                for (int i = 0; i < H(s)[l].n; ++i) sysOf[rowOff[l][s] + i] = s;
             }
             if (Lv.sell.csrG > 0) {
                 Lv.rowSys.upload(sysOf, st);
@@ -206,7 +227,7 @@ void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const doubl
             std::vector<char> flip(Lv.nFaces, 0);
             for (int s : order[l]) {
                 if (B->nLev[s] - 1 <= l) continue;
-                const HLevel& F = H[s][l];
+                const HLevel& F = H(s)[l];
                 for (int i = 0; i < F.n; ++i) agg[rowOff[l][s] + i] = F.agg[i] + rowOff[l + 1][s];
                 for (size_t f = 0; f < F.faceMap.size(); ++f) {
                     const int cf = F.faceMap[f];
@@ -240,8 +261,8 @@ void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const doubl
         const int c = B->nLev[s] - 1;
         BLevel& Lv = *B->L[c];
         CoarseDesc& d = B->cdesc[s];
-        d.n = H[s][c].n;
-        d.nf = int(H[s][c].l.size());
+        d.n = H(s)[c].n;
+        d.nf = int(H(s)[c].l.size());
         d.rowBase = rowOff[c][s];
         d.level = c;
         d.luOff = luSize;
@@ -524,18 +545,73 @@ void bpcgIteration(vol_batch B, const BPlan& P, const BKrylov<T>& k, DevScalars*
     launch_bcount(st, S, B->nSys, B->nDone.p);
 }
 
-// Runs `iterate` until every system is done, one iteration ahead of the host.
-template <class F> void bloop(vol_batch B, int maxLoops, F&& iterate) {
+// Whether batched iterations replay a captured graph (VOLUMA_BATCH_GRAPH=0: launch
+// every kernel of every iteration instead; both give the same bits).
+bool useGraph() {
+    const char* e = std::getenv("VOLUMA_BATCH_GRAPH");
+    return !(e && std::string(e) == "0");
+}
+
+// The iteration `iterate` captured into B->graph (once per key). Every launch in it is
+// a kernel predicated on the systems' device flags, so the same graph serves every
+// iteration; nothing in it is a memset or copy node.
+template <class F> void captureIteration(vol_batch B, const std::string& key, F&& iterate) {
+    for (auto& g : B->graphs)
+        if (g.first == key) {
+            B->graph = g.second;
+            return;
+        }
+    if (B->graphs.size() >= vol_batch_s::MAX_GRAPHS) {
+        (void)hipGraphExecDestroy(B->graphs.front().second);
+        B->graphs.erase(B->graphs.begin());
+    }
+    B->graph = nullptr;
+    hipStream_t st = B->ctx->stream;
+    hipGraph_t g = nullptr;
+    HIPCHECK(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    try {
+        iterate();
+    } catch (...) {
+        (void)hipStreamEndCapture(st, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+    }
+    HIPCHECK(hipStreamEndCapture(st, &g));
+    hipGraphExec_t ex = nullptr;
+    const hipError_t e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    HIPCHECK(e);
+    B->graphs.push_back({key, ex});
+    B->graph = ex;
+}
+
+// Runs `iterate` until every system is done, one iteration ahead of the host. With a
+// graph key, each iteration is a replay of the captured iteration.
+template <class F> void bloop(vol_batch B, int maxLoops, F&& iterate, const std::string& graphKey = "") {
     hipStream_t st = B->ctx->stream;
     hipEvent_t evs[2];
     HIPCHECK(hipEventCreateWithFlags(&evs[0], hipEventDisableTiming));
     HIPCHECK(hipEventCreateWithFlags(&evs[1], hipEventDisableTiming));
     int* hd = B->hostDone;
-    iterate();
+    bool graph = false;  // the first iteration runs eagerly (it makes any lazy allocations)
+    auto step = [&] {
+        if (graph) {
+            HIPCHECK(hipGraphLaunch(B->graph, st));
+            ++B->graphLaunches;
+        } else {
+            iterate();
+            ++B->eagerIterations;
+        }
+    };
+    step();
     HIPCHECK(hipMemcpyAsync(&hd[0], B->nDone.p, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPCHECK(hipEventRecord(evs[0], st));
+    if (!graphKey.empty()) {
+        captureIteration(B, graphKey, iterate);
+        graph = true;
+    }
     for (int k = 1; k <= maxLoops; ++k) {
-        iterate();
+        step();
         HIPCHECK(hipMemcpyAsync(&hd[k & 1], B->nDone.p, sizeof(int), hipMemcpyDeviceToHost, st));
         HIPCHECK(hipEventRecord(evs[k & 1], st));
         HIPCHECK(hipEventSynchronize(evs[(k - 1) & 1]));
@@ -544,6 +620,17 @@ template <class F> void bloop(vol_batch B, int maxLoops, F&& iterate) {
     HIPCHECK(hipStreamSynchronize(st));
     (void)hipEventDestroy(evs[0]);
     (void)hipEventDestroy(evs[1]);
+}
+
+// Key of a captured iteration: everything its launches depend on (the plan and the
+// buffers they name; x is the caller's solution vector in FP64).
+std::string graphKey(vol_batch B, const BPlan& P, const char* prec, const double* x) {
+    if (!useGraph()) return "";
+    char buf[256];
+    std::snprintf(buf, sizeof(buf), "%s %s %d %d %.17g %p %p %p %p %p %p %p %p %p", prec, P.precond.c_str(), P.nPre,
+                  P.nPost, P.omega, (const void*)x, (void*)B->vr.p, (void*)B->vw.p, (void*)B->vp.p, (void*)B->vq.p,
+                  (void*)B->ir.p, (void*)B->ie.p, (void*)B->ip.p, (void*)B->iq.p);
+    return buf;
 }
 
 int readDone(vol_batch B, const DevScalars* S) {
@@ -602,7 +689,7 @@ void bsolve(vol_batch B, const double* b, double* x, const vol_solver_controls& 
     if (readDone(B, S) < B->nSys) {
         if (!mixed) {
             BKrylov<double> k{x, B->vr.p, B->vw.p, B->vp.p, B->vq.p};
-            bloop(B, maxLoops, [&] { bpcgIteration<double>(B, P, k, S); });
+            bloop(B, maxLoops, [&] { bpcgIteration<double>(B, P, k, S); }, graphKey(B, P, "fp64", x));
         } else {
             for (DevBuf<float>* v : {&B->ir, &B->iw, &B->ip, &B->iq, &B->ie}) v->alloc(n);
             DevScalars* SI = B->SI.p;
@@ -610,7 +697,7 @@ void bsolve(vol_batch B, const double* b, double* x, const vol_solver_controls& 
             launch_bfinalize(st, FIN_INNER_START, 0, B->red, S, SI);
             launch_bmix(st, MIX_TO32, B->red, S, B->vr.p, B->ir.p, nullptr, B->ie.p, nullptr, nullptr);
             for (int outer = 0; outer < maxLoops; ++outer) {
-                bloop(B, maxLoops, [&] { bpcgIteration<float>(B, P, k, SI); });
+                bloop(B, maxLoops, [&] { bpcgIteration<float>(B, P, k, SI); }, graphKey(B, P, "mixed", nullptr));
                 launch_bmix(st, MIX_ADD, B->red, S, nullptr, nullptr, nullptr, nullptr, x, B->ie.p);
                 bsell<double>(st, OP_AMUL, XL_PLAIN, V, L0.coef.p, B->diag.p, nullptr, x, nullptr, B->wA.p, 1.0,
                               nullptr, nullptr, &L0, S);
@@ -646,6 +733,59 @@ void bsolve(vol_batch B, const double* b, double* x, const vol_solver_controls& 
 }
 
 }  // namespace
+
+// (declared with ctxRelease in host_common.hpp, outside the namespaces)
+vol_batch batchOfCopies(vol_context ctx, vol_mesh m, int K) {
+    const MeshView v = meshView(m);
+    if (!v.H) throw VolError(VOL_ERR_NO_GAMG_WEIGHTS, "batched copies need a mesh with a GAMG hierarchy");
+    if (K < 1 || int64_t(v.n) * K > (int64_t(1) << 30) || int64_t(v.nf) * K > (int64_t(1) << 29))
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "bad number of copies");
+    auto B = std::make_unique<vol_batch_s>();
+    B->ctx = ctx;
+    B->nSys = K;
+    B->n.assign(K, v.n);
+    B->nf.assign(K, v.nf);
+    B->nLev.assign(K, 0);
+    std::vector<int32_t> own(size_t(v.nf) * K), nei(size_t(v.nf) * K);
+    for (int s = 0; s < K; ++s) {
+        std::copy(v.own, v.own + v.nf, own.begin() + size_t(s) * v.nf);
+        std::copy(v.nei, v.nei + v.nf, nei.begin() + size_t(s) * v.nf);
+    }
+    const std::vector<const std::vector<HLevel>*> pre(K, v.H);
+    B->layoutHint = v.layout;  // the mesh's coarse layouts suit its copies
+    HIPCHECK(hipSetDevice(ctx->device));
+    buildBatch(B.get(), own.data(), nei.data(), nullptr, &pre);
+    ctx->refs.fetch_add(1);
+    return B.release();
+}
+
+void batchSetSystems(vol_batch B, const std::vector<const double*>& diag, const std::vector<const double*>& upper) {
+    if (int(diag.size()) != B->nSys || int(upper.size()) != B->nSys)
+        throw VolError(VOL_ERR_INVALID_ARGUMENT, "one matrix per system");
+    HIPCHECK(hipSetDevice(B->ctx->device));
+    hipStream_t st = B->ctx->stream;
+    B->symmetric = true;
+    B->diag.alloc(B->nTotal);
+    B->upper.alloc(B->nfTotal);
+    for (int s = 0; s < B->nSys; ++s) {
+        double* d = B->diag.p + B->cellOff[s];
+        double* u = B->upper.p + B->faceOff[s];
+        const size_t n = size_t(B->n[s]), nf = size_t(B->nf[s]);
+        if (s > 0 && diag[s] == diag[s - 1] && upper[s] == upper[s - 1] && B->n[s] == B->n[s - 1]) {
+            // the same matrix as the previous system: copy on the device
+            HIPCHECK(hipMemcpyAsync(d, B->diag.p + B->cellOff[s - 1], sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+            if (nf)
+                HIPCHECK(hipMemcpyAsync(u, B->upper.p + B->faceOff[s - 1], sizeof(double) * nf,
+                                        hipMemcpyDeviceToDevice, st));
+        } else {
+            HIPCHECK(hipMemcpyAsync(d, diag[s], sizeof(double) * n, hipMemcpyHostToDevice, st));
+            if (nf) HIPCHECK(hipMemcpyAsync(u, upper[s], sizeof(double) * nf, hipMemcpyHostToDevice, st));
+        }
+    }
+    B->set = false;
+    buildCoeffs(B);
+    B->set = true;
+}
 
 extern "C" {
 
@@ -683,6 +823,15 @@ int vol_batch_destroy(vol_batch B) {
     if (B->hostDone) (void)hipHostFree(B->hostDone);
     delete B;
     ctxRelease(c);
+    API_END
+}
+
+int vol_batch_graph_stats(vol_batch B, int* graphLaunches, int* eagerIterations, int reset) {
+    API_BEGIN
+    if (!B) throw VolError(VOL_ERR_INVALID_ARGUMENT, "null batch");
+    if (graphLaunches) *graphLaunches = B->graphLaunches;
+    if (eagerIterations) *eagerIterations = B->eagerIterations;
+    if (reset) B->graphLaunches = B->eagerIterations = 0;
     API_END
 }
 

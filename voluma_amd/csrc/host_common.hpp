@@ -32,6 +32,10 @@ inline void hipCheck(hipError_t e, const char* what) {
 }
 #define HIPCHECK(x) hipCheck((x), #x)
 
+// Device bytes held by DevBufs, net of frees, counted per host thread (the solve
+// server's workers read it around their work to account what each cached entry holds).
+inline thread_local int64_t tlDeviceBytes = 0;
+
 template <class T> struct DevBuf {
     T* p = nullptr;
     size_t n = 0;
@@ -40,7 +44,10 @@ template <class T> struct DevBuf {
     DevBuf& operator=(const DevBuf&) = delete;
     ~DevBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            (void)hipFree(p);
+            tlDeviceBytes -= int64_t(sizeof(T) * std::max<size_t>(n, 1));
+        }
         p = nullptr;
         n = 0;
     }
@@ -49,6 +56,7 @@ template <class T> struct DevBuf {
         release();
         HIPCHECK(hipMalloc(&p, sizeof(T) * std::max<size_t>(count, 1)));
         n = count;
+        tlDeviceBytes += int64_t(sizeof(T) * std::max<size_t>(count, 1));
     }
     void upload(const std::vector<T>& v, hipStream_t st) {
         alloc(v.size());
@@ -460,30 +468,32 @@ inline double timeLayout(const SellDev& S, hipStream_t st, void* flush, size_t f
 }
 
 // Layout of a coarse GAMG level: the CSR-vector form (G lanes per row) or SELL-64,
-// whichever sweeps faster on this GPU from cold caches (measured once at mesh setup;
-// both give the same results, row by row in slot order). Levels above 4M rows use SELL.
-inline void buildCoarseLayout(int n, const std::vector<int>& off, const std::vector<int>& col,
-                              const std::vector<int>& src, SellDev& S, hipStream_t st) {
+// whichever sweeps faster on this GPU from cold caches (measured on the two sweeps a 1/1
+// V-cycle runs on the level; both give the same results, row by row in slot order).
+// Levels above 4M rows use SELL. `choice` (0 CSR, 1 SELL) skips the measurement: copies
+// of a mesh take the choice made for the original. VOLUMA_COARSE_LAYOUT=csr / sell fixes
+// the choice. Returns the layout built (0 CSR, 1 SELL).
+inline int buildCoarseLayout(int n, const std::vector<int>& off, const std::vector<int>& col,
+                             const std::vector<int>& src, SellDev& S, hipStream_t st, int choice = -1) {
     constexpr int TUNE_MAX_ROWS = 1 << 22;
-    bool csr = false;
-    // VOLUMA_COARSE_LAYOUT=csr / sell fixes the choice (measurement override)
     const char* force = std::getenv("VOLUMA_COARSE_LAYOUT");
     const std::string f = force ? force : "";
-    if (f == "csr" || f == "sell") {
-        if (!(f == "csr" && buildCsr(n, off, col, src, S, st))) buildSell(n, off, col, src, S, st);
-        return;
-    }
-    if (n <= TUNE_MAX_ROWS) {
+    if (f == "csr") choice = 0;
+    if (f == "sell" || n > TUNE_MAX_ROWS) choice = 1;
+    if (choice < 0) {
+        choice = 1;
         SellDev a, b;
         if (buildCsr(n, off, col, src, a, st)) {
             buildSell(n, off, col, src, b, st);
             const size_t fb = size_t(512) << 20;  // larger than L2 + the 256 MB Infinity Cache
             DevBuf<char> flush;
             flush.alloc(fb);
-            csr = timeLayout(a, st, flush.p, fb) <= timeLayout(b, st, flush.p, fb);
+            if (timeLayout(a, st, flush.p, fb) <= timeLayout(b, st, flush.p, fb)) choice = 0;
         }
     }
-    if (!(csr && buildCsr(n, off, col, src, S, st))) buildSell(n, off, col, src, S, st);
+    if (choice == 0 && buildCsr(n, off, col, src, S, st)) return 0;
+    buildSell(n, off, col, src, S, st);
+    return 1;
 }
 
 // ------------------------------------------------------------------ GAMG hierarchy (host setup)
@@ -621,9 +631,25 @@ inline std::vector<HLevel> buildHierarchy(int n, int nf, const int* own, const i
     return H;
 }
 
+}  // namespace vol::host
+
+namespace vol {
+struct AgglomerationStats {
+    int passes = 0;  // pairing passes
+    int rounds = 0;  // decision rounds over all passes
+};
+// buildHierarchy computed on the GPU of stream st (agglomeration.hip): the same levels,
+// maps, coarse faces, weights and adjacency.
+std::vector<host::HLevel> buildHierarchyDevice(hipStream_t st, int n, int nf, const int* own, const int* nei,
+                                               const double* w, int mergeLevels, AgglomerationStats* stats);
+}  // namespace vol
+
+namespace vol::host {
+
 struct LevelDev {
     int n = 0, nf = 0;
     SellDev sell;                       // coarse levels (level 0 uses the mesh SELL)
+    int layout = -1;                    // of sell: 0 CSR, 1 SELL (-1 at level 0)
     DevBuf<int> agg;                    // → next level
     DevBuf<int> mOff, mList;            // next-level cell → its fine cells (ascending)
     DevBuf<int> iOff, iList;            // next-level cell → intra-aggregate fine faces (ascending)
@@ -735,6 +761,18 @@ struct vol_context_s {
     // GAMG levels this small run in the one-workgroup tail kernel (VOLUMA_TAIL_MAX_CELLS;
     // 0 = off, the default: on MI355X separate launches for the small levels measured faster)
     int tailMax = 0;
+    // GAMG levels of at most gridMax cells (and all below them) run in one persistent
+    // grid launch of gridBlocks workgroups (VOLUMA_GRID_MAX_CELLS, VOLUMA_GRID_BLOCKS;
+    // gridMax 0 = off)
+    int gridMax = 0, gridBlocks = 0;
+    // Jacobi V-cycle forms that give the same bits (measurement switches, read from the
+    // environment at context creation): coarse levels take z = ω·(r·rD) from the
+    // restriction (VOLUMA_COARSE_Z=0: gather r and 1/diag instead); FP64 PCG keeps the
+    // level-0 z with the residual (VOLUMA_LEVEL0_Z=0); VOLUMA_PROLONG_SPLIT=1: the
+    // level-0 post-sweep reads z + xc[agg] from a streaming prolongation pass instead of
+    // gathering it per neighbour (measured slower on the 2.06M-cell tets: 24.75 against
+    // 24.60 ms per solve)
+    bool coarseZ = true, level0Z = true, prolongSplit = false;
     RedScratch red;
     DevBuf<double> part, result;
     DevBuf<long long> xpart;
@@ -770,5 +808,19 @@ struct vol_context_s {
 
 // Drops one reference of a context and frees it with the last one (api.cpp).
 void ctxRelease(vol_context c);
+// Addressing and GAMG hierarchy of a mesh handle (api.cpp).
+struct MeshView {
+    int n = 0, nf = 0;
+    const int* own = nullptr;
+    const int* nei = nullptr;
+    const std::vector<HLevel>* H = nullptr;  // null without agglomeration weights
+    std::vector<int> layout;                 // per level: 0 CSR, 1 SELL (-1 at level 0)
+};
+MeshView meshView(vol_mesh m);
+// A batch of K copies of mesh m's system on context ctx, its hierarchy reused (batch.cpp).
+vol_batch batchOfCopies(vol_context ctx, vol_mesh m, int K);
+// Symmetric matrices of the batch's systems from host arrays (diag folded, upper), then
+// the GAMG coefficients of every system (batch.cpp).
+void batchSetSystems(vol_batch B, const std::vector<const double*>& diag, const std::vector<const double*>& upper);
 // Copy of a mesh (addressing, GAMG hierarchy, boundary lists) for another context (api.cpp).
 vol_mesh meshCloneTo(vol_mesh src, vol_context dst);

@@ -148,11 +148,11 @@ void launch_sell_coef(hipStream_t st, const SellView& S, const double* upper, co
 // `exact` selects the ExactSum accumulator.
 enum VecOp : int {
     V_SUM_X = 0,          // red0 = Σ a
-    V_INIT = 1,           // rA = b − wA; red0 = Σ(|wA − pA| + |b − pA|), red1 = Σ|rA|
+    V_INIT = 1,           // rA = b − wA; red0 = Σ(|wA − pA| + |b − pA|), red1 = Σ|rA|; y (nullable) = ω·(rA·rD)
     V_PRECOND_DOT = 2,    // w = rD∘r (rD == null: w = r); red0 = Σ w·r
     V_DOT = 3,            // red0 = Σ a·b
     V_PCG_P = 4,          // p = (it == 0) ? w : w + β p
-    V_PCG_XR = 5,         // x += α p; r −= α q; red0 = Σ|r|
+    V_PCG_XR = 5,         // x += α p; r −= α q; red0 = Σ|r|; w (nullable) = ω·(r·rD)
     V_BI_P = 6,           // p = (it == 0) ? r : r + β(p − ω Ay)
     V_SCALE = 7,          // w = rD∘r (or copy)
     V_BI_S = 8,           // s = r − α Ay; red0 = Σ|s|
@@ -190,7 +190,9 @@ enum MixOp : int {
 };
 void launch_mix(hipStream_t st, int op, int n, const double* a, float* w, float* w2, float* z, double* x,
                 const float* e, const int* done);
-void launch_finalize(hipStream_t st, int op, int nred, bool exact, int nparts, const RedScratch& R, DevScalars* S);
+// hostOut (nullable, host-visible memory): receives a copy of *S after the step
+void launch_finalize(hipStream_t st, int op, int nred, bool exact, int nparts, const RedScratch& R, DevScalars* S,
+                     DevScalars* hostOut = nullptr);
 int vec_has_reduction(int op);
 
 void launch_clear_exit(hipStream_t st, DevScalars* S);
@@ -198,10 +200,11 @@ void launch_init_scalars(hipStream_t st, DevScalars* S, double tol, double relTo
                          int nSweeps, int n, double innerRel, int keepKrylov = 0);
 
 // ------------------------------------------------------------------ GAMG kernels
+// rc[I] = Σ res over the fine cells of I (ascending); zc (nullable): ω·(rc·rDc) as well
 void launch_restrict(hipStream_t st, int nc, const int* mOff, const int* mList, const double* res, double* rc,
-                     const int* done);
+                     const int* done, double* zc = nullptr, const double* rDc = nullptr, double omega = 1.0);
 void launch_restrict32(hipStream_t st, int nc, const int* mOff, const int* mList, const float* res, float* rc,
-                       const int* done);
+                       const int* done, float* zc = nullptr, const float* rDc = nullptr, double omega = 1.0);
 void launch_galerkin(hipStream_t st, int nc, int ncf, const int* mOff, const int* mList, const int* iOff,
                      const int* iList, const int* cfOff, const int* cfList, const double* Fd, const double* Fu,
                      const double* Fl, double* Cd, double* Cu, double* Cl);
@@ -260,6 +263,53 @@ using TailLevel32 = TailLevelT<float>;
 using TailArgs32 = TailArgsT<float>;
 void launch_vcycle_tail(hipStream_t st, const TailArgs& t);
 void launch_vcycle_tail32(hipStream_t st, const TailArgs32& t);
+// Coarse levels of the V-cycle on a persistent grid (k_vcycle_grid): the levels from
+// `first` down to the coarsest in one launch of GRID_THREADS-thread workgroups, all
+// resident at once, the phases of the per-level kernels separated by grid barriers.
+constexpr int GRID_THREADS = 256;
+template <class T> struct GridLevelT {
+    SellView S;
+    const T* coef = nullptr;
+    const T* diag = nullptr;
+    const T* rD = nullptr;
+    T* r = nullptr;        // right-hand side of the level (written by the restriction above)
+    T* bufA = nullptr;     // work buffers (pre-smoothed x, residual)
+    T* bufB = nullptr;
+    T* x = nullptr;        // V-cycle result of the level
+    T* z = nullptr;        // ω·(r·rD) written with r (used with one pre-sweep)
+    const int* agg = nullptr;  // → next level
+    const int* mOff = nullptr;
+    const int* mList = nullptr;
+    int nNext = 0;
+};
+// Barrier state in device memory: zero when no launch is running (every barrier leaves
+// count at 0 and advances gen). fault: a barrier wait ran past its spin limit.
+struct GridBarrier {
+    unsigned count, gen;
+    int fault;
+    int pad;
+};
+template <class T> struct GridArgsT {
+    int nLevels = 0;                  // levels in the launch, the last is the coarsest
+    const GridLevelT<T>* L = nullptr; // device array
+    int nCoarse = 0;
+    const double* lu = nullptr;       // coarsest LU factors, always FP64
+    const double* luT = nullptr;
+    int nPre = 2, nPost = 2;
+    double omega = 0.8;
+    int useZ = 0;                     // levels above the coarsest keep z (one pre-sweep)
+    const int* done = nullptr;
+    GridBarrier* bar = nullptr;
+};
+using GridLevel = GridLevelT<double>;
+using GridArgs = GridArgsT<double>;
+using GridLevel32 = GridLevelT<float>;
+using GridArgs32 = GridArgsT<float>;
+void launch_vcycle_grid(hipStream_t st, int nBlocks, const GridArgs& g);
+void launch_vcycle_grid32(hipStream_t st, int nBlocks, const GridArgs32& g);
+// Workgroups of k_vcycle_grid<T> that one CU holds at once (occupancy), for sizing grids.
+int vcycle_grid_blocks_per_cu(bool fp32);
+
 // buffer (0 = bufA, 1 = bufB) holding the pre-smoothed x after nPre sweeps
 __host__ __device__ inline int pre_smoothed_buffer(int nPre) { return nPre >= 2 ? ((nPre - 2) & 1) : 0; }
 

@@ -59,11 +59,15 @@ def build_pool(ctx, specs):
     return pool
 
 
-def register_pool(server, pool):
+def register_pool(server, pool, prefetch=False):
+    """Register every pool mesh; with prefetch, also make each resident on every worker."""
     for e in pool:
         m = e["mesh"]
         e["id"] = server.add_mesh(m.n_cells, m.owner[: m.n_internal], m.neighbour, m.agglomeration_weights(),
                                   m.owner[m.n_internal:])
+    if prefetch:
+        for e in pool:
+            server.prefetch(e["id"])
 
 
 def request_stream(pool, rate, n_requests, seed=9):
