@@ -5,8 +5,8 @@ context switch read from the environment when the context is created:
 * FP64 PCG keeps the level-0 z with its residual (VOLUMA_LEVEL0_Z);
 * the level-0 post-sweep reads z + xc[agg] from a streaming prolongation pass
   (VOLUMA_PROLONG_SPLIT=1);
-* the small levels run in one persistent grid launch with grid barriers
-  (VOLUMA_GRID_MAX_CELLS, VOLUMA_GRID_BLOCKS).
+* the levels of at most 512 cells and the coarsest run in one workgroup
+  (VOLUMA_TAIL_MAX_CELLS).
 
 Every form computes the same per-row arithmetic in the same order, so solutions,
 iteration counts and residuals must equal the plain per-level launches bit for bit, in
@@ -22,16 +22,14 @@ from voluma_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-PLAIN = {"VOLUMA_COARSE_Z": "0", "VOLUMA_LEVEL0_Z": "0", "VOLUMA_PROLONG_SPLIT": "0", "VOLUMA_GRID_MAX_CELLS": "0"}
+PLAIN = {"VOLUMA_COARSE_Z": "0", "VOLUMA_LEVEL0_Z": "0", "VOLUMA_PROLONG_SPLIT": "0", "VOLUMA_TAIL_MAX_CELLS": "0"}
 FORMS = {
     "default": {},
     "prolong_split": {"VOLUMA_PROLONG_SPLIT": "1"},
-    "grid": {"VOLUMA_GRID_MAX_CELLS": "40000"},
-    "grid_few_blocks": {"VOLUMA_GRID_MAX_CELLS": "40000", "VOLUMA_GRID_BLOCKS": "7"},
-    "grid_no_coarse_z": {"VOLUMA_GRID_MAX_CELLS": "40000", "VOLUMA_COARSE_Z": "0"},
-    "grid_from_level1": {"VOLUMA_GRID_MAX_CELLS": "100000000"},
+    "tail": {"VOLUMA_TAIL_MAX_CELLS": "512"},
+    "tail_no_coarse_z": {"VOLUMA_TAIL_MAX_CELLS": "128", "VOLUMA_COARSE_Z": "0"},
 }
-KEYS = sorted(set(PLAIN) | {"VOLUMA_GRID_BLOCKS"})
+KEYS = sorted(PLAIN)
 
 
 def _context(env):

@@ -166,10 +166,6 @@ struct vol_matrix_s {
     int tailStart = -1;  // first GAMG level handled by the tail kernel (−1: none)
     int tailLdsBytes = 0, tailLdsBytes32 = 0;
     DevBuf<TailLevel> tail;
-    int gridStart = -1;  // first GAMG level of the persistent grid launch (−1: none)
-    DevBuf<GridLevel> grid;
-    DevBuf<GridLevel32> grid32;
-    DevBuf<GridBarrier> gridBar;
     DevBuf<int> status;
     // FP32 copies for mixed precision, built lazily after each vol_matrix_set
     bool mixedReady = false, mixedGamg = false;
@@ -339,46 +335,6 @@ template <class T> int tailLayout(std::vector<TailLevelT<T>>& V, int coarseN) {
     return int(bytes);
 }
 
-// Descriptors of the grid launch's levels (gridStart … coarsest) in precision T.
-template <class T> std::vector<GridLevelT<T>> gridLevels(vol_matrix A) {
-    vol_mesh m = A->mesh;
-    const int nL = int(m->lv.size());
-    std::vector<GridLevelT<T>> G;
-    for (int k = A->gridStart; k < nL; ++k) {
-        const LevelDev& D = *m->lv[k];
-        MatLevel& M = *A->L[k];
-        GridLevelT<T> g;
-        g.S = D.sell.view();
-        if constexpr (std::is_same_v<T, double>) {
-            g.coef = M.coef.p;
-            g.diag = M.diag.p;
-            g.rD = M.rD.p;
-            g.r = M.r.p;
-            g.bufA = M.bufA.p;
-            g.bufB = M.bufB.p;
-            g.x = M.x.p;
-            g.z = M.z.p;
-        } else {
-            g.coef = M.coef32.p;
-            g.diag = M.diag32.p;
-            g.rD = M.rD32.p;
-            g.r = M.r32.p;
-            g.bufA = M.bufA32.p;
-            g.bufB = M.bufB32.p;
-            g.x = M.x32.p;
-            g.z = M.z32.p;
-        }
-        if (k + 1 < nL) {
-            g.agg = D.agg.p;
-            g.mOff = D.mOff.p;
-            g.mList = D.mList.p;
-            g.nNext = m->lv[k + 1]->n;
-        }
-        G.push_back(g);
-    }
-    return G;
-}
-
 void buildGamgCoeffs(vol_matrix A) {
     vol_mesh m = A->mesh;
     hipStream_t st = m->ctx->stream;
@@ -460,22 +416,6 @@ void buildGamgCoeffs(vol_matrix A) {
         }
         A->tailLdsBytes = tailLayout(T, DL.n);
         A->tail.upload(T, st);
-    }
-    // without the tail: the levels from the first of ≤ gridMax cells (below level 0, above
-    // the coarsest) down to the coarsest in one persistent grid launch
-    A->gridStart = -1;
-    if (A->tailStart < 0 && m->ctx->gridMax > 0)
-        for (int k = 1; k + 1 < nL; ++k)
-            if (m->lv[k]->n <= m->ctx->gridMax) {
-                A->gridStart = k;
-                break;
-            }
-    if (A->gridStart >= 0) {
-        A->grid.upload(gridLevels<double>(A), st);
-        if (!A->gridBar.p) {
-            A->gridBar.alloc(1);
-            HIPCHECK(hipMemsetAsync(A->gridBar.p, 0, sizeof(GridBarrier), st));
-        }
     }
     int status = 0;
     HIPCHECK(hipMemcpyAsync(&status, A->status.p, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -869,7 +809,6 @@ void buildMixed(vol_matrix A, bool gamg) {
             A->tailLdsBytes32 = tailLayout(T, DL.n);
             A->tail32.upload(T, st);
         }
-        if (A->gridStart >= 0) A->grid32.upload(gridLevels<float>(A), st);
     }
     HIPCHECK(hipStreamSynchronize(st));
     A->mixedReady = true;
@@ -1276,42 +1215,6 @@ void tailLaunch(const Ctx& C, vol_matrix A, const Plan& P, int l, const float* r
     t.done = C.done;
     launch_vcycle_tail32(C.st, t);
 }
-// After a solve (stream synchronised): a grid launch whose barrier wait gave up has set
-// the fault flag; its results are void. The barrier is reset for the next launch.
-void checkGridBarrier(vol_matrix A, hipStream_t st) {
-    if (!A->gridBar.p) return;
-    GridBarrier b{};
-    HIPCHECK(hipMemcpyAsync(&b, A->gridBar.p, sizeof(b), hipMemcpyDeviceToHost, st));
-    HIPCHECK(hipStreamSynchronize(st));
-    if (b.fault) {
-        HIPCHECK(hipMemsetAsync(A->gridBar.p, 0, sizeof(GridBarrier), st));
-        HIPCHECK(hipStreamSynchronize(st));
-        throw VolError(VOL_ERR_HIP, "GAMG grid launch: a barrier wait timed out (workgroups not co-resident)");
-    }
-}
-
-template <class T> void gridLaunch(const Ctx& C, vol_matrix A, const Plan& P, int l, T*) {
-    GridArgsT<T> g;
-    g.nLevels = int(A->mesh->lv.size()) - l;
-    if constexpr (std::is_same_v<T, double>)
-        g.L = A->grid.p;
-    else
-        g.L = A->grid32.p;
-    g.nCoarse = A->nCoarse;
-    g.lu = A->lu.p;
-    g.luT = A->luT.p;
-    g.nPre = P.nPre;
-    g.nPost = P.nPost;
-    g.omega = P.omega;
-    g.useZ = P.nPre == 1 && A->mesh->ctx->coarseZ ? 1 : 0;
-    g.done = C.done;
-    g.bar = A->gridBar.p;
-    if constexpr (std::is_same_v<T, double>)
-        launch_vcycle_grid(C.st, A->mesh->ctx->gridBlocks, g);
-    else
-        launch_vcycle_grid32(C.st, A->mesh->ctx->gridBlocks, g);
-}
-
 void coarseSolve(const Ctx& C, vol_matrix A, const double* r, double* x) {
     launch_coarse_solve(C.st, A->nCoarse, A->lu.p, A->luT.p, r, x, C.done);
 }
@@ -1346,10 +1249,6 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
     const int nL = int(m->lv.size());
     if (l == A->tailStart) {
         tailLaunch(C, A, P, l, r, xout);
-        return;
-    }
-    if (l == A->gridStart) {  // r and xout are the level's own r and x buffers
-        gridLaunch(C, A, P, l, (T*)nullptr);
         return;
     }
     if (l == nL - 1) {
@@ -1778,7 +1677,6 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
         readScalars(c, C.S, hs);
         if (!hs.done) throw VolError(VOL_ERR_HIP, "solver loop did not terminate");
     }
-    checkGridBarrier(A, C.st);
     collectTiming(c);
     perf.initialResidual = hs.initialResidual;
     perf.finalResidual = hs.finalResidual;
@@ -1969,7 +1867,6 @@ int vol_context_create(int device, vol_context* out) {
     HIPCHECK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
     if (const char* e = std::getenv("CFD_REPRODUCIBLE")) c->reproducible = std::string(e) != "0" && std::string(e) != "";
     if (const char* e = std::getenv("VOLUMA_TAIL_MAX_CELLS")) c->tailMax = std::min(std::atoi(e), TAIL_MAX_CELLS);
-    if (const char* e = std::getenv("VOLUMA_GRID_MAX_CELLS")) c->gridMax = std::max(0, std::atoi(e));
     auto envOff = [](const char* name) {
         const char* e = std::getenv(name);
         return e && std::string(e) == "0";
@@ -1977,17 +1874,6 @@ int vol_context_create(int device, vol_context* out) {
     c->coarseZ = !envOff("VOLUMA_COARSE_Z");
     c->level0Z = !envOff("VOLUMA_LEVEL0_Z");
     if (const char* e = std::getenv("VOLUMA_PROLONG_SPLIT")) c->prolongSplit = std::string(e) == "1";
-    {
-        // all workgroups of a grid launch must be resident together, with room for the
-        // grid launches of other contexts on the same GPU (at most a quarter each)
-        int cus = 0;
-        HIPCHECK(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
-        const int per = std::min(vcycle_grid_blocks_per_cu(false), vcycle_grid_blocks_per_cu(true));
-        c->gridBlocks = std::min(128, std::max(1, cus * per / 4));
-        if (const char* e = std::getenv("VOLUMA_GRID_BLOCKS"))
-            c->gridBlocks = std::min(c->gridBlocks, std::max(1, std::atoi(e)));
-        if (per <= 0) c->gridMax = 0;
-    }
     c->scal.alloc(2);
     HIPCHECK(hipHostMalloc((void**)&c->hostScal, 2 * sizeof(DevScalars), hipHostMallocDefault));
     c->ensureScratch(RED_BLOCKS);

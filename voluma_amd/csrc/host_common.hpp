@@ -761,10 +761,6 @@ struct vol_context_s {
     // GAMG levels this small run in the one-workgroup tail kernel (VOLUMA_TAIL_MAX_CELLS;
     // 0 = off, the default: on MI355X separate launches for the small levels measured faster)
     int tailMax = 0;
-    // GAMG levels of at most gridMax cells (and all below them) run in one persistent
-    // grid launch of gridBlocks workgroups (VOLUMA_GRID_MAX_CELLS, VOLUMA_GRID_BLOCKS;
-    // gridMax 0 = off)
-    int gridMax = 0, gridBlocks = 0;
     // Jacobi V-cycle forms that give the same bits (measurement switches, read from the
     // environment at context creation): coarse levels take z = ω·(r·rD) from the
     // restriction (VOLUMA_COARSE_Z=0: gather r and 1/diag instead); FP64 PCG keeps the

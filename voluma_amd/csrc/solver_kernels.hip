@@ -1345,214 +1345,5 @@ static void launch_tail_t(hipStream_t st, const TailArgsT<T>& t) {
 void launch_vcycle_tail(hipStream_t st, const TailArgs& t) { launch_tail_t(st, t); }
 void launch_vcycle_tail32(hipStream_t st, const TailArgs32& t) { launch_tail_t(st, t); }
 
-// ============================================================================ V-cycle on a grid
-// The coarse levels of the V-cycle in one launch of workgroups that are all resident at
-// once: each phase of the per-level kernels (sweep, residual, restriction, coarsest
-// solve, prolongation sweep) is spread over the whole grid, and a grid barrier separates
-// it from the next. Per-row arithmetic is sell_row / csr_row and the restriction sums
-// in the same order, so the results equal the per-level launches bit for bit.
-constexpr unsigned GRID_SPIN_LIMIT = 1u << 25;  // ≈ 1–2 s of s_sleep(1) per barrier
-
-// Barrier over all workgroups of the grid. Thread 0 of each workgroup arrives with a
-// release (the workgroup's writes, ordered before it by the block barrier, become visible
-// at device scope) and waits with an acquire; the last to arrive resets the count and
-// advances the generation. A wait that runs past the spin limit sets bar->fault and
-// gives up, so every launch ends; the host turns the flag into an error.
-__device__ __forceinline__ void grid_sync(GridBarrier* bar) {
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned gen = __hip_atomic_load(&bar->gen, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        const unsigned arrived =
-            __hip_atomic_fetch_add(&bar->count, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) + 1u;
-        if (arrived == gridDim.x) {
-            __hip_atomic_store(&bar->count, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            __hip_atomic_store(&bar->gen, gen + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-        } else {
-            unsigned spins = 0;
-            while (__hip_atomic_load(&bar->gen, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == gen) {
-                __builtin_amdgcn_s_sleep(1);
-                if (++spins > GRID_SPIN_LIMIT) {
-                    __hip_atomic_store(&bar->fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    break;
-                }
-            }
-        }
-    }
-    __syncthreads();
-}
-
-// CSR rows with G lanes each, the grid's threads striding over them; the loop bound is
-// uniform per wavefront (csr_row needs every lane of the wave).
-template <int OP, int XL, int G, int IT, class T>
-__device__ __forceinline__ void grid_csr(const SellArgsT<T>& a) {
-    const long total = long(a.S.n) * G;
-    const long stride = long(gridDim.x) * blockDim.x;
-    const int lane = threadIdx.x & (WAVE - 1);
-    for (long base = long(blockIdx.x) * blockDim.x + threadIdx.x; base - lane < total; base += stride) {
-        const int row = int(base / G), g = int(base % G);
-        csr_row<OP, XL, G, IT>(a, row, g, row < a.S.n, T(0));
-    }
-}
-
-template <int OP, int XL, class T>
-__device__ __forceinline__ void grid_sell(const SellArgsT<T>& a) {
-    const int stride = gridDim.x * blockDim.x;
-    for (int row = blockIdx.x * blockDim.x + threadIdx.x; row < a.S.n; row += stride)
-        sell_row<OP, XL, 4>(a, row, T(0));
-}
-
-// One gather phase over all rows of a level (SELL: a wavefront per 64-row slice).
-template <int OP, int XL, class T>
-__device__ __forceinline__ void grid_rows(const SellArgsT<T>& a) {
-    const int G = a.S.csrG, IT = a.S.csrIt;
-    if (G > 0) {
-        if (G == 4 && IT <= 4) grid_csr<OP, XL, 4, 4>(a);
-        else if (G == 8 && IT <= 4) grid_csr<OP, XL, 8, 4>(a);
-        else if (G == 16 && IT <= 4) grid_csr<OP, XL, 16, 4>(a);
-        else grid_csr<OP, XL, 16, 8>(a);
-    } else {
-        grid_sell<OP, XL>(a);
-    }
-}
-
-template <class T>
-__device__ __forceinline__ SellArgsT<T> grid_level_args(const GridLevelT<T>& L, double omega) {
-    SellArgsT<T> a;
-    a.S = L.S;
-    a.coef = L.coef;
-    a.diag = L.diag;
-    a.rD = L.rD;
-    a.r = L.r;
-    a.omega = omega;
-    return a;
-}
-
-template <class T>
-__global__ __launch_bounds__(GRID_THREADS) void k_vcycle_grid(GridArgsT<T> g) {
-    if (g.done && *g.done) return;
-    extern __shared__ double gsh[];
-    const int nl = g.nLevels;
-    const int cur = pre_smoothed_buffer(g.nPre), freeB = 1 - cur;
-    const int gtid = blockIdx.x * blockDim.x + threadIdx.x, gthreads = gridDim.x * blockDim.x;
-    for (int l = 0; l + 1 < nl; ++l) {
-        const GridLevelT<T>& L = g.L[l];
-        SellArgsT<T> a = grid_level_args(L, g.omega);
-        T* const z = g.useZ ? L.z : nullptr;
-        T* bufs[2] = {L.bufA, L.bufB};
-        int c = 0;
-        for (int s = 1; s < g.nPre; ++s) {
-            const int dst = (s == 1) ? 0 : 1 - c;
-            a.x = bufs[c];
-            a.out = bufs[dst];
-            if (s == 1)
-                grid_rows<OP_JACOBI, XL_FROMZERO>(a);
-            else
-                grid_rows<OP_JACOBI, XL_PLAIN>(a);
-            grid_sync(g.bar);
-            c = dst;
-        }
-        a.out = bufs[freeB];
-        if (z) {
-            a.x = z;
-            grid_rows<OP_RESID, XL_PLAIN>(a);
-        } else {
-            a.x = bufs[cur];
-            if (g.nPre == 1)
-                grid_rows<OP_RESID, XL_FROMZERO>(a);
-            else
-                grid_rows<OP_RESID, XL_PLAIN>(a);
-        }
-        grid_sync(g.bar);
-        // restriction into level l + 1, in ascending fine-cell order (k_restrict)
-        const GridLevelT<T>& N = g.L[l + 1];
-        T* const zc = (g.useZ && l + 2 < nl) ? N.z : nullptr;
-        const T* res = bufs[freeB];
-        for (int I = gtid; I < L.nNext; I += gthreads) {
-            T sum = T(0);
-            for (int k = L.mOff[I]; k < L.mOff[I + 1]; ++k) sum += res[L.mList[k]];
-            N.r[I] = sum;
-            if (zc) zc[I] = T(g.omega) * (sum * N.rD[I]);
-        }
-        grid_sync(g.bar);
-    }
-    if (blockIdx.x == 0) {  // coarsest level: the operations of k_coarse_solve64 / k_coarse_solve
-        const GridLevelT<T>& LC = g.L[nl - 1];
-        const int n = g.nCoarse;
-        if (n <= 64) {  // factors staged in LDS, one wavefront substitutes
-            for (int q = threadIdx.x; q < n * n; q += blockDim.x) gsh[q] = g.luT[q];
-            __syncthreads();
-            if (threadIdx.x < WAVE) {
-                const int i = threadIdx.x;
-                double v = i < n ? double(LC.r[i]) : 0.0;
-                for (int k = 0; k < n; ++k) {
-                    const double vk = lane_bcast(v, k);
-                    if (i > k && i < n) v -= gsh[k * n + i] * vk;
-                }
-                for (int k = n - 1; k >= 0; --k) {
-                    if (i == k) v /= gsh[k * n + k];
-                    const double vk = lane_bcast(v, k);
-                    if (i < k) v -= gsh[k * n + i] * vk;
-                }
-                if (i < n) LC.x[i] = T(v);
-            }
-        } else {
-            coarse_solve_body(n, g.lu, LC.r, LC.x, gsh);
-        }
-    }
-    grid_sync(g.bar);
-    for (int l = nl - 2; l >= 0; --l) {
-        const GridLevelT<T>& L = g.L[l];
-        SellArgsT<T> a = grid_level_args(L, g.omega);
-        a.xc = g.L[l + 1].x;
-        a.agg = L.agg;
-        T* const z = g.useZ ? L.z : nullptr;
-        T* bufs[2] = {L.bufA, L.bufB};
-        int src = cur, other = freeB;
-        for (int s = 0; s < g.nPost; ++s) {
-            const bool last = s == g.nPost - 1;
-            a.out = last ? L.x : bufs[other];
-            if (s == 0 && z) {
-                a.x = z;
-                grid_rows<OP_JACOBI, XL_PROLONG>(a);
-            } else if (s == 0) {
-                a.x = bufs[src];
-                if (g.nPre == 1)
-                    grid_rows<OP_JACOBI, XL_ZEROPROLONG>(a);
-                else
-                    grid_rows<OP_JACOBI, XL_PROLONG>(a);
-            } else {
-                a.x = bufs[src];
-                grid_rows<OP_JACOBI, XL_PLAIN>(a);
-            }
-            if (!(l == 0 && last)) grid_sync(g.bar);  // the launch's end orders the last phase
-            if (!last) {
-                const int t = src;
-                src = other;
-                other = t;
-            }
-        }
-    }
-}
-
-template <class T> static size_t grid_lds_bytes(int nCoarse) {
-    return sizeof(double) * size_t(nCoarse <= 64 ? nCoarse * nCoarse : nCoarse);
-}
-
-template <class T>
-static void launch_grid_t(hipStream_t st, int nBlocks, const GridArgsT<T>& g) {
-    hipLaunchKernelGGL(k_vcycle_grid<T>, dim3(nBlocks), dim3(GRID_THREADS), grid_lds_bytes<T>(g.nCoarse), st, g);
-}
-void launch_vcycle_grid(hipStream_t st, int nBlocks, const GridArgs& g) { launch_grid_t(st, nBlocks, g); }
-void launch_vcycle_grid32(hipStream_t st, int nBlocks, const GridArgs32& g) { launch_grid_t(st, nBlocks, g); }
-
-int vcycle_grid_blocks_per_cu(bool fp32) {
-    int nb = 0;
-    const hipError_t e =
-        fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_vcycle_grid<float>, GRID_THREADS,
-                                                            sizeof(double) * 4096)
-             : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_vcycle_grid<double>, GRID_THREADS,
-                                                            sizeof(double) * 4096);
-    return e == hipSuccess ? nb : 0;
-}
 
 }  // namespace vol

@@ -263,53 +263,6 @@ using TailLevel32 = TailLevelT<float>;
 using TailArgs32 = TailArgsT<float>;
 void launch_vcycle_tail(hipStream_t st, const TailArgs& t);
 void launch_vcycle_tail32(hipStream_t st, const TailArgs32& t);
-// Coarse levels of the V-cycle on a persistent grid (k_vcycle_grid): the levels from
-// `first` down to the coarsest in one launch of GRID_THREADS-thread workgroups, all
-// resident at once, the phases of the per-level kernels separated by grid barriers.
-constexpr int GRID_THREADS = 256;
-template <class T> struct GridLevelT {
-    SellView S;
-    const T* coef = nullptr;
-    const T* diag = nullptr;
-    const T* rD = nullptr;
-    T* r = nullptr;        // right-hand side of the level (written by the restriction above)
-    T* bufA = nullptr;     // work buffers (pre-smoothed x, residual)
-    T* bufB = nullptr;
-    T* x = nullptr;        // V-cycle result of the level
-    T* z = nullptr;        // ω·(r·rD) written with r (used with one pre-sweep)
-    const int* agg = nullptr;  // → next level
-    const int* mOff = nullptr;
-    const int* mList = nullptr;
-    int nNext = 0;
-};
-// Barrier state in device memory: zero when no launch is running (every barrier leaves
-// count at 0 and advances gen). fault: a barrier wait ran past its spin limit.
-struct GridBarrier {
-    unsigned count, gen;
-    int fault;
-    int pad;
-};
-template <class T> struct GridArgsT {
-    int nLevels = 0;                  // levels in the launch, the last is the coarsest
-    const GridLevelT<T>* L = nullptr; // device array
-    int nCoarse = 0;
-    const double* lu = nullptr;       // coarsest LU factors, always FP64
-    const double* luT = nullptr;
-    int nPre = 2, nPost = 2;
-    double omega = 0.8;
-    int useZ = 0;                     // levels above the coarsest keep z (one pre-sweep)
-    const int* done = nullptr;
-    GridBarrier* bar = nullptr;
-};
-using GridLevel = GridLevelT<double>;
-using GridArgs = GridArgsT<double>;
-using GridLevel32 = GridLevelT<float>;
-using GridArgs32 = GridArgsT<float>;
-void launch_vcycle_grid(hipStream_t st, int nBlocks, const GridArgs& g);
-void launch_vcycle_grid32(hipStream_t st, int nBlocks, const GridArgs32& g);
-// Workgroups of k_vcycle_grid<T> that one CU holds at once (occupancy), for sizing grids.
-int vcycle_grid_blocks_per_cu(bool fp32);
-
 // buffer (0 = bufA, 1 = bufB) holding the pre-smoothed x after nPre sweeps
 __host__ __device__ inline int pre_smoothed_buffer(int nPre) { return nPre >= 2 ? ((nPre - 2) & 1) : 0; }
 
