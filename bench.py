@@ -394,6 +394,7 @@ def main():
     t0 = time.perf_counter()
     lm, A, b = W.pressure_system(ctx, m, seed=1, gamg=True, nonorth=True)
     levels = [L["n"] for L in lm.gamg_levels()]
+    level_faces = [int(L["faceMap"].size) for L in lm.gamg_levels()]
     log(f"GPU setup (assembly + hierarchy) {time.perf_counter() - t0:.1f} s, levels {levels}")
     agg_s, agg_passes, agg_rounds = lm.gamg_setup_info()
     os.environ["VOLUMA_AGGLOMERATION"] = "host"  # the same hierarchy by the host loop, for comparison
@@ -547,7 +548,8 @@ def main():
         "config": {"workload": f"{args.workload}: pressure Poisson (laplacian with non-orthogonal correction), "
                                f"GAMG-PCG (damped-Jacobi V-cycle {args.pre}/{args.post}, omega {args.omega}) to "
                                f"normalised residual {args.tol:g}, FP64",
-                   "cells": n, "internal_faces": nf, "gamg_levels": levels, "precision": "fp64"},
+                   "cells": n, "internal_faces": nf, "gamg_levels": levels, "gamg_level_faces": level_faces,
+                   "precision": "fp64"},
         "latency_ms": {"p50": float(np.percentile(lat_ms, 50)), "p99": float(np.percentile(lat_ms, 99)),
                        "min": float(lat_ms.min())},
         "mixed": mixed,

@@ -5,8 +5,8 @@ context switch read from the environment when the context is created:
 * FP64 PCG keeps the level-0 z with its residual (VOLUMA_LEVEL0_Z);
 * the level-0 post-sweep reads z + xc[agg] from a streaming prolongation pass
   (VOLUMA_PROLONG_SPLIT=1);
-* the levels of at most 512 cells and the coarsest run in one workgroup
-  (VOLUMA_TAIL_MAX_CELLS).
+* the levels from the first of at most VOLUMA_TAIL_MAX_CELLS cells (up to 4096) down
+  to the coarsest run in one workgroup, matrices preloaded into LDS where they fit.
 
 Every form computes the same per-row arithmetic in the same order, so solutions,
 iteration counts and residuals must equal the plain per-level launches bit for bit, in
@@ -28,6 +28,7 @@ FORMS = {
     "prolong_split": {"VOLUMA_PROLONG_SPLIT": "1"},
     "tail": {"VOLUMA_TAIL_MAX_CELLS": "512"},
     "tail_no_coarse_z": {"VOLUMA_TAIL_MAX_CELLS": "128", "VOLUMA_COARSE_Z": "0"},
+    "tail_4096": {"VOLUMA_TAIL_MAX_CELLS": "4096"},
 }
 KEYS = sorted(PLAIN)
 

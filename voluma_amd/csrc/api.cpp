@@ -166,6 +166,8 @@ struct vol_matrix_s {
     int tailStart = -1;  // first GAMG level handled by the tail kernel (−1: none)
     int tailLdsBytes = 0, tailLdsBytes32 = 0;
     DevBuf<TailLevel> tail;
+    DevBuf<TailChunk> tailChunks, tailChunks32;  // LDS preload of the tail kernel
+    int tailNChunks = 0, tailNChunks32 = 0;
     DevBuf<int> status;
     // FP32 copies for mixed precision, built lazily after each vol_matrix_set
     bool mixedReady = false, mixedGamg = false;
@@ -335,6 +337,42 @@ template <class T> int tailLayout(std::vector<TailLevelT<T>>& V, int coarseN) {
     return int(bytes);
 }
 
+// The chunks the tail kernel preloads into LDS: the matrices of the levels that fit
+// (layout of tailLayout), the coarsest LU factors when the coarsest has ≤ 64 cells, and
+// rTop (the first tail level's r) when given (its address is fixed below level 0).
+template <class T>
+std::vector<TailChunk> tailChunks(const std::vector<TailLevelT<T>>& V, int coarseN, const double* luT, const T* rTop) {
+    std::vector<TailChunk> C;
+    auto add = [&](const void* src, size_t byteOff, size_t bytes) {
+        const unsigned* s = static_cast<const unsigned*>(src);
+        const int dst = int(byteOff / 4);
+        const int words = int(bytes / 4);
+        for (int w = 0; w < words; w += TAIL_CHUNK_WORDS) {
+            TailChunk c;
+            c.src = s + w;
+            c.dst = dst + w;
+            c.n = std::min(TAIL_CHUNK_WORDS, words - w);
+            C.push_back(c);
+        }
+    };
+    if (coarseN <= 64) add(luT, 0, sizeof(double) * size_t(coarseN) * coarseN);
+    if (rTop) add(rTop, sizeof(T) * size_t(V[0].ldsR), sizeof(T) * size_t(V[0].S.n));
+    for (size_t k = 0; k + 1 < V.size(); ++k) {
+        const TailLevelT<T>& t = V[k];
+        if (!t.ldsMat) continue;
+        const size_t n = t.S.n, ns = t.S.nslots;
+        add(t.S.sliceOff, t.ldsRowOff, 4 * size_t(t.nRowOff));
+        add(t.S.col, t.ldsCol, 4 * ns);
+        add(t.coef, t.ldsCoef, sizeof(T) * ns);
+        add(t.diag, t.ldsDiag, sizeof(T) * n);
+        add(t.rD, t.ldsRD, sizeof(T) * n);
+        if (t.agg) add(t.agg, t.ldsAgg, 4 * n);
+        if (t.mOff) add(t.mOff, t.ldsMOff, 4 * (size_t(t.nNext) + 1));
+        if (t.mList) add(t.mList, t.ldsMList, 4 * n);
+    }
+    return C;
+}
+
 void buildGamgCoeffs(vol_matrix A) {
     vol_mesh m = A->mesh;
     hipStream_t st = m->ctx->stream;
@@ -416,6 +454,10 @@ void buildGamgCoeffs(vol_matrix A) {
         }
         A->tailLdsBytes = tailLayout(T, DL.n);
         A->tail.upload(T, st);
+        const double* rTop = A->tailStart > 0 ? A->L[A->tailStart]->r.p : nullptr;
+        const std::vector<TailChunk> ch = tailChunks<double>(T, DL.n, A->luT.p, rTop);
+        A->tailChunks.upload(ch, st);
+        A->tailNChunks = int(ch.size());
     }
     int status = 0;
     HIPCHECK(hipMemcpyAsync(&status, A->status.p, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -808,6 +850,10 @@ void buildMixed(vol_matrix A, bool gamg) {
             }
             A->tailLdsBytes32 = tailLayout(T, DL.n);
             A->tail32.upload(T, st);
+            const float* rTop = A->tailStart > 0 ? A->L[A->tailStart]->r32.p : nullptr;
+            const std::vector<TailChunk> ch = tailChunks<float>(T, DL.n, A->luT.p, rTop);
+            A->tailChunks32.upload(ch, st);
+            A->tailNChunks32 = int(ch.size());
         }
     }
     HIPCHECK(hipStreamSynchronize(st));
@@ -1196,6 +1242,9 @@ void tailLaunch(const Ctx& C, vol_matrix A, const Plan& P, int l, const double* 
     t.nPost = P.nPost;
     t.omega = P.omega;
     t.done = C.done;
+    t.chunks = A->tailChunks.p;
+    t.nChunks = A->tailNChunks;
+    t.rInChunks = l > 0 && r == A->L[l]->r.p ? 1 : 0;
     launch_vcycle_tail(C.st, t);
 }
 void tailLaunch(const Ctx& C, vol_matrix A, const Plan& P, int l, const float* r, float* xout) {
@@ -1213,6 +1262,9 @@ void tailLaunch(const Ctx& C, vol_matrix A, const Plan& P, int l, const float* r
     t.nPost = P.nPost;
     t.omega = P.omega;
     t.done = C.done;
+    t.chunks = A->tailChunks32.p;
+    t.nChunks = A->tailNChunks32;
+    t.rInChunks = l > 0 && r == A->L[l]->r32.p ? 1 : 0;
     launch_vcycle_tail32(C.st, t);
 }
 void coarseSolve(const Ctx& C, vol_matrix A, const double* r, double* x) {
@@ -1277,11 +1329,16 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
                 P.nPre == 1 ? nullptr : bufs[cur], r, bufs[freeB], P.omega, nullptr, nullptr, nullptr, nullptr, nullptr,
                 L.rD);
     const LevelPtrs<T> LC = levelPtrs(A, l + 1, (T*)nullptr);
-    if (levelUsesZ(A, P, l + 1))
-        restrictLevel(C, m->lv[l + 1]->n, D, bufs[freeB], LC.r, LC.z, LC.rD, P.omega);
-    else
-        restrictLevel(C, m->lv[l + 1]->n, D, bufs[freeB], LC.r);
-    vcycle<T>(C, A, P, l + 1, LC.r, LC.x);
+    if (l + 1 == nL - 1 && A->nCoarse <= 64 && A->tailStart != l + 1) {
+        // the coarsest level: restriction and substitution in one launch
+        launch_restrict_coarse64<T>(C.st, A->nCoarse, D.mOff.p, D.mList.p, bufs[freeB], A->luT.p, LC.x, C.done);
+    } else {
+        if (levelUsesZ(A, P, l + 1))
+            restrictLevel(C, m->lv[l + 1]->n, D, bufs[freeB], LC.r, LC.z, LC.rD, P.omega);
+        else
+            restrictLevel(C, m->lv[l + 1]->n, D, bufs[freeB], LC.r);
+        vcycle<T>(C, A, P, l + 1, LC.r, LC.x);
+    }
     // prolongation fused into the first post-sweep
     int src = cur, other = freeB;
     for (int s = 0; s < P.nPost; ++s) {

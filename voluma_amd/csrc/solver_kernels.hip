@@ -28,10 +28,17 @@ __device__ __forceinline__ double lane_bcast(double v, int lane) {
 constexpr int SELL_WAVES = 4;  // 256 threads per block
 constexpr int NXCD = 8;
 
-int sell_grid_blocks(int nslices) {
-    const int logical = (nslices + SELL_WAVES - 1) / SELL_WAVES;
-    return ((logical + NXCD - 1) / NXCD) * NXCD;
+// Grid rounded up to a multiple of the XCD count, and the logical block of this block
+// under the XCD-contiguous remap: with blocks dispatched round-robin over the XCDs, XCD
+// x gets logical blocks [x·nb/8, (x+1)·nb/8), so rows that share cache lines (of the
+// vectors they write, read or gather) meet in one XCD's L2 instead of up to eight.
+inline int xcd_grid(long blocks) { return int(((blocks + NXCD - 1) / NXCD) * NXCD); }
+__device__ __forceinline__ int xcd_block() {
+    const int per = gridDim.x / NXCD;
+    return (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
 }
+
+int sell_grid_blocks(int nslices) { return xcd_grid((nslices + SELL_WAVES - 1) / SELL_WAVES); }
 
 // Value of x at row j for the gather ops; T is the storage/arithmetic type of the level
 // (double, or float in the mixed-precision inner solve).
@@ -117,8 +124,7 @@ __device__ __forceinline__ T sell_row(const SellArgsT<T>& a, int row, T xu) {
 template <int OP, int XL, bool DOT, int U, class T>
 __global__ __launch_bounds__(256) void k_sell(SellArgsT<T> a) {
     if (a.done && *a.done) return;
-    const int nb = gridDim.x, per = nb / NXCD;
-    const int lb = (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;  // XCD-contiguous logical block
+    const int lb = xcd_block();
     const int slice = lb * SELL_WAVES + threadIdx.x / WAVE;
     const int lane = threadIdx.x & (WAVE - 1);
     const int row = slice * WAVE + lane;
@@ -200,7 +206,7 @@ __device__ __forceinline__ T csr_row(const SellArgsT<T>& a, int row, int g, bool
 template <int OP, int XL, int G, int IT, class T>
 __global__ __launch_bounds__(256) void k_csrv(SellArgsT<T> a) {
     if (a.done && *a.done) return;
-    const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int gid = xcd_block() * blockDim.x + threadIdx.x;
     const int row = gid / G, g = gid % G;
     T xu = T(0);
     if constexpr (XL == XL_UNIFORM) xu = *a.xuPtr;
@@ -219,7 +225,7 @@ template <class T, class K> static void sell_launch(K kernel, int nb, hipStream_
 template <int OP, int XL, class T>
 static void csr_dispatch(hipStream_t st, const SellArgsT<T>& a) {
     const long threads = long(a.S.n) * a.S.csrG;
-    const int nb = int((threads + 255) / 256);
+    const int nb = xcd_grid((threads + 255) / 256);
     if (nb == 0) return;
     const int G = a.S.csrG, IT = a.S.csrIt;
     if (G == 4 && IT <= 4) sell_launch(k_csrv<OP, XL, 4, 4, T>, nb, st, a);
@@ -751,10 +757,24 @@ template <class T>
 __global__ void k_restrict(int nc, const int* mOff, const int* mList, const T* res, T* rc, const int* done, T* zc,
                            const T* rDc, double omega) {
     if (done && *done) return;
-    const int I = blockIdx.x * blockDim.x + threadIdx.x;
+    const int I = xcd_block() * blockDim.x + threadIdx.x;
     if (I >= nc) return;
+    // members in groups of 8: every index load of a group, then every value load, are in
+    // flight together; the sum still adds them one by one in ascending order
+    constexpr int U = 8;
+    const int b = mOff[I], e = mOff[I + 1];
     T s = T(0);
-    for (int k = mOff[I]; k < mOff[I + 1]; ++k) s += res[mList[k]];
+    for (int k0 = b; k0 < e; k0 += U) {
+        int j[U];
+        T v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) j[u] = k0 + u < e ? mList[k0 + u] : -1;
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = j[u] >= 0 ? res[j[u]] : T(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (j[u] >= 0) s += v[u];
+    }
     rc[I] = s;
     if (zc) zc[I] = T(omega) * (s * rDc[I]);  // as the first sweep from zero computes it (xload)
 }
@@ -762,14 +782,14 @@ __global__ void k_restrict(int nc, const int* mOff, const int* mList, const T* r
 void launch_restrict(hipStream_t st, int nc, const int* mOff, const int* mList, const double* res, double* rc,
                      const int* done, double* zc, const double* rDc, double omega) {
     if (nc == 0) return;
-    hipLaunchKernelGGL(k_restrict<double>, dim3((nc + 255) / 256), dim3(256), 0, st, nc, mOff, mList, res, rc, done,
-                       zc, rDc, omega);
+    hipLaunchKernelGGL(k_restrict<double>, dim3(xcd_grid((nc + 255) / 256)), dim3(256), 0, st, nc, mOff, mList, res,
+                       rc, done, zc, rDc, omega);
 }
 void launch_restrict32(hipStream_t st, int nc, const int* mOff, const int* mList, const float* res, float* rc,
                        const int* done, float* zc, const float* rDc, double omega) {
     if (nc == 0) return;
-    hipLaunchKernelGGL(k_restrict<float>, dim3((nc + 255) / 256), dim3(256), 0, st, nc, mOff, mList, res, rc, done,
-                       zc, rDc, omega);
+    hipLaunchKernelGGL(k_restrict<float>, dim3(xcd_grid((nc + 255) / 256)), dim3(256), 0, st, nc, mOff, mList, res,
+                       rc, done, zc, rDc, omega);
 }
 
 // Galerkin coarsening (GAMG.cpp:279-301) as gathers over fine contributions listed in
@@ -931,12 +951,11 @@ __global__ void k_coarse_solve(int n, const double* lu, const T* r, T* x, const 
 // column steps exchange g_k by a cross-lane broadcast. Same operations and order as
 // k_coarse_solve.
 template <class T>
-__device__ void coarse_solve64_body(int n, const double* luT, const T* r, T* x) {
+__device__ void coarse_solve64_g(int n, const double* luT, double g, T* x) {
     const int i = threadIdx.x;
     double row[64];
 #pragma unroll
     for (int k = 0; k < 64; ++k) row[k] = (k < n && i < n) ? luT[size_t(k) * n + i] : 0.0;
-    double g = i < n ? double(r[i]) : 0.0;
 #pragma unroll
     for (int k = 0; k < 64; ++k) {
         if (k < n) {
@@ -956,10 +975,41 @@ __device__ void coarse_solve64_body(int n, const double* luT, const T* r, T* x) 
 }
 
 template <class T>
+__device__ void coarse_solve64_body(int n, const double* luT, const T* r, T* x) {
+    const int i = threadIdx.x;
+    coarse_solve64_g(n, luT, i < n ? double(r[i]) : 0.0, x);
+}
+
+template <class T>
 __global__ __launch_bounds__(64) void k_coarse_solve64(int n, const double* luT, const T* r, T* x, const int* done) {
     if (done && *done) return;
     coarse_solve64_body(n, luT, r, x);
 }
+
+// The restriction into a coarsest level of ≤ 64 cells and its substitution in one
+// wavefront: lane I sums the residuals of aggregate I (ascending, as k_restrict) into
+// its right-hand side and the substitution starts from it (as k_coarse_solve64).
+template <class T>
+__global__ __launch_bounds__(64) void k_restrict_coarse64(int nc, const int* mOff, const int* mList, const T* res,
+                                                          const double* luT, T* x, const int* done) {
+    if (done && *done) return;
+    const int I = threadIdx.x;
+    T s = T(0);
+    if (I < nc) {
+        for (int k = mOff[I]; k < mOff[I + 1]; ++k) s += res[mList[k]];
+    }
+    coarse_solve64_g(nc, luT, I < nc ? double(s) : 0.0, x);
+}
+
+template <class T>
+void launch_restrict_coarse64(hipStream_t st, int nc, const int* mOff, const int* mList, const T* res,
+                              const double* luT, T* x, const int* done) {
+    hipLaunchKernelGGL(k_restrict_coarse64<T>, dim3(1), dim3(64), 0, st, nc, mOff, mList, res, luT, x, done);
+}
+template void launch_restrict_coarse64<double>(hipStream_t, int, const int*, const int*, const double*, const double*,
+                                               double*, const int*);
+template void launch_restrict_coarse64<float>(hipStream_t, int, const int*, const int*, const float*, const double*,
+                                              float*, const int*);
 
 template <class T>
 static void coarse_solve_t(hipStream_t st, int n, const double* lu, const double* luT, const T* r, T* x,
@@ -1172,26 +1222,6 @@ __device__ __forceinline__ TailLevelT<T> tail_level(const TailLevelT<T>& G, char
     return L;
 }
 
-// Global → LDS with eight loads in flight per thread before the stores.
-template <class V>
-__device__ __forceinline__ void lds_copy(char* base, int off, const V* src, int n) {
-    V* dst = reinterpret_cast<V*>(base + off);
-    constexpr int UNR = 8;
-    for (int i0 = threadIdx.x; i0 < n; i0 += UNR * blockDim.x) {
-        V v[UNR];
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            const int i = i0 + u * blockDim.x;
-            v[u] = i < n ? src[i] : V(0);
-        }
-#pragma unroll
-        for (int u = 0; u < UNR; ++u) {
-            const int i = i0 + u * blockDim.x;
-            if (i < n) dst[i] = v[u];
-        }
-    }
-}
-
 template <class T>
 __global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(TailArgsT<T> t) {
     if (t.done && *t.done) return;
@@ -1201,20 +1231,29 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(TailArgsT<T> t) {
     const int nl = t.nLevels;
     const int cur = pre_smoothed_buffer(t.nPre), freeB = 1 - cur;
     {
-        const TailLevelT<T>& L0 = t.L[0];
-        for (int i = threadIdx.x; i < L0.S.n; i += blockDim.x) ldsT[L0.ldsR + i] = t.rTop[i];
-        // matrices into LDS: every later phase then reads LDS only
-        for (int l = 0; l + 1 < nl; ++l) {
-            const TailLevelT<T>& G = t.L[l];
-            if (!G.ldsMat) continue;
-            lds_copy(base, G.ldsRowOff, G.S.sliceOff, G.nRowOff);
-            lds_copy(base, G.ldsCol, G.S.col, G.S.nslots);
-            lds_copy(base, G.ldsCoef, G.coef, G.S.nslots);
-            lds_copy(base, G.ldsDiag, G.diag, G.S.n);
-            lds_copy(base, G.ldsRD, G.rD, G.S.n);
-            if (G.agg) lds_copy(base, G.ldsAgg, G.agg, G.S.n);
-            if (G.mOff) lds_copy(base, G.ldsMOff, G.mOff, G.nNext + 1);
-            if (G.mList) lds_copy(base, G.ldsMList, G.mList, G.S.n);
+        // matrices (and the coarsest factors, and rTop when its address is fixed) into
+        // LDS: every later phase then reads LDS only. One chunk per wave at a time, all
+        // its loads in flight before its stores.
+        constexpr int PER = TAIL_CHUNK_WORDS / WAVE;
+        unsigned* lw = reinterpret_cast<unsigned*>(lds);
+        const int wave = threadIdx.x / WAVE, lane = threadIdx.x % WAVE, nw = blockDim.x / WAVE;
+        for (int c = wave; c < t.nChunks; c += nw) {
+            const TailChunk ch = t.chunks[c];
+            unsigned v[PER];
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int i = lane + u * WAVE;
+                v[u] = i < ch.n ? ch.src[i] : 0u;
+            }
+#pragma unroll
+            for (int u = 0; u < PER; ++u) {
+                const int i = lane + u * WAVE;
+                if (i < ch.n) lw[ch.dst + i] = v[u];
+            }
+        }
+        if (!t.rInChunks) {
+            const TailLevelT<T>& L0 = t.L[0];
+            for (int i = threadIdx.x; i < L0.S.n; i += blockDim.x) ldsT[L0.ldsR + i] = t.rTop[i];
         }
         __syncthreads();
     }
@@ -1260,9 +1299,7 @@ __global__ __launch_bounds__(TAIL_THREADS) void k_vcycle_tail(TailArgsT<T> t) {
         T* x = (nl == 1) ? t.xTop : ldsT + LC.ldsX;
         const int n = t.nCoarse;
         double* cl = lds;
-        if (n <= 64) {
-            for (int q = threadIdx.x; q < n * n; q += blockDim.x) cl[q] = t.luT[q];
-            __syncthreads();
+        if (n <= 64) {  // the factors were preloaded into cl
             if (threadIdx.x < WAVE) {
                 const int i = threadIdx.x;
                 double g = i < n ? double(r[i]) : 0.0;

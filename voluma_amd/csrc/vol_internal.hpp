@@ -216,13 +216,17 @@ void launch_coarse_assemble(hipStream_t st, int n, int nf, const int* l, const i
 void launch_coarse_lu(hipStream_t st, int n, double* lu, double* luT, int* status);
 void launch_coarse_solve(hipStream_t st, int n, const double* lu, const double* luT, const double* r, double* x,
                          const int* done);
+// Restriction into a coarsest level of nc ≤ 64 cells fused with its substitution.
+template <class T>
+void launch_restrict_coarse64(hipStream_t st, int nc, const int* mOff, const int* mList, const T* res,
+                              const double* luT, T* x, const int* done);
 // Same substitution in FP64 on FP32 vectors (r widened, x rounded).
 void launch_coarse_solve32(hipStream_t st, int n, const double* lu, const double* luT, const float* r, float* x,
                            const int* done);
 
 // Coarse levels of the V-cycle run inside one workgroup (k_vcycle_tail): the levels
 // from `first` down to the coarsest, with block barriers between the phases.
-constexpr int TAIL_MAX_CELLS = 512;   // levels at or below this size go to the tail kernel
+constexpr int TAIL_MAX_CELLS = 4096;  // largest first level the tail kernel takes (VOLUMA_TAIL_MAX_CELLS)
 constexpr int TAIL_THREADS = 1024;
 // The tail's vectors (r, two work buffers, x per level) live in LDS at the given
 // offsets (in doubles); only matrix data is read from global memory.
@@ -244,6 +248,16 @@ template <class T> struct TailLevelT {
     int nRowOff = 0;
     int ldsRowOff = 0, ldsCol = 0, ldsCoef = 0, ldsDiag = 0, ldsRD = 0, ldsAgg = 0, ldsMOff = 0, ldsMList = 0;
 };
+// A run of 4-byte words that the tail kernel copies from global memory into its LDS at
+// start (dst: word offset in LDS). Every wave loads a whole chunk with all its loads in
+// flight, and the waves take different chunks, so the preload costs a few memory round
+// trips instead of one per array.
+constexpr int TAIL_CHUNK_WORDS = 1024;
+struct TailChunk {
+    const unsigned* src = nullptr;
+    int dst = 0;
+    int n = 0;  // ≤ TAIL_CHUNK_WORDS
+};
 template <class T> struct TailArgsT {
     int nLevels = 0;                  // levels in the tail, the last is the coarsest
     const TailLevelT<T>* L = nullptr; // device array
@@ -256,6 +270,9 @@ template <class T> struct TailArgsT {
     int nPre = 2, nPost = 2;
     double omega = 0.8;
     const int* done = nullptr;
+    const TailChunk* chunks = nullptr;  // device array: matrices, coarsest factors (n ≤ 64), rTop
+    int nChunks = 0;
+    int rInChunks = 0;                  // rTop is among the chunks (its address is fixed)
 };
 using TailLevel = TailLevelT<double>;
 using TailArgs = TailArgsT<double>;

@@ -1,0 +1,171 @@
+"""Summaries of a rocprofv3 kernel trace of the solver (``rocprofv3 --kernel-trace -d OUT
+-o run -- python3 bench.py ...`` writes ``OUT/run_results.db``, an SQLite database whose
+``kernels`` view lists every dispatch with its grid and start/end times in ns).
+
+    python -m voluma_amd.profiling DB stats [N]        per (kernel, grid): calls, total, avg, min, max
+    python -m voluma_amd.profiling DB levels BENCH     one PCG iteration of the headline solve,
+                                                       per dispatch: level, average time over all
+                                                       iterations, bytes, GB/s (BENCH: a file
+                                                       holding bench.py's JSON line, for the
+                                                       level sizes)
+
+The ``levels`` table splits the trace into PCG iterations at the level-0 A·p launches
+(``k_sell<0, 0, true, ...>``), keeps the iterations whose dispatch sequence is the most
+common one, and walks the V-cycle's fixed order (residual and restriction down the
+levels, coarsest solve, prolongation sweeps up) to name each dispatch's level. Bytes are
+what the kernel has to move given the stored layout, without its padding: per row of a
+level the vectors it reads and writes, per off-diagonal entry a coefficient and a column
+(8 + 4 bytes in FP64), the restriction's member lists and the coarsest LU factors; the
+x/r update counts the level-0 z it writes for the V-cycle (FP64 PCG)."""
+import json
+import re
+import sqlite3
+import sys
+from collections import Counter
+
+
+def _short(name):
+    name = re.sub(r"void vol::(\(anonymous namespace\)::)?", "", name)
+    name = re.sub(r"\(.*\)$", "", name)
+    return name.replace("vol::", "")
+
+
+def dispatches(db):
+    """[(kernel, grid in workgroups, start ns, duration ns)] in start order."""
+    con = sqlite3.connect(db)
+    rows = con.execute("select name, grid_x, workgroup_x, start, duration from kernels order by start").fetchall()
+    con.close()
+    return [(_short(n), g // max(wg, 1), s, d) for n, g, wg, s, d in rows]
+
+
+def stats(db, top=30):
+    rows = dispatches(db)
+    busy = sum(r[3] for r in rows)
+    agg = {}
+    for k, g, _, d in rows:
+        a = agg.setdefault((k, g), [0, 0, 1 << 62, 0])
+        a[0] += 1
+        a[1] += d
+        a[2] = min(a[2], d)
+        a[3] = max(a[3], d)
+    out = [f"busy {busy / 1e6:.3f} ms, {len(rows)} dispatches", "",
+           "| kernel | grid (WG) | calls | total ms | avg us | min us | max us | % |", "|---|---|---|---|---|---|---|---|"]
+    for (k, g), a in sorted(agg.items(), key=lambda kv: -kv[1][1])[:top]:
+        out.append(f"| `{k}` | {g} | {a[0]} | {a[1] / 1e6:.3f} | {a[1] / a[0] / 1e3:.2f} | {a[2] / 1e3:.2f} | "
+                   f"{a[3] / 1e3:.2f} | {100 * a[1] / busy:.1f} |")
+    return "\n".join(out)
+
+
+def _template_args(kernel):
+    m = re.search(r"<(.*)>", kernel)
+    return [a.strip() for a in m.group(1).split(",")] if m else []
+
+
+def _is_amul(kernel):
+    a = _template_args(kernel)
+    return kernel.startswith("k_sell<") and len(a) >= 3 and a[0] == "0" and a[1] == "0" and a[2] == "true"
+
+
+def _iterations(rows):
+    """Dispatch sequences between consecutive level-0 A·p launches."""
+    starts = [i for i, r in enumerate(rows) if _is_amul(r[0])]
+    return [rows[a:b] for a, b in zip(starts, starts[1:])]
+
+
+def _row_bytes(op_kind, n, nnz, nc, elem):
+    """Bytes of one gather sweep over n rows with nnz off-diagonal entries."""
+    mat = elem * n + (elem + 4) * nnz  # diag + (coefficient, column) per entry
+    if op_kind == "amul":               # x, A·x, dot vector
+        return mat + 3 * elem * n
+    if op_kind == "resid":              # x (or z), r, residual
+        return mat + 3 * elem * n
+    if op_kind == "post":               # z, r, result, agg, coarse x
+        return mat + 3 * elem * n + 4 * n + elem * nc
+    return mat + 3 * elem * n
+
+
+def levels(db, bench_json):
+    cfg = None
+    with open(bench_json) as fh:  # the bench's JSON line, wherever it is in the file
+        for line in reversed(fh.read().splitlines()):
+            try:
+                cfg = json.loads(line)["config"]
+                break
+            except (ValueError, KeyError, TypeError):
+                continue
+    if cfg is None:
+        raise SystemExit(f"no bench.py JSON line in {bench_json}")
+    cells = cfg["gamg_levels"]
+    faces = cfg["gamg_level_faces"]
+    rows = dispatches(db)
+    its = _iterations(rows)
+    if not its:
+        raise SystemExit("no level-0 A·p launches in the trace")
+    shape = Counter(tuple((k, g) for k, g, _, _ in it) for it in its).most_common(1)[0][0]
+    keep = [it for it in its if tuple((k, g) for k, g, _, _ in it) == shape]
+    avg = [sum(it[i][3] for it in keep) / len(keep) / 1e3 for i in range(len(shape))]
+    nl = len(cells)
+    elem = 8
+    out = [f"{len(keep)} of {len(its)} iterations with the common sequence of {len(shape)} dispatches; "
+           f"{sum(avg):.1f} us of kernel time per iteration", "",
+           "| # | kernel | grid | level | rows | entries | avg us | MB | GB/s |", "|---|---|---|---|---|---|---|---|---|"]
+    # the V-cycle's order: residual at l, restriction l → l+1 (l = 0 …), coarsest solve,
+    # then prolongation sweeps at l = L-2 … 0 (a streaming prolongation, when split,
+    # right before the level-0 sweep)
+    level = 0
+    for i, ((k, g), us) in enumerate(zip(shape, avg)):
+        lev, nbytes, rws, ent = "", None, "", ""
+        if _is_amul(k):
+            lev, nbytes = "0", _row_bytes("amul", cells[0], 2 * faces[0], 0, elem)
+        elif k.startswith("k_sell<") or k.startswith("k_csrv<"):
+            if _template_args(k)[0] == "2":  # prolongation sweep one level up
+                level -= 1
+                kind = "post"
+            else:
+                kind = "resid"
+            nc = cells[level + 1] if level + 1 < nl else 0
+            lev, nbytes = str(level), _row_bytes(kind, cells[level], 2 * faces[level], nc, elem)
+        elif k.startswith("k_restrict"):
+            n, nc = cells[level], cells[level + 1]
+            lev = f"{level}→{level + 1}"
+            nbytes = elem * n + 4 * n + 4 * (nc + 1) + 3 * elem * nc  # res, lists; r, z, 1/diag
+            level += 1
+        elif k.startswith("k_coarse_solve"):
+            nc = cells[-1]
+            lev = f"{nl - 1}"
+            nbytes = 8 * nc * nc + 2 * elem * nc
+        elif k.startswith("k_prolong"):
+            lev = f"{level}→{level - 1}"
+            nbytes = 2 * elem * cells[level - 1] + 4 * cells[level - 1] + elem * cells[level]
+        elif k.startswith("k_vec_fast"):
+            lev = "0"
+            op = _template_args(k)[0]
+            if op == "5":    # x += αp, r −= αq (and z = ω·(r·rD)): p, q, x, r, 1/diag in; x, r, z out
+                nbytes = 8 * elem * cells[0]
+            elif op == "4":  # p = w + βp
+                nbytes = 3 * elem * cells[0]
+        if lev.isdigit():
+            rws, ent = cells[int(lev)], 2 * faces[int(lev)]
+        mb = f"{nbytes / 1e6:.1f}" if nbytes else ""
+        gbs = f"{nbytes / us / 1e3:.0f}" if nbytes else ""
+        out.append(f"| {i} | `{k}` | {g} | {lev} | {rws} | {ent} | {us:.2f} | {mb} | {gbs} |")
+    return "\n".join(out)
+
+
+def main(argv):
+    if len(argv) < 3:
+        print(__doc__)
+        return 2
+    db, mode = argv[1], argv[2]
+    if mode == "stats":
+        print(stats(db, int(argv[3]) if len(argv) > 3 else 30))
+    elif mode == "levels":
+        print(levels(db, argv[3]))
+    else:
+        print(__doc__)
+        return 2
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main(sys.argv))
