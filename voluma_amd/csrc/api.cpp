@@ -271,7 +271,8 @@ void uploadLevels(vol_mesh m, const std::vector<int>* layout) {
         if (k > 0) {  // SELL rows in adjacency (face) order: A(i, j) = upper if i is the l side
             std::vector<int> srcL(F.adjFace.size());
             for (size_t q = 0; q < srcL.size(); ++q) srcL[q] = (F.adjFace[q] << 1) | (F.adjIsL[q] ? 0 : 1);
-            D->layout = buildCoarseLayout(F.n, F.off, F.adjCell, srcL, D->sell, st, layout ? (*layout)[k] : -1);
+            D->layout = buildCoarseLayout(F.n, F.off, F.adjCell, srcL, D->sell, st, layout ? (*layout)[k] : -1,
+                                          k + 1 < nL ? &F.agg : nullptr);
         }
         if (k + 1 < nL) {
             const HLevel& C = m->H[k + 1];

@@ -418,63 +418,72 @@ inline bool buildCsr(int n, const std::vector<int>& off, const std::vector<int>&
     return true;
 }
 
-// Time the two sweeps a 1/1 V-cycle runs on a level (residual of the first sweep from
-// zero; first post-sweep with prolongation) over layout S on stream st, with the caches
-// flushed before each launch (in a V-cycle the finer levels have evicted the level's
-// data): median of three, seconds. Values are zero; agg maps four rows to one.
-inline double timeLayout(const SellDev& S, hipStream_t st, void* flush, size_t flushBytes) {
+// Time the two sweeps a 1/1 V-cycle runs on a level in the forms the solve uses (the
+// residual gathering z; the post-sweep gathering z + xc[agg]) over layout S on stream st,
+// with the caches flushed before each launch (in a V-cycle the finer levels have evicted
+// most of the level's data): median of five, seconds. Values are zero; aggDev is the
+// level's map to the next level (nullable: four consecutive rows per aggregate).
+inline double timeLayout(const SellDev& S, hipStream_t st, void* flush, size_t flushBytes, const int* aggDev) {
     DevBuf<double> coef, v;
     DevBuf<int> agg;
-    const size_t n = S.n, nc = n / 4 + 1;
+    const size_t n = S.n, nc = n;
     coef.alloc(S.nslots);
     v.alloc(5 * n + nc);
     HIPCHECK(hipMemsetAsync(coef.p, 0, sizeof(double) * std::max<size_t>(S.nslots, 1), st));
     HIPCHECK(hipMemsetAsync(v.p, 0, sizeof(double) * (5 * n + nc), st));
-    std::vector<int> ag(n);
-    for (size_t i = 0; i < n; ++i) ag[i] = int(i / 4);
-    agg.upload(ag, st);
+    if (!aggDev) {
+        std::vector<int> ag(n);
+        for (size_t i = 0; i < n; ++i) ag[i] = int(i / 4);
+        agg.upload(ag, st);
+        aggDev = agg.p;
+    }
     SellArgs a;
     a.S = S.view();
     a.coef = coef.p;
     a.diag = v.p;
     a.rD = v.p + n;
     a.r = v.p + 2 * n;
-    a.x = v.p + 3 * n;
+    a.x = v.p + 3 * n;  // z
     a.out = v.p + 4 * n;
     a.xc = v.p + 5 * n;
-    a.agg = agg.p;
+    a.agg = aggDev;
     a.omega = 0.9;
     hipEvent_t e[4];
     for (auto& x : e) HIPCHECK(hipEventCreate(&x));
-    float t[3];
-    for (int k = 0; k < 3; ++k) {
+    constexpr int REPS = 5;
+    float t[REPS];
+    for (int k = 0; k < REPS; ++k) {
         float t1 = 0, t2 = 0;
         HIPCHECK(hipMemsetAsync(flush, k + 1, flushBytes, st));
         a.evStart = e[0];
         a.evStop = e[1];
-        launch_sell(st, OP_RESID, XL_FROMZERO, a);
+        launch_sell(st, OP_RESID, XL_PLAIN, a);
         HIPCHECK(hipMemsetAsync(flush, k + 2, flushBytes, st));
         a.evStart = e[2];
         a.evStop = e[3];
-        launch_sell(st, OP_JACOBI, XL_ZEROPROLONG, a);
+        launch_sell(st, OP_JACOBI, XL_PROLONG, a);
         HIPCHECK(hipEventSynchronize(e[3]));
         HIPCHECK(hipEventElapsedTime(&t1, e[0], e[1]));
         HIPCHECK(hipEventElapsedTime(&t2, e[2], e[3]));
         t[k] = t1 + t2;
     }
     for (auto& x : e) (void)hipEventDestroy(x);
-    std::sort(t, t + 3);
-    return t[1] / 1e3;
+    std::sort(t, t + REPS);
+    return t[REPS / 2] / 1e3;
 }
 
-// Layout of a coarse GAMG level: the CSR-vector form (G lanes per row) or SELL-64,
-// whichever sweeps faster on this GPU from cold caches (measured on the two sweeps a 1/1
-// V-cycle runs on the level; both give the same results, row by row in slot order).
-// Levels above 4M rows use SELL. `choice` (0 CSR, 1 SELL) skips the measurement: copies
-// of a mesh take the choice made for the original. VOLUMA_COARSE_LAYOUT=csr / sell fixes
-// the choice. Returns the layout built (0 CSR, 1 SELL).
+// Layout of a coarse GAMG level: the CSR-vector form (G lanes per row) or SELL-64 (both
+// give the same results, row by row in slot order). The two sweeps of a 1/1 V-cycle are
+// timed in each from cold caches on this GPU, and CSR is taken when it is at least 5 %
+// faster: SELL reads one wavefront per 64 rows and measured faster inside the solve on
+// the big coarse levels when the cold timings were close. Levels above 4M rows use SELL.
+// `choice` (0 CSR, 1 SELL) skips the measurement: copies of a mesh take the choice made
+// for the original. VOLUMA_COARSE_LAYOUT=csr / sell fixes the choice. agg (nullable):
+// the level's map to the next level, for the timing. Returns the layout built (0 CSR,
+// 1 SELL).
 inline int buildCoarseLayout(int n, const std::vector<int>& off, const std::vector<int>& col,
-                             const std::vector<int>& src, SellDev& S, hipStream_t st, int choice = -1) {
+                             const std::vector<int>& src, SellDev& S, hipStream_t st, int choice = -1,
+                             const std::vector<int>* agg = nullptr) {
     constexpr int TUNE_MAX_ROWS = 1 << 22;
     const char* force = std::getenv("VOLUMA_COARSE_LAYOUT");
     const std::string f = force ? force : "";
@@ -488,7 +497,11 @@ inline int buildCoarseLayout(int n, const std::vector<int>& off, const std::vect
             const size_t fb = size_t(512) << 20;  // larger than L2 + the 256 MB Infinity Cache
             DevBuf<char> flush;
             flush.alloc(fb);
-            if (timeLayout(a, st, flush.p, fb) <= timeLayout(b, st, flush.p, fb)) choice = 0;
+            DevBuf<int> aggD;
+            if (agg && int(agg->size()) == n) aggD.upload(*agg, st);
+            const double tc = timeLayout(a, st, flush.p, fb, aggD.p);
+            const double ts = timeLayout(b, st, flush.p, fb, aggD.p);
+            if (tc < 0.95 * ts) choice = 0;
         }
     }
     if (choice == 0 && buildCsr(n, off, col, src, S, st)) return 0;
