@@ -136,6 +136,38 @@ def batch_leg(va, W, ctx, args):
     return out
 
 
+def amul_cache_states(va, ctx, lm, A, n):
+    """Level-0 A·x (unfused, vol_amul) timed by events around the kernel in two cache
+    states: rotating over three copies of the matrix and of x, y, so that each launch
+    finds its data evicted from the 256 MB Infinity Cache by the two launches before it
+    (DRAM-bound), and the same copy back to back (warm). Returns (dram_us, warm_us)."""
+    from voluma_amd import workloads as W
+
+    d_h, u_h, _ = A.get()
+    mats = [A] + [va.LduMatrix(lm) for _ in range(2)]
+    for M in mats[1:]:
+        M.set(d_h, u_h)
+    xs = [va.DeviceArray.from_host(ctx, W.splitmix_uniform(n, 50 + k)) for k in range(3)]
+    ys = [va.DeviceArray(ctx, n) for _ in range(3)]
+    for k in range(3):
+        va.amul(mats[k], xs[k], ys[k])
+    ctx.timing(True)
+    ctx.read_timing(reset=True)
+    for _ in range(20):
+        for k in range(3):
+            va.amul(mats[k], xs[k], ys[k])
+    ms_d, cnt_d = ctx.read_timing(reset=True)
+    for _ in range(60):
+        va.amul(A, xs[0], ys[0])
+    ms_w, cnt_w = ctx.read_timing(reset=True)
+    ctx.timing(False)
+    for a in xs + ys:
+        a.free()
+    for M in mats[1:]:
+        M.close()
+    return ms_d / cnt_d * 1e3, ms_w / cnt_w * 1e3
+
+
 def serve_leg(va, ctx, args):
     """Solve server: 4 workers (own HIP streams), 200 GB HBM budget, a pool of 8 meshes
     with log-uniform sizes 10k–2M cells; Poisson arrivals at --serve-rate of 60 %
@@ -170,6 +202,10 @@ def serve_leg(va, ctx, args):
         out[name] = st
         log(f"serve {name}: p50 {st['latency_ms']['p50']:.1f} ms, p99 {st['latency_ms']['p99']:.1f} ms at "
             f"{args.serve_rate:g} req/s; {st['solves_per_s']:.0f} solves/s; saturated {st_sat['solves_per_s']:.0f}")
+        for kind, v in st["by_kind"].items():
+            log(f"  {kind} ({v['requests']}): p50 {v['latency_ms']['p50']:.1f} p99 {v['latency_ms']['p99']:.1f} ms; "
+                f"queued p99 {v['queued_ms']['p99']:.1f}, running p50 {v['running_ms']['p50']:.1f} "
+                f"p99 {v['running_ms']['p99']:.1f} ms")
     server.close()
     return out
 
@@ -473,6 +509,7 @@ def main():
     setup_ms = (float(np.median(t_set)) - float(np.median(lat))) * 1e3
     dd.free()
     du.free()
+    amul_dram_us, amul_warm_us = amul_cache_states(va, ctx, lm, A, n)
     amul_bytes = 24 * n + 16 * nf
     amul_avg_s = amul_ms / max(amul_cnt, 1) / 1e3
     achieved = amul_bytes / amul_avg_s / 1e9 if amul_cnt else None
@@ -566,7 +603,11 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": None if achieved is None else achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "k_sell<OP_AMUL> level-0 A·p", "algorithmic_bytes": amul_bytes,
-                     "avg_us": amul_avg_s * 1e6, "launches": amul_cnt, "stream_copy_gbs": stream_gbs},
+                     "avg_us": amul_avg_s * 1e6, "launches": amul_cnt, "stream_copy_gbs": stream_gbs,
+                     "traffic_kind": "FETCH_SIZE/WRITE_SIZE counter bytes between L2 and the fabric per launch "
+                                     "(Infinity Cache hits included)",
+                     "dram_us": amul_dram_us, "warm_us": amul_warm_us,
+                     "dram_gbs": None if traffic is None else traffic / amul_dram_us / 1e3},
     }
     if not args.no_cpu_baseline:
         log("CPU baseline (oracle, 1 thread) ...")

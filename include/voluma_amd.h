@@ -468,10 +468,10 @@ int vol_batch_solve(vol_batch batch, const double* b, double* x, const vol_solve
  * estimate until its first request ends, then at what its allocations measured.
  * A request names a mesh registered with vol_server_add_mesh and carries one matrix and
  * nRhs right-hand sides (e.g. PISO correctors) on that matrix, solved with `controls`.
- * Pressure requests (PCG + GAMG, damped-Jacobi V-cycle, fast sums) run as systems of the
- * worker's batch of copies, together with the requests queued behind them for the same
- * mesh and controls (at most 4 right-hand sides); other requests solve their
- * right-hand sides one after another. All request buffers are host memory owned by the
+ * Pressure requests (PCG + GAMG, damped-Jacobi V-cycle, fast sums) on meshes of at most
+ * 262144 cells run as systems of the worker's batch of copies, together with the requests
+ * queued behind them for the same mesh and controls (at most 4 right-hand sides); other
+ * requests solve their right-hand sides one after another. All request buffers are host memory owned by the
  * caller and must stay valid until vol_server_wait returns for the ticket. A request's
  * result does not depend on what else runs. */
 typedef struct {
@@ -503,7 +503,8 @@ int vol_server_stats(vol_server s, uint64_t* bytesInUse, int* queued, int* cache
 int vol_server_destroy(vol_server s); /* stops the workers after their current request */
 
 /* ---- measurement hooks (no reference counterpart): time every level-0 A·p
- * product issued by vol_solve with HIP events on the context stream. */
+ * product issued by vol_solve or vol_amul with HIP events on the context stream
+ * (recorded by the launch itself, so they bracket the kernel only). */
 int vol_timing_enable(vol_context ctx, int on);
 int vol_timing_read(vol_context ctx, double* amulTotalMs, int64_t* amulCount, int reset);
 

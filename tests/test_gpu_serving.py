@@ -26,9 +26,9 @@ def pool(ctx):
     return SV.build_pool(ctx, SV.pool_specs(n_meshes=3, seed=5, n_min=2000, n_max=30000))
 
 
-def _est(e):  # the server's admission estimate per resident mesh copy + matrix + batch of 4 copies
+def _est(e):  # the server's admission estimate per resident mesh copy + matrix (+ batch of 4 copies)
     m = e["mesh"]
-    return (m.n_cells * 400 + m.n_internal * 96 + (1 << 20)) * 5
+    return (m.n_cells * 400 + m.n_internal * 96 + (1 << 20)) * (5 if m.n_cells <= 1 << 18 else 1)
 
 
 def _alone(server, pool, stream, precision):

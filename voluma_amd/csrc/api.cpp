@@ -2822,9 +2822,10 @@ int vol_amul(vol_matrix A, const double* x, double* y, int location) {
         dy = A->hy.p;
     }
     Ctx C{c, st, nullptr, c->scal.p};
-    sell<double>(C, OP_AMUL, XL_PLAIN, m->sell.view(), A->coef.p, A->diag.p, dx, nullptr, dy);
+    amulTimed<double>(C, A, dx, dy, nullptr, false);  // timed when timing is on
     if (location != VOL_DEVICE) HIPCHECK(hipMemcpyAsync(y, dy, sizeof(double) * n, hipMemcpyDeviceToHost, st));
     HIPCHECK(hipStreamSynchronize(st));
+    collectTiming(c);
     API_END
 }
 

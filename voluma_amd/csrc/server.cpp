@@ -9,15 +9,17 @@
 // cached per (worker, mesh) and evicted, least recently used first, when memory is
 // needed.
 //
-// Continuous batching: pressure requests (PCG + GAMG with the damped-Jacobi V-cycle)
-// are solved as systems of a batch of BATCH_SYSTEMS copies of their mesh (vol_batch,
-// the mesh's hierarchy reused): a worker that takes one such request also takes the
-// requests queued behind it for the same mesh with the same controls, as many as the
-// free systems hold, and their right-hand sides run in the same launches; systems left
-// over get b = 0 and finish at once. A system's arithmetic in a batch does not depend
-// on its batch mates or its position, so a request's result is the same bits alone and
-// in the mix. Other requests (PBiCGStab momentum solves) run one right-hand side after
-// the other with vol_solve.
+// Continuous batching: pressure requests (PCG + GAMG with the damped-Jacobi V-cycle) on
+// meshes of at most BATCH_MAX_CELLS cells are solved as systems of a batch of
+// BATCH_SYSTEMS copies of their mesh (vol_batch, the mesh's hierarchy reused): a worker
+// that takes one such request also takes the requests queued behind it for the same mesh
+// with the same controls, as many as the free systems hold, and their right-hand sides
+// run in the same launches; systems left over get b = 0 and finish at once. A system's
+// arithmetic in a batch does not depend on its batch mates or its position, so a
+// request's result is the same bits alone and in the mix. Other requests (PBiCGStab
+// momentum solves, pressure on larger meshes, which fill the GPU on their own and run
+// faster through vol_solve's fused iteration) solve their right-hand sides one after the
+// other with vol_solve.
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -35,11 +37,12 @@ namespace {
 struct ProtoMesh {
     vol_mesh mesh = nullptr;  // on the server's setup context; workers use copies
     int n = 0, nf = 0;
-    bool gamg = false;        // has a hierarchy: its pressure requests are batched
+    bool batched = false;     // has a hierarchy and ≤ BATCH_MAX_CELLS cells: pressure requests batch
     uint64_t bytes = 0;       // device estimate of one resident copy + matrix (+ batch)
 };
 
-constexpr int BATCH_SYSTEMS = 4;  // right-hand sides a worker solves together per mesh
+constexpr int BATCH_SYSTEMS = 4;           // right-hand sides a worker solves together per mesh
+constexpr int BATCH_MAX_CELLS = 1 << 18;  // larger meshes: one right-hand side at a time
 
 struct CacheEntry {
     vol_mesh mesh = nullptr;  // this worker's copy
@@ -219,14 +222,14 @@ void runBatched(Worker& w, CacheEntry* e, const std::vector<Req*>& group) {
     HIPCHECK(hipStreamSynchronize(st));
 }
 
-// A worker's resident copy of its mesh: the copy, matrix storage and, for a mesh with
-// a hierarchy, the batch of copies its pressure requests run in.
-void prefetch(Worker& w, CacheEntry* e, bool gamg, int n) {
+// A worker's resident copy of its mesh: the copy, matrix storage and, for a mesh whose
+// pressure requests batch, the batch of copies they run in.
+void prefetch(Worker& w, CacheEntry* e, bool batchMesh, int n) {
     if (!e->A) {
         const int rc = vol_matrix_create(e->mesh, &e->A);
         if (rc != VOL_OK) throw VolError(rc, vol_last_error());
     }
-    if (gamg && !e->batch) {
+    if (batchMesh && !e->batch) {
         e->batch = batchOfCopies(w.ctx, e->mesh, BATCH_SYSTEMS);
         e->bb.alloc(size_t(n) * BATCH_SYSTEMS);
         e->bx.alloc(size_t(n) * BATCH_SYSTEMS);
@@ -236,7 +239,7 @@ void prefetch(Worker& w, CacheEntry* e, bool gamg, int n) {
 void runRequest(vol_server S, Worker& w, const std::vector<Req*>& group, bool batched) {
     Req* q = group[0];
     const int64_t bytes0 = tlDeviceBytes;
-    bool gamg;
+    bool batchMesh;
     int nCells;
     CacheEntry* e;
     vol_mesh proto;
@@ -244,7 +247,7 @@ void runRequest(vol_server S, Worker& w, const std::vector<Req*>& group, bool ba
         std::lock_guard<std::mutex> lk(S->mu);
         e = &w.cache[q->r.mesh];
         proto = S->meshes[q->r.mesh].mesh;
-        gamg = S->meshes[q->r.mesh].gamg;
+        batchMesh = S->meshes[q->r.mesh].batched;
         nCells = S->meshes[q->r.mesh].n;
     }
     int rc = VOL_OK;
@@ -259,7 +262,7 @@ void runRequest(vol_server S, Worker& w, const std::vector<Req*>& group, bool ba
     try {
         if (!e->mesh) e->mesh = meshCloneTo(proto, w.ctx);
         if (q->prefetchWorker >= 0) {
-            prefetch(w, e, gamg, nCells);
+            prefetch(w, e, batchMesh, nCells);
         } else if (batched) {
             runBatched(w, e, group);
         } else {
@@ -303,7 +306,7 @@ void workerLoop(vol_server S, int wi) {
                 S->cvWork.wait(lk);
             }
             Req* q = group[0];
-            batched = q->prefetchWorker < 0 && batchable(q, S->meshes[q->r.mesh].gamg) && q->r.nRhs <= BATCH_SYSTEMS;
+            batched = q->prefetchWorker < 0 && batchable(q, S->meshes[q->r.mesh].batched) && q->r.nRhs <= BATCH_SYSTEMS;
             if (batched) {  // the requests queued behind it that share its solve join the batch
                 int used = q->r.nRhs;
                 for (auto it = S->queue.begin(); it != S->queue.end();) {
@@ -368,9 +371,9 @@ int vol_server_add_mesh(vol_server S, int nCells, int nInternalFaces, const int3
     }
     p.n = nCells;
     p.nf = nInternalFaces;
-    p.gamg = gamgWeights != nullptr;
+    p.batched = gamgWeights != nullptr && nCells <= BATCH_MAX_CELLS;
     // one resident copy + matrix, and for pressure requests the batch of copies
-    p.bytes = estimateBytes(nCells, nInternalFaces) * (p.gamg ? 1 + BATCH_SYSTEMS : 1);
+    p.bytes = estimateBytes(nCells, nInternalFaces) * (p.batched ? 1 + BATCH_SYSTEMS : 1);
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->registered + p.bytes / 2 > S->budget) {
         vol_mesh_destroy(p.mesh);

@@ -132,9 +132,23 @@ def run_stream(server, pool, stream, precision, timeout=600.0):
     # behind the scheduled arrival
     lat = np.array([service[k]["finished"] - service[k]["queued"] + max(0.0, submitted[k] - (t0 + stream[k]["t"]))
                     for k in range(len(stream))])
+    wait = np.array([service[k]["started"] - service[k]["queued"] for k in range(len(stream))])
+    run = np.array([service[k]["finished"] - service[k]["started"] for k in range(len(stream))])
     n_solves = sum(rq["nrhs"] for rq in stream)
+
+    def pct(v):
+        return {"p50": float(np.percentile(v, 50) * 1e3), "p99": float(np.percentile(v, 99) * 1e3),
+                "max": float(v.max() * 1e3)}
+
+    # where the time goes: per request kind, and for the largest mesh, queueing and solving
+    big = max(range(len(pool)), key=lambda i: pool[i]["n"])
+    parts = {}
+    for name, sel in (("pressure", [rq["kind"] == "pressure" for rq in stream]),
+                      ("momentum", [rq["kind"] == "momentum" for rq in stream]),
+                      ("largest_mesh", [rq["pool"] == big for rq in stream])):
+        sel = np.array(sel)
+        if sel.any():
+            parts[name] = {"requests": int(sel.sum()), "latency_ms": pct(lat[sel]), "queued_ms": pct(wait[sel]),
+                           "running_ms": pct(run[sel])}
     return results, {"requests": len(stream), "solves": n_solves, "wall_s": wall, "solves_per_s": n_solves / wall,
-                     "requests_per_s": len(stream) / wall,
-                     "latency_ms": {"p50": float(np.percentile(lat, 50) * 1e3),
-                                    "p99": float(np.percentile(lat, 99) * 1e3),
-                                    "max": float(lat.max() * 1e3)}}
+                     "requests_per_s": len(stream) / wall, "latency_ms": pct(lat), "by_kind": parts}
