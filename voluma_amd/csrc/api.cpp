@@ -185,6 +185,7 @@ struct vol_matrix_s {
     DevBuf<float> coef32, diag32, rD32;
     DevBuf<TailLevel32> tail32;
     DevBuf<float> ir, iw, ip, iq, ie, ir0, iy, iAy, is, iz, it;  // inner-solve vectors
+    DevBuf<float> iz0;  // inner PCG: ω·(r·rD) kept with the inner residual
     // solve work vectors
     DevBuf<double> vx, vb, vr, vw, vp, vq, wA, pA, r0, y, Ay, s, z, t;
     DevBuf<double> hx, hy;  // staging for host-pointer calls
@@ -1664,10 +1665,22 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
                             A->it.p};
             DevScalars* SI = c->scal.p + 1;
             Ctx Ci{c, st, &SI->done, SI};
-            // inner right-hand side: the FP64 residual rounded; correction e = 0
-            fin(C, FIN_INNER_START, 0, false, 0);
-            launch_mix(st, MIX_TO32, n, A->vr.p, A->ir.p, isBi ? A->ir0.p : nullptr, A->ie.p, nullptr, nullptr,
-                       C.done);
+            // the inner PCG with the fused Jacobi V-cycle keeps z0 = ω·(r·rD) with r, as
+            // the FP64 one does (set from each inner right-hand side, then by x/r updates)
+            const bool useZ0f = isPCG && P.precond == "GAMG" && !P.gamgGS && P.nPre == 1 && c->level0Z;
+            if (useZ0f) {
+                A->iz0.alloc(n);
+                k.z0 = A->iz0.p;
+            }
+            auto innerRhs = [&] {  // inner right-hand side: the FP64 residual rounded; e = 0
+                fin(C, FIN_INNER_START, 0, false, 0);
+                launch_mix(st, MIX_TO32, n, A->vr.p, A->ir.p, isBi ? A->ir0.p : nullptr, A->ie.p, nullptr, nullptr,
+                           C.done);
+                if (useZ0f)
+                    vec<float>(C, V_ZSCALE, n, false, A->ir.p, nullptr, nullptr, nullptr, A->iz0.p, nullptr, nullptr,
+                               A->rD32.p, P.omega);
+            };
+            innerRhs();
             for (int outer = 0; outer < maxLoops; ++outer) {
                 runLoop(c, Ci, maxLoops, [&] { krylovIteration<float>(Ci, A, P, k, isPCG, false); });
                 // x += e; r = b − A·x in FP64; convergence; next inner solve
@@ -1675,9 +1688,7 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
                 sell<double>(C, OP_AMUL, XL_PLAIN, V, A->coef.p, A->diag.p, x, nullptr, A->wA.p);
                 vec<double>(C, V_BMINUSAX_NORM, n, false, A->wA.p, b, nullptr, nullptr, A->vr.p);
                 fin(C, FIN_OUTER, 1, false, RED_BLOCKS);
-                fin(C, FIN_INNER_START, 0, false, 0);
-                launch_mix(st, MIX_TO32, n, A->vr.p, A->ir.p, isBi ? A->ir0.p : nullptr, A->ie.p, nullptr, nullptr,
-                           C.done);
+                innerRhs();
                 readScalars(c, C.S, hs);
                 if (hs.done) break;
             }

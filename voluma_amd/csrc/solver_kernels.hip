@@ -373,6 +373,9 @@ __device__ __forceinline__ void vec_elem(const VecArgsT<T>& A, int i, double& r0
         A.w[i] = T(A.omega * (ld(A.a, i) / ld(A.b, i)));
     } else if constexpr (OP == V_FILL_XREF) {
         A.w[i] = T(S->xRef);
+    } else if constexpr (OP == V_ZSCALE) {  // as xload's first sweep from zero computes it
+        const T r = A.a[i];
+        A.w[i] = T(A.omega) * (r * A.rD[i]);
     }
 }
 
@@ -475,6 +478,7 @@ static void launch_vec_t(hipStream_t st, int op, const VecArgsT<T>& a, bool exac
     case V_COPY: vec_dispatch<V_COPY>(st, a, exact, R); break;
     case V_RECIP: vec_dispatch<V_RECIP>(st, a, exact, R); break;
     case V_FROMZERO: vec_dispatch<V_FROMZERO>(st, a, exact, R); break;
+    case V_ZSCALE: vec_dispatch<V_ZSCALE>(st, a, exact, R); break;
     }
 }
 

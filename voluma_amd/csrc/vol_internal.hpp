@@ -164,6 +164,7 @@ enum VecOp : int {
     V_RECIP = 14,         // w = 1/a
     V_FROMZERO = 15,      // w = ω·(r/d)
     V_FILL_XREF = 16,     // w = xRef of the row's system (batches)
+    V_ZSCALE = 17,        // w = ω·(a·rD) in the vector type (the first Jacobi sweep from zero)
 };
 
 // Vector element type T (double, or float for the inner solve of mixed precision);
