@@ -186,6 +186,8 @@ struct vol_matrix_s {
     DevBuf<TailLevel32> tail32;
     DevBuf<float> ir, iw, ip, iq, ie, ir0, iy, iAy, is, iz, it;  // inner-solve vectors
     DevBuf<float> iz0;  // inner PCG: ω·(r·rD) kept with the inner residual
+    DevBuf<double> diluTmp;  // sync-free DILU: the forward substitution's result
+    DevBuf<float> diluTmp32;
     // solve work vectors
     DevBuf<double> vx, vb, vr, vw, vp, vq, wA, pA, r0, y, Ay, s, z, t;
     DevBuf<double> hx, hy;  // staging for host-pointer calls
@@ -1367,6 +1369,33 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
     }
 }
 
+// After a solve with sync-free sweeps (stream synchronised): a wait that ran past its
+// limit voids the results.
+void checkSweepFault(vol_context c, hipStream_t st) {
+    int f = 0;
+    HIPCHECK(hipMemcpyAsync(&f, c->sweepFault.p, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPCHECK(hipStreamSynchronize(st));
+    if (f) {
+        HIPCHECK(hipMemsetAsync(c->sweepFault.p, 0, sizeof(int), st));
+        HIPCHECK(hipStreamSynchronize(st));
+        throw VolError(VOL_ERR_HIP, "sync-free sweep: a wait for a neighbour's value timed out");
+    }
+}
+
+// `sweeps` sync-free Gauss–Seidel sweeps of x (in place from the caller's view):
+// alternately x → tmp and tmp → x, and a copy back after an odd count.
+template <class T>
+void gsSweepsFree(const Ctx& C, const LevelEll& E, bool backward, const T* coef, const T* diag, const T* b, T* x, T* tmp,
+                  int n, int sweeps) {
+    T* src = x;
+    T* dst = tmp;
+    for (int s = 0; s < sweeps; ++s) {
+        launch_gs_sweep_free<T>(C.st, E, backward, coef, diag, b, src, dst, C.c->sweepFault.p, C.done);
+        std::swap(src, dst);
+    }
+    if (src != x) vec<T>(C, V_COPY, n, false, src, nullptr, nullptr, nullptr, x);
+}
+
 // One V-cycle at level l with the reference's Gauss–Seidel smoother (GAMG.cpp:403-443):
 // x = 0, nPre forward sweeps, residual r − A·x, restriction, recursion, x += xc[agg],
 // nPost backward sweeps. x is xout itself.
@@ -1392,7 +1421,9 @@ template <class T> void vcycleGS(const Ctx& C, vol_matrix A, const Plan& P, int 
         cB = l == 0 ? A->gsCB.p : A->L[l]->gsCB.p;
     }
     launch_fill_zero<T>(C.st, n, xout, C.done);
-    if constexpr (std::is_same_v<T, double>) {  // the sweeps as one pipelined schedule
+    if (C.c->freeSweeps) {  // sync-free sweeps alternating between xout and bufB
+        gsSweepsFree<T>(C, sF, false, cF, L.diag, r, xout, L.bufB, n, P.nPre);
+    } else if constexpr (std::is_same_v<T, double>) {  // the sweeps as one pipelined schedule
         if (P.nPre > 1) {
             launch_gs_sweep<T>(C.st, gsMultiEll(m, l, false, P.nPre).view(), gsMultiCoef(A, l, false, P.nPre),
                                L.diag, r, xout, C.done);
@@ -1408,7 +1439,9 @@ template <class T> void vcycleGS(const Ctx& C, vol_matrix A, const Plan& P, int 
     restrictLevel(C, m->lv[l + 1]->n, D, L.bufA, LC.r);
     vcycleGS<T>(C, A, P, l + 1, LC.r, LC.x);
     launch_prolong<T>(C.st, n, D.agg.p, LC.x, xout, C.done);
-    if constexpr (std::is_same_v<T, double>) {
+    if (C.c->freeSweeps) {
+        gsSweepsFree<T>(C, sB, true, cB, L.diag, r, xout, L.bufB, n, P.nPost);
+    } else if constexpr (std::is_same_v<T, double>) {
         if (P.nPost > 1) {
             launch_gs_sweep<T>(C.st, gsMultiEll(m, l, true, P.nPost).view(), gsMultiCoef(A, l, true, P.nPost),
                                L.diag, r, xout, C.done);
@@ -1438,7 +1471,20 @@ void precondApply(const Ctx& C, vol_matrix A, const Plan& P, const T* r, T* w, i
             ef = A->dluEf.p;
             eb = A->dluEb.p;
         }
-        launch_dilu_apply<T>(C.st, m->diluF.view(), m->diluB.view(), rD, ef, eb, r, w, C.done);
+        if (C.c->freeSweeps) {
+            T* tmp;
+            if constexpr (std::is_same_v<T, float>) {
+                A->diluTmp32.alloc(n);
+                tmp = A->diluTmp32.p;
+            } else {
+                A->diluTmp.alloc(n);
+                tmp = A->diluTmp.p;
+            }
+            launch_dilu_apply_free<T>(C.st, m->diluF.view(), m->diluB.view(), rD, ef, eb, r, tmp, w,
+                                      C.c->sweepFault.p, C.done);
+        } else {
+            launch_dilu_apply<T>(C.st, m->diluF.view(), m->diluB.view(), rD, ef, eb, r, w, C.done);
+        }
     } else
         vec<T>(C, V_SCALE, n, false, r, nullptr, nullptr, nullptr, w, nullptr, nullptr,
                P.precond == "diagonal" ? level0(A, (T*)nullptr).rD : nullptr);
@@ -1730,6 +1776,17 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
             const bool sym = P.solver == "symGaussSeidel";
             runLoop(c, C, maxLoops, [&] {
                 for (int sw = 0; sw < ct.nSweeps; ++sw) {
+                    if (isGS && c->freeSweeps) {  // x → vw (forward), then vw → x (backward) or a copy
+                        int* f = c->sweepFault.p;
+                        launch_gs_sweep_free<double>(st, m->gsF.view(), false, A->gsCF.p, A->diag.p, b, x, A->vw.p, f,
+                                                     C.done);
+                        if (sym)
+                            launch_gs_sweep_free<double>(st, m->gsB.view(), true, A->gsCB.p, A->diag.p, b, A->vw.p, x,
+                                                         f, C.done);
+                        else
+                            vec<double>(C, V_COPY, n, false, A->vw.p, nullptr, nullptr, nullptr, x);
+                        continue;
+                    }
                     if (isGS) {
                         launch_gs_sweep<double>(st, m->gsF.view(), A->gsCF.p, A->diag.p, b, x, C.done);
                         if (sym) launch_gs_sweep<double>(st, m->gsB.view(), A->gsCB.p, A->diag.p, b, x, C.done);
@@ -1746,6 +1803,7 @@ void solveDevice(vol_matrix A, const double* b, double* x, const vol_solver_cont
         readScalars(c, C.S, hs);
         if (!hs.done) throw VolError(VOL_ERR_HIP, "solver loop did not terminate");
     }
+    if (c->freeSweeps && (isGS || P.precond == "DIC" || P.precond == "DILU" || P.gamgGS)) checkSweepFault(c, st);
     collectTiming(c);
     perf.initialResidual = hs.initialResidual;
     perf.finalResidual = hs.finalResidual;
@@ -1943,6 +2001,9 @@ int vol_context_create(int device, vol_context* out) {
     c->coarseZ = !envOff("VOLUMA_COARSE_Z");
     c->level0Z = !envOff("VOLUMA_LEVEL0_Z");
     if (const char* e = std::getenv("VOLUMA_PROLONG_SPLIT")) c->prolongSplit = std::string(e) == "1";
+    if (const char* e = std::getenv("VOLUMA_SWEEP")) c->freeSweeps = std::string(e) == "free";
+    c->sweepFault.alloc(2);  // fault flag, work counter
+    HIPCHECK(hipMemsetAsync(c->sweepFault.p, 0, 2 * sizeof(int), c->stream));
     c->scal.alloc(2);
     HIPCHECK(hipHostMalloc((void**)&c->hostScal, 2 * sizeof(DevScalars), hipHostMallocDefault));
     c->ensureScratch(RED_BLOCKS);

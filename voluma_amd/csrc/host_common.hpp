@@ -111,6 +111,9 @@ struct LevelEllDev {
     std::vector<int> off, width, base, grp, runOff, runCnt;
     std::vector<char> kind;
     DevBuf<int> cell, slot, runCells, col, src, offD, widthD, baseD;
+    DevBuf<int2> waves;  // sync-free sweeps: (level, first row) per wavefront
+    DevBuf<int> runBaseD;  // per level: first runCells entry of its run (−1: not in a run)
+    int nWaves = 0;
     size_t nEntries = 0;
     void build(const std::vector<int>& levOff, const std::vector<int>& perm, const std::vector<int>& rowOff,
                const std::vector<int>& rowCol, const std::vector<int>& rowSrc, hipStream_t st) {
@@ -129,6 +132,13 @@ struct LevelEllDev {
         }
         if (total >= (size_t(1) << 31)) throw VolError(VOL_ERR_INVALID_ARGUMENT, "sweep rows exceed 2^31 entries");
         nEntries = total;
+        {  // wavefronts of one level each, in level order (the sync-free sweeps' work order)
+            std::vector<int2> wv;
+            for (int L = 0; L < nl; ++L)
+                for (int t0 = 0; t0 < off[L + 1] - off[L]; t0 += WAVE) wv.push_back(make_int2(L, t0));
+            nWaves = int(wv.size());
+            waves.upload(wv, st);
+        }
         // at least one entry, so a sweep's stand-in reads of entry 0 stay in bounds
         std::vector<int> ec(std::max<size_t>(total, 1), -1), es(total, -1);
         for (int L = 0; L < nl; ++L) {
@@ -192,6 +202,7 @@ struct LevelEllDev {
         // as a grid level); slots in first-appearance order (a row's cell, then its
         // neighbours), run rows' entries and own cells re-coded as slots
         std::vector<int> slotv(std::max<size_t>(perm.size(), 1), 0), cells;
+        std::vector<int> runBase(std::max(nl, 1), -1);  // entries of run levels hold slots
         {
             std::vector<int> g2, ro, rn;
             std::vector<char> k2;
@@ -238,6 +249,7 @@ struct LevelEllDev {
                     close(L, kind[g], start, int(cells.size()) - start);
                     for (int Lq = L; Lq < R; ++Lq) {
                         const int size = off[Lq + 1] - off[Lq];
+                        runBase[Lq] = start;
                         for (int t = 0; t < size; ++t) {
                             const int c = perm[off[Lq] + t];
                             slotv[off[Lq] + t] = map.at(c);
@@ -260,6 +272,7 @@ struct LevelEllDev {
         slot.upload(slotv, st);
         if (cells.empty()) cells.push_back(0);
         runCells.upload(cells, st);
+        runBaseD.upload(runBase, st);
         col.upload(ec, st);
         src.upload(es, st);
         offD.upload(off, st);
@@ -285,6 +298,9 @@ struct LevelEllDev {
         v.grpH = grp.data();
         v.grpKindH = kind.data();
         v.nGrp = int(grp.size()) - 1;
+        v.waves = waves.p;
+        v.nWaves = nWaves;
+        v.runBaseD = runBaseD.p;
         return v;
     }
 };
@@ -782,6 +798,12 @@ struct vol_context_s {
     // gathering it per neighbour (measured slower on the 2.06M-cell tets: 24.75 against
     // 24.60 ms per solve)
     bool coarseZ = true, level0Z = true, prolongSplit = false;
+    // triangular sweeps (Gauss–Seidel, DIC/DILU substitutions) in their sync-free form,
+    // one launch per sweep (VOLUMA_SWEEP=free; the same bits; measured slower than the
+    // default level-scheduled launches); sweepFault: [0] set by a sync-free wait that ran
+    // past its limit, [1] the sync-free sweeps' work counter
+    bool freeSweeps = false;
+    DevBuf<int> sweepFault;
     RedScratch red;
     DevBuf<double> part, result;
     DevBuf<long long> xpart;

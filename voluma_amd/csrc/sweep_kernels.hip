@@ -378,6 +378,172 @@ template void launch_gs_sweep<double>(hipStream_t, const LevelEll&, const double
 template void launch_gs_sweep<float>(hipStream_t, const LevelEll&, const float*, const float*, const float*, float*,
                                      const int*);
 
+// ---------------------------------------------------------------- sync-free sweeps
+// One launch per sweep. The work items are the wavefronts of E.waves (64 rows of one
+// level each, in level order), and a wavefront takes the next item from a counter when
+// it starts, not from its block index: the order in which workgroups start is not the
+// order of their indices across the XCDs. Every value a row waits for then belongs to an
+// item taken earlier, by a wavefront that has started and is resident or finished, and
+// never to one of its own lanes, so the earliest unfinished item can always proceed. A row reads the new value of a neighbour on the sweep's side (GS forward and
+// the DILU forward substitution: lower-numbered; backward: higher-numbered) from xNew,
+// which starts filled with a sentinel NaN that arithmetic does not produce, and waits
+// until it is no longer the sentinel; other neighbours' values come from xOld. Values
+// are published by an atomic exchange (a read-modify-write is performed where all XCDs
+// see it; a plain relaxed store may stay in the writer's L2) and read with device-scope
+// atomic loads: the value is its own flag, so no fence is needed. A wait that runs past
+// its limit sets *fault, and once it is set no wait spins any more, so every launch ends
+// soon. Row arithmetic and order are row_finish's.
+constexpr int FREE_GROUP = 8;                    // entries whose loads go out together
+constexpr unsigned FREE_SPIN_LIMIT = 1u << 18;   // ≈ 10–20 ms of s_sleep(1) per value
+
+template <class T> struct Sentinel;
+template <> struct Sentinel<double> {
+    static constexpr unsigned long long bits = 0x7FF4A5A5DEAD0001ull;  // signalling NaN
+    __device__ static bool is(double v) { return (unsigned long long)__double_as_longlong(v) == bits; }
+    __device__ static double value() { return __longlong_as_double((long long)bits); }
+};
+template <> struct Sentinel<float> {
+    static constexpr unsigned bits = 0x7FA5DEADu;  // signalling NaN
+    __device__ static bool is(float v) { return (unsigned)__float_as_uint(v) == bits; }
+    __device__ static float value() { return __uint_as_float(bits); }
+};
+
+// xNew ← sentinel, and the sweep's work counter ← 0 (stream-ordered before the sweep).
+template <class T>
+__global__ void k_fill_sentinel(int n, T* x, int* counter, const int* done) {
+    if (done && *done) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) x[i] = Sentinel<T>::value();
+    if (i == 0) *counter = 0;
+}
+
+template <class T>
+__device__ __forceinline__ T free_wait(T* xNew, int j, int* fault) {
+    T v = __hip_atomic_load(&xNew[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned spins = 0;
+    while (Sentinel<T>::is(v)) {
+        if ((spins & 1023) == 0 && __hip_atomic_load(fault, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) break;
+        __builtin_amdgcn_s_sleep(1);
+        v = __hip_atomic_load(&xNew[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (++spins > FREE_SPIN_LIMIT) {
+            __hip_atomic_fetch_or(fault, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+        }
+    }
+    return v;
+}
+
+template <int OP, bool BACK, class T>
+__global__ __launch_bounds__(SWEEP_THREADS) void k_ell_free(LevelEll E, SweepArgs<T> a, const T* xOld, T* xNew,
+                                                            int* counter, int* fault, const int* done) {
+    if (done && *done) return;
+    int w = 0;
+    if ((threadIdx.x % WAVE) == 0) w = __hip_atomic_fetch_add(counter, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    w = __builtin_amdgcn_readfirstlane(w);  // lane 0's item for the whole wavefront
+    if (w >= E.nWaves) return;              // whole wavefronts
+    const int2 wl = E.waves[w];
+    const int L = wl.x;
+    const int pos0 = E.offD[L], size = E.offD[L + 1] - pos0, width = E.widthD[L], base = E.baseD[L];
+    const int rb = E.runBaseD[L];  // ≥ 0: entries hold LDS slots of a run (cell = runCells[rb + slot])
+    const int t = wl.y + int(threadIdx.x % WAVE);
+    const bool has = t < size;
+    const int c = E.cell[pos0 + (has ? t : 0)];
+    T acc, dg = T(1);
+    if constexpr (OP == SW_GS) {
+        acc = a.b[c];
+        dg = a.diag[c];
+    } else if constexpr (OP == SW_FWD) {
+        acc = a.rD[c] * a.r[c];
+    } else {
+        acc = xOld[c];
+    }
+    for (int k0 = 0; k0 < width; k0 += FREE_GROUP) {
+        int j[FREE_GROUP];
+        T cf[FREE_GROUP], v[FREE_GROUP];
+#pragma unroll
+        for (int u = 0; u < FREE_GROUP; ++u) {
+            const bool in = has && k0 + u < width;
+            const int e = in ? base + (k0 + u) * size + t : 0;
+            const int jj = E.col[e];
+            cf[u] = a.coef[e];
+            j[u] = in ? jj : -1;
+        }
+        if (rb >= 0) {  // LDS slots → cells
+#pragma unroll
+            for (int u = 0; u < FREE_GROUP; ++u)
+                if (j[u] >= 0) j[u] = E.runCells[rb + j[u]];
+        }
+        // first reads of the group together; then wait on the ones not published yet
+#pragma unroll
+        for (int u = 0; u < FREE_GROUP; ++u) {
+            const bool fresh = j[u] >= 0 && (BACK ? j[u] > c : j[u] < c);
+            v[u] = j[u] < 0 ? T(0)
+                   : fresh ? __hip_atomic_load(&xNew[j[u]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                           : xOld[j[u]];
+        }
+#pragma unroll
+        for (int u = 0; u < FREE_GROUP; ++u) {
+            const bool fresh = j[u] >= 0 && (BACK ? j[u] > c : j[u] < c);
+            if (fresh && Sentinel<T>::is(v[u])) v[u] = free_wait(xNew, j[u], fault);
+        }
+#pragma unroll
+        for (int u = 0; u < FREE_GROUP; ++u)
+            if (j[u] >= 0) acc = acc - cf[u] * v[u];
+    }
+    if (has) {
+        const T res = OP == SW_GS ? acc / dg : acc;
+        (void)__hip_atomic_exchange(&xNew[c], res, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// fault points at two ints: [0] the fault flag, [1] the work counter.
+template <int OP, bool BACK, class T>
+static void launch_free(hipStream_t st, const LevelEll& E, int n, const SweepArgs<T>& a, const T* xOld, T* xNew,
+                        int* fault, const int* done) {
+    if (n <= 0 || E.nWaves == 0) return;
+    int* counter = fault + 1;
+    hipLaunchKernelGGL(k_fill_sentinel<T>, sweep_grid(n), dim3(SWEEP_THREADS), 0, st, n, xNew, counter, done);
+    const long threads = long(E.nWaves) * WAVE;
+    hipLaunchKernelGGL((k_ell_free<OP, BACK, T>), dim3(unsigned((threads + SWEEP_THREADS - 1) / SWEEP_THREADS)),
+                       dim3(SWEEP_THREADS), 0, st, E, a, xOld, xNew, counter, fault, done);
+}
+
+template <class T>
+void launch_gs_sweep_free(hipStream_t st, const LevelEll& E, bool backward, const T* coef, const T* diag, const T* b,
+                          const T* xOld, T* xNew, int* fault, const int* done) {
+    SweepArgs<T> a;
+    a.coef = coef;
+    a.diag = diag;
+    a.b = b;
+    const int n = E.offH[E.nLev];
+    if (backward)
+        launch_free<SW_GS, true, T>(st, E, n, a, xOld, xNew, fault, done);
+    else
+        launch_free<SW_GS, false, T>(st, E, n, a, xOld, xNew, fault, done);
+}
+template void launch_gs_sweep_free<double>(hipStream_t, const LevelEll&, bool, const double*, const double*,
+                                           const double*, const double*, double*, int*, const int*);
+template void launch_gs_sweep_free<float>(hipStream_t, const LevelEll&, bool, const float*, const float*,
+                                          const float*, const float*, float*, int*, const int*);
+
+template <class T>
+void launch_dilu_apply_free(hipStream_t st, const LevelEll& F, const LevelEll& B, const T* rD, const T* ef,
+                            const T* eb, const T* r, T* tmp, T* w, int* fault, const int* done) {
+    SweepArgs<T> a;
+    a.rD = rD;
+    a.r = r;
+    a.coef = ef;
+    const int n = F.offH[F.nLev];
+    launch_free<SW_FWD, false, T>(st, F, n, a, nullptr, tmp, fault, done);  // tmp: forward result
+    a.coef = eb;
+    launch_free<SW_BWD, true, T>(st, B, n, a, tmp, w, fault, done);  // every cell, level 0 included
+}
+template void launch_dilu_apply_free<double>(hipStream_t, const LevelEll&, const LevelEll&, const double*,
+                                             const double*, const double*, const double*, double*, double*, int*,
+                                             const int*);
+template void launch_dilu_apply_free<float>(hipStream_t, const LevelEll&, const LevelEll&, const float*, const float*,
+                                            const float*, const float*, float*, float*, int*, const int*);
+
 // Per-entry coefficients of a LevelEll from their source array: out[e] = from[src[e]]
 // (0 for padding, src < 0).
 template <class T>

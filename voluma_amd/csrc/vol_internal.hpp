@@ -483,6 +483,12 @@ struct LevelEll {
     const int* grpH = nullptr;
     const char* grpKindH = nullptr;
     int nGrp = 0;
+    // sync-free sweeps: per wavefront (64 rows of one level) its level and first row
+    const int2* waves = nullptr;  // device
+    int nWaves = 0;
+    // per level, the first runCells entry of the LDS run it belongs to (−1: none): a run
+    // level's entries hold LDS slots, cell = runCells[runBase + slot]
+    const int* runBaseD = nullptr;  // device
 };
 enum SweepOp { SW_GS = 0, SW_FWD = 1, SW_BWD = 2 };
 template <class T> struct SweepArgs {
@@ -493,6 +499,19 @@ template <class T> struct SweepArgs {
     const T* r = nullptr;     // SW_FWD
     T* x = nullptr;           // updated in place (w for DILU)
 };
+// Sync-free forms of the sweeps (one launch per sweep, no level barriers): a row waits
+// for each neighbour whose new value it reads until that value is published in xNew,
+// which starts filled with a sentinel; values it reads old come from xOld (GS: the
+// neighbours on the other side; backward DILU: its own forward value). The arithmetic of
+// every row is the level-scheduled one, so the bits are too. fault (two device ints):
+// [0] set when a wait ran past its spin limit (the sweep's results are then void), [1]
+// the launch's work counter.
+template <class T>
+void launch_gs_sweep_free(hipStream_t st, const LevelEll& E, bool backward, const T* coef, const T* diag, const T* b,
+                          const T* xOld, T* xNew, int* fault, const int* done);
+template <class T>
+void launch_dilu_apply_free(hipStream_t st, const LevelEll& F, const LevelEll& B, const T* rD, const T* ef,
+                            const T* eb, const T* r, T* tmp, T* w, int* fault, const int* done);
 // w = M⁻¹ r of DIC/DILU: forward then backward substitution (ef, eb: rD·A products per
 // entry of F and B), every launch a no-op once *done
 template <class T>
