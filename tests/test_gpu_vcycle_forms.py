@@ -6,7 +6,9 @@ context switch read from the environment when the context is created:
 * the level-0 post-sweep reads z + xc[agg] from a streaming prolongation pass
   (VOLUMA_PROLONG_SPLIT=1);
 * the levels from the first of at most VOLUMA_TAIL_MAX_CELLS cells (up to 4096) down
-  to the coarsest run in one workgroup, matrices preloaded into LDS where they fit.
+  to the coarsest run in one workgroup, matrices preloaded into LDS where they fit;
+* levels in the CSR layout compute their residual and its restriction (into the
+  coarsest level: and its substitution) in one launch (VOLUMA_FUSED_RESTRICT, default).
 
 Every form computes the same per-row arithmetic in the same order, so solutions,
 iteration counts and residuals must equal the plain per-level launches bit for bit, in
@@ -22,9 +24,11 @@ from voluma_amd import workloads as W
 
 pytestmark = pytest.mark.gpu
 
-PLAIN = {"VOLUMA_COARSE_Z": "0", "VOLUMA_LEVEL0_Z": "0", "VOLUMA_PROLONG_SPLIT": "0", "VOLUMA_TAIL_MAX_CELLS": "0"}
+PLAIN = {"VOLUMA_COARSE_Z": "0", "VOLUMA_LEVEL0_Z": "0", "VOLUMA_PROLONG_SPLIT": "0", "VOLUMA_TAIL_MAX_CELLS": "0",
+         "VOLUMA_FUSED_RESTRICT": "0"}
 FORMS = {
     "default": {},
+    "fused_restrict_no_coarse_z": {"VOLUMA_COARSE_Z": "0"},
     "prolong_split": {"VOLUMA_PROLONG_SPLIT": "1"},
     "tail": {"VOLUMA_TAIL_MAX_CELLS": "512"},
     "tail_no_coarse_z": {"VOLUMA_TAIL_MAX_CELLS": "128", "VOLUMA_COARSE_Z": "0"},

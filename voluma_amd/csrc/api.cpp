@@ -282,6 +282,21 @@ void uploadLevels(vol_mesh m, const std::vector<int>* layout) {
             D->agg.upload(F.agg, st);
             const GatherLists G = buildGatherLists(F.n, C.n, F.agg, F.l, F.faceMap, F.faceFlip, int(C.l.size()));
             G.upload(*D, st);
+            if (D->layout == 0) {
+                const std::vector<int> blk = restrictBlocks(G.mOff, D->sell.csrG);
+                if (!blk.empty()) {
+                    std::vector<int2> b2(blk.size());
+                    for (size_t q = 0; q < blk.size(); ++q) b2[q] = make_int2(blk[q], G.mOff[blk[q]]);
+                    std::vector<int4> rows(G.mList.size());
+                    for (size_t p = 0; p < rows.size(); ++p) {
+                        const int i = G.mList[p];
+                        rows[p] = make_int4(i, F.off[i], F.off[i + 1] - F.off[i], 0);
+                    }
+                    D->rrBlk.upload(b2, st);
+                    D->rrRows.upload(rows, st);
+                    D->rrBlocks = int(blk.size()) - 1;
+                }
+            }
         }
         m->lv.push_back(std::move(D));
     }
@@ -1325,23 +1340,59 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
         cur = dst;
     }
     const int freeB = 1 - cur;
-    // residual of the pre-smoothed x, then restriction
-    if (z)
-        sell<T>(C, OP_RESID, XL_PLAIN, V, L.coef, L.diag, z, r, bufs[freeB], P.omega);
-    else
-        sell<T>(C, OP_RESID, P.nPre == 1 ? XL_FROMZERO : XL_PLAIN, V, L.coef, L.diag,
-                P.nPre == 1 ? nullptr : bufs[cur], r, bufs[freeB], P.omega, nullptr, nullptr, nullptr, nullptr, nullptr,
-                L.rD);
     const LevelPtrs<T> LC = levelPtrs(A, l + 1, (T*)nullptr);
-    if (l + 1 == nL - 1 && A->nCoarse <= 64 && A->tailStart != l + 1) {
-        // the coarsest level: restriction and substitution in one launch
-        launch_restrict_coarse64<T>(C.st, A->nCoarse, D.mOff.p, D.mList.p, bufs[freeB], A->luT.p, LC.x, C.done);
+    const bool toCoarsest = l + 1 == nL - 1 && A->nCoarse <= 64 && A->tailStart != l + 1;
+    if (m->ctx->fusedRestrict && D.rrBlocks > 0 && (!toCoarsest || (D.n <= RR_COARSE_ROWS && V.maxWidth <= 64))) {
+        // residual and restriction in one launch (no residual vector); into the coarsest
+        // level also its substitution
+        SellArgsT<T> a;
+        a.S = V;
+        a.coef = L.coef;
+        a.diag = L.diag;
+        a.rD = L.rD;
+        a.x = z ? z : (P.nPre == 1 ? nullptr : bufs[cur]);
+        a.r = r;
+        a.omega = P.omega;
+        a.done = C.done;
+        const int xl = (z || P.nPre != 1) ? XL_PLAIN : XL_FROMZERO;
+        RestrictArgsT<T> R;
+        R.nc = m->lv[l + 1]->n;
+        R.blk = D.rrBlk.p;
+        R.nBlocks = D.rrBlocks;
+        R.rows = D.rrRows.p;
+        R.mOff = D.mOff.p;
+        R.omega = P.omega;
+        if (toCoarsest) {
+            R.rc = LC.x;
+            R.luT = A->luT.p;
+            launch_resid_restrict_coarse64<T>(C.st, xl, V.csrG, V.csrIt, a, R);
+        } else {
+            R.rc = LC.r;
+            if (levelUsesZ(A, P, l + 1)) {
+                R.zc = LC.z;
+                R.rDc = LC.rD;
+            }
+            launch_resid_restrict<T>(C.st, xl, V.csrG, V.csrIt, a, R);
+            vcycle<T>(C, A, P, l + 1, LC.r, LC.x);
+        }
     } else {
-        if (levelUsesZ(A, P, l + 1))
-            restrictLevel(C, m->lv[l + 1]->n, D, bufs[freeB], LC.r, LC.z, LC.rD, P.omega);
+        // residual of the pre-smoothed x, then restriction
+        if (z)
+            sell<T>(C, OP_RESID, XL_PLAIN, V, L.coef, L.diag, z, r, bufs[freeB], P.omega);
         else
-            restrictLevel(C, m->lv[l + 1]->n, D, bufs[freeB], LC.r);
-        vcycle<T>(C, A, P, l + 1, LC.r, LC.x);
+            sell<T>(C, OP_RESID, P.nPre == 1 ? XL_FROMZERO : XL_PLAIN, V, L.coef, L.diag,
+                    P.nPre == 1 ? nullptr : bufs[cur], r, bufs[freeB], P.omega, nullptr, nullptr, nullptr, nullptr,
+                    nullptr, L.rD);
+        if (toCoarsest) {
+            // the coarsest level: restriction and substitution in one launch
+            launch_restrict_coarse64<T>(C.st, A->nCoarse, D.mOff.p, D.mList.p, bufs[freeB], A->luT.p, LC.x, C.done);
+        } else {
+            if (levelUsesZ(A, P, l + 1))
+                restrictLevel(C, m->lv[l + 1]->n, D, bufs[freeB], LC.r, LC.z, LC.rD, P.omega);
+            else
+                restrictLevel(C, m->lv[l + 1]->n, D, bufs[freeB], LC.r);
+            vcycle<T>(C, A, P, l + 1, LC.r, LC.x);
+        }
     }
     // prolongation fused into the first post-sweep
     int src = cur, other = freeB;
@@ -2001,6 +2052,7 @@ int vol_context_create(int device, vol_context* out) {
     c->coarseZ = !envOff("VOLUMA_COARSE_Z");
     c->level0Z = !envOff("VOLUMA_LEVEL0_Z");
     if (const char* e = std::getenv("VOLUMA_PROLONG_SPLIT")) c->prolongSplit = std::string(e) == "1";
+    c->fusedRestrict = !envOff("VOLUMA_FUSED_RESTRICT");
     if (const char* e = std::getenv("VOLUMA_SWEEP")) c->freeSweeps = std::string(e) == "free";
     c->sweepFault.alloc(2);  // fault flag, work counter
     HIPCHECK(hipMemsetAsync(c->sweepFault.p, 0, 2 * sizeof(int), c->stream));

@@ -684,7 +684,29 @@ struct LevelDev {
     DevBuf<int> iOff, iList;            // next-level cell → intra-aggregate fine faces (ascending)
     DevBuf<int> cfOff, cfList;          // next-level face → (fine face << 1 | flip), ascending
     DevBuf<int> l, u;                   // faces (coarsest: dense assembly)
+    // fused residual + restriction (CSR levels): per block (first aggregate, first member
+    // position) of its range; per member position (row, first slot, slot count)
+    DevBuf<int2> rrBlk;
+    DevBuf<int4> rrRows;
+    int rrBlocks = 0;
 };
+
+// Block ranges of the fused residual + restriction over the aggregates of a level with G
+// lanes per row: consecutive aggregates while their member rows number at most one per
+// lane group (256/G), and at most RR_POS aggregates; an aggregate alone takes a block
+// whatever its size. Empty when an aggregate has more than RR_POS member rows.
+inline std::vector<int> restrictBlocks(const std::vector<int>& mOff, int G) {
+    const int nc = int(mOff.size()) - 1, cap = 256 / G;
+    std::vector<int> blk{0};
+    int I = 0;
+    while (I < nc) {
+        if (mOff[I + 1] - mOff[I] > RR_POS) return {};
+        const int start = I++;
+        while (I < nc && I - start < RR_POS && mOff[I + 1] - mOff[start] <= cap) ++I;
+        blk.push_back(I);
+    }
+    return blk;
+}
 
 
 // Per-thread message of the last failed C ABI call (vol_last_error).
@@ -798,6 +820,9 @@ struct vol_context_s {
     // gathering it per neighbour (measured slower on the 2.06M-cell tets: 24.75 against
     // 24.60 ms per solve)
     bool coarseZ = true, level0Z = true, prolongSplit = false;
+    // levels in the CSR layout compute their residual and its restriction in one launch
+    // (VOLUMA_FUSED_RESTRICT=0: a residual gather, then the restriction)
+    bool fusedRestrict = true;
     // triangular sweeps (Gauss–Seidel, DIC/DILU substitutions) in their sync-free form,
     // one launch per sweep (VOLUMA_SWEEP=free; the same bits; measured slower than the
     // default level-scheduled launches); sweepFault: [0] set by a sync-free wait that ran

@@ -951,31 +951,41 @@ __global__ void k_coarse_solve(int n, const double* lu, const T* r, T* x, const 
     coarse_solve_body(n, lu, r, x, g);
 }
 
-// n ≤ 64: one wavefront, lane i holds row i of the factors in registers and g_i; the
-// column steps exchange g_k by a cross-lane broadcast. Same operations and order as
-// k_coarse_solve.
+// n ≤ 64: one wavefront, lane i holds g_i and reads row i of the factors (row(k): the
+// factor in column k); the column steps exchange g_k by a cross-lane broadcast. Same
+// operations and order as k_coarse_solve.
+// The steps are branch-free: every lane computes, and a select keeps the result only
+// where the step applies (lane i > k forward; lane k's division, lanes i < k backward),
+// so a lane's value sees exactly the operations of the branching form. Steps k ≥ n
+// change nothing: no lane is both > k and < n, and lanes ≥ n hold g = 0.
+template <class T, class Row>
+__device__ __forceinline__ void coarse_subst64(int n, Row row, double g, T* x) {
+    const int i = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < 64; ++k) {
+        const double gk = lane_bcast(g, k);
+        const double t = g - row(k) * gk;
+        g = (i > k && i < n) ? t : g;
+    }
+#pragma unroll
+    for (int k = 63; k >= 0; --k) {
+        const double q = g / row(k);
+        g = (i == k && k < n) ? q : g;
+        const double gk = lane_bcast(g, k);
+        const double t = g - row(k) * gk;
+        g = (i < k && k < n) ? t : g;
+    }
+    if (i < n) x[i] = T(g);
+}
+
+// ... with the factors of row i in registers
 template <class T>
 __device__ void coarse_solve64_g(int n, const double* luT, double g, T* x) {
     const int i = threadIdx.x;
     double row[64];
 #pragma unroll
     for (int k = 0; k < 64; ++k) row[k] = (k < n && i < n) ? luT[size_t(k) * n + i] : 0.0;
-#pragma unroll
-    for (int k = 0; k < 64; ++k) {
-        if (k < n) {
-            const double gk = lane_bcast(g, k);
-            if (i > k && i < n) g -= row[k] * gk;
-        }
-    }
-#pragma unroll
-    for (int k = 63; k >= 0; --k) {
-        if (k < n) {
-            if (i == k) g /= row[k];
-            const double gk = lane_bcast(g, k);
-            if (i < k) g -= row[k] * gk;
-        }
-    }
-    if (i < n) x[i] = T(g);
+    coarse_subst64(n, [&](int k) { return row[k]; }, g, x);
 }
 
 template <class T>
@@ -1014,6 +1024,158 @@ template void launch_restrict_coarse64<double>(hipStream_t, int, const int*, con
                                                double*, const int*);
 template void launch_restrict_coarse64<float>(hipStream_t, int, const int*, const int*, const float*, const double*,
                                               float*, const int*);
+
+// ============================================================================ residual + restriction
+// Residual (r − A·x) of CSR row `row` (slots s0 .. s0 + width) by G lanes of a
+// wavefront: lane g loads slots g, g+G, …, then every lane of the group subtracts the
+// products in slot order, read from their lanes, so the result has the bits of the
+// residual gather (sell_row / csr_row). All lanes of the wavefront call.
+template <int XL, int G, int IT, class T>
+__device__ __forceinline__ T resid_row(const SellArgsT<T>& a, int row, int s0, int width, bool valid) {
+    const int g = threadIdx.x % G;
+    T p[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int k = g + G * it;
+        p[it] = T(0);
+        if (k < width) p[it] = a.coef[s0 + k] * xload<XL>(a, a.S.col[s0 + k], T(0));
+    }
+    T acc = T(0);
+    if (valid) acc = a.r[row] - a.diag[row] * xload<XL>(a, row, T(0));
+    const int base = (threadIdx.x & (WAVE - 1)) - g;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const T t = __shfl(p[it], base + q, WAVE);
+            if (q + G * it < width) acc = acc - t;
+        }
+    }
+    return acc;
+}
+
+// Residuals of the member rows at positions [p0, p1) of the restriction lists into LDS
+// (res[p − p0]); G lanes per row, the whole block of NT threads in step.
+template <int XL, int G, int IT, int NT, class T>
+__device__ __forceinline__ void resid_members(const SellArgsT<T>& a, const int4* rows, int p0, int p1, T* res) {
+    const int grp = threadIdx.x / G;
+    for (int q = p0; q < p1; q += NT / G) {
+        const int p = q + grp;
+        const bool valid = p < p1;
+        const int4 rw = valid ? rows[p] : make_int4(0, 0, 0, 0);
+        const T v = resid_row<XL, G, IT>(a, rw.x, rw.y, rw.z, valid);
+        if (valid && threadIdx.x % G == 0) res[p - p0] = v;
+    }
+}
+
+// rc[I] for the aggregates of block b's range: member residuals into LDS, then one lane
+// per aggregate adds its members in ascending order (as k_restrict).
+template <int XL, int G, int IT, class T>
+__global__ __launch_bounds__(256) void k_resid_restrict(SellArgsT<T> a, RestrictArgsT<T> R) {
+    if (a.done && *a.done) return;
+    const int b = xcd_block();
+    if (b >= R.nBlocks) return;
+    __shared__ T res[RR_POS];
+    const int2 lo = R.blk[b], hi = R.blk[b + 1];
+    const int I = lo.x + int(threadIdx.x);
+    int mb = 0, me = 0;
+    if (I < hi.x) {  // the aggregate's member range, loaded ahead of the residuals
+        mb = R.mOff[I] - lo.y;
+        me = R.mOff[I + 1] - lo.y;
+    }
+    resid_members<XL, G, IT, 256>(a, R.rows, lo.y, hi.y, res);
+    __syncthreads();
+    if (I < hi.x) {
+        T s = T(0);
+        for (int k = mb; k < me; ++k) s += res[k];
+        R.rc[I] = s;
+        if (R.zc) R.zc[I] = T(R.omega) * (s * R.rDc[I]);
+    }
+}
+
+// Every member residual of the level into LDS (512 threads, the LU factors copied into
+// LDS alongside), then wavefront 0 restricts into the coarsest level (lane I: aggregate
+// I) and runs its substitution from registers (as k_restrict_coarse64).
+constexpr int RR_COARSE_THREADS = 512;
+template <int XL, int G, int IT, class T>
+__global__ __launch_bounds__(RR_COARSE_THREADS) void k_resid_restrict_coarse64(SellArgsT<T> a, RestrictArgsT<T> R) {
+    if (a.done && *a.done) return;
+    __shared__ T res[RR_COARSE_ROWS];
+    __shared__ double lu[64 * 64];
+    const int i = threadIdx.x, n = R.nc;
+    // the factors (≤ 64·64) in flight together with the residuals' loads, stored after
+    constexpr int LU_PER = 64 * 64 / RR_COARSE_THREADS;
+    double luv[LU_PER];
+#pragma unroll
+    for (int u = 0; u < LU_PER; ++u) {
+        const int k = i + u * RR_COARSE_THREADS;
+        luv[u] = k < n * n ? R.luT[k] : 0.0;
+    }
+    int mb = 0, me = 0;
+    if (i < n) {
+        mb = R.mOff[i];
+        me = R.mOff[i + 1];
+    }
+    resid_members<XL, G, IT, RR_COARSE_THREADS>(a, R.rows, 0, a.S.n, res);
+#pragma unroll
+    for (int u = 0; u < LU_PER; ++u) {
+        const int k = i + u * RR_COARSE_THREADS;
+        if (k < n * n) lu[k] = luv[u];
+    }
+    __syncthreads();
+    if (i < WAVE) {
+        double row[64];  // LU row i, all read from LDS before the substitution starts
+#pragma unroll
+        for (int k = 0; k < 64; ++k) row[k] = (k < n && i < n) ? lu[k * n + i] : 0.0;
+        T s = T(0);
+        for (int k = mb; k < me; ++k) s += res[k];
+        coarse_subst64(n, [&](int k) { return row[k]; }, i < n ? double(s) : 0.0, R.rc);
+    }
+}
+
+template <int XL, int G, int IT, class T>
+static void rr_launch(hipStream_t st, bool coarse, const SellArgsT<T>& a, const RestrictArgsT<T>& R) {
+    if (coarse)
+        hipLaunchKernelGGL((k_resid_restrict_coarse64<XL, G, IT, T>), dim3(1), dim3(RR_COARSE_THREADS), 0, st, a, R);
+    else
+        hipLaunchKernelGGL((k_resid_restrict<XL, G, IT, T>), dim3(xcd_grid(R.nBlocks)), dim3(256), 0, st, a, R);
+}
+// the grid form takes the level's lanes per row (the block ranges are cut for them); the
+// coarsest form as few lanes per row as its longest row allows, so that its one block
+// has every row in flight at once
+template <int XL, class T>
+static void rr_dispatch(hipStream_t st, bool coarse, int G, int IT, const SellArgsT<T>& a, const RestrictArgsT<T>& R) {
+    if (coarse) {
+        const int w = a.S.maxWidth;
+        if (w <= 16) rr_launch<XL, 4, 4>(st, true, a, R);
+        else if (w <= 32) rr_launch<XL, 4, 8>(st, true, a, R);
+        else rr_launch<XL, 8, 8>(st, true, a, R);  // rows ≤ 64 slots (the host checks)
+    } else if (G == 4 && IT <= 4) rr_launch<XL, 4, 4>(st, false, a, R);
+    else if (G == 8 && IT <= 4) rr_launch<XL, 8, 4>(st, false, a, R);
+    else if (G == 16 && IT <= 4) rr_launch<XL, 16, 4>(st, false, a, R);
+    else rr_launch<XL, 16, 8>(st, false, a, R);
+}
+template <class T>
+static void rr_any(hipStream_t st, bool coarse, int xl, int G, int IT, const SellArgsT<T>& a,
+                   const RestrictArgsT<T>& R) {
+    if (xl == XL_FROMZERO) rr_dispatch<XL_FROMZERO>(st, coarse, G, IT, a, R);
+    else rr_dispatch<XL_PLAIN>(st, coarse, G, IT, a, R);
+}
+template <class T>
+void launch_resid_restrict(hipStream_t st, int xl, int G, int IT, const SellArgsT<T>& a, const RestrictArgsT<T>& R) {
+    if (R.nBlocks > 0) rr_any(st, false, xl, G, IT, a, R);
+}
+template <class T>
+void launch_resid_restrict_coarse64(hipStream_t st, int xl, int G, int IT, const SellArgsT<T>& a,
+                                    const RestrictArgsT<T>& R) {
+    rr_any(st, true, xl, G, IT, a, R);
+}
+template void launch_resid_restrict<double>(hipStream_t, int, int, int, const SellArgs&, const RestrictArgsT<double>&);
+template void launch_resid_restrict<float>(hipStream_t, int, int, int, const SellArgs32&, const RestrictArgsT<float>&);
+template void launch_resid_restrict_coarse64<double>(hipStream_t, int, int, int, const SellArgs&,
+                                                     const RestrictArgsT<double>&);
+template void launch_resid_restrict_coarse64<float>(hipStream_t, int, int, int, const SellArgs32&,
+                                                    const RestrictArgsT<float>&);
 
 template <class T>
 static void coarse_solve_t(hipStream_t st, int n, const double* lu, const double* luT, const T* r, T* x,

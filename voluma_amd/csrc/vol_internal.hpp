@@ -221,6 +221,35 @@ void launch_coarse_solve(hipStream_t st, int n, const double* lu, const double* 
 template <class T>
 void launch_restrict_coarse64(hipStream_t st, int nc, const int* mOff, const int* mList, const T* res,
                               const double* luT, T* x, const int* done);
+// Residual and restriction of a level in one launch: rc[I] = Σ over the fine cells i of
+// aggregate I, ascending, of (r − A·x)_i, each row's residual computed in slot order as
+// the residual gather does (x: `a.x` under XL_PLAIN, ω·(r·rD) under XL_FROMZERO); zc
+// (nullable) as launch_restrict. No residual vector is written. CSR levels only.
+// blk: per block (first aggregate, first member position) of its range (nBlocks + 1
+// entries; a range holds ≤ RR_POS member rows unless it is one aggregate, and ≤ RR_POS
+// aggregates); rows: per member position (row, its first slot, its slot count), in the
+// order of the restriction lists. G lanes per row, G·IT ≥ the longest row.
+constexpr int RR_POS = 256;
+constexpr int RR_COARSE_ROWS = 1024;  // rows of a level whose restriction goes into the coarsest
+template <class T> struct RestrictArgsT {
+    int nc = 0;
+    const int2* blk = nullptr;
+    int nBlocks = 0;
+    const int4* rows = nullptr;
+    const int* mOff = nullptr;
+    T* rc = nullptr;
+    T* zc = nullptr;
+    const T* rDc = nullptr;
+    double omega = 1.0;
+    const double* luT = nullptr;  // coarsest form: LU factors (transposed), rc is then x
+};
+template <class T> void launch_resid_restrict(hipStream_t st, int xl, int G, int IT, const SellArgsT<T>& a,
+                                              const RestrictArgsT<T>& R);
+// The same into a coarsest level of nc ≤ 64 cells, followed in the same launch by its
+// substitution (as launch_restrict_coarse64); one workgroup, level rows ≤ RR_COARSE_ROWS,
+// each of ≤ 64 slots.
+template <class T> void launch_resid_restrict_coarse64(hipStream_t st, int xl, int G, int IT, const SellArgsT<T>& a,
+                                                       const RestrictArgsT<T>& R);
 // Same substitution in FP64 on FP32 vectors (r widened, x rounded).
 void launch_coarse_solve32(hipStream_t st, int n, const double* lu, const double* luT, const float* r, float* x,
                            const int* done);
