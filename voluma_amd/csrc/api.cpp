@@ -1342,6 +1342,7 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
     const int freeB = 1 - cur;
     const LevelPtrs<T> LC = levelPtrs(A, l + 1, (T*)nullptr);
     const bool toCoarsest = l + 1 == nL - 1 && A->nCoarse <= 64 && A->tailStart != l + 1;
+    bool postDone = false;  // the post-sweep ran in the coarsest launch
     if (m->ctx->fusedRestrict && D.rrBlocks > 0 && (!toCoarsest || (D.n <= RR_COARSE_ROWS && V.maxWidth <= 64))) {
         // residual and restriction in one launch (no residual vector); into the coarsest
         // level also its substitution
@@ -1365,6 +1366,13 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
         if (toCoarsest) {
             R.rc = LC.x;
             R.luT = A->luT.p;
+            // one post-sweep from z above level 0 (no fused dot): in the same launch
+            postDone = z && l > 0 && P.nPost == 1 && V.n <= RR_COARSE_THREADS / rrCoarseLanes(V.maxWidth);
+            if (postDone) {
+                R.post = 1;
+                a.agg = D.agg.p;
+                a.out = xout;
+            }
             launch_resid_restrict_coarse64<T>(C.st, xl, V.csrG, V.csrIt, a, R);
         } else {
             R.rc = LC.r;
@@ -1394,6 +1402,7 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
             vcycle<T>(C, A, P, l + 1, LC.r, LC.x);
         }
     }
+    if (postDone) return;
     // prolongation fused into the first post-sweep
     int src = cur, other = freeB;
     for (int s = 0; s < P.nPost; ++s) {

@@ -1096,7 +1096,6 @@ __global__ __launch_bounds__(256) void k_resid_restrict(SellArgsT<T> a, Restrict
 // Every member residual of the level into LDS (512 threads, the LU factors copied into
 // LDS alongside), then wavefront 0 restricts into the coarsest level (lane I: aggregate
 // I) and runs its substitution from registers (as k_restrict_coarse64).
-constexpr int RR_COARSE_THREADS = 512;
 template <int XL, int G, int IT, class T>
 __global__ __launch_bounds__(RR_COARSE_THREADS) void k_resid_restrict_coarse64(SellArgsT<T> a, RestrictArgsT<T> R) {
     if (a.done && *a.done) return;
@@ -1133,9 +1132,108 @@ __global__ __launch_bounds__(RR_COARSE_THREADS) void k_resid_restrict_coarse64(S
     }
 }
 
+// The same from x = z (XL_PLAIN), followed by the level's damped-Jacobi post-sweep from
+// x = z + xc[agg]: one row per lane group, which keeps its row's coefficients and the
+// gathered z and agg of its neighbours from the residual for the sweep, and reads xc from
+// LDS. Each row's arithmetic is that of the residual and prolongation-sweep gathers
+// (sell_row / csr_row with XL_PLAIN, then OP_JACOBI with XL_PROLONG).
+template <int G, int IT, class T>
+__global__ __launch_bounds__(RR_COARSE_THREADS) void k_resid_restrict_coarse64_post(SellArgsT<T> a,
+                                                                                   RestrictArgsT<T> R) {
+    if (a.done && *a.done) return;
+    __shared__ T res[RR_COARSE_THREADS / G];
+    __shared__ double lu[64 * 64];
+    __shared__ T xcL[64];
+    const int i = threadIdx.x, n = R.nc;
+    constexpr int LU_PER = 64 * 64 / RR_COARSE_THREADS;
+    double luv[LU_PER];
+#pragma unroll
+    for (int u = 0; u < LU_PER; ++u) {
+        const int k = i + u * RR_COARSE_THREADS;
+        luv[u] = k < n * n ? R.luT[k] : 0.0;
+    }
+    int mb = 0, me = 0;
+    if (i < n) {
+        mb = R.mOff[i];
+        me = R.mOff[i + 1];
+    }
+    // this lane group's member row (position p of the restriction lists)
+    const int p = i / G, g = i % G;
+    const bool valid = p < a.S.n;
+    const int4 rw = valid ? R.rows[p] : make_int4(0, 0, 0, 0);
+    T cf[IT], zv[IT];
+    int ag[IT];
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const int k = g + G * it;
+        cf[it] = T(0);
+        zv[it] = T(0);
+        ag[it] = 0;
+        if (k < rw.z) {
+            const int j = a.S.col[rw.y + k];
+            cf[it] = a.coef[rw.y + k];
+            zv[it] = a.x[j];
+            ag[it] = a.agg[j];
+        }
+    }
+    T rr = T(0), dd = T(1), zr = T(0), acc = T(0);
+    int agr = 0;
+    if (valid) {
+        rr = a.r[rw.x];
+        dd = a.diag[rw.x];
+        zr = a.x[rw.x];
+        agr = a.agg[rw.x];
+        acc = rr - dd * zr;
+    }
+    const int base = (i & (WAVE - 1)) - g;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const T pr = cf[it] * zv[it];
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const T t = __shfl(pr, base + q, WAVE);
+            if (q + G * it < rw.z) acc = acc - t;
+        }
+    }
+    if (valid && g == 0) res[p] = acc;
+#pragma unroll
+    for (int u = 0; u < LU_PER; ++u) {
+        const int k = i + u * RR_COARSE_THREADS;
+        if (k < n * n) lu[k] = luv[u];
+    }
+    __syncthreads();
+    if (i < WAVE) {
+        double row[64];
+#pragma unroll
+        for (int k = 0; k < 64; ++k) row[k] = (k < n && i < n) ? lu[k * n + i] : 0.0;
+        T s = T(0);
+        for (int k = mb; k < me; ++k) s += res[k];
+        coarse_subst64(n, [&](int k) { return row[k]; }, i < n ? double(s) : 0.0, xcL);
+        if (i < n) R.rc[i] = xcL[i];
+    }
+    __syncthreads();
+    T acc2 = rr;
+#pragma unroll
+    for (int it = 0; it < IT; ++it) {
+        const T pr = cf[it] * (zv[it] + xcL[ag[it]]);
+#pragma unroll
+        for (int q = 0; q < G; ++q) {
+            const T t = __shfl(pr, base + q, WAVE);
+            if (q + G * it < rw.z) acc2 = acc2 - t;
+        }
+    }
+    if (valid && g == 0) {
+        const T J = acc2 / dd;
+        const T xo = zr + xcL[agr];
+        a.out[rw.x] = xo + T(a.omega) * (J - xo);
+    }
+}
+
 template <int XL, int G, int IT, class T>
 static void rr_launch(hipStream_t st, bool coarse, const SellArgsT<T>& a, const RestrictArgsT<T>& R) {
-    if (coarse)
+    if (coarse && XL == XL_PLAIN && R.post)
+        hipLaunchKernelGGL((k_resid_restrict_coarse64_post<G, IT, T>), dim3(1), dim3(RR_COARSE_THREADS), 0, st, a, R);
+    else if (coarse)
         hipLaunchKernelGGL((k_resid_restrict_coarse64<XL, G, IT, T>), dim3(1), dim3(RR_COARSE_THREADS), 0, st, a, R);
     else
         hipLaunchKernelGGL((k_resid_restrict<XL, G, IT, T>), dim3(xcd_grid(R.nBlocks)), dim3(256), 0, st, a, R);
@@ -1150,6 +1248,7 @@ static void rr_dispatch(hipStream_t st, bool coarse, int G, int IT, const SellAr
         if (w <= 16) rr_launch<XL, 4, 4>(st, true, a, R);
         else if (w <= 32) rr_launch<XL, 4, 8>(st, true, a, R);
         else rr_launch<XL, 8, 8>(st, true, a, R);  // rows ≤ 64 slots (the host checks)
+        static_assert(RR_COARSE_THREADS / 4 >= 64, "lane groups of the coarsest form");
     } else if (G == 4 && IT <= 4) rr_launch<XL, 4, 4>(st, false, a, R);
     else if (G == 8 && IT <= 4) rr_launch<XL, 8, 4>(st, false, a, R);
     else if (G == 16 && IT <= 4) rr_launch<XL, 16, 4>(st, false, a, R);

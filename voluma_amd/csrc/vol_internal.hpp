@@ -242,7 +242,14 @@ template <class T> struct RestrictArgsT {
     const T* rDc = nullptr;
     double omega = 1.0;
     const double* luT = nullptr;  // coarsest form: LU factors (transposed), rc is then x
+    // coarsest form, XL_PLAIN: also the level's one damped-Jacobi post-sweep into a.out from
+    // x = a.x + xc[a.agg] (xc: the coarsest solution); needs level rows ≤ RR_COARSE_THREADS
+    // / rrCoarseLanes(longest row)
+    int post = 0;
 };
+constexpr int RR_COARSE_THREADS = 512;
+// lanes per row of the coarsest form
+inline int rrCoarseLanes(int maxWidth) { return maxWidth <= 32 ? 4 : 8; }
 template <class T> void launch_resid_restrict(hipStream_t st, int xl, int G, int IT, const SellArgsT<T>& a,
                                               const RestrictArgsT<T>& R);
 // The same into a coarsest level of nc ≤ 64 cells, followed in the same launch by its
