@@ -7,8 +7,10 @@ context switch read from the environment when the context is created:
   (VOLUMA_PROLONG_SPLIT=1);
 * the levels from the first of at most VOLUMA_TAIL_MAX_CELLS cells (up to 4096) down
   to the coarsest run in one workgroup, matrices preloaded into LDS where they fit;
-* levels in the CSR layout compute their residual and its restriction (into the
-  coarsest level: and its substitution) in one launch (VOLUMA_FUSED_RESTRICT, default).
+* levels compute their residual and its restriction (into the coarsest level: and its
+  substitution, and the post-sweep of the level) in one launch (VOLUMA_FUSED_RESTRICT,
+  default), SELL levels from a copy of their rows in restriction order
+  (VOLUMA_FUSED_SELL_MAX: up to that many rows).
 
 Every form computes the same per-row arithmetic in the same order, so solutions,
 iteration counts and residuals must equal the plain per-level launches bit for bit, in
@@ -28,13 +30,14 @@ PLAIN = {"VOLUMA_COARSE_Z": "0", "VOLUMA_LEVEL0_Z": "0", "VOLUMA_PROLONG_SPLIT":
          "VOLUMA_FUSED_RESTRICT": "0"}
 FORMS = {
     "default": {},
+    "fused_sell_levels": {"VOLUMA_FUSED_SELL_MAX": "1048576"},
     "fused_restrict_no_coarse_z": {"VOLUMA_COARSE_Z": "0"},
     "prolong_split": {"VOLUMA_PROLONG_SPLIT": "1"},
     "tail": {"VOLUMA_TAIL_MAX_CELLS": "512"},
     "tail_no_coarse_z": {"VOLUMA_TAIL_MAX_CELLS": "128", "VOLUMA_COARSE_Z": "0"},
     "tail_4096": {"VOLUMA_TAIL_MAX_CELLS": "4096"},
 }
-KEYS = sorted(PLAIN)
+KEYS = sorted(PLAIN) + ["VOLUMA_FUSED_SELL_MAX"]
 
 
 def _context(env):

@@ -150,6 +150,8 @@ struct MatLevel {
     DevBuf<float> diag32, rD32, coef32, bufA32, bufB32, r32, x32, z32;  // mixed precision
     DevBuf<double> gsCF, gsCB;  // Gauss–Seidel smoother coefficients (LevelEll layouts)
     DevBuf<float> gsCF32, gsCB32;
+    DevBuf<double> coefR;  // the level's rows in restriction order (LevelDev::rrCsr)
+    DevBuf<float> coefR32;
 };
 
 struct vol_matrix_s {
@@ -282,20 +284,39 @@ void uploadLevels(vol_mesh m, const std::vector<int>* layout) {
             D->agg.upload(F.agg, st);
             const GatherLists G = buildGatherLists(F.n, C.n, F.agg, F.l, F.faceMap, F.faceFlip, int(C.l.size()));
             G.upload(*D, st);
-            if (D->layout == 0) {
-                const std::vector<int> blk = restrictBlocks(G.mOff, D->sell.csrG);
-                if (!blk.empty()) {
-                    std::vector<int2> b2(blk.size());
-                    for (size_t q = 0; q < blk.size(); ++q) b2[q] = make_int2(blk[q], G.mOff[blk[q]]);
-                    std::vector<int4> rows(G.mList.size());
-                    for (size_t p = 0; p < rows.size(); ++p) {
-                        const int i = G.mList[p];
-                        rows[p] = make_int4(i, F.off[i], F.off[i + 1] - F.off[i], 0);
-                    }
-                    D->rrBlk.upload(b2, st);
-                    D->rrRows.upload(rows, st);
-                    D->rrBlocks = int(blk.size()) - 1;
+            // fused residual + restriction: CSR levels read their own rows; SELL levels up
+            // to the size limit a copy of their rows in the order of the restriction lists
+            int lanes = 0;
+            std::vector<int4> rows(G.mList.size());
+            if (k > 0 && D->layout == 0) {
+                lanes = D->sell.csrG;
+                for (size_t p = 0; p < rows.size(); ++p) {
+                    const int i = G.mList[p];
+                    rows[p] = make_int4(i, F.off[i], F.off[i + 1] - F.off[i], 0);
                 }
+            } else if (k > 0 && D->layout == 1 && F.n <= m->ctx->fusedSellMax) {
+                std::vector<int> offP(1, 0), colP, srcP;
+                offP.reserve(F.n + 1);
+                colP.reserve(F.adjCell.size());
+                srcP.reserve(F.adjCell.size());
+                for (size_t p = 0; p < rows.size(); ++p) {
+                    const int i = G.mList[p];
+                    rows[p] = make_int4(i, offP.back(), F.off[i + 1] - F.off[i], 0);
+                    for (int q = F.off[i]; q < F.off[i + 1]; ++q) {
+                        colP.push_back(F.adjCell[q]);
+                        srcP.push_back((F.adjFace[q] << 1) | (F.adjIsL[q] ? 0 : 1));
+                    }
+                    offP.push_back(int(colP.size()));
+                }
+                if (buildCsr(int(rows.size()), offP, colP, srcP, D->rrCsr, st)) lanes = D->rrCsr.csrG;
+            }
+            const std::vector<int> blk = lanes ? restrictBlocks(G.mOff, lanes) : std::vector<int>();
+            if (!blk.empty()) {
+                std::vector<int2> b2(blk.size());
+                for (size_t q = 0; q < blk.size(); ++q) b2[q] = make_int2(blk[q], G.mOff[blk[q]]);
+                D->rrBlk.upload(b2, st);
+                D->rrRows.upload(rows, st);
+                D->rrBlocks = int(blk.size()) - 1;
             }
         }
         m->lv.push_back(std::move(D));
@@ -425,6 +446,10 @@ void buildGamgCoeffs(vol_matrix A) {
         launch_galerkin(st, DC.n, DC.nf, D.mOff.p, D.mList.p, D.iOff.p, D.iList.p, D.cfOff.p, D.cfList.p, Fd,
                         A->L[k]->Fu.p, A->L[k]->Fl.p, A->L[k + 1]->diag.p, A->L[k + 1]->Fu.p, A->L[k + 1]->Fl.p);
         launch_sell_coef(st, DC.sell.view(), A->L[k + 1]->Fu.p, A->L[k + 1]->Fl.p, A->L[k + 1]->coef.p, nullptr);
+        if (DC.rrCsr.nslots > 0) {
+            A->L[k + 1]->coefR.alloc(DC.rrCsr.nslots);
+            launch_sell_coef(st, DC.rrCsr.view(), A->L[k + 1]->Fu.p, A->L[k + 1]->Fl.p, A->L[k + 1]->coefR.p, nullptr);
+        }
         VecArgs v;  // 1/diag of the coarse level
         v.n = DC.n;
         v.a = A->L[k + 1]->diag.p;
@@ -844,6 +869,10 @@ void buildMixed(vol_matrix A, bool gamg) {
                 toFloat(st, M.diag.p, M.diag32.p, D.n);
                 toFloat(st, M.rD.p, M.rD32.p, D.n);
                 toFloat(st, M.coef.p, M.coef32.p, D.sell.nslots);
+                if (D.rrCsr.nslots > 0) {
+                    M.coefR32.alloc(D.rrCsr.nslots);
+                    toFloat(st, M.coefR.p, M.coefR32.p, D.rrCsr.nslots);
+                }
             }
             M.bufA32.alloc(D.n);
             M.bufB32.alloc(D.n);
@@ -1247,6 +1276,8 @@ LevelPtrs<float> levelPtrs(vol_matrix A, int l, float*) {
     if (l == 0) return {A->coef32.p, A->diag32.p, A->rD32.p, M.bufA32.p, M.bufB32.p, nullptr, nullptr, nullptr};
     return {M.coef32.p, M.diag32.p, M.rD32.p, M.bufA32.p, M.bufB32.p, M.r32.p, M.x32.p, M.z32.p};
 }
+const double* restrictCoef(const MatLevel* M, double*) { return M->coefR.p; }
+const float* restrictCoef(const MatLevel* M, float*) { return M->coefR32.p; }
 void tailLaunch(const Ctx& C, vol_matrix A, const Plan& P, int l, const double* r, double* xout) {
     TailArgs t;
     t.nLevels = int(A->mesh->lv.size()) - l;
@@ -1343,12 +1374,14 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
     const LevelPtrs<T> LC = levelPtrs(A, l + 1, (T*)nullptr);
     const bool toCoarsest = l + 1 == nL - 1 && A->nCoarse <= 64 && A->tailStart != l + 1;
     bool postDone = false;  // the post-sweep ran in the coarsest launch
-    if (m->ctx->fusedRestrict && D.rrBlocks > 0 && (!toCoarsest || (D.n <= RR_COARSE_ROWS && V.maxWidth <= 64))) {
+    if (m->ctx->fusedRestrict && D.rrBlocks > 0 &&
+        (!toCoarsest || (D.n <= RR_COARSE_ROWS && V.maxWidth <= 64))) {
         // residual and restriction in one launch (no residual vector); into the coarsest
         // level also its substitution
         SellArgsT<T> a;
-        a.S = V;
-        a.coef = L.coef;
+        const bool own = D.layout == 0;  // CSR level: its own rows; SELL: the restriction-order copy
+        a.S = own ? V : D.rrCsr.view();
+        a.coef = own ? L.coef : restrictCoef(A->L[l].get(), (T*)nullptr);
         a.diag = L.diag;
         a.rD = L.rD;
         a.x = z ? z : (P.nPre == 1 ? nullptr : bufs[cur]);
@@ -1367,20 +1400,20 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
             R.rc = LC.x;
             R.luT = A->luT.p;
             // one post-sweep from z above level 0 (no fused dot): in the same launch
-            postDone = z && l > 0 && P.nPost == 1 && V.n <= RR_COARSE_THREADS / rrCoarseLanes(V.maxWidth);
+            postDone = z && l > 0 && P.nPost == 1 && V.n <= RR_COARSE_THREADS / rrCoarseLanes(a.S.maxWidth);
             if (postDone) {
                 R.post = 1;
                 a.agg = D.agg.p;
                 a.out = xout;
             }
-            launch_resid_restrict_coarse64<T>(C.st, xl, V.csrG, V.csrIt, a, R);
+            launch_resid_restrict_coarse64<T>(C.st, xl, a.S.csrG, a.S.csrIt, a, R);
         } else {
             R.rc = LC.r;
             if (levelUsesZ(A, P, l + 1)) {
                 R.zc = LC.z;
                 R.rDc = LC.rD;
             }
-            launch_resid_restrict<T>(C.st, xl, V.csrG, V.csrIt, a, R);
+            launch_resid_restrict<T>(C.st, xl, a.S.csrG, a.S.csrIt, a, R);
             vcycle<T>(C, A, P, l + 1, LC.r, LC.x);
         }
     } else {
@@ -2062,6 +2095,7 @@ int vol_context_create(int device, vol_context* out) {
     c->level0Z = !envOff("VOLUMA_LEVEL0_Z");
     if (const char* e = std::getenv("VOLUMA_PROLONG_SPLIT")) c->prolongSplit = std::string(e) == "1";
     c->fusedRestrict = !envOff("VOLUMA_FUSED_RESTRICT");
+    if (const char* e = std::getenv("VOLUMA_FUSED_SELL_MAX")) c->fusedSellMax = std::atoi(e);
     if (const char* e = std::getenv("VOLUMA_SWEEP")) c->freeSweeps = std::string(e) == "free";
     c->sweepFault.alloc(2);  // fault flag, work counter
     HIPCHECK(hipMemsetAsync(c->sweepFault.p, 0, 2 * sizeof(int), c->stream));

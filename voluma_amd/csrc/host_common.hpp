@@ -684,11 +684,14 @@ struct LevelDev {
     DevBuf<int> iOff, iList;            // next-level cell → intra-aggregate fine faces (ascending)
     DevBuf<int> cfOff, cfList;          // next-level face → (fine face << 1 | flip), ascending
     DevBuf<int> l, u;                   // faces (coarsest: dense assembly)
-    // fused residual + restriction (CSR levels): per block (first aggregate, first member
-    // position) of its range; per member position (row, first slot, slot count)
+    // fused residual + restriction: per block (first aggregate, first member position) of
+    // its range; per member position (row, first slot, slot count) in the level's CSR
+    // layout, or, for a SELL level, in rrCsr: the level's rows again, in the order of the
+    // restriction lists, as CSR (its coefficients are filled beside the SELL ones)
     DevBuf<int2> rrBlk;
     DevBuf<int4> rrRows;
     int rrBlocks = 0;
+    SellDev rrCsr;
 };
 
 // Block ranges of the fused residual + restriction over the aggregates of a level with G
@@ -821,8 +824,13 @@ struct vol_context_s {
     // 24.60 ms per solve)
     bool coarseZ = true, level0Z = true, prolongSplit = false;
     // levels in the CSR layout compute their residual and its restriction in one launch
-    // (VOLUMA_FUSED_RESTRICT=0: a residual gather, then the restriction)
+    // (VOLUMA_FUSED_RESTRICT=0: a residual gather, then the restriction); so can SELL
+    // levels of at most fusedSellMax rows (VOLUMA_FUSED_SELL_MAX, default 0), from a copy
+    // of their rows in restriction order: measured slower on the 2.06M-cell tets (level 1
+    // 31.0 µs against 23.5 for the residual and restriction launches, and the following
+    // post-sweeps slower as the SELL rows are no longer in cache; 22.8 against 21.4 ms)
     bool fusedRestrict = true;
+    int fusedSellMax = 0;
     // triangular sweeps (Gauss–Seidel, DIC/DILU substitutions) in their sync-free form,
     // one launch per sweep (VOLUMA_SWEEP=free; the same bits; measured slower than the
     // default level-scheduled launches); sweepFault: [0] set by a sync-free wait that ran
