@@ -1,0 +1,83 @@
+"""CPU tests of the rocprofv3 trace summaries (voluma_amd/profiling.py) on a synthetic
+trace: the per-kernel statistics and the per-level walk of one PCG iteration, with the
+separate and the fused residual/restriction launches."""
+import json
+import sqlite3
+
+from voluma_amd import profiling
+
+CELLS = [1000, 250, 60]
+FACES = [2000, 600, 150]
+
+
+def _db(path, iterations):
+    con = sqlite3.connect(path)
+    con.execute("create table kernels (name text, grid_x int, workgroup_x int, start int, duration int)")
+    t = 0
+    for seq in iterations:
+        for name, grid, dur in seq:
+            con.execute("insert into kernels values (?, ?, ?, ?, ?)",
+                        (f"void vol::{name}(vol::SellArgsT<double>)", grid * 256, 256, t, dur))
+            t += dur + 100
+    con.commit()
+    con.close()
+
+
+def _bench(path):
+    line = {"metric": "m", "config": {"gamg_levels": CELLS, "gamg_level_faces": FACES}}
+    with open(path, "w") as fh:
+        fh.write("[bench] something\n" + json.dumps(line) + "\n")
+
+
+AMUL = ("k_sell<0, 0, true, 4, double>", 8, 2000)
+SEPARATE = [AMUL, ("k_finalize", 1, 500), ("k_sell<1, 0, false, 4, double>", 8, 1000),
+            ("k_restrict<double>", 2, 300), ("k_csrv<1, 0, 8, 4, double>", 4, 400),
+            ("k_restrict_coarse64<double>", 1, 900), ("k_csrv<2, 3, 8, 4, double>", 4, 450),
+            ("k_sell<2, 3, true, 4, double>", 8, 3000)]
+FUSED = [AMUL, ("k_finalize", 1, 500), ("k_sell<1, 0, false, 4, double>", 8, 1000),
+         ("k_restrict<double>", 2, 300), ("k_resid_restrict_coarse64_post<4, 8, double>", 1, 1200),
+         ("k_sell<2, 3, true, 4, double>", 8, 3000)]
+
+
+def _rows(table):
+    return [[c.strip() for c in line.strip("|").split("|")] for line in table.splitlines()
+            if line.startswith("| ") and not line.startswith("| #") and not line.startswith("| kernel")]
+
+
+def test_stats_totals(tmp_path):
+    db = str(tmp_path / "run.db")
+    _db(db, [SEPARATE] * 3)
+    out = profiling.stats(db)
+    assert out.startswith(f"busy {3 * sum(d for _, _, d in SEPARATE) / 1e6:.3f} ms, {3 * len(SEPARATE)} dispatches")
+    rows = _rows(out)
+    top = rows[0]
+    assert top[0] == "`k_sell<2, 3, true, 4, double>`" and top[2] == "3" and float(top[4]) == 3.0
+
+
+def test_levels_separate_launches(tmp_path):
+    db, bench = str(tmp_path / "run.db"), str(tmp_path / "bench.log")
+    _db(db, [SEPARATE] * 4)
+    _bench(bench)
+    out = profiling.levels(db, bench)
+    assert out.startswith(f"3 of 3 iterations with the common sequence of {len(SEPARATE)} dispatches")
+    lev = {r[1]: r[3] for r in _rows(out)}
+    assert lev["`k_restrict<double>`"] == "0→1"
+    assert lev["`k_csrv<1, 0, 8, 4, double>`"] == "1"
+    assert lev["`k_restrict_coarse64<double>`"] == "1→2"
+    assert lev["`k_csrv<2, 3, 8, 4, double>`"] == "1"   # the post-sweep one level up
+    assert lev["`k_sell<2, 3, true, 4, double>`"] == "0"
+
+
+def test_levels_fused_launches(tmp_path):
+    db, bench = str(tmp_path / "run.db"), str(tmp_path / "bench.log")
+    _db(db, [FUSED] * 3)
+    _bench(bench)
+    rows = {r[1]: r for r in _rows(profiling.levels(db, bench))}
+    fused = rows["`k_resid_restrict_coarse64_post<4, 8, double>`"]
+    assert fused[3] == "1→2" and fused[4] == str(CELLS[1]) and fused[5] == str(2 * FACES[1])
+    # its own post-sweep is inside: the next sweep is level 0's
+    assert rows["`k_sell<2, 3, true, 4, double>`"][3] == "0"
+    # bytes of the fused launch: level-1 rows and entries, lists, coarse vectors, LU, the sweep
+    n, nc, e = CELLS[1], CELLS[2], 2 * FACES[1]
+    mb = (8 * n + 12 * e + 16 * n + 16 * n + 4 * (nc + 1) + 24 * nc + 8 * nc * nc + 16 * n + 4 * n + 8 * n) / 1e6
+    assert fused[7] == f"{mb:.1f}"

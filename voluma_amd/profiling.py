@@ -12,7 +12,8 @@
 The ``levels`` table splits the trace into PCG iterations at the level-0 A·p launches
 (``k_sell<0, 0, true, ...>``), keeps the iterations whose dispatch sequence is the most
 common one, and walks the V-cycle's fixed order (residual and restriction down the
-levels, coarsest solve, prolongation sweeps up) to name each dispatch's level. Bytes are
+levels, separate or fused, coarsest solve, prolongation sweeps up) to name each
+dispatch's level. Bytes are
 what the kernel has to move given the stored layout, without its padding: per row of a
 level the vectors it reads and writes, per off-diagonal entry a coefficient and a column
 (8 + 4 bytes in FP64), the restriction's member lists and the coarsest LU factors; the
@@ -125,6 +126,21 @@ def levels(db, bench_json):
                 kind = "resid"
             nc = cells[level + 1] if level + 1 < nl else 0
             lev, nbytes = str(level), _row_bytes(kind, cells[level], 2 * faces[level], nc, elem)
+        elif k.startswith("k_resid_restrict"):
+            # residual of level l (its rows through the restriction lists) summed into l + 1;
+            # into the coarsest also the LU substitution, and with _post the level's sweep
+            n, nc = cells[level], cells[level + 1]
+            lev = f"{level}→{level + 1}"
+            rws, ent = n, 2 * faces[level]
+            mat = elem * n + (elem + 4) * ent
+            nbytes = mat + 2 * elem * n + 16 * n + 4 * (nc + 1) + 3 * elem * nc  # r, z; rows; lists; r, z, 1/diag
+            if "coarse64" in k:
+                nbytes += 8 * nc * nc
+                if "_post" in k:  # the sweep re-reads r, z, agg of the rows and writes x
+                    nbytes += 2 * elem * n + 4 * n + elem * n
+            level += 1
+            if "_post" in k:
+                level -= 1  # its post-sweep is done: the next sweep is one level up
         elif k.startswith("k_restrict"):
             n, nc = cells[level], cells[level + 1]
             lev = f"{level}→{level + 1}"
@@ -144,7 +160,7 @@ def levels(db, bench_json):
                 nbytes = 8 * elem * cells[0]
             elif op == "4":  # p = w + βp
                 nbytes = 3 * elem * cells[0]
-        if lev.isdigit():
+        if lev.isdigit() and not rws:
             rws, ent = cells[int(lev)], 2 * faces[int(lev)]
         mb = f"{nbytes / 1e6:.1f}" if nbytes else ""
         gbs = f"{nbytes / us / 1e3:.0f}" if nbytes else ""
