@@ -81,3 +81,17 @@ def test_levels_fused_launches(tmp_path):
     n, nc, e = CELLS[1], CELLS[2], 2 * FACES[1]
     mb = (8 * n + 12 * e + 16 * n + 16 * n + 4 * (nc + 1) + 24 * nc + 8 * nc * nc + 16 * n + 4 * n + 8 * n) / 1e6
     assert fused[7] == f"{mb:.1f}"
+
+
+def test_precisions_pairs_dispatches(tmp_path):
+    db = str(tmp_path / "run.db")
+    f32 = [(k.replace("double", "float"), g, d // 2) for k, g, d in FUSED]
+    _db(db, [SEPARATE] * 2 + [FUSED] * 4 + [f32] * 3)
+    out = profiling.precisions(db)
+    # FP64 iterations run between FP64 A·p launches: 5, three of them the fused sequence;
+    # FP32: 2
+    assert out.startswith(f"FP64: 3 of 5 iterations with the common sequence of {len(FUSED)} dispatches")
+    assert "FP32 inner: 2 of 2 iterations" in out
+    rows = _rows(out)
+    assert len(rows) == len(FUSED)
+    assert all(r[6] == "2.00" for r in rows)

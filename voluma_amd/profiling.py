@@ -8,6 +8,9 @@
                                                        iterations, bytes, GB/s (BENCH: a file
                                                        holding bench.py's JSON line, for the
                                                        level sizes)
+    python -m voluma_amd.profiling DB precisions       the FP64 PCG iteration beside the FP32
+                                                       inner iteration of mixed precision,
+                                                       dispatch by dispatch
 
 The ``levels`` table splits the trace into PCG iterations at the level-0 A·p launches
 (``k_sell<0, 0, true, ...>``), keeps the iterations whose dispatch sequence is the most
@@ -67,10 +70,38 @@ def _is_amul(kernel):
     return kernel.startswith("k_sell<") and len(a) >= 3 and a[0] == "0" and a[1] == "0" and a[2] == "true"
 
 
-def _iterations(rows):
-    """Dispatch sequences between consecutive level-0 A·p launches."""
-    starts = [i for i, r in enumerate(rows) if _is_amul(r[0])]
+def _iterations(rows, elem_type=None):
+    """Dispatch sequences between consecutive level-0 A·p launches (of one value type)."""
+    starts = [i for i, r in enumerate(rows)
+              if _is_amul(r[0]) and (elem_type is None or _template_args(r[0])[-1] == elem_type)]
     return [rows[a:b] for a, b in zip(starts, starts[1:])]
+
+
+def _common(its):
+    """The most common dispatch sequence, the iterations that follow it, and the
+    average duration (µs) of each of its dispatches."""
+    shape = Counter(tuple((k, g) for k, g, _, _ in it) for it in its).most_common(1)[0][0]
+    keep = [it for it in its if tuple((k, g) for k, g, _, _ in it) == shape]
+    avg = [sum(it[i][3] for it in keep) / len(keep) / 1e3 for i in range(len(shape))]
+    return shape, keep, avg
+
+
+def precisions(db):
+    rows = dispatches(db)
+    i64, i32 = _iterations(rows, "double"), _iterations(rows, "float")
+    if not i64 or not i32:
+        raise SystemExit("the trace needs FP64 and FP32 (mixed precision) A·p launches")
+    s64, k64, a64 = _common(i64)
+    s32, k32, a32 = _common(i32)
+    if [g for _, g in s64] != [g for _, g in s32]:
+        raise SystemExit("the FP64 and FP32 iterations launch different grids")
+    out = [f"FP64: {len(k64)} of {len(i64)} iterations with the common sequence of {len(s64)} dispatches, "
+           f"{sum(a64):.1f} us of kernel time per iteration; FP32 inner: {len(k32)} of {len(i32)} iterations, "
+           f"{sum(a32):.1f} us ({sum(a64) / sum(a32):.2f}x)", "",
+           "| # | FP64 kernel | grid | avg us | FP32 kernel | avg us | FP64/FP32 |", "|---|---|---|---|---|---|---|"]
+    for i, ((k, g), (k2, _)) in enumerate(zip(s64, s32)):
+        out.append(f"| {i} | `{k}` | {g} | {a64[i]:.2f} | `{k2}` | {a32[i]:.2f} | {a64[i] / a32[i]:.2f} |")
+    return "\n".join(out)
 
 
 def _row_bytes(op_kind, n, nnz, nc, elem):
@@ -102,9 +133,7 @@ def levels(db, bench_json):
     its = _iterations(rows)
     if not its:
         raise SystemExit("no level-0 A·p launches in the trace")
-    shape = Counter(tuple((k, g) for k, g, _, _ in it) for it in its).most_common(1)[0][0]
-    keep = [it for it in its if tuple((k, g) for k, g, _, _ in it) == shape]
-    avg = [sum(it[i][3] for it in keep) / len(keep) / 1e3 for i in range(len(shape))]
+    shape, keep, avg = _common(its)
     nl = len(cells)
     elem = 8
     out = [f"{len(keep)} of {len(its)} iterations with the common sequence of {len(shape)} dispatches; "
@@ -177,6 +206,8 @@ def main(argv):
         print(stats(db, int(argv[3]) if len(argv) > 3 else 30))
     elif mode == "levels":
         print(levels(db, argv[3]))
+    elif mode == "precisions":
+        print(precisions(db))
     else:
         print(__doc__)
         return 2
