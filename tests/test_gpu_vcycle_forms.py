@@ -92,3 +92,54 @@ def test_vcycle_forms_bit_identical(mesh):
             assert pg.iterations == pr.iterations, (name, case)
             assert pg.finalResidual == pr.finalResidual and pg.initialResidual == pr.initialResidual, (name, case)
             assert np.array_equal(xg, xr), (name, case)
+
+
+def _packed_solves(packed, m):
+    """Level-0 gathers with packed rows (the default where the mesh allows them) or
+    slot-major (VOLUMA_PACKED=0, read when the mesh is set up): pressure solves of every
+    case above, diagonal PCG, A·x, and the nonsymmetric momentum system."""
+    old = os.environ.get("VOLUMA_PACKED")
+    try:
+        if packed:
+            os.environ.pop("VOLUMA_PACKED", None)
+        else:
+            os.environ["VOLUMA_PACKED"] = "0"
+        ctx = va.Context(0)
+        out = []
+        lm, A, b = W.pressure_system(ctx, m, seed=1, gamg=True, nonorth=False)
+        for pre, prec, sums, tol in CASES:
+            x = np.zeros(m.n_cells)
+            out.append((x, va.solve_ldu(A, b, x, _controls(pre, prec, sums, tol))))
+        x = np.zeros(m.n_cells)
+        out.append((x, va.solve_ldu(A, b, x, va.SolverControls("PCG", "diagonal", 1e-8, 0.0, 400))))
+        out.append((va.amul(A, b), None))
+        A.close()
+        lm.close()
+        lm, A, b, inp = W.momentum_system(ctx, m, seed=1, comp=0)
+        for prec in (va.FP64, va.MIXED):
+            x = np.ascontiguousarray(inp["U0"][:, 0]).copy()
+            out.append((x, va.solve_ldu(A, b, x, va.SolverControls("PBiCGStab", "diagonal", 1e-10, 0.0, 50,
+                                                                      precision=prec))))
+        out.append((va.amul(A, b), None))
+        A.close()
+        lm.close()
+        ctx.close()
+        return out
+    finally:
+        if old is None:
+            os.environ.pop("VOLUMA_PACKED", None)
+        else:
+            os.environ["VOLUMA_PACKED"] = old
+
+
+@pytest.mark.parametrize("mesh", ["box64", "tet"])
+def test_packed_rows_bit_identical(mesh):
+    m = W.make_mesh("box64") if mesh == "box64" else M.tet(34, 30, 28, 0.2, 3)
+    print(f"[packed] {mesh}: {m.n_cells} cells, slot-major level 0", flush=True)
+    ref = _packed_solves(False, m)
+    print(f"[packed] {mesh}: packed rows", flush=True)
+    got = _packed_solves(True, m)
+    for k, ((xr, pr), (xg, pg)) in enumerate(zip(ref, got)):
+        if pr is not None:
+            assert pg.iterations == pr.iterations and pg.finalResidual == pr.finalResidual, k
+        assert np.array_equal(xg, xr), k

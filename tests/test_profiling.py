@@ -95,3 +95,14 @@ def test_precisions_pairs_dispatches(tmp_path):
     rows = _rows(out)
     assert len(rows) == len(FUSED)
     assert all(r[6] == "2.00" for r in rows)
+
+
+def test_packed_gathers_count_as_level0(tmp_path):
+    db, bench = str(tmp_path / "run.db"), str(tmp_path / "bench.log")
+    packed = [(k.replace("k_sell<", "k_sell_packed<"), g, d) for k, g, d in FUSED]
+    _db(db, [packed] * 3)
+    _bench(bench)
+    rows = {r[1]: r for r in _rows(profiling.levels(db, bench))}
+    assert rows["`k_sell_packed<0, 0, true, 4, double>`"][3] == "0"
+    assert rows["`k_sell_packed<1, 0, false, 4, double>`"][3] == "0"
+    assert rows["`k_sell_packed<2, 3, true, 4, double>`"][3] == "0"

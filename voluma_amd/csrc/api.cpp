@@ -236,7 +236,7 @@ void setupMesh(vol_mesh m, const double* w, int mergeLevels) {
             src[k] = (f << 1) | 1;
         }
     }
-    buildSell(n, off, col, src, m->sell, st, true);
+    buildSell(n, off, col, src, m->sell, st, true, true);
     if (!w) return;
     m->hasGamg = true;
     // agglomeration on the GPU for meshes of ≥ 32768 cells (below, the launches of the
@@ -848,10 +848,10 @@ void buildDilu32(vol_matrix A) {
 void buildMixed(vol_matrix A, bool gamg) {
     vol_mesh m = A->mesh;
     hipStream_t st = m->ctx->stream;
-    A->coef32.alloc(m->sell.nslots);
+    A->coef32.alloc(size_t(m->sell.nslots) + m->sell.nslotsP);
     A->diag32.alloc(m->n);
     A->rD32.alloc(m->n);
-    toFloat(st, A->coef.p, A->coef32.p, m->sell.nslots);
+    toFloat(st, A->coef.p, A->coef32.p, size_t(m->sell.nslots) + m->sell.nslotsP);
     toFloat(st, A->diag.p, A->diag32.p, m->n);
     toFloat(st, A->rD.p, A->rD32.p, m->n);
     if (gamg) {
@@ -913,7 +913,7 @@ void buildMixed(vol_matrix A, bool gamg) {
 void matrixFinish(vol_matrix A) {
     vol_mesh m = A->mesh;
     hipStream_t st = m->ctx->stream;
-    A->coef.alloc(m->sell.nslots);
+    A->coef.alloc(size_t(m->sell.nslots) + m->sell.nslotsP);  // slot-major values (+ the pair copy)
     launch_sell_coef(st, m->sell.view(), A->upper.p, A->symmetric ? A->upper.p : A->lower.p, A->coef.p, nullptr);
     A->rD.alloc(m->n);
     VecArgs v;

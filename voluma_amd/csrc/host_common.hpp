@@ -309,6 +309,8 @@ struct SellDev {
     int n = 0, nslices = 0, nslots = 0, maxWidth = 0, csrG = 0, csrIt = 0, ellWidth = 0;
     DevBuf<int> sliceOff, col, src;
     DevBuf<int16_t> col16;
+    DevBuf<int> colP;  // packed rows (SellView::colP)
+    int colPW = 0, nslotsP = 0;
     SellView view() const {
         SellView v;
         v.n = n;
@@ -322,14 +324,19 @@ struct SellDev {
         v.src = src.p;
         v.col16 = col16.p;
         v.ellWidth = ellWidth;
+        v.colP = colP.p;
+        v.colPW = colPW;
+        v.nslotsP = nslotsP;
         return v;
     }
 };
 
-// Row lists (row → (col, src) in the given order) → SELL-64 on the device. With
-// try16, int16 column offsets are added when every neighbour lies within ±32767 rows.
+// Row lists (row → (col, src) in the given order) → SELL-64 on the device. With try16,
+// int16 slot-major column offsets under VOLUMA_COL16=1 when every neighbour lies within
+// ±32767 rows; with packed, the packed rows (SellView::colP) when the form allows them
+// (VOLUMA_PACKED=0: never); the owner of the coefficients then allocates nslots + nslotsP.
 inline void buildSell(int n, const std::vector<int>& off, const std::vector<int>& colL, const std::vector<int>& srcL,
-               SellDev& S, hipStream_t st, bool try16 = false) {
+               SellDev& S, hipStream_t st, bool try16 = false, bool packed = false) {
     S.n = n;
     S.nslices = (n + WAVE - 1) / WAVE;
     std::vector<int> sliceOff(S.nslices + 1, 0);
@@ -372,6 +379,29 @@ inline void buildSell(int n, const std::vector<int>& off, const std::vector<int>
             }
         }
         if (fits) S.col16.upload(c16, st);
+    }
+    const char* packEnv = std::getenv("VOLUMA_PACKED");
+    if (packed && S.ellWidth > 0 && S.ellWidth <= 8 && !(packEnv && std::string(packEnv) == "0")) {
+        const int W = S.ellWidth, PW = W <= 4 ? 4 : 8;  // offsets per row, padded
+        std::vector<int16_t> p16(size_t(S.nslices) * WAVE * PW, int16_t(COL16_PAD));
+        bool fits = true;
+        for (int r = 0; r < n && fits; ++r)
+            for (int k = off[r]; k < off[r + 1]; ++k) {
+                const int d = colL[k] - r;
+                if (d <= COL16_PAD || d > 32767) {
+                    fits = false;
+                    break;
+                }
+                p16[size_t(r) * PW + (k - off[r])] = int16_t(d);
+            }
+        if (fits) {
+            std::vector<int> words(p16.size() / 2);
+            for (size_t q = 0; q < words.size(); ++q)
+                words[q] = int((uint32_t(uint16_t(p16[2 * q + 1])) << 16) | uint16_t(p16[2 * q]));
+            S.colP.upload(words, st);
+            S.colPW = PW / 2;
+            S.nslotsP = S.nslices * WAVE * 2 * ((W + 1) / 2);
+        }
     }
 }
 

@@ -121,6 +121,98 @@ __device__ __forceinline__ T sell_row(const SellArgsT<T>& a, int row, T xu) {
     return outv;
 }
 
+// One row of a gather op over the packed rows (SellView::colP): its column offsets in
+// one vector load, its coefficients in ⌈W/2⌉ pair loads, then the x gathers; the
+// accumulation keeps the slot order, so the result has the bits of sell_row. PW: offsets
+// per row (4 or 8, ≥ ellWidth).
+template <class T> struct Pair;
+template <> struct Pair<double> { using type = double2; };
+template <> struct Pair<float> { using type = float2; };
+
+template <int OP, int XL, int PW, class T>
+__device__ __forceinline__ T sell_row_packed(const SellArgsT<T>& a, int row, T xu) {
+    using T2 = typename Pair<T>::type;
+    const int W = a.S.ellWidth, P = (W + 1) / 2;
+    int j[PW];
+    {
+        int w[PW / 2];
+        if constexpr (PW == 4) {
+            const int2 v = reinterpret_cast<const int2*>(a.S.colP)[row];
+            w[0] = v.x;
+            w[1] = v.y;
+        } else {
+            const int4 v = reinterpret_cast<const int4*>(a.S.colP)[row];
+            w[0] = v.x;
+            w[1] = v.y;
+            w[2] = v.z;
+            w[3] = v.w;
+        }
+#pragma unroll
+        for (int k = 0; k < PW; ++k) {
+            const int d = (k & 1) ? (w[k / 2] >> 16) : int(short(w[k / 2] & 0xffff));
+            j[k] = (k < W && d != COL16_PAD) ? row + d : -1;
+        }
+    }
+    T av[PW];
+    {
+        const T2* cp = reinterpret_cast<const T2*>(a.coef + a.S.nslots) + size_t(row / WAVE) * WAVE * P + row % WAVE;
+#pragma unroll
+        for (int p = 0; p < PW / 2; ++p) {
+            T2 v;
+            v.x = T(0);
+            v.y = T(0);
+            if (p < P) v = cp[p * WAVE];
+            av[2 * p] = v.x;
+            av[2 * p + 1] = v.y;
+        }
+    }
+    T acc;
+    if constexpr (OP == OP_AMUL) acc = a.diag[row] * xload<XL>(a, row, xu);
+    else if constexpr (OP == OP_RESID) acc = a.r[row] - a.diag[row] * xload<XL>(a, row, xu);
+    else acc = a.r[row];
+    T xv[PW];
+#pragma unroll
+    for (int k = 0; k < PW; ++k) xv[k] = j[k] >= 0 ? xload<XL>(a, j[k], xu) : T(0);
+#pragma unroll
+    for (int k = 0; k < PW; ++k) {
+        if (j[k] >= 0) {
+            const T t = av[k] * xv[k];
+            if constexpr (OP == OP_AMUL) acc = acc + t;
+            else acc = acc - t;
+        }
+    }
+    T outv;
+    if constexpr (OP == OP_JACOBI) {
+        const T J = acc / a.diag[row];
+        const T xo = xload<XL>(a, row, xu);
+        outv = xo + T(a.omega) * (J - xo);
+    } else if constexpr (OP == OP_JACREF) {
+        outv = acc / a.diag[row];
+    } else {
+        outv = acc;
+    }
+    a.out[row] = outv;
+    return outv;
+}
+
+template <int OP, int XL, bool DOT, int PW, class T>
+__global__ __launch_bounds__(256) void k_sell_packed(SellArgsT<T> a) {
+    if (a.done && *a.done) return;
+    const int lb = xcd_block();
+    const int slice = lb * SELL_WAVES + threadIdx.x / WAVE;
+    const int row = slice * WAVE + (threadIdx.x & (WAVE - 1));
+    const bool valid = slice < a.S.nslices && row < a.S.n;
+    T xu = T(0);
+    if constexpr (XL == XL_UNIFORM) xu = *a.xuPtr;
+    T outv = T(0);
+    if (valid) outv = sell_row_packed<OP, XL, PW>(a, row, xu);
+    if constexpr (DOT) {
+        __shared__ double sh[256 / WAVE];
+        const double v = block_sum(valid ? double(outv) * double(a.dotv[row]) : 0.0, sh);
+        if (threadIdx.x == 0) a.part[lb] = v;
+    }
+}
+
 template <int OP, int XL, bool DOT, int U, class T>
 __global__ __launch_bounds__(256) void k_sell(SellArgsT<T> a) {
     if (a.done && *a.done) return;
@@ -243,6 +335,21 @@ static void sell_dispatch2(hipStream_t st, const SellArgsT<T>& a) {
     }
     const int nb = sell_grid_blocks(a.S.nslices);
     if (nb == 0) return;
+    if (a.S.colP && !a.sliceSys) {  // packed rows (level 0 of a single system)
+        const bool fused = a.part != nullptr;
+        if constexpr (OP == OP_AMUL || (OP == OP_JACOBI && XL != XL_UNIFORM && XL != XL_FROMZERO)) {
+            if (fused) {
+                if (a.S.colPW == 2) sell_launch(k_sell_packed<OP, XL, true, 4, T>, nb, st, a);
+                else sell_launch(k_sell_packed<OP, XL, true, 8, T>, nb, st, a);
+                return;
+            }
+        }
+        if (!fused) {
+            if (a.S.colPW == 2) sell_launch(k_sell_packed<OP, XL, false, 4, T>, nb, st, a);
+            else sell_launch(k_sell_packed<OP, XL, false, 8, T>, nb, st, a);
+        }
+        return;
+    }
     const bool wide = a.S.maxWidth > 8;
     if (a.part) {  // fused Σ out·dotv: A·p, and the last sweep of the V-cycle (Σ w·r)
         if constexpr (OP == OP_AMUL || (OP == OP_JACOBI && XL != XL_UNIFORM && XL != XL_FROMZERO)) {
@@ -295,6 +402,13 @@ __global__ void k_sell_coef(SellView S, const double* upper, const double* lower
     const double v = s < 0 ? 0.0 : ((s & 1) ? lower[s >> 1] : upper[s >> 1]);
     if (coef) coef[i] = v;
     if (coef32) coef32[i] = float(v);
+    if (S.colP) {  // the pair copy: ELL slot k of row 64·slice + lane → pair k/2, element k%2
+        const int W = S.ellWidth, P = (W + 1) / 2;
+        const int slice = i / (WAVE * W), rem = i - slice * WAVE * W, k = rem / WAVE, lane = rem % WAVE;
+        const size_t j = size_t(S.nslots) + (size_t(slice) * WAVE * P + size_t(k / 2) * WAVE + lane) * 2 + (k & 1);
+        if (coef) coef[j] = v;
+        if (coef32) coef32[j] = float(v);
+    }
 }
 
 void launch_sell_coef(hipStream_t st, const SellView& S, const double* upper, const double* lower, double* coef,

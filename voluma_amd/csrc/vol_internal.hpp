@@ -37,6 +37,16 @@ struct SellView {
     // ELL form (ellWidth > 0): every slice has ellWidth slots, so slot k of row r sits at
     // (r/64)·64·ellWidth + 64k + r%64 and no slice offset has to be loaded first.
     int ellWidth = 0;
+    // Packed rows (level 0 in ELL form, ellWidth ≤ 8, every neighbour within ±32767 rows;
+    // colP != null): row r's column offsets (neighbour − r, COL16_PAD = padding) as int16
+    // in colPW words at colP + r·colPW (2 offsets per word, colPW = 2 or 4), and the
+    // coefficient array holds, after its nslots slot-major values, nslotsP more: the same
+    // values in pairs, pair p of row r at (r/64)·64·P + 64p + r%64 counted in pairs
+    // (P = ⌈ellWidth/2⌉). A gather then reads one vector load of offsets and P pair loads
+    // per row instead of 2·ellWidth scalar loads, and 2 bytes per column instead of 4.
+    const int* colP = nullptr;
+    int colPW = 0;
+    int nslotsP = 0;
 };
 constexpr int COL16_PAD = -32768;
 

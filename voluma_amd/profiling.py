@@ -65,9 +65,14 @@ def _template_args(kernel):
     return [a.strip() for a in m.group(1).split(",")] if m else []
 
 
+def _is_sell(kernel):
+    """A SELL-64 gather (slot-major or packed rows)."""
+    return kernel.startswith("k_sell<") or kernel.startswith("k_sell_packed<")
+
+
 def _is_amul(kernel):
     a = _template_args(kernel)
-    return kernel.startswith("k_sell<") and len(a) >= 3 and a[0] == "0" and a[1] == "0" and a[2] == "true"
+    return _is_sell(kernel) and len(a) >= 3 and a[0] == "0" and a[1] == "0" and a[2] == "true"
 
 
 def _iterations(rows, elem_type=None):
@@ -147,7 +152,7 @@ def levels(db, bench_json):
         lev, nbytes, rws, ent = "", None, "", ""
         if _is_amul(k):
             lev, nbytes = "0", _row_bytes("amul", cells[0], 2 * faces[0], 0, elem)
-        elif k.startswith("k_sell<") or k.startswith("k_csrv<"):
+        elif _is_sell(k) or k.startswith("k_csrv<"):
             if _template_args(k)[0] == "2":  # prolongation sweep one level up
                 level -= 1
                 kind = "post"
