@@ -275,14 +275,39 @@ __global__ __launch_bounds__(THREADS) void k_ell_run(LevelEll E, SweepArgs<T> a,
     __shared__ int sOff[ELL_RUN_LEVELS + 1], sWidth[ELL_RUN_LEVELS], sBase[ELL_RUN_LEVELS];
     __shared__ T xs[RUN_LDS_CELLS];
     const int nl = L1 - L0, t = int(threadIdx.x);
-    for (int i = t; i <= nl; i += THREADS) {
-        sOff[i] = E.offD[L0 + i];
-        if (i < nl) {
-            sWidth[i] = E.widthD[L0 + i];
-            sBase[i] = E.baseD[L0 + i];
+    // staging in batches: every load of a batch in flight together, then the stores (a
+    // plain loop waits for each element's load, or pair of dependent loads, in turn)
+    constexpr int SB = 16;
+    for (int i0 = t; i0 <= nl; i0 += SB * THREADS) {
+        int o[SB], w[SB], b[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const int i = i0 + u * THREADS;
+            o[u] = i <= nl ? E.offD[L0 + i] : 0;
+            w[u] = i < nl ? E.widthD[L0 + i] : 0;
+            b[u] = i < nl ? E.baseD[L0 + i] : 0;
+        }
+#pragma unroll
+        for (int u = 0; u < SB; ++u) {
+            const int i = i0 + u * THREADS;
+            if (i <= nl) sOff[i] = o[u];
+            if (i < nl) {
+                sWidth[i] = w[u];
+                sBase[i] = b[u];
+            }
         }
     }
-    for (int i = t; i < nCells; i += THREADS) xs[i] = a.x[cells[i]];
+    for (int i0 = t; i0 < nCells; i0 += SB * THREADS) {
+        int c[SB];
+        T v[SB];
+#pragma unroll
+        for (int u = 0; u < SB; ++u) c[u] = i0 + u * THREADS < nCells ? cells[i0 + u * THREADS] : -1;
+#pragma unroll
+        for (int u = 0; u < SB; ++u) v[u] = c[u] >= 0 ? a.x[c[u]] : T(0);
+#pragma unroll
+        for (int u = 0; u < SB; ++u)
+            if (c[u] >= 0) xs[i0 + u * THREADS] = v[u];
+    }
     __syncthreads();
     // past the last level, a thread loads the last level's data again (unused)
     auto lev = [nl](int i) { return i < nl ? i : nl - 1; };
@@ -577,8 +602,17 @@ __global__ __launch_bounds__(64) void k_coarse_solve_rows(int n, const double* l
     double* gv = sh;
     double* f = sh + n;
     for (int i = threadIdx.x; i < n; i += blockDim.x) gv[i] = double(r[i]);
-    if (small)
-        for (int i = threadIdx.x; i < n * n; i += blockDim.x) f[i] = lu[i];
+    if (small) {  // 16 loads in flight per lane, then their stores
+        constexpr int SB = 16;
+        for (int i0 = threadIdx.x; i0 < n * n; i0 += SB * 64) {
+            double v[SB];
+#pragma unroll
+            for (int u = 0; u < SB; ++u) v[u] = i0 + u * 64 < n * n ? lu[i0 + u * 64] : 0.0;
+#pragma unroll
+            for (int u = 0; u < SB; ++u)
+                if (i0 + u * 64 < n * n) f[i0 + u * 64] = v[u];
+        }
+    }
     __syncthreads();
     const double* a = small ? f : lu;
     for (int k = 0; k < n; ++k) {
