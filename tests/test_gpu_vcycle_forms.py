@@ -7,6 +7,9 @@ context switch read from the environment when the context is created:
   (VOLUMA_PROLONG_SPLIT=1);
 * the levels from the first of at most VOLUMA_TAIL_MAX_CELLS cells (up to 4096) down
   to the coarsest run in one workgroup, matrices preloaded into LDS where they fit;
+* the last post-sweep of a coarse level adds its result to the finer level's vector
+  (the pushed prolongation, VOLUMA_PUSH_PROLONG, default) instead of the finer sweep
+  gathering x + xc[agg];
 * levels compute their residual and its restriction (into the coarsest level: and its
   substitution, and the post-sweep of the level) in one launch (VOLUMA_FUSED_RESTRICT,
   default), SELL levels from a copy of their rows in restriction order
@@ -27,9 +30,11 @@ from voluma_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 PLAIN = {"VOLUMA_COARSE_Z": "0", "VOLUMA_LEVEL0_Z": "0", "VOLUMA_PROLONG_SPLIT": "0", "VOLUMA_TAIL_MAX_CELLS": "0",
-         "VOLUMA_FUSED_RESTRICT": "0"}
+         "VOLUMA_FUSED_RESTRICT": "0", "VOLUMA_PUSH_PROLONG": "0"}
 FORMS = {
     "default": {},
+    "gathered_prolongation": {"VOLUMA_PUSH_PROLONG": "0"},
+    "pushed_prolongation_unfused": {"VOLUMA_FUSED_RESTRICT": "0"},
     "fused_sell_levels": {"VOLUMA_FUSED_SELL_MAX": "1048576"},
     "fused_restrict_no_coarse_z": {"VOLUMA_COARSE_Z": "0"},
     "prolong_split": {"VOLUMA_PROLONG_SPLIT": "1"},

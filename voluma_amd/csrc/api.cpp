@@ -1160,11 +1160,25 @@ const RedScratch& redOf(const Ctx& C) { return C.red ? *C.red : C.c->red; }
 void launchSell(hipStream_t st, int op, int xl, const SellArgs& a) { launch_sell(st, op, xl, a); }
 void launchSell(hipStream_t st, int op, int xl, const SellArgs32& a) { launch_sell32(st, op, xl, a); }
 
+// Where a level's final x goes besides its output: added to the finer level's vector z
+// over the aggregates' member lists (the pushed prolongation).
+template <class T> struct PushTarget {
+    T* z = nullptr;
+    const int* mOff = nullptr;
+    const int* mList = nullptr;
+};
+
 template <class T>
 void sell(const Ctx& C, int op, int xl, const SellView& V, const T* coef, const T* diag, const T* x, const T* r,
           T* out, double omega = 1.0, const T* xc = nullptr, const int* agg = nullptr, const T* dotv = nullptr,
-          double* part = nullptr, const T* xu = nullptr, const T* rD = nullptr) {
+          double* part = nullptr, const T* xu = nullptr, const T* rD = nullptr,
+          const PushTarget<T>* push = nullptr) {
     SellArgsT<T> a;
+    if (push) {
+        a.pushZ = push->z;
+        a.pushOff = push->mOff;
+        a.pushList = push->mList;
+    }
     a.S = V;
     a.coef = coef;
     a.diag = diag;
@@ -1343,19 +1357,21 @@ bool levelUsesZ(vol_matrix A, const Plan& P, int l) {
 
 // One V-cycle at level l: r → x (GAMG.cpp:403-443 structure, damped-Jacobi smoothing).
 // With part != nullptr (level 0 outside the tail), the last post-sweep also leaves the
-// per-block partials of Σ x·dotv in part.
+// per-block partials of Σ x·dotv in part. With push, the launch that produces the final
+// x also adds it to the finer level's vector over the aggregates (returns true; false
+// when it was not pushed and the finer level has to gather x itself).
 template <class T>
-void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xout, const T* dotv = nullptr,
-            double* part = nullptr, T* z0 = nullptr) {
+bool vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xout, const T* dotv = nullptr,
+            double* part = nullptr, T* z0 = nullptr, const PushTarget<T>* push = nullptr) {
     vol_mesh m = A->mesh;
     const int nL = int(m->lv.size());
     if (l == A->tailStart) {
         tailLaunch(C, A, P, l, r, xout);
-        return;
+        return false;
     }
     if (l == nL - 1) {
         coarseSolve(C, A, r, xout);
-        return;
+        return false;
     }
     const LevelDev& D = *m->lv[l];
     const SellView V = l == 0 ? m->sell.view() : D.sell.view();
@@ -1374,6 +1390,16 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
     const LevelPtrs<T> LC = levelPtrs(A, l + 1, (T*)nullptr);
     const bool toCoarsest = l + 1 == nL - 1 && A->nCoarse <= 64 && A->tailStart != l + 1;
     bool postDone = false;  // the post-sweep ran in the coarsest launch
+    // the vector the first post-sweep gathers with the prolongation (x + xc[agg]): the
+    // coarser level may push xc into it instead
+    T* const prolongX = z ? z : (P.nPre == 1 ? nullptr : bufs[cur]);
+    PushTarget<T> down;
+    if (m->ctx->pushProlong && prolongX && !(l == 0 && m->ctx->prolongSplit)) {
+        down.z = prolongX;
+        down.mOff = D.mOff.p;
+        down.mList = D.mList.p;
+    }
+    bool pushed = false;  // xc already added to prolongX
     if (m->ctx->fusedRestrict && D.rrBlocks > 0 &&
         (!toCoarsest || (D.n <= RR_COARSE_ROWS && V.maxWidth <= 64))) {
         // residual and restriction in one launch (no residual vector); into the coarsest
@@ -1405,6 +1431,11 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
                 R.post = 1;
                 a.agg = D.agg.p;
                 a.out = xout;
+                if (push) {
+                    a.pushZ = push->z;
+                    a.pushOff = push->mOff;
+                    a.pushList = push->mList;
+                }
             }
             launch_resid_restrict_coarse64<T>(C.st, xl, a.S.csrG, a.S.csrIt, a, R);
         } else {
@@ -1414,7 +1445,7 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
                 R.rDc = LC.rD;
             }
             launch_resid_restrict<T>(C.st, xl, a.S.csrG, a.S.csrIt, a, R);
-            vcycle<T>(C, A, P, l + 1, LC.r, LC.x);
+            pushed = vcycle<T>(C, A, P, l + 1, LC.r, LC.x, nullptr, nullptr, nullptr, down.z ? &down : nullptr);
         }
     } else {
         // residual of the pre-smoothed x, then restriction
@@ -1432,34 +1463,41 @@ void vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
                 restrictLevel(C, m->lv[l + 1]->n, D, bufs[freeB], LC.r, LC.z, LC.rD, P.omega);
             else
                 restrictLevel(C, m->lv[l + 1]->n, D, bufs[freeB], LC.r);
-            vcycle<T>(C, A, P, l + 1, LC.r, LC.x);
+            pushed = vcycle<T>(C, A, P, l + 1, LC.r, LC.x, nullptr, nullptr, nullptr, down.z ? &down : nullptr);
         }
     }
-    if (postDone) return;
-    // prolongation fused into the first post-sweep
+    if (postDone) return push != nullptr;
+    // prolongation fused into the first post-sweep (or already pushed into prolongX)
     int src = cur, other = freeB;
     for (int s = 0; s < P.nPost; ++s) {
         const bool last = s == P.nPost - 1;
         T* dst = last ? xout : bufs[other];
         const T* dv = last ? dotv : nullptr;
         double* pt = last ? part : nullptr;
-        if (s == 0 && z && l == 0 && m->ctx->prolongSplit) {
+        const PushTarget<T>* pu = last ? push : nullptr;
+        if (s == 0 && pushed) {
+            sell<T>(C, OP_JACOBI, XL_PLAIN, V, L.coef, L.diag, prolongX, r, dst, P.omega, nullptr, nullptr, dv, pt,
+                    nullptr, nullptr, pu);
+        } else if (s == 0 && z && l == 0 && m->ctx->prolongSplit) {
             // z += xc[agg] streamed first, so that the sweep gathers one vector
             launch_prolong<T>(C.st, m->n, D.agg.p, LC.x, z, C.done);
             sell<T>(C, OP_JACOBI, XL_PLAIN, V, L.coef, L.diag, z, r, dst, P.omega, nullptr, nullptr, dv, pt);
         } else if (s == 0 && z)
-            sell<T>(C, OP_JACOBI, XL_PROLONG, V, L.coef, L.diag, z, r, dst, P.omega, LC.x, D.agg.p, dv, pt);
+            sell<T>(C, OP_JACOBI, XL_PROLONG, V, L.coef, L.diag, z, r, dst, P.omega, LC.x, D.agg.p, dv, pt, nullptr,
+                    nullptr, pu);
         else if (s == 0)
             sell<T>(C, OP_JACOBI, P.nPre == 1 ? XL_ZEROPROLONG : XL_PROLONG, V, L.coef, L.diag,
-                    P.nPre == 1 ? nullptr : bufs[src], r, dst, P.omega, LC.x, D.agg.p, dv, pt, nullptr, L.rD);
+                    P.nPre == 1 ? nullptr : bufs[src], r, dst, P.omega, LC.x, D.agg.p, dv, pt, nullptr, L.rD, pu);
         else
-            sell<T>(C, OP_JACOBI, XL_PLAIN, V, L.coef, L.diag, bufs[src], r, dst, P.omega, nullptr, nullptr, dv, pt);
+            sell<T>(C, OP_JACOBI, XL_PLAIN, V, L.coef, L.diag, bufs[src], r, dst, P.omega, nullptr, nullptr, dv, pt,
+                    nullptr, nullptr, pu);
         if (s != P.nPost - 1) {
             const int t = src;
             src = other;
             other = t;
         }
     }
+    return push != nullptr;
 }
 
 // After a solve with sync-free sweeps (stream synchronised): a wait that ran past its
@@ -2095,6 +2133,7 @@ int vol_context_create(int device, vol_context* out) {
     c->level0Z = !envOff("VOLUMA_LEVEL0_Z");
     if (const char* e = std::getenv("VOLUMA_PROLONG_SPLIT")) c->prolongSplit = std::string(e) == "1";
     c->fusedRestrict = !envOff("VOLUMA_FUSED_RESTRICT");
+    c->pushProlong = !envOff("VOLUMA_PUSH_PROLONG");
     if (const char* e = std::getenv("VOLUMA_FUSED_SELL_MAX")) c->fusedSellMax = std::atoi(e);
     if (const char* e = std::getenv("VOLUMA_SWEEP")) c->freeSweeps = std::string(e) == "free";
     c->sweepFault.alloc(2);  // fault flag, work counter

@@ -860,6 +860,10 @@ struct vol_context_s {
     // 31.0 µs against 23.5 for the residual and restriction launches, and the following
     // post-sweeps slower as the SELL rows are no longer in cache; 22.8 against 21.4 ms)
     bool fusedRestrict = true;
+    // the last post-sweep of a coarse level adds its result to the finer level's vector
+    // over the aggregates, so that the finer post-sweep gathers one vector instead of
+    // x + xc[agg] per neighbour (VOLUMA_PUSH_PROLONG=0: gather)
+    bool pushProlong = true;
     int fusedSellMax = 0;
     // triangular sweeps (Gauss–Seidel, DIC/DILU substitutions) in their sync-free form,
     // one launch per sweep (VOLUMA_SWEEP=free; the same bits; measured slower than the

@@ -54,6 +54,17 @@ __device__ __forceinline__ T xload(const SellArgsT<T>& a, int j, T xu) {
 }
 
 
+// Pushed prolongation: row I's new value v added to the finer cells of aggregate I,
+// members g, g + G, … of its list (lane g of a group of G lanes that all hold v). Each
+// finer cell belongs to one aggregate, so no two lanes write the same cell.
+template <class T>
+__device__ __forceinline__ void push_prolongation(const SellArgsT<T>& a, int I, T v, int g, int G) {
+    for (int k = a.pushOff[I] + g, e = a.pushOff[I + 1]; k < e; k += G) {
+        const int m = a.pushList[k];
+        a.pushZ[m] = a.pushZ[m] + v;
+    }
+}
+
 // One row of a SELL gather op (the arithmetic shared by the grid kernel and the
 // one-workgroup coarse-level kernel). Slots are taken in groups of U: all column and
 // coefficient loads of a group, then all x gathers, are in flight together, and the
@@ -233,6 +244,9 @@ __global__ __launch_bounds__(256) void k_sell(SellArgsT<T> a) {
     if constexpr (XL == XL_UNIFORM) xu = *a.xuPtr;
     T outv = T(0);
     if (valid) outv = sell_row<OP, XL, U>(a, row, xu);
+    if constexpr (OP == OP_JACOBI) {
+        if (a.pushZ && valid) push_prolongation(a, row, outv, 0, 1);
+    }
     if constexpr (DOT) {  // Σ out·dotv, accumulated in double
         __shared__ double sh[256 / WAVE];
         const double v = block_sum(valid ? double(outv) * double(a.dotv[row]) : 0.0, sh);
@@ -304,7 +318,10 @@ __global__ __launch_bounds__(256) void k_csrv(SellArgsT<T> a) {
     if constexpr (XL == XL_UNIFORM) xu = *a.xuPtr;
     bool valid = row < a.S.n;
     if (valid && a.rowSys) valid = !a.sysS[a.rowSys[row]].done;  // batches: skip finished systems
-    csr_row<OP, XL, G, IT>(a, row, g, valid, xu);
+    const T v = csr_row<OP, XL, G, IT>(a, row, g, valid, xu);
+    if constexpr (OP == OP_JACOBI) {
+        if (a.pushZ && valid) push_prolongation(a, row, v, g, G);
+    }
 }
 
 template <class T, class K> static void sell_launch(K kernel, int nb, hipStream_t st, const SellArgsT<T>& a) {
@@ -1336,10 +1353,12 @@ __global__ __launch_bounds__(RR_COARSE_THREADS) void k_resid_restrict_coarse64_p
             if (q + G * it < rw.z) acc2 = acc2 - t;
         }
     }
-    if (valid && g == 0) {
+    if (valid) {
         const T J = acc2 / dd;
         const T xo = zr + xcL[agr];
-        a.out[rw.x] = xo + T(a.omega) * (J - xo);
+        const T v = xo + T(a.omega) * (J - xo);
+        if (g == 0) a.out[rw.x] = v;
+        if (a.pushZ) push_prolongation(a, rw.x, v, g, G);
     }
 }
 

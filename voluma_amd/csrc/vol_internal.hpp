@@ -139,6 +139,12 @@ template <class T> struct SellArgsT {
     const int2* sliceSys = nullptr;
     const int* rowSys = nullptr;
     const DevScalars* sysS = nullptr;
+    // prolongation pushed down (OP_JACOBI, nullable): row I's result is also added to
+    // pushZ[m] for the finer cells m of aggregate I (pushList[pushOff[I] .. pushOff[I+1])),
+    // which is the value x[m] + xc[agg[m]] the finer level's sweep would gather
+    T* pushZ = nullptr;
+    const int* pushOff = nullptr;
+    const int* pushList = nullptr;
     // host side: when set, the launch records these events at the kernel's start and end
     // (hipExtLaunchKernelGGL), which times the kernel itself
     hipEvent_t evStart = nullptr, evStop = nullptr;
