@@ -16,11 +16,14 @@ The ``levels`` table splits the trace into PCG iterations at the level-0 A·p la
 (``k_sell<0, 0, true, ...>``), keeps the iterations whose dispatch sequence is the most
 common one, and walks the V-cycle's fixed order (residual and restriction down the
 levels, separate or fused, coarsest solve, prolongation sweeps up) to name each
-dispatch's level. Bytes are
-what the kernel has to move given the stored layout, without its padding: per row of a
-level the vectors it reads and writes, per off-diagonal entry a coefficient and a column
-(8 + 4 bytes in FP64), the restriction's member lists and the coarsest LU factors; the
-x/r update counts the level-0 z it writes for the V-cycle (FP64 PCG)."""
+dispatch's level. Bytes are what the kernel has to move given the stored layout,
+without its padding: per row of a level the vectors it reads and writes, per
+off-diagonal entry a coefficient and a column (8 + 4 bytes in FP64; 8 + 2 on the packed
+level-0 rows, ``k_sell_packed``), the restriction's member lists and the coarsest LU
+factors; a post-sweep that gathers the prolongation (x-load 3) its agg and coarse x, one
+with a plain x-load above level 0 the member lists and the finer z it adds its result
+to (the pushed prolongation); the x/r update counts the level-0 z it writes for the
+V-cycle (FP64 PCG)."""
 import json
 import re
 import sqlite3
@@ -109,15 +112,18 @@ def precisions(db):
     return "\n".join(out)
 
 
-def _row_bytes(op_kind, n, nnz, nc, elem):
-    """Bytes of one gather sweep over n rows with nnz off-diagonal entries."""
-    mat = elem * n + (elem + 4) * nnz  # diag + (coefficient, column) per entry
+def _row_bytes(op_kind, n, nnz, nc, elem, col_bytes=4, n_fine=0):
+    """Bytes of one gather sweep over n rows with nnz off-diagonal entries (col_bytes per
+    entry for its column: 4, or 2 for the packed level-0 rows)."""
+    mat = elem * n + (elem + col_bytes) * nnz  # diag + (coefficient, column) per entry
     if op_kind == "amul":               # x, A·x, dot vector
         return mat + 3 * elem * n
     if op_kind == "resid":              # x (or z), r, residual
         return mat + 3 * elem * n
-    if op_kind == "post":               # z, r, result, agg, coarse x
+    if op_kind == "post":               # z, r, result, agg, coarse x (gathered prolongation)
         return mat + 3 * elem * n + 4 * n + elem * nc
+    if op_kind == "post_push":          # z, r, result; pushed into the finer level: lists, z read and written
+        return mat + 3 * elem * n + 4 * (n + 1) + 4 * n_fine + 2 * elem * n_fine
     return mat + 3 * elem * n
 
 
@@ -150,16 +156,20 @@ def levels(db, bench_json):
     level = 0
     for i, ((k, g), us) in enumerate(zip(shape, avg)):
         lev, nbytes, rws, ent = "", None, "", ""
+        cb = 2 if k.startswith("k_sell_packed<") else 4
         if _is_amul(k):
-            lev, nbytes = "0", _row_bytes("amul", cells[0], 2 * faces[0], 0, elem)
+            lev, nbytes = "0", _row_bytes("amul", cells[0], 2 * faces[0], 0, elem, cb)
         elif _is_sell(k) or k.startswith("k_csrv<"):
-            if _template_args(k)[0] == "2":  # prolongation sweep one level up
+            args = _template_args(k)
+            if args[0] == "2":  # post-sweep one level up
                 level -= 1
-                kind = "post"
+                # x-load 3: gathers x + xc[agg]; plain above level 0: pushes into the finer level
+                kind = "post" if args[1] == "3" else ("post_push" if level > 0 else "resid")
             else:
                 kind = "resid"
             nc = cells[level + 1] if level + 1 < nl else 0
-            lev, nbytes = str(level), _row_bytes(kind, cells[level], 2 * faces[level], nc, elem)
+            nfine = cells[level - 1] if level > 0 else 0
+            lev, nbytes = str(level), _row_bytes(kind, cells[level], 2 * faces[level], nc, elem, cb, nfine)
         elif k.startswith("k_resid_restrict"):
             # residual of level l (its rows through the restriction lists) summed into l + 1;
             # into the coarsest also the LU substitution, and with _post the level's sweep
