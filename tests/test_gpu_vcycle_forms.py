@@ -12,7 +12,8 @@ context switch read from the environment when the context is created:
   gathering x + xc[agg];
 * levels compute their residual and its restriction (into the coarsest level: and its
   substitution, and the post-sweep of the level) in one launch (VOLUMA_FUSED_RESTRICT,
-  default), SELL levels from a copy of their rows in restriction order
+  default), the packed level 0 reading its rows in restriction order
+  (VOLUMA_FUSED_LEVEL0), SELL levels from a copy of their rows in restriction order
   (VOLUMA_FUSED_SELL_MAX: up to that many rows).
 
 Every form computes the same per-row arithmetic in the same order, so solutions,
@@ -34,6 +35,7 @@ PLAIN = {"VOLUMA_COARSE_Z": "0", "VOLUMA_LEVEL0_Z": "0", "VOLUMA_PROLONG_SPLIT":
 FORMS = {
     "default": {},
     "gathered_prolongation": {"VOLUMA_PUSH_PROLONG": "0"},
+    "level0_residual_and_restriction_separate": {"VOLUMA_FUSED_LEVEL0": "0"},
     "pushed_prolongation_unfused": {"VOLUMA_FUSED_RESTRICT": "0"},
     "fused_sell_levels": {"VOLUMA_FUSED_SELL_MAX": "1048576"},
     "fused_restrict_no_coarse_z": {"VOLUMA_COARSE_Z": "0"},
@@ -42,7 +44,7 @@ FORMS = {
     "tail_no_coarse_z": {"VOLUMA_TAIL_MAX_CELLS": "128", "VOLUMA_COARSE_Z": "0"},
     "tail_4096": {"VOLUMA_TAIL_MAX_CELLS": "4096"},
 }
-KEYS = sorted(PLAIN) + ["VOLUMA_FUSED_SELL_MAX"]
+KEYS = sorted(PLAIN) + ["VOLUMA_FUSED_SELL_MAX", "VOLUMA_FUSED_LEVEL0"]
 
 
 def _context(env):

@@ -288,7 +288,10 @@ void uploadLevels(vol_mesh m, const std::vector<int>* layout) {
             // to the size limit a copy of their rows in the order of the restriction lists
             int lanes = 0;
             std::vector<int4> rows(G.mList.size());
-            if (k > 0 && D->layout == 0) {
+            if (k == 0 && m->sell.colP.p && m->ctx->fusedLevel0) {
+                lanes = 1;  // packed level-0 rows: one lane per member row, through mList
+                rows.clear();
+            } else if (k > 0 && D->layout == 0) {
                 lanes = D->sell.csrG;
                 for (size_t p = 0; p < rows.size(); ++p) {
                     const int i = G.mList[p];
@@ -315,7 +318,7 @@ void uploadLevels(vol_mesh m, const std::vector<int>* layout) {
                 std::vector<int2> b2(blk.size());
                 for (size_t q = 0; q < blk.size(); ++q) b2[q] = make_int2(blk[q], G.mOff[blk[q]]);
                 D->rrBlk.upload(b2, st);
-                D->rrRows.upload(rows, st);
+                if (!rows.empty()) D->rrRows.upload(rows, st);
                 D->rrBlocks = int(blk.size()) - 1;
             }
         }
@@ -1401,11 +1404,12 @@ bool vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
     }
     bool pushed = false;  // xc already added to prolongX
     if (m->ctx->fusedRestrict && D.rrBlocks > 0 &&
-        (!toCoarsest || (D.n <= RR_COARSE_ROWS && V.maxWidth <= 64))) {
+        (!toCoarsest || (l > 0 && D.n <= RR_COARSE_ROWS && V.maxWidth <= 64))) {
         // residual and restriction in one launch (no residual vector); into the coarsest
         // level also its substitution
         SellArgsT<T> a;
-        const bool own = D.layout == 0;  // CSR level: its own rows; SELL: the restriction-order copy
+        // CSR level or packed level 0: its own rows; SELL: the restriction-order copy
+        const bool own = D.layout == 0 || l == 0;
         a.S = own ? V : D.rrCsr.view();
         a.coef = own ? L.coef : restrictCoef(A->L[l].get(), (T*)nullptr);
         a.diag = L.diag;
@@ -1421,6 +1425,7 @@ bool vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
         R.nBlocks = D.rrBlocks;
         R.rows = D.rrRows.p;
         R.mOff = D.mOff.p;
+        R.mList = D.mList.p;
         R.omega = P.omega;
         if (toCoarsest) {
             R.rc = LC.x;
@@ -2134,6 +2139,7 @@ int vol_context_create(int device, vol_context* out) {
     if (const char* e = std::getenv("VOLUMA_PROLONG_SPLIT")) c->prolongSplit = std::string(e) == "1";
     c->fusedRestrict = !envOff("VOLUMA_FUSED_RESTRICT");
     c->pushProlong = !envOff("VOLUMA_PUSH_PROLONG");
+    c->fusedLevel0 = !envOff("VOLUMA_FUSED_LEVEL0");
     if (const char* e = std::getenv("VOLUMA_FUSED_SELL_MAX")) c->fusedSellMax = std::atoi(e);
     if (const char* e = std::getenv("VOLUMA_SWEEP")) c->freeSweeps = std::string(e) == "free";
     c->sweepFault.alloc(2);  // fault flag, work counter

@@ -864,6 +864,10 @@ struct vol_context_s {
     // over the aggregates, so that the finer post-sweep gathers one vector instead of
     // x + xc[agg] per neighbour (VOLUMA_PUSH_PROLONG=0: gather)
     bool pushProlong = true;
+    // level 0 on packed rows computes its residual and restriction in one launch too,
+    // reading the rows in restriction order (VOLUMA_FUSED_LEVEL0=0: separate launches;
+    // read when a mesh is set up)
+    bool fusedLevel0 = true;
     int fusedSellMax = 0;
     // triangular sweeps (Gauss–Seidel, DIC/DILU substitutions) in their sync-free form,
     // one launch per sweep (VOLUMA_SWEEP=free; the same bits; measured slower than the

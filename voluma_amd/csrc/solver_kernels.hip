@@ -202,7 +202,7 @@ __device__ __forceinline__ T sell_row_packed(const SellArgsT<T>& a, int row, T x
     } else {
         outv = acc;
     }
-    a.out[row] = outv;
+    if (a.out) a.out[row] = outv;  // (the fused restriction keeps the value only)
     return outv;
 }
 
@@ -1362,6 +1362,39 @@ __global__ __launch_bounds__(RR_COARSE_THREADS) void k_resid_restrict_coarse64_p
     }
 }
 
+// The same on the packed level-0 rows (SellView::colP): one lane per member row, reached
+// through the restriction list (R.mList), with sell_row_packed's arithmetic.
+template <int XL, int PW, class T>
+__global__ __launch_bounds__(256) void k_resid_restrict_packed(SellArgsT<T> a, RestrictArgsT<T> R) {
+    if (a.done && *a.done) return;
+    const int b = xcd_block();
+    if (b >= R.nBlocks) return;
+    __shared__ T res[RR_POS];
+    const int2 lo = R.blk[b], hi = R.blk[b + 1];
+    const int I = lo.x + int(threadIdx.x);
+    int mb = 0, me = 0;
+    if (I < hi.x) {
+        mb = R.mOff[I] - lo.y;
+        me = R.mOff[I + 1] - lo.y;
+    }
+    for (int p = lo.y + int(threadIdx.x); p < hi.y; p += 256) res[p - lo.y] = sell_row_packed<OP_RESID, XL, PW>(a, R.mList[p], T(0));
+    __syncthreads();
+    if (I < hi.x) {
+        T s = T(0);
+        for (int k = mb; k < me; ++k) s += res[k];
+        R.rc[I] = s;
+        if (R.zc) R.zc[I] = T(R.omega) * (s * R.rDc[I]);
+    }
+}
+
+template <int XL, class T>
+static void rr_packed(hipStream_t st, const SellArgsT<T>& a, const RestrictArgsT<T>& R) {
+    if (a.S.colPW == 2)
+        hipLaunchKernelGGL((k_resid_restrict_packed<XL, 4, T>), dim3(xcd_grid(R.nBlocks)), dim3(256), 0, st, a, R);
+    else
+        hipLaunchKernelGGL((k_resid_restrict_packed<XL, 8, T>), dim3(xcd_grid(R.nBlocks)), dim3(256), 0, st, a, R);
+}
+
 template <int XL, int G, int IT, class T>
 static void rr_launch(hipStream_t st, bool coarse, const SellArgsT<T>& a, const RestrictArgsT<T>& R) {
     if (coarse && XL == XL_PLAIN && R.post)
@@ -1395,7 +1428,13 @@ static void rr_any(hipStream_t st, bool coarse, int xl, int G, int IT, const Sel
 }
 template <class T>
 void launch_resid_restrict(hipStream_t st, int xl, int G, int IT, const SellArgsT<T>& a, const RestrictArgsT<T>& R) {
-    if (R.nBlocks > 0) rr_any(st, false, xl, G, IT, a, R);
+    if (R.nBlocks == 0) return;
+    if (a.S.colP) {  // packed level-0 rows
+        if (xl == XL_FROMZERO) rr_packed<XL_FROMZERO>(st, a, R);
+        else rr_packed<XL_PLAIN>(st, a, R);
+        return;
+    }
+    rr_any(st, false, xl, G, IT, a, R);
 }
 template <class T>
 void launch_resid_restrict_coarse64(hipStream_t st, int xl, int G, int IT, const SellArgsT<T>& a,

@@ -253,6 +253,7 @@ template <class T> struct RestrictArgsT {
     int nBlocks = 0;
     const int4* rows = nullptr;
     const int* mOff = nullptr;
+    const int* mList = nullptr;  // packed level-0 rows: the member rows by position
     T* rc = nullptr;
     T* zc = nullptr;
     const T* rDc = nullptr;
@@ -266,6 +267,8 @@ template <class T> struct RestrictArgsT {
 constexpr int RR_COARSE_THREADS = 512;
 // lanes per row of the coarsest form
 inline int rrCoarseLanes(int maxWidth) { return maxWidth <= 32 ? 4 : 8; }
+// (a.S.colP set: the packed level-0 rows, one lane per member row through R.mList; a.out
+// must be null)
 template <class T> void launch_resid_restrict(hipStream_t st, int xl, int G, int IT, const SellArgsT<T>& a,
                                               const RestrictArgsT<T>& R);
 // The same into a coarsest level of nc ≤ 64 cells, followed in the same launch by its
