@@ -36,6 +36,7 @@ FORMS = {
     "default": {},
     "gathered_prolongation": {"VOLUMA_PUSH_PROLONG": "0"},
     "level0_residual_and_restriction_separate": {"VOLUMA_FUSED_LEVEL0": "0"},
+    "sell_levels_fused_direct": {"VOLUMA_FUSED_SELL_DIRECT": "1"},
     "pushed_prolongation_unfused": {"VOLUMA_FUSED_RESTRICT": "0"},
     "fused_sell_levels": {"VOLUMA_FUSED_SELL_MAX": "1048576"},
     "fused_restrict_no_coarse_z": {"VOLUMA_COARSE_Z": "0"},
@@ -44,7 +45,7 @@ FORMS = {
     "tail_no_coarse_z": {"VOLUMA_TAIL_MAX_CELLS": "128", "VOLUMA_COARSE_Z": "0"},
     "tail_4096": {"VOLUMA_TAIL_MAX_CELLS": "4096"},
 }
-KEYS = sorted(PLAIN) + ["VOLUMA_FUSED_SELL_MAX", "VOLUMA_FUSED_LEVEL0"]
+KEYS = sorted(PLAIN) + ["VOLUMA_FUSED_SELL_MAX", "VOLUMA_FUSED_LEVEL0", "VOLUMA_FUSED_SELL_DIRECT"]
 
 
 def _context(env):

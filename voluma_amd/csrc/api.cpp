@@ -291,6 +291,9 @@ void uploadLevels(vol_mesh m, const std::vector<int>* layout) {
             if (k == 0 && m->sell.colP.p && m->ctx->fusedLevel0) {
                 lanes = 1;  // packed level-0 rows: one lane per member row, through mList
                 rows.clear();
+            } else if (k > 0 && D->layout == 1 && m->ctx->fusedSellDirect && F.n > m->ctx->fusedSellMax) {
+                lanes = 1;  // a SELL level's own rows, likewise
+                rows.clear();
             } else if (k > 0 && D->layout == 0) {
                 lanes = D->sell.csrG;
                 for (size_t p = 0; p < rows.size(); ++p) {
@@ -1404,12 +1407,12 @@ bool vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
     }
     bool pushed = false;  // xc already added to prolongX
     if (m->ctx->fusedRestrict && D.rrBlocks > 0 &&
-        (!toCoarsest || (l > 0 && D.n <= RR_COARSE_ROWS && V.maxWidth <= 64))) {
+        (!toCoarsest || (l > 0 && D.rrRows.p && D.n <= RR_COARSE_ROWS && V.maxWidth <= 64))) {
         // residual and restriction in one launch (no residual vector); into the coarsest
         // level also its substitution
         SellArgsT<T> a;
-        // CSR level or packed level 0: its own rows; SELL: the restriction-order copy
-        const bool own = D.layout == 0 || l == 0;
+        // its own rows (CSR, packed level 0, SELL read directly) or the restriction-order copy
+        const bool own = D.rrCsr.nslots == 0;
         a.S = own ? V : D.rrCsr.view();
         a.coef = own ? L.coef : restrictCoef(A->L[l].get(), (T*)nullptr);
         a.diag = L.diag;
@@ -2140,6 +2143,7 @@ int vol_context_create(int device, vol_context* out) {
     c->fusedRestrict = !envOff("VOLUMA_FUSED_RESTRICT");
     c->pushProlong = !envOff("VOLUMA_PUSH_PROLONG");
     c->fusedLevel0 = !envOff("VOLUMA_FUSED_LEVEL0");
+    if (const char* e = std::getenv("VOLUMA_FUSED_SELL_DIRECT")) c->fusedSellDirect = std::string(e) == "1";
     if (const char* e = std::getenv("VOLUMA_FUSED_SELL_MAX")) c->fusedSellMax = std::atoi(e);
     if (const char* e = std::getenv("VOLUMA_SWEEP")) c->freeSweeps = std::string(e) == "free";
     c->sweepFault.alloc(2);  // fault flag, work counter

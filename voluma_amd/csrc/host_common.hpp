@@ -868,6 +868,11 @@ struct vol_context_s {
     // reading the rows in restriction order (VOLUMA_FUSED_LEVEL0=0: separate launches;
     // read when a mesh is set up)
     bool fusedLevel0 = true;
+    // the SELL coarse levels likewise, reading their own SELL rows in restriction order
+    // (VOLUMA_FUSED_SELL_DIRECT=1; default off: measured slower on the 2.06M-cell tets,
+    // level 1 35.8 µs against 21 µs for the two launches, as a wave's rows then share few
+    // slot cache lines; 20.9 against 19.7 ms per solve)
+    bool fusedSellDirect = false;
     int fusedSellMax = 0;
     // triangular sweeps (Gauss–Seidel, DIC/DILU substitutions) in their sync-free form,
     // one launch per sweep (VOLUMA_SWEEP=free; the same bits; measured slower than the

@@ -128,7 +128,7 @@ __device__ __forceinline__ T sell_row(const SellArgsT<T>& a, int row, T xu) {
     } else {
         outv = acc;
     }
-    a.out[row] = outv;
+    if (a.out) a.out[row] = outv;  // (the fused restriction keeps the value only)
     return outv;
 }
 
@@ -1387,6 +1387,38 @@ __global__ __launch_bounds__(256) void k_resid_restrict_packed(SellArgsT<T> a, R
     }
 }
 
+// The same on a SELL level's own rows (one lane per member row, sell_row's arithmetic).
+template <int XL, int U, class T>
+__global__ __launch_bounds__(256) void k_resid_restrict_sell(SellArgsT<T> a, RestrictArgsT<T> R) {
+    if (a.done && *a.done) return;
+    const int b = xcd_block();
+    if (b >= R.nBlocks) return;
+    __shared__ T res[RR_POS];
+    const int2 lo = R.blk[b], hi = R.blk[b + 1];
+    const int I = lo.x + int(threadIdx.x);
+    int mb = 0, me = 0;
+    if (I < hi.x) {
+        mb = R.mOff[I] - lo.y;
+        me = R.mOff[I + 1] - lo.y;
+    }
+    for (int p = lo.y + int(threadIdx.x); p < hi.y; p += 256) res[p - lo.y] = sell_row<OP_RESID, XL, U>(a, R.mList[p], T(0));
+    __syncthreads();
+    if (I < hi.x) {
+        T s = T(0);
+        for (int k = mb; k < me; ++k) s += res[k];
+        R.rc[I] = s;
+        if (R.zc) R.zc[I] = T(R.omega) * (s * R.rDc[I]);
+    }
+}
+
+template <int XL, class T>
+static void rr_sell(hipStream_t st, const SellArgsT<T>& a, const RestrictArgsT<T>& R) {
+    if (a.S.maxWidth > 8)
+        hipLaunchKernelGGL((k_resid_restrict_sell<XL, 16, T>), dim3(xcd_grid(R.nBlocks)), dim3(256), 0, st, a, R);
+    else
+        hipLaunchKernelGGL((k_resid_restrict_sell<XL, 4, T>), dim3(xcd_grid(R.nBlocks)), dim3(256), 0, st, a, R);
+}
+
 template <int XL, class T>
 static void rr_packed(hipStream_t st, const SellArgsT<T>& a, const RestrictArgsT<T>& R) {
     if (a.S.colPW == 2)
@@ -1432,6 +1464,11 @@ void launch_resid_restrict(hipStream_t st, int xl, int G, int IT, const SellArgs
     if (a.S.colP) {  // packed level-0 rows
         if (xl == XL_FROMZERO) rr_packed<XL_FROMZERO>(st, a, R);
         else rr_packed<XL_PLAIN>(st, a, R);
+        return;
+    }
+    if (a.S.csrG == 0) {  // a SELL level's own rows
+        if (xl == XL_FROMZERO) rr_sell<XL_FROMZERO>(st, a, R);
+        else rr_sell<XL_PLAIN>(st, a, R);
         return;
     }
     rr_any(st, false, xl, G, IT, a, R);
