@@ -1438,7 +1438,7 @@ bool vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
             if (postDone) {
                 R.post = 1;
                 a.agg = D.agg.p;
-                a.out = xout;
+                a.out = push ? nullptr : xout;  // pushed: nothing reads this level's x
                 if (push) {
                     a.pushZ = push->z;
                     a.pushOff = push->mOff;
@@ -1479,10 +1479,10 @@ bool vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
     int src = cur, other = freeB;
     for (int s = 0; s < P.nPost; ++s) {
         const bool last = s == P.nPost - 1;
-        T* dst = last ? xout : bufs[other];
+        const PushTarget<T>* pu = last ? push : nullptr;
+        T* dst = last ? (pu ? nullptr : xout) : bufs[other];  // pushed: nothing reads this level's x
         const T* dv = last ? dotv : nullptr;
         double* pt = last ? part : nullptr;
-        const PushTarget<T>* pu = last ? push : nullptr;
         if (s == 0 && pushed) {
             sell<T>(C, OP_JACOBI, XL_PLAIN, V, L.coef, L.diag, prolongX, r, dst, P.omega, nullptr, nullptr, dv, pt,
                     nullptr, nullptr, pu);

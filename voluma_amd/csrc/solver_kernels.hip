@@ -304,7 +304,7 @@ __device__ __forceinline__ T csr_row(const SellArgsT<T>& a, int row, int g, bool
         } else {
             outv = acc;
         }
-        if (g == 0) a.out[row] = outv;
+        if (g == 0 && a.out) a.out[row] = outv;
     }
     return outv;
 }
@@ -1357,7 +1357,7 @@ __global__ __launch_bounds__(RR_COARSE_THREADS) void k_resid_restrict_coarse64_p
         const T J = acc2 / dd;
         const T xo = zr + xcL[agr];
         const T v = xo + T(a.omega) * (J - xo);
-        if (g == 0) a.out[rw.x] = v;
+        if (g == 0 && a.out) a.out[rw.x] = v;
         if (a.pushZ) push_prolongation(a, rw.x, v, g, G);
     }
 }
