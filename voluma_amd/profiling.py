@@ -122,8 +122,8 @@ def _row_bytes(op_kind, n, nnz, nc, elem, col_bytes=4, n_fine=0):
         return mat + 3 * elem * n
     if op_kind == "post":               # z, r, result, agg, coarse x (gathered prolongation)
         return mat + 3 * elem * n + 4 * n + elem * nc
-    if op_kind == "post_push":          # z, r, result; pushed into the finer level: lists, z read and written
-        return mat + 3 * elem * n + 4 * (n + 1) + 4 * n_fine + 2 * elem * n_fine
+    if op_kind == "post_push":          # z, r; pushed into the finer level: lists, z read and written
+        return mat + 2 * elem * n + 4 * (n + 1) + 4 * n_fine + 2 * elem * n_fine
     return mat + 3 * elem * n
 
 
@@ -176,11 +176,14 @@ def levels(db, bench_json):
             n, nc = cells[level], cells[level + 1]
             lev = f"{level}→{level + 1}"
             rws, ent = n, 2 * faces[level]
-            mat = elem * n + (elem + 4) * ent
-            nbytes = mat + 2 * elem * n + 16 * n + 4 * (nc + 1) + 3 * elem * nc  # r, z; rows; lists; r, z, 1/diag
+            if k.startswith("k_resid_restrict_packed"):  # packed rows, reached through the member list
+                mat, rowref = elem * n + (elem + 2) * ent, 4 * n
+            else:  # CSR rows (or a SELL level's), reached through row descriptors
+                mat, rowref = elem * n + (elem + 4) * ent, 16 * n
+            nbytes = mat + 2 * elem * n + rowref + 4 * (nc + 1) + 3 * elem * nc  # r, z; rows; lists; r, z, 1/diag
             if "coarse64" in k:
                 nbytes += 8 * nc * nc
-                if "_post" in k:  # the sweep re-reads r, z, agg of the rows and writes x
+                if "_post" in k:  # the sweep re-reads r, z, agg of the rows (and writes x or pushes it)
                     nbytes += 2 * elem * n + 4 * n + elem * n
             level += 1
             if "_post" in k:
