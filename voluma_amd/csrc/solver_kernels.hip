@@ -57,11 +57,22 @@ __device__ __forceinline__ T xload(const SellArgsT<T>& a, int j, T xu) {
 // Pushed prolongation: row I's new value v added to the finer cells of aggregate I,
 // members g, g + G, … of its list (lane g of a group of G lanes that all hold v). Each
 // finer cell belongs to one aggregate, so no two lanes write the same cell.
+// Members in groups of 8: every index load of a group, then every z load, are in flight
+// together.
 template <class T>
 __device__ __forceinline__ void push_prolongation(const SellArgsT<T>& a, int I, T v, int g, int G) {
-    for (int k = a.pushOff[I] + g, e = a.pushOff[I + 1]; k < e; k += G) {
-        const int m = a.pushList[k];
-        a.pushZ[m] = a.pushZ[m] + v;
+    constexpr int U = 8;
+    const int e = a.pushOff[I + 1];
+    for (int k0 = a.pushOff[I] + g; k0 < e; k0 += U * G) {
+        int m[U];
+        T z[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) m[u] = k0 + u * G < e ? a.pushList[k0 + u * G] : -1;
+#pragma unroll
+        for (int u = 0; u < U; ++u) z[u] = m[u] >= 0 ? a.pushZ[m[u]] : T(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (m[u] >= 0) a.pushZ[m[u]] = z[u] + v;
     }
 }
 
