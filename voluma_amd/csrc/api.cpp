@@ -1680,7 +1680,7 @@ void krylovIteration(const Ctx& C, vol_matrix A, const Plan& P, const Krylov<T>&
             fin(C, FIN_PCG_WAPA, 1, ex, nparts);
         }
         vec<T>(C, V_PCG_XR, n, ex, k.p, k.q, nullptr, nullptr, k.z0, k.x, k.r, k.z0 ? L0.rD : nullptr, P.omega);
-        fin(C, FIN_PCG_NORM, 1, ex, nparts, C.hostOut);
+        fin(C, FIN_PCG_NORM, 1, ex, ex ? nparts : vec_fast_blocks(V_PCG_XR, n, redOf(C).maxBlk), C.hostOut);
     } else {  // PBiCGStab, LinearSolver.cpp:202-237
         vec<T>(C, V_DOT, n, ex, k.r0, k.r);
         fin(C, FIN_BI_RHO, 1, ex, nparts);

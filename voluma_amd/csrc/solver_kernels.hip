@@ -446,6 +446,18 @@ void launch_sell_coef(hipStream_t st, const SellView& S, const double* upper, co
 }
 
 // ============================================================================ vector kernels
+// Grid of a fast vector op, which is also the partial count of its reduction. An element
+// step stores before the next element's loads may issue (the vectors may alias), so a
+// thread's elements are one memory round trip each: PCG's x/r update, the widest of
+// them, runs up to 4·RED_BLOCKS blocks (within the partial scratch, maxBlk) like the ops
+// without a reduction; other reductions use RED_BLOCKS blocks.
+int vec_fast_blocks(int op, int n, int maxBlk) {
+    const int rows = std::max((n + RED_THREADS - 1) / RED_THREADS, 1);
+    if (op == V_PCG_XR) return std::max(std::min({rows, 4 * RED_BLOCKS, maxBlk}), 1);
+    if (vec_has_reduction(op) > 0) return RED_BLOCKS;
+    return std::min(rows, 4 * RED_BLOCKS);
+}
+
 int vec_has_reduction(int op) {
     switch (op) {
     case V_SUM_X: case V_PRECOND_DOT: case V_DOT: case V_PCG_XR: case V_BI_S: case V_BMINUSAX_NORM: return 1;
@@ -590,15 +602,8 @@ static void vec_dispatch(hipStream_t st, const VecArgsT<T>& a, bool exact, const
             return;
         }
     }
-    {
-        int nb = RED_BLOCKS;
-        if (NRED == 0) {
-            nb = (a.n + RED_THREADS - 1) / RED_THREADS;
-            if (nb > 4 * RED_BLOCKS) nb = 4 * RED_BLOCKS;
-            if (nb < 1) nb = 1;
-        }
-        hipLaunchKernelGGL((k_vec_fast<OP, NRED, T>), dim3(nb), dim3(RED_THREADS), 0, st, a, R.part, R.maxBlk);
-    }
+    hipLaunchKernelGGL((k_vec_fast<OP, NRED, T>), dim3(vec_fast_blocks(OP, a.n, R.maxBlk)), dim3(RED_THREADS), 0, st,
+                       a, R.part, R.maxBlk);
 }
 
 template <class T>

@@ -211,6 +211,8 @@ void launch_mix(hipStream_t st, int op, int n, const double* a, float* w, float*
 void launch_finalize(hipStream_t st, int op, int nred, bool exact, int nparts, const RedScratch& R, DevScalars* S,
                      DevScalars* hostOut = nullptr);
 int vec_has_reduction(int op);
+// blocks of a fast-sum vector op (= partials of its reduction; maxBlk: the scratch's)
+int vec_fast_blocks(int op, int n, int maxBlk);
 
 void launch_clear_exit(hipStream_t st, DevScalars* S);
 void launch_init_scalars(hipStream_t st, DevScalars* S, double tol, double relTol, int maxIter, int minIter,
