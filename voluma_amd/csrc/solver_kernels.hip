@@ -661,7 +661,8 @@ void launch_mix(hipStream_t st, int op, int n, const double* a, float* w, float*
 }
 
 // ============================================================================ finalize
-constexpr int FIN_THREADS = 1024;
+constexpr int FIN_THREADS = 256;
+constexpr int FIN_LOADS = 32;  // partials in flight per thread
 // One block: reduce the per-block partials in a fixed order (fast) or sum the exact
 // bins (exact), then run the solver's scalar step on thread 0.
 __device__ __forceinline__ bool dev_converged(const DevScalars& S) {
@@ -802,18 +803,18 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(int op, int nred, int 
     __shared__ int nfl;
     for (int q = 0; q < nred; ++q) {
         if (!exact) {
-            // eight partials in flight per thread, summed in a fixed order
+            // FIN_LOADS partials in flight per thread, summed in a fixed order
             double s = 0.0;
             const double* pq = part + size_t(q) * maxBlk;
-            for (int k0 = threadIdx.x; k0 < nparts; k0 += 8 * blockDim.x) {
-                double v[8];
+            for (int k0 = threadIdx.x; k0 < nparts; k0 += FIN_LOADS * blockDim.x) {
+                double v[FIN_LOADS];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < FIN_LOADS; ++u) {
                     const int k = k0 + u * blockDim.x;
                     v[u] = k < nparts ? pq[k] : 0.0;
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) s += v[u];
+                for (int u = 0; u < FIN_LOADS; ++u) s += v[u];
             }
             const double t = block_sum(s, sh);
             if (threadIdx.x == 0) vals[q] = t;
