@@ -1591,10 +1591,21 @@ __global__ void k_bfinalize(int op, int nred, const double* part, int maxChunks,
         inner_start(S[s], I[s]);
         return;
     }
-    if (S[s].done) return;
+    const int done = S[s].done;
+    // the system's chunk partials in order, 16 loads in flight at a time
+    constexpr int U = 16;
     double v[MAX_RED] = {0.0, 0.0};
+    const int b = sysChunkOff[s], e = sysChunkOff[s + 1];
     for (int q = 0; q < nred; ++q)
-        for (int k = sysChunkOff[s]; k < sysChunkOff[s + 1]; ++k) v[q] += part[size_t(q) * maxChunks + k];
+        for (int k0 = b; k0 < e; k0 += U) {
+            double t[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) t[u] = k0 + u < e ? part[size_t(q) * maxChunks + k0 + u] : 0.0;
+#pragma unroll
+            for (int u = 0; u < U; ++u)
+                if (k0 + u < e) v[q] += t[u];
+        }
+    if (done) return;
     fin_step(op, S[s], I ? I[s] : S[s], v[0], v[1]);
 }
 
