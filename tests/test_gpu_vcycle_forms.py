@@ -7,6 +7,8 @@ context switch read from the environment when the context is created:
   (VOLUMA_PROLONG_SPLIT=1);
 * the levels from the first of at most VOLUMA_TAIL_MAX_CELLS cells (up to 4096) down
   to the coarsest run in one workgroup, matrices preloaded into LDS where they fit;
+* the last two levels above the coarsest and the coarsest run in one workgroup
+  (VOLUMA_LAST2=1, opt-in, where they are small CSR levels);
 * the last post-sweep of a coarse level adds its result to the finer level's vector
   (the pushed prolongation, VOLUMA_PUSH_PROLONG, default) instead of the finer sweep
   gathering x + xc[agg];
@@ -31,9 +33,11 @@ from voluma_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 PLAIN = {"VOLUMA_COARSE_Z": "0", "VOLUMA_LEVEL0_Z": "0", "VOLUMA_PROLONG_SPLIT": "0", "VOLUMA_TAIL_MAX_CELLS": "0",
-         "VOLUMA_FUSED_RESTRICT": "0", "VOLUMA_PUSH_PROLONG": "0"}
+         "VOLUMA_FUSED_RESTRICT": "0", "VOLUMA_PUSH_PROLONG": "0", "VOLUMA_LAST2": "0"}
 FORMS = {
     "default": {},
+    "last_two_levels_in_one_workgroup": {"VOLUMA_LAST2": "1"},
+    "last_two_levels_in_one_workgroup_gathered": {"VOLUMA_LAST2": "1", "VOLUMA_PUSH_PROLONG": "0"},
     "gathered_prolongation": {"VOLUMA_PUSH_PROLONG": "0"},
     "level0_residual_and_restriction_separate": {"VOLUMA_FUSED_LEVEL0": "0"},
     "sell_levels_fused_direct": {"VOLUMA_FUSED_SELL_DIRECT": "1"},

@@ -1383,6 +1383,49 @@ bool vcycle(const Ctx& C, vol_matrix A, const Plan& P, int l, const T* r, T* xou
     const SellView V = l == 0 ? m->sell.view() : D.sell.view();
     const LevelPtrs<T> L = levelPtrs(A, l, (T*)nullptr);
     const bool useZ = levelUsesZ(A, P, l);  // then L.z holds the first sweep from zero
+    if (useZ && l + 3 == nL && m->ctx->last2 && P.nPost == 1 && A->tailStart < 0 && A->nCoarse <= 64) {
+        // this level, the next and the coarsest in one workgroup
+        const LevelDev& DB = *m->lv[l + 1];
+        constexpr int W = LAST2_BATCHES * 2 * LAST2_IT;
+        if (D.layout == 0 && DB.layout == 0 && D.n <= LAST2_ROWS && DB.n <= LAST2_ROWS && D.sell.maxWidth <= W &&
+            DB.sell.maxWidth <= W) {
+            const LevelPtrs<T> LB = levelPtrs(A, l + 1, (T*)nullptr);
+            Last2ArgsT<T> a;
+            a.nA = D.n;
+            a.nB = DB.n;
+            a.nC = A->nCoarse;
+            a.batchesA = std::max(1, (D.sell.maxWidth + 2 * LAST2_IT - 1) / (2 * LAST2_IT));
+            a.batchesB = std::max(1, (DB.sell.maxWidth + 2 * LAST2_IT - 1) / (2 * LAST2_IT));
+            a.offA = D.sell.sliceOff.p;
+            a.colA = D.sell.col.p;
+            a.aggA = D.agg.p;
+            a.mOffA = D.mOff.p;
+            a.mListA = D.mList.p;
+            a.coefA = L.coef;
+            a.diagA = L.diag;
+            a.rA = r;
+            a.zA = L.z;
+            a.offB = DB.sell.sliceOff.p;
+            a.colB = DB.sell.col.p;
+            a.aggB = DB.agg.p;
+            a.mOffB = DB.mOff.p;
+            a.mListB = DB.mList.p;
+            a.coefB = LB.coef;
+            a.diagB = LB.diag;
+            a.rDB = LB.rD;
+            a.luT = A->luT.p;
+            a.omega = P.omega;
+            a.xA = push ? nullptr : xout;  // pushed: nothing reads this level's x
+            if (push) {
+                a.pushZ = push->z;
+                a.pushOff = push->mOff;
+                a.pushList = push->mList;
+            }
+            a.done = C.done;
+            launch_vcycle_last2<T>(C.st, a);
+            return push != nullptr;
+        }
+    }
     T* const z = useZ ? L.z : (l == 0 && P.nPre == 1 ? z0 : nullptr);  // given: ω·(r·rD)
     T* bufs[2] = {L.bufA, L.bufB};
     int cur = 0;  // index of the buffer holding the pre-smoothed x (nPre ≥ 2)
@@ -2143,6 +2186,7 @@ int vol_context_create(int device, vol_context* out) {
     c->fusedRestrict = !envOff("VOLUMA_FUSED_RESTRICT");
     c->pushProlong = !envOff("VOLUMA_PUSH_PROLONG");
     c->fusedLevel0 = !envOff("VOLUMA_FUSED_LEVEL0");
+    if (const char* e = std::getenv("VOLUMA_LAST2")) c->last2 = std::string(e) == "1";
     if (const char* e = std::getenv("VOLUMA_FUSED_SELL_DIRECT")) c->fusedSellDirect = std::string(e) == "1";
     if (const char* e = std::getenv("VOLUMA_FUSED_SELL_MAX")) c->fusedSellMax = std::atoi(e);
     if (const char* e = std::getenv("VOLUMA_SWEEP")) c->freeSweeps = std::string(e) == "free";

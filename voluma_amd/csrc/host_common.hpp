@@ -873,6 +873,12 @@ struct vol_context_s {
     // level 1 35.8 µs against 21 µs for the two launches, as a wave's rows then share few
     // slot cache lines; 20.9 against 19.7 ms per solve)
     bool fusedSellDirect = false;
+    // the last two levels above the coarsest and the coarsest in one workgroup when they
+    // are small CSR levels of a 1/1 V-cycle with z kept (VOLUMA_LAST2=1; default off:
+    // measured slower on the 2.06M-cell tets, 41.1 µs against 29.9 for the three launches
+    // it replaces, 19.7 against 19.1 ms per solve: one workgroup pays every phase's
+    // global-load round trips in turn)
+    bool last2 = false;
     int fusedSellMax = 0;
     // triangular sweeps (Gauss–Seidel, DIC/DILU substitutions) in their sync-free form,
     // one launch per sweep (VOLUMA_SWEEP=free; the same bits; measured slower than the

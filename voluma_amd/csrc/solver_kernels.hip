@@ -1436,6 +1436,165 @@ static void rr_sell(hipStream_t st, const SellArgsT<T>& a, const RestrictArgsT<T
         hipLaunchKernelGGL((k_resid_restrict_sell<XL, 4, T>), dim3(xcd_grid(R.nBlocks)), dim3(256), 0, st, a, R);
 }
 
+// ============================================================================ last two levels
+// coarse_subst64's steps (same operations, order and selects) with the factors read
+// from LDS (lu[k·n + i]) and the loops unrolled four steps at a time, for a workgroup
+// whose register budget cannot hold a factor row.
+template <class T>
+__device__ __forceinline__ void coarse_subst64_lds(int n, const double* lu, double g, T* x) {
+    const int i = threadIdx.x, ic = i < n ? i : 0;
+#pragma unroll 4
+    for (int k = 0; k < 64; ++k) {
+        const double gk = lane_bcast(g, k);
+        const double rk = k < n ? lu[k * n + ic] : 0.0;
+        const double t = g - rk * gk;
+        g = (i > k && i < n) ? t : g;
+    }
+#pragma unroll 4
+    for (int k = 63; k >= 0; --k) {
+        const double rk = k < n ? lu[k * n + ic] : 0.0;
+        const double q = g / rk;
+        g = (i == k && k < n) ? q : g;
+        const double gk = lane_bcast(g, k);
+        const double t = g - rk * gk;
+        g = (i < k && k < n) ? t : g;
+    }
+    if (i < n) x[i] = T(g);
+}
+
+// acc0 − Σ coef·x(j) over CSR row `row`, by the 2 lanes g of a group (lane g takes slots
+// g, g+2, … of each batch of 2·LAST2_IT; every lane subtracts the products in slot
+// order, read from their lanes): csr_row's arithmetic. All lanes of the wavefront call,
+// with the same batch count.
+template <class T, class XF>
+__device__ __forceinline__ T last2_row(const int* off, const int* col, const T* coef, int row, bool valid, int g,
+                                       T acc0, int batches, XF xf) {
+    int s0 = 0, len = 0;
+    if (valid) {
+        s0 = off[row];
+        len = off[row + 1] - s0;
+    }
+    T acc = acc0;
+    const int base = (threadIdx.x & (WAVE - 1)) - g;
+    for (int b = 0; b < batches; ++b) {
+        const int k0 = b * 2 * LAST2_IT;
+        T p[LAST2_IT];
+#pragma unroll
+        for (int it = 0; it < LAST2_IT; ++it) {
+            const int k = k0 + g + 2 * it;
+            p[it] = T(0);
+            if (k < len) p[it] = coef[s0 + k] * xf(col[s0 + k]);
+        }
+#pragma unroll
+        for (int it = 0; it < LAST2_IT; ++it) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) {
+                const T t = __shfl(p[it], base + q, WAVE);
+                if (k0 + q + 2 * it < len) acc = acc - t;
+            }
+        }
+    }
+    return acc;
+}
+
+template <class T>
+__global__ __launch_bounds__(LAST2_THREADS) void k_vcycle_last2(Last2ArgsT<T> a) {
+    if (a.done && *a.done) return;
+    __shared__ T sZA[LAST2_ROWS], sRes[LAST2_ROWS], sRB[LAST2_ROWS], sZB[LAST2_ROWS], sXB[LAST2_ROWS];
+    __shared__ int sAggA[LAST2_ROWS], sAggB[LAST2_ROWS];
+    __shared__ double sLU[64 * 64];
+    __shared__ T sXC[64];
+    const int t = threadIdx.x, grp = t / 2, g = t % 2;
+    const int nA = a.nA, nB = a.nB, nC = a.nC;
+    const T om = T(a.omega);
+    // staging: z_A, the aggregate maps, the LU factors, and each row's r and diagonals
+    constexpr int LU_PER = 64 * 64 / LAST2_THREADS;
+    double luv[LU_PER];
+#pragma unroll
+    for (int u = 0; u < LU_PER; ++u) {
+        const int k = t + u * LAST2_THREADS;
+        luv[u] = k < nC * nC ? a.luT[k] : 0.0;
+    }
+    if (t < nA) {
+        sZA[t] = a.zA[t];
+        sAggA[t] = a.aggA[t];
+    }
+    if (t < nB) sAggB[t] = a.aggB[t];
+    const bool vA = grp < nA, vB = grp < nB;
+    T rA = T(0), dA = T(1), dB = T(1), rDB = T(0);
+    if (vA) {
+        rA = a.rA[grp];
+        dA = a.diagA[grp];
+    }
+    if (vB) dB = a.diagB[grp];
+    if (t < nB) rDB = a.rDB[t];
+#pragma unroll
+    for (int u = 0; u < LU_PER; ++u) {
+        const int k = t + u * LAST2_THREADS;
+        if (k < nC * nC) sLU[k] = luv[u];
+    }
+    __syncthreads();
+    {  // A's residual
+        const T res = last2_row(a.offA, a.colA, a.coefA, grp, vA, g, vA ? rA - dA * sZA[grp] : T(0), a.batchesA,
+                                [&](int j) { return sZA[j]; });
+        if (vA && g == 0) sRes[grp] = res;
+    }
+    __syncthreads();
+    if (t < nB) {  // restriction A → B, members ascending, and z_B
+        T s = T(0);
+        for (int k = a.mOffA[t]; k < a.mOffA[t + 1]; ++k) s += sRes[a.mListA[k]];
+        sRB[t] = s;
+        sZB[t] = om * (s * rDB);
+    }
+    __syncthreads();
+    {  // B's residual
+        const T res = last2_row(a.offB, a.colB, a.coefB, grp, vB, g, vB ? sRB[grp] - dB * sZB[grp] : T(0), a.batchesB,
+                                [&](int j) { return sZB[j]; });
+        __syncthreads();  // every lane has read sRes (A's residuals) before it is overwritten
+        if (vB && g == 0) sRes[grp] = res;
+    }
+    __syncthreads();
+    if (t < WAVE) {  // restriction B → C and the substitution (factors read from LDS)
+        T s = T(0);
+        if (t < nC)
+            for (int k = a.mOffB[t]; k < a.mOffB[t + 1]; ++k) s += sRes[a.mListB[k]];
+        coarse_subst64_lds(nC, sLU, t < nC ? double(s) : 0.0, sXC);
+    }
+    __syncthreads();
+    {  // B's post-sweep from z_B + x_C[agg]
+        auto xf = [&](int j) { return sZB[j] + sXC[sAggB[j]]; };
+        const T acc = last2_row(a.offB, a.colB, a.coefB, grp, vB, g, vB ? sRB[grp] : T(0), a.batchesB, xf);
+        if (vB && g == 0) {
+            const T J = acc / dB;
+            const T xo = xf(grp);
+            sXB[grp] = xo + om * (J - xo);
+        }
+    }
+    __syncthreads();
+    {  // A's post-sweep from z_A + x_B[agg]
+        auto xf = [&](int j) { return sZA[j] + sXB[sAggA[j]]; };
+        const T acc = last2_row(a.offA, a.colA, a.coefA, grp, vA, g, vA ? rA : T(0), a.batchesA, xf);
+        if (vA) {
+            const T J = acc / dA;
+            const T xo = xf(grp);
+            const T v = xo + om * (J - xo);
+            if (g == 0 && a.xA) a.xA[grp] = v;
+            if (a.pushZ) {  // members g, g + 2, … of the finer cells of aggregate grp
+                for (int k = a.pushOff[grp] + g, e = a.pushOff[grp + 1]; k < e; k += 2) {
+                    const int m = a.pushList[k];
+                    a.pushZ[m] = a.pushZ[m] + v;
+                }
+            }
+        }
+    }
+}
+
+template <class T> void launch_vcycle_last2(hipStream_t st, const Last2ArgsT<T>& a) {
+    hipLaunchKernelGGL(k_vcycle_last2<T>, dim3(1), dim3(LAST2_THREADS), 0, st, a);
+}
+template void launch_vcycle_last2<double>(hipStream_t, const Last2ArgsT<double>&);
+template void launch_vcycle_last2<float>(hipStream_t, const Last2ArgsT<float>&);
+
 template <int XL, class T>
 static void rr_packed(hipStream_t st, const SellArgsT<T>& a, const RestrictArgsT<T>& R) {
     if (a.S.colPW == 2)

@@ -278,6 +278,28 @@ template <class T> void launch_resid_restrict(hipStream_t st, int xl, int G, int
 // each of ≤ 64 slots.
 template <class T> void launch_resid_restrict_coarse64(hipStream_t st, int xl, int G, int IT, const SellArgsT<T>& a,
                                                        const RestrictArgsT<T>& R);
+// The last two levels above the coarsest (A = coarsest − 2, B = coarsest − 1, both CSR,
+// ≤ LAST2_ROWS rows of ≤ LAST2_BATCHES·2·LAST2_IT slots) and the coarsest C (≤ 64 cells) of a 1/1
+// damped-Jacobi V-cycle with z kept, in one workgroup: A's residual, restriction into B
+// (with z_B), B's residual, restriction into C, C's substitution, B's post-sweep, A's
+// post-sweep, every row's arithmetic and every sum's order as in the separate launches.
+// A's result goes to xA (nullable) and, with pushZ, is added to the finer level's z.
+constexpr int LAST2_THREADS = 1024, LAST2_ROWS = 512, LAST2_IT = 8, LAST2_BATCHES = 2;
+template <class T> struct Last2ArgsT {
+    int nA = 0, nB = 0, nC = 0;
+    int batchesA = 1, batchesB = 1;  // ⌈longest row / (2·LAST2_IT)⌉ of A and B
+    const int *offA = nullptr, *colA = nullptr, *aggA = nullptr, *mOffA = nullptr, *mListA = nullptr;
+    const T *coefA = nullptr, *diagA = nullptr, *rA = nullptr, *zA = nullptr;
+    const int *offB = nullptr, *colB = nullptr, *aggB = nullptr, *mOffB = nullptr, *mListB = nullptr;
+    const T *coefB = nullptr, *diagB = nullptr, *rDB = nullptr;
+    const double* luT = nullptr;
+    double omega = 1.0;
+    T* xA = nullptr;
+    T* pushZ = nullptr;
+    const int *pushOff = nullptr, *pushList = nullptr;
+    const int* done = nullptr;
+};
+template <class T> void launch_vcycle_last2(hipStream_t st, const Last2ArgsT<T>& a);
 // Same substitution in FP64 on FP32 vectors (r widened, x rounded).
 void launch_coarse_solve32(hipStream_t st, int n, const double* lu, const double* luT, const float* r, float* x,
                            const int* done);
