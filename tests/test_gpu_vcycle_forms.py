@@ -12,6 +12,9 @@ context switch read from the environment when the context is created:
 * the last post-sweep of a coarse level adds its result to the finer level's vector
   (the pushed prolongation, VOLUMA_PUSH_PROLONG, default) instead of the finer sweep
   gathering x + xc[agg];
+* SELL coarse levels hold their rows sorted by length within windows of rows
+  (VOLUMA_SELL_SORT=<rows>, opt-in), read back in row order by the Gauss–Seidel level
+  tables;
 * levels compute their residual and its restriction (into the coarsest level: and its
   substitution, and the post-sweep of the level) in one launch (VOLUMA_FUSED_RESTRICT,
   default), the packed level 0 reading its rows in restriction order
@@ -33,7 +36,7 @@ from voluma_amd import workloads as W
 pytestmark = pytest.mark.gpu
 
 PLAIN = {"VOLUMA_COARSE_Z": "0", "VOLUMA_LEVEL0_Z": "0", "VOLUMA_PROLONG_SPLIT": "0", "VOLUMA_TAIL_MAX_CELLS": "0",
-         "VOLUMA_FUSED_RESTRICT": "0", "VOLUMA_PUSH_PROLONG": "0", "VOLUMA_LAST2": "0"}
+         "VOLUMA_FUSED_RESTRICT": "0", "VOLUMA_PUSH_PROLONG": "0", "VOLUMA_LAST2": "0", "VOLUMA_SELL_SORT": "0"}
 FORMS = {
     "default": {},
     "last_two_levels_in_one_workgroup": {"VOLUMA_LAST2": "1"},
@@ -155,3 +158,49 @@ def test_packed_rows_bit_identical(mesh):
         if pr is not None:
             assert pg.iterations == pr.iterations and pg.finalResidual == pr.finalResidual, k
         assert np.array_equal(xg, xr), k
+
+
+def _sell_solves(env, m):
+    """Every coarse level in SELL form (VOLUMA_COARSE_LAYOUT=sell, read when the mesh is
+    set up) with the switches of env: the Jacobi V-cycle cases above and a Gauss–Seidel
+    GAMG-PCG solve (its level tables are built from the SELL rows read back)."""
+    keys = ["VOLUMA_COARSE_LAYOUT", "VOLUMA_SELL_SORT", "VOLUMA_FUSED_SELL_DIRECT"]
+    old = {k: os.environ.get(k) for k in keys}
+    try:
+        for k in keys:
+            os.environ.pop(k, None)
+        os.environ["VOLUMA_COARSE_LAYOUT"] = "sell"
+        os.environ.update(env)
+        ctx = va.Context(0)
+        lm, A, b = W.pressure_system(ctx, m, seed=1, gamg=True, nonorth=False)
+        out = []
+        for pre, prec, sums, tol in CASES:
+            x = np.zeros(m.n_cells)
+            out.append((x, va.solve_ldu(A, b, x, _controls(pre, prec, sums, tol))))
+        x = np.zeros(m.n_cells)
+        out.append((x, va.solve_ldu(A, b, x, va.SolverControls("PCG", "GAMG", 1e-7, 0.0, 40, sums=va.SUMS_EXACT,
+                                                                  gamgSmoother=va.GAMG_GAUSS_SEIDEL))))
+        A.close()
+        lm.close()
+        ctx.close()
+        return out
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+
+
+@pytest.mark.parametrize("mesh", ["box64", "tet"])
+def test_sorted_sell_rows_bit_identical(mesh):
+    m = W.make_mesh("box64") if mesh == "box64" else M.tet(34, 30, 28, 0.2, 3)
+    print(f"[sell-sort] {mesh}: {m.n_cells} cells, SELL levels in row order", flush=True)
+    ref = _sell_solves({"VOLUMA_SELL_SORT": "0"}, m)
+    for name, env in (("sorted", {"VOLUMA_SELL_SORT": "1024"}),
+                      ("sorted_fused_direct", {"VOLUMA_SELL_SORT": "1024", "VOLUMA_FUSED_SELL_DIRECT": "1"})):
+        print(f"[sell-sort] {mesh}: {name}", flush=True)
+        got = _sell_solves(env, m)
+        for k, ((xr, pr), (xg, pg)) in enumerate(zip(ref, got)):
+            assert pg.iterations == pr.iterations and pg.finalResidual == pr.finalResidual, (name, k)
+            assert np.array_equal(xg, xr), (name, k)

@@ -311,6 +311,7 @@ struct SellDev {
     DevBuf<int16_t> col16;
     DevBuf<int> colP;  // packed rows (SellView::colP)
     int colPW = 0, nslotsP = 0;
+    DevBuf<int> perm, iperm;  // sorted rows (SellView::perm)
     SellView view() const {
         SellView v;
         v.n = n;
@@ -327,6 +328,8 @@ struct SellDev {
         v.colP = colP.p;
         v.colPW = colPW;
         v.nslotsP = nslotsP;
+        v.perm = perm.p;
+        v.iperm = iperm.p;
         return v;
     }
 };
@@ -335,14 +338,27 @@ struct SellDev {
 // int16 slot-major column offsets under VOLUMA_COL16=1 when every neighbour lies within
 // ±32767 rows; with packed, the packed rows (SellView::colP) when the form allows them
 // (VOLUMA_PACKED=0: never); the owner of the coefficients then allocates nslots + nslotsP.
+// sortWindow > 0 (levels of more than sortWindow rows): rows sorted by length, longest
+// first and ties in row order, within each window of sortWindow consecutive rows
+// (SellView::perm); the int16 and packed forms are then not built.
 inline void buildSell(int n, const std::vector<int>& off, const std::vector<int>& colL, const std::vector<int>& srcL,
-               SellDev& S, hipStream_t st, bool try16 = false, bool packed = false) {
+               SellDev& S, hipStream_t st, bool try16 = false, bool packed = false, int sortWindow = 0) {
     S.n = n;
     S.nslices = (n + WAVE - 1) / WAVE;
+    std::vector<int> posRow(n);  // position → row
+    for (int r = 0; r < n; ++r) posRow[r] = r;
+    const bool sorted = sortWindow > 0 && n > sortWindow;
+    if (sorted) {
+        for (int w0 = 0; w0 < n; w0 += sortWindow)
+            std::stable_sort(posRow.begin() + w0, posRow.begin() + std::min(n, w0 + sortWindow),
+                             [&](int a, int b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
+        try16 = packed = false;
+    }
     std::vector<int> sliceOff(S.nslices + 1, 0);
     for (int s = 0; s < S.nslices; ++s) {
         int w = 0;
-        for (int r = s * WAVE; r < std::min(n, (s + 1) * WAVE); ++r) w = std::max(w, off[r + 1] - off[r]);
+        for (int p = s * WAVE; p < std::min(n, (s + 1) * WAVE); ++p)
+            w = std::max(w, off[posRow[p] + 1] - off[posRow[p]]);
         sliceOff[s + 1] = sliceOff[s] + w * WAVE;
         S.maxWidth = std::max(S.maxWidth, w);
     }
@@ -353,8 +369,8 @@ inline void buildSell(int n, const std::vector<int>& off, const std::vector<int>
     }
     S.nslots = sliceOff[S.nslices];
     std::vector<int> col(S.nslots, -1), src(S.nslots, -1);
-    for (int r = 0; r < n; ++r) {
-        const int s = r / WAVE, lane = r % WAVE;
+    for (int p = 0; p < n; ++p) {
+        const int s = p / WAVE, lane = p % WAVE, r = posRow[p];
         for (int k = off[r]; k < off[r + 1]; ++k) {
             const int idx = sliceOff[s] + (k - off[r]) * WAVE + lane;
             col[idx] = colL[k];
@@ -364,6 +380,12 @@ inline void buildSell(int n, const std::vector<int>& off, const std::vector<int>
     S.sliceOff.upload(sliceOff, st);
     S.col.upload(col, st);
     S.src.upload(src, st);
+    if (sorted) {
+        std::vector<int> rowPos(n);
+        for (int p = 0; p < n; ++p) rowPos[posRow[p]] = p;
+        S.perm.upload(posRow, st);
+        S.iperm.upload(rowPos, st);
+    }
     if (try16 && std::getenv("VOLUMA_COL16") && std::string(std::getenv("VOLUMA_COL16")) == "1") {
         std::vector<int16_t> c16(S.nslots, int16_t(COL16_PAD));
         bool fits = true;
@@ -412,25 +434,30 @@ inline void buildSell(int n, const std::vector<int>& off, const std::vector<int>
 inline void sellRows(const SellDev& S, std::vector<int>& off, std::vector<int>& col, std::vector<int>& slot) {
     const int n = S.n;
     const int nOff = S.csrG > 0 ? n + 1 : S.nslices + 1;
-    std::vector<int> so(nOff), sc(S.nslots);
+    std::vector<int> so(nOff), sc(S.nslots), ip;
     HIPCHECK(hipMemcpy(so.data(), S.sliceOff.p, sizeof(int) * so.size(), hipMemcpyDeviceToHost));
     if (S.nslots) HIPCHECK(hipMemcpy(sc.data(), S.col.p, sizeof(int) * sc.size(), hipMemcpyDeviceToHost));
+    if (S.iperm.p) {
+        ip.resize(n);
+        HIPCHECK(hipMemcpy(ip.data(), S.iperm.p, sizeof(int) * n, hipMemcpyDeviceToHost));
+    }
     off.assign(n + 1, 0);
     col.clear();
     slot.clear();
     for (int r = 0; r < n; ++r) {
         int s0, width, stride;
+        const int q = ip.empty() ? r : ip[r];  // the row's position in the slices
         if (S.csrG > 0) {
             s0 = so[r];
             width = so[r + 1] - s0;
             stride = 1;
         } else if (S.ellWidth > 0) {
             width = S.ellWidth;
-            s0 = (r / WAVE) * WAVE * width + r % WAVE;
+            s0 = (q / WAVE) * WAVE * width + q % WAVE;
             stride = WAVE;
         } else {
-            width = (so[r / WAVE + 1] - so[r / WAVE]) / WAVE;
-            s0 = so[r / WAVE] + r % WAVE;
+            width = (so[q / WAVE + 1] - so[q / WAVE]) / WAVE;
+            s0 = so[q / WAVE] + q % WAVE;
             stride = WAVE;
         }
         for (int k = 0; k < width; ++k) {
@@ -529,7 +556,7 @@ inline double timeLayout(const SellDev& S, hipStream_t st, void* flush, size_t f
 // 1 SELL).
 inline int buildCoarseLayout(int n, const std::vector<int>& off, const std::vector<int>& col,
                              const std::vector<int>& src, SellDev& S, hipStream_t st, int choice = -1,
-                             const std::vector<int>* agg = nullptr) {
+                             const std::vector<int>* agg = nullptr, int sortWindow = 0) {
     constexpr int TUNE_MAX_ROWS = 1 << 22;
     const char* force = std::getenv("VOLUMA_COARSE_LAYOUT");
     const std::string f = force ? force : "";
@@ -539,7 +566,7 @@ inline int buildCoarseLayout(int n, const std::vector<int>& off, const std::vect
         choice = 1;
         SellDev a, b;
         if (buildCsr(n, off, col, src, a, st)) {
-            buildSell(n, off, col, src, b, st);
+            buildSell(n, off, col, src, b, st, false, false, sortWindow);
             const size_t fb = size_t(512) << 20;  // larger than L2 + the 256 MB Infinity Cache
             DevBuf<char> flush;
             flush.alloc(fb);
@@ -551,7 +578,7 @@ inline int buildCoarseLayout(int n, const std::vector<int>& off, const std::vect
         }
     }
     if (choice == 0 && buildCsr(n, off, col, src, S, st)) return 0;
-    buildSell(n, off, col, src, S, st);
+    buildSell(n, off, col, src, S, st, false, false, sortWindow);
     return 1;
 }
 
@@ -880,6 +907,13 @@ struct vol_context_s {
     // global-load round trips in turn)
     bool last2 = false;
     int fusedSellMax = 0;
+    // SELL coarse levels: rows sorted by length within windows of this many rows
+    // (SellView::perm; VOLUMA_SELL_SORT=<rows>, default 0: natural row order). Measured
+    // slower on the 2.06M-cell tets (19.8 against 19.0 ms per solve): the sort cuts the
+    // padded slots of levels 1–2 from 1.31–1.33× the entries to 1.03×, but each row's own
+    // vectors are then read and written out of order, and the timed layout choice takes
+    // CSR for level 1
+    int sellSort = 0;
     // triangular sweeps (Gauss–Seidel, DIC/DILU substitutions) in their sync-free form,
     // one launch per sweep (VOLUMA_SWEEP=free; the same bits; measured slower than the
     // default level-scheduled launches); sweepFault: [0] set by a sync-free wait that ran

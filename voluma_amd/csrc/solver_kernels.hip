@@ -79,9 +79,10 @@ __device__ __forceinline__ void push_prolongation(const SellArgsT<T>& a, int I, 
 // One row of a SELL gather op (the arithmetic shared by the grid kernel and the
 // one-workgroup coarse-level kernel). Slots are taken in groups of U: all column and
 // coefficient loads of a group, then all x gathers, are in flight together, and the
-// accumulation keeps the slot order.
+// accumulation keeps the slot order. pos: the row's position in the slices (the row
+// itself unless the rows are sorted, SellView::perm).
 template <int OP, int XL, int U, class T>
-__device__ __forceinline__ T sell_row(const SellArgsT<T>& a, int row, T xu) {
+__device__ __forceinline__ T sell_row_at(const SellArgsT<T>& a, int pos, int row, T xu) {
     // SELL: slot k of the row at sliceOff[slice] + 64k + lane; CSR: at sliceOff[row] + k
     int s0, width, stride;
     if (a.S.csrG > 0) {
@@ -90,13 +91,13 @@ __device__ __forceinline__ T sell_row(const SellArgsT<T>& a, int row, T xu) {
         stride = 1;
     } else if (a.S.ellWidth > 0) {
         width = a.S.ellWidth;
-        s0 = (row / WAVE) * WAVE * width + row % WAVE;
+        s0 = (pos / WAVE) * WAVE * width + pos % WAVE;
         stride = WAVE;
     } else {
-        const int slice = row / WAVE;
+        const int slice = pos / WAVE;
         const int b = a.S.sliceOff[slice];
         width = (a.S.sliceOff[slice + 1] - b) / WAVE;
-        s0 = b + row % WAVE;
+        s0 = b + pos % WAVE;
         stride = WAVE;
     }
     T acc;
@@ -141,6 +142,11 @@ __device__ __forceinline__ T sell_row(const SellArgsT<T>& a, int row, T xu) {
     }
     if (a.out) a.out[row] = outv;  // (the fused restriction keeps the value only)
     return outv;
+}
+
+template <int OP, int XL, int U, class T>
+__device__ __forceinline__ T sell_row(const SellArgsT<T>& a, int row, T xu) {
+    return sell_row_at<OP, XL, U>(a, a.S.iperm ? a.S.iperm[row] : row, row, xu);
 }
 
 // One row of a gather op over the packed rows (SellView::colP): its column offsets in
@@ -241,8 +247,9 @@ __global__ __launch_bounds__(256) void k_sell(SellArgsT<T> a) {
     const int lb = xcd_block();
     const int slice = lb * SELL_WAVES + threadIdx.x / WAVE;
     const int lane = threadIdx.x & (WAVE - 1);
-    const int row = slice * WAVE + lane;
-    const bool valid = slice < a.S.nslices && row < a.S.n;
+    const int pos = slice * WAVE + lane;
+    const bool valid = slice < a.S.nslices && pos < a.S.n;
+    const int row = valid && a.S.perm ? a.S.perm[pos] : pos;
     if constexpr (!DOT) {  // batches: a slice whose systems are all done is skipped (wave-uniform)
         if (a.sliceSys && slice < a.S.nslices) {
             const int2 r = a.sliceSys[slice];
@@ -254,7 +261,7 @@ __global__ __launch_bounds__(256) void k_sell(SellArgsT<T> a) {
     T xu = T(0);
     if constexpr (XL == XL_UNIFORM) xu = *a.xuPtr;
     T outv = T(0);
-    if (valid) outv = sell_row<OP, XL, U>(a, row, xu);
+    if (valid) outv = sell_row_at<OP, XL, U>(a, pos, row, xu);
     if constexpr (OP == OP_JACOBI) {
         if (a.pushZ && valid) push_prolongation(a, row, outv, 0, 1);
     }

@@ -47,6 +47,13 @@ struct SellView {
     const int* colP = nullptr;
     int colPW = 0;
     int nslotsP = 0;
+    // Sorted SELL rows (perm != null): position p of the slices (slice p/64, lane p%64)
+    // holds row perm[p], and row r sits at position iperm[r]. Rows are sorted by length
+    // within windows of consecutive rows so that the rows sharing a slice have nearly
+    // equal lengths (less padding); each row keeps its slot order, so every row's result
+    // is unchanged. The vectors stay in row order.
+    const int* perm = nullptr;
+    const int* iperm = nullptr;
 };
 constexpr int COL16_PAD = -32768;
 
