@@ -106,3 +106,16 @@ def test_packed_gathers_count_as_level0(tmp_path):
     assert rows["`k_sell_packed<0, 0, true, 4, double>`"][3] == "0"
     assert rows["`k_sell_packed<1, 0, false, 4, double>`"][3] == "0"
     assert rows["`k_sell_packed<2, 3, true, 4, double>`"][3] == "0"
+
+
+def test_packed_coarse_rows_keep_their_level(tmp_path):
+    db, bench = str(tmp_path / "run.db"), str(tmp_path / "bench.log")
+    seq = [AMUL, ("k_finalize", 1, 500), ("k_sell_pk<1, 0, double>", 8, 1000), ("k_restrict<double>", 2, 300),
+           ("k_resid_restrict_coarse64_post<4, 8, double>", 1, 1200), ("k_sell_pk<2, 0, double>", 8, 3000)]
+    _db(db, [seq] * 3)
+    _bench(bench)
+    rows = _rows(profiling.levels(db, bench))
+    assert [r[3] for r in rows] == ["0", "", "0", "0→1", "1→2", "0"]
+    # level-0 residual: diag, offsets at 2 bytes, coefficients; r, z read and the residual written
+    n, e = CELLS[0], 2 * FACES[0]
+    assert rows[2][7] == f"{(8 * n + 10 * e + 3 * 8 * n) / 1e6:.1f}"

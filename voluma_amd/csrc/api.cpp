@@ -277,7 +277,7 @@ void uploadLevels(vol_mesh m, const std::vector<int>* layout) {
             std::vector<int> srcL(F.adjFace.size());
             for (size_t q = 0; q < srcL.size(); ++q) srcL[q] = (F.adjFace[q] << 1) | (F.adjIsL[q] ? 0 : 1);
             D->layout = buildCoarseLayout(F.n, F.off, F.adjCell, srcL, D->sell, st, layout ? (*layout)[k] : -1,
-                                          k + 1 < nL ? &F.agg : nullptr, m->ctx->sellSort);
+                                          k + 1 < nL ? &F.agg : nullptr, m->ctx->sellSort, m->ctx->coarsePairs);
         }
         if (k + 1 < nL) {
             const HLevel& C = m->H[k + 1];
@@ -431,7 +431,7 @@ void buildGamgCoeffs(vol_matrix A) {
             if (k > 0) {
                 M->diag.alloc(D.n);
                 M->rD.alloc(D.n);
-                M->coef.alloc(D.sell.nslots);
+                M->coef.alloc(size_t(D.sell.nslots) + D.sell.nslotsP);  // (+ the pair copy)
                 M->r.alloc(D.n);
                 M->x.alloc(D.n);
                 M->z.alloc(D.n);
@@ -868,13 +868,13 @@ void buildMixed(vol_matrix A, bool gamg) {
             if (k > 0) {
                 M.diag32.alloc(D.n);
                 M.rD32.alloc(D.n);
-                M.coef32.alloc(D.sell.nslots);
+                M.coef32.alloc(size_t(D.sell.nslots) + D.sell.nslotsP);
                 M.r32.alloc(D.n);
                 M.x32.alloc(D.n);
                 M.z32.alloc(D.n);
                 toFloat(st, M.diag.p, M.diag32.p, D.n);
                 toFloat(st, M.rD.p, M.rD32.p, D.n);
-                toFloat(st, M.coef.p, M.coef32.p, D.sell.nslots);
+                toFloat(st, M.coef.p, M.coef32.p, size_t(D.sell.nslots) + D.sell.nslotsP);
                 if (D.rrCsr.nslots > 0) {
                     M.coefR32.alloc(D.rrCsr.nslots);
                     toFloat(st, M.coefR.p, M.coefR32.p, D.rrCsr.nslots);
@@ -2188,6 +2188,7 @@ int vol_context_create(int device, vol_context* out) {
     c->fusedLevel0 = !envOff("VOLUMA_FUSED_LEVEL0");
     if (const char* e = std::getenv("VOLUMA_LAST2")) c->last2 = std::string(e) == "1";
     if (const char* e = std::getenv("VOLUMA_SELL_SORT")) c->sellSort = std::max(0, std::atoi(e));
+    c->coarsePairs = !envOff("VOLUMA_COARSE_PAIRS");
     if (const char* e = std::getenv("VOLUMA_FUSED_SELL_DIRECT")) c->fusedSellDirect = std::string(e) == "1";
     if (const char* e = std::getenv("VOLUMA_FUSED_SELL_MAX")) c->fusedSellMax = std::atoi(e);
     if (const char* e = std::getenv("VOLUMA_SWEEP")) c->freeSweeps = std::string(e) == "free";

@@ -312,6 +312,8 @@ struct SellDev {
     DevBuf<int> colP;  // packed rows (SellView::colP)
     int colPW = 0, nslotsP = 0;
     DevBuf<int> perm, iperm;  // sorted rows (SellView::perm)
+    DevBuf<int2> colS;        // packed SELL rows (SellView::colS)
+    DevBuf<int4> sliceP;
     SellView view() const {
         SellView v;
         v.n = n;
@@ -330,6 +332,8 @@ struct SellDev {
         v.nslotsP = nslotsP;
         v.perm = perm.p;
         v.iperm = iperm.p;
+        v.colS = colS.p;
+        v.sliceP = sliceP.p;
         return v;
     }
 };
@@ -340,9 +344,12 @@ struct SellDev {
 // (VOLUMA_PACKED=0: never); the owner of the coefficients then allocates nslots + nslotsP.
 // sortWindow > 0 (levels of more than sortWindow rows): rows sorted by length, longest
 // first and ties in row order, within each window of sortWindow consecutive rows
-// (SellView::perm); the int16 and packed forms are then not built.
+// (SellView::perm); the int16 and packed forms are then not built. pairs: the packed
+// SELL rows as well (SellView::colS) when every column offset fits int16; the owner of
+// the coefficients then allocates nslots + nslotsP.
 inline void buildSell(int n, const std::vector<int>& off, const std::vector<int>& colL, const std::vector<int>& srcL,
-               SellDev& S, hipStream_t st, bool try16 = false, bool packed = false, int sortWindow = 0) {
+               SellDev& S, hipStream_t st, bool try16 = false, bool packed = false, int sortWindow = 0,
+               bool pairs = false) {
     S.n = n;
     S.nslices = (n + WAVE - 1) / WAVE;
     std::vector<int> posRow(n);  // position → row
@@ -352,7 +359,7 @@ inline void buildSell(int n, const std::vector<int>& off, const std::vector<int>
         for (int w0 = 0; w0 < n; w0 += sortWindow)
             std::stable_sort(posRow.begin() + w0, posRow.begin() + std::min(n, w0 + sortWindow),
                              [&](int a, int b) { return off[a + 1] - off[a] > off[b + 1] - off[b]; });
-        try16 = packed = false;
+        try16 = packed = pairs = false;
     }
     std::vector<int> sliceOff(S.nslices + 1, 0);
     for (int s = 0; s < S.nslices; ++s) {
@@ -385,6 +392,42 @@ inline void buildSell(int n, const std::vector<int>& off, const std::vector<int>
         for (int p = 0; p < n; ++p) rowPos[posRow[p]] = p;
         S.perm.upload(posRow, st);
         S.iperm.upload(rowPos, st);
+    }
+    if (pairs) {
+        bool fits = true;
+        for (int r = 0; r < n && fits; ++r)
+            for (int k = off[r]; k < off[r + 1] && fits; ++k) {
+                const int d = colL[k] - r;
+                fits = d > COL16_PAD && d <= 32767;
+            }
+        if (fits) {
+            const int padWord = int((uint32_t(uint16_t(COL16_PAD)) << 16) | uint16_t(COL16_PAD));
+            std::vector<int4> sp(S.nslices);
+            std::vector<int2> words;
+            int ob = 0, pb = 0;
+            for (int s = 0; s < S.nslices; ++s) {
+                const int w = (sliceOff[s + 1] - sliceOff[s]) / WAVE, Q = (w + 3) / 4, P = (w + 1) / 2;
+                sp[s] = make_int4(ob, pb, w, 0);
+                words.resize(size_t(ob) + size_t(WAVE) * Q, make_int2(padWord, padWord));
+                for (int lane = 0; lane < WAVE; ++lane) {
+                    const int r = s * WAVE + lane;
+                    if (r >= n) break;
+                    for (int k = off[r]; k < off[r + 1]; ++k) {
+                        const int q = (k - off[r]) / 4, e = (k - off[r]) % 4;
+                        int& word = e < 2 ? words[size_t(ob) + size_t(WAVE) * q + lane].x
+                                          : words[size_t(ob) + size_t(WAVE) * q + lane].y;
+                        const uint32_t h = uint16_t(int16_t(colL[k] - r));
+                        word = (e & 1) ? int((uint32_t(word) & 0xffffu) | (h << 16))
+                                       : int((uint32_t(word) & 0xffff0000u) | h);
+                    }
+                }
+                ob += WAVE * Q;
+                pb += WAVE * P;
+            }
+            S.colS.upload(words, st);
+            S.sliceP.upload(sp, st);
+            S.nslotsP = 2 * pb;
+        }
     }
     if (try16 && std::getenv("VOLUMA_COL16") && std::string(std::getenv("VOLUMA_COL16")) == "1") {
         std::vector<int16_t> c16(S.nslots, int16_t(COL16_PAD));
@@ -556,7 +599,7 @@ inline double timeLayout(const SellDev& S, hipStream_t st, void* flush, size_t f
 // 1 SELL).
 inline int buildCoarseLayout(int n, const std::vector<int>& off, const std::vector<int>& col,
                              const std::vector<int>& src, SellDev& S, hipStream_t st, int choice = -1,
-                             const std::vector<int>* agg = nullptr, int sortWindow = 0) {
+                             const std::vector<int>* agg = nullptr, int sortWindow = 0, bool pairs = false) {
     constexpr int TUNE_MAX_ROWS = 1 << 22;
     const char* force = std::getenv("VOLUMA_COARSE_LAYOUT");
     const std::string f = force ? force : "";
@@ -578,7 +621,7 @@ inline int buildCoarseLayout(int n, const std::vector<int>& off, const std::vect
         }
     }
     if (choice == 0 && buildCsr(n, off, col, src, S, st)) return 0;
-    buildSell(n, off, col, src, S, st, false, false, sortWindow);
+    buildSell(n, off, col, src, S, st, false, false, sortWindow, pairs);
     return 1;
 }
 
@@ -914,6 +957,9 @@ struct vol_context_s {
     // vectors are then read and written out of order, and the timed layout choice takes
     // CSR for level 1
     int sellSort = 0;
+    // SELL coarse levels also in the packed SELL form (SellView::colS; VOLUMA_COARSE_PAIRS=0:
+    // slot-major only)
+    bool coarsePairs = true;
     // triangular sweeps (Gauss–Seidel, DIC/DILU substitutions) in their sync-free form,
     // one launch per sweep (VOLUMA_SWEEP=free; the same bits; measured slower than the
     // default level-scheduled launches); sweepFault: [0] set by a sync-free wait that ran

@@ -223,6 +223,85 @@ __device__ __forceinline__ T sell_row_packed(const SellArgsT<T>& a, int row, T x
     return outv;
 }
 
+// One row of a gather op over the packed SELL rows of a coarse level (SellView::colS):
+// slots in groups of 16, each group's offsets in four 8-byte loads and its coefficients
+// in eight pair loads, then its x gathers; the accumulation keeps the slot order, so the
+// result has the bits of sell_row.
+template <int OP, int XL, class T>
+__device__ __forceinline__ T sell_row_pk(const SellArgsT<T>& a, int slice, int lane, int row, T xu) {
+    using T2 = typename Pair<T>::type;
+    const int4 d = a.S.sliceP[slice];
+    const int W = d.z;
+    const int2* cw = a.S.colS + d.x + lane;
+    const T2* cp = reinterpret_cast<const T2*>(a.coef + a.S.nslots) + d.y + lane;
+    T acc;
+    if constexpr (OP == OP_AMUL) acc = a.diag[row] * xload<XL>(a, row, xu);
+    else if constexpr (OP == OP_RESID) acc = a.r[row] - a.diag[row] * xload<XL>(a, row, xu);
+    else acc = a.r[row];
+    for (int k0 = 0; k0 < W; k0 += 16) {
+        int j[16];
+        T av[16], xv[16];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int k = k0 + 4 * q;
+            int2 v = make_int2(0, 0);
+            if (k < W) v = cw[WAVE * (k / 4)];
+            const int o[4] = {int(short(v.x & 0xffff)), v.x >> 16, int(short(v.y & 0xffff)), v.y >> 16};
+#pragma unroll
+            for (int e = 0; e < 4; ++e) j[4 * q + e] = (k + e < W && o[e] != COL16_PAD) ? row + o[e] : -1;
+        }
+#pragma unroll
+        for (int p = 0; p < 8; ++p) {
+            const int k = k0 + 2 * p;
+            T2 v;
+            v.x = T(0);
+            v.y = T(0);
+            if (k < W) v = cp[WAVE * (k / 2)];
+            av[2 * p] = v.x;
+            av[2 * p + 1] = v.y;
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) xv[u] = j[u] >= 0 ? xload<XL>(a, j[u], xu) : T(0);
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            if (j[u] >= 0) {
+                const T t = av[u] * xv[u];
+                if constexpr (OP == OP_AMUL) acc = acc + t;
+                else acc = acc - t;
+            }
+        }
+    }
+    T outv;
+    if constexpr (OP == OP_JACOBI) {
+        const T J = acc / a.diag[row];
+        const T xo = xload<XL>(a, row, xu);
+        outv = xo + T(a.omega) * (J - xo);
+    } else if constexpr (OP == OP_JACREF) {
+        outv = acc / a.diag[row];
+    } else {
+        outv = acc;
+    }
+    if (a.out) a.out[row] = outv;
+    return outv;
+}
+
+// A gather op over a coarse level's packed SELL rows (no reduction), with the pushed
+// prolongation after a post-sweep like k_sell.
+template <int OP, int XL, class T>
+__global__ __launch_bounds__(256) void k_sell_pk(SellArgsT<T> a) {
+    if (a.done && *a.done) return;
+    const int slice = xcd_block() * SELL_WAVES + threadIdx.x / WAVE;
+    const int lane = threadIdx.x & (WAVE - 1);
+    const int row = slice * WAVE + lane;
+    if (slice >= a.S.nslices || row >= a.S.n) return;
+    T xu = T(0);
+    if constexpr (XL == XL_UNIFORM) xu = *a.xuPtr;
+    const T outv = sell_row_pk<OP, XL>(a, slice, lane, row, xu);
+    if constexpr (OP == OP_JACOBI) {
+        if (a.pushZ) push_prolongation(a, row, outv, 0, 1);
+    }
+}
+
 template <int OP, int XL, bool DOT, int PW, class T>
 __global__ __launch_bounds__(256) void k_sell_packed(SellArgsT<T> a) {
     if (a.done && *a.done) return;
@@ -385,6 +464,10 @@ static void sell_dispatch2(hipStream_t st, const SellArgsT<T>& a) {
         }
         return;
     }
+    if (a.S.colS && !a.sliceSys && !a.part) {  // packed SELL rows (a coarse level)
+        sell_launch(k_sell_pk<OP, XL, T>, nb, st, a);
+        return;
+    }
     const bool wide = a.S.maxWidth > 8;
     if (a.part) {  // fused Σ out·dotv: A·p, and the last sweep of the V-cycle (Σ w·r)
         if constexpr (OP == OP_AMUL || (OP == OP_JACOBI && XL != XL_UNIFORM && XL != XL_FROMZERO)) {
@@ -443,6 +526,23 @@ __global__ void k_sell_coef(SellView S, const double* upper, const double* lower
         const size_t j = size_t(S.nslots) + (size_t(slice) * WAVE * P + size_t(k / 2) * WAVE + lane) * 2 + (k & 1);
         if (coef) coef[j] = v;
         if (coef32) coef32[j] = float(v);
+    }
+    if (S.colS) {  // the packed SELL pair copy: the slice by bisection of the slice offsets
+        int lo = 0, hi = S.nslices;  // sliceOff[lo] ≤ i < sliceOff[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) / 2;
+            if (S.sliceOff[mid] <= i) lo = mid;
+            else hi = mid;
+        }
+        const int4 d = S.sliceP[lo];
+        const int rem = i - S.sliceOff[lo], k = rem / WAVE, lane = rem % WAVE;
+        const size_t j = size_t(S.nslots) + (size_t(d.y) + size_t(k / 2) * WAVE + lane) * 2 + (k & 1);
+        if (coef) coef[j] = v;
+        if (coef32) coef32[j] = float(v);
+        if ((d.z & 1) && k == d.z - 1) {  // the unused half of an odd-width slice's last pair
+            if (coef) coef[j + 1] = 0.0;
+            if (coef32) coef32[j + 1] = 0.0f;
+        }
     }
 }
 

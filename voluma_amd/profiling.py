@@ -19,7 +19,8 @@ levels, separate or fused, coarsest solve, prolongation sweeps up) to name each
 dispatch's level. Bytes are what the kernel has to move given the stored layout,
 without its padding: per row of a level the vectors it reads and writes, per
 off-diagonal entry a coefficient and a column (8 + 4 bytes in FP64; 8 + 2 on the packed
-level-0 rows, ``k_sell_packed``), the restriction's member lists and the coarsest LU
+level-0 rows, ``k_sell_packed``, and the packed coarse SELL rows, ``k_sell_pk``), the
+restriction's member lists and the coarsest LU
 factors; a post-sweep that gathers the prolongation (x-load 3) its agg and coarse x, one
 with a plain x-load above level 0 the member lists and the finer z it adds its result
 to (the pushed prolongation); the x/r update counts the level-0 z it writes for the
@@ -70,7 +71,7 @@ def _template_args(kernel):
 
 def _is_sell(kernel):
     """A SELL-64 gather (slot-major or packed rows)."""
-    return kernel.startswith("k_sell<") or kernel.startswith("k_sell_packed<")
+    return kernel.startswith(("k_sell<", "k_sell_packed<", "k_sell_pk<"))
 
 
 def _is_amul(kernel):
@@ -156,7 +157,7 @@ def levels(db, bench_json):
     level = 0
     for i, ((k, g), us) in enumerate(zip(shape, avg)):
         lev, nbytes, rws, ent = "", None, "", ""
-        cb = 2 if k.startswith("k_sell_packed<") else 4
+        cb = 2 if k.startswith(("k_sell_packed<", "k_sell_pk<")) else 4
         if _is_amul(k):
             lev, nbytes = "0", _row_bytes("amul", cells[0], 2 * faces[0], 0, elem, cb)
         elif _is_sell(k) or k.startswith("k_csrv<"):
