@@ -59,6 +59,34 @@ def test_batch_matches_single_system_solver(ctx, small_batch, pre):
         assert np.linalg.norm(xb - xs) <= 1e-8 * np.linalg.norm(xs), s
 
 
+def test_batch_packed_rows_bit_identical(ctx, small_batch):
+    """Batch levels read through their packed SELL rows (the default) or slot-major
+    (VOLUMA_SELL_PAIRS=0, read when the context is created): the same bits, FP64 and
+    mixed precision."""
+    import os
+    specs, B, b, meshes = small_batch
+    old = os.environ.get("VOLUMA_SELL_PAIRS")
+    os.environ["VOLUMA_SELL_PAIRS"] = "0"
+    try:
+        c0 = va.Context(0)
+    finally:
+        if old is None:
+            os.environ.pop("VOLUMA_SELL_PAIRS", None)
+        else:
+            os.environ["VOLUMA_SELL_PAIRS"] = old
+    B0, b0, _ = W.batch_system(c0, specs)
+    assert np.array_equal(b0, b)
+    for prec in (va.FP64, va.MIXED):
+        x, x0 = np.zeros(B.n_total), np.zeros(B.n_total)
+        perf = B.solve(b, x, _ctl(precision=prec))
+        perf0 = B0.solve(b0, x0, _ctl(precision=prec))
+        for p, p0 in zip(perf, perf0):
+            assert p.iterations == p0.iterations and p.finalResidual == p0.finalResidual, prec
+        assert np.array_equal(x, x0), prec
+    B0.close()
+    c0.close()
+
+
 def test_batch_result_independent_of_batch_mates(ctx, small_batch):
     """System k solved alone (batch of one) is bit-identical to system k in the batch."""
     specs, B, b, meshes = small_batch

@@ -16,7 +16,7 @@ context switch read from the environment when the context is created:
   (VOLUMA_SELL_SORT=<rows>, opt-in), read back in row order by the Gauss–Seidel level
   tables;
 * SELL coarse levels are read through their packed rows, int16 column offsets and
-  coefficient pairs (VOLUMA_COARSE_PAIRS, default);
+  coefficient pairs (VOLUMA_SELL_PAIRS, default);
 * levels compute their residual and its restriction (into the coarsest level: and its
   substitution, and the post-sweep of the level) in one launch (VOLUMA_FUSED_RESTRICT,
   default), the packed level 0 reading its rows in restriction order
@@ -39,7 +39,7 @@ pytestmark = pytest.mark.gpu
 
 PLAIN = {"VOLUMA_COARSE_Z": "0", "VOLUMA_LEVEL0_Z": "0", "VOLUMA_PROLONG_SPLIT": "0", "VOLUMA_TAIL_MAX_CELLS": "0",
          "VOLUMA_FUSED_RESTRICT": "0", "VOLUMA_PUSH_PROLONG": "0", "VOLUMA_LAST2": "0", "VOLUMA_SELL_SORT": "0",
-         "VOLUMA_COARSE_PAIRS": "0"}
+         "VOLUMA_SELL_PAIRS": "0"}
 FORMS = {
     "default": {},
     "last_two_levels_in_one_workgroup": {"VOLUMA_LAST2": "1"},
@@ -167,7 +167,7 @@ def _sell_solves(env, m):
     """Every coarse level in SELL form (VOLUMA_COARSE_LAYOUT=sell, read when the mesh is
     set up) with the switches of env: the Jacobi V-cycle cases above and a Gauss–Seidel
     GAMG-PCG solve (its level tables are built from the SELL rows read back)."""
-    keys = ["VOLUMA_COARSE_LAYOUT", "VOLUMA_SELL_SORT", "VOLUMA_FUSED_SELL_DIRECT", "VOLUMA_COARSE_PAIRS"]
+    keys = ["VOLUMA_COARSE_LAYOUT", "VOLUMA_SELL_SORT", "VOLUMA_FUSED_SELL_DIRECT", "VOLUMA_SELL_PAIRS"]
     old = {k: os.environ.get(k) for k in keys}
     try:
         for k in keys:
@@ -199,7 +199,7 @@ def _sell_solves(env, m):
 def test_sell_row_forms_bit_identical(mesh):
     m = W.make_mesh("box64") if mesh == "box64" else M.tet(34, 30, 28, 0.2, 3)
     print(f"[sell-sort] {mesh}: {m.n_cells} cells, SELL levels in row order, slot-major", flush=True)
-    ref = _sell_solves({"VOLUMA_SELL_SORT": "0", "VOLUMA_COARSE_PAIRS": "0"}, m)
+    ref = _sell_solves({"VOLUMA_SELL_SORT": "0", "VOLUMA_SELL_PAIRS": "0"}, m)
     for name, env in (("packed_pairs", {}), ("sorted", {"VOLUMA_SELL_SORT": "1024"}),
                       ("sorted_fused_direct", {"VOLUMA_SELL_SORT": "1024", "VOLUMA_FUSED_SELL_DIRECT": "1"})):
         print(f"[sell-sort] {mesh}: {name}", flush=True)

@@ -277,7 +277,7 @@ void uploadLevels(vol_mesh m, const std::vector<int>* layout) {
             std::vector<int> srcL(F.adjFace.size());
             for (size_t q = 0; q < srcL.size(); ++q) srcL[q] = (F.adjFace[q] << 1) | (F.adjIsL[q] ? 0 : 1);
             D->layout = buildCoarseLayout(F.n, F.off, F.adjCell, srcL, D->sell, st, layout ? (*layout)[k] : -1,
-                                          k + 1 < nL ? &F.agg : nullptr, m->ctx->sellSort, m->ctx->coarsePairs);
+                                          k + 1 < nL ? &F.agg : nullptr, m->ctx->sellSort, m->ctx->sellPairs);
         }
         if (k + 1 < nL) {
             const HLevel& C = m->H[k + 1];
@@ -2188,7 +2188,7 @@ int vol_context_create(int device, vol_context* out) {
     c->fusedLevel0 = !envOff("VOLUMA_FUSED_LEVEL0");
     if (const char* e = std::getenv("VOLUMA_LAST2")) c->last2 = std::string(e) == "1";
     if (const char* e = std::getenv("VOLUMA_SELL_SORT")) c->sellSort = std::max(0, std::atoi(e));
-    c->coarsePairs = !envOff("VOLUMA_COARSE_PAIRS");
+    c->sellPairs = !envOff("VOLUMA_SELL_PAIRS");
     if (const char* e = std::getenv("VOLUMA_FUSED_SELL_DIRECT")) c->fusedSellDirect = std::string(e) == "1";
     if (const char* e = std::getenv("VOLUMA_FUSED_SELL_MAX")) c->fusedSellMax = std::atoi(e);
     if (const char* e = std::getenv("VOLUMA_SWEEP")) c->freeSweeps = std::string(e) == "free";

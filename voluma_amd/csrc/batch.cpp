@@ -173,7 +173,7 @@ void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const doubl
                     col[k] = o;
                     src[k] = (gf << 1) | 1;
                 }
-            buildSell(Lv.nSmooth, off, col, src, Lv.sell, st, true);
+            buildSell(Lv.nSmooth, off, col, src, Lv.sell, st, true, false, 0, B->ctx->sellPairs);
             std::vector<char> oil(B->nfTotal);
             for (int s = 0; s < nS; ++s)
                 for (int f = 0; f < B->nf[s]; ++f)
@@ -200,7 +200,7 @@ void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const doubl
                 }
             }
             buildCoarseLayout(Lv.nSmooth, off, col, src, Lv.sell, st,
-                              l < int(B->layoutHint.size()) ? B->layoutHint[l] : -1);
+                              l < int(B->layoutHint.size()) ? B->layoutHint[l] : -1, nullptr, 0, B->ctx->sellPairs);
         }
         // system of every smoothing row, for skipping finished systems
         if (Lv.nSmooth > 0) {
@@ -247,7 +247,7 @@ void buildBatch(vol_batch B, const int32_t* own, const int32_t* nei, const doubl
             Lv.rD.alloc(Lv.nRows);
             Lv.r.alloc(Lv.nRows);
             Lv.x.alloc(Lv.nRows);
-            if (Lv.nSmooth > 0) Lv.coef.alloc(Lv.sell.nslots);
+            if (Lv.nSmooth > 0) Lv.coef.alloc(size_t(Lv.sell.nslots) + Lv.sell.nslotsP);  // (+ the pair copy)
         }
         Lv.Fu.alloc(Lv.nFaces);
         Lv.Fl.alloc(Lv.nFaces);
@@ -320,7 +320,7 @@ void buildCoeffs(vol_batch B) {
     hipStream_t st = B->ctx->stream;
     const int LM = B->nLevMax;
     BLevel& L0 = *B->L[0];
-    L0.coef.alloc(L0.sell.nslots);
+    L0.coef.alloc(size_t(L0.sell.nslots) + L0.sell.nslotsP);
     launch_sell_coef(st, L0.sell.view(), B->upper.p, B->symmetric ? B->upper.p : B->lower.p, L0.coef.p, nullptr);
     L0.rD.alloc(B->nTotal);
     VecArgs v;
@@ -364,8 +364,8 @@ void buildMixedB(vol_batch B) {
         toFloatB(st, d, Lv.diag32.p, nr);
         toFloatB(st, Lv.rD.p, Lv.rD32.p, nr);
         if (Lv.nSmooth > 0) {
-            Lv.coef32.alloc(Lv.sell.nslots);
-            toFloatB(st, Lv.coef.p, Lv.coef32.p, Lv.sell.nslots);
+            Lv.coef32.alloc(size_t(Lv.sell.nslots) + Lv.sell.nslotsP);
+            toFloatB(st, Lv.coef.p, Lv.coef32.p, size_t(Lv.sell.nslots) + Lv.sell.nslotsP);
         }
         Lv.bufA32.alloc(Lv.nSmooth);
         Lv.bufB32.alloc(Lv.nSmooth);

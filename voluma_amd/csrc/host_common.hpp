@@ -957,9 +957,9 @@ struct vol_context_s {
     // vectors are then read and written out of order, and the timed layout choice takes
     // CSR for level 1
     int sellSort = 0;
-    // SELL coarse levels also in the packed SELL form (SellView::colS; VOLUMA_COARSE_PAIRS=0:
-    // slot-major only)
-    bool coarsePairs = true;
+    // SELL levels also in the packed SELL form (SellView::colS): the coarse levels of a
+    // mesh, and every level of a batch (VOLUMA_SELL_PAIRS=0: slot-major only)
+    bool sellPairs = true;
     // triangular sweeps (Gauss–Seidel, DIC/DILU substitutions) in their sync-free form,
     // one launch per sweep (VOLUMA_SWEEP=free; the same bits; measured slower than the
     // default level-scheduled launches); sweepFault: [0] set by a sync-free wait that ran

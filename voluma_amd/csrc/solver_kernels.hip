@@ -285,15 +285,22 @@ __device__ __forceinline__ T sell_row_pk(const SellArgsT<T>& a, int slice, int l
     return outv;
 }
 
-// A gather op over a coarse level's packed SELL rows (no reduction), with the pushed
-// prolongation after a post-sweep like k_sell.
+// A gather op over packed SELL rows (no reduction), with the pushed prolongation after
+// a post-sweep and the skip of finished batch systems like k_sell.
 template <int OP, int XL, class T>
 __global__ __launch_bounds__(256) void k_sell_pk(SellArgsT<T> a) {
     if (a.done && *a.done) return;
     const int slice = xcd_block() * SELL_WAVES + threadIdx.x / WAVE;
     const int lane = threadIdx.x & (WAVE - 1);
     const int row = slice * WAVE + lane;
-    if (slice >= a.S.nslices || row >= a.S.n) return;
+    if (slice >= a.S.nslices) return;
+    if (a.sliceSys) {  // batches: a slice whose systems are all done is skipped (wave-uniform)
+        const int2 r = a.sliceSys[slice];
+        bool all = true;
+        for (int q = r.x; q <= r.y; ++q) all = all && a.sysS[q].done;
+        if (all) return;
+    }
+    if (row >= a.S.n) return;
     T xu = T(0);
     if constexpr (XL == XL_UNIFORM) xu = *a.xuPtr;
     const T outv = sell_row_pk<OP, XL>(a, slice, lane, row, xu);
@@ -464,7 +471,7 @@ static void sell_dispatch2(hipStream_t st, const SellArgsT<T>& a) {
         }
         return;
     }
-    if (a.S.colS && !a.sliceSys && !a.part) {  // packed SELL rows (a coarse level)
+    if (a.S.colS && !a.part) {  // packed SELL rows (coarse levels, batches)
         sell_launch(k_sell_pk<OP, XL, T>, nb, st, a);
         return;
     }

@@ -54,8 +54,8 @@ struct SellView {
     // is unchanged. The vectors stay in row order.
     const int* perm = nullptr;
     const int* iperm = nullptr;
-    // Packed SELL rows (a coarse level in SELL form whose column offsets all fit int16;
-    // colS != null): slice s has sliceP[s] = (word base, pair base, width w, 0); the column
+    // Packed SELL rows (a coarse level in SELL form, or a batch level, whose column offsets
+    // all fit int16; colS != null): slice s has sliceP[s] = (word base, pair base, width w, 0); the column
     // offsets (neighbour − row, COL16_PAD = padding) of lane l, four per int2 word, word q
     // at colS[word base + 64q + l]; and the coefficient array holds, after its nslots
     // slot-major values, nslotsP more: the same values in pairs, pair p of lane l at pair
