@@ -200,7 +200,8 @@ def test_sell_row_forms_bit_identical(mesh):
     m = W.make_mesh("box64") if mesh == "box64" else M.tet(34, 30, 28, 0.2, 3)
     print(f"[sell-sort] {mesh}: {m.n_cells} cells, SELL levels in row order, slot-major", flush=True)
     ref = _sell_solves({"VOLUMA_SELL_SORT": "0", "VOLUMA_SELL_PAIRS": "0"}, m)
-    for name, env in (("packed_pairs", {}), ("sorted", {"VOLUMA_SELL_SORT": "1024"}),
+    for name, env in (("packed_pairs", {}), ("packed_pairs_fused_direct", {"VOLUMA_FUSED_SELL_DIRECT": "1"}),
+                      ("sorted", {"VOLUMA_SELL_SORT": "1024"}),
                       ("sorted_fused_direct", {"VOLUMA_SELL_SORT": "1024", "VOLUMA_FUSED_SELL_DIRECT": "1"})):
         print(f"[sell-sort] {mesh}: {name}", flush=True)
         got = _sell_solves(env, m)

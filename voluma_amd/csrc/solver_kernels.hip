@@ -1542,8 +1542,39 @@ __global__ __launch_bounds__(256) void k_resid_restrict_sell(SellArgsT<T> a, Res
     }
 }
 
+// The same on a SELL level's packed rows (SellView::colS), sell_row_pk's arithmetic.
+template <int XL, class T>
+__global__ __launch_bounds__(256) void k_resid_restrict_pk(SellArgsT<T> a, RestrictArgsT<T> R) {
+    if (a.done && *a.done) return;
+    const int b = xcd_block();
+    if (b >= R.nBlocks) return;
+    __shared__ T res[RR_POS];
+    const int2 lo = R.blk[b], hi = R.blk[b + 1];
+    const int I = lo.x + int(threadIdx.x);
+    int mb = 0, me = 0;
+    if (I < hi.x) {
+        mb = R.mOff[I] - lo.y;
+        me = R.mOff[I + 1] - lo.y;
+    }
+    for (int p = lo.y + int(threadIdx.x); p < hi.y; p += 256) {
+        const int r = R.mList[p];
+        res[p - lo.y] = sell_row_pk<OP_RESID, XL>(a, r / WAVE, r % WAVE, r, T(0));
+    }
+    __syncthreads();
+    if (I < hi.x) {
+        T s = T(0);
+        for (int k = mb; k < me; ++k) s += res[k];
+        R.rc[I] = s;
+        if (R.zc) R.zc[I] = T(R.omega) * (s * R.rDc[I]);
+    }
+}
+
 template <int XL, class T>
 static void rr_sell(hipStream_t st, const SellArgsT<T>& a, const RestrictArgsT<T>& R) {
+    if (a.S.colS) {
+        hipLaunchKernelGGL((k_resid_restrict_pk<XL, T>), dim3(xcd_grid(R.nBlocks)), dim3(256), 0, st, a, R);
+        return;
+    }
     if (a.S.maxWidth > 8)
         hipLaunchKernelGGL((k_resid_restrict_sell<XL, 16, T>), dim3(xcd_grid(R.nBlocks)), dim3(256), 0, st, a, R);
     else
