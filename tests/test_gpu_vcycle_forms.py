@@ -17,6 +17,8 @@ context switch read from the environment when the context is created:
   tables;
 * SELL coarse levels are read through their packed rows, int16 column offsets and
   coefficient pairs (VOLUMA_SELL_PAIRS, default);
+* the fast-mode A·p finishes its dot product in its last block and runs the scalar
+  step there instead of in a finalize launch (VOLUMA_LAST_BLOCK_FIN=1, opt-in);
 * levels compute their residual and its restriction (into the coarsest level: and its
   substitution, and the post-sweep of the level) in one launch (VOLUMA_FUSED_RESTRICT,
   default), the packed level 0 reading its rows in restriction order
@@ -50,12 +52,14 @@ FORMS = {
     "pushed_prolongation_unfused": {"VOLUMA_FUSED_RESTRICT": "0"},
     "fused_sell_levels": {"VOLUMA_FUSED_SELL_MAX": "1048576"},
     "fused_restrict_no_coarse_z": {"VOLUMA_COARSE_Z": "0"},
+    "last_block_finalize": {"VOLUMA_LAST_BLOCK_FIN": "1"},
     "prolong_split": {"VOLUMA_PROLONG_SPLIT": "1"},
     "tail": {"VOLUMA_TAIL_MAX_CELLS": "512"},
     "tail_no_coarse_z": {"VOLUMA_TAIL_MAX_CELLS": "128", "VOLUMA_COARSE_Z": "0"},
     "tail_4096": {"VOLUMA_TAIL_MAX_CELLS": "4096"},
 }
-KEYS = sorted(PLAIN) + ["VOLUMA_FUSED_SELL_MAX", "VOLUMA_FUSED_LEVEL0", "VOLUMA_FUSED_SELL_DIRECT"]
+KEYS = sorted(PLAIN) + ["VOLUMA_FUSED_SELL_MAX", "VOLUMA_FUSED_LEVEL0", "VOLUMA_FUSED_SELL_DIRECT",
+                        "VOLUMA_LAST_BLOCK_FIN"]
 
 
 def _context(env):
@@ -208,3 +212,34 @@ def test_sell_row_forms_bit_identical(mesh):
         for k, ((xr, pr), (xg, pg)) in enumerate(zip(ref, got)):
             assert pg.iterations == pr.iterations and pg.finalResidual == pr.finalResidual, (name, k)
             assert np.array_equal(xg, xr), (name, k)
+
+
+def _krylov_solves(env, m):
+    """Diagonal PCG and PBiCGStab (FP64 and mixed) on the pressure and momentum systems
+    in a context made with env."""
+    ctx = _context(env)
+    out = []
+    lm, A, b = W.pressure_system(ctx, m, seed=1, gamg=True, nonorth=False)
+    for prec in (va.FP64, va.MIXED):
+        x = np.zeros(m.n_cells)
+        out.append((x, va.solve_ldu(A, b, x, va.SolverControls("PCG", "diagonal", 1e-8, 0.0, 400, precision=prec))))
+    A.close()
+    lm.close()
+    lm, A, b, inp = W.momentum_system(ctx, m, seed=1, comp=0)
+    for prec in (va.FP64, va.MIXED):
+        x = np.ascontiguousarray(inp["U0"][:, 0]).copy()
+        out.append((x, va.solve_ldu(A, b, x, va.SolverControls("PBiCGStab", "diagonal", 1e-10, 0.0, 50,
+                                                                  precision=prec))))
+    A.close()
+    lm.close()
+    ctx.close()
+    return out
+
+
+def test_last_block_finalize_krylov_bit_identical():
+    m = M.tet(34, 30, 28, 0.2, 3)
+    ref = _krylov_solves({}, m)
+    got = _krylov_solves({"VOLUMA_LAST_BLOCK_FIN": "1"}, m)
+    for k, ((xr, pr), (xg, pg)) in enumerate(zip(ref, got)):
+        assert pg.iterations == pr.iterations and pg.finalResidual == pr.finalResidual, k
+        assert np.array_equal(xg, xr), k

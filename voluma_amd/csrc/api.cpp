@@ -1238,9 +1238,11 @@ template <class T> struct Level0 {
 Level0<double> level0(vol_matrix A, double*) { return {A->coef.p, A->diag.p, A->rD.p}; }
 Level0<float> level0(vol_matrix A, float*) { return {A->coef32.p, A->diag32.p, A->rD32.p}; }
 
-// Level-0 A·p with optional fused dot (fast mode), timed when timing is on.
+// Level-0 A·p with optional fused dot (fast mode), timed when timing is on. With
+// finOp and the context's lastBlockFin, the launch's last block also runs that
+// finalize step; returns whether it does (then the caller launches no finalize).
 template <class T>
-void amulTimed(const Ctx& C, vol_matrix A, const T* p, T* q, const T* dotv, bool fused) {
+bool amulTimed(const Ctx& C, vol_matrix A, const T* p, T* q, const T* dotv, bool fused, int finOp = 0) {
     vol_context c = C.c;
     const SellView V = A->mesh->sell.view();
     const Level0<T> L0 = level0(A, (T*)nullptr);
@@ -1267,7 +1269,14 @@ void amulTimed(const Ctx& C, vol_matrix A, const T* p, T* q, const T* dotv, bool
     a.done = C.done;
     a.evStart = e0;
     a.evStop = e1;
+    const bool last = fused && finOp != 0 && c->lastBlockFin && redOf(C).arrive != nullptr;
+    if (last) {
+        a.arrive = redOf(C).arrive;
+        a.finS = C.S;
+        a.finOp = finOp;
+    }
     launchSell(C.st, OP_AMUL, XL_PLAIN, a);
+    return last;
 }
 
 void collectTiming(vol_context c) {
@@ -1715,9 +1724,9 @@ void krylovIteration(const Ctx& C, vol_matrix A, const Plan& P, const Krylov<T>&
             fin(C, FIN_PCG_WRA, 1, ex, nparts);
         }
         vec<T>(C, V_PCG_P, n, ex, k.w, nullptr, nullptr, nullptr, nullptr, k.p);
-        amulTimed<T>(C, A, k.p, k.q, k.p, fuse);
+        const bool finDone = amulTimed<T>(C, A, k.p, k.q, k.p, fuse, FIN_PCG_WAPA);
         if (fuse) {
-            fin(C, FIN_PCG_WAPA, 1, false, sellParts);
+            if (!finDone) fin(C, FIN_PCG_WAPA, 1, false, sellParts);
         } else {
             vec<T>(C, V_DOT, n, ex, k.q, k.p);
             fin(C, FIN_PCG_WAPA, 1, ex, nparts);
@@ -1729,9 +1738,9 @@ void krylovIteration(const Ctx& C, vol_matrix A, const Plan& P, const Krylov<T>&
         fin(C, FIN_BI_RHO, 1, ex, nparts);
         vec<T>(C, V_BI_P, n, ex, k.r, k.Ay, nullptr, nullptr, nullptr, k.p);
         precondApply<T>(C, A, P, k.p, k.y, n);
-        amulTimed<T>(C, A, k.y, k.Ay, k.r0, fuse);
+        const bool finDone = amulTimed<T>(C, A, k.y, k.Ay, k.r0, fuse, FIN_BI_ALPHA);
         if (fuse) {
-            fin(C, FIN_BI_ALPHA, 1, false, sellParts);
+            if (!finDone) fin(C, FIN_BI_ALPHA, 1, false, sellParts);
         } else {
             vec<T>(C, V_DOT, n, ex, k.r0, k.Ay);
             fin(C, FIN_BI_ALPHA, 1, ex, nparts);
@@ -2189,6 +2198,7 @@ int vol_context_create(int device, vol_context* out) {
     if (const char* e = std::getenv("VOLUMA_LAST2")) c->last2 = std::string(e) == "1";
     if (const char* e = std::getenv("VOLUMA_SELL_SORT")) c->sellSort = std::max(0, std::atoi(e));
     c->sellPairs = !envOff("VOLUMA_SELL_PAIRS");
+    if (const char* e = std::getenv("VOLUMA_LAST_BLOCK_FIN")) c->lastBlockFin = std::string(e) == "1";
     if (const char* e = std::getenv("VOLUMA_FUSED_SELL_DIRECT")) c->fusedSellDirect = std::string(e) == "1";
     if (const char* e = std::getenv("VOLUMA_FUSED_SELL_MAX")) c->fusedSellMax = std::atoi(e);
     if (const char* e = std::getenv("VOLUMA_SWEEP")) c->freeSweeps = std::string(e) == "free";

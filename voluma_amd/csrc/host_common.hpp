@@ -969,7 +969,10 @@ struct vol_context_s {
     RedScratch red;
     DevBuf<double> part, result;
     DevBuf<long long> xpart;
-    DevBuf<int> xnf;
+    DevBuf<int> xnf, arrive;
+    // the fast-mode A·p finishes its own reduction in its last block (SellArgsT::arrive;
+    // VOLUMA_LAST_BLOCK_FIN=1; same bits as the finalize launch)
+    bool lastBlockFin = false;
     DevBuf<DevScalars> scal;
     DevScalars* hostScal = nullptr;  // pinned, 2 slots
     hipEvent_t loopEv[2] = {nullptr, nullptr};  // the solve loop's look-ahead events
@@ -995,6 +998,11 @@ struct vol_context_s {
             red.xnf = xnf.p;
             red.result = result.p;
             red.maxBlkExact = RED_BLOCKS_EXACT;
+        }
+        if (!red.arrive) {
+            arrive.alloc(1);
+            HIPCHECK(hipMemset(arrive.p, 0, sizeof(int)));
+            red.arrive = arrive.p;
         }
     }
 };

@@ -38,6 +38,51 @@ __device__ __forceinline__ int xcd_block() {
     return (blockIdx.x % NXCD) * per + blockIdx.x / NXCD;
 }
 
+// ============================================================================ partial sums
+constexpr int FIN_THREADS = 256;
+constexpr int FIN_LOADS = 32;  // partials in flight per thread
+
+// Sum of nparts block partials in a fixed order by a block of FIN_THREADS threads:
+// FIN_LOADS partials in flight per thread, then the block sum (the value on every thread).
+__device__ __forceinline__ double fin_sum(const double* pq, int nparts, double* sh) {
+    double s = 0.0;
+    for (int k0 = threadIdx.x; k0 < nparts; k0 += FIN_LOADS * blockDim.x) {
+        double v[FIN_LOADS];
+#pragma unroll
+        for (int u = 0; u < FIN_LOADS; ++u) {
+            const int k = k0 + u * blockDim.x;
+            v[u] = k < nparts ? pq[k] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < FIN_LOADS; ++u) s += v[u];
+    }
+    return block_sum(s, sh);
+}
+
+__device__ void fin_step(int op, DevScalars& S, const DevScalars& I, double v0, double v1);
+
+// Last-block finalize of a fused dot (SellArgsT::arrive): after its partial is stored,
+// each block's thread 0 makes it visible device-wide and takes a ticket; the block that
+// takes the last one reads every partial after an acquire, sums them as the finalize
+// kernel does (the same block size and order, so the same bits), runs the scalar step
+// and resets the counter for the next launch. A block of FIN_THREADS threads.
+__device__ __forceinline__ void last_block_fin(const double* part, int nparts, int* arrive, DevScalars* S, int op) {
+    __shared__ int isLast;
+    __shared__ double sh[FIN_THREADS / WAVE];
+    if (threadIdx.x == 0) {
+        __threadfence();
+        isLast = atomicAdd(arrive, 1) == nparts - 1;
+    }
+    __syncthreads();
+    if (!isLast) return;
+    __threadfence();
+    const double t = fin_sum(part, nparts, sh);
+    if (threadIdx.x == 0) {
+        *arrive = 0;
+        fin_step(op, *S, S[1], t, 0.0);
+    }
+}
+
 int sell_grid_blocks(int nslices) { return xcd_grid((nslices + SELL_WAVES - 1) / SELL_WAVES); }
 
 // Value of x at row j for the gather ops; T is the storage/arithmetic type of the level
@@ -324,6 +369,7 @@ __global__ __launch_bounds__(256) void k_sell_packed(SellArgsT<T> a) {
         __shared__ double sh[256 / WAVE];
         const double v = block_sum(valid ? double(outv) * double(a.dotv[row]) : 0.0, sh);
         if (threadIdx.x == 0) a.part[lb] = v;
+        if (a.arrive) last_block_fin(a.part, int(gridDim.x), a.arrive, a.finS, a.finOp);
     }
 }
 
@@ -355,6 +401,7 @@ __global__ __launch_bounds__(256) void k_sell(SellArgsT<T> a) {
         __shared__ double sh[256 / WAVE];
         const double v = block_sum(valid ? double(outv) * double(a.dotv[row]) : 0.0, sh);
         if (threadIdx.x == 0) a.part[lb] = v;
+        if (a.arrive) last_block_fin(a.part, int(gridDim.x), a.arrive, a.finS, a.finOp);
     }
 }
 
@@ -775,8 +822,6 @@ void launch_mix(hipStream_t st, int op, int n, const double* a, float* w, float*
 }
 
 // ============================================================================ finalize
-constexpr int FIN_THREADS = 256;
-constexpr int FIN_LOADS = 32;  // partials in flight per thread
 // One block: reduce the per-block partials in a fixed order (fast) or sum the exact
 // bins (exact), then run the solver's scalar step on thread 0.
 __device__ __forceinline__ bool dev_converged(const DevScalars& S) {
@@ -917,20 +962,7 @@ __global__ __launch_bounds__(FIN_THREADS) void k_finalize(int op, int nred, int 
     __shared__ int nfl;
     for (int q = 0; q < nred; ++q) {
         if (!exact) {
-            // FIN_LOADS partials in flight per thread, summed in a fixed order
-            double s = 0.0;
-            const double* pq = part + size_t(q) * maxBlk;
-            for (int k0 = threadIdx.x; k0 < nparts; k0 += FIN_LOADS * blockDim.x) {
-                double v[FIN_LOADS];
-#pragma unroll
-                for (int u = 0; u < FIN_LOADS; ++u) {
-                    const int k = k0 + u * blockDim.x;
-                    v[u] = k < nparts ? pq[k] : 0.0;
-                }
-#pragma unroll
-                for (int u = 0; u < FIN_LOADS; ++u) s += v[u];
-            }
-            const double t = block_sum(s, sh);
+            const double t = fin_sum(part + size_t(q) * maxBlk, nparts, sh);
             if (threadIdx.x == 0) vals[q] = t;
         } else {
             if (threadIdx.x == 0) nfl = 0;

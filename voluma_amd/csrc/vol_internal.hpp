@@ -111,6 +111,7 @@ struct RedScratch {
     long long* xpart = nullptr; // [maxRed][maxBlkExact][72]
     int* xnf = nullptr;         // [maxRed][maxBlkExact]
     double* result = nullptr;   // [maxRed] (FIN_STORE)
+    int* arrive = nullptr;      // ticket counter of the last-block finalize (nullable; 0 between launches)
     int maxBlk = 0, maxBlkExact = 0;
 };
 
@@ -163,6 +164,12 @@ template <class T> struct SellArgsT {
     // host side: when set, the launch records these events at the kernel's start and end
     // (hipExtLaunchKernelGGL), which times the kernel itself
     hipEvent_t evStart = nullptr, evStop = nullptr;
+    // fused dot only (nullable): every block takes a ticket from arrive after storing its
+    // partial; the last one sums the partials in the finalize kernel's order and runs the
+    // scalar step finOp on finS, so no separate finalize launch follows
+    int* arrive = nullptr;
+    DevScalars* finS = nullptr;
+    int finOp = 0;
 };
 using SellArgs = SellArgsT<double>;
 using SellArgs32 = SellArgsT<float>;
