@@ -354,8 +354,11 @@ __global__ __launch_bounds__(256) void k_sell_pk(SellArgsT<T> a) {
     }
 }
 
-template <int OP, int XL, bool DOT, int PW, class T>
-__global__ __launch_bounds__(256) void k_sell_packed(SellArgsT<T> a) {
+// Body of k_sell_packed and k_sell_packed_fin. LASTFIN: the last-block finalize of the
+// fused dot (SellArgsT::arrive), in a kernel of its own so that its registers do not
+// lower the occupancy of the others.
+template <int OP, int XL, bool DOT, int PW, class T, bool LASTFIN>
+__device__ __forceinline__ void sell_packed_body(const SellArgsT<T>& a) {
     if (a.done && *a.done) return;
     const int lb = xcd_block();
     const int slice = lb * SELL_WAVES + threadIdx.x / WAVE;
@@ -369,12 +372,22 @@ __global__ __launch_bounds__(256) void k_sell_packed(SellArgsT<T> a) {
         __shared__ double sh[256 / WAVE];
         const double v = block_sum(valid ? double(outv) * double(a.dotv[row]) : 0.0, sh);
         if (threadIdx.x == 0) a.part[lb] = v;
-        if (a.arrive) last_block_fin(a.part, int(gridDim.x), a.arrive, a.finS, a.finOp);
+        if constexpr (LASTFIN) last_block_fin(a.part, int(gridDim.x), a.arrive, a.finS, a.finOp);
     }
 }
 
-template <int OP, int XL, bool DOT, int U, class T>
-__global__ __launch_bounds__(256) void k_sell(SellArgsT<T> a) {
+template <int OP, int XL, bool DOT, int PW, class T>
+__global__ __launch_bounds__(256) void k_sell_packed(SellArgsT<T> a) {
+    sell_packed_body<OP, XL, DOT, PW, T, false>(a);
+}
+template <int OP, int XL, int PW, class T>
+__global__ __launch_bounds__(256) void k_sell_packed_fin(SellArgsT<T> a) {
+    sell_packed_body<OP, XL, true, PW, T, true>(a);
+}
+
+// Body of k_sell and k_sell_fin (LASTFIN as for the packed rows).
+template <int OP, int XL, bool DOT, int U, class T, bool LASTFIN>
+__device__ __forceinline__ void sell_body(const SellArgsT<T>& a) {
     if (a.done && *a.done) return;
     const int lb = xcd_block();
     const int slice = lb * SELL_WAVES + threadIdx.x / WAVE;
@@ -401,8 +414,17 @@ __global__ __launch_bounds__(256) void k_sell(SellArgsT<T> a) {
         __shared__ double sh[256 / WAVE];
         const double v = block_sum(valid ? double(outv) * double(a.dotv[row]) : 0.0, sh);
         if (threadIdx.x == 0) a.part[lb] = v;
-        if (a.arrive) last_block_fin(a.part, int(gridDim.x), a.arrive, a.finS, a.finOp);
+        if constexpr (LASTFIN) last_block_fin(a.part, int(gridDim.x), a.arrive, a.finS, a.finOp);
     }
+}
+
+template <int OP, int XL, bool DOT, int U, class T>
+__global__ __launch_bounds__(256) void k_sell(SellArgsT<T> a) {
+    sell_body<OP, XL, DOT, U, T, false>(a);
+}
+template <int OP, int XL, int U, class T>
+__global__ __launch_bounds__(256) void k_sell_fin(SellArgsT<T> a) {
+    sell_body<OP, XL, true, U, T, true>(a);
 }
 
 // One CSR row processed by G lanes (lane g takes slots g, g+G, …): all loads of the row
@@ -505,6 +527,13 @@ static void sell_dispatch2(hipStream_t st, const SellArgsT<T>& a) {
     if (nb == 0) return;
     if (a.S.colP && !a.sliceSys) {  // packed rows (level 0 of a single system)
         const bool fused = a.part != nullptr;
+        if constexpr (OP == OP_AMUL && XL == XL_PLAIN) {
+            if (fused && a.arrive) {
+                if (a.S.colPW == 2) sell_launch(k_sell_packed_fin<OP, XL, 4, T>, nb, st, a);
+                else sell_launch(k_sell_packed_fin<OP, XL, 8, T>, nb, st, a);
+                return;
+            }
+        }
         if constexpr (OP == OP_AMUL || (OP == OP_JACOBI && XL != XL_UNIFORM && XL != XL_FROMZERO)) {
             if (fused) {
                 if (a.S.colPW == 2) sell_launch(k_sell_packed<OP, XL, true, 4, T>, nb, st, a);
@@ -524,6 +553,15 @@ static void sell_dispatch2(hipStream_t st, const SellArgsT<T>& a) {
     }
     const bool wide = a.S.maxWidth > 8;
     if (a.part) {  // fused Σ out·dotv: A·p, and the last sweep of the V-cycle (Σ w·r)
+        if constexpr (OP == OP_AMUL && XL == XL_PLAIN) {
+            if (a.arrive) {
+                if (wide)
+                    sell_launch(k_sell_fin<OP, XL, 16, T>, nb, st, a);
+                else
+                    sell_launch(k_sell_fin<OP, XL, 4, T>, nb, st, a);
+                return;
+            }
+        }
         if constexpr (OP == OP_AMUL || (OP == OP_JACOBI && XL != XL_UNIFORM && XL != XL_FROMZERO)) {
             if (wide)
                 sell_launch(k_sell<OP, XL, true, 16, T>, nb, st, a);

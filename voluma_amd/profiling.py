@@ -71,11 +71,13 @@ def _template_args(kernel):
 
 def _is_sell(kernel):
     """A SELL-64 gather (slot-major or packed rows)."""
-    return kernel.startswith(("k_sell<", "k_sell_packed<", "k_sell_pk<"))
+    return kernel.startswith(("k_sell<", "k_sell_packed<", "k_sell_pk<", "k_sell_fin<", "k_sell_packed_fin<"))
 
 
 def _is_amul(kernel):
     a = _template_args(kernel)
+    if kernel.startswith(("k_sell_fin<", "k_sell_packed_fin<")):  # always the fused dot
+        return len(a) >= 2 and a[0] == "0" and a[1] == "0"
     return _is_sell(kernel) and len(a) >= 3 and a[0] == "0" and a[1] == "0" and a[2] == "true"
 
 
@@ -157,7 +159,7 @@ def levels(db, bench_json):
     level = 0
     for i, ((k, g), us) in enumerate(zip(shape, avg)):
         lev, nbytes, rws, ent = "", None, "", ""
-        cb = 2 if k.startswith(("k_sell_packed<", "k_sell_pk<")) else 4
+        cb = 2 if k.startswith(("k_sell_packed<", "k_sell_pk<", "k_sell_packed_fin<")) else 4
         if _is_amul(k):
             lev, nbytes = "0", _row_bytes("amul", cells[0], 2 * faces[0], 0, elem, cb)
         elif _is_sell(k) or k.startswith("k_csrv<"):
