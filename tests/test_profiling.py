@@ -119,3 +119,38 @@ def test_packed_coarse_rows_keep_their_level(tmp_path):
     # level-0 residual: diag, offsets at 2 bytes, coefficients; r, z read and the residual written
     n, e = CELLS[0], 2 * FACES[0]
     assert rows[2][7] == f"{(8 * n + 10 * e + 3 * 8 * n) / 1e6:.1f}"
+
+
+def _pmc_db(path, iterations, counter, per_dispatch, copy_kib):
+    """A counter run: the kernels view with dispatch ids, and counters_collection."""
+    con = sqlite3.connect(path)
+    con.execute("create table kernels (name text, grid_x int, workgroup_x int, start int, duration int, "
+                "dispatch_id int)")
+    con.execute("create table counters_collection (dispatch_id int, counter_name text, value real)")
+    t, did = 0, 0
+    seq = [("k_vec_fast<13, 0, double>", 4096, 200000, copy_kib)] + [
+        (name, grid, dur, per_dispatch[name]) for it in iterations for name, grid, dur in it]
+    for name, grid, dur, val in seq:
+        did += 1
+        con.execute("insert into kernels values (?, ?, ?, ?, ?, ?)",
+                    (f"void vol::{name}(vol::SellArgsT<double>)", grid * 256, 256, t, dur, did))
+        con.execute("insert into counters_collection values (?, ?, ?)", (did, counter, val))
+        t += dur + 100
+    con.commit()
+    con.close()
+
+
+def test_traffic_scales_counters_by_the_copy_kernel(tmp_path):
+    fdb, wdb, bench = str(tmp_path / "f.db"), str(tmp_path / "w.db"), str(tmp_path / "bench.log")
+    names = {k: 0.0 for k, _, _ in FUSED}
+    fetch = dict(names, **{AMUL[0]: 1000.0})
+    write = dict(names, **{AMUL[0]: 250.0})
+    # the copy's FETCH_SIZE counts half its 1 GiB (factor 2), WRITE_SIZE all of it
+    _pmc_db(fdb, [FUSED] * 3, "FETCH_SIZE", fetch, 524288.0)
+    _pmc_db(wdb, [FUSED] * 3, "WRITE_SIZE", write, 1048576.0)
+    _bench(bench)
+    out = profiling.traffic(fdb, wdb, bench)
+    assert "FETCH_SIZE 2.0000 KiB, WRITE_SIZE 1.0000 KiB" in out
+    row = _rows(out)[0]
+    assert row[1] == f"`{AMUL[0]}`" and row[2] == "0"
+    assert row[4] == f"{1000 * 2 * 1024 / 1e6:.1f}" and row[5] == f"{250 * 1024 / 1e6:.1f}"

@@ -11,6 +11,12 @@
     python -m voluma_amd.profiling DB precisions       the FP64 PCG iteration beside the FP32
                                                        inner iteration of mixed precision,
                                                        dispatch by dispatch
+    python -m voluma_amd.profiling FDB traffic WDB BENCH
+                                                       counter bytes per dispatch of one PCG
+                                                       iteration: FDB and WDB from two
+                                                       ``rocprofv3 --pmc FETCH_SIZE`` and
+                                                       ``--pmc WRITE_SIZE`` runs of the bench,
+                                                       scaled by its copy kernel
 
 The ``levels`` table splits the trace into PCG iterations at the level-0 A·p launches
 (``k_sell<0, 0, true, ...>``), keeps the iterations whose dispatch sequence is the most
@@ -91,8 +97,8 @@ def _iterations(rows, elem_type=None):
 def _common(its):
     """The most common dispatch sequence, the iterations that follow it, and the
     average duration (µs) of each of its dispatches."""
-    shape = Counter(tuple((k, g) for k, g, _, _ in it) for it in its).most_common(1)[0][0]
-    keep = [it for it in its if tuple((k, g) for k, g, _, _ in it) == shape]
+    shape = Counter(tuple((r[0], r[1]) for r in it) for it in its).most_common(1)[0][0]
+    keep = [it for it in its if tuple((r[0], r[1]) for r in it) == shape]
     avg = [sum(it[i][3] for it in keep) / len(keep) / 1e3 for i in range(len(shape))]
     return shape, keep, avg
 
@@ -130,7 +136,8 @@ def _row_bytes(op_kind, n, nnz, nc, elem, col_bytes=4, n_fine=0):
     return mat + 3 * elem * n
 
 
-def levels(db, bench_json):
+def _level_sizes(bench_json):
+    """Cells and internal faces per GAMG level from the bench's JSON line."""
     cfg = None
     with open(bench_json) as fh:  # the bench's JSON line, wherever it is in the file
         for line in reversed(fh.read().splitlines()):
@@ -141,23 +148,19 @@ def levels(db, bench_json):
                 continue
     if cfg is None:
         raise SystemExit(f"no bench.py JSON line in {bench_json}")
-    cells = cfg["gamg_levels"]
-    faces = cfg["gamg_level_faces"]
-    rows = dispatches(db)
-    its = _iterations(rows)
-    if not its:
-        raise SystemExit("no level-0 A·p launches in the trace")
-    shape, keep, avg = _common(its)
+    return cfg["gamg_levels"], cfg["gamg_level_faces"]
+
+
+def _walk(shape, cells, faces):
+    """Per dispatch of one iteration: (level, rows, entries, model bytes or None)."""
     nl = len(cells)
     elem = 8
-    out = [f"{len(keep)} of {len(its)} iterations with the common sequence of {len(shape)} dispatches; "
-           f"{sum(avg):.1f} us of kernel time per iteration", "",
-           "| # | kernel | grid | level | rows | entries | avg us | MB | GB/s |", "|---|---|---|---|---|---|---|---|---|"]
+    out = []
     # the V-cycle's order: residual at l, restriction l → l+1 (l = 0 …), coarsest solve,
     # then prolongation sweeps at l = L-2 … 0 (a streaming prolongation, when split,
     # right before the level-0 sweep)
     level = 0
-    for i, ((k, g), us) in enumerate(zip(shape, avg)):
+    for k, g in shape:
         lev, nbytes, rws, ent = "", None, "", ""
         cb = 2 if k.startswith(("k_sell_packed<", "k_sell_pk<", "k_sell_packed_fin<")) else 4
         if _is_amul(k):
@@ -212,9 +215,76 @@ def levels(db, bench_json):
                 nbytes = 3 * elem * cells[0]
         if lev.isdigit() and not rws:
             rws, ent = cells[int(lev)], 2 * faces[int(lev)]
+        out.append((lev, rws, ent, nbytes))
+    return out
+
+
+def levels(db, bench_json):
+    cells, faces = _level_sizes(bench_json)
+    rows = dispatches(db)
+    its = _iterations(rows)
+    if not its:
+        raise SystemExit("no level-0 A·p launches in the trace")
+    shape, keep, avg = _common(its)
+    out = [f"{len(keep)} of {len(its)} iterations with the common sequence of {len(shape)} dispatches; "
+           f"{sum(avg):.1f} us of kernel time per iteration", "",
+           "| # | kernel | grid | level | rows | entries | avg us | MB | GB/s |", "|---|---|---|---|---|---|---|---|---|"]
+    for i, ((k, g), us, (lev, rws, ent, nbytes)) in enumerate(zip(shape, avg, _walk(shape, cells, faces))):
         mb = f"{nbytes / 1e6:.1f}" if nbytes else ""
         gbs = f"{nbytes / us / 1e3:.0f}" if nbytes else ""
         out.append(f"| {i} | `{k}` | {g} | {lev} | {rws} | {ent} | {us:.2f} | {mb} | {gbs} |")
+    return "\n".join(out)
+
+
+COPY_KERNEL = "k_vec_fast<13, 0, double>"  # the bench's streaming copy: 2^27 doubles each way
+COPY_BYTES = 1 << 30
+
+
+def _counted(db, counter):
+    """Dispatches as in dispatches(db), each with the counter's value (KiB) appended, and
+    the bytes per KiB counted, from the largest launch of the bench's copy kernel."""
+    con = sqlite3.connect(db)
+    vals = dict(con.execute("select dispatch_id, value from counters_collection where counter_name = ?",
+                            (counter,)).fetchall())
+    rows = con.execute("select name, grid_x, workgroup_x, start, duration, dispatch_id from kernels "
+                       "order by start").fetchall()
+    con.close()
+    out = [(_short(n), g // max(wg, 1), s, d, float(vals.get(di, 0.0))) for n, g, wg, s, d, di in rows]
+    copies = [r[4] for r in out if r[0] == COPY_KERNEL]
+    if not copies or max(copies) <= 0:
+        raise SystemExit(f"{db}: no {COPY_KERNEL} launch with {counter} to calibrate against")
+    return out, COPY_BYTES / (max(copies) * 1024.0)
+
+
+def traffic(fetch_db, write_db, bench_json):
+    """Counter bytes per dispatch of one PCG iteration (FETCH_SIZE and WRITE_SIZE from
+    two runs, each scaled by the copy kernel's bytes per counted KiB in its run) beside
+    the model bytes of the levels table."""
+    cells, faces = _level_sizes(bench_json)
+    res = []
+    for db, counter in ((fetch_db, "FETCH_SIZE"), (write_db, "WRITE_SIZE")):
+        rows, scale = _counted(db, counter)
+        its = _iterations(rows)
+        if not its:
+            raise SystemExit(f"{db}: no level-0 A·p launches")
+        shape, keep, _ = _common(its)
+        per = [sum(it[i][4] for it in keep) / len(keep) * 1024.0 * scale for i in range(len(shape))]
+        res.append((shape, keep, scale, per))
+    (s_f, k_f, sc_f, fetch), (s_w, k_w, sc_w, write) = res
+    if s_f != s_w:
+        raise SystemExit("the two runs have different dispatch sequences")
+    walk = _walk(s_f, cells, faces)
+    tot_model = sum(w[3] or 0 for w in walk)
+    tot_ctr = sum(fetch) + sum(write)
+    out = [f"{len(k_f)} and {len(k_w)} iterations with the common sequence of {len(s_f)} dispatches; bytes per "
+           f"counted KiB from {COPY_KERNEL}: FETCH_SIZE {sc_f:.4f} KiB, WRITE_SIZE {sc_w:.4f} KiB; counted "
+           f"{tot_ctr / 1e6:.1f} MB per iteration against {tot_model / 1e6:.1f} MB of model bytes", "",
+           "| # | kernel | level | model MB | fetched MB | written MB | counted MB | counted / model |",
+           "|---|---|---|---|---|---|---|---|"]
+    for i, ((k, _), (lev, _, _, nbytes), f, w) in enumerate(zip(s_f, walk, fetch, write)):
+        mb = f"{nbytes / 1e6:.1f}" if nbytes else ""
+        ratio = f"{(f + w) / nbytes:.2f}" if nbytes else ""
+        out.append(f"| {i} | `{k}` | {lev} | {mb} | {f / 1e6:.1f} | {w / 1e6:.1f} | {(f + w) / 1e6:.1f} | {ratio} |")
     return "\n".join(out)
 
 
@@ -229,6 +299,8 @@ def main(argv):
         print(levels(db, argv[3]))
     elif mode == "precisions":
         print(precisions(db))
+    elif mode == "traffic":
+        print(traffic(db, argv[3], argv[4]))
     else:
         print(__doc__)
         return 2
